@@ -1,0 +1,334 @@
+// ORACLE — test infrastructure only (see cpu_ref.hpp).  C-ABI over the CPU
+// restatement so the Python tests and the CPU-baseline leg of bench.py can run
+// it through ctypes.  The world description and the state records use the same
+// little-endian layout as oracle/ref_driver.cpp writes, so the reference's own
+// description can be fed in and the outputs digested identically.
+#include "cpu_ref.hpp"
+
+#include <chrono>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <thread>
+
+namespace {
+
+struct In {
+    const unsigned char* p;
+    size_t n, o = 0;
+    std::uint32_t u32() {
+        std::uint32_t v;
+        std::memcpy(&v, p + o, 4);
+        o += 4;
+        return v;
+    }
+    float f32() {
+        float v;
+        std::memcpy(&v, p + o, 4);
+        o += 4;
+        return v;
+    }
+};
+
+struct Out {
+    std::vector<unsigned char> buf;
+    void u32(std::uint32_t v) { put(&v, 4); }
+    void f32(float v) { put(&v, 4); }
+    void put(const void* q, size_t k) {
+        const auto* c = static_cast<const unsigned char*>(q);
+        buf.insert(buf.end(), c, c + k);
+    }
+};
+
+template <class R>
+struct Handle {
+    oracle::World<R> world;
+    oracle::StepConf<R> conf{};
+    oracle::StepStats last{};
+    std::uint32_t steps = 0;
+};
+
+// Reads the description (shapes, bodies) and the StepConf record.
+template <class R>
+void Load(Handle<R>& h, In& in) {
+    using namespace oracle;
+    const std::uint32_t ns = in.u32();
+    for (std::uint32_t i = 0; i < ns; ++i) {
+        Shape<R> s;
+        const std::uint32_t nv = in.u32();
+        s.radius = R(in.f32());
+        s.density = R(in.f32());
+        s.friction = R(in.f32());
+        s.restitution = R(in.f32());
+        s.sensor = in.u32();
+        s.catmask = in.u32();
+        s.group = std::int32_t(in.u32());
+        for (std::uint32_t k = 0; k < nv; ++k) {
+            const R x = R(in.f32());
+            const R y = R(in.f32());
+            s.v.push_back(V2<R>{x, y});
+        }
+        for (std::uint32_t k = 0; k < nv; ++k) {
+            const R x = R(in.f32());
+            const R y = R(in.f32());
+            s.nrm.push_back(V2<R>{x, y});
+        }
+        h.world.shapes.push_back(std::move(s));
+    }
+    const std::uint32_t nb = in.u32();
+    for (std::uint32_t i = 0; i < nb; ++i) {
+        Body<R> b;
+        b.type = in.u32();
+        b.flags = in.u32();
+        R f[19];
+        for (auto& x : f) x = R(in.f32());
+        b.pos0 = Pos<R>{V2<R>{f[0], f[1]}, f[2]};
+        b.pos1 = Pos<R>{V2<R>{f[3], f[4]}, f[5]};
+        b.lc = V2<R>{f[6], f[7]};
+        b.vel = Vel<R>{V2<R>{f[8], f[9]}, f[10]};
+        b.lacc = V2<R>{f[11], f[12]};
+        b.aacc = f[13];
+        b.ldamp = f[14];
+        b.adamp = f[15];
+        b.invMass = f[16];
+        b.invRotI = f[17];
+        b.underActive = f[18];
+        const std::uint32_t nsh = in.u32();
+        for (std::uint32_t k = 0; k < nsh; ++k) b.shapes.push_back(in.u32());
+        h.world.AddBody(b);
+    }
+    const std::uint32_t nf = in.u32();
+    R cf[32] = {};
+    for (std::uint32_t k = 0; k < nf && k < 32; ++k) cf[k] = R(in.f32());
+    auto& c = h.conf;
+    c.dt = cf[0];
+    c.dtRatio = cf[1];
+    c.minStillTimeToSleep = cf[2];
+    c.linearSlop = cf[3];
+    c.angularSlop = cf[4];
+    c.regResolutionRate = cf[5];
+    c.regMinSeparation = cf[6];
+    c.regMinMomentum = cf[7];
+    c.velocityThreshold = cf[8];
+    c.maxTranslation = cf[9];
+    c.maxRotation = cf[10];
+    c.maxLinearCorrection = cf[11];
+    c.maxAngularCorrection = cf[12];
+    c.linearSleepTolerance = cf[13];
+    c.angularSleepTolerance = cf[14];
+    c.displaceMultiplier = cf[15];
+    c.aabbExtension = cf[16];
+    c.maxCirclesRatio = cf[17];
+    c.velIters = in.u32();
+    c.posIters = in.u32();
+    const std::uint32_t fl = in.u32();
+    c.warmStart = (fl & 1u) != 0;
+    c.toi = (fl & 2u) != 0;
+    c.blockSolve = (fl & 4u) != 0;
+}
+
+template <class R>
+void WriteState(const Handle<R>& h, Out& o) {
+    using namespace oracle;
+    const auto& w = h.world;
+    o.u32(h.steps);
+    const auto& s = h.last;
+    const std::uint32_t st[8] = {s.preProxiesMoved, s.preDestroyed, s.preAdded, s.preUpdated,
+                                 s.islandsFound, s.regAdded, s.regProxiesMoved, s.bodiesSlept};
+    o.put(st, sizeof(st));
+    o.u32(std::uint32_t(w.bodies.size()));
+    for (const auto& b : w.bodies) {
+        const float v[10] = {float(b.pos0.lin.x), float(b.pos0.lin.y), float(b.pos0.ang),
+                             float(b.pos1.lin.x), float(b.pos1.lin.y), float(b.pos1.ang),
+                             float(b.vel.lin.x),  float(b.vel.lin.y),  float(b.vel.ang),
+                             float(b.underActive)};
+        o.put(v, sizeof(v));
+        o.u32(b.flags);
+    }
+    o.u32(std::uint32_t(w.contacts.size()));
+    for (const auto& kc : w.contacts) {
+        const auto& c = w.contactBuf[kc.id];
+        const auto& m = w.manifoldBuf[kc.id];
+        const std::uint32_t u[4] = {kc.kmin, kc.kmax, kc.id, c.flags};
+        o.put(u, sizeof(u));
+        const std::uint32_t t[3] = {m.type | (m.n << 8), m.n > 0 ? m.pts[0].cf : kDefaultCf,
+                                    m.n > 1 ? m.pts[1].cf : kDefaultCf};
+        o.put(t, sizeof(t));
+        float f[14] = {float(c.friction), float(c.restitution)};
+        if (m.type == kFaceA || m.type == kFaceB) {
+            f[2] = float(m.ln.x);
+            f[3] = float(m.ln.y);
+        }
+        if (m.type != kUnset) {
+            f[4] = float(m.lpt.x);
+            f[5] = float(m.lpt.y);
+        }
+        for (std::uint32_t p = 0; p < m.n; ++p) {
+            f[6 + 4 * p + 0] = float(m.pts[p].lp.x);
+            f[6 + 4 * p + 1] = float(m.pts[p].lp.y);
+            f[6 + 4 * p + 2] = float(m.pts[p].ni);
+            f[6 + 4 * p + 3] = float(m.pts[p].ti);
+        }
+        o.put(f, sizeof(f));
+    }
+    std::uint32_t np = 0;
+    for (const auto& lf : w.tree) np += lf.used ? 1u : 0u;
+    o.u32(np);
+    for (std::uint32_t id = 0; id < w.tree.size(); ++id) {
+        const auto& lf = w.tree[id];
+        if (!lf.used) continue;
+        o.u32(id);
+        const float a[4] = {float(lf.lo[0]), float(lf.lo[1]), float(lf.hi[0]), float(lf.hi[1])};
+        o.put(a, sizeof(a));
+    }
+}
+
+struct AnyHandle {
+    int precision = 32;
+    void* h = nullptr;
+};
+
+template <class R>
+void DoStep(Handle<R>& h) {
+    h.last = h.world.Step(h.conf);
+    ++h.steps;
+}
+
+unsigned char* Copy(const std::vector<unsigned char>& v, size_t* n) {
+    auto* p = static_cast<unsigned char*>(std::malloc(v.size() ? v.size() : 1));
+    if (!v.empty()) std::memcpy(p, v.data(), v.size());
+    *n = v.size();
+    return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Creates an oracle world from a description + conf record (layout of
+// ref_driver.cpp's WriteDescription/WriteConf).  precision: 32 or 64.
+void* oracle_create(const unsigned char* desc, size_t len, int precision) {
+    In in{desc, len};
+    auto* a = new AnyHandle;
+    a->precision = precision;
+    if (precision == 64) {
+        auto* h = new Handle<double>;
+        Load(*h, in);
+        a->h = h;
+    } else {
+        auto* h = new Handle<float>;
+        Load(*h, in);
+        a->h = h;
+    }
+    return a;
+}
+
+void oracle_destroy(void* p) {
+    auto* a = static_cast<AnyHandle*>(p);
+    if (!a) return;
+    if (a->precision == 64) delete static_cast<Handle<double>*>(a->h);
+    else delete static_cast<Handle<float>*>(a->h);
+    delete a;
+}
+
+// Steps n times; when `records` is non-null, appends one state record per step
+// (and returns the malloc'd buffer, free with oracle_free).
+unsigned char* oracle_step(void* p, int n, int record, size_t* outlen) {
+    auto* a = static_cast<AnyHandle*>(p);
+    Out o;
+    for (int i = 0; i < n; ++i) {
+        if (a->precision == 64) {
+            auto& h = *static_cast<Handle<double>*>(a->h);
+            DoStep(h);
+            if (record) WriteState(h, o);
+        } else {
+            auto& h = *static_cast<Handle<float>*>(a->h);
+            DoStep(h);
+            if (record) WriteState(h, o);
+        }
+    }
+    return Copy(o.buf, outlen);
+}
+
+// Current state record (same layout as a per-step record).
+unsigned char* oracle_state(void* p, size_t* outlen) {
+    auto* a = static_cast<AnyHandle*>(p);
+    Out o;
+    if (a->precision == 64) WriteState(*static_cast<Handle<double>*>(a->h), o);
+    else WriteState(*static_cast<Handle<float>*>(a->h), o);
+    return Copy(o.buf, outlen);
+}
+
+// Full body positions/velocities in the oracle's own precision, as doubles:
+// out[b*6 + {x, y, angle, vx, vy, w}].
+void oracle_bodies_f64(void* p, double* out) {
+    auto* a = static_cast<AnyHandle*>(p);
+    auto fill = [&](const auto& w) {
+        size_t k = 0;
+        for (const auto& b : w.bodies) {
+            out[k++] = double(b.pos1.lin.x);
+            out[k++] = double(b.pos1.lin.y);
+            out[k++] = double(b.pos1.ang);
+            out[k++] = double(b.vel.lin.x);
+            out[k++] = double(b.vel.lin.y);
+            out[k++] = double(b.vel.ang);
+        }
+    };
+    if (a->precision == 64) fill(static_cast<Handle<double>*>(a->h)->world);
+    else fill(static_cast<Handle<float>*>(a->h)->world);
+}
+
+void oracle_free(void* p) { std::free(p); }
+
+// CPU-baseline helper: steps many independent worlds (descriptions packed
+// back-to-back, each prefixed by its byte length) with `threads` std::threads,
+// each owning a static contiguous partition balanced by body count.  Returns
+// wall seconds for `steps` steps of every world; *body_steps receives
+// sum(dynamic bodies) * steps.
+double oracle_time_batch(const unsigned char* packed, size_t len, int nworlds, int steps, int threads,
+                         double* body_steps) {
+    std::vector<std::unique_ptr<Handle<float>>> ws;
+    std::vector<double> weight;
+    size_t off = 0;
+    double dyn = 0;
+    for (int i = 0; i < nworlds && off + 8 <= len; ++i) {
+        std::uint64_t n;
+        std::memcpy(&n, packed + off, 8);
+        off += 8;
+        In in{packed + off, size_t(n)};
+        auto h = std::make_unique<Handle<float>>();
+        Load(*h, in);
+        off += size_t(n);
+        double nb = 0;
+        for (const auto& b : h->world.bodies) nb += (b.flags & oracle::kAccelerable) ? 1.0 : 0.0;
+        dyn += nb;
+        weight.push_back(nb);
+        ws.push_back(std::move(h));
+    }
+    if (threads < 1) threads = 1;
+    // Static partition by cumulative body count.
+    std::vector<size_t> cut(size_t(threads) + 1, ws.size());
+    cut[0] = 0;
+    double total = 0;
+    for (double x : weight) total += x;
+    double acc = 0;
+    int t = 1;
+    for (size_t i = 0; i < ws.size() && t < threads; ++i) {
+        acc += weight[i];
+        while (t < threads && acc >= total * t / threads) cut[size_t(t++)] = i + 1;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<std::thread> pool;
+    for (int k = 0; k < threads; ++k) {
+        pool.emplace_back([&, k] {
+            for (size_t i = cut[size_t(k)]; i < cut[size_t(k) + 1]; ++i)
+                for (int s = 0; s < steps; ++s) DoStep(*ws[i]);
+        });
+    }
+    for (auto& th : pool) th.join();
+    const auto t1 = std::chrono::steady_clock::now();
+    *body_steps = dyn * steps;
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
+}  // extern "C"

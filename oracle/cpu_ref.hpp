@@ -1,0 +1,1504 @@
+// ORACLE — test infrastructure only.  A CPU restatement of PlayRho's per-step
+// hot path, `Step(AabbTreeWorld&, const StepConf&)` with doToi == false, used as
+// the checker for the HIP implementation in playrho_amd/.  Nothing in the
+// product links, loads or calls this code; tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg use it as the checker / CPU baseline only.
+//
+// Templated on the real type: the float instantiation is pinned bit-for-bit to
+// the reference float build (oracle/_ref, tests/golden); the double
+// instantiation gives the FP64 comparison figure.
+//
+// Every function cites the reference file:line it restates (paths relative to
+// the PlayRho checkout).  Operation order inside each arithmetic expression
+// follows the reference so that float results are bit-identical; compile with
+// -ffp-contract=off and without -ffast-math.
+//
+// Pinning (what checks each part):
+//  - whole step, float: per-step digests of full state vs the reference build
+//    for the hello / pyramid / drop / pile / billiards goldens (tests/golden);
+//  - ContactFeature / manifold: the same digests include every manifold;
+//  - double instantiation: closed-form free fall (v = a t, p += v h), and
+//    agreement with the float instantiation to float precision on short runs.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <tuple>
+#include <vector>
+
+namespace oracle {
+
+template <class R>
+struct V2 {
+    R x = 0, y = 0;
+};
+template <class R> inline V2<R> operator+(V2<R> a, V2<R> b) { return {a.x + b.x, a.y + b.y}; }
+template <class R> inline V2<R> operator-(V2<R> a, V2<R> b) { return {a.x - b.x, a.y - b.y}; }
+template <class R> inline V2<R> operator-(V2<R> a) { return {-a.x, -a.y}; }
+// Vector * scalar, element-wise (Vector.hpp:496-507).
+template <class R> inline V2<R> operator*(V2<R> a, R s) { return {a.x * s, a.y * s}; }
+// Scalar * vector (Vector.hpp:479-490).
+template <class R> inline V2<R> operator*(R s, V2<R> a) { return {s * a.x, s * a.y}; }
+// Vector / scalar multiplies by the reciprocal (Vector.hpp:511-521).
+template <class R> inline V2<R> operator/(V2<R> a, R s) {
+    const R inv = R(1) / s;
+    return {a.x * inv, a.y * inv};
+}
+template <class R> inline bool operator==(V2<R> a, V2<R> b) { return a.x == b.x && a.y == b.y; }
+
+// Dot starts its sum from zero (Math.hpp:320-334).
+template <class R> inline R Dot(V2<R> a, V2<R> b) {
+    R r = 0;
+    r += a.x * b.x;
+    r += a.y * b.y;
+    return r;
+}
+// 2-D cross (Math.hpp:365-386).
+template <class R> inline R Cross(V2<R> a, V2<R> b) {
+    const R m = a.x * b.y;
+    const R s = a.y * b.x;
+    return m - s;
+}
+// Math.hpp:274-283.
+template <class R> inline R MagSq(V2<R> v) {
+    R r = 0;
+    r += v.x * v.x;
+    r += v.y * v.y;
+    return r;
+}
+template <class R> inline V2<R> RevPerp(V2<R> v) { return {-v.y, v.x}; }  // UnitVec.hpp:364-368
+template <class R> inline V2<R> FwdPerp(V2<R> v) { return {v.y, -v.x}; }  // UnitVec.hpp:374-378
+
+// Rotate / InverseRotate (d2/Math.hpp:107-134); q = (cos, sin).
+template <class R> inline V2<R> Rotate(V2<R> v, V2<R> q) {
+    const R nx = (q.x * v.x) - (q.y * v.y);
+    const R ny = (q.y * v.x) + (q.x * v.y);
+    return {nx, ny};
+}
+template <class R> inline V2<R> InvRotate(V2<R> v, V2<R> q) {
+    const R nx = (q.x * v.x) + (q.y * v.y);
+    const R ny = (q.x * v.y) - (q.y * v.x);
+    return {nx, ny};
+}
+// UnitVec::Rotate (UnitVec.hpp:353-358).
+template <class R> inline V2<R> RotateUnit(V2<R> a, V2<R> amount) {
+    return {a.x * amount.x - a.y * amount.y, a.y * amount.x + a.x * amount.y};
+}
+
+template <class R>
+struct Xf {
+    V2<R> p;
+    V2<R> q;  // cos, sin
+};
+// d2/Math.hpp:194-212
+template <class R> inline V2<R> Transform(V2<R> v, const Xf<R>& xf) { return Rotate(v, xf.q) + xf.p; }
+template <class R> inline V2<R> InvTransform(V2<R> v, const Xf<R>& xf) { return InvRotate(v - xf.p, xf.q); }
+// MulT (d2/Math.hpp:224-229).
+template <class R> inline Xf<R> MulT(const Xf<R>& A, const Xf<R>& B) {
+    const V2<R> dp = B.p - A.p;
+    return Xf<R>{InvRotate(dp, A.q), RotateUnit(B.q, V2<R>{A.q.x, -A.q.y})};
+}
+
+// UnitVec::Get(angle) = {cos, sin} (UnitVec.cpp:27-30).
+template <class R> inline V2<R> UnitFromAngle(R a) { return {std::cos(a), std::sin(a)}; }
+
+template <class R>
+struct Pos {
+    V2<R> lin;
+    R ang = 0;
+};
+template <class R> inline Pos<R> operator+(Pos<R> a, Pos<R> b) { return {a.lin + b.lin, a.ang + b.ang}; }
+template <class R> inline Pos<R> operator-(Pos<R> a) { return {-a.lin, -a.ang}; }
+template <class R> inline bool operator!=(Pos<R> a, Pos<R> b) { return !(a.lin == b.lin) || a.ang != b.ang; }
+
+template <class R>
+struct Vel {
+    V2<R> lin;
+    R ang = 0;
+};
+
+// GetTransformation(pos, localCenter) (d2/Math.hpp:232-245).
+template <class R> inline Xf<R> GetXf(const Pos<R>& pos, V2<R> lc) {
+    const V2<R> q = UnitFromAngle(pos.ang);
+    return Xf<R>{pos.lin - Rotate(lc, q), q};
+}
+
+// UnitVec::Get(x, y, fallback) (UnitVec.hpp:175-236).
+template <class R> inline V2<R> UnitVector(V2<R> v, V2<R> fallback) {
+    const R m2 = v.x * v.x + v.y * v.y;
+    if (std::isnormal(m2)) {
+        const R mag = std::sqrt(m2);
+        const R inv = R(1) / mag;
+        return {v.x * inv, v.y * inv};
+    }
+    const R mag = std::hypot(v.x, v.y);
+    if (!std::isnormal(mag)) return fallback;
+    return {v.x / mag, v.y / mag};
+}
+
+// std::max / std::min / std::clamp comparison forms.
+template <class R> inline R Max(R a, R b) { return (a < b) ? b : a; }
+template <class R> inline R Min(R a, R b) { return (b < a) ? b : a; }
+template <class R> inline R Clamp(R v, R lo, R hi) { return (v < lo) ? lo : ((hi < v) ? hi : v); }
+
+// AlmostEqual with 4 ULP via nextafter (Math.hpp:186-194).
+template <class R> inline bool AlmostEqual(R a, R b, int ulp = 4) {
+    for (; (a != b) && (ulp > 0); --ulp) a = std::nextafter(a, b);
+    return a == b;
+}
+template <class R> inline bool AlmostZero(R v) { return std::abs(v) < std::numeric_limits<R>::min(); }
+
+// ---------------------------------------------------------------------------
+// ContactFeature (ContactFeature.hpp:30-85), packed as
+// typeA | typeB << 1 | indexA << 8 | indexB << 16.
+using Cf = std::uint32_t;
+constexpr std::uint32_t kVertex = 0, kFace = 1, kInvalidVertex = 255;
+inline Cf MakeCf(std::uint32_t tA, std::uint32_t iA, std::uint32_t tB, std::uint32_t iB) {
+    return tA | (tB << 1) | ((iA & 0xffu) << 8) | ((iB & 0xffu) << 16);
+}
+inline Cf FlipCf(Cf c) {
+    return MakeCf((c >> 1) & 1u, (c >> 16) & 0xffu, c & 1u, (c >> 8) & 0xffu);
+}
+inline std::uint32_t CfIndexB(Cf c) { return (c >> 16) & 0xffu; }
+constexpr Cf kDefaultCf = (kInvalidVertex << 8) | (kInvalidVertex << 16);
+
+enum ManifoldType : std::uint32_t { kUnset = 0, kCircles = 1, kFaceA = 2, kFaceB = 3 };
+
+template <class R>
+struct MPoint {
+    V2<R> lp;
+    Cf cf = kDefaultCf;
+    R ni = 0, ti = 0;
+};
+
+template <class R>
+struct Manifold {
+    std::uint32_t type = kUnset;
+    std::uint32_t n = 0;
+    V2<R> ln;  // local normal
+    V2<R> lpt; // local point
+    MPoint<R> pts[2];
+    void Add(const MPoint<R>& p) { pts[n++] = p; }
+};
+// Manifold factories (d2/Manifold.hpp:168-243).
+template <class R> inline Manifold<R> ForCircles(V2<R> vA, std::uint32_t iA, V2<R> vB, std::uint32_t iB) {
+    Manifold<R> m;
+    m.type = kCircles;
+    m.lpt = vA;
+    m.Add(MPoint<R>{vB, MakeCf(kVertex, iA, kVertex, iB)});
+    return m;
+}
+template <class R> inline Manifold<R> ForFace(bool faceB, V2<R> ln, V2<R> lp) {
+    Manifold<R> m;
+    m.type = faceB ? kFaceB : kFaceA;
+    m.ln = ln;
+    m.lpt = lp;
+    return m;
+}
+
+// DistanceProxy: vertices, forward normals, vertex radius.
+template <class R>
+struct Proxy {
+    const V2<R>* v = nullptr;
+    const V2<R>* nrm = nullptr;
+    std::uint32_t n = 0;
+    R radius = 0;
+};
+
+inline std::uint32_t ModNext(std::uint32_t i, std::uint32_t n) { return (i + 1 < n) ? i + 1 : 0; }
+inline std::uint32_t ModPrev(std::uint32_t i, std::uint32_t n) { return i ? i - 1 : n - 1; }
+
+template <class R>
+struct ClipV {
+    V2<R> v;
+    Cf cf;
+};
+template <class R>
+struct ClipList {
+    ClipV<R> e[2];
+    std::uint32_t n = 0;
+    void push(const ClipV<R>& c) { e[n++] = c; }
+};
+
+// Manifold.cpp:61-102
+template <class R>
+ClipList<R> ClipSegmentToLine(const ClipList<R>& vIn, V2<R> normal, R offset, std::uint32_t indexA) {
+    ClipList<R> out;
+    if (vIn.n == 2) {
+        const R d0 = Dot(normal, vIn.e[0].v) - offset;
+        const R d1 = Dot(normal, vIn.e[1].v) - offset;
+        if (d0 <= R(0) || AlmostZero(d0)) out.push(vIn.e[0]);
+        if (d1 <= R(0) || AlmostZero(d1)) out.push(vIn.e[1]);
+        if (out.n < 2 && std::signbit(d0) != std::signbit(d1)) {
+            const R interp = d0 / (d0 - d1);
+            const V2<R> vertex = vIn.e[0].v + (vIn.e[1].v - vIn.e[0].v) * interp;
+            out.push(ClipV<R>{vertex, MakeCf(kVertex, indexA, kFace, CfIndexB(vIn.e[0].cf))});
+        }
+    }
+    return out;
+}
+
+// Manifold.cpp:122-141
+template <class R>
+inline void MostAntiParallelEdge(V2<R> n0, const Xf<R>& xf0, const Proxy<R>& s1, const Xf<R>& xf1,
+                                 std::uint32_t first, std::uint32_t second,
+                                 std::uint32_t& e0, std::uint32_t& e1) {
+    if (second == kInvalidVertex) {
+        const V2<R> normal = InvRotate(Rotate(n0, xf0.q), xf1.q);
+        const std::uint32_t prev = ModPrev(first, s1.n);
+        const R prevDot = Dot(normal, s1.nrm[prev]);
+        const R currDot = Dot(normal, s1.nrm[first]);
+        if (prevDot < currDot) {
+            e0 = prev;
+            e1 = first;
+        } else {
+            e0 = first;
+            e1 = ModNext(first, s1.n);
+        }
+        return;
+    }
+    if ((second > first) && ((first + 1) == second)) {
+        e0 = first;
+        e1 = second;
+    } else {
+        e0 = second;
+        e1 = first;
+    }
+}
+
+// Polygon-polygon manifold (Manifold.cpp:168-291).
+template <class R>
+Manifold<R> PolyManifold(bool flipped, const Proxy<R>& s0, const Xf<R>& xf0, std::uint32_t idx0,
+                         const Proxy<R>& s1, const Xf<R>& xf1, std::uint32_t i1a, std::uint32_t i1b,
+                         R maxCirclesRatio) {
+    const R r0 = s0.radius;
+    const R totalRadius = r0 + s1.radius;
+    const std::uint32_t idx0Next = ModNext(idx0, s0.n);
+    const V2<R> rv0 = s0.v[idx0];
+    const V2<R> rv1 = s0.v[idx0Next];
+    const V2<R> av0 = Transform(rv0, xf0);
+    const V2<R> av1 = Transform(rv1, xf0);
+    const R lenEdge0 = MagSq(rv1 - rv0);
+    const V2<R> rn0 = s0.nrm[idx0];
+    const V2<R> re0dir = RevPerp(rn0);
+    std::uint32_t e1first, e1second;
+    MostAntiParallelEdge(rn0, xf0, s1, xf1, i1a, i1b, e1first, e1second);
+    const V2<R> s1rv0 = s1.v[e1first];
+    const V2<R> s1av0 = Transform(s1rv0, xf1);
+    const V2<R> s1rv1 = s1.v[e1second];
+    const V2<R> s1av1 = Transform(s1rv1, xf1);
+    {
+        const V2<R> ae0dir = Rotate(re0dir, xf0.q);
+        // GetClipPoints (Manifold.cpp:143-156)
+        ClipList<R> ie;
+        ie.push(ClipV<R>{s1av0, MakeCf(kFace, idx0, kVertex, e1first)});
+        ie.push(ClipV<R>{s1av1, MakeCf(kFace, idx0, kVertex, e1second)});
+        const R dp0 = -Dot(ae0dir, av0);
+        const R dp1 = +Dot(ae0dir, av1);
+        const ClipList<R> pts = ClipSegmentToLine(ie, -ae0dir, dp0, idx0);
+        const ClipList<R> clip = ClipSegmentToLine(pts, ae0dir, dp1, idx0Next);
+        if (clip.n == 2) {
+            const V2<R> absNormal = FwdPerp(ae0dir);
+            const V2<R> relMid = (rv0 + rv1) / R(2);
+            const R absOffset = Dot(absNormal, av0);
+            const V2<R> normal = FwdPerp(re0dir);
+            Manifold<R> m = ForFace(flipped, normal, relMid);
+            for (std::uint32_t k = 0; k < clip.n; ++k) {
+                const auto& cp = clip.e[k];
+                if ((Dot(absNormal, cp.v) - absOffset) <= totalRadius) {
+                    m.Add(MPoint<R>{InvTransform(cp.v, xf1), flipped ? FlipCf(cp.cf) : cp.cf});
+                }
+            }
+            if (m.n > 0) return m;
+        }
+    }
+    const R trSq = totalRadius * totalRadius;
+    const R cr = maxCirclesRatio * r0;
+    const bool mustFace = lenEdge0 > cr * cr;
+    auto vertexCase = [&](V2<R> v0rel, std::uint32_t i0, V2<R> v1rel, std::uint32_t i1) {
+        if (mustFace) {
+            Manifold<R> m = ForFace(flipped, FwdPerp(re0dir), v0rel);
+            m.Add(MPoint<R>{v1rel, flipped ? MakeCf(kVertex, i1, kFace, i0) : MakeCf(kFace, i0, kVertex, i1)});
+            return m;
+        }
+        return !flipped ? ForCircles(v0rel, i0, v1rel, i1) : ForCircles(v1rel, i1, v0rel, i0);
+    };
+    if (MagSq(av0 - s1av0) <= trSq) return vertexCase(rv0, idx0, s1rv0, e1first);
+    if (MagSq(av1 - s1av1) <= trSq) return vertexCase(rv1, idx0Next, s1rv1, e1second);
+    if (MagSq(av0 - s1av1) <= trSq) return vertexCase(rv0, idx0, s1rv1, e1second);
+    if (MagSq(av1 - s1av0) <= trSq) return vertexCase(rv1, idx0Next, s1rv0, e1first);
+    return Manifold<R>{};
+}
+
+// Polygon-circle manifold (Manifold.cpp:293-372).
+template <class R>
+Manifold<R> PolyCircleManifold(bool flipped, R totalRadius, const Proxy<R>& s, const Xf<R>& sxf,
+                               V2<R> point, const Xf<R>& xf) {
+    const std::uint32_t n = s.n;
+    const V2<R> cLocal = InvTransform(Transform(point, xf), sxf);
+    std::uint32_t indexOfMax = 0;
+    R maxSep = -std::numeric_limits<R>::max();  // -MaxFloat (Settings.hpp:64)
+    for (std::uint32_t i = 0; i < n; ++i) {
+        const R sep = Dot(s.nrm[i], cLocal - s.v[i]);
+        if (sep > totalRadius) return Manifold<R>{};
+        if (maxSep < sep) {
+            maxSep = sep;
+            indexOfMax = i;
+        }
+    }
+    const std::uint32_t indexOfMax2 = ModNext(indexOfMax, n);
+    const V2<R> v1 = s.v[indexOfMax];
+    const V2<R> v2 = s.v[indexOfMax2];
+    auto face = [&](V2<R> fc) {
+        Manifold<R> m = ForFace(flipped, s.nrm[indexOfMax], fc);
+        m.Add(MPoint<R>{point, flipped ? MakeCf(kVertex, 0, kFace, indexOfMax) : MakeCf(kFace, indexOfMax, kVertex, 0)});
+        return m;
+    };
+    if (maxSep < R(0)) return face((v1 + v2) / R(2));
+    const V2<R> cv1 = cLocal - v1;
+    if (Dot(cv1, v2 - v1) <= R(0)) {
+        if (MagSq(cv1) > totalRadius * totalRadius) return Manifold<R>{};
+        return !flipped ? ForCircles(v1, indexOfMax, point, 0u) : ForCircles(point, 0u, v1, indexOfMax);
+    }
+    const V2<R> cv2 = cLocal - v2;
+    if (Dot(cv2, v1 - v2) <= R(0)) {
+        if (MagSq(cv2) > totalRadius * totalRadius) return Manifold<R>{};
+        return !flipped ? ForCircles(v2, indexOfMax2, point, 0u) : ForCircles(point, 0u, v2, indexOfMax2);
+    }
+    const V2<R> fc = (v1 + v2) / R(2);
+    if (Dot(cLocal - fc, s.nrm[indexOfMax]) > totalRadius) return Manifold<R>{};
+    return face(fc);
+}
+
+// GetMinSeparationInfo (ShapeSeparation.cpp:39-64)
+template <class R>
+inline void MinSepInfo(V2<R> origin, V2<R> dir, const V2<R>* verts, std::uint32_t n,
+                       R& minSep, std::uint32_t& first, std::uint32_t& second) {
+    minSep = std::numeric_limits<R>::infinity();
+    first = kInvalidVertex;
+    second = kInvalidVertex;
+    for (std::uint32_t i = 0; i < n; ++i) {
+        const R s = Dot(dir, verts[i] - origin);
+        if (minSep > s) {
+            minSep = s;
+            first = i;
+            second = kInvalidVertex;
+        } else if (minSep == s) {
+            second = i;
+        }
+    }
+}
+
+template <class R>
+struct SepInfo {
+    R distance;
+    std::uint32_t first, second0, second1;
+};
+
+// GetMaxSeparation4x4 (ShapeSeparation.cpp:66-104)
+template <class R>
+SepInfo<R> MaxSep4x4(const Proxy<R>& p1, const Xf<R>& xf1, const Proxy<R>& p2, const Xf<R>& xf2) {
+    SepInfo<R> out{-std::numeric_limits<R>::infinity(), kInvalidVertex, kInvalidVertex, kInvalidVertex};
+    const Xf<R> xf = MulT(xf1, xf2);
+    V2<R> verts[4];
+    for (int k = 0; k < 4; ++k) verts[k] = Transform(p2.v[k], xf);
+    for (std::uint32_t i = 0; i < 4; ++i) {
+        R d;
+        std::uint32_t f, s;
+        MinSepInfo(p1.v[i], p1.nrm[i], verts, 4u, d, f, s);
+        if (out.distance < d) {
+            out.distance = d;
+            out.second0 = f;
+            out.second1 = s;
+            out.first = i;
+        }
+    }
+    return out;
+}
+
+// GetMaxSeparation (ShapeSeparation.cpp:106-128)
+template <class R>
+SepInfo<R> MaxSep(const Proxy<R>& p1, const Xf<R>& xf1, const Proxy<R>& p2, const Xf<R>& xf2) {
+    SepInfo<R> out{-std::numeric_limits<R>::infinity(), kInvalidVertex, kInvalidVertex, kInvalidVertex};
+    const Xf<R> xf = MulT(xf2, xf1);
+    for (std::uint32_t i = 0; i < p1.n; ++i) {
+        const V2<R> origin = Transform(p1.v[i], xf);
+        const V2<R> normal = Rotate(p1.nrm[i], xf.q);
+        R d;
+        std::uint32_t f, s;
+        MinSepInfo(origin, normal, p2.v, p2.n, d, f, s);
+        if (out.distance < d) {
+            out.distance = d;
+            out.second0 = f;
+            out.second1 = s;
+            out.first = i;
+        }
+    }
+    return out;
+}
+
+// CollideShapes (Manifold.cpp:393-438)
+template <class R>
+Manifold<R> CollideShapes(const Proxy<R>& A, const Xf<R>& xfA, const Proxy<R>& B, const Xf<R>& xfB,
+                          R linearSlop, R maxCirclesRatio) {
+    const R totalRadius = A.radius + B.radius;
+    if (A.n == 1 && B.n == 1) {
+        // Manifold.cpp:374-386
+        const V2<R> pA = Transform(A.v[0], xfA);
+        const V2<R> pB = Transform(B.v[0], xfB);
+        const R lenSq = MagSq(pB - pA);
+        const R totSq = totalRadius * totalRadius;
+        if (lenSq > totSq) return Manifold<R>{};
+        return ForCircles(A.v[0], 0u, B.v[0], 0u);
+    }
+    if (A.n == 1) return PolyCircleManifold(true, totalRadius, B, xfB, A.v[0], xfA);
+    if (B.n == 1) return PolyCircleManifold(false, totalRadius, A, xfA, B.v[0], xfB);
+    const bool do4x4 = (A.n == 4) && (B.n == 4);
+    const SepInfo<R> sa = do4x4 ? MaxSep4x4(A, xfA, B, xfB) : MaxSep(A, xfA, B, xfB);
+    if (sa.distance > totalRadius) return Manifold<R>{};
+    const SepInfo<R> sb = do4x4 ? MaxSep4x4(B, xfB, A, xfA) : MaxSep(B, xfB, A, xfA);
+    if (sb.distance > totalRadius) return Manifold<R>{};
+    const R kTol = linearSlop / R(10);
+    return (sb.distance > (sa.distance + kTol))
+               ? PolyManifold(true, B, xfB, sb.first, A, xfA, sb.second0, sb.second1, maxCirclesRatio)
+               : PolyManifold(false, A, xfA, sa.first, B, xfB, sa.second0, sa.second1, maxCirclesRatio);
+}
+
+// ---------------------------------------------------------------------------
+// World manifold (WorldManifold.cpp:37-130)
+template <class R>
+struct WorldManifold {
+    V2<R> normal;
+    std::uint32_t n = 0;
+    V2<R> p[2];
+    R ni[2] = {0, 0}, ti[2] = {0, 0};
+    R sep[2] = {0, 0};
+};
+
+template <class R>
+WorldManifold<R> GetWorldManifold(const Manifold<R>& m, const Xf<R>& xfA, R rA, const Xf<R>& xfB, R rB) {
+    WorldManifold<R> w;
+    if (m.type == kCircles) {
+        const V2<R> pA = Transform(m.lpt, xfA);
+        const V2<R> pB = Transform(m.pts[0].lp, xfB);
+        const V2<R> normal = UnitVector(pB - pA, V2<R>{R(1), R(0)});
+        const V2<R> cA = pA + (rA * normal);
+        const V2<R> cB = pB - (rB * normal);
+        w.normal = normal;
+        w.n = 1;
+        w.p[0] = (cA + cB) / R(2);
+        w.ni[0] = m.pts[0].ni;
+        w.ti[0] = m.pts[0].ti;
+        w.sep[0] = Dot(cB - cA, normal);
+        return w;
+    }
+    if (m.type == kFaceA) {
+        const V2<R> normal = Rotate(m.ln, xfA.q);
+        const V2<R> plane = Transform(m.lpt, xfA);
+        w.normal = normal;
+        w.n = m.n;
+        for (std::uint32_t i = 0; i < m.n; ++i) {
+            const V2<R> clip = Transform(m.pts[i].lp, xfB);
+            const V2<R> cA = clip + (rA - Dot(clip - plane, normal)) * normal;
+            const V2<R> cB = clip - (rB * normal);
+            w.p[i] = (cA + cB) / R(2);
+            w.ni[i] = m.pts[i].ni;
+            w.ti[i] = m.pts[i].ti;
+            w.sep[i] = Dot(cB - cA, normal);
+        }
+        return w;
+    }
+    if (m.type == kFaceB) {
+        const V2<R> normal = Rotate(m.ln, xfB.q);
+        const V2<R> plane = Transform(m.lpt, xfB);
+        w.normal = -normal;
+        w.n = m.n;
+        for (std::uint32_t i = 0; i < m.n; ++i) {
+            const V2<R> clip = Transform(m.pts[i].lp, xfA);
+            const V2<R> cB = clip + (rB - Dot(clip - plane, normal)) * normal;
+            const V2<R> cA = clip - (rA * normal);
+            w.p[i] = (cA + cB) / R(2);
+            w.ni[i] = m.pts[i].ni;
+            w.ti[i] = m.pts[i].ti;
+            w.sep[i] = Dot(cA - cB, normal);
+        }
+        return w;
+    }
+    return w;
+}
+
+// Position solver manifold (PositionSolverManifold.cpp:42-121)
+template <class R>
+struct PSM {
+    V2<R> normal, point;
+    R sep;
+};
+template <class R>
+PSM<R> GetPSM(const Manifold<R>& m, std::uint32_t idx, const Xf<R>& xfA, const Xf<R>& xfB) {
+    if (m.type == kCircles) {
+        const V2<R> pA = Transform(m.lpt, xfA);
+        const V2<R> pB = Transform(m.pts[idx].lp, xfB);
+        const V2<R> delta = pB - pA;
+        const V2<R> normal = UnitVector(delta, V2<R>{R(0), R(0)});
+        const V2<R> mid = (pA + pB) / R(2);
+        return PSM<R>{normal, mid, Dot(delta, normal)};
+    }
+    if (m.type == kFaceA) {
+        const V2<R> plane = Transform(m.lpt, xfA);
+        const V2<R> normal = Rotate(m.ln, xfA.q);
+        const V2<R> clip = Transform(m.pts[idx].lp, xfB);
+        return PSM<R>{normal, clip, Dot(clip - plane, normal)};
+    }
+    if (m.type == kFaceB) {
+        const V2<R> plane = Transform(m.lpt, xfB);
+        const V2<R> normal = Rotate(m.ln, xfB.q);
+        const V2<R> clip = Transform(m.pts[idx].lp, xfA);
+        return PSM<R>{-normal, clip, Dot(clip - plane, normal)};
+    }
+    return PSM<R>{V2<R>{}, V2<R>{}, std::numeric_limits<R>::quiet_NaN()};
+}
+
+// ---------------------------------------------------------------------------
+// Step configuration: the StepConf fields the regular step uses
+// (StepConf.hpp:49-392).
+template <class R>
+struct StepConf {
+    R dt, dtRatio, minStillTimeToSleep, linearSlop, angularSlop, regResolutionRate,
+        regMinSeparation, regMinMomentum, velocityThreshold, maxTranslation, maxRotation,
+        maxLinearCorrection, maxAngularCorrection, linearSleepTolerance, angularSleepTolerance,
+        displaceMultiplier, aabbExtension, maxCirclesRatio;
+    std::uint32_t velIters, posIters;
+    bool warmStart, toi, blockSolve;
+};
+
+// Body flags as in the dump format.
+enum BodyFlag : std::uint32_t {
+    kSpeedable = 1, kAccelerable = 2, kImpenetrable = 4, kAwake = 8, kAutoSleep = 16,
+    kEnabled = 32, kFixedRotation = 64
+};
+// Contact flags as in the dump format.
+enum ContactFlag : std::uint32_t {
+    kTouching = 1, kCEnabled = 2, kSensor = 4, kCImpenetrable = 8, kNeedsUpdating = 16,
+    kNeedsFiltering = 32
+};
+
+template <class R>
+struct Shape {
+    std::vector<V2<R>> v, nrm;
+    R radius, density, friction, restitution;
+    std::uint32_t sensor, catmask;
+    std::int32_t group;
+    Proxy<R> proxy() const { return Proxy<R>{v.data(), nrm.data(), std::uint32_t(v.size()), radius}; }
+};
+
+template <class R>
+struct Body {
+    std::uint32_t type = 0, flags = 0;
+    Pos<R> pos0, pos1;
+    V2<R> lc;
+    Vel<R> vel;
+    V2<R> lacc;
+    R aacc = 0, ldamp = 0, adamp = 0, invMass = 0, invRotI = 0, underActive = 0;
+    Xf<R> xf;  // cached transform of pos1 (Body.hpp:697-715)
+    std::vector<std::uint32_t> shapes;
+    std::vector<std::uint32_t> proxies;
+    bool awake() const { return (flags & kAwake) != 0; }
+    bool speedable() const { return (flags & kSpeedable) != 0; }
+    bool accelerable() const { return (flags & kAccelerable) != 0; }
+    // Body::SetAwake (Body.cpp:163-171)
+    void SetAwake() {
+        if (flags & kSpeedable) {
+            flags |= kAwake;
+            underActive = 0;
+        }
+    }
+    void SetAwakeFlag() { flags |= kAwake; }
+    // Unawaken -> UnsetAwake (Body.hpp:1203-1210, Body.cpp:173-182)
+    bool Unawaken() {
+        if ((flags & kAwake) && (flags & kAutoSleep)) {
+            flags &= ~kAwake;
+            underActive = 0;
+            vel = Vel<R>{};
+            return true;
+        }
+        return false;
+    }
+};
+
+template <class R>
+struct Leaf {
+    bool used = false;
+    std::uint32_t body = 0, shape = 0, child = 0;
+    R lo[2], hi[2];
+};
+
+template <class R>
+struct Contact {
+    std::uint32_t kmin = 0, kmax = 0;
+    std::uint32_t bodyA = 0, bodyB = 0, shapeA = 0, shapeB = 0;
+    std::uint32_t flags = 0;
+    R friction = 0, restitution = 0, tangentSpeed = 0;
+};
+
+struct StepStats {
+    std::uint32_t preProxiesMoved = 0, preDestroyed = 0, preAdded = 0, preUpdated = 0;
+    std::uint32_t islandsFound = 0, regAdded = 0, regProxiesMoved = 0, bodiesSlept = 0;
+};
+
+// Solver scratch records.
+template <class R>
+struct BodyConstraint {
+    R invMass = 0, invRotI = 0;
+    V2<R> lc;
+    Pos<R> pos;
+    Vel<R> vel;
+};
+
+template <class R>
+struct VcPoint {
+    V2<R> relA, relB;
+    R ni = 0, ti = 0, normalMass = 0, tangentMass = 0, velocityBias = 0;
+};
+
+template <class R>
+struct VelocityConstraint {
+    std::uint32_t bodyA = 0, bodyB = 0, n = 0;
+    V2<R> normal;
+    R friction = 0, restitution = 0, tangentSpeed = 0;
+    VcPoint<R> p[2];
+    R K[3] = {0, 0, 0};           // k00, k11, k01
+    R normalMass[3] = {0, 0, 0};  // inverse of K (m00, m11, m01)
+    V2<R> tangent() const { return FwdPerp(normal); }
+};
+
+template <class R>
+struct PositionConstraint {
+    Manifold<R> m;
+    std::uint32_t bodyA, bodyB;
+    R totalRadius;
+};
+
+template <class R>
+class World {
+public:
+    std::vector<Shape<R>> shapes;
+    std::vector<Body<R>> bodies;
+    std::vector<std::uint32_t> bodyOrder;  // m_bodies
+    std::vector<Leaf<R>> tree;             // by proxy (leaf) id
+    std::uint32_t leafCount = 0;
+    std::vector<std::pair<std::uint32_t, std::uint32_t>> fixturesForProxies;  // (body, shape)
+    std::vector<std::uint32_t> proxiesForContacts;
+    // Contact pool (ObjectPool.hpp:62-116) with manifolds at the same index.
+    std::vector<Contact<R>> contactBuf;
+    std::vector<Manifold<R>> manifoldBuf;
+    std::vector<std::uint32_t> contactFree;
+    // m_contacts and per-body contact lists: (kmin, kmax, id).
+    struct KeyedId {
+        std::uint32_t kmin, kmax, id;
+    };
+    std::vector<KeyedId> contacts;
+    std::vector<std::vector<KeyedId>> bodyContacts;
+    std::vector<char> islandedBodies, islandedContacts;
+
+    // Leaf id allocation of DynamicTree (DynamicTree.cpp:456-537) on a tree
+    // that only ever grows: the first leaf takes node 0, every later leaf the
+    // next free node, followed by its new parent node.
+    std::uint32_t AllocLeafId() {
+        const std::uint32_t id = (leafCount == 0) ? 0u : 2u * leafCount - 1u;
+        ++leafCount;
+        if (tree.size() <= id) tree.resize(id + 1);
+        return id;
+    }
+
+    // ComputeAABB(proxy, xf0, xf1) (AABB.cpp:51-62) + GetFattenedAABB +
+    // GetDisplacedAABB (detail/AABB.hpp:305-319, Interval.hpp:186-220).
+    void FatAabb(const Proxy<R>& px, const Xf<R>& xf0, const Xf<R>& xf1, const StepConf<R>& conf,
+                 R lo[2], R hi[2], bool fatten) const {
+        R mn[2] = {std::numeric_limits<R>::infinity(), std::numeric_limits<R>::infinity()};
+        R mx[2] = {-std::numeric_limits<R>::infinity(), -std::numeric_limits<R>::infinity()};
+        auto include = [&](V2<R> p) {
+            mn[0] = Min(p.x, mn[0]);
+            mx[0] = Max(p.x, mx[0]);
+            mn[1] = Min(p.y, mn[1]);
+            mx[1] = Max(p.y, mx[1]);
+        };
+        for (std::uint32_t i = 0; i < px.n; ++i) {
+            include(Transform(px.v[i], xf0));
+            include(Transform(px.v[i], xf1));
+        }
+        for (int d = 0; d < 2; ++d) {
+            mn[d] = mn[d] - px.radius;
+            mx[d] = mx[d] + px.radius;
+        }
+        if (fatten) {
+            for (int d = 0; d < 2; ++d) {
+                mn[d] = mn[d] - conf.aabbExtension;
+                mx[d] = mx[d] + conf.aabbExtension;
+            }
+            const V2<R> disp = conf.displaceMultiplier * (xf1.p - xf0.p);
+            const R dd[2] = {disp.x, disp.y};
+            for (int d = 0; d < 2; ++d) {
+                if (dd[d] < R(0)) mn[d] += dd[d];
+                else mx[d] += dd[d];
+            }
+        }
+        for (int d = 0; d < 2; ++d) {
+            lo[d] = mn[d];
+            hi[d] = mx[d];
+        }
+    }
+
+    static bool Overlap(const R alo[2], const R ahi[2], const R blo[2], const R bhi[2]) {
+        for (int d = 0; d < 2; ++d) {
+            const R maxOfMins = Max(alo[d], blo[d]);
+            const R minOfMaxs = Min(ahi[d], bhi[d]);
+            if (!(minOfMaxs >= maxOfMins)) return false;
+        }
+        return true;
+    }
+
+    Xf<R> Xf0(const Body<R>& b) const { return GetXf(b.pos0, b.lc); }
+
+    // CreateProxies (AabbTreeWorld.cpp:557-579)
+    void CreateProxies(std::uint32_t bodyId, std::uint32_t shapeId, const StepConf<R>& conf) {
+        Body<R>& b = bodies[bodyId];
+        const Xf<R> xf0 = Xf0(b);
+        const Xf<R> xf1 = b.xf;
+        const std::uint32_t id = AllocLeafId();
+        Leaf<R>& lf = tree[id];
+        lf.used = true;
+        lf.body = bodyId;
+        lf.shape = shapeId;
+        lf.child = 0;
+        FatAabb(shapes[shapeId].proxy(), xf0, xf1, conf, lf.lo, lf.hi, true);
+        b.proxies.push_back(id);
+        proxiesForContacts.push_back(id);
+    }
+
+    // Synchronize (AabbTreeWorld.cpp:2564-2585)
+    std::uint32_t Synchronize(std::uint32_t bodyId, const StepConf<R>& conf) {
+        Body<R>& b = bodies[bodyId];
+        const Xf<R> xf0 = Xf0(b);
+        const Xf<R> xf1 = b.xf;
+        std::uint32_t moved = 0;
+        for (std::uint32_t id : b.proxies) {
+            Leaf<R>& lf = tree[id];
+            R lo[2], hi[2];
+            FatAabb(shapes[lf.shape].proxy(), xf0, xf1, conf, lo, hi, false);
+            const bool contains = lf.lo[0] <= lo[0] && lf.hi[0] >= hi[0] && lf.lo[1] <= lo[1] && lf.hi[1] >= hi[1];
+            if (!contains) {
+                FatAabb(shapes[lf.shape].proxy(), xf0, xf1, conf, lf.lo, lf.hi, true);
+                proxiesForContacts.push_back(id);
+                ++moved;
+            }
+        }
+        return moved;
+    }
+
+    // InternalDestroy (AabbTreeWorld.cpp:2263-2288), called by DestroyContacts.
+    void InternalDestroy(std::uint32_t cid) {
+        Contact<R>& c = contactBuf[cid];
+        auto eraseFrom = [&](std::uint32_t body) {
+            auto& lst = bodyContacts[body];
+            for (auto it = lst.begin(); it != lst.end(); ++it) {
+                if (it->id == cid) {
+                    lst.erase(it);
+                    break;
+                }
+            }
+        };
+        eraseFrom(c.bodyA);
+        eraseFrom(c.bodyB);
+        if (manifoldBuf[cid].n > 0 && !(c.flags & kSensor)) {
+            bodies[c.bodyA].SetAwake();
+            bodies[c.bodyB].SetAwake();
+        }
+        contactFree.push_back(cid);
+        contactBuf[cid] = Contact<R>{};
+        manifoldBuf[cid] = Manifold<R>{};
+    }
+
+    // DestroyContacts (AabbTreeWorld.cpp:2290-2331); filtering is never flagged
+    // by the operations this restatement supports.
+    std::uint32_t DestroyContacts() {
+        std::uint32_t destroyed = 0;
+        std::vector<KeyedId> kept;
+        kept.reserve(contacts.size());
+        for (const auto& kc : contacts) {
+            const Leaf<R>& a = tree[kc.kmin];
+            const Leaf<R>& b = tree[kc.kmax];
+            if (!Overlap(a.lo, a.hi, b.lo, b.hi)) {
+                InternalDestroy(kc.id);
+                ++destroyed;
+            } else {
+                kept.push_back(kc);
+            }
+        }
+        contacts.swap(kept);
+        return destroyed;
+    }
+
+    // Update (AabbTreeWorld.cpp:2587-2721) for a non-sensor contact.
+    void Update(std::uint32_t cid, const StepConf<R>& conf) {
+        Contact<R>& c = contactBuf[cid];
+        Manifold<R>& manifold = manifoldBuf[cid];
+        const Manifold<R> old = manifold;
+        const bool oldTouching = (c.flags & kTouching) != 0;
+        bool newTouching = false;
+        const Body<R>& bA = bodies[c.bodyA];
+        const Body<R>& bB = bodies[c.bodyB];
+        if (c.flags & kSensor) {
+            // Sensors are not produced by the supported scene API.
+            manifold = Manifold<R>{};
+        } else {
+            Manifold<R> nm = CollideShapes(shapes[c.shapeA].proxy(), bA.xf, shapes[c.shapeB].proxy(), bB.xf,
+                                           conf.linearSlop, conf.maxCirclesRatio);
+            const std::uint32_t oldN = old.n, newN = nm.n;
+            newTouching = newN > 0;
+            bool found[2] = {false, newN < 2};
+            for (std::uint32_t i = 0; i < newN; ++i) {
+                for (std::uint32_t j = 0; j < oldN; ++j) {
+                    if (nm.pts[i].cf == old.pts[j].cf) {
+                        found[i] = true;
+                        nm.pts[i].ni = old.pts[j].ni;
+                        nm.pts[i].ti = old.pts[j].ti;
+                        break;
+                    }
+                }
+            }
+            for (std::uint32_t i = 0; i < newN; ++i) {
+                if (!found[i]) {
+                    R least = std::numeric_limits<R>::infinity();
+                    for (std::uint32_t j = 0; j < oldN; ++j) {
+                        const R d = MagSq(old.pts[j].lp - nm.pts[i].lp);
+                        if (least > d) {
+                            least = d;
+                            nm.pts[i].ni = old.pts[j].ni;
+                            nm.pts[i].ti = old.pts[j].ti;
+                        }
+                    }
+                }
+            }
+            manifold = nm;
+        }
+        c.flags &= ~kNeedsUpdating;
+        if (!oldTouching && newTouching) c.flags |= kTouching;
+        else if (oldTouching && !newTouching) c.flags &= ~kTouching;
+    }
+
+    // UpdateContacts (AabbTreeWorld.cpp:2333-2421)
+    std::uint32_t UpdateContacts(const StepConf<R>& conf) {
+        std::uint32_t updated = 0;
+        for (const auto& kc : contacts) {
+            if (contactBuf[kc.id].flags & kNeedsUpdating) {
+                Update(kc.id, conf);
+                ++updated;
+            }
+        }
+        return updated;
+    }
+
+    // FindContacts (AabbTreeWorld.cpp:709-751) + AddContacts (:2424-2547).
+    std::uint32_t FindAndAddContacts(const StepConf<R>& conf) {
+        struct PK {
+            std::uint32_t kmin, kmax;
+        };
+        std::vector<PK> keys;
+        for (std::uint32_t pid : proxiesForContacts) {
+            const Leaf<R>& l0 = tree[pid];
+            for (std::uint32_t nid = 0; nid < tree.size(); ++nid) {
+                const Leaf<R>& l1 = tree[nid];
+                if (!l1.used) continue;
+                if (!Overlap(l0.lo, l0.hi, l1.lo, l1.hi)) continue;
+                if (nid != pid && l0.body != l1.body) {
+                    keys.push_back(PK{std::min(pid, nid), std::max(pid, nid)});
+                }
+            }
+        }
+        proxiesForContacts.clear();
+        std::sort(keys.begin(), keys.end(), [](const PK& a, const PK& b) {
+            return (a.kmin < b.kmin) || (a.kmin == b.kmin && a.kmax < b.kmax);
+        });
+        keys.erase(std::unique(keys.begin(), keys.end(),
+                               [](const PK& a, const PK& b) { return a.kmin == b.kmin && a.kmax == b.kmax; }),
+                   keys.end());
+        std::uint32_t added = 0;
+        for (const PK& k : keys) {
+            const Leaf<R>& la = tree[k.kmin];
+            const Leaf<R>& lb = tree[k.kmax];
+            Body<R>& bA = bodies[la.body];
+            Body<R>& bB = bodies[lb.body];
+            const Shape<R>& sA = shapes[la.shape];
+            const Shape<R>& sB = shapes[lb.shape];
+            if (!(bA.accelerable() || bB.accelerable())) continue;
+            // ShouldCollide(shapes) (Filter.hpp:81-88); joints are not modelled.
+            {
+                const std::int32_t gA = sA.group, gB = sB.group;
+                bool collide;
+                if (gA == gB && gA != 0) {
+                    collide = gA > 0;
+                } else {
+                    const std::uint32_t catA = sA.catmask & 0xffffu, maskA = sA.catmask >> 16;
+                    const std::uint32_t catB = sB.catmask & 0xffffu, maskB = sB.catmask >> 16;
+                    collide = ((maskA & catB) != 0) && ((maskB & catA) != 0);
+                }
+                if (!collide) continue;
+            }
+            auto& lA = bodyContacts[la.body];
+            auto& lB = bodyContacts[lb.body];
+            const auto& search = (lA.size() < lB.size()) ? lA : lB;
+            bool exists = false;
+            for (const auto& e : search) {
+                if (e.kmin == k.kmin && e.kmax == k.kmax) {
+                    exists = true;
+                    break;
+                }
+            }
+            if (exists) continue;
+            std::uint32_t cid;
+            if (!contactFree.empty()) {
+                cid = contactFree.back();
+                contactFree.pop_back();
+            } else {
+                cid = std::uint32_t(contactBuf.size());
+                contactBuf.emplace_back();
+                manifoldBuf.emplace_back();
+            }
+            Contact<R>& c = contactBuf[cid];
+            c = Contact<R>{};
+            manifoldBuf[cid] = Manifold<R>{};
+            c.kmin = k.kmin;
+            c.kmax = k.kmax;
+            c.bodyA = la.body;
+            c.bodyB = lb.body;
+            c.shapeA = la.shape;
+            c.shapeB = lb.shape;
+            c.flags = kCEnabled | kNeedsUpdating;
+            if ((bA.flags & kImpenetrable) || (bB.flags & kImpenetrable)) c.flags |= kCImpenetrable;
+            if (sA.sensor || sB.sensor) c.flags |= kSensor;
+            c.friction = std::sqrt(sA.friction * sB.friction);                       // Contact.hpp:57-60
+            c.restitution = (sA.restitution > sB.restitution) ? sA.restitution : sB.restitution;  // :66-69
+            contacts.push_back(KeyedId{k.kmin, k.kmax, cid});
+            lA.push_back(KeyedId{k.kmin, k.kmax, cid});
+            lB.push_back(KeyedId{k.kmin, k.kmax, cid});
+            if (!(c.flags & kSensor)) {
+                if (bA.speedable()) bA.SetAwakeFlag();
+                if (bB.speedable()) bB.SetAwakeFlag();
+            }
+            Update(cid, conf);
+            ++added;
+        }
+        return added;
+    }
+
+    // GetVelocity(body, h) (Body.cpp:290-309) and Cap (Velocity.cpp:27-45).
+    Vel<R> IntegratedVelocity(const Body<R>& b, const StepConf<R>& conf) const {
+        Vel<R> v = b.vel;
+        const R h = conf.dt;
+        if (b.accelerable()) {
+            v.lin = v.lin + h * b.lacc;
+            v.ang = v.ang + h * b.aacc;
+            const R ldiv = R(1) + h * b.ldamp;
+            const R linv = R(1) / ldiv;
+            v.lin = V2<R>{v.lin.x * linv, v.lin.y * linv};
+            v.ang = v.ang / (R(1) + h * b.adamp);
+        }
+        const V2<R> translation = h * v.lin;
+        const R lsq = MagSq(translation);
+        if (lsq > conf.maxTranslation * conf.maxTranslation) {
+            const R ratio = conf.maxTranslation / std::sqrt(lsq);
+            v.lin = V2<R>{v.lin.x * ratio, v.lin.y * ratio};
+        }
+        const R absRot = std::abs(h * v.ang);
+        if (absRot > conf.maxRotation) {
+            const R ratio = conf.maxRotation / absRot;
+            v.ang = v.ang * ratio;
+        }
+        return v;
+    }
+
+    // GetContactRelVelocity (d2/Math.cpp:73-103)
+    static V2<R> RelVel(const Vel<R>& vA, V2<R> relA, const Vel<R>& vB, V2<R> relB) {
+        const V2<R> velBrot = RevPerp(relB) * (vB.ang / R(1));
+        const V2<R> velArot = RevPerp(relA) * (vA.ang / R(1));
+        return (vB.lin + velBrot) - (vA.lin + velArot);
+    }
+
+    // VelocityConstraint ctor + GetPoint + ComputeK (VelocityConstraint.cpp:38-185)
+    VelocityConstraint<R> MakeVC(const Contact<R>& c, const WorldManifold<R>& wm,
+                                 const std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) const {
+        VelocityConstraint<R> vc;
+        vc.bodyA = c.bodyA;
+        vc.bodyB = c.bodyB;
+        vc.normal = wm.normal;
+        vc.friction = c.friction;
+        vc.restitution = c.restitution;
+        vc.tangentSpeed = c.tangentSpeed;
+        const BodyConstraint<R>& A = bcs[c.bodyA];
+        const BodyConstraint<R>& B = bcs[c.bodyB];
+        const R dtRatio = conf.warmStart ? conf.dtRatio : R(0);
+        const R invMass = A.invMass + B.invMass;
+        for (std::uint32_t j = 0; j < wm.n; ++j) {
+            VcPoint<R>& p = vc.p[vc.n];
+            const V2<R> relA = wm.p[j] - A.pos.lin;
+            const V2<R> relB = wm.p[j] - B.pos.lin;
+            p.ni = wm.ni[j] * dtRatio;
+            p.ti = wm.ti[j] * dtRatio;
+            {
+                const V2<R> dv = RelVel(A.vel, relA, B.vel, relB);
+                const R vn = Dot(dv, vc.normal);
+                p.velocityBias = (vn < -conf.velocityThreshold) ? -vc.restitution * vn : R(0);
+            }
+            p.relA = relA;
+            p.relB = relB;
+            {
+                const R cA = Cross(relA, vc.normal);
+                const R cB = Cross(relB, vc.normal);
+                const R rmA = A.invRotI * (cA * cA) / R(1);
+                const R rmB = B.invRotI * (cB * cB) / R(1);
+                const R value = invMass + rmA + rmB;
+                p.normalMass = (value != R(0)) ? R(1) / value : R(0);
+            }
+            {
+                const V2<R> t = vc.tangent();
+                const R cA = Cross(relA, t);
+                const R cB = Cross(relB, t);
+                const R rmA = A.invRotI * (cA * cA) / R(1);
+                const R rmB = B.invRotI * (cB * cB) / R(1);
+                const R value = invMass + rmA + rmB;
+                p.tangentMass = (value != R(0)) ? R(1) / value : R(0);
+            }
+            ++vc.n;
+        }
+        if (conf.blockSolve && vc.n == 2) {
+            const R rnA0 = Cross(vc.p[0].relA, vc.normal);
+            const R rnB0 = Cross(vc.p[0].relB, vc.normal);
+            const R rnA1 = Cross(vc.p[1].relA, vc.normal);
+            const R rnB1 = Cross(vc.p[1].relB, vc.normal);
+            const R irmA0 = (A.invRotI * (rnA0 * rnA0)) / R(1);
+            const R irmA1 = (A.invRotI * (rnA1 * rnA1)) / R(1);
+            const R irmB0 = (B.invRotI * (rnB0 * rnB0)) / R(1);
+            const R irmB1 = (B.invRotI * (rnB1 * rnB1)) / R(1);
+            const R irmA = (A.invRotI * rnA0 * rnA1) / R(1);
+            const R irmB = (B.invRotI * rnB0 * rnB1) / R(1);
+            const R k00 = invMass + irmA0 + irmB0;
+            const R k11 = invMass + irmA1 + irmB1;
+            const R k01 = invMass + irmA + irmB;
+            const R k00sq = k00 * k00;
+            const R kdiff = k00 * k11 - k01 * k01;
+            if (k00sq < R(1000) * kdiff) {
+                vc.K[0] = k00;
+                vc.K[1] = k11;
+                vc.K[2] = k01;
+                // Invert (Math.hpp:428-442) of [[k00, k01], [k01, k11]].
+                const R cp = k00 * k11 - k01 * k01;
+                if (!AlmostZero(cp)) {
+                    const R inv = R(1) / cp;
+                    vc.normalMass[0] = k11 * inv;
+                    vc.normalMass[1] = k00 * inv;
+                    vc.normalMass[2] = -k01 * inv;
+                }
+            } else {
+                vc.n = 1;  // RemovePoint
+            }
+        }
+        return vc;
+    }
+
+    // SolveTangentConstraint / SeqSolveNormalConstraint (ContactSolver.cpp:337-454)
+    static R SolveDirectional(VelocityConstraint<R>& vc, std::vector<BodyConstraint<R>>& bcs, bool tangent) {
+        R maxInc = 0;
+        BodyConstraint<R>& A = bcs[vc.bodyA];
+        BodyConstraint<R>& B = bcs[vc.bodyB];
+        const V2<R> dir = tangent ? vc.tangent() : vc.normal;
+        Vel<R> vA = A.vel, vB = B.vel;
+        auto proc = [&](std::uint32_t i) {
+            VcPoint<R>& p = vc.p[i];
+            const V2<R> closing = RelVel(vA, p.relA, vB, p.relB);
+            R oldImpulse, newImpulse;
+            if (tangent) {
+                const R dirVel = vc.tangentSpeed - Dot(closing, dir);
+                const R lambda = p.tangentMass * dirVel;
+                const R maxImpulse = vc.friction * p.ni;
+                oldImpulse = p.ti;
+                newImpulse = Clamp(oldImpulse + lambda, -maxImpulse, maxImpulse);
+            } else {
+                const R dirVel = Dot(closing, dir);
+                const R lambda = p.normalMass * (p.velocityBias - dirVel);
+                oldImpulse = p.ni;
+                newImpulse = Max(oldImpulse + lambda, R(0));
+            }
+            const R inc = newImpulse - oldImpulse;
+            const V2<R> P = inc * dir;
+            const R LA = Cross(p.relA, P) / R(1);
+            const R LB = Cross(p.relB, P) / R(1);
+            vA.lin = vA.lin - A.invMass * P;
+            vA.ang = vA.ang - A.invRotI * LA;
+            vB.lin = vB.lin + B.invMass * P;
+            vB.ang = vB.ang + B.invRotI * LB;
+            maxInc = Max(maxInc, std::abs(inc));
+            if (tangent) p.ti = oldImpulse + inc;
+            else p.ni = oldImpulse + inc;
+        };
+        if (vc.n == 2) proc(1);
+        proc(0);
+        A.vel = vA;
+        B.vel = vB;
+        return maxInc;
+    }
+
+    // BlockSolveNormalConstraint and its cases (ContactSolver.cpp:65-335)
+    static R BlockSolve(VelocityConstraint<R>& vc, std::vector<BodyConstraint<R>>& bcs) {
+        BodyConstraint<R>& A = bcs[vc.bodyA];
+        BodyConstraint<R>& B = bcs[vc.bodyB];
+        const V2<R> normal = vc.normal;
+        const R k00 = vc.K[0], k11 = vc.K[1], k01 = vc.K[2];
+        V2<R> bprime;
+        {
+            const V2<R> dv0 = RelVel(A.vel, vc.p[0].relA, B.vel, vc.p[0].relB);
+            const V2<R> dv1 = RelVel(A.vel, vc.p[1].relA, B.vel, vc.p[1].relB);
+            const R vn0 = Dot(dv0, normal);
+            const R vn1 = Dot(dv1, normal);
+            const V2<R> b{vn0 - vc.p[0].velocityBias, vn1 - vc.p[1].velocityBias};
+            // Transform(impulses, K) = K * impulses with K rows {k00,k01},{k01,k11}
+            // (Vector.hpp:458-470); InvMass3 unpacks to a 2x2 (Math.hpp).
+            R t0 = 0;
+            t0 += k00 * vc.p[0].ni;
+            t0 += k01 * vc.p[1].ni;
+            R t1 = 0;
+            t1 += k01 * vc.p[0].ni;
+            t1 += k11 * vc.p[1].ni;
+            bprime = V2<R>{b.x - t0, b.y - t1};
+        }
+        auto update = [&](R n0, R n1) {
+            // BlockSolveUpdate + GetVelocityDelta (ContactSolver.cpp:65-111)
+            const R d0 = n0 - vc.p[0].ni;
+            const R d1 = n1 - vc.p[1].ni;
+            const V2<R> P0 = d0 * normal;
+            const V2<R> P1 = d1 * normal;
+            const V2<R> P = P0 + P1;
+            const R LA = (Cross(vc.p[0].relA, P0) + Cross(vc.p[1].relA, P1)) / R(1);
+            const R LB = (Cross(vc.p[0].relB, P0) + Cross(vc.p[1].relB, P1)) / R(1);
+            const V2<R> dvAlin = -(A.invMass * P);
+            const R dvAang = -(A.invRotI * LA);
+            const V2<R> dvBlin = B.invMass * P;
+            const R dvBang = B.invRotI * LB;
+            A.vel.lin = A.vel.lin + dvAlin;
+            A.vel.ang = A.vel.ang + dvAang;
+            B.vel.lin = B.vel.lin + dvBlin;
+            B.vel.ang = B.vel.ang + dvBang;
+            vc.p[0].ni = n0;
+            vc.p[1].ni = n1;
+            return Max(std::abs(n0), std::abs(n1));
+        };
+        {
+            // Case 1: -Transform(b', normalMass) (ContactSolver.cpp:113-141)
+            const R m00 = vc.normalMass[0], m11 = vc.normalMass[1], m01 = vc.normalMass[2];
+            R x0 = 0;
+            x0 += m00 * bprime.x;
+            x0 += m01 * bprime.y;
+            R x1 = 0;
+            x1 += m01 * bprime.x;
+            x1 += m11 * bprime.y;
+            const R n0 = -x0, n1 = -x1;
+            if (n0 >= R(0) && n1 >= R(0)) return update(n0, n1);
+        }
+        {
+            // Case 2 (ContactSolver.cpp:143-172)
+            const R n0 = -vc.p[0].normalMass * bprime.x;
+            const R vn2 = k01 * n0 + bprime.y;
+            if (n0 >= R(0) && vn2 >= R(0)) return update(n0, R(0));
+        }
+        {
+            // Case 3 (ContactSolver.cpp:174-203)
+            const R n1 = -vc.p[1].normalMass * bprime.y;
+            const R vn1 = k01 * n1 + bprime.x;
+            if (n1 >= R(0) && vn1 >= R(0)) return update(R(0), n1);
+        }
+        {
+            // Case 4 (ContactSolver.cpp:205-216)
+            if (bprime.x >= R(0) && bprime.y >= R(0)) return update(R(0), R(0));
+        }
+        return R(0);
+    }
+
+    // GaussSeidel::SolveVelocityConstraint (ContactSolver.cpp:456-492)
+    static R SolveVC(VelocityConstraint<R>& vc, std::vector<BodyConstraint<R>>& bcs) {
+        R maxInc = 0;
+        maxInc = Max(maxInc, SolveDirectional(vc, bcs, true));
+        R n;
+        if (vc.n == 1 || (vc.K[0] == R(0) && vc.K[1] == R(0) && vc.K[2] == R(0))) {
+            n = SolveDirectional(vc, bcs, false);
+        } else {
+            n = BlockSolve(vc, bcs);
+        }
+        maxInc = Max(maxInc, n);
+        return maxInc;
+    }
+
+    // GaussSeidel::SolvePositionConstraint (ContactSolver.cpp:494-643)
+    static R SolvePC(const PositionConstraint<R>& pc, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        BodyConstraint<R>& A = bcs[pc.bodyA];
+        BodyConstraint<R>& B = bcs[pc.bodyB];
+        const R invMassA = A.invMass, invRotIA = A.invRotI;
+        const R invMassB = B.invMass, invRotIB = B.invRotI;
+        const V2<R> lcA = A.lc, lcB = B.lc;
+        const R invMassTotal = invMassA + invMassB;
+        const R totalRadius = pc.totalRadius;
+        struct Sol {
+            Pos<R> a, b;
+            R sep;
+        };
+        auto solve = [&](const PSM<R>& psm, V2<R> pA, V2<R> pB) {
+            const R separation = psm.sep - totalRadius;
+            const V2<R> rA = psm.point - pA;
+            const V2<R> rB = psm.point - pB;
+            const R crA = Cross(rA, psm.normal) / R(1);
+            const R crB = Cross(rB, psm.normal) / R(1);
+            const R K = invMassTotal + invRotIA * (crA * crA) + invRotIB * (crB * crB);
+            const R C = -Clamp(conf.regResolutionRate * (separation + conf.linearSlop), -conf.maxLinearCorrection, R(0));
+            const V2<R> P = (psm.normal * C) / K;
+            const R LA = Cross(rA, P) / R(1);
+            const R LB = Cross(rB, P) / R(1);
+            return Sol{-Pos<R>{invMassA * P, invRotIA * LA}, Pos<R>{invMassB * P, invRotIB * LB}, separation};
+        };
+        Pos<R> posA = A.pos, posB = B.pos;
+        R minSep = std::numeric_limits<R>::infinity();
+        if (pc.m.n == 1) {
+            const PSM<R> psm0 = GetPSM(pc.m, 0, GetXf(posA, lcA), GetXf(posB, lcB));
+            const Sol s = solve(psm0, posA.lin, posB.lin);
+            posA = posA + s.a;
+            posB = posB + s.b;
+            minSep = R(0) + s.sep;
+        } else if (pc.m.n == 2) {
+            const Xf<R> xfA = GetXf(posA, lcA);
+            const Xf<R> xfB = GetXf(posB, lcB);
+            const PSM<R> psm0 = GetPSM(pc.m, 0, xfA, xfB);
+            const PSM<R> psm1 = GetPSM(pc.m, 1, xfA, xfB);
+            if (AlmostEqual(psm0.sep, psm1.sep)) {
+                const Sol s0 = solve(psm0, posA.lin, posB.lin);
+                const Sol s1 = solve(psm1, posA.lin, posB.lin);
+                posA = posA + s0.a + s1.a;
+                posB = posB + s0.b + s1.b;
+                minSep = s0.sep;
+            } else if (psm0.sep < psm1.sep) {
+                const Sol s0 = solve(psm0, posA.lin, posB.lin);
+                posA = posA + s0.a;
+                posB = posB + s0.b;
+                const PSM<R> p1 = GetPSM(pc.m, 1, GetXf(posA, lcA), GetXf(posB, lcB));
+                const Sol s1 = solve(p1, posA.lin, posB.lin);
+                posA = posA + s1.a;
+                posB = posB + s1.b;
+                minSep = s0.sep;
+            } else if (psm1.sep < psm0.sep) {
+                const Sol s1 = solve(psm1, posA.lin, posB.lin);
+                posA = posA + s1.a;
+                posB = posB + s1.b;
+                const PSM<R> p0 = GetPSM(pc.m, 0, GetXf(posA, lcA), GetXf(posB, lcB));
+                const Sol s0 = solve(p0, posA.lin, posB.lin);
+                posA = posA + s0.a;
+                posB = posB + s0.b;
+                minSep = s1.sep;
+            }
+        }
+        A.pos = posA;
+        B.pos = posB;
+        return minSep;
+    }
+
+    struct Island {
+        std::vector<std::uint32_t> bodies, contacts;
+    };
+
+    // AddToIsland (AabbTreeWorld.cpp:1358-1444)
+    void AddToIsland(Island& island, std::uint32_t seed) {
+        std::vector<std::uint32_t> stack;
+        stack.push_back(seed);
+        islandedBodies[seed] = 1;
+        while (!stack.empty()) {
+            const std::uint32_t bid = stack.back();
+            stack.pop_back();
+            Body<R>& b = bodies[bid];
+            island.bodies.push_back(bid);
+            if (!b.speedable()) continue;
+            b.SetAwakeFlag();
+            for (const auto& kc : bodyContacts[bid]) {
+                const std::uint32_t cid = kc.id;
+                if (islandedContacts[cid]) continue;
+                const Contact<R>& c = contactBuf[cid];
+                if ((c.flags & kCEnabled) && (c.flags & kTouching) && !(c.flags & kSensor)) {
+                    const std::uint32_t other = (c.bodyA == bid) ? c.bodyB : c.bodyA;
+                    island.contacts.push_back(cid);
+                    islandedContacts[cid] = 1;
+                    if (!islandedBodies[other]) {
+                        islandedBodies[other] = 1;
+                        stack.push_back(other);
+                    }
+                }
+            }
+        }
+    }
+
+    // SolveRegIslandViaGS (AabbTreeWorld.cpp:1563-1682)
+    std::uint32_t SolveIsland(const Island& island, const StepConf<R>& conf) {
+        const R h = conf.dt;
+        for (std::uint32_t bid : island.bodies) bodies[bid].pos0 = bodies[bid].pos1;
+        std::vector<BodyConstraint<R>> bcs(bodies.size());
+        for (std::uint32_t bid : island.bodies) {
+            const Body<R>& b = bodies[bid];
+            bcs[bid] = BodyConstraint<R>{b.invMass, b.invRotI, b.lc, b.pos1, IntegratedVelocity(b, conf)};
+        }
+        std::vector<PositionConstraint<R>> pcs;
+        std::vector<VelocityConstraint<R>> vcs;
+        for (std::uint32_t cid : island.contacts) {
+            const Contact<R>& c = contactBuf[cid];
+            const R rA = shapes[c.shapeA].radius, rB = shapes[c.shapeB].radius;
+            pcs.push_back(PositionConstraint<R>{manifoldBuf[cid], c.bodyA, c.bodyB, rA + rB});
+        }
+        for (std::uint32_t cid : island.contacts) {
+            const Contact<R>& c = contactBuf[cid];
+            const R rA = shapes[c.shapeA].radius, rB = shapes[c.shapeB].radius;
+            const Xf<R> xfA = GetXf(bcs[c.bodyA].pos, bcs[c.bodyA].lc);
+            const Xf<R> xfB = GetXf(bcs[c.bodyB].pos, bcs[c.bodyB].lc);
+            const WorldManifold<R> wm = GetWorldManifold(manifoldBuf[cid], xfA, rA, xfB, rB);
+            vcs.push_back(MakeVC(c, wm, bcs, conf));
+        }
+        if (conf.warmStart) {
+            // WarmStartVelocities + CalcWarmStartVelocityDeltas (AabbTreeWorld.cpp:230-274)
+            for (const auto& vc : vcs) {
+                BodyConstraint<R>& A = bcs[vc.bodyA];
+                BodyConstraint<R>& B = bcs[vc.bodyB];
+                Vel<R> dA{}, dB{};
+                for (std::uint32_t j = 0; j < vc.n; ++j) {
+                    const VcPoint<R>& p = vc.p[j];
+                    const V2<R> P = p.ni * vc.normal + p.ti * vc.tangent();
+                    const R LA = Cross(p.relA, P) / R(1);
+                    const R LB = Cross(p.relB, P) / R(1);
+                    dA.lin = dA.lin - A.invMass * P;
+                    dA.ang = dA.ang - A.invRotI * LA;
+                    dB.lin = dB.lin + B.invMass * P;
+                    dB.ang = dB.ang + B.invRotI * LB;
+                }
+                A.vel.lin = A.vel.lin + dA.lin;
+                A.vel.ang = A.vel.ang + dA.ang;
+                B.vel.lin = B.vel.lin + dB.lin;
+                B.vel.ang = B.vel.ang + dB.ang;
+            }
+        }
+        for (std::uint32_t i = 0; i < conf.velIters; ++i) {
+            R newInc = 0;
+            for (auto& vc : vcs) newInc = Max(newInc, SolveVC(vc, bcs));
+            if (newInc <= conf.regMinMomentum) break;
+        }
+        for (std::uint32_t bid : island.bodies) {
+            BodyConstraint<R>& bc = bcs[bid];
+            const V2<R> tr = h * bc.vel.lin;
+            const R rot = h * bc.vel.ang;
+            bc.pos = bc.pos + Pos<R>{tr, rot};
+        }
+        bool solved = false;
+        for (std::uint32_t i = 0; i < conf.posIters; ++i) {
+            R minSep = std::numeric_limits<R>::infinity();
+            for (const auto& pc : pcs) minSep = Min(minSep, SolvePC(pc, bcs, conf));
+            if (minSep >= conf.regMinSeparation) {
+                solved = true;
+                break;
+            }
+        }
+        for (std::size_t i = 0; i < vcs.size(); ++i) {
+            Manifold<R>& m = manifoldBuf[island.contacts[i]];
+            for (std::uint32_t j = 0; j < vcs[i].n; ++j) {
+                m.pts[j].ni = vcs[i].p[j].ni;
+                m.pts[j].ti = vcs[i].p[j].ti;
+            }
+        }
+        for (std::uint32_t bid : island.bodies) {
+            Body<R>& b = bodies[bid];
+            const BodyConstraint<R>& bc = bcs[bid];
+            b.vel = bc.vel;
+            if (b.pos1 != bc.pos) {
+                b.pos1 = bc.pos;
+                b.xf = GetXf(b.pos1, b.lc);
+                for (const auto& kc : bodyContacts[bid]) contactBuf[kc.id].flags |= kNeedsUpdating;
+            }
+        }
+        // UpdateUnderActiveTimes (AabbTreeWorld.cpp:403-417)
+        R minUnder = std::numeric_limits<R>::infinity();
+        for (std::uint32_t bid : island.bodies) {
+            Body<R>& b = bodies[bid];
+            if (!b.speedable()) continue;
+            const R linSq = MagSq(b.vel.lin);
+            const R angSq = b.vel.ang * b.vel.ang;
+            const bool under = (angSq <= conf.angularSleepTolerance * conf.angularSleepTolerance) &&
+                               (linSq <= conf.linearSleepTolerance * conf.linearSleepTolerance);
+            const bool sleepable = (b.flags & kAutoSleep) != 0;
+            const R t = (sleepable && under) ? b.underActive + conf.dt : R(0);
+            // Body::SetUnderActiveTime (Body.hpp:640-645)
+            if (t == R(0) || b.accelerable()) b.underActive = t;
+            minUnder = Min(minUnder, t);
+        }
+        std::uint32_t slept = 0;
+        if (minUnder >= conf.minStillTimeToSleep && solved) {
+            for (std::uint32_t bid : island.bodies) slept += bodies[bid].Unawaken() ? 1u : 0u;
+        }
+        return slept;
+    }
+
+    // SolveReg (AabbTreeWorld.cpp:1472-1561)
+    void SolveReg(const StepConf<R>& conf, StepStats& st) {
+        islandedBodies.assign(bodies.size(), 0);
+        islandedContacts.assign(contactBuf.size(), 0);
+        for (std::uint32_t bid : bodyOrder) {
+            if (islandedBodies[bid]) continue;
+            Body<R>& b = bodies[bid];
+            if (b.awake() && (b.flags & kEnabled)) {
+                ++st.islandsFound;
+                Island island;
+                AddToIsland(island, bid);
+                for (std::uint32_t ib : island.bodies) {
+                    if (!bodies[ib].speedable()) islandedBodies[ib] = 0;
+                }
+                st.bodiesSlept += SolveIsland(island, conf);
+            }
+        }
+        for (std::uint32_t bid : bodyOrder) {
+            if (islandedBodies[bid] && bodies[bid].speedable()) st.regProxiesMoved += Synchronize(bid, conf);
+        }
+        islandedBodies.assign(bodies.size(), 0);
+        islandedContacts.assign(contactBuf.size(), 0);
+        UpdateContacts(conf);
+        st.regAdded = FindAndAddContacts(conf);
+    }
+
+    // Step (AabbTreeWorld.cpp:2144-2219) with doToi == false.
+    StepStats Step(const StepConf<R>& conf) {
+        StepStats st;
+        for (const auto& fs : fixturesForProxies) CreateProxies(fs.first, fs.second, conf);
+        fixturesForProxies.clear();
+        st.preDestroyed = DestroyContacts();
+        st.preUpdated = UpdateContacts(conf);
+        st.preAdded = FindAndAddContacts(conf);
+        if (conf.dt != R(0)) SolveReg(conf, st);
+        return st;
+    }
+
+    // Adds a body described as in the dump format; its shapes are queued for
+    // proxy creation as CreateBody does (AabbTreeWorld.cpp:1019-1044).
+    std::uint32_t AddBody(const Body<R>& b) {
+        const std::uint32_t id = std::uint32_t(bodies.size());
+        bodies.push_back(b);
+        bodies.back().proxies.clear();
+        bodies.back().xf = GetXf(b.pos1, b.lc);
+        bodyOrder.push_back(id);
+        bodyContacts.emplace_back();
+        if (b.flags & kEnabled) {
+            for (std::uint32_t s : b.shapes) fixturesForProxies.emplace_back(id, s);
+        }
+        return id;
+    }
+};
+
+}  // namespace oracle

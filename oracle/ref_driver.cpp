@@ -1,0 +1,309 @@
+// Golden-vector driver: builds a named scene through the REFERENCE PlayRho API
+// (float build in oracle/_ref/libplayrho_ref.a), steps it, and dumps the
+// reference's own state after every step as a raw little-endian record stream.
+// This program is test infrastructure only: it is our code, linked against the
+// reference library built from its sources by oracle/Makefile.ref.
+//
+//   ref_driver <scene> <steps> <doToi 0|1> <out.bin> [int params...]
+//
+// Record layout (all u32/f32):
+//   "PRXD" 1
+//   nshapes, per shape: nverts, radius, density, friction, restitution,
+//                       sensor, (cat|mask<<16), group, verts[2n], normals[2n]
+//   nbodies, per body:  type, flags, f32[19] (pos0 3, pos1 3, lc 2, vel 3,
+//                       linAcc 2, angAcc, linDamp, angDamp, invMass, invRotI,
+//                       underActive), nshape, shapeIds[]
+//   conf: f32/u32 StepConf fields used by the step (see WriteConf)
+//   nrecords, per record: step, stats u32[8],
+//                         B, per body f32[10] (pos0, pos1, vel, underActive), flags
+//                         C, per contact u32[4] (kmin,kmax,id,flags), u32[3]
+//                           (type|npts<<8, cf0, cf1), f32[14]
+//                         P, per proxy u32 id, f32[4]
+#include <playrho/d2/BasicAPI.hpp>
+#include <playrho/d2/World.hpp>
+#include <playrho/d2/WorldBody.hpp>
+#include <playrho/d2/WorldContact.hpp>
+#include <playrho/d2/WorldMisc.hpp>
+#include <playrho/d2/WorldShape.hpp>
+#include <playrho/d2/DynamicTree.hpp>
+#include <playrho/d2/PolygonShapeConf.hpp>
+#include <playrho/d2/DiskShapeConf.hpp>
+#include <playrho/d2/EdgeShapeConf.hpp>
+#include <playrho/StepConf.hpp>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../playrho_amd/csrc/scenes.hpp"
+
+using namespace playrho;
+using namespace playrho::d2;
+
+namespace {
+
+struct Out {
+    std::vector<unsigned char> buf;
+    void u32(std::uint32_t v) { put(&v, 4); }
+    void f32(float v) { put(&v, 4); }
+    void put(const void* p, size_t n) {
+        const auto* c = static_cast<const unsigned char*>(p);
+        buf.insert(buf.end(), c, c + n);
+    }
+};
+
+std::uint32_t PackCf(const ContactFeature& cf) {
+    return std::uint32_t(cf.typeA) | (std::uint32_t(cf.typeB) << 1) |
+           (std::uint32_t(cf.indexA) << 8) | (std::uint32_t(cf.indexB) << 16);
+}
+
+std::uint32_t BodyFlags(const Body& b) {
+    std::uint32_t f = 0;
+    if (IsSpeedable(b)) f |= 1u;
+    if (IsAccelerable(b)) f |= 2u;
+    if (IsImpenetrable(b)) f |= 4u;
+    if (IsAwake(b)) f |= 8u;
+    if (IsSleepingAllowed(b)) f |= 16u;
+    if (IsEnabled(b)) f |= 32u;
+    if (IsFixedRotation(b)) f |= 64u;
+    return f;
+}
+
+ShapeID MakeShape(World& world, const prx_scene::ShapeSpec& s) {
+    switch (s.kind) {
+    case prx_scene::kDisk: {
+        auto conf = DiskShapeConf{}.UseRadius(s.radius);
+        conf.UseDensity(s.density).UseFriction(s.friction).UseRestitution(s.restitution);
+        return CreateShape(world, conf);
+    }
+    case prx_scene::kBox: {
+        auto conf = PolygonShapeConf{};
+        if (s.vertexRadius >= 0) conf.UseVertexRadius(s.vertexRadius);
+        conf.SetAsBox(s.hx, s.hy);
+        conf.UseDensity(s.density).UseFriction(s.friction).UseRestitution(s.restitution);
+        return CreateShape(world, conf);
+    }
+    case prx_scene::kPolygon: {
+        auto conf = PolygonShapeConf{};
+        if (s.vertexRadius >= 0) conf.UseVertexRadius(s.vertexRadius);
+        std::vector<Length2> pts;
+        for (const auto& p : s.pts) pts.push_back(Length2{p.x, p.y});
+        conf.Set(Span<const Length2>(pts.data(), pts.size()));
+        conf.UseDensity(s.density).UseFriction(s.friction).UseRestitution(s.restitution);
+        return CreateShape(world, conf);
+    }
+    default: {
+        auto conf = EdgeShapeConf{Length2{s.pts[0].x, s.pts[0].y}, Length2{s.pts[1].x, s.pts[1].y}};
+        if (s.vertexRadius >= 0) conf.UseVertexRadius(s.vertexRadius);
+        conf.UseDensity(s.density).UseFriction(s.friction).UseRestitution(s.restitution);
+        return CreateShape(world, conf);
+    }
+    }
+}
+
+void WriteDescription(Out& o, const World& world) {
+    const auto nshapes = GetShapeRange(world);
+    o.u32(nshapes);
+    for (auto i = 0u; i < nshapes; ++i) {
+        const auto shape = GetShape(world, ShapeID(i));
+        const auto child = GetChild(shape, 0);
+        const auto n = child.GetVertexCount();
+        o.u32(n);
+        o.f32(child.GetVertexRadius());
+        o.f32(GetDensity(shape));
+        o.f32(GetFriction(shape));
+        o.f32(GetRestitution(shape));
+        o.u32(IsSensor(shape) ? 1u : 0u);
+        const auto filt = GetFilter(shape);
+        o.u32(std::uint32_t(filt.categoryBits) | (std::uint32_t(filt.maskBits) << 16));
+        o.u32(std::uint32_t(std::int32_t(filt.groupIndex)));
+        for (auto v = 0u; v < n; ++v) {
+            o.f32(GetX(child.GetVertex(v)));
+            o.f32(GetY(child.GetVertex(v)));
+        }
+        for (auto v = 0u; v < n; ++v) {
+            // A one-vertex proxy (disk) carries no normals.
+            o.f32(n > 1 ? float(GetX(child.GetNormal(v))) : 0.0f);
+            o.f32(n > 1 ? float(GetY(child.GetNormal(v))) : 0.0f);
+        }
+    }
+    const auto nbodies = GetBodyRange(world);
+    o.u32(nbodies);
+    for (auto i = 0u; i < nbodies; ++i) {
+        const auto b = GetBody(world, BodyID(i));
+        o.u32(std::uint32_t(GetType(b)));
+        o.u32(BodyFlags(b));
+        const auto sw = GetSweep(b);
+        const float vals[19] = {
+            GetX(sw.pos0.linear), GetY(sw.pos0.linear), sw.pos0.angular,
+            GetX(sw.pos1.linear), GetY(sw.pos1.linear), sw.pos1.angular,
+            GetX(sw.localCenter), GetY(sw.localCenter),
+            GetX(GetVelocity(b).linear), GetY(GetVelocity(b).linear), GetVelocity(b).angular,
+            GetX(GetLinearAcceleration(b)), GetY(GetLinearAcceleration(b)),
+            GetAngularAcceleration(b), GetLinearDamping(b), GetAngularDamping(b),
+            GetInvMass(b), GetInvRotInertia(b), GetUnderActiveTime(b)};
+        o.put(vals, sizeof(vals));
+        const auto shapes = GetShapes(world, BodyID(i));
+        o.u32(std::uint32_t(shapes.size()));
+        for (auto s : shapes) o.u32(to_underlying(s));
+    }
+}
+
+void WriteConf(Out& o, const StepConf& c) {
+    const float f[] = {c.deltaTime, c.dtRatio, c.minStillTimeToSleep, c.linearSlop,
+                       c.angularSlop, c.regResolutionRate, c.regMinSeparation,
+                       c.regMinMomentum, c.velocityThreshold, c.maxTranslation,
+                       c.maxRotation, c.maxLinearCorrection, c.maxAngularCorrection,
+                       c.linearSleepTolerance, c.angularSleepTolerance,
+                       c.displaceMultiplier, c.aabbExtension, c.maxCirclesRatio};
+    o.u32(sizeof(f) / 4);
+    o.put(f, sizeof(f));
+    o.u32(c.regVelocityIters);
+    o.u32(c.regPositionIters);
+    o.u32((c.doWarmStart ? 1u : 0u) | (c.doToi ? 2u : 0u) | (c.doBlocksolve ? 4u : 0u));
+}
+
+void WriteState(Out& o, const World& world, std::uint32_t step, const StepStats& st) {
+    o.u32(step);
+    const std::uint32_t stats[8] = {st.pre.proxiesMoved, st.pre.contactsDestroyed,
+                                    st.pre.contactsAdded, st.pre.contactsUpdated,
+                                    std::uint32_t(st.reg.islandsFound), st.reg.contactsAdded,
+                                    st.reg.proxiesMoved, std::uint32_t(st.reg.bodiesSlept)};
+    o.put(stats, sizeof(stats));
+    const auto nbodies = GetBodyRange(world);
+    o.u32(nbodies);
+    for (auto i = 0u; i < nbodies; ++i) {
+        const auto b = GetBody(world, BodyID(i));
+        const auto sw = GetSweep(b);
+        const float v[10] = {GetX(sw.pos0.linear), GetY(sw.pos0.linear), sw.pos0.angular,
+                             GetX(sw.pos1.linear), GetY(sw.pos1.linear), sw.pos1.angular,
+                             GetX(GetVelocity(b).linear), GetY(GetVelocity(b).linear),
+                             GetVelocity(b).angular, GetUnderActiveTime(b)};
+        o.put(v, sizeof(v));
+        o.u32(BodyFlags(b));
+    }
+    const auto contacts = GetContacts(world);
+    o.u32(std::uint32_t(contacts.size()));
+    for (const auto& kc : contacts) {
+        const auto key = std::get<ContactKey>(kc);
+        const auto id = std::get<ContactID>(kc);
+        const auto c = GetContact(world, id);
+        std::uint32_t flags = 0;
+        if (IsTouching(c)) flags |= 1u;
+        if (IsEnabled(c)) flags |= 2u;
+        if (IsSensor(c)) flags |= 4u;
+        if (IsImpenetrable(c)) flags |= 8u;
+        if (NeedsUpdating(c)) flags |= 16u;
+        if (NeedsFiltering(c)) flags |= 32u;
+        const std::uint32_t u[4] = {key.GetMin(), key.GetMax(), to_underlying(id), flags};
+        o.put(u, sizeof(u));
+        const auto m = GetManifold(world, id);
+        const auto npts = m.GetPointCount();
+        const auto type = m.GetType();
+        std::uint32_t cf[2] = {PackCf(ContactFeature{}), PackCf(ContactFeature{})};
+        float f[14] = {GetFriction(c), GetRestitution(c)};
+        if (type == Manifold::e_faceA || type == Manifold::e_faceB) {
+            f[2] = GetX(m.GetLocalNormal());
+            f[3] = GetY(m.GetLocalNormal());
+        }
+        if (type != Manifold::e_unset) {
+            f[4] = GetX(m.GetLocalPoint());
+            f[5] = GetY(m.GetLocalPoint());
+        }
+        for (auto p = 0u; p < npts; ++p) {
+            const auto mp = m.GetPoint(p);
+            cf[p] = PackCf(mp.contactFeature);
+            f[6 + 4 * p + 0] = GetX(mp.localPoint);
+            f[6 + 4 * p + 1] = GetY(mp.localPoint);
+            f[6 + 4 * p + 2] = mp.normalImpulse;
+            f[6 + 4 * p + 3] = mp.tangentImpulse;
+        }
+        const std::uint32_t t[3] = {std::uint32_t(type) | (std::uint32_t(npts) << 8), cf[0], cf[1]};
+        o.put(t, sizeof(t));
+        o.put(f, sizeof(f));
+    }
+    const auto& tree = GetTree(world);
+    const auto cap = tree.GetNodeCapacity();
+    std::vector<std::uint32_t> leaves;
+    for (auto i = 0u; i < cap; ++i) {
+        if (DynamicTree::IsLeaf(tree.GetHeight(i))) leaves.push_back(i);
+    }
+    o.u32(std::uint32_t(leaves.size()));
+    for (auto id : leaves) {
+        o.u32(id);
+        const auto aabb = tree.GetAABB(id);
+        const float a[4] = {aabb.ranges[0].GetMin(), aabb.ranges[1].GetMin(),
+                            aabb.ranges[0].GetMax(), aabb.ranges[1].GetMax()};
+        o.put(a, sizeof(a));
+    }
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 5) {
+        std::fprintf(stderr, "usage: %s scene steps doToi out.bin [params...]\n", argv[0]);
+        return 2;
+    }
+    const std::string name = argv[1];
+    const int steps = std::atoi(argv[2]);
+    const bool doToi = std::atoi(argv[3]) != 0;
+    std::vector<int> params;
+    for (int i = 5; i < argc; ++i) params.push_back(std::atoi(argv[i]));
+    prx_scene::SceneSpec spec;
+    if (!prx_scene::Make(name, params.data(), int(params.size()), spec)) {
+        std::fprintf(stderr, "unknown scene %s\n", name.c_str());
+        return 2;
+    }
+
+    auto world = World{};
+    // Shapes are created in spec order, each immediately before the first body
+    // that uses it, matching the order the product's scene builder uses.
+    std::vector<ShapeID> shapeIds(spec.shapes.size(), InvalidShapeID);
+    for (const auto& bs : spec.bodies) {
+        auto conf = BodyConf{};
+        conf.Use(bs.type == prx_scene::kDynamic ? BodyType::Dynamic
+                 : bs.type == prx_scene::kKinematic ? BodyType::Kinematic
+                                                    : BodyType::Static);
+        conf.UseLocation(Length2{bs.x, bs.y});
+        conf.UseAngle(bs.angle);
+        conf.UseLinearVelocity(LinearVelocity2{bs.vx, bs.vy});
+        conf.UseAngularVelocity(bs.w);
+        conf.UseLinearAcceleration(LinearAcceleration2{bs.ax, bs.ay});
+        for (int si : bs.shapes) {
+            if (shapeIds[size_t(si)] == InvalidShapeID) {
+                shapeIds[size_t(si)] = MakeShape(world, spec.shapes[size_t(si)]);
+            }
+            conf.Use(shapeIds[size_t(si)]);
+        }
+        CreateBody(world, conf);
+    }
+
+    auto conf = StepConf{};
+    conf.deltaTime = 1.0f / 60;
+    conf.regVelocityIters = 8;
+    conf.regPositionIters = 3;
+    conf.doToi = doToi;
+
+    Out o;
+    o.put("PRXD", 4);
+    o.u32(1);
+    WriteDescription(o, world);
+    WriteConf(o, conf);
+    o.u32(std::uint32_t(steps + 1));
+    WriteState(o, world, 0, StepStats{});
+    for (int s = 1; s <= steps; ++s) {
+        const auto st = Step(world, conf);
+        WriteState(o, world, std::uint32_t(s), st);
+    }
+    FILE* f = std::fopen(argv[4], "wb");
+    if (!f) {
+        std::perror("fopen");
+        return 1;
+    }
+    std::fwrite(o.buf.data(), 1, o.buf.size(), f);
+    std::fclose(f);
+    return 0;
+}

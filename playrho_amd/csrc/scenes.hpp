@@ -1,0 +1,295 @@
+// Scene specifications for the step workloads: plain data (shapes, bodies) built
+// from a portable random stream, so that the same scene can be applied through
+// any World API that follows playrho::d2 (this library's, or the reference's in
+// the golden-vector driver) and come out identical.
+//
+// Geometry sources:
+//  - "hello":     HelloWorld/HelloWorld.cpp:31-92 (ground box 50x10 at (0,-10),
+//                 disk r=1 at (0,4), gravity (0,-9.8)).
+//  - "pyramid":   Testbed/Tests/Pyramid.cpp:30-56 with 0.5 half-extent boxes,
+//                 density 1, friction 0.6, ground edge (-40,0)-(40,0).
+//  - "drop":      N bodies (half disks, half 3..8-gons) dropped into a walled box.
+//  - "pile":      one world of the batched-uneven-worlds workload: n bodies
+//                 (log-uniform in [nmin,nmax]) of boxes / disks / 3..8-gons in
+//                 stacks or piles on a ground edge.
+//  - "billiards": 64 disks r=0.5 in a walled 20x20 box with seeded velocities,
+//                 friction and restitution.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <random>
+#include <string>
+#include <vector>
+
+namespace prx_scene {
+
+enum ShapeKind : int { kDisk = 0, kPolygon = 1, kBox = 2, kEdge = 3 };
+enum BodyKind : int { kStatic = 0, kKinematic = 1, kDynamic = 2 };
+
+struct Pt {
+    float x = 0, y = 0;
+};
+
+struct ShapeSpec {
+    int kind = kDisk;
+    float radius = 0.01f;        // disk radius (vertex radius of a disk)
+    float hx = 0, hy = 0;        // box half extents
+    std::vector<Pt> pts;         // polygon points (hull taken) or the 2 edge ends
+    float vertexRadius = -1.0f;  // <0: the conf's default (0.01 m)
+    float density = 0, friction = 0.2f, restitution = 0;
+};
+
+struct BodySpec {
+    int type = kStatic;
+    float x = 0, y = 0, angle = 0;
+    float vx = 0, vy = 0, w = 0;
+    float ax = 0, ay = 0;
+    std::vector<int> shapes;
+};
+
+struct SceneSpec {
+    std::vector<ShapeSpec> shapes;
+    std::vector<BodySpec> bodies;
+    // Order in which the scene was authored: shape ids are allocated in
+    // ShapeSpec order and body ids in BodySpec order; when interleave is true
+    // shape i+1... are created right before body i uses them (see Apply*).
+};
+
+// Deterministic uniform floats from std::mt19937 (whose output sequence the C++
+// standard fixes); the mapping to [a,b) is done here so no library-specific
+// distribution is involved.
+struct Rng {
+    std::mt19937 gen;
+    explicit Rng(std::uint32_t seed) : gen(seed) {}
+    double u01() { return double(gen() >> 8) * (1.0 / 16777216.0); }
+    float uniform(double a, double b) { return float(a + (b - a) * u01()); }
+    int below(int n) { return int(gen() % std::uint32_t(n)); }
+    double normal() {
+        // Box-Muller on two draws; deterministic given the draws.
+        double u1 = u01();
+        double u2 = u01();
+        if (u1 < 1e-12) u1 = 1e-12;
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    }
+};
+
+inline ShapeSpec Edge(float x0, float y0, float x1, float y1) {
+    ShapeSpec s;
+    s.kind = kEdge;
+    s.pts = {Pt{x0, y0}, Pt{x1, y1}};
+    return s;
+}
+
+inline ShapeSpec Box(float hx, float hy, float density, float friction, float restitution) {
+    ShapeSpec s;
+    s.kind = kBox;
+    s.hx = hx;
+    s.hy = hy;
+    s.density = density;
+    s.friction = friction;
+    s.restitution = restitution;
+    return s;
+}
+
+inline ShapeSpec Disk(float r, float density, float friction, float restitution) {
+    ShapeSpec s;
+    s.kind = kDisk;
+    s.radius = r;
+    s.density = density;
+    s.friction = friction;
+    s.restitution = restitution;
+    return s;
+}
+
+inline ShapeSpec Ngon(Rng& rng, int n, float R, float density, float friction) {
+    std::vector<double> ang(n);
+    for (auto& a : ang) a = rng.u01() * 6.283185307179586;
+    // Sort so the points go round the circle; the hull keeps all of them unless
+    // two coincide after rounding.
+    for (int i = 1; i < n; ++i)
+        for (int j = i; j > 0 && ang[j] < ang[j - 1]; --j) std::swap(ang[j], ang[j - 1]);
+    ShapeSpec s;
+    s.kind = kPolygon;
+    s.density = density;
+    s.friction = friction;
+    for (double a : ang) s.pts.push_back(Pt{float(R * std::cos(a)), float(R * std::sin(a))});
+    return s;
+}
+
+inline SceneSpec Hello() {
+    SceneSpec sc;
+    sc.shapes.push_back(Box(50.0f, 10.0f, 0.0f, 0.2f, 0.0f));
+    sc.shapes.push_back(Disk(1.0f, 0.0f, 0.2f, 0.0f));
+    BodySpec ground;
+    ground.type = kStatic;
+    ground.x = 0.0f;
+    ground.y = -10.0f;
+    ground.shapes = {0};
+    BodySpec ball;
+    ball.type = kDynamic;
+    ball.x = 0.0f;
+    ball.y = 4.0f;
+    ball.ay = -9.8f;
+    ball.shapes = {1};
+    sc.bodies = {ground, ball};
+    return sc;
+}
+
+inline SceneSpec Pyramid(int rows = 20) {
+    SceneSpec sc;
+    sc.shapes.push_back(Edge(-40.0f, 0.0f, 40.0f, 0.0f));
+    sc.shapes.push_back(Box(0.5f, 0.5f, 1.0f, 0.6f, 0.0f));
+    BodySpec ground;
+    ground.shapes = {0};
+    sc.bodies.push_back(ground);
+    float xx = -7.0f, xy = 0.75f;
+    const float dxx = 0.5625f, dxy = 1.25f;
+    const float dyx = 1.125f, dyy = 0.0f;
+    for (int i = 0; i < rows; ++i) {
+        float yx = xx, yy = xy;
+        for (int j = i; j < rows; ++j) {
+            BodySpec b;
+            b.type = kDynamic;
+            b.x = yx;
+            b.y = yy;
+            b.ay = -10.0f;
+            b.shapes = {1};
+            sc.bodies.push_back(b);
+            yx += dyx;
+            yy += dyy;
+        }
+        xx += dxx;
+        xy += dxy;
+    }
+    return sc;
+}
+
+// Walled container: one static body with four edges around [-half,half]x[0,height].
+inline void AddContainer(SceneSpec& sc, float half, float height) {
+    const int s0 = int(sc.shapes.size());
+    sc.shapes.push_back(Edge(-half, 0.0f, half, 0.0f));
+    sc.shapes.push_back(Edge(half, 0.0f, half, height));
+    sc.shapes.push_back(Edge(half, height, -half, height));
+    sc.shapes.push_back(Edge(-half, height, -half, 0.0f));
+    BodySpec c;
+    c.shapes = {s0, s0 + 1, s0 + 2, s0 + 3};
+    sc.bodies.push_back(c);
+}
+
+inline SceneSpec Drop(int n, float half, float height, std::uint32_t seed) {
+    SceneSpec sc;
+    Rng rng(seed);
+    AddContainer(sc, half, height);
+    for (int i = 0; i < n; ++i) {
+        BodySpec b;
+        b.type = kDynamic;
+        b.x = rng.uniform(-half + 1.0, half - 1.0);
+        b.y = rng.uniform(1.0, height - 1.0);
+        b.ay = -10.0f;
+        const float friction = rng.uniform(0.2, 0.8);
+        if ((i % 2) == 0) {
+            sc.shapes.push_back(Disk(rng.uniform(0.25, 0.75), 1.0f, friction, 0.0f));
+        } else {
+            const int nv = 3 + rng.below(6);
+            const float R = rng.uniform(0.3, 0.8);
+            sc.shapes.push_back(Ngon(rng, nv, R, 1.0f, friction));
+        }
+        b.shapes = {int(sc.shapes.size()) - 1};
+        sc.bodies.push_back(b);
+    }
+    return sc;
+}
+
+// Body count of world w of the batched workload: round(exp(U[ln nmin, ln nmax])).
+inline int PileBodyCount(std::uint32_t seed, int w, int nmin, int nmax) {
+    Rng rng(seed * 1000003u + std::uint32_t(w) * 7919u + 17u);
+    const double lo = std::log(double(nmin)), hi = std::log(double(nmax));
+    return int(std::lround(std::exp(lo + (hi - lo) * rng.u01())));
+}
+
+inline SceneSpec Pile(std::uint32_t seed, int w, int nmin = 8, int nmax = 512) {
+    SceneSpec sc;
+    const int n = PileBodyCount(seed, w, nmin, nmax);
+    Rng rng(seed * 2654435761u + std::uint32_t(w) * 40503u + 3u);
+    const bool pile = rng.below(2) == 1;  // else: stacks
+    int ncols = int(std::lround(std::sqrt(double(n)) * 0.8));
+    if (ncols < 1) ncols = 1;
+    const float spacing = 1.25f;
+    const float halfw = float(ncols) * spacing * 0.5f + 5.0f;
+    sc.shapes.push_back(Edge(-halfw, 0.0f, halfw, 0.0f));
+    BodySpec ground;
+    ground.shapes = {0};
+    sc.bodies.push_back(ground);
+    for (int k = 0; k < n; ++k) {
+        const int col = k % ncols, row = k / ncols;
+        BodySpec b;
+        b.type = kDynamic;
+        b.x = float((double(col) - double(ncols - 1) * 0.5) * spacing);
+        b.y = float(0.6 + row * 1.15);
+        if (pile) {
+            b.x += rng.uniform(-0.15, 0.15);
+            b.y += rng.uniform(0.0, 0.1);
+            b.angle = rng.uniform(-0.3, 0.3);
+        }
+        b.ay = -10.0f;
+        const float friction = rng.uniform(0.2, 0.8);
+        const int kind = rng.below(3);
+        if (kind == 0) {
+            sc.shapes.push_back(Box(rng.uniform(0.3, 0.5), rng.uniform(0.3, 0.5), 1.0f, friction, 0.0f));
+        } else if (kind == 1) {
+            sc.shapes.push_back(Disk(rng.uniform(0.3, 0.5), 1.0f, friction, 0.0f));
+        } else {
+            const int nv = 3 + rng.below(6);
+            sc.shapes.push_back(Ngon(rng, nv, rng.uniform(0.35, 0.55), 1.0f, friction));
+        }
+        b.shapes = {int(sc.shapes.size()) - 1};
+        sc.bodies.push_back(b);
+    }
+    return sc;
+}
+
+inline SceneSpec Billiards(std::uint32_t seed, int w) {
+    SceneSpec sc;
+    Rng rng(seed * 69069u + std::uint32_t(w) * 1664525u + 1013904223u);
+    AddContainer(sc, 10.0f, 20.0f);
+    for (int k = 0; k < 64; ++k) {
+        const int col = k % 8, row = k / 8;
+        BodySpec b;
+        b.type = kDynamic;
+        b.x = float(-7.7 + col * 2.2) + rng.uniform(-0.2, 0.2);
+        b.y = float(1.5 + row * 2.2) + rng.uniform(-0.2, 0.2);
+        b.vx = float(3.0 * rng.normal());
+        b.vy = float(3.0 * rng.normal());
+        b.ay = -10.0f;
+        const float friction = rng.uniform(0.0, 1.0);
+        const float restitution = rng.uniform(0.0, 1.0);
+        sc.shapes.push_back(Disk(0.5f, 1.0f, friction, restitution));
+        b.shapes = {int(sc.shapes.size()) - 1};
+        sc.bodies.push_back(b);
+    }
+    return sc;
+}
+
+// Named scenes with integer parameters, as used by tests and benchmarks.
+//   "hello", "pyramid"(rows), "drop"(n, half, height, seed),
+//   "pile"(seed, w, nmin, nmax), "billiards"(seed, w)
+inline bool Make(const std::string& name, const int* p, int np, SceneSpec& out) {
+    auto arg = [&](int i, int dflt) { return (i < np) ? p[i] : dflt; };
+    if (name == "hello") {
+        out = Hello();
+    } else if (name == "pyramid") {
+        out = Pyramid(arg(0, 20));
+    } else if (name == "drop") {
+        out = Drop(arg(0, 20000), float(arg(1, 200)), float(arg(2, 400)), std::uint32_t(arg(3, 1)));
+    } else if (name == "pile") {
+        out = Pile(std::uint32_t(arg(0, 1)), arg(1, 0), arg(2, 8), arg(3, 512));
+    } else if (name == "billiards") {
+        out = Billiards(std::uint32_t(arg(0, 1)), arg(1, 0));
+    } else {
+        return false;
+    }
+    return true;
+}
+
+}  // namespace prx_scene

@@ -1,0 +1,114 @@
+"""Reader for the per-step state record stream written by oracle/ref_driver.cpp
+and by the oracle restatement (same layout), plus canonical per-step digests.
+
+A record holds, after each step: bodies (pos0, pos1, velocity, under-active
+time, flags), the contact list in the world's contact order (keys, ids, flags,
+manifold with ContactFeatures and impulses) and the broadphase proxies (id and
+fat AABB).
+"""
+import hashlib
+import struct
+
+import numpy as np
+
+BODY_F = 10
+CONTACT_U = 4
+CONTACT_T = 3
+CONTACT_F = 14
+
+
+class Reader:
+    def __init__(self, buf):
+        self.b = memoryview(buf)
+        self.o = 0
+
+    def u32(self):
+        v = struct.unpack_from("<I", self.b, self.o)[0]
+        self.o += 4
+        return v
+
+    def arr(self, dtype, n):
+        a = np.frombuffer(self.b, dtype=dtype, count=n, offset=self.o).copy()
+        self.o += 4 * n
+        return a
+
+
+def read_description(r):
+    shapes = []
+    for _ in range(r.u32()):
+        n = r.u32()
+        hdr = r.arr("<f4", 4)
+        sensor = r.u32()
+        catmask = r.u32()
+        group = r.u32()
+        verts = r.arr("<f4", 2 * n).reshape(n, 2)
+        normals = r.arr("<f4", 2 * n).reshape(n, 2)
+        shapes.append(dict(radius=hdr[0], density=hdr[1], friction=hdr[2],
+                           restitution=hdr[3], sensor=sensor, catmask=catmask,
+                           group=group, verts=verts, normals=normals))
+    bodies = []
+    for _ in range(r.u32()):
+        btype = r.u32()
+        flags = r.u32()
+        vals = r.arr("<f4", 19)
+        ns = r.u32()
+        sids = r.arr("<u4", ns)
+        bodies.append(dict(type=btype, flags=flags, vals=vals, shapes=sids))
+    return shapes, bodies
+
+
+def read_conf(r):
+    nf = r.u32()
+    f = r.arr("<f4", nf)
+    vi = r.u32()
+    pi = r.u32()
+    flags = r.u32()
+    return dict(floats=f, vel_iters=vi, pos_iters=pi, flags=flags)
+
+
+def read_state(r):
+    step = r.u32()
+    stats = r.arr("<u4", 8)
+    nb = r.u32()
+    raw = r.arr("<u4", nb * (BODY_F + 1)).reshape(nb, BODY_F + 1)
+    bodies = raw[:, :BODY_F].copy().view("<f4")
+    bflags = raw[:, BODY_F].copy()
+    nc = r.u32()
+    per = CONTACT_U + CONTACT_T + CONTACT_F
+    raw = r.arr("<u4", nc * per).reshape(nc, per)
+    cu = raw[:, :CONTACT_U].copy()
+    ct = raw[:, CONTACT_U:CONTACT_U + CONTACT_T].copy()
+    cf = raw[:, CONTACT_U + CONTACT_T:].copy().view("<f4")
+    npx = r.u32()
+    raw = r.arr("<u4", npx * 5).reshape(npx, 5)
+    pid = raw[:, 0].copy()
+    paabb = raw[:, 1:].copy().view("<f4")
+    return dict(step=step, stats=stats, bodies=bodies, bflags=bflags,
+                cu=cu, ct=ct, cf=cf, pid=pid, paabb=paabb)
+
+
+def read_dump(buf):
+    r = Reader(buf)
+    assert bytes(r.b[0:4]) == b"PRXD", "bad magic"
+    r.o = 4
+    assert r.u32() == 1
+    shapes, bodies = read_description(r)
+    conf = read_conf(r)
+    states = [read_state(r) for _ in range(r.u32())]
+    return dict(shapes=shapes, bodies=bodies, conf=conf, states=states)
+
+
+def digest(state, parts=("bodies", "contacts", "proxies")):
+    """SHA-256 over the canonical bytes of one state (selected parts)."""
+    h = hashlib.sha256()
+    if "bodies" in parts:
+        h.update(np.ascontiguousarray(state["bodies"]).view("<u4").tobytes())
+        h.update(np.ascontiguousarray(state["bflags"]).tobytes())
+    if "contacts" in parts:
+        h.update(np.ascontiguousarray(state["cu"]).tobytes())
+        h.update(np.ascontiguousarray(state["ct"]).tobytes())
+        h.update(np.ascontiguousarray(state["cf"]).view("<u4").tobytes())
+    if "proxies" in parts:
+        h.update(np.ascontiguousarray(state["pid"]).tobytes())
+        h.update(np.ascontiguousarray(state["paabb"]).view("<u4").tobytes())
+    return h.digest()

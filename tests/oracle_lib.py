@@ -1,0 +1,110 @@
+"""ctypes wrapper over the oracle's C-ABI (oracle/cpu_ref.cpp) for tests and
+the CPU-baseline leg of bench.py.  The oracle is the checker, never the thing
+measured on the GPU path."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from . import dumpfmt  # noqa: F401  (re-exported for callers)
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+REF_DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError("oracle library not built: run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB)
+        L.oracle_create.restype = ctypes.c_void_p
+        L.oracle_create.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int]
+        L.oracle_destroy.argtypes = [ctypes.c_void_p]
+        L.oracle_step.restype = ctypes.c_void_p
+        L.oracle_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_size_t)]
+        L.oracle_state.restype = ctypes.c_void_p
+        L.oracle_state.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]
+        L.oracle_bodies_f64.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+        L.oracle_free.argtypes = [ctypes.c_void_p]
+        L.oracle_time_batch.restype = ctypes.c_double
+        L.oracle_time_batch.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_double)]
+        _lib = L
+    return _lib
+
+
+def _take(ptr, n):
+    buf = ctypes.string_at(ptr, n.value)
+    lib().oracle_free(ptr)
+    return buf
+
+
+class OracleWorld:
+    """One oracle world built from a description+conf blob."""
+
+    def __init__(self, desc_conf: bytes, precision=32):
+        self.h = lib().oracle_create(desc_conf, len(desc_conf), precision)
+        self.nbodies = None
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_destroy(self.h)
+            self.h = None
+
+    def step(self, n=1, record=True):
+        ln = ctypes.c_size_t()
+        p = lib().oracle_step(self.h, n, 1 if record else 0, ctypes.byref(ln))
+        buf = _take(p, ln)
+        if not record:
+            return []
+        r = dumpfmt.Reader(buf)
+        out = []
+        while r.o < len(buf):
+            out.append(dumpfmt.read_state(r))
+        return out
+
+    def state(self):
+        ln = ctypes.c_size_t()
+        p = lib().oracle_state(self.h, ctypes.byref(ln))
+        buf = _take(p, ln)
+        return dumpfmt.read_state(dumpfmt.Reader(buf))
+
+    def bodies_f64(self, nbodies):
+        out = np.zeros(nbodies * 6, dtype=np.float64)
+        lib().oracle_bodies_f64(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return out.reshape(nbodies, 6)
+
+
+def split_dump(buf):
+    """Returns (desc_conf_bytes, parsed_dump) for a ref_driver dump."""
+    r = dumpfmt.Reader(buf)
+    r.o = 8
+    dumpfmt.read_description(r)
+    dumpfmt.read_conf(r)
+    desc_conf = bytes(buf[8:r.o])
+    return desc_conf, dumpfmt.read_dump(buf)
+
+
+def run_ref_driver(scene, steps, do_toi, params, path):
+    """Runs the reference build's driver (present only where oracle/_ref was
+    built) and returns the raw dump."""
+    cmd = [REF_DRIVER, scene, str(steps), "1" if do_toi else "0", path] + [str(p) for p in params]
+    subprocess.run(cmd, check=True)
+    with open(path, "rb") as f:
+        return f.read()
+
+
+def time_batch(descs, steps, threads):
+    """Times the oracle on independent worlds with `threads` threads.
+    Returns (seconds, body_steps)."""
+    packed = b"".join(len(d).to_bytes(8, "little") + d for d in descs)
+    bs = ctypes.c_double()
+    secs = lib().oracle_time_batch(packed, len(packed), len(descs), steps, threads, ctypes.byref(bs))
+    return secs, bs.value
