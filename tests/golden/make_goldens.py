@@ -1,0 +1,69 @@
+"""Writes the committed golden fixtures from the reference float build.
+
+Run from the repository root, in a container where the reference checkout is
+present and oracle/_ref has been built (make -f oracle/Makefile.ref):
+
+    python tests/golden/make_goldens.py
+
+For each case it runs oracle/_ref/ref_driver (our driver linked against the
+reference library), and stores in tests/golden/<case>.npz:
+  desc_conf  the world description and StepConf as the reference built them
+  digests    per step, SHA-256 of bodies / contact list / proxies
+  full_*     complete state at a few steps (for diffs when a digest differs)
+HelloWorld/ExpectedOutput.txt (the reference's own golden output) is copied
+as data to tests/golden/hello_expected.txt.
+"""
+import os
+import shutil
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from tests import dumpfmt, oracle_lib  # noqa: E402
+
+CASES = [
+    # name, scene, steps, params, full-state steps
+    ("hello", "hello", 60, [], [1, 30, 60]),
+    ("pyramid", "pyramid", 300, [20], [1, 2, 50, 150, 300]),
+    ("drop300", "drop", 200, [300, 20, 40, 7], [1, 100, 200]),
+    ("billiards0", "billiards", 120, [1, 0], [1, 60, 120]),
+    ("billiards1", "billiards", 120, [1, 1], [1, 120]),
+] + [("pile%d" % w, "pile", 150, [1, w], [1, 150]) for w in range(8)]
+
+
+def main():
+    tmp = tempfile.mkdtemp()
+    try:
+        for name, scene, steps, params, full in CASES:
+            path = os.path.join(tmp, name + ".bin")
+            buf = oracle_lib.run_ref_driver(scene, steps, False, params, path)
+            desc_conf, d = oracle_lib.split_dump(buf)
+            states = d["states"]
+            dig = np.zeros((len(states), 3, 32), dtype=np.uint8)
+            for i, s in enumerate(states):
+                for k, part in enumerate(("bodies", "contacts", "proxies")):
+                    dig[i, k] = np.frombuffer(dumpfmt.digest(s, (part,)), dtype=np.uint8)
+            arrays = dict(desc_conf=np.frombuffer(desc_conf, dtype=np.uint8),
+                          digests=dig, scene=np.array(scene), params=np.array(params, dtype=np.int32),
+                          steps=np.array(steps), full_steps=np.array(full, dtype=np.int32))
+            for st in full:
+                s = states[st]
+                for key in ("stats", "bodies", "bflags", "cu", "ct", "cf", "pid", "paabb"):
+                    arrays["full%d_%s" % (st, key)] = s[key]
+            out = os.path.join(HERE, name + ".npz")
+            np.savez_compressed(out, **arrays)
+            print("%-12s %4d steps  %6d bytes" % (name, steps, os.path.getsize(out)))
+    finally:
+        shutil.rmtree(tmp)
+    ref_hello = "/root/reference/HelloWorld/ExpectedOutput.txt"
+    if os.path.exists(ref_hello):
+        shutil.copyfile(ref_hello, os.path.join(HERE, "hello_expected.txt"))
+
+
+if __name__ == "__main__":
+    main()

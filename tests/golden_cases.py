@@ -1,0 +1,35 @@
+"""Loads the committed golden fixtures (tests/golden/*.npz)."""
+import glob
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+PARTS = ("bodies", "contacts", "proxies")
+
+
+def names():
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+
+
+def load(name):
+    z = np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
+    g = dict(desc_conf=z["desc_conf"].tobytes(), digests=z["digests"], scene=str(z["scene"]),
+             params=[int(x) for x in z["params"]], steps=int(z["steps"]), full={})
+    for st in z["full_steps"]:
+        st = int(st)
+        g["full"][st] = {k: z["full%d_%s" % (st, k)] for k in
+                         ("stats", "bodies", "bflags", "cu", "ct", "cf", "pid", "paabb")}
+    return g
+
+
+def first_mismatch(digests, states, dumpfmt, start=1):
+    """Compares per-step digests of `states` (steps start..) against the golden
+    digests; returns (step, part) of the first difference or None."""
+    for i, s in enumerate(states):
+        step = start + i
+        for k, part in enumerate(PARTS):
+            if dumpfmt.digest(s, (part,)) != digests[step, k].tobytes():
+                return step, part
+    return None
