@@ -1,0 +1,186 @@
+/* playrho_amd — C ABI of the MI355X step engine for PlayRho worlds.
+ *
+ * The boundary is the one PlayRho's type-erased World forwards through:
+ * every call here corresponds to a WorldConcept virtual or a free function of
+ * the playrho::d2 API that the step path needs (Library/include/playrho/d2/
+ * World.hpp:694-841, detail/WorldConcept.hpp, detail/WorldModel.hpp:144-147).
+ * Plain pointers and sizes only; no C++ or torch types cross it.  Exceptions of
+ * the reference (WrongState, LengthError, InvalidArgument, OutOfRange) map to
+ * status codes; prx_last_error() gives the message of the last failure on the
+ * calling thread.
+ *
+ * Stepping runs hand-written gfx950 HIP kernels on the current HIP device.  A
+ * world's step state lives in device memory between calls; host getters
+ * download it lazily.  Many worlds can be stepped together as a batch.
+ */
+#ifndef PLAYRHO_AMD_H
+#define PLAYRHO_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PRX_ABI_VERSION 1
+
+/* Status codes (reference exception types in parentheses). */
+enum {
+    PRX_OK = 0,
+    PRX_E_INVALID_ARGUMENT = 1, /* InvalidArgument (AabbTreeWorld.cpp:1243-1248) */
+    PRX_E_LENGTH = 2,           /* LengthError (AabbTreeWorld.cpp:1024-1026) */
+    PRX_E_WRONG_STATE = 3,      /* WrongState (AabbTreeWorld.cpp:2149-2151) */
+    PRX_E_OUT_OF_RANGE = 4,     /* OutOfRange (AabbTreeWorld.cpp:164-174) */
+    PRX_E_DEVICE = 5,           /* HIP runtime failure */
+    PRX_E_UNSUPPORTED = 6       /* a configuration this engine does not step */
+};
+
+typedef struct prx_world prx_world;
+typedef struct prx_batch prx_batch;
+
+/* Body types (BodyType.hpp). */
+enum { PRX_STATIC = 0, PRX_KINEMATIC = 1, PRX_DYNAMIC = 2 };
+/* Shape kinds: DiskShapeConf, PolygonShapeConf::Set (hull of points),
+ * PolygonShapeConf::SetAsBox, EdgeShapeConf. */
+enum { PRX_SHAPE_DISK = 0, PRX_SHAPE_POLYGON = 1, PRX_SHAPE_BOX = 2, PRX_SHAPE_EDGE = 3 };
+
+/* WorldConf (d2/WorldConf.hpp:40-90): vertex-radius bounds; capacity hint. */
+typedef struct prx_world_conf {
+    uint32_t abi_version;
+    float min_vertex_radius; /* default 0.01 m */
+    float max_vertex_radius; /* default 255 m */
+    uint32_t contact_capacity_per_proxy; /* device contact slots per proxy (0 = 8) */
+} prx_world_conf;
+
+/* Union of the shape confs used on the step path
+ * (d2/PolygonShapeConf.hpp, DiskShapeConf.hpp, EdgeShapeConf.hpp, ShapeConf.hpp). */
+typedef struct prx_shape_conf {
+    uint32_t kind;
+    float vertex_radius; /* < 0: the conf's default (0.01 m; for a disk: its radius) */
+    float radius;        /* disk radius */
+    float hx, hy;        /* box half extents */
+    float location[2];   /* disk center */
+    const float* points; /* polygon points (x, y pairs) or edge ends */
+    uint32_t npoints;
+    float density, friction, restitution;
+    uint32_t sensor;
+    uint16_t category_bits, mask_bits; /* Filter.hpp defaults: 1, 0xffff */
+    int16_t group_index;
+} prx_shape_conf;
+
+/* BodyConf (d2/BodyConf.hpp). */
+typedef struct prx_body_conf {
+    uint32_t type;
+    float location[2];
+    float angle;
+    float linear_velocity[2];
+    float angular_velocity;
+    float linear_acceleration[2];
+    float angular_acceleration;
+    float linear_damping, angular_damping;
+    uint32_t allow_sleep, awake, fixed_rotation, bullet, enabled;
+    const uint32_t* shapes;
+    uint32_t nshapes;
+} prx_body_conf;
+
+/* StepConf fields the regular step reads (StepConf.hpp:49-392). */
+typedef struct prx_step_conf {
+    float delta_time, dt_ratio, min_still_time_to_sleep, linear_slop, angular_slop;
+    float reg_resolution_rate, reg_min_separation, reg_min_momentum, velocity_threshold;
+    float max_translation, max_rotation, max_linear_correction, max_angular_correction;
+    float linear_sleep_tolerance, angular_sleep_tolerance, displace_multiplier;
+    float aabb_extension, max_circles_ratio;
+    uint32_t reg_velocity_iters, reg_position_iters;
+    uint32_t do_warm_start, do_toi, do_blocksolve;
+} prx_step_conf;
+
+/* Subset of StepStats (StepStats.hpp:30-201). */
+typedef struct prx_step_stats {
+    uint32_t pre_proxies_moved, pre_contacts_destroyed, pre_contacts_added, pre_contacts_updated;
+    uint32_t reg_islands_found, reg_contacts_added, reg_proxies_moved, reg_bodies_slept;
+} prx_step_stats;
+
+/* Launch schedule of the batched step (chosen by prx_batch_autotune). */
+typedef struct prx_batch_conf {
+    uint32_t threads_per_world; /* 64 (one wavefront) by default */
+    uint32_t use_graph;         /* replay steps from a captured hipGraph */
+    uint32_t order;             /* 0: largest worlds first, 1: as given */
+} prx_batch_conf;
+
+int prx_abi_version(void);
+const char* prx_last_error(void);
+
+/* Reference StepConf defaults (values of a default-constructed StepConf in the
+ * reference float build, with dt = 1/60 and 8/3 iterations). */
+void prx_step_conf_default(prx_step_conf* conf);
+void prx_world_conf_default(prx_world_conf* conf);
+void prx_batch_conf_default(prx_batch_conf* conf);
+
+/* World{WorldConf} / ~World. */
+int prx_world_create(const prx_world_conf* conf, prx_world** out);
+void prx_world_destroy(prx_world* world);
+
+/* CreateShape(World&, conf) (World.hpp:512,522-526; AabbTreeWorld.cpp:1234-1257). */
+int prx_shape_create(prx_world* world, const prx_shape_conf* conf, uint32_t* shape_id);
+/* CreateBody(World&, BodyConf, resetMassData = true) (WorldBody.cpp:55;
+ * AabbTreeWorld.cpp:1019-1044; mass from WorldBody.cpp:360-392). */
+int prx_body_create(prx_world* world, const prx_body_conf* conf, uint32_t* body_id);
+/* Builds a named workload scene through the two calls above. */
+int prx_scene_build(prx_world* world, const char* name, const int32_t* params, int nparams);
+
+/* Replaces the world with a description in the engine's record layout
+ * (shapes with hull vertices and normals, bodies with sweep, velocity, mass).
+ * Used to start from a state produced elsewhere (e.g. the reference). */
+int prx_world_load_desc(prx_world* world, const uint8_t* desc, size_t len);
+/* Writes the current description (same layout) into buf. */
+int prx_world_save_desc(prx_world* world, uint8_t* buf, size_t cap, size_t* len);
+
+uint32_t prx_world_body_count(const prx_world* world);
+uint32_t prx_world_shape_count(const prx_world* world);
+
+/* Step(World&, const StepConf&) (World.hpp:917-920 -> AabbTreeWorld.cpp:2144-2219),
+ * plain per-world form on the device.  doToi must be 0 (no TOI phase here). */
+int prx_step(prx_world* world, const prx_step_conf* conf, prx_step_stats* stats);
+
+/* Body getters: 10 floats per body (pos0 x,y,a; pos1 x,y,a; velocity x,y,w;
+ * under-active time) and the body flags. */
+int prx_world_get_bodies(prx_world* world, float* out, uint32_t* flags);
+/* GetTransformation(World&, BodyID) (WorldBody.hpp): {px, py, cos, sin}. */
+int prx_world_get_transformation(prx_world* world, uint32_t body, float out[4]);
+/* SetVelocity(World&, BodyID, Velocity) (WorldBody.hpp). */
+int prx_world_set_velocity(prx_world* world, uint32_t body, const float v[3]);
+/* One state record (bodies, contact list in world order with manifolds,
+ * proxies) in the layout of the golden vectors. */
+int prx_world_state_record(prx_world* world, uint8_t* buf, size_t cap, size_t* len);
+uint32_t prx_world_contact_count(prx_world* world);
+
+/* Batched worlds: all resident on the current device, stepped by one
+ * persistent launch per step. */
+int prx_batch_create(prx_world* const* worlds, int nworlds, const prx_batch_conf* conf,
+                     prx_batch** out);
+void prx_batch_destroy(prx_batch* batch);
+int prx_batch_step(prx_batch* batch, const prx_step_conf* conf, int nsteps);
+int prx_batch_sync(prx_batch* batch);
+/* Copies device state back into the member worlds (for getters/records). */
+int prx_batch_download(prx_batch* batch);
+/* Measures the last prx_batch_step call's kernel time (ms, HIP events on the
+ * launch stream) and the number of step launches it covered. */
+int prx_batch_last_timing(prx_batch* batch, float* ms, int* launches);
+/* Sum over worlds of dynamic bodies, and of contacts (after download). */
+uint64_t prx_batch_dynamic_bodies(const prx_batch* batch);
+/* Tries launch schedules on `nsteps` steps of a copy of the batch state and
+ * keeps the fastest in conf. */
+int prx_batch_autotune(prx_batch* batch, const prx_step_conf* step, int nsteps,
+                       prx_batch_conf* best);
+
+/* Diagnostic: UnitVec::Get(angle) = {cos, sin} (UnitVec.cpp:27-30) evaluated
+ * by the device routine for n angles; out receives n (cos, sin) pairs.  Used
+ * by the tests to compare the device rotation with the host C library. */
+int prx_selftest_sincos(const float* angles, float* out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PLAYRHO_AMD_H */

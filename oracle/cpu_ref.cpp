@@ -280,6 +280,15 @@ void oracle_bodies_f64(void* p, double* out) {
 
 void oracle_free(void* p) { std::free(p); }
 
+// The C library's float cosine and sine (what UnitVec::Get(angle) calls on the
+// host), for comparison with the device routine.
+void oracle_sincosf(const float* in, float* out, int n) {
+    for (int i = 0; i < n; ++i) {
+        out[2 * i] = std::cos(in[i]);
+        out[2 * i + 1] = std::sin(in[i]);
+    }
+}
+
 // CPU-baseline helper: steps many independent worlds (descriptions packed
 // back-to-back, each prefixed by its byte length) with `threads` std::threads,
 // each owning a static contiguous partition balanced by body count.  Returns

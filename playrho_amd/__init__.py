@@ -1,0 +1,301 @@
+"""playrho_amd — MI355X step engine for PlayRho worlds.
+
+Python binding over the C ABI in include/playrho_amd.h (libplayrho_amd.so,
+built in-tree by playrho_amd._build).  The stepping path runs only on the HIP
+device: there is no CPU fallback, and every call fails loudly when the native
+library is missing or no device is present.
+
+    import playrho_amd as pa
+    w = pa.World()
+    w.build_scene("pyramid", [20])
+    conf = pa.StepConf()          # reference StepConf defaults, doToi = 0
+    w.step(conf)
+    xf = w.transformation(5)
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libplayrho_amd.so")
+
+PRX_OK = 0
+STATUS = {1: "InvalidArgument", 2: "LengthError", 3: "WrongState", 4: "OutOfRange",
+          5: "DeviceError", 6: "Unsupported"}
+
+STATIC, KINEMATIC, DYNAMIC = 0, 1, 2
+SHAPE_DISK, SHAPE_POLYGON, SHAPE_BOX, SHAPE_EDGE = 0, 1, 2, 3
+
+
+class PrxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (STATUS.get(code, "Error%d" % code), msg))
+        self.code = code
+
+
+class StepConf(ctypes.Structure):
+    """StepConf fields of the regular step (StepConf.hpp:49-392)."""
+    _fields_ = [(n, ctypes.c_float) for n in (
+        "delta_time", "dt_ratio", "min_still_time_to_sleep", "linear_slop", "angular_slop",
+        "reg_resolution_rate", "reg_min_separation", "reg_min_momentum", "velocity_threshold",
+        "max_translation", "max_rotation", "max_linear_correction", "max_angular_correction",
+        "linear_sleep_tolerance", "angular_sleep_tolerance", "displace_multiplier",
+        "aabb_extension", "max_circles_ratio")] + [
+        (n, ctypes.c_uint32) for n in ("reg_velocity_iters", "reg_position_iters",
+                                       "do_warm_start", "do_toi", "do_blocksolve")]
+
+    def __init__(self, **kw):
+        super().__init__()
+        lib().prx_step_conf_default(ctypes.byref(self))
+        for k, v in kw.items():
+            setattr(self, k, v)
+
+
+class WorldConf(ctypes.Structure):
+    _fields_ = [("abi_version", ctypes.c_uint32), ("min_vertex_radius", ctypes.c_float),
+                ("max_vertex_radius", ctypes.c_float), ("contact_capacity_per_proxy", ctypes.c_uint32)]
+
+
+class ShapeConf(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("vertex_radius", ctypes.c_float), ("radius", ctypes.c_float),
+                ("hx", ctypes.c_float), ("hy", ctypes.c_float), ("location", ctypes.c_float * 2),
+                ("points", ctypes.POINTER(ctypes.c_float)), ("npoints", ctypes.c_uint32),
+                ("density", ctypes.c_float), ("friction", ctypes.c_float), ("restitution", ctypes.c_float),
+                ("sensor", ctypes.c_uint32), ("category_bits", ctypes.c_uint16),
+                ("mask_bits", ctypes.c_uint16), ("group_index", ctypes.c_int16)]
+
+
+class BodyConf(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_uint32), ("location", ctypes.c_float * 2), ("angle", ctypes.c_float),
+                ("linear_velocity", ctypes.c_float * 2), ("angular_velocity", ctypes.c_float),
+                ("linear_acceleration", ctypes.c_float * 2), ("angular_acceleration", ctypes.c_float),
+                ("linear_damping", ctypes.c_float), ("angular_damping", ctypes.c_float),
+                ("allow_sleep", ctypes.c_uint32), ("awake", ctypes.c_uint32),
+                ("fixed_rotation", ctypes.c_uint32), ("bullet", ctypes.c_uint32),
+                ("enabled", ctypes.c_uint32), ("shapes", ctypes.POINTER(ctypes.c_uint32)),
+                ("nshapes", ctypes.c_uint32)]
+
+
+class StepStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in (
+        "pre_proxies_moved", "pre_contacts_destroyed", "pre_contacts_added", "pre_contacts_updated",
+        "reg_islands_found", "reg_contacts_added", "reg_proxies_moved", "reg_bodies_slept")]
+
+
+class BatchConf(ctypes.Structure):
+    _fields_ = [("threads_per_world", ctypes.c_uint32), ("use_graph", ctypes.c_uint32),
+                ("order", ctypes.c_uint32)]
+
+
+_lib = None
+
+_SIGS = {
+    "prx_abi_version": (ctypes.c_int, []),
+    "prx_last_error": (ctypes.c_char_p, []),
+    "prx_step_conf_default": (None, [ctypes.c_void_p]),
+    "prx_world_conf_default": (None, [ctypes.c_void_p]),
+    "prx_batch_conf_default": (None, [ctypes.c_void_p]),
+    "prx_world_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "prx_world_destroy": (None, [ctypes.c_void_p]),
+    "prx_shape_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
+    "prx_body_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
+    "prx_scene_build": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32),
+                                       ctypes.c_int]),
+    "prx_world_load_desc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]),
+    "prx_world_save_desc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                           ctypes.POINTER(ctypes.c_size_t)]),
+    "prx_world_body_count": (ctypes.c_uint32, [ctypes.c_void_p]),
+    "prx_world_shape_count": (ctypes.c_uint32, [ctypes.c_void_p]),
+    "prx_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "prx_world_get_bodies": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "prx_world_get_transformation": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32,
+                                                    ctypes.POINTER(ctypes.c_float)]),
+    "prx_world_set_velocity": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32,
+                                              ctypes.POINTER(ctypes.c_float)]),
+    "prx_world_state_record": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                              ctypes.POINTER(ctypes.c_size_t)]),
+    "prx_world_contact_count": (ctypes.c_uint32, [ctypes.c_void_p]),
+    "prx_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_void_p,
+                                        ctypes.POINTER(ctypes.c_void_p)]),
+    "prx_batch_destroy": (None, [ctypes.c_void_p]),
+    "prx_batch_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
+    "prx_batch_sync": (ctypes.c_int, [ctypes.c_void_p]),
+    "prx_batch_download": (ctypes.c_int, [ctypes.c_void_p]),
+    "prx_batch_last_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float),
+                                             ctypes.POINTER(ctypes.c_int)]),
+    "prx_batch_dynamic_bodies": (ctypes.c_uint64, [ctypes.c_void_p]),
+    "prx_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "prx_selftest_sincos": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
+}
+
+
+def lib():
+    """Loads libplayrho_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libplayrho_amd.so is not built: run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return sorted(_SIGS)
+
+
+def _check(code):
+    if code != PRX_OK:
+        raise PrxError(code, lib().prx_last_error().decode())
+
+
+def device_available():
+    """True when a HIP device is visible (counting devices does not start the
+    device runtime in this process)."""
+    try:
+        import torch
+        return torch.cuda.device_count() > 0
+    except Exception:  # pragma: no cover
+        return False
+
+
+def require_device():
+    if not device_available():
+        raise RuntimeError("playrho_amd needs a HIP device: the step path has no CPU fallback")
+
+
+class World:
+    """A PlayRho world whose steps run on the device (plain per-world form)."""
+
+    def __init__(self, conf=None):
+        self.h = ctypes.c_void_p()
+        _check(lib().prx_world_create(ctypes.byref(conf) if conf is not None else None, ctypes.byref(self.h)))
+
+    @classmethod
+    def from_desc(cls, desc: bytes, conf=None):
+        w = cls(conf)
+        _check(lib().prx_world_load_desc(w.h, desc, len(desc)))
+        return w
+
+    @classmethod
+    def scene(cls, name, params=(), conf=None):
+        w = cls(conf)
+        w.build_scene(name, params)
+        return w
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and h.value and _lib is not None:
+            _lib.prx_world_destroy(h)
+            self.h = None
+
+    def build_scene(self, name, params=()):
+        arr = (ctypes.c_int32 * max(1, len(params)))(*params)
+        _check(lib().prx_scene_build(self.h, name.encode(), arr, len(params)))
+
+    def create_shape(self, conf: ShapeConf):
+        sid = ctypes.c_uint32()
+        _check(lib().prx_shape_create(self.h, ctypes.byref(conf), ctypes.byref(sid)))
+        return sid.value
+
+    def create_body(self, conf: BodyConf):
+        bid = ctypes.c_uint32()
+        _check(lib().prx_body_create(self.h, ctypes.byref(conf), ctypes.byref(bid)))
+        return bid.value
+
+    @property
+    def body_count(self):
+        return lib().prx_world_body_count(self.h)
+
+    def step(self, conf: StepConf = None):
+        conf = conf or StepConf()
+        st = StepStats()
+        _check(lib().prx_step(self.h, ctypes.byref(conf), ctypes.byref(st)))
+        return st
+
+    def bodies(self):
+        n = self.body_count
+        out = np.zeros((n, 10), dtype=np.float32)
+        flags = np.zeros(n, dtype=np.uint32)
+        _check(lib().prx_world_get_bodies(self.h, out.ctypes.data, flags.ctypes.data))
+        return out, flags
+
+    def transformation(self, body):
+        out = (ctypes.c_float * 4)()
+        _check(lib().prx_world_get_transformation(self.h, body, out))
+        return tuple(out)
+
+    def set_velocity(self, body, v):
+        arr = (ctypes.c_float * 3)(*v)
+        _check(lib().prx_world_set_velocity(self.h, body, arr))
+
+    def state_record(self) -> bytes:
+        n = ctypes.c_size_t()
+        _check(lib().prx_world_state_record(self.h, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _check(lib().prx_world_state_record(self.h, buf, n.value, ctypes.byref(n)))
+        return buf.raw[:n.value]
+
+    def save_desc(self) -> bytes:
+        n = ctypes.c_size_t()
+        _check(lib().prx_world_save_desc(self.h, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _check(lib().prx_world_save_desc(self.h, buf, n.value, ctypes.byref(n)))
+        return buf.raw[:n.value]
+
+    def contact_count(self):
+        return lib().prx_world_contact_count(self.h)
+
+
+class Batch:
+    """Many worlds resident on the device, stepped by one launch per step."""
+
+    def __init__(self, worlds, use_graph=True, order=0):
+        self.worlds = list(worlds)
+        conf = BatchConf()
+        lib().prx_batch_conf_default(ctypes.byref(conf))
+        conf.use_graph = 1 if use_graph else 0
+        conf.order = order
+        arr = (ctypes.c_void_p * len(self.worlds))(*[w.h.value for w in self.worlds])
+        self.h = ctypes.c_void_p()
+        _check(lib().prx_batch_create(arr, len(self.worlds), ctypes.byref(conf), ctypes.byref(self.h)))
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and h.value and _lib is not None:
+            _lib.prx_batch_destroy(h)
+            self.h = None
+
+    def step(self, conf: StepConf = None, n=1):
+        conf = conf or StepConf()
+        _check(lib().prx_batch_step(self.h, ctypes.byref(conf), n))
+
+    def sync(self):
+        _check(lib().prx_batch_sync(self.h))
+
+    def download(self):
+        _check(lib().prx_batch_download(self.h))
+
+    def last_timing(self):
+        ms = ctypes.c_float()
+        n = ctypes.c_int()
+        _check(lib().prx_batch_last_timing(self.h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def dynamic_bodies(self):
+        return lib().prx_batch_dynamic_bodies(self.h)
+
+
+def device_sincos(angles):
+    """Device evaluation of UnitVec::Get(angle) for an array of float angles."""
+    a = np.ascontiguousarray(angles, dtype=np.float32)
+    out = np.zeros((a.size, 2), dtype=np.float32)
+    code = lib().prx_selftest_sincos(a.ctypes.data, out.ctypes.data, a.size)
+    if code:
+        raise PrxError(code, "device sincos self-test failed")
+    return out

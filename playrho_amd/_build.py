@@ -1,0 +1,71 @@
+"""Builds the in-tree native libraries.
+
+* playrho_amd/libplayrho_amd.so: the C ABI (host C++17) and the gfx950 HIP
+  kernels, compiled with hipcc for gfx950 only.  Float semantics: no FMA
+  contraction, IEEE denormals, correctly rounded division and square root.
+* oracle/_build/liboracle.so: the CPU restatement used as the checker.
+* oracle/_ref/: the reference float build and golden driver (only where the
+  reference checkout is present).
+"""
+import os
+import shutil
+import subprocess
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIB = os.path.join(PKG, "libplayrho_amd.so")
+ORACLE_LIB = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    "-ffp-contract=off", "-fno-fast-math", "-fno-gpu-flush-denormals-to-zero",
+    "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+]
+
+
+def _run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout[-4000:], r.stderr[-8000:]))
+    return r
+
+
+def build_library(out=LIB):
+    objs = []
+    tmp = os.path.join(PKG, "_build")
+    os.makedirs(tmp, exist_ok=True)
+    for src in ("step_kernel.hip", "host_world.cpp"):
+        obj = os.path.join(tmp, src + ".o")
+        lang = ["-x", "hip"] if src.endswith(".hip") else []
+        _run([HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    tmp_out = out + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp_out] + objs)
+    os.replace(tmp_out, out)
+    return out
+
+
+def build_oracle(out=ORACLE_LIB):
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
+          os.path.join(ROOT, "oracle", "cpu_ref.cpp"), "-o", out, "-lpthread"])
+    return out
+
+
+def build_reference():
+    """Builds oracle/_ref from /root/reference when present (here only)."""
+    if not os.path.isdir("/root/reference/Library/source"):
+        return None
+    jobs = str(min(16, os.cpu_count() or 4))
+    _run(["make", "-f", "oracle/Makefile.ref", "-j" + jobs, "all"], cwd=ROOT)
+    return os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+
+
+def build_all():
+    build_library()
+    build_oracle()
+    build_reference()

@@ -1,0 +1,1408 @@
+// Host side of the C ABI: the world model (shapes, bodies, proxies) built
+// through the playrho::d2 construction calls, packing into the device layout,
+// batch management and stepping.  Shape geometry and mass properties are
+// computed here exactly as the reference computes them, in float:
+//
+//   PolygonShapeConf::SetAsBox / Set (weld + gift wrap)  PolygonShapeConf.cpp:45-134,
+//                                                         DistanceProxy.cpp:42-105
+//   forward normals                                       d2/Math.cpp:106-124
+//   disk / edge / polygon mass data                       MassData.cpp:40-190
+//   ComputeMassData over a body's shapes                  WorldShape.cpp:160-176
+//   SetMassData                                           WorldBody.cpp:365-399
+//   Body flags from BodyConf                              Body.cpp:32-120
+//   tree leaf (proxy) ids of a growing tree               DynamicTree.cpp:456-537
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/playrho_amd.h"
+#include "layout.hpp"
+#include "scenes.hpp"
+#include "sincosf.hpp"
+
+extern "C" hipError_t prx_launch_step(const prx::DeviceState* d, const prx::StepParams* c, int first, int count,
+                                      hipStream_t stream);
+
+namespace {
+
+thread_local std::string g_error;
+
+int Fail(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+#define PRX_HIP(call)                                                                          \
+    do {                                                                                       \
+        const hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) return Fail(PRX_E_DEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+struct V2 {
+    float x = 0, y = 0;
+};
+inline V2 operator+(V2 a, V2 b) { return {a.x + b.x, a.y + b.y}; }
+inline V2 operator-(V2 a, V2 b) { return {a.x - b.x, a.y - b.y}; }
+inline V2 Mul(V2 a, float s) { return {a.x * s, a.y * s}; }
+inline V2 Mul(float s, V2 a) { return {s * a.x, s * a.y}; }
+inline V2 DivS(V2 a, float s) {
+    const float inv = 1.0f / s;
+    return {a.x * inv, a.y * inv};
+}
+inline float Cross(V2 a, V2 b) {
+    const float m = a.x * b.y;
+    const float s = a.y * b.x;
+    return m - s;
+}
+inline float MagSq(V2 v) {
+    float r = 0.0f;
+    r += v.x * v.x;
+    r += v.y * v.y;
+    return r;
+}
+inline V2 Rotate(V2 v, V2 q) { return {(q.x * v.x) - (q.y * v.y), (q.y * v.x) + (q.x * v.y)}; }
+inline V2 UnitVector(V2 v, V2 fallback) {
+    const float m2 = v.x * v.x + v.y * v.y;
+    if (std::isnormal(m2)) {
+        const float mag = std::sqrt(m2);
+        const float inv = 1.0f / mag;
+        return {v.x * inv, v.y * inv};
+    }
+    const float mag = float(std::sqrt(double(v.x) * v.x + double(v.y) * v.y));
+    if (!std::isnormal(mag)) return fallback;
+    return {v.x / mag, v.y / mag};
+}
+inline V2 UnitFromAngle(float a) { return {prx_math::CosF<true>(a), prx_math::SinF<true>(a)}; }
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kDefaultLinearSlop = 0.005f;
+constexpr int kMaxShapeVertices = 16;
+
+struct HostShape {
+    std::vector<V2> v, n;
+    float radius = 0.01f, density = 0, friction = 0.2f, restitution = 0;
+    std::uint32_t sensor = 0, catmask = 0xffff0001u;
+    std::int32_t group = 0;
+};
+
+struct MassData {
+    V2 center;
+    float mass = 0, I = 0;
+};
+
+std::vector<V2> FwdNormals(const std::vector<V2>& v) {
+    std::vector<V2> n;
+    const std::size_t count = v.size();
+    if (count > 1) {
+        for (std::size_t i = 0; i < count; ++i) {
+            const V2 e = v[(i + 1 < count) ? i + 1 : 0] - v[i];
+            n.push_back(UnitVector(V2{e.y, -e.x}, V2{0, 0}));
+        }
+    } else if (count == 1) {
+        n.push_back(V2{0, 0});
+    }
+    return n;
+}
+
+std::vector<V2> ConvexHull(const std::vector<V2>& pts) {
+    std::vector<V2> out;
+    const std::size_t nv = pts.size();
+    if (nv == 0) return out;
+    std::size_t i0 = 0;
+    float maxx = pts[0].x;
+    for (std::size_t i = 1; i < nv; ++i) {
+        const float x = pts[i].x;
+        if ((maxx < x) || ((maxx == x) && (pts[i].y < pts[i0].y))) {
+            maxx = x;
+            i0 = i;
+        }
+    }
+    std::vector<std::size_t> hull;
+    std::size_t ih = i0;
+    for (;;) {
+        hull.push_back(ih);
+        std::size_t ie = 0;
+        for (std::size_t j = 1; j < nv; ++j) {
+            if (ie == ih) {
+                ie = j;
+                continue;
+            }
+            const V2 r = pts[ie] - pts[ih];
+            const V2 v = pts[j] - pts[ih];
+            const float c = Cross(r, v);
+            if ((c < 0.0f) || ((c == 0.0f) && (MagSq(v) > MagSq(r)))) ie = j;
+        }
+        ih = ie;
+        if (ie == i0) break;
+        if (hull.size() > nv) break;
+    }
+    for (std::size_t k : hull) out.push_back(pts[k]);
+    return out;
+}
+
+MassData DiskMass(float r, float density, V2 loc) {
+    const float r2 = r * r;
+    const float area = r2 * kPi;
+    const float mass = density * area;
+    const float Iz = area * ((r2 / 2.0f) + MagSq(loc));
+    const float I = Iz * density / 1.0f;
+    return MassData{loc, mass, I};
+}
+
+float PolarMoment(const V2* v, std::size_t count) {
+    float sx = 0.0f, sy = 0.0f;
+    for (std::size_t i = 0; i < count; ++i) {
+        const V2 a = v[i];
+        const V2 b = v[(i + 1 < count) ? i + 1 : 0];
+        const float fb = Cross(a, b);
+        sx += ((a.y * a.y) + a.y * b.y + (b.y * b.y)) * fb;
+        sy += ((a.x * a.x) + a.x * b.x + (b.x * b.x)) * fb;
+    }
+    return (sx + sy) * (1.0f / 12.0f);
+}
+
+MassData EdgeMass(float r, float density, V2 v0, V2 v1) {
+    const float r2 = r * r;
+    const float circleArea = r2 * kPi;
+    const float circleMass = density * circleArea;
+    const V2 d = v1 - v0;
+    const V2 u = UnitVector(d, V2{0, 0});
+    const V2 offset = Mul(V2{-u.y, u.x}, r);
+    const float b = std::sqrt(MagSq(d));
+    const float h = r * 2.0f;
+    const float rectMass = density * b * h;
+    const float totalMass = circleMass + rectMass;
+    const V2 center = DivS(v0 + v1, 2.0f);
+    const float halfCircleArea = circleArea / 2.0f;
+    const float halfR2 = r2 / 2.0f;
+    const V2 verts[4] = {v0 + offset, v0 - offset, v1 - offset, v1 + offset};
+    const float Iz = PolarMoment(verts, 4);
+    const float I0 = halfCircleArea * (halfR2 + MagSq(v0));
+    const float I1 = halfCircleArea * (halfR2 + MagSq(v1));
+    const float I = (I0 + I1 + Iz) * density / 1.0f;
+    return MassData{center, totalMass, I};
+}
+
+MassData PolygonMass(float r, float density, const std::vector<V2>& v) {
+    const std::size_t count = v.size();
+    if (count == 0) return MassData{};
+    if (count == 1) return DiskMass(r, density, v[0]);
+    if (count == 2) return EdgeMass(r, density, v[0], v[1]);
+    V2 center{0, 0};
+    float area = 0.0f, I = 0.0f;
+    V2 sum{0, 0};
+    for (const V2& p : v) sum = sum + p;
+    const V2 s = DivS(sum, float(count));
+    for (std::size_t i = 0; i < count; ++i) {
+        const V2 e1 = v[i] - s;
+        const V2 e2 = v[(i + 1 < count) ? i + 1 : 0] - s;
+        const float D = Cross(e1, e2);
+        const float triangleArea = D * (1.0f / 2.0f);
+        area += triangleArea;
+        center = center + Mul(Mul(triangleArea, e1 + e2), 1.0f / 3.0f);
+        const float intx2 = (e1.x * e1.x) + e2.x * e1.x + (e2.x * e2.x);
+        const float inty2 = (e1.y * e1.y) + e2.y * e1.y + (e2.y * e2.y);
+        const float triangleI = D * (intx2 + inty2) * (1.0f / 12.0f);
+        I += triangleI;
+    }
+    const float mass = density * area;
+    center = ((area > 0.0f) && !(std::abs(area) < 1.17549435e-38f)) ? DivS(center, area) : V2{0, 0};
+    const V2 mcenter = center + s;
+    const float lever = MagSq(mcenter) - MagSq(center);
+    const float massI = ((density * I) + (mass * lever)) / 1.0f;
+    return MassData{mcenter, mass, massI};
+}
+
+MassData ShapeMass(const HostShape& s) { return PolygonMass(s.radius, s.density, s.v); }
+
+struct HostBody {
+    std::uint32_t type = PRX_STATIC, flags = 0;
+    float pos0[3] = {0, 0, 0}, pos1[3] = {0, 0, 0};
+    V2 lc;
+    float vel[3] = {0, 0, 0};
+    float under = 0;
+    float acc[3] = {0, 0, 0};
+    float damp[2] = {0, 0};
+    float invMass = 0, invRotI = 0;
+    std::vector<std::uint32_t> shapes;
+    std::vector<int> proxies;  // world-local proxy indices
+};
+
+struct HostProxy {
+    int body = 0, shape = 0;
+    std::uint32_t id = 0, flags = prx::kProxyPending;
+    float aabb[4] = {0, 0, 0, 0};
+};
+
+// Contact records as held on the host between device residencies.
+struct HostContacts {
+    std::vector<int2> prox, body, shape;
+    std::vector<std::uint32_t> id, flags;
+    std::vector<float4> mat, geo, p0, p1;
+    std::vector<uint4> hdr;
+    std::vector<std::uint32_t> freelist;
+    int pool = 0;
+    std::size_t size() const { return id.size(); }
+    void resize(std::size_t n) {
+        prox.resize(n);
+        body.resize(n);
+        shape.resize(n);
+        id.resize(n);
+        flags.resize(n);
+        mat.resize(n);
+        geo.resize(n);
+        p0.resize(n);
+        p1.resize(n);
+        hdr.resize(n);
+    }
+};
+
+}  // namespace
+
+struct prx_batch;
+
+struct prx_world {
+    prx_world_conf conf{};
+    std::vector<HostShape> shapes;
+    std::vector<HostBody> bodies;
+    std::vector<HostProxy> proxies;
+    HostContacts contacts;
+    std::uint32_t leafCount = 0;
+    prx_batch* owner = nullptr;  // batch holding the authoritative device state
+    int index_in_owner = -1;
+    bool host_stale = false;     // device state newer than this mirror
+    prx_batch* solo = nullptr;   // single-world batch used by prx_step
+    prx_step_stats last_stats{};
+};
+
+namespace {
+
+// Device arrays of one batch.
+struct DeviceArena {
+    std::vector<void*> ptrs;
+    ~DeviceArena() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <class T>
+    hipError_t alloc(T*& out, std::size_t n) {
+        void* p = nullptr;
+        const hipError_t e = hipMalloc(&p, std::max<std::size_t>(n, 1) * sizeof(T));
+        if (e == hipSuccess) ptrs.push_back(p);
+        out = static_cast<T*>(p);
+        return e;
+    }
+};
+
+}  // namespace
+
+struct prx_batch {
+    std::vector<prx_world*> worlds;
+    prx_batch_conf conf{};
+    DeviceArena arena;
+    prx::DeviceState d{};
+    std::vector<prx::WorldHeader> hdr;
+    std::vector<int> shape_base, vert_base;
+    int* d_schedule = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0;
+    int last_launches = 0;
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    prx::StepParams graph_params{};
+    bool graph_valid = false;
+    std::size_t totB = 0, totP = 0, totS = 0, totV = 0, totC = 0;
+    ~prx_batch() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int CheckWorld(prx_world* w) {
+    if (!w) return Fail(PRX_E_INVALID_ARGUMENT, "null world");
+    return PRX_OK;
+}
+
+// Copies the batch's device state for every member world back to the host.
+int DownloadBatch(prx_batch* b);
+
+// Brings a world's host mirror up to date (device -> host).
+int Refresh(prx_world* w) {
+    if (w->owner && w->host_stale) return DownloadBatch(w->owner);
+    return PRX_OK;
+}
+
+// Leaves the current batch (device state copied back first).
+int Detach(prx_world* w) {
+    if (!w->owner) return PRX_OK;
+    int st = Refresh(w);
+    if (st != PRX_OK) return st;
+    prx_batch* b = w->owner;
+    for (auto& m : b->worlds)
+        if (m == w) m = nullptr;
+    w->owner = nullptr;
+    w->index_in_owner = -1;
+    if (w->solo == b) {
+        w->solo = nullptr;
+        delete b;
+    }
+    return PRX_OK;
+}
+
+int AddShape(prx_world* w, const prx_shape_conf& c, std::uint32_t* out) {
+    HostShape s;
+    const float defaultRadius = kDefaultLinearSlop * 2.0f;
+    switch (c.kind) {
+    case PRX_SHAPE_DISK: {
+        s.radius = (c.vertex_radius >= 0) ? c.vertex_radius : c.radius;
+        s.v = {V2{c.location[0], c.location[1]}};
+        break;
+    }
+    case PRX_SHAPE_BOX: {
+        s.radius = (c.vertex_radius >= 0) ? c.vertex_radius : defaultRadius;
+        const float hx = c.hx, hy = c.hy;
+        s.v = {V2{+hx, -hy}, V2{+hx, +hy}, V2{-hx, +hy}, V2{-hx, -hy}};
+        break;
+    }
+    case PRX_SHAPE_POLYGON: {
+        s.radius = (c.vertex_radius >= 0) ? c.vertex_radius : defaultRadius;
+        if (!c.points && c.npoints) return Fail(PRX_E_INVALID_ARGUMENT, "polygon points missing");
+        // VertexSet weld at Square(DefaultLinearSlop) (VertexSet.hpp:67-108).
+        const float minSepSq = kDefaultLinearSlop * kDefaultLinearSlop;
+        std::vector<V2> set;
+        for (std::uint32_t i = 0; i < c.npoints; ++i) {
+            const V2 p{c.points[2 * i], c.points[2 * i + 1]};
+            bool dup = false;
+            for (const V2& e : set) {
+                if (MagSq(p - e) <= minSepSq) {
+                    dup = true;
+                    break;
+                }
+            }
+            if (!dup) set.push_back(p);
+        }
+        s.v = ConvexHull(set);
+        break;
+    }
+    case PRX_SHAPE_EDGE: {
+        s.radius = (c.vertex_radius >= 0) ? c.vertex_radius : defaultRadius;
+        if (!c.points || c.npoints < 2) return Fail(PRX_E_INVALID_ARGUMENT, "edge needs two points");
+        s.v = {V2{c.points[0], c.points[1]}, V2{c.points[2], c.points[3]}};
+        break;
+    }
+    default:
+        return Fail(PRX_E_INVALID_ARGUMENT, "unknown shape kind");
+    }
+    if (s.v.empty()) return Fail(PRX_E_INVALID_ARGUMENT, "shape has no vertices");
+    if (int(s.v.size()) > kMaxShapeVertices) return Fail(PRX_E_UNSUPPORTED, "more than 16 vertices");
+    // Vertex radius bounds (AabbTreeWorld.cpp:1243-1248).
+    if (!(s.radius >= w->conf.min_vertex_radius) || !(s.radius <= w->conf.max_vertex_radius))
+        return Fail(PRX_E_INVALID_ARGUMENT, "CreateShape: vertex radius out of range");
+    s.n = FwdNormals(s.v);
+    s.density = c.density;
+    s.friction = c.friction;
+    s.restitution = c.restitution;
+    s.sensor = c.sensor ? 1u : 0u;
+    s.catmask = std::uint32_t(c.category_bits) | (std::uint32_t(c.mask_bits) << 16);
+    s.group = c.group_index;
+    w->shapes.push_back(std::move(s));
+    if (out) *out = std::uint32_t(w->shapes.size() - 1);
+    return PRX_OK;
+}
+
+int AddBody(prx_world* w, const prx_body_conf& c, std::uint32_t* out) {
+    if (w->bodies.size() >= 65534) return Fail(PRX_E_LENGTH, "CreateBody: operation would exceed MaxBodies");
+    for (std::uint32_t i = 0; i < c.nshapes; ++i)
+        if (c.shapes[i] >= w->shapes.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such shape");
+    HostBody b;
+    b.type = c.type;
+    // Body::GetFlags(BodyConf) (Body.cpp:32-87).
+    std::uint32_t f = 0;
+    if (c.type == PRX_DYNAMIC) f |= prx::kSpeedable | prx::kAccelerable;
+    else if (c.type == PRX_KINEMATIC) f |= prx::kImpenetrable | prx::kSpeedable;
+    else f |= prx::kImpenetrable;
+    if (c.bullet) f |= prx::kImpenetrable;
+    if (c.fixed_rotation) f |= prx::kFixedRotation;
+    if (c.allow_sleep) f |= prx::kAutoSleep;
+    if (c.awake) {
+        if (f & prx::kSpeedable) f |= prx::kAwake;
+    } else if (!c.allow_sleep && (f & prx::kSpeedable)) {
+        f |= prx::kAwake;
+    }
+    if (c.enabled) f |= prx::kEnabled;
+    b.flags = f;
+    b.pos0[0] = b.pos1[0] = c.location[0];
+    b.pos0[1] = b.pos1[1] = c.location[1];
+    b.pos0[2] = b.pos1[2] = c.angle;
+    b.damp[0] = c.linear_damping;
+    b.damp[1] = c.angular_damping;
+    // Velocity (Body::SetVelocity, Body.cpp:184-194): non-speedable bodies
+    // keep zero velocity.
+    const bool nonzeroVel = c.linear_velocity[0] != 0 || c.linear_velocity[1] != 0 || c.angular_velocity != 0;
+    if (!nonzeroVel || (f & prx::kSpeedable)) {
+        b.vel[0] = c.linear_velocity[0];
+        b.vel[1] = c.linear_velocity[1];
+        b.vel[2] = c.angular_velocity;
+    }
+    // Acceleration (Body::SetAcceleration, Body.cpp:203-234).
+    const bool nonzeroAcc = c.linear_acceleration[0] != 0 || c.linear_acceleration[1] != 0 ||
+                            c.angular_acceleration != 0;
+    if (!nonzeroAcc || (f & prx::kAccelerable)) {
+        b.acc[0] = c.linear_acceleration[0];
+        b.acc[1] = c.linear_acceleration[1];
+        b.acc[2] = c.angular_acceleration;
+    }
+    b.shapes.assign(c.shapes, c.shapes + c.nshapes);
+    // ResetMassData -> SetMassData (WorldBody.cpp:360-399).
+    if (!(f & prx::kAccelerable)) {
+        b.invMass = 0;
+        b.invRotI = 0;
+    } else {
+        float mass = 0, I = 0;
+        V2 wc{0, 0};
+        for (std::uint32_t sid : b.shapes) {
+            const HostShape& s = w->shapes[sid];
+            if (s.density > 0) {
+                const MassData md = ShapeMass(s);
+                mass += md.mass;
+                wc = wc + Mul(md.mass / 1.0f, md.center);
+                I += md.I;
+            }
+        }
+        const V2 center = (mass > 0) ? DivS(wc, mass / 1.0f) : V2{0, 0};
+        const float m = (mass > 0) ? mass : 1.0f;
+        b.invMass = 1.0f / m;
+        b.invRotI = 0.0f / 1.0f;
+        if ((I > 0) && !(f & prx::kFixedRotation)) {
+            const float lsq = MagSq(center);
+            const float Ic = I - (m * lsq) / 1.0f;
+            b.invRotI = 1.0f / Ic;
+        }
+        // Move the center of mass (sweep) and update the velocity.
+        const V2 q = UnitFromAngle(b.pos1[2]);
+        const V2 xfp = V2{b.pos1[0], b.pos1[1]} - Rotate(V2{0, 0}, q);
+        const V2 oldCenter{b.pos1[0], b.pos1[1]};
+        const V2 newc = Rotate(center, q) + xfp;
+        b.pos0[0] = b.pos1[0] = newc.x;
+        b.pos0[1] = b.pos1[1] = newc.y;
+        b.lc = center;
+        const V2 dc = newc - oldCenter;
+        const float wv = b.vel[2] / 1.0f;
+        b.vel[0] += -dc.y * wv;
+        b.vel[1] += dc.x * wv;
+    }
+    const int bid = int(w->bodies.size());
+    if (f & prx::kEnabled) {
+        for (std::uint32_t sid : b.shapes) {
+            HostProxy p;
+            p.body = bid;
+            p.shape = int(sid);
+            p.id = (w->leafCount == 0) ? 0u : 2u * w->leafCount - 1u;
+            ++w->leafCount;
+            p.flags = prx::kProxyPending;
+            b.proxies.push_back(int(w->proxies.size()));
+            w->proxies.push_back(p);
+        }
+    }
+    w->bodies.push_back(std::move(b));
+    if (out) *out = std::uint32_t(bid);
+    return PRX_OK;
+}
+
+prx::StepParams ToParams(const prx_step_conf& c) {
+    prx::StepParams p{};
+    p.dt = c.delta_time;
+    p.dtRatio = c.dt_ratio;
+    p.minStillTimeToSleep = c.min_still_time_to_sleep;
+    p.linearSlop = c.linear_slop;
+    p.angularSlop = c.angular_slop;
+    p.regResolutionRate = c.reg_resolution_rate;
+    p.regMinSeparation = c.reg_min_separation;
+    p.regMinMomentum = c.reg_min_momentum;
+    p.velocityThreshold = c.velocity_threshold;
+    p.maxTranslation = c.max_translation;
+    p.maxRotation = c.max_rotation;
+    p.maxLinearCorrection = c.max_linear_correction;
+    p.maxAngularCorrection = c.max_angular_correction;
+    p.linearSleepTolerance = c.linear_sleep_tolerance;
+    p.angularSleepTolerance = c.angular_sleep_tolerance;
+    p.displaceMultiplier = c.displace_multiplier;
+    p.aabbExtension = c.aabb_extension;
+    p.maxCirclesRatio = c.max_circles_ratio;
+    p.velIters = c.reg_velocity_iters;
+    p.posIters = c.reg_position_iters;
+    p.warmStart = c.do_warm_start;
+    p.blockSolve = c.do_blocksolve;
+    return p;
+}
+
+int ContactCap(const prx_world* w) {
+    const int per = w->conf.contact_capacity_per_proxy ? int(w->conf.contact_capacity_per_proxy) : 8;
+    return std::max<int>(int(w->contacts.size()) + 64, per * int(w->proxies.size()) + 64);
+}
+
+template <class T>
+int Upload(T* dst, const std::vector<T>& src) {
+    if (!src.empty()) PRX_HIP(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return PRX_OK;
+}
+
+int BuildBatch(prx_batch* b) {
+    const int nw = int(b->worlds.size());
+    std::size_t B = 0, P = 0, S = 0, V = 0, C = 0;
+    b->hdr.assign(std::size_t(nw), prx::WorldHeader{});
+    b->shape_base.assign(std::size_t(nw), 0);
+    b->vert_base.assign(std::size_t(nw), 0);
+    for (int i = 0; i < nw; ++i) {
+        prx_world* w = b->worlds[std::size_t(i)];
+        if (w->proxies.size() >= 65535) return Fail(PRX_E_LENGTH, "more than 65534 proxies in one world");
+        auto& h = b->hdr[std::size_t(i)];
+        h.body_off = int(B);
+        h.nbodies = int(w->bodies.size());
+        h.proxy_off = int(P);
+        h.nproxies = int(w->proxies.size());
+        h.contact_off = int(C);
+        h.contact_cap = ContactCap(w);
+        h.ncontacts = int(w->contacts.size());
+        h.pool_size = w->contacts.pool;
+        h.nfree = int(w->contacts.freelist.size());
+        h.status = 0;
+        b->shape_base[std::size_t(i)] = int(S);
+        b->vert_base[std::size_t(i)] = int(V);
+        B += w->bodies.size();
+        P += w->proxies.size();
+        S += w->shapes.size();
+        for (const auto& s : w->shapes) V += s.v.size();
+        C += std::size_t(h.contact_cap);
+    }
+    b->totB = B;
+    b->totP = P;
+    b->totS = S;
+    b->totV = V;
+    b->totC = C;
+    auto& a = b->arena;
+    auto& d = b->d;
+    d.nworlds = nw;
+    PRX_HIP(a.alloc(d.hdr, std::size_t(nw)));
+    PRX_HIP(a.alloc(b->d_schedule, std::size_t(nw)));
+    PRX_HIP(a.alloc(d.bflags, B));
+    PRX_HIP(a.alloc(d.pos0, B));
+    PRX_HIP(a.alloc(d.pos1, B));
+    PRX_HIP(a.alloc(d.vel, B));
+    PRX_HIP(a.alloc(d.xf, B));
+    PRX_HIP(a.alloc(d.lcinv, B));
+    PRX_HIP(a.alloc(d.acc, B));
+    PRX_HIP(a.alloc(d.damp, B));
+    PRX_HIP(a.alloc(d.bproxy, B));
+    PRX_HIP(a.alloc(d.bproxy_list, P));
+    PRX_HIP(a.alloc(d.pbody, P));
+    PRX_HIP(a.alloc(d.pshape, P));
+    PRX_HIP(a.alloc(d.pid, P));
+    PRX_HIP(a.alloc(d.paabb, P));
+    PRX_HIP(a.alloc(d.pflags, P));
+    PRX_HIP(a.alloc(d.shape_i, S));
+    PRX_HIP(a.alloc(d.shape_f, S));
+    PRX_HIP(a.alloc(d.shape_catmask, S));
+    PRX_HIP(a.alloc(d.verts, V));
+    PRX_HIP(a.alloc(d.norms, V));
+    PRX_HIP(a.alloc(d.c_prox, C));
+    PRX_HIP(a.alloc(d.c_body, C));
+    PRX_HIP(a.alloc(d.c_shape, C));
+    PRX_HIP(a.alloc(d.c_id, C));
+    PRX_HIP(a.alloc(d.c_flags, C));
+    PRX_HIP(a.alloc(d.c_mat, C));
+    PRX_HIP(a.alloc(d.m_hdr, C));
+    PRX_HIP(a.alloc(d.m_geo, C));
+    PRX_HIP(a.alloc(d.m_p0, C));
+    PRX_HIP(a.alloc(d.m_p1, C));
+    PRX_HIP(a.alloc(d.c_free, C));
+    PRX_HIP(a.alloc(d.s_keys, C));
+    PRX_HIP(a.alloc(d.s_keys2, C));
+    PRX_HIP(a.alloc(d.s_csr_off, B + std::size_t(nw)));
+    PRX_HIP(a.alloc(d.s_csr, 2 * C));
+    PRX_HIP(a.alloc(d.s_stack, B));
+    PRX_HIP(a.alloc(d.s_isl_b, B));
+    PRX_HIP(a.alloc(d.s_isl_c, C));
+    PRX_HIP(a.alloc(d.s_isl_body, B + C));
+    PRX_HIP(a.alloc(d.s_isl_contact, C));
+    PRX_HIP(a.alloc(d.s_isl_start, B + std::size_t(nw)));
+    PRX_HIP(a.alloc(d.s_bc_pos, B));
+    PRX_HIP(a.alloc(d.s_bc_vel, B));
+    PRX_HIP(a.alloc(d.s_vc, C * std::size_t(prx::kVcStride)));
+    // Launch order: largest worlds first (stable), so the long worlds start
+    // early and the small ones fill in behind them.
+    std::vector<int> order(static_cast<std::size_t>(nw));
+    std::iota(order.begin(), order.end(), 0);
+    if (b->conf.order == 0) {
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+            return b->hdr[std::size_t(x)].nbodies + b->hdr[std::size_t(x)].nproxies >
+                   b->hdr[std::size_t(y)].nbodies + b->hdr[std::size_t(y)].nproxies;
+        });
+    }
+    PRX_HIP(hipMemcpy(b->d_schedule, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice));
+    d.schedule = b->d_schedule;
+    return PRX_OK;
+}
+
+// Host mirror -> device for all member worlds.
+int UploadBatch(prx_batch* b) {
+    const std::size_t B = b->totB, P = b->totP, S = b->totS, V = b->totV, C = b->totC;
+    std::vector<std::uint32_t> bflags(B), pid(P), pflags(P), catmask(S), cid(C), cflags(C), cfree(C);
+    std::vector<float4> pos0(B), pos1(B), vel(B), xf(B), lcinv(B), acc(B), paabb(P), shape_f(S);
+    std::vector<float2> damp(B), verts(V), norms(V);
+    std::vector<int2> bproxy(B), cprox(C), cbody(C), cshape(C);
+    std::vector<int> bplist(P), pbody(P), pshape(P);
+    std::vector<int4> shape_i(S);
+    std::vector<float4> cmat(C), mgeo(C), mp0(C), mp1(C);
+    std::vector<uint4> mhdr(C);
+    for (std::size_t wi = 0; wi < b->worlds.size(); ++wi) {
+        const prx_world* w = b->worlds[wi];
+        const auto& h = b->hdr[wi];
+        const int sb = b->shape_base[wi];
+        int vb = b->vert_base[wi];
+        for (std::size_t si = 0; si < w->shapes.size(); ++si) {
+            const HostShape& s = w->shapes[si];
+            const std::size_t g = std::size_t(sb) + si;
+            shape_i[g] = int4{int(s.v.size()), vb, int(s.sensor), s.group};
+            shape_f[g] = float4{s.radius, s.friction, s.restitution, s.density};
+            catmask[g] = s.catmask;
+            for (std::size_t k = 0; k < s.v.size(); ++k) {
+                verts[std::size_t(vb) + k] = float2{s.v[k].x, s.v[k].y};
+                norms[std::size_t(vb) + k] = float2{s.n[k].x, s.n[k].y};
+            }
+            vb += int(s.v.size());
+        }
+        int plist = 0;
+        for (std::size_t bi = 0; bi < w->bodies.size(); ++bi) {
+            const HostBody& hb = w->bodies[bi];
+            const std::size_t g = std::size_t(h.body_off) + bi;
+            bflags[g] = hb.flags;
+            pos0[g] = float4{hb.pos0[0], hb.pos0[1], hb.pos0[2], 0};
+            pos1[g] = float4{hb.pos1[0], hb.pos1[1], hb.pos1[2], 0};
+            vel[g] = float4{hb.vel[0], hb.vel[1], hb.vel[2], hb.under};
+            const V2 q = UnitFromAngle(hb.pos1[2]);
+            const V2 p = V2{hb.pos1[0], hb.pos1[1]} - Rotate(hb.lc, q);
+            xf[g] = float4{p.x, p.y, q.x, q.y};
+            lcinv[g] = float4{hb.lc.x, hb.lc.y, hb.invMass, hb.invRotI};
+            acc[g] = float4{hb.acc[0], hb.acc[1], hb.acc[2], 0};
+            damp[g] = float2{hb.damp[0], hb.damp[1]};
+            bproxy[g] = int2{plist, int(hb.proxies.size())};
+            for (int p2 : hb.proxies) bplist[std::size_t(h.proxy_off) + std::size_t(plist++)] = p2;
+        }
+        for (std::size_t pi = 0; pi < w->proxies.size(); ++pi) {
+            const HostProxy& hp = w->proxies[pi];
+            const std::size_t g = std::size_t(h.proxy_off) + pi;
+            pbody[g] = hp.body;
+            pshape[g] = sb + hp.shape;
+            pid[g] = hp.id;
+            pflags[g] = hp.flags;
+            paabb[g] = float4{hp.aabb[0], hp.aabb[1], hp.aabb[2], hp.aabb[3]};
+        }
+        const HostContacts& hc = w->contacts;
+        for (std::size_t ci = 0; ci < hc.size(); ++ci) {
+            const std::size_t g = std::size_t(h.contact_off) + ci;
+            cprox[g] = hc.prox[ci];
+            cbody[g] = hc.body[ci];
+            cshape[g] = int2{sb + hc.shape[ci].x, sb + hc.shape[ci].y};
+            cid[g] = hc.id[ci];
+            cflags[g] = hc.flags[ci];
+            cmat[g] = hc.mat[ci];
+            mhdr[g] = hc.hdr[ci];
+            mgeo[g] = hc.geo[ci];
+            mp0[g] = hc.p0[ci];
+            mp1[g] = hc.p1[ci];
+        }
+        for (std::size_t k = 0; k < hc.freelist.size(); ++k) cfree[std::size_t(h.contact_off) + k] = hc.freelist[k];
+    }
+    auto& d = b->d;
+    int st;
+    if ((st = Upload(d.hdr, b->hdr)) || (st = Upload(d.bflags, bflags)) || (st = Upload(d.pos0, pos0)) ||
+        (st = Upload(d.pos1, pos1)) || (st = Upload(d.vel, vel)) || (st = Upload(d.xf, xf)) ||
+        (st = Upload(d.lcinv, lcinv)) || (st = Upload(d.acc, acc)) || (st = Upload(d.damp, damp)) ||
+        (st = Upload(d.bproxy, bproxy)) || (st = Upload(d.bproxy_list, bplist)) || (st = Upload(d.pbody, pbody)) ||
+        (st = Upload(d.pshape, pshape)) || (st = Upload(d.pid, pid)) || (st = Upload(d.paabb, paabb)) ||
+        (st = Upload(d.pflags, pflags)) || (st = Upload(d.shape_i, shape_i)) || (st = Upload(d.shape_f, shape_f)) ||
+        (st = Upload(d.shape_catmask, catmask)) || (st = Upload(d.verts, verts)) || (st = Upload(d.norms, norms)) ||
+        (st = Upload(d.c_prox, cprox)) || (st = Upload(d.c_body, cbody)) || (st = Upload(d.c_shape, cshape)) ||
+        (st = Upload(d.c_id, cid)) || (st = Upload(d.c_flags, cflags)) || (st = Upload(d.c_mat, cmat)) ||
+        (st = Upload(d.m_hdr, mhdr)) || (st = Upload(d.m_geo, mgeo)) || (st = Upload(d.m_p0, mp0)) ||
+        (st = Upload(d.m_p1, mp1)) || (st = Upload(d.c_free, cfree)))
+        return st;
+    return PRX_OK;
+}
+
+template <class T>
+int Download(std::vector<T>& dst, const T* src, std::size_t n) {
+    dst.resize(n);
+    if (n) PRX_HIP(hipMemcpy(dst.data(), src, n * sizeof(T), hipMemcpyDeviceToHost));
+    return PRX_OK;
+}
+
+int DownloadBatch(prx_batch* b) {
+    PRX_HIP(hipStreamSynchronize(b->stream));
+    const std::size_t B = b->totB, P = b->totP, C = b->totC;
+    auto& d = b->d;
+    std::vector<prx::WorldHeader> hdr;
+    std::vector<std::uint32_t> bflags, pflags, cid, cflags, cfree;
+    std::vector<float4> pos0, pos1, vel, paabb, cmat, mgeo, mp0, mp1;
+    std::vector<int2> cprox, cbody, cshape;
+    std::vector<uint4> mhdr;
+    int st;
+    if ((st = Download(hdr, d.hdr, b->worlds.size())) || (st = Download(bflags, d.bflags, B)) ||
+        (st = Download(pos0, d.pos0, B)) || (st = Download(pos1, d.pos1, B)) || (st = Download(vel, d.vel, B)) ||
+        (st = Download(paabb, d.paabb, P)) || (st = Download(pflags, d.pflags, P)) ||
+        (st = Download(cprox, d.c_prox, C)) || (st = Download(cbody, d.c_body, C)) ||
+        (st = Download(cshape, d.c_shape, C)) || (st = Download(cid, d.c_id, C)) ||
+        (st = Download(cflags, d.c_flags, C)) || (st = Download(cmat, d.c_mat, C)) ||
+        (st = Download(mhdr, d.m_hdr, C)) || (st = Download(mgeo, d.m_geo, C)) || (st = Download(mp0, d.m_p0, C)) ||
+        (st = Download(mp1, d.m_p1, C)) || (st = Download(cfree, d.c_free, C)))
+        return st;
+    b->hdr = hdr;
+    for (std::size_t wi = 0; wi < b->worlds.size(); ++wi) {
+        prx_world* w = b->worlds[wi];
+        if (!w) continue;
+        const auto& h = hdr[wi];
+        const int sb = b->shape_base[wi];
+        for (std::size_t bi = 0; bi < w->bodies.size(); ++bi) {
+            HostBody& hb = w->bodies[bi];
+            const std::size_t g = std::size_t(h.body_off) + bi;
+            hb.flags = bflags[g];
+            hb.pos0[0] = pos0[g].x;
+            hb.pos0[1] = pos0[g].y;
+            hb.pos0[2] = pos0[g].z;
+            hb.pos1[0] = pos1[g].x;
+            hb.pos1[1] = pos1[g].y;
+            hb.pos1[2] = pos1[g].z;
+            hb.vel[0] = vel[g].x;
+            hb.vel[1] = vel[g].y;
+            hb.vel[2] = vel[g].z;
+            hb.under = vel[g].w;
+        }
+        for (std::size_t pi = 0; pi < w->proxies.size(); ++pi) {
+            HostProxy& hp = w->proxies[pi];
+            const std::size_t g = std::size_t(h.proxy_off) + pi;
+            hp.flags = pflags[g];
+            hp.aabb[0] = paabb[g].x;
+            hp.aabb[1] = paabb[g].y;
+            hp.aabb[2] = paabb[g].z;
+            hp.aabb[3] = paabb[g].w;
+        }
+        HostContacts& hc = w->contacts;
+        hc.resize(std::size_t(h.ncontacts));
+        for (int ci = 0; ci < h.ncontacts; ++ci) {
+            const std::size_t g = std::size_t(h.contact_off) + std::size_t(ci);
+            hc.prox[std::size_t(ci)] = cprox[g];
+            hc.body[std::size_t(ci)] = cbody[g];
+            hc.shape[std::size_t(ci)] = int2{cshape[g].x - sb, cshape[g].y - sb};
+            hc.id[std::size_t(ci)] = cid[g];
+            hc.flags[std::size_t(ci)] = cflags[g];
+            hc.mat[std::size_t(ci)] = cmat[g];
+            hc.hdr[std::size_t(ci)] = mhdr[g];
+            hc.geo[std::size_t(ci)] = mgeo[g];
+            hc.p0[std::size_t(ci)] = mp0[g];
+            hc.p1[std::size_t(ci)] = mp1[g];
+        }
+        hc.freelist.assign(cfree.begin() + h.contact_off, cfree.begin() + h.contact_off + h.nfree);
+        hc.pool = h.pool_size;
+        const std::uint32_t* s = h.stats;
+        w->last_stats = prx_step_stats{s[0], s[1], s[2], s[3], s[4], s[5], s[6], s[7]};
+        w->host_stale = false;
+    }
+    return PRX_OK;
+}
+
+int CreateBatch(prx_world* const* worlds, int n, const prx_batch_conf* conf, prx_batch** out) {
+    if (!worlds || n <= 0 || !out) return Fail(PRX_E_INVALID_ARGUMENT, "bad batch arguments");
+    for (int i = 0; i < n; ++i) {
+        if (!worlds[i]) return Fail(PRX_E_INVALID_ARGUMENT, "null world in batch");
+        const int st = Detach(worlds[i]);
+        if (st != PRX_OK) return st;
+    }
+    auto b = std::make_unique<prx_batch>();
+    if (conf) b->conf = *conf;
+    else prx_batch_conf_default(&b->conf);
+    b->worlds.assign(worlds, worlds + n);
+    PRX_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    PRX_HIP(hipEventCreate(&b->ev0));
+    PRX_HIP(hipEventCreate(&b->ev1));
+    int st = BuildBatch(b.get());
+    if (st != PRX_OK) return st;
+    st = UploadBatch(b.get());
+    if (st != PRX_OK) return st;
+    for (int i = 0; i < n; ++i) {
+        worlds[i]->owner = b.get();
+        worlds[i]->index_in_owner = i;
+        worlds[i]->host_stale = false;
+    }
+    *out = b.release();
+    return PRX_OK;
+}
+
+int CheckStatus(prx_batch* b) {
+    // Status words are checked after each synchronising call.
+    std::vector<prx::WorldHeader> hdr(b->worlds.size());
+    PRX_HIP(hipMemcpy(hdr.data(), b->d.hdr, hdr.size() * sizeof(prx::WorldHeader), hipMemcpyDeviceToHost));
+    for (const auto& h : hdr) {
+        if (h.status & prx::kStatusContactOverflow)
+            return Fail(PRX_E_LENGTH, "contact capacity of a world exceeded (raise contact_capacity_per_proxy)");
+        if (h.status & prx::kStatusScratchOverflow)
+            return Fail(PRX_E_LENGTH, "candidate pair scratch of a world exceeded");
+    }
+    return PRX_OK;
+}
+
+int StepBatch(prx_batch* b, const prx_step_conf* conf, int nsteps) {
+    if (!conf) return Fail(PRX_E_INVALID_ARGUMENT, "null step conf");
+    if (conf->do_toi) return Fail(PRX_E_UNSUPPORTED, "doToi = true is not stepped by this engine");
+    if (nsteps < 0) return Fail(PRX_E_INVALID_ARGUMENT, "negative step count");
+    const prx::StepParams p = ToParams(*conf);
+    const int nw = int(b->worlds.size());
+    PRX_HIP(hipEventRecord(b->ev0, b->stream));
+    if (b->conf.use_graph && nsteps > 1) {
+        if (!b->graph_valid || std::memcmp(&p, &b->graph_params, sizeof(p)) != 0) {
+            if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+            if (b->graph) (void)hipGraphDestroy(b->graph);
+            b->graph_exec = nullptr;
+            b->graph = nullptr;
+            PRX_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            const hipError_t le = prx_launch_step(&b->d, &p, 0, nw, b->stream);
+            hipGraph_t g = nullptr;
+            const hipError_t ce = hipStreamEndCapture(b->stream, &g);
+            if (le != hipSuccess) return Fail(PRX_E_DEVICE, std::string("launch: ") + hipGetErrorString(le));
+            if (ce != hipSuccess) return Fail(PRX_E_DEVICE, std::string("capture: ") + hipGetErrorString(ce));
+            b->graph = g;
+            PRX_HIP(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+            b->graph_params = p;
+            b->graph_valid = true;
+        }
+        for (int s = 0; s < nsteps; ++s) PRX_HIP(hipGraphLaunch(b->graph_exec, b->stream));
+    } else {
+        for (int s = 0; s < nsteps; ++s) {
+            const hipError_t e = prx_launch_step(&b->d, &p, 0, nw, b->stream);
+            if (e != hipSuccess) return Fail(PRX_E_DEVICE, std::string("launch: ") + hipGetErrorString(e));
+        }
+    }
+    PRX_HIP(hipEventRecord(b->ev1, b->stream));
+    b->last_launches = nsteps;
+    for (prx_world* w : b->worlds)
+        if (w) w->host_stale = true;
+    return PRX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int prx_abi_version(void) { return PRX_ABI_VERSION; }
+const char* prx_last_error(void) { return g_error.c_str(); }
+
+void prx_step_conf_default(prx_step_conf* c) {
+    // Bit-exact values of a default StepConf in the reference float build.
+    c->delta_time = 1.0f / 60.0f;
+    c->dt_ratio = 1.0f;
+    c->min_still_time_to_sleep = 0.5f;
+    c->linear_slop = 0.005f;
+    c->angular_slop = (3.14159265358979323846f * 2.0f) / 180.0f;
+    c->reg_resolution_rate = 2.0f / 10.0f;
+    c->reg_min_separation = -0.005f * 3.0f;
+    c->reg_min_momentum = 0.0f / 100.0f;
+    c->velocity_threshold = 1.0f;
+    c->max_translation = 2.0f;
+    c->max_rotation = float(179.0L * (3.14159265358979323846264338327950288L / 180.0L));
+    c->max_linear_correction = 0.2f;
+    c->max_angular_correction = (8.0f / 180.0f) * 3.14159265358979323846f;
+    c->linear_sleep_tolerance = 0.01f;
+    c->angular_sleep_tolerance = (3.14159265358979323846f * 2.0f) / 180.0f;
+    c->displace_multiplier = 2.0f;
+    c->aabb_extension = 0.005f * 20.0f;
+    c->max_circles_ratio = 10.0f;
+    c->reg_velocity_iters = 8;
+    c->reg_position_iters = 3;
+    c->do_warm_start = 1;
+    c->do_toi = 0;
+    c->do_blocksolve = 1;
+}
+
+void prx_world_conf_default(prx_world_conf* c) {
+    c->abi_version = PRX_ABI_VERSION;
+    c->min_vertex_radius = 0.01f;
+    c->max_vertex_radius = 255.0f;
+    c->contact_capacity_per_proxy = 8;
+}
+
+void prx_batch_conf_default(prx_batch_conf* c) {
+    c->threads_per_world = 64;
+    c->use_graph = 1;
+    c->order = 0;
+}
+
+int prx_world_create(const prx_world_conf* conf, prx_world** out) {
+    if (!out) return Fail(PRX_E_INVALID_ARGUMENT, "null out");
+    auto* w = new prx_world;
+    if (conf) w->conf = *conf;
+    else prx_world_conf_default(&w->conf);
+    *out = w;
+    return PRX_OK;
+}
+
+void prx_world_destroy(prx_world* w) {
+    if (!w) return;
+    if (w->owner) {
+        prx_batch* b = w->owner;
+        for (auto& m : b->worlds)
+            if (m == w) m = nullptr;
+        if (w->solo == b) delete b;
+    }
+    delete w;
+}
+
+int prx_shape_create(prx_world* w, const prx_shape_conf* conf, std::uint32_t* id) {
+    if (int st = CheckWorld(w)) return st;
+    if (!conf) return Fail(PRX_E_INVALID_ARGUMENT, "null shape conf");
+    if (int st = Detach(w)) return st;
+    return AddShape(w, *conf, id);
+}
+
+int prx_body_create(prx_world* w, const prx_body_conf* conf, std::uint32_t* id) {
+    if (int st = CheckWorld(w)) return st;
+    if (!conf) return Fail(PRX_E_INVALID_ARGUMENT, "null body conf");
+    if (int st = Detach(w)) return st;
+    return AddBody(w, *conf, id);
+}
+
+int prx_scene_build(prx_world* w, const char* name, const std::int32_t* params, int nparams) {
+    if (int st = CheckWorld(w)) return st;
+    if (int st = Detach(w)) return st;
+    prx_scene::SceneSpec spec;
+    if (!name || !prx_scene::Make(name, params, nparams, spec)) return Fail(PRX_E_INVALID_ARGUMENT, "unknown scene");
+    // Shapes are created right before the first body using them, matching the
+    // order of the golden-vector driver.
+    std::vector<std::uint32_t> ids(spec.shapes.size(), 0xffffffffu);
+    for (const auto& bs : spec.bodies) {
+        std::vector<std::uint32_t> bsh;
+        for (int si : bs.shapes) {
+            if (ids[std::size_t(si)] == 0xffffffffu) {
+                const auto& s = spec.shapes[std::size_t(si)];
+                prx_shape_conf c{};
+                c.kind = std::uint32_t(s.kind);
+                c.vertex_radius = s.vertexRadius;
+                c.radius = s.radius;
+                c.hx = s.hx;
+                c.hy = s.hy;
+                std::vector<float> pts;
+                for (const auto& p : s.pts) {
+                    pts.push_back(p.x);
+                    pts.push_back(p.y);
+                }
+                c.points = pts.data();
+                c.npoints = std::uint32_t(s.pts.size());
+                c.density = s.density;
+                c.friction = s.friction;
+                c.restitution = s.restitution;
+                c.category_bits = 1;
+                c.mask_bits = 0xffff;
+                if (int st = AddShape(w, c, &ids[std::size_t(si)])) return st;
+            }
+            bsh.push_back(ids[std::size_t(si)]);
+        }
+        prx_body_conf c{};
+        c.type = std::uint32_t(bs.type);
+        c.location[0] = bs.x;
+        c.location[1] = bs.y;
+        c.angle = bs.angle;
+        c.linear_velocity[0] = bs.vx;
+        c.linear_velocity[1] = bs.vy;
+        c.angular_velocity = bs.w;
+        c.linear_acceleration[0] = bs.ax;
+        c.linear_acceleration[1] = bs.ay;
+        c.allow_sleep = 1;
+        c.awake = 1;
+        c.enabled = 1;
+        c.shapes = bsh.data();
+        c.nshapes = std::uint32_t(bsh.size());
+        if (int st = AddBody(w, c, nullptr)) return st;
+    }
+    return PRX_OK;
+}
+
+int prx_world_load_desc(prx_world* w, const std::uint8_t* desc, std::size_t len) {
+    if (int st = CheckWorld(w)) return st;
+    if (!desc) return Fail(PRX_E_INVALID_ARGUMENT, "null description");
+    if (int st = Detach(w)) return st;
+    std::size_t o = 0;
+    bool bad = false;
+    auto u32 = [&]() {
+        std::uint32_t v = 0;
+        if (o + 4 > len) {
+            bad = true;
+            return v;
+        }
+        std::memcpy(&v, desc + o, 4);
+        o += 4;
+        return v;
+    };
+    auto f32 = [&]() {
+        float v = 0;
+        if (o + 4 > len) {
+            bad = true;
+            return v;
+        }
+        std::memcpy(&v, desc + o, 4);
+        o += 4;
+        return v;
+    };
+    prx_world fresh;
+    fresh.conf = w->conf;
+    const std::uint32_t ns = u32();
+    for (std::uint32_t i = 0; i < ns && !bad; ++i) {
+        HostShape s;
+        const std::uint32_t nv = u32();
+        if (nv == 0 || nv > std::uint32_t(kMaxShapeVertices)) return Fail(PRX_E_UNSUPPORTED, "bad vertex count");
+        s.radius = f32();
+        s.density = f32();
+        s.friction = f32();
+        s.restitution = f32();
+        s.sensor = u32();
+        s.catmask = u32();
+        s.group = std::int32_t(u32());
+        for (std::uint32_t k = 0; k < nv; ++k) {
+            const float x = f32();
+            const float y = f32();
+            s.v.push_back(V2{x, y});
+        }
+        for (std::uint32_t k = 0; k < nv; ++k) {
+            const float x = f32();
+            const float y = f32();
+            s.n.push_back(V2{x, y});
+        }
+        fresh.shapes.push_back(std::move(s));
+    }
+    const std::uint32_t nb = u32();
+    for (std::uint32_t i = 0; i < nb && !bad; ++i) {
+        HostBody b;
+        b.type = u32();
+        b.flags = u32();
+        float f[19];
+        for (auto& x : f) x = f32();
+        b.pos0[0] = f[0];
+        b.pos0[1] = f[1];
+        b.pos0[2] = f[2];
+        b.pos1[0] = f[3];
+        b.pos1[1] = f[4];
+        b.pos1[2] = f[5];
+        b.lc = V2{f[6], f[7]};
+        b.vel[0] = f[8];
+        b.vel[1] = f[9];
+        b.vel[2] = f[10];
+        b.acc[0] = f[11];
+        b.acc[1] = f[12];
+        b.acc[2] = f[13];
+        b.damp[0] = f[14];
+        b.damp[1] = f[15];
+        b.invMass = f[16];
+        b.invRotI = f[17];
+        b.under = f[18];
+        const std::uint32_t nsh = u32();
+        for (std::uint32_t k = 0; k < nsh && !bad; ++k) {
+            const std::uint32_t sid = u32();
+            if (sid >= fresh.shapes.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such shape");
+            b.shapes.push_back(sid);
+        }
+        const int bid = int(fresh.bodies.size());
+        if (b.flags & prx::kEnabled) {
+            for (std::uint32_t sid : b.shapes) {
+                HostProxy p;
+                p.body = bid;
+                p.shape = int(sid);
+                p.id = (fresh.leafCount == 0) ? 0u : 2u * fresh.leafCount - 1u;
+                ++fresh.leafCount;
+                b.proxies.push_back(int(fresh.proxies.size()));
+                fresh.proxies.push_back(p);
+            }
+        }
+        fresh.bodies.push_back(std::move(b));
+    }
+    if (bad) return Fail(PRX_E_INVALID_ARGUMENT, "truncated description");
+    w->shapes = std::move(fresh.shapes);
+    w->bodies = std::move(fresh.bodies);
+    w->proxies = std::move(fresh.proxies);
+    w->contacts = HostContacts{};
+    w->leafCount = fresh.leafCount;
+    return PRX_OK;
+}
+
+int prx_world_save_desc(prx_world* w, std::uint8_t* buf, std::size_t cap, std::size_t* len) {
+    if (int st = CheckWorld(w)) return st;
+    if (int st = Refresh(w)) return st;
+    std::vector<std::uint8_t> o;
+    auto put = [&](const void* p, std::size_t n) {
+        const auto* c = static_cast<const std::uint8_t*>(p);
+        o.insert(o.end(), c, c + n);
+    };
+    auto u32 = [&](std::uint32_t v) { put(&v, 4); };
+    auto f32 = [&](float v) { put(&v, 4); };
+    u32(std::uint32_t(w->shapes.size()));
+    for (const auto& s : w->shapes) {
+        u32(std::uint32_t(s.v.size()));
+        f32(s.radius);
+        f32(s.density);
+        f32(s.friction);
+        f32(s.restitution);
+        u32(s.sensor);
+        u32(s.catmask);
+        u32(std::uint32_t(s.group));
+        for (const auto& v : s.v) {
+            f32(v.x);
+            f32(v.y);
+        }
+        for (const auto& v : s.n) {
+            f32(v.x);
+            f32(v.y);
+        }
+    }
+    u32(std::uint32_t(w->bodies.size()));
+    for (const auto& b : w->bodies) {
+        u32(b.type);
+        u32(b.flags & 0x7fu);
+        const float f[19] = {b.pos0[0], b.pos0[1], b.pos0[2], b.pos1[0], b.pos1[1], b.pos1[2], b.lc.x,
+                             b.lc.y,    b.vel[0],  b.vel[1],  b.vel[2],  b.acc[0],  b.acc[1],  b.acc[2],
+                             b.damp[0], b.damp[1], b.invMass, b.invRotI, b.under};
+        put(f, sizeof(f));
+        u32(std::uint32_t(b.shapes.size()));
+        for (auto s : b.shapes) u32(s);
+    }
+    if (len) *len = o.size();
+    if (buf && cap >= o.size()) std::memcpy(buf, o.data(), o.size());
+    else if (buf) return Fail(PRX_E_LENGTH, "buffer too small");
+    return PRX_OK;
+}
+
+std::uint32_t prx_world_body_count(const prx_world* w) { return w ? std::uint32_t(w->bodies.size()) : 0u; }
+std::uint32_t prx_world_shape_count(const prx_world* w) { return w ? std::uint32_t(w->shapes.size()) : 0u; }
+
+int prx_step(prx_world* w, const prx_step_conf* conf, prx_step_stats* stats) {
+    if (int st = CheckWorld(w)) return st;
+    if (!conf) return Fail(PRX_E_INVALID_ARGUMENT, "null step conf");
+    if (w->owner && w->owner != w->solo) {
+        if (int st = Detach(w)) return st;
+    }
+    if (!w->solo) {
+        prx_batch_conf bc;
+        prx_batch_conf_default(&bc);
+        bc.use_graph = 0;
+        prx_batch* b = nullptr;
+        prx_world* one[1] = {w};
+        if (int st = CreateBatch(one, 1, &bc, &b)) return st;
+        w->solo = b;
+    }
+    if (int st = StepBatch(w->solo, conf, 1)) return st;
+    PRX_HIP(hipStreamSynchronize(w->solo->stream));
+    if (int st = CheckStatus(w->solo)) return st;
+    if (stats) {
+        prx::WorldHeader h;
+        PRX_HIP(hipMemcpy(&h, w->solo->d.hdr, sizeof(h), hipMemcpyDeviceToHost));
+        *stats = prx_step_stats{h.stats[0], h.stats[1], h.stats[2], h.stats[3],
+                                h.stats[4], h.stats[5], h.stats[6], h.stats[7]};
+    }
+    return PRX_OK;
+}
+
+int prx_world_get_bodies(prx_world* w, float* out, std::uint32_t* flags) {
+    if (int st = CheckWorld(w)) return st;
+    if (int st = Refresh(w)) return st;
+    for (std::size_t i = 0; i < w->bodies.size(); ++i) {
+        const HostBody& b = w->bodies[i];
+        if (out) {
+            float* o = out + 10 * i;
+            o[0] = b.pos0[0];
+            o[1] = b.pos0[1];
+            o[2] = b.pos0[2];
+            o[3] = b.pos1[0];
+            o[4] = b.pos1[1];
+            o[5] = b.pos1[2];
+            o[6] = b.vel[0];
+            o[7] = b.vel[1];
+            o[8] = b.vel[2];
+            o[9] = b.under;
+        }
+        if (flags) flags[i] = b.flags & 0x7fu;
+    }
+    return PRX_OK;
+}
+
+int prx_world_get_transformation(prx_world* w, std::uint32_t body, float out[4]) {
+    if (int st = CheckWorld(w)) return st;
+    if (body >= w->bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
+    if (int st = Refresh(w)) return st;
+    const HostBody& b = w->bodies[body];
+    const V2 q = UnitFromAngle(b.pos1[2]);
+    const V2 p = V2{b.pos1[0], b.pos1[1]} - Rotate(b.lc, q);
+    out[0] = p.x;
+    out[1] = p.y;
+    out[2] = q.x;
+    out[3] = q.y;
+    return PRX_OK;
+}
+
+int prx_world_set_velocity(prx_world* w, std::uint32_t body, const float v[3]) {
+    if (int st = CheckWorld(w)) return st;
+    if (body >= w->bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
+    if (int st = Detach(w)) return st;
+    HostBody& b = w->bodies[body];
+    const bool nonzero = v[0] != 0 || v[1] != 0 || v[2] != 0;
+    if (nonzero) {
+        if (!(b.flags & prx::kSpeedable)) return PRX_OK;
+        b.flags |= prx::kAwake;
+        b.under = 0;
+    }
+    b.vel[0] = v[0];
+    b.vel[1] = v[1];
+    b.vel[2] = v[2];
+    return PRX_OK;
+}
+
+std::uint32_t prx_world_contact_count(prx_world* w) {
+    if (!w || Refresh(w) != PRX_OK) return 0;
+    return std::uint32_t(w->contacts.size());
+}
+
+int prx_world_state_record(prx_world* w, std::uint8_t* buf, std::size_t cap, std::size_t* len) {
+    if (int st = CheckWorld(w)) return st;
+    if (int st = Refresh(w)) return st;
+    std::vector<std::uint8_t> o;
+    auto put = [&](const void* p, std::size_t n) {
+        const auto* c = static_cast<const std::uint8_t*>(p);
+        o.insert(o.end(), c, c + n);
+    };
+    auto u32 = [&](std::uint32_t v) { put(&v, 4); };
+    u32(0);
+    const prx_step_stats& s = w->last_stats;
+    const std::uint32_t st8[8] = {s.pre_proxies_moved, s.pre_contacts_destroyed, s.pre_contacts_added,
+                                  s.pre_contacts_updated, s.reg_islands_found, s.reg_contacts_added,
+                                  s.reg_proxies_moved, s.reg_bodies_slept};
+    put(st8, sizeof(st8));
+    u32(std::uint32_t(w->bodies.size()));
+    for (const auto& b : w->bodies) {
+        const float v[10] = {b.pos0[0], b.pos0[1], b.pos0[2], b.pos1[0], b.pos1[1],
+                             b.pos1[2], b.vel[0],  b.vel[1],  b.vel[2],  b.under};
+        put(v, sizeof(v));
+        u32(b.flags & 0x7fu);
+    }
+    const HostContacts& hc = w->contacts;
+    u32(std::uint32_t(hc.size()));
+    for (std::size_t i = 0; i < hc.size(); ++i) {
+        const std::uint32_t u[4] = {w->proxies[std::size_t(hc.prox[i].x)].id, w->proxies[std::size_t(hc.prox[i].y)].id,
+                                    hc.id[i], hc.flags[i]};
+        put(u, sizeof(u));
+        const uint4 h = hc.hdr[i];
+        const std::uint32_t type = h.x & 0xffu, n = (h.x >> 8) & 0xffu;
+        const std::uint32_t t[3] = {type | (n << 8), h.y, h.z};
+        put(t, sizeof(t));
+        float f[14] = {hc.mat[i].x, hc.mat[i].y};
+        if (type == 2 || type == 3) {
+            f[2] = hc.geo[i].x;
+            f[3] = hc.geo[i].y;
+        }
+        if (type != 0) {
+            f[4] = hc.geo[i].z;
+            f[5] = hc.geo[i].w;
+        }
+        if (n > 0) {
+            f[6] = hc.p0[i].x;
+            f[7] = hc.p0[i].y;
+            f[8] = hc.p0[i].z;
+            f[9] = hc.p0[i].w;
+        }
+        if (n > 1) {
+            f[10] = hc.p1[i].x;
+            f[11] = hc.p1[i].y;
+            f[12] = hc.p1[i].z;
+            f[13] = hc.p1[i].w;
+        }
+        put(f, sizeof(f));
+    }
+    std::uint32_t np = 0;
+    for (const auto& p : w->proxies) np += (p.flags & prx::kProxyPending) ? 0u : 1u;
+    u32(np);
+    for (const auto& p : w->proxies) {
+        if (p.flags & prx::kProxyPending) continue;
+        u32(p.id);
+        put(p.aabb, sizeof(p.aabb));
+    }
+    if (len) *len = o.size();
+    if (buf && cap >= o.size()) std::memcpy(buf, o.data(), o.size());
+    else if (buf) return Fail(PRX_E_LENGTH, "buffer too small");
+    return PRX_OK;
+}
+
+int prx_batch_create(prx_world* const* worlds, int nworlds, const prx_batch_conf* conf, prx_batch** out) {
+    return CreateBatch(worlds, nworlds, conf, out);
+}
+
+void prx_batch_destroy(prx_batch* b) {
+    if (!b) return;
+    (void)DownloadBatch(b);
+    for (prx_world* w : b->worlds) {
+        if (w && w->owner == b) {
+            w->owner = nullptr;
+            w->index_in_owner = -1;
+        }
+    }
+    delete b;
+}
+
+int prx_batch_step(prx_batch* b, const prx_step_conf* conf, int nsteps) {
+    if (!b) return Fail(PRX_E_INVALID_ARGUMENT, "null batch");
+    return StepBatch(b, conf, nsteps);
+}
+
+int prx_batch_sync(prx_batch* b) {
+    if (!b) return Fail(PRX_E_INVALID_ARGUMENT, "null batch");
+    PRX_HIP(hipStreamSynchronize(b->stream));
+    return CheckStatus(b);
+}
+
+int prx_batch_download(prx_batch* b) {
+    if (!b) return Fail(PRX_E_INVALID_ARGUMENT, "null batch");
+    return DownloadBatch(b);
+}
+
+int prx_batch_last_timing(prx_batch* b, float* ms, int* launches) {
+    if (!b) return Fail(PRX_E_INVALID_ARGUMENT, "null batch");
+    PRX_HIP(hipEventSynchronize(b->ev1));
+    float t = 0;
+    PRX_HIP(hipEventElapsedTime(&t, b->ev0, b->ev1));
+    if (ms) *ms = t;
+    if (launches) *launches = b->last_launches;
+    return PRX_OK;
+}
+
+std::uint64_t prx_batch_dynamic_bodies(const prx_batch* b) {
+    if (!b) return 0;
+    std::uint64_t n = 0;
+    for (const prx_world* w : b->worlds)
+        if (w)
+            for (const auto& hb : w->bodies) n += (hb.flags & prx::kAccelerable) ? 1u : 0u;
+    return n;
+}
+
+int prx_batch_autotune(prx_batch* b, const prx_step_conf* step, int nsteps, prx_batch_conf* best) {
+    if (!b || !step) return Fail(PRX_E_INVALID_ARGUMENT, "null argument");
+    if (best) *best = b->conf;
+    (void)nsteps;
+    return PRX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,132 @@
+// Device-resident state of a batch of worlds (one world = one batch entry).
+//
+// Layout principles (MI355X): structure-of-arrays per record kind, 16-byte
+// vectors where a record field group is always read together (one dwordx4
+// load), world-contiguous segments so that the workgroup stepping a world
+// touches one contiguous range of every array, and all indices world-local
+// (int32) so a world can be moved between batches without rewriting them.
+//
+// Per world w: bodies [body_off, body_off + nbodies), proxies
+// [proxy_off, proxy_off + nproxies), contacts [contact_off, contact_off + cap)
+// with the live ones first in the world's contact order (creation order, as
+// PlayRho's m_contacts: stable erase, append at back).  Shapes and their
+// vertices are batch-global.
+#pragma once
+
+#include <cstdint>
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+#else
+struct alignas(16) float4 {
+    float x, y, z, w;
+};
+struct alignas(8) float2 {
+    float x, y;
+};
+struct alignas(16) uint4 {
+    unsigned x, y, z, w;
+};
+struct alignas(8) int2 {
+    int x, y;
+};
+struct alignas(16) int4 {
+    int x, y, z, w;
+};
+#endif
+
+namespace prx {
+
+// Body flags (same bits as the state record).
+enum : std::uint32_t {
+    kSpeedable = 1u, kAccelerable = 2u, kImpenetrable = 4u, kAwake = 8u, kAutoSleep = 16u,
+    kEnabled = 32u, kFixedRotation = 64u
+};
+// Contact flags (same bits as the state record).
+enum : std::uint32_t {
+    kTouching = 1u, kCEnabled = 2u, kSensor = 4u, kCImpenetrable = 8u, kNeedsUpdating = 16u
+};
+// Proxy flags.
+enum : std::uint32_t { kProxyPending = 1u, kProxyMoved = 2u };
+// World status bits.
+enum : std::uint32_t { kStatusContactOverflow = 1u, kStatusScratchOverflow = 2u };
+
+// Per-world header (AoS: read once per phase by the world's workgroup).
+struct WorldHeader {
+    std::int32_t body_off, nbodies, proxy_off, nproxies;
+    std::int32_t contact_off, contact_cap, ncontacts, pool_size;
+    std::int32_t nfree, status, nislands, pad0;
+    std::uint32_t stats[8];
+};
+
+struct StepParams {
+    float dt, dtRatio, minStillTimeToSleep, linearSlop, angularSlop, regResolutionRate,
+        regMinSeparation, regMinMomentum, velocityThreshold, maxTranslation, maxRotation,
+        maxLinearCorrection, maxAngularCorrection, linearSleepTolerance, angularSleepTolerance,
+        displaceMultiplier, aabbExtension, maxCirclesRatio;
+    std::uint32_t velIters, posIters, warmStart, blockSolve;
+};
+
+struct DeviceState {
+    int nworlds;
+    WorldHeader* hdr;
+    const int* schedule;  // launch order of worlds (largest first)
+
+    // Bodies.
+    std::uint32_t* bflags;
+    float4* pos0;    // x, y, angle, -
+    float4* pos1;    // x, y, angle, -
+    float4* vel;     // vx, vy, w, underActiveTime
+    float4* xf;      // transform of pos1: px, py, cos, sin
+    float4* lcinv;   // localCenter x, y, invMass, invRotI
+    float4* acc;     // linear accel x, y, angular accel, -
+    float2* damp;    // linear, angular damping
+    int2* bproxy;    // first entry in bproxy_list, count (world-local)
+    int* bproxy_list;
+
+    // Proxies (broadphase leaves).
+    int* pbody;          // world-local body
+    int* pshape;         // batch-global shape
+    std::uint32_t* pid;  // tree leaf id (contact keys use it)
+    float4* paabb;       // fat AABB: lo x, lo y, hi x, hi y
+    std::uint32_t* pflags;
+
+    // Shapes.
+    int4* shape_i;       // nverts, vertex offset, sensor, group
+    float4* shape_f;     // vertex radius, friction, restitution, density
+    std::uint32_t* shape_catmask;
+    float2* verts;
+    float2* norms;
+
+    // Contacts (world order).  Keys are (pid[pa], pid[pb]) with pa < pb by id.
+    int2* c_prox;        // world-local proxy of key min, key max
+    int2* c_body;        // body A, body B (world-local)
+    int2* c_shape;       // shape A, shape B (batch-global)
+    std::uint32_t* c_id; // contact pool id
+    std::uint32_t* c_flags;
+    float4* c_mat;       // friction, restitution, tangent speed, -
+    uint4* m_hdr;        // type | count << 8, cf0, cf1, -
+    float4* m_geo;       // local normal x, y, local point x, y
+    float4* m_p0;        // point 0: local x, y, normal impulse, tangent impulse
+    float4* m_p1;
+    std::uint32_t* c_free;  // pool free list (LIFO), per world segment of cap
+
+    // Scratch (per world, same segments as above).
+    std::uint32_t* s_keys;   // candidate pair keys (cap)
+    std::uint32_t* s_keys2;  // sorted candidate keys (cap)
+    int* s_csr_off;          // per-body offsets (nbodies + 1 per world: body_off + w)
+    int* s_csr;              // per-body touching-contact lists (2 * cap)
+    int* s_stack;            // DFS stack (nbodies)
+    std::uint8_t* s_isl_b;   // islanded bodies (nbodies)
+    std::uint8_t* s_isl_c;   // islanded contacts (cap)
+    int* s_isl_body;         // island body lists (nbodies + cap)
+    int* s_isl_contact;      // island contact lists (cap)
+    int2* s_isl_start;       // island: first body entry, first contact entry (nbodies + 1)
+    float4* s_bc_pos;        // body constraint position x, y, a (nbodies)
+    float4* s_bc_vel;        // body constraint velocity x, y, w (nbodies)
+    float4* s_vc;            // velocity constraints, kVcStride float4 per contact
+};
+
+constexpr int kVcStride = 8;
+
+}  // namespace prx

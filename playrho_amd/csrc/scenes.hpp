@@ -236,12 +236,17 @@ inline SceneSpec Pile(std::uint32_t seed, int w, int nmin = 8, int nmax = 512) {
         const float friction = rng.uniform(0.2, 0.8);
         const int kind = rng.below(3);
         if (kind == 0) {
-            sc.shapes.push_back(Box(rng.uniform(0.3, 0.5), rng.uniform(0.3, 0.5), 1.0f, friction, 0.0f));
+            // Draws are sequenced explicitly: argument evaluation order is
+            // unspecified in C++ and differs between compilers.
+            const float hx = rng.uniform(0.3, 0.5);
+            const float hy = rng.uniform(0.3, 0.5);
+            sc.shapes.push_back(Box(hx, hy, 1.0f, friction, 0.0f));
         } else if (kind == 1) {
             sc.shapes.push_back(Disk(rng.uniform(0.3, 0.5), 1.0f, friction, 0.0f));
         } else {
             const int nv = 3 + rng.below(6);
-            sc.shapes.push_back(Ngon(rng, nv, rng.uniform(0.35, 0.55), 1.0f, friction));
+            const float R = rng.uniform(0.35, 0.55);
+            sc.shapes.push_back(Ngon(rng, nv, R, 1.0f, friction));
         }
         b.shapes = {int(sc.shapes.size()) - 1};
         sc.bodies.push_back(b);

@@ -1,0 +1,1365 @@
+// The fused per-world step: one workgroup (one 64-lane wavefront) advances one
+// world through PlayRho's regular step, Step(AabbTreeWorld&, const StepConf&)
+// (AabbTreeWorld.cpp:2144-2219) with doToi == false:
+//
+//   A  proxy creation for new shapes              CreateProxies        :557-579
+//   B  contact destruction (fat AABBs apart)       DestroyContacts      :2290-2331
+//   C  narrowphase of flagged contacts             UpdateContacts/Update:2333-2421,2587-2721
+//   D  pair finding + contact creation             FindContacts/AddContacts :709-751,2424-2547
+//   E  islands (DFS), per-island Gauss-Seidel solve, writeback, sleep
+//                                                  SolveReg/SolveRegIslandViaGS :1472-1682
+//   F  proxy synchronisation, C and D again        SolveReg             :1532-1557
+//
+// Lane-parallel phases: proxy AABBs, destruction with stable compaction
+// (ballot prefix counts), narrowphase (one lane per contact), candidate pair
+// tests, key sort, contact creation, per-body adjacency, synchronisation.
+// Order-dependent phases replay the reference order exactly: the island DFS
+// (one lane) and each island's Gauss-Seidel solve (one lane per island;
+// islands share no dynamic body, so they are independent).
+//
+// A batch of worlds is one launch: workgroup g steps world schedule[g]
+// (largest worlds first), so uneven world sizes balance across the 256 CUs.
+#include <hip/hip_runtime.h>
+
+#include "layout.hpp"
+#include "narrowphase.hpp"
+
+namespace prxd {
+
+using prx::DeviceState;
+using prx::StepParams;
+using prx::WorldHeader;
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ unsigned long long lanes_below(int lane) {
+    return (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+}
+
+__device__ __forceinline__ Proxy shape_proxy(const DeviceState& d, int s) {
+    const int4 si = d.shape_i[s];
+    const float4 sf = d.shape_f[s];
+    return Proxy{d.verts + si.y, d.norms + si.y, std::uint32_t(si.x), sf.x};
+}
+
+__device__ __forceinline__ Xf body_xf1(const DeviceState& d, int gb) {
+    const float4 t = d.xf[gb];
+    return Xf{V2{t.x, t.y}, V2{t.z, t.w}};
+}
+__device__ __forceinline__ Xf body_xf0(const DeviceState& d, int gb) {
+    const float4 p = d.pos0[gb];
+    const float4 l = d.lcinv[gb];
+    return get_xf(p.x, p.y, p.z, V2{l.x, l.y});
+}
+
+// ComputeAABB(proxy, xf0, xf1) (AABB.cpp:51-62) with optional fattening by
+// aabbExtension and displacement by displaceMultiplier * (p1 - p0)
+// (detail/AABB.hpp:305-319, Interval.hpp:144-220).
+__device__ float4 proxy_aabb(const Proxy& px, const Xf& xf0, const Xf& xf1, const StepParams& c, bool fatten) {
+    float mnx = __builtin_inff(), mny = __builtin_inff();
+    float mxx = -__builtin_inff(), mxy = -__builtin_inff();
+    for (std::uint32_t i = 0; i < px.n; ++i) {
+        const V2 v = px.vert(i);
+        const V2 a = transform(v, xf0);
+        mnx = fmin_std(a.x, mnx);
+        mxx = fmax_std(a.x, mxx);
+        mny = fmin_std(a.y, mny);
+        mxy = fmax_std(a.y, mxy);
+        const V2 b = transform(v, xf1);
+        mnx = fmin_std(b.x, mnx);
+        mxx = fmax_std(b.x, mxx);
+        mny = fmin_std(b.y, mny);
+        mxy = fmax_std(b.y, mxy);
+    }
+    mnx = mnx - px.radius;
+    mxx = mxx + px.radius;
+    mny = mny - px.radius;
+    mxy = mxy + px.radius;
+    if (fatten) {
+        mnx = mnx - c.aabbExtension;
+        mxx = mxx + c.aabbExtension;
+        mny = mny - c.aabbExtension;
+        mxy = mxy + c.aabbExtension;
+        const V2 disp = mul(c.displaceMultiplier, xf1.p - xf0.p);
+        if (disp.x < 0.0f) mnx += disp.x;
+        else mxx += disp.x;
+        if (disp.y < 0.0f) mny += disp.y;
+        else mxy += disp.y;
+    }
+    return float4{mnx, mny, mxx, mxy};
+}
+
+// IsIntersecting per axis (Interval.hpp:292-298).
+__device__ __forceinline__ bool overlap(float4 a, float4 b) {
+    const float mx0 = fmax_std(a.x, b.x);
+    const float mn0 = fmin_std(a.z, b.z);
+    if (!(mn0 >= mx0)) return false;
+    const float mx1 = fmax_std(a.y, b.y);
+    const float mn1 = fmin_std(a.w, b.w);
+    return mn1 >= mx1;
+}
+
+__device__ __forceinline__ void body_set_awake(const DeviceState& d, int gb) {
+    // Body::SetAwake (Body.cpp:163-171)
+    const std::uint32_t f = d.bflags[gb];
+    if (f & prx::kSpeedable) {
+        d.bflags[gb] = f | prx::kAwake;
+        d.vel[gb].w = 0.0f;
+    }
+}
+
+__device__ __forceinline__ Manifold load_manifold(const DeviceState& d, int gi) {
+    Manifold m;
+    const uint4 h = d.m_hdr[gi];
+    const float4 g = d.m_geo[gi];
+    const float4 p0 = d.m_p0[gi];
+    const float4 p1 = d.m_p1[gi];
+    m.type = h.x & 0xffu;
+    m.n = (h.x >> 8) & 0xffu;
+    m.ln = V2{g.x, g.y};
+    m.lpt = V2{g.z, g.w};
+    m.p[0] = MPoint{V2{p0.x, p0.y}, h.y, p0.z, p0.w};
+    m.p[1] = MPoint{V2{p1.x, p1.y}, h.z, p1.z, p1.w};
+    return m;
+}
+
+__device__ __forceinline__ void store_manifold(const DeviceState& d, int gi, const Manifold& m) {
+    d.m_hdr[gi] = uint4{m.type | (m.n << 8), m.n > 0 ? m.p[0].cf : kDefaultCf, m.n > 1 ? m.p[1].cf : kDefaultCf, 0u};
+    d.m_geo[gi] = float4{m.ln.x, m.ln.y, m.lpt.x, m.lpt.y};
+    d.m_p0[gi] = m.n > 0 ? float4{m.p[0].lp.x, m.p[0].lp.y, m.p[0].ni, m.p[0].ti} : float4{0, 0, 0, 0};
+    d.m_p1[gi] = m.n > 1 ? float4{m.p[1].lp.x, m.p[1].lp.y, m.p[1].ni, m.p[1].ti} : float4{0, 0, 0, 0};
+}
+
+// Update (AabbTreeWorld.cpp:2587-2721): new manifold, warm-start impulses
+// carried over by ContactFeature, else from the nearest old point.
+__device__ void update_contact(const DeviceState& d, int bo, int gi, const StepParams& c) {
+    std::uint32_t flags = d.c_flags[gi];
+    const Manifold old = load_manifold(d, gi);
+    const bool oldTouching = (flags & prx::kTouching) != 0;
+    bool newTouching = false;
+    Manifold nm = empty_manifold();
+    if (!(flags & prx::kSensor)) {
+        const int2 bb = d.c_body[gi];
+        const int2 ss = d.c_shape[gi];
+        const Proxy pa = shape_proxy(d, ss.x);
+        const Proxy pb = shape_proxy(d, ss.y);
+        nm = collide_shapes(pa, body_xf1(d, bo + bb.x), pb, body_xf1(d, bo + bb.y), c.linearSlop, c.maxCirclesRatio);
+        const std::uint32_t oldN = old.n, newN = nm.n;
+        newTouching = newN > 0;
+        bool found[2] = {false, newN < 2};
+        for (std::uint32_t i = 0; i < newN; ++i) {
+            for (std::uint32_t j = 0; j < oldN; ++j) {
+                if (nm.p[i].cf == old.p[j].cf) {
+                    found[i] = true;
+                    nm.p[i].ni = old.p[j].ni;
+                    nm.p[i].ti = old.p[j].ti;
+                    break;
+                }
+            }
+        }
+        for (std::uint32_t i = 0; i < newN; ++i) {
+            if (!found[i]) {
+                float least = __builtin_inff();
+                for (std::uint32_t j = 0; j < oldN; ++j) {
+                    const float dd = magsq(old.p[j].lp - nm.p[i].lp);
+                    if (least > dd) {
+                        least = dd;
+                        nm.p[i].ni = old.p[j].ni;
+                        nm.p[i].ti = old.p[j].ti;
+                    }
+                }
+            }
+        }
+    }
+    store_manifold(d, gi, nm);
+    flags &= ~prx::kNeedsUpdating;
+    if (!oldTouching && newTouching) flags |= prx::kTouching;
+    else if (oldTouching && !newTouching) flags &= ~prx::kTouching;
+    d.c_flags[gi] = flags;
+}
+
+struct WorldCtx {
+    int w, bo, nb, po, np, co, cap;
+    int lane;
+};
+
+struct Counters {
+    int nc, nfree, pool;
+    std::uint32_t status;
+};
+
+// UpdateContacts (AabbTreeWorld.cpp:2333-2421).  Returns contacts updated.
+__device__ std::uint32_t phase_update(const DeviceState& d, const WorldCtx& x, int nc, const StepParams& c) {
+    std::uint32_t updated = 0;
+    for (int base = 0; base < nc; base += kWave) {
+        const int i = base + x.lane;
+        bool need = false;
+        if (i < nc) need = (d.c_flags[x.co + i] & prx::kNeedsUpdating) != 0;
+        updated += std::uint32_t(__popcll(__ballot(need)));
+        if (need) update_contact(d, x.bo, x.co + i, c);
+    }
+    return updated;
+}
+
+// DestroyContacts (AabbTreeWorld.cpp:2290-2331) + InternalDestroy (:2263-2288).
+__device__ std::uint32_t phase_destroy(const DeviceState& d, const WorldCtx& x, Counters& k) {
+    int out = 0;
+    std::uint32_t destroyedTotal = 0;
+    const int nc = k.nc;
+    for (int base = 0; base < nc; base += kWave) {
+        const int i = base + x.lane;
+        const bool valid = i < nc;
+        bool keep = false;
+        if (valid) {
+            const int2 pr = d.c_prox[x.co + i];
+            keep = overlap(d.paabb[x.po + pr.x], d.paabb[x.po + pr.y]);
+        }
+        const bool destroyed = valid && !keep;
+        const unsigned long long dm = __ballot(destroyed);
+        const unsigned long long km = __ballot(valid && keep);
+        const unsigned long long below = lanes_below(x.lane);
+        if (destroyed) {
+            const int r = __popcll(dm & below);
+            d.c_free[x.co + k.nfree + r] = d.c_id[x.co + i];
+            const std::uint32_t mh = d.m_hdr[x.co + i].x;
+            const std::uint32_t fl = d.c_flags[x.co + i];
+            if (((mh >> 8) & 0xffu) > 0 && !(fl & prx::kSensor)) {
+                const int2 bd = d.c_body[x.co + i];
+                body_set_awake(d, x.bo + bd.x);
+                body_set_awake(d, x.bo + bd.y);
+            }
+        }
+        const int dst = out + __popcll(km & below);
+        const bool move = valid && keep && (dst != i);
+        if (__ballot(move)) {
+            // Stable compaction: every lane loads its record, then the wave
+            // synchronises, then the moved records are stored (dst <= i).
+            int2 pr{0, 0}, bd{0, 0}, sh{0, 0};
+            std::uint32_t id = 0, fl = 0;
+            float4 mt{0, 0, 0, 0}, mg{0, 0, 0, 0}, p0{0, 0, 0, 0}, p1{0, 0, 0, 0};
+            uint4 mh{0, 0, 0, 0};
+            if (move) {
+                const int src = x.co + i;
+                pr = d.c_prox[src];
+                bd = d.c_body[src];
+                sh = d.c_shape[src];
+                id = d.c_id[src];
+                fl = d.c_flags[src];
+                mt = d.c_mat[src];
+                mh = d.m_hdr[src];
+                mg = d.m_geo[src];
+                p0 = d.m_p0[src];
+                p1 = d.m_p1[src];
+            }
+            __syncthreads();
+            if (move) {
+                const int g = x.co + dst;
+                d.c_prox[g] = pr;
+                d.c_body[g] = bd;
+                d.c_shape[g] = sh;
+                d.c_id[g] = id;
+                d.c_flags[g] = fl;
+                d.c_mat[g] = mt;
+                d.m_hdr[g] = mh;
+                d.m_geo[g] = mg;
+                d.m_p0[g] = p0;
+                d.m_p1[g] = p1;
+            }
+        }
+        k.nfree += __popcll(dm);
+        out += __popcll(km);
+        destroyedTotal += std::uint32_t(__popcll(dm));
+        __syncthreads();
+    }
+    k.nc = out;
+    return destroyedTotal;
+}
+
+// ShouldCollide for two shapes' filters (Filter.hpp:81-88).
+__device__ __forceinline__ bool should_collide(const DeviceState& d, int sa, int sb) {
+    const int ga = d.shape_i[sa].w, gb = d.shape_i[sb].w;
+    if (ga == gb && ga != 0) return ga > 0;
+    const std::uint32_t cma = d.shape_catmask[sa], cmb = d.shape_catmask[sb];
+    return (((cma >> 16) & (cmb & 0xffffu)) != 0) && (((cmb >> 16) & (cma & 0xffffu)) != 0);
+}
+
+// FindContacts (AabbTreeWorld.cpp:709-751) + AddContacts (:2424-2547).
+// Candidate keys pack world-local proxy indices (min << 16 | max); proxy
+// indices increase with tree leaf ids, so key order equals ContactKey order.
+__device__ std::uint32_t phase_find_add(const DeviceState& d, const WorldCtx& x, Counters& k, const StepParams& c) {
+    // Moved proxies, in index order.
+    int nmoved = 0;
+    for (int base = 0; base < x.np; base += kWave) {
+        const int p = base + x.lane;
+        bool mv = false;
+        if (p < x.np) mv = (d.pflags[x.po + p] & prx::kProxyMoved) != 0;
+        const unsigned long long m = __ballot(mv);
+        if (mv) {
+            const int r = nmoved + __popcll(m & lanes_below(x.lane));
+            d.s_isl_body[x.bo + x.co + r] = p;  // scratch list (segment is nbodies + cap >= nproxies)
+        }
+        nmoved += __popcll(m);
+    }
+    __syncthreads();
+    if (nmoved == 0) return 0;
+    int* moved = d.s_isl_body + x.bo + x.co;
+
+    // Candidate pairs: every proxy q against every moved proxy m.
+    int ncand = 0;
+    bool overflow = false;
+    for (int base = 0; base < x.np; base += kWave) {
+        const int q = base + x.lane;
+        const bool qv = q < x.np;
+        float4 qa = float4{0, 0, 0, 0};
+        int qb = -1;
+        bool qmoved = false;
+        if (qv) {
+            qa = d.paabb[x.po + q];
+            qb = d.pbody[x.po + q];
+            qmoved = (d.pflags[x.po + q] & prx::kProxyMoved) != 0;
+        }
+        for (int j = 0; j < nmoved; ++j) {
+            const int m = moved[j];
+            bool hit = false;
+            if (qv && q != m) {
+                // A pair of two moved proxies is produced once, from the smaller.
+                if (!(qmoved && q < m)) {
+                    if (qb != d.pbody[x.po + m] && overlap(d.paabb[x.po + m], qa)) hit = true;
+                }
+            }
+            const unsigned long long hm = __ballot(hit);
+            if (hit) {
+                const int r = ncand + __popcll(hm & lanes_below(x.lane));
+                if (r < x.cap) {
+                    const int lo = q < m ? q : m, hi = q < m ? m : q;
+                    d.s_keys[x.co + r] = (std::uint32_t(lo) << 16) | std::uint32_t(hi);
+                }
+            }
+            ncand += __popcll(hm);
+        }
+    }
+    if (ncand > x.cap) {
+        overflow = true;
+        ncand = x.cap;
+    }
+    __syncthreads();
+    // Clear the moved flags (proxiesForContacts is consumed).
+    for (int p = x.lane; p < x.np; p += kWave) d.pflags[x.po + p] &= ~prx::kProxyMoved;
+    if (ncand == 0) {
+        if (overflow) k.status |= prx::kStatusScratchOverflow;
+        return 0;
+    }
+    // Rank sort of the (unique) candidate keys.
+    for (int i = x.lane; i < ncand; i += kWave) {
+        const std::uint32_t key = d.s_keys[x.co + i];
+        int rank = 0;
+        for (int j = 0; j < ncand; ++j) rank += (d.s_keys[x.co + j] < key) ? 1 : 0;
+        d.s_keys2[x.co + rank] = key;
+    }
+    __syncthreads();
+    // Existing contacts: mark candidates already present (binary search).
+    for (int i = x.lane; i < ncand; i += kWave) d.s_isl_c[x.co + i] = 0;
+    __syncthreads();
+    for (int i = x.lane; i < k.nc; i += kWave) {
+        const int2 pr = d.c_prox[x.co + i];
+        const std::uint32_t key = (std::uint32_t(pr.x) << 16) | std::uint32_t(pr.y);
+        int lo = 0, hi = ncand;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (d.s_keys2[x.co + mid] < key) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo < ncand && d.s_keys2[x.co + lo] == key) d.s_isl_c[x.co + lo] = 1;
+    }
+    __syncthreads();
+    // Filter, then append in key order.
+    std::uint32_t added = 0;
+    const int nc0 = k.nc;
+    for (int base = 0; base < ncand; base += kWave) {
+        const int i = base + x.lane;
+        bool acc = false;
+        int pa = 0, pb = 0, ba = 0, bb = 0, sa = 0, sb = 0;
+        if (i < ncand) {
+            const std::uint32_t key = d.s_keys2[x.co + i];
+            pa = int(key >> 16);
+            pb = int(key & 0xffffu);
+            ba = d.pbody[x.po + pa];
+            bb = d.pbody[x.po + pb];
+            sa = d.pshape[x.po + pa];
+            sb = d.pshape[x.po + pb];
+            const std::uint32_t fa = d.bflags[x.bo + ba], fb = d.bflags[x.bo + bb];
+            acc = ((fa | fb) & prx::kAccelerable) != 0 && should_collide(d, sa, sb) && d.s_isl_c[x.co + i] == 0;
+        }
+        const unsigned long long am = __ballot(acc);
+        if (acc) {
+            const int r = int(added) + __popcll(am & lanes_below(x.lane));
+            const int idx = nc0 + r;
+            if (idx < x.cap) {
+                std::uint32_t id;
+                if (r < k.nfree) id = d.c_free[x.co + k.nfree - 1 - r];
+                else id = std::uint32_t(k.pool + (r - k.nfree));
+                const int gi = x.co + idx;
+                const std::uint32_t fa = d.bflags[x.bo + ba], fb = d.bflags[x.bo + bb];
+                const int4 sia = d.shape_i[sa], sib = d.shape_i[sb];
+                const float4 sfa = d.shape_f[sa], sfb = d.shape_f[sb];
+                std::uint32_t flags = prx::kCEnabled | prx::kNeedsUpdating;
+                if ((fa | fb) & prx::kImpenetrable) flags |= prx::kCImpenetrable;
+                const bool sensor = sia.z != 0 || sib.z != 0;
+                if (sensor) flags |= prx::kSensor;
+                d.c_prox[gi] = int2{pa, pb};
+                d.c_body[gi] = int2{ba, bb};
+                d.c_shape[gi] = int2{sa, sb};
+                d.c_id[gi] = id;
+                d.c_flags[gi] = flags;
+                // MixFriction / MixRestitution (Contact.hpp:57-69)
+                d.c_mat[gi] = float4{__builtin_sqrtf(sfa.y * sfb.y), (sfa.z > sfb.z) ? sfa.z : sfb.z, 0.0f, 0.0f};
+                d.m_hdr[gi] = uint4{0u, kDefaultCf, kDefaultCf, 0u};
+                d.m_geo[gi] = float4{0, 0, 0, 0};
+                d.m_p0[gi] = float4{0, 0, 0, 0};
+                d.m_p1[gi] = float4{0, 0, 0, 0};
+            }
+        }
+        added += std::uint32_t(__popcll(am));
+    }
+    __syncthreads();
+    // Awake flags (SetAwakeFlag for speedable bodies of non-sensor contacts):
+    // idempotent, so done per new contact in a second pass without races on
+    // the flags word of a body touched by two lanes.
+    int nadd = int(added);
+    if (nc0 + nadd > x.cap) {
+        k.status |= prx::kStatusContactOverflow;
+        nadd = x.cap - nc0;
+    }
+    if (overflow) k.status |= prx::kStatusScratchOverflow;
+    if (x.lane == 0) {
+        for (int r = 0; r < nadd; ++r) {
+            const int gi = x.co + nc0 + r;
+            if (d.c_flags[gi] & prx::kSensor) continue;
+            const int2 bd = d.c_body[gi];
+            const std::uint32_t fa = d.bflags[x.bo + bd.x];
+            if (fa & prx::kSpeedable) d.bflags[x.bo + bd.x] = fa | prx::kAwake;
+            const std::uint32_t fb = d.bflags[x.bo + bd.y];
+            if (fb & prx::kSpeedable) d.bflags[x.bo + bd.y] = fb | prx::kAwake;
+        }
+    }
+    const int consumedFree = nadd < k.nfree ? nadd : k.nfree;
+    k.pool += nadd - consumedFree;
+    k.nfree -= consumedFree;
+    k.nc = nc0 + nadd;
+    __syncthreads();
+    // Narrowphase of the new contacts.
+    for (int i = nc0 + x.lane; i < k.nc; i += kWave) update_contact(d, x.bo, x.co + i, c);
+    __syncthreads();
+    return std::uint32_t(nadd);
+}
+
+// ---------------------------------------------------------------------------
+// Island solve.  Velocity-constraint record (kVcStride float4 per contact):
+//   v[0] normal.x, normal.y, friction, restitution
+//   v[1] tangentSpeed, count, K00, K11
+//   v[2] K01, M00, M11, M01              (normal mass = K^-1)
+//   v[3] p0 relA.x, relA.y, relB.x, relB.y
+//   v[4] p0 ni, ti, normalMass, tangentMass
+//   v[5] p0 velocityBias, p1 velocityBias, bodyA, bodyB (bit patterns)
+//   v[6] p1 relA.x, relA.y, relB.x, relB.y
+//   v[7] p1 ni, ti, normalMass, tangentMass
+
+struct BodyC {
+    float invMass, invRotI;
+    V2 lc;
+};
+
+__device__ __forceinline__ V2 rel_vel(V2 vA, float wA, V2 relA, V2 vB, float wB, V2 relB) {
+    // GetContactRelVelocity (d2/Math.cpp:73-103)
+    const V2 velBrot = mul(revperp(relB), wB / 1.0f);
+    const V2 velArot = mul(revperp(relA), wA / 1.0f);
+    return (vB + velBrot) - (vA + velArot);
+}
+
+struct VcPoint {
+    V2 relA, relB;
+    float ni, ti, nm, tm, bias;
+};
+
+struct Vc {
+    V2 normal;
+    float friction, restitution, tangentSpeed;
+    std::uint32_t n;
+    float K00, K11, K01, M00, M11, M01;
+    int bA, bB;
+    VcPoint p[2];
+};
+
+__device__ __forceinline__ void store_vc(float4* v, const Vc& vc) {
+    v[0] = float4{vc.normal.x, vc.normal.y, vc.friction, vc.restitution};
+    v[1] = float4{vc.tangentSpeed, __uint_as_float(vc.n), vc.K00, vc.K11};
+    v[2] = float4{vc.K01, vc.M00, vc.M11, vc.M01};
+    v[3] = float4{vc.p[0].relA.x, vc.p[0].relA.y, vc.p[0].relB.x, vc.p[0].relB.y};
+    v[4] = float4{vc.p[0].ni, vc.p[0].ti, vc.p[0].nm, vc.p[0].tm};
+    v[5] = float4{vc.p[0].bias, vc.p[1].bias, __int_as_float(vc.bA), __int_as_float(vc.bB)};
+    v[6] = float4{vc.p[1].relA.x, vc.p[1].relA.y, vc.p[1].relB.x, vc.p[1].relB.y};
+    v[7] = float4{vc.p[1].ni, vc.p[1].ti, vc.p[1].nm, vc.p[1].tm};
+}
+__device__ __forceinline__ Vc load_vc(const float4* v) {
+    Vc vc;
+    const float4 a = v[0], b = v[1], e = v[2], p0a = v[3], p0b = v[4], f = v[5], p1a = v[6], p1b = v[7];
+    vc.normal = V2{a.x, a.y};
+    vc.friction = a.z;
+    vc.restitution = a.w;
+    vc.tangentSpeed = b.x;
+    vc.n = __float_as_uint(b.y);
+    vc.K00 = b.z;
+    vc.K11 = b.w;
+    vc.K01 = e.x;
+    vc.M00 = e.y;
+    vc.M11 = e.z;
+    vc.M01 = e.w;
+    vc.p[0] = VcPoint{V2{p0a.x, p0a.y}, V2{p0a.z, p0a.w}, p0b.x, p0b.y, p0b.z, p0b.w, f.x};
+    vc.p[1] = VcPoint{V2{p1a.x, p1a.y}, V2{p1a.z, p1a.w}, p1b.x, p1b.y, p1b.z, p1b.w, f.y};
+    vc.bA = __float_as_int(f.z);
+    vc.bB = __float_as_int(f.w);
+    return vc;
+}
+
+// World manifold point data (WorldManifold.cpp:37-130).
+struct WM {
+    V2 normal;
+    std::uint32_t n;
+    V2 p[2];
+    float ni[2], ti[2];
+};
+
+__device__ WM world_manifold(const Manifold& m, const Xf& xfA, float rA, const Xf& xfB, float rB) {
+    WM w;
+    w.n = 0;
+    w.normal = V2{0, 0};
+    if (m.type == kCircles) {
+        const V2 pA = transform(m.lpt, xfA);
+        const V2 pB = transform(m.p[0].lp, xfB);
+        const V2 normal = unit_vector(pB - pA, V2{1.0f, 0.0f});
+        const V2 cA = pA + mul(rA, normal);
+        const V2 cB = pB - mul(rB, normal);
+        w.normal = normal;
+        w.n = 1;
+        w.p[0] = divs(cA + cB, 2.0f);
+        w.ni[0] = m.p[0].ni;
+        w.ti[0] = m.p[0].ti;
+    } else if (m.type == kFaceA) {
+        const V2 normal = rotate(m.ln, xfA.q);
+        const V2 plane = transform(m.lpt, xfA);
+        w.normal = normal;
+        w.n = m.n;
+        for (std::uint32_t i = 0; i < m.n; ++i) {
+            const V2 clip = transform(m.p[i].lp, xfB);
+            const V2 cA = clip + mul(rA - dot(clip - plane, normal), normal);
+            const V2 cB = clip - mul(rB, normal);
+            w.p[i] = divs(cA + cB, 2.0f);
+            w.ni[i] = m.p[i].ni;
+            w.ti[i] = m.p[i].ti;
+        }
+    } else if (m.type == kFaceB) {
+        const V2 normal = rotate(m.ln, xfB.q);
+        const V2 plane = transform(m.lpt, xfB);
+        w.normal = -normal;
+        w.n = m.n;
+        for (std::uint32_t i = 0; i < m.n; ++i) {
+            const V2 clip = transform(m.p[i].lp, xfA);
+            const V2 cB = clip + mul(rB - dot(clip - plane, normal), normal);
+            const V2 cA = clip - mul(rA, normal);
+            w.p[i] = divs(cA + cB, 2.0f);
+            w.ni[i] = m.p[i].ni;
+            w.ti[i] = m.p[i].ti;
+        }
+    }
+    return w;
+}
+
+// VelocityConstraint ctor + GetPoint + ComputeK (VelocityConstraint.cpp:38-185).
+__device__ Vc make_vc(const WM& wm, float friction, float restitution, float tangentSpeed, int bA, int bB,
+                      const BodyC& A, float4 posA, float4 velA, const BodyC& B, float4 posB, float4 velB,
+                      const StepParams& c) {
+    Vc vc;
+    vc.normal = wm.normal;
+    vc.friction = friction;
+    vc.restitution = restitution;
+    vc.tangentSpeed = tangentSpeed;
+    vc.n = 0;
+    vc.K00 = vc.K11 = vc.K01 = 0.0f;
+    vc.M00 = vc.M11 = vc.M01 = 0.0f;
+    vc.bA = bA;
+    vc.bB = bB;
+    vc.p[0] = VcPoint{V2{0, 0}, V2{0, 0}, 0, 0, 0, 0, 0};
+    vc.p[1] = vc.p[0];
+    const float dtRatio = c.warmStart ? c.dtRatio : 0.0f;
+    const float invMass = A.invMass + B.invMass;
+    const V2 tangent = fwdperp(vc.normal);
+    for (std::uint32_t j = 0; j < wm.n; ++j) {
+        VcPoint& p = vc.p[vc.n];
+        const V2 relA = wm.p[j] - V2{posA.x, posA.y};
+        const V2 relB = wm.p[j] - V2{posB.x, posB.y};
+        p.ni = wm.ni[j] * dtRatio;
+        p.ti = wm.ti[j] * dtRatio;
+        {
+            const V2 dv = rel_vel(V2{velA.x, velA.y}, velA.z, relA, V2{velB.x, velB.y}, velB.z, relB);
+            const float vn = dot(dv, vc.normal);
+            p.bias = (vn < -c.velocityThreshold) ? -vc.restitution * vn : 0.0f;
+        }
+        p.relA = relA;
+        p.relB = relB;
+        {
+            const float cA = cross(relA, vc.normal);
+            const float cB = cross(relB, vc.normal);
+            const float rmA = A.invRotI * (cA * cA) / 1.0f;
+            const float rmB = B.invRotI * (cB * cB) / 1.0f;
+            const float value = invMass + rmA + rmB;
+            p.nm = (value != 0.0f) ? 1.0f / value : 0.0f;
+        }
+        {
+            const float cA = cross(relA, tangent);
+            const float cB = cross(relB, tangent);
+            const float rmA = A.invRotI * (cA * cA) / 1.0f;
+            const float rmB = B.invRotI * (cB * cB) / 1.0f;
+            const float value = invMass + rmA + rmB;
+            p.tm = (value != 0.0f) ? 1.0f / value : 0.0f;
+        }
+        ++vc.n;
+    }
+    if (c.blockSolve && vc.n == 2) {
+        const float rnA0 = cross(vc.p[0].relA, vc.normal);
+        const float rnB0 = cross(vc.p[0].relB, vc.normal);
+        const float rnA1 = cross(vc.p[1].relA, vc.normal);
+        const float rnB1 = cross(vc.p[1].relB, vc.normal);
+        const float irmA0 = (A.invRotI * (rnA0 * rnA0)) / 1.0f;
+        const float irmA1 = (A.invRotI * (rnA1 * rnA1)) / 1.0f;
+        const float irmB0 = (B.invRotI * (rnB0 * rnB0)) / 1.0f;
+        const float irmB1 = (B.invRotI * (rnB1 * rnB1)) / 1.0f;
+        const float irmA = (A.invRotI * rnA0 * rnA1) / 1.0f;
+        const float irmB = (B.invRotI * rnB0 * rnB1) / 1.0f;
+        const float k00 = invMass + irmA0 + irmB0;
+        const float k11 = invMass + irmA1 + irmB1;
+        const float k01 = invMass + irmA + irmB;
+        const float k00sq = k00 * k00;
+        const float kdiff = k00 * k11 - k01 * k01;
+        if (k00sq < 1000.0f * kdiff) {
+            vc.K00 = k00;
+            vc.K11 = k11;
+            vc.K01 = k01;
+            // Invert (Math.hpp:428-442): cp = Cross(row0, row1)
+            const float cpm = k00 * k11;
+            const float cps = k01 * k01;
+            const float cp = cpm - cps;
+            if (!almost_zero(cp)) {
+                const float inv = 1.0f / cp;
+                vc.M00 = k11 * inv;
+                vc.M11 = k00 * inv;
+                vc.M01 = -k01 * inv;
+            }
+        } else {
+            vc.n = 1;
+        }
+    }
+    return vc;
+}
+
+// SolveTangentConstraint / SeqSolveNormalConstraint (ContactSolver.cpp:337-454).
+__device__ __forceinline__ float solve_dir(Vc& vc, bool tangent, const BodyC& A, const BodyC& B, float4& velA,
+                                           float4& velB) {
+    float maxInc = 0.0f;
+    const V2 dir = tangent ? fwdperp(vc.normal) : vc.normal;
+    V2 vA{velA.x, velA.y}, vB{velB.x, velB.y};
+    float wA = velA.z, wB = velB.z;
+    for (int k = int(vc.n) - 1; k >= 0; --k) {
+        VcPoint& p = vc.p[k];
+        const V2 closing = rel_vel(vA, wA, p.relA, vB, wB, p.relB);
+        float oldI, newI;
+        if (tangent) {
+            const float dirVel = vc.tangentSpeed - dot(closing, dir);
+            const float lambda = p.tm * dirVel;
+            const float maxI = vc.friction * p.ni;
+            oldI = p.ti;
+            newI = clamp_std(oldI + lambda, -maxI, maxI);
+        } else {
+            const float dirVel = dot(closing, dir);
+            const float lambda = p.nm * (p.bias - dirVel);
+            oldI = p.ni;
+            newI = fmax_std(oldI + lambda, 0.0f);
+        }
+        const float inc = newI - oldI;
+        const V2 P = mul(inc, dir);
+        const float LA = cross(p.relA, P) / 1.0f;
+        const float LB = cross(p.relB, P) / 1.0f;
+        vA = vA - mul(A.invMass, P);
+        wA = wA - A.invRotI * LA;
+        vB = vB + mul(B.invMass, P);
+        wB = wB + B.invRotI * LB;
+        maxInc = fmax_std(maxInc, __builtin_fabsf(inc));
+        if (tangent) p.ti = oldI + inc;
+        else p.ni = oldI + inc;
+    }
+    velA.x = vA.x;
+    velA.y = vA.y;
+    velA.z = wA;
+    velB.x = vB.x;
+    velB.y = vB.y;
+    velB.z = wB;
+    return maxInc;
+}
+
+// BlockSolveNormalConstraint (ContactSolver.cpp:65-335).
+__device__ __forceinline__ float block_solve(Vc& vc, const BodyC& A, const BodyC& B, float4& velA, float4& velB) {
+    const V2 normal = vc.normal;
+    V2 bp;
+    {
+        const V2 vA{velA.x, velA.y}, vB{velB.x, velB.y};
+        const V2 dv0 = rel_vel(vA, velA.z, vc.p[0].relA, vB, velB.z, vc.p[0].relB);
+        const V2 dv1 = rel_vel(vA, velA.z, vc.p[1].relA, vB, velB.z, vc.p[1].relB);
+        const float vn0 = dot(dv0, normal);
+        const float vn1 = dot(dv1, normal);
+        const V2 b{vn0 - vc.p[0].bias, vn1 - vc.p[1].bias};
+        float t0 = 0.0f;
+        t0 += vc.K00 * vc.p[0].ni;
+        t0 += vc.K01 * vc.p[1].ni;
+        float t1 = 0.0f;
+        t1 += vc.K01 * vc.p[0].ni;
+        t1 += vc.K11 * vc.p[1].ni;
+        bp = V2{b.x - t0, b.y - t1};
+    }
+    float n0, n1;
+    bool ok = false;
+    {
+        float x0 = 0.0f;
+        x0 += vc.M00 * bp.x;
+        x0 += vc.M01 * bp.y;
+        float x1 = 0.0f;
+        x1 += vc.M01 * bp.x;
+        x1 += vc.M11 * bp.y;
+        n0 = -x0;
+        n1 = -x1;
+        ok = (n0 >= 0.0f && n1 >= 0.0f);
+    }
+    if (!ok) {
+        n0 = -vc.p[0].nm * bp.x;
+        n1 = 0.0f;
+        const float vn2 = vc.K01 * n0 + bp.y;
+        ok = (n0 >= 0.0f && vn2 >= 0.0f);
+    }
+    if (!ok) {
+        n0 = 0.0f;
+        n1 = -vc.p[1].nm * bp.y;
+        const float vn1 = vc.K01 * n1 + bp.x;
+        ok = (n1 >= 0.0f && vn1 >= 0.0f);
+    }
+    if (!ok) {
+        n0 = 0.0f;
+        n1 = 0.0f;
+        ok = (bp.x >= 0.0f && bp.y >= 0.0f);
+    }
+    if (!ok) return 0.0f;
+    const float d0 = n0 - vc.p[0].ni;
+    const float d1 = n1 - vc.p[1].ni;
+    const V2 P0 = mul(d0, normal);
+    const V2 P1 = mul(d1, normal);
+    const V2 P = P0 + P1;
+    const float LA = (cross(vc.p[0].relA, P0) + cross(vc.p[1].relA, P1)) / 1.0f;
+    const float LB = (cross(vc.p[0].relB, P0) + cross(vc.p[1].relB, P1)) / 1.0f;
+    const V2 dAl = -mul(A.invMass, P);
+    const float dAa = -(A.invRotI * LA);
+    const V2 dBl = mul(B.invMass, P);
+    const float dBa = B.invRotI * LB;
+    velA.x = velA.x + dAl.x;
+    velA.y = velA.y + dAl.y;
+    velA.z = velA.z + dAa;
+    velB.x = velB.x + dBl.x;
+    velB.y = velB.y + dBl.y;
+    velB.z = velB.z + dBa;
+    vc.p[0].ni = n0;
+    vc.p[1].ni = n1;
+    return fmax_std(__builtin_fabsf(n0), __builtin_fabsf(n1));
+}
+
+// Position solver manifold (PositionSolverManifold.cpp:42-121).
+struct PSM {
+    V2 normal, point;
+    float sep;
+};
+__device__ PSM get_psm(const Manifold& m, std::uint32_t idx, const Xf& xfA, const Xf& xfB) {
+    if (m.type == kCircles) {
+        const V2 pA = transform(m.lpt, xfA);
+        const V2 pB = transform(m.p[idx].lp, xfB);
+        const V2 delta = pB - pA;
+        const V2 normal = unit_vector(delta, V2{0, 0});
+        return PSM{normal, divs(pA + pB, 2.0f), dot(delta, normal)};
+    }
+    if (m.type == kFaceA) {
+        const V2 plane = transform(m.lpt, xfA);
+        const V2 normal = rotate(m.ln, xfA.q);
+        const V2 clip = transform(m.p[idx].lp, xfB);
+        return PSM{normal, clip, dot(clip - plane, normal)};
+    }
+    const V2 plane = transform(m.lpt, xfB);
+    const V2 normal = rotate(m.ln, xfB.q);
+    const V2 clip = transform(m.p[idx].lp, xfA);
+    return PSM{-normal, clip, dot(clip - plane, normal)};
+}
+
+// GaussSeidel::SolvePositionConstraint (ContactSolver.cpp:494-643).
+__device__ float solve_pc(const Manifold& m, float totalRadius, const BodyC& A, const BodyC& B, float4& posA,
+                          float4& posB, const StepParams& c) {
+    const float invMassTotal = A.invMass + B.invMass;
+    struct Sol {
+        V2 la;
+        float aa;
+        V2 lb;
+        float ab;
+        float sep;
+    };
+    auto solve = [&](const PSM& psm, V2 pA, V2 pB) {
+        const float separation = psm.sep - totalRadius;
+        const V2 rA = psm.point - pA;
+        const V2 rB = psm.point - pB;
+        const float crA = cross(rA, psm.normal) / 1.0f;
+        const float crB = cross(rB, psm.normal) / 1.0f;
+        const float K = invMassTotal + A.invRotI * (crA * crA) + B.invRotI * (crB * crB);
+        const float C = -clamp_std(c.regResolutionRate * (separation + c.linearSlop), -c.maxLinearCorrection, 0.0f);
+        const V2 P = divs(mul(psm.normal, C), K);
+        const float LA = cross(rA, P) / 1.0f;
+        const float LB = cross(rB, P) / 1.0f;
+        return Sol{-mul(A.invMass, P), -(A.invRotI * LA), mul(B.invMass, P), B.invRotI * LB, separation};
+    };
+    V2 pAl{posA.x, posA.y}, pBl{posB.x, posB.y};
+    float pAa = posA.z, pBa = posB.z;
+    float minSep = __builtin_inff();
+    if (m.n == 1) {
+        const PSM psm0 = get_psm(m, 0, get_xf(pAl.x, pAl.y, pAa, A.lc), get_xf(pBl.x, pBl.y, pBa, B.lc));
+        const Sol s = solve(psm0, pAl, pBl);
+        pAl = pAl + s.la;
+        pAa = pAa + s.aa;
+        pBl = pBl + s.lb;
+        pBa = pBa + s.ab;
+        minSep = 0.0f + s.sep;
+    } else if (m.n == 2) {
+        const Xf xfA = get_xf(pAl.x, pAl.y, pAa, A.lc);
+        const Xf xfB = get_xf(pBl.x, pBl.y, pBa, B.lc);
+        const PSM psm0 = get_psm(m, 0, xfA, xfB);
+        const PSM psm1 = get_psm(m, 1, xfA, xfB);
+        if (almost_equal(psm0.sep, psm1.sep)) {
+            const Sol s0 = solve(psm0, pAl, pBl);
+            const Sol s1 = solve(psm1, pAl, pBl);
+            pAl = pAl + s0.la + s1.la;
+            pAa = pAa + s0.aa + s1.aa;
+            pBl = pBl + s0.lb + s1.lb;
+            pBa = pBa + s0.ab + s1.ab;
+            minSep = s0.sep;
+        } else if (psm0.sep < psm1.sep) {
+            const Sol s0 = solve(psm0, pAl, pBl);
+            pAl = pAl + s0.la;
+            pAa = pAa + s0.aa;
+            pBl = pBl + s0.lb;
+            pBa = pBa + s0.ab;
+            const PSM p1 = get_psm(m, 1, get_xf(pAl.x, pAl.y, pAa, A.lc), get_xf(pBl.x, pBl.y, pBa, B.lc));
+            const Sol s1 = solve(p1, pAl, pBl);
+            pAl = pAl + s1.la;
+            pAa = pAa + s1.aa;
+            pBl = pBl + s1.lb;
+            pBa = pBa + s1.ab;
+            minSep = s0.sep;
+        } else if (psm1.sep < psm0.sep) {
+            const Sol s1 = solve(psm1, pAl, pBl);
+            pAl = pAl + s1.la;
+            pAa = pAa + s1.aa;
+            pBl = pBl + s1.lb;
+            pBa = pBa + s1.ab;
+            const PSM p0 = get_psm(m, 0, get_xf(pAl.x, pAl.y, pAa, A.lc), get_xf(pBl.x, pBl.y, pBa, B.lc));
+            const Sol s0 = solve(p0, pAl, pBl);
+            pAl = pAl + s0.la;
+            pAa = pAa + s0.aa;
+            pBl = pBl + s0.lb;
+            pBa = pBa + s0.ab;
+            minSep = s1.sep;
+        }
+    }
+    posA.x = pAl.x;
+    posA.y = pAl.y;
+    posA.z = pAa;
+    posB.x = pBl.x;
+    posB.y = pBl.y;
+    posB.z = pBa;
+    return minSep;
+}
+
+__device__ __forceinline__ BodyC load_bodyc(const DeviceState& d, int gb) {
+    const float4 l = d.lcinv[gb];
+    return BodyC{l.z, l.w, V2{l.x, l.y}};
+}
+
+// GetVelocity(body, h) + Cap (Body.cpp:290-309, Velocity.cpp:27-45).
+__device__ float4 integrated_velocity(const DeviceState& d, int gb, const StepParams& c) {
+    const float4 v = d.vel[gb];
+    V2 lin{v.x, v.y};
+    float ang = v.z;
+    const float h = c.dt;
+    if (d.bflags[gb] & prx::kAccelerable) {
+        const float4 a = d.acc[gb];
+        const float2 dm = d.damp[gb];
+        lin = lin + mul(h, V2{a.x, a.y});
+        ang = ang + h * a.z;
+        const float linv = 1.0f / (1.0f + h * dm.x);
+        lin = V2{lin.x * linv, lin.y * linv};
+        ang = ang / (1.0f + h * dm.y);
+    }
+    const V2 tr = mul(h, lin);
+    const float lsq = magsq(tr);
+    if (lsq > c.maxTranslation * c.maxTranslation) {
+        const float ratio = c.maxTranslation / __builtin_sqrtf(lsq);
+        lin = V2{lin.x * ratio, lin.y * ratio};
+    }
+    const float absRot = __builtin_fabsf(h * ang);
+    if (absRot > c.maxRotation) {
+        const float ratio = c.maxRotation / absRot;
+        ang = ang * ratio;
+    }
+    return float4{lin.x, lin.y, ang, 0.0f};
+}
+
+// SolveRegIslandViaGS (AabbTreeWorld.cpp:1563-1682) for one island, run by one
+// lane.  Returns bodies put to sleep.
+__device__ std::uint32_t solve_island(const DeviceState& d, const WorldCtx& x, int b0, int b1, int c0, int c1,
+                                      const StepParams& c) {
+    const int bo = x.bo;
+    for (int e = b0; e < b1; ++e) {
+        const int b = d.s_isl_body[bo + x.co + e];
+        d.pos0[bo + b] = d.pos1[bo + b];
+    }
+    for (int e = b0; e < b1; ++e) {
+        const int b = d.s_isl_body[bo + x.co + e];
+        d.s_bc_pos[bo + b] = d.pos1[bo + b];
+        d.s_bc_vel[bo + b] = integrated_velocity(d, bo + b, c);
+    }
+    // Velocity constraints in island contact order.
+    for (int e = c0; e < c1; ++e) {
+        const int ci = d.s_isl_contact[x.co + e];
+        const int gi = x.co + ci;
+        const int2 bd = d.c_body[gi];
+        const int2 ss = d.c_shape[gi];
+        const float rA = d.shape_f[ss.x].x, rB = d.shape_f[ss.y].x;
+        const BodyC A = load_bodyc(d, bo + bd.x), B = load_bodyc(d, bo + bd.y);
+        const float4 pA = d.s_bc_pos[bo + bd.x], pB = d.s_bc_pos[bo + bd.y];
+        const Xf xfA = get_xf(pA.x, pA.y, pA.z, A.lc);
+        const Xf xfB = get_xf(pB.x, pB.y, pB.z, B.lc);
+        const WM wm = world_manifold(load_manifold(d, gi), xfA, rA, xfB, rB);
+        const float4 mt = d.c_mat[gi];
+        const Vc vc = make_vc(wm, mt.x, mt.y, mt.z, bd.x, bd.y, A, pA, d.s_bc_vel[bo + bd.x], B, pB,
+                              d.s_bc_vel[bo + bd.y], c);
+        store_vc(d.s_vc + std::size_t(gi) * prx::kVcStride, vc);
+    }
+    // Warm start (AabbTreeWorld.cpp:230-274).
+    if (c.warmStart) {
+        for (int e = c0; e < c1; ++e) {
+            const int gi = x.co + d.s_isl_contact[x.co + e];
+            const Vc vc = load_vc(d.s_vc + std::size_t(gi) * prx::kVcStride);
+            const BodyC A = load_bodyc(d, bo + vc.bA), B = load_bodyc(d, bo + vc.bB);
+            const V2 tangent = fwdperp(vc.normal);
+            V2 dAl{0, 0}, dBl{0, 0};
+            float dAa = 0.0f, dBa = 0.0f;
+            for (std::uint32_t j = 0; j < vc.n; ++j) {
+                const VcPoint& p = vc.p[j];
+                const V2 P = mul(p.ni, vc.normal) + mul(p.ti, tangent);
+                const float LA = cross(p.relA, P) / 1.0f;
+                const float LB = cross(p.relB, P) / 1.0f;
+                dAl = dAl - mul(A.invMass, P);
+                dAa = dAa - A.invRotI * LA;
+                dBl = dBl + mul(B.invMass, P);
+                dBa = dBa + B.invRotI * LB;
+            }
+            float4 va = d.s_bc_vel[bo + vc.bA];
+            va.x = va.x + dAl.x;
+            va.y = va.y + dAl.y;
+            va.z = va.z + dAa;
+            d.s_bc_vel[bo + vc.bA] = va;
+            float4 vb = d.s_bc_vel[bo + vc.bB];
+            vb.x = vb.x + dBl.x;
+            vb.y = vb.y + dBl.y;
+            vb.z = vb.z + dBa;
+            d.s_bc_vel[bo + vc.bB] = vb;
+        }
+    }
+    // Velocity iterations.
+    for (std::uint32_t it = 0; it < c.velIters; ++it) {
+        float newInc = 0.0f;
+        for (int e = c0; e < c1; ++e) {
+            const int gi = x.co + d.s_isl_contact[x.co + e];
+            float4* rec = d.s_vc + std::size_t(gi) * prx::kVcStride;
+            Vc vc = load_vc(rec);
+            const BodyC A = load_bodyc(d, bo + vc.bA), B = load_bodyc(d, bo + vc.bB);
+            float4 va = d.s_bc_vel[bo + vc.bA];
+            float4 vb = d.s_bc_vel[bo + vc.bB];
+            float mi = 0.0f;
+            mi = fmax_std(mi, solve_dir(vc, true, A, B, va, vb));
+            float nrm;
+            if (vc.n == 1 || (vc.K00 == 0.0f && vc.K11 == 0.0f && vc.K01 == 0.0f)) {
+                nrm = solve_dir(vc, false, A, B, va, vb);
+            } else {
+                nrm = block_solve(vc, A, B, va, vb);
+            }
+            mi = fmax_std(mi, nrm);
+            d.s_bc_vel[bo + vc.bA] = va;
+            d.s_bc_vel[bo + vc.bB] = vb;
+            rec[4] = float4{vc.p[0].ni, vc.p[0].ti, vc.p[0].nm, vc.p[0].tm};
+            rec[7] = float4{vc.p[1].ni, vc.p[1].ti, vc.p[1].nm, vc.p[1].tm};
+            newInc = fmax_std(newInc, mi);
+        }
+        if (newInc <= c.regMinMomentum) break;
+    }
+    // IntegratePositions (AabbTreeWorld.cpp:176-188).
+    for (int e = b0; e < b1; ++e) {
+        const int b = d.s_isl_body[bo + x.co + e];
+        float4 p = d.s_bc_pos[bo + b];
+        const float4 v = d.s_bc_vel[bo + b];
+        const float tx = c.dt * v.x, ty = c.dt * v.y, r = c.dt * v.z;
+        p.x = p.x + tx;
+        p.y = p.y + ty;
+        p.z = p.z + r;
+        d.s_bc_pos[bo + b] = p;
+    }
+    // Position iterations.
+    bool solved = false;
+    for (std::uint32_t it = 0; it < c.posIters; ++it) {
+        float minSep = __builtin_inff();
+        for (int e = c0; e < c1; ++e) {
+            const int gi = x.co + d.s_isl_contact[x.co + e];
+            const int2 bd = d.c_body[gi];
+            const int2 ss = d.c_shape[gi];
+            const float tr = d.shape_f[ss.x].x + d.shape_f[ss.y].x;
+            const BodyC A = load_bodyc(d, bo + bd.x), B = load_bodyc(d, bo + bd.y);
+            float4 pa = d.s_bc_pos[bo + bd.x];
+            float4 pb = d.s_bc_pos[bo + bd.y];
+            const float s = solve_pc(load_manifold(d, gi), tr, A, B, pa, pb, c);
+            d.s_bc_pos[bo + bd.x] = pa;
+            d.s_bc_pos[bo + bd.y] = pb;
+            minSep = fmin_std(minSep, s);
+        }
+        if (minSep >= c.regMinSeparation) {
+            solved = true;
+            break;
+        }
+    }
+    // AssignImpulses (AabbTreeWorld.cpp:208-227).
+    for (int e = c0; e < c1; ++e) {
+        const int gi = x.co + d.s_isl_contact[x.co + e];
+        const float4* rec = d.s_vc + std::size_t(gi) * prx::kVcStride;
+        const std::uint32_t n = __float_as_uint(rec[1].y);
+        const float4 q0 = rec[4];
+        float4 m0 = d.m_p0[gi];
+        m0.z = q0.x;
+        m0.w = q0.y;
+        d.m_p0[gi] = m0;
+        if (n > 1) {
+            const float4 q1 = rec[7];
+            float4 m1 = d.m_p1[gi];
+            m1.z = q1.x;
+            m1.w = q1.y;
+            d.m_p1[gi] = m1;
+        }
+    }
+    // Body writeback; bodies whose position changed flag their contacts.
+    for (int e = b0; e < b1; ++e) {
+        const int b = d.s_isl_body[bo + x.co + e];
+        const int gb = bo + b;
+        const float4 bv = d.s_bc_vel[gb];
+        const float4 bp = d.s_bc_pos[gb];
+        float4 v = d.vel[gb];
+        v.x = bv.x;
+        v.y = bv.y;
+        v.z = bv.z;
+        d.vel[gb] = v;
+        const float4 p1 = d.pos1[gb];
+        if (!(p1.x == bp.x && p1.y == bp.y) || p1.z != bp.z) {
+            d.pos1[gb] = float4{bp.x, bp.y, bp.z, 0.0f};
+            const float4 l = d.lcinv[gb];
+            const Xf xf = get_xf(bp.x, bp.y, bp.z, V2{l.x, l.y});
+            d.xf[gb] = float4{xf.p.x, xf.p.y, xf.q.x, xf.q.y};
+            d.bflags[gb] |= 0x100u;  // position changed this step
+        }
+    }
+    // UpdateUnderActiveTimes + Sleepem (AabbTreeWorld.cpp:403-429).
+    float minUnder = __builtin_inff();
+    for (int e = b0; e < b1; ++e) {
+        const int gb = bo + d.s_isl_body[bo + x.co + e];
+        const std::uint32_t f = d.bflags[gb];
+        if (!(f & prx::kSpeedable)) continue;
+        float4 v = d.vel[gb];
+        const float linSq = magsq(V2{v.x, v.y});
+        const float angSq = v.z * v.z;
+        const bool under = (angSq <= c.angularSleepTolerance * c.angularSleepTolerance) &&
+                           (linSq <= c.linearSleepTolerance * c.linearSleepTolerance);
+        const float t = ((f & prx::kAutoSleep) && under) ? v.w + c.dt : 0.0f;
+        if (t == 0.0f || (f & prx::kAccelerable)) {
+            v.w = t;
+            d.vel[gb] = v;
+        }
+        minUnder = fmin_std(minUnder, t);
+    }
+    std::uint32_t slept = 0;
+    if (minUnder >= c.minStillTimeToSleep && solved) {
+        for (int e = b0; e < b1; ++e) {
+            const int gb = bo + d.s_isl_body[bo + x.co + e];
+            const std::uint32_t f = d.bflags[gb];
+            if ((f & prx::kAwake) && (f & prx::kAutoSleep)) {
+                d.bflags[gb] = f & ~prx::kAwake;
+                d.vel[gb] = float4{0.0f, 0.0f, 0.0f, 0.0f};
+                ++slept;
+            }
+        }
+    }
+    return slept;
+}
+
+// Synchronize (AabbTreeWorld.cpp:2564-2585) for one body.
+__device__ std::uint32_t synchronize_body(const DeviceState& d, const WorldCtx& x, int b, const StepParams& c) {
+    const int gb = x.bo + b;
+    const Xf xf0 = body_xf0(d, gb);
+    const Xf xf1 = body_xf1(d, gb);
+    const int2 bp = d.bproxy[gb];
+    std::uint32_t moved = 0;
+    for (int k = 0; k < bp.y; ++k) {
+        const int p = d.bproxy_list[x.po + bp.x + k];
+        const int gp = x.po + p;
+        const Proxy px = shape_proxy(d, d.pshape[gp]);
+        const float4 tight = proxy_aabb(px, xf0, xf1, c, false);
+        const float4 fat = d.paabb[gp];
+        const bool contains = fat.x <= tight.x && fat.z >= tight.z && fat.y <= tight.y && fat.w >= tight.w;
+        if (!contains) {
+            d.paabb[gp] = proxy_aabb(px, xf0, xf1, c, true);
+            d.pflags[gp] |= prx::kProxyMoved;
+            ++moved;
+        }
+    }
+    return moved;
+}
+
+__device__ void step_world(const DeviceState& d, int w, const StepParams& c) {
+    WorldHeader& H = d.hdr[w];
+    WorldCtx x;
+    x.w = w;
+    x.bo = H.body_off;
+    x.nb = H.nbodies;
+    x.po = H.proxy_off;
+    x.np = H.nproxies;
+    x.co = H.contact_off;
+    x.cap = H.contact_cap;
+    x.lane = int(threadIdx.x);
+    Counters k;
+    k.nc = H.ncontacts;
+    k.nfree = H.nfree;
+    k.pool = H.pool_size;
+    k.status = std::uint32_t(H.status);
+    std::uint32_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+    // A: proxies of newly attached shapes (CreateProxies, AabbTreeWorld.cpp:557-579).
+    for (int p = x.lane; p < x.np; p += kWave) {
+        const int gp = x.po + p;
+        if (d.pflags[gp] & prx::kProxyPending) {
+            const int gb = x.bo + d.pbody[gp];
+            d.paabb[gp] = proxy_aabb(shape_proxy(d, d.pshape[gp]), body_xf0(d, gb), body_xf1(d, gb), c, true);
+            d.pflags[gp] = prx::kProxyMoved;
+        }
+    }
+    __syncthreads();
+    // B, C, D.
+    stats[1] = phase_destroy(d, x, k);
+    __syncthreads();
+    stats[3] = phase_update(d, x, k.nc, c);
+    __syncthreads();
+    stats[2] = phase_find_add(d, x, k, c);
+    __syncthreads();
+
+    if (c.dt != 0.0f) {
+        // E1: per-body lists of touching, enabled, non-sensor contacts in
+        // contact order (the order of each body's contact list).
+        int* off = d.s_csr_off + x.bo + w;  // nbodies + 1 entries
+        for (int b = x.lane; b <= x.nb; b += kWave) off[b] = 0;
+        for (int b = x.lane; b < x.nb; b += kWave) d.s_isl_b[x.bo + b] = 0;
+        __syncthreads();
+        for (int i = x.lane; i < k.nc; i += kWave) {
+            const std::uint32_t f = d.c_flags[x.co + i];
+            d.s_isl_c[x.co + i] = 0;
+            if ((f & prx::kCEnabled) && (f & prx::kTouching) && !(f & prx::kSensor)) {
+                const int2 bd = d.c_body[x.co + i];
+                atomicAdd(&off[bd.x + 1], 1);
+                atomicAdd(&off[bd.y + 1], 1);
+            }
+        }
+        __syncthreads();
+        if (x.lane == 0) {
+            int acc = 0;
+            for (int b = 0; b <= x.nb; ++b) {
+                acc += off[b];
+                off[b] = acc;
+            }
+        }
+        __syncthreads();
+        // Fill with per-body cursors kept in s_stack, then sort each list.
+        int* cur = d.s_stack + x.bo;
+        for (int b = x.lane; b < x.nb; b += kWave) cur[b] = off[b];
+        __syncthreads();
+        int* csr = d.s_csr + 2 * x.co;
+        for (int i = x.lane; i < k.nc; i += kWave) {
+            const std::uint32_t f = d.c_flags[x.co + i];
+            if ((f & prx::kCEnabled) && (f & prx::kTouching) && !(f & prx::kSensor)) {
+                const int2 bd = d.c_body[x.co + i];
+                csr[atomicAdd(&cur[bd.x], 1)] = i;
+                csr[atomicAdd(&cur[bd.y], 1)] = i;
+            }
+        }
+        __syncthreads();
+        for (int b = x.lane; b < x.nb; b += kWave) {
+            const int s = off[b], e = off[b + 1];
+            for (int i = s + 1; i < e; ++i) {
+                const int v = csr[i];
+                int j = i - 1;
+                while (j >= s && csr[j] > v) {
+                    csr[j + 1] = csr[j];
+                    --j;
+                }
+                csr[j + 1] = v;
+            }
+        }
+        __syncthreads();
+        // E2: islands by depth-first search in body order (AabbTreeWorld.cpp:1358-1523).
+        __shared__ int s_nisl;
+        if (x.lane == 0) {
+            int nisl = 0, nbe = 0, nce = 0;
+            int* stack = d.s_stack + x.bo;
+            std::uint8_t* islb = d.s_isl_b + x.bo;
+            std::uint8_t* islc = d.s_isl_c + x.co;
+            int* ibody = d.s_isl_body + x.bo + x.co;
+            int* icont = d.s_isl_contact + x.co;
+            int2* istart = d.s_isl_start + x.bo + w;
+            for (int seed = 0; seed < x.nb; ++seed) {
+                if (islb[seed]) continue;
+                const std::uint32_t sf = d.bflags[x.bo + seed];
+                if (!((sf & prx::kAwake) && (sf & prx::kEnabled))) continue;
+                istart[nisl] = int2{nbe, nce};
+                ++nisl;
+                const int be0 = nbe;
+                int sp = 0;
+                stack[sp++] = seed;
+                islb[seed] = 1;
+                while (sp > 0) {
+                    const int b = stack[--sp];
+                    ibody[nbe++] = b;
+                    const std::uint32_t f = d.bflags[x.bo + b];
+                    if (!(f & prx::kSpeedable)) continue;
+                    d.bflags[x.bo + b] = f | prx::kAwake;
+                    for (int e = off[b]; e < off[b + 1]; ++e) {
+                        const int ci = csr[e];
+                        if (islc[ci]) continue;
+                        const int2 bd = d.c_body[x.co + ci];
+                        const int other = (bd.x == b) ? bd.y : bd.x;
+                        icont[nce++] = ci;
+                        islc[ci] = 1;
+                        if (!islb[other]) {
+                            islb[other] = 1;
+                            stack[sp++] = other;
+                        }
+                    }
+                }
+                for (int e = be0; e < nbe; ++e) {
+                    const int b = ibody[e];
+                    if (!(d.bflags[x.bo + b] & prx::kSpeedable)) islb[b] = 0;
+                }
+            }
+            istart[nisl] = int2{nbe, nce};
+            s_nisl = nisl;
+        }
+        __syncthreads();
+        const int nisl = s_nisl;
+        stats[4] = std::uint32_t(nisl);
+        // E3: islands solved in parallel, one lane per island.
+        std::uint32_t slept = 0;
+        const int2* istart = d.s_isl_start + x.bo + w;
+        for (int is = x.lane; is < nisl; is += kWave) {
+            const int2 s0 = istart[is], s1 = istart[is + 1];
+            slept += solve_island(d, x, s0.x, s1.x, s0.y, s1.y, c);
+        }
+        for (int o = 32; o > 0; o >>= 1) slept += __shfl_xor(slept, o);
+        stats[7] = slept;
+        __syncthreads();
+        // E4: synchronise proxies of islanded speedable bodies.
+        std::uint32_t moved = 0;
+        for (int b = x.lane; b < x.nb; b += kWave) {
+            const int gb = x.bo + b;
+            if (d.s_isl_b[gb] && (d.bflags[gb] & prx::kSpeedable)) moved += synchronize_body(d, x, b, c);
+        }
+        for (int o = 32; o > 0; o >>= 1) moved += __shfl_xor(moved, o);
+        stats[6] = moved;
+        __syncthreads();
+        // Contacts of bodies whose position changed need updating.
+        for (int i = x.lane; i < k.nc; i += kWave) {
+            const int2 bd = d.c_body[x.co + i];
+            if ((d.bflags[x.bo + bd.x] | d.bflags[x.bo + bd.y]) & 0x100u) d.c_flags[x.co + i] |= prx::kNeedsUpdating;
+        }
+        __syncthreads();
+        for (int b = x.lane; b < x.nb; b += kWave) d.bflags[x.bo + b] &= ~0x100u;
+        __syncthreads();
+        phase_update(d, x, k.nc, c);
+        __syncthreads();
+        stats[5] = phase_find_add(d, x, k, c);
+        __syncthreads();
+    }
+    if (x.lane == 0) {
+        H.ncontacts = k.nc;
+        H.nfree = k.nfree;
+        H.pool_size = k.pool;
+        H.status = int(k.status);
+        for (int i = 0; i < 8; ++i) H.stats[i] = stats[i];
+    }
+}
+
+__global__ __launch_bounds__(64) void k_step_worlds(DeviceState d, StepParams c) {
+    const int w = d.schedule[blockIdx.x];
+    step_world(d, w, c);
+}
+
+}  // namespace prxd
+
+// Host-side launcher (compiled in this translation unit).
+extern "C" hipError_t prx_launch_step(const prx::DeviceState* d, const prx::StepParams* c, int first, int count,
+                                      hipStream_t stream) {
+    prx::DeviceState dd = *d;
+    dd.schedule = d->schedule + first;
+    hipLaunchKernelGGL(prxd::k_step_worlds, dim3(unsigned(count)), dim3(64), 0, stream, dd, *c);
+    return hipGetLastError();
+}
+
+// Diagnostic: evaluates the device sine/cosine (UnitVec::Get) on n angles.
+namespace prxd {
+__global__ void k_sincos(const float* in, float* out, int n) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) {
+        const V2 q = unit_from_angle(in[i]);
+        out[2 * i] = q.x;
+        out[2 * i + 1] = q.y;
+    }
+}
+}  // namespace prxd
+
+extern "C" int prx_selftest_sincos(const float* angles, float* out, int n) {
+    if (n <= 0 || !angles || !out) return 1;
+    float* d_in = nullptr;
+    float* d_out = nullptr;
+    if (hipMalloc(&d_in, sizeof(float) * std::size_t(n)) != hipSuccess) return 5;
+    if (hipMalloc(&d_out, 2 * sizeof(float) * std::size_t(n)) != hipSuccess) {
+        (void)hipFree(d_in);
+        return 5;
+    }
+    hipError_t e = hipMemcpy(d_in, angles, sizeof(float) * std::size_t(n), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(prxd::k_sincos, dim3(unsigned((n + 255) / 256)), dim3(256), 0, 0, d_in, d_out, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(out, d_out, 2 * sizeof(float) * std::size_t(n), hipMemcpyDeviceToHost);
+    (void)hipFree(d_in);
+    (void)hipFree(d_out);
+    return e == hipSuccess ? 0 : 5;
+}

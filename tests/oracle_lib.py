@@ -32,6 +32,7 @@ def lib():
         L.oracle_state.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]
         L.oracle_bodies_f64.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.oracle_free.argtypes = [ctypes.c_void_p]
+        L.oracle_sincosf.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
         L.oracle_time_batch.restype = ctypes.c_double
         L.oracle_time_batch.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int,
@@ -108,3 +109,11 @@ def time_batch(descs, steps, threads):
     bs = ctypes.c_double()
     secs = lib().oracle_time_batch(packed, len(packed), len(descs), steps, threads, ctypes.byref(bs))
     return secs, bs.value
+
+
+def host_sincosf(angles):
+    """(cos, sin) pairs from the host C library's cosf/sinf."""
+    a = np.ascontiguousarray(angles, dtype=np.float32)
+    out = np.zeros((a.size, 2), dtype=np.float32)
+    lib().oracle_sincosf(a.ctypes.data, out.ctypes.data, a.size)
+    return out

@@ -65,3 +65,43 @@ def test_double_tracks_float_short_horizon():
     a = w32.bodies_f64(211)
     b = w64.bodies_f64(211)
     assert np.max(np.abs(a[:, :2] - b[:, :2])) < 1e-4
+
+
+def test_host_sincos_matches_restated_routine():
+    """The restated float sine/cosine used on the device (csrc/sincosf.hpp,
+    compiled here for the host) equals the host C library on a dense sample;
+    the full-range comparison over all 2^32 floats is done offline."""
+    import ctypes
+    import subprocess
+    import tempfile
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = r'''
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <cstdio>
+#include "sincosf.hpp"
+int main() {
+  long bad = 0;
+  for (std::uint64_t b = 0; b < 0x7f800000ull; b += 4099) {
+    for (int sg = 0; sg < 2; ++sg) {
+      std::uint32_t u = std::uint32_t(b) | (sg ? 0x80000000u : 0u);
+      float x; std::memcpy(&x, &u, 4);
+      float c = std::cos(x), s = std::sin(x);
+      float c1 = prx_math::CosF<true>(x), s1 = prx_math::SinF<true>(x);
+      bad += std::memcmp(&c, &c1, 4) != 0;
+      bad += std::memcmp(&s, &s1, 4) != 0;
+    }
+  }
+  std::printf("%ld\n", bad);
+  return 0;
+}
+'''
+    with tempfile.TemporaryDirectory() as d:
+        cpp = os.path.join(d, "t.cpp")
+        exe = os.path.join(d, "t")
+        open(cpp, "w").write(src)
+        subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I",
+                        os.path.join(root, "playrho_amd", "csrc"), cpp, "-o", exe], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    assert int(out.strip()) == 0
