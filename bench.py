@@ -1,0 +1,188 @@
+"""Benchmark of the batched step: body-steps/s at 4096 uneven worlds on one
+MI355X, with the plain per-world GPU form and the reference CPU build timed
+in the same run.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--worlds 4096] [--seed 1]
+
+Workload: 4096 independent "pile" worlds, world w with a seeded body count
+drawn log-uniform in [8, 512] (boxes, disks and 3..8-gons in stacks or piles
+on a ground edge), PlayRho's regular step (dt = 1/60, 8 velocity and 3
+position iterations, doToi = false), all worlds advanced by one hipGraph
+launch per step.  W untimed warmup steps are followed by K timed steps; inputs
+are resident in device memory when the timed region starts.
+
+Prints one JSON line (rank 0, one GPU).
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "body-steps/s at 4096 batched worlds vs plain GPU and CPU; adjoint steps/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=200)
+    p.add_argument("--warmup", type=int, default=30)
+    p.add_argument("--worlds", type=int, default=4096)
+    p.add_argument("--seed", type=int, default=1)
+    p.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cores)")
+    p.add_argument("--cpu-worlds", type=int, default=1024, help="worlds in the CPU sample")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--plain-worlds", type=int, default=64)
+    p.add_argument("--plain-steps", type=int, default=10)
+    return p.parse_args()
+
+
+def algorithmic_bytes(awake, proxies, shape_bytes, contacts, joints=0):
+    # B_step = 116 B_awake + 48 P + sum(8 + 16 n_v) + 136 C + 88 J
+    return 116 * awake + 48 * proxies + shape_bytes + 136 * contacts + 88 * joints
+
+
+def cpu_baseline(args, threads):
+    ref = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    n = min(args.cpu_worlds, args.worlds)
+    if os.path.exists(ref):
+        out = subprocess.run([ref, "--time", "pile", str(args.warmup), str(args.steps), "0", str(threads),
+                              str(n), str(args.seed), "0"], check=True, capture_output=True, text=True).stdout
+        r = json.loads(out.strip().splitlines()[-1])
+        return dict(value=r["body_steps"] / r["seconds"], unit="body-steps/s", cores=threads, kind="reference",
+                    sample="reference PlayRho float build (oracle/_ref, g++ -O3), worlds 0..%d of the same "
+                           "workload, steps %d..%d timed after %d warmup, %d std::threads, one world per "
+                           "thread at a time, doToi=false" % (n - 1, args.warmup + 1, args.warmup + args.steps,
+                                                              args.warmup, threads),
+                    seconds=r["seconds"])
+    from tests import oracle_lib
+    import playrho_amd as pa
+    descs = []
+    for w in range(n):
+        wd = pa.World.scene("pile", [args.seed, w])
+        descs.append(wd.save_desc() + conf_record(pa))
+    secs, bs = oracle_lib.time_batch(descs, args.steps, threads)
+    return dict(value=bs / secs, unit="body-steps/s", cores=threads, kind="port",
+                sample="oracle restatement, worlds 0..%d, %d steps from step 1" % (n - 1, args.steps),
+                seconds=secs)
+
+
+def conf_record(pa):
+    import struct
+    c = pa.StepConf()
+    floats = [getattr(c, f) for f, _ in pa.StepConf._fields_[:18]]
+    return (struct.pack("<I", 18) + struct.pack("<18f", *floats) +
+            struct.pack("<III", c.reg_velocity_iters, c.reg_position_iters,
+                        (1 if c.do_warm_start else 0) | (4 if c.do_blocksolve else 0)))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU; worlds would shard across GPUs without "
+              "communication, but no figure at several GPUs is produced here (use --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+    import playrho_amd as pa
+    pa.require_device()
+    conf = pa.StepConf()
+
+    t0 = time.time()
+    worlds = [pa.World.scene("pile", [args.seed, w]) for w in range(args.worlds)]
+    build_s = time.time() - t0
+    total_bodies = sum(w.body_count for w in worlds)
+    batch = pa.Batch(worlds, use_graph=True)
+    dyn = batch.dynamic_bodies()
+
+    batch.step(conf, args.warmup)
+    batch.sync()
+    batch.download()
+    c0 = batch.counts()
+
+    batch.sync()
+    t1 = time.perf_counter()
+    batch.step(conf, args.steps)
+    batch.sync()
+    wall = time.perf_counter() - t1
+    kernel_ms, launches = batch.last_timing()
+
+    batch.download()
+    c1 = batch.counts()
+    contacts_avg = 0.5 * (c0[4] + c1[4])
+    bytes_step = 0.5 * (algorithmic_bytes(*c0[:4]) + algorithmic_bytes(*c1[:4]))
+    ms_launch = kernel_ms / max(1, launches)
+    achieved = bytes_step / (ms_launch * 1e-3) / 1e9
+
+    value = dyn * args.steps / wall
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "%d batched uneven pile worlds (bodies log-uniform in [8,512], seed %d), regular "
+                        "step dt=1/60 8/3 iterations doToi=false, one hipGraph launch per step"
+                        % (args.worlds, args.seed),
+            "worlds": args.worlds,
+            "bodies": total_bodies,
+            "dynamic_bodies": int(dyn),
+        },
+        "contacts_per_s": contacts_avg * args.steps / wall,
+        "state": {"awake_dynamic_bodies": [c0[0], c1[0]], "proxies": c1[1], "contacts": [c0[3], c1[3]],
+                  "touching_contacts": [c0[4], c1[4]]},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "k_step_worlds",
+            "algorithmic_bytes_per_launch": bytes_step,
+            "kernel_ms_per_launch": ms_launch,
+        },
+    }
+
+    # Plain per-world form: one world per launch, synchronised, on a sample.
+    step_idx = max(1, args.worlds // max(1, args.plain_worlds))
+    sample = [pa.World.scene("pile", [args.seed, w]) for w in range(0, args.worlds, step_idx)][:args.plain_worlds]
+    for w in sample:
+        for _ in range(args.warmup):
+            w.step(conf)
+    nb = sum(sum(1 for f in w.bodies()[1] if f & 2) for w in sample)
+    t2 = time.perf_counter()
+    for w in sample:
+        for _ in range(args.plain_steps):
+            w.step(conf)
+    plain_s = time.perf_counter() - t2
+    plain = nb * args.plain_steps / plain_s
+    result["plain_gpu"] = {"value": plain, "unit": "body-steps/s",
+                           "sample": "%d worlds (every %dth), one synchronised launch per world per step, "
+                                     "%d steps after %d warmup" % (len(sample), step_idx, args.plain_steps,
+                                                                  args.warmup)}
+    result["ratio_vs_plain_gpu"] = value / plain
+
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cb = cpu_baseline(args, threads)
+        result["cpu_baseline"] = cb
+        result["ratio_vs_cpu"] = value / cb["value"]
+    result["setup_s"] = build_s
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -169,6 +169,10 @@ int prx_batch_download(prx_batch* batch);
 int prx_batch_last_timing(prx_batch* batch, float* ms, int* launches);
 /* Sum over worlds of dynamic bodies, and of contacts (after download). */
 uint64_t prx_batch_dynamic_bodies(const prx_batch* batch);
+/* Totals over the batch's worlds from the last download: out[0] awake
+ * dynamic bodies, out[1] proxies, out[2] sum over proxies of (8 + 16 x vertex
+ * count) bytes, out[3] contacts, out[4] touching contacts. */
+int prx_batch_counts(prx_batch* batch, uint64_t out[5]);
 /* Tries launch schedules on `nsteps` steps of a copy of the batch state and
  * keeps the fastest in conf. */
 int prx_batch_autotune(prx_batch* batch, const prx_step_conf* step, int nsteps,

@@ -5,6 +5,7 @@
 // reference library built from its sources by oracle/Makefile.ref.
 //
 //   ref_driver <scene> <steps> <doToi 0|1> <out.bin> [int params...]
+//   ref_driver --time <scene> <warmup> <steps> <doToi> <threads> <nworlds> [params...]
 //
 // Record layout (all u32/f32):
 //   "PRXD" 1
@@ -31,7 +32,9 @@
 #include <playrho/d2/EdgeShapeConf.hpp>
 #include <playrho/StepConf.hpp>
 
+#include <chrono>
 #include <cstdio>
+#include <thread>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -242,22 +245,7 @@ void WriteState(Out& o, const World& world, std::uint32_t step, const StepStats&
 
 }  // namespace
 
-int main(int argc, char** argv) {
-    if (argc < 5) {
-        std::fprintf(stderr, "usage: %s scene steps doToi out.bin [params...]\n", argv[0]);
-        return 2;
-    }
-    const std::string name = argv[1];
-    const int steps = std::atoi(argv[2]);
-    const bool doToi = std::atoi(argv[3]) != 0;
-    std::vector<int> params;
-    for (int i = 5; i < argc; ++i) params.push_back(std::atoi(argv[i]));
-    prx_scene::SceneSpec spec;
-    if (!prx_scene::Make(name, params.data(), int(params.size()), spec)) {
-        std::fprintf(stderr, "unknown scene %s\n", name.c_str());
-        return 2;
-    }
-
+World BuildWorld(const prx_scene::SceneSpec& spec) {
     auto world = World{};
     // Shapes are created in spec order, each immediately before the first body
     // that uses it, matching the order the product's scene builder uses.
@@ -280,12 +268,102 @@ int main(int argc, char** argv) {
         }
         CreateBody(world, conf);
     }
+    return world;
+}
 
+StepConf RegularStepConf(bool doToi) {
     auto conf = StepConf{};
     conf.deltaTime = 1.0f / 60;
     conf.regVelocityIters = 8;
     conf.regPositionIters = 3;
     conf.doToi = doToi;
+    return conf;
+}
+
+// --time <scene> <warmup> <steps> <doToi> <threads> <nworlds> [params...]:
+// builds nworlds copies of a scene (for "pile", param 1 is the first world
+// index and world i uses index first + i), steps each `warmup` times untimed,
+// then `steps` times timed, on `threads` std::threads (static partition
+// balanced by body count; PlayRho steps one world on one thread), and prints
+// the timing as one JSON line.
+int TimeWorlds(int argc, char** argv) {
+    if (argc < 8) return 2;
+    const std::string name = argv[2];
+    const int warmup = std::atoi(argv[3]);
+    const int steps = std::atoi(argv[4]);
+    const bool doToi = std::atoi(argv[5]) != 0;
+    int threads = std::atoi(argv[6]);
+    const int nworlds = std::atoi(argv[7]);
+    std::vector<int> params;
+    for (int i = 8; i < argc; ++i) params.push_back(std::atoi(argv[i]));
+    std::vector<World> worlds;
+    std::vector<double> weight;
+    double dyn = 0;
+    for (int k = 0; k < nworlds; ++k) {
+        std::vector<int> p = params;
+        if (name == "pile") {
+            if (p.size() < 2) p.resize(2, 0);
+            p[1] += k;
+        }
+        prx_scene::SceneSpec spec;
+        if (!prx_scene::Make(name, p.data(), int(p.size()), spec)) return 2;
+        double nb = 0;
+        for (const auto& b : spec.bodies) nb += (b.type == prx_scene::kDynamic) ? 1.0 : 0.0;
+        dyn += nb;
+        weight.push_back(nb);
+        worlds.push_back(BuildWorld(spec));
+    }
+    if (threads < 1) threads = 1;
+    std::vector<size_t> cut(size_t(threads) + 1, worlds.size());
+    cut[0] = 0;
+    double total = 0;
+    for (double x : weight) total += x;
+    double acc = 0;
+    int t = 1;
+    for (size_t i = 0; i < worlds.size() && t < threads; ++i) {
+        acc += weight[i];
+        while (t < threads && acc >= total * t / threads) cut[size_t(t++)] = i + 1;
+    }
+    const auto conf = RegularStepConf(doToi);
+    auto run = [&](int nsteps) {
+        std::vector<std::thread> pool;
+        for (int k = 0; k < threads; ++k) {
+            pool.emplace_back([&, k] {
+                for (size_t i = cut[size_t(k)]; i < cut[size_t(k) + 1]; ++i)
+                    for (int s = 0; s < nsteps; ++s) Step(worlds[i], conf);
+            });
+        }
+        for (auto& th : pool) th.join();
+    };
+    run(warmup);
+    const auto t0 = std::chrono::steady_clock::now();
+    run(steps);
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("{\"seconds\": %.6f, \"body_steps\": %.0f, \"threads\": %d, \"worlds\": %d, \"steps\": %d}\n",
+                secs, dyn * steps, threads, nworlds, steps);
+    return 0;
+}
+
+int main(int argc, char** argv) {
+    if (argc > 1 && std::string(argv[1]) == "--time") return TimeWorlds(argc, argv);
+    if (argc < 5) {
+        std::fprintf(stderr, "usage: %s scene steps doToi out.bin [params...]\n", argv[0]);
+        return 2;
+    }
+    const std::string name = argv[1];
+    const int steps = std::atoi(argv[2]);
+    const bool doToi = std::atoi(argv[3]) != 0;
+    std::vector<int> params;
+    for (int i = 5; i < argc; ++i) params.push_back(std::atoi(argv[i]));
+    prx_scene::SceneSpec spec;
+    if (!prx_scene::Make(name, params.data(), int(params.size()), spec)) {
+        std::fprintf(stderr, "unknown scene %s\n", name.c_str());
+        return 2;
+    }
+
+    auto world = BuildWorld(spec);
+
+    const auto conf = RegularStepConf(doToi);
 
     Out o;
     o.put("PRXD", 4);

@@ -126,6 +126,7 @@ _SIGS = {
                                              ctypes.POINTER(ctypes.c_int)]),
     "prx_batch_dynamic_bodies": (ctypes.c_uint64, [ctypes.c_void_p]),
     "prx_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "prx_batch_counts": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "prx_selftest_sincos": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
 }
 
@@ -289,6 +290,12 @@ class Batch:
 
     def dynamic_bodies(self):
         return lib().prx_batch_dynamic_bodies(self.h)
+
+    def counts(self):
+        """(awake dynamic bodies, proxies, shape bytes, contacts, touching) after download()."""
+        out = np.zeros(5, dtype=np.uint64)
+        _check(lib().prx_batch_counts(self.h, out.ctypes.data))
+        return tuple(int(x) for x in out)
 
 
 def device_sincos(angles):

@@ -1398,6 +1398,23 @@ std::uint64_t prx_batch_dynamic_bodies(const prx_batch* b) {
     return n;
 }
 
+int prx_batch_counts(prx_batch* b, std::uint64_t out[5]) {
+    if (!b || !out) return Fail(PRX_E_INVALID_ARGUMENT, "null argument");
+    for (int i = 0; i < 5; ++i) out[i] = 0;
+    for (const prx_world* w : b->worlds) {
+        if (!w) continue;
+        for (const auto& hb : w->bodies)
+            out[0] += ((hb.flags & prx::kAccelerable) && (hb.flags & prx::kAwake)) ? 1u : 0u;
+        for (const auto& p : w->proxies) {
+            ++out[1];
+            out[2] += 8u + 16u * std::uint64_t(w->shapes[std::size_t(p.shape)].v.size());
+        }
+        out[3] += w->contacts.size();
+        for (std::size_t i = 0; i < w->contacts.size(); ++i) out[4] += (w->contacts.flags[i] & prx::kTouching) ? 1u : 0u;
+    }
+    return PRX_OK;
+}
+
 int prx_batch_autotune(prx_batch* b, const prx_step_conf* step, int nsteps, prx_batch_conf* best) {
     if (!b || !step) return Fail(PRX_E_INVALID_ARGUMENT, "null argument");
     if (best) *best = b->conf;
