@@ -157,6 +157,15 @@ def main():
     }
 
     # Plain per-world form: one world per launch, synchronised, on a sample.
+    if args.plain_worlds <= 0:
+        result["setup_s"] = build_s
+        if not args.no_cpu:
+            threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+            cb = cpu_baseline(args, threads)
+            result["cpu_baseline"] = cb
+            result["ratio_vs_cpu"] = value / cb["value"]
+        print(json.dumps(result))
+        return
     step_idx = max(1, args.worlds // max(1, args.plain_worlds))
     sample = [pa.World.scene("pile", [args.seed, w]) for w in range(0, args.worlds, step_idx)][:args.plain_worlds]
     for w in sample:
