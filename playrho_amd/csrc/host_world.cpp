@@ -319,6 +319,7 @@ struct prx_batch {
     prx::StepParams graph_params{};
     bool graph_valid = false;
     std::size_t totB = 0, totP = 0, totS = 0, totV = 0, totC = 0;
+    int lds_bytes = 16 * 1024;
     ~prx_batch() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -550,7 +551,8 @@ prx::StepParams ToParams(const prx_step_conf& c) {
 
 int ContactCap(const prx_world* w) {
     const int per = w->conf.contact_capacity_per_proxy ? int(w->conf.contact_capacity_per_proxy) : 8;
-    return std::max<int>(int(w->contacts.size()) + 64, per * int(w->proxies.size()) + 64);
+    const int c = std::max<int>(int(w->contacts.size()) + 64, per * int(w->proxies.size()) + 64);
+    return std::max<int>(c, int(w->bodies.size()) + 64);
 }
 
 template <class T>
@@ -587,6 +589,9 @@ int BuildBatch(prx_batch* b) {
         for (const auto& s : w->shapes) V += s.v.size();
         C += std::size_t(h.contact_cap);
     }
+    int maxNb = 0;
+    for (const prx_world* w : b->worlds) maxNb = std::max<int>(maxNb, int(w->bodies.size()));
+    b->lds_bytes = std::min<int>(64 * 1024, std::max<int>(1024, ((maxNb * 32 + 255) / 256) * 256));
     b->totB = B;
     b->totP = P;
     b->totS = S;
@@ -631,8 +636,8 @@ int BuildBatch(prx_batch* b) {
     PRX_HIP(a.alloc(d.s_keys, C));
     PRX_HIP(a.alloc(d.s_keys2, C));
     PRX_HIP(a.alloc(d.s_csr_off, B + std::size_t(nw)));
-    PRX_HIP(a.alloc(d.s_csr, 2 * C));
-    PRX_HIP(a.alloc(d.s_stack, B));
+    PRX_HIP(a.alloc(d.s_csr, 4 * C));
+    PRX_HIP(a.alloc(d.s_stack, 2 * B));
     PRX_HIP(a.alloc(d.s_isl_b, B));
     PRX_HIP(a.alloc(d.s_isl_c, C));
     PRX_HIP(a.alloc(d.s_isl_body, B + C));
@@ -867,7 +872,8 @@ int StepBatch(prx_batch* b, const prx_step_conf* conf, int nsteps) {
     if (!conf) return Fail(PRX_E_INVALID_ARGUMENT, "null step conf");
     if (conf->do_toi) return Fail(PRX_E_UNSUPPORTED, "doToi = true is not stepped by this engine");
     if (nsteps < 0) return Fail(PRX_E_INVALID_ARGUMENT, "negative step count");
-    const prx::StepParams p = ToParams(*conf);
+    prx::StepParams p = ToParams(*conf);
+    p.ldsBytes = std::uint32_t(b->lds_bytes);
     const int nw = int(b->worlds.size());
     PRX_HIP(hipEventRecord(b->ev0, b->stream));
     if (b->conf.use_graph && nsteps > 1) {

@@ -65,6 +65,7 @@ struct StepParams {
         maxLinearCorrection, maxAngularCorrection, linearSleepTolerance, angularSleepTolerance,
         displaceMultiplier, aabbExtension, maxCirclesRatio;
     std::uint32_t velIters, posIters, warmStart, blockSolve;
+    std::uint32_t ldsBytes;  // dynamic LDS per workgroup (island solve state)
 };
 
 struct DeviceState {
@@ -115,8 +116,8 @@ struct DeviceState {
     std::uint32_t* s_keys;   // candidate pair keys (cap)
     std::uint32_t* s_keys2;  // sorted candidate keys (cap)
     int* s_csr_off;          // per-body offsets (nbodies + 1 per world: body_off + w)
-    int* s_csr;              // per-body touching-contact lists (2 * cap)
-    int* s_stack;            // DFS stack (nbodies)
+    int* s_csr;              // per-body touching-contact lists: (contact, other body) pairs (4 * cap ints)
+    int* s_stack;            // DFS stack / adjacency cursors, then island of each body (2 * nbodies)
     std::uint8_t* s_isl_b;   // islanded bodies (nbodies)
     std::uint8_t* s_isl_c;   // islanded contacts (cap)
     int* s_isl_body;         // island body lists (nbodies + cap)

@@ -921,38 +921,207 @@ __device__ float4 integrated_velocity(const DeviceState& d, int gb, const StepPa
     return float4{lin.x, lin.y, ang, 0.0f};
 }
 
-// SolveRegIslandViaGS (AabbTreeWorld.cpp:1563-1682) for one island, run by one
-// lane.  Returns bodies put to sleep.
-__device__ std::uint32_t solve_island(const DeviceState& d, const WorldCtx& x, int b0, int b1, int c0, int c1,
-                                      const StepParams& c) {
+// ---------------------------------------------------------------------------
+// SolveReg (AabbTreeWorld.cpp:1472-1561) and SolveRegIslandViaGS (:1563-1682).
+//
+// Only two things in an island solve depend on order: the island's contact
+// order (the DFS) and the Gauss-Seidel sweeps over it.  Everything else is
+// independent per body or per contact and runs lane-parallel over all islands
+// of the world at once.  Static bodies are left out of island body lists: the
+// reference adds them but they never propagate the search, their solver
+// updates are exact no-ops (zero inverse mass and velocity) and they are not
+// written back, so islands become disjoint in every body.
+//
+// Body-constraint positions and velocities live in LDS when the world fits
+// (32 bytes per body), else in global scratch.
+
+extern __shared__ float4 g_smem[];
+
+template <bool kLds>
+struct Bodies {
+    float4* gpos;
+    float4* gvel;
+    int nb;
+    __device__ __forceinline__ float4 pos(int b) const {
+        if constexpr (kLds) return g_smem[b];
+        else return gpos[b];
+    }
+    __device__ __forceinline__ float4 vel(int b) const {
+        if constexpr (kLds) return g_smem[nb + b];
+        else return gvel[b];
+    }
+    __device__ __forceinline__ void set_pos(int b, float4 v) const {
+        if constexpr (kLds) g_smem[b] = v;
+        else gpos[b] = v;
+    }
+    __device__ __forceinline__ void set_vel(int b, float4 v) const {
+        if constexpr (kLds) g_smem[nb + b] = v;
+        else gvel[b] = v;
+    }
+};
+
+// Inclusive wave scan of a 64-lane value.
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int t = __shfl_up(v, o);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Exclusive prefix sum of a[0..n) in place (wave-parallel, carry across chunks);
+// returns the total.
+__device__ int wave_exclusive_scan(int* a, int n, int lane) {
+    int carry = 0;
+    for (int base = 0; base < n; base += kWave) {
+        const int i = base + lane;
+        const int v = (i < n) ? a[i] : 0;
+        const int inc = wave_incl_scan(v, lane);
+        if (i < n) a[i] = carry + inc - v;
+        carry += __shfl(inc, kWave - 1);
+    }
+    return carry;
+}
+
+struct IslandScratch {
+    int* off;         // per-body CSR offsets (nb + 1)
+    int2* csr;        // per-body touching contacts: contact index, other body
+    int* ibody;       // island body lists
+    int* icont;       // island contact lists (contact indices)
+    int2* istart;     // island i: first body entry, first contact entry
+    int* bisland;     // island of each body (-1: none)
+    std::uint32_t* isl_word;  // per island: bit 0 solved, bits 1..: unused; min under-active time bits
+    std::uint32_t* isl_under;
+};
+
+// Per-body CSR of touching, enabled, non-sensor contacts in contact order
+// (each body's contact list order), entries carry the other body.
+__device__ void build_adjacency(const DeviceState& d, const WorldCtx& x, int nc, const IslandScratch& S) {
+    for (int b = x.lane; b <= x.nb; b += kWave) S.off[b] = 0;
+    __syncthreads();
+    for (int i = x.lane; i < nc; i += kWave) {
+        const std::uint32_t f = d.c_flags[x.co + i];
+        if ((f & prx::kCEnabled) && (f & prx::kTouching) && !(f & prx::kSensor)) {
+            const int2 bd = d.c_body[x.co + i];
+            atomicAdd(&S.off[bd.x], 1);
+            atomicAdd(&S.off[bd.y], 1);
+        }
+    }
+    __syncthreads();
+    wave_exclusive_scan(S.off, x.nb + 1, x.lane);
+    __syncthreads();
+    int* cur = d.s_stack + 2 * x.bo;
+    for (int b = x.lane; b < x.nb; b += kWave) cur[b] = S.off[b];
+    __syncthreads();
+    for (int i = x.lane; i < nc; i += kWave) {
+        const std::uint32_t f = d.c_flags[x.co + i];
+        if ((f & prx::kCEnabled) && (f & prx::kTouching) && !(f & prx::kSensor)) {
+            const int2 bd = d.c_body[x.co + i];
+            S.csr[atomicAdd(&cur[bd.x], 1)] = int2{i, bd.y};
+            S.csr[atomicAdd(&cur[bd.y], 1)] = int2{i, bd.x};
+        }
+    }
+    __syncthreads();
+    for (int b = x.lane; b < x.nb; b += kWave) {
+        const int s0 = S.off[b], e0 = S.off[b + 1];
+        for (int i = s0 + 1; i < e0; ++i) {
+            const int2 v = S.csr[i];
+            int j = i - 1;
+            while (j >= s0 && S.csr[j].x > v.x) {
+                S.csr[j + 1] = S.csr[j];
+                --j;
+            }
+            S.csr[j + 1] = v;
+        }
+    }
+    __syncthreads();
+}
+
+// Islands (AddToIsland, AabbTreeWorld.cpp:1358-1444; seeds in body order
+// :1495-1523).  Awake speedable bodies without touching contacts are
+// one-body islands and are emitted lane-parallel; the rest are found by a
+// depth-first search on one lane with the islanded flags in LDS when they
+// fit.  Returns the island count.
+template <bool kLdsFlags>
+__device__ int find_islands(const DeviceState& d, const WorldCtx& x, int nc, const IslandScratch& S) {
+    __shared__ int s_cnt[2];
+    std::uint8_t* islb = kLdsFlags ? reinterpret_cast<std::uint8_t*>(g_smem) : d.s_isl_b + x.bo;
+    std::uint8_t* islc = kLdsFlags ? reinterpret_cast<std::uint8_t*>(g_smem) + x.nb : d.s_isl_c + x.co;
+    for (int b = x.lane; b < x.nb; b += kWave) {
+        islb[b] = 0;
+        S.bisland[b] = -1;
+    }
+    for (int i = x.lane; i < nc; i += kWave) islc[i] = 0;
+    __syncthreads();
+    // One-body islands.
+    int nsingle = 0;
+    for (int base = 0; base < x.nb; base += kWave) {
+        const int b = base + x.lane;
+        bool single = false;
+        if (b < x.nb) {
+            const std::uint32_t f = d.bflags[x.bo + b];
+            single = (f & prx::kAwake) && (f & prx::kEnabled) && (f & prx::kSpeedable) && S.off[b] == S.off[b + 1];
+        }
+        const unsigned long long m = __ballot(single);
+        if (single) {
+            const int r = nsingle + __popcll(m & lanes_below(x.lane));
+            S.ibody[r] = b;
+            S.istart[r] = int2{r, 0};
+            S.bisland[b] = r;
+            islb[b] = 1;
+        }
+        nsingle += __popcll(m);
+    }
+    __syncthreads();
+    if (x.lane == 0) {
+        int nisl = nsingle, nbe = nsingle, nce = 0;
+        int* stack = d.s_stack + 2 * x.bo;
+        for (int seed = 0; seed < x.nb; ++seed) {
+            if (islb[seed]) continue;
+            if (S.off[seed] == S.off[seed + 1]) continue;
+            const std::uint32_t sf = d.bflags[x.bo + seed];
+            if (!((sf & prx::kAwake) && (sf & prx::kEnabled) && (sf & prx::kSpeedable))) continue;
+            S.istart[nisl] = int2{nbe, nce};
+            int sp = 0;
+            stack[sp++] = seed;
+            islb[seed] = 1;
+            while (sp > 0) {
+                const int b = stack[--sp];
+                S.ibody[nbe++] = b;
+                S.bisland[b] = nisl;
+                d.bflags[x.bo + b] |= prx::kAwake;
+                const int e1 = S.off[b + 1];
+                for (int e = S.off[b]; e < e1; ++e) {
+                    const int2 en = S.csr[e];
+                    if (islc[en.x]) continue;
+                    S.icont[nce++] = en.x;
+                    islc[en.x] = 1;
+                    if (!islb[en.y]) {
+                        islb[en.y] = 1;
+                        if (d.bflags[x.bo + en.y] & prx::kSpeedable) stack[sp++] = en.y;
+                    }
+                }
+            }
+            ++nisl;
+        }
+        S.istart[nisl] = int2{nbe, nce};
+        s_cnt[0] = nisl;
+    }
+    __syncthreads();
+    const int nisl = s_cnt[0];
+    // One-body islands have no contacts: their contact range is empty at the
+    // start of the contact list.
+    return nisl;
+}
+
+// Lane-serial Gauss-Seidel velocity sweeps of one island (warm start first),
+// ContactSolver.cpp:481-492 per constraint in island order; the next
+// constraint's record is loaded while the current one is solved.
+template <bool kLds>
+__device__ void island_velocity(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs, int c0, int c1,
+                                const StepParams& c) {
+    if (c0 >= c1) return;
     const int bo = x.bo;
-    for (int e = b0; e < b1; ++e) {
-        const int b = d.s_isl_body[bo + x.co + e];
-        d.pos0[bo + b] = d.pos1[bo + b];
-    }
-    for (int e = b0; e < b1; ++e) {
-        const int b = d.s_isl_body[bo + x.co + e];
-        d.s_bc_pos[bo + b] = d.pos1[bo + b];
-        d.s_bc_vel[bo + b] = integrated_velocity(d, bo + b, c);
-    }
-    // Velocity constraints in island contact order.
-    for (int e = c0; e < c1; ++e) {
-        const int ci = d.s_isl_contact[x.co + e];
-        const int gi = x.co + ci;
-        const int2 bd = d.c_body[gi];
-        const int2 ss = d.c_shape[gi];
-        const float rA = d.shape_f[ss.x].x, rB = d.shape_f[ss.y].x;
-        const BodyC A = load_bodyc(d, bo + bd.x), B = load_bodyc(d, bo + bd.y);
-        const float4 pA = d.s_bc_pos[bo + bd.x], pB = d.s_bc_pos[bo + bd.y];
-        const Xf xfA = get_xf(pA.x, pA.y, pA.z, A.lc);
-        const Xf xfB = get_xf(pB.x, pB.y, pB.z, B.lc);
-        const WM wm = world_manifold(load_manifold(d, gi), xfA, rA, xfB, rB);
-        const float4 mt = d.c_mat[gi];
-        const Vc vc = make_vc(wm, mt.x, mt.y, mt.z, bd.x, bd.y, A, pA, d.s_bc_vel[bo + bd.x], B, pB,
-                              d.s_bc_vel[bo + bd.y], c);
-        store_vc(d.s_vc + std::size_t(gi) * prx::kVcStride, vc);
-    }
-    // Warm start (AabbTreeWorld.cpp:230-274).
     if (c.warmStart) {
         for (int e = c0; e < c1; ++e) {
             const int gi = x.co + d.s_isl_contact[x.co + e];
@@ -971,81 +1140,162 @@ __device__ std::uint32_t solve_island(const DeviceState& d, const WorldCtx& x, i
                 dBl = dBl + mul(B.invMass, P);
                 dBa = dBa + B.invRotI * LB;
             }
-            float4 va = d.s_bc_vel[bo + vc.bA];
+            float4 va = bs.vel(vc.bA);
             va.x = va.x + dAl.x;
             va.y = va.y + dAl.y;
             va.z = va.z + dAa;
-            d.s_bc_vel[bo + vc.bA] = va;
-            float4 vb = d.s_bc_vel[bo + vc.bB];
+            bs.set_vel(vc.bA, va);
+            float4 vb = bs.vel(vc.bB);
             vb.x = vb.x + dBl.x;
             vb.y = vb.y + dBl.y;
             vb.z = vb.z + dBa;
-            d.s_bc_vel[bo + vc.bB] = vb;
+            bs.set_vel(vc.bB, vb);
         }
     }
-    // Velocity iterations.
     for (std::uint32_t it = 0; it < c.velIters; ++it) {
         float newInc = 0.0f;
+        int gi = x.co + d.s_isl_contact[x.co + c0];
+        Vc vc = load_vc(d.s_vc + std::size_t(gi) * prx::kVcStride);
+        BodyC A = load_bodyc(d, bo + vc.bA), B = load_bodyc(d, bo + vc.bB);
         for (int e = c0; e < c1; ++e) {
-            const int gi = x.co + d.s_isl_contact[x.co + e];
-            float4* rec = d.s_vc + std::size_t(gi) * prx::kVcStride;
-            Vc vc = load_vc(rec);
-            const BodyC A = load_bodyc(d, bo + vc.bA), B = load_bodyc(d, bo + vc.bB);
-            float4 va = d.s_bc_vel[bo + vc.bA];
-            float4 vb = d.s_bc_vel[bo + vc.bB];
+            // Prefetch the next record (independent of this solve).
+            const int gn = (e + 1 < c1) ? x.co + d.s_isl_contact[x.co + e + 1] : gi;
+            const Vc nvc = load_vc(d.s_vc + std::size_t(gn) * prx::kVcStride);
+            const BodyC nA = load_bodyc(d, bo + nvc.bA), nB = load_bodyc(d, bo + nvc.bB);
+            float4 va = bs.vel(vc.bA);
+            float4 vb = bs.vel(vc.bB);
             float mi = 0.0f;
             mi = fmax_std(mi, solve_dir(vc, true, A, B, va, vb));
             float nrm;
-            if (vc.n == 1 || (vc.K00 == 0.0f && vc.K11 == 0.0f && vc.K01 == 0.0f)) {
-                nrm = solve_dir(vc, false, A, B, va, vb);
-            } else {
-                nrm = block_solve(vc, A, B, va, vb);
-            }
+            if (vc.n == 1 || (vc.K00 == 0.0f && vc.K11 == 0.0f && vc.K01 == 0.0f)) nrm = solve_dir(vc, false, A, B, va, vb);
+            else nrm = block_solve(vc, A, B, va, vb);
             mi = fmax_std(mi, nrm);
-            d.s_bc_vel[bo + vc.bA] = va;
-            d.s_bc_vel[bo + vc.bB] = vb;
+            bs.set_vel(vc.bA, va);
+            bs.set_vel(vc.bB, vb);
+            float4* rec = d.s_vc + std::size_t(gi) * prx::kVcStride;
             rec[4] = float4{vc.p[0].ni, vc.p[0].ti, vc.p[0].nm, vc.p[0].tm};
             rec[7] = float4{vc.p[1].ni, vc.p[1].ti, vc.p[1].nm, vc.p[1].tm};
             newInc = fmax_std(newInc, mi);
+            gi = gn;
+            vc = nvc;
+            A = nA;
+            B = nB;
         }
         if (newInc <= c.regMinMomentum) break;
     }
+}
+
+// Lane-serial position sweeps of one island (ContactSolver.cpp:494-643,
+// AabbTreeWorld.cpp:1627-1644).  Returns whether the island was solved.
+template <bool kLds>
+__device__ bool island_position(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs, int c0, int c1,
+                                const StepParams& c) {
+    const int bo = x.bo;
+    for (std::uint32_t it = 0; it < c.posIters; ++it) {
+        float minSep = __builtin_inff();
+        if (c0 < c1) {
+            int gi = x.co + d.s_isl_contact[x.co + c0];
+            Manifold m = load_manifold(d, gi);
+            int2 bd = d.c_body[gi];
+            int2 ss = d.c_shape[gi];
+            float tr = d.shape_f[ss.x].x + d.shape_f[ss.y].x;
+            BodyC A = load_bodyc(d, bo + bd.x), B = load_bodyc(d, bo + bd.y);
+            for (int e = c0; e < c1; ++e) {
+                const int gn = (e + 1 < c1) ? x.co + d.s_isl_contact[x.co + e + 1] : gi;
+                const Manifold nm = load_manifold(d, gn);
+                const int2 nbd = d.c_body[gn];
+                const int2 nss = d.c_shape[gn];
+                const float ntr = d.shape_f[nss.x].x + d.shape_f[nss.y].x;
+                const BodyC nA = load_bodyc(d, bo + nbd.x), nB = load_bodyc(d, bo + nbd.y);
+                float4 pa = bs.pos(bd.x);
+                float4 pb = bs.pos(bd.y);
+                const float sep = solve_pc(m, tr, A, B, pa, pb, c);
+                bs.set_pos(bd.x, pa);
+                bs.set_pos(bd.y, pb);
+                minSep = fmin_std(minSep, sep);
+                gi = gn;
+                m = nm;
+                bd = nbd;
+                ss = nss;
+                tr = ntr;
+                A = nA;
+                B = nB;
+            }
+        }
+        if (minSep >= c.regMinSeparation) return true;
+    }
+    return false;
+}
+
+template <bool kLds>
+__device__ std::uint32_t solve_islands(const DeviceState& d, const WorldCtx& x, int nisl, const IslandScratch& S,
+                                       const StepParams& c) {
+    const int bo = x.bo;
+    const Bodies<kLds> bs{d.s_bc_pos + bo, d.s_bc_vel + bo, x.nb};
+    const int nbe = S.istart[nisl].x;
+    const int nce = S.istart[nisl].y;
+    // Body constraints (GetBodyConstraints, AabbTreeWorld.cpp:276-287) for
+    // island bodies; static bodies get their fixed pose and zero velocity.
+    for (int b = x.lane; b < x.nb; b += kWave) {
+        const int gb = bo + b;
+        if (S.bisland[b] >= 0) {
+            const float4 p1 = d.pos1[gb];
+            d.pos0[gb] = p1;
+            bs.set_pos(b, p1);
+            bs.set_vel(b, integrated_velocity(d, gb, c));
+        } else if (!(d.bflags[gb] & prx::kSpeedable)) {
+            bs.set_pos(b, d.pos1[gb]);
+            bs.set_vel(b, float4{0.0f, 0.0f, 0.0f, 0.0f});
+        }
+    }
+    for (int i = x.lane; i < nisl; i += kWave) {
+        S.isl_word[i] = 0u;
+        S.isl_under[i] = 0x7f800000u;
+    }
+    __syncthreads();
+    // Velocity constraints for all island contacts (independent per contact).
+    for (int e = x.lane; e < nce; e += kWave) {
+        const int gi = x.co + S.icont[e];
+        const int2 bd = d.c_body[gi];
+        const int2 ss = d.c_shape[gi];
+        const float rA = d.shape_f[ss.x].x, rB = d.shape_f[ss.y].x;
+        const BodyC A = load_bodyc(d, bo + bd.x), B = load_bodyc(d, bo + bd.y);
+        const float4 pA = bs.pos(bd.x), pB = bs.pos(bd.y);
+        const Xf xfA = get_xf(pA.x, pA.y, pA.z, A.lc);
+        const Xf xfB = get_xf(pB.x, pB.y, pB.z, B.lc);
+        const WM wm = world_manifold(load_manifold(d, gi), xfA, rA, xfB, rB);
+        const float4 mt = d.c_mat[gi];
+        const Vc vc = make_vc(wm, mt.x, mt.y, mt.z, bd.x, bd.y, A, pA, bs.vel(bd.x), B, pB, bs.vel(bd.y), c);
+        store_vc(d.s_vc + std::size_t(gi) * prx::kVcStride, vc);
+    }
+    __syncthreads();
+    // Warm start and velocity sweeps: one lane per island.
+    for (int is = x.lane; is < nisl; is += kWave) {
+        const int2 s0 = S.istart[is], s1 = S.istart[is + 1];
+        island_velocity<kLds>(d, x, bs, s0.y, s1.y, c);
+    }
+    __syncthreads();
     // IntegratePositions (AabbTreeWorld.cpp:176-188).
-    for (int e = b0; e < b1; ++e) {
-        const int b = d.s_isl_body[bo + x.co + e];
-        float4 p = d.s_bc_pos[bo + b];
-        const float4 v = d.s_bc_vel[bo + b];
+    for (int e = x.lane; e < nbe; e += kWave) {
+        const int b = S.ibody[e];
+        float4 p = bs.pos(b);
+        const float4 v = bs.vel(b);
         const float tx = c.dt * v.x, ty = c.dt * v.y, r = c.dt * v.z;
         p.x = p.x + tx;
         p.y = p.y + ty;
         p.z = p.z + r;
-        d.s_bc_pos[bo + b] = p;
+        bs.set_pos(b, p);
     }
-    // Position iterations.
-    bool solved = false;
-    for (std::uint32_t it = 0; it < c.posIters; ++it) {
-        float minSep = __builtin_inff();
-        for (int e = c0; e < c1; ++e) {
-            const int gi = x.co + d.s_isl_contact[x.co + e];
-            const int2 bd = d.c_body[gi];
-            const int2 ss = d.c_shape[gi];
-            const float tr = d.shape_f[ss.x].x + d.shape_f[ss.y].x;
-            const BodyC A = load_bodyc(d, bo + bd.x), B = load_bodyc(d, bo + bd.y);
-            float4 pa = d.s_bc_pos[bo + bd.x];
-            float4 pb = d.s_bc_pos[bo + bd.y];
-            const float s = solve_pc(load_manifold(d, gi), tr, A, B, pa, pb, c);
-            d.s_bc_pos[bo + bd.x] = pa;
-            d.s_bc_pos[bo + bd.y] = pb;
-            minSep = fmin_std(minSep, s);
-        }
-        if (minSep >= c.regMinSeparation) {
-            solved = true;
-            break;
-        }
+    __syncthreads();
+    // Position sweeps: one lane per island.
+    for (int is = x.lane; is < nisl; is += kWave) {
+        const int2 s0 = S.istart[is], s1 = S.istart[is + 1];
+        if (island_position<kLds>(d, x, bs, s0.y, s1.y, c)) S.isl_word[is] = 1u;
     }
+    __syncthreads();
     // AssignImpulses (AabbTreeWorld.cpp:208-227).
-    for (int e = c0; e < c1; ++e) {
-        const int gi = x.co + d.s_isl_contact[x.co + e];
+    for (int e = x.lane; e < nce; e += kWave) {
+        const int gi = x.co + S.icont[e];
         const float4* rec = d.s_vc + std::size_t(gi) * prx::kVcStride;
         const std::uint32_t n = __float_as_uint(rec[1].y);
         const float4 q0 = rec[4];
@@ -1061,48 +1311,44 @@ __device__ std::uint32_t solve_island(const DeviceState& d, const WorldCtx& x, i
             d.m_p1[gi] = m1;
         }
     }
-    // Body writeback; bodies whose position changed flag their contacts.
-    for (int e = b0; e < b1; ++e) {
-        const int b = d.s_isl_body[bo + x.co + e];
+    // Body writeback, under-active times (AabbTreeWorld.cpp:1652-1664, 403-417).
+    for (int e = x.lane; e < nbe; e += kWave) {
+        const int b = S.ibody[e];
         const int gb = bo + b;
-        const float4 bv = d.s_bc_vel[gb];
-        const float4 bp = d.s_bc_pos[gb];
-        float4 v = d.vel[gb];
-        v.x = bv.x;
-        v.y = bv.y;
-        v.z = bv.z;
-        d.vel[gb] = v;
+        const float4 bv = bs.vel(b);
+        const float4 bp = bs.pos(b);
         const float4 p1 = d.pos1[gb];
+        std::uint32_t f = d.bflags[gb];
         if (!(p1.x == bp.x && p1.y == bp.y) || p1.z != bp.z) {
             d.pos1[gb] = float4{bp.x, bp.y, bp.z, 0.0f};
             const float4 l = d.lcinv[gb];
             const Xf xf = get_xf(bp.x, bp.y, bp.z, V2{l.x, l.y});
             d.xf[gb] = float4{xf.p.x, xf.p.y, xf.q.x, xf.q.y};
-            d.bflags[gb] |= 0x100u;  // position changed this step
+            f |= 0x100u;  // position changed this step
+            d.bflags[gb] = f;
         }
-    }
-    // UpdateUnderActiveTimes + Sleepem (AabbTreeWorld.cpp:403-429).
-    float minUnder = __builtin_inff();
-    for (int e = b0; e < b1; ++e) {
-        const int gb = bo + d.s_isl_body[bo + x.co + e];
-        const std::uint32_t f = d.bflags[gb];
-        if (!(f & prx::kSpeedable)) continue;
         float4 v = d.vel[gb];
+        v.x = bv.x;
+        v.y = bv.y;
+        v.z = bv.z;
         const float linSq = magsq(V2{v.x, v.y});
         const float angSq = v.z * v.z;
         const bool under = (angSq <= c.angularSleepTolerance * c.angularSleepTolerance) &&
                            (linSq <= c.linearSleepTolerance * c.linearSleepTolerance);
         const float t = ((f & prx::kAutoSleep) && under) ? v.w + c.dt : 0.0f;
-        if (t == 0.0f || (f & prx::kAccelerable)) {
-            v.w = t;
-            d.vel[gb] = v;
-        }
-        minUnder = fmin_std(minUnder, t);
+        if (t == 0.0f || (f & prx::kAccelerable)) v.w = t;
+        d.vel[gb] = v;
+        // Under-active times are non-negative: their bit patterns order as floats.
+        atomicMin(&S.isl_under[S.bisland[b]], __float_as_uint(t));
     }
+    __syncthreads();
+    // Sleepem (AabbTreeWorld.cpp:419-429) for islands still long enough and solved.
     std::uint32_t slept = 0;
-    if (minUnder >= c.minStillTimeToSleep && solved) {
-        for (int e = b0; e < b1; ++e) {
-            const int gb = bo + d.s_isl_body[bo + x.co + e];
+    for (int e = x.lane; e < nbe; e += kWave) {
+        const int b = S.ibody[e];
+        const int is = S.bisland[b];
+        if (__uint_as_float(S.isl_under[is]) >= c.minStillTimeToSleep && S.isl_word[is]) {
+            const int gb = bo + b;
             const std::uint32_t f = d.bflags[gb];
             if ((f & prx::kAwake) && (f & prx::kAutoSleep)) {
                 d.bflags[gb] = f & ~prx::kAwake;
@@ -1111,6 +1357,8 @@ __device__ std::uint32_t solve_island(const DeviceState& d, const WorldCtx& x, i
             }
         }
     }
+    for (int o = 32; o > 0; o >>= 1) slept += __shfl_xor(slept, o);
+    __syncthreads();
     return slept;
 }
 
@@ -1174,122 +1422,25 @@ __device__ void step_world(const DeviceState& d, int w, const StepParams& c) {
     __syncthreads();
 
     if (c.dt != 0.0f) {
-        // E1: per-body lists of touching, enabled, non-sensor contacts in
-        // contact order (the order of each body's contact list).
-        int* off = d.s_csr_off + x.bo + w;  // nbodies + 1 entries
-        for (int b = x.lane; b <= x.nb; b += kWave) off[b] = 0;
-        for (int b = x.lane; b < x.nb; b += kWave) d.s_isl_b[x.bo + b] = 0;
-        __syncthreads();
-        for (int i = x.lane; i < k.nc; i += kWave) {
-            const std::uint32_t f = d.c_flags[x.co + i];
-            d.s_isl_c[x.co + i] = 0;
-            if ((f & prx::kCEnabled) && (f & prx::kTouching) && !(f & prx::kSensor)) {
-                const int2 bd = d.c_body[x.co + i];
-                atomicAdd(&off[bd.x + 1], 1);
-                atomicAdd(&off[bd.y + 1], 1);
-            }
-        }
-        __syncthreads();
-        if (x.lane == 0) {
-            int acc = 0;
-            for (int b = 0; b <= x.nb; ++b) {
-                acc += off[b];
-                off[b] = acc;
-            }
-        }
-        __syncthreads();
-        // Fill with per-body cursors kept in s_stack, then sort each list.
-        int* cur = d.s_stack + x.bo;
-        for (int b = x.lane; b < x.nb; b += kWave) cur[b] = off[b];
-        __syncthreads();
-        int* csr = d.s_csr + 2 * x.co;
-        for (int i = x.lane; i < k.nc; i += kWave) {
-            const std::uint32_t f = d.c_flags[x.co + i];
-            if ((f & prx::kCEnabled) && (f & prx::kTouching) && !(f & prx::kSensor)) {
-                const int2 bd = d.c_body[x.co + i];
-                csr[atomicAdd(&cur[bd.x], 1)] = i;
-                csr[atomicAdd(&cur[bd.y], 1)] = i;
-            }
-        }
-        __syncthreads();
-        for (int b = x.lane; b < x.nb; b += kWave) {
-            const int s = off[b], e = off[b + 1];
-            for (int i = s + 1; i < e; ++i) {
-                const int v = csr[i];
-                int j = i - 1;
-                while (j >= s && csr[j] > v) {
-                    csr[j + 1] = csr[j];
-                    --j;
-                }
-                csr[j + 1] = v;
-            }
-        }
-        __syncthreads();
-        // E2: islands by depth-first search in body order (AabbTreeWorld.cpp:1358-1523).
-        __shared__ int s_nisl;
-        if (x.lane == 0) {
-            int nisl = 0, nbe = 0, nce = 0;
-            int* stack = d.s_stack + x.bo;
-            std::uint8_t* islb = d.s_isl_b + x.bo;
-            std::uint8_t* islc = d.s_isl_c + x.co;
-            int* ibody = d.s_isl_body + x.bo + x.co;
-            int* icont = d.s_isl_contact + x.co;
-            int2* istart = d.s_isl_start + x.bo + w;
-            for (int seed = 0; seed < x.nb; ++seed) {
-                if (islb[seed]) continue;
-                const std::uint32_t sf = d.bflags[x.bo + seed];
-                if (!((sf & prx::kAwake) && (sf & prx::kEnabled))) continue;
-                istart[nisl] = int2{nbe, nce};
-                ++nisl;
-                const int be0 = nbe;
-                int sp = 0;
-                stack[sp++] = seed;
-                islb[seed] = 1;
-                while (sp > 0) {
-                    const int b = stack[--sp];
-                    ibody[nbe++] = b;
-                    const std::uint32_t f = d.bflags[x.bo + b];
-                    if (!(f & prx::kSpeedable)) continue;
-                    d.bflags[x.bo + b] = f | prx::kAwake;
-                    for (int e = off[b]; e < off[b + 1]; ++e) {
-                        const int ci = csr[e];
-                        if (islc[ci]) continue;
-                        const int2 bd = d.c_body[x.co + ci];
-                        const int other = (bd.x == b) ? bd.y : bd.x;
-                        icont[nce++] = ci;
-                        islc[ci] = 1;
-                        if (!islb[other]) {
-                            islb[other] = 1;
-                            stack[sp++] = other;
-                        }
-                    }
-                }
-                for (int e = be0; e < nbe; ++e) {
-                    const int b = ibody[e];
-                    if (!(d.bflags[x.bo + b] & prx::kSpeedable)) islb[b] = 0;
-                }
-            }
-            istart[nisl] = int2{nbe, nce};
-            s_nisl = nisl;
-        }
-        __syncthreads();
-        const int nisl = s_nisl;
+        IslandScratch S;
+        S.off = d.s_csr_off + x.bo + w;
+        S.csr = reinterpret_cast<int2*>(d.s_csr) + 2 * x.co;
+        S.ibody = d.s_isl_body + x.bo + x.co;
+        S.icont = d.s_isl_contact + x.co;
+        S.istart = d.s_isl_start + x.bo + w;
+        S.bisland = d.s_stack + 2 * x.bo + x.nb;
+        S.isl_word = d.s_keys + x.co;
+        S.isl_under = d.s_keys2 + x.co;
+        build_adjacency(d, x, k.nc, S);
+        const bool ldsFlags = (x.nb + k.nc) <= int(c.ldsBytes);
+        const int nisl = ldsFlags ? find_islands<true>(d, x, k.nc, S) : find_islands<false>(d, x, k.nc, S);
         stats[4] = std::uint32_t(nisl);
-        // E3: islands solved in parallel, one lane per island.
-        std::uint32_t slept = 0;
-        const int2* istart = d.s_isl_start + x.bo + w;
-        for (int is = x.lane; is < nisl; is += kWave) {
-            const int2 s0 = istart[is], s1 = istart[is + 1];
-            slept += solve_island(d, x, s0.x, s1.x, s0.y, s1.y, c);
-        }
-        for (int o = 32; o > 0; o >>= 1) slept += __shfl_xor(slept, o);
-        stats[7] = slept;
-        __syncthreads();
-        // E4: synchronise proxies of islanded speedable bodies.
+        const bool ldsBodies = x.nb * 32 <= int(c.ldsBytes);
+        stats[7] = ldsBodies ? solve_islands<true>(d, x, nisl, S, c) : solve_islands<false>(d, x, nisl, S, c);
+        // Synchronise proxies of islanded speedable bodies (AabbTreeWorld.cpp:1532-1544).
         std::uint32_t moved = 0;
         for (int b = x.lane; b < x.nb; b += kWave) {
-            const int gb = x.bo + b;
-            if (d.s_isl_b[gb] && (d.bflags[gb] & prx::kSpeedable)) moved += synchronize_body(d, x, b, c);
+            if (S.bisland[b] >= 0) moved += synchronize_body(d, x, b, c);
         }
         for (int o = 32; o > 0; o >>= 1) moved += __shfl_xor(moved, o);
         stats[6] = moved;
@@ -1328,7 +1479,7 @@ extern "C" hipError_t prx_launch_step(const prx::DeviceState* d, const prx::Step
                                       hipStream_t stream) {
     prx::DeviceState dd = *d;
     dd.schedule = d->schedule + first;
-    hipLaunchKernelGGL(prxd::k_step_worlds, dim3(unsigned(count)), dim3(64), 0, stream, dd, *c);
+    hipLaunchKernelGGL(prxd::k_step_worlds, dim3(unsigned(count)), dim3(64), c->ldsBytes, stream, dd, *c);
     return hipGetLastError();
 }
 
