@@ -173,6 +173,12 @@ uint64_t prx_batch_dynamic_bodies(const prx_batch* batch);
  * dynamic bodies, out[1] proxies, out[2] sum over proxies of (8 + 16 x vertex
  * count) bytes, out[3] contacts, out[4] touching contacts. */
 int prx_batch_counts(prx_batch* batch, uint64_t out[5]);
+/* Per-world phase timings: enable != 0 turns collection on (subsequent steps
+ * accumulate shader-clock cycles per phase); out receives up to max_worlds x
+ * 16 counters (proxies, destroy, update, find/add, adjacency, islands,
+ * constraints, velocity, position, writeback, sync, update, find/add, total)
+ * and the counters are then cleared.  enable = 0 turns collection off. */
+int prx_batch_phase_cycles(prx_batch* batch, int enable, uint64_t* out, int max_worlds);
 /* Tries launch schedules on `nsteps` steps of a copy of the batch state and
  * keeps the fastest in conf. */
 int prx_batch_autotune(prx_batch* batch, const prx_step_conf* step, int nsteps,

@@ -127,6 +127,7 @@ _SIGS = {
     "prx_batch_dynamic_bodies": (ctypes.c_uint64, [ctypes.c_void_p]),
     "prx_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
     "prx_batch_counts": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "prx_batch_phase_cycles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]),
     "prx_selftest_sincos": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
 }
 
@@ -290,6 +291,17 @@ class Batch:
 
     def dynamic_bodies(self):
         return lib().prx_batch_dynamic_bodies(self.h)
+
+    PHASES = ("proxies", "destroy", "update", "find_add", "adjacency", "islands", "constraints", "velocity",
+              "position", "writeback", "sync", "update2", "find_add2", "total")
+
+    def phase_cycles(self, enable=True):
+        """Per-world phase cycle counters accumulated since the last call
+        (array [worlds, 16]); enable turns collection on or off afterwards."""
+        n = len(self.worlds)
+        out = np.zeros((n, 16), dtype=np.uint64)
+        _check(lib().prx_batch_phase_cycles(self.h, 1 if enable else 0, out.ctypes.data, n))
+        return out
 
     def counts(self):
         """(awake dynamic bodies, proxies, shape bytes, contacts, touching) after download()."""

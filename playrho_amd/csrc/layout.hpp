@@ -126,6 +126,16 @@ struct DeviceState {
     float4* s_bc_pos;        // body constraint position x, y, a (nbodies)
     float4* s_bc_vel;        // body constraint velocity x, y, w (nbodies)
     float4* s_vc;            // velocity constraints, kVcStride float4 per contact
+
+    // Optional per-world phase timings of the last step (shader clock cycles),
+    // kPhases entries per world; null when not collected.
+    unsigned long long* phase_cycles;
+};
+
+// Phases timed by phase_cycles.
+enum Phase : int {
+    kPhProxies = 0, kPhDestroy, kPhUpdate, kPhFindAdd, kPhAdjacency, kPhIslands, kPhConstraints,
+    kPhVelocity, kPhPosition, kPhWriteback, kPhSync, kPhUpdate2, kPhFindAdd2, kPhTotal, kPhases = 16
 };
 
 constexpr int kVcStride = 8;

@@ -32,6 +32,26 @@ using prx::WorldHeader;
 
 constexpr int kWave = 64;
 
+// Phase timer: lane 0 records shader-clock deltas into phase_cycles[w].
+struct PhaseTimer {
+    unsigned long long* out;
+    unsigned long long t0, start;
+    __device__ void begin(unsigned long long* o) {
+        out = o;
+        t0 = start = (o && threadIdx.x == 0) ? clock64() : 0ull;
+    }
+    __device__ void mark(int phase) {
+        if (out && threadIdx.x == 0) {
+            const unsigned long long t = clock64();
+            out[phase] += t - t0;
+            t0 = t;
+        }
+    }
+    __device__ void end() {
+        if (out && threadIdx.x == 0) out[prx::kPhTotal] += clock64() - start;
+    }
+};
+
 __device__ __forceinline__ unsigned long long lanes_below(int lane) {
     return (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 }
@@ -1229,7 +1249,7 @@ __device__ bool island_position(const DeviceState& d, const WorldCtx& x, const B
 
 template <bool kLds>
 __device__ std::uint32_t solve_islands(const DeviceState& d, const WorldCtx& x, int nisl, const IslandScratch& S,
-                                       const StepParams& c) {
+                                       const StepParams& c, PhaseTimer& T) {
     const int bo = x.bo;
     const Bodies<kLds> bs{d.s_bc_pos + bo, d.s_bc_vel + bo, x.nb};
     const int nbe = S.istart[nisl].x;
@@ -1269,12 +1289,14 @@ __device__ std::uint32_t solve_islands(const DeviceState& d, const WorldCtx& x, 
         store_vc(d.s_vc + std::size_t(gi) * prx::kVcStride, vc);
     }
     __syncthreads();
+    T.mark(prx::kPhConstraints);
     // Warm start and velocity sweeps: one lane per island.
     for (int is = x.lane; is < nisl; is += kWave) {
         const int2 s0 = S.istart[is], s1 = S.istart[is + 1];
         island_velocity<kLds>(d, x, bs, s0.y, s1.y, c);
     }
     __syncthreads();
+    T.mark(prx::kPhVelocity);
     // IntegratePositions (AabbTreeWorld.cpp:176-188).
     for (int e = x.lane; e < nbe; e += kWave) {
         const int b = S.ibody[e];
@@ -1293,6 +1315,7 @@ __device__ std::uint32_t solve_islands(const DeviceState& d, const WorldCtx& x, 
         if (island_position<kLds>(d, x, bs, s0.y, s1.y, c)) S.isl_word[is] = 1u;
     }
     __syncthreads();
+    T.mark(prx::kPhPosition);
     // AssignImpulses (AabbTreeWorld.cpp:208-227).
     for (int e = x.lane; e < nce; e += kWave) {
         const int gi = x.co + S.icont[e];
@@ -1359,6 +1382,7 @@ __device__ std::uint32_t solve_islands(const DeviceState& d, const WorldCtx& x, 
     }
     for (int o = 32; o > 0; o >>= 1) slept += __shfl_xor(slept, o);
     __syncthreads();
+    T.mark(prx::kPhWriteback);
     return slept;
 }
 
@@ -1402,6 +1426,8 @@ __device__ void step_world(const DeviceState& d, int w, const StepParams& c) {
     k.pool = H.pool_size;
     k.status = std::uint32_t(H.status);
     std::uint32_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    PhaseTimer T;
+    T.begin(d.phase_cycles ? d.phase_cycles + std::size_t(w) * prx::kPhases : nullptr);
 
     // A: proxies of newly attached shapes (CreateProxies, AabbTreeWorld.cpp:557-579).
     for (int p = x.lane; p < x.np; p += kWave) {
@@ -1413,13 +1439,17 @@ __device__ void step_world(const DeviceState& d, int w, const StepParams& c) {
         }
     }
     __syncthreads();
+    T.mark(prx::kPhProxies);
     // B, C, D.
     stats[1] = phase_destroy(d, x, k);
     __syncthreads();
+    T.mark(prx::kPhDestroy);
     stats[3] = phase_update(d, x, k.nc, c);
     __syncthreads();
+    T.mark(prx::kPhUpdate);
     stats[2] = phase_find_add(d, x, k, c);
     __syncthreads();
+    T.mark(prx::kPhFindAdd);
 
     if (c.dt != 0.0f) {
         IslandScratch S;
@@ -1432,11 +1462,13 @@ __device__ void step_world(const DeviceState& d, int w, const StepParams& c) {
         S.isl_word = d.s_keys + x.co;
         S.isl_under = d.s_keys2 + x.co;
         build_adjacency(d, x, k.nc, S);
+        T.mark(prx::kPhAdjacency);
         const bool ldsFlags = (x.nb + k.nc) <= int(c.ldsBytes);
         const int nisl = ldsFlags ? find_islands<true>(d, x, k.nc, S) : find_islands<false>(d, x, k.nc, S);
         stats[4] = std::uint32_t(nisl);
+        T.mark(prx::kPhIslands);
         const bool ldsBodies = x.nb * 32 <= int(c.ldsBytes);
-        stats[7] = ldsBodies ? solve_islands<true>(d, x, nisl, S, c) : solve_islands<false>(d, x, nisl, S, c);
+        stats[7] = ldsBodies ? solve_islands<true>(d, x, nisl, S, c, T) : solve_islands<false>(d, x, nisl, S, c, T);
         // Synchronise proxies of islanded speedable bodies (AabbTreeWorld.cpp:1532-1544).
         std::uint32_t moved = 0;
         for (int b = x.lane; b < x.nb; b += kWave) {
@@ -1445,6 +1477,7 @@ __device__ void step_world(const DeviceState& d, int w, const StepParams& c) {
         for (int o = 32; o > 0; o >>= 1) moved += __shfl_xor(moved, o);
         stats[6] = moved;
         __syncthreads();
+        T.mark(prx::kPhSync);
         // Contacts of bodies whose position changed need updating.
         for (int i = x.lane; i < k.nc; i += kWave) {
             const int2 bd = d.c_body[x.co + i];
@@ -1455,9 +1488,12 @@ __device__ void step_world(const DeviceState& d, int w, const StepParams& c) {
         __syncthreads();
         phase_update(d, x, k.nc, c);
         __syncthreads();
+        T.mark(prx::kPhUpdate2);
         stats[5] = phase_find_add(d, x, k, c);
         __syncthreads();
+        T.mark(prx::kPhFindAdd2);
     }
+    T.end();
     if (x.lane == 0) {
         H.ncontacts = k.nc;
         H.nfree = k.nfree;
