@@ -1424,6 +1424,15 @@ int prx_batch_counts(prx_batch* b, std::uint64_t out[5]) {
 int prx_batch_phase_cycles(prx_batch* b, int enable, std::uint64_t* out, int max_worlds) {
     if (!b) return Fail(PRX_E_INVALID_ARGUMENT, "null batch");
     const std::size_t n = b->worlds.size() * std::size_t(prx::kPhases);
+    if (b->d.phase_cycles) {
+        PRX_HIP(hipStreamSynchronize(b->stream));
+        if (out) {
+            const std::size_t m = std::min<std::size_t>(n, std::size_t(std::max(0, max_worlds)) * prx::kPhases);
+            PRX_HIP(hipMemcpy(out, b->d.phase_cycles, m * sizeof(std::uint64_t), hipMemcpyDeviceToHost));
+        }
+    } else if (out) {
+        std::memset(out, 0, std::size_t(std::max(0, max_worlds)) * prx::kPhases * sizeof(std::uint64_t));
+    }
     if (enable && !b->d.phase_cycles) {
         PRX_HIP(b->arena.alloc(b->d.phase_cycles, n));
         b->graph_valid = false;
@@ -1432,14 +1441,7 @@ int prx_batch_phase_cycles(prx_batch* b, int enable, std::uint64_t* out, int max
         b->d.phase_cycles = nullptr;
         b->graph_valid = false;
     }
-    if (b->d.phase_cycles) {
-        PRX_HIP(hipStreamSynchronize(b->stream));
-        if (out) {
-            const std::size_t m = std::min<std::size_t>(n, std::size_t(std::max(0, max_worlds)) * prx::kPhases);
-            PRX_HIP(hipMemcpy(out, b->d.phase_cycles, m * sizeof(std::uint64_t), hipMemcpyDeviceToHost));
-        }
-        PRX_HIP(hipMemset(b->d.phase_cycles, 0, n * sizeof(std::uint64_t)));
-    }
+    if (b->d.phase_cycles) PRX_HIP(hipMemset(b->d.phase_cycles, 0, n * sizeof(std::uint64_t)));
     return PRX_OK;
 }
 

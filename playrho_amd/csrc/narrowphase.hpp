@@ -40,10 +40,12 @@ struct Manifold {
     V2 ln, lpt;
     MPoint p[2];
     __device__ __forceinline__ void add(V2 lp, std::uint32_t cf) {
-        p[n].lp = lp;
-        p[n].cf = cf;
-        p[n].ni = 0.0f;
-        p[n].ti = 0.0f;
+        // Static indices only, so the points stay in registers.
+        if (n == 0) {
+            p[0] = MPoint{lp, cf, 0.0f, 0.0f};
+        } else {
+            p[1] = MPoint{lp, cf, 0.0f, 0.0f};
+        }
         ++n;
     }
 };
@@ -103,21 +105,29 @@ struct ClipList {
 __device__ __forceinline__ ClipList clip_segment(const ClipList& in, V2 normal, float offset, std::uint32_t indexA) {
     ClipList out;
     out.n = 0;
+    out.e[0] = in.e[0];
+    out.e[1] = in.e[1];
     if (in.n == 2) {
         const float d0 = dot(normal, in.e[0].v) - offset;
         const float d1 = dot(normal, in.e[1].v) - offset;
-        if (d0 <= 0.0f || almost_zero(d0)) out.e[out.n++] = in.e[0];
-        if (d1 <= 0.0f || almost_zero(d1)) out.e[out.n++] = in.e[1];
+        const bool k0 = d0 <= 0.0f || almost_zero(d0);
+        const bool k1 = d1 <= 0.0f || almost_zero(d1);
+        // Kept vertices in order: e[0] if k0, then e[1] if k1.
+        if (!k0 && k1) out.e[0] = in.e[1];
+        out.n = (k0 ? 1u : 0u) + (k1 ? 1u : 0u);
         if (out.n < 2 && (__builtin_signbit(d0) != 0) != (__builtin_signbit(d1) != 0)) {
             const float interp = d0 / (d0 - d1);
             const V2 vertex = in.e[0].v + mul(in.e[1].v - in.e[0].v, interp);
-            out.e[out.n++] = ClipV{vertex, make_cf(kVertex, indexA, kFace, (in.e[0].cf >> 16) & 0xffu)};
+            const ClipV cv{vertex, make_cf(kVertex, indexA, kFace, (in.e[0].cf >> 16) & 0xffu)};
+            if (out.n == 0) out.e[0] = cv;
+            else out.e[1] = cv;
+            ++out.n;
         }
     }
     return out;
 }
 
-__device__ Manifold poly_manifold(bool flipped, const Proxy& s0, const Xf& xf0, std::uint32_t idx0,
+__device__ __forceinline__ Manifold poly_manifold(bool flipped, const Proxy& s0, const Xf& xf0, std::uint32_t idx0,
                                   const Proxy& s1, const Xf& xf1, std::uint32_t first, std::uint32_t second,
                                   float maxCirclesRatio) {
     const float r0 = s0.radius;
@@ -170,6 +180,7 @@ __device__ Manifold poly_manifold(bool flipped, const Proxy& s0, const Xf& xf0, 
             const float absOffset = dot(absNormal, av0);
             const V2 normal = fwdperp(re0dir);
             Manifold m = for_face(flipped, normal, relMid);
+#pragma unroll
             for (std::uint32_t k = 0; k < 2; ++k) {
                 if ((dot(absNormal, clip.e[k].v) - absOffset) <= totalRadius) {
                     m.add(invtransform(clip.e[k].v, xf1), flipped ? flip_cf(clip.e[k].cf) : clip.e[k].cf);
@@ -202,7 +213,7 @@ __device__ Manifold poly_manifold(bool flipped, const Proxy& s0, const Xf& xf0, 
     return !flipped ? for_circles(v0rel, i0, v1rel, i1) : for_circles(v1rel, i1, v0rel, i0);
 }
 
-__device__ Manifold poly_circle_manifold(bool flipped, float totalRadius, const Proxy& s, const Xf& sxf,
+__device__ __forceinline__ Manifold poly_circle_manifold(bool flipped, float totalRadius, const Proxy& s, const Xf& sxf,
                                          V2 point, const Xf& xf) {
     const std::uint32_t n = s.n;
     const V2 cLocal = invtransform(transform(point, xf), sxf);
@@ -250,34 +261,32 @@ struct SepInfo {
     std::uint32_t first, s0, s1;
 };
 
-__device__ __forceinline__ void min_sep_info(V2 origin, V2 dir, const V2* verts, std::uint32_t n, float& minSep,
-                                             std::uint32_t& first, std::uint32_t& second) {
-    minSep = __builtin_inff();
-    first = kInvalidVertex;
-    second = kInvalidVertex;
-    for (std::uint32_t i = 0; i < n; ++i) {
-        const float s = dot(dir, verts[i] - origin);
-        if (minSep > s) {
-            minSep = s;
-            first = i;
-            second = kInvalidVertex;
-        } else if (minSep == s) {
-            second = i;
-        }
+// Minimum separation of vertices from a face (ShapeSeparation.cpp:39-64).
+__device__ __forceinline__ void min_sep_step(float s, std::uint32_t i, float& minSep, std::uint32_t& first,
+                                             std::uint32_t& second) {
+    if (minSep > s) {
+        minSep = s;
+        first = i;
+        second = kInvalidVertex;
+    } else if (minSep == s) {
+        second = i;
     }
 }
 
-constexpr int kMaxVerts = 16;
-
-__device__ SepInfo max_sep_4x4(const Proxy& p1, const Xf& xf1, const Proxy& p2, const Xf& xf2) {
+__device__ __forceinline__ SepInfo max_sep_4x4(const Proxy& p1, const Xf& xf1, const Proxy& p2, const Xf& xf2) {
     SepInfo out{-__builtin_inff(), kInvalidVertex, kInvalidVertex, kInvalidVertex};
     const Xf xf = mulT(xf1, xf2);
     V2 verts[4];
+#pragma unroll
     for (int k = 0; k < 4; ++k) verts[k] = transform(p2.vert(std::uint32_t(k)), xf);
+#pragma unroll
     for (std::uint32_t i = 0; i < 4; ++i) {
-        float d;
-        std::uint32_t f, s;
-        min_sep_info(p1.vert(i), p1.norm(i), verts, 4u, d, f, s);
+        const V2 origin = p1.vert(i);
+        const V2 normal = p1.norm(i);
+        float d = __builtin_inff();
+        std::uint32_t f = kInvalidVertex, s = kInvalidVertex;
+#pragma unroll
+        for (std::uint32_t k = 0; k < 4; ++k) min_sep_step(dot(normal, verts[k] - origin), k, d, f, s);
         if (out.d < d) {
             out.d = d;
             out.s0 = f;
@@ -288,17 +297,15 @@ __device__ SepInfo max_sep_4x4(const Proxy& p1, const Xf& xf1, const Proxy& p2, 
     return out;
 }
 
-__device__ SepInfo max_sep(const Proxy& p1, const Xf& xf1, const Proxy& p2, const Xf& xf2) {
+__device__ __forceinline__ SepInfo max_sep(const Proxy& p1, const Xf& xf1, const Proxy& p2, const Xf& xf2) {
     SepInfo out{-__builtin_inff(), kInvalidVertex, kInvalidVertex, kInvalidVertex};
     const Xf xf = mulT(xf2, xf1);
-    V2 verts[kMaxVerts];
-    for (std::uint32_t k = 0; k < p2.n; ++k) verts[k] = p2.vert(k);
     for (std::uint32_t i = 0; i < p1.n; ++i) {
         const V2 origin = transform(p1.vert(i), xf);
         const V2 normal = rotate(p1.norm(i), xf.q);
-        float d;
-        std::uint32_t f, s;
-        min_sep_info(origin, normal, verts, p2.n, d, f, s);
+        float d = __builtin_inff();
+        std::uint32_t f = kInvalidVertex, s = kInvalidVertex;
+        for (std::uint32_t k = 0; k < p2.n; ++k) min_sep_step(dot(normal, p2.vert(k) - origin), k, d, f, s);
         if (out.d < d) {
             out.d = d;
             out.s0 = f;
@@ -309,7 +316,7 @@ __device__ SepInfo max_sep(const Proxy& p1, const Xf& xf1, const Proxy& p2, cons
     return out;
 }
 
-__device__ Manifold collide_shapes(const Proxy& A, const Xf& xfA, const Proxy& B, const Xf& xfB, float linearSlop,
+__device__ __forceinline__ Manifold collide_shapes(const Proxy& A, const Xf& xfA, const Proxy& B, const Xf& xfB, float linearSlop,
                                    float maxCirclesRatio) {
     const float totalRadius = A.radius + B.radius;
     if (A.n == 1 && B.n == 1) {

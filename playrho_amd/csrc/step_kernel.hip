@@ -36,18 +36,18 @@ constexpr int kWave = 64;
 struct PhaseTimer {
     unsigned long long* out;
     unsigned long long t0, start;
-    __device__ void begin(unsigned long long* o) {
+    __device__ __forceinline__ void begin(unsigned long long* o) {
         out = o;
         t0 = start = (o && threadIdx.x == 0) ? clock64() : 0ull;
     }
-    __device__ void mark(int phase) {
+    __device__ __forceinline__ void mark(int phase) {
         if (out && threadIdx.x == 0) {
             const unsigned long long t = clock64();
             out[phase] += t - t0;
             t0 = t;
         }
     }
-    __device__ void end() {
+    __device__ __forceinline__ void end() {
         if (out && threadIdx.x == 0) out[prx::kPhTotal] += clock64() - start;
     }
 };
@@ -75,7 +75,7 @@ __device__ __forceinline__ Xf body_xf0(const DeviceState& d, int gb) {
 // ComputeAABB(proxy, xf0, xf1) (AABB.cpp:51-62) with optional fattening by
 // aabbExtension and displacement by displaceMultiplier * (p1 - p0)
 // (detail/AABB.hpp:305-319, Interval.hpp:144-220).
-__device__ float4 proxy_aabb(const Proxy& px, const Xf& xf0, const Xf& xf1, const StepParams& c, bool fatten) {
+__device__ __forceinline__ float4 proxy_aabb(const Proxy& px, const Xf& xf0, const Xf& xf1, const StepParams& c, bool fatten) {
     float mnx = __builtin_inff(), mny = __builtin_inff();
     float mxx = -__builtin_inff(), mxy = -__builtin_inff();
     for (std::uint32_t i = 0; i < px.n; ++i) {
@@ -152,7 +152,7 @@ __device__ __forceinline__ void store_manifold(const DeviceState& d, int gi, con
 
 // Update (AabbTreeWorld.cpp:2587-2721): new manifold, warm-start impulses
 // carried over by ContactFeature, else from the nearest old point.
-__device__ void update_contact(const DeviceState& d, int bo, int gi, const StepParams& c) {
+__device__ __forceinline__ void update_contact(const DeviceState& d, int bo, int gi, const StepParams& c) {
     std::uint32_t flags = d.c_flags[gi];
     const Manifold old = load_manifold(d, gi);
     const bool oldTouching = (flags & prx::kTouching) != 0;
@@ -167,26 +167,33 @@ __device__ void update_contact(const DeviceState& d, int bo, int gi, const StepP
         const std::uint32_t oldN = old.n, newN = nm.n;
         newTouching = newN > 0;
         bool found[2] = {false, newN < 2};
-        for (std::uint32_t i = 0; i < newN; ++i) {
-            for (std::uint32_t j = 0; j < oldN; ++j) {
+#pragma unroll
+        for (std::uint32_t i = 0; i < 2; ++i) {
+            if (i >= newN) continue;
+            bool done = false;
+#pragma unroll
+            for (std::uint32_t j = 0; j < 2; ++j) {
+                if (done || j >= oldN) continue;
                 if (nm.p[i].cf == old.p[j].cf) {
                     found[i] = true;
                     nm.p[i].ni = old.p[j].ni;
                     nm.p[i].ti = old.p[j].ti;
-                    break;
+                    done = true;
                 }
             }
         }
-        for (std::uint32_t i = 0; i < newN; ++i) {
-            if (!found[i]) {
-                float least = __builtin_inff();
-                for (std::uint32_t j = 0; j < oldN; ++j) {
-                    const float dd = magsq(old.p[j].lp - nm.p[i].lp);
-                    if (least > dd) {
-                        least = dd;
-                        nm.p[i].ni = old.p[j].ni;
-                        nm.p[i].ti = old.p[j].ti;
-                    }
+#pragma unroll
+        for (std::uint32_t i = 0; i < 2; ++i) {
+            if (i >= newN || found[i]) continue;
+            float least = __builtin_inff();
+#pragma unroll
+            for (std::uint32_t j = 0; j < 2; ++j) {
+                if (j >= oldN) continue;
+                const float dd = magsq(old.p[j].lp - nm.p[i].lp);
+                if (least > dd) {
+                    least = dd;
+                    nm.p[i].ni = old.p[j].ni;
+                    nm.p[i].ti = old.p[j].ti;
                 }
             }
         }
@@ -209,7 +216,7 @@ struct Counters {
 };
 
 // UpdateContacts (AabbTreeWorld.cpp:2333-2421).  Returns contacts updated.
-__device__ std::uint32_t phase_update(const DeviceState& d, const WorldCtx& x, int nc, const StepParams& c) {
+__device__ __forceinline__ std::uint32_t phase_update(const DeviceState& d, const WorldCtx& x, int nc, const StepParams& c) {
     std::uint32_t updated = 0;
     for (int base = 0; base < nc; base += kWave) {
         const int i = base + x.lane;
@@ -222,7 +229,7 @@ __device__ std::uint32_t phase_update(const DeviceState& d, const WorldCtx& x, i
 }
 
 // DestroyContacts (AabbTreeWorld.cpp:2290-2331) + InternalDestroy (:2263-2288).
-__device__ std::uint32_t phase_destroy(const DeviceState& d, const WorldCtx& x, Counters& k) {
+__device__ __forceinline__ std::uint32_t phase_destroy(const DeviceState& d, const WorldCtx& x, Counters& k) {
     int out = 0;
     std::uint32_t destroyedTotal = 0;
     const int nc = k.nc;
@@ -306,7 +313,7 @@ __device__ __forceinline__ bool should_collide(const DeviceState& d, int sa, int
 // FindContacts (AabbTreeWorld.cpp:709-751) + AddContacts (:2424-2547).
 // Candidate keys pack world-local proxy indices (min << 16 | max); proxy
 // indices increase with tree leaf ids, so key order equals ContactKey order.
-__device__ std::uint32_t phase_find_add(const DeviceState& d, const WorldCtx& x, Counters& k, const StepParams& c) {
+__device__ __forceinline__ std::uint32_t phase_find_add(const DeviceState& d, const WorldCtx& x, Counters& k, const StepParams& c) {
     // Moved proxies, in index order.
     int nmoved = 0;
     for (int base = 0; base < x.np; base += kWave) {
@@ -549,7 +556,7 @@ struct WM {
     float ni[2], ti[2];
 };
 
-__device__ WM world_manifold(const Manifold& m, const Xf& xfA, float rA, const Xf& xfB, float rB) {
+__device__ __forceinline__ WM world_manifold(const Manifold& m, const Xf& xfA, float rA, const Xf& xfB, float rB) {
     WM w;
     w.n = 0;
     w.normal = V2{0, 0};
@@ -569,7 +576,9 @@ __device__ WM world_manifold(const Manifold& m, const Xf& xfA, float rA, const X
         const V2 plane = transform(m.lpt, xfA);
         w.normal = normal;
         w.n = m.n;
-        for (std::uint32_t i = 0; i < m.n; ++i) {
+#pragma unroll
+        for (std::uint32_t i = 0; i < 2; ++i) {
+            if (i >= m.n) continue;
             const V2 clip = transform(m.p[i].lp, xfB);
             const V2 cA = clip + mul(rA - dot(clip - plane, normal), normal);
             const V2 cB = clip - mul(rB, normal);
@@ -582,7 +591,9 @@ __device__ WM world_manifold(const Manifold& m, const Xf& xfA, float rA, const X
         const V2 plane = transform(m.lpt, xfB);
         w.normal = -normal;
         w.n = m.n;
-        for (std::uint32_t i = 0; i < m.n; ++i) {
+#pragma unroll
+        for (std::uint32_t i = 0; i < 2; ++i) {
+            if (i >= m.n) continue;
             const V2 clip = transform(m.p[i].lp, xfA);
             const V2 cB = clip + mul(rB - dot(clip - plane, normal), normal);
             const V2 cA = clip - mul(rA, normal);
@@ -595,7 +606,7 @@ __device__ WM world_manifold(const Manifold& m, const Xf& xfA, float rA, const X
 }
 
 // VelocityConstraint ctor + GetPoint + ComputeK (VelocityConstraint.cpp:38-185).
-__device__ Vc make_vc(const WM& wm, float friction, float restitution, float tangentSpeed, int bA, int bB,
+__device__ __forceinline__ Vc make_vc(const WM& wm, float friction, float restitution, float tangentSpeed, int bA, int bB,
                       const BodyC& A, float4 posA, float4 velA, const BodyC& B, float4 posB, float4 velB,
                       const StepParams& c) {
     Vc vc;
@@ -613,8 +624,10 @@ __device__ Vc make_vc(const WM& wm, float friction, float restitution, float tan
     const float dtRatio = c.warmStart ? c.dtRatio : 0.0f;
     const float invMass = A.invMass + B.invMass;
     const V2 tangent = fwdperp(vc.normal);
-    for (std::uint32_t j = 0; j < wm.n; ++j) {
-        VcPoint& p = vc.p[vc.n];
+#pragma unroll
+    for (std::uint32_t j = 0; j < 2; ++j) {
+        if (j >= wm.n) continue;
+        VcPoint p;
         const V2 relA = wm.p[j] - V2{posA.x, posA.y};
         const V2 relB = wm.p[j] - V2{posB.x, posB.y};
         p.ni = wm.ni[j] * dtRatio;
@@ -642,6 +655,8 @@ __device__ Vc make_vc(const WM& wm, float friction, float restitution, float tan
             const float value = invMass + rmA + rmB;
             p.tm = (value != 0.0f) ? 1.0f / value : 0.0f;
         }
+        // wm points fill vc points in order, so slot j receives point j.
+        vc.p[j] = p;
         ++vc.n;
     }
     if (c.blockSolve && vc.n == 2) {
@@ -688,7 +703,10 @@ __device__ __forceinline__ float solve_dir(Vc& vc, bool tangent, const BodyC& A,
     const V2 dir = tangent ? fwdperp(vc.normal) : vc.normal;
     V2 vA{velA.x, velA.y}, vB{velB.x, velB.y};
     float wA = velA.z, wB = velB.z;
-    for (int k = int(vc.n) - 1; k >= 0; --k) {
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+        const int k = 1 - kk;  // point 1 first, then point 0
+        if (k >= int(vc.n)) continue;
         VcPoint& p = vc.p[k];
         const V2 closing = rel_vel(vA, wA, p.relA, vB, wB, p.relB);
         float oldI, newI;
@@ -802,10 +820,11 @@ struct PSM {
     V2 normal, point;
     float sep;
 };
-__device__ PSM get_psm(const Manifold& m, std::uint32_t idx, const Xf& xfA, const Xf& xfB) {
+__device__ __forceinline__ PSM get_psm(const Manifold& m, std::uint32_t idx, const Xf& xfA, const Xf& xfB) {
+    const V2 plp = idx ? m.p[1].lp : m.p[0].lp;
     if (m.type == kCircles) {
         const V2 pA = transform(m.lpt, xfA);
-        const V2 pB = transform(m.p[idx].lp, xfB);
+        const V2 pB = transform(plp, xfB);
         const V2 delta = pB - pA;
         const V2 normal = unit_vector(delta, V2{0, 0});
         return PSM{normal, divs(pA + pB, 2.0f), dot(delta, normal)};
@@ -813,17 +832,17 @@ __device__ PSM get_psm(const Manifold& m, std::uint32_t idx, const Xf& xfA, cons
     if (m.type == kFaceA) {
         const V2 plane = transform(m.lpt, xfA);
         const V2 normal = rotate(m.ln, xfA.q);
-        const V2 clip = transform(m.p[idx].lp, xfB);
+        const V2 clip = transform(plp, xfB);
         return PSM{normal, clip, dot(clip - plane, normal)};
     }
     const V2 plane = transform(m.lpt, xfB);
     const V2 normal = rotate(m.ln, xfB.q);
-    const V2 clip = transform(m.p[idx].lp, xfA);
+    const V2 clip = transform(plp, xfA);
     return PSM{-normal, clip, dot(clip - plane, normal)};
 }
 
 // GaussSeidel::SolvePositionConstraint (ContactSolver.cpp:494-643).
-__device__ float solve_pc(const Manifold& m, float totalRadius, const BodyC& A, const BodyC& B, float4& posA,
+__device__ __forceinline__ float solve_pc(const Manifold& m, float totalRadius, const BodyC& A, const BodyC& B, float4& posA,
                           float4& posB, const StepParams& c) {
     const float invMassTotal = A.invMass + B.invMass;
     struct Sol {
@@ -913,7 +932,7 @@ __device__ __forceinline__ BodyC load_bodyc(const DeviceState& d, int gb) {
 }
 
 // GetVelocity(body, h) + Cap (Body.cpp:290-309, Velocity.cpp:27-45).
-__device__ float4 integrated_velocity(const DeviceState& d, int gb, const StepParams& c) {
+__device__ __forceinline__ float4 integrated_velocity(const DeviceState& d, int gb, const StepParams& c) {
     const float4 v = d.vel[gb];
     V2 lin{v.x, v.y};
     float ang = v.z;
@@ -991,7 +1010,7 @@ __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
 
 // Exclusive prefix sum of a[0..n) in place (wave-parallel, carry across chunks);
 // returns the total.
-__device__ int wave_exclusive_scan(int* a, int n, int lane) {
+__device__ __forceinline__ int wave_exclusive_scan(int* a, int n, int lane) {
     int carry = 0;
     for (int base = 0; base < n; base += kWave) {
         const int i = base + lane;
@@ -1016,7 +1035,7 @@ struct IslandScratch {
 
 // Per-body CSR of touching, enabled, non-sensor contacts in contact order
 // (each body's contact list order), entries carry the other body.
-__device__ void build_adjacency(const DeviceState& d, const WorldCtx& x, int nc, const IslandScratch& S) {
+__device__ __forceinline__ void build_adjacency(const DeviceState& d, const WorldCtx& x, int nc, const IslandScratch& S) {
     for (int b = x.lane; b <= x.nb; b += kWave) S.off[b] = 0;
     __syncthreads();
     for (int i = x.lane; i < nc; i += kWave) {
@@ -1063,7 +1082,7 @@ __device__ void build_adjacency(const DeviceState& d, const WorldCtx& x, int nc,
 // depth-first search on one lane with the islanded flags in LDS when they
 // fit.  Returns the island count.
 template <bool kLdsFlags>
-__device__ int find_islands(const DeviceState& d, const WorldCtx& x, int nc, const IslandScratch& S) {
+__device__ __forceinline__ int find_islands(const DeviceState& d, const WorldCtx& x, int nc, const IslandScratch& S) {
     __shared__ int s_cnt[2];
     std::uint8_t* islb = kLdsFlags ? reinterpret_cast<std::uint8_t*>(g_smem) : d.s_isl_b + x.bo;
     std::uint8_t* islc = kLdsFlags ? reinterpret_cast<std::uint8_t*>(g_smem) + x.nb : d.s_isl_c + x.co;
@@ -1138,7 +1157,7 @@ __device__ int find_islands(const DeviceState& d, const WorldCtx& x, int nc, con
 // ContactSolver.cpp:481-492 per constraint in island order; the next
 // constraint's record is loaded while the current one is solved.
 template <bool kLds>
-__device__ void island_velocity(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs, int c0, int c1,
+__device__ __forceinline__ void island_velocity(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs, int c0, int c1,
                                 const StepParams& c) {
     if (c0 >= c1) return;
     const int bo = x.bo;
@@ -1150,7 +1169,9 @@ __device__ void island_velocity(const DeviceState& d, const WorldCtx& x, const B
             const V2 tangent = fwdperp(vc.normal);
             V2 dAl{0, 0}, dBl{0, 0};
             float dAa = 0.0f, dBa = 0.0f;
-            for (std::uint32_t j = 0; j < vc.n; ++j) {
+#pragma unroll
+            for (std::uint32_t j = 0; j < 2; ++j) {
+                if (j >= vc.n) continue;
                 const VcPoint& p = vc.p[j];
                 const V2 P = mul(p.ni, vc.normal) + mul(p.ti, tangent);
                 const float LA = cross(p.relA, P) / 1.0f;
@@ -1208,7 +1229,7 @@ __device__ void island_velocity(const DeviceState& d, const WorldCtx& x, const B
 // Lane-serial position sweeps of one island (ContactSolver.cpp:494-643,
 // AabbTreeWorld.cpp:1627-1644).  Returns whether the island was solved.
 template <bool kLds>
-__device__ bool island_position(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs, int c0, int c1,
+__device__ __forceinline__ bool island_position(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs, int c0, int c1,
                                 const StepParams& c) {
     const int bo = x.bo;
     for (std::uint32_t it = 0; it < c.posIters; ++it) {
@@ -1248,7 +1269,7 @@ __device__ bool island_position(const DeviceState& d, const WorldCtx& x, const B
 }
 
 template <bool kLds>
-__device__ std::uint32_t solve_islands(const DeviceState& d, const WorldCtx& x, int nisl, const IslandScratch& S,
+__device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, const WorldCtx& x, int nisl, const IslandScratch& S,
                                        const StepParams& c, PhaseTimer& T) {
     const int bo = x.bo;
     const Bodies<kLds> bs{d.s_bc_pos + bo, d.s_bc_vel + bo, x.nb};
@@ -1387,7 +1408,7 @@ __device__ std::uint32_t solve_islands(const DeviceState& d, const WorldCtx& x, 
 }
 
 // Synchronize (AabbTreeWorld.cpp:2564-2585) for one body.
-__device__ std::uint32_t synchronize_body(const DeviceState& d, const WorldCtx& x, int b, const StepParams& c) {
+__device__ __forceinline__ std::uint32_t synchronize_body(const DeviceState& d, const WorldCtx& x, int b, const StepParams& c) {
     const int gb = x.bo + b;
     const Xf xf0 = body_xf0(d, gb);
     const Xf xf1 = body_xf1(d, gb);
@@ -1409,7 +1430,7 @@ __device__ std::uint32_t synchronize_body(const DeviceState& d, const WorldCtx& 
     return moved;
 }
 
-__device__ void step_world(const DeviceState& d, int w, const StepParams& c) {
+__device__ __forceinline__ void step_world(const DeviceState& d, int w, const StepParams& c) {
     WorldHeader& H = d.hdr[w];
     WorldCtx x;
     x.w = w;
