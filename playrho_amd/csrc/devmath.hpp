@@ -82,7 +82,9 @@ __device__ __forceinline__ Xf mulT(const Xf& A, const Xf& B) {
 }
 
 __device__ __forceinline__ V2 unit_from_angle(float a) {
-    return V2{prx_math::CosF<true>(a), prx_math::SinF<true>(a)};
+    float sn, cs;
+    prx_math::SinCosF<true>(a, &sn, &cs);
+    return V2{cs, sn};
 }
 
 // GetTransformation(pos, localCenter)

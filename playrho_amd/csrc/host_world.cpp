@@ -646,6 +646,7 @@ int BuildBatch(prx_batch* b) {
     PRX_HIP(a.alloc(d.s_bc_pos, B));
     PRX_HIP(a.alloc(d.s_bc_vel, B));
     PRX_HIP(a.alloc(d.s_vc, C * std::size_t(prx::kVcStride)));
+    PRX_HIP(a.alloc(d.s_pc, C * std::size_t(prx::kPcStride)));
     // Launch order: largest worlds first (stable), so the long worlds start
     // early and the small ones fill in behind them.
     std::vector<int> order(static_cast<std::size_t>(nw));

@@ -125,7 +125,8 @@ struct DeviceState {
     int2* s_isl_start;       // island: first body entry, first contact entry (nbodies + 1)
     float4* s_bc_pos;        // body constraint position x, y, a (nbodies)
     float4* s_bc_vel;        // body constraint velocity x, y, w (nbodies)
-    float4* s_vc;            // velocity constraints, kVcStride float4 per contact
+    float4* s_vc;            // velocity constraints in island order, kVcStride float4 each
+    float4* s_pc;            // position constraints in island order, kPcStride float4 each
 
     // Optional per-world phase timings of the last step (shader clock cycles),
     // kPhases entries per world; null when not collected.
@@ -138,6 +139,7 @@ enum Phase : int {
     kPhVelocity, kPhPosition, kPhWriteback, kPhSync, kPhUpdate2, kPhFindAdd2, kPhTotal, kPhases = 16
 };
 
-constexpr int kVcStride = 8;
+constexpr int kVcStride = 9;
+constexpr int kPcStride = 5;
 
 }  // namespace prx

@@ -201,4 +201,40 @@ PRX_HD float CosF(float y) {
     return Poly<Fma>(x * s, x * x, *p, n ^ 1);
 }
 
+// Both values with one argument reduction; each equals SinF / CosF above
+// (same reduction, same table, polynomials of opposite parity).
+template <bool Fma>
+PRX_HD void SinCosF(float y, float* sinOut, float* cosOut) {
+    double x = y;
+    const SinCosTab* p = &Tab(0);
+    int n;
+    const std::uint32_t t = AbsTop12(y);
+    if (t < AbsTop12(0x1.921FB6p-1f)) {
+        const double x2 = x * x;
+        if (t < AbsTop12(0x1p-12f)) {
+            *sinOut = y;
+            *cosOut = 1.0f;
+            return;
+        }
+        *sinOut = Poly<Fma>(x, x2, *p, 0);
+        *cosOut = Poly<Fma>(x, x2, *p, 1);
+        return;
+    }
+    double s;
+    if (t < AbsTop12(120.0f)) {
+        x = ReduceFast<Fma>(x, *p, &n);
+        s = p->sign[n & 3];
+        if (n & 2) p = &Tab(1);
+    } else {
+        const std::uint32_t xi = AsU32(y);
+        const int sign = int(xi >> 31);
+        x = ReduceLarge(xi, &n);
+        s = p->sign[(n + sign) & 3];
+        if ((n + sign) & 2) p = &Tab(1);
+    }
+    const double xs = x * s, x2 = x * x;
+    *sinOut = Poly<Fma>(xs, x2, *p, n);
+    *cosOut = Poly<Fma>(xs, x2, *p, n ^ 1);
+}
+
 }  // namespace prx_math

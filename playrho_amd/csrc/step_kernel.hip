@@ -481,15 +481,23 @@ __device__ __forceinline__ std::uint32_t phase_find_add(const DeviceState& d, co
 }
 
 // ---------------------------------------------------------------------------
-// Island solve.  Velocity-constraint record (kVcStride float4 per contact):
-//   v[0] normal.x, normal.y, friction, restitution
-//   v[1] tangentSpeed, count, K00, K11
-//   v[2] K01, M00, M11, M01              (normal mass = K^-1)
+// Island solve.  Velocity-constraint record (kVcStride float4, stored in
+// island order so a sweep reads consecutive records):
+//   v[0] normal.x, normal.y, friction, tangentSpeed
+//   v[1] K00, K11, K01, count
+//   v[2] M00, M11, M01, bodyA            (normal mass = K^-1)
 //   v[3] p0 relA.x, relA.y, relB.x, relB.y
 //   v[4] p0 ni, ti, normalMass, tangentMass
-//   v[5] p0 velocityBias, p1 velocityBias, bodyA, bodyB (bit patterns)
+//   v[5] p0 velocityBias, p1 velocityBias, bodyB, restitution
 //   v[6] p1 relA.x, relA.y, relB.x, relB.y
 //   v[7] p1 ni, ti, normalMass, tangentMass
+//   v[8] invMassA, invRotIA, invMassB, invRotIB
+// Position-constraint record (kPcStride float4, island order):
+//   u[0] type | count << 8, bodyA, bodyB, totalRadius
+//   u[1] local normal x, y, local point x, y
+//   u[2] point 0 local x, y, point 1 local x, y
+//   u[3] invMassA, invRotIA, localCenterA x, y
+//   u[4] invMassB, invRotIB, localCenterB x, y
 
 struct BodyC {
     float invMass, invRotI;
@@ -517,34 +525,38 @@ struct Vc {
     VcPoint p[2];
 };
 
-__device__ __forceinline__ void store_vc(float4* v, const Vc& vc) {
-    v[0] = float4{vc.normal.x, vc.normal.y, vc.friction, vc.restitution};
-    v[1] = float4{vc.tangentSpeed, __uint_as_float(vc.n), vc.K00, vc.K11};
-    v[2] = float4{vc.K01, vc.M00, vc.M11, vc.M01};
+__device__ __forceinline__ void store_vc(float4* v, const Vc& vc, const BodyC& A, const BodyC& B) {
+    v[0] = float4{vc.normal.x, vc.normal.y, vc.friction, vc.tangentSpeed};
+    v[1] = float4{vc.K00, vc.K11, vc.K01, __uint_as_float(vc.n)};
+    v[2] = float4{vc.M00, vc.M11, vc.M01, __int_as_float(vc.bA)};
     v[3] = float4{vc.p[0].relA.x, vc.p[0].relA.y, vc.p[0].relB.x, vc.p[0].relB.y};
     v[4] = float4{vc.p[0].ni, vc.p[0].ti, vc.p[0].nm, vc.p[0].tm};
-    v[5] = float4{vc.p[0].bias, vc.p[1].bias, __int_as_float(vc.bA), __int_as_float(vc.bB)};
+    v[5] = float4{vc.p[0].bias, vc.p[1].bias, __int_as_float(vc.bB), vc.restitution};
     v[6] = float4{vc.p[1].relA.x, vc.p[1].relA.y, vc.p[1].relB.x, vc.p[1].relB.y};
     v[7] = float4{vc.p[1].ni, vc.p[1].ti, vc.p[1].nm, vc.p[1].tm};
+    v[8] = float4{A.invMass, A.invRotI, B.invMass, B.invRotI};
 }
-__device__ __forceinline__ Vc load_vc(const float4* v) {
+__device__ __forceinline__ Vc load_vc(const float4* v, BodyC& A, BodyC& B) {
     Vc vc;
-    const float4 a = v[0], b = v[1], e = v[2], p0a = v[3], p0b = v[4], f = v[5], p1a = v[6], p1b = v[7];
+    const float4 a = v[0], b = v[1], e = v[2], p0a = v[3], p0b = v[4], f = v[5], p1a = v[6], p1b = v[7],
+                 m = v[8];
     vc.normal = V2{a.x, a.y};
     vc.friction = a.z;
-    vc.restitution = a.w;
-    vc.tangentSpeed = b.x;
-    vc.n = __float_as_uint(b.y);
-    vc.K00 = b.z;
-    vc.K11 = b.w;
-    vc.K01 = e.x;
-    vc.M00 = e.y;
-    vc.M11 = e.z;
-    vc.M01 = e.w;
+    vc.tangentSpeed = a.w;
+    vc.K00 = b.x;
+    vc.K11 = b.y;
+    vc.K01 = b.z;
+    vc.n = __float_as_uint(b.w);
+    vc.M00 = e.x;
+    vc.M11 = e.y;
+    vc.M01 = e.z;
+    vc.bA = __float_as_int(e.w);
     vc.p[0] = VcPoint{V2{p0a.x, p0a.y}, V2{p0a.z, p0a.w}, p0b.x, p0b.y, p0b.z, p0b.w, f.x};
     vc.p[1] = VcPoint{V2{p1a.x, p1a.y}, V2{p1a.z, p1a.w}, p1b.x, p1b.y, p1b.z, p1b.w, f.y};
-    vc.bA = __float_as_int(f.z);
-    vc.bB = __float_as_int(f.w);
+    vc.bB = __float_as_int(f.z);
+    vc.restitution = f.w;
+    A = BodyC{m.x, m.y, V2{0, 0}};
+    B = BodyC{m.z, m.w, V2{0, 0}};
     return vc;
 }
 
@@ -1079,15 +1091,35 @@ __device__ __forceinline__ void build_adjacency(const DeviceState& d, const Worl
 // Islands (AddToIsland, AabbTreeWorld.cpp:1358-1444; seeds in body order
 // :1495-1523).  Awake speedable bodies without touching contacts are
 // one-body islands and are emitted lane-parallel; the rest are found by a
-// depth-first search on one lane with the islanded flags in LDS when they
-// fit.  Returns the island count.
-template <bool kLdsFlags>
+// depth-first search on one lane.  kMode 2: adjacency and flags in LDS,
+// 1: flags in LDS, 0: everything in global scratch.  Flag bytes per body:
+// 0 free, 1 islanded, 2 static (never pushed).  Returns the island count.
+template <int kMode>
 __device__ __forceinline__ int find_islands(const DeviceState& d, const WorldCtx& x, int nc, const IslandScratch& S) {
     __shared__ int s_cnt[2];
-    std::uint8_t* islb = kLdsFlags ? reinterpret_cast<std::uint8_t*>(g_smem) : d.s_isl_b + x.bo;
-    std::uint8_t* islc = kLdsFlags ? reinterpret_cast<std::uint8_t*>(g_smem) + x.nb : d.s_isl_c + x.co;
+    const int nent = S.off[x.nb];
+    int* off = S.off;
+    int2* csr = S.csr;
+    std::uint8_t* islb;
+    std::uint8_t* islc;
+    if constexpr (kMode == 2) {
+        int* loff = reinterpret_cast<int*>(g_smem);
+        int2* lcsr = reinterpret_cast<int2*>(g_smem) + ((x.nb + 2) >> 1);
+        for (int b = x.lane; b <= x.nb; b += kWave) loff[b] = S.off[b];
+        for (int e = x.lane; e < nent; e += kWave) lcsr[e] = S.csr[e];
+        off = loff;
+        csr = lcsr;
+        islb = reinterpret_cast<std::uint8_t*>(lcsr + nent);
+        islc = islb + x.nb;
+    } else if constexpr (kMode == 1) {
+        islb = reinterpret_cast<std::uint8_t*>(g_smem);
+        islc = islb + x.nb;
+    } else {
+        islb = d.s_isl_b + x.bo;
+        islc = d.s_isl_c + x.co;
+    }
     for (int b = x.lane; b < x.nb; b += kWave) {
-        islb[b] = 0;
+        islb[b] = (d.bflags[x.bo + b] & prx::kSpeedable) ? 0 : 2;
         S.bisland[b] = -1;
     }
     for (int i = x.lane; i < nc; i += kWave) islc[i] = 0;
@@ -1099,7 +1131,7 @@ __device__ __forceinline__ int find_islands(const DeviceState& d, const WorldCtx
         bool single = false;
         if (b < x.nb) {
             const std::uint32_t f = d.bflags[x.bo + b];
-            single = (f & prx::kAwake) && (f & prx::kEnabled) && (f & prx::kSpeedable) && S.off[b] == S.off[b + 1];
+            single = (f & prx::kAwake) && (f & prx::kEnabled) && (f & prx::kSpeedable) && off[b] == off[b + 1];
         }
         const unsigned long long m = __ballot(single);
         if (single) {
@@ -1117,27 +1149,26 @@ __device__ __forceinline__ int find_islands(const DeviceState& d, const WorldCtx
         int* stack = d.s_stack + 2 * x.bo;
         for (int seed = 0; seed < x.nb; ++seed) {
             if (islb[seed]) continue;
-            if (S.off[seed] == S.off[seed + 1]) continue;
+            if (off[seed] == off[seed + 1]) continue;
             const std::uint32_t sf = d.bflags[x.bo + seed];
-            if (!((sf & prx::kAwake) && (sf & prx::kEnabled) && (sf & prx::kSpeedable))) continue;
+            if (!((sf & prx::kAwake) && (sf & prx::kEnabled))) continue;
             S.istart[nisl] = int2{nbe, nce};
             int sp = 0;
             stack[sp++] = seed;
             islb[seed] = 1;
             while (sp > 0) {
-                const int b = stack[--sp];
-                S.ibody[nbe++] = b;
-                S.bisland[b] = nisl;
-                d.bflags[x.bo + b] |= prx::kAwake;
-                const int e1 = S.off[b + 1];
-                for (int e = S.off[b]; e < e1; ++e) {
-                    const int2 en = S.csr[e];
+                const int bb = stack[--sp];
+                S.ibody[nbe++] = bb;
+                S.bisland[bb] = nisl;
+                const int e1 = off[bb + 1];
+                for (int e = off[bb]; e < e1; ++e) {
+                    const int2 en = csr[e];
                     if (islc[en.x]) continue;
                     S.icont[nce++] = en.x;
                     islc[en.x] = 1;
-                    if (!islb[en.y]) {
+                    if (islb[en.y] == 0) {
                         islb[en.y] = 1;
-                        if (d.bflags[x.bo + en.y] & prx::kSpeedable) stack[sp++] = en.y;
+                        stack[sp++] = en.y;
                     }
                 }
             }
@@ -1147,25 +1178,58 @@ __device__ __forceinline__ int find_islands(const DeviceState& d, const WorldCtx
         s_cnt[0] = nisl;
     }
     __syncthreads();
-    const int nisl = s_cnt[0];
-    // One-body islands have no contacts: their contact range is empty at the
-    // start of the contact list.
-    return nisl;
+    return s_cnt[0];
+}
+
+
+// Position-constraint record (see the layout above).
+struct Pc {
+    Manifold m;
+    int bA, bB;
+    float totalRadius;
+    BodyC A, B;
+};
+
+__device__ __forceinline__ void store_pc(float4* u, const Manifold& m, int bA, int bB, float tr, const BodyC& A,
+                                         const BodyC& B) {
+    u[0] = float4{__uint_as_float(m.type | (m.n << 8)), __int_as_float(bA), __int_as_float(bB), tr};
+    u[1] = float4{m.ln.x, m.ln.y, m.lpt.x, m.lpt.y};
+    u[2] = float4{m.p[0].lp.x, m.p[0].lp.y, m.p[1].lp.x, m.p[1].lp.y};
+    u[3] = float4{A.invMass, A.invRotI, A.lc.x, A.lc.y};
+    u[4] = float4{B.invMass, B.invRotI, B.lc.x, B.lc.y};
+}
+__device__ __forceinline__ Pc load_pc(const float4* u) {
+    const float4 a = u[0], g = u[1], q = u[2], ma = u[3], mb = u[4];
+    Pc pc;
+    const std::uint32_t h = __float_as_uint(a.x);
+    pc.m.type = h & 0xffu;
+    pc.m.n = (h >> 8) & 0xffu;
+    pc.m.ln = V2{g.x, g.y};
+    pc.m.lpt = V2{g.z, g.w};
+    pc.m.p[0] = MPoint{V2{q.x, q.y}, 0u, 0.0f, 0.0f};
+    pc.m.p[1] = MPoint{V2{q.z, q.w}, 0u, 0.0f, 0.0f};
+    pc.bA = __float_as_int(a.y);
+    pc.bB = __float_as_int(a.z);
+    pc.totalRadius = a.w;
+    pc.A = BodyC{ma.x, ma.y, V2{ma.z, ma.w}};
+    pc.B = BodyC{mb.x, mb.y, V2{mb.z, mb.w}};
+    return pc;
 }
 
 // Lane-serial Gauss-Seidel velocity sweeps of one island (warm start first),
-// ContactSolver.cpp:481-492 per constraint in island order; the next
-// constraint's record is loaded while the current one is solved.
+// ContactSolver.cpp:481-492 per constraint in island order.  Records are
+// consecutive; the next one is loaded while the current one is solved, so the
+// only dependence left between constraints is through the bodies' velocities
+// (in LDS).
 template <bool kLds>
-__device__ __forceinline__ void island_velocity(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs, int c0, int c1,
-                                const StepParams& c) {
+__device__ __forceinline__ void island_velocity(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
+                                                int c0, int c1, const StepParams& c) {
     if (c0 >= c1) return;
-    const int bo = x.bo;
+    const float4* base = d.s_vc + std::size_t(x.co) * prx::kVcStride;
     if (c.warmStart) {
         for (int e = c0; e < c1; ++e) {
-            const int gi = x.co + d.s_isl_contact[x.co + e];
-            const Vc vc = load_vc(d.s_vc + std::size_t(gi) * prx::kVcStride);
-            const BodyC A = load_bodyc(d, bo + vc.bA), B = load_bodyc(d, bo + vc.bB);
+            BodyC A, B;
+            const Vc vc = load_vc(base + std::size_t(e) * prx::kVcStride, A, B);
             const V2 tangent = fwdperp(vc.normal);
             V2 dAl{0, 0}, dBl{0, 0};
             float dAa = 0.0f, dBa = 0.0f;
@@ -1195,14 +1259,12 @@ __device__ __forceinline__ void island_velocity(const DeviceState& d, const Worl
     }
     for (std::uint32_t it = 0; it < c.velIters; ++it) {
         float newInc = 0.0f;
-        int gi = x.co + d.s_isl_contact[x.co + c0];
-        Vc vc = load_vc(d.s_vc + std::size_t(gi) * prx::kVcStride);
-        BodyC A = load_bodyc(d, bo + vc.bA), B = load_bodyc(d, bo + vc.bB);
+        BodyC A, B;
+        Vc vc = load_vc(base + std::size_t(c0) * prx::kVcStride, A, B);
         for (int e = c0; e < c1; ++e) {
-            // Prefetch the next record (independent of this solve).
-            const int gn = (e + 1 < c1) ? x.co + d.s_isl_contact[x.co + e + 1] : gi;
-            const Vc nvc = load_vc(d.s_vc + std::size_t(gn) * prx::kVcStride);
-            const BodyC nA = load_bodyc(d, bo + nvc.bA), nB = load_bodyc(d, bo + nvc.bB);
+            const int en = (e + 1 < c1) ? e + 1 : e;
+            BodyC nA, nB;
+            const Vc nvc = load_vc(base + std::size_t(en) * prx::kVcStride, nA, nB);
             float4 va = bs.vel(vc.bA);
             float4 vb = bs.vel(vc.bB);
             float mi = 0.0f;
@@ -1213,11 +1275,10 @@ __device__ __forceinline__ void island_velocity(const DeviceState& d, const Worl
             mi = fmax_std(mi, nrm);
             bs.set_vel(vc.bA, va);
             bs.set_vel(vc.bB, vb);
-            float4* rec = d.s_vc + std::size_t(gi) * prx::kVcStride;
+            float4* rec = d.s_vc + (std::size_t(x.co) + std::size_t(e)) * prx::kVcStride;
             rec[4] = float4{vc.p[0].ni, vc.p[0].ti, vc.p[0].nm, vc.p[0].tm};
             rec[7] = float4{vc.p[1].ni, vc.p[1].ti, vc.p[1].nm, vc.p[1].tm};
             newInc = fmax_std(newInc, mi);
-            gi = gn;
             vc = nvc;
             A = nA;
             B = nB;
@@ -1229,38 +1290,23 @@ __device__ __forceinline__ void island_velocity(const DeviceState& d, const Worl
 // Lane-serial position sweeps of one island (ContactSolver.cpp:494-643,
 // AabbTreeWorld.cpp:1627-1644).  Returns whether the island was solved.
 template <bool kLds>
-__device__ __forceinline__ bool island_position(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs, int c0, int c1,
-                                const StepParams& c) {
-    const int bo = x.bo;
+__device__ __forceinline__ bool island_position(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
+                                                int c0, int c1, const StepParams& c) {
+    const float4* base = d.s_pc + std::size_t(x.co) * prx::kPcStride;
     for (std::uint32_t it = 0; it < c.posIters; ++it) {
         float minSep = __builtin_inff();
         if (c0 < c1) {
-            int gi = x.co + d.s_isl_contact[x.co + c0];
-            Manifold m = load_manifold(d, gi);
-            int2 bd = d.c_body[gi];
-            int2 ss = d.c_shape[gi];
-            float tr = d.shape_f[ss.x].x + d.shape_f[ss.y].x;
-            BodyC A = load_bodyc(d, bo + bd.x), B = load_bodyc(d, bo + bd.y);
+            Pc pc = load_pc(base + std::size_t(c0) * prx::kPcStride);
             for (int e = c0; e < c1; ++e) {
-                const int gn = (e + 1 < c1) ? x.co + d.s_isl_contact[x.co + e + 1] : gi;
-                const Manifold nm = load_manifold(d, gn);
-                const int2 nbd = d.c_body[gn];
-                const int2 nss = d.c_shape[gn];
-                const float ntr = d.shape_f[nss.x].x + d.shape_f[nss.y].x;
-                const BodyC nA = load_bodyc(d, bo + nbd.x), nB = load_bodyc(d, bo + nbd.y);
-                float4 pa = bs.pos(bd.x);
-                float4 pb = bs.pos(bd.y);
-                const float sep = solve_pc(m, tr, A, B, pa, pb, c);
-                bs.set_pos(bd.x, pa);
-                bs.set_pos(bd.y, pb);
+                const int en = (e + 1 < c1) ? e + 1 : e;
+                const Pc npc = load_pc(base + std::size_t(en) * prx::kPcStride);
+                float4 pa = bs.pos(pc.bA);
+                float4 pb = bs.pos(pc.bB);
+                const float sep = solve_pc(pc.m, pc.totalRadius, pc.A, pc.B, pa, pb, c);
+                bs.set_pos(pc.bA, pa);
+                bs.set_pos(pc.bB, pb);
                 minSep = fmin_std(minSep, sep);
-                gi = gn;
-                m = nm;
-                bd = nbd;
-                ss = nss;
-                tr = ntr;
-                A = nA;
-                B = nB;
+                pc = npc;
             }
         }
         if (minSep >= c.regMinSeparation) return true;
@@ -1269,19 +1315,21 @@ __device__ __forceinline__ bool island_position(const DeviceState& d, const Worl
 }
 
 template <bool kLds>
-__device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, const WorldCtx& x, int nisl, const IslandScratch& S,
-                                       const StepParams& c, PhaseTimer& T) {
+__device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, const WorldCtx& x, int nisl,
+                                                       const IslandScratch& S, const StepParams& c, PhaseTimer& T) {
     const int bo = x.bo;
     const Bodies<kLds> bs{d.s_bc_pos + bo, d.s_bc_vel + bo, x.nb};
     const int nbe = S.istart[nisl].x;
     const int nce = S.istart[nisl].y;
     // Body constraints (GetBodyConstraints, AabbTreeWorld.cpp:276-287) for
-    // island bodies; static bodies get their fixed pose and zero velocity.
+    // island bodies (which the island search also marks awake, :1410);
+    // static bodies get their fixed pose and zero velocity.
     for (int b = x.lane; b < x.nb; b += kWave) {
         const int gb = bo + b;
         if (S.bisland[b] >= 0) {
             const float4 p1 = d.pos1[gb];
             d.pos0[gb] = p1;
+            d.bflags[gb] |= prx::kAwake;
             bs.set_pos(b, p1);
             bs.set_vel(b, integrated_velocity(d, gb, c));
         } else if (!(d.bflags[gb] & prx::kSpeedable)) {
@@ -1294,7 +1342,8 @@ __device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, con
         S.isl_under[i] = 0x7f800000u;
     }
     __syncthreads();
-    // Velocity constraints for all island contacts (independent per contact).
+    // Velocity and position constraints of all island contacts, in island
+    // order (independent per contact).
     for (int e = x.lane; e < nce; e += kWave) {
         const int gi = x.co + S.icont[e];
         const int2 bd = d.c_body[gi];
@@ -1304,10 +1353,13 @@ __device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, con
         const float4 pA = bs.pos(bd.x), pB = bs.pos(bd.y);
         const Xf xfA = get_xf(pA.x, pA.y, pA.z, A.lc);
         const Xf xfB = get_xf(pB.x, pB.y, pB.z, B.lc);
-        const WM wm = world_manifold(load_manifold(d, gi), xfA, rA, xfB, rB);
+        const Manifold m = load_manifold(d, gi);
+        const WM wm = world_manifold(m, xfA, rA, xfB, rB);
         const float4 mt = d.c_mat[gi];
         const Vc vc = make_vc(wm, mt.x, mt.y, mt.z, bd.x, bd.y, A, pA, bs.vel(bd.x), B, pB, bs.vel(bd.y), c);
-        store_vc(d.s_vc + std::size_t(gi) * prx::kVcStride, vc);
+        const std::size_t slot = std::size_t(x.co) + std::size_t(e);
+        store_vc(d.s_vc + slot * prx::kVcStride, vc, A, B);
+        store_pc(d.s_pc + slot * prx::kPcStride, m, bd.x, bd.y, rA + rB, A, B);
     }
     __syncthreads();
     T.mark(prx::kPhConstraints);
@@ -1340,8 +1392,8 @@ __device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, con
     // AssignImpulses (AabbTreeWorld.cpp:208-227).
     for (int e = x.lane; e < nce; e += kWave) {
         const int gi = x.co + S.icont[e];
-        const float4* rec = d.s_vc + std::size_t(gi) * prx::kVcStride;
-        const std::uint32_t n = __float_as_uint(rec[1].y);
+        const float4* rec = d.s_vc + (std::size_t(x.co) + std::size_t(e)) * prx::kVcStride;
+        const std::uint32_t n = __float_as_uint(rec[1].w);
         const float4 q0 = rec[4];
         float4 m0 = d.m_p0[gi];
         m0.z = q0.x;
@@ -1484,8 +1536,12 @@ __device__ __forceinline__ void step_world(const DeviceState& d, int w, const St
         S.isl_under = d.s_keys2 + x.co;
         build_adjacency(d, x, k.nc, S);
         T.mark(prx::kPhAdjacency);
-        const bool ldsFlags = (x.nb + k.nc) <= int(c.ldsBytes);
-        const int nisl = ldsFlags ? find_islands<true>(d, x, k.nc, S) : find_islands<false>(d, x, k.nc, S);
+        const int nent = S.off[x.nb];
+        const int ldsAdj = 4 * (((x.nb + 2) >> 1) * 2) + 8 * nent + x.nb + k.nc;
+        int nisl;
+        if (ldsAdj <= int(c.ldsBytes)) nisl = find_islands<2>(d, x, k.nc, S);
+        else if (x.nb + k.nc <= int(c.ldsBytes)) nisl = find_islands<1>(d, x, k.nc, S);
+        else nisl = find_islands<0>(d, x, k.nc, S);
         stats[4] = std::uint32_t(nisl);
         T.mark(prx::kPhIslands);
         const bool ldsBodies = x.nb * 32 <= int(c.ldsBytes);
