@@ -39,7 +39,37 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--plain-worlds", type=int, default=64)
     p.add_argument("--plain-steps", type=int, default=10)
+    p.add_argument("--coop-min", type=int, default=0, help="0: library default")
+    p.add_argument("--lds-bytes", type=int, default=0, help="0: library default")
+    p.add_argument("--autotune", type=int, default=10, metavar="K",
+                   help="steps per candidate schedule for prx_batch_autotune after warmup (0: off)")
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                   help="only run a plain device copy of MIB MiB five times (counter calibration)")
     return p.parse_args()
+
+
+def copy_calibration(mib):
+    """A plain device-to-device copy of known size: the HBM counters read for
+    it scale the counters read for the step kernel."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.arange(n, dtype=torch.int32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4, "copies": 5, "ok": bool(torch.equal(a, b))}))
+
+
+def measured_traffic():
+    """HBM bytes per step launch from the committed counter summary (two
+    rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE, scaled by a plain copy's
+    counters), or None."""
+    path = os.path.join(ROOT, "profiles", "k_step_worlds_pile4096_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
 
 
 def algorithmic_bytes(awake, proxies, shape_bytes, contacts, joints=0):
@@ -83,6 +113,9 @@ def conf_record(pa):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; worlds would shard across GPUs without "
               "communication, but no figure at several GPUs is produced here (use --gpus 1)",
@@ -96,11 +129,16 @@ def main():
     worlds = [pa.World.scene("pile", [args.seed, w]) for w in range(args.worlds)]
     build_s = time.time() - t0
     total_bodies = sum(w.body_count for w in worlds)
-    batch = pa.Batch(worlds, use_graph=True)
+    batch = pa.Batch(worlds, use_graph=True, coop_min=args.coop_min or None, lds_bytes=args.lds_bytes or None)
     dyn = batch.dynamic_bodies()
 
     batch.step(conf, args.warmup)
     batch.sync()
+    schedule = None
+    if args.autotune > 0:
+        t_at = time.time()
+        schedule = batch.autotune(conf, args.autotune)
+        schedule["autotune_s"] = round(time.time() - t_at, 3)
     batch.download()
     c0 = batch.counts()
 
@@ -139,6 +177,7 @@ def main():
             "worlds": args.worlds,
             "bodies": total_bodies,
             "dynamic_bodies": int(dyn),
+            "schedule": schedule,
         },
         "contacts_per_s": contacts_avg * args.steps / wall,
         "state": {"awake_dynamic_bodies": [c0[0], c1[0]], "proxies": c1[1], "contacts": [c0[3], c1[3]],
@@ -155,6 +194,11 @@ def main():
             "kernel_ms_per_launch": ms_launch,
         },
     }
+
+    tr = measured_traffic()
+    if tr is not None:
+        result["roofline"]["traffic"] = tr["bytes_per_launch"]
+        result["roofline"]["traffic_note"] = tr["note"]
 
     # Plain per-world form: one world per launch, synchronised, on a sample.
     if args.plain_worlds <= 0:

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PRX_ABI_VERSION 1
+#define PRX_ABI_VERSION 2
 
 /* Status codes (reference exception types in parentheses). */
 enum {
@@ -84,6 +84,19 @@ typedef struct prx_body_conf {
     uint32_t nshapes;
 } prx_body_conf;
 
+/* RevoluteJointConf (d2/RevoluteJointConf.hpp:27-123) with its JointConf base
+ * (bodyA, bodyB, collideConnected; d2/JointConf.hpp). */
+typedef struct prx_revolute_joint_conf {
+    uint32_t body_a, body_b;
+    float local_anchor_a[2], local_anchor_b[2];
+    float reference_angle;
+    uint32_t enable_limit;
+    float lower_angle, upper_angle;
+    uint32_t enable_motor;
+    float motor_speed, max_motor_torque;
+    uint32_t collide_connected;
+} prx_revolute_joint_conf;
+
 /* StepConf fields the regular step reads (StepConf.hpp:49-392). */
 typedef struct prx_step_conf {
     float delta_time, dt_ratio, min_still_time_to_sleep, linear_slop, angular_slop;
@@ -106,6 +119,13 @@ typedef struct prx_batch_conf {
     uint32_t threads_per_world; /* 64 (one wavefront) by default */
     uint32_t use_graph;         /* replay steps from a captured hipGraph */
     uint32_t order;             /* 0: largest worlds first, 1: as given */
+    uint32_t coop_min_contacts; /* islands with at least this many constraints are
+                                   swept by the whole wavefront (default 32) */
+    uint32_t lds_bytes;         /* dynamic LDS per workgroup; 0: the minimum the
+                                   batch needs.  Larger values leave fewer
+                                   workgroups per CU (160 KB of LDS per CU), so
+                                   the long worlds share their SIMD with fewer
+                                   waves */
 } prx_batch_conf;
 
 int prx_abi_version(void);
@@ -126,7 +146,14 @@ int prx_shape_create(prx_world* world, const prx_shape_conf* conf, uint32_t* sha
 /* CreateBody(World&, BodyConf, resetMassData = true) (WorldBody.cpp:55;
  * AabbTreeWorld.cpp:1019-1044; mass from WorldBody.cpp:360-392). */
 int prx_body_create(prx_world* world, const prx_body_conf* conf, uint32_t* body_id);
-/* Builds a named workload scene through the two calls above. */
+/* CreateJoint(World&, const RevoluteJointConf&) (World.hpp:448-458;
+ * AabbTreeWorld.cpp:1132-1175).  Joints must be created before the world's
+ * first step that finds contacts between the two bodies (joints added to a
+ * world with contacts would need contact re-filtering, which is not
+ * modelled): PRX_E_UNSUPPORTED otherwise. */
+int prx_joint_create(prx_world* world, const prx_revolute_joint_conf* conf, uint32_t* joint_id);
+uint32_t prx_world_joint_count(const prx_world* world);
+/* Builds a named workload scene through the calls above. */
 int prx_scene_build(prx_world* world, const char* name, const int32_t* params, int nparams);
 
 /* Replaces the world with a description in the engine's record layout
@@ -179,8 +206,11 @@ int prx_batch_counts(prx_batch* batch, uint64_t out[5]);
  * constraints, velocity, position, writeback, sync, update, find/add, total)
  * and the counters are then cleared.  enable = 0 turns collection off. */
 int prx_batch_phase_cycles(prx_batch* batch, int enable, uint64_t* out, int max_worlds);
-/* Tries launch schedules on `nsteps` steps of a copy of the batch state and
- * keeps the fastest in conf. */
+/* Times candidate launch schedules (LDS per workgroup, i.e. workgroups per CU;
+ * the wave-wide sweep threshold; launch order) on `nsteps` steps each,
+ * starting every trial from the batch's current state, keeps the fastest in
+ * the batch (and in *best when non-null) and restores the state: the
+ * simulation is not advanced. */
 int prx_batch_autotune(prx_batch* batch, const prx_step_conf* step, int nsteps,
                        prx_batch_conf* best);
 

@@ -48,7 +48,7 @@ struct Handle {
     std::uint32_t steps = 0;
 };
 
-// Reads the description (shapes, bodies) and the StepConf record.
+// Reads the description (shapes, bodies, joints) and the StepConf record.
 template <class R>
 void Load(Handle<R>& h, In& in) {
     using namespace oracle;
@@ -96,6 +96,26 @@ void Load(Handle<R>& h, In& in) {
         const std::uint32_t nsh = in.u32();
         for (std::uint32_t k = 0; k < nsh; ++k) b.shapes.push_back(in.u32());
         h.world.AddBody(b);
+    }
+    const std::uint32_t nj = in.u32();
+    for (std::uint32_t i = 0; i < nj; ++i) {
+        Joint<R> j;
+        j.bodyA = in.u32();
+        j.bodyB = in.u32();
+        const std::uint32_t fl = in.u32();
+        j.enableLimit = (fl & 1u) != 0;
+        j.enableMotor = (fl & 2u) != 0;
+        j.collideConnected = (fl & 4u) != 0;
+        R f[9];
+        for (auto& x : f) x = R(in.f32());
+        j.laA = V2<R>{f[0], f[1]};
+        j.laB = V2<R>{f[2], f[3]};
+        j.referenceAngle = f[4];
+        j.lower = f[5];
+        j.upper = f[6];
+        j.motorSpeed = f[7];
+        j.maxMotorTorque = f[8];
+        h.world.AddJoint(j);
     }
     const std::uint32_t nf = in.u32();
     R cf[32] = {};
@@ -180,6 +200,12 @@ void WriteState(const Handle<R>& h, Out& o) {
         o.u32(id);
         const float a[4] = {float(lf.lo[0]), float(lf.lo[1]), float(lf.hi[0]), float(lf.hi[1])};
         o.put(a, sizeof(a));
+    }
+    o.u32(std::uint32_t(w.joints.size()));
+    for (const auto& j : w.joints) {
+        o.u32(j.limitState);
+        const float f[4] = {float(j.impulse.x), float(j.impulse.y), float(j.impulse.z), float(j.motorImpulse)};
+        o.put(f, sizeof(f));
     }
 }
 

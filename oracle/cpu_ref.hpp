@@ -681,9 +681,64 @@ struct PositionConstraint {
     R totalRadius;
 };
 
+// Revolute joint (RevoluteJointConf.hpp:27-123): its configuration, the
+// state carried from step to step (impulse, angular motor impulse, limit
+// state) and the per-step solver data (rA, rB, mass, angularMass).
+enum : std::uint32_t { kLimitInactive = 0, kLimitLower = 1, kLimitUpper = 2, kLimitEqual = 3 };
+
+template <class R>
+struct V3 {
+    R x = 0, y = 0, z = 0;
+};
+template <class R> inline R Dot3(V3<R> a, V3<R> b) {  // Math.hpp:320-333
+    R r = 0;
+    r += a.x * b.x;
+    r += a.y * b.y;
+    r += a.z * b.z;
+    return r;
+}
+template <class R> inline V3<R> Cross3(V3<R> a, V3<R> b) {  // Math.hpp:397-409
+    return V3<R>{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+// Mat33 as its three vectors GetX, GetY, GetZ (symmetric here).
+template <class R>
+struct M33 {
+    V3<R> ex, ey, ez;
+};
+template <class R> inline V3<R> Solve33(const M33<R>& m, V3<R> b) {  // Math.hpp:446-456
+    const R dp = Dot3(m.ex, Cross3(m.ey, m.ez));
+    const R det = (dp != R(0)) ? (R(1) / dp) : dp;
+    return V3<R>{det * Dot3(b, Cross3(m.ey, m.ez)), det * Dot3(m.ex, Cross3(b, m.ez)),
+                 det * Dot3(m.ex, Cross3(m.ey, b))};
+}
+template <class R> inline V2<R> Solve22(const M33<R>& m, V2<R> b) {  // Math.hpp:461-474
+    const R xx = m.ex.x, xy = m.ey.x, yx = m.ex.y, yy = m.ey.y;
+    const R cp = xx * yy - xy * yx;
+    const R det = (cp != R(0)) ? (R(1) / cp) : cp;
+    return V2<R>{det * (yy * b.x - xy * b.y), det * (xx * b.y - yx * b.x)};
+}
+
+template <class R>
+struct Joint {
+    std::uint32_t bodyA = 0, bodyB = 0;
+    bool enableLimit = false, enableMotor = false, collideConnected = false;
+    V2<R> laA, laB;
+    R referenceAngle = 0, lower = 0, upper = 0, motorSpeed = 0, maxMotorTorque = 0;
+    V3<R> impulse;
+    R motorImpulse = 0;
+    std::uint32_t limitState = kLimitInactive;
+    V2<R> rA, rB;
+    M33<R> mass;
+    R angularMass = 0;
+};
+
 template <class R>
 class World {
 public:
+    std::vector<Joint<R>> joints;
+    // m_bodyJoints: per body (other body, joint) in creation order.
+    std::vector<std::vector<std::pair<std::uint32_t, std::uint32_t>>> bodyJoints;
+    std::vector<char> islandedJoints;
     std::vector<Shape<R>> shapes;
     std::vector<Body<R>> bodies;
     std::vector<std::uint32_t> bodyOrder;  // m_bodies
@@ -934,7 +989,15 @@ public:
             const Shape<R>& sA = shapes[la.shape];
             const Shape<R>& sB = shapes[lb.shape];
             if (!(bA.accelerable() || bB.accelerable())) continue;
-            // ShouldCollide(shapes) (Filter.hpp:81-88); joints are not modelled.
+            // ShouldCollide(joints) (AabbTreeWorld.cpp:479-490).
+            {
+                bool jointed = false;
+                for (const auto& bj : bodyJoints[la.body]) {
+                    if (bj.first == lb.body && !joints[bj.second].collideConnected) jointed = true;
+                }
+                if (jointed) continue;
+            }
+            // ShouldCollide(shapes) (Filter.hpp:81-88).
             {
                 const std::int32_t gA = sA.group, gB = sB.group;
                 bool collide;
@@ -1310,8 +1373,207 @@ public:
     }
 
     struct Island {
-        std::vector<std::uint32_t> bodies, contacts;
+        std::vector<std::uint32_t> bodies, contacts, joints;
     };
+
+    // GetMat33 (RevoluteJointConf.cpp:36-62).
+    static M33<R> JointMass(R mA, V2<R> rA, R iA, R mB, V2<R> rB, R iB) {
+        const R totInvI = iA + iB;
+        const R exx = mA + ((rA.y * rA.y) * iA / R(1)) + mB + ((rB.y * rB.y) * iB / R(1));
+        const R eyx = (-rA.y * rA.x * iA / R(1)) + (-rB.y * rB.x * iB / R(1));
+        const R ezx = (-rA.y * iA * R(1) / R(1)) + (-rB.y * iB * R(1) / R(1));
+        const R eyy = mA + ((rA.x * rA.x) * iA / R(1)) + mB + ((rB.x * rB.x) * iB / R(1));
+        const R ezy = (rA.x * iA * R(1) / R(1)) + (rB.x * iB * R(1) / R(1));
+        return M33<R>{V3<R>{exx, eyx, ezx}, V3<R>{eyx, eyy, ezy}, V3<R>{ezx, ezy, totInvI}};
+    }
+
+    // InitVelocity(RevoluteJointConf&) (RevoluteJointConf.cpp:127-226).
+    static void JointInitVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        const R mA = A.invMass, iA = A.invRotI, aA = A.pos.ang;
+        const R mB = B.invMass, iB = B.invRotI, aB = B.pos.ang;
+        Vel<R> velA = A.vel, velB = B.vel;
+        const V2<R> qA = UnitFromAngle(aA);
+        const V2<R> qB = UnitFromAngle(aB);
+        j.rA = Rotate(j.laA - A.lc, qA);
+        j.rB = Rotate(j.laB - B.lc, qB);
+        const R totInvI = iA + iB;
+        const bool fixedRotation = (totInvI == R(0));
+        j.mass = JointMass(mA, j.rA, iA, mB, j.rB, iB);
+        j.angularMass = (totInvI > R(0)) ? R(R(1) / totInvI) : R(0);
+        if (!j.enableMotor || fixedRotation) j.motorImpulse = 0;
+        if (j.enableLimit && !fixedRotation) {
+            const R jointAngle = aB - aA - j.referenceAngle;
+            if (std::abs(j.upper - j.lower) < (conf.angularSlop * R(2))) {
+                j.limitState = kLimitEqual;
+            } else if (jointAngle <= j.lower) {
+                if (j.limitState != kLimitLower) {
+                    j.limitState = kLimitLower;
+                    j.impulse.z = 0;
+                }
+            } else if (jointAngle >= j.upper) {
+                if (j.limitState != kLimitUpper) {
+                    j.limitState = kLimitUpper;
+                    j.impulse.z = 0;
+                }
+            } else {
+                j.limitState = kLimitInactive;
+                j.impulse.z = 0;
+            }
+        } else {
+            j.limitState = kLimitInactive;
+        }
+        if (conf.warmStart) {
+            j.impulse.x *= conf.dtRatio;
+            j.impulse.y *= conf.dtRatio;
+            j.impulse.z *= conf.dtRatio;
+            j.motorImpulse *= conf.dtRatio;
+            const V2<R> P{j.impulse.x, j.impulse.y};
+            const R L = j.motorImpulse + (j.impulse.z * R(1) * R(1) / (R(1) * R(1)));
+            const R LA = Cross(j.rA, P) / R(1) + L;
+            const R LB = Cross(j.rB, P) / R(1) + L;
+            velA.lin = velA.lin - mA * P;
+            velA.ang = velA.ang - iA * LA;
+            velB.lin = velB.lin + mB * P;
+            velB.ang = velB.ang + iB * LB;
+        } else {
+            j.impulse = V3<R>{};
+            j.motorImpulse = 0;
+        }
+        A.vel = velA;
+        B.vel = velB;
+    }
+
+    // SolveVelocity(RevoluteJointConf&) (RevoluteJointConf.cpp:228-331).
+    static bool JointSolveVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        const Vel<R> oldVelA = A.vel, oldVelB = B.vel;
+        Vel<R> velA = oldVelA, velB = oldVelB;
+        const R mA = A.invMass, iA = A.invRotI, mB = B.invMass, iB = B.invRotI;
+        const bool fixedRotation = (iA + iB == R(0));
+        if (j.enableMotor && j.limitState != kLimitEqual && !fixedRotation) {
+            const R impulse = -j.angularMass * (velB.ang - velA.ang - j.motorSpeed);
+            const R oldImpulse = j.motorImpulse;
+            const R maxImpulse = conf.dt * j.maxMotorTorque;
+            j.motorImpulse = Clamp(j.motorImpulse + impulse, -maxImpulse, maxImpulse);
+            const R inc = j.motorImpulse - oldImpulse;
+            velA.ang = velA.ang - iA * inc;
+            velB.ang = velB.ang + iB * inc;
+        }
+        const V2<R> vb = velB.lin + RevPerp(j.rB) * (velB.ang / R(1));
+        const V2<R> va = velA.lin + RevPerp(j.rA) * (velA.ang / R(1));
+        const V2<R> vDelta = vb - va;
+        V3<R> impulse;
+        if (j.enableLimit && j.limitState != kLimitInactive && !fixedRotation) {
+            const V3<R> cdot{vDelta.x / R(1), vDelta.y / R(1), (velB.ang - velA.ang) / R(1)};
+            const V3<R> s = Solve33(j.mass, cdot);
+            impulse = V3<R>{-s.x, -s.y, -s.z};
+            auto update = [&](V3<R>& imp) {
+                const V2<R> rhs = -V2<R>{vDelta.x / R(1), vDelta.y / R(1)} + j.impulse.z * V2<R>{j.mass.ez.x, j.mass.ez.y};
+                const V2<R> reduced = Solve22(j.mass, rhs);
+                imp = V3<R>{reduced.x, reduced.y, -j.impulse.z};
+                return V3<R>{j.impulse.x + reduced.x, j.impulse.y + reduced.y, R(0)};
+            };
+            auto add = [](V3<R> a, V3<R> b) { return V3<R>{a.x + b.x, a.y + b.y, a.z + b.z}; };
+            if (j.limitState == kLimitLower) {
+                const R newZ = j.impulse.z + impulse.z;
+                j.impulse = (newZ < R(0)) ? update(impulse) : add(j.impulse, impulse);
+            } else if (j.limitState == kLimitUpper) {
+                const R newZ = j.impulse.z + impulse.z;
+                j.impulse = (newZ > R(0)) ? update(impulse) : add(j.impulse, impulse);
+            } else {
+                j.impulse = add(j.impulse, impulse);
+            }
+            const V2<R> P{impulse.x, impulse.y};
+            const R L = impulse.z * R(1) * R(1) / (R(1) * R(1));
+            const R LA = Cross(j.rA, P) / R(1) + L;
+            const R LB = Cross(j.rB, P) / R(1) + L;
+            velA.lin = velA.lin - mA * P;
+            velA.ang = velA.ang - iA * LA;
+            velB.lin = velB.lin + mB * P;
+            velB.ang = velB.ang + iB * LB;
+        } else {
+            const V2<R> imp = Solve22(j.mass, -V2<R>{vDelta.x / R(1), vDelta.y / R(1)});
+            j.impulse.x += imp.x;
+            j.impulse.y += imp.y;
+            const V2<R> P{imp.x, imp.y};
+            const R LA = Cross(j.rA, P) / R(1);
+            const R LB = Cross(j.rB, P) / R(1);
+            velA.lin = velA.lin - mA * P;
+            velA.ang = velA.ang - iA * LA;
+            velB.lin = velB.lin + mB * P;
+            velB.ang = velB.ang + iB * LB;
+        }
+        const bool changed = !(velA.lin == oldVelA.lin) || velA.ang != oldVelA.ang || !(velB.lin == oldVelB.lin) ||
+                             velB.ang != oldVelB.ang;
+        if (changed) {
+            A.vel = velA;
+            B.vel = velB;
+            return false;
+        }
+        return true;
+    }
+
+    // SolvePosition(const RevoluteJointConf&) (RevoluteJointConf.cpp:333-433).
+    static bool JointSolvePosition(const Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        Pos<R> posA = A.pos, posB = B.pos;
+        const R iA = A.invRotI, iB = B.invRotI;
+        const bool fixedRotation = ((iA + iB) == R(0));
+        R angularError = 0;
+        if (j.enableLimit && j.limitState != kLimitInactive && !fixedRotation) {
+            const R angle = posB.ang - posA.ang - j.referenceAngle;
+            R limitImpulse = R(0) * R(1) * R(1) / R(1);
+            if (j.limitState == kLimitLower) {
+                R C = angle - j.lower;
+                angularError = -C;
+                C = Clamp(C + conf.angularSlop, -conf.maxAngularCorrection, R(0));
+                limitImpulse = -j.angularMass * C;
+            } else if (j.limitState == kLimitUpper) {
+                R C = angle - j.upper;
+                angularError = C;
+                C = Clamp(C - conf.angularSlop, R(0), conf.maxAngularCorrection);
+                limitImpulse = -j.angularMass * C;
+            } else {
+                const R C = Clamp(angle - j.lower, -conf.maxAngularCorrection, conf.maxAngularCorrection);
+                limitImpulse = -j.angularMass * C;
+                angularError = std::abs(C);
+            }
+            posA.ang = posA.ang - iA * limitImpulse;
+            posB.ang = posB.ang + iB * limitImpulse;
+        }
+        R positionError;
+        {
+            const V2<R> qA = UnitFromAngle(posA.ang);
+            const V2<R> qB = UnitFromAngle(posB.ang);
+            const V2<R> rA = Rotate(j.laA - A.lc, qA);
+            const V2<R> rB = Rotate(j.laB - B.lc, qB);
+            const V2<R> C = (posB.lin + rB) - (posA.lin + rA);
+            positionError = MagSq(C);
+            const R mA = A.invMass, mB = B.invMass;
+            const R exx = mA + (iA * (rA.y * rA.y) / R(1)) + mB + (iB * (rB.y * rB.y) / R(1));
+            const R exy = (-iA * rA.x * rA.y / R(1)) + (-iB * rB.x * rB.y / R(1));
+            const R eyy = mA + (iA * (rA.x * rA.x) / R(1)) + mB + (iB * (rB.x * rB.x) / R(1));
+            // Solve(Matrix22, Vector2) (Math.hpp:413-426): columns (exx, exy), (exy, eyy).
+            V2<R> sol{};
+            const R cp = Cross(V2<R>{exx, exy}, V2<R>{exy, eyy});
+            if (!AlmostZero(cp)) {
+                const R inverse = R(1) / cp;
+                sol = V2<R>{(eyy * C.x - exy * C.y) * inverse, (exx * C.y - exy * C.x) * inverse};
+            }
+            const V2<R> P = -sol;
+            posA.lin = posA.lin - mA * P;
+            posA.ang = posA.ang - iA * Cross(rA, P) / R(1);
+            posB.lin = posB.lin + mB * P;
+            posB.ang = posB.ang + iB * Cross(rB, P) / R(1);
+        }
+        A.pos = posA;
+        B.pos = posB;
+        return (positionError <= conf.linearSlop * conf.linearSlop) && (angularError <= conf.angularSlop);
+    }
 
     // AddToIsland (AabbTreeWorld.cpp:1358-1444)
     void AddToIsland(Island& island, std::uint32_t seed) {
@@ -1333,6 +1595,20 @@ public:
                     const std::uint32_t other = (c.bodyA == bid) ? c.bodyB : c.bodyA;
                     island.contacts.push_back(cid);
                     islandedContacts[cid] = 1;
+                    if (!islandedBodies[other]) {
+                        islandedBodies[other] = 1;
+                        stack.push_back(other);
+                    }
+                }
+            }
+            // AddJointsToIsland (AabbTreeWorld.cpp:1446-1470).
+            for (const auto& bj : bodyJoints[bid]) {
+                const std::uint32_t jid = bj.second;
+                if (islandedJoints[jid]) continue;
+                const std::uint32_t other = bj.first;
+                if (bodies[other].flags & kEnabled) {
+                    islandedJoints[jid] = 1;
+                    island.joints.push_back(jid);
                     if (!islandedBodies[other]) {
                         islandedBodies[other] = 1;
                         stack.push_back(other);
@@ -1388,10 +1664,13 @@ public:
                 B.vel.ang = B.vel.ang + dB.ang;
             }
         }
+        for (std::uint32_t jid : island.joints) JointInitVelocity(joints[jid], bcs, conf);
         for (std::uint32_t i = 0; i < conf.velIters; ++i) {
+            bool jointsOkay = true;
+            for (std::uint32_t jid : island.joints) jointsOkay &= JointSolveVelocity(joints[jid], bcs, conf);
             R newInc = 0;
             for (auto& vc : vcs) newInc = Max(newInc, SolveVC(vc, bcs));
-            if (newInc <= conf.regMinMomentum) break;
+            if (jointsOkay && newInc <= conf.regMinMomentum) break;
         }
         for (std::uint32_t bid : island.bodies) {
             BodyConstraint<R>& bc = bcs[bid];
@@ -1403,7 +1682,10 @@ public:
         for (std::uint32_t i = 0; i < conf.posIters; ++i) {
             R minSep = std::numeric_limits<R>::infinity();
             for (const auto& pc : pcs) minSep = Min(minSep, SolvePC(pc, bcs, conf));
-            if (minSep >= conf.regMinSeparation) {
+            const bool contactsOkay = minSep >= conf.regMinSeparation;
+            bool jointsOkay = true;
+            for (std::uint32_t jid : island.joints) jointsOkay &= JointSolvePosition(joints[jid], bcs, conf);
+            if (contactsOkay && jointsOkay) {
                 solved = true;
                 break;
             }
@@ -1451,6 +1733,7 @@ public:
     void SolveReg(const StepConf<R>& conf, StepStats& st) {
         islandedBodies.assign(bodies.size(), 0);
         islandedContacts.assign(contactBuf.size(), 0);
+        islandedJoints.assign(joints.size(), 0);
         for (std::uint32_t bid : bodyOrder) {
             if (islandedBodies[bid]) continue;
             Body<R>& b = bodies[bid];
@@ -1494,9 +1777,20 @@ public:
         bodies.back().xf = GetXf(b.pos1, b.lc);
         bodyOrder.push_back(id);
         bodyContacts.emplace_back();
+        bodyJoints.emplace_back();
         if (b.flags & kEnabled) {
             for (std::uint32_t s : b.shapes) fixturesForProxies.emplace_back(id, s);
         }
+        return id;
+    }
+
+    // CreateJoint + AabbTreeWorld::Add (AabbTreeWorld.cpp:1132-1175); joints
+    // are created before any contact exists, so no contact needs filtering.
+    std::uint32_t AddJoint(const Joint<R>& j) {
+        const std::uint32_t id = std::uint32_t(joints.size());
+        joints.push_back(j);
+        bodyJoints[j.bodyA].emplace_back(j.bodyB, id);
+        bodyJoints[j.bodyB].emplace_back(j.bodyA, id);
         return id;
     }
 };

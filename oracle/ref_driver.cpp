@@ -8,18 +8,23 @@
 //   ref_driver --time <scene> <warmup> <steps> <doToi> <threads> <nworlds> [params...]
 //
 // Record layout (all u32/f32):
-//   "PRXD" 1
+//   "PRXD" 2
 //   nshapes, per shape: nverts, radius, density, friction, restitution,
 //                       sensor, (cat|mask<<16), group, verts[2n], normals[2n]
 //   nbodies, per body:  type, flags, f32[19] (pos0 3, pos1 3, lc 2, vel 3,
 //                       linAcc 2, angAcc, linDamp, angDamp, invMass, invRotI,
 //                       underActive), nshape, shapeIds[]
+//   njoints, per joint: bodyA, bodyB, flags (limit 1, motor 2, collideConnected 4),
+//                       f32[9] (laA 2, laB 2, referenceAngle, lower, upper,
+//                       motorSpeed, maxMotorTorque)
 //   conf: f32/u32 StepConf fields used by the step (see WriteConf)
 //   nrecords, per record: step, stats u32[8],
 //                         B, per body f32[10] (pos0, pos1, vel, underActive), flags
 //                         C, per contact u32[4] (kmin,kmax,id,flags), u32[3]
 //                           (type|npts<<8, cf0, cf1), f32[14]
 //                         P, per proxy u32 id, f32[4]
+//                         J, per joint u32 limitState, f32[4] (impulse 3,
+//                           angular motor impulse)
 #include <playrho/d2/BasicAPI.hpp>
 #include <playrho/d2/World.hpp>
 #include <playrho/d2/WorldBody.hpp>
@@ -30,6 +35,8 @@
 #include <playrho/d2/PolygonShapeConf.hpp>
 #include <playrho/d2/DiskShapeConf.hpp>
 #include <playrho/d2/EdgeShapeConf.hpp>
+#include <playrho/d2/RevoluteJointConf.hpp>
+#include <playrho/d2/WorldJoint.hpp>
 #include <playrho/StepConf.hpp>
 
 #include <chrono>
@@ -152,6 +159,18 @@ void WriteDescription(Out& o, const World& world) {
         o.u32(std::uint32_t(shapes.size()));
         for (auto s : shapes) o.u32(to_underlying(s));
     }
+    const auto njoints = GetJointRange(world);
+    o.u32(njoints);
+    for (auto i = 0u; i < njoints; ++i) {
+        const auto j = TypeCast<RevoluteJointConf>(GetJoint(world, JointID(i)));
+        o.u32(to_underlying(GetBodyA(j)));
+        o.u32(to_underlying(GetBodyB(j)));
+        o.u32((j.enableLimit ? 1u : 0u) | (j.enableMotor ? 2u : 0u) | (j.collideConnected ? 4u : 0u));
+        const float f[9] = {GetX(j.localAnchorA), GetY(j.localAnchorA), GetX(j.localAnchorB),
+                            GetY(j.localAnchorB), j.referenceAngle, j.lowerAngle, j.upperAngle,
+                            j.motorSpeed, j.maxMotorTorque};
+        o.put(f, sizeof(f));
+    }
 }
 
 void WriteConf(Out& o, const StepConf& c) {
@@ -241,6 +260,14 @@ void WriteState(Out& o, const World& world, std::uint32_t step, const StepStats&
                             aabb.ranges[0].GetMax(), aabb.ranges[1].GetMax()};
         o.put(a, sizeof(a));
     }
+    const auto njoints = GetJointRange(world);
+    o.u32(njoints);
+    for (auto i = 0u; i < njoints; ++i) {
+        const auto j = TypeCast<RevoluteJointConf>(GetJoint(world, JointID(i)));
+        o.u32(std::uint32_t(j.limitState));
+        const float f[4] = {GetX(j.impulse), GetY(j.impulse), GetZ(j.impulse), j.angularMotorImpulse};
+        o.put(f, sizeof(f));
+    }
 }
 
 }  // namespace
@@ -267,6 +294,15 @@ World BuildWorld(const prx_scene::SceneSpec& spec) {
             conf.Use(shapeIds[size_t(si)]);
         }
         CreateBody(world, conf);
+    }
+    for (const auto& js : spec.joints) {
+        auto jc = RevoluteJointConf{BodyID(static_cast<BodyID::underlying_type>(js.a)),
+                                    BodyID(static_cast<BodyID::underlying_type>(js.b)),
+                                    Length2{js.lax, js.lay}, Length2{js.lbx, js.lby}, js.referenceAngle};
+        jc.UseEnableLimit(js.enableLimit).UseLowerAngle(js.lower).UseUpperAngle(js.upper);
+        jc.UseEnableMotor(js.enableMotor).UseMotorSpeed(js.motorSpeed).UseMaxMotorTorque(js.maxMotorTorque);
+        jc.UseCollideConnected(js.collideConnected);
+        CreateJoint(world, jc);
     }
     return world;
 }
@@ -367,7 +403,7 @@ int main(int argc, char** argv) {
 
     Out o;
     o.put("PRXD", 4);
-    o.u32(1);
+    o.u32(2);
     WriteDescription(o, world);
     WriteConf(o, conf);
     o.u32(std::uint32_t(steps + 1));

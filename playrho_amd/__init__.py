@@ -77,6 +77,15 @@ class BodyConf(ctypes.Structure):
                 ("nshapes", ctypes.c_uint32)]
 
 
+class RevoluteJointConf(ctypes.Structure):
+    _fields_ = [("body_a", ctypes.c_uint32), ("body_b", ctypes.c_uint32),
+                ("local_anchor_a", ctypes.c_float * 2), ("local_anchor_b", ctypes.c_float * 2),
+                ("reference_angle", ctypes.c_float), ("enable_limit", ctypes.c_uint32),
+                ("lower_angle", ctypes.c_float), ("upper_angle", ctypes.c_float),
+                ("enable_motor", ctypes.c_uint32), ("motor_speed", ctypes.c_float),
+                ("max_motor_torque", ctypes.c_float), ("collide_connected", ctypes.c_uint32)]
+
+
 class StepStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in (
         "pre_proxies_moved", "pre_contacts_destroyed", "pre_contacts_added", "pre_contacts_updated",
@@ -85,7 +94,8 @@ class StepStats(ctypes.Structure):
 
 class BatchConf(ctypes.Structure):
     _fields_ = [("threads_per_world", ctypes.c_uint32), ("use_graph", ctypes.c_uint32),
-                ("order", ctypes.c_uint32)]
+                ("order", ctypes.c_uint32), ("coop_min_contacts", ctypes.c_uint32),
+                ("lds_bytes", ctypes.c_uint32)]
 
 
 _lib = None
@@ -106,6 +116,8 @@ _SIGS = {
     "prx_world_save_desc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
                                            ctypes.POINTER(ctypes.c_size_t)]),
     "prx_world_body_count": (ctypes.c_uint32, [ctypes.c_void_p]),
+    "prx_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
+    "prx_world_joint_count": (ctypes.c_uint32, [ctypes.c_void_p]),
     "prx_world_shape_count": (ctypes.c_uint32, [ctypes.c_void_p]),
     "prx_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "prx_world_get_bodies": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
@@ -210,6 +222,15 @@ class World:
         _check(lib().prx_body_create(self.h, ctypes.byref(conf), ctypes.byref(bid)))
         return bid.value
 
+    def create_joint(self, conf: RevoluteJointConf):
+        jid = ctypes.c_uint32()
+        _check(lib().prx_joint_create(self.h, ctypes.byref(conf), ctypes.byref(jid)))
+        return jid.value
+
+    @property
+    def joint_count(self):
+        return lib().prx_world_joint_count(self.h)
+
     @property
     def body_count(self):
         return lib().prx_world_body_count(self.h)
@@ -257,12 +278,16 @@ class World:
 class Batch:
     """Many worlds resident on the device, stepped by one launch per step."""
 
-    def __init__(self, worlds, use_graph=True, order=0):
+    def __init__(self, worlds, use_graph=True, order=0, coop_min=None, lds_bytes=None):
         self.worlds = list(worlds)
         conf = BatchConf()
         lib().prx_batch_conf_default(ctypes.byref(conf))
         conf.use_graph = 1 if use_graph else 0
         conf.order = order
+        if coop_min is not None:
+            conf.coop_min_contacts = int(coop_min)
+        if lds_bytes is not None:
+            conf.lds_bytes = int(lds_bytes)
         arr = (ctypes.c_void_p * len(self.worlds))(*[w.h.value for w in self.worlds])
         self.h = ctypes.c_void_p()
         _check(lib().prx_batch_create(arr, len(self.worlds), ctypes.byref(conf), ctypes.byref(self.h)))
@@ -302,6 +327,14 @@ class Batch:
         out = np.zeros((n, 16), dtype=np.uint64)
         _check(lib().prx_batch_phase_cycles(self.h, 1 if enable else 0, out.ctypes.data, n))
         return out
+
+    def autotune(self, conf=None, nsteps=10):
+        """Chooses the launch schedule (prx_batch_autotune) by timing nsteps
+        steps per candidate from the current state, which is restored."""
+        conf = conf or StepConf()
+        best = BatchConf()
+        _check(lib().prx_batch_autotune(self.h, ctypes.byref(conf), int(nsteps), ctypes.byref(best)))
+        return {k: getattr(best, k) for k, _ in BatchConf._fields_}
 
     def counts(self):
         """(awake dynamic bodies, proxies, shape bytes, contacts, touching) after download()."""

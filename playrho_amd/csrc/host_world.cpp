@@ -241,6 +241,15 @@ struct HostProxy {
     float aabb[4] = {0, 0, 0, 0};
 };
 
+struct HostJoint {
+    std::uint32_t a = 0, b = 0, flags = 0;  // prx::kJointLimit / kJointMotor / kJointCollideConnected
+    float la[4] = {0, 0, 0, 0};             // local anchor A x, y, B x, y
+    float cfg[4] = {0, 0, 0, 0};            // reference angle, lower, upper, motor speed
+    float maxTorque = 0;
+    float imp[4] = {0, 0, 0, 0};            // impulse x, y, z, angular motor impulse
+    std::uint32_t limit = 0;
+};
+
 // Contact records as held on the host between device residencies.
 struct HostContacts {
     std::vector<int2> prox, body, shape;
@@ -273,6 +282,7 @@ struct prx_world {
     std::vector<HostShape> shapes;
     std::vector<HostBody> bodies;
     std::vector<HostProxy> proxies;
+    std::vector<HostJoint> joints;
     HostContacts contacts;
     std::uint32_t leafCount = 0;
     prx_batch* owner = nullptr;  // batch holding the authoritative device state
@@ -318,8 +328,9 @@ struct prx_batch {
     hipGraph_t graph = nullptr;
     prx::StepParams graph_params{};
     bool graph_valid = false;
-    std::size_t totB = 0, totP = 0, totS = 0, totV = 0, totC = 0;
+    std::size_t totB = 0, totP = 0, totS = 0, totV = 0, totC = 0, totJ = 0;
     int lds_bytes = 16 * 1024;
+    int lds_min = 16 * 1024;
     ~prx_batch() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -549,6 +560,35 @@ prx::StepParams ToParams(const prx_step_conf& c) {
     return p;
 }
 
+int AddJoint(prx_world* w, const prx_revolute_joint_conf& c, std::uint32_t* out) {
+    if (c.body_a >= w->bodies.size() || c.body_b >= w->bodies.size())
+        return Fail(PRX_E_OUT_OF_RANGE, "no such body");
+    if (!c.collide_connected) {
+        for (std::size_t i = 0; i < w->contacts.size(); ++i) {
+            const int2 bd = w->contacts.body[i];
+            if ((bd.x == int(c.body_a) && bd.y == int(c.body_b)) || (bd.x == int(c.body_b) && bd.y == int(c.body_a)))
+                return Fail(PRX_E_UNSUPPORTED, "joint between bodies that already have a contact");
+        }
+    }
+    HostJoint j;
+    j.a = c.body_a;
+    j.b = c.body_b;
+    j.flags = (c.enable_limit ? prx::kJointLimit : 0u) | (c.enable_motor ? prx::kJointMotor : 0u) |
+              (c.collide_connected ? prx::kJointCollideConnected : 0u);
+    j.la[0] = c.local_anchor_a[0];
+    j.la[1] = c.local_anchor_a[1];
+    j.la[2] = c.local_anchor_b[0];
+    j.la[3] = c.local_anchor_b[1];
+    j.cfg[0] = c.reference_angle;
+    j.cfg[1] = c.lower_angle;
+    j.cfg[2] = c.upper_angle;
+    j.cfg[3] = c.motor_speed;
+    j.maxTorque = c.max_motor_torque;
+    if (out) *out = std::uint32_t(w->joints.size());
+    w->joints.push_back(j);
+    return PRX_OK;
+}
+
 int ContactCap(const prx_world* w) {
     const int per = w->conf.contact_capacity_per_proxy ? int(w->conf.contact_capacity_per_proxy) : 8;
     const int c = std::max<int>(int(w->contacts.size()) + 64, per * int(w->proxies.size()) + 64);
@@ -561,9 +601,31 @@ int Upload(T* dst, const std::vector<T>& src) {
     return PRX_OK;
 }
 
+// Launch schedule from the batch conf: dynamic LDS per workgroup (at least
+// what the largest world needs) and the launch order of the worlds.
+int ApplySchedule(prx_batch* b) {
+    int lds = b->lds_min;
+    if (b->conf.lds_bytes) lds = std::max<int>(lds, int(std::min<std::uint32_t>(b->conf.lds_bytes, 64u * 1024u)));
+    b->lds_bytes = ((lds + 255) / 256) * 256;
+    // Launch order: largest worlds first (stable), so the long worlds start
+    // early and the small ones fill in behind them.
+    const int nw = int(b->worlds.size());
+    std::vector<int> order(static_cast<std::size_t>(nw));
+    std::iota(order.begin(), order.end(), 0);
+    if (b->conf.order == 0) {
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+            return b->hdr[std::size_t(x)].nbodies + b->hdr[std::size_t(x)].nproxies >
+                   b->hdr[std::size_t(y)].nbodies + b->hdr[std::size_t(y)].nproxies;
+        });
+    }
+    PRX_HIP(hipMemcpy(b->d_schedule, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice));
+    b->graph_valid = false;
+    return PRX_OK;
+}
+
 int BuildBatch(prx_batch* b) {
     const int nw = int(b->worlds.size());
-    std::size_t B = 0, P = 0, S = 0, V = 0, C = 0;
+    std::size_t B = 0, P = 0, S = 0, V = 0, C = 0, J = 0;
     b->hdr.assign(std::size_t(nw), prx::WorldHeader{});
     b->shape_base.assign(std::size_t(nw), 0);
     b->vert_base.assign(std::size_t(nw), 0);
@@ -581,6 +643,9 @@ int BuildBatch(prx_batch* b) {
         h.pool_size = w->contacts.pool;
         h.nfree = int(w->contacts.freelist.size());
         h.status = 0;
+        h.joint_off = int(J);
+        h.njoints = int(w->joints.size());
+        J += w->joints.size();
         b->shape_base[std::size_t(i)] = int(S);
         b->vert_base[std::size_t(i)] = int(V);
         B += w->bodies.size();
@@ -591,12 +656,16 @@ int BuildBatch(prx_batch* b) {
     }
     int maxNb = 0;
     for (const prx_world* w : b->worlds) maxNb = std::max<int>(maxNb, int(w->bodies.size()));
-    b->lds_bytes = std::min<int>(64 * 1024, std::max<int>(1024, ((maxNb * 32 + 255) / 256) * 256));
+    // Body constraint state (32 B per body) plus the record stage; worlds whose
+    // bodies do not fit keep them in global scratch.
+    const int stageBytes = prx::kStageFloat4 * 16;
+    b->lds_min = std::min<int>(64 * 1024, std::max<int>(1024 + stageBytes, ((maxNb * 32 + stageBytes + 255) / 256) * 256));
     b->totB = B;
     b->totP = P;
     b->totS = S;
     b->totV = V;
     b->totC = C;
+    b->totJ = J;
     auto& a = b->arena;
     auto& d = b->d;
     d.nworlds = nw;
@@ -647,19 +716,20 @@ int BuildBatch(prx_batch* b) {
     PRX_HIP(a.alloc(d.s_bc_vel, B));
     PRX_HIP(a.alloc(d.s_vc, C * std::size_t(prx::kVcStride)));
     PRX_HIP(a.alloc(d.s_pc, C * std::size_t(prx::kPcStride)));
-    // Launch order: largest worlds first (stable), so the long worlds start
-    // early and the small ones fill in behind them.
-    std::vector<int> order(static_cast<std::size_t>(nw));
-    std::iota(order.begin(), order.end(), 0);
-    if (b->conf.order == 0) {
-        std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-            return b->hdr[std::size_t(x)].nbodies + b->hdr[std::size_t(x)].nproxies >
-                   b->hdr[std::size_t(y)].nbodies + b->hdr[std::size_t(y)].nproxies;
-        });
-    }
-    PRX_HIP(hipMemcpy(b->d_schedule, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice));
+    PRX_HIP(a.alloc(d.j_body, J));
+    PRX_HIP(a.alloc(d.j_anchor, J));
+    PRX_HIP(a.alloc(d.j_cfg, J));
+    PRX_HIP(a.alloc(d.j_cfg2, J));
+    PRX_HIP(a.alloc(d.j_imp, J));
+    PRX_HIP(a.alloc(d.j_limit, J));
+    PRX_HIP(a.alloc(d.bjoint, B));
+    PRX_HIP(a.alloc(d.bjoint_list, 2 * J));
+    PRX_HIP(a.alloc(d.s_jc, J * std::size_t(prx::kJcStride)));
+    PRX_HIP(a.alloc(d.s_isl_joint, J));
+    PRX_HIP(a.alloc(d.s_isl_jstart, B + std::size_t(nw)));
+    PRX_HIP(a.alloc(d.s_isl_j, J));
     d.schedule = b->d_schedule;
-    return PRX_OK;
+    return ApplySchedule(b);
 }
 
 // Host mirror -> device for all member worlds.
@@ -673,9 +743,36 @@ int UploadBatch(prx_batch* b) {
     std::vector<int4> shape_i(S);
     std::vector<float4> cmat(C), mgeo(C), mp0(C), mp1(C);
     std::vector<uint4> mhdr(C);
+    const std::size_t J = b->totJ;
+    std::vector<int2> jbody(J), bjoint(B), bjlist(2 * J);
+    std::vector<float4> janchor(J), jcfg(J), jcfg2(J), jimp(J);
+    std::vector<std::uint32_t> jlimit(J);
     for (std::size_t wi = 0; wi < b->worlds.size(); ++wi) {
         const prx_world* w = b->worlds[wi];
         const auto& h = b->hdr[wi];
+        // Joints and per-body joint lists in creation order (m_bodyJoints).
+        {
+            std::vector<std::vector<int2>> lists(w->bodies.size());
+            for (std::size_t ji = 0; ji < w->joints.size(); ++ji) {
+                const HostJoint& j = w->joints[ji];
+                const std::size_t g = std::size_t(h.joint_off) + ji;
+                jbody[g] = int2{int(j.a), int(j.b)};
+                janchor[g] = float4{j.la[0], j.la[1], j.la[2], j.la[3]};
+                jcfg[g] = float4{j.cfg[0], j.cfg[1], j.cfg[2], j.cfg[3]};
+                float fl;
+                std::memcpy(&fl, &j.flags, 4);
+                jcfg2[g] = float4{j.maxTorque, fl, 0.0f, 0.0f};
+                jimp[g] = float4{j.imp[0], j.imp[1], j.imp[2], j.imp[3]};
+                jlimit[g] = j.limit;
+                lists[j.a].push_back(int2{int(ji), int(j.b)});
+                lists[j.b].push_back(int2{int(ji), int(j.a)});
+            }
+            int at = 0;
+            for (std::size_t bi = 0; bi < w->bodies.size(); ++bi) {
+                bjoint[std::size_t(h.body_off) + bi] = int2{at, int(lists[bi].size())};
+                for (const int2& e : lists[bi]) bjlist[2 * std::size_t(h.joint_off) + std::size_t(at++)] = e;
+            }
+        }
         const int sb = b->shape_base[wi];
         int vb = b->vert_base[wi];
         for (std::size_t si = 0; si < w->shapes.size(); ++si) {
@@ -744,7 +841,10 @@ int UploadBatch(prx_batch* b) {
         (st = Upload(d.c_prox, cprox)) || (st = Upload(d.c_body, cbody)) || (st = Upload(d.c_shape, cshape)) ||
         (st = Upload(d.c_id, cid)) || (st = Upload(d.c_flags, cflags)) || (st = Upload(d.c_mat, cmat)) ||
         (st = Upload(d.m_hdr, mhdr)) || (st = Upload(d.m_geo, mgeo)) || (st = Upload(d.m_p0, mp0)) ||
-        (st = Upload(d.m_p1, mp1)) || (st = Upload(d.c_free, cfree)))
+        (st = Upload(d.m_p1, mp1)) || (st = Upload(d.c_free, cfree)) || (st = Upload(d.j_body, jbody)) ||
+        (st = Upload(d.j_anchor, janchor)) || (st = Upload(d.j_cfg, jcfg)) || (st = Upload(d.j_cfg2, jcfg2)) ||
+        (st = Upload(d.j_imp, jimp)) || (st = Upload(d.j_limit, jlimit)) || (st = Upload(d.bjoint, bjoint)) ||
+        (st = Upload(d.bjoint_list, bjlist)))
         return st;
     return PRX_OK;
 }
@@ -775,6 +875,9 @@ int DownloadBatch(prx_batch* b) {
         (st = Download(mhdr, d.m_hdr, C)) || (st = Download(mgeo, d.m_geo, C)) || (st = Download(mp0, d.m_p0, C)) ||
         (st = Download(mp1, d.m_p1, C)) || (st = Download(cfree, d.c_free, C)))
         return st;
+    std::vector<float4> jimp;
+    std::vector<std::uint32_t> jlimit;
+    if ((st = Download(jimp, d.j_imp, b->totJ)) || (st = Download(jlimit, d.j_limit, b->totJ))) return st;
     b->hdr = hdr;
     for (std::size_t wi = 0; wi < b->worlds.size(); ++wi) {
         prx_world* w = b->worlds[wi];
@@ -819,6 +922,15 @@ int DownloadBatch(prx_batch* b) {
             hc.geo[std::size_t(ci)] = mgeo[g];
             hc.p0[std::size_t(ci)] = mp0[g];
             hc.p1[std::size_t(ci)] = mp1[g];
+        }
+        for (std::size_t ji = 0; ji < w->joints.size(); ++ji) {
+            HostJoint& j = w->joints[ji];
+            const std::size_t g = std::size_t(h.joint_off) + ji;
+            j.imp[0] = jimp[g].x;
+            j.imp[1] = jimp[g].y;
+            j.imp[2] = jimp[g].z;
+            j.imp[3] = jimp[g].w;
+            j.limit = jlimit[g];
         }
         hc.freelist.assign(cfree.begin() + h.contact_off, cfree.begin() + h.contact_off + h.nfree);
         hc.pool = h.pool_size;
@@ -875,6 +987,7 @@ int StepBatch(prx_batch* b, const prx_step_conf* conf, int nsteps) {
     if (nsteps < 0) return Fail(PRX_E_INVALID_ARGUMENT, "negative step count");
     prx::StepParams p = ToParams(*conf);
     p.ldsBytes = std::uint32_t(b->lds_bytes);
+    p.coopMin = b->conf.coop_min_contacts;
     const int nw = int(b->worlds.size());
     PRX_HIP(hipEventRecord(b->ev0, b->stream));
     if (b->conf.use_graph && nsteps > 1) {
@@ -953,6 +1066,8 @@ void prx_batch_conf_default(prx_batch_conf* c) {
     c->threads_per_world = 64;
     c->use_graph = 1;
     c->order = 0;
+    c->coop_min_contacts = 32;
+    c->lds_bytes = 0;
 }
 
 int prx_world_create(const prx_world_conf* conf, prx_world** out) {
@@ -1041,8 +1156,35 @@ int prx_scene_build(prx_world* w, const char* name, const std::int32_t* params, 
         c.nshapes = std::uint32_t(bsh.size());
         if (int st = AddBody(w, c, nullptr)) return st;
     }
+    for (const auto& js : spec.joints) {
+        prx_revolute_joint_conf jc{};
+        jc.body_a = std::uint32_t(js.a);
+        jc.body_b = std::uint32_t(js.b);
+        jc.local_anchor_a[0] = js.lax;
+        jc.local_anchor_a[1] = js.lay;
+        jc.local_anchor_b[0] = js.lbx;
+        jc.local_anchor_b[1] = js.lby;
+        jc.reference_angle = js.referenceAngle;
+        jc.enable_limit = js.enableLimit ? 1u : 0u;
+        jc.lower_angle = js.lower;
+        jc.upper_angle = js.upper;
+        jc.enable_motor = js.enableMotor ? 1u : 0u;
+        jc.motor_speed = js.motorSpeed;
+        jc.max_motor_torque = js.maxMotorTorque;
+        jc.collide_connected = js.collideConnected ? 1u : 0u;
+        if (int st = AddJoint(w, jc, nullptr)) return st;
+    }
     return PRX_OK;
 }
+
+int prx_joint_create(prx_world* w, const prx_revolute_joint_conf* conf, std::uint32_t* id) {
+    if (int st = CheckWorld(w)) return st;
+    if (!conf) return Fail(PRX_E_INVALID_ARGUMENT, "null joint conf");
+    if (int st = Detach(w)) return st;
+    return AddJoint(w, *conf, id);
+}
+
+std::uint32_t prx_world_joint_count(const prx_world* w) { return w ? std::uint32_t(w->joints.size()) : 0u; }
 
 int prx_world_load_desc(prx_world* w, const std::uint8_t* desc, std::size_t len) {
     if (int st = CheckWorld(w)) return st;
@@ -1141,9 +1283,22 @@ int prx_world_load_desc(prx_world* w, const std::uint8_t* desc, std::size_t len)
         }
         fresh.bodies.push_back(std::move(b));
     }
+    const std::uint32_t nj = u32();
+    for (std::uint32_t i = 0; i < nj && !bad; ++i) {
+        HostJoint j;
+        j.a = u32();
+        j.b = u32();
+        j.flags = u32() & 7u;
+        if (j.a >= fresh.bodies.size() || j.b >= fresh.bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
+        for (float& v : j.la) v = f32();
+        for (float& v : j.cfg) v = f32();
+        j.maxTorque = f32();
+        fresh.joints.push_back(j);
+    }
     if (bad) return Fail(PRX_E_INVALID_ARGUMENT, "truncated description");
     w->shapes = std::move(fresh.shapes);
     w->bodies = std::move(fresh.bodies);
+    w->joints = std::move(fresh.joints);
     w->proxies = std::move(fresh.proxies);
     w->contacts = HostContacts{};
     w->leafCount = fresh.leafCount;
@@ -1189,6 +1344,15 @@ int prx_world_save_desc(prx_world* w, std::uint8_t* buf, std::size_t cap, std::s
         put(f, sizeof(f));
         u32(std::uint32_t(b.shapes.size()));
         for (auto s : b.shapes) u32(s);
+    }
+    u32(std::uint32_t(w->joints.size()));
+    for (const auto& j : w->joints) {
+        u32(j.a);
+        u32(j.b);
+        u32(j.flags);
+        put(j.la, sizeof(j.la));
+        put(j.cfg, sizeof(j.cfg));
+        f32(j.maxTorque);
     }
     if (len) *len = o.size();
     if (buf && cap >= o.size()) std::memcpy(buf, o.data(), o.size());
@@ -1348,6 +1512,11 @@ int prx_world_state_record(prx_world* w, std::uint8_t* buf, std::size_t cap, std
         u32(p.id);
         put(p.aabb, sizeof(p.aabb));
     }
+    u32(std::uint32_t(w->joints.size()));
+    for (const auto& j : w->joints) {
+        u32(j.limit);
+        put(j.imp, sizeof(j.imp));
+    }
     if (len) *len = o.size();
     if (buf && cap >= o.size()) std::memcpy(buf, o.data(), o.size());
     else if (buf) return Fail(PRX_E_LENGTH, "buffer too small");
@@ -1448,8 +1617,71 @@ int prx_batch_phase_cycles(prx_batch* b, int enable, std::uint64_t* out, int max
 
 int prx_batch_autotune(prx_batch* b, const prx_step_conf* step, int nsteps, prx_batch_conf* best) {
     if (!b || !step) return Fail(PRX_E_INVALID_ARGUMENT, "null argument");
-    if (best) *best = b->conf;
-    (void)nsteps;
+    if (nsteps < 1) return Fail(PRX_E_INVALID_ARGUMENT, "nsteps < 1");
+    for (prx_world* w : b->worlds)
+        if (!w) return Fail(PRX_E_WRONG_STATE, "a member world left the batch");
+    // Start state: the host mirror after a download; every trial starts from
+    // it and it is restored at the end.
+    if (int st = DownloadBatch(b)) return st;
+    const prx_batch_conf start = b->conf;
+    auto trial = [&](const prx_batch_conf& c, float* ms) -> int {
+        b->conf = c;
+        if (int st = ApplySchedule(b)) return st;
+        if (int st = UploadBatch(b)) return st;
+        // Two untimed steps capture the graph for this schedule.
+        if (int st = StepBatch(b, step, 2)) return st;
+        PRX_HIP(hipStreamSynchronize(b->stream));
+        if (int st = UploadBatch(b)) return st;
+        if (int st = StepBatch(b, step, nsteps)) return st;
+        PRX_HIP(hipEventSynchronize(b->ev1));
+        PRX_HIP(hipEventElapsedTime(ms, b->ev0, b->ev1));
+        return CheckStatus(b);
+    };
+    // Coordinate search: workgroups per CU (LDS per workgroup), then the
+    // wave-wide sweep threshold, then the launch order.
+    prx_batch_conf cur = start;
+    float bestMs = 0;
+    if (int st = trial(cur, &bestMs)) return st;
+    const std::uint32_t ldsCands[] = {0u, 20u * 1024u, 26u * 1024u + 512u, 32u * 1024u + 256u, 40u * 1024u};
+    for (std::uint32_t l : ldsCands) {
+        prx_batch_conf c = cur;
+        c.lds_bytes = l;
+        if (c.lds_bytes == cur.lds_bytes) continue;
+        float ms = 0;
+        if (int st = trial(c, &ms)) return st;
+        if (ms < bestMs) {
+            bestMs = ms;
+            cur = c;
+        }
+    }
+    const std::uint32_t coopCands[] = {8u, 16u, 32u, 64u, 128u, 0xffffffffu};
+    for (std::uint32_t k : coopCands) {
+        prx_batch_conf c = cur;
+        c.coop_min_contacts = k;
+        if (c.coop_min_contacts == cur.coop_min_contacts) continue;
+        float ms = 0;
+        if (int st = trial(c, &ms)) return st;
+        if (ms < bestMs) {
+            bestMs = ms;
+            cur = c;
+        }
+    }
+    {
+        prx_batch_conf c = cur;
+        c.order = cur.order ? 0u : 1u;
+        float ms = 0;
+        if (int st = trial(c, &ms)) return st;
+        if (ms < bestMs) {
+            bestMs = ms;
+            cur = c;
+        }
+    }
+    b->conf = cur;
+    if (int st = ApplySchedule(b)) return st;
+    if (int st = UploadBatch(b)) return st;
+    for (prx_world* w : b->worlds) w->host_stale = false;
+    b->last_launches = 0;
+    if (best) *best = cur;
     return PRX_OK;
 }
 

@@ -55,7 +55,7 @@ enum : std::uint32_t { kStatusContactOverflow = 1u, kStatusScratchOverflow = 2u 
 struct WorldHeader {
     std::int32_t body_off, nbodies, proxy_off, nproxies;
     std::int32_t contact_off, contact_cap, ncontacts, pool_size;
-    std::int32_t nfree, status, nislands, pad0;
+    std::int32_t nfree, status, joint_off, njoints;
     std::uint32_t stats[8];
 };
 
@@ -65,7 +65,8 @@ struct StepParams {
         maxLinearCorrection, maxAngularCorrection, linearSleepTolerance, angularSleepTolerance,
         displaceMultiplier, aabbExtension, maxCirclesRatio;
     std::uint32_t velIters, posIters, warmStart, blockSolve;
-    std::uint32_t ldsBytes;  // dynamic LDS per workgroup (island solve state)
+    std::uint32_t ldsBytes;  // dynamic LDS per workgroup (island solve state + record stage)
+    std::uint32_t coopMin;   // islands with at least this many contacts are swept by the whole wave
 };
 
 struct DeviceState {
@@ -112,6 +113,16 @@ struct DeviceState {
     float4* m_p1;
     std::uint32_t* c_free;  // pool free list (LIFO), per world segment of cap
 
+    // Revolute joints (world order = creation order).
+    int2* j_body;        // body A, body B (world-local)
+    float4* j_anchor;    // local anchor A x, y, local anchor B x, y
+    float4* j_cfg;       // reference angle, lower angle, upper angle, motor speed
+    float4* j_cfg2;      // max motor torque, flags (limit 1, motor 2, collideConnected 4), -, -
+    float4* j_imp;       // impulse x, y, z, angular motor impulse
+    std::uint32_t* j_limit;  // limit state (0 inactive, 1 lower, 2 upper, 3 equal)
+    int2* bjoint;        // per body: first entry in bjoint_list, count (world-local)
+    int2* bjoint_list;   // per body, in joint creation order: joint, other body (2 * njoints per world)
+
     // Scratch (per world, same segments as above).
     std::uint32_t* s_keys;   // candidate pair keys (cap)
     std::uint32_t* s_keys2;  // sorted candidate keys (cap)
@@ -127,6 +138,10 @@ struct DeviceState {
     float4* s_bc_vel;        // body constraint velocity x, y, w (nbodies)
     float4* s_vc;            // velocity constraints in island order, kVcStride float4 each
     float4* s_pc;            // position constraints in island order, kPcStride float4 each
+    float4* s_jc;            // joint constraints in island order, kJcStride float4 each (njoints)
+    int* s_isl_joint;        // island joint lists (njoints)
+    int* s_isl_jstart;       // island i: first joint entry (nbodies + 1, offset body_off + w)
+    std::uint8_t* s_isl_j;   // islanded joints (njoints)
 
     // Optional per-world phase timings of the last step (shader clock cycles),
     // kPhases entries per world; null when not collected.
@@ -141,5 +156,11 @@ enum Phase : int {
 
 constexpr int kVcStride = 9;
 constexpr int kPcStride = 5;
+constexpr int kJcStride = 8;
+enum : std::uint32_t { kJointLimit = 1u, kJointMotor = 2u, kJointCollideConnected = 4u };
+// Constraint records staged through LDS per chunk by the wave-wide sweeps;
+// the stage sits at the end of the dynamic LDS.
+constexpr int kStageRecords = 16;
+constexpr int kStageFloat4 = kStageRecords * kVcStride;
 
 }  // namespace prx

@@ -48,9 +48,22 @@ struct BodySpec {
     std::vector<int> shapes;
 };
 
+// Revolute joint between bodies a and b with local anchors (RevoluteJointConf).
+struct JointSpec {
+    int a = 0, b = 0;
+    float lax = 0, lay = 0, lbx = 0, lby = 0;
+    float referenceAngle = 0;
+    bool enableLimit = false;
+    float lower = 0, upper = 0;
+    bool enableMotor = false;
+    float motorSpeed = 0, maxMotorTorque = 0;
+    bool collideConnected = false;
+};
+
 struct SceneSpec {
     std::vector<ShapeSpec> shapes;
     std::vector<BodySpec> bodies;
+    std::vector<JointSpec> joints;  // created after all bodies, in order
     // Order in which the scene was authored: shape ids are allocated in
     // ShapeSpec order and body ids in BodySpec order; when interleave is true
     // shape i+1... are created right before body i uses them (see Apply*).
@@ -276,9 +289,73 @@ inline SceneSpec Billiards(std::uint32_t seed, int w) {
     return sc;
 }
 
+// Articulated world: a static anchor at (0, 40) and a chain of `links` boxes
+// (half extents 1 x 0.125, density 1) joined end to end by revolute joints
+// (anchor -> link 0, link i -> link i+1, collideConnected false), laid out
+// horizontally so it swings down, plus `falling` seeded boxes and disks
+// dropped onto it and onto a ground edge.  Joint motors act as joint friction
+// (motor speed 0, maximum torque `torque` N m); every third joint has a
+// +-60 degree limit.
+inline SceneSpec Chain(std::uint32_t seed, int w, int links = 32, int falling = 64, float torque = 2.0f) {
+    SceneSpec sc;
+    Rng rng(seed * 362437u + std::uint32_t(w) * 521288629u + 7u);
+    sc.shapes.push_back(Edge(-80.0f, 0.0f, 80.0f, 0.0f));
+    BodySpec ground;
+    ground.shapes = {0};
+    sc.bodies.push_back(ground);
+    BodySpec anchor;
+    anchor.x = 0.0f;
+    anchor.y = 40.0f;
+    sc.bodies.push_back(anchor);
+    sc.shapes.push_back(Box(1.0f, 0.125f, 1.0f, 0.4f, 0.0f));
+    const int linkShape = int(sc.shapes.size()) - 1;
+    for (int i = 0; i < links; ++i) {
+        BodySpec b;
+        b.type = kDynamic;
+        b.x = float(2 * i + 1);
+        b.y = 40.0f;
+        b.ay = -10.0f;
+        b.shapes = {linkShape};
+        sc.bodies.push_back(b);
+        JointSpec j;
+        j.a = (i == 0) ? 1 : 1 + i;
+        j.b = 2 + i;
+        j.lax = (i == 0) ? 0.0f : 1.0f;
+        j.lbx = -1.0f;
+        j.enableMotor = true;
+        j.motorSpeed = 0.0f;
+        j.maxMotorTorque = torque;
+        if (i % 3 == 2) {
+            j.enableLimit = true;
+            j.lower = -1.04719758f;
+            j.upper = 1.04719758f;
+        }
+        sc.joints.push_back(j);
+    }
+    for (int k = 0; k < falling; ++k) {
+        BodySpec b;
+        b.type = kDynamic;
+        b.x = rng.uniform(-30.0, 60.0);
+        b.y = rng.uniform(45.0, 90.0);
+        b.ay = -10.0f;
+        const float friction = rng.uniform(0.2, 0.8);
+        if (rng.below(2) == 0) {
+            const float hx = rng.uniform(0.3, 0.6);
+            const float hy = rng.uniform(0.3, 0.6);
+            sc.shapes.push_back(Box(hx, hy, 1.0f, friction, 0.0f));
+        } else {
+            sc.shapes.push_back(Disk(rng.uniform(0.3, 0.6), 1.0f, friction, 0.0f));
+        }
+        b.shapes = {int(sc.shapes.size()) - 1};
+        sc.bodies.push_back(b);
+    }
+    return sc;
+}
+
 // Named scenes with integer parameters, as used by tests and benchmarks.
 //   "hello", "pyramid"(rows), "drop"(n, half, height, seed),
-//   "pile"(seed, w, nmin, nmax), "billiards"(seed, w)
+//   "pile"(seed, w, nmin, nmax), "billiards"(seed, w),
+//   "chain"(seed, w, links, falling, torque)
 inline bool Make(const std::string& name, const int* p, int np, SceneSpec& out) {
     auto arg = [&](int i, int dflt) { return (i < np) ? p[i] : dflt; };
     if (name == "hello") {
@@ -291,6 +368,8 @@ inline bool Make(const std::string& name, const int* p, int np, SceneSpec& out) 
         out = Pile(std::uint32_t(arg(0, 1)), arg(1, 0), arg(2, 8), arg(3, 512));
     } else if (name == "billiards") {
         out = Billiards(std::uint32_t(arg(0, 1)), arg(1, 0));
+    } else if (name == "chain") {
+        out = Chain(std::uint32_t(arg(0, 1)), arg(1, 0), arg(2, 32), arg(3, 64), float(arg(4, 2)));
     } else {
         return false;
     }

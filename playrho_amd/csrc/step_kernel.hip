@@ -31,6 +31,7 @@ using prx::StepParams;
 using prx::WorldHeader;
 
 constexpr int kWave = 64;
+constexpr int kChunk = prx::kStageRecords;
 
 // Phase timer: lane 0 records shader-clock deltas into phase_cycles[w].
 struct PhaseTimer {
@@ -206,7 +207,7 @@ __device__ __forceinline__ void update_contact(const DeviceState& d, int bo, int
 }
 
 struct WorldCtx {
-    int w, bo, nb, po, np, co, cap;
+    int w, bo, nb, po, np, co, cap, jo, nj;
     int lane;
 };
 
@@ -308,6 +309,18 @@ __device__ __forceinline__ bool should_collide(const DeviceState& d, int sa, int
     if (ga == gb && ga != 0) return ga > 0;
     const std::uint32_t cma = d.shape_catmask[sa], cmb = d.shape_catmask[sb];
     return (((cma >> 16) & (cmb & 0xffffu)) != 0) && (((cmb >> 16) & (cma & 0xffffu)) != 0);
+}
+
+// ShouldCollide for two bodies' joints (AabbTreeWorld.cpp:479-490): false if
+// a joint between them does not let its bodies collide.
+__device__ __forceinline__ bool joint_allows(const DeviceState& d, const WorldCtx& x, int ba, int bb) {
+    if (x.nj == 0) return true;
+    const int2 bj = d.bjoint[x.bo + ba];
+    for (int k = 0; k < bj.y; ++k) {
+        const int2 e = d.bjoint_list[2 * x.jo + bj.x + k];
+        if (e.y == bb && !(__float_as_uint(d.j_cfg2[x.jo + e.x].y) & prx::kJointCollideConnected)) return false;
+    }
+    return true;
 }
 
 // FindContacts (AabbTreeWorld.cpp:709-751) + AddContacts (:2424-2547).
@@ -415,7 +428,8 @@ __device__ __forceinline__ std::uint32_t phase_find_add(const DeviceState& d, co
             sa = d.pshape[x.po + pa];
             sb = d.pshape[x.po + pb];
             const std::uint32_t fa = d.bflags[x.bo + ba], fb = d.bflags[x.bo + bb];
-            acc = ((fa | fb) & prx::kAccelerable) != 0 && should_collide(d, sa, sb) && d.s_isl_c[x.co + i] == 0;
+            acc = ((fa | fb) & prx::kAccelerable) != 0 && should_collide(d, sa, sb) && d.s_isl_c[x.co + i] == 0 &&
+                  joint_allows(d, x, ba, bb);
         }
         const unsigned long long am = __ballot(acc);
         if (acc) {
@@ -938,6 +952,296 @@ __device__ __forceinline__ float solve_pc(const Manifold& m, float totalRadius, 
     return minSep;
 }
 
+// ---------------------------------------------------------------------------
+// Revolute joints (RevoluteJointConf.cpp:127-433).  A joint's solver record in
+// island order (kJcStride float4):
+//   u[0] rA x, y, rB x, y               u[4] invMassA, invRotIA, invMassB, invRotIB
+//   u[1] mass exx, eyx, ezx, eyy        u[5] motor speed, dt * max motor torque, body A, body B
+//   u[2] ezy, ezz, angular mass, flags | limit state << 8
+//   u[3] impulse x, y, z, angular motor impulse (updated by the sweeps)
+//   u[6] local anchor A - local center A, local anchor B - local center B
+//   u[7] reference angle, lower angle, upper angle, -
+struct Jc {
+    V2 rA, rB;
+    float exx, eyx, ezx, eyy, ezy, ezz, angularMass;
+    std::uint32_t flags, limit;
+    float ix, iy, iz, motorImp;
+    BodyC A, B;
+    float motorSpeed, maxImpulse;
+    int bA, bB;
+};
+
+__device__ __forceinline__ Jc load_jc_velocity(const float4* u) {
+    const float4 r = u[0], m = u[1], n = u[2], im = u[3], ms = u[4], mo = u[5];
+    Jc j;
+    j.rA = V2{r.x, r.y};
+    j.rB = V2{r.z, r.w};
+    j.exx = m.x;
+    j.eyx = m.y;
+    j.ezx = m.z;
+    j.eyy = m.w;
+    j.ezy = n.x;
+    j.ezz = n.y;
+    j.angularMass = n.z;
+    j.flags = __float_as_uint(n.w) & 0xffu;
+    j.limit = __float_as_uint(n.w) >> 8;
+    j.ix = im.x;
+    j.iy = im.y;
+    j.iz = im.z;
+    j.motorImp = im.w;
+    j.A = BodyC{ms.x, ms.y, V2{0, 0}};
+    j.B = BodyC{ms.z, ms.w, V2{0, 0}};
+    j.motorSpeed = mo.x;
+    j.maxImpulse = mo.y;
+    j.bA = __float_as_int(mo.z);
+    j.bB = __float_as_int(mo.w);
+    return j;
+}
+
+// Solve33 / Solve22 of the symmetric joint mass (Math.hpp:446-474); vectors
+// GetX = (exx, eyx, ezx), GetY = (eyx, eyy, ezy), GetZ = (ezx, ezy, ezz).
+__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
+    float r = 0.0f;
+    r += ax * bx;
+    r += ay * by;
+    r += az * bz;
+    return r;
+}
+__device__ __forceinline__ void cross3(float ax, float ay, float az, float bx, float by, float bz, float& cx,
+                                       float& cy, float& cz) {
+    cx = ay * bz - az * by;
+    cy = az * bx - ax * bz;
+    cz = ax * by - ay * bx;
+}
+__device__ __forceinline__ void solve33(const Jc& j, float bx, float by, float bz, float& ox, float& oy, float& oz) {
+    float yzx, yzy, yzz;
+    cross3(j.eyx, j.eyy, j.ezy, j.ezx, j.ezy, j.ezz, yzx, yzy, yzz);
+    const float dp = dot3(j.exx, j.eyx, j.ezx, yzx, yzy, yzz);
+    const float det = (dp != 0.0f) ? (1.0f / dp) : dp;
+    ox = det * dot3(bx, by, bz, yzx, yzy, yzz);
+    float bzx, bzy, bzz;
+    cross3(bx, by, bz, j.ezx, j.ezy, j.ezz, bzx, bzy, bzz);
+    oy = det * dot3(j.exx, j.eyx, j.ezx, bzx, bzy, bzz);
+    float ybx, yby, ybz;
+    cross3(j.eyx, j.eyy, j.ezy, bx, by, bz, ybx, yby, ybz);
+    oz = det * dot3(j.exx, j.eyx, j.ezx, ybx, yby, ybz);
+}
+__device__ __forceinline__ V2 solve22(const Jc& j, V2 b) {
+    const float xx = j.exx, xy = j.eyx, yx = j.eyx, yy = j.eyy;
+    const float cp = xx * yy - xy * yx;
+    const float det = (cp != 0.0f) ? (1.0f / cp) : cp;
+    return V2{det * (yy * b.x - xy * b.y), det * (xx * b.y - yx * b.x)};
+}
+
+// Warm-start half of InitVelocity (RevoluteJointConf.cpp:199-224).
+__device__ __forceinline__ void joint_warm_start(const Jc& j, float4& velA, float4& velB) {
+    const V2 P{j.ix, j.iy};
+    const float L = j.motorImp + j.iz;
+    const float LA = cross(j.rA, P) / 1.0f + L;
+    const float LB = cross(j.rB, P) / 1.0f + L;
+    velA.x = velA.x - j.A.invMass * P.x;
+    velA.y = velA.y - j.A.invMass * P.y;
+    velA.z = velA.z - j.A.invRotI * LA;
+    velB.x = velB.x + j.B.invMass * P.x;
+    velB.y = velB.y + j.B.invMass * P.y;
+    velB.z = velB.z + j.B.invRotI * LB;
+}
+
+// SolveVelocity (RevoluteJointConf.cpp:228-331).  Returns true if the bodies'
+// velocities were left unchanged.
+__device__ __forceinline__ bool joint_solve_velocity(Jc& j, float4& velA, float4& velB) {
+    const float4 oldA = velA, oldB = velB;
+    const float mA = j.A.invMass, iA = j.A.invRotI, mB = j.B.invMass, iB = j.B.invRotI;
+    const bool fixedRotation = (iA + iB == 0.0f);
+    if ((j.flags & prx::kJointMotor) && j.limit != 3u && !fixedRotation) {
+        const float impulse = -j.angularMass * (velB.z - velA.z - j.motorSpeed);
+        const float oldImpulse = j.motorImp;
+        j.motorImp = clamp_std(j.motorImp + impulse, -j.maxImpulse, j.maxImpulse);
+        const float inc = j.motorImp - oldImpulse;
+        velA.z = velA.z - iA * inc;
+        velB.z = velB.z + iB * inc;
+    }
+    const V2 vb = V2{velB.x, velB.y} + mul(revperp(j.rB), velB.z / 1.0f);
+    const V2 va = V2{velA.x, velA.y} + mul(revperp(j.rA), velA.z / 1.0f);
+    const V2 vDelta = vb - va;
+    V2 P;
+    float L = 0.0f;
+    if ((j.flags & prx::kJointLimit) && j.limit != 0u && !fixedRotation) {
+        float sx, sy, sz;
+        solve33(j, vDelta.x / 1.0f, vDelta.y / 1.0f, (velB.z - velA.z) / 1.0f, sx, sy, sz);
+        float ix = -sx, iy = -sy, iz = -sz;
+        const bool lowerCase = j.limit == 1u, upperCase = j.limit == 2u;
+        const float newZ = j.iz + iz;
+        if ((lowerCase && newZ < 0.0f) || (upperCase && newZ > 0.0f)) {
+            const V2 rhs = -V2{vDelta.x / 1.0f, vDelta.y / 1.0f} + mul(j.iz, V2{j.ezx, j.ezy});
+            const V2 reduced = solve22(j, rhs);
+            ix = reduced.x;
+            iy = reduced.y;
+            iz = -j.iz;
+            j.ix = j.ix + reduced.x;
+            j.iy = j.iy + reduced.y;
+            j.iz = 0.0f;
+        } else {
+            j.ix = j.ix + ix;
+            j.iy = j.iy + iy;
+            j.iz = j.iz + iz;
+        }
+        P = V2{ix, iy};
+        L = iz;
+        const float LA = cross(j.rA, P) / 1.0f + L;
+        const float LB = cross(j.rB, P) / 1.0f + L;
+        velA.x = velA.x - mA * P.x;
+        velA.y = velA.y - mA * P.y;
+        velA.z = velA.z - iA * LA;
+        velB.x = velB.x + mB * P.x;
+        velB.y = velB.y + mB * P.y;
+        velB.z = velB.z + iB * LB;
+    } else {
+        const V2 imp = solve22(j, -V2{vDelta.x / 1.0f, vDelta.y / 1.0f});
+        j.ix = j.ix + imp.x;
+        j.iy = j.iy + imp.y;
+        P = imp;
+        const float LA = cross(j.rA, P) / 1.0f;
+        const float LB = cross(j.rB, P) / 1.0f;
+        velA.x = velA.x - mA * P.x;
+        velA.y = velA.y - mA * P.y;
+        velA.z = velA.z - iA * LA;
+        velB.x = velB.x + mB * P.x;
+        velB.y = velB.y + mB * P.y;
+        velB.z = velB.z + iB * LB;
+    }
+    const bool same = velA.x == oldA.x && velA.y == oldA.y && velA.z == oldA.z && velB.x == oldB.x &&
+                      velB.y == oldB.y && velB.z == oldB.z;
+    if (same) {
+        velA = oldA;
+        velB = oldB;
+    }
+    return same;
+}
+
+// SolvePosition (RevoluteJointConf.cpp:333-433).  Returns whether the joint
+// is within tolerance.
+__device__ __forceinline__ bool joint_solve_position(const float4* u, float4& posA, float4& posB, const StepParams& c) {
+    const float4 n = u[2], ms = u[4], la = u[6], ang = u[7];
+    const float angularMass = n.z;
+    const std::uint32_t flags = __float_as_uint(n.w) & 0xffu, limit = __float_as_uint(n.w) >> 8;
+    const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
+    const bool fixedRotation = ((iA + iB) == 0.0f);
+    float angularError = 0.0f;
+    if ((flags & prx::kJointLimit) && limit != 0u && !fixedRotation) {
+        const float angle = posB.z - posA.z - ang.x;
+        float limitImpulse = 0.0f;
+        if (limit == 1u) {
+            float C = angle - ang.y;
+            angularError = -C;
+            C = clamp_std(C + c.angularSlop, -c.maxAngularCorrection, 0.0f);
+            limitImpulse = -angularMass * C;
+        } else if (limit == 2u) {
+            float C = angle - ang.z;
+            angularError = C;
+            C = clamp_std(C - c.angularSlop, 0.0f, c.maxAngularCorrection);
+            limitImpulse = -angularMass * C;
+        } else {
+            const float C = clamp_std(angle - ang.y, -c.maxAngularCorrection, c.maxAngularCorrection);
+            limitImpulse = -angularMass * C;
+            angularError = __builtin_fabsf(C);
+        }
+        posA.z = posA.z - iA * limitImpulse;
+        posB.z = posB.z + iB * limitImpulse;
+    }
+    float positionError;
+    {
+        const V2 qA = unit_from_angle(posA.z);
+        const V2 qB = unit_from_angle(posB.z);
+        const V2 rA = rotate(V2{la.x, la.y}, qA);
+        const V2 rB = rotate(V2{la.z, la.w}, qB);
+        const V2 C = (V2{posB.x, posB.y} + rB) - (V2{posA.x, posA.y} + rA);
+        positionError = magsq(C);
+        const float exx = mA + (iA * (rA.y * rA.y) / 1.0f) + mB + (iB * (rB.y * rB.y) / 1.0f);
+        const float exy = (-iA * rA.x * rA.y / 1.0f) + (-iB * rB.x * rB.y / 1.0f);
+        const float eyy = mA + (iA * (rA.x * rA.x) / 1.0f) + mB + (iB * (rB.x * rB.x) / 1.0f);
+        V2 sol{0.0f, 0.0f};
+        const float cp = cross(V2{exx, exy}, V2{exy, eyy});
+        if (!almost_zero(cp)) {
+            const float inverse = 1.0f / cp;
+            sol = V2{(eyy * C.x - exy * C.y) * inverse, (exx * C.y - exy * C.x) * inverse};
+        }
+        const V2 P = -sol;
+        posA.x = posA.x - mA * P.x;
+        posA.y = posA.y - mA * P.y;
+        posA.z = posA.z - iA * cross(rA, P) / 1.0f;
+        posB.x = posB.x + mB * P.x;
+        posB.y = posB.y + mB * P.y;
+        posB.z = posB.z + iB * cross(rB, P) / 1.0f;
+    }
+    return (positionError <= c.linearSlop * c.linearSlop) && (angularError <= c.angularSlop);
+}
+
+// Per-step joint set-up half of InitVelocity (RevoluteJointConf.cpp:127-197),
+// writes the joint's solver record.
+__device__ __forceinline__ void build_jc(const DeviceState& d, const WorldCtx& x, int jg, float4* u, const BodyC& A,
+                                         const float4& pA, const BodyC& B, const float4& pB, int bA, int bB,
+                                         const StepParams& c) {
+    const float4 an = d.j_anchor[jg], cf = d.j_cfg[jg], cf2 = d.j_cfg2[jg];
+    float4 im = d.j_imp[jg];
+    std::uint32_t limit = d.j_limit[jg];
+    const std::uint32_t flags = __float_as_uint(cf2.y);
+    const V2 laA = V2{an.x, an.y} - A.lc;
+    const V2 laB = V2{an.z, an.w} - B.lc;
+    const V2 qA = unit_from_angle(pA.z);
+    const V2 qB = unit_from_angle(pB.z);
+    const V2 rA = rotate(laA, qA);
+    const V2 rB = rotate(laB, qB);
+    const float mA = A.invMass, iA = A.invRotI, mB = B.invMass, iB = B.invRotI;
+    const float totInvI = iA + iB;
+    const bool fixedRotation = (totInvI == 0.0f);
+    // GetMat33 (RevoluteJointConf.cpp:36-62).
+    const float exx = mA + ((rA.y * rA.y) * iA / 1.0f) + mB + ((rB.y * rB.y) * iB / 1.0f);
+    const float eyx = (-rA.y * rA.x * iA / 1.0f) + (-rB.y * rB.x * iB / 1.0f);
+    const float ezx = (-rA.y * iA) + (-rB.y * iB);
+    const float eyy = mA + ((rA.x * rA.x) * iA / 1.0f) + mB + ((rB.x * rB.x) * iB / 1.0f);
+    const float ezy = (rA.x * iA) + (rB.x * iB);
+    const float angularMass = (totInvI > 0.0f) ? (1.0f / totInvI) : 0.0f;
+    if (!(flags & prx::kJointMotor) || fixedRotation) im.w = 0.0f;
+    if ((flags & prx::kJointLimit) && !fixedRotation) {
+        const float jointAngle = pB.z - pA.z - cf.x;
+        if (__builtin_fabsf(cf.z - cf.y) < (c.angularSlop * 2.0f)) {
+            limit = 3u;
+        } else if (jointAngle <= cf.y) {
+            if (limit != 1u) {
+                limit = 1u;
+                im.z = 0.0f;
+            }
+        } else if (jointAngle >= cf.z) {
+            if (limit != 2u) {
+                limit = 2u;
+                im.z = 0.0f;
+            }
+        } else {
+            limit = 0u;
+            im.z = 0.0f;
+        }
+    } else {
+        limit = 0u;
+    }
+    if (c.warmStart) {
+        im.x = im.x * c.dtRatio;
+        im.y = im.y * c.dtRatio;
+        im.z = im.z * c.dtRatio;
+        im.w = im.w * c.dtRatio;
+    } else {
+        im = float4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    u[0] = float4{rA.x, rA.y, rB.x, rB.y};
+    u[1] = float4{exx, eyx, ezx, eyy};
+    u[2] = float4{ezy, totInvI, angularMass, __uint_as_float(flags | (limit << 8))};
+    u[3] = im;
+    u[4] = float4{mA, iA, mB, iB};
+    u[5] = float4{cf.w, c.dt * cf2.x, __int_as_float(bA), __int_as_float(bB)};
+    u[6] = float4{laA.x, laA.y, laB.x, laB.y};
+    u[7] = float4{cf.x, cf.y, cf.z, 0.0f};
+}
+
 __device__ __forceinline__ BodyC load_bodyc(const DeviceState& d, int gb) {
     const float4 l = d.lcinv[gb];
     return BodyC{l.z, l.w, V2{l.x, l.y}};
@@ -1039,7 +1343,9 @@ struct IslandScratch {
     int2* csr;        // per-body touching contacts: contact index, other body
     int* ibody;       // island body lists
     int* icont;       // island contact lists (contact indices)
+    int* ijoint;      // island joint lists (joint indices)
     int2* istart;     // island i: first body entry, first contact entry
+    int* jstart;      // island i: first joint entry
     int* bisland;     // island of each body (-1: none)
     std::uint32_t* isl_word;  // per island: bit 0 solved, bits 1..: unused; min under-active time bits
     std::uint32_t* isl_under;
@@ -1102,6 +1408,7 @@ __device__ __forceinline__ int find_islands(const DeviceState& d, const WorldCtx
     int2* csr = S.csr;
     std::uint8_t* islb;
     std::uint8_t* islc;
+    std::uint8_t* islj;
     if constexpr (kMode == 2) {
         int* loff = reinterpret_cast<int*>(g_smem);
         int2* lcsr = reinterpret_cast<int2*>(g_smem) + ((x.nb + 2) >> 1);
@@ -1111,18 +1418,22 @@ __device__ __forceinline__ int find_islands(const DeviceState& d, const WorldCtx
         csr = lcsr;
         islb = reinterpret_cast<std::uint8_t*>(lcsr + nent);
         islc = islb + x.nb;
+        islj = islc + nc;
     } else if constexpr (kMode == 1) {
         islb = reinterpret_cast<std::uint8_t*>(g_smem);
         islc = islb + x.nb;
+        islj = islc + nc;
     } else {
         islb = d.s_isl_b + x.bo;
         islc = d.s_isl_c + x.co;
+        islj = d.s_isl_j + x.jo;
     }
     for (int b = x.lane; b < x.nb; b += kWave) {
         islb[b] = (d.bflags[x.bo + b] & prx::kSpeedable) ? 0 : 2;
         S.bisland[b] = -1;
     }
     for (int i = x.lane; i < nc; i += kWave) islc[i] = 0;
+    for (int i = x.lane; i < x.nj; i += kWave) islj[i] = 0;
     __syncthreads();
     // One-body islands.
     int nsingle = 0;
@@ -1131,13 +1442,15 @@ __device__ __forceinline__ int find_islands(const DeviceState& d, const WorldCtx
         bool single = false;
         if (b < x.nb) {
             const std::uint32_t f = d.bflags[x.bo + b];
-            single = (f & prx::kAwake) && (f & prx::kEnabled) && (f & prx::kSpeedable) && off[b] == off[b + 1];
+            single = (f & prx::kAwake) && (f & prx::kEnabled) && (f & prx::kSpeedable) && off[b] == off[b + 1] &&
+                     (x.nj == 0 || d.bjoint[x.bo + b].y == 0);
         }
         const unsigned long long m = __ballot(single);
         if (single) {
             const int r = nsingle + __popcll(m & lanes_below(x.lane));
             S.ibody[r] = b;
             S.istart[r] = int2{r, 0};
+            S.jstart[r] = 0;
             S.bisland[b] = r;
             islb[b] = 1;
         }
@@ -1145,14 +1458,14 @@ __device__ __forceinline__ int find_islands(const DeviceState& d, const WorldCtx
     }
     __syncthreads();
     if (x.lane == 0) {
-        int nisl = nsingle, nbe = nsingle, nce = 0;
+        int nisl = nsingle, nbe = nsingle, nce = 0, nje = 0;
         int* stack = d.s_stack + 2 * x.bo;
         for (int seed = 0; seed < x.nb; ++seed) {
             if (islb[seed]) continue;
-            if (off[seed] == off[seed + 1]) continue;
             const std::uint32_t sf = d.bflags[x.bo + seed];
             if (!((sf & prx::kAwake) && (sf & prx::kEnabled))) continue;
             S.istart[nisl] = int2{nbe, nce};
+            S.jstart[nisl] = nje;
             int sp = 0;
             stack[sp++] = seed;
             islb[seed] = 1;
@@ -1171,10 +1484,26 @@ __device__ __forceinline__ int find_islands(const DeviceState& d, const WorldCtx
                         stack[sp++] = en.y;
                     }
                 }
+                // AddJointsToIsland (AabbTreeWorld.cpp:1446-1470).
+                if (x.nj) {
+                    const int2 bj = d.bjoint[x.bo + bb];
+                    for (int k = 0; k < bj.y; ++k) {
+                        const int2 je = d.bjoint_list[2 * x.jo + bj.x + k];
+                        if (islj[je.x]) continue;
+                        if (!(d.bflags[x.bo + je.y] & prx::kEnabled)) continue;
+                        islj[je.x] = 1;
+                        S.ijoint[nje++] = je.x;
+                        if (islb[je.y] == 0) {
+                            islb[je.y] = 1;
+                            stack[sp++] = je.y;
+                        }
+                    }
+                }
             }
             ++nisl;
         }
         S.istart[nisl] = int2{nbe, nce};
+        S.jstart[nisl] = nje;
         s_cnt[0] = nisl;
     }
     __syncthreads();
@@ -1216,66 +1545,150 @@ __device__ __forceinline__ Pc load_pc(const float4* u) {
     return pc;
 }
 
-// Lane-serial Gauss-Seidel velocity sweeps of one island (warm start first),
-// ContactSolver.cpp:481-492 per constraint in island order.  Records are
-// consecutive; the next one is loaded while the current one is solved, so the
-// only dependence left between constraints is through the bodies' velocities
-// (in LDS).
+// Gauss-Seidel sweeps of one island.  Two forms with identical arithmetic:
+//
+// * lane form (island_velocity / island_position): one lane walks one
+//   island's records, the next record loaded while the current one is
+//   solved; many small islands proceed side by side on the wave's lanes;
+// * wave form (island_velocity_wave / island_position_wave): the whole wave
+//   serves one large island.  Records are staged through LDS in chunks of
+//   kChunk; all lanes load the next chunk from global memory (coalesced) while
+//   lane 0 solves the current chunk out of LDS, so the serial chain waits on
+//   LDS, not on global memory.
+//
+// Velocity-record impulses are written back to global memory chunk by chunk
+// (every later read of a record comes after its write-back in the wave's
+// program order).
+
+// Warm start of one velocity constraint (WarmStartVelocities,
+// AabbTreeWorld.cpp:230-274).
+template <bool kLds>
+__device__ __forceinline__ void warm_start_one(const Vc& vc, const BodyC& A, const BodyC& B, const Bodies<kLds>& bs) {
+    const V2 tangent = fwdperp(vc.normal);
+    V2 dAl{0, 0}, dBl{0, 0};
+    float dAa = 0.0f, dBa = 0.0f;
+#pragma unroll
+    for (std::uint32_t j = 0; j < 2; ++j) {
+        if (j >= vc.n) continue;
+        const VcPoint& p = vc.p[j];
+        const V2 P = mul(p.ni, vc.normal) + mul(p.ti, tangent);
+        const float LA = cross(p.relA, P) / 1.0f;
+        const float LB = cross(p.relB, P) / 1.0f;
+        dAl = dAl - mul(A.invMass, P);
+        dAa = dAa - A.invRotI * LA;
+        dBl = dBl + mul(B.invMass, P);
+        dBa = dBa + B.invRotI * LB;
+    }
+    float4 va = bs.vel(vc.bA);
+    va.x = va.x + dAl.x;
+    va.y = va.y + dAl.y;
+    va.z = va.z + dAa;
+    bs.set_vel(vc.bA, va);
+    float4 vb = bs.vel(vc.bB);
+    vb.x = vb.x + dBl.x;
+    vb.y = vb.y + dBl.y;
+    vb.z = vb.z + dBa;
+    bs.set_vel(vc.bB, vb);
+}
+
+// GaussSeidel::SolveVelocityConstraint (ContactSolver.cpp:481-492): tangent
+// first, then normal (block solve for two points).  Returns the largest
+// impulse increment.
+template <bool kLds>
+__device__ __forceinline__ float solve_velocity_one(Vc& vc, const BodyC& A, const BodyC& B, const Bodies<kLds>& bs) {
+    float4 va = bs.vel(vc.bA);
+    float4 vb = bs.vel(vc.bB);
+    float mi = 0.0f;
+    mi = fmax_std(mi, solve_dir(vc, true, A, B, va, vb));
+    float nrm;
+    if (vc.n == 1 || (vc.K00 == 0.0f && vc.K11 == 0.0f && vc.K01 == 0.0f)) nrm = solve_dir(vc, false, A, B, va, vb);
+    else nrm = block_solve(vc, A, B, va, vb);
+    mi = fmax_std(mi, nrm);
+    bs.set_vel(vc.bA, va);
+    bs.set_vel(vc.bB, vb);
+    return mi;
+}
+
+// One pass over an island's joints in island order: the warm start of
+// InitVelocity, or SolveVelocity (returns whether every joint left its
+// bodies' velocities unchanged).
+template <bool kLds>
+__device__ __forceinline__ bool joints_velocity(float4* jbase, int j0, int j1, const Bodies<kLds>& bs, bool warm) {
+    bool ok = true;
+    for (int e = j0; e < j1; ++e) {
+        float4* u = jbase + std::size_t(e) * prx::kJcStride;
+        Jc j = load_jc_velocity(u);
+        float4 va = bs.vel(j.bA);
+        float4 vb = bs.vel(j.bB);
+        if (warm) {
+            joint_warm_start(j, va, vb);
+        } else {
+            ok &= joint_solve_velocity(j, va, vb);
+            u[3] = float4{j.ix, j.iy, j.iz, j.motorImp};
+        }
+        bs.set_vel(j.bA, va);
+        bs.set_vel(j.bB, vb);
+    }
+    return ok;
+}
+
+// One SolvePosition pass over an island's joints; returns whether all are
+// within tolerance.
+template <bool kLds>
+__device__ __forceinline__ bool joints_position(const float4* jbase, int j0, int j1, const Bodies<kLds>& bs,
+                                                const StepParams& c) {
+    bool ok = true;
+    for (int e = j0; e < j1; ++e) {
+        const float4* u = jbase + std::size_t(e) * prx::kJcStride;
+        const float4 mo = u[5];
+        const int bA = __float_as_int(mo.z), bB = __float_as_int(mo.w);
+        float4 pa = bs.pos(bA);
+        float4 pb = bs.pos(bB);
+        ok &= joint_solve_position(u, pa, pb, c);
+        bs.set_pos(bA, pa);
+        bs.set_pos(bB, pb);
+    }
+    return ok;
+}
+
+// Lane form of the warm start and velocity sweeps of one island
+// (AabbTreeWorld.cpp:1590-1625): contact warm start, joint warm start, then
+// per iteration the joints followed by the contacts.
 template <bool kLds>
 __device__ __forceinline__ void island_velocity(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
-                                                int c0, int c1, const StepParams& c) {
-    if (c0 >= c1) return;
-    const float4* base = d.s_vc + std::size_t(x.co) * prx::kVcStride;
-    if (c.warmStart) {
-        for (int e = c0; e < c1; ++e) {
-            BodyC A, B;
-            const Vc vc = load_vc(base + std::size_t(e) * prx::kVcStride, A, B);
-            const V2 tangent = fwdperp(vc.normal);
-            V2 dAl{0, 0}, dBl{0, 0};
-            float dAa = 0.0f, dBa = 0.0f;
-#pragma unroll
-            for (std::uint32_t j = 0; j < 2; ++j) {
-                if (j >= vc.n) continue;
-                const VcPoint& p = vc.p[j];
-                const V2 P = mul(p.ni, vc.normal) + mul(p.ti, tangent);
-                const float LA = cross(p.relA, P) / 1.0f;
-                const float LB = cross(p.relB, P) / 1.0f;
-                dAl = dAl - mul(A.invMass, P);
-                dAa = dAa - A.invRotI * LA;
-                dBl = dBl + mul(B.invMass, P);
-                dBa = dBa + B.invRotI * LB;
-            }
-            float4 va = bs.vel(vc.bA);
-            va.x = va.x + dAl.x;
-            va.y = va.y + dAl.y;
-            va.z = va.z + dAa;
-            bs.set_vel(vc.bA, va);
-            float4 vb = bs.vel(vc.bB);
-            vb.x = vb.x + dBl.x;
-            vb.y = vb.y + dBl.y;
-            vb.z = vb.z + dBa;
-            bs.set_vel(vc.bB, vb);
-        }
-    }
-    for (std::uint32_t it = 0; it < c.velIters; ++it) {
-        float newInc = 0.0f;
+                                                int c0, int c1, int j0, int j1, const StepParams& c) {
+    if (c0 >= c1 && j0 >= j1) return;
+    float4* base = d.s_vc + std::size_t(x.co) * prx::kVcStride;
+    float4* jbase = d.s_jc + std::size_t(x.jo) * prx::kJcStride;
+    if (c.warmStart && c0 < c1) {
         BodyC A, B;
         Vc vc = load_vc(base + std::size_t(c0) * prx::kVcStride, A, B);
         for (int e = c0; e < c1; ++e) {
             const int en = (e + 1 < c1) ? e + 1 : e;
             BodyC nA, nB;
             const Vc nvc = load_vc(base + std::size_t(en) * prx::kVcStride, nA, nB);
-            float4 va = bs.vel(vc.bA);
-            float4 vb = bs.vel(vc.bB);
-            float mi = 0.0f;
-            mi = fmax_std(mi, solve_dir(vc, true, A, B, va, vb));
-            float nrm;
-            if (vc.n == 1 || (vc.K00 == 0.0f && vc.K11 == 0.0f && vc.K01 == 0.0f)) nrm = solve_dir(vc, false, A, B, va, vb);
-            else nrm = block_solve(vc, A, B, va, vb);
-            mi = fmax_std(mi, nrm);
-            bs.set_vel(vc.bA, va);
-            bs.set_vel(vc.bB, vb);
-            float4* rec = d.s_vc + (std::size_t(x.co) + std::size_t(e)) * prx::kVcStride;
+            warm_start_one(vc, A, B, bs);
+            vc = nvc;
+            A = nA;
+            B = nB;
+        }
+    }
+    if (c.warmStart) joints_velocity(jbase, j0, j1, bs, true);
+    for (std::uint32_t it = 0; it < c.velIters; ++it) {
+        const bool jointsOkay = joints_velocity(jbase, j0, j1, bs, false);
+        float newInc = 0.0f;
+        if (c0 >= c1) {
+            if (jointsOkay && newInc <= c.regMinMomentum) break;
+            continue;
+        }
+        BodyC A, B;
+        Vc vc = load_vc(base + std::size_t(c0) * prx::kVcStride, A, B);
+        for (int e = c0; e < c1; ++e) {
+            const int en = (e + 1 < c1) ? e + 1 : e;
+            BodyC nA, nB;
+            const Vc nvc = load_vc(base + std::size_t(en) * prx::kVcStride, nA, nB);
+            const float mi = solve_velocity_one(vc, A, B, bs);
+            float4* rec = base + std::size_t(e) * prx::kVcStride;
             rec[4] = float4{vc.p[0].ni, vc.p[0].ti, vc.p[0].nm, vc.p[0].tm};
             rec[7] = float4{vc.p[1].ni, vc.p[1].ti, vc.p[1].nm, vc.p[1].tm};
             newInc = fmax_std(newInc, mi);
@@ -1283,16 +1696,121 @@ __device__ __forceinline__ void island_velocity(const DeviceState& d, const Worl
             A = nA;
             B = nB;
         }
-        if (newInc <= c.regMinMomentum) break;
+        if (jointsOkay && newInc <= c.regMinMomentum) break;
     }
 }
 
-// Lane-serial position sweeps of one island (ContactSolver.cpp:494-643,
+// Chunk staging of kStride-float4 records [c0, c1) of a global array through
+// registers into the LDS stage: fetch issues the loads, commit stores them.
+template <int kStride>
+struct Stager {
+    static constexpr int kFloats = kChunk * kStride;
+    static constexpr int kPer = (kFloats + kWave - 1) / kWave;
+    const float4* src;  // first record of the island
+    float4* stage;
+    int n;              // records in the island
+    int lane;
+    float4 reg[kPer];
+    __device__ __forceinline__ int limit(int ch) const {
+        const int e = (ch + 1) * kChunk;
+        return ((e < n) ? e : n) * kStride - ch * kFloats;
+    }
+    __device__ __forceinline__ void fetch(int ch) {
+        const int lim = limit(ch);
+        const float4* s = src + std::size_t(ch) * kFloats;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int f = lane + k * kWave;
+            reg[k] = (f < lim) ? s[f] : float4{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+    }
+    __device__ __forceinline__ void commit(int ch) {
+        const int lim = limit(ch);
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int f = lane + k * kWave;
+            if (f < lim) stage[f] = reg[k];
+        }
+    }
+};
+
+template <bool kLds>
+__device__ __forceinline__ void island_velocity_wave(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
+                                                     float4* stage, int c0, int c1, int j0, int j1,
+                                                     const StepParams& c) {
+    float4* jbase = d.s_jc + std::size_t(x.jo) * prx::kJcStride;
+    float4* gbase = d.s_vc + (std::size_t(x.co) + std::size_t(c0)) * prx::kVcStride;
+    const int n = c1 - c0;
+    const int nch = (n + kChunk - 1) / kChunk;
+    const bool resident = nch == 1;  // the whole island stays in the stage
+    Stager<prx::kVcStride> sg{gbase, stage, n, x.lane, {}};
+    sg.fetch(0);
+    sg.commit(0);
+    __syncthreads();
+    const int npass = int(c.velIters) + (c.warmStart ? 1 : 0);
+    for (int pass = 0; pass < npass; ++pass) {
+        const bool warm = c.warmStart && pass == 0;
+        float newInc = 0.0f;
+        bool jointsOkay = true;
+        if (!warm && x.lane == 0) jointsOkay = joints_velocity(jbase, j0, j1, bs, false);
+        for (int ch = 0; ch < nch; ++ch) {
+            const int nx = (ch + 1 < nch) ? ch + 1 : 0;
+            const bool more = !resident && (ch + 1 < nch || pass + 1 < npass);
+            if (more) sg.fetch(nx);
+            const int r1 = ((ch + 1) * kChunk < n ? kChunk : n - ch * kChunk);
+            if (x.lane == 0) {
+                BodyC A, B;
+                Vc vc = load_vc(stage, A, B);
+                for (int r = 0; r < r1; ++r) {
+                    const int rn = (r + 1 < r1) ? r + 1 : r;
+                    BodyC nA, nB;
+                    const Vc nvc = load_vc(stage + rn * prx::kVcStride, nA, nB);
+                    if (warm) {
+                        warm_start_one(vc, A, B, bs);
+                    } else {
+                        const float mi = solve_velocity_one(vc, A, B, bs);
+                        float4* rec = stage + r * prx::kVcStride;
+                        rec[4] = float4{vc.p[0].ni, vc.p[0].ti, vc.p[0].nm, vc.p[0].tm};
+                        rec[7] = float4{vc.p[1].ni, vc.p[1].ti, vc.p[1].nm, vc.p[1].tm};
+                        newInc = fmax_std(newInc, mi);
+                    }
+                    vc = nvc;
+                    A = nA;
+                    B = nB;
+                }
+            }
+            __syncthreads();
+            if (!warm && !resident && x.lane < 2 * r1) {
+                const int r = x.lane >> 1;
+                const int f = r * prx::kVcStride + ((x.lane & 1) ? 7 : 4);
+                gbase[std::size_t(ch) * kChunk * prx::kVcStride + f] = stage[f];
+            }
+            if (more) {
+                __syncthreads();
+                sg.commit(nx);
+                __syncthreads();
+            }
+        }
+        if (warm && x.lane == 0) joints_velocity(jbase, j0, j1, bs, true);
+        newInc = __shfl(newInc, 0);
+        jointsOkay = __shfl(int(jointsOkay), 0) != 0;
+        if (!warm && jointsOkay && newInc <= c.regMinMomentum) break;
+    }
+    if (resident && x.lane < 2 * n) {
+        const int r = x.lane >> 1;
+        const int f = r * prx::kVcStride + ((x.lane & 1) ? 7 : 4);
+        gbase[f] = stage[f];
+    }
+    __syncthreads();
+}
+
+// Lane form of the position sweeps of one island (ContactSolver.cpp:494-643,
 // AabbTreeWorld.cpp:1627-1644).  Returns whether the island was solved.
 template <bool kLds>
 __device__ __forceinline__ bool island_position(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
-                                                int c0, int c1, const StepParams& c) {
+                                                int c0, int c1, int j0, int j1, const StepParams& c) {
     const float4* base = d.s_pc + std::size_t(x.co) * prx::kPcStride;
+    const float4* jbase = d.s_jc + std::size_t(x.jo) * prx::kJcStride;
     for (std::uint32_t it = 0; it < c.posIters; ++it) {
         float minSep = __builtin_inff();
         if (c0 < c1) {
@@ -1309,9 +1827,66 @@ __device__ __forceinline__ bool island_position(const DeviceState& d, const Worl
                 pc = npc;
             }
         }
-        if (minSep >= c.regMinSeparation) return true;
+        const bool contactsOkay = minSep >= c.regMinSeparation;
+        const bool jointsOkay = joints_position(jbase, j0, j1, bs, c);
+        if (contactsOkay && jointsOkay) return true;
     }
     return false;
+}
+
+template <bool kLds>
+__device__ __forceinline__ bool island_position_wave(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
+                                                     float4* stage, int c0, int c1, int j0, int j1,
+                                                     const StepParams& c) {
+    const float4* jbase = d.s_jc + std::size_t(x.jo) * prx::kJcStride;
+    const float4* gbase = d.s_pc + (std::size_t(x.co) + std::size_t(c0)) * prx::kPcStride;
+    const int n = c1 - c0;
+    const int nch = (n + kChunk - 1) / kChunk;
+    const bool resident = nch == 1;
+    Stager<prx::kPcStride> sg{gbase, stage, n, x.lane, {}};
+    sg.fetch(0);
+    sg.commit(0);
+    __syncthreads();
+    const int npass = int(c.posIters);
+    bool solved = false;
+    for (int pass = 0; pass < npass; ++pass) {
+        float minSep = __builtin_inff();
+        for (int ch = 0; ch < nch; ++ch) {
+            const int nx = (ch + 1 < nch) ? ch + 1 : 0;
+            const bool more = !resident && (ch + 1 < nch || pass + 1 < npass);
+            if (more) sg.fetch(nx);
+            const int r1 = ((ch + 1) * kChunk < n ? kChunk : n - ch * kChunk);
+            if (x.lane == 0) {
+                Pc pc = load_pc(stage);
+                for (int r = 0; r < r1; ++r) {
+                    const int rn = (r + 1 < r1) ? r + 1 : r;
+                    const Pc npc = load_pc(stage + rn * prx::kPcStride);
+                    float4 pa = bs.pos(pc.bA);
+                    float4 pb = bs.pos(pc.bB);
+                    const float sep = solve_pc(pc.m, pc.totalRadius, pc.A, pc.B, pa, pb, c);
+                    bs.set_pos(pc.bA, pa);
+                    bs.set_pos(pc.bB, pb);
+                    minSep = fmin_std(minSep, sep);
+                    pc = npc;
+                }
+            }
+            __syncthreads();
+            if (more) {
+                sg.commit(nx);
+                __syncthreads();
+            }
+        }
+        bool jointsOkay = true;
+        if (x.lane == 0) jointsOkay = joints_position(jbase, j0, j1, bs, c);
+        minSep = __shfl(minSep, 0);
+        jointsOkay = __shfl(int(jointsOkay), 0) != 0;
+        if (minSep >= c.regMinSeparation && jointsOkay) {
+            solved = true;
+            break;
+        }
+    }
+    __syncthreads();
+    return solved;
 }
 
 template <bool kLds>
@@ -1361,14 +1936,38 @@ __device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, con
         store_vc(d.s_vc + slot * prx::kVcStride, vc, A, B);
         store_pc(d.s_pc + slot * prx::kPcStride, m, bd.x, bd.y, rA + rB, A, B);
     }
-    __syncthreads();
-    T.mark(prx::kPhConstraints);
-    // Warm start and velocity sweeps: one lane per island.
-    for (int is = x.lane; is < nisl; is += kWave) {
-        const int2 s0 = S.istart[is], s1 = S.istart[is + 1];
-        island_velocity<kLds>(d, x, bs, s0.y, s1.y, c);
+    // Joint set-up (the non-warm-start half of InitVelocity), island order.
+    const int nje = S.jstart[nisl];
+    for (int e = x.lane; e < nje; e += kWave) {
+        const int jg = x.jo + S.ijoint[e];
+        const int2 bd = d.j_body[jg];
+        const BodyC A = load_bodyc(d, bo + bd.x), B = load_bodyc(d, bo + bd.y);
+        build_jc(d, x, jg, d.s_jc + (std::size_t(x.jo) + std::size_t(e)) * prx::kJcStride, A, bs.pos(bd.x), B,
+                 bs.pos(bd.y), bd.x, bd.y, c);
     }
     __syncthreads();
+    T.mark(prx::kPhConstraints);
+    // Warm start and velocity sweeps: islands below coopMin contacts one lane
+    // each, larger ones by the whole wave, one after another.
+    float4* stage = g_smem + (int(c.ldsBytes) / 16 - prx::kStageFloat4);
+    const int coopMin = c.coopMin > 0 ? int(c.coopMin) : 1;
+    auto constraints = [&](int is) {
+        return (S.istart[is + 1].y - S.istart[is].y) + (S.jstart[is + 1] - S.jstart[is]);
+    };
+    for (int is = x.lane; is < nisl; is += kWave) {
+        if (constraints(is) < coopMin)
+            island_velocity<kLds>(d, x, bs, S.istart[is].y, S.istart[is + 1].y, S.jstart[is], S.jstart[is + 1], c);
+    }
+    __syncthreads();
+    for (int base = 0; base < nisl; base += kWave) {
+        const int is = base + x.lane;
+        const bool big = is < nisl && constraints(is) >= coopMin;
+        for (unsigned long long m = __ballot(big); m; m &= m - 1) {
+            const int isl = base + __ffsll(m) - 1;
+            island_velocity_wave<kLds>(d, x, bs, stage, S.istart[isl].y, S.istart[isl + 1].y, S.jstart[isl],
+                                       S.jstart[isl + 1], c);
+        }
+    }
     T.mark(prx::kPhVelocity);
     // IntegratePositions (AabbTreeWorld.cpp:176-188).
     for (int e = x.lane; e < nbe; e += kWave) {
@@ -1382,10 +1981,22 @@ __device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, con
         bs.set_pos(b, p);
     }
     __syncthreads();
-    // Position sweeps: one lane per island.
+    // Position sweeps, split like the velocity sweeps.
     for (int is = x.lane; is < nisl; is += kWave) {
-        const int2 s0 = S.istart[is], s1 = S.istart[is + 1];
-        if (island_position<kLds>(d, x, bs, s0.y, s1.y, c)) S.isl_word[is] = 1u;
+        if (constraints(is) < coopMin &&
+            island_position<kLds>(d, x, bs, S.istart[is].y, S.istart[is + 1].y, S.jstart[is], S.jstart[is + 1], c))
+            S.isl_word[is] = 1u;
+    }
+    __syncthreads();
+    for (int base = 0; base < nisl; base += kWave) {
+        const int is = base + x.lane;
+        const bool big = is < nisl && constraints(is) >= coopMin;
+        for (unsigned long long m = __ballot(big); m; m &= m - 1) {
+            const int isl = base + __ffsll(m) - 1;
+            const bool solved = island_position_wave<kLds>(d, x, bs, stage, S.istart[isl].y, S.istart[isl + 1].y,
+                                                           S.jstart[isl], S.jstart[isl + 1], c);
+            if (solved && x.lane == 0) S.isl_word[isl] = 1u;
+        }
     }
     __syncthreads();
     T.mark(prx::kPhPosition);
@@ -1406,6 +2017,13 @@ __device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, con
             m1.w = q1.y;
             d.m_p1[gi] = m1;
         }
+    }
+    // Joint state carried to the next step: impulses and limit state.
+    for (int e = x.lane; e < nje; e += kWave) {
+        const int jg = x.jo + S.ijoint[e];
+        const float4* u = d.s_jc + (std::size_t(x.jo) + std::size_t(e)) * prx::kJcStride;
+        d.j_imp[jg] = u[3];
+        d.j_limit[jg] = __float_as_uint(u[2].w) >> 8;
     }
     // Body writeback, under-active times (AabbTreeWorld.cpp:1652-1664, 403-417).
     for (int e = x.lane; e < nbe; e += kWave) {
@@ -1492,6 +2110,8 @@ __device__ __forceinline__ void step_world(const DeviceState& d, int w, const St
     x.np = H.nproxies;
     x.co = H.contact_off;
     x.cap = H.contact_cap;
+    x.jo = H.joint_off;
+    x.nj = H.njoints;
     x.lane = int(threadIdx.x);
     Counters k;
     k.nc = H.ncontacts;
@@ -1531,20 +2151,22 @@ __device__ __forceinline__ void step_world(const DeviceState& d, int w, const St
         S.ibody = d.s_isl_body + x.bo + x.co;
         S.icont = d.s_isl_contact + x.co;
         S.istart = d.s_isl_start + x.bo + w;
+        S.ijoint = d.s_isl_joint + x.jo;
+        S.jstart = d.s_isl_jstart + x.bo + w;
         S.bisland = d.s_stack + 2 * x.bo + x.nb;
         S.isl_word = d.s_keys + x.co;
         S.isl_under = d.s_keys2 + x.co;
         build_adjacency(d, x, k.nc, S);
         T.mark(prx::kPhAdjacency);
         const int nent = S.off[x.nb];
-        const int ldsAdj = 4 * (((x.nb + 2) >> 1) * 2) + 8 * nent + x.nb + k.nc;
+        const int ldsAdj = 4 * (((x.nb + 2) >> 1) * 2) + 8 * nent + x.nb + k.nc + x.nj;
         int nisl;
         if (ldsAdj <= int(c.ldsBytes)) nisl = find_islands<2>(d, x, k.nc, S);
-        else if (x.nb + k.nc <= int(c.ldsBytes)) nisl = find_islands<1>(d, x, k.nc, S);
+        else if (x.nb + k.nc + x.nj <= int(c.ldsBytes)) nisl = find_islands<1>(d, x, k.nc, S);
         else nisl = find_islands<0>(d, x, k.nc, S);
         stats[4] = std::uint32_t(nisl);
         T.mark(prx::kPhIslands);
-        const bool ldsBodies = x.nb * 32 <= int(c.ldsBytes);
+        const bool ldsBodies = x.nb * 32 + prx::kStageFloat4 * 16 <= int(c.ldsBytes);
         stats[7] = ldsBodies ? solve_islands<true>(d, x, nisl, S, c, T) : solve_islands<false>(d, x, nisl, S, c, T);
         // Synchronise proxies of islanded speedable bodies (AabbTreeWorld.cpp:1532-1544).
         std::uint32_t moved = 0;

@@ -3,8 +3,8 @@ and by the oracle restatement (same layout), plus canonical per-step digests.
 
 A record holds, after each step: bodies (pos0, pos1, velocity, under-active
 time, flags), the contact list in the world's contact order (keys, ids, flags,
-manifold with ContactFeatures and impulses) and the broadphase proxies (id and
-fat AABB).
+manifold with ContactFeatures and impulses), the broadphase proxies (id and
+fat AABB) and the joints (limit state, impulses).
 """
 import hashlib
 import struct
@@ -15,6 +15,8 @@ BODY_F = 10
 CONTACT_U = 4
 CONTACT_T = 3
 CONTACT_F = 14
+JOINT_DESC_F = 9
+JOINT_F = 4
 
 
 class Reader:
@@ -54,7 +56,12 @@ def read_description(r):
         ns = r.u32()
         sids = r.arr("<u4", ns)
         bodies.append(dict(type=btype, flags=flags, vals=vals, shapes=sids))
-    return shapes, bodies
+    joints = []
+    for _ in range(r.u32()):
+        ab = r.arr("<u4", 3)
+        vals = r.arr("<f4", JOINT_DESC_F)
+        joints.append(dict(a=int(ab[0]), b=int(ab[1]), flags=int(ab[2]), vals=vals))
+    return shapes, bodies, joints
 
 
 def read_conf(r):
@@ -83,22 +90,26 @@ def read_state(r):
     raw = r.arr("<u4", npx * 5).reshape(npx, 5)
     pid = raw[:, 0].copy()
     paabb = raw[:, 1:].copy().view("<f4")
+    nj = r.u32()
+    raw = r.arr("<u4", nj * (1 + JOINT_F)).reshape(nj, 1 + JOINT_F)
+    jlimit = raw[:, 0].copy()
+    jimp = raw[:, 1:].copy().view("<f4")
     return dict(step=step, stats=stats, bodies=bodies, bflags=bflags,
-                cu=cu, ct=ct, cf=cf, pid=pid, paabb=paabb)
+                cu=cu, ct=ct, cf=cf, pid=pid, paabb=paabb, jlimit=jlimit, jimp=jimp)
 
 
 def read_dump(buf):
     r = Reader(buf)
     assert bytes(r.b[0:4]) == b"PRXD", "bad magic"
     r.o = 4
-    assert r.u32() == 1
-    shapes, bodies = read_description(r)
+    assert r.u32() == 2
+    shapes, bodies, joints = read_description(r)
     conf = read_conf(r)
     states = [read_state(r) for _ in range(r.u32())]
-    return dict(shapes=shapes, bodies=bodies, conf=conf, states=states)
+    return dict(shapes=shapes, bodies=bodies, joints=joints, conf=conf, states=states)
 
 
-def digest(state, parts=("bodies", "contacts", "proxies")):
+def digest(state, parts=("bodies", "contacts", "proxies", "joints")):
     """SHA-256 over the canonical bytes of one state (selected parts)."""
     h = hashlib.sha256()
     if "bodies" in parts:
@@ -111,4 +122,7 @@ def digest(state, parts=("bodies", "contacts", "proxies")):
     if "proxies" in parts:
         h.update(np.ascontiguousarray(state["pid"]).tobytes())
         h.update(np.ascontiguousarray(state["paabb"]).view("<u4").tobytes())
+    if "joints" in parts:
+        h.update(np.ascontiguousarray(state["jlimit"]).tobytes())
+        h.update(np.ascontiguousarray(state["jimp"]).view("<u4").tobytes())
     return h.digest()

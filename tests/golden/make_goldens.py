@@ -33,7 +33,10 @@ CASES = [
     ("drop300", "drop", 200, [300, 20, 40, 7], [1, 100, 200]),
     ("billiards0", "billiards", 120, [1, 0], [1, 60, 120]),
     ("billiards1", "billiards", 120, [1, 1], [1, 120]),
-] + [("pile%d" % w, "pile", 150, [1, w], [1, 150]) for w in range(8)]
+] + [("pile%d" % w, "pile", 150, [1, w], [1, 150]) for w in range(8)] + [
+    ("chain0", "chain", 240, [1, 0], [1, 120, 240]),
+    ("chain1", "chain", 240, [1, 1, 12, 24, 40], [1, 240]),
+]
 
 
 def main():
@@ -44,16 +47,16 @@ def main():
             buf = oracle_lib.run_ref_driver(scene, steps, False, params, path)
             desc_conf, d = oracle_lib.split_dump(buf)
             states = d["states"]
-            dig = np.zeros((len(states), 3, 32), dtype=np.uint8)
+            dig = np.zeros((len(states), 4, 32), dtype=np.uint8)
             for i, s in enumerate(states):
-                for k, part in enumerate(("bodies", "contacts", "proxies")):
+                for k, part in enumerate(("bodies", "contacts", "proxies", "joints")):
                     dig[i, k] = np.frombuffer(dumpfmt.digest(s, (part,)), dtype=np.uint8)
             arrays = dict(desc_conf=np.frombuffer(desc_conf, dtype=np.uint8),
                           digests=dig, scene=np.array(scene), params=np.array(params, dtype=np.int32),
                           steps=np.array(steps), full_steps=np.array(full, dtype=np.int32))
             for st in full:
                 s = states[st]
-                for key in ("stats", "bodies", "bflags", "cu", "ct", "cf", "pid", "paabb"):
+                for key in ("stats", "bodies", "bflags", "cu", "ct", "cf", "pid", "paabb", "jlimit", "jimp"):
                     arrays["full%d_%s" % (st, key)] = s[key]
             out = os.path.join(HERE, name + ".npz")
             np.savez_compressed(out, **arrays)

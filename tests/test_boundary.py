@@ -27,7 +27,7 @@ def test_library_exports_every_declared_function():
 
 
 def test_abi_version():
-    assert pa.lib().prx_abi_version() == 1
+    assert pa.lib().prx_abi_version() == 2
 
 
 def _desc_only(desc_conf):

@@ -48,12 +48,16 @@ def test_plain_step_matches_reference_every_step(name):
                 pytest.fail("%s: step %d differs from the reference in %s%s" % (name, step, part, detail))
 
 
-def test_batched_step_matches_reference():
-    """All golden worlds in one batch, one launch per step."""
+@pytest.mark.parametrize("coop_min", [None, 1, 1 << 30])
+def test_batched_step_matches_reference(coop_min):
+    """All golden worlds in one batch, one launch per step.  coop_min 1 sweeps
+    every island with contacts by the whole wavefront (LDS-staged records),
+    1 << 30 sweeps every island on one lane; both equal the reference."""
     names = golden_cases.names()
     gs = [golden_cases.load(n) for n in names]
     worlds = [pa.World.from_desc(g["desc_conf"]) for g in gs]
-    batch = pa.Batch(worlds, use_graph=False)
+    print("coop_min", coop_min, flush=True)
+    batch = pa.Batch(worlds, use_graph=False, coop_min=coop_min)
     conf = pa.StepConf()
     steps = max(g["steps"] for g in gs)
     done = {}
@@ -88,8 +92,54 @@ def test_graph_replay_matches_eager():
             assert a.state_record() == b.state_record(), rep
 
 
+def test_autotune_restores_state_and_keeps_results():
+    """prx_batch_autotune times schedules from the current state, restores it,
+    and the chosen schedule steps to the same bits as the default one."""
+    descs = [golden_cases.load(n)["desc_conf"] for n in ("pile3", "chain0", "pyramid", "billiards0")]
+    a = [pa.World.from_desc(d) for d in descs]
+    b = [pa.World.from_desc(d) for d in descs]
+    ba = pa.Batch(a, use_graph=True)
+    bb = pa.Batch(b, use_graph=True)
+    conf = pa.StepConf()
+    ba.step(conf, 5)
+    bb.step(conf, 5)
+    ba.sync()
+    print("autotune", flush=True)
+    chosen = ba.autotune(conf, nsteps=3)
+    print("chosen", chosen, flush=True)
+    ba.download()
+    bb.download()
+    for x, y in zip(a, b):
+        assert x.state_record() == y.state_record()
+    ba.step(conf, 20)
+    bb.step(conf, 20)
+    ba.download()
+    bb.download()
+    for x, y in zip(a, b):
+        assert x.state_record() == y.state_record()
+
+
+@pytest.mark.parametrize("lds_bytes", [26 * 1024, 40 * 1024])
+def test_lds_schedule_matches_reference(lds_bytes):
+    """Padding the workgroup's LDS (fewer workgroups per CU) changes nothing in
+    the results."""
+    names = ["pile1", "pile3", "chain1", "drop300"]
+    gs = [golden_cases.load(n) for n in names]
+    worlds = [pa.World.from_desc(g["desc_conf"]) for g in gs]
+    batch = pa.Batch(worlds, use_graph=True, lds_bytes=lds_bytes)
+    conf = pa.StepConf()
+    steps = 100
+    batch.step(conf, steps)
+    batch.download()
+    for n, g, w in zip(names, gs, worlds):
+        s = _state(w)
+        for k, part in enumerate(golden_cases.PARTS):
+            assert dumpfmt.digest(s, (part,)) == g["digests"][steps, k].tobytes(), (n, part)
+
+
 @pytest.mark.parametrize("scene,params,steps", [
-    ("pile", [3, 11], 60), ("pile", [5, 40], 60), ("drop", [1000, 40, 80, 3], 40)])
+    ("pile", [3, 11], 60), ("pile", [5, 40], 60), ("drop", [1000, 40, 80, 3], 40),
+    ("chain", [3, 5, 32, 64, 5], 150)])
 def test_device_matches_oracle(scene, params, steps):
     w = pa.World.scene(scene, params)
     ow = oracle_lib.OracleWorld(w.save_desc() + _conf_bytes())
