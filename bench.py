@@ -43,6 +43,10 @@ def parse():
     p.add_argument("--lds-bytes", type=int, default=0, help="0: library default")
     p.add_argument("--autotune", type=int, default=10, metavar="K",
                    help="steps per candidate schedule for prx_batch_autotune after warmup (0: off)")
+    p.add_argument("--big-bodies", type=int, default=20000, help="large single world (0: skip)")
+    p.add_argument("--big-steps", type=int, default=600)
+    p.add_argument("--big-cpu-steps", type=int, default=30,
+                   help="leading steps of the large world timed on the CPU reference and the GPU alike")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only run a plain device copy of MIB MiB five times (counter calibration)")
     return p.parse_args()
@@ -100,6 +104,53 @@ def cpu_baseline(args, threads):
     return dict(value=bs / secs, unit="body-steps/s", cores=threads, kind="port",
                 sample="oracle restatement, worlds 0..%d, %d steps from step 1" % (n - 1, args.steps),
                 seconds=secs)
+
+
+def large_world(args, pa, conf):
+    """One 20,000-body drop world (disks and 3..8-gons into a 400 x 400 box),
+    graph-coloured mode: body-steps/s over the whole run, and over its first
+    steps next to the reference CPU build (one thread: PlayRho steps a world
+    on one thread) on the same steps."""
+    params = [args.big_bodies, 200, 400, args.seed]
+    w = pa.World.scene("drop", params)
+    dyn = sum(1 for f in w.bodies()[1] if f & 2)
+    big = pa.BigWorld(w)
+    k = min(args.big_cpu_steps, args.big_steps)
+    big.sync()
+    t0 = time.perf_counter()
+    big.step(conf, k)
+    big.sync()
+    t_first = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    big.step(conf, args.big_steps - k)
+    big.sync()
+    t_rest = time.perf_counter() - t1
+    wall = t_first + t_rest
+    met = big.metrics()
+    out = {
+        "workload": "one drop world: %d bodies (half disks r U[0.25,0.75], half 3..8-gons R U[0.3,0.8], "
+                    "friction U[0.2,0.8]) into a 400 x 400 static box of 4 edges, seed %d, dt=1/60 8/3 "
+                    "iterations doToi=false, LBVH broadphase, graph-coloured Gauss-Seidel"
+                    % (args.big_bodies, args.seed),
+        "steps": args.big_steps,
+        "value": dyn * args.big_steps / wall,
+        "unit": "body-steps/s",
+        "ms_per_step": 1e3 * wall / max(1, args.big_steps),
+        "first_steps": {"steps": k, "value": dyn * k / t_first if k else None},
+        "end_state": met,
+    }
+    ref = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    if k and not args.no_cpu and os.path.exists(ref):
+        r = subprocess.run([ref, "--time", "drop", "0", str(k), "0", "1", "1"] + [str(x) for x in params],
+                           check=True, capture_output=True, text=True).stdout
+        r = json.loads(r.strip().splitlines()[-1])
+        cpu = r["body_steps"] / r["seconds"]
+        out["cpu_baseline"] = {"value": cpu, "unit": "body-steps/s", "cores": 1, "kind": "reference",
+                               "sample": "reference float build, steps 1..%d of the same world, one thread "
+                                         "(the reference steps one world on one thread)" % k,
+                               "seconds": r["seconds"]}
+        out["ratio_vs_cpu_first_steps"] = out["first_steps"]["value"] / cpu
+    return out
 
 
 def conf_record(pa):
@@ -199,6 +250,9 @@ def main():
     if tr is not None:
         result["roofline"]["traffic"] = tr["bytes_per_launch"]
         result["roofline"]["traffic_note"] = tr["note"]
+
+    if args.big_bodies > 0 and args.big_steps > 0:
+        result["large_world"] = large_world(args, pa, conf)
 
     # Plain per-world form: one world per launch, synchronised, on a sample.
     if args.plain_worlds <= 0:

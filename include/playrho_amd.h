@@ -214,6 +214,32 @@ int prx_batch_phase_cycles(prx_batch* batch, int enable, uint64_t* out, int max_
 int prx_batch_autotune(prx_batch* batch, const prx_step_conf* step, int nsteps,
                        prx_batch_conf* best);
 
+/* One very large world on the whole device (graph-coloured mode).  No
+ * reference counterpart as an API: it is Step(World&, const StepConf&)
+ * (World.hpp:917-920) for one world too large for a single wavefront, with
+ * the DynamicTree broadphase (DynamicTree.cpp:673-700, AabbTreeWorld.cpp:
+ * 709-751) replaced by a linear BVH giving the same pair sets, and the
+ * island Gauss-Seidel solve (AabbTreeWorld.cpp:1563-1682) replaced by a
+ * colour-parallel Gauss-Seidel over all awake bodies (no sleeping).  The
+ * world's state moves to the device (its getters read it back). */
+typedef struct prx_big prx_big;
+int prx_big_create(prx_world* world, prx_big** out);
+void prx_big_destroy(prx_big* big);
+int prx_big_step(prx_big* big, const prx_step_conf* conf, int nsteps);
+int prx_big_sync(prx_big* big);
+int prx_big_last_timing(prx_big* big, float* ms, int* launches);
+/* out[0] smallest separation over touching contacts at the current poses
+ * (negative: penetration), out[1] touching contacts and out[2] colours of the
+ * last solve, out[3] contacts. */
+int prx_big_metrics(prx_big* big, float out[4]);
+/* Broadphase pair finding alone (FindContacts, AabbTreeWorld.cpp:709-751, with
+ * the LBVH): n proxies with fat AABBs (lo x, lo y, hi x, hi y), owning body
+ * and moved flag; writes up to cap pairs (lower index, higher index) of
+ * proxies of different bodies whose AABBs overlap, at least one moved;
+ * *count receives the number found. */
+int prx_lbvh_pairs(const float* aabbs, const int32_t* bodies, const uint8_t* moved, int n, int32_t* pairs,
+                   int cap, int* count);
+
 /* Diagnostic: UnitVec::Get(angle) = {cos, sin} (UnitVec.cpp:27-30) evaluated
  * by the device routine for n angles; out receives n (cos, sin) pairs.  Used
  * by the tests to compare the device rotation with the host C library. */

@@ -141,6 +141,15 @@ _SIGS = {
     "prx_batch_counts": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "prx_batch_phase_cycles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]),
     "prx_selftest_sincos": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
+    "prx_big_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "prx_big_destroy": (None, [ctypes.c_void_p]),
+    "prx_big_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
+    "prx_big_sync": (ctypes.c_int, [ctypes.c_void_p]),
+    "prx_big_last_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float),
+                                           ctypes.POINTER(ctypes.c_int)]),
+    "prx_big_metrics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "prx_lbvh_pairs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                      ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
 }
 
 
@@ -351,3 +360,55 @@ def device_sincos(angles):
     if code:
         raise PrxError(code, "device sincos self-test failed")
     return out
+
+
+class BigWorld:
+    """One large world stepped on the whole device (graph-coloured mode):
+    LBVH broadphase with the reference's pair sets, colour-parallel
+    Gauss-Seidel solve.  The World's getters read the device state back."""
+
+    def __init__(self, world):
+        self.world = world
+        self.h = ctypes.c_void_p()
+        _check(lib().prx_big_create(world.h, ctypes.byref(self.h)))
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and h.value and _lib is not None:
+            _lib.prx_big_destroy(h)
+            self.h = None
+
+    def step(self, conf=None, nsteps=1):
+        conf = conf or StepConf()
+        _check(lib().prx_big_step(self.h, ctypes.byref(conf), int(nsteps)))
+
+    def sync(self):
+        _check(lib().prx_big_sync(self.h))
+
+    def last_timing(self):
+        ms = ctypes.c_float()
+        n = ctypes.c_int()
+        _check(lib().prx_big_last_timing(self.h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def metrics(self):
+        """(min separation over touching contacts, touching contacts and
+        colours of the last solve, contacts)."""
+        out = np.zeros(4, dtype=np.float32)
+        _check(lib().prx_big_metrics(self.h, out.ctypes.data))
+        return dict(min_separation=float(out[0]), touching=int(out[1]), colors=int(out[2]), contacts=int(out[3]))
+
+
+def lbvh_pairs(aabbs, bodies, moved, cap=None):
+    """Broadphase pairs by the device LBVH: (k, 2) int32 array of (lower, higher)
+    proxy indices of different bodies with overlapping AABBs, one of them moved."""
+    aabbs = np.ascontiguousarray(aabbs, dtype=np.float32).reshape(-1, 4)
+    n = aabbs.shape[0]
+    bodies = np.ascontiguousarray(bodies, dtype=np.int32)
+    moved = np.ascontiguousarray(moved, dtype=np.uint8)
+    cap = cap if cap is not None else max(16 * n, 1024)
+    out = np.zeros((cap, 2), dtype=np.int32)
+    count = ctypes.c_int()
+    _check(lib().prx_lbvh_pairs(aabbs.ctypes.data, bodies.ctypes.data, moved.ctypes.data, n,
+                                out.ctypes.data, cap, ctypes.byref(count)))
+    return out[:min(count.value, cap)].copy()

@@ -23,12 +23,17 @@
 #include <vector>
 
 #include "../../include/playrho_amd.h"
+#include "big_world.hpp"
 #include "layout.hpp"
 #include "scenes.hpp"
 #include "sincosf.hpp"
 
 extern "C" hipError_t prx_launch_step(const prx::DeviceState* d, const prx::StepParams* c, int first, int count,
                                       hipStream_t stream);
+extern "C" hipError_t prx_big_launch_step(const prx::DeviceState* d, const prx::BigState* g,
+                                          const prx::StepParams* c, hipStream_t s);
+extern "C" hipError_t prx_big_launch_pairs(const prx::DeviceState* d, const prx::BigState* g, hipStream_t s);
+extern "C" hipError_t prx_big_launch_metrics(const prx::DeviceState* d, const prx::BigState* g, hipStream_t s);
 
 namespace {
 
@@ -1682,6 +1687,210 @@ int prx_batch_autotune(prx_batch* b, const prx_step_conf* step, int nsteps, prx_
     for (prx_world* w : b->worlds) w->host_stale = false;
     b->last_launches = 0;
     if (best) *best = cur;
+    return PRX_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Single large world (graph-coloured mode) and the LBVH pair finder.
+
+}  // extern "C"
+
+namespace {
+
+std::size_t NextPow2(std::size_t v) {
+    std::size_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+int AllocBig(DeviceArena& a, prx::BigState& g, int P, int B, int cap) {
+    g.P = std::max(P, 1);
+    g.B = std::max(B, 1);
+    g.cap = std::max(cap, 1);
+    g.pairCap = std::max(16 * g.P, 1024);
+    g.hashMask = int(NextPow2(std::size_t(2 * g.cap))) - 1;
+    const std::size_t n = std::size_t(g.P), nodes = 2 * n;
+    PRX_HIP(a.alloc(g.morton, n));
+    PRX_HIP(a.alloc(g.morton2, n));
+    PRX_HIP(a.alloc(g.order, n));
+    PRX_HIP(a.alloc(g.order2, n));
+    PRX_HIP(a.alloc(g.left, n));
+    PRX_HIP(a.alloc(g.right, n));
+    PRX_HIP(a.alloc(g.parent, nodes));
+    PRX_HIP(a.alloc(g.escape, nodes));
+    PRX_HIP(a.alloc(g.visit, n));
+    PRX_HIP(a.alloc(g.naabb, nodes));
+    PRX_HIP(a.alloc(g.bounds, 1));
+    PRX_HIP(a.alloc(g.pcount, n + 1));
+    PRX_HIP(a.alloc(g.pairs, std::size_t(g.pairCap)));
+    PRX_HIP(a.alloc(g.paccept, std::size_t(g.pairCap)));
+    PRX_HIP(a.alloc(g.hash, std::size_t(g.hashMask) + 1));
+    PRX_HIP(a.alloc(g.list, std::size_t(g.cap)));
+    PRX_HIP(a.alloc(g.list2, std::size_t(g.cap)));
+    PRX_HIP(a.alloc(g.color, std::size_t(g.cap)));
+    PRX_HIP(a.alloc(g.claim, std::size_t(g.B)));
+    PRX_HIP(a.alloc(g.colorStart, std::size_t(prx::kBigMaxColors + 2)));
+    PRX_HIP(a.alloc(g.counters, std::size_t(prx::kBigCounters)));
+    PRX_HIP(a.alloc(g.metrics, std::size_t(prx::kBigMetrics)));
+    PRX_HIP(hipMemset(g.counters, 0, sizeof(int) * prx::kBigCounters));
+    PRX_HIP(hipMemset(g.metrics, 0, sizeof(float) * prx::kBigMetrics));
+    return PRX_OK;
+}
+
+}  // namespace
+
+struct prx_big {
+    prx_world* world = nullptr;
+    prx_batch* batch = nullptr;  // the world's one-world device state
+    DeviceArena arena;
+    prx::BigState g{};
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    prx::StepParams gparams{};
+    bool graph_valid = false;
+    int last_launches = 0;
+    ~prx_big() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+extern "C" {
+
+int prx_big_create(prx_world* w, prx_big** out) {
+    if (int st = CheckWorld(w)) return st;
+    if (!out) return Fail(PRX_E_INVALID_ARGUMENT, "null out");
+    if (!w->joints.empty()) return Fail(PRX_E_UNSUPPORTED, "joints are not stepped by the coloured large-world mode");
+    if (w->proxies.empty()) return Fail(PRX_E_INVALID_ARGUMENT, "world has no shapes");
+    prx_batch_conf bc;
+    prx_batch_conf_default(&bc);
+    bc.use_graph = 0;
+    prx_batch* b = nullptr;
+    prx_world* one[1] = {w};
+    if (int st = CreateBatch(one, 1, &bc, &b)) return st;
+    auto big = std::make_unique<prx_big>();
+    big->world = w;
+    big->batch = b;
+    if (int st = AllocBig(big->arena, big->g, int(b->totP), int(b->totB), int(b->totC))) {
+        prx_batch_destroy(b);
+        return st;
+    }
+    *out = big.release();
+    return PRX_OK;
+}
+
+void prx_big_destroy(prx_big* big) {
+    if (!big) return;
+    prx_batch_destroy(big->batch);
+    delete big;
+}
+
+int prx_big_step(prx_big* big, const prx_step_conf* conf, int nsteps) {
+    if (!big || !conf) return Fail(PRX_E_INVALID_ARGUMENT, "null argument");
+    if (conf->do_toi) return Fail(PRX_E_UNSUPPORTED, "doToi = true is not stepped by this engine");
+    if (nsteps < 0) return Fail(PRX_E_INVALID_ARGUMENT, "negative step count");
+    prx_batch* b = big->batch;
+    if (b->worlds[0] != big->world || big->world->owner != b)
+        return Fail(PRX_E_WRONG_STATE, "the world left its large-world state");
+    prx::StepParams p = ToParams(*conf);
+    p.ldsBytes = 0;
+    p.coopMin = 0;
+    PRX_HIP(hipEventRecord(b->ev0, b->stream));
+    if (!big->graph_valid || std::memcmp(&p, &big->gparams, sizeof(p)) != 0) {
+        if (big->exec) (void)hipGraphExecDestroy(big->exec);
+        if (big->graph) (void)hipGraphDestroy(big->graph);
+        big->exec = nullptr;
+        big->graph = nullptr;
+        PRX_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        const hipError_t le = prx_big_launch_step(&b->d, &big->g, &p, b->stream);
+        hipGraph_t gr = nullptr;
+        const hipError_t ce = hipStreamEndCapture(b->stream, &gr);
+        if (le != hipSuccess) return Fail(PRX_E_DEVICE, std::string("launch: ") + hipGetErrorString(le));
+        if (ce != hipSuccess) return Fail(PRX_E_DEVICE, std::string("capture: ") + hipGetErrorString(ce));
+        big->graph = gr;
+        PRX_HIP(hipGraphInstantiate(&big->exec, gr, nullptr, nullptr, 0));
+        big->gparams = p;
+        big->graph_valid = true;
+        PRX_HIP(hipEventRecord(b->ev0, b->stream));
+    }
+    for (int s = 0; s < nsteps; ++s) PRX_HIP(hipGraphLaunch(big->exec, b->stream));
+    PRX_HIP(hipEventRecord(b->ev1, b->stream));
+    b->last_launches = nsteps;
+    big->world->host_stale = true;
+    return PRX_OK;
+}
+
+int prx_big_sync(prx_big* big) {
+    if (!big) return Fail(PRX_E_INVALID_ARGUMENT, "null argument");
+    PRX_HIP(hipStreamSynchronize(big->batch->stream));
+    return CheckStatus(big->batch);
+}
+
+int prx_big_last_timing(prx_big* big, float* ms, int* launches) {
+    if (!big) return Fail(PRX_E_INVALID_ARGUMENT, "null argument");
+    return prx_batch_last_timing(big->batch, ms, launches);
+}
+
+int prx_big_metrics(prx_big* big, float out[4]) {
+    if (!big || !out) return Fail(PRX_E_INVALID_ARGUMENT, "null argument");
+    prx_batch* b = big->batch;
+    const int init[1] = {0x7fffffff};
+    PRX_HIP(hipMemcpyAsync(big->g.counters + prx::kBigMinSepBits, init, sizeof(int), hipMemcpyHostToDevice,
+                           b->stream));
+    PRX_HIP(prx_big_launch_metrics(&b->d, &big->g, b->stream));
+    PRX_HIP(hipStreamSynchronize(b->stream));
+    int counters[prx::kBigCounters];
+    PRX_HIP(hipMemcpy(counters, big->g.counters, sizeof(counters), hipMemcpyDeviceToHost));
+    prx::WorldHeader h;
+    PRX_HIP(hipMemcpy(&h, b->d.hdr, sizeof(h), hipMemcpyDeviceToHost));
+    const int key = counters[prx::kBigMinSepBits];
+    float minSep = __builtin_inff();
+    if (key != 0x7fffffff) {
+        const int bits = key >= 0 ? key : (key ^ 0x7fffffff);
+        std::memcpy(&minSep, &bits, 4);
+    }
+    out[0] = minSep;
+    out[1] = float(counters[prx::kBigTouching]);
+    out[2] = float(counters[prx::kBigColors]);
+    out[3] = float(h.ncontacts);
+    return PRX_OK;
+}
+
+int prx_lbvh_pairs(const float* aabbs, const std::int32_t* bodies, const std::uint8_t* moved, int n,
+                   std::int32_t* pairs, int cap, int* count) {
+    if (!aabbs || !bodies || !moved || n <= 0 || !count || (cap > 0 && !pairs))
+        return Fail(PRX_E_INVALID_ARGUMENT, "bad argument");
+    DeviceArena a;
+    prx::DeviceState d{};
+    prx::BigState g{};
+    d.nworlds = 1;
+    PRX_HIP(a.alloc(d.hdr, 1));
+    PRX_HIP(a.alloc(d.paabb, std::size_t(n)));
+    PRX_HIP(a.alloc(d.pbody, std::size_t(n)));
+    PRX_HIP(a.alloc(d.pflags, std::size_t(n)));
+    if (int st = AllocBig(a, g, n, 1, 1)) return st;
+    prx::WorldHeader h{};
+    h.nproxies = n;
+    h.nbodies = 0;
+    std::vector<float4> ab(static_cast<std::size_t>(n));
+    std::vector<std::uint32_t> fl(static_cast<std::size_t>(n));
+    for (int i = 0; i < n; ++i) {
+        ab[std::size_t(i)] = float4{aabbs[4 * i], aabbs[4 * i + 1], aabbs[4 * i + 2], aabbs[4 * i + 3]};
+        fl[std::size_t(i)] = moved[i] ? prx::kProxyMoved : 0u;
+    }
+    PRX_HIP(hipMemcpy(d.hdr, &h, sizeof(h), hipMemcpyHostToDevice));
+    PRX_HIP(hipMemcpy(d.paabb, ab.data(), ab.size() * sizeof(float4), hipMemcpyHostToDevice));
+    PRX_HIP(hipMemcpy(d.pbody, bodies, std::size_t(n) * sizeof(int), hipMemcpyHostToDevice));
+    PRX_HIP(hipMemcpy(d.pflags, fl.data(), fl.size() * sizeof(std::uint32_t), hipMemcpyHostToDevice));
+    PRX_HIP(prx_big_launch_pairs(&d, &g, nullptr));
+    PRX_HIP(hipDeviceSynchronize());
+    int total = 0;
+    PRX_HIP(hipMemcpy(&total, g.pcount + n, sizeof(int), hipMemcpyDeviceToHost));
+    *count = total;
+    const int m = std::min(std::min(total, cap), g.pairCap);
+    if (m > 0) PRX_HIP(hipMemcpy(pairs, g.pairs, std::size_t(m) * sizeof(int2), hipMemcpyDeviceToHost));
+    if (total > g.pairCap) return Fail(PRX_E_LENGTH, "more pairs than the pair capacity");
     return PRX_OK;
 }
 

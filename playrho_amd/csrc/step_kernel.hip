@@ -19,2188 +19,9 @@
 //
 // A batch of worlds is one launch: workgroup g steps world schedule[g]
 // (largest worlds first), so uneven world sizes balance across the 256 CUs.
-#include <hip/hip_runtime.h>
-
-#include "layout.hpp"
-#include "narrowphase.hpp"
+#include "step_device.hpp"
 
 namespace prxd {
-
-using prx::DeviceState;
-using prx::StepParams;
-using prx::WorldHeader;
-
-constexpr int kWave = 64;
-constexpr int kChunk = prx::kStageRecords;
-
-// Phase timer: lane 0 records shader-clock deltas into phase_cycles[w].
-struct PhaseTimer {
-    unsigned long long* out;
-    unsigned long long t0, start;
-    __device__ __forceinline__ void begin(unsigned long long* o) {
-        out = o;
-        t0 = start = (o && threadIdx.x == 0) ? clock64() : 0ull;
-    }
-    __device__ __forceinline__ void mark(int phase) {
-        if (out && threadIdx.x == 0) {
-            const unsigned long long t = clock64();
-            out[phase] += t - t0;
-            t0 = t;
-        }
-    }
-    __device__ __forceinline__ void end() {
-        if (out && threadIdx.x == 0) out[prx::kPhTotal] += clock64() - start;
-    }
-};
-
-__device__ __forceinline__ unsigned long long lanes_below(int lane) {
-    return (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-}
-
-__device__ __forceinline__ Proxy shape_proxy(const DeviceState& d, int s) {
-    const int4 si = d.shape_i[s];
-    const float4 sf = d.shape_f[s];
-    return Proxy{d.verts + si.y, d.norms + si.y, std::uint32_t(si.x), sf.x};
-}
-
-__device__ __forceinline__ Xf body_xf1(const DeviceState& d, int gb) {
-    const float4 t = d.xf[gb];
-    return Xf{V2{t.x, t.y}, V2{t.z, t.w}};
-}
-__device__ __forceinline__ Xf body_xf0(const DeviceState& d, int gb) {
-    const float4 p = d.pos0[gb];
-    const float4 l = d.lcinv[gb];
-    return get_xf(p.x, p.y, p.z, V2{l.x, l.y});
-}
-
-// ComputeAABB(proxy, xf0, xf1) (AABB.cpp:51-62) with optional fattening by
-// aabbExtension and displacement by displaceMultiplier * (p1 - p0)
-// (detail/AABB.hpp:305-319, Interval.hpp:144-220).
-__device__ __forceinline__ float4 proxy_aabb(const Proxy& px, const Xf& xf0, const Xf& xf1, const StepParams& c, bool fatten) {
-    float mnx = __builtin_inff(), mny = __builtin_inff();
-    float mxx = -__builtin_inff(), mxy = -__builtin_inff();
-    for (std::uint32_t i = 0; i < px.n; ++i) {
-        const V2 v = px.vert(i);
-        const V2 a = transform(v, xf0);
-        mnx = fmin_std(a.x, mnx);
-        mxx = fmax_std(a.x, mxx);
-        mny = fmin_std(a.y, mny);
-        mxy = fmax_std(a.y, mxy);
-        const V2 b = transform(v, xf1);
-        mnx = fmin_std(b.x, mnx);
-        mxx = fmax_std(b.x, mxx);
-        mny = fmin_std(b.y, mny);
-        mxy = fmax_std(b.y, mxy);
-    }
-    mnx = mnx - px.radius;
-    mxx = mxx + px.radius;
-    mny = mny - px.radius;
-    mxy = mxy + px.radius;
-    if (fatten) {
-        mnx = mnx - c.aabbExtension;
-        mxx = mxx + c.aabbExtension;
-        mny = mny - c.aabbExtension;
-        mxy = mxy + c.aabbExtension;
-        const V2 disp = mul(c.displaceMultiplier, xf1.p - xf0.p);
-        if (disp.x < 0.0f) mnx += disp.x;
-        else mxx += disp.x;
-        if (disp.y < 0.0f) mny += disp.y;
-        else mxy += disp.y;
-    }
-    return float4{mnx, mny, mxx, mxy};
-}
-
-// IsIntersecting per axis (Interval.hpp:292-298).
-__device__ __forceinline__ bool overlap(float4 a, float4 b) {
-    const float mx0 = fmax_std(a.x, b.x);
-    const float mn0 = fmin_std(a.z, b.z);
-    if (!(mn0 >= mx0)) return false;
-    const float mx1 = fmax_std(a.y, b.y);
-    const float mn1 = fmin_std(a.w, b.w);
-    return mn1 >= mx1;
-}
-
-__device__ __forceinline__ void body_set_awake(const DeviceState& d, int gb) {
-    // Body::SetAwake (Body.cpp:163-171)
-    const std::uint32_t f = d.bflags[gb];
-    if (f & prx::kSpeedable) {
-        d.bflags[gb] = f | prx::kAwake;
-        d.vel[gb].w = 0.0f;
-    }
-}
-
-__device__ __forceinline__ Manifold load_manifold(const DeviceState& d, int gi) {
-    Manifold m;
-    const uint4 h = d.m_hdr[gi];
-    const float4 g = d.m_geo[gi];
-    const float4 p0 = d.m_p0[gi];
-    const float4 p1 = d.m_p1[gi];
-    m.type = h.x & 0xffu;
-    m.n = (h.x >> 8) & 0xffu;
-    m.ln = V2{g.x, g.y};
-    m.lpt = V2{g.z, g.w};
-    m.p[0] = MPoint{V2{p0.x, p0.y}, h.y, p0.z, p0.w};
-    m.p[1] = MPoint{V2{p1.x, p1.y}, h.z, p1.z, p1.w};
-    return m;
-}
-
-__device__ __forceinline__ void store_manifold(const DeviceState& d, int gi, const Manifold& m) {
-    d.m_hdr[gi] = uint4{m.type | (m.n << 8), m.n > 0 ? m.p[0].cf : kDefaultCf, m.n > 1 ? m.p[1].cf : kDefaultCf, 0u};
-    d.m_geo[gi] = float4{m.ln.x, m.ln.y, m.lpt.x, m.lpt.y};
-    d.m_p0[gi] = m.n > 0 ? float4{m.p[0].lp.x, m.p[0].lp.y, m.p[0].ni, m.p[0].ti} : float4{0, 0, 0, 0};
-    d.m_p1[gi] = m.n > 1 ? float4{m.p[1].lp.x, m.p[1].lp.y, m.p[1].ni, m.p[1].ti} : float4{0, 0, 0, 0};
-}
-
-// Update (AabbTreeWorld.cpp:2587-2721): new manifold, warm-start impulses
-// carried over by ContactFeature, else from the nearest old point.
-__device__ __forceinline__ void update_contact(const DeviceState& d, int bo, int gi, const StepParams& c) {
-    std::uint32_t flags = d.c_flags[gi];
-    const Manifold old = load_manifold(d, gi);
-    const bool oldTouching = (flags & prx::kTouching) != 0;
-    bool newTouching = false;
-    Manifold nm = empty_manifold();
-    if (!(flags & prx::kSensor)) {
-        const int2 bb = d.c_body[gi];
-        const int2 ss = d.c_shape[gi];
-        const Proxy pa = shape_proxy(d, ss.x);
-        const Proxy pb = shape_proxy(d, ss.y);
-        nm = collide_shapes(pa, body_xf1(d, bo + bb.x), pb, body_xf1(d, bo + bb.y), c.linearSlop, c.maxCirclesRatio);
-        const std::uint32_t oldN = old.n, newN = nm.n;
-        newTouching = newN > 0;
-        bool found[2] = {false, newN < 2};
-#pragma unroll
-        for (std::uint32_t i = 0; i < 2; ++i) {
-            if (i >= newN) continue;
-            bool done = false;
-#pragma unroll
-            for (std::uint32_t j = 0; j < 2; ++j) {
-                if (done || j >= oldN) continue;
-                if (nm.p[i].cf == old.p[j].cf) {
-                    found[i] = true;
-                    nm.p[i].ni = old.p[j].ni;
-                    nm.p[i].ti = old.p[j].ti;
-                    done = true;
-                }
-            }
-        }
-#pragma unroll
-        for (std::uint32_t i = 0; i < 2; ++i) {
-            if (i >= newN || found[i]) continue;
-            float least = __builtin_inff();
-#pragma unroll
-            for (std::uint32_t j = 0; j < 2; ++j) {
-                if (j >= oldN) continue;
-                const float dd = magsq(old.p[j].lp - nm.p[i].lp);
-                if (least > dd) {
-                    least = dd;
-                    nm.p[i].ni = old.p[j].ni;
-                    nm.p[i].ti = old.p[j].ti;
-                }
-            }
-        }
-    }
-    store_manifold(d, gi, nm);
-    flags &= ~prx::kNeedsUpdating;
-    if (!oldTouching && newTouching) flags |= prx::kTouching;
-    else if (oldTouching && !newTouching) flags &= ~prx::kTouching;
-    d.c_flags[gi] = flags;
-}
-
-struct WorldCtx {
-    int w, bo, nb, po, np, co, cap, jo, nj;
-    int lane;
-};
-
-struct Counters {
-    int nc, nfree, pool;
-    std::uint32_t status;
-};
-
-// UpdateContacts (AabbTreeWorld.cpp:2333-2421).  Returns contacts updated.
-__device__ __forceinline__ std::uint32_t phase_update(const DeviceState& d, const WorldCtx& x, int nc, const StepParams& c) {
-    std::uint32_t updated = 0;
-    for (int base = 0; base < nc; base += kWave) {
-        const int i = base + x.lane;
-        bool need = false;
-        if (i < nc) need = (d.c_flags[x.co + i] & prx::kNeedsUpdating) != 0;
-        updated += std::uint32_t(__popcll(__ballot(need)));
-        if (need) update_contact(d, x.bo, x.co + i, c);
-    }
-    return updated;
-}
-
-// DestroyContacts (AabbTreeWorld.cpp:2290-2331) + InternalDestroy (:2263-2288).
-__device__ __forceinline__ std::uint32_t phase_destroy(const DeviceState& d, const WorldCtx& x, Counters& k) {
-    int out = 0;
-    std::uint32_t destroyedTotal = 0;
-    const int nc = k.nc;
-    for (int base = 0; base < nc; base += kWave) {
-        const int i = base + x.lane;
-        const bool valid = i < nc;
-        bool keep = false;
-        if (valid) {
-            const int2 pr = d.c_prox[x.co + i];
-            keep = overlap(d.paabb[x.po + pr.x], d.paabb[x.po + pr.y]);
-        }
-        const bool destroyed = valid && !keep;
-        const unsigned long long dm = __ballot(destroyed);
-        const unsigned long long km = __ballot(valid && keep);
-        const unsigned long long below = lanes_below(x.lane);
-        if (destroyed) {
-            const int r = __popcll(dm & below);
-            d.c_free[x.co + k.nfree + r] = d.c_id[x.co + i];
-            const std::uint32_t mh = d.m_hdr[x.co + i].x;
-            const std::uint32_t fl = d.c_flags[x.co + i];
-            if (((mh >> 8) & 0xffu) > 0 && !(fl & prx::kSensor)) {
-                const int2 bd = d.c_body[x.co + i];
-                body_set_awake(d, x.bo + bd.x);
-                body_set_awake(d, x.bo + bd.y);
-            }
-        }
-        const int dst = out + __popcll(km & below);
-        const bool move = valid && keep && (dst != i);
-        if (__ballot(move)) {
-            // Stable compaction: every lane loads its record, then the wave
-            // synchronises, then the moved records are stored (dst <= i).
-            int2 pr{0, 0}, bd{0, 0}, sh{0, 0};
-            std::uint32_t id = 0, fl = 0;
-            float4 mt{0, 0, 0, 0}, mg{0, 0, 0, 0}, p0{0, 0, 0, 0}, p1{0, 0, 0, 0};
-            uint4 mh{0, 0, 0, 0};
-            if (move) {
-                const int src = x.co + i;
-                pr = d.c_prox[src];
-                bd = d.c_body[src];
-                sh = d.c_shape[src];
-                id = d.c_id[src];
-                fl = d.c_flags[src];
-                mt = d.c_mat[src];
-                mh = d.m_hdr[src];
-                mg = d.m_geo[src];
-                p0 = d.m_p0[src];
-                p1 = d.m_p1[src];
-            }
-            __syncthreads();
-            if (move) {
-                const int g = x.co + dst;
-                d.c_prox[g] = pr;
-                d.c_body[g] = bd;
-                d.c_shape[g] = sh;
-                d.c_id[g] = id;
-                d.c_flags[g] = fl;
-                d.c_mat[g] = mt;
-                d.m_hdr[g] = mh;
-                d.m_geo[g] = mg;
-                d.m_p0[g] = p0;
-                d.m_p1[g] = p1;
-            }
-        }
-        k.nfree += __popcll(dm);
-        out += __popcll(km);
-        destroyedTotal += std::uint32_t(__popcll(dm));
-        __syncthreads();
-    }
-    k.nc = out;
-    return destroyedTotal;
-}
-
-// ShouldCollide for two shapes' filters (Filter.hpp:81-88).
-__device__ __forceinline__ bool should_collide(const DeviceState& d, int sa, int sb) {
-    const int ga = d.shape_i[sa].w, gb = d.shape_i[sb].w;
-    if (ga == gb && ga != 0) return ga > 0;
-    const std::uint32_t cma = d.shape_catmask[sa], cmb = d.shape_catmask[sb];
-    return (((cma >> 16) & (cmb & 0xffffu)) != 0) && (((cmb >> 16) & (cma & 0xffffu)) != 0);
-}
-
-// ShouldCollide for two bodies' joints (AabbTreeWorld.cpp:479-490): false if
-// a joint between them does not let its bodies collide.
-__device__ __forceinline__ bool joint_allows(const DeviceState& d, const WorldCtx& x, int ba, int bb) {
-    if (x.nj == 0) return true;
-    const int2 bj = d.bjoint[x.bo + ba];
-    for (int k = 0; k < bj.y; ++k) {
-        const int2 e = d.bjoint_list[2 * x.jo + bj.x + k];
-        if (e.y == bb && !(__float_as_uint(d.j_cfg2[x.jo + e.x].y) & prx::kJointCollideConnected)) return false;
-    }
-    return true;
-}
-
-// FindContacts (AabbTreeWorld.cpp:709-751) + AddContacts (:2424-2547).
-// Candidate keys pack world-local proxy indices (min << 16 | max); proxy
-// indices increase with tree leaf ids, so key order equals ContactKey order.
-__device__ __forceinline__ std::uint32_t phase_find_add(const DeviceState& d, const WorldCtx& x, Counters& k, const StepParams& c) {
-    // Moved proxies, in index order.
-    int nmoved = 0;
-    for (int base = 0; base < x.np; base += kWave) {
-        const int p = base + x.lane;
-        bool mv = false;
-        if (p < x.np) mv = (d.pflags[x.po + p] & prx::kProxyMoved) != 0;
-        const unsigned long long m = __ballot(mv);
-        if (mv) {
-            const int r = nmoved + __popcll(m & lanes_below(x.lane));
-            d.s_isl_body[x.bo + x.co + r] = p;  // scratch list (segment is nbodies + cap >= nproxies)
-        }
-        nmoved += __popcll(m);
-    }
-    __syncthreads();
-    if (nmoved == 0) return 0;
-    int* moved = d.s_isl_body + x.bo + x.co;
-
-    // Candidate pairs: every proxy q against every moved proxy m.
-    int ncand = 0;
-    bool overflow = false;
-    for (int base = 0; base < x.np; base += kWave) {
-        const int q = base + x.lane;
-        const bool qv = q < x.np;
-        float4 qa = float4{0, 0, 0, 0};
-        int qb = -1;
-        bool qmoved = false;
-        if (qv) {
-            qa = d.paabb[x.po + q];
-            qb = d.pbody[x.po + q];
-            qmoved = (d.pflags[x.po + q] & prx::kProxyMoved) != 0;
-        }
-        for (int j = 0; j < nmoved; ++j) {
-            const int m = moved[j];
-            bool hit = false;
-            if (qv && q != m) {
-                // A pair of two moved proxies is produced once, from the smaller.
-                if (!(qmoved && q < m)) {
-                    if (qb != d.pbody[x.po + m] && overlap(d.paabb[x.po + m], qa)) hit = true;
-                }
-            }
-            const unsigned long long hm = __ballot(hit);
-            if (hit) {
-                const int r = ncand + __popcll(hm & lanes_below(x.lane));
-                if (r < x.cap) {
-                    const int lo = q < m ? q : m, hi = q < m ? m : q;
-                    d.s_keys[x.co + r] = (std::uint32_t(lo) << 16) | std::uint32_t(hi);
-                }
-            }
-            ncand += __popcll(hm);
-        }
-    }
-    if (ncand > x.cap) {
-        overflow = true;
-        ncand = x.cap;
-    }
-    __syncthreads();
-    // Clear the moved flags (proxiesForContacts is consumed).
-    for (int p = x.lane; p < x.np; p += kWave) d.pflags[x.po + p] &= ~prx::kProxyMoved;
-    if (ncand == 0) {
-        if (overflow) k.status |= prx::kStatusScratchOverflow;
-        return 0;
-    }
-    // Rank sort of the (unique) candidate keys.
-    for (int i = x.lane; i < ncand; i += kWave) {
-        const std::uint32_t key = d.s_keys[x.co + i];
-        int rank = 0;
-        for (int j = 0; j < ncand; ++j) rank += (d.s_keys[x.co + j] < key) ? 1 : 0;
-        d.s_keys2[x.co + rank] = key;
-    }
-    __syncthreads();
-    // Existing contacts: mark candidates already present (binary search).
-    for (int i = x.lane; i < ncand; i += kWave) d.s_isl_c[x.co + i] = 0;
-    __syncthreads();
-    for (int i = x.lane; i < k.nc; i += kWave) {
-        const int2 pr = d.c_prox[x.co + i];
-        const std::uint32_t key = (std::uint32_t(pr.x) << 16) | std::uint32_t(pr.y);
-        int lo = 0, hi = ncand;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (d.s_keys2[x.co + mid] < key) lo = mid + 1;
-            else hi = mid;
-        }
-        if (lo < ncand && d.s_keys2[x.co + lo] == key) d.s_isl_c[x.co + lo] = 1;
-    }
-    __syncthreads();
-    // Filter, then append in key order.
-    std::uint32_t added = 0;
-    const int nc0 = k.nc;
-    for (int base = 0; base < ncand; base += kWave) {
-        const int i = base + x.lane;
-        bool acc = false;
-        int pa = 0, pb = 0, ba = 0, bb = 0, sa = 0, sb = 0;
-        if (i < ncand) {
-            const std::uint32_t key = d.s_keys2[x.co + i];
-            pa = int(key >> 16);
-            pb = int(key & 0xffffu);
-            ba = d.pbody[x.po + pa];
-            bb = d.pbody[x.po + pb];
-            sa = d.pshape[x.po + pa];
-            sb = d.pshape[x.po + pb];
-            const std::uint32_t fa = d.bflags[x.bo + ba], fb = d.bflags[x.bo + bb];
-            acc = ((fa | fb) & prx::kAccelerable) != 0 && should_collide(d, sa, sb) && d.s_isl_c[x.co + i] == 0 &&
-                  joint_allows(d, x, ba, bb);
-        }
-        const unsigned long long am = __ballot(acc);
-        if (acc) {
-            const int r = int(added) + __popcll(am & lanes_below(x.lane));
-            const int idx = nc0 + r;
-            if (idx < x.cap) {
-                std::uint32_t id;
-                if (r < k.nfree) id = d.c_free[x.co + k.nfree - 1 - r];
-                else id = std::uint32_t(k.pool + (r - k.nfree));
-                const int gi = x.co + idx;
-                const std::uint32_t fa = d.bflags[x.bo + ba], fb = d.bflags[x.bo + bb];
-                const int4 sia = d.shape_i[sa], sib = d.shape_i[sb];
-                const float4 sfa = d.shape_f[sa], sfb = d.shape_f[sb];
-                std::uint32_t flags = prx::kCEnabled | prx::kNeedsUpdating;
-                if ((fa | fb) & prx::kImpenetrable) flags |= prx::kCImpenetrable;
-                const bool sensor = sia.z != 0 || sib.z != 0;
-                if (sensor) flags |= prx::kSensor;
-                d.c_prox[gi] = int2{pa, pb};
-                d.c_body[gi] = int2{ba, bb};
-                d.c_shape[gi] = int2{sa, sb};
-                d.c_id[gi] = id;
-                d.c_flags[gi] = flags;
-                // MixFriction / MixRestitution (Contact.hpp:57-69)
-                d.c_mat[gi] = float4{__builtin_sqrtf(sfa.y * sfb.y), (sfa.z > sfb.z) ? sfa.z : sfb.z, 0.0f, 0.0f};
-                d.m_hdr[gi] = uint4{0u, kDefaultCf, kDefaultCf, 0u};
-                d.m_geo[gi] = float4{0, 0, 0, 0};
-                d.m_p0[gi] = float4{0, 0, 0, 0};
-                d.m_p1[gi] = float4{0, 0, 0, 0};
-            }
-        }
-        added += std::uint32_t(__popcll(am));
-    }
-    __syncthreads();
-    // Awake flags (SetAwakeFlag for speedable bodies of non-sensor contacts):
-    // idempotent, so done per new contact in a second pass without races on
-    // the flags word of a body touched by two lanes.
-    int nadd = int(added);
-    if (nc0 + nadd > x.cap) {
-        k.status |= prx::kStatusContactOverflow;
-        nadd = x.cap - nc0;
-    }
-    if (overflow) k.status |= prx::kStatusScratchOverflow;
-    if (x.lane == 0) {
-        for (int r = 0; r < nadd; ++r) {
-            const int gi = x.co + nc0 + r;
-            if (d.c_flags[gi] & prx::kSensor) continue;
-            const int2 bd = d.c_body[gi];
-            const std::uint32_t fa = d.bflags[x.bo + bd.x];
-            if (fa & prx::kSpeedable) d.bflags[x.bo + bd.x] = fa | prx::kAwake;
-            const std::uint32_t fb = d.bflags[x.bo + bd.y];
-            if (fb & prx::kSpeedable) d.bflags[x.bo + bd.y] = fb | prx::kAwake;
-        }
-    }
-    const int consumedFree = nadd < k.nfree ? nadd : k.nfree;
-    k.pool += nadd - consumedFree;
-    k.nfree -= consumedFree;
-    k.nc = nc0 + nadd;
-    __syncthreads();
-    // Narrowphase of the new contacts.
-    for (int i = nc0 + x.lane; i < k.nc; i += kWave) update_contact(d, x.bo, x.co + i, c);
-    __syncthreads();
-    return std::uint32_t(nadd);
-}
-
-// ---------------------------------------------------------------------------
-// Island solve.  Velocity-constraint record (kVcStride float4, stored in
-// island order so a sweep reads consecutive records):
-//   v[0] normal.x, normal.y, friction, tangentSpeed
-//   v[1] K00, K11, K01, count
-//   v[2] M00, M11, M01, bodyA            (normal mass = K^-1)
-//   v[3] p0 relA.x, relA.y, relB.x, relB.y
-//   v[4] p0 ni, ti, normalMass, tangentMass
-//   v[5] p0 velocityBias, p1 velocityBias, bodyB, restitution
-//   v[6] p1 relA.x, relA.y, relB.x, relB.y
-//   v[7] p1 ni, ti, normalMass, tangentMass
-//   v[8] invMassA, invRotIA, invMassB, invRotIB
-// Position-constraint record (kPcStride float4, island order):
-//   u[0] type | count << 8, bodyA, bodyB, totalRadius
-//   u[1] local normal x, y, local point x, y
-//   u[2] point 0 local x, y, point 1 local x, y
-//   u[3] invMassA, invRotIA, localCenterA x, y
-//   u[4] invMassB, invRotIB, localCenterB x, y
-
-struct BodyC {
-    float invMass, invRotI;
-    V2 lc;
-};
-
-__device__ __forceinline__ V2 rel_vel(V2 vA, float wA, V2 relA, V2 vB, float wB, V2 relB) {
-    // GetContactRelVelocity (d2/Math.cpp:73-103)
-    const V2 velBrot = mul(revperp(relB), wB / 1.0f);
-    const V2 velArot = mul(revperp(relA), wA / 1.0f);
-    return (vB + velBrot) - (vA + velArot);
-}
-
-struct VcPoint {
-    V2 relA, relB;
-    float ni, ti, nm, tm, bias;
-};
-
-struct Vc {
-    V2 normal;
-    float friction, restitution, tangentSpeed;
-    std::uint32_t n;
-    float K00, K11, K01, M00, M11, M01;
-    int bA, bB;
-    VcPoint p[2];
-};
-
-__device__ __forceinline__ void store_vc(float4* v, const Vc& vc, const BodyC& A, const BodyC& B) {
-    v[0] = float4{vc.normal.x, vc.normal.y, vc.friction, vc.tangentSpeed};
-    v[1] = float4{vc.K00, vc.K11, vc.K01, __uint_as_float(vc.n)};
-    v[2] = float4{vc.M00, vc.M11, vc.M01, __int_as_float(vc.bA)};
-    v[3] = float4{vc.p[0].relA.x, vc.p[0].relA.y, vc.p[0].relB.x, vc.p[0].relB.y};
-    v[4] = float4{vc.p[0].ni, vc.p[0].ti, vc.p[0].nm, vc.p[0].tm};
-    v[5] = float4{vc.p[0].bias, vc.p[1].bias, __int_as_float(vc.bB), vc.restitution};
-    v[6] = float4{vc.p[1].relA.x, vc.p[1].relA.y, vc.p[1].relB.x, vc.p[1].relB.y};
-    v[7] = float4{vc.p[1].ni, vc.p[1].ti, vc.p[1].nm, vc.p[1].tm};
-    v[8] = float4{A.invMass, A.invRotI, B.invMass, B.invRotI};
-}
-__device__ __forceinline__ Vc load_vc(const float4* v, BodyC& A, BodyC& B) {
-    Vc vc;
-    const float4 a = v[0], b = v[1], e = v[2], p0a = v[3], p0b = v[4], f = v[5], p1a = v[6], p1b = v[7],
-                 m = v[8];
-    vc.normal = V2{a.x, a.y};
-    vc.friction = a.z;
-    vc.tangentSpeed = a.w;
-    vc.K00 = b.x;
-    vc.K11 = b.y;
-    vc.K01 = b.z;
-    vc.n = __float_as_uint(b.w);
-    vc.M00 = e.x;
-    vc.M11 = e.y;
-    vc.M01 = e.z;
-    vc.bA = __float_as_int(e.w);
-    vc.p[0] = VcPoint{V2{p0a.x, p0a.y}, V2{p0a.z, p0a.w}, p0b.x, p0b.y, p0b.z, p0b.w, f.x};
-    vc.p[1] = VcPoint{V2{p1a.x, p1a.y}, V2{p1a.z, p1a.w}, p1b.x, p1b.y, p1b.z, p1b.w, f.y};
-    vc.bB = __float_as_int(f.z);
-    vc.restitution = f.w;
-    A = BodyC{m.x, m.y, V2{0, 0}};
-    B = BodyC{m.z, m.w, V2{0, 0}};
-    return vc;
-}
-
-// World manifold point data (WorldManifold.cpp:37-130).
-struct WM {
-    V2 normal;
-    std::uint32_t n;
-    V2 p[2];
-    float ni[2], ti[2];
-};
-
-__device__ __forceinline__ WM world_manifold(const Manifold& m, const Xf& xfA, float rA, const Xf& xfB, float rB) {
-    WM w;
-    w.n = 0;
-    w.normal = V2{0, 0};
-    if (m.type == kCircles) {
-        const V2 pA = transform(m.lpt, xfA);
-        const V2 pB = transform(m.p[0].lp, xfB);
-        const V2 normal = unit_vector(pB - pA, V2{1.0f, 0.0f});
-        const V2 cA = pA + mul(rA, normal);
-        const V2 cB = pB - mul(rB, normal);
-        w.normal = normal;
-        w.n = 1;
-        w.p[0] = divs(cA + cB, 2.0f);
-        w.ni[0] = m.p[0].ni;
-        w.ti[0] = m.p[0].ti;
-    } else if (m.type == kFaceA) {
-        const V2 normal = rotate(m.ln, xfA.q);
-        const V2 plane = transform(m.lpt, xfA);
-        w.normal = normal;
-        w.n = m.n;
-#pragma unroll
-        for (std::uint32_t i = 0; i < 2; ++i) {
-            if (i >= m.n) continue;
-            const V2 clip = transform(m.p[i].lp, xfB);
-            const V2 cA = clip + mul(rA - dot(clip - plane, normal), normal);
-            const V2 cB = clip - mul(rB, normal);
-            w.p[i] = divs(cA + cB, 2.0f);
-            w.ni[i] = m.p[i].ni;
-            w.ti[i] = m.p[i].ti;
-        }
-    } else if (m.type == kFaceB) {
-        const V2 normal = rotate(m.ln, xfB.q);
-        const V2 plane = transform(m.lpt, xfB);
-        w.normal = -normal;
-        w.n = m.n;
-#pragma unroll
-        for (std::uint32_t i = 0; i < 2; ++i) {
-            if (i >= m.n) continue;
-            const V2 clip = transform(m.p[i].lp, xfA);
-            const V2 cB = clip + mul(rB - dot(clip - plane, normal), normal);
-            const V2 cA = clip - mul(rA, normal);
-            w.p[i] = divs(cA + cB, 2.0f);
-            w.ni[i] = m.p[i].ni;
-            w.ti[i] = m.p[i].ti;
-        }
-    }
-    return w;
-}
-
-// VelocityConstraint ctor + GetPoint + ComputeK (VelocityConstraint.cpp:38-185).
-__device__ __forceinline__ Vc make_vc(const WM& wm, float friction, float restitution, float tangentSpeed, int bA, int bB,
-                      const BodyC& A, float4 posA, float4 velA, const BodyC& B, float4 posB, float4 velB,
-                      const StepParams& c) {
-    Vc vc;
-    vc.normal = wm.normal;
-    vc.friction = friction;
-    vc.restitution = restitution;
-    vc.tangentSpeed = tangentSpeed;
-    vc.n = 0;
-    vc.K00 = vc.K11 = vc.K01 = 0.0f;
-    vc.M00 = vc.M11 = vc.M01 = 0.0f;
-    vc.bA = bA;
-    vc.bB = bB;
-    vc.p[0] = VcPoint{V2{0, 0}, V2{0, 0}, 0, 0, 0, 0, 0};
-    vc.p[1] = vc.p[0];
-    const float dtRatio = c.warmStart ? c.dtRatio : 0.0f;
-    const float invMass = A.invMass + B.invMass;
-    const V2 tangent = fwdperp(vc.normal);
-#pragma unroll
-    for (std::uint32_t j = 0; j < 2; ++j) {
-        if (j >= wm.n) continue;
-        VcPoint p;
-        const V2 relA = wm.p[j] - V2{posA.x, posA.y};
-        const V2 relB = wm.p[j] - V2{posB.x, posB.y};
-        p.ni = wm.ni[j] * dtRatio;
-        p.ti = wm.ti[j] * dtRatio;
-        {
-            const V2 dv = rel_vel(V2{velA.x, velA.y}, velA.z, relA, V2{velB.x, velB.y}, velB.z, relB);
-            const float vn = dot(dv, vc.normal);
-            p.bias = (vn < -c.velocityThreshold) ? -vc.restitution * vn : 0.0f;
-        }
-        p.relA = relA;
-        p.relB = relB;
-        {
-            const float cA = cross(relA, vc.normal);
-            const float cB = cross(relB, vc.normal);
-            const float rmA = A.invRotI * (cA * cA) / 1.0f;
-            const float rmB = B.invRotI * (cB * cB) / 1.0f;
-            const float value = invMass + rmA + rmB;
-            p.nm = (value != 0.0f) ? 1.0f / value : 0.0f;
-        }
-        {
-            const float cA = cross(relA, tangent);
-            const float cB = cross(relB, tangent);
-            const float rmA = A.invRotI * (cA * cA) / 1.0f;
-            const float rmB = B.invRotI * (cB * cB) / 1.0f;
-            const float value = invMass + rmA + rmB;
-            p.tm = (value != 0.0f) ? 1.0f / value : 0.0f;
-        }
-        // wm points fill vc points in order, so slot j receives point j.
-        vc.p[j] = p;
-        ++vc.n;
-    }
-    if (c.blockSolve && vc.n == 2) {
-        const float rnA0 = cross(vc.p[0].relA, vc.normal);
-        const float rnB0 = cross(vc.p[0].relB, vc.normal);
-        const float rnA1 = cross(vc.p[1].relA, vc.normal);
-        const float rnB1 = cross(vc.p[1].relB, vc.normal);
-        const float irmA0 = (A.invRotI * (rnA0 * rnA0)) / 1.0f;
-        const float irmA1 = (A.invRotI * (rnA1 * rnA1)) / 1.0f;
-        const float irmB0 = (B.invRotI * (rnB0 * rnB0)) / 1.0f;
-        const float irmB1 = (B.invRotI * (rnB1 * rnB1)) / 1.0f;
-        const float irmA = (A.invRotI * rnA0 * rnA1) / 1.0f;
-        const float irmB = (B.invRotI * rnB0 * rnB1) / 1.0f;
-        const float k00 = invMass + irmA0 + irmB0;
-        const float k11 = invMass + irmA1 + irmB1;
-        const float k01 = invMass + irmA + irmB;
-        const float k00sq = k00 * k00;
-        const float kdiff = k00 * k11 - k01 * k01;
-        if (k00sq < 1000.0f * kdiff) {
-            vc.K00 = k00;
-            vc.K11 = k11;
-            vc.K01 = k01;
-            // Invert (Math.hpp:428-442): cp = Cross(row0, row1)
-            const float cpm = k00 * k11;
-            const float cps = k01 * k01;
-            const float cp = cpm - cps;
-            if (!almost_zero(cp)) {
-                const float inv = 1.0f / cp;
-                vc.M00 = k11 * inv;
-                vc.M11 = k00 * inv;
-                vc.M01 = -k01 * inv;
-            }
-        } else {
-            vc.n = 1;
-        }
-    }
-    return vc;
-}
-
-// SolveTangentConstraint / SeqSolveNormalConstraint (ContactSolver.cpp:337-454).
-__device__ __forceinline__ float solve_dir(Vc& vc, bool tangent, const BodyC& A, const BodyC& B, float4& velA,
-                                           float4& velB) {
-    float maxInc = 0.0f;
-    const V2 dir = tangent ? fwdperp(vc.normal) : vc.normal;
-    V2 vA{velA.x, velA.y}, vB{velB.x, velB.y};
-    float wA = velA.z, wB = velB.z;
-#pragma unroll
-    for (int kk = 0; kk < 2; ++kk) {
-        const int k = 1 - kk;  // point 1 first, then point 0
-        if (k >= int(vc.n)) continue;
-        VcPoint& p = vc.p[k];
-        const V2 closing = rel_vel(vA, wA, p.relA, vB, wB, p.relB);
-        float oldI, newI;
-        if (tangent) {
-            const float dirVel = vc.tangentSpeed - dot(closing, dir);
-            const float lambda = p.tm * dirVel;
-            const float maxI = vc.friction * p.ni;
-            oldI = p.ti;
-            newI = clamp_std(oldI + lambda, -maxI, maxI);
-        } else {
-            const float dirVel = dot(closing, dir);
-            const float lambda = p.nm * (p.bias - dirVel);
-            oldI = p.ni;
-            newI = fmax_std(oldI + lambda, 0.0f);
-        }
-        const float inc = newI - oldI;
-        const V2 P = mul(inc, dir);
-        const float LA = cross(p.relA, P) / 1.0f;
-        const float LB = cross(p.relB, P) / 1.0f;
-        vA = vA - mul(A.invMass, P);
-        wA = wA - A.invRotI * LA;
-        vB = vB + mul(B.invMass, P);
-        wB = wB + B.invRotI * LB;
-        maxInc = fmax_std(maxInc, __builtin_fabsf(inc));
-        if (tangent) p.ti = oldI + inc;
-        else p.ni = oldI + inc;
-    }
-    velA.x = vA.x;
-    velA.y = vA.y;
-    velA.z = wA;
-    velB.x = vB.x;
-    velB.y = vB.y;
-    velB.z = wB;
-    return maxInc;
-}
-
-// BlockSolveNormalConstraint (ContactSolver.cpp:65-335).
-__device__ __forceinline__ float block_solve(Vc& vc, const BodyC& A, const BodyC& B, float4& velA, float4& velB) {
-    const V2 normal = vc.normal;
-    V2 bp;
-    {
-        const V2 vA{velA.x, velA.y}, vB{velB.x, velB.y};
-        const V2 dv0 = rel_vel(vA, velA.z, vc.p[0].relA, vB, velB.z, vc.p[0].relB);
-        const V2 dv1 = rel_vel(vA, velA.z, vc.p[1].relA, vB, velB.z, vc.p[1].relB);
-        const float vn0 = dot(dv0, normal);
-        const float vn1 = dot(dv1, normal);
-        const V2 b{vn0 - vc.p[0].bias, vn1 - vc.p[1].bias};
-        float t0 = 0.0f;
-        t0 += vc.K00 * vc.p[0].ni;
-        t0 += vc.K01 * vc.p[1].ni;
-        float t1 = 0.0f;
-        t1 += vc.K01 * vc.p[0].ni;
-        t1 += vc.K11 * vc.p[1].ni;
-        bp = V2{b.x - t0, b.y - t1};
-    }
-    float n0, n1;
-    bool ok = false;
-    {
-        float x0 = 0.0f;
-        x0 += vc.M00 * bp.x;
-        x0 += vc.M01 * bp.y;
-        float x1 = 0.0f;
-        x1 += vc.M01 * bp.x;
-        x1 += vc.M11 * bp.y;
-        n0 = -x0;
-        n1 = -x1;
-        ok = (n0 >= 0.0f && n1 >= 0.0f);
-    }
-    if (!ok) {
-        n0 = -vc.p[0].nm * bp.x;
-        n1 = 0.0f;
-        const float vn2 = vc.K01 * n0 + bp.y;
-        ok = (n0 >= 0.0f && vn2 >= 0.0f);
-    }
-    if (!ok) {
-        n0 = 0.0f;
-        n1 = -vc.p[1].nm * bp.y;
-        const float vn1 = vc.K01 * n1 + bp.x;
-        ok = (n1 >= 0.0f && vn1 >= 0.0f);
-    }
-    if (!ok) {
-        n0 = 0.0f;
-        n1 = 0.0f;
-        ok = (bp.x >= 0.0f && bp.y >= 0.0f);
-    }
-    if (!ok) return 0.0f;
-    const float d0 = n0 - vc.p[0].ni;
-    const float d1 = n1 - vc.p[1].ni;
-    const V2 P0 = mul(d0, normal);
-    const V2 P1 = mul(d1, normal);
-    const V2 P = P0 + P1;
-    const float LA = (cross(vc.p[0].relA, P0) + cross(vc.p[1].relA, P1)) / 1.0f;
-    const float LB = (cross(vc.p[0].relB, P0) + cross(vc.p[1].relB, P1)) / 1.0f;
-    const V2 dAl = -mul(A.invMass, P);
-    const float dAa = -(A.invRotI * LA);
-    const V2 dBl = mul(B.invMass, P);
-    const float dBa = B.invRotI * LB;
-    velA.x = velA.x + dAl.x;
-    velA.y = velA.y + dAl.y;
-    velA.z = velA.z + dAa;
-    velB.x = velB.x + dBl.x;
-    velB.y = velB.y + dBl.y;
-    velB.z = velB.z + dBa;
-    vc.p[0].ni = n0;
-    vc.p[1].ni = n1;
-    return fmax_std(__builtin_fabsf(n0), __builtin_fabsf(n1));
-}
-
-// Position solver manifold (PositionSolverManifold.cpp:42-121).
-struct PSM {
-    V2 normal, point;
-    float sep;
-};
-__device__ __forceinline__ PSM get_psm(const Manifold& m, std::uint32_t idx, const Xf& xfA, const Xf& xfB) {
-    const V2 plp = idx ? m.p[1].lp : m.p[0].lp;
-    if (m.type == kCircles) {
-        const V2 pA = transform(m.lpt, xfA);
-        const V2 pB = transform(plp, xfB);
-        const V2 delta = pB - pA;
-        const V2 normal = unit_vector(delta, V2{0, 0});
-        return PSM{normal, divs(pA + pB, 2.0f), dot(delta, normal)};
-    }
-    if (m.type == kFaceA) {
-        const V2 plane = transform(m.lpt, xfA);
-        const V2 normal = rotate(m.ln, xfA.q);
-        const V2 clip = transform(plp, xfB);
-        return PSM{normal, clip, dot(clip - plane, normal)};
-    }
-    const V2 plane = transform(m.lpt, xfB);
-    const V2 normal = rotate(m.ln, xfB.q);
-    const V2 clip = transform(plp, xfA);
-    return PSM{-normal, clip, dot(clip - plane, normal)};
-}
-
-// GaussSeidel::SolvePositionConstraint (ContactSolver.cpp:494-643).
-__device__ __forceinline__ float solve_pc(const Manifold& m, float totalRadius, const BodyC& A, const BodyC& B, float4& posA,
-                          float4& posB, const StepParams& c) {
-    const float invMassTotal = A.invMass + B.invMass;
-    struct Sol {
-        V2 la;
-        float aa;
-        V2 lb;
-        float ab;
-        float sep;
-    };
-    auto solve = [&](const PSM& psm, V2 pA, V2 pB) {
-        const float separation = psm.sep - totalRadius;
-        const V2 rA = psm.point - pA;
-        const V2 rB = psm.point - pB;
-        const float crA = cross(rA, psm.normal) / 1.0f;
-        const float crB = cross(rB, psm.normal) / 1.0f;
-        const float K = invMassTotal + A.invRotI * (crA * crA) + B.invRotI * (crB * crB);
-        const float C = -clamp_std(c.regResolutionRate * (separation + c.linearSlop), -c.maxLinearCorrection, 0.0f);
-        const V2 P = divs(mul(psm.normal, C), K);
-        const float LA = cross(rA, P) / 1.0f;
-        const float LB = cross(rB, P) / 1.0f;
-        return Sol{-mul(A.invMass, P), -(A.invRotI * LA), mul(B.invMass, P), B.invRotI * LB, separation};
-    };
-    V2 pAl{posA.x, posA.y}, pBl{posB.x, posB.y};
-    float pAa = posA.z, pBa = posB.z;
-    float minSep = __builtin_inff();
-    if (m.n == 1) {
-        const PSM psm0 = get_psm(m, 0, get_xf(pAl.x, pAl.y, pAa, A.lc), get_xf(pBl.x, pBl.y, pBa, B.lc));
-        const Sol s = solve(psm0, pAl, pBl);
-        pAl = pAl + s.la;
-        pAa = pAa + s.aa;
-        pBl = pBl + s.lb;
-        pBa = pBa + s.ab;
-        minSep = 0.0f + s.sep;
-    } else if (m.n == 2) {
-        const Xf xfA = get_xf(pAl.x, pAl.y, pAa, A.lc);
-        const Xf xfB = get_xf(pBl.x, pBl.y, pBa, B.lc);
-        const PSM psm0 = get_psm(m, 0, xfA, xfB);
-        const PSM psm1 = get_psm(m, 1, xfA, xfB);
-        if (almost_equal(psm0.sep, psm1.sep)) {
-            const Sol s0 = solve(psm0, pAl, pBl);
-            const Sol s1 = solve(psm1, pAl, pBl);
-            pAl = pAl + s0.la + s1.la;
-            pAa = pAa + s0.aa + s1.aa;
-            pBl = pBl + s0.lb + s1.lb;
-            pBa = pBa + s0.ab + s1.ab;
-            minSep = s0.sep;
-        } else if (psm0.sep < psm1.sep) {
-            const Sol s0 = solve(psm0, pAl, pBl);
-            pAl = pAl + s0.la;
-            pAa = pAa + s0.aa;
-            pBl = pBl + s0.lb;
-            pBa = pBa + s0.ab;
-            const PSM p1 = get_psm(m, 1, get_xf(pAl.x, pAl.y, pAa, A.lc), get_xf(pBl.x, pBl.y, pBa, B.lc));
-            const Sol s1 = solve(p1, pAl, pBl);
-            pAl = pAl + s1.la;
-            pAa = pAa + s1.aa;
-            pBl = pBl + s1.lb;
-            pBa = pBa + s1.ab;
-            minSep = s0.sep;
-        } else if (psm1.sep < psm0.sep) {
-            const Sol s1 = solve(psm1, pAl, pBl);
-            pAl = pAl + s1.la;
-            pAa = pAa + s1.aa;
-            pBl = pBl + s1.lb;
-            pBa = pBa + s1.ab;
-            const PSM p0 = get_psm(m, 0, get_xf(pAl.x, pAl.y, pAa, A.lc), get_xf(pBl.x, pBl.y, pBa, B.lc));
-            const Sol s0 = solve(p0, pAl, pBl);
-            pAl = pAl + s0.la;
-            pAa = pAa + s0.aa;
-            pBl = pBl + s0.lb;
-            pBa = pBa + s0.ab;
-            minSep = s1.sep;
-        }
-    }
-    posA.x = pAl.x;
-    posA.y = pAl.y;
-    posA.z = pAa;
-    posB.x = pBl.x;
-    posB.y = pBl.y;
-    posB.z = pBa;
-    return minSep;
-}
-
-// ---------------------------------------------------------------------------
-// Revolute joints (RevoluteJointConf.cpp:127-433).  A joint's solver record in
-// island order (kJcStride float4):
-//   u[0] rA x, y, rB x, y               u[4] invMassA, invRotIA, invMassB, invRotIB
-//   u[1] mass exx, eyx, ezx, eyy        u[5] motor speed, dt * max motor torque, body A, body B
-//   u[2] ezy, ezz, angular mass, flags | limit state << 8
-//   u[3] impulse x, y, z, angular motor impulse (updated by the sweeps)
-//   u[6] local anchor A - local center A, local anchor B - local center B
-//   u[7] reference angle, lower angle, upper angle, -
-struct Jc {
-    V2 rA, rB;
-    float exx, eyx, ezx, eyy, ezy, ezz, angularMass;
-    std::uint32_t flags, limit;
-    float ix, iy, iz, motorImp;
-    BodyC A, B;
-    float motorSpeed, maxImpulse;
-    int bA, bB;
-};
-
-__device__ __forceinline__ Jc load_jc_velocity(const float4* u) {
-    const float4 r = u[0], m = u[1], n = u[2], im = u[3], ms = u[4], mo = u[5];
-    Jc j;
-    j.rA = V2{r.x, r.y};
-    j.rB = V2{r.z, r.w};
-    j.exx = m.x;
-    j.eyx = m.y;
-    j.ezx = m.z;
-    j.eyy = m.w;
-    j.ezy = n.x;
-    j.ezz = n.y;
-    j.angularMass = n.z;
-    j.flags = __float_as_uint(n.w) & 0xffu;
-    j.limit = __float_as_uint(n.w) >> 8;
-    j.ix = im.x;
-    j.iy = im.y;
-    j.iz = im.z;
-    j.motorImp = im.w;
-    j.A = BodyC{ms.x, ms.y, V2{0, 0}};
-    j.B = BodyC{ms.z, ms.w, V2{0, 0}};
-    j.motorSpeed = mo.x;
-    j.maxImpulse = mo.y;
-    j.bA = __float_as_int(mo.z);
-    j.bB = __float_as_int(mo.w);
-    return j;
-}
-
-// Solve33 / Solve22 of the symmetric joint mass (Math.hpp:446-474); vectors
-// GetX = (exx, eyx, ezx), GetY = (eyx, eyy, ezy), GetZ = (ezx, ezy, ezz).
-__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
-    float r = 0.0f;
-    r += ax * bx;
-    r += ay * by;
-    r += az * bz;
-    return r;
-}
-__device__ __forceinline__ void cross3(float ax, float ay, float az, float bx, float by, float bz, float& cx,
-                                       float& cy, float& cz) {
-    cx = ay * bz - az * by;
-    cy = az * bx - ax * bz;
-    cz = ax * by - ay * bx;
-}
-__device__ __forceinline__ void solve33(const Jc& j, float bx, float by, float bz, float& ox, float& oy, float& oz) {
-    float yzx, yzy, yzz;
-    cross3(j.eyx, j.eyy, j.ezy, j.ezx, j.ezy, j.ezz, yzx, yzy, yzz);
-    const float dp = dot3(j.exx, j.eyx, j.ezx, yzx, yzy, yzz);
-    const float det = (dp != 0.0f) ? (1.0f / dp) : dp;
-    ox = det * dot3(bx, by, bz, yzx, yzy, yzz);
-    float bzx, bzy, bzz;
-    cross3(bx, by, bz, j.ezx, j.ezy, j.ezz, bzx, bzy, bzz);
-    oy = det * dot3(j.exx, j.eyx, j.ezx, bzx, bzy, bzz);
-    float ybx, yby, ybz;
-    cross3(j.eyx, j.eyy, j.ezy, bx, by, bz, ybx, yby, ybz);
-    oz = det * dot3(j.exx, j.eyx, j.ezx, ybx, yby, ybz);
-}
-__device__ __forceinline__ V2 solve22(const Jc& j, V2 b) {
-    const float xx = j.exx, xy = j.eyx, yx = j.eyx, yy = j.eyy;
-    const float cp = xx * yy - xy * yx;
-    const float det = (cp != 0.0f) ? (1.0f / cp) : cp;
-    return V2{det * (yy * b.x - xy * b.y), det * (xx * b.y - yx * b.x)};
-}
-
-// Warm-start half of InitVelocity (RevoluteJointConf.cpp:199-224).
-__device__ __forceinline__ void joint_warm_start(const Jc& j, float4& velA, float4& velB) {
-    const V2 P{j.ix, j.iy};
-    const float L = j.motorImp + j.iz;
-    const float LA = cross(j.rA, P) / 1.0f + L;
-    const float LB = cross(j.rB, P) / 1.0f + L;
-    velA.x = velA.x - j.A.invMass * P.x;
-    velA.y = velA.y - j.A.invMass * P.y;
-    velA.z = velA.z - j.A.invRotI * LA;
-    velB.x = velB.x + j.B.invMass * P.x;
-    velB.y = velB.y + j.B.invMass * P.y;
-    velB.z = velB.z + j.B.invRotI * LB;
-}
-
-// SolveVelocity (RevoluteJointConf.cpp:228-331).  Returns true if the bodies'
-// velocities were left unchanged.
-__device__ __forceinline__ bool joint_solve_velocity(Jc& j, float4& velA, float4& velB) {
-    const float4 oldA = velA, oldB = velB;
-    const float mA = j.A.invMass, iA = j.A.invRotI, mB = j.B.invMass, iB = j.B.invRotI;
-    const bool fixedRotation = (iA + iB == 0.0f);
-    if ((j.flags & prx::kJointMotor) && j.limit != 3u && !fixedRotation) {
-        const float impulse = -j.angularMass * (velB.z - velA.z - j.motorSpeed);
-        const float oldImpulse = j.motorImp;
-        j.motorImp = clamp_std(j.motorImp + impulse, -j.maxImpulse, j.maxImpulse);
-        const float inc = j.motorImp - oldImpulse;
-        velA.z = velA.z - iA * inc;
-        velB.z = velB.z + iB * inc;
-    }
-    const V2 vb = V2{velB.x, velB.y} + mul(revperp(j.rB), velB.z / 1.0f);
-    const V2 va = V2{velA.x, velA.y} + mul(revperp(j.rA), velA.z / 1.0f);
-    const V2 vDelta = vb - va;
-    V2 P;
-    float L = 0.0f;
-    if ((j.flags & prx::kJointLimit) && j.limit != 0u && !fixedRotation) {
-        float sx, sy, sz;
-        solve33(j, vDelta.x / 1.0f, vDelta.y / 1.0f, (velB.z - velA.z) / 1.0f, sx, sy, sz);
-        float ix = -sx, iy = -sy, iz = -sz;
-        const bool lowerCase = j.limit == 1u, upperCase = j.limit == 2u;
-        const float newZ = j.iz + iz;
-        if ((lowerCase && newZ < 0.0f) || (upperCase && newZ > 0.0f)) {
-            const V2 rhs = -V2{vDelta.x / 1.0f, vDelta.y / 1.0f} + mul(j.iz, V2{j.ezx, j.ezy});
-            const V2 reduced = solve22(j, rhs);
-            ix = reduced.x;
-            iy = reduced.y;
-            iz = -j.iz;
-            j.ix = j.ix + reduced.x;
-            j.iy = j.iy + reduced.y;
-            j.iz = 0.0f;
-        } else {
-            j.ix = j.ix + ix;
-            j.iy = j.iy + iy;
-            j.iz = j.iz + iz;
-        }
-        P = V2{ix, iy};
-        L = iz;
-        const float LA = cross(j.rA, P) / 1.0f + L;
-        const float LB = cross(j.rB, P) / 1.0f + L;
-        velA.x = velA.x - mA * P.x;
-        velA.y = velA.y - mA * P.y;
-        velA.z = velA.z - iA * LA;
-        velB.x = velB.x + mB * P.x;
-        velB.y = velB.y + mB * P.y;
-        velB.z = velB.z + iB * LB;
-    } else {
-        const V2 imp = solve22(j, -V2{vDelta.x / 1.0f, vDelta.y / 1.0f});
-        j.ix = j.ix + imp.x;
-        j.iy = j.iy + imp.y;
-        P = imp;
-        const float LA = cross(j.rA, P) / 1.0f;
-        const float LB = cross(j.rB, P) / 1.0f;
-        velA.x = velA.x - mA * P.x;
-        velA.y = velA.y - mA * P.y;
-        velA.z = velA.z - iA * LA;
-        velB.x = velB.x + mB * P.x;
-        velB.y = velB.y + mB * P.y;
-        velB.z = velB.z + iB * LB;
-    }
-    const bool same = velA.x == oldA.x && velA.y == oldA.y && velA.z == oldA.z && velB.x == oldB.x &&
-                      velB.y == oldB.y && velB.z == oldB.z;
-    if (same) {
-        velA = oldA;
-        velB = oldB;
-    }
-    return same;
-}
-
-// SolvePosition (RevoluteJointConf.cpp:333-433).  Returns whether the joint
-// is within tolerance.
-__device__ __forceinline__ bool joint_solve_position(const float4* u, float4& posA, float4& posB, const StepParams& c) {
-    const float4 n = u[2], ms = u[4], la = u[6], ang = u[7];
-    const float angularMass = n.z;
-    const std::uint32_t flags = __float_as_uint(n.w) & 0xffu, limit = __float_as_uint(n.w) >> 8;
-    const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
-    const bool fixedRotation = ((iA + iB) == 0.0f);
-    float angularError = 0.0f;
-    if ((flags & prx::kJointLimit) && limit != 0u && !fixedRotation) {
-        const float angle = posB.z - posA.z - ang.x;
-        float limitImpulse = 0.0f;
-        if (limit == 1u) {
-            float C = angle - ang.y;
-            angularError = -C;
-            C = clamp_std(C + c.angularSlop, -c.maxAngularCorrection, 0.0f);
-            limitImpulse = -angularMass * C;
-        } else if (limit == 2u) {
-            float C = angle - ang.z;
-            angularError = C;
-            C = clamp_std(C - c.angularSlop, 0.0f, c.maxAngularCorrection);
-            limitImpulse = -angularMass * C;
-        } else {
-            const float C = clamp_std(angle - ang.y, -c.maxAngularCorrection, c.maxAngularCorrection);
-            limitImpulse = -angularMass * C;
-            angularError = __builtin_fabsf(C);
-        }
-        posA.z = posA.z - iA * limitImpulse;
-        posB.z = posB.z + iB * limitImpulse;
-    }
-    float positionError;
-    {
-        const V2 qA = unit_from_angle(posA.z);
-        const V2 qB = unit_from_angle(posB.z);
-        const V2 rA = rotate(V2{la.x, la.y}, qA);
-        const V2 rB = rotate(V2{la.z, la.w}, qB);
-        const V2 C = (V2{posB.x, posB.y} + rB) - (V2{posA.x, posA.y} + rA);
-        positionError = magsq(C);
-        const float exx = mA + (iA * (rA.y * rA.y) / 1.0f) + mB + (iB * (rB.y * rB.y) / 1.0f);
-        const float exy = (-iA * rA.x * rA.y / 1.0f) + (-iB * rB.x * rB.y / 1.0f);
-        const float eyy = mA + (iA * (rA.x * rA.x) / 1.0f) + mB + (iB * (rB.x * rB.x) / 1.0f);
-        V2 sol{0.0f, 0.0f};
-        const float cp = cross(V2{exx, exy}, V2{exy, eyy});
-        if (!almost_zero(cp)) {
-            const float inverse = 1.0f / cp;
-            sol = V2{(eyy * C.x - exy * C.y) * inverse, (exx * C.y - exy * C.x) * inverse};
-        }
-        const V2 P = -sol;
-        posA.x = posA.x - mA * P.x;
-        posA.y = posA.y - mA * P.y;
-        posA.z = posA.z - iA * cross(rA, P) / 1.0f;
-        posB.x = posB.x + mB * P.x;
-        posB.y = posB.y + mB * P.y;
-        posB.z = posB.z + iB * cross(rB, P) / 1.0f;
-    }
-    return (positionError <= c.linearSlop * c.linearSlop) && (angularError <= c.angularSlop);
-}
-
-// Per-step joint set-up half of InitVelocity (RevoluteJointConf.cpp:127-197),
-// writes the joint's solver record.
-__device__ __forceinline__ void build_jc(const DeviceState& d, const WorldCtx& x, int jg, float4* u, const BodyC& A,
-                                         const float4& pA, const BodyC& B, const float4& pB, int bA, int bB,
-                                         const StepParams& c) {
-    const float4 an = d.j_anchor[jg], cf = d.j_cfg[jg], cf2 = d.j_cfg2[jg];
-    float4 im = d.j_imp[jg];
-    std::uint32_t limit = d.j_limit[jg];
-    const std::uint32_t flags = __float_as_uint(cf2.y);
-    const V2 laA = V2{an.x, an.y} - A.lc;
-    const V2 laB = V2{an.z, an.w} - B.lc;
-    const V2 qA = unit_from_angle(pA.z);
-    const V2 qB = unit_from_angle(pB.z);
-    const V2 rA = rotate(laA, qA);
-    const V2 rB = rotate(laB, qB);
-    const float mA = A.invMass, iA = A.invRotI, mB = B.invMass, iB = B.invRotI;
-    const float totInvI = iA + iB;
-    const bool fixedRotation = (totInvI == 0.0f);
-    // GetMat33 (RevoluteJointConf.cpp:36-62).
-    const float exx = mA + ((rA.y * rA.y) * iA / 1.0f) + mB + ((rB.y * rB.y) * iB / 1.0f);
-    const float eyx = (-rA.y * rA.x * iA / 1.0f) + (-rB.y * rB.x * iB / 1.0f);
-    const float ezx = (-rA.y * iA) + (-rB.y * iB);
-    const float eyy = mA + ((rA.x * rA.x) * iA / 1.0f) + mB + ((rB.x * rB.x) * iB / 1.0f);
-    const float ezy = (rA.x * iA) + (rB.x * iB);
-    const float angularMass = (totInvI > 0.0f) ? (1.0f / totInvI) : 0.0f;
-    if (!(flags & prx::kJointMotor) || fixedRotation) im.w = 0.0f;
-    if ((flags & prx::kJointLimit) && !fixedRotation) {
-        const float jointAngle = pB.z - pA.z - cf.x;
-        if (__builtin_fabsf(cf.z - cf.y) < (c.angularSlop * 2.0f)) {
-            limit = 3u;
-        } else if (jointAngle <= cf.y) {
-            if (limit != 1u) {
-                limit = 1u;
-                im.z = 0.0f;
-            }
-        } else if (jointAngle >= cf.z) {
-            if (limit != 2u) {
-                limit = 2u;
-                im.z = 0.0f;
-            }
-        } else {
-            limit = 0u;
-            im.z = 0.0f;
-        }
-    } else {
-        limit = 0u;
-    }
-    if (c.warmStart) {
-        im.x = im.x * c.dtRatio;
-        im.y = im.y * c.dtRatio;
-        im.z = im.z * c.dtRatio;
-        im.w = im.w * c.dtRatio;
-    } else {
-        im = float4{0.0f, 0.0f, 0.0f, 0.0f};
-    }
-    u[0] = float4{rA.x, rA.y, rB.x, rB.y};
-    u[1] = float4{exx, eyx, ezx, eyy};
-    u[2] = float4{ezy, totInvI, angularMass, __uint_as_float(flags | (limit << 8))};
-    u[3] = im;
-    u[4] = float4{mA, iA, mB, iB};
-    u[5] = float4{cf.w, c.dt * cf2.x, __int_as_float(bA), __int_as_float(bB)};
-    u[6] = float4{laA.x, laA.y, laB.x, laB.y};
-    u[7] = float4{cf.x, cf.y, cf.z, 0.0f};
-}
-
-__device__ __forceinline__ BodyC load_bodyc(const DeviceState& d, int gb) {
-    const float4 l = d.lcinv[gb];
-    return BodyC{l.z, l.w, V2{l.x, l.y}};
-}
-
-// GetVelocity(body, h) + Cap (Body.cpp:290-309, Velocity.cpp:27-45).
-__device__ __forceinline__ float4 integrated_velocity(const DeviceState& d, int gb, const StepParams& c) {
-    const float4 v = d.vel[gb];
-    V2 lin{v.x, v.y};
-    float ang = v.z;
-    const float h = c.dt;
-    if (d.bflags[gb] & prx::kAccelerable) {
-        const float4 a = d.acc[gb];
-        const float2 dm = d.damp[gb];
-        lin = lin + mul(h, V2{a.x, a.y});
-        ang = ang + h * a.z;
-        const float linv = 1.0f / (1.0f + h * dm.x);
-        lin = V2{lin.x * linv, lin.y * linv};
-        ang = ang / (1.0f + h * dm.y);
-    }
-    const V2 tr = mul(h, lin);
-    const float lsq = magsq(tr);
-    if (lsq > c.maxTranslation * c.maxTranslation) {
-        const float ratio = c.maxTranslation / __builtin_sqrtf(lsq);
-        lin = V2{lin.x * ratio, lin.y * ratio};
-    }
-    const float absRot = __builtin_fabsf(h * ang);
-    if (absRot > c.maxRotation) {
-        const float ratio = c.maxRotation / absRot;
-        ang = ang * ratio;
-    }
-    return float4{lin.x, lin.y, ang, 0.0f};
-}
-
-// ---------------------------------------------------------------------------
-// SolveReg (AabbTreeWorld.cpp:1472-1561) and SolveRegIslandViaGS (:1563-1682).
-//
-// Only two things in an island solve depend on order: the island's contact
-// order (the DFS) and the Gauss-Seidel sweeps over it.  Everything else is
-// independent per body or per contact and runs lane-parallel over all islands
-// of the world at once.  Static bodies are left out of island body lists: the
-// reference adds them but they never propagate the search, their solver
-// updates are exact no-ops (zero inverse mass and velocity) and they are not
-// written back, so islands become disjoint in every body.
-//
-// Body-constraint positions and velocities live in LDS when the world fits
-// (32 bytes per body), else in global scratch.
-
-extern __shared__ float4 g_smem[];
-
-template <bool kLds>
-struct Bodies {
-    float4* gpos;
-    float4* gvel;
-    int nb;
-    __device__ __forceinline__ float4 pos(int b) const {
-        if constexpr (kLds) return g_smem[b];
-        else return gpos[b];
-    }
-    __device__ __forceinline__ float4 vel(int b) const {
-        if constexpr (kLds) return g_smem[nb + b];
-        else return gvel[b];
-    }
-    __device__ __forceinline__ void set_pos(int b, float4 v) const {
-        if constexpr (kLds) g_smem[b] = v;
-        else gpos[b] = v;
-    }
-    __device__ __forceinline__ void set_vel(int b, float4 v) const {
-        if constexpr (kLds) g_smem[nb + b] = v;
-        else gvel[b] = v;
-    }
-};
-
-// Inclusive wave scan of a 64-lane value.
-__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
-    for (int o = 1; o < kWave; o <<= 1) {
-        const int t = __shfl_up(v, o);
-        if (lane >= o) v += t;
-    }
-    return v;
-}
-
-// Exclusive prefix sum of a[0..n) in place (wave-parallel, carry across chunks);
-// returns the total.
-__device__ __forceinline__ int wave_exclusive_scan(int* a, int n, int lane) {
-    int carry = 0;
-    for (int base = 0; base < n; base += kWave) {
-        const int i = base + lane;
-        const int v = (i < n) ? a[i] : 0;
-        const int inc = wave_incl_scan(v, lane);
-        if (i < n) a[i] = carry + inc - v;
-        carry += __shfl(inc, kWave - 1);
-    }
-    return carry;
-}
-
-struct IslandScratch {
-    int* off;         // per-body CSR offsets (nb + 1)
-    int2* csr;        // per-body touching contacts: contact index, other body
-    int* ibody;       // island body lists
-    int* icont;       // island contact lists (contact indices)
-    int* ijoint;      // island joint lists (joint indices)
-    int2* istart;     // island i: first body entry, first contact entry
-    int* jstart;      // island i: first joint entry
-    int* bisland;     // island of each body (-1: none)
-    std::uint32_t* isl_word;  // per island: bit 0 solved, bits 1..: unused; min under-active time bits
-    std::uint32_t* isl_under;
-};
-
-// Per-body CSR of touching, enabled, non-sensor contacts in contact order
-// (each body's contact list order), entries carry the other body.
-__device__ __forceinline__ void build_adjacency(const DeviceState& d, const WorldCtx& x, int nc, const IslandScratch& S) {
-    for (int b = x.lane; b <= x.nb; b += kWave) S.off[b] = 0;
-    __syncthreads();
-    for (int i = x.lane; i < nc; i += kWave) {
-        const std::uint32_t f = d.c_flags[x.co + i];
-        if ((f & prx::kCEnabled) && (f & prx::kTouching) && !(f & prx::kSensor)) {
-            const int2 bd = d.c_body[x.co + i];
-            atomicAdd(&S.off[bd.x], 1);
-            atomicAdd(&S.off[bd.y], 1);
-        }
-    }
-    __syncthreads();
-    wave_exclusive_scan(S.off, x.nb + 1, x.lane);
-    __syncthreads();
-    int* cur = d.s_stack + 2 * x.bo;
-    for (int b = x.lane; b < x.nb; b += kWave) cur[b] = S.off[b];
-    __syncthreads();
-    for (int i = x.lane; i < nc; i += kWave) {
-        const std::uint32_t f = d.c_flags[x.co + i];
-        if ((f & prx::kCEnabled) && (f & prx::kTouching) && !(f & prx::kSensor)) {
-            const int2 bd = d.c_body[x.co + i];
-            S.csr[atomicAdd(&cur[bd.x], 1)] = int2{i, bd.y};
-            S.csr[atomicAdd(&cur[bd.y], 1)] = int2{i, bd.x};
-        }
-    }
-    __syncthreads();
-    for (int b = x.lane; b < x.nb; b += kWave) {
-        const int s0 = S.off[b], e0 = S.off[b + 1];
-        for (int i = s0 + 1; i < e0; ++i) {
-            const int2 v = S.csr[i];
-            int j = i - 1;
-            while (j >= s0 && S.csr[j].x > v.x) {
-                S.csr[j + 1] = S.csr[j];
-                --j;
-            }
-            S.csr[j + 1] = v;
-        }
-    }
-    __syncthreads();
-}
-
-// Islands (AddToIsland, AabbTreeWorld.cpp:1358-1444; seeds in body order
-// :1495-1523).  Awake speedable bodies without touching contacts are
-// one-body islands and are emitted lane-parallel; the rest are found by a
-// depth-first search on one lane.  kMode 2: adjacency and flags in LDS,
-// 1: flags in LDS, 0: everything in global scratch.  Flag bytes per body:
-// 0 free, 1 islanded, 2 static (never pushed).  Returns the island count.
-template <int kMode>
-__device__ __forceinline__ int find_islands(const DeviceState& d, const WorldCtx& x, int nc, const IslandScratch& S) {
-    __shared__ int s_cnt[2];
-    const int nent = S.off[x.nb];
-    int* off = S.off;
-    int2* csr = S.csr;
-    std::uint8_t* islb;
-    std::uint8_t* islc;
-    std::uint8_t* islj;
-    if constexpr (kMode == 2) {
-        int* loff = reinterpret_cast<int*>(g_smem);
-        int2* lcsr = reinterpret_cast<int2*>(g_smem) + ((x.nb + 2) >> 1);
-        for (int b = x.lane; b <= x.nb; b += kWave) loff[b] = S.off[b];
-        for (int e = x.lane; e < nent; e += kWave) lcsr[e] = S.csr[e];
-        off = loff;
-        csr = lcsr;
-        islb = reinterpret_cast<std::uint8_t*>(lcsr + nent);
-        islc = islb + x.nb;
-        islj = islc + nc;
-    } else if constexpr (kMode == 1) {
-        islb = reinterpret_cast<std::uint8_t*>(g_smem);
-        islc = islb + x.nb;
-        islj = islc + nc;
-    } else {
-        islb = d.s_isl_b + x.bo;
-        islc = d.s_isl_c + x.co;
-        islj = d.s_isl_j + x.jo;
-    }
-    for (int b = x.lane; b < x.nb; b += kWave) {
-        islb[b] = (d.bflags[x.bo + b] & prx::kSpeedable) ? 0 : 2;
-        S.bisland[b] = -1;
-    }
-    for (int i = x.lane; i < nc; i += kWave) islc[i] = 0;
-    for (int i = x.lane; i < x.nj; i += kWave) islj[i] = 0;
-    __syncthreads();
-    // One-body islands.
-    int nsingle = 0;
-    for (int base = 0; base < x.nb; base += kWave) {
-        const int b = base + x.lane;
-        bool single = false;
-        if (b < x.nb) {
-            const std::uint32_t f = d.bflags[x.bo + b];
-            single = (f & prx::kAwake) && (f & prx::kEnabled) && (f & prx::kSpeedable) && off[b] == off[b + 1] &&
-                     (x.nj == 0 || d.bjoint[x.bo + b].y == 0);
-        }
-        const unsigned long long m = __ballot(single);
-        if (single) {
-            const int r = nsingle + __popcll(m & lanes_below(x.lane));
-            S.ibody[r] = b;
-            S.istart[r] = int2{r, 0};
-            S.jstart[r] = 0;
-            S.bisland[b] = r;
-            islb[b] = 1;
-        }
-        nsingle += __popcll(m);
-    }
-    __syncthreads();
-    if (x.lane == 0) {
-        int nisl = nsingle, nbe = nsingle, nce = 0, nje = 0;
-        int* stack = d.s_stack + 2 * x.bo;
-        for (int seed = 0; seed < x.nb; ++seed) {
-            if (islb[seed]) continue;
-            const std::uint32_t sf = d.bflags[x.bo + seed];
-            if (!((sf & prx::kAwake) && (sf & prx::kEnabled))) continue;
-            S.istart[nisl] = int2{nbe, nce};
-            S.jstart[nisl] = nje;
-            int sp = 0;
-            stack[sp++] = seed;
-            islb[seed] = 1;
-            while (sp > 0) {
-                const int bb = stack[--sp];
-                S.ibody[nbe++] = bb;
-                S.bisland[bb] = nisl;
-                const int e1 = off[bb + 1];
-                for (int e = off[bb]; e < e1; ++e) {
-                    const int2 en = csr[e];
-                    if (islc[en.x]) continue;
-                    S.icont[nce++] = en.x;
-                    islc[en.x] = 1;
-                    if (islb[en.y] == 0) {
-                        islb[en.y] = 1;
-                        stack[sp++] = en.y;
-                    }
-                }
-                // AddJointsToIsland (AabbTreeWorld.cpp:1446-1470).
-                if (x.nj) {
-                    const int2 bj = d.bjoint[x.bo + bb];
-                    for (int k = 0; k < bj.y; ++k) {
-                        const int2 je = d.bjoint_list[2 * x.jo + bj.x + k];
-                        if (islj[je.x]) continue;
-                        if (!(d.bflags[x.bo + je.y] & prx::kEnabled)) continue;
-                        islj[je.x] = 1;
-                        S.ijoint[nje++] = je.x;
-                        if (islb[je.y] == 0) {
-                            islb[je.y] = 1;
-                            stack[sp++] = je.y;
-                        }
-                    }
-                }
-            }
-            ++nisl;
-        }
-        S.istart[nisl] = int2{nbe, nce};
-        S.jstart[nisl] = nje;
-        s_cnt[0] = nisl;
-    }
-    __syncthreads();
-    return s_cnt[0];
-}
-
-
-// Position-constraint record (see the layout above).
-struct Pc {
-    Manifold m;
-    int bA, bB;
-    float totalRadius;
-    BodyC A, B;
-};
-
-__device__ __forceinline__ void store_pc(float4* u, const Manifold& m, int bA, int bB, float tr, const BodyC& A,
-                                         const BodyC& B) {
-    u[0] = float4{__uint_as_float(m.type | (m.n << 8)), __int_as_float(bA), __int_as_float(bB), tr};
-    u[1] = float4{m.ln.x, m.ln.y, m.lpt.x, m.lpt.y};
-    u[2] = float4{m.p[0].lp.x, m.p[0].lp.y, m.p[1].lp.x, m.p[1].lp.y};
-    u[3] = float4{A.invMass, A.invRotI, A.lc.x, A.lc.y};
-    u[4] = float4{B.invMass, B.invRotI, B.lc.x, B.lc.y};
-}
-__device__ __forceinline__ Pc load_pc(const float4* u) {
-    const float4 a = u[0], g = u[1], q = u[2], ma = u[3], mb = u[4];
-    Pc pc;
-    const std::uint32_t h = __float_as_uint(a.x);
-    pc.m.type = h & 0xffu;
-    pc.m.n = (h >> 8) & 0xffu;
-    pc.m.ln = V2{g.x, g.y};
-    pc.m.lpt = V2{g.z, g.w};
-    pc.m.p[0] = MPoint{V2{q.x, q.y}, 0u, 0.0f, 0.0f};
-    pc.m.p[1] = MPoint{V2{q.z, q.w}, 0u, 0.0f, 0.0f};
-    pc.bA = __float_as_int(a.y);
-    pc.bB = __float_as_int(a.z);
-    pc.totalRadius = a.w;
-    pc.A = BodyC{ma.x, ma.y, V2{ma.z, ma.w}};
-    pc.B = BodyC{mb.x, mb.y, V2{mb.z, mb.w}};
-    return pc;
-}
-
-// Gauss-Seidel sweeps of one island.  Two forms with identical arithmetic:
-//
-// * lane form (island_velocity / island_position): one lane walks one
-//   island's records, the next record loaded while the current one is
-//   solved; many small islands proceed side by side on the wave's lanes;
-// * wave form (island_velocity_wave / island_position_wave): the whole wave
-//   serves one large island.  Records are staged through LDS in chunks of
-//   kChunk; all lanes load the next chunk from global memory (coalesced) while
-//   lane 0 solves the current chunk out of LDS, so the serial chain waits on
-//   LDS, not on global memory.
-//
-// Velocity-record impulses are written back to global memory chunk by chunk
-// (every later read of a record comes after its write-back in the wave's
-// program order).
-
-// Warm start of one velocity constraint (WarmStartVelocities,
-// AabbTreeWorld.cpp:230-274).
-template <bool kLds>
-__device__ __forceinline__ void warm_start_one(const Vc& vc, const BodyC& A, const BodyC& B, const Bodies<kLds>& bs) {
-    const V2 tangent = fwdperp(vc.normal);
-    V2 dAl{0, 0}, dBl{0, 0};
-    float dAa = 0.0f, dBa = 0.0f;
-#pragma unroll
-    for (std::uint32_t j = 0; j < 2; ++j) {
-        if (j >= vc.n) continue;
-        const VcPoint& p = vc.p[j];
-        const V2 P = mul(p.ni, vc.normal) + mul(p.ti, tangent);
-        const float LA = cross(p.relA, P) / 1.0f;
-        const float LB = cross(p.relB, P) / 1.0f;
-        dAl = dAl - mul(A.invMass, P);
-        dAa = dAa - A.invRotI * LA;
-        dBl = dBl + mul(B.invMass, P);
-        dBa = dBa + B.invRotI * LB;
-    }
-    float4 va = bs.vel(vc.bA);
-    va.x = va.x + dAl.x;
-    va.y = va.y + dAl.y;
-    va.z = va.z + dAa;
-    bs.set_vel(vc.bA, va);
-    float4 vb = bs.vel(vc.bB);
-    vb.x = vb.x + dBl.x;
-    vb.y = vb.y + dBl.y;
-    vb.z = vb.z + dBa;
-    bs.set_vel(vc.bB, vb);
-}
-
-// GaussSeidel::SolveVelocityConstraint (ContactSolver.cpp:481-492): tangent
-// first, then normal (block solve for two points).  Returns the largest
-// impulse increment.
-template <bool kLds>
-__device__ __forceinline__ float solve_velocity_one(Vc& vc, const BodyC& A, const BodyC& B, const Bodies<kLds>& bs) {
-    float4 va = bs.vel(vc.bA);
-    float4 vb = bs.vel(vc.bB);
-    float mi = 0.0f;
-    mi = fmax_std(mi, solve_dir(vc, true, A, B, va, vb));
-    float nrm;
-    if (vc.n == 1 || (vc.K00 == 0.0f && vc.K11 == 0.0f && vc.K01 == 0.0f)) nrm = solve_dir(vc, false, A, B, va, vb);
-    else nrm = block_solve(vc, A, B, va, vb);
-    mi = fmax_std(mi, nrm);
-    bs.set_vel(vc.bA, va);
-    bs.set_vel(vc.bB, vb);
-    return mi;
-}
-
-// One pass over an island's joints in island order: the warm start of
-// InitVelocity, or SolveVelocity (returns whether every joint left its
-// bodies' velocities unchanged).
-template <bool kLds>
-__device__ __forceinline__ bool joints_velocity(float4* jbase, int j0, int j1, const Bodies<kLds>& bs, bool warm) {
-    bool ok = true;
-    for (int e = j0; e < j1; ++e) {
-        float4* u = jbase + std::size_t(e) * prx::kJcStride;
-        Jc j = load_jc_velocity(u);
-        float4 va = bs.vel(j.bA);
-        float4 vb = bs.vel(j.bB);
-        if (warm) {
-            joint_warm_start(j, va, vb);
-        } else {
-            ok &= joint_solve_velocity(j, va, vb);
-            u[3] = float4{j.ix, j.iy, j.iz, j.motorImp};
-        }
-        bs.set_vel(j.bA, va);
-        bs.set_vel(j.bB, vb);
-    }
-    return ok;
-}
-
-// One SolvePosition pass over an island's joints; returns whether all are
-// within tolerance.
-template <bool kLds>
-__device__ __forceinline__ bool joints_position(const float4* jbase, int j0, int j1, const Bodies<kLds>& bs,
-                                                const StepParams& c) {
-    bool ok = true;
-    for (int e = j0; e < j1; ++e) {
-        const float4* u = jbase + std::size_t(e) * prx::kJcStride;
-        const float4 mo = u[5];
-        const int bA = __float_as_int(mo.z), bB = __float_as_int(mo.w);
-        float4 pa = bs.pos(bA);
-        float4 pb = bs.pos(bB);
-        ok &= joint_solve_position(u, pa, pb, c);
-        bs.set_pos(bA, pa);
-        bs.set_pos(bB, pb);
-    }
-    return ok;
-}
-
-// Lane form of the warm start and velocity sweeps of one island
-// (AabbTreeWorld.cpp:1590-1625): contact warm start, joint warm start, then
-// per iteration the joints followed by the contacts.
-template <bool kLds>
-__device__ __forceinline__ void island_velocity(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
-                                                int c0, int c1, int j0, int j1, const StepParams& c) {
-    if (c0 >= c1 && j0 >= j1) return;
-    float4* base = d.s_vc + std::size_t(x.co) * prx::kVcStride;
-    float4* jbase = d.s_jc + std::size_t(x.jo) * prx::kJcStride;
-    if (c.warmStart && c0 < c1) {
-        BodyC A, B;
-        Vc vc = load_vc(base + std::size_t(c0) * prx::kVcStride, A, B);
-        for (int e = c0; e < c1; ++e) {
-            const int en = (e + 1 < c1) ? e + 1 : e;
-            BodyC nA, nB;
-            const Vc nvc = load_vc(base + std::size_t(en) * prx::kVcStride, nA, nB);
-            warm_start_one(vc, A, B, bs);
-            vc = nvc;
-            A = nA;
-            B = nB;
-        }
-    }
-    if (c.warmStart) joints_velocity(jbase, j0, j1, bs, true);
-    for (std::uint32_t it = 0; it < c.velIters; ++it) {
-        const bool jointsOkay = joints_velocity(jbase, j0, j1, bs, false);
-        float newInc = 0.0f;
-        if (c0 >= c1) {
-            if (jointsOkay && newInc <= c.regMinMomentum) break;
-            continue;
-        }
-        BodyC A, B;
-        Vc vc = load_vc(base + std::size_t(c0) * prx::kVcStride, A, B);
-        for (int e = c0; e < c1; ++e) {
-            const int en = (e + 1 < c1) ? e + 1 : e;
-            BodyC nA, nB;
-            const Vc nvc = load_vc(base + std::size_t(en) * prx::kVcStride, nA, nB);
-            const float mi = solve_velocity_one(vc, A, B, bs);
-            float4* rec = base + std::size_t(e) * prx::kVcStride;
-            rec[4] = float4{vc.p[0].ni, vc.p[0].ti, vc.p[0].nm, vc.p[0].tm};
-            rec[7] = float4{vc.p[1].ni, vc.p[1].ti, vc.p[1].nm, vc.p[1].tm};
-            newInc = fmax_std(newInc, mi);
-            vc = nvc;
-            A = nA;
-            B = nB;
-        }
-        if (jointsOkay && newInc <= c.regMinMomentum) break;
-    }
-}
-
-// Chunk staging of kStride-float4 records [c0, c1) of a global array through
-// registers into the LDS stage: fetch issues the loads, commit stores them.
-template <int kStride>
-struct Stager {
-    static constexpr int kFloats = kChunk * kStride;
-    static constexpr int kPer = (kFloats + kWave - 1) / kWave;
-    const float4* src;  // first record of the island
-    float4* stage;
-    int n;              // records in the island
-    int lane;
-    float4 reg[kPer];
-    __device__ __forceinline__ int limit(int ch) const {
-        const int e = (ch + 1) * kChunk;
-        return ((e < n) ? e : n) * kStride - ch * kFloats;
-    }
-    __device__ __forceinline__ void fetch(int ch) {
-        const int lim = limit(ch);
-        const float4* s = src + std::size_t(ch) * kFloats;
-#pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            const int f = lane + k * kWave;
-            reg[k] = (f < lim) ? s[f] : float4{0.0f, 0.0f, 0.0f, 0.0f};
-        }
-    }
-    __device__ __forceinline__ void commit(int ch) {
-        const int lim = limit(ch);
-#pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            const int f = lane + k * kWave;
-            if (f < lim) stage[f] = reg[k];
-        }
-    }
-};
-
-template <bool kLds>
-__device__ __forceinline__ void island_velocity_wave(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
-                                                     float4* stage, int c0, int c1, int j0, int j1,
-                                                     const StepParams& c) {
-    float4* jbase = d.s_jc + std::size_t(x.jo) * prx::kJcStride;
-    float4* gbase = d.s_vc + (std::size_t(x.co) + std::size_t(c0)) * prx::kVcStride;
-    const int n = c1 - c0;
-    const int nch = (n + kChunk - 1) / kChunk;
-    const bool resident = nch == 1;  // the whole island stays in the stage
-    Stager<prx::kVcStride> sg{gbase, stage, n, x.lane, {}};
-    sg.fetch(0);
-    sg.commit(0);
-    __syncthreads();
-    const int npass = int(c.velIters) + (c.warmStart ? 1 : 0);
-    for (int pass = 0; pass < npass; ++pass) {
-        const bool warm = c.warmStart && pass == 0;
-        float newInc = 0.0f;
-        bool jointsOkay = true;
-        if (!warm && x.lane == 0) jointsOkay = joints_velocity(jbase, j0, j1, bs, false);
-        for (int ch = 0; ch < nch; ++ch) {
-            const int nx = (ch + 1 < nch) ? ch + 1 : 0;
-            const bool more = !resident && (ch + 1 < nch || pass + 1 < npass);
-            if (more) sg.fetch(nx);
-            const int r1 = ((ch + 1) * kChunk < n ? kChunk : n - ch * kChunk);
-            if (x.lane == 0) {
-                BodyC A, B;
-                Vc vc = load_vc(stage, A, B);
-                for (int r = 0; r < r1; ++r) {
-                    const int rn = (r + 1 < r1) ? r + 1 : r;
-                    BodyC nA, nB;
-                    const Vc nvc = load_vc(stage + rn * prx::kVcStride, nA, nB);
-                    if (warm) {
-                        warm_start_one(vc, A, B, bs);
-                    } else {
-                        const float mi = solve_velocity_one(vc, A, B, bs);
-                        float4* rec = stage + r * prx::kVcStride;
-                        rec[4] = float4{vc.p[0].ni, vc.p[0].ti, vc.p[0].nm, vc.p[0].tm};
-                        rec[7] = float4{vc.p[1].ni, vc.p[1].ti, vc.p[1].nm, vc.p[1].tm};
-                        newInc = fmax_std(newInc, mi);
-                    }
-                    vc = nvc;
-                    A = nA;
-                    B = nB;
-                }
-            }
-            __syncthreads();
-            if (!warm && !resident && x.lane < 2 * r1) {
-                const int r = x.lane >> 1;
-                const int f = r * prx::kVcStride + ((x.lane & 1) ? 7 : 4);
-                gbase[std::size_t(ch) * kChunk * prx::kVcStride + f] = stage[f];
-            }
-            if (more) {
-                __syncthreads();
-                sg.commit(nx);
-                __syncthreads();
-            }
-        }
-        if (warm && x.lane == 0) joints_velocity(jbase, j0, j1, bs, true);
-        newInc = __shfl(newInc, 0);
-        jointsOkay = __shfl(int(jointsOkay), 0) != 0;
-        if (!warm && jointsOkay && newInc <= c.regMinMomentum) break;
-    }
-    if (resident && x.lane < 2 * n) {
-        const int r = x.lane >> 1;
-        const int f = r * prx::kVcStride + ((x.lane & 1) ? 7 : 4);
-        gbase[f] = stage[f];
-    }
-    __syncthreads();
-}
-
-// Lane form of the position sweeps of one island (ContactSolver.cpp:494-643,
-// AabbTreeWorld.cpp:1627-1644).  Returns whether the island was solved.
-template <bool kLds>
-__device__ __forceinline__ bool island_position(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
-                                                int c0, int c1, int j0, int j1, const StepParams& c) {
-    const float4* base = d.s_pc + std::size_t(x.co) * prx::kPcStride;
-    const float4* jbase = d.s_jc + std::size_t(x.jo) * prx::kJcStride;
-    for (std::uint32_t it = 0; it < c.posIters; ++it) {
-        float minSep = __builtin_inff();
-        if (c0 < c1) {
-            Pc pc = load_pc(base + std::size_t(c0) * prx::kPcStride);
-            for (int e = c0; e < c1; ++e) {
-                const int en = (e + 1 < c1) ? e + 1 : e;
-                const Pc npc = load_pc(base + std::size_t(en) * prx::kPcStride);
-                float4 pa = bs.pos(pc.bA);
-                float4 pb = bs.pos(pc.bB);
-                const float sep = solve_pc(pc.m, pc.totalRadius, pc.A, pc.B, pa, pb, c);
-                bs.set_pos(pc.bA, pa);
-                bs.set_pos(pc.bB, pb);
-                minSep = fmin_std(minSep, sep);
-                pc = npc;
-            }
-        }
-        const bool contactsOkay = minSep >= c.regMinSeparation;
-        const bool jointsOkay = joints_position(jbase, j0, j1, bs, c);
-        if (contactsOkay && jointsOkay) return true;
-    }
-    return false;
-}
-
-template <bool kLds>
-__device__ __forceinline__ bool island_position_wave(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
-                                                     float4* stage, int c0, int c1, int j0, int j1,
-                                                     const StepParams& c) {
-    const float4* jbase = d.s_jc + std::size_t(x.jo) * prx::kJcStride;
-    const float4* gbase = d.s_pc + (std::size_t(x.co) + std::size_t(c0)) * prx::kPcStride;
-    const int n = c1 - c0;
-    const int nch = (n + kChunk - 1) / kChunk;
-    const bool resident = nch == 1;
-    Stager<prx::kPcStride> sg{gbase, stage, n, x.lane, {}};
-    sg.fetch(0);
-    sg.commit(0);
-    __syncthreads();
-    const int npass = int(c.posIters);
-    bool solved = false;
-    for (int pass = 0; pass < npass; ++pass) {
-        float minSep = __builtin_inff();
-        for (int ch = 0; ch < nch; ++ch) {
-            const int nx = (ch + 1 < nch) ? ch + 1 : 0;
-            const bool more = !resident && (ch + 1 < nch || pass + 1 < npass);
-            if (more) sg.fetch(nx);
-            const int r1 = ((ch + 1) * kChunk < n ? kChunk : n - ch * kChunk);
-            if (x.lane == 0) {
-                Pc pc = load_pc(stage);
-                for (int r = 0; r < r1; ++r) {
-                    const int rn = (r + 1 < r1) ? r + 1 : r;
-                    const Pc npc = load_pc(stage + rn * prx::kPcStride);
-                    float4 pa = bs.pos(pc.bA);
-                    float4 pb = bs.pos(pc.bB);
-                    const float sep = solve_pc(pc.m, pc.totalRadius, pc.A, pc.B, pa, pb, c);
-                    bs.set_pos(pc.bA, pa);
-                    bs.set_pos(pc.bB, pb);
-                    minSep = fmin_std(minSep, sep);
-                    pc = npc;
-                }
-            }
-            __syncthreads();
-            if (more) {
-                sg.commit(nx);
-                __syncthreads();
-            }
-        }
-        bool jointsOkay = true;
-        if (x.lane == 0) jointsOkay = joints_position(jbase, j0, j1, bs, c);
-        minSep = __shfl(minSep, 0);
-        jointsOkay = __shfl(int(jointsOkay), 0) != 0;
-        if (minSep >= c.regMinSeparation && jointsOkay) {
-            solved = true;
-            break;
-        }
-    }
-    __syncthreads();
-    return solved;
-}
-
-template <bool kLds>
-__device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, const WorldCtx& x, int nisl,
-                                                       const IslandScratch& S, const StepParams& c, PhaseTimer& T) {
-    const int bo = x.bo;
-    const Bodies<kLds> bs{d.s_bc_pos + bo, d.s_bc_vel + bo, x.nb};
-    const int nbe = S.istart[nisl].x;
-    const int nce = S.istart[nisl].y;
-    // Body constraints (GetBodyConstraints, AabbTreeWorld.cpp:276-287) for
-    // island bodies (which the island search also marks awake, :1410);
-    // static bodies get their fixed pose and zero velocity.
-    for (int b = x.lane; b < x.nb; b += kWave) {
-        const int gb = bo + b;
-        if (S.bisland[b] >= 0) {
-            const float4 p1 = d.pos1[gb];
-            d.pos0[gb] = p1;
-            d.bflags[gb] |= prx::kAwake;
-            bs.set_pos(b, p1);
-            bs.set_vel(b, integrated_velocity(d, gb, c));
-        } else if (!(d.bflags[gb] & prx::kSpeedable)) {
-            bs.set_pos(b, d.pos1[gb]);
-            bs.set_vel(b, float4{0.0f, 0.0f, 0.0f, 0.0f});
-        }
-    }
-    for (int i = x.lane; i < nisl; i += kWave) {
-        S.isl_word[i] = 0u;
-        S.isl_under[i] = 0x7f800000u;
-    }
-    __syncthreads();
-    // Velocity and position constraints of all island contacts, in island
-    // order (independent per contact).
-    for (int e = x.lane; e < nce; e += kWave) {
-        const int gi = x.co + S.icont[e];
-        const int2 bd = d.c_body[gi];
-        const int2 ss = d.c_shape[gi];
-        const float rA = d.shape_f[ss.x].x, rB = d.shape_f[ss.y].x;
-        const BodyC A = load_bodyc(d, bo + bd.x), B = load_bodyc(d, bo + bd.y);
-        const float4 pA = bs.pos(bd.x), pB = bs.pos(bd.y);
-        const Xf xfA = get_xf(pA.x, pA.y, pA.z, A.lc);
-        const Xf xfB = get_xf(pB.x, pB.y, pB.z, B.lc);
-        const Manifold m = load_manifold(d, gi);
-        const WM wm = world_manifold(m, xfA, rA, xfB, rB);
-        const float4 mt = d.c_mat[gi];
-        const Vc vc = make_vc(wm, mt.x, mt.y, mt.z, bd.x, bd.y, A, pA, bs.vel(bd.x), B, pB, bs.vel(bd.y), c);
-        const std::size_t slot = std::size_t(x.co) + std::size_t(e);
-        store_vc(d.s_vc + slot * prx::kVcStride, vc, A, B);
-        store_pc(d.s_pc + slot * prx::kPcStride, m, bd.x, bd.y, rA + rB, A, B);
-    }
-    // Joint set-up (the non-warm-start half of InitVelocity), island order.
-    const int nje = S.jstart[nisl];
-    for (int e = x.lane; e < nje; e += kWave) {
-        const int jg = x.jo + S.ijoint[e];
-        const int2 bd = d.j_body[jg];
-        const BodyC A = load_bodyc(d, bo + bd.x), B = load_bodyc(d, bo + bd.y);
-        build_jc(d, x, jg, d.s_jc + (std::size_t(x.jo) + std::size_t(e)) * prx::kJcStride, A, bs.pos(bd.x), B,
-                 bs.pos(bd.y), bd.x, bd.y, c);
-    }
-    __syncthreads();
-    T.mark(prx::kPhConstraints);
-    // Warm start and velocity sweeps: islands below coopMin contacts one lane
-    // each, larger ones by the whole wave, one after another.
-    float4* stage = g_smem + (int(c.ldsBytes) / 16 - prx::kStageFloat4);
-    const int coopMin = c.coopMin > 0 ? int(c.coopMin) : 1;
-    auto constraints = [&](int is) {
-        return (S.istart[is + 1].y - S.istart[is].y) + (S.jstart[is + 1] - S.jstart[is]);
-    };
-    for (int is = x.lane; is < nisl; is += kWave) {
-        if (constraints(is) < coopMin)
-            island_velocity<kLds>(d, x, bs, S.istart[is].y, S.istart[is + 1].y, S.jstart[is], S.jstart[is + 1], c);
-    }
-    __syncthreads();
-    for (int base = 0; base < nisl; base += kWave) {
-        const int is = base + x.lane;
-        const bool big = is < nisl && constraints(is) >= coopMin;
-        for (unsigned long long m = __ballot(big); m; m &= m - 1) {
-            const int isl = base + __ffsll(m) - 1;
-            island_velocity_wave<kLds>(d, x, bs, stage, S.istart[isl].y, S.istart[isl + 1].y, S.jstart[isl],
-                                       S.jstart[isl + 1], c);
-        }
-    }
-    T.mark(prx::kPhVelocity);
-    // IntegratePositions (AabbTreeWorld.cpp:176-188).
-    for (int e = x.lane; e < nbe; e += kWave) {
-        const int b = S.ibody[e];
-        float4 p = bs.pos(b);
-        const float4 v = bs.vel(b);
-        const float tx = c.dt * v.x, ty = c.dt * v.y, r = c.dt * v.z;
-        p.x = p.x + tx;
-        p.y = p.y + ty;
-        p.z = p.z + r;
-        bs.set_pos(b, p);
-    }
-    __syncthreads();
-    // Position sweeps, split like the velocity sweeps.
-    for (int is = x.lane; is < nisl; is += kWave) {
-        if (constraints(is) < coopMin &&
-            island_position<kLds>(d, x, bs, S.istart[is].y, S.istart[is + 1].y, S.jstart[is], S.jstart[is + 1], c))
-            S.isl_word[is] = 1u;
-    }
-    __syncthreads();
-    for (int base = 0; base < nisl; base += kWave) {
-        const int is = base + x.lane;
-        const bool big = is < nisl && constraints(is) >= coopMin;
-        for (unsigned long long m = __ballot(big); m; m &= m - 1) {
-            const int isl = base + __ffsll(m) - 1;
-            const bool solved = island_position_wave<kLds>(d, x, bs, stage, S.istart[isl].y, S.istart[isl + 1].y,
-                                                           S.jstart[isl], S.jstart[isl + 1], c);
-            if (solved && x.lane == 0) S.isl_word[isl] = 1u;
-        }
-    }
-    __syncthreads();
-    T.mark(prx::kPhPosition);
-    // AssignImpulses (AabbTreeWorld.cpp:208-227).
-    for (int e = x.lane; e < nce; e += kWave) {
-        const int gi = x.co + S.icont[e];
-        const float4* rec = d.s_vc + (std::size_t(x.co) + std::size_t(e)) * prx::kVcStride;
-        const std::uint32_t n = __float_as_uint(rec[1].w);
-        const float4 q0 = rec[4];
-        float4 m0 = d.m_p0[gi];
-        m0.z = q0.x;
-        m0.w = q0.y;
-        d.m_p0[gi] = m0;
-        if (n > 1) {
-            const float4 q1 = rec[7];
-            float4 m1 = d.m_p1[gi];
-            m1.z = q1.x;
-            m1.w = q1.y;
-            d.m_p1[gi] = m1;
-        }
-    }
-    // Joint state carried to the next step: impulses and limit state.
-    for (int e = x.lane; e < nje; e += kWave) {
-        const int jg = x.jo + S.ijoint[e];
-        const float4* u = d.s_jc + (std::size_t(x.jo) + std::size_t(e)) * prx::kJcStride;
-        d.j_imp[jg] = u[3];
-        d.j_limit[jg] = __float_as_uint(u[2].w) >> 8;
-    }
-    // Body writeback, under-active times (AabbTreeWorld.cpp:1652-1664, 403-417).
-    for (int e = x.lane; e < nbe; e += kWave) {
-        const int b = S.ibody[e];
-        const int gb = bo + b;
-        const float4 bv = bs.vel(b);
-        const float4 bp = bs.pos(b);
-        const float4 p1 = d.pos1[gb];
-        std::uint32_t f = d.bflags[gb];
-        if (!(p1.x == bp.x && p1.y == bp.y) || p1.z != bp.z) {
-            d.pos1[gb] = float4{bp.x, bp.y, bp.z, 0.0f};
-            const float4 l = d.lcinv[gb];
-            const Xf xf = get_xf(bp.x, bp.y, bp.z, V2{l.x, l.y});
-            d.xf[gb] = float4{xf.p.x, xf.p.y, xf.q.x, xf.q.y};
-            f |= 0x100u;  // position changed this step
-            d.bflags[gb] = f;
-        }
-        float4 v = d.vel[gb];
-        v.x = bv.x;
-        v.y = bv.y;
-        v.z = bv.z;
-        const float linSq = magsq(V2{v.x, v.y});
-        const float angSq = v.z * v.z;
-        const bool under = (angSq <= c.angularSleepTolerance * c.angularSleepTolerance) &&
-                           (linSq <= c.linearSleepTolerance * c.linearSleepTolerance);
-        const float t = ((f & prx::kAutoSleep) && under) ? v.w + c.dt : 0.0f;
-        if (t == 0.0f || (f & prx::kAccelerable)) v.w = t;
-        d.vel[gb] = v;
-        // Under-active times are non-negative: their bit patterns order as floats.
-        atomicMin(&S.isl_under[S.bisland[b]], __float_as_uint(t));
-    }
-    __syncthreads();
-    // Sleepem (AabbTreeWorld.cpp:419-429) for islands still long enough and solved.
-    std::uint32_t slept = 0;
-    for (int e = x.lane; e < nbe; e += kWave) {
-        const int b = S.ibody[e];
-        const int is = S.bisland[b];
-        if (__uint_as_float(S.isl_under[is]) >= c.minStillTimeToSleep && S.isl_word[is]) {
-            const int gb = bo + b;
-            const std::uint32_t f = d.bflags[gb];
-            if ((f & prx::kAwake) && (f & prx::kAutoSleep)) {
-                d.bflags[gb] = f & ~prx::kAwake;
-                d.vel[gb] = float4{0.0f, 0.0f, 0.0f, 0.0f};
-                ++slept;
-            }
-        }
-    }
-    for (int o = 32; o > 0; o >>= 1) slept += __shfl_xor(slept, o);
-    __syncthreads();
-    T.mark(prx::kPhWriteback);
-    return slept;
-}
-
-// Synchronize (AabbTreeWorld.cpp:2564-2585) for one body.
-__device__ __forceinline__ std::uint32_t synchronize_body(const DeviceState& d, const WorldCtx& x, int b, const StepParams& c) {
-    const int gb = x.bo + b;
-    const Xf xf0 = body_xf0(d, gb);
-    const Xf xf1 = body_xf1(d, gb);
-    const int2 bp = d.bproxy[gb];
-    std::uint32_t moved = 0;
-    for (int k = 0; k < bp.y; ++k) {
-        const int p = d.bproxy_list[x.po + bp.x + k];
-        const int gp = x.po + p;
-        const Proxy px = shape_proxy(d, d.pshape[gp]);
-        const float4 tight = proxy_aabb(px, xf0, xf1, c, false);
-        const float4 fat = d.paabb[gp];
-        const bool contains = fat.x <= tight.x && fat.z >= tight.z && fat.y <= tight.y && fat.w >= tight.w;
-        if (!contains) {
-            d.paabb[gp] = proxy_aabb(px, xf0, xf1, c, true);
-            d.pflags[gp] |= prx::kProxyMoved;
-            ++moved;
-        }
-    }
-    return moved;
-}
-
-__device__ __forceinline__ void step_world(const DeviceState& d, int w, const StepParams& c) {
-    WorldHeader& H = d.hdr[w];
-    WorldCtx x;
-    x.w = w;
-    x.bo = H.body_off;
-    x.nb = H.nbodies;
-    x.po = H.proxy_off;
-    x.np = H.nproxies;
-    x.co = H.contact_off;
-    x.cap = H.contact_cap;
-    x.jo = H.joint_off;
-    x.nj = H.njoints;
-    x.lane = int(threadIdx.x);
-    Counters k;
-    k.nc = H.ncontacts;
-    k.nfree = H.nfree;
-    k.pool = H.pool_size;
-    k.status = std::uint32_t(H.status);
-    std::uint32_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    PhaseTimer T;
-    T.begin(d.phase_cycles ? d.phase_cycles + std::size_t(w) * prx::kPhases : nullptr);
-
-    // A: proxies of newly attached shapes (CreateProxies, AabbTreeWorld.cpp:557-579).
-    for (int p = x.lane; p < x.np; p += kWave) {
-        const int gp = x.po + p;
-        if (d.pflags[gp] & prx::kProxyPending) {
-            const int gb = x.bo + d.pbody[gp];
-            d.paabb[gp] = proxy_aabb(shape_proxy(d, d.pshape[gp]), body_xf0(d, gb), body_xf1(d, gb), c, true);
-            d.pflags[gp] = prx::kProxyMoved;
-        }
-    }
-    __syncthreads();
-    T.mark(prx::kPhProxies);
-    // B, C, D.
-    stats[1] = phase_destroy(d, x, k);
-    __syncthreads();
-    T.mark(prx::kPhDestroy);
-    stats[3] = phase_update(d, x, k.nc, c);
-    __syncthreads();
-    T.mark(prx::kPhUpdate);
-    stats[2] = phase_find_add(d, x, k, c);
-    __syncthreads();
-    T.mark(prx::kPhFindAdd);
-
-    if (c.dt != 0.0f) {
-        IslandScratch S;
-        S.off = d.s_csr_off + x.bo + w;
-        S.csr = reinterpret_cast<int2*>(d.s_csr) + 2 * x.co;
-        S.ibody = d.s_isl_body + x.bo + x.co;
-        S.icont = d.s_isl_contact + x.co;
-        S.istart = d.s_isl_start + x.bo + w;
-        S.ijoint = d.s_isl_joint + x.jo;
-        S.jstart = d.s_isl_jstart + x.bo + w;
-        S.bisland = d.s_stack + 2 * x.bo + x.nb;
-        S.isl_word = d.s_keys + x.co;
-        S.isl_under = d.s_keys2 + x.co;
-        build_adjacency(d, x, k.nc, S);
-        T.mark(prx::kPhAdjacency);
-        const int nent = S.off[x.nb];
-        const int ldsAdj = 4 * (((x.nb + 2) >> 1) * 2) + 8 * nent + x.nb + k.nc + x.nj;
-        int nisl;
-        if (ldsAdj <= int(c.ldsBytes)) nisl = find_islands<2>(d, x, k.nc, S);
-        else if (x.nb + k.nc + x.nj <= int(c.ldsBytes)) nisl = find_islands<1>(d, x, k.nc, S);
-        else nisl = find_islands<0>(d, x, k.nc, S);
-        stats[4] = std::uint32_t(nisl);
-        T.mark(prx::kPhIslands);
-        const bool ldsBodies = x.nb * 32 + prx::kStageFloat4 * 16 <= int(c.ldsBytes);
-        stats[7] = ldsBodies ? solve_islands<true>(d, x, nisl, S, c, T) : solve_islands<false>(d, x, nisl, S, c, T);
-        // Synchronise proxies of islanded speedable bodies (AabbTreeWorld.cpp:1532-1544).
-        std::uint32_t moved = 0;
-        for (int b = x.lane; b < x.nb; b += kWave) {
-            if (S.bisland[b] >= 0) moved += synchronize_body(d, x, b, c);
-        }
-        for (int o = 32; o > 0; o >>= 1) moved += __shfl_xor(moved, o);
-        stats[6] = moved;
-        __syncthreads();
-        T.mark(prx::kPhSync);
-        // Contacts of bodies whose position changed need updating.
-        for (int i = x.lane; i < k.nc; i += kWave) {
-            const int2 bd = d.c_body[x.co + i];
-            if ((d.bflags[x.bo + bd.x] | d.bflags[x.bo + bd.y]) & 0x100u) d.c_flags[x.co + i] |= prx::kNeedsUpdating;
-        }
-        __syncthreads();
-        for (int b = x.lane; b < x.nb; b += kWave) d.bflags[x.bo + b] &= ~0x100u;
-        __syncthreads();
-        phase_update(d, x, k.nc, c);
-        __syncthreads();
-        T.mark(prx::kPhUpdate2);
-        stats[5] = phase_find_add(d, x, k, c);
-        __syncthreads();
-        T.mark(prx::kPhFindAdd2);
-    }
-    T.end();
-    if (x.lane == 0) {
-        H.ncontacts = k.nc;
-        H.nfree = k.nfree;
-        H.pool_size = k.pool;
-        H.status = int(k.status);
-        for (int i = 0; i < 8; ++i) H.stats[i] = stats[i];
-    }
-}
 
 __global__ __launch_bounds__(64) void k_step_worlds(DeviceState d, StepParams c) {
     const int w = d.schedule[blockIdx.x];

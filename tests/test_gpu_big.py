@@ -1,0 +1,205 @@
+"""The single-large-world mode on the GPU: LBVH broadphase pair sets against
+brute force, the pre-phase (contact creation and narrowphase) against the
+reference restatement bit for bit, and the colour-parallel solve judged by
+penetration, containment and energy against the reference's exact solve."""
+import numpy as np
+import pytest
+
+import playrho_amd as pa
+from tests import dumpfmt, oracle_lib
+
+pytestmark = pytest.mark.gpu
+
+
+def _brute_pairs(aabbs, bodies, moved):
+    """All (i < j) pairs of different bodies whose AABBs overlap (the
+    reference's TestOverlap: max of lows <= min of highs on both axes), with
+    at least one of i, j moved."""
+    n = aabbs.shape[0]
+    lo = aabbs[:, :2]
+    hi = aabbs[:, 2:]
+    out = []
+    idx = np.nonzero(moved)[0]
+    for s in range(0, idx.size, 512):
+        rows = idx[s:s + 512]
+        mx = np.maximum(lo[rows, None, :], lo[None, :, :])
+        mn = np.minimum(hi[rows, None, :], hi[None, :, :])
+        ov = np.all(mn >= mx, axis=2)
+        ov &= bodies[rows, None] != bodies[None, :]
+        ov[np.arange(rows.size), rows] = False
+        r, c = np.nonzero(ov)
+        a, b = rows[r], c
+        out.append(np.stack([np.minimum(a, b), np.maximum(a, b)], axis=1))
+    if not out:
+        return np.zeros((0, 2), dtype=np.int64)
+    p = np.concatenate(out)
+    return np.unique(p, axis=0)
+
+
+@pytest.mark.parametrize("n,seed", [(1, 0), (2, 1), (3, 2), (64, 3), (1000, 4), (20000, 5)])
+def test_lbvh_pairs_match_brute_force(n, seed):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0, 10 + np.sqrt(n) * 2, size=(n, 2)).astype(np.float32)
+    # Repeated centres (equal Morton codes) and touching edges.
+    k = n // 10
+    if k:
+        c[:k] = c[n - k:]
+    h = rng.uniform(0.05, 1.5, size=(n, 2)).astype(np.float32)
+    aabbs = np.concatenate([c - h, c + h], axis=1).astype(np.float32)
+    if n >= 4:
+        aabbs[1, 0] = aabbs[0, 2]  # exactly touching on x
+        aabbs[1, 1] = aabbs[0, 1]
+    bodies = (np.arange(n) // 2 if n > 8 else np.arange(n)).astype(np.int32)
+    moved = (rng.uniform(size=n) < 0.5).astype(np.uint8)
+    if n <= 3:
+        moved[:] = 1
+    print("lbvh n", n, "seed", seed, flush=True)
+    got = pa.lbvh_pairs(aabbs, bodies, moved)
+    got = np.unique(got.astype(np.int64), axis=0) if got.size else np.zeros((0, 2), dtype=np.int64)
+    want = _brute_pairs(aabbs, bodies, moved.astype(bool))
+    assert got.shape == want.shape, (got.shape, want.shape)
+    assert np.array_equal(got, want)
+
+
+def _contacts_by_key(state):
+    order = np.lexsort((state["cu"][:, 1], state["cu"][:, 0]))
+    return (state["cu"][order][:, [0, 1, 3]], state["ct"][order], state["cf"][order].view("<u4"))
+
+
+def test_big_prephase_matches_reference():
+    """dt = 0 steps do only the pre-phase: proxies, contact destruction, the
+    narrowphase and pair finding.  The contact set (keys, flags, manifolds with
+    ContactFeatures) equals the reference restatement's, bit for bit."""
+    w = pa.World.scene("drop", [1500, 30, 60, 5])
+    desc = w.save_desc()
+    conf = pa.StepConf(delta_time=0.0)
+    ow = oracle_lib.OracleWorld(desc + _conf_bytes(conf))
+    big = pa.BigWorld(w)
+    for step in range(2):
+        print("prephase step", step + 1, flush=True)
+        big.step(conf, 1)
+        big.sync()
+        got = dumpfmt.read_state(dumpfmt.Reader(w.state_record()))
+        want = ow.step(1)[0]
+        assert len(got["cu"]) > 100
+        for a, b in zip(_contacts_by_key(got), _contacts_by_key(want)):
+            assert np.array_equal(a, b)
+        assert np.array_equal(got["bodies"].view("<u4"), want["bodies"].view("<u4"))
+        assert np.array_equal(got["paabb"].view("<u4"), want["paabb"].view("<u4"))
+        assert list(got["stats"][1:4]) == list(want["stats"][1:4])
+
+
+def _conf_bytes(c):
+    import struct
+    floats = [getattr(c, f) for f, _ in pa.StepConf._fields_[:18]]
+    return (struct.pack("<I", 18) + struct.pack("<18f", *floats) +
+            struct.pack("<III", c.reg_velocity_iters, c.reg_position_iters,
+                        (1 if c.do_warm_start else 0) | (4 if c.do_blocksolve else 0)))
+
+
+def _min_separation(desc_bytes, state):
+    """Smallest separation over touching contacts at the state's poses, from
+    the manifolds (PositionSolverManifold.cpp:42-121, float64)."""
+    r = dumpfmt.Reader(desc_bytes)
+    shapes, bodies, _ = dumpfmt.read_description(r)
+    # Proxy (tree leaf) ids in creation order: body order, shape order.
+    leaf = {}
+    k = 0
+    for bi, b in enumerate(bodies):
+        if not (b["flags"] & 32):
+            continue
+        for sid in b["shapes"]:
+            leaf[0 if k == 0 else 2 * k - 1] = (bi, int(sid))
+            k += 1
+    lc = np.array([b["vals"][6:8] for b in bodies], dtype=np.float64)
+    pos = state["bodies"][:, 3:6].astype(np.float64)
+    q = np.stack([np.cos(pos[:, 2]), np.sin(pos[:, 2])], axis=1)
+
+    def xf(bi, v):
+        c, s = q[bi]
+        p = pos[bi, :2] - np.array([c * lc[bi, 0] - s * lc[bi, 1], s * lc[bi, 0] + c * lc[bi, 1]])
+        return p + np.array([c * v[0] - s * v[1], s * v[0] + c * v[1]])
+
+    def rot(bi, v):
+        c, s = q[bi]
+        return np.array([c * v[0] - s * v[1], s * v[0] + c * v[1]])
+
+    best = np.inf
+    for cu, ct, cf in zip(state["cu"], state["ct"], state["cf"]):
+        if not (cu[3] & 1) or (cu[3] & 4):
+            continue
+        (ba, sa), (bb, sb) = leaf[int(cu[0])], leaf[int(cu[1])]
+        tr = float(shapes[sa]["radius"]) + float(shapes[sb]["radius"])
+        mtype, npts = int(ct[0]) & 0xff, int(ct[0]) >> 8
+        ln, lpt = cf[2:4], cf[4:6]
+        for i in range(npts):
+            lp = cf[6 + 4 * i:8 + 4 * i]
+            if mtype == 1:  # circles
+                pa_, pb_ = xf(ba, lpt), xf(bb, lp)
+                sep = np.linalg.norm(pb_ - pa_)
+            elif mtype == 2:  # face A
+                n = rot(ba, ln)
+                sep = np.dot(xf(bb, lp) - xf(ba, lpt), n)
+            else:  # face B
+                n = rot(bb, ln)
+                sep = np.dot(xf(ba, lp) - xf(bb, lpt), n)
+            best = min(best, sep - tr)
+    return best
+
+
+def test_big_solve_quality_against_reference():
+    """An 800-body drop into a 50 x 30 box, 300 steps: the coloured solve
+    against the reference's exact island solve (restatement).  Judged by the
+    deepest penetration, containment, resting height and kinetic energy."""
+    scene = [800, 25, 30, 3]
+    w = pa.World.scene("drop", scene)
+    desc = w.save_desc()
+    conf = pa.StepConf()
+    ow = oracle_lib.OracleWorld(desc + _conf_bytes(conf))
+    big = pa.BigWorld(w)
+    steps = 300
+    print("big solve", steps, flush=True)
+    big.step(conf, steps)
+    big.sync()
+    met = big.metrics()
+    print("metrics", met, flush=True)
+    got = dumpfmt.read_state(dumpfmt.Reader(w.state_record()))
+    want = ow.step(steps)[-1]
+    gb, wb = got["bodies"].astype(np.float64), want["bodies"].astype(np.float64)
+    assert np.all(np.isfinite(gb))
+    dyn = np.array([bool(f & 2) for f in got["bflags"]])
+
+    # Containment in the [-25, 25] x [0, 30] box.  Without TOI a fast body can
+    # pass through a thin edge in one step; the reference loses bodies the
+    # same way, so the coloured solve may not lose more.
+    def inside(b):
+        return dyn & (np.abs(b[:, 3]) < 25.5) & (b[:, 4] > -0.5)
+
+    ig, iw = inside(gb), inside(wb)
+    print("escaped big %d reference %d" % (dyn.sum() - ig.sum(), dyn.sum() - iw.sum()), flush=True)
+    assert ig.sum() >= iw.sum()
+    sep_got = _min_separation(desc, got)
+    sep_want = _min_separation(desc, want)
+    print("min separation big %.5f reference %.5f" % (sep_got, sep_want), flush=True)
+    assert sep_got > sep_want - 0.03
+    assert abs(met["min_separation"] - sep_got) < 1e-3
+    both = ig & iw
+    hg, hw = gb[both, 4].mean(), wb[both, 4].mean()
+    print("mean height big %.4f reference %.4f" % (hg, hw), flush=True)
+    assert abs(hg - hw) < 0.05 * hw + 0.1
+    ke_g = 0.5 * np.sum(gb[both, 6] ** 2 + gb[both, 7] ** 2)
+    ke_w = 0.5 * np.sum(wb[both, 6] ** 2 + wb[both, 7] ** 2)
+    print("kinetic (unit mass) big %.4f reference %.4f" % (ke_g, ke_w), flush=True)
+    assert ke_g < 2.0 * ke_w + 1.0
+
+
+def test_big_deterministic():
+    a = pa.World.scene("drop", [800, 15, 30, 9])
+    b = pa.World.scene("drop", [800, 15, 30, 9])
+    ba, bb = pa.BigWorld(a), pa.BigWorld(b)
+    conf = pa.StepConf()
+    ba.step(conf, 60)
+    bb.step(conf, 60)
+    ba.sync()
+    bb.sync()
+    assert a.state_record() == b.state_record()
