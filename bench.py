@@ -47,6 +47,9 @@ def parse():
     p.add_argument("--big-steps", type=int, default=600)
     p.add_argument("--big-cpu-steps", type=int, default=30,
                    help="leading steps of the large world timed on the CPU reference and the GPU alike")
+    p.add_argument("--adjoint-worlds", type=int, default=1024, help="differentiable billiards worlds (0: skip)")
+    p.add_argument("--adjoint-steps", type=int, default=120)
+    p.add_argument("--adjoint-every", type=int, default=16)
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only run a plain device copy of MIB MiB five times (counter calibration)")
     return p.parse_args()
@@ -153,6 +156,41 @@ def large_world(args, pa, conf):
     return out
 
 
+def adjoint_bench(args, pa, conf):
+    """Differentiable batched billiards: 1024 worlds of 64 disks (radius 0.5)
+    on a frictional floor in a walled 20 x 20 box, initial velocities
+    N(0, 3^2), friction and restitution U[0, 1]; loss = sum of squared
+    distances of the final positions to seeded targets after 120 steps;
+    gradients with respect to initial velocities, friction and restitution
+    through the adjoint, checkpointing every 16 steps."""
+    import numpy as np
+    worlds = [pa.World.scene("billiards", [args.seed, w]) for w in range(args.adjoint_worlds)]
+    batch = pa.Batch(worlds, use_graph=False)
+    adj = pa.Adjoint(batch, args.adjoint_steps, args.adjoint_every, conf)
+    rng = np.random.default_rng(args.seed)
+    nb = sum(w.body_count for w in worlds)
+    targets = np.stack([rng.uniform(-8, 8, nb), rng.uniform(1, 10, nb)], axis=1).astype(np.float32)
+    adj.run(targets)  # warm-up (allocations, code paths)
+    t0 = time.perf_counter()
+    out = adj.run(targets)
+    wall = time.perf_counter() - t0
+    ws = args.adjoint_worlds * args.adjoint_steps
+    return {
+        "workload": "%d billiards worlds x 64 disks, %d steps, checkpoint every %d, loss sum |x_T - target|^2, "
+                    "d loss / d (initial velocity, friction, restitution)" % (args.adjoint_worlds,
+                                                                           args.adjoint_steps,
+                                                                           args.adjoint_every),
+        "value": ws / wall,
+        "unit": "adjoint world-steps/s",
+        "note": "one run = the forward pass with checkpoints, the recomputation of every segment with the "
+                "solve recorded, and the reversed steps; value = worlds x steps / wall time of the run",
+        "seconds": wall,
+        "device_ms": adj.timing_ms(),
+        "mean_loss": float(np.mean(out["loss"])),
+        "grad_norm": float(np.linalg.norm(out["grad_velocity"])),
+    }
+
+
 def conf_record(pa):
     import struct
     c = pa.StepConf()
@@ -253,6 +291,8 @@ def main():
 
     if args.big_bodies > 0 and args.big_steps > 0:
         result["large_world"] = large_world(args, pa, conf)
+    if args.adjoint_worlds > 0 and args.adjoint_steps > 0:
+        result["adjoint"] = adjoint_bench(args, pa, conf)
 
     # Plain per-world form: one world per launch, synchronised, on a sample.
     if args.plain_worlds <= 0:

@@ -240,6 +240,30 @@ int prx_big_metrics(prx_big* big, float out[4]);
 int prx_lbvh_pairs(const float* aabbs, const int32_t* bodies, const uint8_t* moved, int n, int32_t* pairs,
                    int cap, int* count);
 
+/* Adjoint of K regular steps of a batch, for worlds of disks against each
+ * other and against static edges/polygons (one-point manifolds; no joints).
+ * No reference counterpart: it differentiates Step (AabbTreeWorld.cpp:
+ * 1563-1682) with each step's active set (touching contacts, islands, clamp
+ * branches, early exits) held as the forward pass took it.  Loss per world:
+ * sum over dynamic bodies of |final position - target|^2.  The forward pass
+ * keeps a checkpoint every `every` steps; the backward pass recomputes each
+ * segment from its checkpoint, recording the solves, and reverses them. */
+typedef struct prx_adjoint prx_adjoint;
+int prx_adjoint_create(prx_batch* batch, int nsteps, int every, const prx_step_conf* conf, prx_adjoint** out);
+void prx_adjoint_destroy(prx_adjoint* adj);
+/* targets: 2 floats per batch body (batch body order: world by world).
+ * Outputs (each may be null): loss per world; d loss / d initial velocity,
+ * 3 floats (vx, vy, w) per batch body; d loss / d friction and d loss / d
+ * restitution per batch shape.  The batch is left at its starting state. */
+int prx_adjoint_run(prx_adjoint* adj, const prx_step_conf* conf, const float* targets, float* loss,
+                    float* grad_velocity, float* grad_friction, float* grad_restitution);
+/* Counters of the last run: constraint slots reversed, those with a
+ * restitution bias, position-solver updates and velocity updates reversed. */
+int prx_adjoint_diagnostics(prx_adjoint* adj, float out[4]);
+/* Device time (ms) of the last run: K forward steps, K recorded steps and K
+ * reversed steps. */
+int prx_adjoint_timing(prx_adjoint* adj, float* ms);
+
 /* Diagnostic: UnitVec::Get(angle) = {cos, sin} (UnitVec.cpp:27-30) evaluated
  * by the device routine for n angles; out receives n (cos, sin) pairs.  Used
  * by the tests to compare the device rotation with the host C library. */

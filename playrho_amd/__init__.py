@@ -141,6 +141,13 @@ _SIGS = {
     "prx_batch_counts": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "prx_batch_phase_cycles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]),
     "prx_selftest_sincos": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
+    "prx_adjoint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                          ctypes.POINTER(ctypes.c_void_p)]),
+    "prx_adjoint_destroy": (None, [ctypes.c_void_p]),
+    "prx_adjoint_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "prx_adjoint_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    "prx_adjoint_diagnostics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "prx_big_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     "prx_big_destroy": (None, [ctypes.c_void_p]),
     "prx_big_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
@@ -412,3 +419,84 @@ def lbvh_pairs(aabbs, bodies, moved, cap=None):
     _check(lib().prx_lbvh_pairs(aabbs.ctypes.data, bodies.ctypes.data, moved.ctypes.data, n,
                                 out.ctypes.data, cap, ctypes.byref(count)))
     return out[:min(count.value, cap)].copy()
+
+
+class Adjoint:
+    """Gradients of sum over dynamic bodies of |x_K - target|^2 after K steps
+    with respect to the bodies' initial velocities and the shapes' friction and
+    restitution (prx_adjoint_*).  Arrays are in batch order (worlds in the
+    order given to the Batch, bodies and shapes within each)."""
+
+    def __init__(self, batch, nsteps, every=16, conf=None):
+        self.batch = batch
+        self.conf = conf or StepConf()
+        self.h = ctypes.c_void_p()
+        _check(lib().prx_adjoint_create(batch.h, int(nsteps), int(every), ctypes.byref(self.conf),
+                                        ctypes.byref(self.h)))
+        self.nbodies = sum(w.body_count for w in batch.worlds)
+        self.nshapes = sum(lib().prx_world_shape_count(w.h) for w in batch.worlds)
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and h.value and _lib is not None:
+            _lib.prx_adjoint_destroy(h)
+            self.h = None
+
+    def run(self, targets):
+        t = np.ascontiguousarray(targets, dtype=np.float32).reshape(self.nbodies, 2)
+        loss = np.zeros(len(self.batch.worlds), dtype=np.float32)
+        gv = np.zeros((self.nbodies, 3), dtype=np.float32)
+        gf = np.zeros(self.nshapes, dtype=np.float32)
+        ge = np.zeros(self.nshapes, dtype=np.float32)
+        _check(lib().prx_adjoint_run(self.h, ctypes.byref(self.conf), t.ctypes.data, loss.ctypes.data,
+                                     gv.ctypes.data, gf.ctypes.data, ge.ctypes.data))
+        return dict(loss=loss, grad_velocity=gv, grad_friction=gf, grad_restitution=ge)
+
+    def diagnostics(self):
+        out = np.zeros(4, dtype=np.float32)
+        _check(lib().prx_adjoint_diagnostics(self.h, out.ctypes.data))
+        return dict(slots=int(out[0]), biased=int(out[1]), position_updates=int(out[2]),
+                    velocity_updates=int(out[3]))
+
+    def timing_ms(self):
+        ms = ctypes.c_float()
+        _check(lib().prx_adjoint_timing(self.h, ctypes.byref(ms)))
+        return ms.value
+
+
+def desc_offsets(desc):
+    """Byte offsets of each shape record and each body record in a world
+    description (layout of prx_world_save_desc)."""
+    import struct
+    u32 = lambda o: struct.unpack_from("<I", desc, o)[0]
+    ns = u32(0)
+    off = 4
+    shapes = []
+    for _ in range(ns):
+        n = u32(off)
+        shapes.append(off)
+        off += 4 + 16 + 12 + 16 * n
+    nb = u32(off)
+    off += 4
+    bodies = []
+    for _ in range(nb):
+        bodies.append(off)
+        nsh = u32(off + 8 + 76)
+        off += 8 + 76 + 4 + 4 * nsh
+    return shapes, bodies
+
+
+def desc_with(desc, velocities=None, friction=None, restitution=None):
+    """A copy of a world description with body velocities (dict body ->
+    (vx, vy, w)) and shape friction / restitution (dict shape -> value)
+    replaced."""
+    import struct
+    buf = bytearray(desc)
+    shapes, bodies = desc_offsets(desc)
+    for b, v in (velocities or {}).items():
+        struct.pack_into("<3f", buf, bodies[b] + 8 + 32, *[float(x) for x in v])
+    for sidx, f in (friction or {}).items():
+        struct.pack_into("<f", buf, shapes[sidx] + 12, float(f))
+    for sidx, e in (restitution or {}).items():
+        struct.pack_into("<f", buf, shapes[sidx] + 16, float(e))
+    return bytes(buf)

@@ -38,7 +38,7 @@ def build_library(out=LIB):
     objs = []
     tmp = os.path.join(PKG, "_build")
     os.makedirs(tmp, exist_ok=True)
-    for src in ("step_kernel.hip", "big_world.hip", "host_world.cpp"):
+    for src in ("step_kernel.hip", "big_world.hip", "adjoint.hip", "host_world.cpp"):
         obj = os.path.join(tmp, src + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
         _run([HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj])

@@ -637,31 +637,66 @@ __global__ __launch_bounds__(kBlock) void k_big_solve(DeviceState d, BigState g,
         m += tot;
     }
     for (int e = int(threadIdx.x); e < m; e += kBlock) g.color[e] = -1;
+    for (int b = int(threadIdx.x); b < nb; b += kBlock) g.used[b] = 0ull;
     __syncthreads();
+    // Rounds of speculative first-fit: every uncoloured contact proposes the
+    // lowest colour free at both its dynamic bodies; at each (body, colour)
+    // the lowest list position wins; winners take the colour.
     int ncolors = 0;
-    for (int k = 0; k < prx::kBigMaxColors; ++k) {
-        for (int b = int(threadIdx.x); b < nb; b += kBlock) g.claim[b] = 0x7fffffff;
-        __syncthreads();
+    for (int round = 0; round < prx::kBigColorRounds; ++round) {
         for (int e = int(threadIdx.x); e < m; e += kBlock) {
             if (g.color[e] >= 0) continue;
             const int2 bd = d.c_body[g.list[e]];
-            if (moves(d, bd.x)) atomicMin(&g.claim[bd.x], e);
-            if (moves(d, bd.y)) atomicMin(&g.claim[bd.y], e);
+            std::uint64_t mask = 0ull;
+            if (moves(d, bd.x)) mask |= g.used[bd.x];
+            if (moves(d, bd.y)) mask |= g.used[bd.y];
+            const int col = (mask == ~0ull) ? -1 : (__ffsll((unsigned long long)(~mask)) - 1);
+            g.tent[e] = col;
+            if (col < 0) continue;
+            if (moves(d, bd.x)) atomicMin(&g.claim[bd.x * prx::kBigMaxColors + col], e);
+            if (moves(d, bd.y)) atomicMin(&g.claim[bd.y * prx::kBigMaxColors + col], e);
+        }
+        __syncthreads();
+        for (int e = int(threadIdx.x); e < m; e += kBlock) {
+            if (g.color[e] >= 0) continue;
+            const int col = g.tent[e];
+            if (col < 0) continue;
+            const int2 bd = d.c_body[g.list[e]];
+            const bool wa = !moves(d, bd.x) || g.claim[bd.x * prx::kBigMaxColors + col] == e;
+            const bool wb = !moves(d, bd.y) || g.claim[bd.y * prx::kBigMaxColors + col] == e;
+            g.win[e] = (wa && wb) ? 1 : 0;
         }
         __syncthreads();
         int left = 0;
         for (int e = int(threadIdx.x); e < m; e += kBlock) {
             if (g.color[e] >= 0) continue;
+            const int col = g.tent[e];
+            if (col < 0) {
+                ++left;
+                continue;
+            }
             const int2 bd = d.c_body[g.list[e]];
-            const bool wa = !moves(d, bd.x) || g.claim[bd.x] == e;
-            const bool wb = !moves(d, bd.y) || g.claim[bd.y] == e;
-            if (wa && wb) g.color[e] = k;
-            else ++left;
+            if (moves(d, bd.x)) g.claim[bd.x * prx::kBigMaxColors + col] = 0x7fffffff;
+            if (moves(d, bd.y)) g.claim[bd.y * prx::kBigMaxColors + col] = 0x7fffffff;
+            if (g.win[e]) {
+                g.color[e] = col;
+                const unsigned long long bit = 1ull << col;
+                if (moves(d, bd.x)) atomicOr(reinterpret_cast<unsigned long long*>(g.used + bd.x), bit);
+                if (moves(d, bd.y)) atomicOr(reinterpret_cast<unsigned long long*>(g.used + bd.y), bit);
+                if (col + 1 > ncolors) ncolors = col + 1;
+            } else {
+                ++left;
+            }
         }
-        ncolors = k + 1;
         int tot = 0;
         block_exclusive_scan(left, s_scan, tot);
         if (tot == 0) break;
+    }
+    {
+        // Colours used: the largest over threads.
+        float f = float(ncolors);
+        f = block_reduce_max(f, s_red);
+        ncolors = int(f);
     }
     // Anything left goes to a last bucket solved by one thread in list order.
     const int serial = prx::kBigMaxColors;

@@ -10,7 +10,8 @@
 namespace prx {
 
 constexpr int kBigBlock = 1024;     // threads of the single-workgroup kernels
-constexpr int kBigMaxColors = 64;   // colours tried before the serial bucket (< 128)
+constexpr int kBigMaxColors = 64;   // colours (one bit each in a body's mask) before the serial bucket
+constexpr int kBigColorRounds = 48; // colouring rounds before the rest goes to the serial bucket
 
 // Indices into BigState::counters.
 enum : int { kBigPairs = 0, kBigColors = 1, kBigTouching = 2, kBigMinSepBits = 3, kBigCounters = 8 };
@@ -40,7 +41,10 @@ struct BigState {
     int* list;
     int* list2;
     int* color;
-    int* claim;       // B
+    int* claim;       // B * kBigMaxColors: lowest contact proposing (body, colour); kept at INT_MAX between rounds
+    std::uint64_t* used;  // B: colours taken at the body
+    int* tent;        // cap: proposed colour
+    int* win;         // cap
     int* colorStart;  // kBigMaxColors + 2
     int* counters;    // kBigCounters
     float* metrics;   // kBigMetrics

@@ -34,6 +34,10 @@ extern "C" hipError_t prx_big_launch_step(const prx::DeviceState* d, const prx::
                                           const prx::StepParams* c, hipStream_t s);
 extern "C" hipError_t prx_big_launch_pairs(const prx::DeviceState* d, const prx::BigState* g, hipStream_t s);
 extern "C" hipError_t prx_big_launch_metrics(const prx::DeviceState* d, const prx::BigState* g, hipStream_t s);
+extern "C" hipError_t prx_launch_step_tape(const prx::DeviceState* d, const prx::StepParams* c, int count,
+                                           hipStream_t stream);
+extern "C" hipError_t prx_launch_adjoint_step(const prx::DeviceState* d, const prx::Tape* t, const prx::AdjState* a,
+                                              const prx::StepParams* c, int nworlds, hipStream_t s);
 
 namespace {
 
@@ -1729,7 +1733,11 @@ int AllocBig(DeviceArena& a, prx::BigState& g, int P, int B, int cap) {
     PRX_HIP(a.alloc(g.list, std::size_t(g.cap)));
     PRX_HIP(a.alloc(g.list2, std::size_t(g.cap)));
     PRX_HIP(a.alloc(g.color, std::size_t(g.cap)));
-    PRX_HIP(a.alloc(g.claim, std::size_t(g.B)));
+    PRX_HIP(a.alloc(g.claim, std::size_t(g.B) * prx::kBigMaxColors));
+    PRX_HIP(hipMemset(g.claim, 0x7f, sizeof(int) * std::size_t(g.B) * prx::kBigMaxColors));
+    PRX_HIP(a.alloc(g.used, std::size_t(g.B)));
+    PRX_HIP(a.alloc(g.tent, std::size_t(g.cap)));
+    PRX_HIP(a.alloc(g.win, std::size_t(g.cap)));
     PRX_HIP(a.alloc(g.colorStart, std::size_t(prx::kBigMaxColors + 2)));
     PRX_HIP(a.alloc(g.counters, std::size_t(prx::kBigCounters)));
     PRX_HIP(a.alloc(g.metrics, std::size_t(prx::kBigMetrics)));
@@ -1891,6 +1899,217 @@ int prx_lbvh_pairs(const float* aabbs, const std::int32_t* bodies, const std::ui
     const int m = std::min(std::min(total, cap), g.pairCap);
     if (m > 0) PRX_HIP(hipMemcpy(pairs, g.pairs, std::size_t(m) * sizeof(int2), hipMemcpyDeviceToHost));
     if (total > g.pairCap) return Fail(PRX_E_LENGTH, "more pairs than the pair capacity");
+    return PRX_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Adjoint of K steps of a batch (loss = sum of squared distances of the
+// dynamic bodies' final positions to targets).
+
+}  // extern "C"
+
+namespace {
+
+// Arrays a step changes (the checkpoint of a batch).
+std::vector<std::pair<void*, std::size_t>> DynamicArrays(prx_batch* b) {
+    const std::size_t B = b->totB, P = b->totP, C = b->totC, J = b->totJ, nw = b->worlds.size();
+    auto& d = b->d;
+    return {{d.hdr, nw * sizeof(prx::WorldHeader)}, {d.bflags, B * 4}, {d.pos0, B * 16}, {d.pos1, B * 16},
+            {d.vel, B * 16}, {d.xf, B * 16}, {d.paabb, P * 16}, {d.pflags, P * 4}, {d.c_prox, C * 8},
+            {d.c_body, C * 8}, {d.c_shape, C * 8}, {d.c_id, C * 4}, {d.c_flags, C * 4}, {d.c_mat, C * 16},
+            {d.m_hdr, C * 16}, {d.m_geo, C * 16}, {d.m_p0, C * 16}, {d.m_p1, C * 16}, {d.c_free, C * 4},
+            {d.j_imp, J * 16}, {d.j_limit, J * 4}};
+}
+
+}  // namespace
+
+struct prx_adjoint {
+    prx_batch* b = nullptr;
+    int K = 0, S = 0, nseg = 0, itV = 0, itP = 0;
+    DeviceArena arena;
+    std::vector<prx::Tape> tapes;
+    prx::AdjState a{};
+    std::vector<std::vector<void*>> ckpt;
+};
+
+extern "C" {
+
+int prx_adjoint_create(prx_batch* b, int nsteps, int every, const prx_step_conf* conf, prx_adjoint** out) {
+    if (!b || !out || !conf || nsteps < 1 || every < 1) return Fail(PRX_E_INVALID_ARGUMENT, "bad argument");
+    for (prx_world* w : b->worlds) {
+        if (!w) return Fail(PRX_E_WRONG_STATE, "a member world left the batch");
+        if (!w->joints.empty()) return Fail(PRX_E_UNSUPPORTED, "the adjoint does not model joints");
+        for (const auto& hb : w->bodies) {
+            if (!(hb.flags & prx::kAccelerable)) continue;
+            if (hb.lc.x != 0.0f || hb.lc.y != 0.0f)
+                return Fail(PRX_E_UNSUPPORTED, "the adjoint models dynamic disks centred on their body");
+            for (std::uint32_t sid : hb.shapes) {
+                const HostShape& sh = w->shapes[sid];
+                if (sh.v.size() != 1 || sh.v[0].x != 0.0f || sh.v[0].y != 0.0f)
+                    return Fail(PRX_E_UNSUPPORTED, "the adjoint models dynamic disks centred on their body");
+            }
+        }
+    }
+    auto adj = std::make_unique<prx_adjoint>();
+    adj->b = b;
+    adj->K = nsteps;
+    adj->S = std::min(every, nsteps);
+    adj->nseg = (nsteps + adj->S - 1) / adj->S;
+    adj->itV = int(std::max<std::uint32_t>(1u, conf->reg_velocity_iters));
+    adj->itP = int(std::max<std::uint32_t>(1u, conf->reg_position_iters));
+    auto& A = adj->arena;
+    const std::size_t B = b->totB, C = b->totC, S = b->totS, nw = b->worlds.size();
+    adj->tapes.resize(std::size_t(adj->S));
+    for (auto& t : adj->tapes) {
+        PRX_HIP(A.alloc(t.vc, C * prx::kTapeVc));
+        PRX_HIP(A.alloc(t.gs, C * std::size_t(adj->itV) * 2));
+        PRX_HIP(A.alloc(t.ngs, C * std::size_t(adj->itP) * 2));
+        PRX_HIP(A.alloc(t.isl, B + nw));
+        PRX_HIP(A.alloc(t.nisl, nw));
+        PRX_HIP(A.alloc(t.inisl, B));
+        t.itV = adj->itV;
+        t.itP = adj->itP;
+    }
+    PRX_HIP(A.alloc(adj->a.xbar, B));
+    PRX_HIP(A.alloc(adj->a.vbar, B));
+    PRX_HIP(A.alloc(adj->a.lambar, C));
+    PRX_HIP(A.alloc(adj->a.lamtmp, C));
+    PRX_HIP(A.alloc(adj->a.cbar, 3 * C));
+    PRX_HIP(A.alloc(adj->a.fbar, S));
+    PRX_HIP(A.alloc(adj->a.ebar, S));
+    PRX_HIP(A.alloc(adj->a.status, 1));
+    PRX_HIP(A.alloc(adj->a.diag, 4));
+    const auto arrays = DynamicArrays(b);
+    adj->ckpt.resize(std::size_t(adj->nseg));
+    for (auto& ck : adj->ckpt) {
+        for (const auto& ar : arrays) {
+            std::uint8_t* p = nullptr;
+            PRX_HIP(A.alloc(p, std::max<std::size_t>(ar.second, 1)));
+            ck.push_back(p);
+        }
+    }
+    *out = adj.release();
+    return PRX_OK;
+}
+
+void prx_adjoint_destroy(prx_adjoint* adj) { delete adj; }
+
+int prx_adjoint_run(prx_adjoint* adj, const prx_step_conf* conf, const float* targets, float* loss,
+                    float* grad_velocity, float* grad_friction, float* grad_restitution) {
+    if (!adj || !conf || !targets) return Fail(PRX_E_INVALID_ARGUMENT, "null argument");
+    if (conf->do_toi) return Fail(PRX_E_UNSUPPORTED, "doToi = true is not stepped by this engine");
+    if (int(conf->reg_velocity_iters) > adj->itV || int(conf->reg_position_iters) > adj->itP)
+        return Fail(PRX_E_INVALID_ARGUMENT, "iteration counts above those the adjoint was created for");
+    prx_batch* b = adj->b;
+    for (prx_world* w : b->worlds)
+        if (!w) return Fail(PRX_E_WRONG_STATE, "a member world left the batch");
+    const int nw = int(b->worlds.size());
+    const std::size_t B = b->totB, S = b->totS;
+    hipStream_t st = b->stream;
+    prx::StepParams p = ToParams(*conf);
+    p.ldsBytes = std::uint32_t(b->lds_bytes);
+    p.coopMin = 0xffffffffu;  // every island on one lane (the recorded form)
+    const auto arrays = DynamicArrays(b);
+    auto save = [&](int k) -> int {
+        for (std::size_t i = 0; i < arrays.size(); ++i)
+            if (arrays[i].second)
+                PRX_HIP(hipMemcpyAsync(adj->ckpt[std::size_t(k)][i], arrays[i].first, arrays[i].second,
+                                       hipMemcpyDeviceToDevice, st));
+        return PRX_OK;
+    };
+    auto restore = [&](int k) -> int {
+        for (std::size_t i = 0; i < arrays.size(); ++i)
+            if (arrays[i].second)
+                PRX_HIP(hipMemcpyAsync(arrays[i].first, adj->ckpt[std::size_t(k)][i], arrays[i].second,
+                                       hipMemcpyDeviceToDevice, st));
+        return PRX_OK;
+    };
+    // Forward with checkpoints.
+    PRX_HIP(hipEventRecord(b->ev0, st));
+    for (int k = 0; k < adj->K; ++k) {
+        if (k % adj->S == 0)
+            if (int e = save(k / adj->S)) return e;
+        const hipError_t le = prx_launch_step(&b->d, &p, 0, nw, st);
+        if (le != hipSuccess) return Fail(PRX_E_DEVICE, std::string("launch: ") + hipGetErrorString(le));
+    }
+    // Loss and its gradient with respect to the final positions.
+    std::vector<float4> pos(B);
+    std::vector<std::uint32_t> flags(B);
+    PRX_HIP(hipStreamSynchronize(st));
+    if (int e = CheckStatus(b)) return e;
+    PRX_HIP(hipMemcpy(pos.data(), b->d.pos1, B * sizeof(float4), hipMemcpyDeviceToHost));
+    PRX_HIP(hipMemcpy(flags.data(), b->d.bflags, B * sizeof(std::uint32_t), hipMemcpyDeviceToHost));
+    std::vector<float4> xbar(B, float4{0, 0, 0, 0});
+    for (int w = 0; w < nw; ++w) {
+        const auto& h = b->hdr[std::size_t(w)];
+        double l = 0.0;
+        for (int i = 0; i < h.nbodies; ++i) {
+            const std::size_t g = std::size_t(h.body_off + i);
+            if (!(flags[g] & prx::kAccelerable)) continue;
+            const float dx = pos[g].x - targets[2 * g], dy = pos[g].y - targets[2 * g + 1];
+            l += double(dx) * dx + double(dy) * dy;
+            xbar[g] = float4{2.0f * dx, 2.0f * dy, 0.0f, 0.0f};
+        }
+        if (loss) loss[w] = float(l);
+    }
+    PRX_HIP(hipMemcpy(adj->a.xbar, xbar.data(), B * sizeof(float4), hipMemcpyHostToDevice));
+    PRX_HIP(hipMemset(adj->a.vbar, 0, B * sizeof(float4)));
+    PRX_HIP(hipMemset(adj->a.lambar, 0, b->totC * sizeof(float4)));
+    PRX_HIP(hipMemset(adj->a.fbar, 0, S * sizeof(float)));
+    PRX_HIP(hipMemset(adj->a.ebar, 0, S * sizeof(float)));
+    PRX_HIP(hipMemset(adj->a.status, 0, sizeof(int)));
+    PRX_HIP(hipMemset(adj->a.diag, 0, 4 * sizeof(float)));
+    // Backward, segment by segment: recompute the segment from its
+    // checkpoint recording every step, then reverse its steps.
+    for (int seg = adj->nseg - 1; seg >= 0; --seg) {
+        if (int e = restore(seg)) return e;
+        const int n = std::min(adj->S, adj->K - seg * adj->S);
+        for (int s = 0; s < n; ++s) {
+            prx::DeviceState dd = b->d;
+            dd.tape = adj->tapes[std::size_t(s)];
+            const hipError_t le = prx_launch_step_tape(&dd, &p, nw, st);
+            if (le != hipSuccess) return Fail(PRX_E_DEVICE, std::string("launch: ") + hipGetErrorString(le));
+        }
+        for (int s = n - 1; s >= 0; --s) {
+            const hipError_t le = prx_launch_adjoint_step(&b->d, &adj->tapes[std::size_t(s)], &adj->a, &p, nw, st);
+            if (le != hipSuccess) return Fail(PRX_E_DEVICE, std::string("launch: ") + hipGetErrorString(le));
+        }
+    }
+    PRX_HIP(hipEventRecord(b->ev1, st));
+    b->last_launches = 2 * adj->K;
+    PRX_HIP(hipStreamSynchronize(st));
+    int status = 0;
+    PRX_HIP(hipMemcpy(&status, adj->a.status, sizeof(int), hipMemcpyDeviceToHost));
+    if (grad_velocity) {
+        std::vector<float4> vb(B);
+        PRX_HIP(hipMemcpy(vb.data(), adj->a.vbar, B * sizeof(float4), hipMemcpyDeviceToHost));
+        for (std::size_t i = 0; i < B; ++i) {
+            grad_velocity[3 * i] = vb[i].x;
+            grad_velocity[3 * i + 1] = vb[i].y;
+            grad_velocity[3 * i + 2] = vb[i].z;
+        }
+    }
+    if (grad_friction) PRX_HIP(hipMemcpy(grad_friction, adj->a.fbar, S * sizeof(float), hipMemcpyDeviceToHost));
+    if (grad_restitution)
+        PRX_HIP(hipMemcpy(grad_restitution, adj->a.ebar, S * sizeof(float), hipMemcpyDeviceToHost));
+    // The batch is left at its starting state.
+    if (int e = restore(0)) return e;
+    PRX_HIP(hipStreamSynchronize(st));
+    for (prx_world* w : b->worlds) w->host_stale = true;
+    if (status) return Fail(PRX_E_UNSUPPORTED, "a contact the adjoint does not model (two-point manifold or face B)");
+    return PRX_OK;
+}
+
+int prx_adjoint_diagnostics(prx_adjoint* adj, float out[4]) {
+    if (!adj || !out) return Fail(PRX_E_INVALID_ARGUMENT, "null argument");
+    PRX_HIP(hipMemcpy(out, adj->a.diag, 4 * sizeof(float), hipMemcpyDeviceToHost));
+    return PRX_OK;
+}
+
+int prx_adjoint_timing(prx_adjoint* adj, float* ms) {
+    if (!adj || !ms) return Fail(PRX_E_INVALID_ARGUMENT, "null argument");
+    PRX_HIP(hipEventElapsedTime(ms, adj->b->ev0, adj->b->ev1));
     return PRX_OK;
 }
 

@@ -69,6 +69,38 @@ struct StepParams {
     std::uint32_t coopMin;   // islands with at least this many contacts are swept by the whole wave
 };
 
+// Record of one step's solve for the adjoint (one-point manifolds): per
+// island-order constraint slot the velocity-constraint record as built plus
+// its geometry inputs, the inputs of every Gauss-Seidel update and of every
+// position-solver update, and the island table.
+constexpr int kTapeVc = 15;
+struct Tape {
+    float4* vc;           // slot * kTapeVc: VelocityConstraint record [0..8], body positions [9],
+                          // (vn0, contact id, manifold type, radius A) [10], (radius B, shape A, shape B, -) [11],
+                          // velocities of A [12] and B [13] at creation, final (normal, tangent) impulse [14]
+    float4* gs;           // (slot * itV + it) * 2: (vA, wA, tangent impulse), (vB, wB, normal impulse)
+    float4* ngs;          // (slot * itP + it) * 2: position A (w: separation), position B before the update
+    int4* isl;            // per island: first slot, end slot, velocity iterations run, position iterations run
+    int* nisl;            // per world
+    std::uint8_t* inisl;  // per body: part of an island
+    int itV, itP;
+};
+
+// Adjoint state of a batch (one backward step per launch).
+struct AdjState {
+    float4* xbar;   // per body: d loss / d (x, y, angle)
+    float4* vbar;   // per body: d loss / d (vx, vy, w)
+    float4* lambar; // per world contact pool id (segment of contact capacity): d loss / d (normal, tangent)
+                    // impulse at the start of the later step, and those impulses' values
+    float2* lamtmp; // per island-order slot
+    float4* cbar;   // per slot, 3 float4: accumulated adjoints of the constraint's constants
+    float* fbar;    // per shape: d loss / d friction
+    float* ebar;    // per shape: d loss / d restitution
+    int* status;    // 1: a record the adjoint does not model (two-point manifold, face B, joints)
+    float* diag;    // counters of the last run: slots reversed, of them with a restitution bias,
+                    // position-solver updates reversed, velocity updates reversed
+};
+
 struct DeviceState {
     int nworlds;
     WorldHeader* hdr;
@@ -146,6 +178,8 @@ struct DeviceState {
     // Optional per-world phase timings of the last step (shader clock cycles),
     // kPhases entries per world; null when not collected.
     unsigned long long* phase_cycles;
+    // Solve record for the adjoint (only read by the recording kernel).
+    Tape tape;
 };
 
 // Phases timed by phase_cycles.

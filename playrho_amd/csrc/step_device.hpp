@@ -1639,9 +1639,11 @@ __device__ __forceinline__ bool joints_position(const float4* jbase, int j0, int
 // Lane form of the warm start and velocity sweeps of one island
 // (AabbTreeWorld.cpp:1590-1625): contact warm start, joint warm start, then
 // per iteration the joints followed by the contacts.
-template <bool kLds>
+template <bool kLds, bool kTape = false>
 __device__ __forceinline__ void island_velocity(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
-                                                int c0, int c1, int j0, int j1, const StepParams& c) {
+                                                int c0, int c1, int j0, int j1, const StepParams& c,
+                                                int islandSlot = 0) {
+    if constexpr (kTape) d.tape.isl[islandSlot].z = 0;
     if (c0 >= c1 && j0 >= j1) return;
     float4* base = d.s_vc + std::size_t(x.co) * prx::kVcStride;
     float4* jbase = d.s_jc + std::size_t(x.jo) * prx::kJcStride;
@@ -1660,6 +1662,7 @@ __device__ __forceinline__ void island_velocity(const DeviceState& d, const Worl
     }
     if (c.warmStart) joints_velocity(jbase, j0, j1, bs, true);
     for (std::uint32_t it = 0; it < c.velIters; ++it) {
+        if constexpr (kTape) d.tape.isl[islandSlot].z = int(it) + 1;
         const bool jointsOkay = joints_velocity(jbase, j0, j1, bs, false);
         float newInc = 0.0f;
         if (c0 >= c1) {
@@ -1672,6 +1675,12 @@ __device__ __forceinline__ void island_velocity(const DeviceState& d, const Worl
             const int en = (e + 1 < c1) ? e + 1 : e;
             BodyC nA, nB;
             const Vc nvc = load_vc(base + std::size_t(en) * prx::kVcStride, nA, nB);
+            if constexpr (kTape) {
+                const float4 va = bs.vel(vc.bA), vb = bs.vel(vc.bB);
+                float4* g = d.tape.gs + ((std::size_t(x.co) + std::size_t(e)) * std::size_t(d.tape.itV) + it) * 2;
+                g[0] = float4{va.x, va.y, va.z, vc.p[0].ti};
+                g[1] = float4{vb.x, vb.y, vb.z, vc.p[0].ni};
+            }
             const float mi = solve_velocity_one(vc, A, B, bs);
             float4* rec = base + std::size_t(e) * prx::kVcStride;
             rec[4] = float4{vc.p[0].ni, vc.p[0].ti, vc.p[0].nm, vc.p[0].tm};
@@ -1791,12 +1800,15 @@ __device__ __forceinline__ void island_velocity_wave(const DeviceState& d, const
 
 // Lane form of the position sweeps of one island (ContactSolver.cpp:494-643,
 // AabbTreeWorld.cpp:1627-1644).  Returns whether the island was solved.
-template <bool kLds>
+template <bool kLds, bool kTape = false>
 __device__ __forceinline__ bool island_position(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
-                                                int c0, int c1, int j0, int j1, const StepParams& c) {
+                                                int c0, int c1, int j0, int j1, const StepParams& c,
+                                                int islandSlot = 0) {
     const float4* base = d.s_pc + std::size_t(x.co) * prx::kPcStride;
     const float4* jbase = d.s_jc + std::size_t(x.jo) * prx::kJcStride;
+    if constexpr (kTape) d.tape.isl[islandSlot].w = 0;
     for (std::uint32_t it = 0; it < c.posIters; ++it) {
+        if constexpr (kTape) d.tape.isl[islandSlot].w = int(it) + 1;
         float minSep = __builtin_inff();
         if (c0 < c1) {
             Pc pc = load_pc(base + std::size_t(c0) * prx::kPcStride);
@@ -1805,7 +1817,16 @@ __device__ __forceinline__ bool island_position(const DeviceState& d, const Worl
                 const Pc npc = load_pc(base + std::size_t(en) * prx::kPcStride);
                 float4 pa = bs.pos(pc.bA);
                 float4 pb = bs.pos(pc.bB);
+                if constexpr (kTape) {
+                    float4* g = d.tape.ngs + ((std::size_t(x.co) + std::size_t(e)) * std::size_t(d.tape.itP) + it) * 2;
+                    g[0] = pa;
+                    g[1] = pb;
+                }
                 const float sep = solve_pc(pc.m, pc.totalRadius, pc.A, pc.B, pa, pb, c);
+                if constexpr (kTape) {
+                    float4* g = d.tape.ngs + ((std::size_t(x.co) + std::size_t(e)) * std::size_t(d.tape.itP) + it) * 2;
+                    g[0].w = sep;
+                }
                 bs.set_pos(pc.bA, pa);
                 bs.set_pos(pc.bB, pb);
                 minSep = fmin_std(minSep, sep);
@@ -1874,7 +1895,7 @@ __device__ __forceinline__ bool island_position_wave(const DeviceState& d, const
     return solved;
 }
 
-template <bool kLds>
+template <bool kLds, bool kTape = false>
 __device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, const WorldCtx& x, int nisl,
                                                        const IslandScratch& S, const StepParams& c, PhaseTimer& T) {
     const int bo = x.bo;
@@ -1920,6 +1941,28 @@ __device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, con
         const std::size_t slot = std::size_t(x.co) + std::size_t(e);
         store_vc(d.s_vc + slot * prx::kVcStride, vc, A, B);
         store_pc(d.s_pc + slot * prx::kPcStride, m, bd.x, bd.y, rA + rB, A, B);
+        if constexpr (kTape) {
+            float4* t = d.tape.vc + slot * prx::kTapeVc;
+            store_vc(t, vc, A, B);
+            const float4 vA = bs.vel(bd.x), vB = bs.vel(bd.y);
+            const float vn0 =
+                dot(rel_vel(V2{vA.x, vA.y}, vA.z, vc.p[0].relA, V2{vB.x, vB.y}, vB.z, vc.p[0].relB), vc.normal);
+            // Manifold points in world coordinates (for circles: the two
+            // circle centres; for faces: body A's position and the clip point).
+            const V2 wA = (m.type == kCircles) ? transform(m.lpt, xfA) : V2{pA.x, pA.y};
+            const V2 wB = transform(m.p[0].lp, xfB);
+            t[9] = float4{wA.x, wA.y, wB.x, wB.y};
+            t[10] = float4{vn0, __uint_as_float(d.c_id[gi]), __uint_as_float(m.type | (m.n << 8)), rA};
+            t[11] = float4{rB, __int_as_float(ss.x), __int_as_float(ss.y), 0.0f};
+            t[12] = vA;
+            t[13] = vB;
+        }
+    }
+    if constexpr (kTape) {
+        if (x.lane == 0) d.tape.nisl[x.w] = nisl;
+        for (int is = x.lane; is < nisl; is += kWave)
+            d.tape.isl[x.bo + x.w + is] = int4{S.istart[is].y, S.istart[is + 1].y, 0, 0};
+        for (int b = x.lane; b < x.nb; b += kWave) d.tape.inisl[x.bo + b] = S.bisland[b] >= 0 ? 1 : 0;
     }
     // Joint set-up (the non-warm-start half of InitVelocity), island order.
     const int nje = S.jstart[nisl];
@@ -1935,13 +1978,14 @@ __device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, con
     // Warm start and velocity sweeps: islands below coopMin contacts one lane
     // each, larger ones by the whole wave, one after another.
     float4* stage = g_smem + (int(c.ldsBytes) / 16 - prx::kStageFloat4);
-    const int coopMin = c.coopMin > 0 ? int(c.coopMin) : 1;
+    const int coopMin = (c.coopMin == 0u) ? 1 : (c.coopMin > 0x7fffffffu ? 0x7fffffff : int(c.coopMin));
     auto constraints = [&](int is) {
         return (S.istart[is + 1].y - S.istart[is].y) + (S.jstart[is + 1] - S.jstart[is]);
     };
     for (int is = x.lane; is < nisl; is += kWave) {
         if (constraints(is) < coopMin)
-            island_velocity<kLds>(d, x, bs, S.istart[is].y, S.istart[is + 1].y, S.jstart[is], S.jstart[is + 1], c);
+            island_velocity<kLds, kTape>(d, x, bs, S.istart[is].y, S.istart[is + 1].y, S.jstart[is],
+                                         S.jstart[is + 1], c, x.bo + x.w + is);
     }
     __syncthreads();
     for (int base = 0; base < nisl; base += kWave) {
@@ -1969,7 +2013,8 @@ __device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, con
     // Position sweeps, split like the velocity sweeps.
     for (int is = x.lane; is < nisl; is += kWave) {
         if (constraints(is) < coopMin &&
-            island_position<kLds>(d, x, bs, S.istart[is].y, S.istart[is + 1].y, S.jstart[is], S.jstart[is + 1], c))
+            island_position<kLds, kTape>(d, x, bs, S.istart[is].y, S.istart[is + 1].y, S.jstart[is],
+                                         S.jstart[is + 1], c, x.bo + x.w + is))
             S.isl_word[is] = 1u;
     }
     __syncthreads();
@@ -1989,6 +2034,8 @@ __device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, con
     for (int e = x.lane; e < nce; e += kWave) {
         const int gi = x.co + S.icont[e];
         const float4* rec = d.s_vc + (std::size_t(x.co) + std::size_t(e)) * prx::kVcStride;
+        if constexpr (kTape)
+            d.tape.vc[(std::size_t(x.co) + std::size_t(e)) * prx::kTapeVc + 14] = float4{rec[4].x, rec[4].y, 0.0f, 0.0f};
         const std::uint32_t n = __float_as_uint(rec[1].w);
         const float4 q0 = rec[4];
         float4 m0 = d.m_p0[gi];
@@ -2085,6 +2132,7 @@ __device__ __forceinline__ std::uint32_t synchronize_body(const DeviceState& d, 
     return moved;
 }
 
+template <bool kTape = false>
 __device__ __forceinline__ void step_world(const DeviceState& d, int w, const StepParams& c) {
     WorldHeader& H = d.hdr[w];
     WorldCtx x;
@@ -2152,7 +2200,8 @@ __device__ __forceinline__ void step_world(const DeviceState& d, int w, const St
         stats[4] = std::uint32_t(nisl);
         T.mark(prx::kPhIslands);
         const bool ldsBodies = x.nb * 32 + prx::kStageFloat4 * 16 <= int(c.ldsBytes);
-        stats[7] = ldsBodies ? solve_islands<true>(d, x, nisl, S, c, T) : solve_islands<false>(d, x, nisl, S, c, T);
+        stats[7] = ldsBodies ? solve_islands<true, kTape>(d, x, nisl, S, c, T)
+                             : solve_islands<false, kTape>(d, x, nisl, S, c, T);
         // Synchronise proxies of islanded speedable bodies (AabbTreeWorld.cpp:1532-1544).
         std::uint32_t moved = 0;
         for (int b = x.lane; b < x.nb; b += kWave) {

@@ -25,7 +25,14 @@ namespace prxd {
 
 __global__ __launch_bounds__(64) void k_step_worlds(DeviceState d, StepParams c) {
     const int w = d.schedule[blockIdx.x];
-    step_world(d, w, c);
+    step_world<false>(d, w, c);
+}
+
+// The same step recording its solve for the adjoint (every island swept by
+// one lane: coopMin must exceed every island's constraint count).
+__global__ __launch_bounds__(64) void k_step_worlds_tape(DeviceState d, StepParams c) {
+    const int w = d.schedule[blockIdx.x];
+    step_world<true>(d, w, c);
 }
 
 }  // namespace prxd
@@ -36,6 +43,12 @@ extern "C" hipError_t prx_launch_step(const prx::DeviceState* d, const prx::Step
     prx::DeviceState dd = *d;
     dd.schedule = d->schedule + first;
     hipLaunchKernelGGL(prxd::k_step_worlds, dim3(unsigned(count)), dim3(64), c->ldsBytes, stream, dd, *c);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t prx_launch_step_tape(const prx::DeviceState* d, const prx::StepParams* c, int count,
+                                           hipStream_t stream) {
+    hipLaunchKernelGGL(prxd::k_step_worlds_tape, dim3(unsigned(count)), dim3(64), c->ldsBytes, stream, *d, *c);
     return hipGetLastError();
 }
 

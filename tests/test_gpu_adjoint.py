@@ -1,0 +1,112 @@
+"""The adjoint kernel against finite differences of the reference restatement
+in double precision (the FP64 build of the same step), on the differentiable
+billiards workload: 64 disks in a walled box with a frictional floor,
+loss = sum |x_K - target|^2, gradients with respect to initial velocities,
+friction and restitution."""
+import numpy as np
+import pytest
+
+import playrho_amd as pa
+from tests import dumpfmt, oracle_lib
+
+pytestmark = pytest.mark.gpu
+
+
+def _conf_bytes(c):
+    import struct
+    floats = [getattr(c, f) for f, _ in pa.StepConf._fields_[:18]]
+    return (struct.pack("<I", 18) + struct.pack("<18f", *floats) +
+            struct.pack("<III", c.reg_velocity_iters, c.reg_position_iters,
+                        (1 if c.do_warm_start else 0) | (4 if c.do_blocksolve else 0)))
+
+
+def _loss_f64(desc, conf, steps, targets, dyn):
+    ow = oracle_lib.OracleWorld(desc + _conf_bytes(conf), precision=64)
+    ow.step(steps, record=False)
+    b = ow.bodies_f64(len(dyn))
+    d = b[dyn, :2] - targets[dyn]
+    return float(np.sum(d * d))
+
+
+def _setup(seed, world, steps):
+    w = pa.World.scene("billiards", [seed, world])
+    desc = w.save_desc()
+    _, bodies, _ = dumpfmt.read_description(dumpfmt.Reader(desc))
+    dyn = np.array([bool(b["flags"] & 2) for b in bodies])
+    rng = np.random.default_rng(100 + world)
+    targets = np.zeros((len(bodies), 2), dtype=np.float64)
+    targets[:, 0] = rng.uniform(-8, 8, len(bodies))
+    targets[:, 1] = rng.uniform(1, 10, len(bodies))
+    targets = targets.astype(np.float32).astype(np.float64)
+    return w, desc, bodies, dyn, targets
+
+
+@pytest.mark.parametrize("steps,every", [(12, 4), (30, 16), (60, 16)])
+def test_adjoint_matches_fp64_finite_differences(steps, every):
+    conf = pa.StepConf()
+    w, desc, bodies, dyn, targets = _setup(1, 0, steps)
+    batch = pa.Batch([w], use_graph=False)
+    adj = pa.Adjoint(batch, steps, every, conf)
+    print("adjoint steps", steps, "every", every, flush=True)
+    out = adj.run(targets.astype(np.float32))
+    print("diagnostics", adj.diagnostics(), flush=True)
+    loss64 = _loss_f64(desc, conf, steps, targets, dyn)
+    print("loss device %.6f fp64 %.6f" % (out["loss"][0], loss64), flush=True)
+    assert abs(out["loss"][0] - loss64) < 1e-3 * loss64
+
+    shapes, _ = pa.desc_offsets(desc)
+    sdesc, _, _ = dumpfmt.read_description(dumpfmt.Reader(desc))
+    checks = []
+    for b in (1, 9, 30, 63):
+        v = bodies[b]["vals"][8:11].astype(np.float64)
+        for k in (0, 1):
+            def f(eps, b=b, k=k, v=v):
+                vv = v.copy()
+                vv[k] += eps
+                return _loss_f64(pa.desc_with(desc, velocities={b: vv}), conf, steps, targets, dyn)
+            checks.append(("v%d%s" % (b, "xy"[k]), out["grad_velocity"][b, k], f, 1e-4))
+    for s in (4, 12, 40, 66):
+        fr = float(sdesc[s]["friction"])
+        re = float(sdesc[s]["restitution"])
+        checks.append(("f%d" % s, out["grad_friction"][s],
+                       lambda eps, s=s, fr=fr: _loss_f64(pa.desc_with(desc, friction={s: fr + eps}), conf, steps,
+                                                         targets, dyn), 1e-4))
+        checks.append(("e%d" % s, out["grad_restitution"][s],
+                       lambda eps, s=s, re=re: _loss_f64(pa.desc_with(desc, restitution={s: re + eps}), conf,
+                                                         steps, targets, dyn), 1e-4))
+    got, want = [], []
+    for name, g, f, eps in checks:
+        fd = (f(eps) - f(-eps)) / (2 * eps)
+        got.append(float(g))
+        want.append(fd)
+        print("%-6s adjoint %+.6e  fd %+.6e" % (name, g, fd), flush=True)
+    got, want = np.array(got), np.array(want)
+    rel = np.linalg.norm(got - want) / np.linalg.norm(want)
+    print("relative error (all parameters) %.3e" % rel, flush=True)
+    assert rel < 1e-3
+
+
+def test_adjoint_batched_equals_single():
+    """Every world of a batch gets the gradients it gets alone."""
+    conf = pa.StepConf()
+    steps = 10
+    singles = []
+    for wi in range(3):
+        w, desc, bodies, dyn, targets = _setup(1, wi, steps)
+        out = pa.Adjoint(pa.Batch([w], use_graph=False), steps, 4, conf).run(targets.astype(np.float32))
+        singles.append((desc, targets, out))
+    worlds = [pa.World.from_desc(d) for d, _, _ in singles]
+    tg = np.concatenate([t for _, t, _ in singles]).astype(np.float32)
+    out = pa.Adjoint(pa.Batch(worlds, use_graph=False), steps, 4, conf).run(tg)
+    nb = [len(t) for _, t, _ in singles]
+    ns = [pa.World.from_desc(d).body_count for d, _, _ in singles]
+    ob = 0
+    osh = 0
+    for k, (desc, _, o1) in enumerate(singles):
+        assert out["loss"][k] == o1["loss"][0]
+        assert np.array_equal(out["grad_velocity"][ob:ob + nb[k]], o1["grad_velocity"])
+        nsh = len(pa.desc_offsets(desc)[0])
+        assert np.array_equal(out["grad_friction"][osh:osh + nsh], o1["grad_friction"])
+        ob += nb[k]
+        osh += nsh
+    del ns
