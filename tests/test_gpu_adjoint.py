@@ -41,9 +41,21 @@ def _setup(seed, world, steps):
     return w, desc, bodies, dyn, targets
 
 
-@pytest.mark.parametrize("steps,every", [(12, 4), (30, 16), (60, 16)])
-def test_adjoint_matches_fp64_finite_differences(steps, every):
+# The gradient is that of the float32 step as executed, and finite differences
+# of the FP64 step differentiate the same function only while both precisions
+# take the same discrete path.  The velocity loop's early exit
+# (newInc <= regMinMomentum, default 0) is the first to part: float32 keeps
+# iterating on rounding-level increments where float64 reaches exactly zero
+# (on this world from step 9 on), and gradients of different iteration counts
+# differ.  The long horizon therefore runs every velocity iteration
+# (reg_min_momentum < 0); with it, this world's float32 and float64 runs agree
+# in every iteration count and impulse clamp over 60 steps (they part at
+# step 72, on a normal impulse clamp).
+@pytest.mark.parametrize("steps,every,full_iters", [(12, 4, False), (30, 16, False), (60, 16, True)])
+def test_adjoint_matches_fp64_finite_differences(steps, every, full_iters):
     conf = pa.StepConf()
+    if full_iters:
+        conf.reg_min_momentum = -1.0
     w, desc, bodies, dyn, targets = _setup(1, 0, steps)
     batch = pa.Batch([w], use_graph=False)
     adj = pa.Adjoint(batch, steps, every, conf)
@@ -56,33 +68,54 @@ def test_adjoint_matches_fp64_finite_differences(steps, every):
 
     shapes, _ = pa.desc_offsets(desc)
     sdesc, _, _ = dumpfmt.read_description(dumpfmt.Reader(desc))
+    # Parameters enter the description as float32: a perturbation is the
+    # difference of the two float32 values actually stored.
+    def f32(x):
+        return float(np.float32(x))
+
     checks = []
     for b in (1, 9, 30, 63):
         v = bodies[b]["vals"][8:11].astype(np.float64)
         for k in (0, 1):
             def f(eps, b=b, k=k, v=v):
                 vv = v.copy()
-                vv[k] += eps
-                return _loss_f64(pa.desc_with(desc, velocities={b: vv}), conf, steps, targets, dyn)
-            checks.append(("v%d%s" % (b, "xy"[k]), out["grad_velocity"][b, k], f, 1e-4))
+                vv[k] = f32(v[k] + eps)
+                return _loss_f64(pa.desc_with(desc, velocities={b: vv}), conf, steps, targets, dyn), vv[k]
+            checks.append(("v%d%s" % (b, "xy"[k]), out["grad_velocity"][b, k], f, 1e-3))
     for s in (4, 12, 40, 66):
         fr = float(sdesc[s]["friction"])
         re = float(sdesc[s]["restitution"])
         checks.append(("f%d" % s, out["grad_friction"][s],
-                       lambda eps, s=s, fr=fr: _loss_f64(pa.desc_with(desc, friction={s: fr + eps}), conf, steps,
-                                                         targets, dyn), 1e-4))
+                       lambda eps, s=s, fr=fr: (_loss_f64(pa.desc_with(desc, friction={s: f32(fr + eps)}), conf,
+                                                          steps, targets, dyn), f32(fr + eps)), 1e-3))
         checks.append(("e%d" % s, out["grad_restitution"][s],
-                       lambda eps, s=s, re=re: _loss_f64(pa.desc_with(desc, restitution={s: re + eps}), conf,
-                                                         steps, targets, dyn), 1e-4))
-    got, want = [], []
+                       lambda eps, s=s, re=re: (_loss_f64(pa.desc_with(desc, restitution={s: f32(re + eps)}),
+                                                          conf, steps, targets, dyn), f32(re + eps)), 1e-3))
+
+    def central(f, eps):
+        (lp, xp), (lm, xm) = f(eps), f(-eps)
+        return (lp - lm) / (xp - xm)
+
+    # Central differences at two step sizes: where they disagree, a contact
+    # event (touching onset, a clamp switching, an early exit) lies within the
+    # perturbation and the finite difference is not a derivative; such
+    # parameters are reported and left out.
+    got, want, skipped = [], [], []
     for name, g, f, eps in checks:
-        fd = (f(eps) - f(-eps)) / (2 * eps)
+        fd = central(f, eps)
+        fd2 = central(f, eps / 10)
+        ok = abs(fd - fd2) <= 1e-4 * max(abs(fd), 1.0)
+        print("%-6s adjoint %+.6e  fd %+.6e  fd/10 %+.6e%s" % (name, g, fd, fd2, "" if ok else "  (event)"),
+              flush=True)
+        if not ok:
+            skipped.append(name)
+            continue
         got.append(float(g))
         want.append(fd)
-        print("%-6s adjoint %+.6e  fd %+.6e" % (name, g, fd), flush=True)
     got, want = np.array(got), np.array(want)
     rel = np.linalg.norm(got - want) / np.linalg.norm(want)
-    print("relative error (all parameters) %.3e" % rel, flush=True)
+    print("relative error over %d parameters %.3e (left out: %s)" % (len(got), rel, skipped), flush=True)
+    assert len(skipped) <= len(checks) // 4
     assert rel < 1e-3
 
 
@@ -110,3 +143,27 @@ def test_adjoint_batched_equals_single():
         ob += nb[k]
         osh += nsh
     del ns
+
+
+def test_autograd_function_matches_adjoint():
+    """torch.autograd binding: gradients equal the adjoint's, and a gradient
+    step on the initial velocities lowers the loss."""
+    import torch
+    from playrho_amd.autograd import BilliardsLoss
+    steps = 16
+    w, desc, bodies, dyn, targets = _setup(2, 0, steps)
+    sdesc, _, _ = dumpfmt.read_description(dumpfmt.Reader(desc))
+    v0 = torch.tensor(np.array([[b["vals"][8:11] for b in bodies]]), dtype=torch.float64, requires_grad=True)
+    fr = torch.tensor([[s["friction"] for s in sdesc]], dtype=torch.float64, requires_grad=True)
+    re = torch.tensor([[s["restitution"] for s in sdesc]], dtype=torch.float64, requires_grad=True)
+    loss = BilliardsLoss.apply(v0, fr, re, [desc], targets, steps, 8)
+    loss.sum().backward()
+    direct = pa.Adjoint(pa.Batch([pa.World.from_desc(desc)], use_graph=False), steps, 8).run(
+        targets.astype(np.float32))
+    assert np.allclose(v0.grad.numpy()[0], direct["grad_velocity"], rtol=0, atol=0)
+    assert np.allclose(re.grad.numpy()[0], direct["grad_restitution"], rtol=0, atol=0)
+    with torch.no_grad():
+        v1 = v0 - 1e-3 * v0.grad
+    loss1 = BilliardsLoss.apply(v1, fr, re, [desc], targets, steps, 8)
+    print("loss %.4f -> %.4f" % (float(loss), float(loss1)), flush=True)
+    assert float(loss1) < float(loss)
