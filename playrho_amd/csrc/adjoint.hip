@@ -237,29 +237,29 @@ __global__ __launch_bounds__(64) void k_adjoint_step(DeviceState d, Tape t, AdjS
                 a.cbar[slot * 3 + 2] = float4{nmbar, tmbar, biasbar, fbar};
             }
         }
-        // Warm start and constraint creation, per slot (order-free: warm
-        // start deltas do not depend on velocities).
+        // Warm start, per slot.  Its deltas do not depend on velocities, so
+        // the velocity adjoints here are those after every slot's warm start
+        // and before any creation-time term below is added: constraint
+        // creation (the restitution bias reads the velocities) precedes the
+        // whole island's warm start in the forward step.
         for (int e = rng.x; e < rng.y; ++e) {
             const std::size_t slot = std::size_t(co) + e;
             const float4* vc = t.vc + slot * prx::kTapeVc;
             if ((__float_as_uint(vc[10].z) >> 8) != 1u) continue;
-            const std::uint32_t type = __float_as_uint(vc[10].z) & 0xffu;
             const V2 n = v2(vc[0]);
             const V2 tg = fwdperp(n);
             const V2 rA{vc[3].x, vc[3].y}, rB{vc[3].z, vc[3].w};
             const float lnInit = vc[4].x, ltInit = vc[4].y;
-            const float nm = vc[4].z, tm = vc[4].w;
             const float4 ms = vc[8];
             const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
             const int bA = __float_as_int(vc[2].w), bB = __float_as_int(vc[5].z);
-            const float4 va4 = a.vbar[bo + bA], vb4 = a.vbar[bo + bB];
-            V2 vbA = v2(va4), vbB = v2(vb4);
-            float wbA = va4.z, wbB = vb4.z;
             float2 lb = a.lamtmp[slot];
-            const float4 c0 = a.cbar[slot * 3 + 0], c1 = a.cbar[slot * 3 + 1], c2 = a.cbar[slot * 3 + 2];
-            V2 nbar{c0.x, c0.y}, rAbar{c0.z, c0.w}, rBbar{c1.x, c1.y}, tbar{c1.z, c1.w};
-            const float nmbar = c2.x, tmbar = c2.y, biasbar = c2.z, fbarC = c2.w;
             if (c.warmStart) {
+                const float4 va4 = a.vbar[bo + bA], vb4 = a.vbar[bo + bB];
+                const V2 vbA = v2(va4), vbB = v2(vb4);
+                const float wbA = va4.z, wbB = vb4.z;
+                const float4 c0 = a.cbar[slot * 3 + 0], c1 = a.cbar[slot * 3 + 1];
+                V2 nbar{c0.x, c0.y}, rAbar{c0.z, c0.w}, rBbar{c1.x, c1.y}, tbar{c1.z, c1.w};
                 const V2 P = mul(lnInit, n) + mul(ltInit, tg);
                 const V2 Pbar = mul(mB, vbB) - mul(mA, vbA) + mul(revperp(rA), -iA * wbA) + mul(revperp(rB), iB * wbB);
                 rAbar = rAbar + mul(perp_grad(P), -iA * wbA);
@@ -268,11 +268,33 @@ __global__ __launch_bounds__(64) void k_adjoint_step(DeviceState d, Tape t, AdjS
                 lb.y += dot(Pbar, tg);
                 nbar = nbar + mul(Pbar, lnInit);
                 tbar = tbar + mul(Pbar, ltInit);
+                a.cbar[slot * 3 + 0] = float4{nbar.x, nbar.y, rAbar.x, rAbar.y};
+                a.cbar[slot * 3 + 1] = float4{rBbar.x, rBbar.y, tbar.x, tbar.y};
             }
             // Impulses as carried from the previous step (dtRatio scaling is
             // part of the record: warm start uses them as stored).
             const std::uint32_t id = __float_as_uint(vc[10].y);
             a.lambar[co + int(id)] = float4{lb.x, lb.y, lnInit, ltInit};
+        }
+        // Constraint creation, per slot.
+        for (int e = rng.x; e < rng.y; ++e) {
+            const std::size_t slot = std::size_t(co) + e;
+            const float4* vc = t.vc + slot * prx::kTapeVc;
+            if ((__float_as_uint(vc[10].z) >> 8) != 1u) continue;
+            const std::uint32_t type = __float_as_uint(vc[10].z) & 0xffu;
+            const V2 n = v2(vc[0]);
+            const V2 tg = fwdperp(n);
+            const V2 rA{vc[3].x, vc[3].y}, rB{vc[3].z, vc[3].w};
+            const float nm = vc[4].z, tm = vc[4].w;
+            const float4 ms = vc[8];
+            const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
+            const int bA = __float_as_int(vc[2].w), bB = __float_as_int(vc[5].z);
+            const float4 va4 = a.vbar[bo + bA], vb4 = a.vbar[bo + bB];
+            V2 vbA = v2(va4), vbB = v2(vb4);
+            float wbA = va4.z, wbB = vb4.z;
+            const float4 c0 = a.cbar[slot * 3 + 0], c1 = a.cbar[slot * 3 + 1], c2 = a.cbar[slot * 3 + 2];
+            V2 nbar{c0.x, c0.y}, rAbar{c0.z, c0.w}, rBbar{c1.x, c1.y}, tbar{c1.z, c1.w};
+            const float nmbar = c2.x, tmbar = c2.y, biasbar = c2.z, fbarC = c2.w;
             // Effective masses (VelocityConstraint.cpp:136-185).
             {
                 const float cnA = cross(rA, n), cnB = cross(rB, n);
