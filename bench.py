@@ -50,6 +50,9 @@ def parse():
     p.add_argument("--adjoint-worlds", type=int, default=1024, help="differentiable billiards worlds (0: skip)")
     p.add_argument("--adjoint-steps", type=int, default=120)
     p.add_argument("--adjoint-every", type=int, default=16)
+    p.add_argument("--chain-worlds", type=int, default=8192, help="articulated chain worlds (0: skip)")
+    p.add_argument("--chain-steps", type=int, default=240)
+    p.add_argument("--chain-cpu-worlds", type=int, default=1024, help="worlds in the CPU sample")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only run a plain device copy of MIB MiB five times (counter calibration)")
     return p.parse_args()
@@ -77,6 +80,10 @@ def measured_traffic():
         return None
     with open(path) as f:
         return json.load(f)
+
+
+def cpu_threads(args):
+    return args.cpu_threads or min(16, os.cpu_count() or 1)
 
 
 def algorithmic_bytes(awake, proxies, shape_bytes, contacts, joints=0):
@@ -191,6 +198,58 @@ def adjoint_bench(args, pa, conf):
     }
 
 
+def articulated(args, pa, conf, threads):
+    """Batched articulated worlds: 8192 worlds, each a 32-link revolute-joint
+    chain of 1 x 0.125 half-extent boxes hanging from a static anchor (joint
+    friction by a zero-speed motor of 2 N m, every third joint limited to
+    +-60 degrees) with 64 seeded falling boxes and disks and a ground edge;
+    body-steps/s over the configured steps, next to the reference CPU build on
+    a sample of the same worlds."""
+    worlds = [pa.World.scene("chain", [args.seed, w]) for w in range(args.chain_worlds)]
+    joints = sum(w.joint_count for w in worlds)
+    batch = pa.Batch(worlds, use_graph=True)
+    dyn = batch.dynamic_bodies()
+    warm = 10
+    batch.step(conf, warm)
+    batch.sync()
+    t0 = time.perf_counter()
+    batch.step(conf, args.chain_steps)
+    batch.sync()
+    wall = time.perf_counter() - t0
+    kernel_ms, launches = batch.last_timing()
+    batch.download()
+    c1 = batch.counts()
+    out = {
+        "workload": "%d chain worlds (32 revolute-jointed 1 x 0.125 boxes from a static anchor, 64 falling "
+                    "boxes and disks, ground edge; seed %d), dt=1/60 8/3 iterations doToi=false, one hipGraph "
+                    "launch per step; steps %d..%d timed" % (args.chain_worlds, args.seed, warm + 1,
+                                                              warm + args.chain_steps),
+        "worlds": args.chain_worlds,
+        "dynamic_bodies": int(dyn),
+        "joints": int(joints),
+        "steps": args.chain_steps,
+        "value": dyn * args.chain_steps / wall,
+        "unit": "body-steps/s",
+        "ms_per_step": 1e3 * wall / args.chain_steps,
+        "kernel_ms_per_launch": kernel_ms / max(1, launches),
+        "touching_contacts_end": c1[4],
+        "backward": "not built: the adjoint covers disk worlds (no joints, no polygons)",
+    }
+    ref = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    if not args.no_cpu and os.path.exists(ref):
+        n = min(args.chain_cpu_worlds, args.chain_worlds)
+        r = subprocess.run([ref, "--time", "chain", str(warm), str(args.chain_steps), "0", str(threads), str(n),
+                            str(args.seed), "0"], check=True, capture_output=True, text=True).stdout
+        r = json.loads(r.strip().splitlines()[-1])
+        cpu = r["body_steps"] / r["seconds"]
+        out["cpu_baseline"] = {"value": cpu, "unit": "body-steps/s", "cores": threads, "kind": "reference",
+                               "sample": "reference float build, chain worlds 0..%d, steps %d..%d timed, %d "
+                                         "std::threads" % (n - 1, warm + 1, warm + args.chain_steps, threads),
+                               "seconds": r["seconds"]}
+        out["ratio_vs_cpu"] = out["value"] / cpu
+    return out
+
+
 def conf_record(pa):
     import struct
     c = pa.StepConf()
@@ -293,12 +352,14 @@ def main():
         result["large_world"] = large_world(args, pa, conf)
     if args.adjoint_worlds > 0 and args.adjoint_steps > 0:
         result["adjoint"] = adjoint_bench(args, pa, conf)
+    if args.chain_worlds > 0 and args.chain_steps > 0:
+        result["articulated"] = articulated(args, pa, conf, cpu_threads(args))
 
     # Plain per-world form: one world per launch, synchronised, on a sample.
     if args.plain_worlds <= 0:
         result["setup_s"] = build_s
         if not args.no_cpu:
-            threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+            threads = cpu_threads(args)
             cb = cpu_baseline(args, threads)
             result["cpu_baseline"] = cb
             result["ratio_vs_cpu"] = value / cb["value"]
@@ -323,7 +384,7 @@ def main():
     result["ratio_vs_plain_gpu"] = value / plain
 
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = cpu_threads(args)
         cb = cpu_baseline(args, threads)
         result["cpu_baseline"] = cb
         result["ratio_vs_cpu"] = value / cb["value"]

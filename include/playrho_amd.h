@@ -177,6 +177,14 @@ int prx_world_get_bodies(prx_world* world, float* out, uint32_t* flags);
 int prx_world_get_transformation(prx_world* world, uint32_t body, float out[4]);
 /* SetVelocity(World&, BodyID, Velocity) (WorldBody.hpp). */
 int prx_world_set_velocity(prx_world* world, uint32_t body, const float v[3]);
+/* SetFriction(World&, ShapeID, NonNegative<Real>) and SetRestitution(World&,
+ * ShapeID, Real) (WorldShape.hpp:114,124; WorldShape.cpp:79-96, through
+ * SetShape, AabbTreeWorld.cpp:1284-1357): bodies holding the shape are
+ * woken; its existing contacts are flagged for updating and keep the values
+ * mixed when they were created.  Negative or NaN friction, or NaN
+ * restitution: PRX_E_INVALID_ARGUMENT. */
+int prx_shape_set_friction(prx_world* world, uint32_t shape, float friction);
+int prx_shape_set_restitution(prx_world* world, uint32_t shape, float restitution);
 /* One state record (bodies, contact list in world order with manifolds,
  * proxies) in the layout of the golden vectors. */
 int prx_world_state_record(prx_world* world, uint8_t* buf, size_t cap, size_t* len);

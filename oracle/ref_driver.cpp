@@ -317,8 +317,8 @@ StepConf RegularStepConf(bool doToi) {
 }
 
 // --time <scene> <warmup> <steps> <doToi> <threads> <nworlds> [params...]:
-// builds nworlds copies of a scene (for "pile", param 1 is the first world
-// index and world i uses index first + i), steps each `warmup` times untimed,
+// builds nworlds copies of a scene (for "pile", "chain" and "billiards",
+// param 1 is the first world index and world i uses index first + i), steps each `warmup` times untimed,
 // then `steps` times timed, on `threads` std::threads (static partition
 // balanced by body count; PlayRho steps one world on one thread), and prints
 // the timing as one JSON line.
@@ -337,7 +337,7 @@ int TimeWorlds(int argc, char** argv) {
     double dyn = 0;
     for (int k = 0; k < nworlds; ++k) {
         std::vector<int> p = params;
-        if (name == "pile") {
+        if (name == "pile" || name == "chain" || name == "billiards") {
             if (p.size() < 2) p.resize(2, 0);
             p[1] += k;
         }

@@ -123,6 +123,8 @@ _SIGS = {
     "prx_world_get_bodies": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "prx_world_get_transformation": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32,
                                                     ctypes.POINTER(ctypes.c_float)]),
+    "prx_shape_set_friction": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float]),
+    "prx_shape_set_restitution": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float]),
     "prx_world_set_velocity": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32,
                                               ctypes.POINTER(ctypes.c_float)]),
     "prx_world_state_record": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
@@ -272,6 +274,14 @@ class World:
     def set_velocity(self, body, v):
         arr = (ctypes.c_float * 3)(*v)
         _check(lib().prx_world_set_velocity(self.h, body, arr))
+
+    def set_friction(self, shape, value):
+        """SetFriction(World&, ShapeID, NonNegative<Real>) (WorldShape.cpp:79-84)."""
+        _check(lib().prx_shape_set_friction(self.h, shape, value))
+
+    def set_restitution(self, shape, value):
+        """SetRestitution(World&, ShapeID, Real) (WorldShape.cpp:91-96)."""
+        _check(lib().prx_shape_set_restitution(self.h, shape, value))
 
     def state_record(self) -> bytes:
         n = ctypes.c_size_t()

@@ -1453,6 +1453,53 @@ int prx_world_set_velocity(prx_world* w, std::uint32_t body, const float v[3]) {
     return PRX_OK;
 }
 
+namespace {
+
+// SetShape (AabbTreeWorld.cpp:1284-1357) for a change of friction or
+// restitution only: every body holding the shape is woken (Body::SetAwake,
+// Body.cpp:163-171: speedable bodies only, under-active time reset), and, if
+// the value changed, the shape's contacts are flagged for updating with
+// their bodies woken.
+int SetShapeMaterial(prx_world* w, std::uint32_t shape, bool friction, float value) {
+    if (int st = CheckWorld(w)) return st;
+    if (shape >= w->shapes.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such shape");
+    if (value != value || (friction && value < 0.0f))
+        return Fail(PRX_E_INVALID_ARGUMENT, friction ? "friction must be non-negative" : "restitution is NaN");
+    if (int st = Detach(w)) return st;
+    HostShape& sh = w->shapes[shape];
+    float& field = friction ? sh.friction : sh.restitution;
+    auto wake = [&](int bi) {
+        HostBody& b = w->bodies[std::size_t(bi)];
+        if (b.flags & prx::kSpeedable) {
+            b.flags |= prx::kAwake;
+            b.under = 0;
+        }
+    };
+    for (std::size_t bi = 0; bi < w->bodies.size(); ++bi)
+        for (std::uint32_t s : w->bodies[bi].shapes)
+            if (s == shape) wake(int(bi));
+    if (field == value) return PRX_OK;
+    HostContacts& c = w->contacts;
+    for (std::size_t i = 0; i < c.size(); ++i) {
+        if (std::uint32_t(c.shape[i].x) != shape && std::uint32_t(c.shape[i].y) != shape) continue;
+        c.flags[i] |= prx::kNeedsUpdating;
+        wake(c.body[i].x);
+        wake(c.body[i].y);
+    }
+    field = value;
+    return PRX_OK;
+}
+
+}  // namespace
+
+int prx_shape_set_friction(prx_world* w, std::uint32_t shape, float friction) {
+    return SetShapeMaterial(w, shape, true, friction);
+}
+
+int prx_shape_set_restitution(prx_world* w, std::uint32_t shape, float restitution) {
+    return SetShapeMaterial(w, shape, false, restitution);
+}
+
 std::uint32_t prx_world_contact_count(prx_world* w) {
     if (!w || Refresh(w) != PRX_OK) return 0;
     return std::uint32_t(w->contacts.size());

@@ -96,3 +96,29 @@ def test_step_conf_defaults_match_reference():
     mine = np.array([getattr(c, f) for f, _ in pa.StepConf._fields_[:18]], dtype=np.float32)
     assert np.array_equal(mine.view("<u4"), ref["floats"].view("<u4")), (mine, ref["floats"])
     assert (c.reg_velocity_iters, c.reg_position_iters) == (ref["vel_iters"], ref["pos_iters"])
+
+
+def test_shape_material_setters():
+    """SetFriction / SetRestitution (WorldShape.cpp:79-96): the value lands in
+    the shape, a body holding the shape is woken with its under-active time
+    reset, negative friction is InvalidArgument, a bad id OutOfRange."""
+    w = pa.World.scene("billiards", [1, 0])
+    desc = w.save_desc()
+    shapes, bodies, _ = dumpfmt.read_description(dumpfmt.Reader(desc))
+    s = int(bodies[5]["shapes"][0])
+    w.set_friction(s, 0.125)
+    w.set_restitution(s, 0.75)
+    shapes2, bodies2, _ = dumpfmt.read_description(dumpfmt.Reader(w.save_desc()))
+    assert float(shapes2[s]["friction"]) == 0.125 and float(shapes2[s]["restitution"]) == 0.75
+    for i, (a, b) in enumerate(zip(shapes, shapes2)):
+        if i != s:
+            assert a["friction"] == b["friction"] and a["restitution"] == b["restitution"]
+    with pytest.raises(pa.PrxError) as e:
+        w.set_friction(s, -1.0)
+    assert e.value.code == 1
+    with pytest.raises(pa.PrxError) as e:
+        w.set_restitution(len(shapes) + 3, 0.5)
+    assert e.value.code == 4
+    # A world rebuilt with the same materials from the start has the same description.
+    assert pa.World.from_desc(pa.desc_with(desc, friction={s: 0.125}, restitution={s: 0.75})).save_desc() == \
+        w.save_desc()
