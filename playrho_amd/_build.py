@@ -3,6 +3,9 @@
 * playrho_amd/libplayrho_amd.so: the C ABI (host C++17) and the gfx950 HIP
   kernels, compiled with hipcc for gfx950 only.  Float semantics: no FMA
   contraction, IEEE denormals, correctly rounded division and square root.
+* playrho_amd/libplayrho_amd_torch.so: the libtorch operator
+  torch.ops.playrho_amd.billiards_loss (a torch::autograd::Function over the
+  adjoint's C ABI), compiled with g++ against the installed libtorch.
 * oracle/_build/liboracle.so: the CPU restatement used as the checker.
 * oracle/_ref/: the reference float build and golden driver (only where the
   reference checkout is present).
@@ -49,6 +52,28 @@ def build_library(out=LIB):
     return out
 
 
+TORCH_LIB = os.path.join(PKG, "libplayrho_amd_torch.so")
+
+
+def build_torch_op(out=TORCH_LIB):
+    import torch
+    import torch.utils.cpp_extension as ce
+    abi = "1" if torch._C._GLIBCXX_USE_CXX11_ABI else "0"
+    inc = []
+    for p in ce.include_paths():
+        inc += ["-isystem", p]
+    libs = []
+    for p in ce.library_paths():
+        libs += ["-L" + p, "-Wl,-rpath," + p]
+    tmp_out = out + ".tmp"
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-D_GLIBCXX_USE_CXX11_ABI=" + abi,
+          "-I" + os.path.join(ROOT, "include")] + inc +
+         [os.path.join(CSRC, "torch_adjoint.cpp"), "-o", tmp_out] + libs +
+         ["-ltorch", "-ltorch_cpu", "-lc10", "-L" + PKG, "-lplayrho_amd", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp_out, out)
+    return out
+
+
 def build_oracle(out=ORACLE_LIB):
     os.makedirs(os.path.dirname(out), exist_ok=True)
     _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
@@ -67,5 +92,6 @@ def build_reference():
 
 def build_all():
     build_library()
+    build_torch_op()
     build_oracle()
     build_reference()

@@ -1,53 +1,45 @@
-"""torch.autograd binding of the adjoint kernel (prx_adjoint_*).
+"""torch binding of the adjoint: the libtorch operator
+torch.ops.playrho_amd.billiards_loss, a C++ torch::autograd::Function over
+the library's adjoint (csrc/torch_adjoint.cpp, built into
+libplayrho_amd_torch.so next to this file).
 
-    loss = BilliardsLoss.apply(v0, friction, restitution, descs, targets, nsteps, every)
+    loss = billiards_loss(v0, friction, restitution, descs, targets, nsteps, every)
     loss.sum().backward()       # v0.grad, friction.grad, restitution.grad
 
-The forward pass steps the worlds (with the given initial velocities and
-materials) K steps on the device and returns the per-world loss
-sum |x_K - target|^2; the gradients come from the hand-written adjoint, which
-the forward call runs at the same time (recomputing from its checkpoints).
-The tensors are plumbing: the computation is the library's.
+The forward call steps the worlds (with the given initial velocities and
+materials) nsteps steps on the device and returns the per-world loss
+sum |x_K - target|^2 over dynamic bodies; the gradients come from the
+hand-written adjoint, which the same call runs (recomputing each segment of
+`every` steps from its checkpoint).
 
-Parameters per world w: v0[w] (bodies, 3) initial velocities (vx, vy, w) of
-every body (static bodies' entries are ignored), friction[w] and
-restitution[w] (shapes,) per shape.
+Parameters per world w: v0[w] (bodies, 3) initial velocities (vx, vy, w)
+of every body (static bodies' entries are ignored, as SetVelocity ignores
+them), friction[w] and restitution[w] (shapes,) per shape; descs: the
+worlds' descriptions (World.save_desc()), all with the same body and shape
+counts; targets: (worlds, bodies, 2).
 """
+import os
+
 import numpy as np
 import torch
 
-from . import Adjoint, Batch, StepConf, World, desc_with
+_LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libplayrho_amd_torch.so")
+_loaded = False
 
 
-def _descs_with(descs, v0, friction, restitution):
-    out = []
-    for w, d in enumerate(descs):
-        vel = {b: v0[w, b] for b in range(v0.shape[1])}
-        fr = {s: friction[w, s] for s in range(friction.shape[1])}
-        re = {s: restitution[w, s] for s in range(restitution.shape[1])}
-        out.append(desc_with(d, velocities=vel, friction=fr, restitution=re))
-    return out
+def _op():
+    global _loaded
+    if not _loaded:
+        if not os.path.exists(_LIB):
+            raise RuntimeError("libplayrho_amd_torch.so is not built: run __graft_entry__.build()")
+        torch.ops.load_library(_LIB)
+        _loaded = True
+    return torch.ops.playrho_amd.billiards_loss
 
 
-class BilliardsLoss(torch.autograd.Function):
-    @staticmethod
-    def forward(ctx, v0, friction, restitution, descs, targets, nsteps, every):
-        v = v0.detach().cpu().numpy().astype(np.float64)
-        f = friction.detach().cpu().numpy().astype(np.float64)
-        e = restitution.detach().cpu().numpy().astype(np.float64)
-        worlds = [World.from_desc(d) for d in _descs_with(descs, v, f, e)]
-        batch = Batch(worlds, use_graph=False)
-        adj = Adjoint(batch, int(nsteps), int(every), StepConf())
-        out = adj.run(np.asarray(targets, dtype=np.float32).reshape(-1, 2))
-        nw, nb, ns = v.shape[0], v.shape[1], f.shape[1]
-        ctx.save_for_backward(
-            torch.from_numpy(out["grad_velocity"].reshape(nw, nb, 3)).to(v0.device),
-            torch.from_numpy(out["grad_friction"].reshape(nw, ns)).to(v0.device),
-            torch.from_numpy(out["grad_restitution"].reshape(nw, ns)).to(v0.device))
-        return torch.from_numpy(out["loss"].astype(np.float32)).to(v0.device)
-
-    @staticmethod
-    def backward(ctx, gloss):
-        gv, gf, ge = ctx.saved_tensors
-        g = gloss.to(gv.dtype)
-        return g[:, None, None] * gv, g[:, None] * gf, g[:, None] * ge, None, None, None, None
+def billiards_loss(v0, friction, restitution, descs, targets, nsteps, every):
+    blob = b"".join(descs)
+    offsets = np.cumsum([0] + [len(d) for d in descs]).astype(np.int64)
+    t = torch.as_tensor(np.asarray(targets, dtype=np.float32)).reshape(v0.shape[0], v0.shape[1], 2)
+    return _op()(v0, friction, restitution, torch.frombuffer(bytearray(blob), dtype=torch.uint8),
+                 torch.from_numpy(offsets), t, int(nsteps), int(every))

@@ -1,0 +1,142 @@
+// libtorch binding of the adjoint (prx_adjoint_*): a
+// torch::autograd::Function registered as the operator
+// torch.ops.playrho_amd.billiards_loss.  The tensors carry parameters and
+// gradients in and out; the computation is the library's (forward steps with
+// checkpoints, recorded recomputation and the reversed steps, all on the
+// device).
+//
+//   billiards_loss(v0, friction, restitution, descs, desc_offsets, targets,
+//                  steps, every) -> loss
+//
+//   v0           (W, B, 3)  initial velocity (vx, vy, w) of every body of
+//                           world w (static bodies' entries are ignored, as
+//                           SetVelocity ignores them, WorldBody.cpp)
+//   friction     (W, S)     per shape (SetFriction, WorldShape.cpp:79-84)
+//   restitution  (W, S)     per shape (SetRestitution, WorldShape.cpp:91-96)
+//   descs        uint8      the W world descriptions, concatenated
+//   desc_offsets int64      (W + 1) byte offsets of the descriptions
+//   targets      (W, B, 2)  target positions
+//   loss         (W,)       sum over dynamic bodies of |x_K - target|^2
+//
+// Gradients flow to v0, friction and restitution; every world must have B
+// bodies and S shapes.
+#include <torch/library.h>
+#include <torch/torch.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "playrho_amd.h"
+
+namespace {
+
+using torch::Tensor;
+using torch::autograd::AutogradContext;
+using torch::autograd::tensor_list;
+
+void Check(int status, const char* what) {
+    TORCH_CHECK(status == PRX_OK, "playrho_amd: ", what, ": ", prx_last_error());
+}
+
+// Owns the worlds, the batch and the adjoint of one call.
+struct Run {
+    std::vector<prx_world*> worlds;
+    prx_batch* batch = nullptr;
+    prx_adjoint* adj = nullptr;
+    ~Run() {
+        if (adj) prx_adjoint_destroy(adj);
+        if (batch) prx_batch_destroy(batch);
+        for (prx_world* w : worlds) prx_world_destroy(w);
+    }
+};
+
+class BilliardsLoss : public torch::autograd::Function<BilliardsLoss> {
+public:
+    static Tensor forward(AutogradContext* ctx, const Tensor& v0, const Tensor& friction, const Tensor& restitution,
+                          const Tensor& descs, const Tensor& desc_offsets, const Tensor& targets, int64_t steps,
+                          int64_t every) {
+        TORCH_CHECK(v0.dim() == 3 && v0.size(2) == 3, "v0 must be (worlds, bodies, 3)");
+        const int64_t W = v0.size(0), B = v0.size(1);
+        TORCH_CHECK(friction.dim() == 2 && friction.size(0) == W, "friction must be (worlds, shapes)");
+        TORCH_CHECK(restitution.sizes() == friction.sizes(), "restitution must match friction's shape");
+        const int64_t S = friction.size(1);
+        TORCH_CHECK(targets.numel() == W * B * 2, "targets must hold (worlds, bodies, 2) values");
+        TORCH_CHECK(descs.scalar_type() == torch::kUInt8, "descs must be uint8");
+        TORCH_CHECK(desc_offsets.numel() == W + 1, "desc_offsets must hold worlds + 1 offsets");
+        TORCH_CHECK(steps > 0 && every > 0, "steps and every must be positive");
+
+        const auto cpu = torch::TensorOptions().dtype(torch::kFloat32).device(torch::kCPU);
+        const Tensor v = v0.detach().to(cpu).contiguous();
+        const Tensor f = friction.detach().to(cpu).contiguous();
+        const Tensor e = restitution.detach().to(cpu).contiguous();
+        const Tensor tg = targets.detach().to(cpu).contiguous();
+        const Tensor d = descs.detach().to(torch::kCPU).contiguous();
+        const Tensor off = desc_offsets.detach().to(torch::kCPU, torch::kInt64).contiguous();
+        const float* pv = v.data_ptr<float>();
+        const float* pf = f.data_ptr<float>();
+        const float* pe = e.data_ptr<float>();
+        const std::uint8_t* pd = d.data_ptr<std::uint8_t>();
+        const int64_t* po = off.data_ptr<int64_t>();
+
+        Run run;
+        for (int64_t w = 0; w < W; ++w) {
+            prx_world* h = nullptr;
+            Check(prx_world_create(nullptr, &h), "world create");
+            run.worlds.push_back(h);
+            TORCH_CHECK(po[w] >= 0 && po[w] <= po[w + 1] && po[w + 1] <= d.numel(), "bad description offsets");
+            Check(prx_world_load_desc(h, pd + po[w], std::size_t(po[w + 1] - po[w])), "load description");
+            TORCH_CHECK(int64_t(prx_world_body_count(h)) == B, "world ", w, " has ", prx_world_body_count(h),
+                        " bodies, v0 has ", B);
+            TORCH_CHECK(int64_t(prx_world_shape_count(h)) == S, "world ", w, " has ", prx_world_shape_count(h),
+                        " shapes, friction has ", S);
+            for (int64_t b = 0; b < B; ++b)
+                Check(prx_world_set_velocity(h, std::uint32_t(b), pv + (w * B + b) * 3), "set velocity");
+            for (int64_t s = 0; s < S; ++s) {
+                Check(prx_shape_set_friction(h, std::uint32_t(s), pf[w * S + s]), "set friction");
+                Check(prx_shape_set_restitution(h, std::uint32_t(s), pe[w * S + s]), "set restitution");
+            }
+        }
+        prx_batch_conf bc;
+        prx_batch_conf_default(&bc);
+        bc.use_graph = 0;
+        Check(prx_batch_create(run.worlds.data(), int(W), &bc, &run.batch), "batch create");
+        prx_step_conf sc;
+        prx_step_conf_default(&sc);
+        Check(prx_adjoint_create(run.batch, int(steps), int(every), &sc, &run.adj), "adjoint create");
+
+        Tensor loss = torch::empty({W}, cpu);
+        Tensor gv = torch::empty({W, B, 3}, cpu);
+        Tensor gf = torch::empty({W, S}, cpu);
+        Tensor ge = torch::empty({W, S}, cpu);
+        Check(prx_adjoint_run(run.adj, &sc, tg.data_ptr<float>(), loss.data_ptr<float>(), gv.data_ptr<float>(),
+                              gf.data_ptr<float>(), ge.data_ptr<float>()),
+              "adjoint run");
+        ctx->save_for_backward({gv.to(v0.options()), gf.to(friction.options()), ge.to(restitution.options())});
+        return loss.to(v0.options());
+    }
+
+    static tensor_list backward(AutogradContext* ctx, tensor_list grad_out) {
+        const auto saved = ctx->get_saved_variables();
+        const Tensor g = grad_out[0];
+        return {saved[0] * g.to(saved[0].options()).view({-1, 1, 1}),
+                saved[1] * g.to(saved[1].options()).view({-1, 1}),
+                saved[2] * g.to(saved[2].options()).view({-1, 1}),
+                Tensor(), Tensor(), Tensor(), Tensor(), Tensor()};
+    }
+};
+
+Tensor billiards_loss(const Tensor& v0, const Tensor& friction, const Tensor& restitution, const Tensor& descs,
+                      const Tensor& desc_offsets, const Tensor& targets, int64_t steps, int64_t every) {
+    return BilliardsLoss::apply(v0, friction, restitution, descs, desc_offsets, targets, steps, every);
+}
+
+}  // namespace
+
+TORCH_LIBRARY(playrho_amd, m) {
+    m.def("billiards_loss(Tensor v0, Tensor friction, Tensor restitution, Tensor descs, Tensor desc_offsets, "
+          "Tensor targets, int steps, int every) -> Tensor");
+}
+
+TORCH_LIBRARY_IMPL(playrho_amd, CompositeImplicitAutograd, m) {
+    m.impl("billiards_loss", billiards_loss);
+}

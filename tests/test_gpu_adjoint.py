@@ -146,24 +146,47 @@ def test_adjoint_batched_equals_single():
 
 
 def test_autograd_function_matches_adjoint():
-    """torch.autograd binding: gradients equal the adjoint's, and a gradient
-    step on the initial velocities lowers the loss."""
+    """The libtorch operator (a C++ torch::autograd::Function): gradients
+    equal the adjoint's on the same worlds, two worlds in one call, with the
+    parameters set through the operator's inputs; a gradient step on the
+    initial velocities lowers the loss."""
     import torch
-    from playrho_amd.autograd import BilliardsLoss
+    from playrho_amd.autograd import billiards_loss
     steps = 16
-    w, desc, bodies, dyn, targets = _setup(2, 0, steps)
-    sdesc, _, _ = dumpfmt.read_description(dumpfmt.Reader(desc))
-    v0 = torch.tensor(np.array([[b["vals"][8:11] for b in bodies]]), dtype=torch.float64, requires_grad=True)
-    fr = torch.tensor([[s["friction"] for s in sdesc]], dtype=torch.float64, requires_grad=True)
-    re = torch.tensor([[s["restitution"] for s in sdesc]], dtype=torch.float64, requires_grad=True)
-    loss = BilliardsLoss.apply(v0, fr, re, [desc], targets, steps, 8)
+    descs, tgs, v0s, frs, res = [], [], [], [], []
+    for wi in range(2):
+        w, desc, bodies, dyn, targets = _setup(2, wi, steps)
+        sdesc, _, _ = dumpfmt.read_description(dumpfmt.Reader(desc))
+        descs.append(desc)
+        tgs.append(targets)
+        v0s.append([b["vals"][8:11] for b in bodies])
+        frs.append([s["friction"] for s in sdesc])
+        res.append([s["restitution"] for s in sdesc])
+    # Perturbed materials and velocities enter through the operator only.
+    v0 = torch.tensor(np.array(v0s), dtype=torch.float64) * 0.9
+    fr = torch.tensor(np.array(frs), dtype=torch.float64) * 0.8
+    re = torch.tensor(np.array(res), dtype=torch.float64) * 0.7
+    v0.requires_grad_(True)
+    fr.requires_grad_(True)
+    re.requires_grad_(True)
+    loss = billiards_loss(v0, fr, re, descs, np.array(tgs), steps, 8)
     loss.sum().backward()
-    direct = pa.Adjoint(pa.Batch([pa.World.from_desc(desc)], use_graph=False), steps, 8).run(
-        targets.astype(np.float32))
-    assert np.allclose(v0.grad.numpy()[0], direct["grad_velocity"], rtol=0, atol=0)
-    assert np.allclose(re.grad.numpy()[0], direct["grad_restitution"], rtol=0, atol=0)
+    f32 = lambda x: x.detach().numpy().astype(np.float32)
+    worlds = []
+    for wi, desc in enumerate(descs):
+        nb, ns = v0.shape[1], fr.shape[1]
+        worlds.append(pa.World.from_desc(pa.desc_with(
+            desc, velocities={b: f32(v0[wi, b]) for b in range(nb)},
+            friction={s: float(f32(fr[wi, s])) for s in range(ns)},
+            restitution={s: float(f32(re[wi, s])) for s in range(ns)})))
+    direct = pa.Adjoint(pa.Batch(worlds, use_graph=False), steps, 8).run(
+        np.concatenate(tgs).astype(np.float32))
+    assert np.array_equal(loss.detach().numpy().astype(np.float32), direct["loss"])
+    assert np.array_equal(v0.grad.numpy().reshape(-1, 3).astype(np.float32), direct["grad_velocity"])
+    assert np.array_equal(fr.grad.numpy().reshape(-1).astype(np.float32), direct["grad_friction"])
+    assert np.array_equal(re.grad.numpy().reshape(-1).astype(np.float32), direct["grad_restitution"])
     with torch.no_grad():
         v1 = v0 - 1e-3 * v0.grad
-    loss1 = BilliardsLoss.apply(v1, fr, re, [desc], targets, steps, 8)
-    print("loss %.4f -> %.4f" % (float(loss), float(loss1)), flush=True)
-    assert float(loss1) < float(loss)
+    loss1 = billiards_loss(v1, fr.detach(), re.detach(), descs, np.array(tgs), steps, 8)
+    print("loss %.4f -> %.4f" % (float(loss.sum()), float(loss1.sum())), flush=True)
+    assert float(loss1.sum()) < float(loss.sum())
