@@ -277,7 +277,10 @@ def main():
     worlds = [pa.World.scene("pile", [args.seed, w]) for w in range(args.worlds)]
     build_s = time.time() - t0
     total_bodies = sum(w.body_count for w in worlds)
+    t_up = time.perf_counter()
     batch = pa.Batch(worlds, use_graph=True, coop_min=args.coop_min or None, lds_bytes=args.lds_bytes or None)
+    batch.sync()
+    upload_s = time.perf_counter() - t_up
     dyn = batch.dynamic_bodies()
 
     batch.step(conf, args.warmup)
@@ -297,7 +300,9 @@ def main():
     wall = time.perf_counter() - t1
     kernel_ms, launches = batch.last_timing()
 
+    t_down = time.perf_counter()
     batch.download()
+    download_s = time.perf_counter() - t_down
     c1 = batch.counts()
     contacts_avg = 0.5 * (c0[4] + c1[4])
     bytes_step = 0.5 * (algorithmic_bytes(*c0[:4]) + algorithmic_bytes(*c1[:4]))
@@ -328,6 +333,11 @@ def main():
             "schedule": schedule,
         },
         "contacts_per_s": contacts_avg * args.steps / wall,
+        "host_transfer": {"upload_s": upload_s, "download_s": download_s,
+                          "value_with_download_every_step": dyn / (wall / args.steps + download_s),
+                          "note": "upload = batch creation (allocation + host->device copy of every world); "
+                                  "download = the whole batch state device->host once; the last figure is "
+                                  "body-steps/s if a caller read the full state back after every step"},
         "state": {"awake_dynamic_bodies": [c0[0], c1[0]], "proxies": c1[1], "contacts": [c0[3], c1[3]],
                   "touching_contacts": [c0[4], c1[4]]},
         "roofline": {
