@@ -233,6 +233,7 @@ int prx_batch_autotune(prx_batch* batch, const prx_step_conf* step, int nsteps,
 typedef struct prx_big prx_big;
 int prx_big_create(prx_world* world, prx_big** out);
 void prx_big_destroy(prx_big* big);
+/* At most 16 velocity and 16 position iterations (PRX_E_UNSUPPORTED otherwise). */
 int prx_big_step(prx_big* big, const prx_step_conf* conf, int nsteps);
 int prx_big_sync(prx_big* big);
 int prx_big_last_timing(prx_big* big, float* ms, int* launches);

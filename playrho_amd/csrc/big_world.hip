@@ -32,6 +32,7 @@ using prx::BigState;
 
 constexpr int kBlock = prx::kBigBlock;  // threads of the single-workgroup kernels
 constexpr int kGrid = 256;              // threads per workgroup of the grid-wide kernels
+constexpr int kSweepBlocks = 64;        // workgroups of a colour sweep launch (16,384 threads)
 constexpr std::uint64_t kEmptyKey = ~0ull;
 // Bound on tree depth for the parent walks (a Karras tree over 30-bit codes
 // with index tie-breaks is at most 30 + 32 deep).
@@ -609,20 +610,13 @@ __device__ __forceinline__ bool moves(const DeviceState& d, int b) {
     return (d.bflags[b] & prx::kAccelerable) != 0;
 }
 
-// Graph colouring, constraint records and the coloured Gauss-Seidel sweeps,
-// in one workgroup.  A colour is a set of touching contacts in which no
-// dynamic body appears twice: round k lets every uncoloured contact claim its
-// dynamic bodies (lowest list position wins) and gives colour k to the
-// contacts that won all their claims.  Static and kinematic bodies are never
-// changed by a contact solve, so they do not constrain the colouring.
-__global__ __launch_bounds__(kBlock) void k_big_solve(DeviceState d, BigState g, StepParams c) {
+// Touching, enabled, non-sensor contacts in contact order (one workgroup),
+// and the colouring's initial state: no contact coloured, no colour used at
+// any body, a new claim epoch.
+__global__ __launch_bounds__(kBlock) void k_big_touch_list(DeviceState d, BigState g) {
     __shared__ int s_scan[kBlock / kWave + 1];
-    __shared__ float s_red[kBlock / kWave];
-    __shared__ int s_cnt[prx::kBigMaxColors + 2];
-    WorldHeader& H = d.hdr[0];
-    const int nc = H.ncontacts;
-    const int nb = H.nbodies;
-    // Touching, enabled, non-sensor contacts in contact order.
+    const int nc = d.hdr[0].ncontacts;
+    const int nb = d.hdr[0].nbodies;
     int m = 0;
     for (int base = 0; base < nc; base += kBlock) {
         const int i = base + int(threadIdx.x);
@@ -638,71 +632,107 @@ __global__ __launch_bounds__(kBlock) void k_big_solve(DeviceState d, BigState g,
     }
     for (int e = int(threadIdx.x); e < m; e += kBlock) g.color[e] = -1;
     for (int b = int(threadIdx.x); b < nb; b += kBlock) g.used[b] = 0ull;
-    __syncthreads();
-    // Rounds of speculative first-fit: every uncoloured contact proposes the
-    // lowest colour free at both its dynamic bodies; at each (body, colour)
-    // the lowest list position wins; winners take the colour.
-    int ncolors = 0;
-    for (int round = 0; round < prx::kBigColorRounds; ++round) {
-        for (int e = int(threadIdx.x); e < m; e += kBlock) {
-            if (g.color[e] >= 0) continue;
-            const int2 bd = d.c_body[g.list[e]];
-            std::uint64_t mask = 0ull;
-            if (moves(d, bd.x)) mask |= g.used[bd.x];
-            if (moves(d, bd.y)) mask |= g.used[bd.y];
-            const int col = (mask == ~0ull) ? -1 : (__ffsll((unsigned long long)(~mask)) - 1);
-            g.tent[e] = col;
-            if (col < 0) continue;
-            if (moves(d, bd.x)) atomicMin(&g.claim[bd.x * prx::kBigMaxColors + col], e);
-            if (moves(d, bd.y)) atomicMin(&g.claim[bd.y * prx::kBigMaxColors + col], e);
-        }
-        __syncthreads();
-        for (int e = int(threadIdx.x); e < m; e += kBlock) {
-            if (g.color[e] >= 0) continue;
-            const int col = g.tent[e];
-            if (col < 0) continue;
-            const int2 bd = d.c_body[g.list[e]];
-            const bool wa = !moves(d, bd.x) || g.claim[bd.x * prx::kBigMaxColors + col] == e;
-            const bool wb = !moves(d, bd.y) || g.claim[bd.y * prx::kBigMaxColors + col] == e;
-            g.win[e] = (wa && wb) ? 1 : 0;
-        }
-        __syncthreads();
-        int left = 0;
-        for (int e = int(threadIdx.x); e < m; e += kBlock) {
-            if (g.color[e] >= 0) continue;
-            const int col = g.tent[e];
-            if (col < 0) {
-                ++left;
-                continue;
-            }
-            const int2 bd = d.c_body[g.list[e]];
-            if (moves(d, bd.x)) g.claim[bd.x * prx::kBigMaxColors + col] = 0x7fffffff;
-            if (moves(d, bd.y)) g.claim[bd.y * prx::kBigMaxColors + col] = 0x7fffffff;
-            if (g.win[e]) {
-                g.color[e] = col;
-                const unsigned long long bit = 1ull << col;
-                if (moves(d, bd.x)) atomicOr(reinterpret_cast<unsigned long long*>(g.used + bd.x), bit);
-                if (moves(d, bd.y)) atomicOr(reinterpret_cast<unsigned long long*>(g.used + bd.y), bit);
-                if (col + 1 > ncolors) ncolors = col + 1;
-            } else {
-                ++left;
-            }
-        }
-        int tot = 0;
-        block_exclusive_scan(left, s_scan, tot);
-        if (tot == 0) break;
+    if (threadIdx.x == 0) {
+        g.counters[prx::kBigTouching] = m;
+        g.counters[prx::kBigColors] = 0;
+        g.counters[prx::kBigRounds] = 0;
+        g.counters[prx::kBigLeft + 1] = m;  // read by round 0 as "left after the previous round"
+        g.counters[prx::kBigEpoch] += 1;
     }
-    {
-        // Colours used: the largest over threads.
-        float f = float(ncolors);
-        f = block_reduce_max(f, s_red);
-        ncolors = int(f);
+}
+
+// Graph colouring by rounds of speculative first-fit (two grid-wide launches
+// per round).  A colour is a set of touching contacts in which no dynamic
+// body appears twice: every uncoloured contact proposes the lowest colour
+// free at both its dynamic bodies; at each (body, colour) the lowest list
+// position wins; contacts that won all their claims take the colour.  Static
+// and kinematic bodies are never changed by a contact solve, so they do not
+// constrain the colouring.  A round starts only if the previous one left
+// contacts uncoloured.
+__device__ __forceinline__ unsigned long long claim_key(const BigState& g, int round, int e) {
+    const unsigned epoch = unsigned(g.counters[prx::kBigEpoch]) * 64u + unsigned(round);
+    return (static_cast<unsigned long long>(~epoch) << 32) | static_cast<unsigned>(e);
+}
+
+__global__ __launch_bounds__(kGrid) void k_col_propose(DeviceState d, BigState g, int round) {
+    // The count this round's commit accumulates starts at zero, also when the
+    // colouring is already complete, so that "done" carries to later rounds.
+    const bool done = g.counters[prx::kBigLeft + ((round + 1) & 1)] == 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        g.counters[prx::kBigLeft + (round & 1)] = 0;
+        if (!done) g.counters[prx::kBigRounds] = round + 1;
     }
-    // Anything left goes to a last bucket solved by one thread in list order.
+    if (done) return;
+    const int m = g.counters[prx::kBigTouching];
+    for (int e = int(blockIdx.x * blockDim.x + threadIdx.x); e < m; e += int(gridDim.x * blockDim.x)) {
+        if (g.color[e] >= 0) continue;
+        const int2 bd = d.c_body[g.list[e]];
+        const bool ma = moves(d, bd.x), mb = moves(d, bd.y);
+        std::uint64_t mask = 0ull;
+        if (ma) mask |= g.used[bd.x];
+        if (mb) mask |= g.used[bd.y];
+        const unsigned long long freeCols = ~mask & ((1ull << prx::kBigMaxColors) - 1ull);
+        const int col = freeCols ? (__ffsll(freeCols) - 1) : -1;
+        g.tent[e] = col;
+        if (col < 0) continue;
+        const unsigned long long key = claim_key(g, round, e);
+        if (ma) atomicMin(&g.claim[bd.x * prx::kBigMaxColors + col], key);
+        if (mb) atomicMin(&g.claim[bd.y * prx::kBigMaxColors + col], key);
+    }
+}
+
+__global__ __launch_bounds__(kGrid) void k_col_commit(DeviceState d, BigState g, int round) {
+    if (g.counters[prx::kBigLeft + ((round + 1) & 1)] == 0) return;
+    const int m = g.counters[prx::kBigTouching];
+    int left = 0, ncol = 0;
+    for (int e = int(blockIdx.x * blockDim.x + threadIdx.x); e < m; e += int(gridDim.x * blockDim.x)) {
+        if (g.color[e] >= 0) continue;
+        const int col = g.tent[e];
+        if (col < 0) {
+            ++left;
+            continue;
+        }
+        const int2 bd = d.c_body[g.list[e]];
+        const bool ma = moves(d, bd.x), mb = moves(d, bd.y);
+        const unsigned long long key = claim_key(g, round, e);
+        const bool wa = !ma || g.claim[bd.x * prx::kBigMaxColors + col] == key;
+        const bool wb = !mb || g.claim[bd.y * prx::kBigMaxColors + col] == key;
+        if (wa && wb) {
+            g.color[e] = col;
+            const unsigned long long bit = 1ull << col;
+            if (ma) atomicOr(reinterpret_cast<unsigned long long*>(g.used + bd.x), bit);
+            if (mb) atomicOr(reinterpret_cast<unsigned long long*>(g.used + bd.y), bit);
+            ncol = (col + 1 > ncol) ? col + 1 : ncol;
+        } else {
+            ++left;
+        }
+    }
+    // One atomic per wavefront.
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        left += __shfl_xor(left, off);
+        const int o = __shfl_xor(ncol, off);
+        ncol = (o > ncol) ? o : ncol;
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        if (left) atomicAdd(&g.counters[prx::kBigLeft + (round & 1)], left);
+        if (ncol) atomicMax(&g.counters[prx::kBigColors], ncol);
+    }
+}
+
+
+// Colour counts and the stable colour-bucket order of the touching list (one
+// workgroup), after the colouring; contacts left uncoloured go to a last
+// bucket solved by one thread in list order.  Also clears the per-iteration
+// early-exit slots of the sweeps.
+__global__ __launch_bounds__(kBlock) void k_big_buckets(DeviceState d, BigState g) {
+    __shared__ int s_cnt[prx::kBigMaxColors + 2];
+    const int m = g.counters[prx::kBigTouching];
     const int serial = prx::kBigMaxColors;
     for (int e = int(threadIdx.x); e < m; e += kBlock)
         if (g.color[e] < 0) g.color[e] = serial;
     for (int k = int(threadIdx.x); k < prx::kBigMaxColors + 2; k += kBlock) s_cnt[k] = 0;
+    for (int k = int(threadIdx.x); k < prx::kBigSweepSlots; k += kBlock)
+        g.sweep[k] = (k < prx::kBigPosSlot) ? 0 : 0x7f800000;  // 0.0f; key of +inf
     __syncthreads();
     for (int e = int(threadIdx.x); e < m; e += kBlock) atomicAdd(&s_cnt[g.color[e]], 1);
     __syncthreads();
@@ -714,8 +744,6 @@ __global__ __launch_bounds__(kBlock) void k_big_solve(DeviceState d, BigState g,
             run += cnt;
         }
         g.colorStart[prx::kBigMaxColors + 1] = run;
-        g.counters[prx::kBigColors] = ncolors;
-        g.counters[prx::kBigTouching] = m;
     }
     __syncthreads();
     // Stable bucket order by colour (list order kept within a colour): the
@@ -763,10 +791,14 @@ __global__ __launch_bounds__(kBlock) void k_big_solve(DeviceState d, BigState g,
             __syncthreads();
         }
     }
-    // Constraint records in colour order (GetPositionConstraints /
-    // GetVelocityConstraints, AabbTreeWorld.cpp:289-357).
-    const Bodies<false> bs{d.s_bc_pos, d.s_bc_vel, nb};
-    for (int e = int(threadIdx.x); e < m; e += kBlock) {
+}
+
+// Constraint records in colour order (GetPositionConstraints /
+// GetVelocityConstraints, AabbTreeWorld.cpp:289-357), one thread per contact.
+__global__ __launch_bounds__(kGrid) void k_big_records(DeviceState d, BigState g, StepParams c) {
+    const int m = g.counters[prx::kBigTouching];
+    const Bodies<false> bs{d.s_bc_pos, d.s_bc_vel, d.hdr[0].nbodies};
+    for (int e = int(blockIdx.x * blockDim.x + threadIdx.x); e < m; e += int(gridDim.x * blockDim.x)) {
         const int gi = g.list2[e];
         const int2 bd = d.c_body[gi];
         const int2 ss = d.c_shape[gi];
@@ -782,56 +814,83 @@ __global__ __launch_bounds__(kBlock) void k_big_solve(DeviceState d, BigState g,
         store_vc(d.s_vc + std::size_t(e) * prx::kVcStride, vc, A, B);
         store_pc(d.s_pc + std::size_t(e) * prx::kPcStride, mf, bd.x, bd.y, rA + rB, A, B);
     }
-    __syncthreads();
-    const int* cs = g.colorStart;
-    const int ncol = prx::kBigMaxColors + 1;
-    // Warm start (WarmStartVelocities, AabbTreeWorld.cpp:264-274), colour by colour.
-    if (c.warmStart) {
-        for (int k = 0; k < ncol; ++k) {
-            const int e0 = cs[k], e1 = cs[k + 1];
-            if (k == serial) {
-                if (threadIdx.x == 0) {
-                    for (int e = e0; e < e1; ++e) {
-                        BodyC A, B;
-                        const Vc vc = load_vc(d.s_vc + std::size_t(e) * prx::kVcStride, A, B);
-                        warm_start_one(vc, A, B, bs);
-                    }
-                }
-            } else {
-                for (int e = e0 + int(threadIdx.x); e < e1; e += kBlock) {
-                    BodyC A, B;
-                    const Vc vc = load_vc(d.s_vc + std::size_t(e) * prx::kVcStride, A, B);
-                    warm_start_one(vc, A, B, bs);
-                }
-            }
-            __threadfence_block();
-            __syncthreads();
+}
+
+// Contacts [e0, e1) of colour k for this launch: the whole grid strides over a
+// colour (no dynamic body appears twice in it); the last bucket is walked in
+// order by one thread.
+struct ColourRange {
+    int e, e1, step;
+    __device__ __forceinline__ ColourRange(const BigState& g, int k) {
+        const int a = g.colorStart[k], b = g.colorStart[k + 1];
+        if (k == prx::kBigMaxColors) {
+            const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+            e = a;
+            e1 = first ? b : a;
+            step = 1;
+        } else {
+            e = a + int(blockIdx.x * blockDim.x + threadIdx.x);
+            e1 = b;
+            step = int(gridDim.x * blockDim.x);
         }
     }
-    // Velocity iterations (SolveVelocityConstraintsViaGS per colour).
-    for (std::uint32_t it = 0; it < c.velIters; ++it) {
-        float inc = 0.0f;
-        for (int k = 0; k < ncol; ++k) {
-            const int e0 = cs[k], e1 = cs[k + 1];
-            const bool ser = (k == serial);
-            for (int e = e0 + (ser ? 0 : int(threadIdx.x)); e < e1; e += (ser ? 1 : kBlock)) {
-                if (ser && threadIdx.x != 0) break;
-                float4* rec = d.s_vc + std::size_t(e) * prx::kVcStride;
-                BodyC A, B;
-                Vc vc = load_vc(rec, A, B);
-                const float mi = solve_velocity_one(vc, A, B, bs);
-                rec[4] = float4{vc.p[0].ni, vc.p[0].ti, vc.p[0].nm, vc.p[0].tm};
-                rec[7] = float4{vc.p[1].ni, vc.p[1].ti, vc.p[1].nm, vc.p[1].tm};
-                inc = fmax_std(inc, mi);
-            }
-            __threadfence_block();
-            __syncthreads();
-        }
-        inc = block_reduce_max(inc, s_red);
-        if (inc <= c.regMinMomentum) break;
+};
+
+// Velocity iteration `it` stops the sweeps if an earlier iteration's largest
+// impulse change was at most regMinMomentum (the early exit of
+// SolveRegIslandViaGS, AabbTreeWorld.cpp:1610-1616).
+__device__ __forceinline__ bool velocity_done(const BigState& g, const StepParams& c, int it) {
+    for (int j = 0; j < it; ++j)
+        if (__int_as_float(g.sweep[prx::kBigVelSlot + j]) <= c.regMinMomentum) return true;
+    return false;
+}
+
+// Position iteration `it` stops if an earlier one reached regMinSeparation
+// (AabbTreeWorld.cpp:1627-1642).  Slots hold order-preserving keys of floats.
+__device__ __forceinline__ int sep_key(float f) {
+    const int bits = __float_as_int(f);
+    return bits >= 0 ? bits : (bits ^ 0x7fffffff);
+}
+__device__ __forceinline__ float sep_of_key(int key) { return __int_as_float(key >= 0 ? key : (key ^ 0x7fffffff)); }
+__device__ __forceinline__ bool position_done(const BigState& g, const StepParams& c, int it) {
+    for (int j = 0; j < it; ++j)
+        if (sep_of_key(g.sweep[prx::kBigPosSlot + j]) >= c.regMinSeparation) return true;
+    return false;
+}
+
+// Warm start of colour k (WarmStartVelocities, AabbTreeWorld.cpp:264-274).
+__global__ __launch_bounds__(kGrid) void k_big_warm(DeviceState d, BigState g, int k) {
+    const Bodies<false> bs{d.s_bc_pos, d.s_bc_vel, d.hdr[0].nbodies};
+    for (ColourRange r(g, k); r.e < r.e1; r.e += r.step) {
+        BodyC A, B;
+        const Vc vc = load_vc(d.s_vc + std::size_t(r.e) * prx::kVcStride, A, B);
+        warm_start_one(vc, A, B, bs);
     }
-    // IntegratePositions (AabbTreeWorld.cpp:176-188).
-    for (int b = int(threadIdx.x); b < nb; b += kBlock) {
+}
+
+// Velocity iteration `it` over colour k (SolveVelocityConstraintsViaGS).
+__global__ __launch_bounds__(kGrid) void k_big_velocity(DeviceState d, BigState g, StepParams c, int it, int k) {
+    if (velocity_done(g, c, it)) return;
+    const Bodies<false> bs{d.s_bc_pos, d.s_bc_vel, d.hdr[0].nbodies};
+    float inc = 0.0f;
+    for (ColourRange r(g, k); r.e < r.e1; r.e += r.step) {
+        float4* rec = d.s_vc + std::size_t(r.e) * prx::kVcStride;
+        BodyC A, B;
+        Vc vc = load_vc(rec, A, B);
+        const float mi = solve_velocity_one(vc, A, B, bs);
+        rec[4] = float4{vc.p[0].ni, vc.p[0].ti, vc.p[0].nm, vc.p[0].tm};
+        rec[7] = float4{vc.p[1].ni, vc.p[1].ti, vc.p[1].nm, vc.p[1].tm};
+        inc = fmax_std(inc, mi);
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) inc = fmax_std(inc, __shfl_xor(inc, off));
+    if ((threadIdx.x & (kWave - 1)) == 0 && inc > 0.0f) atomicMax(&g.sweep[prx::kBigVelSlot + it], __float_as_int(inc));
+}
+
+// IntegratePositions (AabbTreeWorld.cpp:176-188).
+__global__ __launch_bounds__(kGrid) void k_big_integrate(DeviceState d, StepParams c) {
+    const int nb = d.hdr[0].nbodies;
+    const Bodies<false> bs{d.s_bc_pos, d.s_bc_vel, nb};
+    for (int b = int(blockIdx.x * blockDim.x + threadIdx.x); b < nb; b += int(gridDim.x * blockDim.x)) {
         const std::uint32_t fb = d.bflags[b];
         if (!((fb & prx::kSpeedable) && (fb & prx::kEnabled))) continue;
         float4 p = bs.pos(b);
@@ -841,35 +900,43 @@ __global__ __launch_bounds__(kBlock) void k_big_solve(DeviceState d, BigState g,
         p.z = p.z + c.dt * v.z;
         bs.set_pos(b, p);
     }
-    __threadfence_block();
-    __syncthreads();
-    // Position iterations (SolvePositionConstraintsViaGS per colour).
-    float minSepAll = __builtin_inff();
-    for (std::uint32_t it = 0; it < c.posIters; ++it) {
-        float ms = __builtin_inff();
-        for (int k = 0; k < ncol; ++k) {
-            const int e0 = cs[k], e1 = cs[k + 1];
-            const bool ser = (k == serial);
-            for (int e = e0 + (ser ? 0 : int(threadIdx.x)); e < e1; e += (ser ? 1 : kBlock)) {
-                if (ser && threadIdx.x != 0) break;
-                const Pc pc = load_pc(d.s_pc + std::size_t(e) * prx::kPcStride);
-                float4 pa = bs.pos(pc.bA);
-                float4 pb = bs.pos(pc.bB);
-                const float sep = solve_pc(pc.m, pc.totalRadius, pc.A, pc.B, pa, pb, c);
-                bs.set_pos(pc.bA, pa);
-                bs.set_pos(pc.bB, pb);
-                ms = fmin_std(ms, sep);
-            }
-            __threadfence_block();
-            __syncthreads();
-        }
-        ms = block_reduce_min(ms, s_red);
-        minSepAll = ms;
-        if (ms >= c.regMinSeparation) break;
+}
+
+// Position iteration `it` over colour k (SolvePositionConstraintsViaGS).
+__global__ __launch_bounds__(kGrid) void k_big_position(DeviceState d, BigState g, StepParams c, int it, int k) {
+    if (position_done(g, c, it)) return;
+    const Bodies<false> bs{d.s_bc_pos, d.s_bc_vel, d.hdr[0].nbodies};
+    float ms = __builtin_inff();
+    for (ColourRange r(g, k); r.e < r.e1; r.e += r.step) {
+        const Pc pc = load_pc(d.s_pc + std::size_t(r.e) * prx::kPcStride);
+        float4 pa = bs.pos(pc.bA);
+        float4 pb = bs.pos(pc.bB);
+        const float sep = solve_pc(pc.m, pc.totalRadius, pc.A, pc.B, pa, pb, c);
+        bs.set_pos(pc.bA, pa);
+        bs.set_pos(pc.bB, pb);
+        ms = fmin_std(ms, sep);
     }
-    if (threadIdx.x == 0) g.metrics[prx::kBigMinSeparation] = minSepAll;
-    // AssignImpulses (AabbTreeWorld.cpp:208-227).
-    for (int e = int(threadIdx.x); e < m; e += kBlock) {
+    for (int off = kWave / 2; off > 0; off >>= 1) ms = fmin_std(ms, __shfl_xor(ms, off));
+    if ((threadIdx.x & (kWave - 1)) == 0 && ms < __builtin_inff())
+        atomicMin(&g.sweep[prx::kBigPosSlot + it], sep_key(ms));
+}
+
+// AssignImpulses (AabbTreeWorld.cpp:208-227) and the body writeback
+// (AabbTreeWorld.cpp:1652-1664); records the smallest separation of the last
+// position iteration run.
+__global__ __launch_bounds__(kGrid) void k_big_finish(DeviceState d, BigState g, StepParams c) {
+    const int m = g.counters[prx::kBigTouching];
+    const int nb = d.hdr[0].nbodies;
+    const int tid = int(blockIdx.x * blockDim.x + threadIdx.x), stride = int(gridDim.x * blockDim.x);
+    if (tid == 0) {
+        float last = __builtin_inff();
+        for (int j = 0; j < int(c.posIters); ++j) {
+            if (position_done(g, c, j)) break;
+            last = sep_of_key(g.sweep[prx::kBigPosSlot + j]);
+        }
+        g.metrics[prx::kBigMinSeparation] = last;
+    }
+    for (int e = tid; e < m; e += stride) {
         const int gi = g.list2[e];
         const float4* rec = d.s_vc + std::size_t(e) * prx::kVcStride;
         const std::uint32_t n = __float_as_uint(rec[1].w);
@@ -884,8 +951,8 @@ __global__ __launch_bounds__(kBlock) void k_big_solve(DeviceState d, BigState g,
             d.m_p1[gi] = m1;
         }
     }
-    // Body writeback (AabbTreeWorld.cpp:1652-1664).
-    for (int b = int(threadIdx.x); b < nb; b += kBlock) {
+    const Bodies<false> bs{d.s_bc_pos, d.s_bc_vel, nb};
+    for (int b = tid; b < nb; b += stride) {
         const std::uint32_t f = d.bflags[b];
         if (!((f & prx::kSpeedable) && (f & prx::kEnabled))) continue;
         const float4 bv = bs.vel(b);
@@ -1001,7 +1068,25 @@ extern "C" hipError_t prx_big_launch_step(const prx::DeviceState* dp, const prx:
     FindAdd(d, g, *c, 2, s);
     if (c->dt != 0.0f) {
         hipLaunchKernelGGL(k_big_bodies, dim3(Blocks(B, kGrid)), dim3(kGrid), 0, s, d, *c);
-        hipLaunchKernelGGL(k_big_solve, dim3(1), dim3(kBlock), 0, s, d, g, *c);
+        hipLaunchKernelGGL(k_big_touch_list, dim3(1), dim3(kBlock), 0, s, d, g);
+        const unsigned colBlocks = Blocks(cap, kGrid) < 1024u ? Blocks(cap, kGrid) : 1024u;
+        for (int r = 0; r < prx::kBigColorRounds; ++r) {
+            hipLaunchKernelGGL(k_col_propose, dim3(colBlocks), dim3(kGrid), 0, s, d, g, r);
+            hipLaunchKernelGGL(k_col_commit, dim3(colBlocks), dim3(kGrid), 0, s, d, g, r);
+        }
+        hipLaunchKernelGGL(k_big_buckets, dim3(1), dim3(kBlock), 0, s, d, g);
+        hipLaunchKernelGGL(k_big_records, dim3(colBlocks), dim3(kGrid), 0, s, d, g, *c);
+        const int ncol = prx::kBigMaxColors + 1;
+        if (c->warmStart)
+            for (int k = 0; k < ncol; ++k) hipLaunchKernelGGL(k_big_warm, dim3(kSweepBlocks), dim3(kGrid), 0, s, d, g, k);
+        for (int it = 0; it < int(c->velIters); ++it)
+            for (int k = 0; k < ncol; ++k)
+                hipLaunchKernelGGL(k_big_velocity, dim3(kSweepBlocks), dim3(kGrid), 0, s, d, g, *c, it, k);
+        hipLaunchKernelGGL(k_big_integrate, dim3(Blocks(B, kGrid)), dim3(kGrid), 0, s, d, *c);
+        for (int it = 0; it < int(c->posIters); ++it)
+            for (int k = 0; k < ncol; ++k)
+                hipLaunchKernelGGL(k_big_position, dim3(kSweepBlocks), dim3(kGrid), 0, s, d, g, *c, it, k);
+        hipLaunchKernelGGL(k_big_finish, dim3(colBlocks), dim3(kGrid), 0, s, d, g, *c);
         hipLaunchKernelGGL(k_big_sync, dim3(Blocks(B, kGrid)), dim3(kGrid), 0, s, d, *c);
         hipLaunchKernelGGL(k_big_flag_contacts, dim3(Blocks(cap, kGrid)), dim3(kGrid), 0, s, d);
         hipLaunchKernelGGL(k_big_clear_bodyflag, dim3(Blocks(B, kGrid)), dim3(kGrid), 0, s, d);

@@ -10,11 +10,21 @@
 namespace prx {
 
 constexpr int kBigBlock = 1024;     // threads of the single-workgroup kernels
-constexpr int kBigMaxColors = 64;   // colours (one bit each in a body's mask) before the serial bucket
-constexpr int kBigColorRounds = 48; // colouring rounds before the rest goes to the serial bucket
+constexpr int kBigMaxColors = 16;   // colours (one bit each in a body's mask) before the serial bucket
+// Early-exit slots of the sweeps (BigState::sweep): the largest impulse change
+// of each velocity iteration (float bits) and the smallest separation of each
+// position iteration (order-preserving keys).
+constexpr int kBigVelSlot = 0, kBigPosSlot = 16, kBigSweepSlots = 32;
+constexpr int kBigColorRounds = 40; // colouring rounds before the rest goes to the serial bucket
 
 // Indices into BigState::counters.
-enum : int { kBigPairs = 0, kBigColors = 1, kBigTouching = 2, kBigMinSepBits = 3, kBigCounters = 8 };
+// kBigEpoch counts colourings (claim keys are stamped with it); kBigLeft and
+// kBigLeft + 1 hold the uncoloured count of the last two rounds; kBigRounds
+// the rounds the last colouring took.
+enum : int {
+    kBigPairs = 0, kBigColors = 1, kBigTouching = 2, kBigMinSepBits = 3, kBigEpoch = 4, kBigLeft = 5,
+    kBigRounds = 7, kBigCounters = 8
+};
 // Indices into BigState::metrics.
 enum : int { kBigMinSeparation = 0, kBigMetrics = 4 };
 
@@ -41,11 +51,15 @@ struct BigState {
     int* list;
     int* list2;
     int* color;
-    int* claim;       // B * kBigMaxColors: lowest contact proposing (body, colour); kept at INT_MAX between rounds
+    // B * kBigMaxColors: the smallest key proposing (body, colour) in a round;
+    // key = (~epoch << 32) | list position, so a later round's keys are
+    // smaller than every stale one and no reset is needed.
+    unsigned long long* claim;
     std::uint64_t* used;  // B: colours taken at the body
     int* tent;        // cap: proposed colour
     int* win;         // cap
     int* colorStart;  // kBigMaxColors + 2
+    int* sweep;       // kBigSweepSlots
     int* counters;    // kBigCounters
     float* metrics;   // kBigMetrics
 };

@@ -1781,11 +1781,12 @@ int AllocBig(DeviceArena& a, prx::BigState& g, int P, int B, int cap) {
     PRX_HIP(a.alloc(g.list2, std::size_t(g.cap)));
     PRX_HIP(a.alloc(g.color, std::size_t(g.cap)));
     PRX_HIP(a.alloc(g.claim, std::size_t(g.B) * prx::kBigMaxColors));
-    PRX_HIP(hipMemset(g.claim, 0x7f, sizeof(int) * std::size_t(g.B) * prx::kBigMaxColors));
+    PRX_HIP(hipMemset(g.claim, 0xff, sizeof(unsigned long long) * std::size_t(g.B) * prx::kBigMaxColors));
     PRX_HIP(a.alloc(g.used, std::size_t(g.B)));
     PRX_HIP(a.alloc(g.tent, std::size_t(g.cap)));
     PRX_HIP(a.alloc(g.win, std::size_t(g.cap)));
     PRX_HIP(a.alloc(g.colorStart, std::size_t(prx::kBigMaxColors + 2)));
+    PRX_HIP(a.alloc(g.sweep, std::size_t(prx::kBigSweepSlots)));
     PRX_HIP(a.alloc(g.counters, std::size_t(prx::kBigCounters)));
     PRX_HIP(a.alloc(g.metrics, std::size_t(prx::kBigMetrics)));
     PRX_HIP(hipMemset(g.counters, 0, sizeof(int) * prx::kBigCounters));
@@ -1848,6 +1849,9 @@ int prx_big_step(prx_big* big, const prx_step_conf* conf, int nsteps) {
     prx_batch* b = big->batch;
     if (b->worlds[0] != big->world || big->world->owner != b)
         return Fail(PRX_E_WRONG_STATE, "the world left its large-world state");
+    if (conf->reg_velocity_iters > std::uint32_t(prx::kBigPosSlot - prx::kBigVelSlot) ||
+        conf->reg_position_iters > std::uint32_t(prx::kBigSweepSlots - prx::kBigPosSlot))
+        return Fail(PRX_E_UNSUPPORTED, "the large-world mode runs at most 16 velocity and 16 position iterations");
     prx::StepParams p = ToParams(*conf);
     p.ldsBytes = 0;
     p.coopMin = 0;
