@@ -53,6 +53,8 @@ def parse():
     p.add_argument("--chain-worlds", type=int, default=8192, help="articulated chain worlds (0: skip)")
     p.add_argument("--chain-steps", type=int, default=240)
     p.add_argument("--chain-cpu-worlds", type=int, default=1024, help="worlds in the CPU sample")
+    p.add_argument("--chain-adjoint-worlds", type=int, default=1024,
+                   help="chain worlds differentiated with respect to joint motor torques (0: skip)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only run a plain device copy of MIB MiB five times (counter calibration)")
     return p.parse_args()
@@ -233,8 +235,31 @@ def articulated(args, pa, conf, threads):
         "ms_per_step": 1e3 * wall / args.chain_steps,
         "kernel_ms_per_launch": kernel_ms / max(1, launches),
         "touching_contacts_end": c1[4],
-        "backward": "not built: the adjoint covers disk worlds (no joints, no polygons)",
     }
+    if args.chain_adjoint_worlds > 0:
+        import numpy as np
+        n = min(args.chain_adjoint_worlds, args.chain_worlds)
+        aw = [pa.World.scene("chain", [args.seed, w]) for w in range(n)]
+        adj = pa.Adjoint(pa.Batch(aw, use_graph=False), args.chain_steps, args.adjoint_every, conf)
+        rng = np.random.default_rng(args.seed)
+        nb = sum(w.body_count for w in aw)
+        targets = np.stack([rng.uniform(-5, 40, nb), rng.uniform(5, 40, nb)], axis=1).astype(np.float32)
+        adj.run(targets)  # warm-up (allocations, code paths)
+        t0 = time.perf_counter()
+        res = adj.run(targets)
+        wall_a = time.perf_counter() - t0
+        out["adjoint"] = {
+            "workload": "%d of the chain worlds, %d steps from their initial state, checkpoint every %d, loss sum "
+                        "|x_T - target|^2, d loss / d (joint maximum motor torques, initial velocities, friction, "
+                        "restitution)" % (n, args.chain_steps, args.adjoint_every),
+            "value": n * args.chain_steps / wall_a,
+            "unit": "adjoint world-steps/s",
+            "seconds": wall_a,
+            "device_ms": adj.timing_ms(),
+            "note": "one run = the forward pass with checkpoints, the recomputation of every segment with the "
+                    "solve recorded, and the reversed steps (general reverse kernel)",
+            "grad_motor_torque_norm": float(np.linalg.norm(res["grad_motor_torque"])),
+        }
     ref = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
     if not args.no_cpu and os.path.exists(ref):
         n = min(args.chain_cpu_worlds, args.chain_worlds)

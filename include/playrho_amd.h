@@ -249,14 +249,19 @@ int prx_big_metrics(prx_big* big, float out[4]);
 int prx_lbvh_pairs(const float* aabbs, const int32_t* bodies, const uint8_t* moved, int n, int32_t* pairs,
                    int cap, int* count);
 
-/* Adjoint of K regular steps of a batch, for worlds of disks against each
- * other and against static edges/polygons (one-point manifolds; no joints).
- * No reference counterpart: it differentiates Step (AabbTreeWorld.cpp:
- * 1563-1682) with each step's active set (touching contacts, islands, clamp
- * branches, early exits) held as the forward pass took it.  Loss per world:
- * sum over dynamic bodies of |final position - target|^2.  The forward pass
- * keeps a checkpoint every `every` steps; the backward pass recomputes each
- * segment from its checkpoint, recording the solves, and reverses them. */
+/* Adjoint of K regular steps of a batch.  No reference counterpart: it
+ * differentiates Step (AabbTreeWorld.cpp:1563-1682; contact solves
+ * ContactSolver.cpp, revolute joints RevoluteJointConf.cpp:127-433) with each
+ * step's active set (touching contacts, islands, clamp branches, block-solver
+ * cases, early exits) held as the forward pass took it.  Worlds of disks
+ * centred on their bodies without joints use a hand-derived reverse kernel;
+ * any other world (polygons, edges, two-point manifolds, revolute joints)
+ * the general one, which takes each update's vector-Jacobian product from
+ * dual-number copies of the solver functions.  Loss per world: sum over
+ * dynamic bodies of |final position - target|^2.  The forward pass keeps a
+ * checkpoint every `every` steps; the backward pass recomputes each segment
+ * from its checkpoint, recording the solves, and reverses them.  Sleeping
+ * is not differentiated (a body put to sleep has its velocity zeroed). */
 typedef struct prx_adjoint prx_adjoint;
 int prx_adjoint_create(prx_batch* batch, int nsteps, int every, const prx_step_conf* conf, prx_adjoint** out);
 void prx_adjoint_destroy(prx_adjoint* adj);
@@ -266,6 +271,13 @@ void prx_adjoint_destroy(prx_adjoint* adj);
  * restitution per batch shape.  The batch is left at its starting state. */
 int prx_adjoint_run(prx_adjoint* adj, const prx_step_conf* conf, const float* targets, float* loss,
                     float* grad_velocity, float* grad_friction, float* grad_restitution);
+/* Diagnostic: on != 0 runs a disk-only batch through the general reverse
+ * kernel (to check the two against each other); on = 0 returns to the
+ * hand-derived one where the batch allows it (else PRX_E_UNSUPPORTED). */
+int prx_adjoint_use_general(prx_adjoint* adj, int on);
+/* d loss / d maximum motor torque (RevoluteJointConf::maxMotorTorque) of the
+ * last run, one float per batch joint (world by world, creation order). */
+int prx_adjoint_joint_gradients(prx_adjoint* adj, float* grad_max_motor_torque);
 /* Counters of the last run: constraint slots reversed, those with a
  * restitution bias, position-solver updates and velocity updates reversed. */
 int prx_adjoint_diagnostics(prx_adjoint* adj, float out[4]);

@@ -149,6 +149,8 @@ _SIGS = {
     "prx_adjoint_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "prx_adjoint_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    "prx_adjoint_use_general": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "prx_adjoint_joint_gradients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "prx_adjoint_diagnostics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "prx_big_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     "prx_big_destroy": (None, [ctypes.c_void_p]),
@@ -445,6 +447,7 @@ class Adjoint:
                                         ctypes.byref(self.h)))
         self.nbodies = sum(w.body_count for w in batch.worlds)
         self.nshapes = sum(lib().prx_world_shape_count(w.h) for w in batch.worlds)
+        self.njoints = sum(w.joint_count for w in batch.worlds)
 
     def __del__(self):
         h = getattr(self, "h", None)
@@ -460,7 +463,16 @@ class Adjoint:
         ge = np.zeros(self.nshapes, dtype=np.float32)
         _check(lib().prx_adjoint_run(self.h, ctypes.byref(self.conf), t.ctypes.data, loss.ctypes.data,
                                      gv.ctypes.data, gf.ctypes.data, ge.ctypes.data))
-        return dict(loss=loss, grad_velocity=gv, grad_friction=gf, grad_restitution=ge)
+        out = dict(loss=loss, grad_velocity=gv, grad_friction=gf, grad_restitution=ge)
+        if self.njoints:
+            gt = np.zeros(self.njoints, dtype=np.float32)
+            _check(lib().prx_adjoint_joint_gradients(self.h, gt.ctypes.data))
+            out["grad_motor_torque"] = gt
+        return out
+
+    def use_general(self, on=True):
+        """Diagnostic: reverse a disk-only batch with the general kernel."""
+        _check(lib().prx_adjoint_use_general(self.h, 1 if on else 0))
 
     def diagnostics(self):
         out = np.zeros(4, dtype=np.float32)
@@ -496,13 +508,28 @@ def desc_offsets(desc):
     return shapes, bodies
 
 
-def desc_with(desc, velocities=None, friction=None, restitution=None):
+def desc_joint_offsets(desc):
+    """Byte offsets of each joint record in a world description."""
+    import struct
+    shapes, bodies = desc_offsets(desc)
+    u32 = lambda o: struct.unpack_from("<I", desc, o)[0]
+    if bodies:
+        off = bodies[-1] + 8 + 76 + 4 + 4 * u32(bodies[-1] + 8 + 76)
+    else:
+        off = shapes[-1] + 4 + 16 + 12 + 16 * u32(shapes[-1]) + 4 if shapes else 8
+    nj = u32(off)
+    return [off + 4 + 48 * k for k in range(nj)]
+
+
+def desc_with(desc, velocities=None, friction=None, restitution=None, motor_torque=None):
     """A copy of a world description with body velocities (dict body ->
-    (vx, vy, w)) and shape friction / restitution (dict shape -> value)
-    replaced."""
+    (vx, vy, w)), shape friction / restitution (dict shape -> value) and
+    joint maximum motor torques (dict joint -> value) replaced."""
     import struct
     buf = bytearray(desc)
     shapes, bodies = desc_offsets(desc)
+    for j, v in (motor_torque or {}).items():
+        struct.pack_into("<f", buf, desc_joint_offsets(desc)[j] + 44, float(v))
     for b, v in (velocities or {}).items():
         struct.pack_into("<3f", buf, bodies[b] + 8 + 32, *[float(x) for x in v])
     for sidx, f in (friction or {}).items():

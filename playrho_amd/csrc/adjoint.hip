@@ -21,6 +21,7 @@
 //
 // One launch reverses one recorded step for every world: one wavefront per
 // world, one lane per island (islands share no dynamic body).
+#include "adjoint_dual.hpp"
 #include "layout.hpp"
 #include "step_device.hpp"
 
@@ -149,7 +150,7 @@ __global__ __launch_bounds__(64) void k_adjoint_step(DeviceState d, Tape t, AdjS
                 const float4 ms = vc[8];
                 const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
                 const int bA = __float_as_int(vc[2].w), bB = __float_as_int(vc[5].z);
-                const float4* g = t.gs + (slot * std::size_t(t.itV) + it) * 2;
+                const float4* g = t.gs + (slot * std::size_t(t.itV) + it) * 3;
                 const V2 vA{g[0].x, g[0].y}, vB{g[1].x, g[1].y};
                 const float wA = g[0].z, wB = g[1].z, ltOld = g[0].w, lnOld = g[1].w;
                 atomicAdd(a.diag + 3, 1.0f);
@@ -396,7 +397,491 @@ __global__ __launch_bounds__(64) void k_adjoint_step(DeviceState d, Tape t, AdjS
     }
 }
 
+// ---------------------------------------------------------------------------
+// General form: any manifold type and point count, revolute joints, bodies
+// with local centres.  The active set is held fixed as recorded; each solver
+// update is reversed by its vector-Jacobian product, which the wave computes
+// from the update's dual copy (adjoint_dual.hpp): lane i evaluates the copy
+// with input i seeded and returns sum_j ybar_j d y_j / d x_i.  One wave per
+// world walks the recorded updates in reverse order.
+
+template <int N, int M, class F>
+__device__ __forceinline__ float lane_vjp(const float (&x)[N], const float (&ybar)[M], F f) {
+    const int lane = int(threadIdx.x);
+    dl::D in[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) in[i] = dl::D{x[i], (i == lane) ? 1.0f : 0.0f};
+    dl::D out[M];
+    f(in, out);
+    float s = 0.0f;
+#pragma unroll
+    for (int j = 0; j < M; ++j) s += ybar[j] * out[j].d;
+    return (lane < N) ? s : 0.0f;
+}
+
+// Velocity-constraint values a contact's updates read, in the order of the
+// creation outputs (kAdjFields): normal x, y; friction; point 0 relA, relB,
+// ni, ti, normal mass, tangent mass, bias; point 1 the same; K00, K11, K01,
+// M00, M11, M01.
+__device__ __forceinline__ void vc_fields(const float4* v, float (&f)[27]) {
+    f[0] = v[0].x; f[1] = v[0].y; f[2] = v[0].z;
+    f[3] = v[3].x; f[4] = v[3].y; f[5] = v[3].z; f[6] = v[3].w;
+    f[7] = v[4].x; f[8] = v[4].y; f[9] = v[4].z; f[10] = v[4].w; f[11] = v[5].x;
+    f[12] = v[6].x; f[13] = v[6].y; f[14] = v[6].z; f[15] = v[6].w;
+    f[16] = v[7].x; f[17] = v[7].y; f[18] = v[7].z; f[19] = v[7].w; f[20] = v[5].y;
+    f[21] = v[1].x; f[22] = v[1].y; f[23] = v[1].z; f[24] = v[2].x; f[25] = v[2].y; f[26] = v[2].z;
+}
+__device__ __forceinline__ dl::Vcd vcd_from(const dl::D* f, const float4* v) {
+    dl::Vcd vc;
+    vc.normal = dl::V{f[0], f[1]};
+    vc.friction = f[2];
+    vc.tangentSpeed = v[0].w;
+    vc.n = __float_as_uint(v[1].w);
+    vc.p[0] = dl::VcPointD{dl::V{f[3], f[4]}, dl::V{f[5], f[6]}, f[7], f[8], f[9], f[10], f[11]};
+    vc.p[1] = dl::VcPointD{dl::V{f[12], f[13]}, dl::V{f[14], f[15]}, f[16], f[17], f[18], f[19], f[20]};
+    vc.K00 = f[21]; vc.K11 = f[22]; vc.K01 = f[23]; vc.M00 = f[24]; vc.M11 = f[25]; vc.M01 = f[26];
+    return vc;
+}
+
+// Joint record values its updates read (kAdjJointFields): rA, rB; exx, eyx,
+// ezx, eyy, ezy, ezz; angular mass; impulses x, y, z, motor; max impulse.
+__device__ __forceinline__ void jc_fields(const float4* u, float (&f)[16]) {
+    f[0] = u[0].x; f[1] = u[0].y; f[2] = u[0].z; f[3] = u[0].w;
+    f[4] = u[1].x; f[5] = u[1].y; f[6] = u[1].z; f[7] = u[1].w;
+    f[8] = u[2].x; f[9] = u[2].y; f[10] = u[2].z;
+    f[11] = u[3].x; f[12] = u[3].y; f[13] = u[3].z; f[14] = u[3].w;
+    f[15] = u[5].y;
+}
+__device__ __forceinline__ dl::Jd jd_from(const dl::D* f) {
+    dl::Jd j;
+    j.rA = dl::V{f[0], f[1]};
+    j.rB = dl::V{f[2], f[3]};
+    j.exx = f[4]; j.eyx = f[5]; j.ezx = f[6]; j.eyy = f[7]; j.ezy = f[8]; j.ezz = f[9];
+    j.angularMass = f[10];
+    j.ix = f[11]; j.iy = f[12]; j.iz = f[13]; j.motorImp = f[14];
+    j.maxImpulse = f[15];
+    return j;
+}
+
+__device__ __forceinline__ void load3(const float4& v, float* o) {
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+}
+__device__ __forceinline__ void load4(const float4& v, float* o) {
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+    o[3] = v.w;
+}
+// Lane k < 3 of a body's three adjoint components (x, y, angle).
+__device__ __forceinline__ float& comp(float4& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+__global__ __launch_bounds__(64) void k_adjoint_general(DeviceState d, Tape t, AdjState a, StepParams c) {
+    const int w = int(blockIdx.x);
+    const int lane = int(threadIdx.x);
+    const WorldHeader H = d.hdr[w];
+    const int bo = H.body_off, co = H.contact_off, nb = H.nbodies, cap = H.contact_cap, jo = H.joint_off;
+    const int nisl = t.nisl[w];
+    const int4* isl = t.isl + bo + w;
+    const int2* islj = t.islj + bo + w;
+    const int nslots = nisl > 0 ? isl[nisl - 1].y : 0;
+    const int njslots = nisl > 0 ? islj[nisl - 1].y : 0;
+    float4* xbar = a.xbar + bo;
+    float4* vbar = a.vbar + bo;
+    auto vcrec = [&](int e) { return t.vc + (std::size_t(co) + std::size_t(e)) * prx::kTapeVc; };
+    auto jcrec = [&](int e) { return t.jc + (std::size_t(jo) + std::size_t(e)) * prx::kTapeJc; };
+
+    // Impulse adjoints carried in from the next step.  Contacts: by pool id,
+    // point by point where the next step started from this step's final
+    // impulses (the manifold update carried them).  Joints: by joint.
+    for (int e = lane; e < nslots; e += kWave) {
+        const float4* vc = vcrec(e);
+        const std::uint32_t id = __float_as_uint(vc[10].y);
+        const float4 lb = a.lamc[(std::size_t(co) + id) * 2];
+        const float4 nv = a.lamc[(std::size_t(co) + id) * 2 + 1];
+        const float4 fin = vc[14];
+        const float fn[2] = {fin.x, fin.z}, ft[2] = {fin.y, fin.w};
+        const float nn[2] = {nv.x, nv.z}, nt[2] = {nv.y, nv.w};
+        const float bn[2] = {lb.x, lb.z}, bt[2] = {lb.y, lb.w};
+        float r[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int k = 0; k < 2; ++k) {
+            for (int q = 0; q < 2; ++q) {
+                const int j = (q == 0) ? k : 1 - k;
+                if (nn[j] == fn[k] && nt[j] == ft[k]) {
+                    r[2 * k] = bn[j];
+                    r[2 * k + 1] = bt[j];
+                    break;
+                }
+            }
+        }
+        a.lam4[std::size_t(co) + e] = float4{r[0], r[1], r[2], r[3]};
+        float* fl = a.fld + (std::size_t(co) + e) * prx::kAdjFields;
+        for (int k = 0; k < prx::kAdjFields; ++k) fl[k] = 0.0f;
+    }
+    for (int e = lane; e < njslots; e += kWave) {
+        const int jg = __float_as_int(jcrec(e)[7].w);
+        a.jlam[std::size_t(jo) + e] = a.jcarry[jg];
+        float* fl = a.jfld + (std::size_t(jo) + e) * prx::kAdjJointFields;
+        for (int k = 0; k < prx::kAdjJointFields; ++k) fl[k] = 0.0f;
+    }
+    __syncthreads();
+    for (int i = lane; i < cap; i += kWave) {
+        a.lamc[(std::size_t(co) + i) * 2] = float4{0.0f, 0.0f, 0.0f, 0.0f};
+        a.lamc[(std::size_t(co) + i) * 2 + 1] = float4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    __syncthreads();
+
+    // Position sweeps, reversed: per iteration the joints (after the
+    // contacts in the forward pass), then the contacts, each in reverse.
+    for (int is = 0; is < nisl; ++is) {
+        const int4 rng = isl[is];
+        const int2 jr = islj[is];
+        for (int it = rng.w - 1; it >= 0; --it) {
+            for (int e = jr.y - 1; e >= jr.x; --e) {
+                const float4* u = jcrec(e);
+                const int bA = __float_as_int(u[5].z), bB = __float_as_int(u[5].w);
+                const float4* g = t.jngs + ((std::size_t(jo) + e) * std::size_t(t.itP) + it) * 2;
+                float x[6], yb[6];
+                load3(g[0], x);
+                load3(g[1], x + 3);
+                const float4 xa = xbar[bA], xb = xbar[bB];
+                load3(xa, yb);
+                load3(xb, yb + 3);
+                const float r = lane_vjp<6, 6>(x, yb, [&](dl::D* in, dl::D* out) {
+                    dl::joint_solve_position(u, in, c);
+                    for (int k = 0; k < 6; ++k) out[k] = in[k];
+                });
+                __syncthreads();
+                if (lane < 3) comp(xbar[bA], lane) = r;
+                else if (lane < 6) comp(xbar[bB], lane - 3) = r;
+                __syncthreads();
+            }
+            for (int e = rng.y - 1; e >= rng.x; --e) {
+                const float4* vc = vcrec(e);
+                const Pc pc = load_pc(vc + 17);
+                const float4* g = t.ngs + ((std::size_t(co) + e) * std::size_t(t.itP) + it) * 2;
+                float x[6], yb[6];
+                load3(g[0], x);
+                load3(g[1], x + 3);
+                load3(xbar[pc.bA], yb);
+                load3(xbar[pc.bB], yb + 3);
+                const float r = lane_vjp<6, 6>(x, yb, [&](dl::D* in, dl::D* out) {
+                    dl::solve_pc(pc.m, pc.totalRadius, pc.A, pc.B, in, c);
+                    for (int k = 0; k < 6; ++k) out[k] = in[k];
+                });
+                __syncthreads();
+                if (lane < 3) comp(xbar[pc.bA], lane) = r;
+                else if (lane < 6) comp(xbar[pc.bB], lane - 3) = r;
+                __syncthreads();
+                if (lane == 0) atomicAdd(a.diag + 2, 1.0f);
+            }
+        }
+    }
+    __syncthreads();
+    // Position integration, reversed: x1 = x0 + h v.
+    for (int b = lane; b < nb; b += kWave) {
+        if (!t.inisl[bo + b]) continue;
+        const float4 xb = xbar[b];
+        add_body(vbar + b, V2{c.dt * xb.x, c.dt * xb.y}, c.dt * xb.z);
+    }
+    __syncthreads();
+
+    for (int is = 0; is < nisl; ++is) {
+        const int4 rng = isl[is];
+        const int2 jr = islj[is];
+        // Velocity sweeps, reversed: per iteration the contacts, then the
+        // joints (solved first in the forward pass), each in reverse.
+        for (int it = rng.z - 1; it >= 0; --it) {
+            for (int e = rng.y - 1; e >= rng.x; --e) {
+                const std::size_t slot = std::size_t(co) + e;
+                const float4* vc = vcrec(e);
+                const int bA = __float_as_int(vc[2].w), bB = __float_as_int(vc[5].z);
+                const BodyC A{vc[8].x, vc[8].y, V2{0, 0}}, B{vc[8].z, vc[8].w, V2{0, 0}};
+                const float4* g = t.gs + (slot * std::size_t(t.itV) + it) * 3;
+                float x[33], yb[10];
+                load3(g[0], x);
+                load3(g[1], x + 3);
+                load4(g[2], x + 6);
+                float f[27];
+                vc_fields(vc, f);
+                // Fields the update reads (creation-output indices).
+                constexpr int kMap[23] = {0, 1, 2, 3, 4, 5, 6, 9, 10, 11, 12, 13, 14, 15, 18, 19, 20, 21, 22, 23, 24, 25, 26};
+#pragma unroll
+                for (int k = 0; k < 23; ++k) x[10 + k] = f[kMap[k]];
+                load3(vbar[bA], yb);
+                load3(vbar[bB], yb + 3);
+                load4(a.lam4[slot], yb + 6);
+                const float r = lane_vjp<33, 10>(x, yb, [&](dl::D* in, dl::D* out) {
+                    dl::D fd[27];
+                    for (int k = 0; k < 27; ++k) fd[k] = dl::cst(f[k]);
+#pragma unroll
+                    for (int k = 0; k < 23; ++k) fd[kMap[k]] = in[10 + k];
+                    fd[7] = in[6];
+                    fd[8] = in[7];
+                    fd[16] = in[8];
+                    fd[17] = in[9];
+                    dl::Vcd v = vcd_from(fd, vc);
+                    dl::D va[3] = {in[0], in[1], in[2]}, vb[3] = {in[3], in[4], in[5]};
+                    dl::solve_velocity(v, A, B, va, vb);
+                    for (int k = 0; k < 3; ++k) {
+                        out[k] = va[k];
+                        out[3 + k] = vb[k];
+                    }
+                    out[6] = v.p[0].ni;
+                    out[7] = v.p[0].ti;
+                    out[8] = v.p[1].ni;
+                    out[9] = v.p[1].ti;
+                });
+                __syncthreads();
+                if (lane < 3) comp(vbar[bA], lane) = r;
+                else if (lane < 6) comp(vbar[bB], lane - 3) = r;
+                else if (lane < 10) (&a.lam4[slot].x)[lane - 6] = r;
+                else if (lane < 33) a.fld[slot * prx::kAdjFields + kMap[lane - 10]] += r;
+                __syncthreads();
+                if (lane == 0) atomicAdd(a.diag + 3, 1.0f);
+            }
+            for (int e = jr.y - 1; e >= jr.x; --e) {
+                const std::size_t js = std::size_t(jo) + e;
+                const float4* u = jcrec(e);
+                const int bA = __float_as_int(u[5].z), bB = __float_as_int(u[5].w);
+                const float4* g = t.jgs + (js * std::size_t(t.itV) + it) * 3;
+                float f[16];
+                jc_fields(u, f);
+                float x[22], yb[10];
+                load3(g[0], x);
+                load3(g[1], x + 3);
+                load4(g[2], x + 6);
+                constexpr int kJMap[12] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 15};
+#pragma unroll
+                for (int k = 0; k < 12; ++k) x[10 + k] = f[kJMap[k]];
+                load3(vbar[bA], yb);
+                load3(vbar[bB], yb + 3);
+                load4(a.jlam[js], yb + 6);
+                const float r = lane_vjp<22, 10>(x, yb, [&](dl::D* in, dl::D* out) {
+                    dl::D fd[16];
+                    for (int k = 0; k < 16; ++k) fd[k] = dl::cst(f[k]);
+#pragma unroll
+                    for (int k = 0; k < 12; ++k) fd[kJMap[k]] = in[10 + k];
+                    fd[11] = in[6];
+                    fd[12] = in[7];
+                    fd[13] = in[8];
+                    fd[14] = in[9];
+                    dl::Jd j = jd_from(fd);
+                    dl::D va[3] = {in[0], in[1], in[2]}, vb[3] = {in[3], in[4], in[5]};
+                    dl::joint_solve_velocity(j, u, va, vb);
+                    for (int k = 0; k < 3; ++k) {
+                        out[k] = va[k];
+                        out[3 + k] = vb[k];
+                    }
+                    out[6] = j.ix;
+                    out[7] = j.iy;
+                    out[8] = j.iz;
+                    out[9] = j.motorImp;
+                });
+                __syncthreads();
+                if (lane < 3) comp(vbar[bA], lane) = r;
+                else if (lane < 6) comp(vbar[bB], lane - 3) = r;
+                else if (lane < 10) (&a.jlam[js].x)[lane - 6] = r;
+                else if (lane < 22) a.jfld[js * prx::kAdjJointFields + kJMap[lane - 10]] += r;
+                __syncthreads();
+            }
+        }
+        // Joint warm start, reversed (after the contacts' in the forward
+        // pass; linear in the velocities, so their values do not enter).
+        for (int e = jr.y - 1; e >= jr.x; --e) {
+            const std::size_t js = std::size_t(jo) + e;
+            const float4* u = jcrec(e);
+            const int bA = __float_as_int(u[5].z), bB = __float_as_int(u[5].w);
+            if (c.warmStart) {
+                float f[16];
+                jc_fields(u, f);
+                float x[14] = {0, 0, 0, 0, 0, 0, f[0], f[1], f[2], f[3], f[11], f[12], f[13], f[14]};
+                float yb[6];
+                load3(vbar[bA], yb);
+                load3(vbar[bB], yb + 3);
+                const float r = lane_vjp<14, 6>(x, yb, [&](dl::D* in, dl::D* out) {
+                    dl::D fd[16];
+                    for (int k = 0; k < 16; ++k) fd[k] = dl::cst(f[k]);
+                    for (int k = 0; k < 4; ++k) fd[k] = in[6 + k];
+                    for (int k = 0; k < 4; ++k) fd[11 + k] = in[10 + k];
+                    const dl::Jd j = jd_from(fd);
+                    dl::D va[3] = {in[0], in[1], in[2]}, vb[3] = {in[3], in[4], in[5]};
+                    dl::joint_warm_start(j, u, va, vb);
+                    for (int k = 0; k < 3; ++k) {
+                        out[k] = va[k];
+                        out[3 + k] = vb[k];
+                    }
+                });
+                __syncthreads();
+                if (lane >= 6 && lane < 10) a.jfld[js * prx::kAdjJointFields + (lane - 6)] += r;
+                else if (lane >= 10 && lane < 14) a.jfld[js * prx::kAdjJointFields + 11 + (lane - 10)] += r;
+                __syncthreads();
+            }
+            // The sweeps started from the set-up's impulses.
+            if (lane < 4) a.jfld[js * prx::kAdjJointFields + 11 + lane] += (&a.jlam[js].x)[lane];
+            __syncthreads();
+        }
+        // Contact warm start, reversed (linear in the velocities).
+        for (int e = rng.y - 1; e >= rng.x; --e) {
+            const std::size_t slot = std::size_t(co) + e;
+            const float4* vc = vcrec(e);
+            const int bA = __float_as_int(vc[2].w), bB = __float_as_int(vc[5].z);
+            const BodyC A{vc[8].x, vc[8].y, V2{0, 0}}, B{vc[8].z, vc[8].w, V2{0, 0}};
+            if (c.warmStart) {
+                float f[27];
+                vc_fields(vc, f);
+                constexpr int kWMap[14] = {0, 1, 3, 4, 5, 6, 7, 8, 12, 13, 14, 15, 16, 17};
+                float x[20] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+                for (int k = 0; k < 14; ++k) x[6 + k] = f[kWMap[k]];
+                float yb[6];
+                load3(vbar[bA], yb);
+                load3(vbar[bB], yb + 3);
+                const float r = lane_vjp<20, 6>(x, yb, [&](dl::D* in, dl::D* out) {
+                    dl::D fd[27];
+                    for (int k = 0; k < 27; ++k) fd[k] = dl::cst(f[k]);
+#pragma unroll
+                    for (int k = 0; k < 14; ++k) fd[kWMap[k]] = in[6 + k];
+                    const dl::Vcd v = vcd_from(fd, vc);
+                    dl::D va[3] = {in[0], in[1], in[2]}, vb[3] = {in[3], in[4], in[5]};
+                    dl::warm_start(v, A, B, va, vb);
+                    for (int k = 0; k < 3; ++k) {
+                        out[k] = va[k];
+                        out[3 + k] = vb[k];
+                    }
+                });
+                __syncthreads();
+                if (lane >= 6 && lane < 20) a.fld[slot * prx::kAdjFields + kWMap[lane - 6]] += r;
+                __syncthreads();
+            }
+            // The sweeps started from the warm-start impulses.
+            if (lane < 4) {
+                constexpr int kImp[4] = {7, 8, 16, 17};
+                a.fld[slot * prx::kAdjFields + kImp[lane]] += (&a.lam4[slot].x)[lane];
+            }
+            __syncthreads();
+        }
+        // Joint set-up, reversed: angles, the impulses it started from and
+        // the maximum motor torque.
+        for (int e = jr.y - 1; e >= jr.x; --e) {
+            const std::size_t js = std::size_t(jo) + e;
+            const float4* u = jcrec(e);
+            const int bA = __float_as_int(u[5].z), bB = __float_as_int(u[5].w);
+            const int jg = __float_as_int(u[7].w);
+            const float4 in8 = u[8], raw = u[9];
+            float x[7] = {in8.x, in8.y, raw.x, raw.y, raw.z, raw.w, in8.z};
+            float yb[16];
+            for (int k = 0; k < 16; ++k) yb[k] = a.jfld[js * prx::kAdjJointFields + k];
+            const std::uint32_t prevLimit = __float_as_uint(in8.w);
+            const float r = lane_vjp<7, 16>(x, yb, [&](dl::D* in, dl::D* out) {
+                const dl::Jd j = dl::build_jc(in[0], in[1], in + 2, in[6], u, prevLimit, c);
+                out[0] = j.rA.x; out[1] = j.rA.y; out[2] = j.rB.x; out[3] = j.rB.y;
+                out[4] = j.exx; out[5] = j.eyx; out[6] = j.ezx; out[7] = j.eyy; out[8] = j.ezy; out[9] = j.ezz;
+                out[10] = j.angularMass;
+                out[11] = j.ix; out[12] = j.iy; out[13] = j.iz; out[14] = j.motorImp;
+                out[15] = j.maxImpulse;
+            });
+            __syncthreads();
+            if (lane == 0) xbar[bA].z += r;
+            else if (lane == 1) xbar[bB].z += r;
+            else if (lane < 6) (&a.jcarry[jg].x)[lane - 2] = r;
+            else if (lane == 6) a.tbar[jg] += r;
+            __syncthreads();
+        }
+        // Contact constraint creation, reversed: positions and velocities at
+        // creation, materials, and the manifold impulses it started from.
+        for (int e = rng.y - 1; e >= rng.x; --e) {
+            const std::size_t slot = std::size_t(co) + e;
+            const float4* vc = vcrec(e);
+            const Pc pc = load_pc(vc + 17);
+            const std::uint32_t id = __float_as_uint(vc[10].y);
+            const float rA = vc[10].w, rB = vc[11].x;
+            float x[18];
+            load3(vc[15], x);
+            load3(vc[16], x + 3);
+            load3(vc[12], x + 6);
+            load3(vc[13], x + 9);
+            x[12] = vc[0].z;
+            x[13] = vc[5].w;
+            load4(vc[22], x + 14);
+            float yb[27];
+            for (int k = 0; k < 27; ++k) yb[k] = a.fld[slot * prx::kAdjFields + k];
+            const float r = lane_vjp<18, 27>(x, yb, [&](dl::D* in, dl::D* out) {
+                const dl::X xfA = dl::get_xf(in[0], in[1], in[2], pc.A.lc);
+                const dl::X xfB = dl::get_xf(in[3], in[4], in[5], pc.B.lc);
+                const dl::WMd wm = dl::world_manifold(pc.m, xfA, rA, xfB, rB);
+                const dl::Vcd v = dl::make_vc(wm, in[12], in[13], vc[0].w, pc.A, in, in + 6, pc.B, in + 3, in + 9,
+                                              in + 14, c);
+                out[0] = v.normal.x; out[1] = v.normal.y; out[2] = v.friction;
+                for (int k = 0; k < 2; ++k) {
+                    const dl::VcPointD& p = v.p[k];
+                    dl::D* o = out + 3 + 9 * k;
+                    o[0] = p.relA.x; o[1] = p.relA.y; o[2] = p.relB.x; o[3] = p.relB.y;
+                    o[4] = p.ni; o[5] = p.ti; o[6] = p.nm; o[7] = p.tm; o[8] = p.bias;
+                }
+                out[21] = v.K00; out[22] = v.K11; out[23] = v.K01; out[24] = v.M00; out[25] = v.M11; out[26] = v.M01;
+            });
+            __syncthreads();
+            if (lane < 3) comp(xbar[pc.bA], lane) += r;
+            else if (lane < 6) comp(xbar[pc.bB], lane - 3) += r;
+            else if (lane < 9) comp(vbar[pc.bA], lane - 6) += r;
+            else if (lane < 12) comp(vbar[pc.bB], lane - 9) += r;
+            else if (lane == 12 || lane == 13) {
+                const int sA = __float_as_int(vc[11].y), sB = __float_as_int(vc[11].z);
+                if (lane == 12) {
+                    // Friction = sqrt(fA fB) (Contact.hpp:57-60).
+                    const float fA = d.shape_f[sA].y, fB = d.shape_f[sB].y, fm = vc[0].z;
+                    if (fm > 0.0f) {
+                        atomicAdd(a.fbar + sA, r * fB / (2.0f * fm));
+                        atomicAdd(a.fbar + sB, r * fA / (2.0f * fm));
+                    }
+                } else {
+                    // Restitution = max(eA, eB), ties to B (Contact.hpp:66-69).
+                    const float eA = d.shape_f[sA].z, eB = d.shape_f[sB].z;
+                    atomicAdd(a.ebar + ((eA > eB) ? sA : sB), r);
+                }
+            } else if (lane < 18) {
+                (&a.lamc[(std::size_t(co) + id) * 2].x)[lane - 14] = r;
+                (&a.lamc[(std::size_t(co) + id) * 2 + 1].x)[lane - 14] = (&vc[22].x)[lane - 14];
+            }
+            __syncthreads();
+            if (lane == 0) atomicAdd(a.diag + 0, 1.0f);
+        }
+    }
+    __syncthreads();
+    // Velocity integration, reversed (damping factors; caps not active).
+    for (int b = lane; b < nb; b += kWave) {
+        if (!t.inisl[bo + b]) continue;
+        const int gb = bo + b;
+        if (!(d.bflags[gb] & prx::kAccelerable)) continue;
+        const float2 dm = d.damp[gb];
+        const float fl = 1.0f / (1.0f + c.dt * dm.x);
+        const float fa = 1.0f / (1.0f + c.dt * dm.y);
+        float4 v = vbar[b];
+        v.x *= fl;
+        v.y *= fl;
+        v.z *= fa;
+        vbar[b] = v;
+    }
+    // Bodies that do not move carry no adjoint (the reverse updates leave
+    // scratch values there, which never reach a moving body: their inverse
+    // masses are zero).
+    for (int b = lane; b < nb; b += kWave) {
+        if (d.bflags[bo + b] & prx::kAccelerable) continue;
+        xbar[b] = float4{0.0f, 0.0f, 0.0f, 0.0f};
+        vbar[b] = float4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+}
+
 }  // namespace prxd
+
+extern "C" hipError_t prx_launch_adjoint_general(const prx::DeviceState* d, const prx::Tape* t,
+                                                 const prx::AdjState* a, const prx::StepParams* c, int nworlds,
+                                                 hipStream_t s) {
+    hipLaunchKernelGGL(prxd::k_adjoint_general, dim3(unsigned(nworlds)), dim3(64), 0, s, *d, *t, *a, *c);
+    return hipGetLastError();
+}
 
 extern "C" hipError_t prx_launch_adjoint_step(const prx::DeviceState* d, const prx::Tape* t, const prx::AdjState* a,
                                               const prx::StepParams* c, int nworlds, hipStream_t s) {

@@ -38,6 +38,9 @@ extern "C" hipError_t prx_launch_step_tape(const prx::DeviceState* d, const prx:
                                            hipStream_t stream);
 extern "C" hipError_t prx_launch_adjoint_step(const prx::DeviceState* d, const prx::Tape* t, const prx::AdjState* a,
                                               const prx::StepParams* c, int nworlds, hipStream_t s);
+extern "C" hipError_t prx_launch_adjoint_general(const prx::DeviceState* d, const prx::Tape* t,
+                                                 const prx::AdjState* a, const prx::StepParams* c, int nworlds,
+                                                 hipStream_t s);
 
 namespace {
 
@@ -1978,6 +1981,7 @@ std::vector<std::pair<void*, std::size_t>> DynamicArrays(prx_batch* b) {
 struct prx_adjoint {
     prx_batch* b = nullptr;
     int K = 0, S = 0, nseg = 0, itV = 0, itP = 0;
+    bool general = false;  // joints, polygons or off-centre disks: the general reverse kernel
     DeviceArena arena;
     std::vector<prx::Tape> tapes;
     prx::AdjState a{};
@@ -1988,21 +1992,21 @@ extern "C" {
 
 int prx_adjoint_create(prx_batch* b, int nsteps, int every, const prx_step_conf* conf, prx_adjoint** out) {
     if (!b || !out || !conf || nsteps < 1 || every < 1) return Fail(PRX_E_INVALID_ARGUMENT, "bad argument");
+    bool general = false;
     for (prx_world* w : b->worlds) {
         if (!w) return Fail(PRX_E_WRONG_STATE, "a member world left the batch");
-        if (!w->joints.empty()) return Fail(PRX_E_UNSUPPORTED, "the adjoint does not model joints");
+        if (!w->joints.empty()) general = true;
         for (const auto& hb : w->bodies) {
             if (!(hb.flags & prx::kAccelerable)) continue;
-            if (hb.lc.x != 0.0f || hb.lc.y != 0.0f)
-                return Fail(PRX_E_UNSUPPORTED, "the adjoint models dynamic disks centred on their body");
+            if (hb.lc.x != 0.0f || hb.lc.y != 0.0f) general = true;
             for (std::uint32_t sid : hb.shapes) {
                 const HostShape& sh = w->shapes[sid];
-                if (sh.v.size() != 1 || sh.v[0].x != 0.0f || sh.v[0].y != 0.0f)
-                    return Fail(PRX_E_UNSUPPORTED, "the adjoint models dynamic disks centred on their body");
+                if (sh.v.size() != 1 || sh.v[0].x != 0.0f || sh.v[0].y != 0.0f) general = true;
             }
         }
     }
     auto adj = std::make_unique<prx_adjoint>();
+    adj->general = general;
     adj->b = b;
     adj->K = nsteps;
     adj->S = std::min(every, nsteps);
@@ -2011,14 +2015,19 @@ int prx_adjoint_create(prx_batch* b, int nsteps, int every, const prx_step_conf*
     adj->itP = int(std::max<std::uint32_t>(1u, conf->reg_position_iters));
     auto& A = adj->arena;
     const std::size_t B = b->totB, C = b->totC, S = b->totS, nw = b->worlds.size();
+    const std::size_t J = std::max<std::size_t>(b->totJ, 1);
     adj->tapes.resize(std::size_t(adj->S));
     for (auto& t : adj->tapes) {
         PRX_HIP(A.alloc(t.vc, C * prx::kTapeVc));
-        PRX_HIP(A.alloc(t.gs, C * std::size_t(adj->itV) * 2));
+        PRX_HIP(A.alloc(t.gs, C * std::size_t(adj->itV) * 3));
         PRX_HIP(A.alloc(t.ngs, C * std::size_t(adj->itP) * 2));
         PRX_HIP(A.alloc(t.isl, B + nw));
+        PRX_HIP(A.alloc(t.islj, B + nw));
         PRX_HIP(A.alloc(t.nisl, nw));
         PRX_HIP(A.alloc(t.inisl, B));
+        PRX_HIP(A.alloc(t.jc, J * prx::kTapeJc));
+        PRX_HIP(A.alloc(t.jgs, J * std::size_t(adj->itV) * 3));
+        PRX_HIP(A.alloc(t.jngs, J * std::size_t(adj->itP) * 2));
         t.itV = adj->itV;
         t.itP = adj->itP;
     }
@@ -2031,6 +2040,13 @@ int prx_adjoint_create(prx_batch* b, int nsteps, int every, const prx_step_conf*
     PRX_HIP(A.alloc(adj->a.ebar, S));
     PRX_HIP(A.alloc(adj->a.status, 1));
     PRX_HIP(A.alloc(adj->a.diag, 4));
+    PRX_HIP(A.alloc(adj->a.fld, C * prx::kAdjFields));
+    PRX_HIP(A.alloc(adj->a.lam4, C));
+    PRX_HIP(A.alloc(adj->a.lamc, 2 * C));
+    PRX_HIP(A.alloc(adj->a.jfld, J * prx::kAdjJointFields));
+    PRX_HIP(A.alloc(adj->a.jlam, J));
+    PRX_HIP(A.alloc(adj->a.jcarry, J));
+    PRX_HIP(A.alloc(adj->a.tbar, J));
     const auto arrays = DynamicArrays(b);
     adj->ckpt.resize(std::size_t(adj->nseg));
     for (auto& ck : adj->ckpt) {
@@ -2111,6 +2127,12 @@ int prx_adjoint_run(prx_adjoint* adj, const prx_step_conf* conf, const float* ta
     PRX_HIP(hipMemset(adj->a.ebar, 0, S * sizeof(float)));
     PRX_HIP(hipMemset(adj->a.status, 0, sizeof(int)));
     PRX_HIP(hipMemset(adj->a.diag, 0, 4 * sizeof(float)));
+    {
+        const std::size_t J = std::max<std::size_t>(b->totJ, 1);
+        PRX_HIP(hipMemset(adj->a.lamc, 0, 2 * b->totC * sizeof(float4)));
+        PRX_HIP(hipMemset(adj->a.jcarry, 0, J * sizeof(float4)));
+        PRX_HIP(hipMemset(adj->a.tbar, 0, J * sizeof(float)));
+    }
     // Backward, segment by segment: recompute the segment from its
     // checkpoint recording every step, then reverse its steps.
     for (int seg = adj->nseg - 1; seg >= 0; --seg) {
@@ -2123,7 +2145,9 @@ int prx_adjoint_run(prx_adjoint* adj, const prx_step_conf* conf, const float* ta
             if (le != hipSuccess) return Fail(PRX_E_DEVICE, std::string("launch: ") + hipGetErrorString(le));
         }
         for (int s = n - 1; s >= 0; --s) {
-            const hipError_t le = prx_launch_adjoint_step(&b->d, &adj->tapes[std::size_t(s)], &adj->a, &p, nw, st);
+            const hipError_t le =
+                adj->general ? prx_launch_adjoint_general(&b->d, &adj->tapes[std::size_t(s)], &adj->a, &p, nw, st)
+                             : prx_launch_adjoint_step(&b->d, &adj->tapes[std::size_t(s)], &adj->a, &p, nw, st);
             if (le != hipSuccess) return Fail(PRX_E_DEVICE, std::string("launch: ") + hipGetErrorString(le));
         }
     }
@@ -2149,6 +2173,33 @@ int prx_adjoint_run(prx_adjoint* adj, const prx_step_conf* conf, const float* ta
     PRX_HIP(hipStreamSynchronize(st));
     for (prx_world* w : b->worlds) w->host_stale = true;
     if (status) return Fail(PRX_E_UNSUPPORTED, "a contact the adjoint does not model (two-point manifold or face B)");
+    return PRX_OK;
+}
+
+int prx_adjoint_use_general(prx_adjoint* adj, int on) {
+    if (!adj) return Fail(PRX_E_INVALID_ARGUMENT, "null argument");
+    bool needed = false;
+    for (prx_world* w : adj->b->worlds) {
+        if (!w) continue;
+        if (!w->joints.empty()) needed = true;
+        for (const auto& hb : w->bodies) {
+            if (!(hb.flags & prx::kAccelerable)) continue;
+            if (hb.lc.x != 0.0f || hb.lc.y != 0.0f) needed = true;
+            for (std::uint32_t sid : hb.shapes) {
+                const HostShape& sh = w->shapes[sid];
+                if (sh.v.size() != 1 || sh.v[0].x != 0.0f || sh.v[0].y != 0.0f) needed = true;
+            }
+        }
+    }
+    if (!on && needed) return Fail(PRX_E_UNSUPPORTED, "this batch needs the general reverse kernel");
+    adj->general = on != 0;
+    return PRX_OK;
+}
+
+int prx_adjoint_joint_gradients(prx_adjoint* adj, float* grad_max_motor_torque) {
+    if (!adj || !grad_max_motor_torque) return Fail(PRX_E_INVALID_ARGUMENT, "null argument");
+    const std::size_t J = adj->b->totJ;
+    if (J) PRX_HIP(hipMemcpy(grad_max_motor_torque, adj->a.tbar, J * sizeof(float), hipMemcpyDeviceToHost));
     return PRX_OK;
 }
 

@@ -73,16 +73,25 @@ struct StepParams {
 // island-order constraint slot the velocity-constraint record as built plus
 // its geometry inputs, the inputs of every Gauss-Seidel update and of every
 // position-solver update, and the island table.
-constexpr int kTapeVc = 15;
+constexpr int kTapeVc = 23;
+constexpr int kTapeJc = 10;
 struct Tape {
     float4* vc;           // slot * kTapeVc: VelocityConstraint record [0..8], body positions [9],
                           // (vn0, contact id, manifold type, radius A) [10], (radius B, shape A, shape B, -) [11],
-                          // velocities of A [12] and B [13] at creation, final (normal, tangent) impulse [14]
-    float4* gs;           // (slot * itV + it) * 2: (vA, wA, tangent impulse), (vB, wB, normal impulse)
+                          // velocities of A [12] and B [13] at creation, final impulses (n0, t0, n1, t1) [14],
+                          // positions of A [15] and B [16] at creation, position-constraint record [17..21],
+                          // manifold impulses before the dtRatio scaling (n0, t0, n1, t1) [22]
+    float4* gs;           // (slot * itV + it) * 3: (vA, wA, point-0 tangent impulse), (vB, wB, point-0 normal
+                          // impulse), impulses (n0, t0, n1, t1) before the update
     float4* ngs;          // (slot * itP + it) * 2: position A (w: separation), position B before the update
     int4* isl;            // per island: first slot, end slot, velocity iterations run, position iterations run
+    int2* islj;           // per island: first and end joint slot
     int* nisl;            // per world
     std::uint8_t* inisl;  // per body: part of an island
+    float4* jc;           // joint slot * kTapeJc: joint record at set-up [0..7], (angle A, angle B, maximum motor
+                          // torque, previous limit state) [8], impulses before the set-up [9]
+    float4* jgs;          // (joint slot * itV + it) * 3: velocities of A, B and the impulses before the update
+    float4* jngs;         // (joint slot * itP + it) * 2: positions of A and B before the update
     int itV, itP;
 };
 
@@ -99,7 +108,18 @@ struct AdjState {
     int* status;    // 1: a record the adjoint does not model (two-point manifold, face B, joints)
     float* diag;    // counters of the last run: slots reversed, of them with a restitution bias,
                     // position-solver updates reversed, velocity updates reversed
+    // General form (polygons, two-point manifolds, joints):
+    float* fld;     // per slot kAdjFields: adjoints of the velocity-constraint values
+    float4* lam4;   // per slot: adjoint of the running impulses (n0, t0, n1, t1)
+    float4* lamc;   // per contact pool id, 2 float4: adjoint of the raw impulses at the later step's start
+                    // and those impulses' values
+    float* jfld;    // per joint slot kAdjJointFields: adjoints of the joint record's values
+    float4* jlam;   // per joint slot: adjoint of the running impulses (x, y, z, motor)
+    float4* jcarry; // per joint: adjoint of the impulses the later step started from
+    float* tbar;    // per joint: d loss / d maximum motor torque
 };
+constexpr int kAdjFields = 28;
+constexpr int kAdjJointFields = 16;
 
 struct DeviceState {
     int nworlds;

@@ -1617,6 +1617,29 @@ __device__ __forceinline__ bool joints_velocity(float4* jbase, int j0, int j1, c
     return ok;
 }
 
+// The same pass recording each joint's inputs (velocities of both bodies
+// and the impulses before the update) for the adjoint.
+template <bool kLds>
+__device__ __forceinline__ bool joints_velocity_tape(float4* jbase, int j0, int j1, const Bodies<kLds>& bs,
+                                                     const prx::Tape& t, int jo, int it) {
+    bool ok = true;
+    for (int e = j0; e < j1; ++e) {
+        float4* u = jbase + std::size_t(e) * prx::kJcStride;
+        Jc j = load_jc_velocity(u);
+        float4 va = bs.vel(j.bA);
+        float4 vb = bs.vel(j.bB);
+        float4* g = t.jgs + ((std::size_t(jo) + std::size_t(e)) * std::size_t(t.itV) + std::size_t(it)) * 3;
+        g[0] = va;
+        g[1] = vb;
+        g[2] = float4{j.ix, j.iy, j.iz, j.motorImp};
+        ok &= joint_solve_velocity(j, va, vb);
+        u[3] = float4{j.ix, j.iy, j.iz, j.motorImp};
+        bs.set_vel(j.bA, va);
+        bs.set_vel(j.bB, vb);
+    }
+    return ok;
+}
+
 // One SolvePosition pass over an island's joints; returns whether all are
 // within tolerance.
 template <bool kLds>
@@ -1629,6 +1652,27 @@ __device__ __forceinline__ bool joints_position(const float4* jbase, int j0, int
         const int bA = __float_as_int(mo.z), bB = __float_as_int(mo.w);
         float4 pa = bs.pos(bA);
         float4 pb = bs.pos(bB);
+        ok &= joint_solve_position(u, pa, pb, c);
+        bs.set_pos(bA, pa);
+        bs.set_pos(bB, pb);
+    }
+    return ok;
+}
+
+// The same pass recording the positions before each update.
+template <bool kLds>
+__device__ __forceinline__ bool joints_position_tape(const float4* jbase, int j0, int j1, const Bodies<kLds>& bs,
+                                                     const StepParams& c, const prx::Tape& t, int jo, int it) {
+    bool ok = true;
+    for (int e = j0; e < j1; ++e) {
+        const float4* u = jbase + std::size_t(e) * prx::kJcStride;
+        const float4 mo = u[5];
+        const int bA = __float_as_int(mo.z), bB = __float_as_int(mo.w);
+        float4 pa = bs.pos(bA);
+        float4 pb = bs.pos(bB);
+        float4* g = t.jngs + ((std::size_t(jo) + std::size_t(e)) * std::size_t(t.itP) + std::size_t(it)) * 2;
+        g[0] = pa;
+        g[1] = pb;
         ok &= joint_solve_position(u, pa, pb, c);
         bs.set_pos(bA, pa);
         bs.set_pos(bB, pb);
@@ -1663,7 +1707,9 @@ __device__ __forceinline__ void island_velocity(const DeviceState& d, const Worl
     if (c.warmStart) joints_velocity(jbase, j0, j1, bs, true);
     for (std::uint32_t it = 0; it < c.velIters; ++it) {
         if constexpr (kTape) d.tape.isl[islandSlot].z = int(it) + 1;
-        const bool jointsOkay = joints_velocity(jbase, j0, j1, bs, false);
+        bool jointsOkay;
+        if constexpr (kTape) jointsOkay = joints_velocity_tape(jbase, j0, j1, bs, d.tape, x.jo, int(it));
+        else jointsOkay = joints_velocity(jbase, j0, j1, bs, false);
         float newInc = 0.0f;
         if (c0 >= c1) {
             if (jointsOkay && newInc <= c.regMinMomentum) break;
@@ -1677,9 +1723,10 @@ __device__ __forceinline__ void island_velocity(const DeviceState& d, const Worl
             const Vc nvc = load_vc(base + std::size_t(en) * prx::kVcStride, nA, nB);
             if constexpr (kTape) {
                 const float4 va = bs.vel(vc.bA), vb = bs.vel(vc.bB);
-                float4* g = d.tape.gs + ((std::size_t(x.co) + std::size_t(e)) * std::size_t(d.tape.itV) + it) * 2;
+                float4* g = d.tape.gs + ((std::size_t(x.co) + std::size_t(e)) * std::size_t(d.tape.itV) + it) * 3;
                 g[0] = float4{va.x, va.y, va.z, vc.p[0].ti};
                 g[1] = float4{vb.x, vb.y, vb.z, vc.p[0].ni};
+                g[2] = float4{vc.p[0].ni, vc.p[0].ti, vc.p[1].ni, vc.p[1].ti};
             }
             const float mi = solve_velocity_one(vc, A, B, bs);
             float4* rec = base + std::size_t(e) * prx::kVcStride;
@@ -1834,7 +1881,9 @@ __device__ __forceinline__ bool island_position(const DeviceState& d, const Worl
             }
         }
         const bool contactsOkay = minSep >= c.regMinSeparation;
-        const bool jointsOkay = joints_position(jbase, j0, j1, bs, c);
+        bool jointsOkay;
+        if constexpr (kTape) jointsOkay = joints_position_tape(jbase, j0, j1, bs, c, d.tape, x.jo, int(it));
+        else jointsOkay = joints_position(jbase, j0, j1, bs, c);
         if (contactsOkay && jointsOkay) return true;
     }
     return false;
@@ -1956,12 +2005,18 @@ __device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, con
             t[11] = float4{rB, __int_as_float(ss.x), __int_as_float(ss.y), 0.0f};
             t[12] = vA;
             t[13] = vB;
+            t[15] = pA;
+            t[16] = pB;
+            store_pc(t + 17, m, bd.x, bd.y, rA + rB, A, B);
+            t[22] = float4{m.p[0].ni, m.p[0].ti, m.p[1].ni, m.p[1].ti};
         }
     }
     if constexpr (kTape) {
         if (x.lane == 0) d.tape.nisl[x.w] = nisl;
-        for (int is = x.lane; is < nisl; is += kWave)
+        for (int is = x.lane; is < nisl; is += kWave) {
             d.tape.isl[x.bo + x.w + is] = int4{S.istart[is].y, S.istart[is + 1].y, 0, 0};
+            d.tape.islj[x.bo + x.w + is] = int2{S.jstart[is], S.jstart[is + 1]};
+        }
         for (int b = x.lane; b < x.nb; b += kWave) d.tape.inisl[x.bo + b] = S.bisland[b] >= 0 ? 1 : 0;
     }
     // Joint set-up (the non-warm-start half of InitVelocity), island order.
@@ -1970,8 +2025,19 @@ __device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, con
         const int jg = x.jo + S.ijoint[e];
         const int2 bd = d.j_body[jg];
         const BodyC A = load_bodyc(d, bo + bd.x), B = load_bodyc(d, bo + bd.y);
+        if constexpr (kTape) {
+            float4* t = d.tape.jc + (std::size_t(x.jo) + std::size_t(e)) * prx::kTapeJc;
+            t[8] = float4{bs.pos(bd.x).z, bs.pos(bd.y).z, d.j_cfg2[jg].x, __uint_as_float(d.j_limit[jg])};
+            t[9] = d.j_imp[jg];
+        }
         build_jc(d, x, jg, d.s_jc + (std::size_t(x.jo) + std::size_t(e)) * prx::kJcStride, A, bs.pos(bd.x), B,
                  bs.pos(bd.y), bd.x, bd.y, c);
+        if constexpr (kTape) {
+            const float4* u = d.s_jc + (std::size_t(x.jo) + std::size_t(e)) * prx::kJcStride;
+            float4* t = d.tape.jc + (std::size_t(x.jo) + std::size_t(e)) * prx::kTapeJc;
+            for (int k = 0; k < prx::kJcStride; ++k) t[k] = u[k];
+            t[7].w = __int_as_float(jg);  // the record leaves .w unused
+        }
     }
     __syncthreads();
     T.mark(prx::kPhConstraints);
@@ -2035,7 +2101,7 @@ __device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, con
         const int gi = x.co + S.icont[e];
         const float4* rec = d.s_vc + (std::size_t(x.co) + std::size_t(e)) * prx::kVcStride;
         if constexpr (kTape)
-            d.tape.vc[(std::size_t(x.co) + std::size_t(e)) * prx::kTapeVc + 14] = float4{rec[4].x, rec[4].y, 0.0f, 0.0f};
+            d.tape.vc[(std::size_t(x.co) + std::size_t(e)) * prx::kTapeVc + 14] = float4{rec[4].x, rec[4].y, rec[7].x, rec[7].y};
         const std::uint32_t n = __float_as_uint(rec[1].w);
         const float4 q0 = rec[4];
         float4 m0 = d.m_p0[gi];
