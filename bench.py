@@ -53,7 +53,7 @@ def parse():
     p.add_argument("--chain-worlds", type=int, default=8192, help="articulated chain worlds (0: skip)")
     p.add_argument("--chain-steps", type=int, default=240)
     p.add_argument("--chain-cpu-worlds", type=int, default=1024, help="worlds in the CPU sample")
-    p.add_argument("--chain-adjoint-worlds", type=int, default=1024,
+    p.add_argument("--chain-adjoint-worlds", type=int, default=8192,
                    help="chain worlds differentiated with respect to joint motor torques (0: skip)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only run a plain device copy of MIB MiB five times (counter calibration)")

@@ -1,7 +1,8 @@
-"""torch binding of the adjoint: the libtorch operator
-torch.ops.playrho_amd.billiards_loss, a C++ torch::autograd::Function over
-the library's adjoint (csrc/torch_adjoint.cpp, built into
-libplayrho_amd_torch.so next to this file).
+"""torch binding of the adjoint: the libtorch operators
+torch.ops.playrho_amd.billiards_loss and articulated_loss, C++
+torch::autograd::Functions over the library's adjoint
+(csrc/torch_adjoint.cpp, built into libplayrho_amd_torch.so next to this
+file).
 
     loss = billiards_loss(v0, friction, restitution, descs, targets, nsteps, every)
     loss.sum().backward()       # v0.grad, friction.grad, restitution.grad
@@ -27,19 +28,32 @@ _LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libplayrho_amd_
 _loaded = False
 
 
-def _op():
+def _ops():
     global _loaded
     if not _loaded:
         if not os.path.exists(_LIB):
             raise RuntimeError("libplayrho_amd_torch.so is not built: run __graft_entry__.build()")
         torch.ops.load_library(_LIB)
         _loaded = True
-    return torch.ops.playrho_amd.billiards_loss
+    return torch.ops.playrho_amd
 
 
-def billiards_loss(v0, friction, restitution, descs, targets, nsteps, every):
+def _pack(descs, v0, targets):
     blob = b"".join(descs)
     offsets = np.cumsum([0] + [len(d) for d in descs]).astype(np.int64)
     t = torch.as_tensor(np.asarray(targets, dtype=np.float32)).reshape(v0.shape[0], v0.shape[1], 2)
-    return _op()(v0, friction, restitution, torch.frombuffer(bytearray(blob), dtype=torch.uint8),
-                 torch.from_numpy(offsets), t, int(nsteps), int(every))
+    return torch.frombuffer(bytearray(blob), dtype=torch.uint8), torch.from_numpy(offsets), t
+
+
+def billiards_loss(v0, friction, restitution, descs, targets, nsteps, every):
+    blob, offsets, t = _pack(descs, v0, targets)
+    return _ops().billiards_loss(v0, friction, restitution, blob, offsets, t, int(nsteps), int(every))
+
+
+def articulated_loss(v0, friction, restitution, torque, descs, targets, nsteps, every):
+    """As billiards_loss, for worlds with revolute joints: torque (worlds,
+    joints) holds every joint's maximum motor torque (each world is built as
+    if its joints were created with these torques) and is differentiated
+    too."""
+    blob, offsets, t = _pack(descs, v0, targets)
+    return _ops().articulated_loss(v0, friction, restitution, torque, blob, offsets, t, int(nsteps), int(every))

@@ -130,3 +130,37 @@ def test_chain_batched_equals_single():
         assert np.array_equal(out["grad_motor_torque"][oj:oj + nj], o1["grad_motor_torque"])
         ob += nb
         oj += nj
+
+
+def test_articulated_torch_operator():
+    """torch.ops.playrho_amd.articulated_loss: its gradients (motor torques,
+    velocities, friction) equal the adjoint's on worlds built with the same
+    parameters, two worlds in one call."""
+    import torch
+    from playrho_amd.autograd import articulated_loss
+    steps = 20
+    descs, tgs, v0s, frs, res, tqs = [], [], [], [], [], []
+    for wi in range(2):
+        w, desc, bodies, joints, dyn, targets = _chain([4, wi, 8, 10, 2], seed_targets=wi)
+        sdesc, _, _ = dumpfmt.read_description(dumpfmt.Reader(desc))
+        descs.append(desc)
+        tgs.append(targets)
+        v0s.append([b["vals"][8:11] for b in bodies])
+        frs.append([s["friction"] for s in sdesc])
+        res.append([s["restitution"] for s in sdesc])
+        tqs.append([j["vals"][8] for j in joints])
+    v0 = torch.tensor(np.array(v0s), dtype=torch.float64, requires_grad=True)
+    fr = torch.tensor(np.array(frs), dtype=torch.float64, requires_grad=True)
+    re = torch.tensor(np.array(res), dtype=torch.float64, requires_grad=True)
+    tq = (torch.tensor(np.array(tqs), dtype=torch.float64) * 1.5).requires_grad_(True)
+    loss = articulated_loss(v0, fr, re, tq, descs, np.array(tgs), steps, 8)
+    loss.sum().backward()
+    worlds = [pa.World.from_desc(pa.desc_with(d, motor_torque={j: float(np.float32(tq[wi, j].item()))
+                                                               for j in range(tq.shape[1])}))
+              for wi, d in enumerate(descs)]
+    direct = pa.Adjoint(pa.Batch(worlds, use_graph=False), steps, 8).run(np.concatenate(tgs).astype(np.float32))
+    assert np.array_equal(loss.detach().numpy().astype(np.float32), direct["loss"])
+    assert np.array_equal(tq.grad.numpy().reshape(-1).astype(np.float32), direct["grad_motor_torque"])
+    assert np.array_equal(v0.grad.numpy().reshape(-1, 3).astype(np.float32), direct["grad_velocity"])
+    assert np.array_equal(fr.grad.numpy().reshape(-1).astype(np.float32), direct["grad_friction"])
+    assert np.any(tq.grad.numpy() != 0.0)
