@@ -728,6 +728,9 @@ int BuildBatch(prx_batch* b) {
     PRX_HIP(a.alloc(d.s_bc_vel, B));
     PRX_HIP(a.alloc(d.s_vc, C * std::size_t(prx::kVcStride)));
     PRX_HIP(a.alloc(d.s_pc, C * std::size_t(prx::kPcStride)));
+    PRX_HIP(a.alloc(d.s_vc2, C * std::size_t(prx::kVcStride)));
+    PRX_HIP(a.alloc(d.s_pc2, C * std::size_t(prx::kPcStride)));
+    PRX_HIP(a.alloc(d.s_lvl, C * std::size_t(prx::kLevelInts)));
     PRX_HIP(a.alloc(d.j_body, J));
     PRX_HIP(a.alloc(d.j_anchor, J));
     PRX_HIP(a.alloc(d.j_cfg, J));

@@ -190,6 +190,9 @@ struct DeviceState {
     float4* s_bc_vel;        // body constraint velocity x, y, w (nbodies)
     float4* s_vc;            // velocity constraints in island order, kVcStride float4 each
     float4* s_pc;            // position constraints in island order, kPcStride float4 each
+    float4* s_vc2;           // level form: velocity constraints of an island in level order
+    float4* s_pc2;           // level form: position constraints of an island in level order
+    int* s_lvl;              // level form: kLevelInts per slot (levels, order, level starts, rounds)
     float4* s_jc;            // joint constraints in island order, kJcStride float4 each (njoints)
     int* s_isl_joint;        // island joint lists (njoints)
     int* s_isl_jstart;       // island i: first joint entry (nbodies + 1, offset body_off + w)
@@ -215,6 +218,9 @@ enum : std::uint32_t { kJointLimit = 1u, kJointMotor = 2u, kJointCollideConnecte
 // Constraint records staged through LDS per chunk by the wave-wide sweeps;
 // the stage sits at the end of the dynamic LDS.
 constexpr int kStageRecords = 16;
+// Level form of the wave-wide sweeps: scratch ints per constraint slot, and
+// the most bodies a world may have (one int per body in the record stage).
+constexpr int kLevelInts = 8;
 constexpr int kStageFloat4 = kStageRecords * kVcStride;
 
 }  // namespace prx

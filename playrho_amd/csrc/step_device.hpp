@@ -1944,6 +1944,237 @@ __device__ __forceinline__ bool island_position_wave(const DeviceState& d, const
     return solved;
 }
 
+// ---------------------------------------------------------------------------
+// Level form of the wave-wide sweeps, for islands without joints in worlds of
+// at most kLevelMaxBodies bodies.  The Gauss-Seidel order is kept exactly: an
+// update's level is one more than the last level that touched either of its
+// bodies (bodies with zero inverse mass and inertia are never changed by an
+// update, so they order nothing), and the updates of one level touch
+// distinct bodies.  The wave runs each level's updates in parallel, level
+// after level, with the inputs the serial sweep gives them; warm-start
+// deltas reach each body in the same order, and the early exits reduce the
+// same maxima and minima.  The plan (levels, level order, rounds of at most
+// one wave) is built once per step and serves the warm start, the velocity
+// iterations and the position iterations; the records are copied into level
+// order so that a round reads consecutive records.
+constexpr int kLevelMaxBodies = prx::kStageFloat4 * 4;
+constexpr int kLevelMinConstraints = 8;
+
+struct LevelPlan {
+    int n, rounds;
+    const int2* rb;    // per round: first and end position in level order
+    const int* order;  // level position -> island position
+};
+
+__device__ __forceinline__ int* level_ints(const DeviceState& d, const WorldCtx& x, int c0) {
+    return d.s_lvl + (std::size_t(x.co) + std::size_t(c0)) * prx::kLevelInts;
+}
+__device__ __forceinline__ int2* level_rounds(int* L, int n) { return reinterpret_cast<int2*>(L + ((3 * n + 3) & ~1)); }
+
+// Builds the plan of island [c0, c1) (n >= kLevelMinConstraints); the
+// record stage serves as the per-body last-level table.
+__device__ __forceinline__ LevelPlan level_plan(const DeviceState& d, const WorldCtx& x, float4* stage, int c0,
+                                                int c1) {
+    const int n = c1 - c0;
+    const int lane = x.lane;
+    const float4* vbase = d.s_vc + (std::size_t(x.co) + std::size_t(c0)) * prx::kVcStride;
+    const float4* pbase = d.s_pc + (std::size_t(x.co) + std::size_t(c0)) * prx::kPcStride;
+    int* L = level_ints(d, x, c0);
+    int* lvl = L;
+    int* order = L + n;
+    int* start = L + 2 * n;  // levels + 1 entries
+    int2* rb = level_rounds(L, n);
+    int* last = reinterpret_cast<int*>(stage);
+    for (int b = lane; b < x.nb; b += kWave) last[b] = 0;
+    __syncthreads();
+    int levels = 0;
+    for (int base = 0; base < n; base += kWave) {
+        const int r = base + lane;
+        int a = -1, bb = -1;
+        if (r < n) {
+            const float4* v = vbase + std::size_t(r) * prx::kVcStride;
+            const float4 m = v[8];
+            if (m.x != 0.0f || m.y != 0.0f) a = __float_as_int(v[2].w);
+            if (m.z != 0.0f || m.w != 0.0f) bb = __float_as_int(v[5].z);
+        }
+        const int cnt = (n - base < kWave) ? n - base : kWave;
+        for (int j = 0; j < cnt; ++j) {
+            const int aj = __shfl(a, j), bj = __shfl(bb, j);
+            if (lane == 0) {
+                const int la = (aj >= 0) ? last[aj] : 0;
+                const int lb = (bj >= 0) ? last[bj] : 0;
+                const int l = ((la > lb) ? la : lb) + 1;
+                if (aj >= 0) last[aj] = l;
+                if (bj >= 0) last[bj] = l;
+                lvl[base + j] = l - 1;
+                levels = (l > levels) ? l : levels;
+            }
+        }
+    }
+    levels = __shfl(levels, 0);
+    __syncthreads();
+    for (int k = lane; k <= levels; k += kWave) start[k] = 0;
+    __syncthreads();
+    for (int r = lane; r < n; r += kWave) atomicAdd(&start[lvl[r]], 1);
+    __syncthreads();
+    {
+        int carry = 0;
+        for (int base = 0; base <= levels; base += kWave) {
+            const int k = base + lane;
+            const int v = (k <= levels) ? start[k] : 0;
+            const int incl = wave_incl_scan(v, lane);
+            if (k <= levels) start[k] = carry + incl - v;
+            carry += __shfl(incl, kWave - 1);
+        }
+    }
+    __syncthreads();
+    int* cur = reinterpret_cast<int*>(rb);
+    for (int k = lane; k < levels; k += kWave) cur[k] = start[k];
+    __syncthreads();
+    for (int r = lane; r < n; r += kWave) order[atomicAdd(&cur[lvl[r]], 1)] = r;
+    __syncthreads();
+    int rounds = 0;
+    for (int base = 0; base < levels; base += kWave) {
+        const int k = base + lane;
+        const int cnt = (k < levels) ? start[k + 1] - start[k] : 0;
+        const int nr = (cnt + kWave - 1) / kWave;
+        const int incl = wave_incl_scan(nr, lane);
+        for (int i = 0; i < nr; ++i) {
+            const int b = start[k] + i * kWave;
+            const int e = (b + kWave < start[k + 1]) ? b + kWave : start[k + 1];
+            rb[rounds + incl - nr + i] = int2{b, e};
+        }
+        rounds += __shfl(incl, kWave - 1);
+    }
+    float4* v2 = d.s_vc2 + (std::size_t(x.co) + std::size_t(c0)) * prx::kVcStride;
+    float4* p2 = d.s_pc2 + (std::size_t(x.co) + std::size_t(c0)) * prx::kPcStride;
+    for (int k = lane; k < n * prx::kVcStride; k += kWave) {
+        const int pos = k / prx::kVcStride, f = k - pos * prx::kVcStride;
+        v2[k] = vbase[std::size_t(order[pos]) * prx::kVcStride + f];
+    }
+    for (int k = lane; k < n * prx::kPcStride; k += kWave) {
+        const int pos = k / prx::kPcStride, f = k - pos * prx::kPcStride;
+        p2[k] = pbase[std::size_t(order[pos]) * prx::kPcStride + f];
+    }
+    __syncthreads();
+    if (lane == 0) lvl[0] = rounds;  // levels are no longer needed
+    __syncthreads();
+    return LevelPlan{n, rounds, rb, order};
+}
+
+__device__ __forceinline__ LevelPlan level_plan_of(const DeviceState& d, const WorldCtx& x, int c0, int c1) {
+    const int n = c1 - c0;
+    int* L = level_ints(d, x, c0);
+    return LevelPlan{n, L[0], level_rounds(L, n), L + n};
+}
+
+// Warm start and velocity sweeps of one island in level form.  Round k+1's
+// records are loaded while round k is solved (a record is read and written
+// by its own update only, once per pass).
+template <bool kLds>
+__device__ __forceinline__ void island_velocity_level(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
+                                                      const LevelPlan& P, int c0, const StepParams& c) {
+    float4* vbase = d.s_vc + (std::size_t(x.co) + std::size_t(c0)) * prx::kVcStride;
+    float4* v2 = d.s_vc2 + (std::size_t(x.co) + std::size_t(c0)) * prx::kVcStride;
+    const int npass = int(c.velIters) + (c.warmStart ? 1 : 0);
+    auto fetch = [&](int2 r, float4 (&buf)[prx::kVcStride]) {
+        const int pos = r.x + x.lane;
+        if (pos < r.y) {
+            const float4* src = v2 + std::size_t(pos) * prx::kVcStride;
+#pragma unroll
+            for (int f = 0; f < prx::kVcStride; ++f) buf[f] = src[f];
+        }
+    };
+    for (int pass = 0; pass < npass; ++pass) {
+        const bool warm = c.warmStart && pass == 0;
+        float newInc = 0.0f;
+        float4 cur[prx::kVcStride], nxt[prx::kVcStride];
+        int2 r = (P.rounds > 0) ? P.rb[0] : int2{0, 0};
+        fetch(r, cur);
+        for (int k = 0; k < P.rounds; ++k) {
+            const int2 rn = (k + 1 < P.rounds) ? P.rb[k + 1] : int2{0, 0};
+            fetch(rn, nxt);
+            const int pos = r.x + x.lane;
+            if (pos < r.y) {
+                BodyC A, B;
+                Vc vc = load_vc(cur, A, B);
+                if (warm) {
+                    warm_start_one(vc, A, B, bs);
+                } else {
+                    const float mi = solve_velocity_one(vc, A, B, bs);
+                    float4* rec = v2 + std::size_t(pos) * prx::kVcStride;
+                    rec[4] = float4{vc.p[0].ni, vc.p[0].ti, vc.p[0].nm, vc.p[0].tm};
+                    rec[7] = float4{vc.p[1].ni, vc.p[1].ti, vc.p[1].nm, vc.p[1].tm};
+                    newInc = fmax_std(newInc, mi);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int f = 0; f < prx::kVcStride; ++f) cur[f] = nxt[f];
+            r = rn;
+        }
+        if (!warm) {
+#pragma unroll
+            for (int off = kWave / 2; off > 0; off >>= 1) newInc = fmax_std(newInc, __shfl_xor(newInc, off));
+            if (newInc <= c.regMinMomentum) break;
+        }
+    }
+    for (int pos = x.lane; pos < P.n; pos += kWave) {
+        float4* dst = vbase + std::size_t(P.order[pos]) * prx::kVcStride;
+        const float4* src = v2 + std::size_t(pos) * prx::kVcStride;
+        dst[4] = src[4];
+        dst[7] = src[7];
+    }
+    __syncthreads();
+}
+
+// Position sweeps of one island in level form (records prefetched a round
+// ahead); returns whether solved.
+template <bool kLds>
+__device__ __forceinline__ bool island_position_level(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
+                                                      const LevelPlan& P, int c0, const StepParams& c) {
+    const float4* p2 = d.s_pc2 + (std::size_t(x.co) + std::size_t(c0)) * prx::kPcStride;
+    auto fetch = [&](int2 r, float4 (&buf)[prx::kPcStride]) {
+        const int pos = r.x + x.lane;
+        if (pos < r.y) {
+            const float4* src = p2 + std::size_t(pos) * prx::kPcStride;
+#pragma unroll
+            for (int f = 0; f < prx::kPcStride; ++f) buf[f] = src[f];
+        }
+    };
+    for (std::uint32_t it = 0; it < c.posIters; ++it) {
+        float minSep = __builtin_inff();
+        float4 cur[prx::kPcStride], nxt[prx::kPcStride];
+        int2 r = (P.rounds > 0) ? P.rb[0] : int2{0, 0};
+        fetch(r, cur);
+        for (int k = 0; k < P.rounds; ++k) {
+            const int2 rn = (k + 1 < P.rounds) ? P.rb[k + 1] : int2{0, 0};
+            fetch(rn, nxt);
+            if (r.x + x.lane < r.y) {
+                const Pc pc = load_pc(cur);
+                float4 pa = bs.pos(pc.bA);
+                float4 pb = bs.pos(pc.bB);
+                const float sep = solve_pc(pc.m, pc.totalRadius, pc.A, pc.B, pa, pb, c);
+                bs.set_pos(pc.bA, pa);
+                bs.set_pos(pc.bB, pb);
+                minSep = fmin_std(minSep, sep);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int f = 0; f < prx::kPcStride; ++f) cur[f] = nxt[f];
+            r = rn;
+        }
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) minSep = fmin_std(minSep, __shfl_xor(minSep, off));
+        if (minSep >= c.regMinSeparation) return true;
+    }
+    return false;
+}
+
+__device__ __forceinline__ bool level_form(const WorldCtx& x, int c0, int c1, int j0, int j1) {
+    return j0 == j1 && x.nb <= kLevelMaxBodies && (c1 - c0) >= kLevelMinConstraints;
+}
+
 template <bool kLds, bool kTape = false>
 __device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, const WorldCtx& x, int nisl,
                                                        const IslandScratch& S, const StepParams& c, PhaseTimer& T) {
@@ -2059,8 +2290,13 @@ __device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, con
         const bool big = is < nisl && constraints(is) >= coopMin;
         for (unsigned long long m = __ballot(big); m; m &= m - 1) {
             const int isl = base + __ffsll(m) - 1;
-            island_velocity_wave<kLds>(d, x, bs, stage, S.istart[isl].y, S.istart[isl + 1].y, S.jstart[isl],
-                                       S.jstart[isl + 1], c);
+            const int c0 = S.istart[isl].y, c1 = S.istart[isl + 1].y;
+            if (level_form(x, c0, c1, S.jstart[isl], S.jstart[isl + 1])) {
+                const LevelPlan P = level_plan(d, x, stage, c0, c1);
+                island_velocity_level<kLds>(d, x, bs, P, c0, c);
+            } else {
+                island_velocity_wave<kLds>(d, x, bs, stage, c0, c1, S.jstart[isl], S.jstart[isl + 1], c);
+            }
         }
     }
     T.mark(prx::kPhVelocity);
@@ -2089,8 +2325,11 @@ __device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, con
         const bool big = is < nisl && constraints(is) >= coopMin;
         for (unsigned long long m = __ballot(big); m; m &= m - 1) {
             const int isl = base + __ffsll(m) - 1;
-            const bool solved = island_position_wave<kLds>(d, x, bs, stage, S.istart[isl].y, S.istart[isl + 1].y,
-                                                           S.jstart[isl], S.jstart[isl + 1], c);
+            const int c0 = S.istart[isl].y, c1 = S.istart[isl + 1].y;
+            const bool solved = level_form(x, c0, c1, S.jstart[isl], S.jstart[isl + 1])
+                                    ? island_position_level<kLds>(d, x, bs, level_plan_of(d, x, c0, c1), c0, c)
+                                    : island_position_wave<kLds>(d, x, bs, stage, c0, c1, S.jstart[isl],
+                                                                 S.jstart[isl + 1], c);
             if (solved && x.lane == 0) S.isl_word[isl] = 1u;
         }
     }
