@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PRX_ABI_VERSION 2
+#define PRX_ABI_VERSION 3
 
 /* Status codes (reference exception types in parentheses). */
 enum {
@@ -186,9 +186,69 @@ int prx_world_set_velocity(prx_world* world, uint32_t body, const float v[3]);
 int prx_shape_set_friction(prx_world* world, uint32_t shape, float friction);
 int prx_shape_set_restitution(prx_world* world, uint32_t shape, float restitution);
 /* One state record (bodies, contact list in world order with manifolds,
- * proxies) in the layout of the golden vectors. */
+ * proxies, joints, then the last step's contact listener calls as
+ * prx_world_contact_events gives them, none unless recorded) in the layout of
+ * the golden vectors. */
 int prx_world_state_record(prx_world* world, uint8_t* buf, size_t cap, size_t* len);
 uint32_t prx_world_contact_count(prx_world* world);
+
+/* Contact listeners.
+ * A contact's manifold as a pre-solve listener receives it: the manifold the
+ * contact had before the step's update (Manifold.hpp:36-120).  type: 0 unset,
+ * 1 circles, 2 face A, 3 face B; fields that the type or point count leaves
+ * unused are zero. */
+typedef struct prx_manifold {
+    uint32_t type, point_count;
+    float local_normal[2], local_point[2];
+    struct {
+        float local_point[2];
+        uint32_t contact_feature; /* type A | type B << 1 | index A << 8 | index B << 16 */
+        float normal_impulse, tangent_impulse;
+    } points[2];
+} prx_manifold;
+/* ContactImpulsesList (ContactImpulsesList.hpp:47-100): the impulses the
+ * solve applied at each point of the contact's velocity constraint. */
+typedef struct prx_contact_impulses {
+    uint32_t count;
+    float normal[2], tangent[2];
+} prx_contact_impulses;
+/* The four contact listeners of the reference (World.hpp:105-139:
+ * SetBeginContactListener, SetEndContactListener, SetPreSolveContactListener,
+ * SetPostSolveContactListener); null members are not called.  post_solve's
+ * solved_iteration is the position iteration that solved the contact's
+ * island, 255 (the 8-bit InvalidIteration) when none did
+ * (AabbTreeWorld.cpp:1671-1674).
+ *
+ * The calls, their order and their arguments are the reference's
+ * (AabbTreeWorld.cpp:2249-2252, 2703-2719, 1671-1674); they are recorded on
+ * the device during the step and delivered once it has completed, in call
+ * order, before prx_step / prx_batch_step return (a batch with listeners is
+ * stepped one step at a time).  A listener sees the world as it is after the
+ * step, and must not step or edit the world. */
+typedef struct prx_contact_listener {
+    void* user;
+    void (*begin_contact)(void* user, uint32_t contact);
+    void (*end_contact)(void* user, uint32_t contact);
+    void (*pre_solve)(void* user, uint32_t contact, const prx_manifold* old_manifold);
+    void (*post_solve)(void* user, uint32_t contact, const prx_contact_impulses* impulses,
+                       uint32_t solved_iteration);
+} prx_contact_listener;
+/* Sets (copies) or clears (null) the world's listeners; setting them turns
+ * on recording of the contact listener calls. */
+int prx_world_set_contact_listener(prx_world* world, const prx_contact_listener* listener);
+/* Turns recording of each step's contact listener calls on or off (the
+ * recorded calls also end each state record).  Not for a world inside a
+ * multi-world batch (PRX_E_WRONG_STATE): switch it before batching. */
+int prx_world_record_contact_events(prx_world* world, int on);
+/* The recorded calls of the world's last step: ncalls calls in nwords 32-bit
+ * words, each u32 kind (0 begin, 1 end, 2 pre-solve, 3 post-solve), u32
+ * contact id, then for pre-solve u32 type | point count << 8, two contact
+ * features and 12 floats (local normal, local point, per point local point,
+ * normal impulse, tangent impulse), for post-solve u32 point count, u32
+ * solved iteration and 4 floats (normal impulses [2], tangent impulses
+ * [2]).  words may be null to query the sizes. */
+int prx_world_contact_events(prx_world* world, uint32_t* words, size_t cap, size_t* nwords,
+                             uint32_t* ncalls);
 
 /* Batched worlds: all resident on the current device, stepped by one
  * persistent launch per step. */

@@ -207,6 +207,8 @@ void WriteState(const Handle<R>& h, Out& o) {
         const float f[4] = {float(j.impulse.x), float(j.impulse.y), float(j.impulse.z), float(j.motorImpulse)};
         o.put(f, sizeof(f));
     }
+    o.u32(w.nevents);
+    if (!w.events.empty()) o.put(w.events.data(), w.events.size() * 4);
 }
 
 struct AnyHandle {

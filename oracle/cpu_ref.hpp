@@ -853,9 +853,52 @@ public:
         return moved;
     }
 
+    // Contact listener calls of the current step in call order (the
+    // reference calls the begin / end / pre-solve / post-solve listeners at
+    // AabbTreeWorld.cpp:2249-2252, 2703-2719 and 1671-1674), serialised as
+    // u32 kind (0 begin, 1 end, 2 pre-solve, 3 post-solve), u32 contact id,
+    // then the pre-solve manifold (u32 type | count << 8, two contact
+    // features, 12 floats) or the post-solve impulses (u32 count, u32 solved
+    // position iteration, normal [2], tangent [2]).
+    std::vector<std::uint32_t> events;
+    std::uint32_t nevents = 0;
+    void EventHead(std::uint32_t kind, std::uint32_t cid) {
+        events.push_back(kind);
+        events.push_back(cid);
+        ++nevents;
+    }
+    static std::uint32_t FloatBits(R v) {
+        const float f = float(v);
+        std::uint32_t u;
+        std::memcpy(&u, &f, 4);
+        return u;
+    }
+    void EventManifold(const Manifold<R>& m) {
+        events.push_back(m.type | (m.n << 8));
+        events.push_back(m.n > 0 ? m.pts[0].cf : kDefaultCf);
+        events.push_back(m.n > 1 ? m.pts[1].cf : kDefaultCf);
+        R f[12] = {};
+        if (m.type == kFaceA || m.type == kFaceB) {
+            f[0] = m.ln.x;
+            f[1] = m.ln.y;
+        }
+        if (m.type != kUnset) {
+            f[2] = m.lpt.x;
+            f[3] = m.lpt.y;
+        }
+        for (std::uint32_t p = 0; p < m.n; ++p) {
+            f[4 + 4 * p + 0] = m.pts[p].lp.x;
+            f[4 + 4 * p + 1] = m.pts[p].lp.y;
+            f[4 + 4 * p + 2] = m.pts[p].ni;
+            f[4 + 4 * p + 3] = m.pts[p].ti;
+        }
+        for (R v : f) events.push_back(FloatBits(v));
+    }
+
     // InternalDestroy (AabbTreeWorld.cpp:2263-2288), called by DestroyContacts.
     void InternalDestroy(std::uint32_t cid) {
         Contact<R>& c = contactBuf[cid];
+        if (c.flags & kTouching) EventHead(1, cid);  // :2249-2252
         auto eraseFrom = [&](std::uint32_t body) {
             auto& lst = bodyContacts[body];
             for (auto it = lst.begin(); it != lst.end(); ++it) {
@@ -940,8 +983,17 @@ public:
             manifold = nm;
         }
         c.flags &= ~kNeedsUpdating;
-        if (!oldTouching && newTouching) c.flags |= kTouching;
-        else if (oldTouching && !newTouching) c.flags &= ~kTouching;
+        if (!oldTouching && newTouching) {
+            c.flags |= kTouching;
+            EventHead(0, cid);
+        } else if (oldTouching && !newTouching) {
+            c.flags &= ~kTouching;
+            EventHead(1, cid);
+        }
+        if (!(c.flags & kSensor) && newTouching) {
+            EventHead(2, cid);
+            EventManifold(old);
+        }
     }
 
     // UpdateContacts (AabbTreeWorld.cpp:2333-2421)
@@ -1679,6 +1731,7 @@ public:
             bc.pos = bc.pos + Pos<R>{tr, rot};
         }
         bool solved = false;
+        std::uint32_t posIters = conf.posIters;
         for (std::uint32_t i = 0; i < conf.posIters; ++i) {
             R minSep = std::numeric_limits<R>::infinity();
             for (const auto& pc : pcs) minSep = Min(minSep, SolvePC(pc, bcs, conf));
@@ -1687,6 +1740,7 @@ public:
             for (std::uint32_t jid : island.joints) jointsOkay &= JointSolvePosition(joints[jid], bcs, conf);
             if (contactsOkay && jointsOkay) {
                 solved = true;
+                posIters = i + 1;
                 break;
             }
         }
@@ -1706,6 +1760,20 @@ public:
                 b.xf = GetXf(b.pos1, b.lc);
                 for (const auto& kc : bodyContacts[bid]) contactBuf[kc.id].flags |= kNeedsUpdating;
             }
+        }
+        // Post-solve reports (AabbTreeWorld.cpp:1671-1674): the velocity
+        // constraints' impulses and the solved position iteration (255, the
+        // 8-bit InvalidIteration, if not solved).
+        for (std::size_t i = 0; i < vcs.size(); ++i) {
+            EventHead(3, island.contacts[i]);
+            events.push_back(vcs[i].n);
+            events.push_back(solved ? std::uint32_t(std::uint8_t(posIters - 1)) : 255u);
+            R f[4] = {};
+            for (std::uint32_t j = 0; j < vcs[i].n; ++j) {
+                f[j] = vcs[i].p[j].ni;
+                f[2 + j] = vcs[i].p[j].ti;
+            }
+            for (R v : f) events.push_back(FloatBits(v));
         }
         // UpdateUnderActiveTimes (AabbTreeWorld.cpp:403-417)
         R minUnder = std::numeric_limits<R>::infinity();
@@ -1759,6 +1827,8 @@ public:
     // Step (AabbTreeWorld.cpp:2144-2219) with doToi == false.
     StepStats Step(const StepConf<R>& conf) {
         StepStats st;
+        events.clear();
+        nevents = 0;
         for (const auto& fs : fixturesForProxies) CreateProxies(fs.first, fs.second, conf);
         fixturesForProxies.clear();
         st.preDestroyed = DestroyContacts();

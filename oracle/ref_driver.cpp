@@ -187,6 +187,69 @@ void WriteConf(Out& o, const StepConf& c) {
     o.u32((c.doWarmStart ? 1u : 0u) | (c.doToi ? 2u : 0u) | (c.doBlocksolve ? 4u : 0u));
 }
 
+// Contact listener calls of the last step, serialised in call order:
+// u32 kind (0 begin, 1 end, 2 pre-solve, 3 post-solve), u32 contact id, then
+// for pre-solve the old manifold (u32 type | count << 8, two packed contact
+// features, 12 floats: local normal, local point, per point local point and
+// normal / tangent impulse) and for post-solve u32 point count, u32 solved
+// position iteration, normal impulses [2], tangent impulses [2].
+struct EventLog {
+    Out o;
+    std::uint32_t n = 0;
+    void head(std::uint32_t kind, ContactID id) {
+        o.u32(kind);
+        o.u32(to_underlying(id));
+        ++n;
+    }
+};
+EventLog g_events;
+
+void PutManifold(Out& o, const Manifold& m) {
+    const auto npts = m.GetPointCount();
+    const auto type = m.GetType();
+    std::uint32_t cf[2] = {PackCf(ContactFeature{}), PackCf(ContactFeature{})};
+    float f[12] = {};
+    if (type == Manifold::e_faceA || type == Manifold::e_faceB) {
+        f[0] = GetX(m.GetLocalNormal());
+        f[1] = GetY(m.GetLocalNormal());
+    }
+    if (type != Manifold::e_unset) {
+        f[2] = GetX(m.GetLocalPoint());
+        f[3] = GetY(m.GetLocalPoint());
+    }
+    for (auto p = 0u; p < npts; ++p) {
+        const auto mp = m.GetPoint(p);
+        cf[p] = PackCf(mp.contactFeature);
+        f[4 + 4 * p + 0] = GetX(mp.localPoint);
+        f[4 + 4 * p + 1] = GetY(mp.localPoint);
+        f[4 + 4 * p + 2] = mp.normalImpulse;
+        f[4 + 4 * p + 3] = mp.tangentImpulse;
+    }
+    const std::uint32_t t[3] = {std::uint32_t(type) | (std::uint32_t(npts) << 8), cf[0], cf[1]};
+    o.put(t, sizeof(t));
+    o.put(f, sizeof(f));
+}
+
+void InstallListeners(World& world) {
+    SetBeginContactListener(world, [](ContactID id) { g_events.head(0, id); });
+    SetEndContactListener(world, [](ContactID id) { g_events.head(1, id); });
+    SetPreSolveContactListener(world, [](ContactID id, const Manifold& m) {
+        g_events.head(2, id);
+        PutManifold(g_events.o, m);
+    });
+    SetPostSolveContactListener(world, [](ContactID id, const ContactImpulsesList& l, unsigned iters) {
+        g_events.head(3, id);
+        g_events.o.u32(std::uint32_t(l.GetCount()));
+        g_events.o.u32(std::uint32_t(iters));
+        float f[4] = {};
+        for (auto i = 0u; i < l.GetCount(); ++i) {
+            f[i] = l.GetEntryNormal(i);
+            f[2 + i] = l.GetEntryTanget(i);
+        }
+        g_events.o.put(f, sizeof(f));
+    });
+}
+
 void WriteState(Out& o, const World& world, std::uint32_t step, const StepStats& st) {
     o.u32(step);
     const std::uint32_t stats[8] = {st.pre.proxiesMoved, st.pre.contactsDestroyed,
@@ -268,6 +331,10 @@ void WriteState(Out& o, const World& world, std::uint32_t step, const StepStats&
         const float f[4] = {GetX(j.impulse), GetY(j.impulse), GetZ(j.impulse), j.angularMotorImpulse};
         o.put(f, sizeof(f));
     }
+    o.u32(g_events.n);
+    o.put(g_events.o.buf.data(), g_events.o.buf.size());
+    g_events.o.buf.clear();
+    g_events.n = 0;
 }
 
 }  // namespace
@@ -398,12 +465,13 @@ int main(int argc, char** argv) {
     }
 
     auto world = BuildWorld(spec);
+    InstallListeners(world);
 
     const auto conf = RegularStepConf(doToi);
 
     Out o;
     o.put("PRXD", 4);
-    o.u32(2);
+    o.u32(3);
     WriteDescription(o, world);
     WriteConf(o, conf);
     o.u32(std::uint32_t(steps + 1));

@@ -52,6 +52,70 @@ class StepConf(ctypes.Structure):
             setattr(self, k, v)
 
 
+class ManifoldPoint(ctypes.Structure):
+    _fields_ = [("local_point", ctypes.c_float * 2), ("contact_feature", ctypes.c_uint32),
+                ("normal_impulse", ctypes.c_float), ("tangent_impulse", ctypes.c_float)]
+
+
+class Manifold(ctypes.Structure):
+    """The manifold a pre-solve listener receives (Manifold.hpp:36-120)."""
+    _fields_ = [("type", ctypes.c_uint32), ("point_count", ctypes.c_uint32),
+                ("local_normal", ctypes.c_float * 2), ("local_point", ctypes.c_float * 2),
+                ("points", ManifoldPoint * 2)]
+
+
+class ContactImpulses(ctypes.Structure):
+    """ContactImpulsesList (ContactImpulsesList.hpp:47-100)."""
+    _fields_ = [("count", ctypes.c_uint32), ("normal", ctypes.c_float * 2), ("tangent", ctypes.c_float * 2)]
+
+
+_BeginEndFn = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint32)
+_PreSolveFn = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(Manifold))
+_PostSolveFn = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ContactImpulses),
+                                ctypes.c_uint32)
+
+
+class ContactListener(ctypes.Structure):
+    _fields_ = [("user", ctypes.c_void_p), ("begin_contact", _BeginEndFn), ("end_contact", _BeginEndFn),
+                ("pre_solve", _PreSolveFn), ("post_solve", _PostSolveFn)]
+
+
+# Listener call kinds and their record sizes in 32-bit words.
+EV_BEGIN, EV_END, EV_PRE_SOLVE, EV_POST_SOLVE = 0, 1, 2, 3
+_EV_WORDS = (2, 2, 17, 8)
+
+
+def parse_contact_events(words, ncalls):
+    """Listener calls from their recorded words, in call order:
+    ("begin", id), ("end", id), ("pre_solve", id, manifold dict),
+    ("post_solve", id, (count, normal impulses, tangent impulses), iteration)."""
+    w = np.asarray(words, dtype=np.uint32)
+    f = w.view(np.float32)
+    out, o = [], 0
+    for _ in range(ncalls):
+        kind, cid = int(w[o]), int(w[o + 1])
+        if kind == EV_BEGIN:
+            out.append(("begin", cid))
+        elif kind == EV_END:
+            out.append(("end", cid))
+        elif kind == EV_PRE_SOLVE:
+            n = int((w[o + 2] >> 8) & 0xff)
+            pts = [dict(local_point=(float(f[o + 9 + 4 * q]), float(f[o + 10 + 4 * q])),
+                        contact_feature=int(w[o + 3 + q]), normal_impulse=float(f[o + 11 + 4 * q]),
+                        tangent_impulse=float(f[o + 12 + 4 * q])) for q in range(n)]
+            out.append(("pre_solve", cid, dict(type=int(w[o + 2] & 0xff), point_count=n,
+                                                local_normal=(float(f[o + 5]), float(f[o + 6])),
+                                                local_point=(float(f[o + 7]), float(f[o + 8])), points=pts)))
+        elif kind == EV_POST_SOLVE:
+            n = int(w[o + 2])
+            out.append(("post_solve", cid, (n, tuple(float(x) for x in f[o + 4:o + 4 + n]),
+                                            tuple(float(x) for x in f[o + 6:o + 6 + n])), int(w[o + 3])))
+        else:
+            raise ValueError("unknown listener call kind %d" % kind)
+        o += _EV_WORDS[kind]
+    return out
+
+
 class WorldConf(ctypes.Structure):
     _fields_ = [("abi_version", ctypes.c_uint32), ("min_vertex_radius", ctypes.c_float),
                 ("max_vertex_radius", ctypes.c_float), ("contact_capacity_per_proxy", ctypes.c_uint32)]
@@ -152,6 +216,10 @@ _SIGS = {
     "prx_adjoint_use_general": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "prx_adjoint_joint_gradients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "prx_adjoint_diagnostics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "prx_world_record_contact_events": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "prx_world_set_contact_listener": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "prx_world_contact_events": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                                ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_uint32)]),
     "prx_big_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     "prx_big_destroy": (None, [ctypes.c_void_p]),
     "prx_big_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
@@ -301,6 +369,45 @@ class World:
 
     def contact_count(self):
         return lib().prx_world_contact_count(self.h)
+
+    def record_contact_events(self, on=True):
+        """Records each step's contact listener calls (before batching the world)."""
+        _check(lib().prx_world_record_contact_events(self.h, 1 if on else 0))
+
+    def contact_event_words(self):
+        """Raw recorded listener calls of the last step: (uint32 words, calls)."""
+        nw, nc = ctypes.c_size_t(), ctypes.c_uint32()
+        _check(lib().prx_world_contact_events(self.h, None, 0, ctypes.byref(nw), ctypes.byref(nc)))
+        out = np.zeros(max(1, nw.value), dtype=np.uint32)
+        _check(lib().prx_world_contact_events(self.h, out.ctypes.data, out.size, ctypes.byref(nw),
+                                              ctypes.byref(nc)))
+        return out[:nw.value], nc.value
+
+    def contact_events(self):
+        """The last step's listener calls, parsed (see parse_contact_events)."""
+        return parse_contact_events(*self.contact_event_words())
+
+    def set_contact_listener(self, begin=None, end=None, pre_solve=None, post_solve=None):
+        """SetBeginContactListener / SetEndContactListener /
+        SetPreSolveContactListener / SetPostSolveContactListener (World.hpp:105-139):
+        begin(id), end(id), pre_solve(id, Manifold), post_solve(id, ContactImpulses,
+        solved_iteration), called through the C ABI after each step in the
+        order the step made them.  All None clears the listeners."""
+        if begin is None and end is None and pre_solve is None and post_solve is None:
+            _check(lib().prx_world_set_contact_listener(self.h, None))
+            self._listener = None
+            return
+        L = ContactListener()  # members left unset are null
+        if begin:
+            L.begin_contact = _BeginEndFn(lambda u, c: begin(c))
+        if end:
+            L.end_contact = _BeginEndFn(lambda u, c: end(c))
+        if pre_solve:
+            L.pre_solve = _PreSolveFn(lambda u, c, m: pre_solve(c, m.contents))
+        if post_solve:
+            L.post_solve = _PostSolveFn(lambda u, c, im, it: post_solve(c, im.contents, it))
+        self._listener = L  # the callbacks live as long as the world uses them
+        _check(lib().prx_world_set_contact_listener(self.h, ctypes.byref(L)))
 
 
 class Batch:

@@ -50,6 +50,8 @@ __device__ __forceinline__ WorldCtx big_ctx(const DeviceState& d) {
     x.cap = H.contact_cap;
     x.jo = 0;
     x.nj = 0;
+    x.eo = 0;
+    x.ecap = 0;
     x.lane = int(threadIdx.x) & (kWave - 1);
     return x;
 }

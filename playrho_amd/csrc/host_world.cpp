@@ -302,6 +302,11 @@ struct prx_world {
     bool host_stale = false;     // device state newer than this mirror
     prx_batch* solo = nullptr;   // single-world batch used by prx_step
     prx_step_stats last_stats{};
+    bool record_events = false;  // the device records the contact listener calls of each step
+    bool has_listener = false;
+    prx_contact_listener listener{};
+    std::vector<std::uint32_t> events;  // listener calls of the last step (device word format)
+    std::uint32_t nevents = 0;
 };
 
 namespace {
@@ -340,7 +345,7 @@ struct prx_batch {
     hipGraph_t graph = nullptr;
     prx::StepParams graph_params{};
     bool graph_valid = false;
-    std::size_t totB = 0, totP = 0, totS = 0, totV = 0, totC = 0, totJ = 0;
+    std::size_t totB = 0, totP = 0, totS = 0, totV = 0, totC = 0, totJ = 0, totE = 0;
     int lds_bytes = 16 * 1024;
     int lds_min = 16 * 1024;
     ~prx_batch() {
@@ -666,6 +671,16 @@ int BuildBatch(prx_batch* b) {
         for (const auto& s : w->shapes) V += s.v.size();
         C += std::size_t(h.contact_cap);
     }
+    // Listener-call segments of the worlds that record them (the most words
+    // a step can write for the world's contact capacity).
+    std::size_t E = 0;
+    for (int i = 0; i < nw; ++i) {
+        auto& h = b->hdr[std::size_t(i)];
+        h.ev_off = int(E);
+        h.ev_cap = b->worlds[std::size_t(i)]->record_events ? prx::kEvWordsPerSlot * h.contact_cap + 16 : 0;
+        E += std::size_t(h.ev_cap);
+        if (E > std::size_t(0x7fffffff)) return Fail(PRX_E_LENGTH, "contact listener records exceed 2^31 words");
+    }
     int maxNb = 0;
     for (const prx_world* w : b->worlds) maxNb = std::max<int>(maxNb, int(w->bodies.size()));
     // Body constraint state (32 B per body) plus the record stage; worlds whose
@@ -678,6 +693,7 @@ int BuildBatch(prx_batch* b) {
     b->totV = V;
     b->totC = C;
     b->totJ = J;
+    b->totE = E;
     auto& a = b->arena;
     auto& d = b->d;
     d.nworlds = nw;
@@ -743,6 +759,8 @@ int BuildBatch(prx_batch* b) {
     PRX_HIP(a.alloc(d.s_isl_joint, J));
     PRX_HIP(a.alloc(d.s_isl_jstart, B + std::size_t(nw)));
     PRX_HIP(a.alloc(d.s_isl_j, J));
+    d.ev = nullptr;
+    if (E) PRX_HIP(a.alloc(d.ev, E));
     d.schedule = b->d_schedule;
     return ApplySchedule(b);
 }
@@ -845,6 +863,18 @@ int UploadBatch(prx_batch* b) {
         for (std::size_t k = 0; k < hc.freelist.size(); ++k) cfree[std::size_t(h.contact_off) + k] = hc.freelist[k];
     }
     auto& d = b->d;
+    // Listener calls of the last step, as the host mirror holds them.
+    for (std::size_t wi = 0; wi < b->worlds.size(); ++wi) {
+        const prx_world* w = b->worlds[wi];
+        auto& h = b->hdr[wi];
+        h.ev_words = 0;
+        h.ev_count = 0;
+        if (h.ev_cap == 0 || w->events.size() > std::size_t(h.ev_cap)) continue;
+        h.ev_words = int(w->events.size());
+        h.ev_count = int(w->nevents);
+        if (!w->events.empty())
+            PRX_HIP(hipMemcpy(d.ev + h.ev_off, w->events.data(), w->events.size() * 4, hipMemcpyHostToDevice));
+    }
     int st;
     if ((st = Upload(d.hdr, b->hdr)) || (st = Upload(d.bflags, bflags)) || (st = Upload(d.pos0, pos0)) ||
         (st = Upload(d.pos1, pos1)) || (st = Upload(d.vel, vel)) || (st = Upload(d.xf, xf)) ||
@@ -868,6 +898,20 @@ template <class T>
 int Download(std::vector<T>& dst, const T* src, std::size_t n) {
     dst.resize(n);
     if (n) PRX_HIP(hipMemcpy(dst.data(), src, n * sizeof(T), hipMemcpyDeviceToHost));
+    return PRX_OK;
+}
+
+// Copies the listener calls of a world's last step to its host mirror.
+int FetchEvents(prx_batch* b, int wi, const prx::WorldHeader& h) {
+    prx_world* w = b->worlds[std::size_t(wi)];
+    if (h.ev_cap == 0) {
+        w->events.clear();
+        w->nevents = 0;
+        return PRX_OK;
+    }
+    const std::size_t words = std::size_t(std::min(h.ev_words, h.ev_cap));
+    if (int st = Download(w->events, b->d.ev + h.ev_off, words)) return st;
+    w->nevents = h.ev_words <= h.ev_cap ? std::uint32_t(h.ev_count) : 0u;
     return PRX_OK;
 }
 
@@ -949,6 +993,7 @@ int DownloadBatch(prx_batch* b) {
         }
         hc.freelist.assign(cfree.begin() + h.contact_off, cfree.begin() + h.contact_off + h.nfree);
         hc.pool = h.pool_size;
+        if (int st2 = FetchEvents(b, int(wi), h)) return st2;
         const std::uint32_t* s = h.stats;
         w->last_stats = prx_step_stats{s[0], s[1], s[2], s[3], s[4], s[5], s[6], s[7]};
         w->host_stale = false;
@@ -992,6 +1037,8 @@ int CheckStatus(prx_batch* b) {
             return Fail(PRX_E_LENGTH, "contact capacity of a world exceeded (raise contact_capacity_per_proxy)");
         if (h.status & prx::kStatusScratchOverflow)
             return Fail(PRX_E_LENGTH, "candidate pair scratch of a world exceeded");
+        if (h.status & prx::kStatusEventOverflow)
+            return Fail(PRX_E_LENGTH, "contact listener record of a world exceeded");
     }
     return PRX_OK;
 }
@@ -1034,6 +1081,79 @@ int StepBatch(prx_batch* b, const prx_step_conf* conf, int nsteps) {
     for (prx_world* w : b->worlds)
         if (w) w->host_stale = true;
     return PRX_OK;
+}
+
+// Calls the world's listeners with the recorded calls of its last step, in
+// the order the step made them.
+int DispatchEvents(prx_world* w) {
+    const prx_contact_listener& L = w->listener;
+    const std::vector<std::uint32_t>& e = w->events;
+    std::size_t o = 0;
+    auto f32 = [&](std::size_t i) {
+        float v;
+        std::memcpy(&v, &e[i], 4);
+        return v;
+    };
+    for (std::uint32_t i = 0; i < w->nevents; ++i) {
+        if (o + 2 > e.size()) return Fail(PRX_E_DEVICE, "truncated contact listener record");
+        const std::uint32_t kind = e[o], cid = e[o + 1];
+        const std::size_t n = kind == prx::kEvPreSolve ? prx::kEvPreWords : kind == prx::kEvPostSolve ? prx::kEvPostWords : 2;
+        if (kind > prx::kEvPostSolve || o + n > e.size()) return Fail(PRX_E_DEVICE, "malformed contact listener record");
+        if (kind == prx::kEvBegin) {
+            if (L.begin_contact) L.begin_contact(L.user, cid);
+        } else if (kind == prx::kEvEnd) {
+            if (L.end_contact) L.end_contact(L.user, cid);
+        } else if (kind == prx::kEvPreSolve) {
+            prx_manifold m{};
+            m.type = e[o + 2] & 0xffu;
+            m.point_count = (e[o + 2] >> 8) & 0xffu;
+            m.local_normal[0] = f32(o + 5);
+            m.local_normal[1] = f32(o + 6);
+            m.local_point[0] = f32(o + 7);
+            m.local_point[1] = f32(o + 8);
+            for (int q = 0; q < 2; ++q) {
+                m.points[q].contact_feature = e[o + 3 + std::size_t(q)];
+                m.points[q].local_point[0] = f32(o + 9 + 4 * std::size_t(q));
+                m.points[q].local_point[1] = f32(o + 10 + 4 * std::size_t(q));
+                m.points[q].normal_impulse = f32(o + 11 + 4 * std::size_t(q));
+                m.points[q].tangent_impulse = f32(o + 12 + 4 * std::size_t(q));
+            }
+            if (L.pre_solve) L.pre_solve(L.user, cid, &m);
+        } else {
+            prx_contact_impulses im{};
+            im.count = e[o + 2];
+            im.normal[0] = f32(o + 4);
+            im.normal[1] = f32(o + 5);
+            im.tangent[0] = f32(o + 6);
+            im.tangent[1] = f32(o + 7);
+            if (L.post_solve) L.post_solve(L.user, cid, &im, e[o + 3]);
+        }
+        o += n;
+    }
+    return PRX_OK;
+}
+
+// After a synchronised step: the listener calls of every world of the batch
+// that has listeners, delivered world by world.
+int DeliverEvents(prx_batch* b) {
+    bool any = false;
+    for (const prx_world* w : b->worlds) any = any || (w && w->has_listener);
+    if (!any) return PRX_OK;
+    std::vector<prx::WorldHeader> hdr;
+    if (int st = Download(hdr, b->d.hdr, b->worlds.size())) return st;
+    for (std::size_t wi = 0; wi < b->worlds.size(); ++wi) {
+        prx_world* w = b->worlds[wi];
+        if (!w || !w->has_listener) continue;
+        if (int st = FetchEvents(b, int(wi), hdr[wi])) return st;
+        if (int st = DispatchEvents(w)) return st;
+    }
+    return PRX_OK;
+}
+
+bool HasListener(const prx_batch* b) {
+    for (const prx_world* w : b->worlds)
+        if (w && w->has_listener) return true;
+    return false;
 }
 
 }  // namespace
@@ -1396,6 +1516,7 @@ int prx_step(prx_world* w, const prx_step_conf* conf, prx_step_stats* stats) {
     if (int st = StepBatch(w->solo, conf, 1)) return st;
     PRX_HIP(hipStreamSynchronize(w->solo->stream));
     if (int st = CheckStatus(w->solo)) return st;
+    if (int st = DeliverEvents(w->solo)) return st;
     if (stats) {
         prx::WorldHeader h;
         PRX_HIP(hipMemcpy(&h, w->solo->d.hdr, sizeof(h), hipMemcpyDeviceToHost));
@@ -1511,6 +1632,47 @@ std::uint32_t prx_world_contact_count(prx_world* w) {
     return std::uint32_t(w->contacts.size());
 }
 
+int prx_world_record_contact_events(prx_world* w, int on) {
+    if (int st = CheckWorld(w)) return st;
+    const bool want = on != 0;
+    if (want == w->record_events) return PRX_OK;
+    if (w->owner && w->owner != w->solo)
+        return Fail(PRX_E_WRONG_STATE, "the world is in a batch: switch contact event recording before batching it");
+    if (int st = Detach(w)) return st;  // prx_step rebuilds its single-world batch with the new segment
+    w->record_events = want;
+    if (!want) {
+        w->events.clear();
+        w->nevents = 0;
+    }
+    return PRX_OK;
+}
+
+int prx_world_set_contact_listener(prx_world* w, const prx_contact_listener* listener) {
+    if (int st = CheckWorld(w)) return st;
+    if (listener) {
+        if (int st = prx_world_record_contact_events(w, 1)) return st;
+        w->listener = *listener;
+        w->has_listener = true;
+    } else {
+        w->listener = prx_contact_listener{};
+        w->has_listener = false;
+    }
+    return PRX_OK;
+}
+
+int prx_world_contact_events(prx_world* w, std::uint32_t* words, std::size_t cap, std::size_t* nwords,
+                             std::uint32_t* ncalls) {
+    if (int st = CheckWorld(w)) return st;
+    if (int st = Refresh(w)) return st;
+    if (nwords) *nwords = w->events.size();
+    if (ncalls) *ncalls = w->nevents;
+    if (words) {
+        if (cap < w->events.size()) return Fail(PRX_E_LENGTH, "buffer too small");
+        if (!w->events.empty()) std::memcpy(words, w->events.data(), w->events.size() * 4);
+    }
+    return PRX_OK;
+}
+
 int prx_world_state_record(prx_world* w, std::uint8_t* buf, std::size_t cap, std::size_t* len) {
     if (int st = CheckWorld(w)) return st;
     if (int st = Refresh(w)) return st;
@@ -1579,6 +1741,8 @@ int prx_world_state_record(prx_world* w, std::uint8_t* buf, std::size_t cap, std
         u32(j.limit);
         put(j.imp, sizeof(j.imp));
     }
+    u32(w->nevents);
+    if (!w->events.empty()) put(w->events.data(), w->events.size() * 4);
     if (len) *len = o.size();
     if (buf && cap >= o.size()) std::memcpy(buf, o.data(), o.size());
     else if (buf) return Fail(PRX_E_LENGTH, "buffer too small");
@@ -1603,7 +1767,16 @@ void prx_batch_destroy(prx_batch* b) {
 
 int prx_batch_step(prx_batch* b, const prx_step_conf* conf, int nsteps) {
     if (!b) return Fail(PRX_E_INVALID_ARGUMENT, "null batch");
-    return StepBatch(b, conf, nsteps);
+    if (!HasListener(b)) return StepBatch(b, conf, nsteps);
+    // Worlds with listeners: one step at a time, each step's calls delivered
+    // before the next step.
+    for (int s = 0; s < nsteps; ++s) {
+        if (int st = StepBatch(b, conf, 1)) return st;
+        PRX_HIP(hipStreamSynchronize(b->stream));
+        if (int st = CheckStatus(b)) return st;
+        if (int st = DeliverEvents(b)) return st;
+    }
+    return PRX_OK;
 }
 
 int prx_batch_sync(prx_batch* b) {
@@ -1824,6 +1997,8 @@ int prx_big_create(prx_world* w, prx_big** out) {
     if (int st = CheckWorld(w)) return st;
     if (!out) return Fail(PRX_E_INVALID_ARGUMENT, "null out");
     if (!w->joints.empty()) return Fail(PRX_E_UNSUPPORTED, "joints are not stepped by the coloured large-world mode");
+    if (w->record_events)
+        return Fail(PRX_E_UNSUPPORTED, "contact listener calls are recorded by the batched step only");
     if (w->proxies.empty()) return Fail(PRX_E_INVALID_ARGUMENT, "world has no shapes");
     prx_batch_conf bc;
     prx_batch_conf_default(&bc);
@@ -1972,11 +2147,14 @@ namespace {
 std::vector<std::pair<void*, std::size_t>> DynamicArrays(prx_batch* b) {
     const std::size_t B = b->totB, P = b->totP, C = b->totC, J = b->totJ, nw = b->worlds.size();
     auto& d = b->d;
-    return {{d.hdr, nw * sizeof(prx::WorldHeader)}, {d.bflags, B * 4}, {d.pos0, B * 16}, {d.pos1, B * 16},
-            {d.vel, B * 16}, {d.xf, B * 16}, {d.paabb, P * 16}, {d.pflags, P * 4}, {d.c_prox, C * 8},
-            {d.c_body, C * 8}, {d.c_shape, C * 8}, {d.c_id, C * 4}, {d.c_flags, C * 4}, {d.c_mat, C * 16},
-            {d.m_hdr, C * 16}, {d.m_geo, C * 16}, {d.m_p0, C * 16}, {d.m_p1, C * 16}, {d.c_free, C * 4},
-            {d.j_imp, J * 16}, {d.j_limit, J * 4}};
+    std::vector<std::pair<void*, std::size_t>> v = {
+        {d.hdr, nw * sizeof(prx::WorldHeader)}, {d.bflags, B * 4}, {d.pos0, B * 16}, {d.pos1, B * 16},
+        {d.vel, B * 16}, {d.xf, B * 16}, {d.paabb, P * 16}, {d.pflags, P * 4}, {d.c_prox, C * 8},
+        {d.c_body, C * 8}, {d.c_shape, C * 8}, {d.c_id, C * 4}, {d.c_flags, C * 4}, {d.c_mat, C * 16},
+        {d.m_hdr, C * 16}, {d.m_geo, C * 16}, {d.m_p0, C * 16}, {d.m_p1, C * 16}, {d.c_free, C * 4},
+        {d.j_imp, J * 16}, {d.j_limit, J * 4}};
+    if (b->totE) v.push_back({d.ev, b->totE * 4});  // listener calls of the last step
+    return v;
 }
 
 }  // namespace

@@ -49,7 +49,7 @@ enum : std::uint32_t {
 // Proxy flags.
 enum : std::uint32_t { kProxyPending = 1u, kProxyMoved = 2u };
 // World status bits.
-enum : std::uint32_t { kStatusContactOverflow = 1u, kStatusScratchOverflow = 2u };
+enum : std::uint32_t { kStatusContactOverflow = 1u, kStatusScratchOverflow = 2u, kStatusEventOverflow = 4u };
 
 // Per-world header (AoS: read once per phase by the world's workgroup).
 struct WorldHeader {
@@ -57,6 +57,10 @@ struct WorldHeader {
     std::int32_t contact_off, contact_cap, ncontacts, pool_size;
     std::int32_t nfree, status, joint_off, njoints;
     std::uint32_t stats[8];
+    // Contact listener calls of the last step: word segment [ev_off, ev_off +
+    // ev_cap) of DeviceState::ev (ev_cap 0: not recorded), words and calls
+    // written (words may exceed ev_cap: then the status has kStatusEventOverflow).
+    std::int32_t ev_off, ev_cap, ev_words, ev_count;
 };
 
 struct StepParams {
@@ -201,6 +205,16 @@ struct DeviceState {
     // Optional per-world phase timings of the last step (shader clock cycles),
     // kPhases entries per world; null when not collected.
     unsigned long long* phase_cycles;
+    // Contact listener calls of the last step, per world segment (see
+    // WorldHeader::ev_off); null when no world records them.  Each call is
+    // u32 kind (kEvBegin, kEvEnd, kEvPreSolve, kEvPostSolve), u32 contact id,
+    // then for pre-solve the contact's manifold before the update (u32 type |
+    // count << 8, two contact features, 12 floats: local normal, local point,
+    // per point local point, normal and tangent impulse; zero where the type
+    // or count has none), for post-solve u32 point count, u32 solved position
+    // iteration (255 when not solved) and 4 floats (normal impulses [2],
+    // tangent impulses [2]).
+    std::uint32_t* ev;
     // Solve record for the adjoint (only read by the recording kernel).
     Tape tape;
 };
@@ -210,6 +224,14 @@ enum Phase : int {
     kPhProxies = 0, kPhDestroy, kPhUpdate, kPhFindAdd, kPhAdjacency, kPhIslands, kPhConstraints,
     kPhVelocity, kPhPosition, kPhWriteback, kPhSync, kPhUpdate2, kPhFindAdd2, kPhTotal, kPhases = 16
 };
+
+enum : std::uint32_t { kEvBegin = 0u, kEvEnd = 1u, kEvPreSolve = 2u, kEvPostSolve = 3u };
+constexpr int kEvPreWords = 17;
+constexpr int kEvPostWords = 8;
+// Most event words one contact slot can produce in a step: end on destroy (2),
+// update or creation before the solve (2 + 17), post-solve (8), update or
+// creation after the solve (2 + 17).
+constexpr int kEvWordsPerSlot = 48;
 
 constexpr int kVcStride = 9;
 constexpr int kPcStride = 5;

@@ -42,6 +42,15 @@ __device__ __forceinline__ unsigned long long lanes_below(int lane) {
     return (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 }
 
+// Inclusive wave scan of a 64-lane value.
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int t = __shfl_up(v, o);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
 __device__ __forceinline__ Proxy shape_proxy(const DeviceState& d, int s) {
     const int4 si = d.shape_i[s];
     const float4 sf = d.shape_f[s];
@@ -137,8 +146,12 @@ __device__ __forceinline__ void store_manifold(const DeviceState& d, int gi, con
 }
 
 // Update (AabbTreeWorld.cpp:2587-2721): new manifold, warm-start impulses
-// carried over by ContactFeature, else from the nearest old point.
-__device__ __forceinline__ void update_contact(const DeviceState& d, int bo, int gi, const StepParams& c) {
+// carried over by ContactFeature, else from the nearest old point.  Returns
+// the listener calls the update makes (:2703-2719): bit 0 begin-contact, bit 1
+// end-contact, bit 2 pre-solve with the manifold before the update (stored
+// to oldOut when given).
+__device__ __forceinline__ std::uint32_t update_contact(const DeviceState& d, int bo, int gi, const StepParams& c,
+                                                        Manifold* oldOut = nullptr) {
     std::uint32_t flags = d.c_flags[gi];
     const Manifold old = load_manifold(d, gi);
     const bool oldTouching = (flags & prx::kTouching) != 0;
@@ -186,30 +199,101 @@ __device__ __forceinline__ void update_contact(const DeviceState& d, int bo, int
     }
     store_manifold(d, gi, nm);
     flags &= ~prx::kNeedsUpdating;
-    if (!oldTouching && newTouching) flags |= prx::kTouching;
-    else if (oldTouching && !newTouching) flags &= ~prx::kTouching;
+    std::uint32_t calls = 0;
+    if (!oldTouching && newTouching) {
+        flags |= prx::kTouching;
+        calls = 1u;
+    } else if (oldTouching && !newTouching) {
+        flags &= ~prx::kTouching;
+        calls = 2u;
+    }
+    if (newTouching && !(flags & prx::kSensor)) calls |= 4u;
     d.c_flags[gi] = flags;
+    if (oldOut) *oldOut = old;
+    return calls;
 }
 
 struct WorldCtx {
     int w, bo, nb, po, np, co, cap, jo, nj;
+    int eo, ecap;  // listener-call words of the world (ecap 0: not recorded)
     int lane;
 };
 
 struct Counters {
     int nc, nfree, pool;
     std::uint32_t status;
+    int evw, evn;  // listener-call words and calls of the step so far
 };
 
+// Listener calls are appended wave-wide in call order: every lane passes the
+// words and calls it appends and gets the first word of its own, behind those
+// of the lanes below it, or -1 (nothing to write, or past the world's
+// segment: the step then reports kStatusEventOverflow).
+__device__ __forceinline__ int ev_claim(const WorldCtx& x, Counters& k, int words, int calls) {
+    const int incl = wave_incl_scan(words, x.lane);
+    const int cincl = wave_incl_scan(calls, x.lane);
+    const int at = k.evw + incl - words;
+    k.evw += __shfl(incl, kWave - 1);
+    k.evn += __shfl(cincl, kWave - 1);
+    const bool over = words > 0 && at + words > x.ecap;
+    if (__ballot(over)) k.status |= prx::kStatusEventOverflow;
+    return (words > 0 && !over) ? x.eo + at : -1;
+}
+
+// The listener calls of one contact update (update_contact's bits), the
+// pre-solve call with the manifold as it was before the update.
+__device__ __forceinline__ void ev_update(const DeviceState& d, const WorldCtx& x, Counters& k, std::uint32_t calls,
+                                          std::uint32_t cid, const Manifold& old) {
+    const bool touch = (calls & 3u) != 0, pre = (calls & 4u) != 0;
+    const int at = ev_claim(x, k, (touch ? 2 : 0) + (pre ? prx::kEvPreWords : 0), (touch ? 1 : 0) + (pre ? 1 : 0));
+    if (at < 0) return;
+    std::uint32_t* o = d.ev + at;
+    if (touch) {
+        o[0] = (calls & 1u) ? prx::kEvBegin : prx::kEvEnd;
+        o[1] = cid;
+        o += 2;
+    }
+    if (pre) {
+        const bool face = old.type == kFaceA || old.type == kFaceB;
+        const bool set = old.type != kUnset;
+        o[0] = prx::kEvPreSolve;
+        o[1] = cid;
+        o[2] = old.type | (old.n << 8);
+        o[3] = old.n > 0 ? old.p[0].cf : kDefaultCf;
+        o[4] = old.n > 1 ? old.p[1].cf : kDefaultCf;
+        o[5] = face ? __float_as_uint(old.ln.x) : 0u;
+        o[6] = face ? __float_as_uint(old.ln.y) : 0u;
+        o[7] = set ? __float_as_uint(old.lpt.x) : 0u;
+        o[8] = set ? __float_as_uint(old.lpt.y) : 0u;
+#pragma unroll
+        for (std::uint32_t q = 0; q < 2; ++q) {
+            const bool has = q < old.n;
+            o[9 + 4 * q] = has ? __float_as_uint(old.p[q].lp.x) : 0u;
+            o[10 + 4 * q] = has ? __float_as_uint(old.p[q].lp.y) : 0u;
+            o[11 + 4 * q] = has ? __float_as_uint(old.p[q].ni) : 0u;
+            o[12 + 4 * q] = has ? __float_as_uint(old.p[q].ti) : 0u;
+        }
+    }
+}
+
 // UpdateContacts (AabbTreeWorld.cpp:2333-2421).  Returns contacts updated.
-__device__ __forceinline__ std::uint32_t phase_update(const DeviceState& d, const WorldCtx& x, int nc, const StepParams& c) {
+__device__ __forceinline__ std::uint32_t phase_update(const DeviceState& d, const WorldCtx& x, Counters& k,
+                                                      const StepParams& c) {
+    const int nc = k.nc;
     std::uint32_t updated = 0;
     for (int base = 0; base < nc; base += kWave) {
         const int i = base + x.lane;
         bool need = false;
         if (i < nc) need = (d.c_flags[x.co + i] & prx::kNeedsUpdating) != 0;
         updated += std::uint32_t(__popcll(__ballot(need)));
-        if (need) update_contact(d, x.bo, x.co + i, c);
+        if (x.ecap > 0) {
+            Manifold old = empty_manifold();
+            std::uint32_t calls = 0;
+            if (need) calls = update_contact(d, x.bo, x.co + i, c, &old);
+            ev_update(d, x, k, calls, need ? d.c_id[x.co + i] : 0u, old);
+        } else if (need) {
+            update_contact(d, x.bo, x.co + i, c);
+        }
     }
     return updated;
 }
@@ -240,6 +324,15 @@ __device__ __forceinline__ std::uint32_t phase_destroy(const DeviceState& d, con
                 const int2 bd = d.c_body[x.co + i];
                 body_set_awake(d, x.bo + bd.x);
                 body_set_awake(d, x.bo + bd.y);
+            }
+        }
+        if (x.ecap > 0) {
+            // End-contact for a destroyed touching contact (AabbTreeWorld.cpp:2249-2252).
+            const bool end = destroyed && (d.c_flags[x.co + i] & prx::kTouching) != 0;
+            const int at = ev_claim(x, k, end ? 2 : 0, end ? 1 : 0);
+            if (at >= 0) {
+                d.ev[at] = prx::kEvEnd;
+                d.ev[at + 1] = d.c_id[x.co + i];
             }
         }
         const int dst = out + __popcll(km & below);
@@ -474,7 +567,18 @@ __device__ __forceinline__ std::uint32_t phase_find_add(const DeviceState& d, co
     k.nc = nc0 + nadd;
     __syncthreads();
     // Narrowphase of the new contacts.
-    for (int i = nc0 + x.lane; i < k.nc; i += kWave) update_contact(d, x.bo, x.co + i, c);
+    for (int base = nc0; base < k.nc; base += kWave) {
+        const int i = base + x.lane;
+        const bool v = i < k.nc;
+        if (x.ecap > 0) {
+            Manifold old = empty_manifold();
+            std::uint32_t calls = 0;
+            if (v) calls = update_contact(d, x.bo, x.co + i, c, &old);
+            ev_update(d, x, k, calls, v ? d.c_id[x.co + i] : 0u, old);
+        } else if (v) {
+            update_contact(d, x.bo, x.co + i, c);
+        }
+    }
     __syncthreads();
     return std::uint32_t(nadd);
 }
@@ -1300,15 +1404,6 @@ struct Bodies {
     }
 };
 
-// Inclusive wave scan of a 64-lane value.
-__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
-    for (int o = 1; o < kWave; o <<= 1) {
-        const int t = __shfl_up(v, o);
-        if (lane >= o) v += t;
-    }
-    return v;
-}
-
 // Exclusive prefix sum of a[0..n) in place (wave-parallel, carry across chunks);
 // returns the total.
 __device__ __forceinline__ int wave_exclusive_scan(int* a, int n, int lane) {
@@ -1332,7 +1427,7 @@ struct IslandScratch {
     int2* istart;     // island i: first body entry, first contact entry
     int* jstart;      // island i: first joint entry
     int* bisland;     // island of each body (-1: none)
-    std::uint32_t* isl_word;  // per island: bit 0 solved, bits 1..: unused; min under-active time bits
+    std::uint32_t* isl_word;  // per island: solved position iteration + 1 (0: not solved)
     std::uint32_t* isl_under;
 };
 
@@ -1847,8 +1942,10 @@ __device__ __forceinline__ void island_velocity_wave(const DeviceState& d, const
 
 // Lane form of the position sweeps of one island (ContactSolver.cpp:494-643,
 // AabbTreeWorld.cpp:1627-1644).  Returns whether the island was solved.
+// The position sweeps return the iteration that solved the island plus one
+// (0: not solved within the iteration limit).
 template <bool kLds, bool kTape = false>
-__device__ __forceinline__ bool island_position(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
+__device__ __forceinline__ int island_position(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
                                                 int c0, int c1, int j0, int j1, const StepParams& c,
                                                 int islandSlot = 0) {
     const float4* base = d.s_pc + std::size_t(x.co) * prx::kPcStride;
@@ -1884,13 +1981,13 @@ __device__ __forceinline__ bool island_position(const DeviceState& d, const Worl
         bool jointsOkay;
         if constexpr (kTape) jointsOkay = joints_position_tape(jbase, j0, j1, bs, c, d.tape, x.jo, int(it));
         else jointsOkay = joints_position(jbase, j0, j1, bs, c);
-        if (contactsOkay && jointsOkay) return true;
+        if (contactsOkay && jointsOkay) return int(it) + 1;
     }
-    return false;
+    return 0;
 }
 
 template <bool kLds>
-__device__ __forceinline__ bool island_position_wave(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
+__device__ __forceinline__ int island_position_wave(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
                                                      float4* stage, int c0, int c1, int j0, int j1,
                                                      const StepParams& c) {
     const float4* jbase = d.s_jc + std::size_t(x.jo) * prx::kJcStride;
@@ -1903,7 +2000,7 @@ __device__ __forceinline__ bool island_position_wave(const DeviceState& d, const
     sg.commit(0);
     __syncthreads();
     const int npass = int(c.posIters);
-    bool solved = false;
+    int solved = 0;
     for (int pass = 0; pass < npass; ++pass) {
         float minSep = __builtin_inff();
         for (int ch = 0; ch < nch; ++ch) {
@@ -1936,7 +2033,7 @@ __device__ __forceinline__ bool island_position_wave(const DeviceState& d, const
         minSep = __shfl(minSep, 0);
         jointsOkay = __shfl(int(jointsOkay), 0) != 0;
         if (minSep >= c.regMinSeparation && jointsOkay) {
-            solved = true;
+            solved = pass + 1;
             break;
         }
     }
@@ -2131,7 +2228,7 @@ __device__ __forceinline__ void island_velocity_level(const DeviceState& d, cons
 // Position sweeps of one island in level form (records prefetched a round
 // ahead); returns whether solved.
 template <bool kLds>
-__device__ __forceinline__ bool island_position_level(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
+__device__ __forceinline__ int island_position_level(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
                                                       const LevelPlan& P, int c0, const StepParams& c) {
     const float4* p2 = d.s_pc2 + (std::size_t(x.co) + std::size_t(c0)) * prx::kPcStride;
     auto fetch = [&](int2 r, float4 (&buf)[prx::kPcStride]) {
@@ -2166,9 +2263,9 @@ __device__ __forceinline__ bool island_position_level(const DeviceState& d, cons
         }
 #pragma unroll
         for (int off = kWave / 2; off > 0; off >>= 1) minSep = fmin_std(minSep, __shfl_xor(minSep, off));
-        if (minSep >= c.regMinSeparation) return true;
+        if (minSep >= c.regMinSeparation) return int(it) + 1;
     }
-    return false;
+    return 0;
 }
 
 __device__ __forceinline__ bool level_form(const WorldCtx& x, int c0, int c1, int j0, int j1) {
@@ -2314,10 +2411,9 @@ __device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, con
     __syncthreads();
     // Position sweeps, split like the velocity sweeps.
     for (int is = x.lane; is < nisl; is += kWave) {
-        if (constraints(is) < coopMin &&
-            island_position<kLds, kTape>(d, x, bs, S.istart[is].y, S.istart[is + 1].y, S.jstart[is],
-                                         S.jstart[is + 1], c, x.bo + x.w + is))
-            S.isl_word[is] = 1u;
+        if (constraints(is) < coopMin)
+            S.isl_word[is] = std::uint32_t(island_position<kLds, kTape>(
+                d, x, bs, S.istart[is].y, S.istart[is + 1].y, S.jstart[is], S.jstart[is + 1], c, x.bo + x.w + is));
     }
     __syncthreads();
     for (int base = 0; base < nisl; base += kWave) {
@@ -2326,11 +2422,11 @@ __device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, con
         for (unsigned long long m = __ballot(big); m; m &= m - 1) {
             const int isl = base + __ffsll(m) - 1;
             const int c0 = S.istart[isl].y, c1 = S.istart[isl + 1].y;
-            const bool solved = level_form(x, c0, c1, S.jstart[isl], S.jstart[isl + 1])
-                                    ? island_position_level<kLds>(d, x, bs, level_plan_of(d, x, c0, c1), c0, c)
-                                    : island_position_wave<kLds>(d, x, bs, stage, c0, c1, S.jstart[isl],
-                                                                 S.jstart[isl + 1], c);
-            if (solved && x.lane == 0) S.isl_word[isl] = 1u;
+            const int solved = level_form(x, c0, c1, S.jstart[isl], S.jstart[isl + 1])
+                                   ? island_position_level<kLds>(d, x, bs, level_plan_of(d, x, c0, c1), c0, c)
+                                   : island_position_wave<kLds>(d, x, bs, stage, c0, c1, S.jstart[isl],
+                                                                S.jstart[isl + 1], c);
+            if (x.lane == 0) S.isl_word[isl] = std::uint32_t(solved);
         }
     }
     __syncthreads();
@@ -2437,6 +2533,40 @@ __device__ __forceinline__ std::uint32_t synchronize_body(const DeviceState& d, 
     return moved;
 }
 
+// Post-solve listener calls (AabbTreeWorld.cpp:1671-1674), island after
+// island in island contact order: the velocity constraint's points and
+// impulses (a constraint can keep one point of a two-point manifold) and the
+// iteration that solved the island's positions (255, the 8-bit
+// InvalidIteration, if none did).
+__device__ __forceinline__ void post_solve_calls(const DeviceState& d, const WorldCtx& x, Counters& k,
+                                                 const IslandScratch& S, int nisl) {
+    for (int is = 0; is < nisl; ++is) {
+        const int c0 = S.istart[is].y, c1 = S.istart[is + 1].y;
+        const std::uint32_t sw = S.isl_word[is];
+        const std::uint32_t iter = sw ? ((sw - 1u) & 0xffu) : 255u;
+        for (int base = c0; base < c1; base += kWave) {
+            const int e = base + x.lane;
+            const bool v = e < c1;
+            const int at = ev_claim(x, k, v ? prx::kEvPostWords : 0, v ? 1 : 0);
+            if (at < 0) continue;
+            const int gi = x.co + S.icont[e];
+            const float4* rec = d.s_vc + (std::size_t(x.co) + std::size_t(e)) * prx::kVcStride;
+            const std::uint32_t n = __float_as_uint(rec[1].w);
+            const float4 q0 = rec[4], q1 = rec[7];
+            std::uint32_t* o = d.ev + at;
+            o[0] = prx::kEvPostSolve;
+            o[1] = d.c_id[gi];
+            o[2] = n;
+            o[3] = iter;
+            o[4] = n > 0 ? __float_as_uint(q0.x) : 0u;
+            o[5] = n > 1 ? __float_as_uint(q1.x) : 0u;
+            o[6] = n > 0 ? __float_as_uint(q0.y) : 0u;
+            o[7] = n > 1 ? __float_as_uint(q1.y) : 0u;
+        }
+    }
+    __syncthreads();
+}
+
 template <bool kTape = false>
 __device__ __forceinline__ void step_world(const DeviceState& d, int w, const StepParams& c) {
     WorldHeader& H = d.hdr[w];
@@ -2450,12 +2580,16 @@ __device__ __forceinline__ void step_world(const DeviceState& d, int w, const St
     x.cap = H.contact_cap;
     x.jo = H.joint_off;
     x.nj = H.njoints;
+    x.eo = H.ev_off;
+    x.ecap = H.ev_cap;
     x.lane = int(threadIdx.x);
     Counters k;
     k.nc = H.ncontacts;
     k.nfree = H.nfree;
     k.pool = H.pool_size;
     k.status = std::uint32_t(H.status);
+    k.evw = 0;
+    k.evn = 0;
     std::uint32_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     PhaseTimer T;
     T.begin(d.phase_cycles ? d.phase_cycles + std::size_t(w) * prx::kPhases : nullptr);
@@ -2475,7 +2609,7 @@ __device__ __forceinline__ void step_world(const DeviceState& d, int w, const St
     stats[1] = phase_destroy(d, x, k);
     __syncthreads();
     T.mark(prx::kPhDestroy);
-    stats[3] = phase_update(d, x, k.nc, c);
+    stats[3] = phase_update(d, x, k, c);
     __syncthreads();
     T.mark(prx::kPhUpdate);
     stats[2] = phase_find_add(d, x, k, c);
@@ -2507,6 +2641,7 @@ __device__ __forceinline__ void step_world(const DeviceState& d, int w, const St
         const bool ldsBodies = x.nb * 32 + prx::kStageFloat4 * 16 <= int(c.ldsBytes);
         stats[7] = ldsBodies ? solve_islands<true, kTape>(d, x, nisl, S, c, T)
                              : solve_islands<false, kTape>(d, x, nisl, S, c, T);
+        if (x.ecap > 0) post_solve_calls(d, x, k, S, nisl);
         // Synchronise proxies of islanded speedable bodies (AabbTreeWorld.cpp:1532-1544).
         std::uint32_t moved = 0;
         for (int b = x.lane; b < x.nb; b += kWave) {
@@ -2524,7 +2659,7 @@ __device__ __forceinline__ void step_world(const DeviceState& d, int w, const St
         __syncthreads();
         for (int b = x.lane; b < x.nb; b += kWave) d.bflags[x.bo + b] &= ~0x100u;
         __syncthreads();
-        phase_update(d, x, k.nc, c);
+        phase_update(d, x, k, c);
         __syncthreads();
         T.mark(prx::kPhUpdate2);
         stats[5] = phase_find_add(d, x, k, c);
@@ -2538,6 +2673,8 @@ __device__ __forceinline__ void step_world(const DeviceState& d, int w, const St
         H.pool_size = k.pool;
         H.status = int(k.status);
         for (int i = 0; i < 8; ++i) H.stats[i] = stats[i];
+        H.ev_words = k.evw;
+        H.ev_count = k.evn;
     }
 }
 

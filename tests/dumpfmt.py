@@ -4,7 +4,9 @@ and by the oracle restatement (same layout), plus canonical per-step digests.
 A record holds, after each step: bodies (pos0, pos1, velocity, under-active
 time, flags), the contact list in the world's contact order (keys, ids, flags,
 manifold with ContactFeatures and impulses), the broadphase proxies (id and
-fat AABB) and the joints (limit state, impulses).
+fat AABB), the joints (limit state, impulses) and the step's contact listener
+calls (begin, end, pre-solve with the old manifold, post-solve with the
+impulses).
 """
 import hashlib
 import struct
@@ -17,6 +19,10 @@ CONTACT_T = 3
 CONTACT_F = 14
 JOINT_DESC_F = 9
 JOINT_F = 4
+# Listener kinds: 0 begin, 1 end, 2 pre-solve (old manifold: type | count << 8,
+# two contact features, 12 floats), 3 post-solve (count, solved position
+# iteration, normal impulses [2], tangent impulses [2]).
+EVENT_PAYLOAD = (0, 0, 15, 6)
 
 
 class Reader:
@@ -94,22 +100,43 @@ def read_state(r):
     raw = r.arr("<u4", nj * (1 + JOINT_F)).reshape(nj, 1 + JOINT_F)
     jlimit = raw[:, 0].copy()
     jimp = raw[:, 1:].copy().view("<f4")
+    # Contact listener calls of the step, in call order: kind, contact id,
+    # then EVENT_PAYLOAD[kind] words.
+    ne = r.u32()
+    words = []
+    for _ in range(ne):
+        kind = r.u32()
+        cid = r.u32()
+        words += [kind, cid] + list(r.arr("<u4", EVENT_PAYLOAD[kind]))
+    events = np.array(words, dtype=np.uint32)
     return dict(step=step, stats=stats, bodies=bodies, bflags=bflags,
-                cu=cu, ct=ct, cf=cf, pid=pid, paabb=paabb, jlimit=jlimit, jimp=jimp)
+                cu=cu, ct=ct, cf=cf, pid=pid, paabb=paabb, jlimit=jlimit, jimp=jimp,
+                nevents=ne, events=events)
+
+
+def iter_events(state):
+    """(kind, contact id, payload words) of each listener call in a state."""
+    w = state["events"]
+    o = 0
+    for _ in range(state["nevents"]):
+        kind = int(w[o])
+        n = EVENT_PAYLOAD[kind]
+        yield kind, int(w[o + 1]), w[o + 2:o + 2 + n]
+        o += 2 + n
 
 
 def read_dump(buf):
     r = Reader(buf)
     assert bytes(r.b[0:4]) == b"PRXD", "bad magic"
     r.o = 4
-    assert r.u32() == 2
+    assert r.u32() == 3
     shapes, bodies, joints = read_description(r)
     conf = read_conf(r)
     states = [read_state(r) for _ in range(r.u32())]
     return dict(shapes=shapes, bodies=bodies, joints=joints, conf=conf, states=states)
 
 
-def digest(state, parts=("bodies", "contacts", "proxies", "joints")):
+def digest(state, parts=("bodies", "contacts", "proxies", "joints", "events")):
     """SHA-256 over the canonical bytes of one state (selected parts)."""
     h = hashlib.sha256()
     if "bodies" in parts:
@@ -125,4 +152,7 @@ def digest(state, parts=("bodies", "contacts", "proxies", "joints")):
     if "joints" in parts:
         h.update(np.ascontiguousarray(state["jlimit"]).tobytes())
         h.update(np.ascontiguousarray(state["jimp"]).view("<u4").tobytes())
+    if "events" in parts:
+        h.update(np.uint32(state["nevents"]).tobytes())
+        h.update(np.ascontiguousarray(state["events"]).tobytes())
     return h.digest()

@@ -8,7 +8,8 @@ present and oracle/_ref has been built (make -f oracle/Makefile.ref):
 For each case it runs oracle/_ref/ref_driver (our driver linked against the
 reference library), and stores in tests/golden/<case>.npz:
   desc_conf  the world description and StepConf as the reference built them
-  digests    per step, SHA-256 of bodies / contact list / proxies
+  digests    per step, SHA-256 of bodies / contact list / proxies / joints /
+             contact listener calls
   full_*     complete state at a few steps (for diffs when a digest differs)
 HelloWorld/ExpectedOutput.txt (the reference's own golden output) is copied
 as data to tests/golden/hello_expected.txt.
@@ -24,7 +25,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
-from tests import dumpfmt, oracle_lib  # noqa: E402
+from tests import dumpfmt, golden_cases, oracle_lib  # noqa: E402
 
 CASES = [
     # name, scene, steps, params, full-state steps
@@ -47,16 +48,16 @@ def main():
             buf = oracle_lib.run_ref_driver(scene, steps, False, params, path)
             desc_conf, d = oracle_lib.split_dump(buf)
             states = d["states"]
-            dig = np.zeros((len(states), 4, 32), dtype=np.uint8)
+            dig = np.zeros((len(states), len(golden_cases.PARTS), 32), dtype=np.uint8)
             for i, s in enumerate(states):
-                for k, part in enumerate(("bodies", "contacts", "proxies", "joints")):
+                for k, part in enumerate(golden_cases.PARTS):
                     dig[i, k] = np.frombuffer(dumpfmt.digest(s, (part,)), dtype=np.uint8)
             arrays = dict(desc_conf=np.frombuffer(desc_conf, dtype=np.uint8),
                           digests=dig, scene=np.array(scene), params=np.array(params, dtype=np.int32),
                           steps=np.array(steps), full_steps=np.array(full, dtype=np.int32))
             for st in full:
                 s = states[st]
-                for key in ("stats", "bodies", "bflags", "cu", "ct", "cf", "pid", "paabb", "jlimit", "jimp"):
+                for key in ("stats", "bodies", "bflags", "cu", "ct", "cf", "pid", "paabb", "jlimit", "jimp", "events"):
                     arrays["full%d_%s" % (st, key)] = s[key]
             out = os.path.join(HERE, name + ".npz")
             np.savez_compressed(out, **arrays)

@@ -6,7 +6,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLDEN = os.path.join(HERE, "golden")
-PARTS = ("bodies", "contacts", "proxies", "joints")
+PARTS = ("bodies", "contacts", "proxies", "joints", "events")
 
 
 def names():
@@ -20,7 +20,7 @@ def load(name):
     for st in z["full_steps"]:
         st = int(st)
         g["full"][st] = {k: z["full%d_%s" % (st, k)] for k in
-                         ("stats", "bodies", "bflags", "cu", "ct", "cf", "pid", "paabb", "jlimit", "jimp")}
+                         ("stats", "bodies", "bflags", "cu", "ct", "cf", "pid", "paabb", "jlimit", "jimp", "events")}
     return g
 
 

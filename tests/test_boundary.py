@@ -27,7 +27,7 @@ def test_library_exports_every_declared_function():
 
 
 def test_abi_version():
-    assert pa.lib().prx_abi_version() == 2
+    assert pa.lib().prx_abi_version() == 3
 
 
 def _desc_only(desc_conf):
@@ -122,3 +122,41 @@ def test_shape_material_setters():
     # A world rebuilt with the same materials from the start has the same description.
     assert pa.World.from_desc(pa.desc_with(desc, friction={s: 0.125}, restitution={s: 0.75})).save_desc() == \
         w.save_desc()
+
+
+def test_contact_event_records_parse_like_the_golden_format():
+    """The package's listener-call parser reads the oracle's recorded calls
+    as the fixture format does (kinds, ids, manifolds, impulses)."""
+    from tests import oracle_lib
+    g = golden_cases.load("pile3")
+    ow = oracle_lib.OracleWorld(g["desc_conf"])
+    seen = set()
+    for s in ow.step(60):
+        calls = pa.parse_contact_events(s["events"], s["nevents"])
+        ref = list(dumpfmt.iter_events(s))
+        assert len(calls) == len(ref)
+        for c, (kind, cid, payload) in zip(calls, ref):
+            assert c[0] == ("begin", "end", "pre_solve", "post_solve")[kind] and c[1] == cid
+            seen.add(kind)
+            if kind == 2:
+                m = c[2]
+                assert m["type"] | (m["point_count"] << 8) == payload[0]
+                for q, p in enumerate(m["points"]):
+                    assert p["contact_feature"] == payload[1 + q]
+                    assert np.float32(p["normal_impulse"]).view(np.uint32) == payload[9 + 4 * q]
+            if kind == 3:
+                assert c[2][0] == payload[0] and c[3] == payload[1]
+                assert np.array_equal(np.array(c[2][1], dtype=np.float32).view(np.uint32), payload[2:2 + payload[0]])
+    assert seen == {0, 2, 3} or seen == {0, 1, 2, 3}
+
+
+def test_contact_event_recording_switch_host_side():
+    """Recording is a world setting; without a step the record holds no calls."""
+    w = pa.World.from_desc(golden_cases.load("hello")["desc_conf"])
+    w.record_contact_events(True)
+    st = dumpfmt.read_state(dumpfmt.Reader(w.state_record()))
+    assert st["nevents"] == 0 and st["events"].size == 0
+    words, n = w.contact_event_words()
+    assert n == 0 and words.size == 0
+    w.record_contact_events(False)
+    w.set_contact_listener()  # clearing with nothing set is allowed

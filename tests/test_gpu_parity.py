@@ -2,8 +2,9 @@
 
 Each golden case starts the device from the reference's own world
 description and compares every step's full state (bodies, contact list with
-manifolds and ContactFeature ids, proxies) against the per-step digests of
-the reference float build: bit-exact.  Larger cases compare against the
+manifolds and ContactFeature ids, proxies, joints, and the step's contact
+listener calls with their arguments) against the per-step digests of the
+reference float build: bit-exact.  Larger cases compare against the
 oracle (CPU restatement) step by step."""
 import numpy as np
 import pytest
@@ -16,6 +17,11 @@ pytestmark = pytest.mark.gpu
 
 def _state(world):
     return dumpfmt.read_state(dumpfmt.Reader(world.state_record()))
+
+
+def _recording(world):
+    world.record_contact_events(True)
+    return world
 
 
 def test_device_sincos_matches_host_library():
@@ -33,7 +39,7 @@ def test_device_sincos_matches_host_library():
 def test_plain_step_matches_reference_every_step(name):
     g = golden_cases.load(name)
     print("case", name, "steps", g["steps"], flush=True)
-    w = pa.World.from_desc(g["desc_conf"])
+    w = _recording(pa.World.from_desc(g["desc_conf"]))
     conf = pa.StepConf()
     for step in range(1, g["steps"] + 1):
         w.step(conf)
@@ -55,7 +61,7 @@ def test_batched_step_matches_reference(coop_min):
     1 << 30 sweeps every island on one lane; both equal the reference."""
     names = golden_cases.names()
     gs = [golden_cases.load(n) for n in names]
-    worlds = [pa.World.from_desc(g["desc_conf"]) for g in gs]
+    worlds = [_recording(pa.World.from_desc(g["desc_conf"])) for g in gs]
     print("coop_min", coop_min, flush=True)
     batch = pa.Batch(worlds, use_graph=False, coop_min=coop_min)
     conf = pa.StepConf()
@@ -78,8 +84,8 @@ def test_batched_step_matches_reference(coop_min):
 def test_graph_replay_matches_eager():
     """Steps replayed from a captured hipGraph equal eager launches."""
     descs = [golden_cases.load("pile%d" % i)["desc_conf"] for i in range(4)]
-    eager = [pa.World.from_desc(d) for d in descs]
-    graph = [pa.World.from_desc(d) for d in descs]
+    eager = [_recording(pa.World.from_desc(d)) for d in descs]
+    graph = [_recording(pa.World.from_desc(d)) for d in descs]
     be = pa.Batch(eager, use_graph=False)
     bg = pa.Batch(graph, use_graph=True)
     conf = pa.StepConf()
@@ -96,8 +102,8 @@ def test_autotune_restores_state_and_keeps_results():
     """prx_batch_autotune times schedules from the current state, restores it,
     and the chosen schedule steps to the same bits as the default one."""
     descs = [golden_cases.load(n)["desc_conf"] for n in ("pile3", "chain0", "pyramid", "billiards0")]
-    a = [pa.World.from_desc(d) for d in descs]
-    b = [pa.World.from_desc(d) for d in descs]
+    a = [_recording(pa.World.from_desc(d)) for d in descs]
+    b = [_recording(pa.World.from_desc(d)) for d in descs]
     ba = pa.Batch(a, use_graph=True)
     bb = pa.Batch(b, use_graph=True)
     conf = pa.StepConf()
@@ -125,7 +131,7 @@ def test_lds_schedule_matches_reference(lds_bytes):
     the results."""
     names = ["pile1", "pile3", "chain1", "drop300"]
     gs = [golden_cases.load(n) for n in names]
-    worlds = [pa.World.from_desc(g["desc_conf"]) for g in gs]
+    worlds = [_recording(pa.World.from_desc(g["desc_conf"])) for g in gs]
     batch = pa.Batch(worlds, use_graph=True, lds_bytes=lds_bytes)
     conf = pa.StepConf()
     steps = 100
@@ -141,7 +147,7 @@ def test_lds_schedule_matches_reference(lds_bytes):
     ("pile", [3, 11], 60), ("pile", [5, 40], 60), ("drop", [1000, 40, 80, 3], 40),
     ("chain", [3, 5, 32, 64, 5], 150)])
 def test_device_matches_oracle(scene, params, steps):
-    w = pa.World.scene(scene, params)
+    w = _recording(pa.World.scene(scene, params))
     ow = oracle_lib.OracleWorld(w.save_desc() + _conf_bytes())
     conf = pa.StepConf()
     for step in range(1, steps + 1):
