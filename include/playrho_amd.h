@@ -108,10 +108,17 @@ typedef struct prx_step_conf {
     uint32_t do_warm_start, do_toi, do_blocksolve;
 } prx_step_conf;
 
-/* Subset of StepStats (StepStats.hpp:30-201). */
+/* StepStats (StepStats.hpp:30-201) of a regular step: PreStepStats
+ * (:38-46) and RegStepStats (:69-82), fields in the reference's order
+ * (ToiStepStats stays default: doToi = false).  reg_min_separation is
+ * +infinity and reg_max_inc_impulse 0 when no island was solved. */
 typedef struct prx_step_stats {
-    uint32_t pre_proxies_moved, pre_contacts_destroyed, pre_contacts_added, pre_contacts_updated;
-    uint32_t reg_islands_found, reg_contacts_added, reg_proxies_moved, reg_bodies_slept;
+    uint32_t pre_proxies_created, pre_proxies_moved, pre_contacts_destroyed, pre_contacts_added;
+    uint32_t pre_contacts_updated, pre_contacts_skipped;
+    float reg_min_separation, reg_max_inc_impulse;
+    uint32_t reg_islands_found, reg_islands_solved, reg_bodies_slept, reg_max_island_bodies;
+    uint32_t reg_contacts_added, reg_contacts_updated, reg_contacts_skipped, reg_proxies_moved;
+    uint32_t reg_sum_pos_iters, reg_sum_vel_iters;
 } prx_step_stats;
 
 /* Launch schedule of the batched step (chosen by prx_batch_autotune). */

@@ -153,8 +153,19 @@ void WriteState(const Handle<R>& h, Out& o) {
     const auto& w = h.world;
     o.u32(h.steps);
     const auto& s = h.last;
-    const std::uint32_t st[8] = {s.preProxiesMoved, s.preDestroyed, s.preAdded, s.preUpdated,
-                                 s.islandsFound, s.regAdded, s.regProxiesMoved, s.bodiesSlept};
+    // PreStepStats then RegStepStats, each in declaration order (StepStats.hpp:38-46, 69-82).
+    auto bits = [](double v) {
+        const float f = float(v);
+        std::uint32_t u;
+        std::memcpy(&u, &f, 4);
+        return u;
+    };
+    const std::uint32_t st[18] = {s.preProxiesCreated, s.preProxiesMoved, s.preDestroyed, s.preAdded,
+                                  s.preUpdated,        s.preSkipped,      bits(s.minSeparation),
+                                  bits(s.maxIncImpulse), s.islandsFound,  s.islandsSolved,
+                                  s.bodiesSlept,       s.maxIslandBodies, s.regAdded,
+                                  s.regUpdated,        s.regSkipped,      s.regProxiesMoved,
+                                  s.sumPosIters,       s.sumVelIters};
     o.put(st, sizeof(st));
     o.u32(std::uint32_t(w.bodies.size()));
     for (const auto& b : w.bodies) {

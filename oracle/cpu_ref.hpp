@@ -643,9 +643,21 @@ struct Contact {
     R friction = 0, restitution = 0, tangentSpeed = 0;
 };
 
+// PreStepStats and RegStepStats (StepStats.hpp:38-46, 69-82); the two real
+// fields are kept in double (exact for either instantiation).
 struct StepStats {
-    std::uint32_t preProxiesMoved = 0, preDestroyed = 0, preAdded = 0, preUpdated = 0;
-    std::uint32_t islandsFound = 0, regAdded = 0, regProxiesMoved = 0, bodiesSlept = 0;
+    std::uint32_t preProxiesCreated = 0, preProxiesMoved = 0, preDestroyed = 0, preAdded = 0, preUpdated = 0,
+                  preSkipped = 0;
+    double minSeparation = std::numeric_limits<double>::infinity(), maxIncImpulse = 0;
+    std::uint32_t islandsFound = 0, islandsSolved = 0, bodiesSlept = 0, maxIslandBodies = 0, regAdded = 0,
+                  regUpdated = 0, regSkipped = 0, regProxiesMoved = 0, sumPosIters = 0, sumVelIters = 0;
+};
+
+// IslandStats of one island solve (StepStats.hpp:149-163).
+struct IslandStats {
+    double minSeparation = std::numeric_limits<double>::infinity(), maxIncImpulse = 0;
+    bool solved = false;
+    std::uint32_t positionIters = 0, velocityIters = 0, bodiesSlept = 0;
 };
 
 // Solver scratch records.
@@ -997,14 +1009,17 @@ public:
     }
 
     // UpdateContacts (AabbTreeWorld.cpp:2333-2421)
-    std::uint32_t UpdateContacts(const StepConf<R>& conf) {
-        std::uint32_t updated = 0;
+    std::uint32_t UpdateContacts(const StepConf<R>& conf, std::uint32_t* skipped = nullptr) {
+        std::uint32_t updated = 0, skip = 0;
         for (const auto& kc : contacts) {
             if (contactBuf[kc.id].flags & kNeedsUpdating) {
                 Update(kc.id, conf);
                 ++updated;
+            } else {
+                ++skip;
             }
         }
+        if (skipped) *skipped = skip;
         return updated;
     }
 
@@ -1671,7 +1686,8 @@ public:
     }
 
     // SolveRegIslandViaGS (AabbTreeWorld.cpp:1563-1682)
-    std::uint32_t SolveIsland(const Island& island, const StepConf<R>& conf) {
+    IslandStats SolveIsland(const Island& island, const StepConf<R>& conf) {
+        IslandStats is;
         const R h = conf.dt;
         for (std::uint32_t bid : island.bodies) bodies[bid].pos0 = bodies[bid].pos1;
         std::vector<BodyConstraint<R>> bcs(bodies.size());
@@ -1717,12 +1733,17 @@ public:
             }
         }
         for (std::uint32_t jid : island.joints) JointInitVelocity(joints[jid], bcs, conf);
+        is.velocityIters = conf.velIters;
         for (std::uint32_t i = 0; i < conf.velIters; ++i) {
             bool jointsOkay = true;
             for (std::uint32_t jid : island.joints) jointsOkay &= JointSolveVelocity(joints[jid], bcs, conf);
             R newInc = 0;
             for (auto& vc : vcs) newInc = Max(newInc, SolveVC(vc, bcs));
-            if (jointsOkay && newInc <= conf.regMinMomentum) break;
+            is.maxIncImpulse = Max(is.maxIncImpulse, double(newInc));
+            if (jointsOkay && newInc <= conf.regMinMomentum) {
+                is.velocityIters = i + 1;
+                break;
+            }
         }
         for (std::uint32_t bid : island.bodies) {
             BodyConstraint<R>& bc = bcs[bid];
@@ -1735,6 +1756,7 @@ public:
         for (std::uint32_t i = 0; i < conf.posIters; ++i) {
             R minSep = std::numeric_limits<R>::infinity();
             for (const auto& pc : pcs) minSep = Min(minSep, SolvePC(pc, bcs, conf));
+            is.minSeparation = Min(is.minSeparation, double(minSep));
             const bool contactsOkay = minSep >= conf.regMinSeparation;
             bool jointsOkay = true;
             for (std::uint32_t jid : island.joints) jointsOkay &= JointSolvePosition(joints[jid], bcs, conf);
@@ -1794,7 +1816,10 @@ public:
         if (minUnder >= conf.minStillTimeToSleep && solved) {
             for (std::uint32_t bid : island.bodies) slept += bodies[bid].Unawaken() ? 1u : 0u;
         }
-        return slept;
+        is.solved = solved;
+        is.positionIters = posIters;
+        is.bodiesSlept = slept;
+        return is;
     }
 
     // SolveReg (AabbTreeWorld.cpp:1472-1561)
@@ -1809,10 +1834,18 @@ public:
                 ++st.islandsFound;
                 Island island;
                 AddToIsland(island, bid);
+                st.maxIslandBodies = std::max(st.maxIslandBodies, std::uint32_t(island.bodies.size()));
                 for (std::uint32_t ib : island.bodies) {
                     if (!bodies[ib].speedable()) islandedBodies[ib] = 0;
                 }
-                st.bodiesSlept += SolveIsland(island, conf);
+                // Update(RegStepStats&, const IslandStats&) (StepStats.cpp:29-38)
+                const IslandStats is = SolveIsland(island, conf);
+                st.maxIncImpulse = Max(st.maxIncImpulse, is.maxIncImpulse);
+                st.minSeparation = Min(st.minSeparation, is.minSeparation);
+                st.islandsSolved += is.solved ? 1u : 0u;
+                st.sumPosIters += is.positionIters;
+                st.sumVelIters += is.velocityIters;
+                st.bodiesSlept += is.bodiesSlept;
             }
         }
         for (std::uint32_t bid : bodyOrder) {
@@ -1820,7 +1853,7 @@ public:
         }
         islandedBodies.assign(bodies.size(), 0);
         islandedContacts.assign(contactBuf.size(), 0);
-        UpdateContacts(conf);
+        st.regUpdated = UpdateContacts(conf, &st.regSkipped);
         st.regAdded = FindAndAddContacts(conf);
     }
 
@@ -1829,10 +1862,13 @@ public:
         StepStats st;
         events.clear();
         nevents = 0;
-        for (const auto& fs : fixturesForProxies) CreateProxies(fs.first, fs.second, conf);
+        for (const auto& fs : fixturesForProxies) {
+            CreateProxies(fs.first, fs.second, conf);
+            ++st.preProxiesCreated;
+        }
         fixturesForProxies.clear();
         st.preDestroyed = DestroyContacts();
-        st.preUpdated = UpdateContacts(conf);
+        st.preUpdated = UpdateContacts(conf, &st.preSkipped);
         st.preAdded = FindAndAddContacts(conf);
         if (conf.dt != R(0)) SolveReg(conf, st);
         return st;

@@ -252,10 +252,19 @@ void InstallListeners(World& world) {
 
 void WriteState(Out& o, const World& world, std::uint32_t step, const StepStats& st) {
     o.u32(step);
-    const std::uint32_t stats[8] = {st.pre.proxiesMoved, st.pre.contactsDestroyed,
-                                    st.pre.contactsAdded, st.pre.contactsUpdated,
-                                    std::uint32_t(st.reg.islandsFound), st.reg.contactsAdded,
-                                    st.reg.proxiesMoved, std::uint32_t(st.reg.bodiesSlept)};
+    // PreStepStats then RegStepStats, each in declaration order.
+    auto bits = [](float f) {
+        std::uint32_t u;
+        std::memcpy(&u, &f, 4);
+        return u;
+    };
+    const std::uint32_t stats[18] = {
+        st.pre.proxiesCreated, st.pre.proxiesMoved, st.pre.contactsDestroyed, st.pre.contactsAdded,
+        st.pre.contactsUpdated, st.pre.contactsSkipped, bits(float(st.reg.minSeparation)),
+        bits(float(st.reg.maxIncImpulse)), std::uint32_t(st.reg.islandsFound),
+        std::uint32_t(st.reg.islandsSolved), std::uint32_t(st.reg.bodiesSlept),
+        std::uint32_t(st.reg.maxIslandBodies), st.reg.contactsAdded, st.reg.contactsUpdated,
+        st.reg.contactsSkipped, st.reg.proxiesMoved, st.reg.sumPosIters, st.reg.sumVelIters};
     o.put(stats, sizeof(stats));
     const auto nbodies = GetBodyRange(world);
     o.u32(nbodies);

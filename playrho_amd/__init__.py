@@ -151,9 +151,15 @@ class RevoluteJointConf(ctypes.Structure):
 
 
 class StepStats(ctypes.Structure):
+    """StepStats of a regular step (StepStats.hpp:30-201): PreStepStats and RegStepStats."""
     _fields_ = [(n, ctypes.c_uint32) for n in (
-        "pre_proxies_moved", "pre_contacts_destroyed", "pre_contacts_added", "pre_contacts_updated",
-        "reg_islands_found", "reg_contacts_added", "reg_proxies_moved", "reg_bodies_slept")]
+        "pre_proxies_created", "pre_proxies_moved", "pre_contacts_destroyed", "pre_contacts_added",
+        "pre_contacts_updated", "pre_contacts_skipped")] + [
+        ("reg_min_separation", ctypes.c_float), ("reg_max_inc_impulse", ctypes.c_float)] + [
+        (n, ctypes.c_uint32) for n in (
+            "reg_islands_found", "reg_islands_solved", "reg_bodies_slept", "reg_max_island_bodies",
+            "reg_contacts_added", "reg_contacts_updated", "reg_contacts_skipped", "reg_proxies_moved",
+            "reg_sum_pos_iters", "reg_sum_vel_iters")]
 
 
 class BatchConf(ctypes.Structure):

@@ -108,7 +108,8 @@ __device__ __forceinline__ float block_reduce_min(float v, float* lds) {
 __global__ void k_big_pending(DeviceState d, StepParams c) {
     const int p = int(blockIdx.x * blockDim.x + threadIdx.x);
     if (p == 0) {
-        for (int i = 0; i < 8; ++i) d.hdr[0].stats[i] = 0u;
+        for (int i = 0; i < prx::kStats; ++i) d.hdr[0].stats[i] = 0u;
+        d.hdr[0].stats[prx::kStRegMinSeparation] = 0x7f800000u;
     }
     if (p >= d.hdr[0].nproxies) return;
     if (d.pflags[p] & prx::kProxyPending) {
@@ -182,7 +183,7 @@ __global__ __launch_bounds__(kBlock) void k_big_destroy(DeviceState d) {
     }
     if (threadIdx.x == 0) {
         H.ncontacts = out;
-        H.stats[1] += destroyed;
+        H.stats[prx::kStPreDestroyed] += destroyed;
     }
 }
 
@@ -983,7 +984,7 @@ __global__ void k_big_sync(DeviceState d, StepParams c) {
     if (!((f & prx::kSpeedable) && (f & prx::kEnabled))) return;
     const WorldCtx x = big_ctx(d);
     const std::uint32_t moved = synchronize_body(d, x, b, c);
-    if (moved) atomicAdd(&d.hdr[0].stats[6], moved);
+    if (moved) atomicAdd(&d.hdr[0].stats[prx::kStRegProxiesMoved], moved);
 }
 
 __global__ void k_big_flag_contacts(DeviceState d) {
@@ -1066,8 +1067,8 @@ extern "C" hipError_t prx_big_launch_step(const prx::DeviceState* dp, const prx:
     const int P = g.P, B = g.B, cap = g.cap;
     hipLaunchKernelGGL(k_big_pending, dim3(Blocks(P, kGrid)), dim3(kGrid), 0, s, d, *c);
     hipLaunchKernelGGL(k_big_destroy, dim3(1), dim3(kBlock), 0, s, d);
-    hipLaunchKernelGGL(k_big_update, dim3(Blocks(cap, kGrid)), dim3(kGrid), 0, s, d, *c, 3);
-    FindAdd(d, g, *c, 2, s);
+    hipLaunchKernelGGL(k_big_update, dim3(Blocks(cap, kGrid)), dim3(kGrid), 0, s, d, *c, prx::kStPreUpdated);
+    FindAdd(d, g, *c, prx::kStPreAdded, s);
     if (c->dt != 0.0f) {
         hipLaunchKernelGGL(k_big_bodies, dim3(Blocks(B, kGrid)), dim3(kGrid), 0, s, d, *c);
         hipLaunchKernelGGL(k_big_touch_list, dim3(1), dim3(kBlock), 0, s, d, g);
@@ -1093,7 +1094,7 @@ extern "C" hipError_t prx_big_launch_step(const prx::DeviceState* dp, const prx:
         hipLaunchKernelGGL(k_big_flag_contacts, dim3(Blocks(cap, kGrid)), dim3(kGrid), 0, s, d);
         hipLaunchKernelGGL(k_big_clear_bodyflag, dim3(Blocks(B, kGrid)), dim3(kGrid), 0, s, d);
         hipLaunchKernelGGL(k_big_update, dim3(Blocks(cap, kGrid)), dim3(kGrid), 0, s, d, *c, -1);
-        FindAdd(d, g, *c, 5, s);
+        FindAdd(d, g, *c, prx::kStRegAdded, s);
     }
     return hipGetLastError();
 }

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <numeric>
 #include <string>
@@ -301,7 +302,7 @@ struct prx_world {
     int index_in_owner = -1;
     bool host_stale = false;     // device state newer than this mirror
     prx_batch* solo = nullptr;   // single-world batch used by prx_step
-    prx_step_stats last_stats{};
+    prx_step_stats last_stats{0, 0, 0, 0, 0, 0, std::numeric_limits<float>::infinity()};  // StepStats{}
     bool record_events = false;  // the device records the contact listener calls of each step
     bool has_listener = false;
     prx_contact_listener listener{};
@@ -358,6 +359,14 @@ struct prx_batch {
 };
 
 namespace {
+
+// The device's StepStats words in the C ABI's struct (same order, same size).
+static_assert(sizeof(prx_step_stats) == sizeof(std::uint32_t) * prx::kStats, "StepStats layout");
+prx_step_stats StatsOf(const prx::WorldHeader& h) {
+    prx_step_stats s;
+    std::memcpy(&s, h.stats, sizeof(s));
+    return s;
+}
 
 int CheckWorld(prx_world* w) {
     if (!w) return Fail(PRX_E_INVALID_ARGUMENT, "null world");
@@ -994,8 +1003,7 @@ int DownloadBatch(prx_batch* b) {
         hc.freelist.assign(cfree.begin() + h.contact_off, cfree.begin() + h.contact_off + h.nfree);
         hc.pool = h.pool_size;
         if (int st2 = FetchEvents(b, int(wi), h)) return st2;
-        const std::uint32_t* s = h.stats;
-        w->last_stats = prx_step_stats{s[0], s[1], s[2], s[3], s[4], s[5], s[6], s[7]};
+        w->last_stats = StatsOf(h);
         w->host_stale = false;
     }
     return PRX_OK;
@@ -1520,8 +1528,7 @@ int prx_step(prx_world* w, const prx_step_conf* conf, prx_step_stats* stats) {
     if (stats) {
         prx::WorldHeader h;
         PRX_HIP(hipMemcpy(&h, w->solo->d.hdr, sizeof(h), hipMemcpyDeviceToHost));
-        *stats = prx_step_stats{h.stats[0], h.stats[1], h.stats[2], h.stats[3],
-                                h.stats[4], h.stats[5], h.stats[6], h.stats[7]};
+        *stats = StatsOf(h);
     }
     return PRX_OK;
 }
@@ -1683,11 +1690,7 @@ int prx_world_state_record(prx_world* w, std::uint8_t* buf, std::size_t cap, std
     };
     auto u32 = [&](std::uint32_t v) { put(&v, 4); };
     u32(0);
-    const prx_step_stats& s = w->last_stats;
-    const std::uint32_t st8[8] = {s.pre_proxies_moved, s.pre_contacts_destroyed, s.pre_contacts_added,
-                                  s.pre_contacts_updated, s.reg_islands_found, s.reg_contacts_added,
-                                  s.reg_proxies_moved, s.reg_bodies_slept};
-    put(st8, sizeof(st8));
+    put(&w->last_stats, sizeof(prx_step_stats));  // the 18 StepStats words, in order
     u32(std::uint32_t(w->bodies.size()));
     for (const auto& b : w->bodies) {
         const float v[10] = {b.pos0[0], b.pos0[1], b.pos0[2], b.pos1[0], b.pos1[1],

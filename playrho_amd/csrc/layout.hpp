@@ -52,11 +52,21 @@ enum : std::uint32_t { kProxyPending = 1u, kProxyMoved = 2u };
 enum : std::uint32_t { kStatusContactOverflow = 1u, kStatusScratchOverflow = 2u, kStatusEventOverflow = 4u };
 
 // Per-world header (AoS: read once per phase by the world's workgroup).
+// Words of WorldHeader::stats: PreStepStats then RegStepStats, each in the
+// reference's declaration order (StepStats.hpp:38-46, 69-82); the two real
+// fields hold float bits.
+enum StatIndex : int {
+    kStPreProxiesCreated = 0, kStPreProxiesMoved, kStPreDestroyed, kStPreAdded, kStPreUpdated, kStPreSkipped,
+    kStRegMinSeparation, kStRegMaxIncImpulse, kStRegIslandsFound, kStRegIslandsSolved, kStRegBodiesSlept,
+    kStRegMaxIslandBodies, kStRegAdded, kStRegUpdated, kStRegSkipped, kStRegProxiesMoved, kStRegSumPosIters,
+    kStRegSumVelIters, kStats
+};
+
 struct WorldHeader {
     std::int32_t body_off, nbodies, proxy_off, nproxies;
     std::int32_t contact_off, contact_cap, ncontacts, pool_size;
     std::int32_t nfree, status, joint_off, njoints;
-    std::uint32_t stats[8];
+    std::uint32_t stats[18];  // StatIndex
     // Contact listener calls of the last step: word segment [ev_off, ev_off +
     // ev_cap) of DeviceState::ev (ev_cap 0: not recorded), words and calls
     // written (words may exceed ev_cap: then the status has kStatusEventOverflow).

@@ -1481,7 +1481,8 @@ __device__ __forceinline__ void build_adjacency(const DeviceState& d, const Worl
 // 1: flags in LDS, 0: everything in global scratch.  Flag bytes per body:
 // 0 free, 1 islanded, 2 static (never pushed).  Returns the island count.
 template <int kMode>
-__device__ __forceinline__ int find_islands(const DeviceState& d, const WorldCtx& x, int nc, const IslandScratch& S) {
+__device__ __forceinline__ int find_islands(const DeviceState& d, const WorldCtx& x, int nc, const IslandScratch& S,
+                                            int& maxBodies) {
     __shared__ int s_cnt[2];
     const int nent = S.off[x.nb];
     int* off = S.off;
@@ -1540,12 +1541,20 @@ __device__ __forceinline__ int find_islands(const DeviceState& d, const WorldCtx
     if (x.lane == 0) {
         int nisl = nsingle, nbe = nsingle, nce = 0, nje = 0;
         int* stack = d.s_stack + 2 * x.bo;
+        // The reference's island body lists also hold the bodies without
+        // velocity (static) that its contacts and joints reach, once per island
+        // (AabbTreeWorld.cpp:1386-1398); they count in maxIslandBodies.  They
+        // are marked 3 while counted and listed from the top of the stack
+        // array (pushes plus listed bodies never exceed the world's bodies).
+        int maxB = nsingle > 0 ? 1 : 0;
         for (int seed = 0; seed < x.nb; ++seed) {
             if (islb[seed]) continue;
             const std::uint32_t sf = d.bflags[x.bo + seed];
             if (!((sf & prx::kAwake) && (sf & prx::kEnabled))) continue;
             S.istart[nisl] = int2{nbe, nce};
             S.jstart[nisl] = nje;
+            const int nbe0 = nbe;
+            int nst = 0;
             int sp = 0;
             stack[sp++] = seed;
             islb[seed] = 1;
@@ -1562,6 +1571,9 @@ __device__ __forceinline__ int find_islands(const DeviceState& d, const WorldCtx
                     if (islb[en.y] == 0) {
                         islb[en.y] = 1;
                         stack[sp++] = en.y;
+                    } else if (islb[en.y] == 2) {
+                        islb[en.y] = 3;
+                        stack[x.nb - 1 - nst++] = en.y;
                     }
                 }
                 // AddJointsToIsland (AabbTreeWorld.cpp:1446-1470).
@@ -1576,18 +1588,27 @@ __device__ __forceinline__ int find_islands(const DeviceState& d, const WorldCtx
                         if (islb[je.y] == 0) {
                             islb[je.y] = 1;
                             stack[sp++] = je.y;
+                        } else if (islb[je.y] == 2) {
+                            islb[je.y] = 3;
+                            stack[x.nb - 1 - nst++] = je.y;
                         }
                     }
                 }
             }
+            for (int q = 0; q < nst; ++q) islb[stack[x.nb - 1 - q]] = 2;
+            maxB = (nbe - nbe0 + nst > maxB) ? nbe - nbe0 + nst : maxB;
             ++nisl;
         }
         S.istart[nisl] = int2{nbe, nce};
         S.jstart[nisl] = nje;
         s_cnt[0] = nisl;
+        s_cnt[1] = maxB;
     }
     __syncthreads();
-    return s_cnt[0];
+    maxBodies = s_cnt[1];
+    const int n = s_cnt[0];
+    __syncthreads();
+    return n;
 }
 
 
@@ -1778,12 +1799,30 @@ __device__ __forceinline__ bool joints_position_tape(const float4* jbase, int j0
 // Lane form of the warm start and velocity sweeps of one island
 // (AabbTreeWorld.cpp:1590-1625): contact warm start, joint warm start, then
 // per iteration the joints followed by the contacts.
+// What one island's sweeps report for the step statistics (IslandStats,
+// StepStats.hpp:149-163): the largest incremental impulse of any velocity
+// iteration and the iterations run; the smallest separation of any position
+// iteration and the iteration that solved the island plus one (0: not solved).
+struct VelResult {
+    float maxInc;
+    int iters;
+};
+struct PosResult {
+    float minSep;
+    int solvedAt;
+};
+
 template <bool kLds, bool kTape = false>
-__device__ __forceinline__ void island_velocity(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
-                                                int c0, int c1, int j0, int j1, const StepParams& c,
-                                                int islandSlot = 0) {
+__device__ __forceinline__ VelResult island_velocity(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
+                                                     int c0, int c1, int j0, int j1, const StepParams& c,
+                                                     int islandSlot = 0) {
+    VelResult res{0.0f, int(c.velIters)};
     if constexpr (kTape) d.tape.isl[islandSlot].z = 0;
-    if (c0 >= c1 && j0 >= j1) return;
+    if (c0 >= c1 && j0 >= j1) {
+        // Nothing to solve: the first iteration's increment is zero.
+        if (c.velIters > 0 && 0.0f <= c.regMinMomentum) res.iters = 1;
+        return res;
+    }
     float4* base = d.s_vc + std::size_t(x.co) * prx::kVcStride;
     float4* jbase = d.s_jc + std::size_t(x.jo) * prx::kJcStride;
     if (c.warmStart && c0 < c1) {
@@ -1807,7 +1846,10 @@ __device__ __forceinline__ void island_velocity(const DeviceState& d, const Worl
         else jointsOkay = joints_velocity(jbase, j0, j1, bs, false);
         float newInc = 0.0f;
         if (c0 >= c1) {
-            if (jointsOkay && newInc <= c.regMinMomentum) break;
+            if (jointsOkay && newInc <= c.regMinMomentum) {
+                res.iters = int(it) + 1;
+                break;
+            }
             continue;
         }
         BodyC A, B;
@@ -1832,8 +1874,13 @@ __device__ __forceinline__ void island_velocity(const DeviceState& d, const Worl
             A = nA;
             B = nB;
         }
-        if (jointsOkay && newInc <= c.regMinMomentum) break;
+        res.maxInc = fmax_std(res.maxInc, newInc);
+        if (jointsOkay && newInc <= c.regMinMomentum) {
+            res.iters = int(it) + 1;
+            break;
+        }
     }
+    return res;
 }
 
 // Chunk staging of kStride-float4 records [c0, c1) of a global array through
@@ -1871,9 +1918,10 @@ struct Stager {
 };
 
 template <bool kLds>
-__device__ __forceinline__ void island_velocity_wave(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
-                                                     float4* stage, int c0, int c1, int j0, int j1,
-                                                     const StepParams& c) {
+__device__ __forceinline__ VelResult island_velocity_wave(const DeviceState& d, const WorldCtx& x,
+                                                          const Bodies<kLds>& bs, float4* stage, int c0, int c1,
+                                                          int j0, int j1, const StepParams& c) {
+    VelResult res{0.0f, int(c.velIters)};
     float4* jbase = d.s_jc + std::size_t(x.jo) * prx::kJcStride;
     float4* gbase = d.s_vc + (std::size_t(x.co) + std::size_t(c0)) * prx::kVcStride;
     const int n = c1 - c0;
@@ -1930,7 +1978,13 @@ __device__ __forceinline__ void island_velocity_wave(const DeviceState& d, const
         if (warm && x.lane == 0) joints_velocity(jbase, j0, j1, bs, true);
         newInc = __shfl(newInc, 0);
         jointsOkay = __shfl(int(jointsOkay), 0) != 0;
-        if (!warm && jointsOkay && newInc <= c.regMinMomentum) break;
+        if (!warm) {
+            res.maxInc = fmax_std(res.maxInc, newInc);
+            if (jointsOkay && newInc <= c.regMinMomentum) {
+                res.iters = pass - (c.warmStart ? 1 : 0) + 1;
+                break;
+            }
+        }
     }
     if (resident && x.lane < 2 * n) {
         const int r = x.lane >> 1;
@@ -1938,18 +1992,18 @@ __device__ __forceinline__ void island_velocity_wave(const DeviceState& d, const
         gbase[f] = stage[f];
     }
     __syncthreads();
+    return res;
 }
 
 // Lane form of the position sweeps of one island (ContactSolver.cpp:494-643,
 // AabbTreeWorld.cpp:1627-1644).  Returns whether the island was solved.
-// The position sweeps return the iteration that solved the island plus one
-// (0: not solved within the iteration limit).
 template <bool kLds, bool kTape = false>
-__device__ __forceinline__ int island_position(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
+__device__ __forceinline__ PosResult island_position(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
                                                 int c0, int c1, int j0, int j1, const StepParams& c,
                                                 int islandSlot = 0) {
     const float4* base = d.s_pc + std::size_t(x.co) * prx::kPcStride;
     const float4* jbase = d.s_jc + std::size_t(x.jo) * prx::kJcStride;
+    PosResult res{__builtin_inff(), 0};
     if constexpr (kTape) d.tape.isl[islandSlot].w = 0;
     for (std::uint32_t it = 0; it < c.posIters; ++it) {
         if constexpr (kTape) d.tape.isl[islandSlot].w = int(it) + 1;
@@ -1977,17 +2031,21 @@ __device__ __forceinline__ int island_position(const DeviceState& d, const World
                 pc = npc;
             }
         }
+        res.minSep = fmin_std(res.minSep, minSep);
         const bool contactsOkay = minSep >= c.regMinSeparation;
         bool jointsOkay;
         if constexpr (kTape) jointsOkay = joints_position_tape(jbase, j0, j1, bs, c, d.tape, x.jo, int(it));
         else jointsOkay = joints_position(jbase, j0, j1, bs, c);
-        if (contactsOkay && jointsOkay) return int(it) + 1;
+        if (contactsOkay && jointsOkay) {
+            res.solvedAt = int(it) + 1;
+            return res;
+        }
     }
-    return 0;
+    return res;
 }
 
 template <bool kLds>
-__device__ __forceinline__ int island_position_wave(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
+__device__ __forceinline__ PosResult island_position_wave(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
                                                      float4* stage, int c0, int c1, int j0, int j1,
                                                      const StepParams& c) {
     const float4* jbase = d.s_jc + std::size_t(x.jo) * prx::kJcStride;
@@ -2000,7 +2058,7 @@ __device__ __forceinline__ int island_position_wave(const DeviceState& d, const 
     sg.commit(0);
     __syncthreads();
     const int npass = int(c.posIters);
-    int solved = 0;
+    PosResult res{__builtin_inff(), 0};
     for (int pass = 0; pass < npass; ++pass) {
         float minSep = __builtin_inff();
         for (int ch = 0; ch < nch; ++ch) {
@@ -2032,13 +2090,14 @@ __device__ __forceinline__ int island_position_wave(const DeviceState& d, const 
         if (x.lane == 0) jointsOkay = joints_position(jbase, j0, j1, bs, c);
         minSep = __shfl(minSep, 0);
         jointsOkay = __shfl(int(jointsOkay), 0) != 0;
+        res.minSep = fmin_std(res.minSep, minSep);
         if (minSep >= c.regMinSeparation && jointsOkay) {
-            solved = pass + 1;
+            res.solvedAt = pass + 1;
             break;
         }
     }
     __syncthreads();
-    return solved;
+    return res;
 }
 
 // ---------------------------------------------------------------------------
@@ -2169,8 +2228,10 @@ __device__ __forceinline__ LevelPlan level_plan_of(const DeviceState& d, const W
 // records are loaded while round k is solved (a record is read and written
 // by its own update only, once per pass).
 template <bool kLds>
-__device__ __forceinline__ void island_velocity_level(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
-                                                      const LevelPlan& P, int c0, const StepParams& c) {
+__device__ __forceinline__ VelResult island_velocity_level(const DeviceState& d, const WorldCtx& x,
+                                                           const Bodies<kLds>& bs, const LevelPlan& P, int c0,
+                                                           const StepParams& c) {
+    VelResult res{0.0f, int(c.velIters)};
     float4* vbase = d.s_vc + (std::size_t(x.co) + std::size_t(c0)) * prx::kVcStride;
     float4* v2 = d.s_vc2 + (std::size_t(x.co) + std::size_t(c0)) * prx::kVcStride;
     const int npass = int(c.velIters) + (c.warmStart ? 1 : 0);
@@ -2213,7 +2274,11 @@ __device__ __forceinline__ void island_velocity_level(const DeviceState& d, cons
         if (!warm) {
 #pragma unroll
             for (int off = kWave / 2; off > 0; off >>= 1) newInc = fmax_std(newInc, __shfl_xor(newInc, off));
-            if (newInc <= c.regMinMomentum) break;
+            res.maxInc = fmax_std(res.maxInc, newInc);
+            if (newInc <= c.regMinMomentum) {
+                res.iters = pass - (c.warmStart ? 1 : 0) + 1;
+                break;
+            }
         }
     }
     for (int pos = x.lane; pos < P.n; pos += kWave) {
@@ -2223,12 +2288,13 @@ __device__ __forceinline__ void island_velocity_level(const DeviceState& d, cons
         dst[7] = src[7];
     }
     __syncthreads();
+    return res;
 }
 
 // Position sweeps of one island in level form (records prefetched a round
 // ahead); returns whether solved.
 template <bool kLds>
-__device__ __forceinline__ int island_position_level(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
+__device__ __forceinline__ PosResult island_position_level(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
                                                       const LevelPlan& P, int c0, const StepParams& c) {
     const float4* p2 = d.s_pc2 + (std::size_t(x.co) + std::size_t(c0)) * prx::kPcStride;
     auto fetch = [&](int2 r, float4 (&buf)[prx::kPcStride]) {
@@ -2239,6 +2305,7 @@ __device__ __forceinline__ int island_position_level(const DeviceState& d, const
             for (int f = 0; f < prx::kPcStride; ++f) buf[f] = src[f];
         }
     };
+    PosResult res{__builtin_inff(), 0};
     for (std::uint32_t it = 0; it < c.posIters; ++it) {
         float minSep = __builtin_inff();
         float4 cur[prx::kPcStride], nxt[prx::kPcStride];
@@ -2263,18 +2330,45 @@ __device__ __forceinline__ int island_position_level(const DeviceState& d, const
         }
 #pragma unroll
         for (int off = kWave / 2; off > 0; off >>= 1) minSep = fmin_std(minSep, __shfl_xor(minSep, off));
-        if (minSep >= c.regMinSeparation) return int(it) + 1;
+        res.minSep = fmin_std(res.minSep, minSep);
+        if (minSep >= c.regMinSeparation) {
+            res.solvedAt = int(it) + 1;
+            return res;
+        }
     }
-    return 0;
+    return res;
 }
 
 __device__ __forceinline__ bool level_form(const WorldCtx& x, int c0, int c1, int j0, int j1) {
     return j0 == j1 && x.nb <= kLevelMaxBodies && (c1 - c0) >= kLevelMinConstraints;
 }
 
+// What the island solves add to RegStepStats (Update(RegStepStats&,
+// IslandStats), StepStats.cpp:29-38).
+struct SolveSummary {
+    std::uint32_t slept, solved, sumPos, sumVel;
+    float minSep, maxInc;
+};
+
 template <bool kLds, bool kTape = false>
-__device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, const WorldCtx& x, int nisl,
-                                                       const IslandScratch& S, const StepParams& c, PhaseTimer& T) {
+__device__ __forceinline__ SolveSummary solve_islands(const DeviceState& d, const WorldCtx& x, int nisl,
+                                                      const IslandScratch& S, const StepParams& c, PhaseTimer& T) {
+    // Per-lane sums and extrema of the islands each lane reports (lane-form
+    // islands on their lane, wave-form islands on lane 0), reduced at the end.
+    // The extrema do not depend on the order: incremental impulses are
+    // magnitudes (never -0) and no separation is -0 (a - b of equal values
+    // is +0 and the total radius is positive).
+    float aMin = __builtin_inff(), aMax = 0.0f;
+    std::uint32_t aVel = 0, aPos = 0, aSolved = 0;
+    auto addVel = [&](const VelResult& r) {
+        aMax = fmax_std(aMax, r.maxInc);
+        aVel += std::uint32_t(r.iters);
+    };
+    auto addPos = [&](const PosResult& r) {
+        aMin = fmin_std(aMin, r.minSep);
+        aPos += r.solvedAt ? std::uint32_t(r.solvedAt) : c.posIters;
+        aSolved += r.solvedAt ? 1u : 0u;
+    };
     const int bo = x.bo;
     const Bodies<kLds> bs{d.s_bc_pos + bo, d.s_bc_vel + bo, x.nb};
     const int nbe = S.istart[nisl].x;
@@ -2378,8 +2472,8 @@ __device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, con
     };
     for (int is = x.lane; is < nisl; is += kWave) {
         if (constraints(is) < coopMin)
-            island_velocity<kLds, kTape>(d, x, bs, S.istart[is].y, S.istart[is + 1].y, S.jstart[is],
-                                         S.jstart[is + 1], c, x.bo + x.w + is);
+            addVel(island_velocity<kLds, kTape>(d, x, bs, S.istart[is].y, S.istart[is + 1].y, S.jstart[is],
+                                                S.jstart[is + 1], c, x.bo + x.w + is));
     }
     __syncthreads();
     for (int base = 0; base < nisl; base += kWave) {
@@ -2388,12 +2482,14 @@ __device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, con
         for (unsigned long long m = __ballot(big); m; m &= m - 1) {
             const int isl = base + __ffsll(m) - 1;
             const int c0 = S.istart[isl].y, c1 = S.istart[isl + 1].y;
+            VelResult r;
             if (level_form(x, c0, c1, S.jstart[isl], S.jstart[isl + 1])) {
                 const LevelPlan P = level_plan(d, x, stage, c0, c1);
-                island_velocity_level<kLds>(d, x, bs, P, c0, c);
+                r = island_velocity_level<kLds>(d, x, bs, P, c0, c);
             } else {
-                island_velocity_wave<kLds>(d, x, bs, stage, c0, c1, S.jstart[isl], S.jstart[isl + 1], c);
+                r = island_velocity_wave<kLds>(d, x, bs, stage, c0, c1, S.jstart[isl], S.jstart[isl + 1], c);
             }
+            if (x.lane == 0) addVel(r);
         }
     }
     T.mark(prx::kPhVelocity);
@@ -2411,9 +2507,12 @@ __device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, con
     __syncthreads();
     // Position sweeps, split like the velocity sweeps.
     for (int is = x.lane; is < nisl; is += kWave) {
-        if (constraints(is) < coopMin)
-            S.isl_word[is] = std::uint32_t(island_position<kLds, kTape>(
-                d, x, bs, S.istart[is].y, S.istart[is + 1].y, S.jstart[is], S.jstart[is + 1], c, x.bo + x.w + is));
+        if (constraints(is) < coopMin) {
+            const PosResult r = island_position<kLds, kTape>(d, x, bs, S.istart[is].y, S.istart[is + 1].y,
+                                                             S.jstart[is], S.jstart[is + 1], c, x.bo + x.w + is);
+            S.isl_word[is] = std::uint32_t(r.solvedAt);
+            addPos(r);
+        }
     }
     __syncthreads();
     for (int base = 0; base < nisl; base += kWave) {
@@ -2422,11 +2521,14 @@ __device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, con
         for (unsigned long long m = __ballot(big); m; m &= m - 1) {
             const int isl = base + __ffsll(m) - 1;
             const int c0 = S.istart[isl].y, c1 = S.istart[isl + 1].y;
-            const int solved = level_form(x, c0, c1, S.jstart[isl], S.jstart[isl + 1])
-                                   ? island_position_level<kLds>(d, x, bs, level_plan_of(d, x, c0, c1), c0, c)
-                                   : island_position_wave<kLds>(d, x, bs, stage, c0, c1, S.jstart[isl],
-                                                                S.jstart[isl + 1], c);
-            if (x.lane == 0) S.isl_word[isl] = std::uint32_t(solved);
+            const PosResult r = level_form(x, c0, c1, S.jstart[isl], S.jstart[isl + 1])
+                                    ? island_position_level<kLds>(d, x, bs, level_plan_of(d, x, c0, c1), c0, c)
+                                    : island_position_wave<kLds>(d, x, bs, stage, c0, c1, S.jstart[isl],
+                                                                 S.jstart[isl + 1], c);
+            if (x.lane == 0) {
+                S.isl_word[isl] = std::uint32_t(r.solvedAt);
+                addPos(r);
+            }
         }
     }
     __syncthreads();
@@ -2504,10 +2606,17 @@ __device__ __forceinline__ std::uint32_t solve_islands(const DeviceState& d, con
             }
         }
     }
-    for (int o = 32; o > 0; o >>= 1) slept += __shfl_xor(slept, o);
+    for (int o = 32; o > 0; o >>= 1) {
+        slept += __shfl_xor(slept, o);
+        aSolved += __shfl_xor(aSolved, o);
+        aPos += __shfl_xor(aPos, o);
+        aVel += __shfl_xor(aVel, o);
+        aMin = fmin_std(aMin, __shfl_xor(aMin, o));
+        aMax = fmax_std(aMax, __shfl_xor(aMax, o));
+    }
     __syncthreads();
     T.mark(prx::kPhWriteback);
-    return slept;
+    return SolveSummary{slept, aSolved, aPos, aVel, aMin, aMax};
 }
 
 // Synchronize (AabbTreeWorld.cpp:2564-2585) for one body.
@@ -2590,29 +2699,44 @@ __device__ __forceinline__ void step_world(const DeviceState& d, int w, const St
     k.status = std::uint32_t(H.status);
     k.evw = 0;
     k.evn = 0;
-    std::uint32_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // Step statistics go to the header as they are produced (lane 0).
+    auto stat = [&](int i, std::uint32_t v) {
+        if (x.lane == 0) H.stats[i] = v;
+    };
+    if (x.lane == 0) {
+        for (int i = 0; i < prx::kStats; ++i) H.stats[i] = 0u;
+        H.stats[prx::kStRegMinSeparation] = 0x7f800000u;  // +infinity: no position iteration ran
+    }
     PhaseTimer T;
     T.begin(d.phase_cycles ? d.phase_cycles + std::size_t(w) * prx::kPhases : nullptr);
 
     // A: proxies of newly attached shapes (CreateProxies, AabbTreeWorld.cpp:557-579).
+    std::uint32_t created = 0;
     for (int p = x.lane; p < x.np; p += kWave) {
         const int gp = x.po + p;
         if (d.pflags[gp] & prx::kProxyPending) {
             const int gb = x.bo + d.pbody[gp];
             d.paabb[gp] = proxy_aabb(shape_proxy(d, d.pshape[gp]), body_xf0(d, gb), body_xf1(d, gb), c, true);
             d.pflags[gp] = prx::kProxyMoved;
+            ++created;
         }
     }
+    for (int o = 32; o > 0; o >>= 1) created += __shfl_xor(created, o);
+    stat(prx::kStPreProxiesCreated, created);
     __syncthreads();
     T.mark(prx::kPhProxies);
     // B, C, D.
-    stats[1] = phase_destroy(d, x, k);
+    stat(prx::kStPreDestroyed, phase_destroy(d, x, k));
     __syncthreads();
     T.mark(prx::kPhDestroy);
-    stats[3] = phase_update(d, x, k, c);
+    {
+        const std::uint32_t updated = phase_update(d, x, k, c);
+        stat(prx::kStPreUpdated, updated);
+        stat(prx::kStPreSkipped, std::uint32_t(k.nc) - updated);
+    }
     __syncthreads();
     T.mark(prx::kPhUpdate);
-    stats[2] = phase_find_add(d, x, k, c);
+    stat(prx::kStPreAdded, phase_find_add(d, x, k, c));
     __syncthreads();
     T.mark(prx::kPhFindAdd);
 
@@ -2632,15 +2756,22 @@ __device__ __forceinline__ void step_world(const DeviceState& d, int w, const St
         T.mark(prx::kPhAdjacency);
         const int nent = S.off[x.nb];
         const int ldsAdj = 4 * (((x.nb + 2) >> 1) * 2) + 8 * nent + x.nb + k.nc + x.nj;
-        int nisl;
-        if (ldsAdj <= int(c.ldsBytes)) nisl = find_islands<2>(d, x, k.nc, S);
-        else if (x.nb + k.nc + x.nj <= int(c.ldsBytes)) nisl = find_islands<1>(d, x, k.nc, S);
-        else nisl = find_islands<0>(d, x, k.nc, S);
-        stats[4] = std::uint32_t(nisl);
+        int nisl, maxBodies;
+        if (ldsAdj <= int(c.ldsBytes)) nisl = find_islands<2>(d, x, k.nc, S, maxBodies);
+        else if (x.nb + k.nc + x.nj <= int(c.ldsBytes)) nisl = find_islands<1>(d, x, k.nc, S, maxBodies);
+        else nisl = find_islands<0>(d, x, k.nc, S, maxBodies);
+        stat(prx::kStRegIslandsFound, std::uint32_t(nisl));
+        stat(prx::kStRegMaxIslandBodies, std::uint32_t(maxBodies));
         T.mark(prx::kPhIslands);
         const bool ldsBodies = x.nb * 32 + prx::kStageFloat4 * 16 <= int(c.ldsBytes);
-        stats[7] = ldsBodies ? solve_islands<true, kTape>(d, x, nisl, S, c, T)
-                             : solve_islands<false, kTape>(d, x, nisl, S, c, T);
+        const SolveSummary sv = ldsBodies ? solve_islands<true, kTape>(d, x, nisl, S, c, T)
+                                          : solve_islands<false, kTape>(d, x, nisl, S, c, T);
+        stat(prx::kStRegBodiesSlept, sv.slept);
+        stat(prx::kStRegIslandsSolved, sv.solved);
+        stat(prx::kStRegSumPosIters, sv.sumPos);
+        stat(prx::kStRegSumVelIters, sv.sumVel);
+        stat(prx::kStRegMinSeparation, __float_as_uint(sv.minSep));
+        stat(prx::kStRegMaxIncImpulse, __float_as_uint(sv.maxInc));
         if (x.ecap > 0) post_solve_calls(d, x, k, S, nisl);
         // Synchronise proxies of islanded speedable bodies (AabbTreeWorld.cpp:1532-1544).
         std::uint32_t moved = 0;
@@ -2648,7 +2779,7 @@ __device__ __forceinline__ void step_world(const DeviceState& d, int w, const St
             if (S.bisland[b] >= 0) moved += synchronize_body(d, x, b, c);
         }
         for (int o = 32; o > 0; o >>= 1) moved += __shfl_xor(moved, o);
-        stats[6] = moved;
+        stat(prx::kStRegProxiesMoved, moved);
         __syncthreads();
         T.mark(prx::kPhSync);
         // Contacts of bodies whose position changed need updating.
@@ -2659,10 +2790,14 @@ __device__ __forceinline__ void step_world(const DeviceState& d, int w, const St
         __syncthreads();
         for (int b = x.lane; b < x.nb; b += kWave) d.bflags[x.bo + b] &= ~0x100u;
         __syncthreads();
-        phase_update(d, x, k, c);
+        {
+            const std::uint32_t updated = phase_update(d, x, k, c);
+            stat(prx::kStRegUpdated, updated);
+            stat(prx::kStRegSkipped, std::uint32_t(k.nc) - updated);
+        }
         __syncthreads();
         T.mark(prx::kPhUpdate2);
-        stats[5] = phase_find_add(d, x, k, c);
+        stat(prx::kStRegAdded, phase_find_add(d, x, k, c));
         __syncthreads();
         T.mark(prx::kPhFindAdd2);
     }
@@ -2672,7 +2807,6 @@ __device__ __forceinline__ void step_world(const DeviceState& d, int w, const St
         H.nfree = k.nfree;
         H.pool_size = k.pool;
         H.status = int(k.status);
-        for (int i = 0; i < 8; ++i) H.stats[i] = stats[i];
         H.ev_words = k.evw;
         H.ev_count = k.evn;
     }

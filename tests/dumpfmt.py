@@ -23,6 +23,12 @@ JOINT_F = 4
 # two contact features, 12 floats), 3 post-solve (count, solved position
 # iteration, normal impulses [2], tangent impulses [2]).
 EVENT_PAYLOAD = (0, 0, 15, 6)
+# StepStats words: PreStepStats (proxies created, proxies moved, contacts
+# destroyed, added, updated, skipped), then RegStepStats (min separation and
+# max incremental impulse as float bits, islands found, solved, bodies slept,
+# max island bodies, contacts added, updated, skipped, proxies moved, sums of
+# position and velocity iterations).
+STATS = 18
 
 
 class Reader:
@@ -81,7 +87,7 @@ def read_conf(r):
 
 def read_state(r):
     step = r.u32()
-    stats = r.arr("<u4", 8)
+    stats = r.arr("<u4", STATS)
     nb = r.u32()
     raw = r.arr("<u4", nb * (BODY_F + 1)).reshape(nb, BODY_F + 1)
     bodies = raw[:, :BODY_F].copy().view("<f4")
@@ -136,7 +142,7 @@ def read_dump(buf):
     return dict(shapes=shapes, bodies=bodies, joints=joints, conf=conf, states=states)
 
 
-def digest(state, parts=("bodies", "contacts", "proxies", "joints", "events")):
+def digest(state, parts=("bodies", "contacts", "proxies", "joints", "events", "stats")):
     """SHA-256 over the canonical bytes of one state (selected parts)."""
     h = hashlib.sha256()
     if "bodies" in parts:
@@ -155,4 +161,6 @@ def digest(state, parts=("bodies", "contacts", "proxies", "joints", "events")):
     if "events" in parts:
         h.update(np.uint32(state["nevents"]).tobytes())
         h.update(np.ascontiguousarray(state["events"]).tobytes())
+    if "stats" in parts:
+        h.update(np.ascontiguousarray(state["stats"]).tobytes())
     return h.digest()

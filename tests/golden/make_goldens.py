@@ -9,7 +9,7 @@ For each case it runs oracle/_ref/ref_driver (our driver linked against the
 reference library), and stores in tests/golden/<case>.npz:
   desc_conf  the world description and StepConf as the reference built them
   digests    per step, SHA-256 of bodies / contact list / proxies / joints /
-             contact listener calls
+             contact listener calls / StepStats
   full_*     complete state at a few steps (for diffs when a digest differs)
 HelloWorld/ExpectedOutput.txt (the reference's own golden output) is copied
 as data to tests/golden/hello_expected.txt.

@@ -6,7 +6,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLDEN = os.path.join(HERE, "golden")
-PARTS = ("bodies", "contacts", "proxies", "joints", "events")
+PARTS = ("bodies", "contacts", "proxies", "joints", "events", "stats")
 
 
 def names():
