@@ -276,12 +276,17 @@ def articulated(args, pa, conf, threads):
 
 
 def conf_record(pa):
+    """The StepConf record the oracle reads (the default conf)."""
     import struct
     c = pa.StepConf()
     floats = [getattr(c, f) for f, _ in pa.StepConf._fields_[:18]]
+    toi_f = [c.toi_resolution_rate, c.toi_min_separation, c.toi_min_momentum, c.target_depth, c.tolerance]
+    toi_i = [c.toi_velocity_iters, c.toi_position_iters, c.max_toi_root_iters, c.max_toi_iters,
+             c.max_distance_iters, c.max_sub_steps]
     return (struct.pack("<I", 18) + struct.pack("<18f", *floats) +
             struct.pack("<III", c.reg_velocity_iters, c.reg_position_iters,
-                        (1 if c.do_warm_start else 0) | (4 if c.do_blocksolve else 0)))
+                        (1 if c.do_warm_start else 0) | (4 if c.do_blocksolve else 0)) +
+            struct.pack("<I", 5) + struct.pack("<5f", *toi_f) + struct.pack("<6I", *toi_i))
 
 
 def main():

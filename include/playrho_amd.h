@@ -106,6 +106,10 @@ typedef struct prx_step_conf {
     float aabb_extension, max_circles_ratio;
     uint32_t reg_velocity_iters, reg_position_iters;
     uint32_t do_warm_start, do_toi, do_blocksolve;
+    /* TOI phase (StepConf.hpp:84-158, 216-374). */
+    float toi_resolution_rate, toi_min_separation, toi_min_momentum, target_depth, tolerance;
+    uint32_t toi_velocity_iters, toi_position_iters, max_toi_root_iters, max_toi_iters;
+    uint32_t max_distance_iters, max_sub_steps;
 } prx_step_conf;
 
 /* StepStats (StepStats.hpp:30-201) of a regular step: PreStepStats

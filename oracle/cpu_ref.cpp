@@ -145,6 +145,23 @@ void Load(Handle<R>& h, In& in) {
     c.warmStart = (fl & 1u) != 0;
     c.toi = (fl & 2u) != 0;
     c.blockSolve = (fl & 4u) != 0;
+    const std::uint32_t nt = in.u32();
+    R tf[8] = {};
+    for (std::uint32_t k = 0; k < nt; ++k) {
+        const R v = R(in.f32());
+        if (k < 8) tf[k] = v;
+    }
+    c.toiResolutionRate = tf[0];
+    c.toiMinSeparation = tf[1];
+    c.toiMinMomentum = tf[2];
+    c.targetDepth = tf[3];
+    c.tolerance = tf[4];
+    c.toiVelIters = in.u32();
+    c.toiPosIters = in.u32();
+    c.maxToiRootIters = in.u32();
+    c.maxToiIters = in.u32();
+    c.maxDistanceIters = in.u32();
+    c.maxSubSteps = in.u32();
 }
 
 template <class R>

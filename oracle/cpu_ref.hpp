@@ -572,6 +572,11 @@ struct StepConf {
         displaceMultiplier, aabbExtension, maxCirclesRatio;
     std::uint32_t velIters, posIters;
     bool warmStart, toi, blockSolve;
+    // TOI phase (StepConf.hpp:84-158, 216-374).
+    R toiResolutionRate = R(0.75f), toiMinSeparation = R(-0.005f * 1.5f), toiMinMomentum = 0,
+      targetDepth = R(0.005f * 3.0f), tolerance = R(0.005f / 4.0f);
+    std::uint32_t toiVelIters = 8, toiPosIters = 20, maxToiRootIters = 30, maxToiIters = 20, maxDistanceIters = 20,
+                  maxSubSteps = 8;
 };
 
 // Body flags as in the dump format.
@@ -602,6 +607,7 @@ struct Body {
     Vel<R> vel;
     V2<R> lacc;
     R aacc = 0, ldamp = 0, adamp = 0, invMass = 0, invRotI = 0, underActive = 0;
+    R alpha0 = 0;  // Sweep::alpha0 (Sweep.hpp:67)
     Xf<R> xf;  // cached transform of pos1 (Body.hpp:697-715)
     std::vector<std::uint32_t> shapes;
     std::vector<std::uint32_t> proxies;
@@ -641,6 +647,10 @@ struct Contact {
     std::uint32_t bodyA = 0, bodyB = 0, shapeA = 0, shapeB = 0;
     std::uint32_t flags = 0;
     R friction = 0, restitution = 0, tangentSpeed = 0;
+    // TOI state (Contact.hpp:334-338, 451-480): value, e_toiFlag, sub-step count.
+    R toi = 0;
+    bool toiValid = false;
+    std::uint32_t toiCount = 0;
 };
 
 // PreStepStats and RegStepStats (StepStats.hpp:38-46, 69-82); the two real
@@ -1857,7 +1867,635 @@ public:
         st.regAdded = FindAndAddContacts(conf);
     }
 
-    // Step (AabbTreeWorld.cpp:2144-2219) with doToi == false.
+    // ---------------------------------------------------------------------
+    // TOI phase (doToi = true): SolveToi (AabbTreeWorld.cpp:1752-1831) and the
+    // functions it calls.
+
+    // Sweep of a body (Sweep.hpp:48-68) and its functions.
+    struct SweepR {
+        Pos<R> pos0, pos1;
+        V2<R> lc;
+        R alpha0;
+    };
+    static constexpr R kPi = R(3.14159265358979323846264338327950288L);  // RealConstants.hpp:57
+    static SweepR SweepOf(const Body<R>& b) { return SweepR{b.pos0, b.pos1, b.lc, b.alpha0}; }
+    // GetPosition(pos0, pos1, beta) = pos0 + (pos1 - pos0) * beta (d2/Math.cpp:30-59).
+    static Pos<R> Lerp(const Pos<R>& p0, const Pos<R>& p1, R beta) {
+        const V2<R> dl = p1.lin - p0.lin;
+        const R da = p1.ang - p0.ang;
+        return Pos<R>{p0.lin + dl * beta, p0.ang + da * beta};
+    }
+    // Advance0(sweep, alpha) (Sweep.cpp:29-33).
+    static SweepR Advance0(const SweepR& s, R alpha) {
+        const R beta = (alpha - s.alpha0) / (R(1) - s.alpha0);
+        return SweepR{Lerp(s.pos0, s.pos1, beta), s.pos1, s.lc, alpha};
+    }
+    // GetNormalized(Angle), trunc form (Math.cpp:33-91).
+    static R NormalizeAngle(R a) {
+        const R twoPi = R(2) * kPi;
+        const R rTwoPi = R(1) / twoPi;
+        a = a - std::trunc(a * rTwoPi) * twoPi;
+        if (a >= kPi) a -= kPi * R(2);
+        else if (a < -kPi) a += kPi * R(2);
+        return a;
+    }
+    // GetNormalized(Sweep) (d2/Math.hpp:160-167).
+    static SweepR Normalized(SweepR s) {
+        const R a0 = NormalizeAngle(s.pos0.ang);
+        const R d = s.pos0.ang - a0;
+        s.pos0.ang = a0;
+        s.pos1.ang -= d;
+        return s;
+    }
+    // GetTransformation(sweep, beta) (d2/Math.hpp:251-254).
+    static Xf<R> XfAt(const SweepR& s, R beta) { return GetXf(Lerp(s.pos0, s.pos1, beta), s.lc); }
+    // Body::Advance0 / Advance / SetSweep (Body.hpp:697-727, 1004-1009).
+    static void BodyAdvance0(Body<R>& b, R alpha) {
+        const SweepR s = Advance0(SweepOf(b), alpha);
+        b.pos0 = s.pos0;
+        b.alpha0 = s.alpha0;
+    }
+    static void BodyAdvance(Body<R>& b, R alpha) {
+        BodyAdvance0(b, alpha);
+        b.pos1 = b.pos0;
+        b.xf = GetXf(b.pos1, b.lc);
+    }
+    static void SetSweep(Body<R>& b, const SweepR& s) {
+        b.pos0 = s.pos0;
+        b.pos1 = s.pos1;
+        b.lc = s.lc;
+        b.alpha0 = s.alpha0;
+        b.xf = GetXf(b.pos1, b.lc);
+    }
+    void FlagForUpdating(std::uint32_t bid) {
+        for (const auto& kc : bodyContacts[bid]) contactBuf[kc.id].flags |= kNeedsUpdating;
+    }
+
+    // GetSupportIndex (DistanceProxy.hpp:233-249).
+    static std::uint32_t SupportIndex(const Proxy<R>& p, V2<R> dir) {
+        std::uint32_t index = 255;
+        R maxValue = -std::numeric_limits<R>::infinity();
+        for (std::uint32_t i = 0; i < p.n; ++i) {
+            const R value = Dot(p.v[i], dir);
+            if (maxValue < value) {
+                maxValue = value;
+                index = i;
+            }
+        }
+        return index;
+    }
+    static bool AlmostZero(R v) { return std::abs(v) < std::numeric_limits<R>::min(); }  // Math.hpp:177-180
+
+    // SimplexEdge / Simplex (SimplexEdge.hpp, Simplex.cpp:25-241).
+    struct SEdge {
+        V2<R> wA, wB;
+        std::uint32_t iA = 255, iB = 255;
+    };
+    struct Simplex {
+        SEdge e[3];
+        R c[3] = {0, 0, 0};
+        std::uint32_t n = 0;
+    };
+    static V2<R> PD(const SEdge& s) { return s.wB - s.wA; }
+    static Simplex Simplex1(const SEdge& s0) {
+        Simplex r;
+        r.e[0] = s0;
+        r.c[0] = 1;
+        r.n = 1;
+        return r;
+    }
+    static Simplex Simplex2(const SEdge& a, const SEdge& b, R ca, R cb) {
+        Simplex r;
+        r.e[0] = a;
+        r.e[1] = b;
+        r.c[0] = ca;
+        r.c[1] = cb;
+        r.n = 2;
+        return r;
+    }
+    static Simplex SolveSimplex(const SEdge* e, std::uint32_t n) {
+        if (n == 1) return Simplex1(e[0]);
+        if (n == 2) {
+            const V2<R> w1 = PD(e[0]), w2 = PD(e[1]);
+            const V2<R> e12 = w2 - w1;
+            const R d12_2 = -Dot(w1, e12);
+            if (d12_2 <= 0) return Simplex1(e[0]);
+            const R d12_1 = Dot(w2, e12);
+            if (d12_1 <= 0) return Simplex1(e[1]);
+            const R inv = R(1) / (d12_1 + d12_2);
+            return Simplex2(e[0], e[1], d12_1 * inv, d12_2 * inv);
+        }
+        const V2<R> w1 = PD(e[0]), w2 = PD(e[1]), w3 = PD(e[2]);
+        const V2<R> e12 = w2 - w1;
+        const R d12_1 = Dot(w2, e12);
+        const R d12_2 = -Dot(w1, e12);
+        const V2<R> e13 = w3 - w1;
+        const R d13_1 = Dot(w3, e13);
+        const R d13_2 = -Dot(w1, e13);
+        const V2<R> e23 = w3 - w2;
+        const R d23_1 = Dot(w3, e23);
+        const R d23_2 = -Dot(w2, e23);
+        if (d12_2 <= 0 && d13_2 <= 0) return Simplex1(e[0]);
+        if (d12_1 <= 0 && d23_2 <= 0) return Simplex1(e[1]);
+        if (d13_1 <= 0 && d23_1 <= 0) return Simplex1(e[2]);
+        const R n123 = Cross(e12, e13);
+        const R d123_3 = n123 * Cross(w1, w2);
+        if (d12_1 > 0 && d12_2 > 0 && d123_3 <= 0) {
+            const R inv = R(1) / (d12_1 + d12_2);
+            return Simplex2(e[0], e[1], d12_1 * inv, d12_2 * inv);
+        }
+        const R d123_2 = n123 * Cross(w3, w1);
+        if (d13_1 > 0 && d13_2 > 0 && d123_2 <= 0) {
+            const R inv = R(1) / (d13_1 + d13_2);
+            return Simplex2(e[0], e[2], d13_1 * inv, d13_2 * inv);
+        }
+        const R d123_1 = n123 * Cross(w2, w3);
+        if (d23_1 > 0 && d23_2 > 0 && d123_1 <= 0) {
+            const R inv = R(1) / (d23_1 + d23_2);
+            return Simplex2(e[2], e[1], d23_2 * inv, d23_1 * inv);
+        }
+        const R inv = R(1) / (d123_1 + d123_2 + d123_3);
+        Simplex r;
+        r.e[0] = e[0];
+        r.e[1] = e[1];
+        r.e[2] = e[2];
+        r.c[0] = d123_1 * inv;
+        r.c[1] = d123_2 * inv;
+        r.c[2] = d123_3 * inv;
+        r.n = 3;
+        return r;
+    }
+    static R CalcMetric(const SEdge* e, std::uint32_t n) {
+        if (n == 2) return std::sqrt(MagSq(PD(e[1]) - PD(e[0])));
+        if (n == 3) return Cross(PD(e[1]) - PD(e[0]), PD(e[2]) - PD(e[0]));
+        return R(0);
+    }
+    // CalcSearchDirection (Simplex.cpp:44-60).
+    static V2<R> SearchDirection(const SEdge* e, std::uint32_t n) {
+        if (n == 1) return -PD(e[0]);
+        if (n == 2) {
+            const V2<R> e12 = PD(e[1]) - PD(e[0]);
+            const V2<R> e0 = PD(e[0]);
+            const R sgn = Cross(e12, -e0);
+            return (sgn > 0) ? RevPerp(e12) : FwdPerp(e12);
+        }
+        return V2<R>{0, 0};
+    }
+    // Simplex::Cache (Simplex.hpp:96-101, 156-159).
+    struct Cache {
+        R metric = std::numeric_limits<R>::quiet_NaN();
+        std::uint32_t ia[3] = {255, 255, 255}, ib[3] = {255, 255, 255};
+        std::uint32_t Count() const {
+            std::uint32_t k = 0;
+            for (int i = 0; i < 3; ++i) k += (ia[i] == 255 && ib[i] == 255) ? 0u : 1u;
+            return k;
+        }
+    };
+    static Cache CacheOf(const Simplex& s) {
+        Cache c;
+        c.metric = CalcMetric(s.e, s.n);
+        for (std::uint32_t i = 0; i < s.n; ++i) {
+            c.ia[i] = s.e[i].iA;
+            c.ib[i] = s.e[i].iB;
+        }
+        return c;
+    }
+    static SEdge EdgeOf(const Proxy<R>& pA, const Xf<R>& xfA, std::uint32_t iA, const Proxy<R>& pB, const Xf<R>& xfB,
+                        std::uint32_t iB) {
+        return SEdge{Transform(pA.v[iA], xfA), Transform(pB.v[iB], xfB), iA, iB};
+    }
+    enum DistState { kDistHitMaxIters, kDistMaxPoints, kDistUnfitSearchDir, kDistDuplicateIndexPair };
+    struct DistOut {
+        Simplex s;
+        std::uint32_t iters;
+        DistState state;
+    };
+    // Distance (Distance.cpp:95-175).
+    static DistOut Distance(const Proxy<R>& pA, const Xf<R>& xfA, const Proxy<R>& pB, const Xf<R>& xfB,
+                            const Cache& cache, std::uint32_t maxIters) {
+        std::uint32_t sa[3] = {cache.ia[0], cache.ia[1], cache.ia[2]};
+        std::uint32_t sb[3] = {cache.ib[0], cache.ib[1], cache.ib[2]};
+        SEdge e[3];
+        std::uint32_t n = cache.Count();
+        for (std::uint32_t i = 0; i < n; ++i) e[i] = EdgeOf(pA, xfA, sa[i], pB, xfB, sb[i]);
+        if (n > 1) {
+            const R metric1 = cache.metric;
+            const R metric2 = CalcMetric(e, n);
+            if ((metric2 < (metric1 / 2)) || (metric2 > (metric1 * 2)) || (metric2 < 0) || AlmostZero(metric2)) n = 0;
+        }
+        if (n == 0) {
+            e[0] = EdgeOf(pA, xfA, 0, pB, xfB, 0);
+            n = 1;
+            sa[0] = 0;
+            sb[0] = 0;
+            sa[1] = sb[1] = sa[2] = sb[2] = 255;
+        }
+        Simplex simplex;
+        DistState state = kDistHitMaxIters;
+        std::uint32_t iter = 0;
+        while (iter < maxIters) {
+            ++iter;
+            simplex = SolveSimplex(e, n);
+            n = simplex.n;
+            for (std::uint32_t i = 0; i < n; ++i) e[i] = simplex.e[i];
+            if (n == 3) {
+                state = kDistMaxPoints;
+                break;
+            }
+            const V2<R> d = SearchDirection(e, n);
+            if (AlmostZero(MagSq(d))) {
+                state = kDistUnfitSearchDir;
+                break;
+            }
+            const std::uint32_t iA = SupportIndex(pA, InvRotate(-d, xfA.q));
+            const std::uint32_t iB = SupportIndex(pB, InvRotate(d, xfB.q));
+            bool dup = false;
+            for (int k = 0; k < 3; ++k) dup = dup || (sa[k] == iA && sb[k] == iB);
+            if (dup) {
+                state = kDistDuplicateIndexPair;
+                break;
+            }
+            e[n++] = EdgeOf(pA, xfA, iA, pB, xfB, iB);
+            for (std::uint32_t k = 0; k < 3; ++k) {
+                sa[k] = k < n ? e[k].iA : 255u;
+                sb[k] = k < n ? e[k].iB : 255u;
+            }
+        }
+        return DistOut{simplex, iter, state};
+    }
+    // GetDelta(GetWitnessPoints(simplex)) (Distance.cpp:75-93, Distance.hpp:55-58).
+    static V2<R> WitnessDelta(const Simplex& s) {
+        V2<R> a{0, 0}, b{0, 0};
+        for (std::uint32_t i = 0; i < s.n; ++i) {
+            a = a + s.e[i].wA * s.c[i];
+            b = b + s.e[i].wB * s.c[i];
+        }
+        return b - a;
+    }
+
+    // SeparationScenario (SeparationScenario.cpp:25-220).
+    enum ScenarioType { kScenPoints, kScenFaceA, kScenFaceB };
+    struct Scenario {
+        ScenarioType type;
+        V2<R> axis, localPoint;
+    };
+    static Scenario GetScenario(const Cache& c, const Proxy<R>& pA, const Xf<R>& xfA, const Proxy<R>& pB,
+                                const Xf<R>& xfB) {
+        const V2<R> zero{0, 0};
+        if (c.Count() >= 2) {
+            if (c.ia[0] == c.ia[1]) {
+                const V2<R> b1 = pB.v[c.ib[0]], b2 = pB.v[c.ib[1]];
+                const V2<R> axis = UnitVector(FwdPerp(b2 - b1), zero);
+                const V2<R> normal = Rotate(axis, xfB.q);
+                const V2<R> lp = (b1 + b2) / R(2);
+                const V2<R> pointB = Transform(lp, xfB);
+                const V2<R> pointA = Transform(pA.v[c.ia[0]], xfA);
+                const V2<R> delta = pointA - pointB;
+                return Scenario{kScenFaceB, (Dot(delta, normal) < 0) ? -axis : axis, lp};
+            }
+            const V2<R> a1 = pA.v[c.ia[0]], a2 = pA.v[c.ia[1]];
+            const V2<R> axis = UnitVector(FwdPerp(a2 - a1), zero);
+            const V2<R> normal = Rotate(axis, xfA.q);
+            const V2<R> lp = (a1 + a2) / R(2);
+            const V2<R> pointA = Transform(lp, xfA);
+            const V2<R> pointB = Transform(pB.v[c.ib[0]], xfB);
+            const V2<R> delta = pointB - pointA;
+            return Scenario{kScenFaceA, (Dot(delta, normal) < 0) ? -axis : axis, lp};
+        }
+        const V2<R> pointA = Transform(pA.v[c.ia[0]], xfA);
+        const V2<R> pointB = Transform(pB.v[c.ib[0]], xfB);
+        return Scenario{kScenPoints, UnitVector(pointB - pointA, zero), zero};
+    }
+    struct SepIdx {
+        R distance;
+        std::uint32_t ia, ib;
+    };
+    static SepIdx FindMinSep(const Scenario& f, const Proxy<R>& pA, const Xf<R>& xfA, const Proxy<R>& pB,
+                             const Xf<R>& xfB) {
+        if (f.type == kScenPoints) {
+            const std::uint32_t ia = SupportIndex(pA, InvRotate(f.axis, xfA.q));
+            const std::uint32_t ib = SupportIndex(pB, InvRotate(-f.axis, xfB.q));
+            const V2<R> delta = Transform(pB.v[ib], xfB) - Transform(pA.v[ia], xfA);
+            return SepIdx{Dot(delta, f.axis), ia, ib};
+        }
+        if (f.type == kScenFaceA) {
+            const V2<R> normal = Rotate(f.axis, xfA.q);
+            const V2<R> pointA = Transform(f.localPoint, xfA);
+            const std::uint32_t ib = SupportIndex(pB, InvRotate(-normal, xfB.q));
+            const V2<R> delta = Transform(pB.v[ib], xfB) - pointA;
+            return SepIdx{Dot(delta, normal), 255u, ib};
+        }
+        const V2<R> normal = Rotate(f.axis, xfB.q);
+        const std::uint32_t ia = SupportIndex(pA, InvRotate(-normal, xfA.q));
+        const V2<R> pointA = Transform(pA.v[ia], xfA);
+        const V2<R> delta = pointA - Transform(f.localPoint, xfB);
+        return SepIdx{Dot(delta, normal), ia, 255u};
+    }
+    static R Evaluate(const Scenario& f, const Proxy<R>& pA, const Xf<R>& xfA, const Proxy<R>& pB, const Xf<R>& xfB,
+                      std::uint32_t ia, std::uint32_t ib) {
+        if (f.type == kScenPoints) {
+            const V2<R> delta = Transform(pB.v[ib], xfB) - Transform(pA.v[ia], xfA);
+            return Dot(delta, f.axis);
+        }
+        if (f.type == kScenFaceA) {
+            const V2<R> normal = Rotate(f.axis, xfA.q);
+            const V2<R> pointA = Transform(f.localPoint, xfA);
+            const V2<R> delta = Transform(pB.v[ib], xfB) - pointA;
+            return Dot(delta, normal);
+        }
+        const V2<R> normal = Rotate(f.axis, xfB.q);
+        const V2<R> pointB = Transform(f.localPoint, xfB);
+        const V2<R> delta = Transform(pA.v[ia], xfA) - pointB;
+        return Dot(delta, normal);
+    }
+
+    enum ToiState {
+        kToiUnknown, kToiTouching, kToiSeparated, kToiOverlapped, kToiMaxRootIters, kToiNextAfter, kToiMaxToiIters,
+        kToiBelowMinTarget, kToiMaxDistIters, kToiTargetDepthExceeds, kToiOverflow
+    };
+    struct ToiOut {
+        R time;
+        ToiState state;
+    };
+    // GetToiViaSat (TimeOfImpact.cpp:35-251) with ToiConf from the StepConf
+    // (ToiConf.cpp:26-35): timeMax 1.
+    static ToiOut ToiViaSat(const Proxy<R>& pA, const SweepR& sA, const Proxy<R>& pB, const SweepR& sB,
+                            const StepConf<R>& conf) {
+        const R totalRadius = pA.radius + pB.radius;
+        if (conf.targetDepth > totalRadius) return ToiOut{0, kToiTargetDepthExceeds};
+        const R target = totalRadius - conf.targetDepth;
+        const R maxTarget = Max(target + conf.tolerance, R(0));
+        const R minTarget = Max(target - conf.tolerance, R(0));
+        const R minTargetSq = minTarget * minTarget;
+        if (!std::isfinite(minTargetSq) && std::isfinite(minTarget)) return ToiOut{0, kToiOverflow};
+        const R maxTargetSq = maxTarget * maxTarget;
+        if (!std::isfinite(maxTargetSq) && std::isfinite(maxTarget)) return ToiOut{0, kToiOverflow};
+        R timeLo = 0;
+        Xf<R> loA = XfAt(sA, timeLo), loB = XfAt(sB, timeLo);
+        Cache cache;
+        std::uint32_t toiIters = 0;
+        while (toiIters < conf.maxToiIters) {
+            const DistOut di = Distance(pA, loA, pB, loB, cache, conf.maxDistanceIters);
+            ++toiIters;
+            if (di.state == kDistHitMaxIters) return ToiOut{timeLo, kToiMaxDistIters};
+            cache = CacheOf(di.s);
+            const R distSq = MagSq(WitnessDelta(di.s));
+            if (distSq < minTargetSq) return ToiOut{timeLo, kToiOverlapped};
+            if (distSq <= maxTargetSq) return ToiOut{timeLo, kToiTouching};
+            const Scenario fcn = GetScenario(cache, pA, loA, pB, loB);
+            R timeHi = R(1);
+            Xf<R> hiA = XfAt(sA, timeHi), hiB = XfAt(sB, timeHi);
+            for (std::uint32_t pbIter = 0; pbIter < 254u; ++pbIter) {
+                const SepIdx hiMin = FindMinSep(fcn, pA, hiA, pB, hiB);
+                if (hiMin.distance > maxTarget) return ToiOut{timeHi, kToiSeparated};
+                if (hiMin.distance >= minTarget) {
+                    if (timeHi == timeLo) return ToiOut{timeHi, kToiSeparated};
+                    timeLo = timeHi;
+                    loA = hiA;
+                    loB = hiB;
+                    break;
+                }
+                const R loEval = Evaluate(fcn, pA, loA, pB, loB, hiMin.ia, hiMin.ib);
+                if (loEval <= maxTarget) return ToiOut{timeLo, (loEval < minTarget) ? kToiBelowMinTarget : kToiTouching};
+                R a1 = timeLo, a2 = timeHi, s1 = loEval, s2 = hiMin.distance;
+                std::uint32_t roots = 0;
+                R t = timeLo;
+                for (;;) {
+                    if (roots == conf.maxToiRootIters) return ToiOut{t, kToiMaxRootIters};
+                    if (std::nextafter(a1, a2) >= a2) return ToiOut{t, kToiNextAfter};
+                    // Secant / Bisect (Math.hpp:95-108).
+                    t = (roots & 1u) ? a1 + (target - s1) * (a2 - a1) / (s2 - s1) : (a1 + a2) / 2;
+                    ++roots;
+                    const Xf<R> tA = XfAt(sA, t), tB = XfAt(sB, t);
+                    const R sv = Evaluate(fcn, pA, tA, pB, tB, hiMin.ia, hiMin.ib);
+                    if (std::abs(sv - target) <= conf.tolerance) {
+                        timeHi = t;
+                        hiA = tA;
+                        hiB = tB;
+                        break;
+                    }
+                    if (sv > target) {
+                        a1 = t;
+                        s1 = sv;
+                    } else {
+                        a2 = t;
+                        s2 = sv;
+                    }
+                }
+            }
+        }
+        return ToiOut{timeLo, kToiMaxToiIters};
+    }
+
+    // UpdateContactTOIs (AabbTreeWorld.cpp:1685-1750).
+    void UpdateContactTOIs(const StepConf<R>& conf) {
+        for (const auto& kc : contacts) {
+            Contact<R>& c = contactBuf[kc.id];
+            if (c.toiValid) continue;
+            if (!(c.flags & kCEnabled) || (c.flags & kSensor) || !(c.flags & kCImpenetrable)) continue;
+            if (c.toiCount >= conf.maxSubSteps) continue;
+            Body<R>& bA = bodies[c.bodyA];
+            Body<R>& bB = bodies[c.bodyB];
+            if (!bA.awake() && !bB.awake()) continue;
+            const R alpha0 = Max(bA.alpha0, bB.alpha0);
+            BodyAdvance0(bA, alpha0);
+            BodyAdvance0(bB, alpha0);
+            const SweepR sA = Normalized(SweepOf(bA));
+            const SweepR sB = Normalized(SweepOf(bB));
+            const ToiOut out = ToiViaSat(shapes[c.shapeA].proxy(), sA, shapes[c.shapeB].proxy(), sB, conf);
+            c.toi = (out.state == kToiTouching) ? Min(alpha0 + (R(1) - alpha0) * out.time, R(1)) : R(1);
+            c.toiValid = true;
+        }
+    }
+
+    // ProcessContactsForTOI (AabbTreeWorld.cpp:2059-2142).
+    void ProcessContactsForTOI(std::uint32_t id, Island& island, R toi, const StepConf<R>& conf) {
+        const bool bodyImpenetrable = (bodies[id].flags & kImpenetrable) != 0;
+        for (const auto& kc : bodyContacts[id]) {
+            const std::uint32_t cid = kc.id;
+            if (islandedContacts[cid]) continue;
+            if (contactBuf[cid].flags & kSensor) continue;
+            const std::uint32_t otherId = (contactBuf[cid].bodyA == id) ? contactBuf[cid].bodyB : contactBuf[cid].bodyA;
+            Body<R>& other = bodies[otherId];
+            if (!(bodyImpenetrable || (other.flags & kImpenetrable))) continue;
+            const bool otherIslanded = islandedBodies[otherId] != 0;
+            const SweepR backup = SweepOf(other);
+            if (!otherIslanded) {
+                BodyAdvance(other, toi);
+                if (other.pos0 != backup.pos0 || other.pos1 != backup.pos1) FlagForUpdating(otherId);
+            }
+            if (contactBuf[cid].flags & kNeedsUpdating) Update(cid, conf);
+            const std::uint32_t fl = contactBuf[cid].flags;
+            if (!(fl & kCEnabled) || !(fl & kTouching)) {
+                SetSweep(other, backup);
+                continue;
+            }
+            island.contacts.push_back(cid);
+            islandedContacts[cid] = 1;
+            if (!otherIslanded) {
+                if (other.speedable()) other.SetAwakeFlag();
+                island.bodies.push_back(otherId);
+                islandedBodies[otherId] = 1;
+            }
+        }
+    }
+
+    // SolveToiViaGS (AabbTreeWorld.cpp:1968-2057) with conf.dt the rest of the step.
+    void SolveToiViaGS(const Island& island, const StepConf<R>& conf) {
+        StepConf<R> still = conf;
+        still.dt = 0;
+        std::vector<BodyConstraint<R>> bcs(bodies.size());
+        for (std::uint32_t bid : island.bodies) {
+            const Body<R>& b = bodies[bid];
+            bcs[bid] = BodyConstraint<R>{b.invMass, b.invRotI, b.lc, b.pos1, IntegratedVelocity(b, still)};
+        }
+        std::vector<PositionConstraint<R>> pcs;
+        for (std::uint32_t cid : island.contacts) {
+            const Contact<R>& c = contactBuf[cid];
+            const R rA = shapes[c.shapeA].radius, rB = shapes[c.shapeB].radius;
+            pcs.push_back(PositionConstraint<R>{manifoldBuf[cid], c.bodyA, c.bodyB, rA + rB});
+        }
+        std::uint32_t positionIters = conf.toiPosIters;
+        StepConf<R> ps = conf;  // GetToiConstraintSolverConf (ConstraintSolverConf.cpp:37-45)
+        ps.regResolutionRate = conf.toiResolutionRate;
+        for (std::uint32_t i = 0; i < conf.toiPosIters; ++i) {
+            R minSep = std::numeric_limits<R>::infinity();
+            for (const auto& pc : pcs) minSep = Min(minSep, SolvePC(pc, bcs, ps));
+            if (minSep >= conf.toiMinSeparation) {
+                positionIters = i + 1;
+                break;
+            }
+        }
+        for (std::uint32_t bid : island.bodies) bodies[bid].pos0 = bcs[bid].pos;
+        StepConf<R> vconf = conf;  // GetToiVelocityConstraintConf (VelocityConstraint.cpp:207-210)
+        vconf.warmStart = false;
+        std::vector<VelocityConstraint<R>> vcs;
+        for (std::uint32_t cid : island.contacts) {
+            const Contact<R>& c = contactBuf[cid];
+            const R rA = shapes[c.shapeA].radius, rB = shapes[c.shapeB].radius;
+            const Xf<R> xfA = GetXf(bcs[c.bodyA].pos, bcs[c.bodyA].lc);
+            const Xf<R> xfB = GetXf(bcs[c.bodyB].pos, bcs[c.bodyB].lc);
+            const WorldManifold<R> wm = GetWorldManifold(manifoldBuf[cid], xfA, rA, xfB, rB);
+            vcs.push_back(MakeVC(c, wm, bcs, vconf));
+        }
+        for (std::uint32_t i = 0; i < conf.toiVelIters; ++i) {
+            R newInc = 0;
+            for (auto& vc : vcs) newInc = Max(newInc, SolveVC(vc, bcs));
+            if (newInc <= conf.toiMinMomentum) break;
+        }
+        const R h = conf.dt;
+        for (std::uint32_t bid : island.bodies) {
+            BodyConstraint<R>& bc = bcs[bid];
+            const V2<R> tr = h * bc.vel.lin;
+            const R rot = h * bc.vel.ang;
+            bc.pos = bc.pos + Pos<R>{tr, rot};
+        }
+        for (std::uint32_t bid : island.bodies) {
+            Body<R>& b = bodies[bid];
+            b.vel = bcs[bid].vel;
+            if (b.pos1 != bcs[bid].pos) {
+                b.pos1 = bcs[bid].pos;
+                b.xf = GetXf(b.pos1, b.lc);
+                FlagForUpdating(bid);
+            }
+        }
+        // Post-solve reports with the position iterations (AabbTreeWorld.cpp:2052-2054).
+        for (std::size_t i = 0; i < vcs.size(); ++i) {
+            EventHead(3, island.contacts[i]);
+            events.push_back(vcs[i].n);
+            events.push_back(positionIters);
+            R f[4] = {};
+            for (std::uint32_t j = 0; j < vcs[i].n; ++j) {
+                f[j] = vcs[i].p[j].ni;
+                f[2 + j] = vcs[i].p[j].ti;
+            }
+            for (R v : f) events.push_back(FloatBits(v));
+        }
+    }
+
+    // SolveToi(ContactID) (AabbTreeWorld.cpp:1833-1966).
+    void SolveToiContact(std::uint32_t cid, const StepConf<R>& conf) {
+        const R toi = contactBuf[cid].toi;
+        const std::uint32_t ia = contactBuf[cid].bodyA, ib = contactBuf[cid].bodyB;
+        Body<R>& bA = bodies[ia];
+        Body<R>& bB = bodies[ib];
+        {
+            const SweepR backupA = SweepOf(bA), backupB = SweepOf(bB);
+            BodyAdvance(bA, toi);
+            if (bA.pos0 != backupA.pos0 || bA.pos1 != backupA.pos1) FlagForUpdating(ia);
+            BodyAdvance(bB, toi);
+            if (bB.pos0 != backupB.pos0 || bB.pos1 != backupB.pos1) FlagForUpdating(ib);
+            if (contactBuf[cid].flags & kNeedsUpdating) Update(cid, conf);
+            contactBuf[cid].toiValid = false;
+            ++contactBuf[cid].toiCount;
+            const std::uint32_t fl = contactBuf[cid].flags;
+            if (!(fl & kCEnabled) || !(fl & kTouching)) {
+                SetSweep(bA, backupA);
+                SetSweep(bB, backupB);
+                return;
+            }
+        }
+        if (bA.speedable()) bA.SetAwakeFlag();
+        if (bB.speedable()) bB.SetAwakeFlag();
+        Island island;
+        islandedBodies[ia] = 1;
+        islandedBodies[ib] = 1;
+        islandedContacts[cid] = 1;
+        island.bodies.push_back(ia);
+        island.bodies.push_back(ib);
+        island.contacts.push_back(cid);
+        if (bA.accelerable()) ProcessContactsForTOI(ia, island, toi, conf);
+        if (bB.accelerable()) ProcessContactsForTOI(ib, island, toi, conf);
+        for (std::uint32_t b : island.bodies) {
+            if (!bodies[b].speedable()) islandedBodies[b] = 0;
+        }
+        StepConf<R> sub = conf;
+        sub.dt = (R(1) - toi) * conf.dt;
+        SolveToiViaGS(island, sub);
+    }
+
+    // SolveToi(const StepConf&) (AabbTreeWorld.cpp:1752-1831), no sub-stepping.
+    void SolveToiPhase(const StepConf<R>& conf) {
+        islandedBodies.assign(bodies.size(), 0);
+        islandedContacts.assign(contactBuf.size(), 0);
+        for (;;) {
+            UpdateContactTOIs(conf);
+            std::uint32_t next = 0xffffffffu;
+            R minToi = R(1);
+            for (const auto& kc : contacts) {
+                const Contact<R>& c = contactBuf[kc.id];
+                if (c.toiValid && minToi > c.toi) {
+                    minToi = c.toi;
+                    next = kc.id;
+                }
+            }
+            if (next == 0xffffffffu) {
+                for (std::uint32_t bid : bodyOrder) bodies[bid].alpha0 = 0;
+                for (const auto& kc : contacts) {
+                    contactBuf[kc.id].toiValid = false;
+                    contactBuf[kc.id].toiCount = 0;
+                }
+                break;
+            }
+            if (!islandedContacts[next]) SolveToiContact(next, conf);
+            for (std::uint32_t bid : bodyOrder) {
+                if (!islandedBodies[bid]) continue;
+                islandedBodies[bid] = 0;
+                if (bodies[bid].accelerable()) {
+                    Synchronize(bid, conf);
+                    for (const auto& kc : bodyContacts[bid]) {
+                        contactBuf[kc.id].toiValid = false;
+                        islandedContacts[kc.id] = 0;
+                    }
+                }
+            }
+            FindAndAddContacts(conf);
+            if (islandedContacts.size() < contactBuf.size()) islandedContacts.resize(contactBuf.size(), 0);
+        }
+        UpdateContacts(conf);
+    }
+
+    // Step (AabbTreeWorld.cpp:2144-2219); the TOI phase when conf.toi.
     StepStats Step(const StepConf<R>& conf) {
         StepStats st;
         events.clear();
@@ -1870,7 +2508,10 @@ public:
         st.preDestroyed = DestroyContacts();
         st.preUpdated = UpdateContacts(conf, &st.preSkipped);
         st.preAdded = FindAndAddContacts(conf);
-        if (conf.dt != R(0)) SolveReg(conf, st);
+        if (conf.dt != R(0)) {
+            SolveReg(conf, st);
+            if (conf.toi) SolveToiPhase(conf);
+        }
         return st;
     }
 

@@ -185,6 +185,13 @@ void WriteConf(Out& o, const StepConf& c) {
     o.u32(c.regVelocityIters);
     o.u32(c.regPositionIters);
     o.u32((c.doWarmStart ? 1u : 0u) | (c.doToi ? 2u : 0u) | (c.doBlocksolve ? 4u : 0u));
+    // TOI phase fields.
+    const float t[] = {c.toiResolutionRate, c.toiMinSeparation, c.toiMinMomentum, c.targetDepth, c.tolerance};
+    o.u32(sizeof(t) / 4);
+    o.put(t, sizeof(t));
+    const std::uint32_t ti[] = {c.toiVelocityIters, c.toiPositionIters, c.maxToiRootIters,
+                                c.maxToiIters,      c.maxDistanceIters, c.maxSubSteps};
+    o.put(ti, sizeof(ti));
 }
 
 // Contact listener calls of the last step, serialised in call order:

@@ -43,7 +43,11 @@ class StepConf(ctypes.Structure):
         "linear_sleep_tolerance", "angular_sleep_tolerance", "displace_multiplier",
         "aabb_extension", "max_circles_ratio")] + [
         (n, ctypes.c_uint32) for n in ("reg_velocity_iters", "reg_position_iters",
-                                       "do_warm_start", "do_toi", "do_blocksolve")]
+                                       "do_warm_start", "do_toi", "do_blocksolve")] + [
+        (n, ctypes.c_float) for n in ("toi_resolution_rate", "toi_min_separation", "toi_min_momentum",
+                                      "target_depth", "tolerance")] + [
+        (n, ctypes.c_uint32) for n in ("toi_velocity_iters", "toi_position_iters", "max_toi_root_iters",
+                                       "max_toi_iters", "max_distance_iters", "max_sub_steps")]
 
     def __init__(self, **kw):
         super().__init__()

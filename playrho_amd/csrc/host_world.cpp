@@ -1196,6 +1196,17 @@ void prx_step_conf_default(prx_step_conf* c) {
     c->do_warm_start = 1;
     c->do_toi = 0;
     c->do_blocksolve = 1;
+    c->toi_resolution_rate = 75.0f / 100.0f;
+    c->toi_min_separation = -0.005f * 1.5f;
+    c->toi_min_momentum = 0.0f / 100.0f;
+    c->target_depth = 0.005f * 3.0f;
+    c->tolerance = 0.005f / 4.0f;
+    c->toi_velocity_iters = 8;
+    c->toi_position_iters = 20;
+    c->max_toi_root_iters = 30;
+    c->max_toi_iters = 20;
+    c->max_distance_iters = 20;
+    c->max_sub_steps = 8;
 }
 
 void prx_world_conf_default(prx_world_conf* c) {

@@ -82,7 +82,27 @@ def read_conf(r):
     vi = r.u32()
     pi = r.u32()
     flags = r.u32()
-    return dict(floats=f, vel_iters=vi, pos_iters=pi, flags=flags)
+    nt = r.u32()
+    tf = r.arr("<f4", nt)
+    ti = r.arr("<u4", len(TOI_ITERS))
+    return dict(floats=f, vel_iters=vi, pos_iters=pi, flags=flags, toi_floats=tf, toi_iters=ti)
+
+
+# StepConf fields of the TOI phase, in record order.
+TOI_FLOATS = ("toi_resolution_rate", "toi_min_separation", "toi_min_momentum", "target_depth", "tolerance")
+TOI_ITERS = ("toi_velocity_iters", "toi_position_iters", "max_toi_root_iters", "max_toi_iters",
+             "max_distance_iters", "max_sub_steps")
+
+
+def conf_record(c):
+    """The StepConf record (dump layout) of a playrho_amd.StepConf."""
+    import struct
+    floats = [getattr(c, f) for f, _ in c._fields_[:18]]
+    flags = (1 if c.do_warm_start else 0) | (2 if c.do_toi else 0) | (4 if c.do_blocksolve else 0)
+    return (struct.pack("<I", 18) + struct.pack("<18f", *floats) +
+            struct.pack("<III", c.reg_velocity_iters, c.reg_position_iters, flags) +
+            struct.pack("<I", len(TOI_FLOATS)) + struct.pack("<5f", *[getattr(c, f) for f in TOI_FLOATS]) +
+            struct.pack("<6I", *[getattr(c, f) for f in TOI_ITERS]))
 
 
 def read_state(r):

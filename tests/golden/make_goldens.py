@@ -38,14 +38,23 @@ CASES = [
     ("chain0", "chain", 240, [1, 0], [1, 120, 240]),
     ("chain1", "chain", 240, [1, 1, 12, 24, 40], [1, 240]),
 ]
+# The same scenes with the TOI phase (the reference's default doToi = true).
+TOI_CASES = [
+    ("pyramid_toi", "pyramid", 300, [20], [1, 150, 300]),
+    ("drop300_toi", "drop", 200, [300, 20, 40, 7], [1, 100, 200]),
+    ("billiards0_toi", "billiards", 120, [1, 0], [1, 60, 120]),
+    ("pile3_toi", "pile", 150, [1, 3], [1, 150]),
+    ("chain0_toi", "chain", 240, [1, 0], [1, 240]),
+]
 
 
 def main():
     tmp = tempfile.mkdtemp()
     try:
-        for name, scene, steps, params, full in CASES:
+        for name, scene, steps, params, full, do_toi in ([c + (False,) for c in CASES] +
+                                                         [c + (True,) for c in TOI_CASES]):
             path = os.path.join(tmp, name + ".bin")
-            buf = oracle_lib.run_ref_driver(scene, steps, False, params, path)
+            buf = oracle_lib.run_ref_driver(scene, steps, do_toi, params, path)
             desc_conf, d = oracle_lib.split_dump(buf)
             states = d["states"]
             dig = np.zeros((len(states), len(golden_cases.PARTS), 32), dtype=np.uint8)
@@ -54,7 +63,8 @@ def main():
                     dig[i, k] = np.frombuffer(dumpfmt.digest(s, (part,)), dtype=np.uint8)
             arrays = dict(desc_conf=np.frombuffer(desc_conf, dtype=np.uint8),
                           digests=dig, scene=np.array(scene), params=np.array(params, dtype=np.int32),
-                          steps=np.array(steps), full_steps=np.array(full, dtype=np.int32))
+                          steps=np.array(steps), full_steps=np.array(full, dtype=np.int32),
+                          do_toi=np.array(do_toi))
             for st in full:
                 s = states[st]
                 for key in ("stats", "bodies", "bflags", "cu", "ct", "cf", "pid", "paabb", "jlimit", "jimp", "events"):

@@ -13,10 +13,21 @@ def names():
     return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
 
 
+def regular_names():
+    """Cases stepped with doToi = false."""
+    return [n for n in names() if not n.endswith("_toi")]
+
+
+def toi_names():
+    """Cases stepped with the TOI phase (doToi = true)."""
+    return [n for n in names() if n.endswith("_toi")]
+
+
 def load(name):
     z = np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
     g = dict(desc_conf=z["desc_conf"].tobytes(), digests=z["digests"], scene=str(z["scene"]),
-             params=[int(x) for x in z["params"]], steps=int(z["steps"]), full={})
+             params=[int(x) for x in z["params"]], steps=int(z["steps"]), full={},
+             do_toi=bool(z["do_toi"]) if "do_toi" in z.files else False)
     for st in z["full_steps"]:
         st = int(st)
         g["full"][st] = {k: z["full%d_%s" % (st, k)] for k in

@@ -96,6 +96,9 @@ def test_step_conf_defaults_match_reference():
     mine = np.array([getattr(c, f) for f, _ in pa.StepConf._fields_[:18]], dtype=np.float32)
     assert np.array_equal(mine.view("<u4"), ref["floats"].view("<u4")), (mine, ref["floats"])
     assert (c.reg_velocity_iters, c.reg_position_iters) == (ref["vel_iters"], ref["pos_iters"])
+    toi = np.array([getattr(c, f) for f in dumpfmt.TOI_FLOATS], dtype=np.float32)
+    assert np.array_equal(toi.view("<u4"), ref["toi_floats"].view("<u4")), (toi, ref["toi_floats"])
+    assert [getattr(c, f) for f in dumpfmt.TOI_ITERS] == list(ref["toi_iters"])
 
 
 def test_shape_material_setters():

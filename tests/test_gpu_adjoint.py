@@ -13,11 +13,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _conf_bytes(c):
-    import struct
-    floats = [getattr(c, f) for f, _ in pa.StepConf._fields_[:18]]
-    return (struct.pack("<I", 18) + struct.pack("<18f", *floats) +
-            struct.pack("<III", c.reg_velocity_iters, c.reg_position_iters,
-                        (1 if c.do_warm_start else 0) | (4 if c.do_blocksolve else 0)))
+    return dumpfmt.conf_record(c)
 
 
 def _loss_f64(desc, conf, steps, targets, dyn):

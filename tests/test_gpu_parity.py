@@ -35,7 +35,7 @@ def test_device_sincos_matches_host_library():
     assert bad.size == 0, (a[bad[:5] // 2], got.reshape(-1)[bad[:5]], want.reshape(-1)[bad[:5]])
 
 
-@pytest.mark.parametrize("name", golden_cases.names())
+@pytest.mark.parametrize("name", golden_cases.regular_names())
 def test_plain_step_matches_reference_every_step(name):
     g = golden_cases.load(name)
     print("case", name, "steps", g["steps"], flush=True)
@@ -59,7 +59,7 @@ def test_batched_step_matches_reference(coop_min):
     """All golden worlds in one batch, one launch per step.  coop_min 1 sweeps
     every island with contacts by the whole wavefront (LDS-staged records),
     1 << 30 sweeps every island on one lane; both equal the reference."""
-    names = golden_cases.names()
+    names = golden_cases.regular_names()
     gs = [golden_cases.load(n) for n in names]
     worlds = [_recording(pa.World.from_desc(g["desc_conf"])) for g in gs]
     print("coop_min", coop_min, flush=True)
@@ -160,9 +160,4 @@ def test_device_matches_oracle(scene, params, steps):
 
 def _conf_bytes():
     """The StepConf record (dump layout) of the default conf."""
-    c = pa.StepConf()
-    import struct
-    floats = [getattr(c, f) for f, _ in pa.StepConf._fields_[:18]]
-    return (struct.pack("<I", 18) + struct.pack("<18f", *floats) +
-            struct.pack("<III", c.reg_velocity_iters, c.reg_position_iters,
-                        (1 if c.do_warm_start else 0) | (2 if c.do_toi else 0) | (4 if c.do_blocksolve else 0)))
+    return dumpfmt.conf_record(pa.StepConf())
