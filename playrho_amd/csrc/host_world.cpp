@@ -583,6 +583,18 @@ prx::StepParams ToParams(const prx_step_conf& c) {
     p.posIters = c.reg_position_iters;
     p.warmStart = c.do_warm_start;
     p.blockSolve = c.do_blocksolve;
+    p.doToi = c.do_toi;
+    p.toiResolutionRate = c.toi_resolution_rate;
+    p.toiMinSeparation = c.toi_min_separation;
+    p.toiMinMomentum = c.toi_min_momentum;
+    p.targetDepth = c.target_depth;
+    p.tolerance = c.tolerance;
+    p.toiVelIters = c.toi_velocity_iters;
+    p.toiPosIters = c.toi_position_iters;
+    p.maxToiRootIters = c.max_toi_root_iters;
+    p.maxToiIters = c.max_toi_iters;
+    p.maxDistanceIters = c.max_distance_iters;
+    p.maxSubSteps = c.max_sub_steps;
     return p;
 }
 
@@ -770,6 +782,13 @@ int BuildBatch(prx_batch* b) {
     PRX_HIP(a.alloc(d.s_isl_j, J));
     d.ev = nullptr;
     if (E) PRX_HIP(a.alloc(d.ev, E));
+    PRX_HIP(a.alloc(d.alpha0, B));
+    PRX_HIP(a.alloc(d.t_isl_b, B));
+    PRX_HIP(a.alloc(d.c_toi, C));
+    PRX_HIP(a.alloc(d.c_toiinfo, C));
+    PRX_HIP(a.alloc(d.t_isl_c, C));
+    PRX_HIP(hipMemset(d.alpha0, 0, std::max<std::size_t>(B, 1) * sizeof(float)));
+    PRX_HIP(hipMemset(d.c_toiinfo, 0, std::max<std::size_t>(C, 1) * sizeof(std::uint32_t)));
     d.schedule = b->d_schedule;
     return ApplySchedule(b);
 }
@@ -1047,13 +1066,14 @@ int CheckStatus(prx_batch* b) {
             return Fail(PRX_E_LENGTH, "candidate pair scratch of a world exceeded");
         if (h.status & prx::kStatusEventOverflow)
             return Fail(PRX_E_LENGTH, "contact listener record of a world exceeded");
+        if (h.status & prx::kStatusToiLimit)
+            return Fail(PRX_E_LENGTH, "TOI events of one step of a world exceeded (maxSubSteps + 1) per contact slot");
     }
     return PRX_OK;
 }
 
 int StepBatch(prx_batch* b, const prx_step_conf* conf, int nsteps) {
     if (!conf) return Fail(PRX_E_INVALID_ARGUMENT, "null step conf");
-    if (conf->do_toi) return Fail(PRX_E_UNSUPPORTED, "doToi = true is not stepped by this engine");
     if (nsteps < 0) return Fail(PRX_E_INVALID_ARGUMENT, "negative step count");
     prx::StepParams p = ToParams(*conf);
     p.ldsBytes = std::uint32_t(b->lds_bytes);
@@ -2039,7 +2059,7 @@ void prx_big_destroy(prx_big* big) {
 
 int prx_big_step(prx_big* big, const prx_step_conf* conf, int nsteps) {
     if (!big || !conf) return Fail(PRX_E_INVALID_ARGUMENT, "null argument");
-    if (conf->do_toi) return Fail(PRX_E_UNSUPPORTED, "doToi = true is not stepped by this engine");
+    if (conf->do_toi) return Fail(PRX_E_UNSUPPORTED, "the coloured large-world mode has no TOI phase (doToi = true)");
     if (nsteps < 0) return Fail(PRX_E_INVALID_ARGUMENT, "negative step count");
     prx_batch* b = big->batch;
     if (b->worlds[0] != big->world || big->world->owner != b)
@@ -2260,7 +2280,7 @@ void prx_adjoint_destroy(prx_adjoint* adj) { delete adj; }
 int prx_adjoint_run(prx_adjoint* adj, const prx_step_conf* conf, const float* targets, float* loss,
                     float* grad_velocity, float* grad_friction, float* grad_restitution) {
     if (!adj || !conf || !targets) return Fail(PRX_E_INVALID_ARGUMENT, "null argument");
-    if (conf->do_toi) return Fail(PRX_E_UNSUPPORTED, "doToi = true is not stepped by this engine");
+    if (conf->do_toi) return Fail(PRX_E_UNSUPPORTED, "the adjoint does not differentiate the TOI phase (doToi = true)");
     if (int(conf->reg_velocity_iters) > adj->itV || int(conf->reg_position_iters) > adj->itP)
         return Fail(PRX_E_INVALID_ARGUMENT, "iteration counts above those the adjoint was created for");
     prx_batch* b = adj->b;

@@ -49,7 +49,9 @@ enum : std::uint32_t {
 // Proxy flags.
 enum : std::uint32_t { kProxyPending = 1u, kProxyMoved = 2u };
 // World status bits.
-enum : std::uint32_t { kStatusContactOverflow = 1u, kStatusScratchOverflow = 2u, kStatusEventOverflow = 4u };
+enum : std::uint32_t {
+    kStatusContactOverflow = 1u, kStatusScratchOverflow = 2u, kStatusEventOverflow = 4u, kStatusToiLimit = 8u
+};
 
 // Per-world header (AoS: read once per phase by the world's workgroup).
 // Words of WorldHeader::stats: PreStepStats then RegStepStats, each in the
@@ -81,6 +83,10 @@ struct StepParams {
     std::uint32_t velIters, posIters, warmStart, blockSolve;
     std::uint32_t ldsBytes;  // dynamic LDS per workgroup (island solve state + record stage)
     std::uint32_t coopMin;   // islands with at least this many contacts are swept by the whole wave
+    // TOI phase (doToi).
+    std::uint32_t doToi;
+    float toiResolutionRate, toiMinSeparation, toiMinMomentum, targetDepth, tolerance;
+    std::uint32_t toiVelIters, toiPosIters, maxToiRootIters, maxToiIters, maxDistanceIters, maxSubSteps;
 };
 
 // Record of one step's solve for the adjoint (one-point manifolds): per
@@ -225,6 +231,15 @@ struct DeviceState {
     // iteration (255 when not solved) and 4 floats (normal impulses [2],
     // tangent impulses [2]).
     std::uint32_t* ev;
+
+    // TOI phase: per body the sweep's alpha0 (0 between steps) and a TOI
+    // island flag; per contact slot the TOI value, its state (bit 31 valid,
+    // low bits the sub-step count) and a TOI island flag.
+    float* alpha0;
+    std::uint8_t* t_isl_b;
+    float* c_toi;
+    std::uint32_t* c_toiinfo;
+    std::uint8_t* t_isl_c;
     // Solve record for the adjoint (only read by the recording kernel).
     Tape tape;
 };

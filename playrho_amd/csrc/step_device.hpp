@@ -1433,12 +1433,14 @@ struct IslandScratch {
 
 // Per-body CSR of touching, enabled, non-sensor contacts in contact order
 // (each body's contact list order), entries carry the other body.
+// kAll: every contact of each body (the TOI phase's per-body contact lists).
+template <bool kAll = false>
 __device__ __forceinline__ void build_adjacency(const DeviceState& d, const WorldCtx& x, int nc, const IslandScratch& S) {
     for (int b = x.lane; b <= x.nb; b += kWave) S.off[b] = 0;
     __syncthreads();
     for (int i = x.lane; i < nc; i += kWave) {
         const std::uint32_t f = d.c_flags[x.co + i];
-        if ((f & prx::kCEnabled) && (f & prx::kTouching) && !(f & prx::kSensor)) {
+        if (kAll || ((f & prx::kCEnabled) && (f & prx::kTouching) && !(f & prx::kSensor))) {
             const int2 bd = d.c_body[x.co + i];
             atomicAdd(&S.off[bd.x], 1);
             atomicAdd(&S.off[bd.y], 1);
@@ -1452,7 +1454,7 @@ __device__ __forceinline__ void build_adjacency(const DeviceState& d, const Worl
     __syncthreads();
     for (int i = x.lane; i < nc; i += kWave) {
         const std::uint32_t f = d.c_flags[x.co + i];
-        if ((f & prx::kCEnabled) && (f & prx::kTouching) && !(f & prx::kSensor)) {
+        if (kAll || ((f & prx::kCEnabled) && (f & prx::kTouching) && !(f & prx::kSensor))) {
             const int2 bd = d.c_body[x.co + i];
             S.csr[atomicAdd(&cur[bd.x], 1)] = int2{i, bd.y};
             S.csr[atomicAdd(&cur[bd.y], 1)] = int2{i, bd.x};
@@ -2642,6 +2644,12 @@ __device__ __forceinline__ std::uint32_t synchronize_body(const DeviceState& d, 
     return moved;
 }
 
+}  // namespace prxd
+
+#include "toi_device.hpp"
+
+namespace prxd {
+
 // Post-solve listener calls (AabbTreeWorld.cpp:1671-1674), island after
 // island in island contact order: the velocity constraint's points and
 // impulses (a constraint can keep one point of a two-point manifold) and the
@@ -2674,6 +2682,54 @@ __device__ __forceinline__ void post_solve_calls(const DeviceState& d, const Wor
         }
     }
     __syncthreads();
+}
+
+// The TOI phase of one world after its regular phase (a kernel of its own:
+// the serial event solves would otherwise set the register budget of the
+// regular step).  Counters and listener calls continue from the header.
+__device__ __forceinline__ void toi_world(const DeviceState& d, int w, const StepParams& c) {
+    WorldHeader& H = d.hdr[w];
+    WorldCtx x;
+    x.w = w;
+    x.bo = H.body_off;
+    x.nb = H.nbodies;
+    x.po = H.proxy_off;
+    x.np = H.nproxies;
+    x.co = H.contact_off;
+    x.cap = H.contact_cap;
+    x.jo = H.joint_off;
+    x.nj = H.njoints;
+    x.eo = H.ev_off;
+    x.ecap = H.ev_cap;
+    x.lane = int(threadIdx.x);
+    Counters k;
+    k.nc = H.ncontacts;
+    k.nfree = H.nfree;
+    k.pool = H.pool_size;
+    k.status = std::uint32_t(H.status);
+    k.evw = H.ev_words;
+    k.evn = H.ev_count;
+    IslandScratch S;
+    S.off = d.s_csr_off + x.bo + w;
+    S.csr = reinterpret_cast<int2*>(d.s_csr) + 2 * x.co;
+    S.ibody = d.s_isl_body + x.bo + x.co;
+    S.icont = d.s_isl_contact + x.co;
+    S.istart = d.s_isl_start + x.bo + w;
+    S.ijoint = d.s_isl_joint + x.jo;
+    S.jstart = d.s_isl_jstart + x.bo + w;
+    S.bisland = d.s_stack + 2 * x.bo + x.nb;
+    S.isl_word = d.s_keys + x.co;
+    S.isl_under = d.s_keys2 + x.co;
+    __syncthreads();
+    toi_phase(d, x, k, S, c);
+    if (x.lane == 0) {
+        H.ncontacts = k.nc;
+        H.nfree = k.nfree;
+        H.pool_size = k.pool;
+        H.status = int(k.status);
+        H.ev_words = k.evw;
+        H.ev_count = k.evn;
+    }
 }
 
 template <bool kTape = false>

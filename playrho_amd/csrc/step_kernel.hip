@@ -1,6 +1,6 @@
 // The fused per-world step: one workgroup (one 64-lane wavefront) advances one
 // world through PlayRho's regular step, Step(AabbTreeWorld&, const StepConf&)
-// (AabbTreeWorld.cpp:2144-2219) with doToi == false:
+// (AabbTreeWorld.cpp:2144-2219):
 //
 //   A  proxy creation for new shapes              CreateProxies        :557-579
 //   B  contact destruction (fat AABBs apart)       DestroyContacts      :2290-2331
@@ -9,6 +9,7 @@
 //   E  islands (DFS), per-island Gauss-Seidel solve, writeback, sleep
 //                                                  SolveReg/SolveRegIslandViaGS :1472-1682
 //   F  proxy synchronisation, C and D again        SolveReg             :1532-1557
+//   G  (doToi) the TOI phase, in a second kernel   SolveToi             :1752-1831
 //
 // Lane-parallel phases: proxy AABBs, destruction with stable compaction
 // (ballot prefix counts), narrowphase (one lane per contact), candidate pair
@@ -28,6 +29,12 @@ __global__ __launch_bounds__(64) void k_step_worlds(DeviceState d, StepParams c)
     step_world<false>(d, w, c);
 }
 
+// The TOI phase (doToi) of every world, after k_step_worlds in the same stream.
+__global__ __launch_bounds__(64) void k_toi_worlds(DeviceState d, StepParams c) {
+    const int w = d.schedule[blockIdx.x];
+    toi_world(d, w, c);
+}
+
 // The same step recording its solve for the adjoint (every island swept by
 // one lane: coopMin must exceed every island's constraint count).
 __global__ __launch_bounds__(64) void k_step_worlds_tape(DeviceState d, StepParams c) {
@@ -43,6 +50,11 @@ extern "C" hipError_t prx_launch_step(const prx::DeviceState* d, const prx::Step
     prx::DeviceState dd = *d;
     dd.schedule = d->schedule + first;
     hipLaunchKernelGGL(prxd::k_step_worlds, dim3(unsigned(count)), dim3(64), c->ldsBytes, stream, dd, *c);
+    if (c->doToi && c->dt != 0.0f) {
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(prxd::k_toi_worlds, dim3(unsigned(count)), dim3(64), 0, stream, dd, *c);
+    }
     return hipGetLastError();
 }
 

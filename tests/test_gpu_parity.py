@@ -35,12 +35,17 @@ def test_device_sincos_matches_host_library():
     assert bad.size == 0, (a[bad[:5] // 2], got.reshape(-1)[bad[:5]], want.reshape(-1)[bad[:5]])
 
 
-@pytest.mark.parametrize("name", golden_cases.regular_names())
+def _conf(g):
+    """The golden case's StepConf: the reference defaults, doToi as recorded."""
+    return pa.StepConf(do_toi=1 if g["do_toi"] else 0)
+
+
+@pytest.mark.parametrize("name", golden_cases.names())
 def test_plain_step_matches_reference_every_step(name):
     g = golden_cases.load(name)
-    print("case", name, "steps", g["steps"], flush=True)
+    print("case", name, "steps", g["steps"], "doToi", g["do_toi"], flush=True)
     w = _recording(pa.World.from_desc(g["desc_conf"]))
-    conf = pa.StepConf()
+    conf = _conf(g)
     for step in range(1, g["steps"] + 1):
         w.step(conf)
         s = _state(w)
@@ -54,17 +59,18 @@ def test_plain_step_matches_reference_every_step(name):
                 pytest.fail("%s: step %d differs from the reference in %s%s" % (name, step, part, detail))
 
 
-@pytest.mark.parametrize("coop_min", [None, 1, 1 << 30])
-def test_batched_step_matches_reference(coop_min):
-    """All golden worlds in one batch, one launch per step.  coop_min 1 sweeps
-    every island with contacts by the whole wavefront (LDS-staged records),
-    1 << 30 sweeps every island on one lane; both equal the reference."""
-    names = golden_cases.regular_names()
+@pytest.mark.parametrize("coop_min,toi", [(None, False), (1, False), (1 << 30, False), (None, True)])
+def test_batched_step_matches_reference(coop_min, toi):
+    """All golden worlds of one doToi setting in one batch, one launch per
+    step.  coop_min 1 sweeps every island with contacts by the whole wavefront
+    (LDS-staged records), 1 << 30 sweeps every island on one lane; all equal
+    the reference."""
+    names = golden_cases.toi_names() if toi else golden_cases.regular_names()
     gs = [golden_cases.load(n) for n in names]
     worlds = [_recording(pa.World.from_desc(g["desc_conf"])) for g in gs]
-    print("coop_min", coop_min, flush=True)
+    print("coop_min", coop_min, "doToi", toi, flush=True)
     batch = pa.Batch(worlds, use_graph=False, coop_min=coop_min)
-    conf = pa.StepConf()
+    conf = pa.StepConf(do_toi=1 if toi else 0)
     steps = max(g["steps"] for g in gs)
     done = {}
     for step in range(1, steps + 1):
@@ -81,14 +87,15 @@ def test_batched_step_matches_reference(coop_min):
     assert not done, done
 
 
-def test_graph_replay_matches_eager():
+@pytest.mark.parametrize("toi", [False, True])
+def test_graph_replay_matches_eager(toi):
     """Steps replayed from a captured hipGraph equal eager launches."""
     descs = [golden_cases.load("pile%d" % i)["desc_conf"] for i in range(4)]
     eager = [_recording(pa.World.from_desc(d)) for d in descs]
     graph = [_recording(pa.World.from_desc(d)) for d in descs]
     be = pa.Batch(eager, use_graph=False)
     bg = pa.Batch(graph, use_graph=True)
-    conf = pa.StepConf()
+    conf = pa.StepConf(do_toi=1 if toi else 0)
     for rep in range(3):
         be.step(conf, 10)
         bg.step(conf, 10)
@@ -143,13 +150,14 @@ def test_lds_schedule_matches_reference(lds_bytes):
             assert dumpfmt.digest(s, (part,)) == g["digests"][steps, k].tobytes(), (n, part)
 
 
-@pytest.mark.parametrize("scene,params,steps", [
-    ("pile", [3, 11], 60), ("pile", [5, 40], 60), ("drop", [1000, 40, 80, 3], 40),
-    ("chain", [3, 5, 32, 64, 5], 150)])
-def test_device_matches_oracle(scene, params, steps):
+@pytest.mark.parametrize("scene,params,steps,toi", [
+    ("pile", [3, 11], 60, False), ("pile", [5, 40], 60, False), ("drop", [1000, 40, 80, 3], 40, False),
+    ("chain", [3, 5, 32, 64, 5], 150, False), ("pile", [3, 11], 60, True), ("drop", [1000, 40, 80, 3], 40, True),
+    ("billiards", [2, 1], 120, True)])
+def test_device_matches_oracle(scene, params, steps, toi):
     w = _recording(pa.World.scene(scene, params))
-    ow = oracle_lib.OracleWorld(w.save_desc() + _conf_bytes())
-    conf = pa.StepConf()
+    conf = pa.StepConf(do_toi=1 if toi else 0)
+    ow = oracle_lib.OracleWorld(w.save_desc() + dumpfmt.conf_record(conf))
     for step in range(1, steps + 1):
         w.step(conf)
         got = _state(w)
