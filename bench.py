@@ -50,6 +50,8 @@ def parse():
     p.add_argument("--adjoint-worlds", type=int, default=1024, help="differentiable billiards worlds (0: skip)")
     p.add_argument("--adjoint-steps", type=int, default=120)
     p.add_argument("--adjoint-every", type=int, default=16)
+    p.add_argument("--toi-worlds", type=int, default=4096,
+                   help="the batched pile worlds again with doToi = true (0: skip)")
     p.add_argument("--chain-worlds", type=int, default=8192, help="articulated chain worlds (0: skip)")
     p.add_argument("--chain-steps", type=int, default=240)
     p.add_argument("--chain-cpu-worlds", type=int, default=1024, help="worlds in the CPU sample")
@@ -93,29 +95,58 @@ def algorithmic_bytes(awake, proxies, shape_bytes, contacts, joints=0):
     return 116 * awake + 48 * proxies + shape_bytes + 136 * contacts + 88 * joints
 
 
-def cpu_baseline(args, threads):
+def cpu_baseline(args, threads, do_toi=False):
     ref = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
     n = min(args.cpu_worlds, args.worlds)
     if os.path.exists(ref):
-        out = subprocess.run([ref, "--time", "pile", str(args.warmup), str(args.steps), "0", str(threads),
-                              str(n), str(args.seed), "0"], check=True, capture_output=True, text=True).stdout
+        out = subprocess.run([ref, "--time", "pile", str(args.warmup), str(args.steps), "1" if do_toi else "0",
+                              str(threads), str(n), str(args.seed), "0"],
+                             check=True, capture_output=True, text=True).stdout
         r = json.loads(out.strip().splitlines()[-1])
         return dict(value=r["body_steps"] / r["seconds"], unit="body-steps/s", cores=threads, kind="reference",
                     sample="reference PlayRho float build (oracle/_ref, g++ -O3), worlds 0..%d of the same "
                            "workload, steps %d..%d timed after %d warmup, %d std::threads, one world per "
-                           "thread at a time, doToi=false" % (n - 1, args.warmup + 1, args.warmup + args.steps,
-                                                              args.warmup, threads),
+                           "thread at a time, doToi=%s" % (n - 1, args.warmup + 1, args.warmup + args.steps,
+                                                           args.warmup, threads, "true" if do_toi else "false"),
                     seconds=r["seconds"])
     from tests import oracle_lib
     import playrho_amd as pa
     descs = []
     for w in range(n):
         wd = pa.World.scene("pile", [args.seed, w])
-        descs.append(wd.save_desc() + conf_record(pa))
+        descs.append(wd.save_desc() + conf_record(pa, do_toi))
     secs, bs = oracle_lib.time_batch(descs, args.steps, threads)
     return dict(value=bs / secs, unit="body-steps/s", cores=threads, kind="port",
                 sample="oracle restatement, worlds 0..%d, %d steps from step 1" % (n - 1, args.steps),
                 seconds=secs)
+
+
+def toi_bench(args, pa, threads, regular_value):
+    """The batched pile workload stepped with the TOI phase (doToi = true, the
+    reference's default StepConf): the regular step kernel then the TOI kernel
+    per step, both in the captured graph."""
+    conf = pa.StepConf(do_toi=1)
+    n = args.toi_worlds
+    worlds = [pa.World.scene("pile", [args.seed, w]) for w in range(n)]
+    batch = pa.Batch(worlds, use_graph=True, coop_min=args.coop_min or None, lds_bytes=args.lds_bytes or None)
+    dyn = batch.dynamic_bodies()
+    batch.step(conf, args.warmup)
+    batch.sync()
+    t1 = time.perf_counter()
+    batch.step(conf, args.steps)
+    batch.sync()
+    wall = time.perf_counter() - t1
+    kernel_ms, launches = batch.last_timing()
+    out = {"value": dyn * args.steps / wall, "unit": "body-steps/s", "ms_per_step": 1e3 * wall / args.steps,
+           "workload": "%d batched uneven pile worlds (seed %d), dt=1/60 8/3 iterations, doToi=true "
+                       "(TOI defaults of StepConf), steps %d..%d" % (n, args.seed, args.warmup + 1,
+                                                                     args.warmup + args.steps),
+           "ratio_vs_doToi_false": (dyn * args.steps / wall) / regular_value}
+    if not args.no_cpu:
+        cb = cpu_baseline(args, threads, do_toi=True)
+        out["cpu_baseline"] = cb
+        out["ratio_vs_cpu"] = out["value"] / cb["value"]
+    return out
 
 
 def large_world(args, pa, conf):
@@ -275,17 +306,17 @@ def articulated(args, pa, conf, threads):
     return out
 
 
-def conf_record(pa):
-    """The StepConf record the oracle reads (the default conf)."""
+def conf_record(pa, do_toi=False):
+    """The StepConf record the oracle reads (the default conf, doToi as given)."""
     import struct
-    c = pa.StepConf()
+    c = pa.StepConf(do_toi=1 if do_toi else 0)
     floats = [getattr(c, f) for f, _ in pa.StepConf._fields_[:18]]
     toi_f = [c.toi_resolution_rate, c.toi_min_separation, c.toi_min_momentum, c.target_depth, c.tolerance]
     toi_i = [c.toi_velocity_iters, c.toi_position_iters, c.max_toi_root_iters, c.max_toi_iters,
              c.max_distance_iters, c.max_sub_steps]
     return (struct.pack("<I", 18) + struct.pack("<18f", *floats) +
             struct.pack("<III", c.reg_velocity_iters, c.reg_position_iters,
-                        (1 if c.do_warm_start else 0) | (4 if c.do_blocksolve else 0)) +
+                        (1 if c.do_warm_start else 0) | (2 if c.do_toi else 0) | (4 if c.do_blocksolve else 0)) +
             struct.pack("<I", 5) + struct.pack("<5f", *toi_f) + struct.pack("<6I", *toi_i))
 
 
@@ -388,6 +419,8 @@ def main():
         result["roofline"]["traffic"] = tr["bytes_per_launch"]
         result["roofline"]["traffic_note"] = tr["note"]
 
+    if args.toi_worlds > 0:
+        result["toi"] = toi_bench(args, pa, cpu_threads(args), value)
     if args.big_bodies > 0 and args.big_steps > 0:
         result["large_world"] = large_world(args, pa, conf)
     if args.adjoint_worlds > 0 and args.adjoint_steps > 0:

@@ -57,6 +57,19 @@ __device__ __forceinline__ Xf xf_at(const SweepD& s, float beta) {
     return get_xf(p.x, p.y, p.z, s.lc);
 }
 
+// A sweep whose two positions are equal has the same transformation at every
+// beta in [0, 1]: each pos1 - pos0 is a zero whose sign does not depend on
+// beta, so pos0 + (pos1 - pos0) * beta is one value.  Such sweeps (static
+// bodies, bodies at rest) are transformed once per time of impact.
+struct SweepXf {
+    SweepD s;
+    bool fixed;
+    Xf xf;
+    __device__ __forceinline__ explicit SweepXf(const SweepD& sw)
+        : s(sw), fixed(sw.p0.x == sw.p1.x && sw.p0.y == sw.p1.y && sw.p0.z == sw.p1.z), xf(xf_at(sw, 0.0f)) {}
+    __device__ __forceinline__ Xf at(float beta) const { return fixed ? xf : xf_at(s, beta); }
+};
+
 // Body::Advance0 / Advance / SetSweep (Body.hpp:697-727, 1004-1009).
 __device__ __forceinline__ void body_advance0(const DeviceState& d, int gb, float alpha) {
     const float a0 = d.alpha0[gb];
@@ -134,7 +147,7 @@ __device__ __forceinline__ Simplex simplex2(const SEdge& a, const SEdge& b, floa
 }
 
 // Simplex::Get (Simplex.cpp:62-175).
-__device__ __noinline__ Simplex solve_simplex(const SEdge* e, std::uint32_t n) {
+__device__ __forceinline__ Simplex solve_simplex(const SEdge* e, std::uint32_t n) {
     if (n == 1) return simplex1(e[0]);
     if (n == 2) {
         const V2 w1 = pdelta(e[0]), w2 = pdelta(e[1]);
@@ -235,7 +248,7 @@ struct DistOut {
 };
 
 // Distance (Distance.cpp:95-175).
-__device__ __noinline__ DistOut gjk_distance(const Proxy& pA, const Xf& xfA, const Proxy& pB, const Xf& xfB,
+__device__ __forceinline__ DistOut gjk_distance(const Proxy& pA, const Xf& xfA, const Proxy& pB, const Xf& xfB,
                                              const SimplexCache& cache, std::uint32_t maxIters) {
     std::uint32_t sa[3] = {cache.ia[0], cache.ia[1], cache.ia[2]};
     std::uint32_t sb[3] = {cache.ib[0], cache.ib[1], cache.ib[2]};
@@ -389,7 +402,7 @@ __device__ __forceinline__ float evaluate_sep(const Scenario& f, const Proxy& pA
 // GetToiViaSat (TimeOfImpact.cpp:35-251) with the step's ToiConf
 // (ToiConf.cpp:26-35, timeMax 1).  Returns the time when the state is
 // e_touching, a negative value otherwise (the caller then uses 1).
-__device__ __noinline__ float toi_via_sat(const Proxy& pA, const SweepD& sA, const Proxy& pB, const SweepD& sB,
+__device__ __forceinline__ float toi_via_sat(const Proxy& pA, const SweepD& sA, const Proxy& pB, const SweepD& sB,
                                           const StepParams& c) {
     const float totalRadius = pA.radius + pB.radius;
     if (c.targetDepth > totalRadius) return -1.0f;
@@ -401,7 +414,8 @@ __device__ __noinline__ float toi_via_sat(const Proxy& pA, const SweepD& sA, con
     const float maxTargetSq = maxTarget * maxTarget;
     if (!__builtin_isfinite(maxTargetSq) && __builtin_isfinite(maxTarget)) return -1.0f;
     float timeLo = 0.0f;
-    Xf loA = xf_at(sA, timeLo), loB = xf_at(sB, timeLo);
+    const SweepXf wA(sA), wB(sB);
+    Xf loA = wA.xf, loB = wB.xf;
     SimplexCache cache = empty_cache();
     for (std::uint32_t toiIters = 0; toiIters < c.maxToiIters;) {
         const DistOut di = gjk_distance(pA, loA, pB, loB, cache, c.maxDistanceIters);
@@ -413,7 +427,7 @@ __device__ __noinline__ float toi_via_sat(const Proxy& pA, const SweepD& sA, con
         if (distSq <= maxTargetSq) return timeLo;  // e_touching
         const Scenario fcn = get_scenario(cache, pA, loA, pB, loB);
         float timeHi = 1.0f;
-        Xf hiA = xf_at(sA, timeHi), hiB = xf_at(sB, timeHi);
+        Xf hiA = wA.at(timeHi), hiB = wB.at(timeHi);
         for (std::uint32_t pbIter = 0; pbIter < 254u; ++pbIter) {
             const SepIdx hiMin = find_min_sep(fcn, pA, hiA, pB, hiB);
             if (hiMin.distance > maxTarget) return -1.0f;  // e_separated
@@ -434,7 +448,7 @@ __device__ __noinline__ float toi_via_sat(const Proxy& pA, const SweepD& sA, con
                 // Secant / Bisect (Math.hpp:95-108).
                 const float t = (roots & 1u) ? a1 + (target - s1) * (a2 - a1) / (s2 - s1) : (a1 + a2) / 2.0f;
                 ++roots;
-                const Xf tA = xf_at(sA, t), tB = xf_at(sB, t);
+                const Xf tA = wA.at(t), tB = wB.at(t);
                 const float sv = evaluate_sep(fcn, pA, tA, pB, tB, hiMin.ia, hiMin.ib);
                 if (__builtin_fabsf(sv - target) <= c.tolerance) {
                     timeHi = t;
@@ -720,7 +734,7 @@ __device__ __forceinline__ void toi_solve_island(const DeviceState& d, const Wor
 }
 
 // SolveToi(ContactID) (AabbTreeWorld.cpp:1833-1966) on lane 0.
-__device__ __noinline__ void toi_event(const DeviceState& d, const WorldCtx& x, Counters& k, const IslandScratch& S,
+__device__ __forceinline__ void toi_event(const DeviceState& d, const WorldCtx& x, Counters& k, const IslandScratch& S,
                                        int i, const StepParams& c) {
     const int gi = x.co + i;
     const float toi = d.c_toi[gi];
@@ -784,9 +798,22 @@ __device__ __forceinline__ void toi_phase(const DeviceState& d, const WorldCtx& 
         if (pass == 0) {
             for (int i = x.lane; i < k.nc; i += kWave)
                 if (needs_toi(d, x, i, c)) contact_toi(d, x, i, c);
-        } else if (x.lane == 0) {
-            for (int i = 0; i < k.nc; ++i)
-                if (needs_toi(d, x, i, c)) contact_toi(d, x, i, c);
+        } else {
+            // The filters do not change during the pass (they read the
+            // contact's own TOI state, its flags and the bodies' awake
+            // flags), so the wave lists the candidates in order and lane 0
+            // computes them one after another.
+            int ncand = 0;
+            for (int base = 0; base < k.nc; base += kWave) {
+                const int i = base + x.lane;
+                const bool need = i < k.nc && needs_toi(d, x, i, c);
+                const unsigned long long m = __ballot(need);
+                if (need) S.icont[ncand + __popcll(m & lanes_below(x.lane))] = i;
+                ncand += __popcll(m);
+            }
+            __syncthreads();
+            if (x.lane == 0)
+                for (int q = 0; q < ncand; ++q) contact_toi(d, x, S.icont[q], c);
         }
         __syncthreads();
         // GetSoonestContact (AabbTreeWorld.cpp:2870-2890): the least TOI below

@@ -59,17 +59,20 @@ class Collector:
 
 
 @pytest.mark.parametrize("name,steps", [("hello", 60), ("pile3", 150), ("billiards0", 120), ("chain0", 120),
-                                        ("drop300", 80)])
+                                        ("drop300", 80), ("pile3_toi", 150), ("billiards0_toi", 120)])
 def test_listener_calls_match_oracle(name, steps):
+    """The TOI cases add the calls of the TOI phase: its updates and its
+    post-solve reports with the position iterations."""
     g = golden_cases.load(name)
-    print("case", name, "steps", steps, flush=True)
+    conf = pa.StepConf(do_toi=1 if g["do_toi"] else 0)
+    print("case", name, "steps", steps, "doToi", g["do_toi"], flush=True)
     w = pa.World.from_desc(g["desc_conf"])
     col = Collector()
     col.attach(w)
     ow = oracle_lib.OracleWorld(g["desc_conf"])
     kinds = np.zeros(4, dtype=np.int64)
     for step in range(1, steps + 1):
-        w.step()
+        w.step(conf)
         want = _oracle_calls(ow.step(1)[0])
         got = col.take()
         if got != want:
