@@ -332,7 +332,7 @@ def main():
         sys.exit(2)
     import playrho_amd as pa
     pa.require_device()
-    conf = pa.StepConf()
+    conf = pa.StepConf(do_toi=0)
 
     t0 = time.time()
     worlds = [pa.World.scene("pile", [args.seed, w]) for w in range(args.worlds)]

@@ -8,7 +8,7 @@ library is missing or no device is present.
     import playrho_amd as pa
     w = pa.World()
     w.build_scene("pyramid", [20])
-    conf = pa.StepConf()          # reference StepConf defaults, doToi = 0
+    conf = pa.StepConf()          # the reference's StepConf defaults (doToi = 1)
     w.step(conf)
     xf = w.transformation(5)
 """
@@ -515,7 +515,7 @@ class BigWorld:
             self.h = None
 
     def step(self, conf=None, nsteps=1):
-        conf = conf or StepConf()
+        conf = conf or StepConf(do_toi=0)  # no TOI phase in the coloured mode
         _check(lib().prx_big_step(self.h, ctypes.byref(conf), int(nsteps)))
 
     def sync(self):
@@ -558,7 +558,7 @@ class Adjoint:
 
     def __init__(self, batch, nsteps, every=16, conf=None):
         self.batch = batch
-        self.conf = conf or StepConf()
+        self.conf = conf or StepConf(do_toi=0)  # the regular step is differentiated
         self.h = ctypes.c_void_p()
         _check(lib().prx_adjoint_create(batch.h, int(nsteps), int(every), ctypes.byref(self.conf),
                                         ctypes.byref(self.h)))

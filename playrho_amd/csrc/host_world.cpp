@@ -1214,7 +1214,7 @@ void prx_step_conf_default(prx_step_conf* c) {
     c->reg_velocity_iters = 8;
     c->reg_position_iters = 3;
     c->do_warm_start = 1;
-    c->do_toi = 0;
+    c->do_toi = 1;
     c->do_blocksolve = 1;
     c->toi_resolution_rate = 75.0f / 100.0f;
     c->toi_min_separation = -0.005f * 1.5f;

@@ -109,6 +109,7 @@ public:
         Check(prx_batch_create(run.worlds.data(), int(W), &bc, &run.batch), "batch create");
         prx_step_conf sc;
         prx_step_conf_default(&sc);
+        sc.do_toi = 0;  // the adjoint differentiates the regular step
         Check(prx_adjoint_create(run.batch, int(steps), int(every), &sc, &run.adj), "adjoint create");
 
         Tensor loss = torch::empty({W}, cpu);
@@ -220,6 +221,7 @@ public:
         Check(prx_batch_create(run.worlds.data(), int(W), &bc, &run.batch), "batch create");
         prx_step_conf sc;
         prx_step_conf_default(&sc);
+        sc.do_toi = 0;  // the adjoint differentiates the regular step
         Check(prx_adjoint_create(run.batch, int(steps), int(every), &sc, &run.adj), "adjoint create");
         Tensor loss = torch::empty({W}, cpu);
         Tensor gv = torch::empty({W, B, 3}, cpu);

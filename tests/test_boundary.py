@@ -96,6 +96,12 @@ def test_step_conf_defaults_match_reference():
     mine = np.array([getattr(c, f) for f, _ in pa.StepConf._fields_[:18]], dtype=np.float32)
     assert np.array_equal(mine.view("<u4"), ref["floats"].view("<u4")), (mine, ref["floats"])
     assert (c.reg_velocity_iters, c.reg_position_iters) == (ref["vel_iters"], ref["pos_iters"])
+    # doWarmStart, doToi, doBlocksolve: the reference's defaults (the golden
+    # driver runs the TOI cases with the default StepConf).
+    flags = (1 if c.do_warm_start else 0) | (2 if c.do_toi else 0) | (4 if c.do_blocksolve else 0)
+    r2 = dumpfmt.Reader(golden_cases.load("pyramid_toi")["desc_conf"])
+    dumpfmt.read_description(r2)
+    assert flags == dumpfmt.read_conf(r2)["flags"] == 7
     toi = np.array([getattr(c, f) for f in dumpfmt.TOI_FLOATS], dtype=np.float32)
     assert np.array_equal(toi.view("<u4"), ref["toi_floats"].view("<u4")), (toi, ref["toi_floats"])
     assert [getattr(c, f) for f in dumpfmt.TOI_ITERS] == list(ref["toi_iters"])

@@ -49,7 +49,7 @@ def _setup(seed, world, steps):
 # step 72, on a normal impulse clamp).
 @pytest.mark.parametrize("steps,every,full_iters", [(12, 4, False), (30, 16, False), (60, 16, True)])
 def test_adjoint_matches_fp64_finite_differences(steps, every, full_iters):
-    conf = pa.StepConf()
+    conf = pa.StepConf(do_toi=0)
     if full_iters:
         conf.reg_min_momentum = -1.0
     w, desc, bodies, dyn, targets = _setup(1, 0, steps)
@@ -117,7 +117,7 @@ def test_adjoint_matches_fp64_finite_differences(steps, every, full_iters):
 
 def test_adjoint_batched_equals_single():
     """Every world of a batch gets the gradients it gets alone."""
-    conf = pa.StepConf()
+    conf = pa.StepConf(do_toi=0)
     steps = 10
     singles = []
     for wi in range(3):

@@ -20,7 +20,7 @@ def _f32(x):
 @pytest.mark.parametrize("steps", [12, 30])
 def test_general_kernel_equals_disk_kernel(steps):
     """Both reverse kernels on the same recorded billiards steps."""
-    conf = pa.StepConf()
+    conf = pa.StepConf(do_toi=0)
     _, desc, _, _, targets = _setup(1, 0, steps)
     disk = pa.Adjoint(pa.Batch([pa.World.from_desc(desc)], use_graph=False), steps, 8, conf)
     a = disk.run(targets.astype(np.float32))
@@ -61,7 +61,7 @@ def test_chain_adjoint_matches_fp64_finite_differences(params, steps, jsel, bsel
     polygon / disk / edge contacts, against central differences of the FP64
     step (two step sizes; parameters where they disagree have an event inside
     the perturbation and are left out)."""
-    conf = pa.StepConf()
+    conf = pa.StepConf(do_toi=0)
     w, desc, bodies, joints, dyn, targets = _chain(params)
     print("chain", params, "steps", steps, flush=True)
     adj = pa.Adjoint(pa.Batch([w], use_graph=False), steps, 8, conf)
@@ -113,7 +113,7 @@ def test_chain_adjoint_matches_fp64_finite_differences(params, steps, jsel, bsel
 
 
 def test_chain_batched_equals_single():
-    conf = pa.StepConf()
+    conf = pa.StepConf(do_toi=0)
     steps = 20
     singles = []
     for wi in range(3):

@@ -72,7 +72,7 @@ def test_big_prephase_matches_reference():
     ContactFeatures) equals the reference restatement's, bit for bit."""
     w = pa.World.scene("drop", [1500, 30, 60, 5])
     desc = w.save_desc()
-    conf = pa.StepConf(delta_time=0.0)
+    conf = pa.StepConf(delta_time=0.0, do_toi=0)
     ow = oracle_lib.OracleWorld(desc + _conf_bytes(conf))
     big = pa.BigWorld(w)
     for step in range(2):
@@ -152,7 +152,7 @@ def test_big_solve_quality_against_reference():
     scene = [800, 25, 30, 3]
     w = pa.World.scene("drop", scene)
     desc = w.save_desc()
-    conf = pa.StepConf()
+    conf = pa.StepConf(do_toi=0)
     ow = oracle_lib.OracleWorld(desc + _conf_bytes(conf))
     big = pa.BigWorld(w)
     steps = 300
@@ -205,7 +205,7 @@ def test_big_deterministic():
     a = pa.World.scene("drop", [800, 15, 30, 9])
     b = pa.World.scene("drop", [800, 15, 30, 9])
     ba, bb = pa.BigWorld(a), pa.BigWorld(b)
-    conf = pa.StepConf()
+    conf = pa.StepConf(do_toi=0)
     ba.step(conf, 60)
     bb.step(conf, 60)
     ba.sync()

@@ -102,7 +102,7 @@ def test_batch_with_listeners_delivers_every_step():
     batch = pa.Batch(worlds, use_graph=True)
     oracles = [oracle_lib.OracleWorld(g["desc_conf"]) for g in gs]
     steps = 40
-    batch.step(pa.StepConf(), steps)
+    batch.step(pa.StepConf(do_toi=0), steps)
     batch.download()
     for c, o, w, g in zip(cols, oracles, worlds, gs):
         states = o.step(steps)

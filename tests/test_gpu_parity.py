@@ -113,7 +113,7 @@ def test_autotune_restores_state_and_keeps_results():
     b = [_recording(pa.World.from_desc(d)) for d in descs]
     ba = pa.Batch(a, use_graph=True)
     bb = pa.Batch(b, use_graph=True)
-    conf = pa.StepConf()
+    conf = pa.StepConf(do_toi=0)
     ba.step(conf, 5)
     bb.step(conf, 5)
     ba.sync()
@@ -140,7 +140,7 @@ def test_lds_schedule_matches_reference(lds_bytes):
     gs = [golden_cases.load(n) for n in names]
     worlds = [_recording(pa.World.from_desc(g["desc_conf"])) for g in gs]
     batch = pa.Batch(worlds, use_graph=True, lds_bytes=lds_bytes)
-    conf = pa.StepConf()
+    conf = pa.StepConf(do_toi=0)
     steps = 100
     batch.step(conf, steps)
     batch.download()
@@ -168,4 +168,4 @@ def test_device_matches_oracle(scene, params, steps, toi):
 
 def _conf_bytes():
     """The StepConf record (dump layout) of the default conf."""
-    return dumpfmt.conf_record(pa.StepConf())
+    return dumpfmt.conf_record(pa.StepConf(do_toi=0))
