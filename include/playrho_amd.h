@@ -112,10 +112,11 @@ typedef struct prx_step_conf {
     uint32_t max_distance_iters, max_sub_steps;
 } prx_step_conf;
 
-/* StepStats (StepStats.hpp:30-201) of a regular step: PreStepStats
- * (:38-46) and RegStepStats (:69-82), fields in the reference's order
- * (ToiStepStats stays default: doToi = false).  reg_min_separation is
- * +infinity and reg_max_inc_impulse 0 when no island was solved. */
+/* StepStats (StepStats.hpp:30-201): PreStepStats (:38-46), RegStepStats
+ * (:69-82) and ToiStepStats (:117-151), fields in the reference's order.
+ * The min separations are +infinity and the max incremental impulses 0 when
+ * no island was solved in the phase; the toi_ fields stay so with doToi =
+ * false. */
 typedef struct prx_step_stats {
     uint32_t pre_proxies_created, pre_proxies_moved, pre_contacts_destroyed, pre_contacts_added;
     uint32_t pre_contacts_updated, pre_contacts_skipped;
@@ -123,6 +124,11 @@ typedef struct prx_step_stats {
     uint32_t reg_islands_found, reg_islands_solved, reg_bodies_slept, reg_max_island_bodies;
     uint32_t reg_contacts_added, reg_contacts_updated, reg_contacts_skipped, reg_proxies_moved;
     uint32_t reg_sum_pos_iters, reg_sum_vel_iters;
+    float toi_min_separation, toi_max_inc_impulse;
+    uint32_t toi_islands_found, toi_islands_solved, toi_contacts_found, toi_contacts_at_max_sub_steps;
+    uint32_t toi_contacts_updated_toi, toi_contacts_updated_touching, toi_contacts_skipped_touching;
+    uint32_t toi_contacts_added, toi_proxies_moved, toi_sum_pos_iters, toi_sum_vel_iters;
+    uint32_t toi_max_dist_iters, toi_max_toi_iters, toi_max_root_iters;
 } prx_step_stats;
 
 /* Launch schedule of the batched step (chosen by prx_batch_autotune). */

@@ -184,6 +184,14 @@ void WriteState(const Handle<R>& h, Out& o) {
                                   s.regUpdated,        s.regSkipped,      s.regProxiesMoved,
                                   s.sumPosIters,       s.sumVelIters};
     o.put(st, sizeof(st));
+    // ToiStepStats in declaration order (StepStats.hpp:117-151).
+    const std::uint32_t ts[16] = {bits(s.toiMinSeparation), bits(s.toiMaxIncImpulse), s.toiIslandsFound,
+                                  s.toiIslandsSolved,       s.toiContactsFound,       s.toiAtMaxSubSteps,
+                                  s.toiUpdatedToi,          s.toiUpdatedTouching,     s.toiSkippedTouching,
+                                  s.toiAdded,               s.toiProxiesMoved,        s.toiSumPosIters,
+                                  s.toiSumVelIters,         s.toiMaxDistIters,        s.toiMaxToiIters,
+                                  s.toiMaxRootIters};
+    o.put(ts, sizeof(ts));
     o.u32(std::uint32_t(w.bodies.size()));
     for (const auto& b : w.bodies) {
         const float v[10] = {float(b.pos0.lin.x), float(b.pos0.lin.y), float(b.pos0.ang),

@@ -661,6 +661,12 @@ struct StepStats {
     double minSeparation = std::numeric_limits<double>::infinity(), maxIncImpulse = 0;
     std::uint32_t islandsFound = 0, islandsSolved = 0, bodiesSlept = 0, maxIslandBodies = 0, regAdded = 0,
                   regUpdated = 0, regSkipped = 0, regProxiesMoved = 0, sumPosIters = 0, sumVelIters = 0;
+    // ToiStepStats (StepStats.hpp:117-151).
+    double toiMinSeparation = std::numeric_limits<double>::infinity(), toiMaxIncImpulse = 0;
+    std::uint32_t toiIslandsFound = 0, toiIslandsSolved = 0, toiContactsFound = 0, toiAtMaxSubSteps = 0,
+                  toiUpdatedToi = 0, toiUpdatedTouching = 0, toiSkippedTouching = 0, toiAdded = 0,
+                  toiProxiesMoved = 0, toiSumPosIters = 0, toiSumVelIters = 0, toiMaxDistIters = 0,
+                  toiMaxToiIters = 0, toiMaxRootIters = 0;
 };
 
 // IslandStats of one island solve (StepStats.hpp:149-163).
@@ -668,6 +674,10 @@ struct IslandStats {
     double minSeparation = std::numeric_limits<double>::infinity(), maxIncImpulse = 0;
     bool solved = false;
     std::uint32_t positionIters = 0, velocityIters = 0, bodiesSlept = 0;
+};
+// The contact counts ProcessContactsForTOI adds (AabbTreeWorld.cpp:2059-2142).
+struct ToiIslandStats : IslandStats {
+    std::uint32_t contactsUpdated = 0, contactsSkipped = 0;
 };
 
 // Solver scratch records.
@@ -2213,9 +2223,11 @@ public:
         kToiUnknown, kToiTouching, kToiSeparated, kToiOverlapped, kToiMaxRootIters, kToiNextAfter, kToiMaxToiIters,
         kToiBelowMinTarget, kToiMaxDistIters, kToiTargetDepthExceeds, kToiOverflow
     };
+    // ToiOutput (ToiOutput.hpp:44-70): time, state and the iteration statistics.
     struct ToiOut {
         R time;
         ToiState state;
+        std::uint32_t toiIters = 0, maxDistIters = 0, maxRootIters = 0;
     };
     // GetToiViaSat (TimeOfImpact.cpp:35-251) with ToiConf from the StepConf
     // (ToiConf.cpp:26-35): timeMax 1.
@@ -2233,36 +2245,50 @@ public:
         R timeLo = 0;
         Xf<R> loA = XfAt(sA, timeLo), loB = XfAt(sB, timeLo);
         Cache cache;
-        std::uint32_t toiIters = 0;
+        std::uint32_t toiIters = 0, maxDist = 0, maxRoot = 0;
+        auto out = [&](R t, ToiState st) {
+            ToiOut o{t, st};
+            o.toiIters = toiIters;
+            o.maxDistIters = maxDist;
+            o.maxRootIters = maxRoot;
+            return o;
+        };
         while (toiIters < conf.maxToiIters) {
             const DistOut di = Distance(pA, loA, pB, loB, cache, conf.maxDistanceIters);
             ++toiIters;
-            if (di.state == kDistHitMaxIters) return ToiOut{timeLo, kToiMaxDistIters};
+            maxDist = std::max(maxDist, di.iters);
+            if (di.state == kDistHitMaxIters) return out(timeLo, kToiMaxDistIters);
             cache = CacheOf(di.s);
             const R distSq = MagSq(WitnessDelta(di.s));
-            if (distSq < minTargetSq) return ToiOut{timeLo, kToiOverlapped};
-            if (distSq <= maxTargetSq) return ToiOut{timeLo, kToiTouching};
+            if (distSq < minTargetSq) return out(timeLo, kToiOverlapped);
+            if (distSq <= maxTargetSq) return out(timeLo, kToiTouching);
             const Scenario fcn = GetScenario(cache, pA, loA, pB, loB);
             R timeHi = R(1);
             Xf<R> hiA = XfAt(sA, timeHi), hiB = XfAt(sB, timeHi);
             for (std::uint32_t pbIter = 0; pbIter < 254u; ++pbIter) {
                 const SepIdx hiMin = FindMinSep(fcn, pA, hiA, pB, hiB);
-                if (hiMin.distance > maxTarget) return ToiOut{timeHi, kToiSeparated};
+                if (hiMin.distance > maxTarget) return out(timeHi, kToiSeparated);
                 if (hiMin.distance >= minTarget) {
-                    if (timeHi == timeLo) return ToiOut{timeHi, kToiSeparated};
+                    if (timeHi == timeLo) return out(timeHi, kToiSeparated);
                     timeLo = timeHi;
                     loA = hiA;
                     loB = hiB;
                     break;
                 }
                 const R loEval = Evaluate(fcn, pA, loA, pB, loB, hiMin.ia, hiMin.ib);
-                if (loEval <= maxTarget) return ToiOut{timeLo, (loEval < minTarget) ? kToiBelowMinTarget : kToiTouching};
+                if (loEval <= maxTarget) return out(timeLo, (loEval < minTarget) ? kToiBelowMinTarget : kToiTouching);
                 R a1 = timeLo, a2 = timeHi, s1 = loEval, s2 = hiMin.distance;
                 std::uint32_t roots = 0;
                 R t = timeLo;
                 for (;;) {
-                    if (roots == conf.maxToiRootIters) return ToiOut{t, kToiMaxRootIters};
-                    if (std::nextafter(a1, a2) >= a2) return ToiOut{t, kToiNextAfter};
+                    if (roots == conf.maxToiRootIters) {
+                        maxRoot = std::max(maxRoot, roots);
+                        return out(t, kToiMaxRootIters);
+                    }
+                    if (std::nextafter(a1, a2) >= a2) {
+                        maxRoot = std::max(maxRoot, roots);
+                        return out(t, kToiNextAfter);
+                    }
                     // Secant / Bisect (Math.hpp:95-108).
                     t = (roots & 1u) ? a1 + (target - s1) * (a2 - a1) / (s2 - s1) : (a1 + a2) / 2;
                     ++roots;
@@ -2282,18 +2308,22 @@ public:
                         s2 = sv;
                     }
                 }
+                maxRoot = std::max(maxRoot, roots);
             }
         }
-        return ToiOut{timeLo, kToiMaxToiIters};
+        return out(timeLo, kToiMaxToiIters);
     }
 
     // UpdateContactTOIs (AabbTreeWorld.cpp:1685-1750).
-    void UpdateContactTOIs(const StepConf<R>& conf) {
+    void UpdateContactTOIs(const StepConf<R>& conf, StepStats& st) {
         for (const auto& kc : contacts) {
             Contact<R>& c = contactBuf[kc.id];
             if (c.toiValid) continue;
             if (!(c.flags & kCEnabled) || (c.flags & kSensor) || !(c.flags & kCImpenetrable)) continue;
-            if (c.toiCount >= conf.maxSubSteps) continue;
+            if (c.toiCount >= conf.maxSubSteps) {
+                ++st.toiAtMaxSubSteps;
+                continue;
+            }
             Body<R>& bA = bodies[c.bodyA];
             Body<R>& bB = bodies[c.bodyB];
             if (!bA.awake() && !bB.awake()) continue;
@@ -2305,11 +2335,16 @@ public:
             const ToiOut out = ToiViaSat(shapes[c.shapeA].proxy(), sA, shapes[c.shapeB].proxy(), sB, conf);
             c.toi = (out.state == kToiTouching) ? Min(alpha0 + (R(1) - alpha0) * out.time, R(1)) : R(1);
             c.toiValid = true;
+            st.toiMaxDistIters = std::max(st.toiMaxDistIters, out.maxDistIters);
+            st.toiMaxToiIters = std::max(st.toiMaxToiIters, out.toiIters);
+            st.toiMaxRootIters = std::max(st.toiMaxRootIters, out.maxRootIters);
+            ++st.toiUpdatedToi;
         }
     }
 
     // ProcessContactsForTOI (AabbTreeWorld.cpp:2059-2142).
-    void ProcessContactsForTOI(std::uint32_t id, Island& island, R toi, const StepConf<R>& conf) {
+    void ProcessContactsForTOI(std::uint32_t id, Island& island, R toi, const StepConf<R>& conf,
+                               ToiIslandStats& is) {
         const bool bodyImpenetrable = (bodies[id].flags & kImpenetrable) != 0;
         for (const auto& kc : bodyContacts[id]) {
             const std::uint32_t cid = kc.id;
@@ -2324,7 +2359,12 @@ public:
                 BodyAdvance(other, toi);
                 if (other.pos0 != backup.pos0 || other.pos1 != backup.pos1) FlagForUpdating(otherId);
             }
-            if (contactBuf[cid].flags & kNeedsUpdating) Update(cid, conf);
+            if (contactBuf[cid].flags & kNeedsUpdating) {
+                Update(cid, conf);
+                ++is.contactsUpdated;
+            } else {
+                ++is.contactsSkipped;
+            }
             const std::uint32_t fl = contactBuf[cid].flags;
             if (!(fl & kCEnabled) || !(fl & kTouching)) {
                 SetSweep(other, backup);
@@ -2341,7 +2381,7 @@ public:
     }
 
     // SolveToiViaGS (AabbTreeWorld.cpp:1968-2057) with conf.dt the rest of the step.
-    void SolveToiViaGS(const Island& island, const StepConf<R>& conf) {
+    void SolveToiViaGS(const Island& island, const StepConf<R>& conf, IslandStats& is) {
         StepConf<R> still = conf;
         still.dt = 0;
         std::vector<BodyConstraint<R>> bcs(bodies.size());
@@ -2361,11 +2401,14 @@ public:
         for (std::uint32_t i = 0; i < conf.toiPosIters; ++i) {
             R minSep = std::numeric_limits<R>::infinity();
             for (const auto& pc : pcs) minSep = Min(minSep, SolvePC(pc, bcs, ps));
+            is.minSeparation = Min(is.minSeparation, double(minSep));
             if (minSep >= conf.toiMinSeparation) {
                 positionIters = i + 1;
+                is.solved = true;
                 break;
             }
         }
+        is.positionIters = positionIters;
         for (std::uint32_t bid : island.bodies) bodies[bid].pos0 = bcs[bid].pos;
         StepConf<R> vconf = conf;  // GetToiVelocityConstraintConf (VelocityConstraint.cpp:207-210)
         vconf.warmStart = false;
@@ -2378,10 +2421,15 @@ public:
             const WorldManifold<R> wm = GetWorldManifold(manifoldBuf[cid], xfA, rA, xfB, rB);
             vcs.push_back(MakeVC(c, wm, bcs, vconf));
         }
+        is.velocityIters = conf.toiVelIters;
         for (std::uint32_t i = 0; i < conf.toiVelIters; ++i) {
             R newInc = 0;
             for (auto& vc : vcs) newInc = Max(newInc, SolveVC(vc, bcs));
-            if (newInc <= conf.toiMinMomentum) break;
+            if (newInc <= conf.toiMinMomentum) {
+                is.velocityIters = i + 1;
+                break;
+            }
+            is.maxIncImpulse = Max(is.maxIncImpulse, double(newInc));
         }
         const R h = conf.dt;
         for (std::uint32_t bid : island.bodies) {
@@ -2414,7 +2462,10 @@ public:
     }
 
     // SolveToi(ContactID) (AabbTreeWorld.cpp:1833-1966).
-    void SolveToiContact(std::uint32_t cid, const StepConf<R>& conf) {
+    IslandStats SolveToiContact(std::uint32_t cid, const StepConf<R>& conf, std::uint32_t& updated,
+                                std::uint32_t& skipped) {
+        IslandStats is;
+        updated = skipped = 0;
         const R toi = contactBuf[cid].toi;
         const std::uint32_t ia = contactBuf[cid].bodyA, ib = contactBuf[cid].bodyB;
         Body<R>& bA = bodies[ia];
@@ -2425,14 +2476,18 @@ public:
             if (bA.pos0 != backupA.pos0 || bA.pos1 != backupA.pos1) FlagForUpdating(ia);
             BodyAdvance(bB, toi);
             if (bB.pos0 != backupB.pos0 || bB.pos1 != backupB.pos1) FlagForUpdating(ib);
-            if (contactBuf[cid].flags & kNeedsUpdating) Update(cid, conf);
+            if (contactBuf[cid].flags & kNeedsUpdating) {
+                Update(cid, conf);
+                ++updated;
+            }
             contactBuf[cid].toiValid = false;
             ++contactBuf[cid].toiCount;
             const std::uint32_t fl = contactBuf[cid].flags;
             if (!(fl & kCEnabled) || !(fl & kTouching)) {
                 SetSweep(bA, backupA);
                 SetSweep(bB, backupB);
-                return;
+                skipped = updated ? 0u : 1u;
+                return is;
             }
         }
         if (bA.speedable()) bA.SetAwakeFlag();
@@ -2444,22 +2499,26 @@ public:
         island.bodies.push_back(ia);
         island.bodies.push_back(ib);
         island.contacts.push_back(cid);
-        if (bA.accelerable()) ProcessContactsForTOI(ia, island, toi, conf);
-        if (bB.accelerable()) ProcessContactsForTOI(ib, island, toi, conf);
+        ToiIslandStats proc;
+        if (bA.accelerable()) ProcessContactsForTOI(ia, island, toi, conf, proc);
+        if (bB.accelerable()) ProcessContactsForTOI(ib, island, toi, conf, proc);
         for (std::uint32_t b : island.bodies) {
             if (!bodies[b].speedable()) islandedBodies[b] = 0;
         }
         StepConf<R> sub = conf;
         sub.dt = (R(1) - toi) * conf.dt;
-        SolveToiViaGS(island, sub);
+        SolveToiViaGS(island, sub, is);
+        updated += proc.contactsUpdated;
+        skipped += proc.contactsSkipped;
+        return is;
     }
 
     // SolveToi(const StepConf&) (AabbTreeWorld.cpp:1752-1831), no sub-stepping.
-    void SolveToiPhase(const StepConf<R>& conf) {
+    void SolveToiPhase(const StepConf<R>& conf, StepStats& st) {
         islandedBodies.assign(bodies.size(), 0);
         islandedContacts.assign(contactBuf.size(), 0);
         for (;;) {
-            UpdateContactTOIs(conf);
+            UpdateContactTOIs(conf, st);
             std::uint32_t next = 0xffffffffu;
             R minToi = R(1);
             for (const auto& kc : contacts) {
@@ -2477,22 +2536,36 @@ public:
                 }
                 break;
             }
-            if (!islandedContacts[next]) SolveToiContact(next, conf);
+            ++st.toiContactsFound;
+            if (!islandedContacts[next]) {
+                std::uint32_t updated = 0, skipped = 0;
+                const IslandStats r = SolveToiContact(next, conf, updated, skipped);
+                st.toiMinSeparation = Min(st.toiMinSeparation, r.minSeparation);
+                st.toiMaxIncImpulse = Max(st.toiMaxIncImpulse, r.maxIncImpulse);
+                st.toiIslandsSolved += r.solved ? 1u : 0u;
+                st.toiSumPosIters += r.positionIters;
+                st.toiSumVelIters += r.velocityIters;
+                if (r.positionIters > 0 || r.velocityIters > 0) ++st.toiIslandsFound;
+                st.toiUpdatedTouching += updated;
+                st.toiSkippedTouching += skipped;
+            }
             for (std::uint32_t bid : bodyOrder) {
                 if (!islandedBodies[bid]) continue;
                 islandedBodies[bid] = 0;
                 if (bodies[bid].accelerable()) {
-                    Synchronize(bid, conf);
+                    st.toiProxiesMoved += Synchronize(bid, conf);
                     for (const auto& kc : bodyContacts[bid]) {
                         contactBuf[kc.id].toiValid = false;
                         islandedContacts[kc.id] = 0;
                     }
                 }
             }
-            FindAndAddContacts(conf);
+            st.toiAdded += FindAndAddContacts(conf);
             if (islandedContacts.size() < contactBuf.size()) islandedContacts.resize(contactBuf.size(), 0);
         }
-        UpdateContacts(conf);
+        std::uint32_t skipped = 0;
+        st.toiUpdatedTouching += UpdateContacts(conf, &skipped);
+        st.toiSkippedTouching += skipped;
     }
 
     // Step (AabbTreeWorld.cpp:2144-2219); the TOI phase when conf.toi.
@@ -2510,7 +2583,7 @@ public:
         st.preAdded = FindAndAddContacts(conf);
         if (conf.dt != R(0)) {
             SolveReg(conf, st);
-            if (conf.toi) SolveToiPhase(conf);
+            if (conf.toi) SolveToiPhase(conf, st);
         }
         return st;
     }

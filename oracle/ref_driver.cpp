@@ -273,6 +273,13 @@ void WriteState(Out& o, const World& world, std::uint32_t step, const StepStats&
         std::uint32_t(st.reg.maxIslandBodies), st.reg.contactsAdded, st.reg.contactsUpdated,
         st.reg.contactsSkipped, st.reg.proxiesMoved, st.reg.sumPosIters, st.reg.sumVelIters};
     o.put(stats, sizeof(stats));
+    const std::uint32_t toi[16] = {
+        bits(float(st.toi.minSeparation)), bits(float(st.toi.maxIncImpulse)), st.toi.islandsFound,
+        st.toi.islandsSolved, st.toi.contactsFound, st.toi.contactsAtMaxSubSteps, st.toi.contactsUpdatedToi,
+        st.toi.contactsUpdatedTouching, st.toi.contactsSkippedTouching, st.toi.contactsAdded,
+        st.toi.proxiesMoved, st.toi.sumPosIters, st.toi.sumVelIters, std::uint32_t(st.toi.maxDistIters),
+        std::uint32_t(st.toi.maxToiIters), std::uint32_t(st.toi.maxRootIters)};
+    o.put(toi, sizeof(toi));
     const auto nbodies = GetBodyRange(world);
     o.u32(nbodies);
     for (auto i = 0u; i < nbodies; ++i) {

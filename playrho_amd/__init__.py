@@ -155,7 +155,7 @@ class RevoluteJointConf(ctypes.Structure):
 
 
 class StepStats(ctypes.Structure):
-    """StepStats of a regular step (StepStats.hpp:30-201): PreStepStats and RegStepStats."""
+    """StepStats (StepStats.hpp:30-201): PreStepStats, RegStepStats and ToiStepStats."""
     _fields_ = [(n, ctypes.c_uint32) for n in (
         "pre_proxies_created", "pre_proxies_moved", "pre_contacts_destroyed", "pre_contacts_added",
         "pre_contacts_updated", "pre_contacts_skipped")] + [
@@ -163,7 +163,13 @@ class StepStats(ctypes.Structure):
         (n, ctypes.c_uint32) for n in (
             "reg_islands_found", "reg_islands_solved", "reg_bodies_slept", "reg_max_island_bodies",
             "reg_contacts_added", "reg_contacts_updated", "reg_contacts_skipped", "reg_proxies_moved",
-            "reg_sum_pos_iters", "reg_sum_vel_iters")]
+            "reg_sum_pos_iters", "reg_sum_vel_iters")] + [
+        ("toi_min_separation", ctypes.c_float), ("toi_max_inc_impulse", ctypes.c_float)] + [
+        (n, ctypes.c_uint32) for n in (
+            "toi_islands_found", "toi_islands_solved", "toi_contacts_found", "toi_contacts_at_max_sub_steps",
+            "toi_contacts_updated_toi", "toi_contacts_updated_touching", "toi_contacts_skipped_touching",
+            "toi_contacts_added", "toi_proxies_moved", "toi_sum_pos_iters", "toi_sum_vel_iters",
+            "toi_max_dist_iters", "toi_max_toi_iters", "toi_max_root_iters")]
 
 
 class BatchConf(ctypes.Structure):

@@ -110,6 +110,7 @@ __global__ void k_big_pending(DeviceState d, StepParams c) {
     if (p == 0) {
         for (int i = 0; i < prx::kStats; ++i) d.hdr[0].stats[i] = 0u;
         d.hdr[0].stats[prx::kStRegMinSeparation] = 0x7f800000u;
+        d.hdr[0].stats[prx::kStToiMinSeparation] = 0x7f800000u;
     }
     if (p >= d.hdr[0].nproxies) return;
     if (d.pflags[p] & prx::kProxyPending) {

@@ -290,6 +290,15 @@ struct HostContacts {
 
 struct prx_batch;
 
+namespace {
+prx_step_stats DefaultStats() {
+    prx_step_stats s{};
+    s.reg_min_separation = std::numeric_limits<float>::infinity();
+    s.toi_min_separation = std::numeric_limits<float>::infinity();
+    return s;
+}
+}  // namespace
+
 struct prx_world {
     prx_world_conf conf{};
     std::vector<HostShape> shapes;
@@ -302,7 +311,7 @@ struct prx_world {
     int index_in_owner = -1;
     bool host_stale = false;     // device state newer than this mirror
     prx_batch* solo = nullptr;   // single-world batch used by prx_step
-    prx_step_stats last_stats{0, 0, 0, 0, 0, 0, std::numeric_limits<float>::infinity()};  // StepStats{}
+    prx_step_stats last_stats = DefaultStats();  // StepStats{}
     bool record_events = false;  // the device records the contact listener calls of each step
     bool has_listener = false;
     prx_contact_listener listener{};
@@ -1721,7 +1730,7 @@ int prx_world_state_record(prx_world* w, std::uint8_t* buf, std::size_t cap, std
     };
     auto u32 = [&](std::uint32_t v) { put(&v, 4); };
     u32(0);
-    put(&w->last_stats, sizeof(prx_step_stats));  // the 18 StepStats words, in order
+    put(&w->last_stats, sizeof(prx_step_stats));  // the 34 StepStats words, in order
     u32(std::uint32_t(w->bodies.size()));
     for (const auto& b : w->bodies) {
         const float v[10] = {b.pos0[0], b.pos0[1], b.pos0[2], b.pos1[0], b.pos1[1],

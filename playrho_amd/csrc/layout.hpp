@@ -54,21 +54,24 @@ enum : std::uint32_t {
 };
 
 // Per-world header (AoS: read once per phase by the world's workgroup).
-// Words of WorldHeader::stats: PreStepStats then RegStepStats, each in the
-// reference's declaration order (StepStats.hpp:38-46, 69-82); the two real
-// fields hold float bits.
+// Words of WorldHeader::stats: PreStepStats, RegStepStats and ToiStepStats,
+// each in the reference's declaration order (StepStats.hpp:38-46, 69-82,
+// 117-151); the real fields hold float bits.
 enum StatIndex : int {
     kStPreProxiesCreated = 0, kStPreProxiesMoved, kStPreDestroyed, kStPreAdded, kStPreUpdated, kStPreSkipped,
     kStRegMinSeparation, kStRegMaxIncImpulse, kStRegIslandsFound, kStRegIslandsSolved, kStRegBodiesSlept,
     kStRegMaxIslandBodies, kStRegAdded, kStRegUpdated, kStRegSkipped, kStRegProxiesMoved, kStRegSumPosIters,
-    kStRegSumVelIters, kStats
+    kStRegSumVelIters, kStToiMinSeparation, kStToiMaxIncImpulse, kStToiIslandsFound, kStToiIslandsSolved,
+    kStToiContactsFound, kStToiAtMaxSubSteps, kStToiUpdatedToi, kStToiUpdatedTouching, kStToiSkippedTouching,
+    kStToiAdded, kStToiProxiesMoved, kStToiSumPosIters, kStToiSumVelIters, kStToiMaxDistIters, kStToiMaxToiIters,
+    kStToiMaxRootIters, kStats
 };
 
 struct WorldHeader {
     std::int32_t body_off, nbodies, proxy_off, nproxies;
     std::int32_t contact_off, contact_cap, ncontacts, pool_size;
     std::int32_t nfree, status, joint_off, njoints;
-    std::uint32_t stats[18];  // StatIndex
+    std::uint32_t stats[34];  // StatIndex
     // Contact listener calls of the last step: word segment [ev_off, ev_off +
     // ev_cap) of DeviceState::ev (ev_cap 0: not recorded), words and calls
     // written (words may exceed ev_cap: then the status has kStatusEventOverflow).

@@ -2721,7 +2721,7 @@ __device__ __forceinline__ void toi_world(const DeviceState& d, int w, const Ste
     S.isl_word = d.s_keys + x.co;
     S.isl_under = d.s_keys2 + x.co;
     __syncthreads();
-    toi_phase(d, x, k, S, c);
+    toi_phase(d, x, k, S, c, H);
     if (x.lane == 0) {
         H.ncontacts = k.nc;
         H.nfree = k.nfree;
@@ -2762,6 +2762,7 @@ __device__ __forceinline__ void step_world(const DeviceState& d, int w, const St
     if (x.lane == 0) {
         for (int i = 0; i < prx::kStats; ++i) H.stats[i] = 0u;
         H.stats[prx::kStRegMinSeparation] = 0x7f800000u;  // +infinity: no position iteration ran
+        H.stats[prx::kStToiMinSeparation] = 0x7f800000u;
     }
     PhaseTimer T;
     T.begin(d.phase_cycles ? d.phase_cycles + std::size_t(w) * prx::kPhases : nullptr);

@@ -245,6 +245,7 @@ enum : int { kDistHitMaxIters = 0, kDistMaxPoints, kDistUnfitSearchDir, kDistDup
 struct DistOut {
     Simplex s;
     int state;
+    std::uint32_t iters;
 };
 
 // Distance (Distance.cpp:95-175).
@@ -308,7 +309,7 @@ __device__ __forceinline__ DistOut gjk_distance(const Proxy& pA, const Xf& xfA, 
             sb[k] = k < n ? e[k].iB : 255u;
         }
     }
-    return DistOut{simplex, state};
+    return DistOut{simplex, state, iter};
 }
 
 // GetDelta(GetWitnessPoints(simplex)) (Distance.cpp:75-93, Distance.hpp:55-58).
@@ -399,20 +400,32 @@ __device__ __forceinline__ float evaluate_sep(const Scenario& f, const Proxy& pA
     return dot(delta, normal);
 }
 
+// ToiOutput (ToiOutput.hpp:44-70): the time when the state is e_touching (a
+// negative value otherwise, for which the caller uses 1) and the iteration
+// statistics.
+struct ToiRes {
+    float time;
+    std::uint32_t toiIters, maxDist, maxRoot;
+};
+
 // GetToiViaSat (TimeOfImpact.cpp:35-251) with the step's ToiConf
-// (ToiConf.cpp:26-35, timeMax 1).  Returns the time when the state is
-// e_touching, a negative value otherwise (the caller then uses 1).
-__device__ __forceinline__ float toi_via_sat(const Proxy& pA, const SweepD& sA, const Proxy& pB, const SweepD& sB,
-                                          const StepParams& c) {
+// (ToiConf.cpp:26-35, timeMax 1).
+__device__ __forceinline__ ToiRes toi_via_sat(const Proxy& pA, const SweepD& sA, const Proxy& pB, const SweepD& sB,
+                                              const StepParams& c) {
+    ToiRes r{-1.0f, 0u, 0u, 0u};
     const float totalRadius = pA.radius + pB.radius;
-    if (c.targetDepth > totalRadius) return -1.0f;
+    if (c.targetDepth > totalRadius) return r;
     const float target = totalRadius - c.targetDepth;
     const float maxTarget = fmax_std(target + c.tolerance, 0.0f);
     const float minTarget = fmax_std(target - c.tolerance, 0.0f);
     const float minTargetSq = minTarget * minTarget;
-    if (!__builtin_isfinite(minTargetSq) && __builtin_isfinite(minTarget)) return -1.0f;
+    if (!__builtin_isfinite(minTargetSq) && __builtin_isfinite(minTarget)) return r;
     const float maxTargetSq = maxTarget * maxTarget;
-    if (!__builtin_isfinite(maxTargetSq) && __builtin_isfinite(maxTarget)) return -1.0f;
+    if (!__builtin_isfinite(maxTargetSq) && __builtin_isfinite(maxTarget)) return r;
+    auto done = [&](float t) {
+        r.time = t;
+        return r;
+    };
     float timeLo = 0.0f;
     const SweepXf wA(sA), wB(sB);
     Xf loA = wA.xf, loB = wB.xf;
@@ -420,31 +433,35 @@ __device__ __forceinline__ float toi_via_sat(const Proxy& pA, const SweepD& sA, 
     for (std::uint32_t toiIters = 0; toiIters < c.maxToiIters;) {
         const DistOut di = gjk_distance(pA, loA, pB, loB, cache, c.maxDistanceIters);
         ++toiIters;
-        if (di.state == kDistHitMaxIters) return -1.0f;
+        r.toiIters = toiIters;
+        r.maxDist = (r.maxDist < di.iters) ? di.iters : r.maxDist;
+        if (di.state == kDistHitMaxIters) return done(-1.0f);
         cache = cache_of(di.s);
         const float distSq = magsq(witness_delta(di.s));
-        if (distSq < minTargetSq) return -1.0f;  // e_overlapped
-        if (distSq <= maxTargetSq) return timeLo;  // e_touching
+        if (distSq < minTargetSq) return done(-1.0f);  // e_overlapped
+        if (distSq <= maxTargetSq) return done(timeLo);  // e_touching
         const Scenario fcn = get_scenario(cache, pA, loA, pB, loB);
         float timeHi = 1.0f;
         Xf hiA = wA.at(timeHi), hiB = wB.at(timeHi);
         for (std::uint32_t pbIter = 0; pbIter < 254u; ++pbIter) {
             const SepIdx hiMin = find_min_sep(fcn, pA, hiA, pB, hiB);
-            if (hiMin.distance > maxTarget) return -1.0f;  // e_separated
+            if (hiMin.distance > maxTarget) return done(-1.0f);  // e_separated
             if (hiMin.distance >= minTarget) {
-                if (timeHi == timeLo) return -1.0f;  // e_separated
+                if (timeHi == timeLo) return done(-1.0f);  // e_separated
                 timeLo = timeHi;
                 loA = hiA;
                 loB = hiB;
                 break;
             }
             const float loEval = evaluate_sep(fcn, pA, loA, pB, loB, hiMin.ia, hiMin.ib);
-            if (loEval <= maxTarget) return (loEval < minTarget) ? -1.0f : timeLo;  // e_belowMinTarget / e_touching
+            if (loEval <= maxTarget) return done((loEval < minTarget) ? -1.0f : timeLo);  // e_belowMinTarget / e_touching
             float a1 = timeLo, a2 = timeHi, s1 = loEval, s2 = hiMin.distance;
             std::uint32_t roots = 0;
             for (;;) {
-                if (roots == c.maxToiRootIters) return -1.0f;  // e_maxRootIters
-                if (nextafter_f(a1, a2) >= a2) return -1.0f;   // e_nextAfter
+                if (roots == c.maxToiRootIters || nextafter_f(a1, a2) >= a2) {  // e_maxRootIters, e_nextAfter
+                    r.maxRoot = (r.maxRoot < roots) ? roots : r.maxRoot;
+                    return done(-1.0f);
+                }
                 // Secant / Bisect (Math.hpp:95-108).
                 const float t = (roots & 1u) ? a1 + (target - s1) * (a2 - a1) / (s2 - s1) : (a1 + a2) / 2.0f;
                 ++roots;
@@ -464,9 +481,10 @@ __device__ __forceinline__ float toi_via_sat(const Proxy& pA, const SweepD& sA, 
                     s2 = sv;
                 }
             }
+            r.maxRoot = (r.maxRoot < roots) ? roots : r.maxRoot;
         }
     }
-    return -1.0f;  // e_maxToiIters
+    return done(-1.0f);  // e_maxToiIters
 }
 
 // ---------------------------------------------------------------------------
@@ -474,8 +492,9 @@ __device__ __forceinline__ float toi_via_sat(const Proxy& pA, const SweepD& sA, 
 // low bits the sub-step count.
 constexpr std::uint32_t kToiValid = 0x80000000u;
 
-// The body of UpdateContactTOIs' loop for a contact that passed its filters.
-__device__ __forceinline__ void contact_toi(const DeviceState& d, const WorldCtx& x, int i, const StepParams& c) {
+// The body of UpdateContactTOIs' loop for a contact that passed its filters;
+// returns the time of impact's iteration statistics.
+__device__ __forceinline__ ToiRes contact_toi(const DeviceState& d, const WorldCtx& x, int i, const StepParams& c) {
     const int gi = x.co + i;
     const int2 bd = d.c_body[gi];
     const int ga = x.bo + bd.x, gb = x.bo + bd.y;
@@ -485,21 +504,29 @@ __device__ __forceinline__ void contact_toi(const DeviceState& d, const WorldCtx
     const SweepD sA = normalized(sweep_of(d, ga));
     const SweepD sB = normalized(sweep_of(d, gb));
     const int2 sh = d.c_shape[gi];
-    const float t = toi_via_sat(shape_proxy(d, sh.x), sA, shape_proxy(d, sh.y), sB, c);
-    d.c_toi[gi] = (t >= 0.0f) ? fmin_std(alpha0 + (1.0f - alpha0) * t, 1.0f) : 1.0f;
+    const ToiRes r = toi_via_sat(shape_proxy(d, sh.x), sA, shape_proxy(d, sh.y), sB, c);
+    d.c_toi[gi] = (r.time >= 0.0f) ? fmin_std(alpha0 + (1.0f - alpha0) * r.time, 1.0f) : 1.0f;
     d.c_toiinfo[gi] |= kToiValid;
+    return r;
 }
 
-// UpdateContactTOIs' filters.
-__device__ __forceinline__ bool needs_toi(const DeviceState& d, const WorldCtx& x, int i, const StepParams& c) {
+// What one event adds (IslandStats, StepStats.hpp:149-163, with the contact counts).
+struct ToiEventStats {
+    float minSep, maxInc;
+    bool solved;
+    std::uint32_t posIters, velIters, updated, skipped;
+};
+
+// UpdateContactTOIs' filters: 1 compute, 2 skipped at maxSubSteps (counted), 0 skipped.
+__device__ __forceinline__ int needs_toi(const DeviceState& d, const WorldCtx& x, int i, const StepParams& c) {
     const int gi = x.co + i;
     const std::uint32_t info = d.c_toiinfo[gi];
-    if (info & kToiValid) return false;
+    if (info & kToiValid) return 0;
     const std::uint32_t f = d.c_flags[gi];
-    if (!(f & prx::kCEnabled) || (f & prx::kSensor) || !(f & prx::kCImpenetrable)) return false;
-    if ((info & 0xffffu) >= c.maxSubSteps) return false;
+    if (!(f & prx::kCEnabled) || (f & prx::kSensor) || !(f & prx::kCImpenetrable)) return 0;
+    if ((info & 0xffffu) >= c.maxSubSteps) return 2;
     const int2 bd = d.c_body[gi];
-    return ((d.bflags[x.bo + bd.x] | d.bflags[x.bo + bd.y]) & prx::kAwake) != 0;
+    return ((d.bflags[x.bo + bd.x] | d.bflags[x.bo + bd.y]) & prx::kAwake) ? 1 : 0;
 }
 
 // Listener calls made on lane 0 (the event solves): appended one by one.
@@ -571,7 +598,7 @@ struct ToiIsland {
 // ProcessContactsForTOI (AabbTreeWorld.cpp:2059-2142).
 __device__ __forceinline__ void toi_process_contacts(const DeviceState& d, const WorldCtx& x, Counters& k,
                                                      const IslandScratch& S, int b, ToiIsland& I, float toi,
-                                                     const StepParams& c) {
+                                                     const StepParams& c, ToiEventStats& es) {
     const bool bodyImp = (d.bflags[x.bo + b] & prx::kImpenetrable) != 0;
     const int e1 = S.off[b + 1];
     for (int e = S.off[b]; e < e1; ++e) {
@@ -588,7 +615,12 @@ __device__ __forceinline__ void toi_process_contacts(const DeviceState& d, const
             body_advance(d, go, toi);
             if (pos_differ(d.pos0[go], backup.p0) || pos_differ(d.pos1[go], backup.p1)) flag_body_contacts(d, x, S, other);
         }
-        if (d.c_flags[gi] & prx::kNeedsUpdating) toi_update(d, x, k, i, c);
+        if (d.c_flags[gi] & prx::kNeedsUpdating) {
+            toi_update(d, x, k, i, c);
+            ++es.updated;
+        } else {
+            ++es.skipped;
+        }
         const std::uint32_t fl = d.c_flags[gi];
         if (!(fl & prx::kCEnabled) || !(fl & prx::kTouching)) {
             set_sweep(d, go, backup);
@@ -608,7 +640,7 @@ __device__ __forceinline__ void toi_process_contacts(const DeviceState& d, const
 // h the rest of the step; lane 0.
 __device__ __forceinline__ void toi_solve_island(const DeviceState& d, const WorldCtx& x, Counters& k,
                                                  const IslandScratch& S, const ToiIsland& I, float h,
-                                                 const StepParams& c) {
+                                                 const StepParams& c, ToiEventStats& es) {
     const Bodies<false> bs{d.s_bc_pos + x.bo, d.s_bc_vel + x.bo, x.nb};
     StepParams still = c;
     still.dt = 0.0f;
@@ -640,11 +672,14 @@ __device__ __forceinline__ void toi_solve_island(const DeviceState& d, const Wor
             bs.set_pos(pc.bA, pa);
             bs.set_pos(pc.bB, pb);
         }
+        es.minSep = fmin_std(es.minSep, minSep);
         if (minSep >= c.toiMinSeparation) {
             positionIters = it + 1;
+            es.solved = true;
             break;
         }
     }
+    es.posIters = positionIters;
     for (int q = 0; q < I.nb; ++q) {
         const int b = S.ibody[q];
         const float4 p = bs.pos(b);
@@ -672,6 +707,7 @@ __device__ __forceinline__ void toi_solve_island(const DeviceState& d, const Wor
         const Vc vc = make_vc(wm, mt.x, mt.y, mt.z, bd.x, bd.y, A, pA, bs.vel(bd.x), B, pB, bs.vel(bd.y), vconf);
         store_vc(vbase + std::size_t(e) * prx::kVcStride, vc, A, B);
     }
+    es.velIters = c.toiVelIters;
     for (std::uint32_t it = 0; it < c.toiVelIters; ++it) {
         float newInc = 0.0f;
         for (int e = 0; e < I.nc; ++e) {
@@ -682,7 +718,11 @@ __device__ __forceinline__ void toi_solve_island(const DeviceState& d, const Wor
             rec[4] = float4{vc.p[0].ni, vc.p[0].ti, vc.p[0].nm, vc.p[0].tm};
             rec[7] = float4{vc.p[1].ni, vc.p[1].ti, vc.p[1].nm, vc.p[1].tm};
         }
-        if (newInc <= c.toiMinMomentum) break;
+        if (newInc <= c.toiMinMomentum) {
+            es.velIters = it + 1;
+            break;
+        }
+        es.maxInc = fmax_std(es.maxInc, newInc);
     }
     // IntegratePositions (AabbTreeWorld.cpp:176-188) over the rest of the step, writeback.
     for (int q = 0; q < I.nb; ++q) {
@@ -734,8 +774,9 @@ __device__ __forceinline__ void toi_solve_island(const DeviceState& d, const Wor
 }
 
 // SolveToi(ContactID) (AabbTreeWorld.cpp:1833-1966) on lane 0.
-__device__ __forceinline__ void toi_event(const DeviceState& d, const WorldCtx& x, Counters& k, const IslandScratch& S,
-                                       int i, const StepParams& c) {
+__device__ __forceinline__ ToiEventStats toi_event(const DeviceState& d, const WorldCtx& x, Counters& k,
+                                                   const IslandScratch& S, int i, const StepParams& c) {
+    ToiEventStats es{__builtin_inff(), 0.0f, false, 0u, 0u, 0u, 0u};
     const int gi = x.co + i;
     const float toi = d.c_toi[gi];
     const int2 bd = d.c_body[gi];
@@ -746,14 +787,18 @@ __device__ __forceinline__ void toi_event(const DeviceState& d, const WorldCtx& 
         if (pos_differ(d.pos0[ga], backupA.p0) || pos_differ(d.pos1[ga], backupA.p1)) flag_body_contacts(d, x, S, bd.x);
         body_advance(d, gb, toi);
         if (pos_differ(d.pos0[gb], backupB.p0) || pos_differ(d.pos1[gb], backupB.p1)) flag_body_contacts(d, x, S, bd.y);
-        if (d.c_flags[gi] & prx::kNeedsUpdating) toi_update(d, x, k, i, c);
+        if (d.c_flags[gi] & prx::kNeedsUpdating) {
+            toi_update(d, x, k, i, c);
+            ++es.updated;
+        }
         const std::uint32_t info = d.c_toiinfo[gi];
         d.c_toiinfo[gi] = ((info & 0xffffu) + 1u) & 0xffffu;  // SetToi({}), IncrementToiCount
         const std::uint32_t fl = d.c_flags[gi];
         if (!(fl & prx::kCEnabled) || !(fl & prx::kTouching)) {
             set_sweep(d, ga, backupA);
             set_sweep(d, gb, backupB);
-            return;
+            es.skipped = es.updated ? 0u : 1u;
+            return es;
         }
     }
     if (d.bflags[ga] & prx::kSpeedable) d.bflags[ga] |= prx::kAwake;
@@ -765,14 +810,15 @@ __device__ __forceinline__ void toi_event(const DeviceState& d, const WorldCtx& 
     S.ibody[I.nb++] = bd.x;
     S.ibody[I.nb++] = bd.y;
     S.icont[I.nc++] = i;
-    if (d.bflags[ga] & prx::kAccelerable) toi_process_contacts(d, x, k, S, bd.x, I, toi, c);
-    if (d.bflags[gb] & prx::kAccelerable) toi_process_contacts(d, x, k, S, bd.y, I, toi, c);
+    if (d.bflags[ga] & prx::kAccelerable) toi_process_contacts(d, x, k, S, bd.x, I, toi, c, es);
+    if (d.bflags[gb] & prx::kAccelerable) toi_process_contacts(d, x, k, S, bd.y, I, toi, c, es);
     // RemoveUnspeedablesFromIslanded.
     for (int q = 0; q < I.nb; ++q) {
         const int g = x.bo + S.ibody[q];
         if (!(d.bflags[g] & prx::kSpeedable)) d.t_isl_b[g] = 0;
     }
-    toi_solve_island(d, x, k, S, I, (1.0f - toi) * c.dt, c);
+    toi_solve_island(d, x, k, S, I, (1.0f - toi) * c.dt, c, es);
+    return es;
 }
 
 // Every solved event raises one contact's sub-step count, which stops at
@@ -783,9 +829,30 @@ __device__ __forceinline__ int toi_event_bound(const WorldCtx& x, const StepPara
     return int(c.maxSubSteps + 1u) * (x.cap + 1) + 64;
 }
 
-// The TOI phase of step_world, after the regular phase.
+__device__ __forceinline__ std::uint32_t wave_sum(std::uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ __forceinline__ std::uint32_t wave_max(std::uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const std::uint32_t u = __shfl_xor(v, o);
+        v = (v < u) ? u : v;
+    }
+    return v;
+}
+
+// The TOI phase of one world, after the regular phase; its ToiStepStats go to
+// the header (lane 0).
 __device__ __forceinline__ void toi_phase(const DeviceState& d, const WorldCtx& x, Counters& k,
-                                          const IslandScratch& S, const StepParams& c) {
+                                          const IslandScratch& S, const StepParams& c, WorldHeader& H) {
+    // The statistics accumulate in the header (lane 0), so that they hold no
+    // registers across the loop.
+    std::uint32_t* st = H.stats;
+    if (x.lane == 0) {
+        for (int i = prx::kStToiMinSeparation; i < prx::kStats; ++i) st[i] = 0u;
+        st[prx::kStToiMinSeparation] = 0x7f800000u;
+    }
+    auto addMax = [&](int i, std::uint32_t v) { st[i] = (st[i] < v) ? v : st[i]; };
     for (int i = x.lane; i < k.nc; i += kWave) {
         d.c_toiinfo[x.co + i] = 0u;
         d.t_isl_c[x.co + i] = 0;
@@ -794,10 +861,21 @@ __device__ __forceinline__ void toi_phase(const DeviceState& d, const WorldCtx& 
     __syncthreads();
     build_adjacency<true>(d, x, k.nc, S);
     for (int pass = 0;; ++pass) {
+        // Per-lane maxima and counts of the times of impact this pass computes.
+        std::uint32_t lMaxDist = 0, lMaxToi = 0, lMaxRoot = 0, lUpdated = 0, lAtMax = 0;
+        auto addToi = [&](const ToiRes& r) {
+            lMaxDist = (lMaxDist < r.maxDist) ? r.maxDist : lMaxDist;
+            lMaxToi = (lMaxToi < r.toiIters) ? r.toiIters : lMaxToi;
+            lMaxRoot = (lMaxRoot < r.maxRoot) ? r.maxRoot : lMaxRoot;
+            ++lUpdated;
+        };
         // UpdateContactTOIs.
         if (pass == 0) {
-            for (int i = x.lane; i < k.nc; i += kWave)
-                if (needs_toi(d, x, i, c)) contact_toi(d, x, i, c);
+            for (int i = x.lane; i < k.nc; i += kWave) {
+                const int need = needs_toi(d, x, i, c);
+                if (need == 1) addToi(contact_toi(d, x, i, c));
+                else if (need == 2) ++lAtMax;
+            }
         } else {
             // The filters do not change during the pass (they read the
             // contact's own TOI state, its flags and the bodies' awake
@@ -806,14 +884,27 @@ __device__ __forceinline__ void toi_phase(const DeviceState& d, const WorldCtx& 
             int ncand = 0;
             for (int base = 0; base < k.nc; base += kWave) {
                 const int i = base + x.lane;
-                const bool need = i < k.nc && needs_toi(d, x, i, c);
+                const int nd = (i < k.nc) ? needs_toi(d, x, i, c) : 0;
+                const bool need = nd == 1;
+                if (nd == 2) ++lAtMax;
                 const unsigned long long m = __ballot(need);
                 if (need) S.icont[ncand + __popcll(m & lanes_below(x.lane))] = i;
                 ncand += __popcll(m);
             }
             __syncthreads();
             if (x.lane == 0)
-                for (int q = 0; q < ncand; ++q) contact_toi(d, x, S.icont[q], c);
+                for (int q = 0; q < ncand; ++q) addToi(contact_toi(d, x, S.icont[q], c));
+        }
+        {
+            const std::uint32_t upd = wave_sum(lUpdated), atm = wave_sum(lAtMax);
+            const std::uint32_t md = wave_max(lMaxDist), mt = wave_max(lMaxToi), mr = wave_max(lMaxRoot);
+            if (x.lane == 0) {
+                st[prx::kStToiUpdatedToi] += upd;
+                st[prx::kStToiAtMaxSubSteps] += atm;
+                addMax(prx::kStToiMaxDistIters, md);
+                addMax(prx::kStToiMaxToiIters, mt);
+                addMax(prx::kStToiMaxRootIters, mr);
+            }
         }
         __syncthreads();
         // GetSoonestContact (AabbTreeWorld.cpp:2870-2890): the least TOI below
@@ -839,17 +930,35 @@ __device__ __forceinline__ void toi_phase(const DeviceState& d, const WorldCtx& 
             if (bi != 0x7fffffff) k.status |= prx::kStatusToiLimit;
             break;
         }
-        if (x.lane == 0 && !d.t_isl_c[x.co + bi]) toi_event(d, x, k, S, bi, c);
+        if (x.lane == 0) {
+            ++st[prx::kStToiContactsFound];
+            if (!d.t_isl_c[x.co + bi]) {
+                const ToiEventStats es = toi_event(d, x, k, S, bi, c);
+                st[prx::kStToiMinSeparation] =
+                    __float_as_uint(fmin_std(__uint_as_float(st[prx::kStToiMinSeparation]), es.minSep));
+                st[prx::kStToiMaxIncImpulse] =
+                    __float_as_uint(fmax_std(__uint_as_float(st[prx::kStToiMaxIncImpulse]), es.maxInc));
+                st[prx::kStToiIslandsSolved] += es.solved ? 1u : 0u;
+                st[prx::kStToiSumPosIters] += es.posIters;
+                st[prx::kStToiSumVelIters] += es.velIters;
+                if (es.posIters > 0 || es.velIters > 0) ++st[prx::kStToiIslandsFound];
+                st[prx::kStToiUpdatedTouching] += es.updated;
+                st[prx::kStToiSkippedTouching] += es.skipped;
+            }
+        }
         k.evw = __shfl(k.evw, 0);
         k.evn = __shfl(k.evn, 0);
         k.status = __shfl(k.status, 0);
         __syncthreads();
         // Island bodies: synchronise the accelerable ones' proxies and
         // invalidate their contacts' TOIs, then clear the island flags.
+        std::uint32_t moved = 0;
         for (int b = x.lane; b < x.nb; b += kWave) {
             const int g = x.bo + b;
-            if (d.t_isl_b[g] && (d.bflags[g] & prx::kAccelerable)) synchronize_body(d, x, b, c);
+            if (d.t_isl_b[g] && (d.bflags[g] & prx::kAccelerable)) moved += synchronize_body(d, x, b, c);
         }
+        moved = wave_sum(moved);
+        if (x.lane == 0) st[prx::kStToiProxiesMoved] += moved;
         for (int i = x.lane; i < k.nc; i += kWave) {
             const int2 bd = d.c_body[x.co + i];
             const int ga = x.bo + bd.x, gb = x.bo + bd.y;
@@ -864,7 +973,8 @@ __device__ __forceinline__ void toi_phase(const DeviceState& d, const WorldCtx& 
         for (int b = x.lane; b < x.nb; b += kWave) d.t_isl_b[x.bo + b] = 0;
         __syncthreads();
         const int nc0 = k.nc;
-        phase_find_add(d, x, k, c);
+        const std::uint32_t added = phase_find_add(d, x, k, c);
+        if (x.lane == 0) st[prx::kStToiAdded] += added;
         __syncthreads();
         if (k.nc > nc0) {
             for (int i = nc0 + x.lane; i < k.nc; i += kWave) {
@@ -879,7 +989,11 @@ __device__ __forceinline__ void toi_phase(const DeviceState& d, const WorldCtx& 
     for (int b = x.lane; b < x.nb; b += kWave) d.alpha0[x.bo + b] = 0.0f;
     for (int i = x.lane; i < k.nc; i += kWave) d.c_toiinfo[x.co + i] = 0u;
     __syncthreads();
-    phase_update(d, x, k, c);
+    const std::uint32_t updated = phase_update(d, x, k, c);
+    if (x.lane == 0) {
+        st[prx::kStToiUpdatedTouching] += updated;
+        st[prx::kStToiSkippedTouching] += std::uint32_t(k.nc) - updated;
+    }
     __syncthreads();
 }
 

@@ -27,8 +27,12 @@ EVENT_PAYLOAD = (0, 0, 15, 6)
 # destroyed, added, updated, skipped), then RegStepStats (min separation and
 # max incremental impulse as float bits, islands found, solved, bodies slept,
 # max island bodies, contacts added, updated, skipped, proxies moved, sums of
-# position and velocity iterations).
-STATS = 18
+# position and velocity iterations), then ToiStepStats (min separation and
+# max incremental impulse as float bits, islands found, solved, contacts
+# found, at max sub-steps, TOIs updated, updated touching, skipped touching,
+# added, proxies moved, sums of position and velocity iterations, max
+# distance, TOI and root iterations).
+STATS = 34
 
 
 class Reader:
