@@ -194,6 +194,20 @@ int prx_world_get_bodies(prx_world* world, float* out, uint32_t* flags);
 int prx_world_get_transformation(prx_world* world, uint32_t body, float out[4]);
 /* SetVelocity(World&, BodyID, Velocity) (WorldBody.hpp). */
 int prx_world_set_velocity(prx_world* world, uint32_t body, const float v[3]);
+/* SetTransformation(World&, BodyID, Transformation) (WorldBody.hpp;
+ * Body.cpp:270-273 through SetBody, AabbTreeWorld.cpp:2723-2795): xf = p.x,
+ * p.y, q.x, q.y.  The body's sweep becomes the single position (p, the angle
+ * of q); a changed transformation flags its contacts for updating and
+ * synchronises its proxies at the next step (pre-step proxies moved). */
+int prx_world_set_transformation(prx_world* world, uint32_t body, const float xf[4]);
+/* SetLocation(World&, BodyID, Length2) (WorldBody.cpp:158-163, Body.cpp:275-278). */
+int prx_world_set_location(prx_world* world, uint32_t body, const float v[2]);
+/* SetAngle(World&, BodyID, Angle) (WorldBody.cpp:165-170, Body.cpp:285-288). */
+int prx_world_set_angle(prx_world* world, uint32_t body, float angle);
+/* ShiftOrigin(World&, Length2) (AabbTreeWorld.cpp:2221-2242): every body's
+ * sweep and every proxy's AABB move by -origin; every contact is updated at
+ * the next step. */
+int prx_world_shift_origin(prx_world* world, const float origin[2]);
 /* SetFriction(World&, ShapeID, NonNegative<Real>) and SetRestitution(World&,
  * ShapeID, Real) (WorldShape.hpp:114,124; WorldShape.cpp:79-96, through
  * SetShape, AabbTreeWorld.cpp:1284-1357): bodies holding the shape are

@@ -267,6 +267,22 @@ unsigned char* Copy(const std::vector<unsigned char>& v, size_t* n) {
 
 }  // namespace
 
+// A world edit between steps: kind 'L' SetLocation(body, (a, b)), 'A'
+// SetAngle(body, a), 'T' SetTransformation(body, {(a, b), UnitVec::Get(c)}),
+// 'O' ShiftOrigin((a, b)).
+template <class R>
+void Edit(Handle<R>& h, int kind, std::uint32_t body, float a, float b, float c) {
+    using namespace oracle;
+    auto& w = h.world;
+    switch (kind) {
+    case 'L': w.SetLocation(body, V2<R>{R(a), R(b)}); break;
+    case 'A': w.SetAngle(body, R(a)); break;
+    case 'T': w.SetTransformation(body, V2<R>{R(a), R(b)}, UnitFromAngle(R(c))); break;
+    case 'O': w.ShiftOrigin(V2<R>{R(a), R(b)}); break;
+    default: break;
+    }
+}
+
 extern "C" {
 
 // Creates an oracle world from a description + conf record (layout of
@@ -293,6 +309,13 @@ void oracle_destroy(void* p) {
     if (a->precision == 64) delete static_cast<Handle<double>*>(a->h);
     else delete static_cast<Handle<float>*>(a->h);
     delete a;
+}
+
+int oracle_edit(void* p, int kind, std::uint32_t body, float a, float b, float c) {
+    auto* an = static_cast<AnyHandle*>(p);
+    if (an->precision == 64) Edit(*static_cast<Handle<double>*>(an->h), kind, body, a, b, c);
+    else Edit(*static_cast<Handle<float>*>(an->h), kind, body, a, b, c);
+    return 0;
 }
 
 // Steps n times; when `records` is non-null, appends one state record per step

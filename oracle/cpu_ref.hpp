@@ -777,6 +777,7 @@ public:
     std::vector<Leaf<R>> tree;             // by proxy (leaf) id
     std::uint32_t leafCount = 0;
     std::vector<std::pair<std::uint32_t, std::uint32_t>> fixturesForProxies;  // (body, shape)
+    std::vector<std::uint32_t> bodiesForSync;  // m_bodiesForSync (AabbTreeWorld.hpp)
     std::vector<std::uint32_t> proxiesForContacts;
     // Contact pool (ObjectPool.hpp:62-116) with manifolds at the same index.
     std::vector<Contact<R>> contactBuf;
@@ -2568,6 +2569,47 @@ public:
         st.toiSkippedTouching += skipped;
     }
 
+    // World edits between steps.  SetBody (AabbTreeWorld.cpp:2723-2795) for a
+    // body whose sweep becomes one position: a new transformation flags the
+    // body's contacts for updating and queues the body for the next step's
+    // Synchronize.
+    void SetSweepPosition(std::uint32_t bid, const Pos<R>& pos) {
+        Body<R>& b = bodies[bid];
+        const Xf<R> old = b.xf;
+        b.pos0 = pos;
+        b.pos1 = pos;
+        b.alpha0 = 0;
+        b.xf = GetXf(pos, b.lc);
+        if (!(old.p == b.xf.p) || !(old.q == b.xf.q)) {
+            FlagForUpdating(bid);
+            bodiesForSync.push_back(bid);
+        }
+    }
+    // SetTransformation(Body&, xfm) (Body.cpp:270-273): the angle of q.
+    void SetTransformation(std::uint32_t bid, V2<R> p, V2<R> q) { SetSweepPosition(bid, Pos<R>{p, std::atan2(q.y, q.x)}); }
+    // SetLocation(Body&, v) (Body.cpp:275-278).
+    void SetLocation(std::uint32_t bid, V2<R> v) { SetTransformation(bid, v, bodies[bid].xf.q); }
+    // SetAngle(Body&, a) (Body.cpp:285-288).
+    void SetAngle(std::uint32_t bid, R a) { SetSweepPosition(bid, Pos<R>{bodies[bid].pos1.lin, a}); }
+    // ShiftOrigin (AabbTreeWorld.cpp:2221-2242, DynamicTree.cpp:650-658).
+    void ShiftOrigin(V2<R> o) {
+        for (std::uint32_t bid : bodyOrder) {
+            Body<R>& b = bodies[bid];
+            b.pos0.lin = b.pos0.lin - o;
+            b.pos1.lin = b.pos1.lin - o;
+            b.xf = GetXf(b.pos1, b.lc);
+            FlagForUpdating(bid);
+        }
+        for (auto& lf : tree) {
+            if (!lf.used) continue;
+            for (int k = 0; k < 2; ++k) {
+                const R ok = k ? o.y : o.x;
+                lf.lo[k] = lf.lo[k] + (-ok);
+                lf.hi[k] = lf.hi[k] + (-ok);
+            }
+        }
+    }
+
     // Step (AabbTreeWorld.cpp:2144-2219); the TOI phase when conf.toi.
     StepStats Step(const StepConf<R>& conf) {
         StepStats st;
@@ -2578,6 +2620,8 @@ public:
             ++st.preProxiesCreated;
         }
         fixturesForProxies.clear();
+        for (std::uint32_t bid : bodiesForSync) st.preProxiesMoved += Synchronize(bid, conf);
+        bodiesForSync.clear();
         st.preDestroyed = DestroyContacts();
         st.preUpdated = UpdateContacts(conf, &st.preSkipped);
         st.preAdded = FindAndAddContacts(conf);

@@ -44,6 +44,7 @@
 #include <thread>
 #include <cstdlib>
 #include <cstring>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -470,6 +471,53 @@ int TimeWorlds(int argc, char** argv) {
     return 0;
 }
 
+// World edits between steps, from the environment variable PRX_EDITS:
+// "step kind body a b c; ...", applied before step `step`:
+//   L  SetLocation(world, body, (a, b))
+//   A  SetAngle(world, body, a)
+//   T  SetTransformation(world, body, {(a, b), UnitVec::Get(c)})
+//   O  ShiftOrigin(world, (a, b))
+struct Edit {
+    int step = 0;
+    char kind = 0;
+    unsigned body = 0;
+    float a = 0, b = 0, c = 0;
+};
+
+std::vector<Edit> ReadEdits() {
+    std::vector<Edit> out;
+    const char* env = std::getenv("PRX_EDITS");
+    if (!env) return out;
+    std::stringstream all(env);
+    std::string item;
+    while (std::getline(all, item, ';')) {
+        std::stringstream in(item);
+        Edit e;
+        if (in >> e.step >> e.kind >> e.body >> e.a >> e.b >> e.c) out.push_back(e);
+    }
+    return out;
+}
+
+void ApplyEdit(World& world, const Edit& e) {
+    const auto id = BodyID(static_cast<BodyID::underlying_type>(e.body));
+    switch (e.kind) {
+    case 'L':
+        SetLocation(world, id, Length2{e.a * Meter, e.b * Meter});
+        break;
+    case 'A':
+        SetAngle(world, id, e.a * Radian);
+        break;
+    case 'T':
+        SetTransformation(world, id, Transformation{Length2{e.a * Meter, e.b * Meter}, UnitVec::Get(e.c * Radian)});
+        break;
+    case 'O':
+        ShiftOrigin(world, Length2{e.a * Meter, e.b * Meter});
+        break;
+    default:
+        break;
+    }
+}
+
 int main(int argc, char** argv) {
     if (argc > 1 && std::string(argv[1]) == "--time") return TimeWorlds(argc, argv);
     if (argc < 5) {
@@ -499,7 +547,10 @@ int main(int argc, char** argv) {
     WriteConf(o, conf);
     o.u32(std::uint32_t(steps + 1));
     WriteState(o, world, 0, StepStats{});
+    const auto edits = ReadEdits();
     for (int s = 1; s <= steps; ++s) {
+        for (const auto& e : edits)
+            if (e.step == s) ApplyEdit(world, e);
         const auto st = Step(world, conf);
         WriteState(o, world, std::uint32_t(s), st);
     }

@@ -233,6 +233,10 @@ _SIGS = {
     "prx_adjoint_joint_gradients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "prx_adjoint_diagnostics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "prx_world_record_contact_events": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "prx_world_set_transformation": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
+    "prx_world_set_location": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
+    "prx_world_set_angle": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float]),
+    "prx_world_shift_origin": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "prx_world_set_contact_listener": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "prx_world_contact_events": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
                                                 ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_uint32)]),
@@ -360,6 +364,25 @@ class World:
     def set_velocity(self, body, v):
         arr = (ctypes.c_float * 3)(*v)
         _check(lib().prx_world_set_velocity(self.h, body, arr))
+
+    def set_transformation(self, body, xf):
+        """SetTransformation (Body.cpp:270-273): xf = (p.x, p.y, q.x, q.y)."""
+        arr = (ctypes.c_float * 4)(*xf)
+        _check(lib().prx_world_set_transformation(self.h, body, arr))
+
+    def set_location(self, body, v):
+        """SetLocation (WorldBody.cpp:158-163)."""
+        arr = (ctypes.c_float * 2)(*v)
+        _check(lib().prx_world_set_location(self.h, body, arr))
+
+    def set_angle(self, body, angle):
+        """SetAngle (WorldBody.cpp:165-170)."""
+        _check(lib().prx_world_set_angle(self.h, body, angle))
+
+    def shift_origin(self, origin):
+        """ShiftOrigin (AabbTreeWorld.cpp:2221-2242)."""
+        arr = (ctypes.c_float * 2)(*origin)
+        _check(lib().prx_world_shift_origin(self.h, arr))
 
     def set_friction(self, shape, value):
         """SetFriction(World&, ShapeID, NonNegative<Real>) (WorldShape.cpp:79-84)."""

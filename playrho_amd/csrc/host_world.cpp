@@ -1120,6 +1120,8 @@ int StepBatch(prx_batch* b, const prx_step_conf* conf, int nsteps) {
     return PRX_OK;
 }
 
+int SetSweepPosition(prx_world* w, std::uint32_t body, float px, float py, float angle);
+
 // Calls the world's listeners with the recorded calls of its last step, in
 // the order the step made them.
 int DispatchEvents(prx_world* w) {
@@ -1610,6 +1612,58 @@ int prx_world_get_transformation(prx_world* w, std::uint32_t body, float out[4])
     return PRX_OK;
 }
 
+int prx_world_set_transformation(prx_world* w, std::uint32_t body, const float xf[4]) {
+    if (int st = CheckWorld(w)) return st;
+    if (!xf) return Fail(PRX_E_INVALID_ARGUMENT, "null transformation");
+    if (body >= w->bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
+    if (int st = Detach(w)) return st;
+    // SetTransformation(Body&, xfm) (Body.cpp:270-273): the angle of q.
+    return SetSweepPosition(w, body, xf[0], xf[1], std::atan2(xf[3], xf[2]));
+}
+
+int prx_world_set_location(prx_world* w, std::uint32_t body, const float v[2]) {
+    if (int st = CheckWorld(w)) return st;
+    if (!v) return Fail(PRX_E_INVALID_ARGUMENT, "null location");
+    if (body >= w->bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
+    if (int st = Detach(w)) return st;
+    // SetLocation(Body&, v) (Body.cpp:275-278): the current rotation, re-read as an angle.
+    const V2 q = UnitFromAngle(w->bodies[body].pos1[2]);
+    return SetSweepPosition(w, body, v[0], v[1], std::atan2(q.y, q.x));
+}
+
+int prx_world_set_angle(prx_world* w, std::uint32_t body, float angle) {
+    if (int st = CheckWorld(w)) return st;
+    if (body >= w->bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
+    if (int st = Detach(w)) return st;
+    // SetAngle(Body&, a) (Body.cpp:285-288): the sweep's linear position kept.
+    const HostBody& b = w->bodies[body];
+    return SetSweepPosition(w, body, b.pos1[0], b.pos1[1], angle);
+}
+
+int prx_world_shift_origin(prx_world* w, const float origin[2]) {
+    if (int st = CheckWorld(w)) return st;
+    if (!origin) return Fail(PRX_E_INVALID_ARGUMENT, "null origin");
+    if (int st = Detach(w)) return st;
+    // ShiftOrigin (AabbTreeWorld.cpp:2221-2242): sweeps, every contact flagged
+    // for updating, and the tree's AABBs (DynamicTree.cpp:650-658).
+    for (HostBody& b : w->bodies) {
+        for (int k = 0; k < 2; ++k) {
+            b.pos0[k] = b.pos0[k] - origin[k];
+            b.pos1[k] = b.pos1[k] - origin[k];
+        }
+    }
+    HostContacts& hc = w->contacts;
+    for (std::size_t i = 0; i < hc.size(); ++i) hc.flags[i] |= prx::kNeedsUpdating;
+    for (HostProxy& p : w->proxies) {
+        if (p.flags & prx::kProxyPending) continue;
+        for (int k = 0; k < 2; ++k) {
+            p.aabb[k] = p.aabb[k] + (-origin[k]);
+            p.aabb[2 + k] = p.aabb[2 + k] + (-origin[k]);
+        }
+    }
+    return PRX_OK;
+}
+
 int prx_world_set_velocity(prx_world* w, std::uint32_t body, const float v[3]) {
     if (int st = CheckWorld(w)) return st;
     if (body >= w->bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
@@ -1634,6 +1688,27 @@ namespace {
 // Body.cpp:163-171: speedable bodies only, under-active time reset), and, if
 // the value changed, the shape's contacts are flagged for updating with
 // their bodies woken.
+// SetBody (AabbTreeWorld.cpp:2723-2795) for a body whose sweep becomes one
+// position (Sweep{Position}, alpha0 0): a new transformation flags the body's
+// contacts for updating and queues the body for the next step's Synchronize.
+int SetSweepPosition(prx_world* w, std::uint32_t body, float px, float py, float angle) {
+    HostBody& b = w->bodies[body];
+    const V2 q0 = UnitFromAngle(b.pos1[2]);
+    const V2 p0 = V2{b.pos1[0], b.pos1[1]} - Rotate(b.lc, q0);
+    b.pos0[0] = b.pos1[0] = px;
+    b.pos0[1] = b.pos1[1] = py;
+    b.pos0[2] = b.pos1[2] = angle;
+    const V2 q1 = UnitFromAngle(angle);
+    const V2 p1 = V2{px, py} - Rotate(b.lc, q1);
+    if (!(p0.x == p1.x && p0.y == p1.y) || !(q0.x == q1.x && q0.y == q1.y)) {
+        HostContacts& hc = w->contacts;
+        for (std::size_t i = 0; i < hc.size(); ++i)
+            if (hc.body[i].x == int(body) || hc.body[i].y == int(body)) hc.flags[i] |= prx::kNeedsUpdating;
+        b.flags |= prx::kNeedsSync;
+    }
+    return PRX_OK;
+}
+
 int SetShapeMaterial(prx_world* w, std::uint32_t shape, bool friction, float value) {
     if (int st = CheckWorld(w)) return st;
     if (shape >= w->shapes.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such shape");

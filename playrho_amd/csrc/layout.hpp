@@ -40,7 +40,10 @@ namespace prx {
 // Body flags (same bits as the state record).
 enum : std::uint32_t {
     kSpeedable = 1u, kAccelerable = 2u, kImpenetrable = 4u, kAwake = 8u, kAutoSleep = 16u,
-    kEnabled = 32u, kFixedRotation = 64u
+    kEnabled = 32u, kFixedRotation = 64u,
+    // Host edits: the body's proxies are synchronised in the next step's
+    // pre-phase (m_bodiesForSync, AabbTreeWorld.cpp:2162-2173).
+    kNeedsSync = 0x200u
 };
 // Contact flags (same bits as the state record).
 enum : std::uint32_t {

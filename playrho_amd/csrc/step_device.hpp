@@ -2781,6 +2781,19 @@ __device__ __forceinline__ void step_world(const DeviceState& d, int w, const St
     for (int o = 32; o > 0; o >>= 1) created += __shfl_xor(created, o);
     stat(prx::kStPreProxiesCreated, created);
     __syncthreads();
+    // Bodies edited since the last step (SetTransformation and its kin):
+    // Synchronize (AabbTreeWorld.cpp:2162-2173).
+    std::uint32_t premoved = 0;
+    for (int b = x.lane; b < x.nb; b += kWave) {
+        const std::uint32_t f = d.bflags[x.bo + b];
+        if (f & prx::kNeedsSync) {
+            premoved += synchronize_body(d, x, b, c);
+            d.bflags[x.bo + b] = f & ~prx::kNeedsSync;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) premoved += __shfl_xor(premoved, o);
+    stat(prx::kStPreProxiesMoved, premoved);
+    __syncthreads();
     T.mark(prx::kPhProxies);
     // B, C, D.
     stat(prx::kStPreDestroyed, phase_destroy(d, x, k));

@@ -38,6 +38,15 @@ CASES = [
     ("chain0", "chain", 240, [1, 0], [1, 120, 240]),
     ("chain1", "chain", 240, [1, 1, 12, 24, 40], [1, 240]),
 ]
+# World edits between steps (step, kind, body, a, b, c): SetLocation 'L',
+# SetAngle 'A', SetTransformation 'T' (c the rotation's angle), ShiftOrigin 'O'.
+EDIT_CASES = [
+    ("pile3_edit", "pile", 120, [1, 3], [1, 120], False,
+     [(20, "L", 5, 1.25, 6.5, 0), (35, "A", 9, 0.7, 0, 0), (50, "T", 12, -1.5, 8.0, 2.5), (70, "O", 0, 3.0, -0.5, 0),
+      (90, "L", 0, 0.25, 0.0, 0)]),
+    ("drop300_edit_toi", "drop", 120, [300, 20, 40, 7], [1, 120], True,
+     [(15, "T", 40, 0.0, 12.0, -1.0), (30, "O", 0, -2.0, 1.5, 0), (45, "A", 77, 3.0, 0, 0), (60, "L", 120, 5.0, 9.0, 0)]),
+]
 # The same scenes with the TOI phase (the reference's default doToi = true).
 TOI_CASES = [
     ("pyramid_toi", "pyramid", 300, [20], [1, 150, 300]),
@@ -51,10 +60,10 @@ TOI_CASES = [
 def main():
     tmp = tempfile.mkdtemp()
     try:
-        for name, scene, steps, params, full, do_toi in ([c + (False,) for c in CASES] +
-                                                         [c + (True,) for c in TOI_CASES]):
+        for name, scene, steps, params, full, do_toi, edits in ([c + (False, []) for c in CASES] +
+                                                                [c + (True, []) for c in TOI_CASES] + EDIT_CASES):
             path = os.path.join(tmp, name + ".bin")
-            buf = oracle_lib.run_ref_driver(scene, steps, do_toi, params, path)
+            buf = oracle_lib.run_ref_driver(scene, steps, do_toi, params, path, edits)
             desc_conf, d = oracle_lib.split_dump(buf)
             states = d["states"]
             dig = np.zeros((len(states), len(golden_cases.PARTS), 32), dtype=np.uint8)
@@ -64,7 +73,9 @@ def main():
             arrays = dict(desc_conf=np.frombuffer(desc_conf, dtype=np.uint8),
                           digests=dig, scene=np.array(scene), params=np.array(params, dtype=np.int32),
                           steps=np.array(steps), full_steps=np.array(full, dtype=np.int32),
-                          do_toi=np.array(do_toi))
+                          do_toi=np.array(do_toi),
+                          edits=np.array([(s, ord(k), b, a, bb, c) for s, k, b, a, bb, c in edits],
+                                         dtype=np.float64).reshape(-1, 6))
             for st in full:
                 s = states[st]
                 for key in ("stats", "bodies", "bflags", "cu", "ct", "cf", "pid", "paabb", "jlimit", "jimp", "events"):

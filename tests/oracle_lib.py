@@ -33,6 +33,8 @@ def lib():
         L.oracle_bodies_f64.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.oracle_free.argtypes = [ctypes.c_void_p]
         L.oracle_sincosf.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        L.oracle_edit.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_float,
+                                  ctypes.c_float, ctypes.c_float]
         L.oracle_time_batch.restype = ctypes.c_double
         L.oracle_time_batch.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int,
@@ -58,6 +60,22 @@ class OracleWorld:
         if getattr(self, "h", None):
             lib().oracle_destroy(self.h)
             self.h = None
+
+    def edit(self, kind, body, a=0.0, b=0.0, c=0.0):
+        """A world edit between steps: 'L' SetLocation, 'A' SetAngle, 'T'
+        SetTransformation (c: the angle of the rotation), 'O' ShiftOrigin."""
+        lib().oracle_edit(self.h, ord(kind), body, a, b, c)
+
+    def step_with_edits(self, steps, edits, first=1):
+        """Steps `steps` steps (numbered from `first`), applying each edit
+        (step, kind, body, a, b, c) before its step; returns the states."""
+        out = []
+        for s in range(first, first + steps):
+            for e in edits:
+                if int(e[0]) == s:
+                    self.edit(chr(int(e[1])), int(e[2]), float(e[3]), float(e[4]), float(e[5]))
+            out += self.step(1)
+        return out
 
     def step(self, n=1, record=True):
         ln = ctypes.c_size_t()
@@ -93,11 +111,15 @@ def split_dump(buf):
     return desc_conf, dumpfmt.read_dump(buf)
 
 
-def run_ref_driver(scene, steps, do_toi, params, path):
+def run_ref_driver(scene, steps, do_toi, params, path, edits=()):
     """Runs the reference build's driver (present only where oracle/_ref was
-    built) and returns the raw dump."""
+    built) and returns the raw dump; edits (step, kind, body, a, b, c) are
+    applied before their steps."""
     cmd = [REF_DRIVER, scene, str(steps), "1" if do_toi else "0", path] + [str(p) for p in params]
-    subprocess.run(cmd, check=True)
+    env = dict(os.environ)
+    if edits:
+        env["PRX_EDITS"] = ";".join("%d %s %d %r %r %r" % (s, k, b, a, bb, c) for s, k, b, a, bb, c in edits)
+    subprocess.run(cmd, check=True, env=env)
     with open(path, "rb") as f:
         return f.read()
 

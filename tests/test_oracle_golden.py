@@ -12,7 +12,7 @@ from tests import dumpfmt, golden_cases, oracle_lib
 def test_oracle_matches_reference_every_step(name):
     g = golden_cases.load(name)
     w = oracle_lib.OracleWorld(g["desc_conf"])
-    states = w.step(g["steps"])
+    states = w.step_with_edits(g["steps"], g["edits"]) if len(g["edits"]) else w.step(g["steps"])
     bad = golden_cases.first_mismatch(g["digests"], states, dumpfmt)
     assert bad is None, "%s: first differing (step, part) = %s" % (name, bad)
     for st, full in g["full"].items():
