@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PRX_ABI_VERSION 3
+#define PRX_ABI_VERSION 4
 
 /* Status codes (reference exception types in parentheses). */
 enum {
@@ -97,6 +97,27 @@ typedef struct prx_revolute_joint_conf {
     uint32_t collide_connected;
 } prx_revolute_joint_conf;
 
+/* DistanceJointConf (d2/DistanceJointConf.hpp:62-130): rest length, and a
+ * soft (spring) constraint when frequency > 0.  The reference's defaults are
+ * length 1, frequency 0, damping ratio 0; a negative or NaN frequency is
+ * PRX_E_INVALID_ARGUMENT (NonNegative<Frequency> there). */
+typedef struct prx_distance_joint_conf {
+    uint32_t body_a, body_b;
+    float local_anchor_a[2], local_anchor_b[2];
+    float length, frequency, damping_ratio;
+    uint32_t collide_connected;
+} prx_distance_joint_conf;
+
+/* WeldJointConf (d2/WeldJointConf.hpp:62-130): rigid, or with a soft angular
+ * part when frequency > 0.  The reference's defaults are reference angle 0,
+ * frequency 0, damping ratio 0; frequency as for the distance joint. */
+typedef struct prx_weld_joint_conf {
+    uint32_t body_a, body_b;
+    float local_anchor_a[2], local_anchor_b[2];
+    float reference_angle, frequency, damping_ratio;
+    uint32_t collide_connected;
+} prx_weld_joint_conf;
+
 /* StepConf fields the regular step reads (StepConf.hpp:49-392). */
 typedef struct prx_step_conf {
     float delta_time, dt_ratio, min_still_time_to_sleep, linear_slop, angular_slop;
@@ -169,6 +190,12 @@ int prx_body_create(prx_world* world, const prx_body_conf* conf, uint32_t* body_
  * world with contacts would need contact re-filtering, which is not
  * modelled): PRX_E_UNSUPPORTED otherwise. */
 int prx_joint_create(prx_world* world, const prx_revolute_joint_conf* conf, uint32_t* joint_id);
+/* CreateJoint(World&, const DistanceJointConf&) and (…, const WeldJointConf&)
+ * (World.hpp:448-458; solvers DistanceJointConf.cpp:81-245,
+ * WeldJointConf.cpp:115-375), with the same restriction as prx_joint_create.
+ * Joint ids are shared by all kinds, in creation order. */
+int prx_distance_joint_create(prx_world* world, const prx_distance_joint_conf* conf, uint32_t* joint_id);
+int prx_weld_joint_create(prx_world* world, const prx_weld_joint_conf* conf, uint32_t* joint_id);
 uint32_t prx_world_joint_count(const prx_world* world);
 /* Builds a named workload scene through the calls above. */
 int prx_scene_build(prx_world* world, const char* name, const int32_t* params, int nparams);

@@ -115,6 +115,16 @@ void Load(Handle<R>& h, In& in) {
         j.upper = f[6];
         j.motorSpeed = f[7];
         j.maxMotorTorque = f[8];
+        j.kind = (fl >> 8) & 0xffu;
+        if (j.kind != 0) {
+            // Distance: length, frequency, damping ratio.  Weld: reference
+            // angle, frequency, damping ratio.
+            j.enableLimit = j.enableMotor = false;
+            if (j.kind == 1) j.length = f[4], j.referenceAngle = 0;
+            j.frequency = f[5];
+            j.dampingRatio = f[6];
+            j.lower = j.upper = j.motorSpeed = j.maxMotorTorque = 0;
+        }
         h.world.AddJoint(j);
     }
     const std::uint32_t nf = in.u32();

@@ -139,6 +139,24 @@ template <class R> inline V2<R> UnitVector(V2<R> v, V2<R> fallback) {
     return {v.x / mag, v.y / mag};
 }
 
+// UnitVec::Get(x, y) with the default (zero) fallback, returning the
+// magnitude as well (UnitVec.hpp:175-236): zero magnitude when no direction.
+template <class R> inline V2<R> UnitPolar(V2<R> v, R& mag) {
+    const R m2 = v.x * v.x + v.y * v.y;
+    if (std::isnormal(m2)) {
+        mag = std::sqrt(m2);
+        const R inv = R(1) / mag;
+        return {v.x * inv, v.y * inv};
+    }
+    const R h = std::hypot(v.x, v.y);
+    if (!std::isnormal(h)) {
+        mag = R(0);
+        return {R(0), R(0)};
+    }
+    mag = h;
+    return {v.x / h, v.y / h};
+}
+
 // std::max / std::min / std::clamp comparison forms.
 template <class R> inline R Max(R a, R b) { return (a < b) ? b : a; }
 template <class R> inline R Min(R a, R b) { return (b < a) ? b : a; }
@@ -762,6 +780,12 @@ struct Joint {
     V2<R> rA, rB;
     M33<R> mass;
     R angularMass = 0;
+    // kind: 0 revolute, 1 distance, 2 weld.  Distance keeps its impulse in
+    // impulse.x; length is the distance joint's rest length.
+    std::uint32_t kind = 0;
+    R length = 0, frequency = 0, dampingRatio = 0;
+    V2<R> u;
+    R dmass = 0, invGamma = 0, bias = 0, gamma = 0;
 };
 
 template <class R>
@@ -1475,8 +1499,261 @@ public:
         return M33<R>{V3<R>{exx, eyx, ezx}, V3<R>{eyx, eyy, ezy}, V3<R>{ezx, ezy, totInvI}};
     }
 
+    // GetInverse22(Mat33) (Math.hpp:478-490).
+    static M33<R> Inverse22(const M33<R>& k) {
+        const R a = k.ex.x, b = k.ey.x, c = k.ex.y, d = k.ey.y;
+        R det = (a * d) - (b * c);
+        if (det != R(0)) det = R(1) / det;
+        return M33<R>{V3<R>{det * d, -det * c, R(0)}, V3<R>{-det * b, det * a, R(0)}, V3<R>{R(0), R(0), R(0)}};
+    }
+    // GetSymInverse33(Mat33) (Math.hpp:493-510).
+    static M33<R> SymInverse33(const M33<R>& k) {
+        const R invDet = Dot3(k.ex, Cross3(k.ey, k.ez));
+        const R det = (invDet != R(0)) ? (R(1) / invDet) : invDet;
+        const R a11 = k.ex.x, a12 = k.ey.x, a13 = k.ez.x, a22 = k.ey.y, a23 = k.ez.y, a33 = k.ez.z;
+        const R exy = det * (a13 * a23 - a12 * a33);
+        const R eyz = det * (a13 * a12 - a11 * a23);
+        const R exz = det * (a12 * a23 - a13 * a22);
+        return M33<R>{V3<R>{det * (a22 * a33 - a23 * a23), exy, exz}, V3<R>{exy, det * (a11 * a33 - a13 * a13), eyz},
+                      V3<R>{exz, eyz, det * (a11 * a22 - a12 * a12)}};
+    }
+    // Matrix times vector, row by row from zero (Vector.hpp:455-470).
+    static V3<R> MulRows(const M33<R>& m, V3<R> v) {
+        auto row = [&](V3<R> r) {
+            R e = 0;
+            e += r.x * v.x;
+            e += r.y * v.y;
+            e += r.z * v.z;
+            return e;
+        };
+        return V3<R>{row(m.ex), row(m.ey), row(m.ez)};
+    }
+    static void ApplyImpulse(BodyConstraint<R>& A, BodyConstraint<R>& B, V2<R> P, R LA, R LB, Vel<R>& velA,
+                             Vel<R>& velB) {
+        velA.lin = velA.lin - A.invMass * P;
+        velA.ang = velA.ang - A.invRotI * LA;
+        velB.lin = velB.lin + B.invMass * P;
+        velB.ang = velB.ang + B.invRotI * LB;
+    }
+
+    // InitVelocity(DistanceJointConf&) (DistanceJointConf.cpp:81-162).
+    static void DistanceInitVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        const R mA = A.invMass, iA = A.invRotI, mB = B.invMass, iB = B.invRotI;
+        Vel<R> velA = A.vel, velB = B.vel;
+        j.rA = Rotate(j.laA - A.lc, UnitFromAngle(A.pos.ang));
+        j.rB = Rotate(j.laB - B.lc, UnitFromAngle(B.pos.ang));
+        R length;
+        j.u = UnitPolar((B.pos.lin + j.rB) - (A.pos.lin + j.rA), length);
+        const R crAu = Cross(j.rA, j.u) / R(1);
+        const R crBu = Cross(j.rB, j.u) / R(1);
+        R invMass = mA + iA * (crAu * crAu) + mB + iB * (crBu * crBu);
+        j.dmass = (invMass != R(0)) ? R(1) / invMass : R(0);
+        if (j.frequency > R(0)) {
+            const R C = length - j.length;
+            const R omega = R(2) * kPi * j.frequency;
+            const R d = R(2) * j.dmass * j.dampingRatio * omega;
+            const R k = j.dmass * (omega * omega);
+            const R h = conf.dt;
+            const R gamma = h * (d + h * k);
+            j.invGamma = (gamma != R(0)) ? R(1) / gamma : R(0);
+            j.bias = C * h * k * j.invGamma;
+            invMass += j.invGamma;
+            j.dmass = (invMass != R(0)) ? R(1) / invMass : R(0);
+        } else {
+            j.invGamma = 0;
+            j.bias = 0;
+        }
+        if (conf.warmStart) {
+            j.impulse.x *= conf.dtRatio;
+            const V2<R> P = j.impulse.x * j.u;
+            ApplyImpulse(A, B, P, Cross(j.rA, P) / R(1), Cross(j.rB, P) / R(1), velA, velB);
+        } else {
+            j.impulse.x = 0;
+        }
+        A.vel = velA;
+        B.vel = velB;
+    }
+
+    // SolveVelocity(DistanceJointConf&) (DistanceJointConf.cpp:164-194).
+    static bool DistanceSolveVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        Vel<R> velA = A.vel, velB = B.vel;
+        const V2<R> vpA = velA.lin + RevPerp(j.rA) * (velA.ang / R(1));
+        const V2<R> vpB = velB.lin + RevPerp(j.rB) * (velB.ang / R(1));
+        const R Cdot = Dot(j.u, vpB - vpA);
+        const R impulse = -j.dmass * (Cdot + j.bias + j.invGamma * j.impulse.x);
+        j.impulse.x += impulse;
+        const V2<R> P = impulse * j.u;
+        ApplyImpulse(A, B, P, Cross(j.rA, P) / R(1), Cross(j.rB, P) / R(1), velA, velB);
+        A.vel = velA;
+        B.vel = velB;
+        return impulse == R(0);
+    }
+
+    // SolvePosition(const DistanceJointConf&) (DistanceJointConf.cpp:196-245).
+    static bool DistanceSolvePosition(const Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        if (j.frequency > R(0)) return true;
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        Pos<R> posA = A.pos, posB = B.pos;
+        const V2<R> rA = Rotate(j.laA - A.lc, UnitFromAngle(posA.ang));
+        const V2<R> rB = Rotate(j.laB - B.lc, UnitFromAngle(posB.ang));
+        R length;
+        const V2<R> u = UnitPolar((posB.lin + rB) - (posA.lin + rA), length);
+        const R C = Clamp(length - j.length, -conf.maxLinearCorrection, conf.maxLinearCorrection);
+        const R impulse = -j.dmass * C;
+        const V2<R> P = impulse * u;
+        posA.lin = posA.lin - A.invMass * P;
+        posA.ang = posA.ang - A.invRotI * (Cross(rA, P) / R(1));
+        posB.lin = posB.lin + B.invMass * P;
+        posB.ang = posB.ang + B.invRotI * (Cross(rB, P) / R(1));
+        A.pos = posA;
+        B.pos = posB;
+        return std::abs(C) < conf.linearSlop;
+    }
+
+    // InitVelocity(WeldJointConf&) (WeldJointConf.cpp:115-213).
+    static void WeldInitVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        const R mA = A.invMass, iA = A.invRotI, mB = B.invMass, iB = B.invRotI;
+        const R aA = A.pos.ang, aB = B.pos.ang;
+        Vel<R> velA = A.vel, velB = B.vel;
+        j.rA = Rotate(j.laA - A.lc, UnitFromAngle(aA));
+        j.rB = Rotate(j.laB - B.lc, UnitFromAngle(aB));
+        const M33<R> K = JointMass(mA, j.rA, iA, mB, j.rB, iB);
+        if (j.frequency > R(0)) {
+            j.mass = Inverse22(K);
+            R invRotI = iA + iB;
+            const R rotI = (invRotI > R(0)) ? R(1) / invRotI : R(0);
+            const R C = aB - aA - j.referenceAngle;
+            const R omega = R(2) * kPi * j.frequency;
+            const R d = R(2) * rotI * j.dampingRatio * omega;
+            const R k = rotI * omega * omega;
+            const R h = conf.dt;
+            const R invGamma = h * (d + h * k);
+            j.gamma = (invGamma != R(0)) ? R(1) / invGamma : R(0);
+            j.bias = C * h * k * j.gamma;
+            invRotI += j.gamma;
+            j.mass.ez.z = (invRotI != R(0)) ? R(1) / invRotI : R(0);
+        } else if (K.ez.z == R(0)) {
+            j.mass = Inverse22(K);
+            j.gamma = 0;
+            j.bias = 0;
+        } else {
+            j.mass = SymInverse33(K);
+            j.gamma = 0;
+            j.bias = 0;
+        }
+        if (conf.warmStart) {
+            j.impulse.x *= conf.dtRatio;
+            j.impulse.y *= conf.dtRatio;
+            j.impulse.z *= conf.dtRatio;
+            const V2<R> P{j.impulse.x, j.impulse.y};
+            const R L = j.impulse.z;
+            ApplyImpulse(A, B, P, L + Cross(j.rA, P) / R(1), L + Cross(j.rB, P) / R(1), velA, velB);
+        } else {
+            j.impulse = V3<R>{};
+        }
+        A.vel = velA;
+        B.vel = velB;
+    }
+
+    // SolveVelocity(WeldJointConf&) (WeldJointConf.cpp:215-303).
+    static bool WeldSolveVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        const Vel<R> oldVelA = A.vel, oldVelB = B.vel;
+        Vel<R> velA = oldVelA, velB = oldVelB;
+        const R iA = A.invRotI, iB = B.invRotI;
+        if (j.frequency > R(0)) {
+            const R Cdot2 = velB.ang - velA.ang;
+            const R g = j.gamma * j.impulse.z;
+            const R impulse2 = -j.mass.ez.z * (Cdot2 + j.bias + g);
+            j.impulse.z += impulse2;
+            velA.ang = velA.ang - iA * impulse2;
+            velB.ang = velB.ang + iB * impulse2;
+            const V2<R> vb = velB.lin + RevPerp(j.rB) * (velB.ang / R(1));
+            const V2<R> va = velA.lin + RevPerp(j.rA) * (velA.ang / R(1));
+            const V2<R> Cdot1 = vb - va;
+            // Transform(Vec2, Mat33) (Math.hpp:522-527).
+            const V2<R> t{j.mass.ex.x * Cdot1.x + j.mass.ey.x * Cdot1.y, j.mass.ex.y * Cdot1.x + j.mass.ey.y * Cdot1.y};
+            const V2<R> impulse1 = -t;
+            j.impulse.x += impulse1.x;
+            j.impulse.y += impulse1.y;
+            ApplyImpulse(A, B, impulse1, Cross(j.rA, impulse1) / R(1), Cross(j.rB, impulse1) / R(1), velA, velB);
+        } else {
+            const V2<R> vb = velB.lin + RevPerp(j.rB) * (velB.ang / R(1));
+            const V2<R> va = velA.lin + RevPerp(j.rA) * (velA.ang / R(1));
+            const V2<R> Cdot1 = vb - va;
+            const R Cdot2 = (velB.ang - velA.ang) / R(1);
+            const V3<R> t = MulRows(j.mass, V3<R>{Cdot1.x, Cdot1.y, Cdot2});
+            const V3<R> impulse{-t.x, -t.y, -t.z};
+            j.impulse.x += impulse.x;
+            j.impulse.y += impulse.y;
+            j.impulse.z += impulse.z;
+            const V2<R> P{impulse.x, impulse.y};
+            const R L = impulse.z;
+            ApplyImpulse(A, B, P, L + Cross(j.rA, P) / R(1), L + Cross(j.rB, P) / R(1), velA, velB);
+        }
+        const bool changed = !(velA.lin == oldVelA.lin) || velA.ang != oldVelA.ang || !(velB.lin == oldVelB.lin) ||
+                             velB.ang != oldVelB.ang;
+        if (changed) {
+            A.vel = velA;
+            B.vel = velB;
+            return false;
+        }
+        return true;
+    }
+
+    // SolvePosition(const WeldJointConf&) (WeldJointConf.cpp:305-375).
+    static bool WeldSolvePosition(const Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        Pos<R> posA = A.pos, posB = B.pos;
+        const R mA = A.invMass, iA = A.invRotI, mB = B.invMass, iB = B.invRotI;
+        const V2<R> rA = Rotate(j.laA - A.lc, UnitFromAngle(posA.ang));
+        const V2<R> rB = Rotate(j.laB - B.lc, UnitFromAngle(posB.ang));
+        const M33<R> K = JointMass(mA, rA, iA, mB, rB, iB);
+        const V2<R> C1 = (posB.lin + rB) - (posA.lin + rA);
+        const R positionError = std::sqrt(MagSq(C1));
+        R angularError = 0;
+        V2<R> P;
+        R L = 0;
+        if (j.frequency > R(0)) {
+            P = -Solve22(K, C1);
+        } else {
+            const R C2 = posB.ang - posA.ang - j.referenceAngle;
+            angularError = std::abs(C2);
+            V3<R> impulse;
+            if (K.ez.z > R(0)) {
+                const V3<R> s = Solve33(K, V3<R>{C1.x, C1.y, C2});
+                impulse = V3<R>{-s.x, -s.y, -s.z};
+            } else {
+                const V2<R> s = Solve22(K, C1);
+                impulse = V3<R>{-s.x, -s.y, R(0)};
+            }
+            P = V2<R>{impulse.x, impulse.y};
+            L = impulse.z;
+        }
+        const R LA = (j.frequency > R(0)) ? Cross(rA, P) / R(1) : L + Cross(rA, P) / R(1);
+        const R LB = (j.frequency > R(0)) ? Cross(rB, P) / R(1) : L + Cross(rB, P) / R(1);
+        posA.lin = posA.lin - mA * P;
+        posA.ang = posA.ang - iA * LA;
+        posB.lin = posB.lin + mB * P;
+        posB.ang = posB.ang + iB * LB;
+        A.pos = posA;
+        B.pos = posB;
+        return (positionError <= conf.linearSlop) && (angularError <= conf.angularSlop);
+    }
+
     // InitVelocity(RevoluteJointConf&) (RevoluteJointConf.cpp:127-226).
     static void JointInitVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        if (j.kind == 1) return DistanceInitVelocity(j, bcs, conf);
+        if (j.kind == 2) return WeldInitVelocity(j, bcs, conf);
         BodyConstraint<R>& A = bcs[j.bodyA];
         BodyConstraint<R>& B = bcs[j.bodyB];
         const R mA = A.invMass, iA = A.invRotI, aA = A.pos.ang;
@@ -1535,6 +1812,8 @@ public:
 
     // SolveVelocity(RevoluteJointConf&) (RevoluteJointConf.cpp:228-331).
     static bool JointSolveVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        if (j.kind == 1) return DistanceSolveVelocity(j, bcs);
+        if (j.kind == 2) return WeldSolveVelocity(j, bcs);
         BodyConstraint<R>& A = bcs[j.bodyA];
         BodyConstraint<R>& B = bcs[j.bodyB];
         const Vel<R> oldVelA = A.vel, oldVelB = B.vel;
@@ -1606,6 +1885,8 @@ public:
 
     // SolvePosition(const RevoluteJointConf&) (RevoluteJointConf.cpp:333-433).
     static bool JointSolvePosition(const Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        if (j.kind == 1) return DistanceSolvePosition(j, bcs, conf);
+        if (j.kind == 2) return WeldSolvePosition(j, bcs, conf);
         BodyConstraint<R>& A = bcs[j.bodyA];
         BodyConstraint<R>& B = bcs[j.bodyB];
         Pos<R> posA = A.pos, posB = B.pos;

@@ -36,6 +36,8 @@
 #include <playrho/d2/DiskShapeConf.hpp>
 #include <playrho/d2/EdgeShapeConf.hpp>
 #include <playrho/d2/RevoluteJointConf.hpp>
+#include <playrho/d2/DistanceJointConf.hpp>
+#include <playrho/d2/WeldJointConf.hpp>
 #include <playrho/d2/WorldJoint.hpp>
 #include <playrho/StepConf.hpp>
 
@@ -162,15 +164,34 @@ void WriteDescription(Out& o, const World& world) {
     }
     const auto njoints = GetJointRange(world);
     o.u32(njoints);
+    // Joint records: flags bits 0-2 (limit, motor, collideConnected), bits 8-15
+    // the kind (0 revolute, 1 distance, 2 weld), then 9 floats.
     for (auto i = 0u; i < njoints; ++i) {
-        const auto j = TypeCast<RevoluteJointConf>(GetJoint(world, JointID(i)));
-        o.u32(to_underlying(GetBodyA(j)));
-        o.u32(to_underlying(GetBodyB(j)));
-        o.u32((j.enableLimit ? 1u : 0u) | (j.enableMotor ? 2u : 0u) | (j.collideConnected ? 4u : 0u));
-        const float f[9] = {GetX(j.localAnchorA), GetY(j.localAnchorA), GetX(j.localAnchorB),
-                            GetY(j.localAnchorB), j.referenceAngle, j.lowerAngle, j.upperAngle,
-                            j.motorSpeed, j.maxMotorTorque};
-        o.put(f, sizeof(f));
+        const auto joint = GetJoint(world, JointID(i));
+        if (const auto* j = TypeCast<const RevoluteJointConf>(&joint)) {
+            o.u32(to_underlying(GetBodyA(*j)));
+            o.u32(to_underlying(GetBodyB(*j)));
+            o.u32((j->enableLimit ? 1u : 0u) | (j->enableMotor ? 2u : 0u) | (j->collideConnected ? 4u : 0u));
+            const float f[9] = {GetX(j->localAnchorA), GetY(j->localAnchorA), GetX(j->localAnchorB),
+                                GetY(j->localAnchorB), j->referenceAngle, j->lowerAngle, j->upperAngle,
+                                j->motorSpeed, j->maxMotorTorque};
+            o.put(f, sizeof(f));
+        } else if (const auto* j = TypeCast<const DistanceJointConf>(&joint)) {
+            o.u32(to_underlying(GetBodyA(*j)));
+            o.u32(to_underlying(GetBodyB(*j)));
+            o.u32((j->collideConnected ? 4u : 0u) | (1u << 8));
+            const float f[9] = {GetX(j->localAnchorA), GetY(j->localAnchorA), GetX(j->localAnchorB),
+                                GetY(j->localAnchorB), j->length, j->frequency, j->dampingRatio, 0.0f, 0.0f};
+            o.put(f, sizeof(f));
+        } else if (const auto* j = TypeCast<const WeldJointConf>(&joint)) {
+            o.u32(to_underlying(GetBodyA(*j)));
+            o.u32(to_underlying(GetBodyB(*j)));
+            o.u32((j->collideConnected ? 4u : 0u) | (2u << 8));
+            const float f[9] = {GetX(j->localAnchorA), GetY(j->localAnchorA), GetX(j->localAnchorB),
+                                GetY(j->localAnchorB), j->referenceAngle, j->frequency, j->dampingRatio,
+                                0.0f, 0.0f};
+            o.put(f, sizeof(f));
+        }
     }
 }
 
@@ -350,10 +371,20 @@ void WriteState(Out& o, const World& world, std::uint32_t step, const StepStats&
     const auto njoints = GetJointRange(world);
     o.u32(njoints);
     for (auto i = 0u; i < njoints; ++i) {
-        const auto j = TypeCast<RevoluteJointConf>(GetJoint(world, JointID(i)));
-        o.u32(std::uint32_t(j.limitState));
-        const float f[4] = {GetX(j.impulse), GetY(j.impulse), GetZ(j.impulse), j.angularMotorImpulse};
-        o.put(f, sizeof(f));
+        const auto joint = GetJoint(world, JointID(i));
+        if (const auto* j = TypeCast<const RevoluteJointConf>(&joint)) {
+            o.u32(std::uint32_t(j->limitState));
+            const float f[4] = {GetX(j->impulse), GetY(j->impulse), GetZ(j->impulse), j->angularMotorImpulse};
+            o.put(f, sizeof(f));
+        } else if (const auto* j = TypeCast<const DistanceJointConf>(&joint)) {
+            o.u32(0u);
+            const float f[4] = {j->impulse, 0.0f, 0.0f, 0.0f};
+            o.put(f, sizeof(f));
+        } else if (const auto* j = TypeCast<const WeldJointConf>(&joint)) {
+            o.u32(0u);
+            const float f[4] = {GetX(j->impulse), GetY(j->impulse), GetZ(j->impulse), 0.0f};
+            o.put(f, sizeof(f));
+        }
     }
     o.u32(g_events.n);
     o.put(g_events.o.buf.data(), g_events.o.buf.size());
@@ -387,13 +418,25 @@ World BuildWorld(const prx_scene::SceneSpec& spec) {
         CreateBody(world, conf);
     }
     for (const auto& js : spec.joints) {
-        auto jc = RevoluteJointConf{BodyID(static_cast<BodyID::underlying_type>(js.a)),
-                                    BodyID(static_cast<BodyID::underlying_type>(js.b)),
-                                    Length2{js.lax, js.lay}, Length2{js.lbx, js.lby}, js.referenceAngle};
-        jc.UseEnableLimit(js.enableLimit).UseLowerAngle(js.lower).UseUpperAngle(js.upper);
-        jc.UseEnableMotor(js.enableMotor).UseMotorSpeed(js.motorSpeed).UseMaxMotorTorque(js.maxMotorTorque);
-        jc.UseCollideConnected(js.collideConnected);
-        CreateJoint(world, jc);
+        const auto ba = BodyID(static_cast<BodyID::underlying_type>(js.a));
+        const auto bb = BodyID(static_cast<BodyID::underlying_type>(js.b));
+        if (js.kind == prx_scene::kDistance) {
+            auto jc = DistanceJointConf{ba, bb, Length2{js.lax, js.lay}, Length2{js.lbx, js.lby}, js.length};
+            jc.UseFrequency(js.frequency).UseDampingRatio(js.dampingRatio);
+            jc.UseCollideConnected(js.collideConnected);
+            CreateJoint(world, jc);
+        } else if (js.kind == prx_scene::kWeld) {
+            auto jc = WeldJointConf{ba, bb, Length2{js.lax, js.lay}, Length2{js.lbx, js.lby}, js.referenceAngle};
+            jc.UseFrequency(js.frequency).UseDampingRatio(js.dampingRatio);
+            jc.UseCollideConnected(js.collideConnected);
+            CreateJoint(world, jc);
+        } else {
+            auto jc = RevoluteJointConf{ba, bb, Length2{js.lax, js.lay}, Length2{js.lbx, js.lby}, js.referenceAngle};
+            jc.UseEnableLimit(js.enableLimit).UseLowerAngle(js.lower).UseUpperAngle(js.upper);
+            jc.UseEnableMotor(js.enableMotor).UseMotorSpeed(js.motorSpeed).UseMaxMotorTorque(js.maxMotorTorque);
+            jc.UseCollideConnected(js.collideConnected);
+            CreateJoint(world, jc);
+        }
     }
     return world;
 }

@@ -154,6 +154,27 @@ class RevoluteJointConf(ctypes.Structure):
                 ("max_motor_torque", ctypes.c_float), ("collide_connected", ctypes.c_uint32)]
 
 
+class DistanceJointConf(ctypes.Structure):
+    """DistanceJointConf (d2/DistanceJointConf.hpp:62-130); the reference's
+    defaults: length 1, frequency 0 (rigid), damping ratio 0."""
+    _fields_ = [("body_a", ctypes.c_uint32), ("body_b", ctypes.c_uint32),
+                ("local_anchor_a", ctypes.c_float * 2), ("local_anchor_b", ctypes.c_float * 2),
+                ("length", ctypes.c_float), ("frequency", ctypes.c_float), ("damping_ratio", ctypes.c_float),
+                ("collide_connected", ctypes.c_uint32)]
+
+    def __init__(self, **kw):
+        kw.setdefault("length", 1.0)
+        super().__init__(**kw)
+
+
+class WeldJointConf(ctypes.Structure):
+    """WeldJointConf (d2/WeldJointConf.hpp:62-130); frequency 0 is rigid."""
+    _fields_ = [("body_a", ctypes.c_uint32), ("body_b", ctypes.c_uint32),
+                ("local_anchor_a", ctypes.c_float * 2), ("local_anchor_b", ctypes.c_float * 2),
+                ("reference_angle", ctypes.c_float), ("frequency", ctypes.c_float), ("damping_ratio", ctypes.c_float),
+                ("collide_connected", ctypes.c_uint32)]
+
+
 class StepStats(ctypes.Structure):
     """StepStats (StepStats.hpp:30-201): PreStepStats, RegStepStats and ToiStepStats."""
     _fields_ = [(n, ctypes.c_uint32) for n in (
@@ -197,6 +218,8 @@ _SIGS = {
                                            ctypes.POINTER(ctypes.c_size_t)]),
     "prx_world_body_count": (ctypes.c_uint32, [ctypes.c_void_p]),
     "prx_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
+    "prx_distance_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
+    "prx_weld_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
     "prx_world_joint_count": (ctypes.c_uint32, [ctypes.c_void_p]),
     "prx_world_shape_count": (ctypes.c_uint32, [ctypes.c_void_p]),
     "prx_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
@@ -330,9 +353,15 @@ class World:
         _check(lib().prx_body_create(self.h, ctypes.byref(conf), ctypes.byref(bid)))
         return bid.value
 
-    def create_joint(self, conf: RevoluteJointConf):
+    def create_joint(self, conf):
+        """CreateJoint with a RevoluteJointConf, DistanceJointConf or
+        WeldJointConf; returns the joint id (shared by all kinds)."""
+        fn = {RevoluteJointConf: lib().prx_joint_create, DistanceJointConf: lib().prx_distance_joint_create,
+              WeldJointConf: lib().prx_weld_joint_create}.get(type(conf))
+        if fn is None:
+            raise TypeError("not a joint conf: %r" % type(conf))
         jid = ctypes.c_uint32()
-        _check(lib().prx_joint_create(self.h, ctypes.byref(conf), ctypes.byref(jid)))
+        _check(fn(self.h, ctypes.byref(conf), ctypes.byref(jid)))
         return jid.value
 
     @property

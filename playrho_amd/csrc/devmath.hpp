@@ -112,6 +112,24 @@ __device__ __forceinline__ V2 unit_vector(V2 v, V2 fallback) {
     return V2{v.x / mag, v.y / mag};
 }
 
+// UnitVec::Get(x, y) with the zero fallback, also giving the magnitude (zero
+// when no direction is found) (UnitVec.hpp:175-236).
+__device__ __forceinline__ V2 unit_polar(V2 v, float& mag) {
+    const float m2 = v.x * v.x + v.y * v.y;
+    if (__builtin_isnormal(m2)) {
+        mag = __builtin_sqrtf(m2);
+        const float inv = 1.0f / mag;
+        return V2{v.x * inv, v.y * inv};
+    }
+    const float h = hypot_f(v.x, v.y);
+    if (!__builtin_isnormal(h)) {
+        mag = 0.0f;
+        return V2{0.0f, 0.0f};
+    }
+    mag = h;
+    return V2{v.x / h, v.y / h};
+}
+
 __device__ __forceinline__ float fmax_std(float a, float b) { return (a < b) ? b : a; }
 __device__ __forceinline__ float fmin_std(float a, float b) { return (b < a) ? b : a; }
 __device__ __forceinline__ float clamp_std(float v, float lo, float hi) {

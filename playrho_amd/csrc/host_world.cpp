@@ -255,9 +255,14 @@ struct HostProxy {
 };
 
 struct HostJoint {
-    std::uint32_t a = 0, b = 0, flags = 0;  // prx::kJointLimit / kJointMotor / kJointCollideConnected
+    // prx::kJointLimit / kJointMotor / kJointCollideConnected, the kind
+    // (prx::kJointRevolute / kJointDistance / kJointWeld) in bits 8-15.
+    std::uint32_t a = 0, b = 0, flags = 0;
     float la[4] = {0, 0, 0, 0};             // local anchor A x, y, B x, y
-    float cfg[4] = {0, 0, 0, 0};            // reference angle, lower, upper, motor speed
+    // Revolute: reference angle, lower, upper, motor speed.  Distance: length,
+    // frequency, damping ratio, 0.  Weld: reference angle, frequency, damping
+    // ratio, 0.
+    float cfg[4] = {0, 0, 0, 0};
     float maxTorque = 0;
     float imp[4] = {0, 0, 0, 0};            // impulse x, y, z, angular motor impulse
     std::uint32_t limit = 0;
@@ -607,16 +612,22 @@ prx::StepParams ToParams(const prx_step_conf& c) {
     return p;
 }
 
-int AddJoint(prx_world* w, const prx_revolute_joint_conf& c, std::uint32_t* out) {
-    if (c.body_a >= w->bodies.size() || c.body_b >= w->bodies.size())
-        return Fail(PRX_E_OUT_OF_RANGE, "no such body");
-    if (!c.collide_connected) {
+// CreateJoint (AabbTreeWorld.cpp:1132-1175) for any joint kind.
+int AddJointRecord(prx_world* w, const HostJoint& j, std::uint32_t* out) {
+    if (j.a >= w->bodies.size() || j.b >= w->bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
+    if (!(j.flags & prx::kJointCollideConnected)) {
         for (std::size_t i = 0; i < w->contacts.size(); ++i) {
             const int2 bd = w->contacts.body[i];
-            if ((bd.x == int(c.body_a) && bd.y == int(c.body_b)) || (bd.x == int(c.body_b) && bd.y == int(c.body_a)))
+            if ((bd.x == int(j.a) && bd.y == int(j.b)) || (bd.x == int(j.b) && bd.y == int(j.a)))
                 return Fail(PRX_E_UNSUPPORTED, "joint between bodies that already have a contact");
         }
     }
+    if (out) *out = std::uint32_t(w->joints.size());
+    w->joints.push_back(j);
+    return PRX_OK;
+}
+
+int AddJoint(prx_world* w, const prx_revolute_joint_conf& c, std::uint32_t* out) {
     HostJoint j;
     j.a = c.body_a;
     j.b = c.body_b;
@@ -631,9 +642,27 @@ int AddJoint(prx_world* w, const prx_revolute_joint_conf& c, std::uint32_t* out)
     j.cfg[2] = c.upper_angle;
     j.cfg[3] = c.motor_speed;
     j.maxTorque = c.max_motor_torque;
-    if (out) *out = std::uint32_t(w->joints.size());
-    w->joints.push_back(j);
-    return PRX_OK;
+    return AddJointRecord(w, j, out);
+}
+
+// A distance or weld joint: kind, anchors, then (length | reference angle,
+// frequency, damping ratio).
+int AddSpringJoint(prx_world* w, std::uint32_t kind, std::uint32_t a, std::uint32_t b, const float la[2],
+                   const float lb[2], float first, float frequency, float dampingRatio, bool collideConnected,
+                   std::uint32_t* out) {
+    if (!(frequency >= 0.0f)) return Fail(PRX_E_INVALID_ARGUMENT, "frequency must be non-negative");
+    HostJoint j;
+    j.a = a;
+    j.b = b;
+    j.flags = (collideConnected ? prx::kJointCollideConnected : 0u) | (kind << 8);
+    j.la[0] = la[0];
+    j.la[1] = la[1];
+    j.la[2] = lb[0];
+    j.la[3] = lb[1];
+    j.cfg[0] = first;
+    j.cfg[1] = frequency;
+    j.cfg[2] = dampingRatio;
+    return AddJointRecord(w, j, out);
 }
 
 int ContactCap(const prx_world* w) {
@@ -1342,6 +1371,15 @@ int prx_scene_build(prx_world* w, const char* name, const std::int32_t* params, 
         if (int st = AddBody(w, c, nullptr)) return st;
     }
     for (const auto& js : spec.joints) {
+        if (js.kind != prx_scene::kRevolute) {
+            const float la[2] = {js.lax, js.lay}, lb[2] = {js.lbx, js.lby};
+            const bool dist = js.kind == prx_scene::kDistance;
+            if (int st = AddSpringJoint(w, dist ? prx::kJointDistance : prx::kJointWeld, std::uint32_t(js.a),
+                                        std::uint32_t(js.b), la, lb, dist ? js.length : js.referenceAngle,
+                                        js.frequency, js.dampingRatio, js.collideConnected, nullptr))
+                return st;
+            continue;
+        }
         prx_revolute_joint_conf jc{};
         jc.body_a = std::uint32_t(js.a);
         jc.body_b = std::uint32_t(js.b);
@@ -1367,6 +1405,24 @@ int prx_joint_create(prx_world* w, const prx_revolute_joint_conf* conf, std::uin
     if (!conf) return Fail(PRX_E_INVALID_ARGUMENT, "null joint conf");
     if (int st = Detach(w)) return st;
     return AddJoint(w, *conf, id);
+}
+
+int prx_distance_joint_create(prx_world* w, const prx_distance_joint_conf* conf, std::uint32_t* id) {
+    if (int st = CheckWorld(w)) return st;
+    if (!conf) return Fail(PRX_E_INVALID_ARGUMENT, "null joint conf");
+    if (int st = Detach(w)) return st;
+    return AddSpringJoint(w, prx::kJointDistance, conf->body_a, conf->body_b, conf->local_anchor_a,
+                          conf->local_anchor_b, conf->length, conf->frequency, conf->damping_ratio,
+                          conf->collide_connected != 0, id);
+}
+
+int prx_weld_joint_create(prx_world* w, const prx_weld_joint_conf* conf, std::uint32_t* id) {
+    if (int st = CheckWorld(w)) return st;
+    if (!conf) return Fail(PRX_E_INVALID_ARGUMENT, "null joint conf");
+    if (int st = Detach(w)) return st;
+    return AddSpringJoint(w, prx::kJointWeld, conf->body_a, conf->body_b, conf->local_anchor_a, conf->local_anchor_b,
+                          conf->reference_angle, conf->frequency, conf->damping_ratio, conf->collide_connected != 0,
+                          id);
 }
 
 std::uint32_t prx_world_joint_count(const prx_world* w) { return w ? std::uint32_t(w->joints.size()) : 0u; }
@@ -1473,8 +1529,10 @@ int prx_world_load_desc(prx_world* w, const std::uint8_t* desc, std::size_t len)
         HostJoint j;
         j.a = u32();
         j.b = u32();
-        j.flags = u32() & 7u;
+        j.flags = u32() & 0xff07u;
         if (j.a >= fresh.bodies.size() || j.b >= fresh.bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
+        if ((j.flags >> 8) > prx::kJointWeld) return Fail(PRX_E_INVALID_ARGUMENT, "unknown joint kind");
+        if (j.flags >> 8) j.flags &= ~(prx::kJointLimit | prx::kJointMotor);
         for (float& v : j.la) v = f32();
         for (float& v : j.cfg) v = f32();
         j.maxTorque = f32();
@@ -2295,6 +2353,8 @@ int prx_adjoint_create(prx_batch* b, int nsteps, int every, const prx_step_conf*
     for (prx_world* w : b->worlds) {
         if (!w) return Fail(PRX_E_WRONG_STATE, "a member world left the batch");
         if (!w->joints.empty()) general = true;
+        for (const auto& j : w->joints)
+            if (j.flags >> 8) return Fail(PRX_E_UNSUPPORTED, "the adjoint differentiates revolute joints only");
         for (const auto& hb : w->bodies) {
             if (!(hb.flags & prx::kAccelerable)) continue;
             if (hb.lc.x != 0.0f || hb.lc.y != 0.0f) general = true;

@@ -194,8 +194,11 @@ struct DeviceState {
     // Revolute joints (world order = creation order).
     int2* j_body;        // body A, body B (world-local)
     float4* j_anchor;    // local anchor A x, y, local anchor B x, y
-    float4* j_cfg;       // reference angle, lower angle, upper angle, motor speed
-    float4* j_cfg2;      // max motor torque, flags (limit 1, motor 2, collideConnected 4), -, -
+    float4* j_cfg;       // revolute: reference angle, lower angle, upper angle, motor speed;
+                         // distance: length, frequency, damping ratio, -; weld: reference
+                         // angle, frequency, damping ratio, -
+    float4* j_cfg2;      // max motor torque, flags (limit 1, motor 2, collideConnected 4,
+                         // kind << 8), -, -
     float4* j_imp;       // impulse x, y, z, angular motor impulse
     std::uint32_t* j_limit;  // limit state (0 inactive, 1 lower, 2 upper, 3 equal)
     int2* bjoint;        // per body: first entry in bjoint_list, count (world-local)
@@ -268,6 +271,10 @@ constexpr int kVcStride = 9;
 constexpr int kPcStride = 5;
 constexpr int kJcStride = 8;
 enum : std::uint32_t { kJointLimit = 1u, kJointMotor = 2u, kJointCollideConnected = 4u };
+// Pi as the float build's Real (RealConstants.hpp:57).
+constexpr float kPiF = 3.14159265358979323846f;
+// Joint kinds, in bits 8-15 of a joint's flags.
+enum : std::uint32_t { kJointRevolute = 0u, kJointDistance = 1u, kJointWeld = 2u };
 // Constraint records staged through LDS per chunk by the wave-wide sweeps;
 // the stage sits at the end of the dynamic LDS.
 constexpr int kStageRecords = 16;

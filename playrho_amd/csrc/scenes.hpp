@@ -48,9 +48,14 @@ struct BodySpec {
     std::vector<int> shapes;
 };
 
-// Revolute joint between bodies a and b with local anchors (RevoluteJointConf).
+// Joint between bodies a and b with local anchors: kind 0 RevoluteJointConf,
+// 1 DistanceJointConf (length, frequency, dampingRatio), 2 WeldJointConf
+// (referenceAngle, frequency, dampingRatio).
+enum JointKind : int { kRevolute = 0, kDistance = 1, kWeld = 2 };
 struct JointSpec {
+    int kind = kRevolute;
     int a = 0, b = 0;
+    float length = 1.0f, frequency = 0.0f, dampingRatio = 0.0f;
     float lax = 0, lay = 0, lbx = 0, lby = 0;
     float referenceAngle = 0;
     bool enableLimit = false;
@@ -352,10 +357,109 @@ inline SceneSpec Chain(std::uint32_t seed, int w, int links = 32, int falling = 
     return sc;
 }
 
+// Distance and weld joints: `n` pendulums hang from a static bar at (0, 30)
+// by distance joints (every other one a spring: frequency 2-4 Hz, damping
+// 0.1-0.7), each pendulum bob welded to a second box (every third weld soft:
+// 3 Hz, damping 0.5); a welded 4-box beam rests on the ground; `falling`
+// seeded boxes and disks drop onto them and onto a ground edge.
+inline SceneSpec Joints(std::uint32_t seed, int w, int n = 8, int falling = 24) {
+    SceneSpec sc;
+    Rng rng(seed * 2654435761u + std::uint32_t(w) * 40503u + 11u);
+    sc.shapes.push_back(Edge(-60.0f, 0.0f, 60.0f, 0.0f));
+    BodySpec ground;
+    ground.shapes = {0};
+    sc.bodies.push_back(ground);
+    BodySpec bar;
+    bar.x = 0.0f;
+    bar.y = 30.0f;
+    sc.bodies.push_back(bar);
+    sc.shapes.push_back(Box(0.5f, 0.5f, 1.0f, 0.5f, 0.0f));
+    const int bob = int(sc.shapes.size()) - 1;
+    sc.shapes.push_back(Box(0.4f, 0.25f, 1.0f, 0.5f, 0.0f));
+    const int tail = int(sc.shapes.size()) - 1;
+    for (int i = 0; i < n; ++i) {
+        const float x = float(3 * i) - 1.5f * float(n);
+        BodySpec b;
+        b.type = kDynamic;
+        b.x = x + rng.uniform(-2.0, 2.0);
+        b.y = 30.0f - rng.uniform(4.0, 9.0);
+        b.ay = -10.0f;
+        b.shapes = {bob};
+        sc.bodies.push_back(b);
+        const int ib = int(sc.bodies.size()) - 1;
+        JointSpec d;
+        d.kind = kDistance;
+        d.a = 1;
+        d.b = ib;
+        d.lax = x;
+        d.length = rng.uniform(5.0, 9.0);
+        if (i % 2 == 1) {
+            d.frequency = rng.uniform(2.0, 4.0);
+            d.dampingRatio = rng.uniform(0.1, 0.7);
+        }
+        sc.joints.push_back(d);
+        BodySpec t;
+        t.type = kDynamic;
+        t.x = b.x + 0.9f;
+        t.y = b.y;
+        t.ay = -10.0f;
+        t.shapes = {tail};
+        sc.bodies.push_back(t);
+        JointSpec wj;
+        wj.kind = kWeld;
+        wj.a = ib;
+        wj.b = int(sc.bodies.size()) - 1;
+        wj.lax = 0.5f;
+        wj.lbx = -0.4f;
+        if (i % 3 == 2) {
+            wj.frequency = 3.0f;
+            wj.dampingRatio = 0.5f;
+        }
+        sc.joints.push_back(wj);
+    }
+    for (int k = 0; k < 4; ++k) {
+        BodySpec b;
+        b.type = kDynamic;
+        b.x = 20.0f + 0.8f * float(k);
+        b.y = 0.3f;
+        b.ay = -10.0f;
+        b.shapes = {tail};
+        sc.bodies.push_back(b);
+        if (k > 0) {
+            JointSpec wj;
+            wj.kind = kWeld;
+            wj.a = int(sc.bodies.size()) - 2;
+            wj.b = int(sc.bodies.size()) - 1;
+            wj.lax = 0.4f;
+            wj.lbx = -0.4f;
+            sc.joints.push_back(wj);
+        }
+    }
+    for (int k = 0; k < falling; ++k) {
+        BodySpec b;
+        b.type = kDynamic;
+        b.x = rng.uniform(-25.0, 25.0);
+        b.y = rng.uniform(32.0, 60.0);
+        b.ay = -10.0f;
+        const float friction = rng.uniform(0.2, 0.8);
+        if (rng.below(2) == 0) {
+            // Drawn in sequence: argument evaluation order differs between compilers.
+            const float hx = rng.uniform(0.3, 0.6);
+            const float hy = rng.uniform(0.3, 0.6);
+            sc.shapes.push_back(Box(hx, hy, 1.0f, friction, 0.0f));
+        } else {
+            sc.shapes.push_back(Disk(rng.uniform(0.3, 0.6), 1.0f, friction, 0.0f));
+        }
+        b.shapes = {int(sc.shapes.size()) - 1};
+        sc.bodies.push_back(b);
+    }
+    return sc;
+}
+
 // Named scenes with integer parameters, as used by tests and benchmarks.
 //   "hello", "pyramid"(rows), "drop"(n, half, height, seed),
 //   "pile"(seed, w, nmin, nmax), "billiards"(seed, w),
-//   "chain"(seed, w, links, falling, torque)
+//   "chain"(seed, w, links, falling, torque), "joints"(seed, w, n, falling)
 inline bool Make(const std::string& name, const int* p, int np, SceneSpec& out) {
     auto arg = [&](int i, int dflt) { return (i < np) ? p[i] : dflt; };
     if (name == "hello") {
@@ -370,6 +474,8 @@ inline bool Make(const std::string& name, const int* p, int np, SceneSpec& out) 
         out = Billiards(std::uint32_t(arg(0, 1)), arg(1, 0));
     } else if (name == "chain") {
         out = Chain(std::uint32_t(arg(0, 1)), arg(1, 0), arg(2, 32), arg(3, 64), float(arg(4, 2)));
+    } else if (name == "joints") {
+        out = Joints(std::uint32_t(arg(0, 1)), arg(1, 0), arg(2, 8), arg(3, 24));
     } else {
         return false;
     }

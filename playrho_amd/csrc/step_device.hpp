@@ -1266,6 +1266,266 @@ __device__ __forceinline__ bool joint_solve_position(const float4* u, float4& po
     return (positionError <= c.linearSlop * c.linearSlop) && (angularError <= c.angularSlop);
 }
 
+// ---------------------------------------------------------------------------
+// Distance and weld joints.  Their records share the revolute record's slots
+// u[0] (rA, rB), u[3] (impulses), u[4] (inverse masses), u[5].zw (bodies),
+// u[6] (anchors less local centers) and u[7].w (kind); u[2].w holds the flags
+// with limit state 0.
+//   distance: u[1] direction x, y, mass, invGamma   u[5].x bias
+//             u[7] -, frequency, length, kind        impulse in u[3].x
+//   weld:     u[1] mass exx, eyx, ezx, eyy           u[2] ezy, ezz, -, flags
+//             u[5].xy gamma, bias                    u[7] reference angle, frequency, -, kind
+__device__ __forceinline__ std::uint32_t jc_kind(const float4* u) { return __float_as_uint(u[7].w); }
+
+__device__ __forceinline__ void apply_joint_impulse(const float4 ms, V2 P, float LA, float LB, float4& velA,
+                                                    float4& velB) {
+    velA.x = velA.x - ms.x * P.x;
+    velA.y = velA.y - ms.x * P.y;
+    velA.z = velA.z - ms.y * LA;
+    velB.x = velB.x + ms.z * P.x;
+    velB.y = velB.y + ms.z * P.y;
+    velB.z = velB.z + ms.w * LB;
+}
+
+// GetMat33 (WeldJointConf.cpp:33-60), the same matrix as the revolute one.
+struct K33 {
+    float exx, eyx, ezx, eyy, ezy, ezz;
+};
+__device__ __forceinline__ K33 joint_k33(float mA, V2 rA, float iA, float mB, V2 rB, float iB) {
+    K33 k;
+    k.exx = mA + ((rA.y * rA.y) * iA / 1.0f) + mB + ((rB.y * rB.y) * iB / 1.0f);
+    k.eyx = (-rA.y * rA.x * iA / 1.0f) + (-rB.y * rB.x * iB / 1.0f);
+    k.ezx = (-rA.y * iA) + (-rB.y * iB);
+    k.eyy = mA + ((rA.x * rA.x) * iA / 1.0f) + mB + ((rB.x * rB.x) * iB / 1.0f);
+    k.ezy = (rA.x * iA) + (rB.x * iB);
+    k.ezz = iA + iB;
+    return k;
+}
+
+// InitVelocity(DistanceJointConf&) less its warm start (DistanceJointConf.cpp:81-141).
+__device__ __forceinline__ void build_jc_distance(float4* u, const float4 cf, float4 im, V2 rA, V2 rB, const float4& pA,
+                                                  const float4& pB, float mA, float iA, float mB, float iB,
+                                                  const StepParams& c) {
+    float length;
+    const V2 dir = unit_polar((V2{pB.x, pB.y} + rB) - (V2{pA.x, pA.y} + rA), length);
+    const float crAu = cross(rA, dir) / 1.0f;
+    const float crBu = cross(rB, dir) / 1.0f;
+    float invMass = mA + iA * (crAu * crAu) + mB + iB * (crBu * crBu);
+    float mass = (invMass != 0.0f) ? 1.0f / invMass : 0.0f;
+    float invGamma = 0.0f, bias = 0.0f;
+    if (cf.y > 0.0f) {
+        const float C = length - cf.x;
+        const float omega = 2.0f * prx::kPiF * cf.y;
+        const float d = 2.0f * mass * cf.z * omega;
+        const float k = mass * (omega * omega);
+        const float h = c.dt;
+        const float gamma = h * (d + h * k);
+        invGamma = (gamma != 0.0f) ? 1.0f / gamma : 0.0f;
+        bias = C * h * k * invGamma;
+        invMass += invGamma;
+        mass = (invMass != 0.0f) ? 1.0f / invMass : 0.0f;
+    }
+    im = c.warmStart ? float4{im.x * c.dtRatio, 0.0f, 0.0f, 0.0f} : float4{0.0f, 0.0f, 0.0f, 0.0f};
+    u[1] = float4{dir.x, dir.y, mass, invGamma};
+    u[3] = im;
+    u[5].x = bias;
+    u[5].y = 0.0f;
+    u[7] = float4{0.0f, cf.y, cf.x, __uint_as_float(prx::kJointDistance)};
+}
+
+// InitVelocity(WeldJointConf&) less its warm start (WeldJointConf.cpp:115-195).
+__device__ __forceinline__ void build_jc_weld(float4* u, const float4 cf, float4 im, V2 rA, V2 rB, const float4& pA,
+                                              const float4& pB, float mA, float iA, float mB, float iB,
+                                              std::uint32_t flags, const StepParams& c) {
+    const K33 K = joint_k33(mA, rA, iA, mB, rB, iB);
+    K33 m;
+    float gamma = 0.0f, bias = 0.0f;
+    if (cf.y > 0.0f || K.ezz == 0.0f) {
+        // GetInverse22 (Math.hpp:478-490); K is symmetric.
+        float det = (K.exx * K.eyy) - (K.eyx * K.eyx);
+        if (det != 0.0f) det = 1.0f / det;
+        m = K33{det * K.eyy, -det * K.eyx, 0.0f, det * K.exx, 0.0f, 0.0f};
+        if (cf.y > 0.0f) {
+            float invRotI = iA + iB;
+            const float rotI = (invRotI > 0.0f) ? 1.0f / invRotI : 0.0f;
+            const float C = pB.z - pA.z - cf.x;
+            const float omega = 2.0f * prx::kPiF * cf.y;
+            const float d = 2.0f * rotI * cf.z * omega;
+            const float k = rotI * omega * omega;
+            const float h = c.dt;
+            const float invGamma = h * (d + h * k);
+            gamma = (invGamma != 0.0f) ? 1.0f / invGamma : 0.0f;
+            bias = C * h * k * gamma;
+            invRotI += gamma;
+            m.ezz = (invRotI != 0.0f) ? 1.0f / invRotI : 0.0f;
+        }
+    } else {
+        // GetSymInverse33 (Math.hpp:493-510).
+        float yzx, yzy, yzz;
+        cross3(K.eyx, K.eyy, K.ezy, K.ezx, K.ezy, K.ezz, yzx, yzy, yzz);
+        const float invDet = dot3(K.exx, K.eyx, K.ezx, yzx, yzy, yzz);
+        const float det = (invDet != 0.0f) ? (1.0f / invDet) : invDet;
+        const float a11 = K.exx, a12 = K.eyx, a13 = K.ezx, a22 = K.eyy, a23 = K.ezy, a33 = K.ezz;
+        m.exx = det * (a22 * a33 - a23 * a23);
+        m.eyx = det * (a13 * a23 - a12 * a33);
+        m.ezx = det * (a12 * a23 - a13 * a22);
+        m.eyy = det * (a11 * a33 - a13 * a13);
+        m.ezy = det * (a13 * a12 - a11 * a23);
+        m.ezz = det * (a11 * a22 - a12 * a12);
+    }
+    im = c.warmStart ? float4{im.x * c.dtRatio, im.y * c.dtRatio, im.z * c.dtRatio, 0.0f}
+                     : float4{0.0f, 0.0f, 0.0f, 0.0f};
+    u[1] = float4{m.exx, m.eyx, m.ezx, m.eyy};
+    u[2] = float4{m.ezy, m.ezz, 0.0f, __uint_as_float(flags)};
+    u[3] = im;
+    u[5].x = gamma;
+    u[5].y = bias;
+    u[7] = float4{cf.x, cf.y, 0.0f, __uint_as_float(prx::kJointWeld)};
+}
+
+// Warm start of a distance or weld joint (DistanceJointConf.cpp:143-158,
+// WeldJointConf.cpp:197-210).
+__device__ __forceinline__ void spring_joint_warm_start(const float4* u, std::uint32_t kind, float4& velA,
+                                                        float4& velB) {
+    const float4 r = u[0], im = u[3], ms = u[4];
+    const V2 rA{r.x, r.y}, rB{r.z, r.w};
+    if (kind == prx::kJointDistance) {
+        const V2 P = mul(im.x, V2{u[1].x, u[1].y});
+        apply_joint_impulse(ms, P, cross(rA, P) / 1.0f, cross(rB, P) / 1.0f, velA, velB);
+    } else {
+        const V2 P{im.x, im.y};
+        apply_joint_impulse(ms, P, im.z + cross(rA, P) / 1.0f, im.z + cross(rB, P) / 1.0f, velA, velB);
+    }
+}
+
+// SolveVelocity(DistanceJointConf&) (DistanceJointConf.cpp:164-194): returns
+// whether the impulse was zero.
+__device__ __forceinline__ bool distance_solve_velocity(float4* u, float4& velA, float4& velB) {
+    const float4 r = u[0], m = u[1], ms = u[4];
+    const V2 rA{r.x, r.y}, rB{r.z, r.w}, dir{m.x, m.y};
+    const V2 vpA = V2{velA.x, velA.y} + mul(revperp(rA), velA.z / 1.0f);
+    const V2 vpB = V2{velB.x, velB.y} + mul(revperp(rB), velB.z / 1.0f);
+    const float Cdot = dot(dir, vpB - vpA);
+    const float acc = u[3].x;
+    const float impulse = -m.z * (Cdot + u[5].x + m.w * acc);
+    u[3].x = acc + impulse;
+    const V2 P = mul(impulse, dir);
+    apply_joint_impulse(ms, P, cross(rA, P) / 1.0f, cross(rB, P) / 1.0f, velA, velB);
+    return impulse == 0.0f;
+}
+
+// SolveVelocity(WeldJointConf&) (WeldJointConf.cpp:215-303): returns whether
+// the bodies' velocities were left unchanged.
+__device__ __forceinline__ bool weld_solve_velocity(float4* u, float4& velA, float4& velB) {
+    const float4 r = u[0], m1 = u[1], m2 = u[2], ms = u[4], gb = u[5], cf = u[7];
+    const V2 rA{r.x, r.y}, rB{r.z, r.w};
+    const float4 oldA = velA, oldB = velB;
+    float4 im = u[3];
+    if (cf.y > 0.0f) {
+        const float Cdot2 = velB.z - velA.z;
+        const float g = gb.x * im.z;
+        const float impulse2 = -m2.y * (Cdot2 + gb.y + g);
+        im.z = im.z + impulse2;
+        velA.z = velA.z - ms.y * impulse2;
+        velB.z = velB.z + ms.w * impulse2;
+        const V2 vb = V2{velB.x, velB.y} + mul(revperp(rB), velB.z / 1.0f);
+        const V2 va = V2{velA.x, velA.y} + mul(revperp(rA), velA.z / 1.0f);
+        const V2 Cdot1 = vb - va;
+        // Transform(Vec2, Mat33) (Math.hpp:522-527).
+        const V2 t{m1.x * Cdot1.x + m1.y * Cdot1.y, m1.y * Cdot1.x + m1.w * Cdot1.y};
+        const V2 impulse1 = -t;
+        im.x = im.x + impulse1.x;
+        im.y = im.y + impulse1.y;
+        apply_joint_impulse(ms, impulse1, cross(rA, impulse1) / 1.0f, cross(rB, impulse1) / 1.0f, velA, velB);
+    } else {
+        const V2 vb = V2{velB.x, velB.y} + mul(revperp(rB), velB.z / 1.0f);
+        const V2 va = V2{velA.x, velA.y} + mul(revperp(rA), velA.z / 1.0f);
+        const V2 Cdot1 = vb - va;
+        const float Cdot2 = (velB.z - velA.z) / 1.0f;
+        // Mat33 times vector, row by row (Vector.hpp:455-470).
+        const float tx = dot3(m1.x, m1.y, m1.z, Cdot1.x, Cdot1.y, Cdot2);
+        const float ty = dot3(m1.y, m1.w, m2.x, Cdot1.x, Cdot1.y, Cdot2);
+        const float tz = dot3(m1.z, m2.x, m2.y, Cdot1.x, Cdot1.y, Cdot2);
+        const V2 P{-tx, -ty};
+        const float L = -tz;
+        im.x = im.x + P.x;
+        im.y = im.y + P.y;
+        im.z = im.z + L;
+        apply_joint_impulse(ms, P, L + cross(rA, P) / 1.0f, L + cross(rB, P) / 1.0f, velA, velB);
+    }
+    u[3] = im;
+    const bool same = velA.x == oldA.x && velA.y == oldA.y && velA.z == oldA.z && velB.x == oldB.x &&
+                      velB.y == oldB.y && velB.z == oldB.z;
+    if (same) {
+        velA = oldA;
+        velB = oldB;
+    }
+    return same;
+}
+
+// SolvePosition of a distance joint (DistanceJointConf.cpp:196-245) or a weld
+// joint (WeldJointConf.cpp:305-375): returns whether it is within tolerance.
+__device__ __forceinline__ bool spring_joint_solve_position(const float4* u, std::uint32_t kind, float4& posA,
+                                                            float4& posB, const StepParams& c) {
+    const float4 ms = u[4], la = u[6], cf = u[7];
+    const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
+    if (kind == prx::kJointDistance && cf.y > 0.0f) return true;
+    const V2 rA = rotate(V2{la.x, la.y}, unit_from_angle(posA.z));
+    const V2 rB = rotate(V2{la.z, la.w}, unit_from_angle(posB.z));
+    const V2 d = (V2{posB.x, posB.y} + rB) - (V2{posA.x, posA.y} + rA);
+    V2 P;
+    float LA, LB;
+    bool ok;
+    if (kind == prx::kJointDistance) {
+        float length;
+        const V2 dir = unit_polar(d, length);
+        const float C = clamp_std(length - cf.z, -c.maxLinearCorrection, c.maxLinearCorrection);
+        const float impulse = -u[1].z * C;
+        P = mul(impulse, dir);
+        LA = cross(rA, P) / 1.0f;
+        LB = cross(rB, P) / 1.0f;
+        ok = __builtin_fabsf(C) < c.linearSlop;
+    } else {
+        const K33 K = joint_k33(mA, rA, iA, mB, rB, iB);
+        const float positionError = __builtin_sqrtf(magsq(d));
+        float angularError = 0.0f, L = 0.0f;
+        Jc kj;
+        kj.exx = K.exx;
+        kj.eyx = K.eyx;
+        kj.ezx = K.ezx;
+        kj.eyy = K.eyy;
+        kj.ezy = K.ezy;
+        kj.ezz = K.ezz;
+        if (cf.y > 0.0f) {
+            P = -solve22(kj, d);
+            LA = cross(rA, P) / 1.0f;
+            LB = cross(rB, P) / 1.0f;
+        } else {
+            const float C2 = posB.z - posA.z - cf.x;
+            angularError = __builtin_fabsf(C2);
+            if (K.ezz > 0.0f) {
+                float sx, sy, sz;
+                solve33(kj, d.x, d.y, C2, sx, sy, sz);
+                P = V2{-sx, -sy};
+                L = -sz;
+            } else {
+                P = -solve22(kj, d);
+                L = 0.0f;
+            }
+            LA = L + cross(rA, P) / 1.0f;
+            LB = L + cross(rB, P) / 1.0f;
+        }
+        ok = (positionError <= c.linearSlop) && (angularError <= c.angularSlop);
+    }
+    posA.x = posA.x - mA * P.x;
+    posA.y = posA.y - mA * P.y;
+    posA.z = posA.z - iA * LA;
+    posB.x = posB.x + mB * P.x;
+    posB.y = posB.y + mB * P.y;
+    posB.z = posB.z + iB * LB;
+    return ok;
+}
+
 // Per-step joint set-up half of InitVelocity (RevoluteJointConf.cpp:127-197),
 // writes the joint's solver record.
 __device__ __forceinline__ void build_jc(const DeviceState& d, const WorldCtx& x, int jg, float4* u, const BodyC& A,
@@ -1282,6 +1542,17 @@ __device__ __forceinline__ void build_jc(const DeviceState& d, const WorldCtx& x
     const V2 rA = rotate(laA, qA);
     const V2 rB = rotate(laB, qB);
     const float mA = A.invMass, iA = A.invRotI, mB = B.invMass, iB = B.invRotI;
+    const std::uint32_t kind = flags >> 8;
+    if (kind != prx::kJointRevolute) {
+        u[0] = float4{rA.x, rA.y, rB.x, rB.y};
+        u[2] = float4{0.0f, 0.0f, 0.0f, __uint_as_float(flags & 0xffu)};
+        u[4] = float4{mA, iA, mB, iB};
+        u[5] = float4{0.0f, 0.0f, __int_as_float(bA), __int_as_float(bB)};
+        u[6] = float4{laA.x, laA.y, laB.x, laB.y};
+        if (kind == prx::kJointDistance) build_jc_distance(u, cf, im, rA, rB, pA, pB, mA, iA, mB, iB, c);
+        else build_jc_weld(u, cf, im, rA, rB, pA, pB, mA, iA, mB, iB, flags & 0xffu, c);
+        return;
+    }
     const float totInvI = iA + iB;
     const bool fixedRotation = (totInvI == 0.0f);
     // GetMat33 (RevoluteJointConf.cpp:36-62).
@@ -1328,7 +1599,7 @@ __device__ __forceinline__ void build_jc(const DeviceState& d, const WorldCtx& x
     u[4] = float4{mA, iA, mB, iB};
     u[5] = float4{cf.w, c.dt * cf2.x, __int_as_float(bA), __int_as_float(bB)};
     u[6] = float4{laA.x, laA.y, laB.x, laB.y};
-    u[7] = float4{cf.x, cf.y, cf.z, 0.0f};
+    u[7] = float4{cf.x, cf.y, cf.z, __uint_as_float(prx::kJointRevolute)};
 }
 
 __device__ __forceinline__ BodyC load_bodyc(const DeviceState& d, int gb) {
@@ -1720,6 +1991,19 @@ __device__ __forceinline__ bool joints_velocity(float4* jbase, int j0, int j1, c
     bool ok = true;
     for (int e = j0; e < j1; ++e) {
         float4* u = jbase + std::size_t(e) * prx::kJcStride;
+        const std::uint32_t kind = jc_kind(u);
+        if (kind != prx::kJointRevolute) {
+            const float4 mo = u[5];
+            const int bA = __float_as_int(mo.z), bB = __float_as_int(mo.w);
+            float4 va = bs.vel(bA);
+            float4 vb = bs.vel(bB);
+            if (warm) spring_joint_warm_start(u, kind, va, vb);
+            else if (kind == prx::kJointDistance) ok &= distance_solve_velocity(u, va, vb);
+            else ok &= weld_solve_velocity(u, va, vb);
+            bs.set_vel(bA, va);
+            bs.set_vel(bB, vb);
+            continue;
+        }
         Jc j = load_jc_velocity(u);
         float4 va = bs.vel(j.bA);
         float4 vb = bs.vel(j.bB);
@@ -1770,7 +2054,9 @@ __device__ __forceinline__ bool joints_position(const float4* jbase, int j0, int
         const int bA = __float_as_int(mo.z), bB = __float_as_int(mo.w);
         float4 pa = bs.pos(bA);
         float4 pb = bs.pos(bB);
-        ok &= joint_solve_position(u, pa, pb, c);
+        const std::uint32_t kind = jc_kind(u);
+        if (kind == prx::kJointRevolute) ok &= joint_solve_position(u, pa, pb, c);
+        else ok &= spring_joint_solve_position(u, kind, pa, pb, c);
         bs.set_pos(bA, pa);
         bs.set_pos(bB, pb);
     }

@@ -37,6 +37,9 @@ CASES = [
 ] + [("pile%d" % w, "pile", 150, [1, w], [1, 150]) for w in range(8)] + [
     ("chain0", "chain", 240, [1, 0], [1, 120, 240]),
     ("chain1", "chain", 240, [1, 1, 12, 24, 40], [1, 240]),
+    # Distance and weld joints, rigid and soft, beside falling boxes.
+    ("joints0", "joints", 240, [1, 0], [1, 120, 240]),
+    ("joints1", "joints", 240, [2, 1, 12, 40], [1, 240]),
 ]
 # World edits between steps (step, kind, body, a, b, c): SetLocation 'L',
 # SetAngle 'A', SetTransformation 'T' (c the rotation's angle), ShiftOrigin 'O'.
@@ -54,14 +57,18 @@ TOI_CASES = [
     ("billiards0_toi", "billiards", 120, [1, 0], [1, 60, 120]),
     ("pile3_toi", "pile", 150, [1, 3], [1, 150]),
     ("chain0_toi", "chain", 240, [1, 0], [1, 240]),
+    ("joints0_toi", "joints", 240, [1, 0], [1, 240]),
 ]
 
 
-def main():
+def main(only=()):
+    """Writes every case, or only the named ones."""
     tmp = tempfile.mkdtemp()
     try:
         for name, scene, steps, params, full, do_toi, edits in ([c + (False, []) for c in CASES] +
                                                                 [c + (True, []) for c in TOI_CASES] + EDIT_CASES):
+            if only and name not in only:
+                continue
             path = os.path.join(tmp, name + ".bin")
             buf = oracle_lib.run_ref_driver(scene, steps, do_toi, params, path, edits)
             desc_conf, d = oracle_lib.split_dump(buf)
@@ -91,4 +98,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(tuple(sys.argv[1:]))

@@ -59,7 +59,8 @@ class Collector:
 
 
 @pytest.mark.parametrize("name,steps", [("hello", 60), ("pile3", 150), ("billiards0", 120), ("chain0", 120),
-                                        ("drop300", 80), ("pile3_toi", 150), ("billiards0_toi", 120)])
+                                        ("drop300", 80), ("pile3_toi", 150), ("billiards0_toi", 120),
+                                        ("joints0_toi", 120)])
 def test_listener_calls_match_oracle(name, steps):
     """The TOI cases add the calls of the TOI phase: its updates and its
     post-solve reports with the position iterations."""
