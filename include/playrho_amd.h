@@ -118,6 +118,26 @@ typedef struct prx_weld_joint_conf {
     uint32_t collide_connected;
 } prx_weld_joint_conf;
 
+/* RopeJointConf (d2/RopeJointConf.hpp:67-106): a maximum distance between the
+ * anchors.  The reference's defaults are local anchors (-1, 0) and (1, 0),
+ * maximum length 0. */
+typedef struct prx_rope_joint_conf {
+    uint32_t body_a, body_b;
+    float local_anchor_a[2], local_anchor_b[2];
+    float max_length;
+    uint32_t collide_connected;
+} prx_rope_joint_conf;
+
+/* FrictionJointConf (d2/FrictionJointConf.hpp:58-117): top-down friction with
+ * maximum force and torque (both default 0; negative or NaN is
+ * PRX_E_INVALID_ARGUMENT, NonNegative there). */
+typedef struct prx_friction_joint_conf {
+    uint32_t body_a, body_b;
+    float local_anchor_a[2], local_anchor_b[2];
+    float max_force, max_torque;
+    uint32_t collide_connected;
+} prx_friction_joint_conf;
+
 /* StepConf fields the regular step reads (StepConf.hpp:49-392). */
 typedef struct prx_step_conf {
     float delta_time, dt_ratio, min_still_time_to_sleep, linear_slop, angular_slop;
@@ -196,6 +216,10 @@ int prx_joint_create(prx_world* world, const prx_revolute_joint_conf* conf, uint
  * Joint ids are shared by all kinds, in creation order. */
 int prx_distance_joint_create(prx_world* world, const prx_distance_joint_conf* conf, uint32_t* joint_id);
 int prx_weld_joint_create(prx_world* world, const prx_weld_joint_conf* conf, uint32_t* joint_id);
+/* CreateJoint(World&, const RopeJointConf&) and (…, const FrictionJointConf&)
+ * (solvers RopeJointConf.cpp:59-212, FrictionJointConf.cpp:76-237). */
+int prx_rope_joint_create(prx_world* world, const prx_rope_joint_conf* conf, uint32_t* joint_id);
+int prx_friction_joint_create(prx_world* world, const prx_friction_joint_conf* conf, uint32_t* joint_id);
 uint32_t prx_world_joint_count(const prx_world* world);
 /* Builds a named workload scene through the calls above. */
 int prx_scene_build(prx_world* world, const char* name, const int32_t* params, int nparams);

@@ -118,11 +118,18 @@ void Load(Handle<R>& h, In& in) {
         j.kind = (fl >> 8) & 0xffu;
         if (j.kind != 0) {
             // Distance: length, frequency, damping ratio.  Weld: reference
-            // angle, frequency, damping ratio.
+            // angle, frequency, damping ratio.  Rope: maximum length.
+            // Friction: maximum force, maximum torque.
             j.enableLimit = j.enableMotor = false;
-            if (j.kind == 1) j.length = f[4], j.referenceAngle = 0;
-            j.frequency = f[5];
-            j.dampingRatio = f[6];
+            if (j.kind == 1 || j.kind == 3) j.length = f[4];
+            if (j.kind != 2) j.referenceAngle = 0;
+            if (j.kind <= 2) {
+                j.frequency = f[5];
+                j.dampingRatio = f[6];
+            } else if (j.kind == 4) {
+                j.maxForce = f[4];
+                j.maxTorque = f[5];
+            }
             j.lower = j.upper = j.motorSpeed = j.maxMotorTorque = 0;
         }
         h.world.AddJoint(j);

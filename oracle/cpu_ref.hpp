@@ -786,6 +786,11 @@ struct Joint {
     R length = 0, frequency = 0, dampingRatio = 0;
     V2<R> u;
     R dmass = 0, invGamma = 0, bias = 0, gamma = 0;
+    // kind 3 rope: length is the maximum length, curLength the length found
+    // by InitVelocity.  kind 4 friction: linear impulse in impulse.x, y and
+    // angular impulse in impulse.z.
+    R curLength = 0, maxForce = 0, maxTorque = 0;
+    R lm00 = 0, lm01 = 0, lm10 = 0, lm11 = 0;
 };
 
 template <class R>
@@ -1750,10 +1755,175 @@ public:
         return (positionError <= conf.linearSlop) && (angularError <= conf.angularSlop);
     }
 
+    // InitVelocity(RopeJointConf&) (RopeJointConf.cpp:59-128).
+    static void RopeInitVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        const R mA = A.invMass, iA = A.invRotI, mB = B.invMass, iB = B.invRotI;
+        Vel<R> velA = A.vel, velB = B.vel;
+        j.rA = Rotate(j.laA - A.lc, UnitFromAngle(A.pos.ang));
+        j.rB = Rotate(j.laB - B.lc, UnitFromAngle(B.pos.ang));
+        R length;
+        const V2<R> uv = UnitPolar((B.pos.lin + j.rB) - (A.pos.lin + j.rA), length);
+        j.curLength = length;
+        const R C = length - j.length;
+        j.limitState = (C > R(0)) ? kLimitUpper : kLimitInactive;
+        if (length > conf.linearSlop) {
+            j.u = uv;
+        } else {
+            j.u = V2<R>{R(0), R(0)};
+            j.dmass = 0;
+            j.impulse.x = 0;
+            return;
+        }
+        const R crA = Cross(j.rA, j.u) / R(1);
+        const R crB = Cross(j.rB, j.u) / R(1);
+        const R invMass = mA + mB + iA * (crA * crA) + iB * (crB * crB);
+        j.dmass = (invMass != R(0)) ? R(1) / invMass : R(0);
+        if (conf.warmStart) {
+            j.impulse.x *= conf.dtRatio;
+            const V2<R> P = j.impulse.x * j.u;
+            ApplyImpulse(A, B, P, Cross(j.rA, P) / R(1), Cross(j.rB, P) / R(1), velA, velB);
+        } else {
+            j.impulse.x = 0;
+        }
+        A.vel = velA;
+        B.vel = velB;
+    }
+
+    // SolveVelocity(RopeJointConf&) (RopeJointConf.cpp:130-167).
+    static bool RopeSolveVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        Vel<R> velA = A.vel, velB = B.vel;
+        const V2<R> vpA = velA.lin + RevPerp(j.rA) * (velA.ang / R(1));
+        const V2<R> vpB = velB.lin + RevPerp(j.rB) * (velB.ang / R(1));
+        const R C = j.curLength - j.length;
+        const R invH = (conf.dt != R(0)) ? R(1) / conf.dt : R(0);
+        const R Cdot = Dot(j.u, vpB - vpA) + ((C < R(0)) ? invH * C : R(0));
+        R local = -j.dmass * Cdot;
+        const R old = j.impulse.x;
+        j.impulse.x = Min(R(0), j.impulse.x + local);
+        local = j.impulse.x - old;
+        const V2<R> P = local * j.u;
+        ApplyImpulse(A, B, P, Cross(j.rA, P) / R(1), Cross(j.rB, P) / R(1), velA, velB);
+        A.vel = velA;
+        B.vel = velB;
+        return local == R(0);
+    }
+
+    // SolvePosition(const RopeJointConf&) (RopeJointConf.cpp:169-212).
+    static bool RopeSolvePosition(const Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        Pos<R> posA = A.pos, posB = B.pos;
+        const V2<R> rA = Rotate(j.laA - A.lc, UnitFromAngle(posA.ang));
+        const V2<R> rB = Rotate(j.laB - B.lc, UnitFromAngle(posB.ang));
+        R length;
+        const V2<R> uv = UnitPolar((posB.lin + rB) - (posA.lin + rA), length);
+        const R C = Clamp(length - j.length, R(0), conf.maxLinearCorrection);
+        const R local = -j.dmass * C;
+        const V2<R> P = local * uv;
+        posA.lin = posA.lin - A.invMass * P;
+        posA.ang = posA.ang - A.invRotI * (Cross(rA, P) / R(1));
+        posB.lin = posB.lin + B.invMass * P;
+        posB.ang = posB.ang + B.invRotI * (Cross(rB, P) / R(1));
+        A.pos = posA;
+        B.pos = posB;
+        return (length - j.length) < conf.linearSlop;
+    }
+
+    // InitVelocity(FrictionJointConf&) (FrictionJointConf.cpp:76-139).
+    static void FrictionInitVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        const R mA = A.invMass, iA = A.invRotI, mB = B.invMass, iB = B.invRotI;
+        Vel<R> velA = A.vel, velB = B.vel;
+        j.rA = Rotate(j.laA - A.lc, UnitFromAngle(A.pos.ang));
+        j.rB = Rotate(j.laB - B.lc, UnitFromAngle(B.pos.ang));
+        const V2<R> rA = j.rA, rB = j.rB;
+        const R exx = mA + iA * (rA.y * rA.y) / R(1) + mB + iB * (rB.y * rB.y) / R(1);
+        const R exy = -iA * rA.x * rA.y / R(1) + -iB * rB.x * rB.y / R(1);
+        const R eyy = mA + iA * (rA.x * rA.x) / R(1) + mB + iB * (rB.x * rB.x) / R(1);
+        // Invert(Matrix22) (Math.hpp:429-442) of columns (exx, exy), (exy, eyy).
+        const R cp = Cross(V2<R>{exx, exy}, V2<R>{exy, eyy});
+        if (!AlmostZero(cp)) {
+            const R inverse = R(1) / cp;
+            j.lm00 = eyy * inverse;
+            j.lm01 = -exy * inverse;
+            j.lm10 = -exy * inverse;
+            j.lm11 = exx * inverse;
+        } else {
+            j.lm00 = j.lm01 = j.lm10 = j.lm11 = 0;
+        }
+        const R invRotI = iA + iB;
+        j.angularMass = (invRotI > R(0)) ? R(1) / invRotI : R(0);
+        if (conf.warmStart) {
+            j.impulse.x *= conf.dtRatio;
+            j.impulse.y *= conf.dtRatio;
+            j.impulse.z *= conf.dtRatio;
+            const V2<R> P{j.impulse.x, j.impulse.y};
+            ApplyImpulse(A, B, P, Cross(rA, P) / R(1) + j.impulse.z, Cross(rB, P) / R(1) + j.impulse.z, velA, velB);
+        } else {
+            j.impulse = V3<R>{};
+        }
+        A.vel = velA;
+        B.vel = velB;
+    }
+
+    // SolveVelocity(FrictionJointConf&) (FrictionJointConf.cpp:141-231).
+    static bool FrictionSolveVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        Vel<R> velA = A.vel, velB = B.vel;
+        const R iA = A.invRotI, iB = B.invRotI;
+        const R h = conf.dt;
+        bool solved = true;
+        {
+            const R angularImpulse = -j.angularMass * (velB.ang - velA.ang);
+            const R old = j.impulse.z;
+            const R maxAngular = h * j.maxTorque;
+            j.impulse.z = Clamp(j.impulse.z + angularImpulse, -maxAngular, maxAngular);
+            const R inc = j.impulse.z - old;
+            if (inc != R(0)) solved = false;
+            velA.ang = velA.ang - iA * inc;
+            velB.ang = velB.ang + iB * inc;
+        }
+        {
+            const V2<R> vb = velB.lin + RevPerp(j.rB) * (velB.ang / R(1));
+            const V2<R> va = velA.lin + RevPerp(j.rA) * (velA.ang / R(1));
+            const V2<R> dv = vb - va;
+            // Transform(v, Mat22): the matrix's vectors dotted with v from zero
+            // (Vector.hpp:455-470).
+            R tx = 0, ty = 0;
+            tx += j.lm00 * dv.x;
+            tx += j.lm01 * dv.y;
+            ty += j.lm10 * dv.x;
+            ty += j.lm11 * dv.y;
+            const V2<R> old{j.impulse.x, j.impulse.y};
+            V2<R> li{old.x + -tx, old.y + -ty};
+            const R maxImpulse = h * j.maxForce;
+            if (MagSq(li) > maxImpulse * maxImpulse) {
+                const V2<R> dir = UnitVector(li, V2<R>{R(0), R(0)});
+                li = V2<R>{dir.x * maxImpulse, dir.y * maxImpulse};
+            }
+            j.impulse.x = li.x;
+            j.impulse.y = li.y;
+            const V2<R> inc = li - old;
+            if (inc.x != R(0) || inc.y != R(0)) solved = false;
+            ApplyImpulse(A, B, inc, Cross(j.rA, inc) / R(1), Cross(j.rB, inc) / R(1), velA, velB);
+        }
+        A.vel = velA;
+        B.vel = velB;
+        return solved;
+    }
+
     // InitVelocity(RevoluteJointConf&) (RevoluteJointConf.cpp:127-226).
     static void JointInitVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
         if (j.kind == 1) return DistanceInitVelocity(j, bcs, conf);
         if (j.kind == 2) return WeldInitVelocity(j, bcs, conf);
+        if (j.kind == 3) return RopeInitVelocity(j, bcs, conf);
+        if (j.kind == 4) return FrictionInitVelocity(j, bcs, conf);
         BodyConstraint<R>& A = bcs[j.bodyA];
         BodyConstraint<R>& B = bcs[j.bodyB];
         const R mA = A.invMass, iA = A.invRotI, aA = A.pos.ang;
@@ -1814,6 +1984,8 @@ public:
     static bool JointSolveVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
         if (j.kind == 1) return DistanceSolveVelocity(j, bcs);
         if (j.kind == 2) return WeldSolveVelocity(j, bcs);
+        if (j.kind == 3) return RopeSolveVelocity(j, bcs, conf);
+        if (j.kind == 4) return FrictionSolveVelocity(j, bcs, conf);
         BodyConstraint<R>& A = bcs[j.bodyA];
         BodyConstraint<R>& B = bcs[j.bodyB];
         const Vel<R> oldVelA = A.vel, oldVelB = B.vel;
@@ -1887,6 +2059,8 @@ public:
     static bool JointSolvePosition(const Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
         if (j.kind == 1) return DistanceSolvePosition(j, bcs, conf);
         if (j.kind == 2) return WeldSolvePosition(j, bcs, conf);
+        if (j.kind == 3) return RopeSolvePosition(j, bcs, conf);
+        if (j.kind == 4) return true;  // FrictionJointConf.cpp:233-237
         BodyConstraint<R>& A = bcs[j.bodyA];
         BodyConstraint<R>& B = bcs[j.bodyB];
         Pos<R> posA = A.pos, posB = B.pos;

@@ -38,6 +38,8 @@
 #include <playrho/d2/RevoluteJointConf.hpp>
 #include <playrho/d2/DistanceJointConf.hpp>
 #include <playrho/d2/WeldJointConf.hpp>
+#include <playrho/d2/RopeJointConf.hpp>
+#include <playrho/d2/FrictionJointConf.hpp>
 #include <playrho/d2/WorldJoint.hpp>
 #include <playrho/StepConf.hpp>
 
@@ -190,6 +192,20 @@ void WriteDescription(Out& o, const World& world) {
             const float f[9] = {GetX(j->localAnchorA), GetY(j->localAnchorA), GetX(j->localAnchorB),
                                 GetY(j->localAnchorB), j->referenceAngle, j->frequency, j->dampingRatio,
                                 0.0f, 0.0f};
+            o.put(f, sizeof(f));
+        } else if (const auto* j = TypeCast<const RopeJointConf>(&joint)) {
+            o.u32(to_underlying(GetBodyA(*j)));
+            o.u32(to_underlying(GetBodyB(*j)));
+            o.u32((j->collideConnected ? 4u : 0u) | (3u << 8));
+            const float f[9] = {GetX(j->localAnchorA), GetY(j->localAnchorA), GetX(j->localAnchorB),
+                                GetY(j->localAnchorB), j->maxLength, 0.0f, 0.0f, 0.0f, 0.0f};
+            o.put(f, sizeof(f));
+        } else if (const auto* j = TypeCast<const FrictionJointConf>(&joint)) {
+            o.u32(to_underlying(GetBodyA(*j)));
+            o.u32(to_underlying(GetBodyB(*j)));
+            o.u32((j->collideConnected ? 4u : 0u) | (4u << 8));
+            const float f[9] = {GetX(j->localAnchorA), GetY(j->localAnchorA), GetX(j->localAnchorB),
+                                GetY(j->localAnchorB), j->maxForce, j->maxTorque, 0.0f, 0.0f, 0.0f};
             o.put(f, sizeof(f));
         }
     }
@@ -384,6 +400,14 @@ void WriteState(Out& o, const World& world, std::uint32_t step, const StepStats&
             o.u32(0u);
             const float f[4] = {GetX(j->impulse), GetY(j->impulse), GetZ(j->impulse), 0.0f};
             o.put(f, sizeof(f));
+        } else if (const auto* j = TypeCast<const RopeJointConf>(&joint)) {
+            o.u32(std::uint32_t(j->limitState));
+            const float f[4] = {j->impulse, 0.0f, 0.0f, 0.0f};
+            o.put(f, sizeof(f));
+        } else if (const auto* j = TypeCast<const FrictionJointConf>(&joint)) {
+            o.u32(0u);
+            const float f[4] = {GetX(j->linearImpulse), GetY(j->linearImpulse), j->angularImpulse, 0.0f};
+            o.put(f, sizeof(f));
         }
     }
     o.u32(g_events.n);
@@ -423,6 +447,18 @@ World BuildWorld(const prx_scene::SceneSpec& spec) {
         if (js.kind == prx_scene::kDistance) {
             auto jc = DistanceJointConf{ba, bb, Length2{js.lax, js.lay}, Length2{js.lbx, js.lby}, js.length};
             jc.UseFrequency(js.frequency).UseDampingRatio(js.dampingRatio);
+            jc.UseCollideConnected(js.collideConnected);
+            CreateJoint(world, jc);
+        } else if (js.kind == prx_scene::kRope) {
+            auto jc = RopeJointConf{ba, bb};
+            jc.localAnchorA = Length2{js.lax, js.lay};
+            jc.localAnchorB = Length2{js.lbx, js.lby};
+            jc.UseMaxLength(js.length);
+            jc.UseCollideConnected(js.collideConnected);
+            CreateJoint(world, jc);
+        } else if (js.kind == prx_scene::kFriction) {
+            auto jc = FrictionJointConf{ba, bb, Length2{js.lax, js.lay}, Length2{js.lbx, js.lby}};
+            jc.UseMaxForce(js.maxForce).UseMaxTorque(js.maxTorque);
             jc.UseCollideConnected(js.collideConnected);
             CreateJoint(world, jc);
         } else if (js.kind == prx_scene::kWeld) {

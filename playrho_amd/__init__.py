@@ -167,6 +167,28 @@ class DistanceJointConf(ctypes.Structure):
         super().__init__(**kw)
 
 
+class RopeJointConf(ctypes.Structure):
+    """RopeJointConf (d2/RopeJointConf.hpp:67-106); the reference's defaults:
+    local anchors (-1, 0) and (1, 0), maximum length 0."""
+    _fields_ = [("body_a", ctypes.c_uint32), ("body_b", ctypes.c_uint32),
+                ("local_anchor_a", ctypes.c_float * 2), ("local_anchor_b", ctypes.c_float * 2),
+                ("max_length", ctypes.c_float), ("collide_connected", ctypes.c_uint32)]
+
+    def __init__(self, **kw):
+        kw.setdefault("local_anchor_a", (-1.0, 0.0))
+        kw.setdefault("local_anchor_b", (1.0, 0.0))
+        super().__init__(**kw)
+
+
+class FrictionJointConf(ctypes.Structure):
+    """FrictionJointConf (d2/FrictionJointConf.hpp:58-117); maximum force and
+    torque default to 0."""
+    _fields_ = [("body_a", ctypes.c_uint32), ("body_b", ctypes.c_uint32),
+                ("local_anchor_a", ctypes.c_float * 2), ("local_anchor_b", ctypes.c_float * 2),
+                ("max_force", ctypes.c_float), ("max_torque", ctypes.c_float),
+                ("collide_connected", ctypes.c_uint32)]
+
+
 class WeldJointConf(ctypes.Structure):
     """WeldJointConf (d2/WeldJointConf.hpp:62-130); frequency 0 is rigid."""
     _fields_ = [("body_a", ctypes.c_uint32), ("body_b", ctypes.c_uint32),
@@ -220,6 +242,8 @@ _SIGS = {
     "prx_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
     "prx_distance_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
     "prx_weld_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
+    "prx_rope_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
+    "prx_friction_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
     "prx_world_joint_count": (ctypes.c_uint32, [ctypes.c_void_p]),
     "prx_world_shape_count": (ctypes.c_uint32, [ctypes.c_void_p]),
     "prx_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
@@ -354,10 +378,12 @@ class World:
         return bid.value
 
     def create_joint(self, conf):
-        """CreateJoint with a RevoluteJointConf, DistanceJointConf or
-        WeldJointConf; returns the joint id (shared by all kinds)."""
+        """CreateJoint with a RevoluteJointConf, DistanceJointConf,
+        WeldJointConf, RopeJointConf or FrictionJointConf; returns the joint id
+        (shared by all kinds)."""
         fn = {RevoluteJointConf: lib().prx_joint_create, DistanceJointConf: lib().prx_distance_joint_create,
-              WeldJointConf: lib().prx_weld_joint_create}.get(type(conf))
+              WeldJointConf: lib().prx_weld_joint_create, RopeJointConf: lib().prx_rope_joint_create,
+              FrictionJointConf: lib().prx_friction_joint_create}.get(type(conf))
         if fn is None:
             raise TypeError("not a joint conf: %r" % type(conf))
         jid = ctypes.c_uint32()

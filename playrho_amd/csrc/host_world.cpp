@@ -261,7 +261,8 @@ struct HostJoint {
     float la[4] = {0, 0, 0, 0};             // local anchor A x, y, B x, y
     // Revolute: reference angle, lower, upper, motor speed.  Distance: length,
     // frequency, damping ratio, 0.  Weld: reference angle, frequency, damping
-    // ratio, 0.
+    // ratio, 0.  Rope: maximum length, 0, 0, 0.  Friction: maximum force,
+    // maximum torque, 0, 0.
     float cfg[4] = {0, 0, 0, 0};
     float maxTorque = 0;
     float imp[4] = {0, 0, 0, 0};            // impulse x, y, z, angular motor impulse
@@ -1371,6 +1372,22 @@ int prx_scene_build(prx_world* w, const char* name, const std::int32_t* params, 
         if (int st = AddBody(w, c, nullptr)) return st;
     }
     for (const auto& js : spec.joints) {
+        if (js.kind == prx_scene::kRope || js.kind == prx_scene::kFriction) {
+            HostJoint j;
+            j.a = std::uint32_t(js.a);
+            j.b = std::uint32_t(js.b);
+            const bool rope = js.kind == prx_scene::kRope;
+            j.flags = (js.collideConnected ? prx::kJointCollideConnected : 0u) |
+                      ((rope ? prx::kJointRope : prx::kJointFriction) << 8);
+            j.la[0] = js.lax;
+            j.la[1] = js.lay;
+            j.la[2] = js.lbx;
+            j.la[3] = js.lby;
+            j.cfg[0] = rope ? js.length : js.maxForce;
+            j.cfg[1] = rope ? 0.0f : js.maxTorque;
+            if (int st = AddJointRecord(w, j, nullptr)) return st;
+            continue;
+        }
         if (js.kind != prx_scene::kRevolute) {
             const float la[2] = {js.lax, js.lay}, lb[2] = {js.lbx, js.lby};
             const bool dist = js.kind == prx_scene::kDistance;
@@ -1414,6 +1431,41 @@ int prx_distance_joint_create(prx_world* w, const prx_distance_joint_conf* conf,
     return AddSpringJoint(w, prx::kJointDistance, conf->body_a, conf->body_b, conf->local_anchor_a,
                           conf->local_anchor_b, conf->length, conf->frequency, conf->damping_ratio,
                           conf->collide_connected != 0, id);
+}
+
+int prx_rope_joint_create(prx_world* w, const prx_rope_joint_conf* conf, std::uint32_t* id) {
+    if (int st = CheckWorld(w)) return st;
+    if (!conf) return Fail(PRX_E_INVALID_ARGUMENT, "null joint conf");
+    if (int st = Detach(w)) return st;
+    HostJoint j;
+    j.a = conf->body_a;
+    j.b = conf->body_b;
+    j.flags = (conf->collide_connected ? prx::kJointCollideConnected : 0u) | (prx::kJointRope << 8);
+    j.la[0] = conf->local_anchor_a[0];
+    j.la[1] = conf->local_anchor_a[1];
+    j.la[2] = conf->local_anchor_b[0];
+    j.la[3] = conf->local_anchor_b[1];
+    j.cfg[0] = conf->max_length;
+    return AddJointRecord(w, j, id);
+}
+
+int prx_friction_joint_create(prx_world* w, const prx_friction_joint_conf* conf, std::uint32_t* id) {
+    if (int st = CheckWorld(w)) return st;
+    if (!conf) return Fail(PRX_E_INVALID_ARGUMENT, "null joint conf");
+    if (!(conf->max_force >= 0.0f) || !(conf->max_torque >= 0.0f))
+        return Fail(PRX_E_INVALID_ARGUMENT, "maximum force and torque must be non-negative");
+    if (int st = Detach(w)) return st;
+    HostJoint j;
+    j.a = conf->body_a;
+    j.b = conf->body_b;
+    j.flags = (conf->collide_connected ? prx::kJointCollideConnected : 0u) | (prx::kJointFriction << 8);
+    j.la[0] = conf->local_anchor_a[0];
+    j.la[1] = conf->local_anchor_a[1];
+    j.la[2] = conf->local_anchor_b[0];
+    j.la[3] = conf->local_anchor_b[1];
+    j.cfg[0] = conf->max_force;
+    j.cfg[1] = conf->max_torque;
+    return AddJointRecord(w, j, id);
 }
 
 int prx_weld_joint_create(prx_world* w, const prx_weld_joint_conf* conf, std::uint32_t* id) {
@@ -1531,7 +1583,7 @@ int prx_world_load_desc(prx_world* w, const std::uint8_t* desc, std::size_t len)
         j.b = u32();
         j.flags = u32() & 0xff07u;
         if (j.a >= fresh.bodies.size() || j.b >= fresh.bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
-        if ((j.flags >> 8) > prx::kJointWeld) return Fail(PRX_E_INVALID_ARGUMENT, "unknown joint kind");
+        if ((j.flags >> 8) > prx::kJointFriction) return Fail(PRX_E_INVALID_ARGUMENT, "unknown joint kind");
         if (j.flags >> 8) j.flags &= ~(prx::kJointLimit | prx::kJointMotor);
         for (float& v : j.la) v = f32();
         for (float& v : j.cfg) v = f32();

@@ -196,7 +196,8 @@ struct DeviceState {
     float4* j_anchor;    // local anchor A x, y, local anchor B x, y
     float4* j_cfg;       // revolute: reference angle, lower angle, upper angle, motor speed;
                          // distance: length, frequency, damping ratio, -; weld: reference
-                         // angle, frequency, damping ratio, -
+                         // angle, frequency, damping ratio, -; rope: maximum length, -, -, -;
+                         // friction: maximum force, maximum torque, -, -
     float4* j_cfg2;      // max motor torque, flags (limit 1, motor 2, collideConnected 4,
                          // kind << 8), -, -
     float4* j_imp;       // impulse x, y, z, angular motor impulse
@@ -274,7 +275,7 @@ enum : std::uint32_t { kJointLimit = 1u, kJointMotor = 2u, kJointCollideConnecte
 // Pi as the float build's Real (RealConstants.hpp:57).
 constexpr float kPiF = 3.14159265358979323846f;
 // Joint kinds, in bits 8-15 of a joint's flags.
-enum : std::uint32_t { kJointRevolute = 0u, kJointDistance = 1u, kJointWeld = 2u };
+enum : std::uint32_t { kJointRevolute = 0u, kJointDistance = 1u, kJointWeld = 2u, kJointRope = 3u, kJointFriction = 4u };
 // Constraint records staged through LDS per chunk by the wave-wide sweeps;
 // the stage sits at the end of the dynamic LDS.
 constexpr int kStageRecords = 16;

@@ -50,12 +50,14 @@ struct BodySpec {
 
 // Joint between bodies a and b with local anchors: kind 0 RevoluteJointConf,
 // 1 DistanceJointConf (length, frequency, dampingRatio), 2 WeldJointConf
-// (referenceAngle, frequency, dampingRatio).
-enum JointKind : int { kRevolute = 0, kDistance = 1, kWeld = 2 };
+// (referenceAngle, frequency, dampingRatio), 3 RopeJointConf (length is the
+// maximum length), 4 FrictionJointConf (maxForce, maxTorque).
+enum JointKind : int { kRevolute = 0, kDistance = 1, kWeld = 2, kRope = 3, kFriction = 4 };
 struct JointSpec {
     int kind = kRevolute;
     int a = 0, b = 0;
     float length = 1.0f, frequency = 0.0f, dampingRatio = 0.0f;
+    float maxForce = 0.0f, maxTorque = 0.0f;
     float lax = 0, lay = 0, lbx = 0, lby = 0;
     float referenceAngle = 0;
     bool enableLimit = false;
@@ -434,6 +436,17 @@ inline SceneSpec Joints(std::uint32_t seed, int w, int n = 8, int falling = 24) 
             wj.lbx = -0.4f;
             sc.joints.push_back(wj);
         }
+        // Top-down friction against the ground (which the box also touches).
+        JointSpec fj;
+        fj.kind = kFriction;
+        fj.a = 0;
+        fj.b = int(sc.bodies.size()) - 1;
+        fj.lax = b.x;
+        fj.lay = b.y;
+        fj.maxForce = 1.5f + 0.5f * float(k);
+        fj.maxTorque = 0.25f * float(k);
+        fj.collideConnected = true;
+        sc.joints.push_back(fj);
     }
     for (int k = 0; k < falling; ++k) {
         BodySpec b;
@@ -452,6 +465,17 @@ inline SceneSpec Joints(std::uint32_t seed, int w, int n = 8, int falling = 24) 
         }
         b.shapes = {int(sc.shapes.size()) - 1};
         sc.bodies.push_back(b);
+        if (k % 3 == 0) {
+            // A rope from the bar, slack at first, taut once the body has
+            // fallen 2 to 6 m below the bar.
+            JointSpec rj;
+            rj.kind = kRope;
+            rj.a = 1;
+            rj.b = int(sc.bodies.size()) - 1;
+            rj.lax = b.x;
+            rj.length = (b.y - 30.0f) + 2.0f + float(k % 5);
+            sc.joints.push_back(rj);
+        }
     }
     return sc;
 }

@@ -1267,7 +1267,7 @@ __device__ __forceinline__ bool joint_solve_position(const float4* u, float4& po
 }
 
 // ---------------------------------------------------------------------------
-// Distance and weld joints.  Their records share the revolute record's slots
+// Distance, weld, rope and friction joints.  Their records share the revolute record's slots
 // u[0] (rA, rB), u[3] (impulses), u[4] (inverse masses), u[5].zw (bodies),
 // u[6] (anchors less local centers) and u[7].w (kind); u[2].w holds the flags
 // with limit state 0.
@@ -1383,15 +1383,142 @@ __device__ __forceinline__ void build_jc_weld(float4* u, const float4 cf, float4
     u[7] = float4{cf.x, cf.y, 0.0f, __uint_as_float(prx::kJointWeld)};
 }
 
+// Rope record: u[1] direction x, y, mass, length; u[2].w flags | limit << 8;
+// u[5].x 1 / dt, u[5].y 1 when too short to act; u[7].z maximum length;
+// impulse in u[3].x.
+// InitVelocity(RopeJointConf&) less its warm start (RopeJointConf.cpp:59-113).
+__device__ __forceinline__ void build_jc_rope(float4* u, const float4 cf, float4 im, V2 rA, V2 rB, const float4& pA,
+                                              const float4& pB, float mA, float iA, float mB, float iB,
+                                              std::uint32_t flags, const StepParams& c) {
+    float length;
+    const V2 uv = unit_polar((V2{pB.x, pB.y} + rB) - (V2{pA.x, pA.y} + rA), length);
+    const float C = length - cf.x;
+    const std::uint32_t limit = (C > 0.0f) ? 2u : 0u;
+    V2 dir{0.0f, 0.0f};
+    float mass = 0.0f;
+    if (length > c.linearSlop) {
+        dir = uv;
+        const float crA = cross(rA, dir) / 1.0f;
+        const float crB = cross(rB, dir) / 1.0f;
+        const float invMass = mA + mB + iA * (crA * crA) + iB * (crB * crB);
+        mass = (invMass != 0.0f) ? 1.0f / invMass : 0.0f;
+        im = c.warmStart ? float4{im.x * c.dtRatio, 0.0f, 0.0f, 0.0f} : float4{0.0f, 0.0f, 0.0f, 0.0f};
+    } else {
+        im = float4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    u[1] = float4{dir.x, dir.y, mass, length};
+    u[2] = float4{0.0f, 0.0f, 0.0f, __uint_as_float(flags | (limit << 8))};
+    u[3] = im;
+    u[5].x = (c.dt != 0.0f) ? 1.0f / c.dt : 0.0f;
+    u[5].y = (length > c.linearSlop) ? 0.0f : 1.0f;  // 1: InitVelocity returned early, no warm start
+    u[7] = float4{0.0f, 0.0f, cf.x, __uint_as_float(prx::kJointRope)};
+}
+
+// Friction record: u[1] linear mass (Invert of the 2x2 K) m00, m01, m10, m11;
+// u[2].x angular mass; u[5].xy dt * maximum force, dt * maximum torque;
+// impulses linear x, y, angular in u[3].
+// InitVelocity(FrictionJointConf&) less its warm start (FrictionJointConf.cpp:76-123).
+__device__ __forceinline__ void build_jc_friction(float4* u, const float4 cf, float4 im, V2 rA, V2 rB, float mA,
+                                                  float iA, float mB, float iB, std::uint32_t flags,
+                                                  const StepParams& c) {
+    const float exx = mA + iA * (rA.y * rA.y) / 1.0f + mB + iB * (rB.y * rB.y) / 1.0f;
+    const float exy = -iA * rA.x * rA.y / 1.0f + -iB * rB.x * rB.y / 1.0f;
+    const float eyy = mA + iA * (rA.x * rA.x) / 1.0f + mB + iB * (rB.x * rB.x) / 1.0f;
+    // Invert(Matrix22) (Math.hpp:429-442).
+    float4 lm{0.0f, 0.0f, 0.0f, 0.0f};
+    const float cp = cross(V2{exx, exy}, V2{exy, eyy});
+    if (!almost_zero(cp)) {
+        const float inverse = 1.0f / cp;
+        lm = float4{eyy * inverse, -exy * inverse, -exy * inverse, exx * inverse};
+    }
+    const float invRotI = iA + iB;
+    const float angularMass = (invRotI > 0.0f) ? 1.0f / invRotI : 0.0f;
+    im = c.warmStart ? float4{im.x * c.dtRatio, im.y * c.dtRatio, im.z * c.dtRatio, 0.0f}
+                     : float4{0.0f, 0.0f, 0.0f, 0.0f};
+    u[1] = lm;
+    u[2] = float4{angularMass, 0.0f, 0.0f, __uint_as_float(flags)};
+    u[3] = im;
+    u[5].x = c.dt * cf.x;
+    u[5].y = c.dt * cf.y;
+    u[7] = float4{0.0f, 0.0f, 0.0f, __uint_as_float(prx::kJointFriction)};
+}
+
+// SolveVelocity(RopeJointConf&) (RopeJointConf.cpp:130-167): returns whether
+// the impulse did not change.
+__device__ __forceinline__ bool rope_solve_velocity(float4* u, float4& velA, float4& velB) {
+    const float4 r = u[0], m = u[1], ms = u[4];
+    const V2 rA{r.x, r.y}, rB{r.z, r.w}, dir{m.x, m.y};
+    const V2 vpA = V2{velA.x, velA.y} + mul(revperp(rA), velA.z / 1.0f);
+    const V2 vpB = V2{velB.x, velB.y} + mul(revperp(rB), velB.z / 1.0f);
+    const float C = m.w - u[7].z;
+    const float Cdot = dot(dir, vpB - vpA) + ((C < 0.0f) ? u[5].x * C : 0.0f);
+    float local = -m.z * Cdot;
+    const float old = u[3].x;
+    const float acc = fmin_std(0.0f, old + local);
+    u[3].x = acc;
+    local = acc - old;
+    const V2 P = mul(local, dir);
+    apply_joint_impulse(ms, P, cross(rA, P) / 1.0f, cross(rB, P) / 1.0f, velA, velB);
+    return local == 0.0f;
+}
+
+// SolveVelocity(FrictionJointConf&) (FrictionJointConf.cpp:141-231): returns
+// whether neither impulse changed.
+__device__ __forceinline__ bool friction_solve_velocity(float4* u, float4& velA, float4& velB) {
+    const float4 r = u[0], lm = u[1], ms = u[4], mx = u[5];
+    const V2 rA{r.x, r.y}, rB{r.z, r.w};
+    float4 im = u[3];
+    bool solved = true;
+    {
+        const float angularImpulse = -u[2].x * (velB.z - velA.z);
+        const float old = im.z;
+        im.z = clamp_std(im.z + angularImpulse, -mx.y, mx.y);
+        const float inc = im.z - old;
+        if (inc != 0.0f) solved = false;
+        velA.z = velA.z - ms.y * inc;
+        velB.z = velB.z + ms.w * inc;
+    }
+    {
+        const V2 vb = V2{velB.x, velB.y} + mul(revperp(rB), velB.z / 1.0f);
+        const V2 va = V2{velA.x, velA.y} + mul(revperp(rA), velA.z / 1.0f);
+        const V2 dv = vb - va;
+        // Transform(v, Mat22), each row dotted from zero (Vector.hpp:455-470).
+        float tx = 0.0f, ty = 0.0f;
+        tx += lm.x * dv.x;
+        tx += lm.y * dv.y;
+        ty += lm.z * dv.x;
+        ty += lm.w * dv.y;
+        const V2 old{im.x, im.y};
+        V2 li{old.x + -tx, old.y + -ty};
+        const float maxImpulse = mx.x;
+        if (magsq(li) > maxImpulse * maxImpulse) {
+            const V2 dir = unit_vector(li, V2{0.0f, 0.0f});
+            li = V2{dir.x * maxImpulse, dir.y * maxImpulse};
+        }
+        im.x = li.x;
+        im.y = li.y;
+        const V2 inc = li - old;
+        if (inc.x != 0.0f || inc.y != 0.0f) solved = false;
+        apply_joint_impulse(ms, inc, cross(rA, inc) / 1.0f, cross(rB, inc) / 1.0f, velA, velB);
+    }
+    u[3] = im;
+    return solved;
+}
+
 // Warm start of a distance or weld joint (DistanceJointConf.cpp:143-158,
-// WeldJointConf.cpp:197-210).
+// WeldJointConf.cpp:197-210), a rope (RopeJointConf.cpp:114-125) or a friction
+// joint (FrictionJointConf.cpp:124-134).
 __device__ __forceinline__ void spring_joint_warm_start(const float4* u, std::uint32_t kind, float4& velA,
                                                         float4& velB) {
     const float4 r = u[0], im = u[3], ms = u[4];
     const V2 rA{r.x, r.y}, rB{r.z, r.w};
-    if (kind == prx::kJointDistance) {
+    if (kind == prx::kJointRope && u[5].y != 0.0f) return;
+    if (kind == prx::kJointDistance || kind == prx::kJointRope) {
         const V2 P = mul(im.x, V2{u[1].x, u[1].y});
         apply_joint_impulse(ms, P, cross(rA, P) / 1.0f, cross(rB, P) / 1.0f, velA, velB);
+    } else if (kind == prx::kJointFriction) {
+        const V2 P{im.x, im.y};
+        apply_joint_impulse(ms, P, cross(rA, P) / 1.0f + im.z, cross(rB, P) / 1.0f + im.z, velA, velB);
     } else {
         const V2 P{im.x, im.y};
         apply_joint_impulse(ms, P, im.z + cross(rA, P) / 1.0f, im.z + cross(rB, P) / 1.0f, velA, velB);
@@ -1469,6 +1596,7 @@ __device__ __forceinline__ bool spring_joint_solve_position(const float4* u, std
                                                             float4& posB, const StepParams& c) {
     const float4 ms = u[4], la = u[6], cf = u[7];
     const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
+    if (kind == prx::kJointFriction) return true;  // FrictionJointConf.cpp:233-237
     if (kind == prx::kJointDistance && cf.y > 0.0f) return true;
     const V2 rA = rotate(V2{la.x, la.y}, unit_from_angle(posA.z));
     const V2 rB = rotate(V2{la.z, la.w}, unit_from_angle(posB.z));
@@ -1476,7 +1604,17 @@ __device__ __forceinline__ bool spring_joint_solve_position(const float4* u, std
     V2 P;
     float LA, LB;
     bool ok;
-    if (kind == prx::kJointDistance) {
+    if (kind == prx::kJointRope) {
+        // RopeJointConf.cpp:169-212.
+        float length;
+        const V2 dir = unit_polar(d, length);
+        const float C = clamp_std(length - cf.z, 0.0f, c.maxLinearCorrection);
+        const float impulse = -u[1].z * C;
+        P = mul(impulse, dir);
+        LA = cross(rA, P) / 1.0f;
+        LB = cross(rB, P) / 1.0f;
+        ok = (length - cf.z) < c.linearSlop;
+    } else if (kind == prx::kJointDistance) {
         float length;
         const V2 dir = unit_polar(d, length);
         const float C = clamp_std(length - cf.z, -c.maxLinearCorrection, c.maxLinearCorrection);
@@ -1550,7 +1688,9 @@ __device__ __forceinline__ void build_jc(const DeviceState& d, const WorldCtx& x
         u[5] = float4{0.0f, 0.0f, __int_as_float(bA), __int_as_float(bB)};
         u[6] = float4{laA.x, laA.y, laB.x, laB.y};
         if (kind == prx::kJointDistance) build_jc_distance(u, cf, im, rA, rB, pA, pB, mA, iA, mB, iB, c);
-        else build_jc_weld(u, cf, im, rA, rB, pA, pB, mA, iA, mB, iB, flags & 0xffu, c);
+        else if (kind == prx::kJointWeld) build_jc_weld(u, cf, im, rA, rB, pA, pB, mA, iA, mB, iB, flags & 0xffu, c);
+        else if (kind == prx::kJointRope) build_jc_rope(u, cf, im, rA, rB, pA, pB, mA, iA, mB, iB, flags & 0xffu, c);
+        else build_jc_friction(u, cf, im, rA, rB, mA, iA, mB, iB, flags & 0xffu, c);
         return;
     }
     const float totInvI = iA + iB;
@@ -1999,7 +2139,9 @@ __device__ __forceinline__ bool joints_velocity(float4* jbase, int j0, int j1, c
             float4 vb = bs.vel(bB);
             if (warm) spring_joint_warm_start(u, kind, va, vb);
             else if (kind == prx::kJointDistance) ok &= distance_solve_velocity(u, va, vb);
-            else ok &= weld_solve_velocity(u, va, vb);
+            else if (kind == prx::kJointWeld) ok &= weld_solve_velocity(u, va, vb);
+            else if (kind == prx::kJointRope) ok &= rope_solve_velocity(u, va, vb);
+            else ok &= friction_solve_velocity(u, va, vb);
             bs.set_vel(bA, va);
             bs.set_vel(bB, vb);
             continue;

@@ -172,30 +172,42 @@ def test_contact_event_recording_switch_host_side():
     w.set_contact_listener()  # clearing with nothing set is allowed
 
 
-def test_distance_and_weld_joint_creation_matches_scene():
-    """Distance and weld joints created through prx_distance_joint_create /
-    prx_weld_joint_create give the same world description as the scene the
-    reference built (kind in bits 8-15 of the flags, then length or
-    reference angle, frequency, damping ratio)."""
+def test_distance_weld_rope_friction_joint_creation_matches_scene():
+    """Distance, weld, rope and friction joints created through
+    prx_distance_joint_create / prx_weld_joint_create / prx_rope_joint_create /
+    prx_friction_joint_create give the same world description as the scene
+    the reference built (kind in bits 8-15 of the flags, then length or
+    reference angle, frequency, damping ratio; maximum length; maximum force
+    and torque)."""
     g = golden_cases.load("joints0")
     desc = _desc_only(g["desc_conf"])
     recs = dumpfmt.read_description(dumpfmt.Reader(desc))[2]
     kinds = {r["flags"] >> 8 for r in recs}
-    assert kinds == {1, 2}
+    assert kinds == {1, 2, 3, 4}
     offs = pa.desc_joint_offsets(desc)
     bare = desc[:offs[0] - 4] + struct.pack("<I", 0)
     w = pa.World.from_desc(bare)
     for r in recs:
         v = [float(x) for x in r["vals"]]
-        common = dict(body_a=r["a"], body_b=r["b"], local_anchor_a=(v[0], v[1]), local_anchor_b=(v[2], v[3]),
-                      frequency=v[5], damping_ratio=v[6], collide_connected=(r["flags"] >> 2) & 1)
-        if r["flags"] >> 8 == 1:
+        base = dict(body_a=r["a"], body_b=r["b"], local_anchor_a=(v[0], v[1]), local_anchor_b=(v[2], v[3]),
+                    collide_connected=(r["flags"] >> 2) & 1)
+        common = dict(frequency=v[5], damping_ratio=v[6], **base)
+        kind = r["flags"] >> 8
+        if kind == 1:
             w.create_joint(pa.DistanceJointConf(length=v[4], **common))
-        else:
+        elif kind == 2:
             w.create_joint(pa.WeldJointConf(reference_angle=v[4], **common))
+        elif kind == 3:
+            w.create_joint(pa.RopeJointConf(max_length=v[4], **base))
+        else:
+            w.create_joint(pa.FrictionJointConf(max_force=v[4], max_torque=v[5], **base))
     assert w.joint_count == len(recs)
     assert w.save_desc() == desc
     with pytest.raises(pa.PrxError) as e:
         w.create_joint(pa.DistanceJointConf(body_a=1, body_b=2, frequency=-1.0))
     assert e.value.code == 1
+    with pytest.raises(pa.PrxError) as e:
+        w.create_joint(pa.FrictionJointConf(body_a=1, body_b=2, max_torque=-1.0))
+    assert e.value.code == 1
     assert pa.DistanceJointConf().length == 1.0
+    assert tuple(pa.RopeJointConf().local_anchor_a) == (-1.0, 0.0)
