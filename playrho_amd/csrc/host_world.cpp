@@ -361,6 +361,7 @@ struct prx_batch {
     hipGraph_t graph = nullptr;
     prx::StepParams graph_params{};
     bool graph_valid = false;
+    bool joint_kinds = false;  // some joint is not revolute
     std::size_t totB = 0, totP = 0, totS = 0, totV = 0, totC = 0, totJ = 0, totE = 0;
     int lds_bytes = 16 * 1024;
     int lds_min = 16 * 1024;
@@ -847,6 +848,7 @@ int UploadBatch(prx_batch* b) {
     std::vector<int2> jbody(J), bjoint(B), bjlist(2 * J);
     std::vector<float4> janchor(J), jcfg(J), jcfg2(J), jimp(J);
     std::vector<std::uint32_t> jlimit(J);
+    b->joint_kinds = false;
     for (std::size_t wi = 0; wi < b->worlds.size(); ++wi) {
         const prx_world* w = b->worlds[wi];
         const auto& h = b->hdr[wi];
@@ -862,6 +864,7 @@ int UploadBatch(prx_batch* b) {
                 float fl;
                 std::memcpy(&fl, &j.flags, 4);
                 jcfg2[g] = float4{j.maxTorque, fl, 0.0f, 0.0f};
+                if (j.flags >> 8) b->joint_kinds = true;
                 jimp[g] = float4{j.imp[0], j.imp[1], j.imp[2], j.imp[3]};
                 jlimit[g] = j.limit;
                 lists[j.a].push_back(int2{int(ji), int(j.b)});
@@ -1117,6 +1120,7 @@ int StepBatch(prx_batch* b, const prx_step_conf* conf, int nsteps) {
     prx::StepParams p = ToParams(*conf);
     p.ldsBytes = std::uint32_t(b->lds_bytes);
     p.coopMin = b->conf.coop_min_contacts;
+    p.jointKinds = b->joint_kinds ? 1u : 0u;
     const int nw = int(b->worlds.size());
     PRX_HIP(hipEventRecord(b->ev0, b->stream));
     if (b->conf.use_graph && nsteps > 1) {

@@ -91,6 +91,7 @@ struct StepParams {
     std::uint32_t coopMin;   // islands with at least this many contacts are swept by the whole wave
     // TOI phase (doToi).
     std::uint32_t doToi;
+    std::uint32_t jointKinds;  // 1: some joint of the batch is not revolute (kind dispatch in the joint sweeps)
     float toiResolutionRate, toiMinSeparation, toiMinMomentum, targetDepth, tolerance;
     std::uint32_t toiVelIters, toiPosIters, maxToiRootIters, maxToiIters, maxDistanceIters, maxSubSteps;
 };

@@ -1269,13 +1269,13 @@ __device__ __forceinline__ bool joint_solve_position(const float4* u, float4& po
 // ---------------------------------------------------------------------------
 // Distance, weld, rope and friction joints.  Their records share the revolute record's slots
 // u[0] (rA, rB), u[3] (impulses), u[4] (inverse masses), u[5].zw (bodies),
-// u[6] (anchors less local centers) and u[7].w (kind); u[2].w holds the flags
-// with limit state 0.
+// u[6] (anchors less local centers); u[2].w holds flags | limit state << 8 |
+// kind << 16 (the revolute record's word, kind 0), limit state 0 but for ropes.
 //   distance: u[1] direction x, y, mass, invGamma   u[5].x bias
 //             u[7] -, frequency, length, kind        impulse in u[3].x
 //   weld:     u[1] mass exx, eyx, ezx, eyy           u[2] ezy, ezz, -, flags
 //             u[5].xy gamma, bias                    u[7] reference angle, frequency, -, kind
-__device__ __forceinline__ std::uint32_t jc_kind(const float4* u) { return __float_as_uint(u[7].w); }
+__device__ __forceinline__ std::uint32_t jc_kind(const float4* u) { return __float_as_uint(u[2].w) >> 16; }
 
 __device__ __forceinline__ void apply_joint_impulse(const float4 ms, V2 P, float LA, float LB, float4& velA,
                                                     float4& velB) {
@@ -1376,14 +1376,14 @@ __device__ __forceinline__ void build_jc_weld(float4* u, const float4 cf, float4
     im = c.warmStart ? float4{im.x * c.dtRatio, im.y * c.dtRatio, im.z * c.dtRatio, 0.0f}
                      : float4{0.0f, 0.0f, 0.0f, 0.0f};
     u[1] = float4{m.exx, m.eyx, m.ezx, m.eyy};
-    u[2] = float4{m.ezy, m.ezz, 0.0f, __uint_as_float(flags)};
+    u[2] = float4{m.ezy, m.ezz, 0.0f, __uint_as_float(flags | (prx::kJointWeld << 16))};
     u[3] = im;
     u[5].x = gamma;
     u[5].y = bias;
     u[7] = float4{cf.x, cf.y, 0.0f, __uint_as_float(prx::kJointWeld)};
 }
 
-// Rope record: u[1] direction x, y, mass, length; u[2].w flags | limit << 8;
+// Rope record: u[1] direction x, y, mass, length; u[2].w flags | limit << 8 | kind << 16;
 // u[5].x 1 / dt, u[5].y 1 when too short to act; u[7].z maximum length;
 // impulse in u[3].x.
 // InitVelocity(RopeJointConf&) less its warm start (RopeJointConf.cpp:59-113).
@@ -1407,7 +1407,7 @@ __device__ __forceinline__ void build_jc_rope(float4* u, const float4 cf, float4
         im = float4{0.0f, 0.0f, 0.0f, 0.0f};
     }
     u[1] = float4{dir.x, dir.y, mass, length};
-    u[2] = float4{0.0f, 0.0f, 0.0f, __uint_as_float(flags | (limit << 8))};
+    u[2] = float4{0.0f, 0.0f, 0.0f, __uint_as_float(flags | (limit << 8) | (prx::kJointRope << 16))};
     u[3] = im;
     u[5].x = (c.dt != 0.0f) ? 1.0f / c.dt : 0.0f;
     u[5].y = (length > c.linearSlop) ? 0.0f : 1.0f;  // 1: InitVelocity returned early, no warm start
@@ -1436,7 +1436,7 @@ __device__ __forceinline__ void build_jc_friction(float4* u, const float4 cf, fl
     im = c.warmStart ? float4{im.x * c.dtRatio, im.y * c.dtRatio, im.z * c.dtRatio, 0.0f}
                      : float4{0.0f, 0.0f, 0.0f, 0.0f};
     u[1] = lm;
-    u[2] = float4{angularMass, 0.0f, 0.0f, __uint_as_float(flags)};
+    u[2] = float4{angularMass, 0.0f, 0.0f, __uint_as_float(flags | (prx::kJointFriction << 16))};
     u[3] = im;
     u[5].x = c.dt * cf.x;
     u[5].y = c.dt * cf.y;
@@ -1683,7 +1683,7 @@ __device__ __forceinline__ void build_jc(const DeviceState& d, const WorldCtx& x
     const std::uint32_t kind = flags >> 8;
     if (kind != prx::kJointRevolute) {
         u[0] = float4{rA.x, rA.y, rB.x, rB.y};
-        u[2] = float4{0.0f, 0.0f, 0.0f, __uint_as_float(flags & 0xffu)};
+        u[2] = float4{0.0f, 0.0f, 0.0f, __uint_as_float((flags & 0xffu) | (kind << 16))};
         u[4] = float4{mA, iA, mB, iB};
         u[5] = float4{0.0f, 0.0f, __int_as_float(bA), __int_as_float(bB)};
         u[6] = float4{laA.x, laA.y, laB.x, laB.y};
@@ -2126,12 +2126,12 @@ __device__ __forceinline__ float solve_velocity_one(Vc& vc, const BodyC& A, cons
 // One pass over an island's joints in island order: the warm start of
 // InitVelocity, or SolveVelocity (returns whether every joint left its
 // bodies' velocities unchanged).
-template <bool kLds>
-__device__ __forceinline__ bool joints_velocity(float4* jbase, int j0, int j1, const Bodies<kLds>& bs, bool warm) {
+template <bool kLds, bool kKinds>
+__device__ __forceinline__ bool joints_velocity_k(float4* jbase, int j0, int j1, const Bodies<kLds>& bs, bool warm) {
     bool ok = true;
     for (int e = j0; e < j1; ++e) {
         float4* u = jbase + std::size_t(e) * prx::kJcStride;
-        const std::uint32_t kind = jc_kind(u);
+        const std::uint32_t kind = kKinds ? jc_kind(u) : prx::kJointRevolute;
         if (kind != prx::kJointRevolute) {
             const float4 mo = u[5];
             const int bA = __float_as_int(mo.z), bB = __float_as_int(mo.w);
@@ -2161,6 +2161,15 @@ __device__ __forceinline__ bool joints_velocity(float4* jbase, int j0, int j1, c
     return ok;
 }
 
+// Batches whose joints are all revolute take the loop without the kind
+// dispatch (StepParams::jointKinds 0).
+template <bool kLds>
+__device__ __forceinline__ bool joints_velocity(float4* jbase, int j0, int j1, const Bodies<kLds>& bs, bool warm,
+                                                const StepParams& c) {
+    return c.jointKinds ? joints_velocity_k<kLds, true>(jbase, j0, j1, bs, warm)
+                        : joints_velocity_k<kLds, false>(jbase, j0, j1, bs, warm);
+}
+
 // The same pass recording each joint's inputs (velocities of both bodies
 // and the impulses before the update) for the adjoint.
 template <bool kLds>
@@ -2186,9 +2195,9 @@ __device__ __forceinline__ bool joints_velocity_tape(float4* jbase, int j0, int 
 
 // One SolvePosition pass over an island's joints; returns whether all are
 // within tolerance.
-template <bool kLds>
-__device__ __forceinline__ bool joints_position(const float4* jbase, int j0, int j1, const Bodies<kLds>& bs,
-                                                const StepParams& c) {
+template <bool kLds, bool kKinds>
+__device__ __forceinline__ bool joints_position_k(const float4* jbase, int j0, int j1, const Bodies<kLds>& bs,
+                                                  const StepParams& c) {
     bool ok = true;
     for (int e = j0; e < j1; ++e) {
         const float4* u = jbase + std::size_t(e) * prx::kJcStride;
@@ -2196,13 +2205,19 @@ __device__ __forceinline__ bool joints_position(const float4* jbase, int j0, int
         const int bA = __float_as_int(mo.z), bB = __float_as_int(mo.w);
         float4 pa = bs.pos(bA);
         float4 pb = bs.pos(bB);
-        const std::uint32_t kind = jc_kind(u);
+        const std::uint32_t kind = kKinds ? jc_kind(u) : prx::kJointRevolute;
         if (kind == prx::kJointRevolute) ok &= joint_solve_position(u, pa, pb, c);
         else ok &= spring_joint_solve_position(u, kind, pa, pb, c);
         bs.set_pos(bA, pa);
         bs.set_pos(bB, pb);
     }
     return ok;
+}
+template <bool kLds>
+__device__ __forceinline__ bool joints_position(const float4* jbase, int j0, int j1, const Bodies<kLds>& bs,
+                                                const StepParams& c) {
+    return c.jointKinds ? joints_position_k<kLds, true>(jbase, j0, j1, bs, c)
+                        : joints_position_k<kLds, false>(jbase, j0, j1, bs, c);
 }
 
 // The same pass recording the positions before each update.
@@ -2268,12 +2283,12 @@ __device__ __forceinline__ VelResult island_velocity(const DeviceState& d, const
             B = nB;
         }
     }
-    if (c.warmStart) joints_velocity(jbase, j0, j1, bs, true);
+    if (c.warmStart) joints_velocity(jbase, j0, j1, bs, true, c);
     for (std::uint32_t it = 0; it < c.velIters; ++it) {
         if constexpr (kTape) d.tape.isl[islandSlot].z = int(it) + 1;
         bool jointsOkay;
         if constexpr (kTape) jointsOkay = joints_velocity_tape(jbase, j0, j1, bs, d.tape, x.jo, int(it));
-        else jointsOkay = joints_velocity(jbase, j0, j1, bs, false);
+        else jointsOkay = joints_velocity(jbase, j0, j1, bs, false, c);
         float newInc = 0.0f;
         if (c0 >= c1) {
             if (jointsOkay && newInc <= c.regMinMomentum) {
@@ -2366,7 +2381,7 @@ __device__ __forceinline__ VelResult island_velocity_wave(const DeviceState& d, 
         const bool warm = c.warmStart && pass == 0;
         float newInc = 0.0f;
         bool jointsOkay = true;
-        if (!warm && x.lane == 0) jointsOkay = joints_velocity(jbase, j0, j1, bs, false);
+        if (!warm && x.lane == 0) jointsOkay = joints_velocity(jbase, j0, j1, bs, false, c);
         for (int ch = 0; ch < nch; ++ch) {
             const int nx = (ch + 1 < nch) ? ch + 1 : 0;
             const bool more = !resident && (ch + 1 < nch || pass + 1 < npass);
@@ -2405,7 +2420,7 @@ __device__ __forceinline__ VelResult island_velocity_wave(const DeviceState& d, 
                 __syncthreads();
             }
         }
-        if (warm && x.lane == 0) joints_velocity(jbase, j0, j1, bs, true);
+        if (warm && x.lane == 0) joints_velocity(jbase, j0, j1, bs, true, c);
         newInc = __shfl(newInc, 0);
         jointsOkay = __shfl(int(jointsOkay), 0) != 0;
         if (!warm) {
@@ -2988,7 +3003,7 @@ __device__ __forceinline__ SolveSummary solve_islands(const DeviceState& d, cons
         const int jg = x.jo + S.ijoint[e];
         const float4* u = d.s_jc + (std::size_t(x.jo) + std::size_t(e)) * prx::kJcStride;
         d.j_imp[jg] = u[3];
-        d.j_limit[jg] = __float_as_uint(u[2].w) >> 8;
+        d.j_limit[jg] = (__float_as_uint(u[2].w) >> 8) & 0xffu;
     }
     // Body writeback, under-active times (AabbTreeWorld.cpp:1652-1664, 403-417).
     for (int e = x.lane; e < nbe; e += kWave) {
