@@ -41,7 +41,7 @@ def parse():
     p.add_argument("--plain-steps", type=int, default=10)
     p.add_argument("--coop-min", type=int, default=0, help="0: library default")
     p.add_argument("--lds-bytes", type=int, default=0, help="0: library default")
-    p.add_argument("--autotune", type=int, default=10, metavar="K",
+    p.add_argument("--autotune", type=int, default=40, metavar="K",
                    help="steps per candidate schedule for prx_batch_autotune after warmup (0: off)")
     p.add_argument("--big-bodies", type=int, default=20000, help="large single world (0: skip)")
     p.add_argument("--big-steps", type=int, default=600)

@@ -360,7 +360,8 @@ int prx_batch_phase_cycles(prx_batch* batch, int enable, uint64_t* out, int max_
  * the wave-wide sweep threshold; launch order) on `nsteps` steps each,
  * starting every trial from the batch's current state, keeps the fastest in
  * the batch (and in *best when non-null) and restores the state: the
- * simulation is not advanced. */
+ * simulation is not advanced.  A candidate must beat the schedule in hand by
+ * 2 % to replace it. */
 int prx_batch_autotune(prx_batch* batch, const prx_step_conf* step, int nsteps,
                        prx_batch_conf* best);
 

@@ -2105,7 +2105,11 @@ int prx_batch_autotune(prx_batch* b, const prx_step_conf* step, int nsteps, prx_
         return CheckStatus(b);
     };
     // Coordinate search: workgroups per CU (LDS per workgroup), then the
-    // wave-wide sweep threshold, then the launch order.
+    // wave-wide sweep threshold, then the launch order.  A candidate replaces
+    // the current schedule only when at least 2 % faster: schedules that tie
+    // over the trial steps can part ways as the worlds evolve, and the one in
+    // hand is kept then.
+    constexpr float kSwitchMargin = 0.98f;
     prx_batch_conf cur = start;
     float bestMs = 0;
     if (int st = trial(cur, &bestMs)) return st;
@@ -2116,7 +2120,7 @@ int prx_batch_autotune(prx_batch* b, const prx_step_conf* step, int nsteps, prx_
         if (c.lds_bytes == cur.lds_bytes) continue;
         float ms = 0;
         if (int st = trial(c, &ms)) return st;
-        if (ms < bestMs) {
+        if (ms < bestMs * kSwitchMargin) {
             bestMs = ms;
             cur = c;
         }
@@ -2128,7 +2132,7 @@ int prx_batch_autotune(prx_batch* b, const prx_step_conf* step, int nsteps, prx_
         if (c.coop_min_contacts == cur.coop_min_contacts) continue;
         float ms = 0;
         if (int st = trial(c, &ms)) return st;
-        if (ms < bestMs) {
+        if (ms < bestMs * kSwitchMargin) {
             bestMs = ms;
             cur = c;
         }
@@ -2138,7 +2142,7 @@ int prx_batch_autotune(prx_batch* b, const prx_step_conf* step, int nsteps, prx_
         c.order = cur.order ? 0u : 1u;
         float ms = 0;
         if (int st = trial(c, &ms)) return st;
-        if (ms < bestMs) {
+        if (ms < bestMs * kSwitchMargin) {
             bestMs = ms;
             cur = c;
         }
