@@ -138,6 +138,17 @@ typedef struct prx_friction_joint_conf {
     uint32_t collide_connected;
 } prx_friction_joint_conf;
 
+/* MotorJointConf (d2/MotorJointConf.hpp:58-138): drives body B towards the
+ * linear and angular offsets from body A.  The reference's defaults are
+ * offsets 0, maximum force 1, maximum torque 1, correction factor 0.3;
+ * negative or NaN maxima are PRX_E_INVALID_ARGUMENT. */
+typedef struct prx_motor_joint_conf {
+    uint32_t body_a, body_b;
+    float linear_offset[2];
+    float angular_offset, max_force, max_torque, correction_factor;
+    uint32_t collide_connected;
+} prx_motor_joint_conf;
+
 /* StepConf fields the regular step reads (StepConf.hpp:49-392). */
 typedef struct prx_step_conf {
     float delta_time, dt_ratio, min_still_time_to_sleep, linear_slop, angular_slop;
@@ -220,6 +231,8 @@ int prx_weld_joint_create(prx_world* world, const prx_weld_joint_conf* conf, uin
  * (solvers RopeJointConf.cpp:59-212, FrictionJointConf.cpp:76-237). */
 int prx_rope_joint_create(prx_world* world, const prx_rope_joint_conf* conf, uint32_t* joint_id);
 int prx_friction_joint_create(prx_world* world, const prx_friction_joint_conf* conf, uint32_t* joint_id);
+/* CreateJoint(World&, const MotorJointConf&) (solver MotorJointConf.cpp:79-234). */
+int prx_motor_joint_create(prx_world* world, const prx_motor_joint_conf* conf, uint32_t* joint_id);
 uint32_t prx_world_joint_count(const prx_world* world);
 /* Builds a named workload scene through the calls above. */
 int prx_scene_build(prx_world* world, const char* name, const int32_t* params, int nparams);

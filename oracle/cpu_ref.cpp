@@ -119,16 +119,24 @@ void Load(Handle<R>& h, In& in) {
         if (j.kind != 0) {
             // Distance: length, frequency, damping ratio.  Weld: reference
             // angle, frequency, damping ratio.  Rope: maximum length.
-            // Friction: maximum force, maximum torque.
+            // Friction: maximum force, maximum torque.  Motor: linear offset,
+            // -, angular offset, maximum force, maximum torque, correction
+            // factor.
             j.enableLimit = j.enableMotor = false;
             if (j.kind == 1 || j.kind == 3) j.length = f[4];
-            if (j.kind != 2) j.referenceAngle = 0;
+            if (j.kind != 2 && j.kind != 5) j.referenceAngle = 0;
             if (j.kind <= 2) {
                 j.frequency = f[5];
                 j.dampingRatio = f[6];
             } else if (j.kind == 4) {
                 j.maxForce = f[4];
                 j.maxTorque = f[5];
+            } else if (j.kind == 5) {
+                // Motor: linear offset in laA, angular offset in referenceAngle.
+                j.laB = V2<R>{R(0), R(0)};
+                j.maxForce = f[5];
+                j.maxTorque = f[6];
+                j.correctionFactor = f[7];
             }
             j.lower = j.upper = j.motorSpeed = j.maxMotorTorque = 0;
         }

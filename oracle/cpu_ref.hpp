@@ -791,6 +791,10 @@ struct Joint {
     // angular impulse in impulse.z.
     R curLength = 0, maxForce = 0, maxTorque = 0;
     R lm00 = 0, lm01 = 0, lm10 = 0, lm11 = 0;
+    // kind 5 motor: linear offset in laA, angular offset in referenceAngle,
+    // impulses as friction.
+    R correctionFactor = 0, angularError = 0;
+    V2<R> linearError;
 };
 
 template <class R>
@@ -1918,12 +1922,114 @@ public:
         return solved;
     }
 
+    // InitVelocity(MotorJointConf&) (MotorJointConf.cpp:79-158).
+    static void MotorInitVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        const R mA = A.invMass, iA = A.invRotI, mB = B.invMass, iB = B.invRotI;
+        Vel<R> velA = A.vel, velB = B.vel;
+        j.rA = Rotate(j.laA - A.lc, UnitFromAngle(A.pos.ang));
+        j.rB = Rotate(-B.lc, UnitFromAngle(B.pos.ang));
+        const V2<R> rA = j.rA, rB = j.rB;
+        const R exx = mA + mB + iA * (rA.y * rA.y) / R(1) + iB * (rB.y * rB.y) / R(1);
+        const R exy = -iA * rA.x * rA.y / R(1) + -iB * rB.x * rB.y / R(1);
+        const R eyy = mA + mB + iA * (rA.x * rA.x) / R(1) + iB * (rB.x * rB.x) / R(1);
+        const R cp = Cross(V2<R>{exx, exy}, V2<R>{exy, eyy});
+        if (!AlmostZero(cp)) {
+            const R inverse = R(1) / cp;
+            j.lm00 = eyy * inverse;
+            j.lm01 = -exy * inverse;
+            j.lm10 = -exy * inverse;
+            j.lm11 = exx * inverse;
+        } else {
+            j.lm00 = j.lm01 = j.lm10 = j.lm11 = 0;
+        }
+        const R invRotI = iA + iB;
+        j.angularMass = (invRotI > R(0)) ? R(1) / invRotI : R(0);
+        j.linearError = (B.pos.lin + rB) - (A.pos.lin + rA);
+        j.angularError = (B.pos.ang - A.pos.ang) - j.referenceAngle;
+        if (conf.warmStart) {
+            j.impulse.x *= conf.dtRatio;
+            j.impulse.y *= conf.dtRatio;
+            j.impulse.z *= conf.dtRatio;
+            const V2<R> P{j.impulse.x, j.impulse.y};
+            ApplyImpulse(A, B, P, Cross(rA, P) / R(1) + j.impulse.z, Cross(rB, P) / R(1) + j.impulse.z, velA, velB);
+        } else {
+            j.impulse = V3<R>{};
+        }
+        A.vel = velA;
+        B.vel = velB;
+    }
+
+    // SolveVelocity(MotorJointConf&) (MotorJointConf.cpp:160-229).
+    static bool MotorSolveVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        Vel<R> velA = A.vel, velB = B.vel;
+        const R mA = A.invMass, iA = A.invRotI, mB = B.invMass, iB = B.invRotI;
+        const R h = conf.dt;
+        const R invH = (h != R(0)) ? R(1) / h : R(0);
+        bool solved = true;
+        {
+            const R Cdot = (velB.ang - velA.ang) + invH * j.correctionFactor * j.angularError;
+            const R angularImpulse = -j.angularMass * Cdot;
+            const R old = j.impulse.z;
+            const R maxAngular = h * j.maxTorque;
+            const R now = Clamp(old + angularImpulse, -maxAngular, maxAngular);
+            j.impulse.z = now;
+            const R inc = now - old;
+            if (inc != R(0)) {
+                // Velocity{LinearVelocity2{}, ...}: the linear parts take a zero.
+                velA.lin = velA.lin - V2<R>{R(0), R(0)};
+                velA.ang = velA.ang - iA * inc;
+                velB.lin = velB.lin + V2<R>{R(0), R(0)};
+                velB.ang = velB.ang + iB * inc;
+                solved = false;
+            }
+        }
+        {
+            const V2<R> vb = velB.lin + RevPerp(j.rB) * (velB.ang / R(1));
+            const V2<R> va = velA.lin + RevPerp(j.rA) * (velA.ang / R(1));
+            const R k = invH * j.correctionFactor;
+            const V2<R> Cdot = (vb - va) + V2<R>{k * j.linearError.x, k * j.linearError.y};
+            R tx = 0, ty = 0;
+            tx += j.lm00 * Cdot.x;
+            tx += j.lm01 * Cdot.y;
+            ty += j.lm10 * Cdot.x;
+            ty += j.lm11 * Cdot.y;
+            const V2<R> old{j.impulse.x, j.impulse.y};
+            V2<R> li{old.x + -tx, old.y + -ty};
+            const R maxImpulse = h * j.maxForce;
+            if (MagSq(li) > maxImpulse * maxImpulse) {
+                const V2<R> dir = UnitVector(li, V2<R>{R(0), R(0)});
+                li = V2<R>{dir.x * maxImpulse, dir.y * maxImpulse};
+            }
+            j.impulse.x = li.x;
+            j.impulse.y = li.y;
+            const V2<R> inc = li - old;
+            const R angA = Cross(j.rA, inc) / R(1);
+            const R angB = Cross(j.rB, inc) / R(1);
+            if (inc.x != R(0) || inc.y != R(0) || angA != R(0) || angB != R(0)) {
+                velA.lin = velA.lin - mA * inc;
+                velA.ang = velA.ang - iA * angA;
+                velB.lin = velB.lin + mB * inc;
+                velB.ang = velB.ang + iB * angB;
+                solved = false;
+            }
+        }
+        if (solved) return true;
+        A.vel = velA;
+        B.vel = velB;
+        return false;
+    }
+
     // InitVelocity(RevoluteJointConf&) (RevoluteJointConf.cpp:127-226).
     static void JointInitVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
         if (j.kind == 1) return DistanceInitVelocity(j, bcs, conf);
         if (j.kind == 2) return WeldInitVelocity(j, bcs, conf);
         if (j.kind == 3) return RopeInitVelocity(j, bcs, conf);
         if (j.kind == 4) return FrictionInitVelocity(j, bcs, conf);
+        if (j.kind == 5) return MotorInitVelocity(j, bcs, conf);
         BodyConstraint<R>& A = bcs[j.bodyA];
         BodyConstraint<R>& B = bcs[j.bodyB];
         const R mA = A.invMass, iA = A.invRotI, aA = A.pos.ang;
@@ -1986,6 +2092,7 @@ public:
         if (j.kind == 2) return WeldSolveVelocity(j, bcs);
         if (j.kind == 3) return RopeSolveVelocity(j, bcs, conf);
         if (j.kind == 4) return FrictionSolveVelocity(j, bcs, conf);
+        if (j.kind == 5) return MotorSolveVelocity(j, bcs, conf);
         BodyConstraint<R>& A = bcs[j.bodyA];
         BodyConstraint<R>& B = bcs[j.bodyB];
         const Vel<R> oldVelA = A.vel, oldVelB = B.vel;
@@ -2061,6 +2168,7 @@ public:
         if (j.kind == 2) return WeldSolvePosition(j, bcs, conf);
         if (j.kind == 3) return RopeSolvePosition(j, bcs, conf);
         if (j.kind == 4) return true;  // FrictionJointConf.cpp:233-237
+        if (j.kind == 5) return true;  // MotorJointConf.cpp:231-234
         BodyConstraint<R>& A = bcs[j.bodyA];
         BodyConstraint<R>& B = bcs[j.bodyB];
         Pos<R> posA = A.pos, posB = B.pos;

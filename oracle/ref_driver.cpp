@@ -40,6 +40,7 @@
 #include <playrho/d2/WeldJointConf.hpp>
 #include <playrho/d2/RopeJointConf.hpp>
 #include <playrho/d2/FrictionJointConf.hpp>
+#include <playrho/d2/MotorJointConf.hpp>
 #include <playrho/d2/WorldJoint.hpp>
 #include <playrho/StepConf.hpp>
 
@@ -206,6 +207,13 @@ void WriteDescription(Out& o, const World& world) {
             o.u32((j->collideConnected ? 4u : 0u) | (4u << 8));
             const float f[9] = {GetX(j->localAnchorA), GetY(j->localAnchorA), GetX(j->localAnchorB),
                                 GetY(j->localAnchorB), j->maxForce, j->maxTorque, 0.0f, 0.0f, 0.0f};
+            o.put(f, sizeof(f));
+        } else if (const auto* j = TypeCast<const MotorJointConf>(&joint)) {
+            o.u32(to_underlying(GetBodyA(*j)));
+            o.u32(to_underlying(GetBodyB(*j)));
+            o.u32((j->collideConnected ? 4u : 0u) | (5u << 8));
+            const float f[9] = {GetX(j->linearOffset), GetY(j->linearOffset), 0.0f, 0.0f, j->angularOffset,
+                                j->maxForce, j->maxTorque, j->correctionFactor, 0.0f};
             o.put(f, sizeof(f));
         }
     }
@@ -408,6 +416,10 @@ void WriteState(Out& o, const World& world, std::uint32_t step, const StepStats&
             o.u32(0u);
             const float f[4] = {GetX(j->linearImpulse), GetY(j->linearImpulse), j->angularImpulse, 0.0f};
             o.put(f, sizeof(f));
+        } else if (const auto* j = TypeCast<const MotorJointConf>(&joint)) {
+            o.u32(0u);
+            const float f[4] = {GetX(j->linearImpulse), GetY(j->linearImpulse), j->angularImpulse, 0.0f};
+            o.put(f, sizeof(f));
         }
     }
     o.u32(g_events.n);
@@ -459,6 +471,11 @@ World BuildWorld(const prx_scene::SceneSpec& spec) {
         } else if (js.kind == prx_scene::kFriction) {
             auto jc = FrictionJointConf{ba, bb, Length2{js.lax, js.lay}, Length2{js.lbx, js.lby}};
             jc.UseMaxForce(js.maxForce).UseMaxTorque(js.maxTorque);
+            jc.UseCollideConnected(js.collideConnected);
+            CreateJoint(world, jc);
+        } else if (js.kind == prx_scene::kMotor) {
+            auto jc = MotorJointConf{ba, bb, Length2{js.lax, js.lay}, js.referenceAngle};
+            jc.UseMaxForce(js.maxForce).UseMaxTorque(js.maxTorque).UseCorrectionFactor(js.correctionFactor);
             jc.UseCollideConnected(js.collideConnected);
             CreateJoint(world, jc);
         } else if (js.kind == prx_scene::kWeld) {

@@ -189,6 +189,21 @@ class FrictionJointConf(ctypes.Structure):
                 ("collide_connected", ctypes.c_uint32)]
 
 
+class MotorJointConf(ctypes.Structure):
+    """MotorJointConf (d2/MotorJointConf.hpp:58-138); the reference's defaults:
+    offsets 0, maximum force 1, maximum torque 1, correction factor 0.3."""
+    _fields_ = [("body_a", ctypes.c_uint32), ("body_b", ctypes.c_uint32),
+                ("linear_offset", ctypes.c_float * 2), ("angular_offset", ctypes.c_float),
+                ("max_force", ctypes.c_float), ("max_torque", ctypes.c_float),
+                ("correction_factor", ctypes.c_float), ("collide_connected", ctypes.c_uint32)]
+
+    def __init__(self, **kw):
+        kw.setdefault("max_force", 1.0)
+        kw.setdefault("max_torque", 1.0)
+        kw.setdefault("correction_factor", 0.3)
+        super().__init__(**kw)
+
+
 class WeldJointConf(ctypes.Structure):
     """WeldJointConf (d2/WeldJointConf.hpp:62-130); frequency 0 is rigid."""
     _fields_ = [("body_a", ctypes.c_uint32), ("body_b", ctypes.c_uint32),
@@ -244,6 +259,7 @@ _SIGS = {
     "prx_weld_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
     "prx_rope_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
     "prx_friction_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
+    "prx_motor_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
     "prx_world_joint_count": (ctypes.c_uint32, [ctypes.c_void_p]),
     "prx_world_shape_count": (ctypes.c_uint32, [ctypes.c_void_p]),
     "prx_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
@@ -379,11 +395,13 @@ class World:
 
     def create_joint(self, conf):
         """CreateJoint with a RevoluteJointConf, DistanceJointConf,
-        WeldJointConf, RopeJointConf or FrictionJointConf; returns the joint id
+        WeldJointConf, RopeJointConf, FrictionJointConf or MotorJointConf;
+        returns the joint id
         (shared by all kinds)."""
         fn = {RevoluteJointConf: lib().prx_joint_create, DistanceJointConf: lib().prx_distance_joint_create,
               WeldJointConf: lib().prx_weld_joint_create, RopeJointConf: lib().prx_rope_joint_create,
-              FrictionJointConf: lib().prx_friction_joint_create}.get(type(conf))
+              FrictionJointConf: lib().prx_friction_joint_create,
+              MotorJointConf: lib().prx_motor_joint_create}.get(type(conf))
         if fn is None:
             raise TypeError("not a joint conf: %r" % type(conf))
         jid = ctypes.c_uint32()

@@ -262,7 +262,8 @@ struct HostJoint {
     // Revolute: reference angle, lower, upper, motor speed.  Distance: length,
     // frequency, damping ratio, 0.  Weld: reference angle, frequency, damping
     // ratio, 0.  Rope: maximum length, 0, 0, 0.  Friction: maximum force,
-    // maximum torque, 0, 0.
+    // maximum torque, 0, 0.  Motor: angular offset, maximum force, maximum
+    // torque, correction factor (linear offset in la[0..1]).
     float cfg[4] = {0, 0, 0, 0};
     float maxTorque = 0;
     float imp[4] = {0, 0, 0, 0};            // impulse x, y, z, angular motor impulse
@@ -644,6 +645,24 @@ int AddJoint(prx_world* w, const prx_revolute_joint_conf& c, std::uint32_t* out)
     j.cfg[2] = c.upper_angle;
     j.cfg[3] = c.motor_speed;
     j.maxTorque = c.max_motor_torque;
+    return AddJointRecord(w, j, out);
+}
+
+// MotorJointConf: the linear offset takes the anchor A slot, then (angular
+// offset, maximum force, maximum torque, correction factor).
+int AddMotorJoint(prx_world* w, const prx_motor_joint_conf& c, std::uint32_t* out) {
+    if (!(c.max_force >= 0.0f) || !(c.max_torque >= 0.0f))
+        return Fail(PRX_E_INVALID_ARGUMENT, "maximum force and torque must be non-negative");
+    HostJoint j;
+    j.a = c.body_a;
+    j.b = c.body_b;
+    j.flags = (c.collide_connected ? prx::kJointCollideConnected : 0u) | (prx::kJointMotorKind << 8);
+    j.la[0] = c.linear_offset[0];
+    j.la[1] = c.linear_offset[1];
+    j.cfg[0] = c.angular_offset;
+    j.cfg[1] = c.max_force;
+    j.cfg[2] = c.max_torque;
+    j.cfg[3] = c.correction_factor;
     return AddJointRecord(w, j, out);
 }
 
@@ -1376,6 +1395,20 @@ int prx_scene_build(prx_world* w, const char* name, const std::int32_t* params, 
         if (int st = AddBody(w, c, nullptr)) return st;
     }
     for (const auto& js : spec.joints) {
+        if (js.kind == prx_scene::kMotor) {
+            prx_motor_joint_conf mc{};
+            mc.body_a = std::uint32_t(js.a);
+            mc.body_b = std::uint32_t(js.b);
+            mc.linear_offset[0] = js.lax;
+            mc.linear_offset[1] = js.lay;
+            mc.angular_offset = js.referenceAngle;
+            mc.max_force = js.maxForce;
+            mc.max_torque = js.maxTorque;
+            mc.correction_factor = js.correctionFactor;
+            mc.collide_connected = js.collideConnected ? 1u : 0u;
+            if (int st = AddMotorJoint(w, mc, nullptr)) return st;
+            continue;
+        }
         if (js.kind == prx_scene::kRope || js.kind == prx_scene::kFriction) {
             HostJoint j;
             j.a = std::uint32_t(js.a);
@@ -1435,6 +1468,13 @@ int prx_distance_joint_create(prx_world* w, const prx_distance_joint_conf* conf,
     return AddSpringJoint(w, prx::kJointDistance, conf->body_a, conf->body_b, conf->local_anchor_a,
                           conf->local_anchor_b, conf->length, conf->frequency, conf->damping_ratio,
                           conf->collide_connected != 0, id);
+}
+
+int prx_motor_joint_create(prx_world* w, const prx_motor_joint_conf* conf, std::uint32_t* id) {
+    if (int st = CheckWorld(w)) return st;
+    if (!conf) return Fail(PRX_E_INVALID_ARGUMENT, "null joint conf");
+    if (int st = Detach(w)) return st;
+    return AddMotorJoint(w, *conf, id);
 }
 
 int prx_rope_joint_create(prx_world* w, const prx_rope_joint_conf* conf, std::uint32_t* id) {
@@ -1587,7 +1627,7 @@ int prx_world_load_desc(prx_world* w, const std::uint8_t* desc, std::size_t len)
         j.b = u32();
         j.flags = u32() & 0xff07u;
         if (j.a >= fresh.bodies.size() || j.b >= fresh.bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
-        if ((j.flags >> 8) > prx::kJointFriction) return Fail(PRX_E_INVALID_ARGUMENT, "unknown joint kind");
+        if ((j.flags >> 8) > prx::kJointMotorKind) return Fail(PRX_E_INVALID_ARGUMENT, "unknown joint kind");
         if (j.flags >> 8) j.flags &= ~(prx::kJointLimit | prx::kJointMotor);
         for (float& v : j.la) v = f32();
         for (float& v : j.cfg) v = f32();

@@ -198,7 +198,9 @@ struct DeviceState {
     float4* j_cfg;       // revolute: reference angle, lower angle, upper angle, motor speed;
                          // distance: length, frequency, damping ratio, -; weld: reference
                          // angle, frequency, damping ratio, -; rope: maximum length, -, -, -;
-                         // friction: maximum force, maximum torque, -, -
+                         // friction: maximum force, maximum torque, -, -; motor: angular
+                         // offset, maximum force, maximum torque, correction factor (the
+                         // linear offset in j_anchor.xy)
     float4* j_cfg2;      // max motor torque, flags (limit 1, motor 2, collideConnected 4,
                          // kind << 8), -, -
     float4* j_imp;       // impulse x, y, z, angular motor impulse
@@ -276,7 +278,14 @@ enum : std::uint32_t { kJointLimit = 1u, kJointMotor = 2u, kJointCollideConnecte
 // Pi as the float build's Real (RealConstants.hpp:57).
 constexpr float kPiF = 3.14159265358979323846f;
 // Joint kinds, in bits 8-15 of a joint's flags.
-enum : std::uint32_t { kJointRevolute = 0u, kJointDistance = 1u, kJointWeld = 2u, kJointRope = 3u, kJointFriction = 4u };
+enum : std::uint32_t {
+    kJointRevolute = 0u,
+    kJointDistance = 1u,
+    kJointWeld = 2u,
+    kJointRope = 3u,
+    kJointFriction = 4u,
+    kJointMotorKind = 5u  // MotorJointConf (kJointMotor is the revolute motor flag)
+};
 // Constraint records staged through LDS per chunk by the wave-wide sweeps;
 // the stage sits at the end of the dynamic LDS.
 constexpr int kStageRecords = 16;

@@ -51,13 +51,15 @@ struct BodySpec {
 // Joint between bodies a and b with local anchors: kind 0 RevoluteJointConf,
 // 1 DistanceJointConf (length, frequency, dampingRatio), 2 WeldJointConf
 // (referenceAngle, frequency, dampingRatio), 3 RopeJointConf (length is the
-// maximum length), 4 FrictionJointConf (maxForce, maxTorque).
-enum JointKind : int { kRevolute = 0, kDistance = 1, kWeld = 2, kRope = 3, kFriction = 4 };
+// maximum length), 4 FrictionJointConf (maxForce, maxTorque), 5 MotorJointConf
+// (linear offset lax, lay; angular offset referenceAngle; maxForce,
+// maxTorque, correctionFactor).
+enum JointKind : int { kRevolute = 0, kDistance = 1, kWeld = 2, kRope = 3, kFriction = 4, kMotor = 5 };
 struct JointSpec {
     int kind = kRevolute;
     int a = 0, b = 0;
     float length = 1.0f, frequency = 0.0f, dampingRatio = 0.0f;
-    float maxForce = 0.0f, maxTorque = 0.0f;
+    float maxForce = 0.0f, maxTorque = 0.0f, correctionFactor = 0.3f;
     float lax = 0, lay = 0, lbx = 0, lby = 0;
     float referenceAngle = 0;
     bool enableLimit = false;
@@ -465,6 +467,20 @@ inline SceneSpec Joints(std::uint32_t seed, int w, int n = 8, int falling = 24) 
         }
         b.shapes = {int(sc.shapes.size()) - 1};
         sc.bodies.push_back(b);
+        if (k == 1 || k == 2) {
+            // Motor joints from the ground pulling the body towards an offset.
+            JointSpec mj;
+            mj.kind = kMotor;
+            mj.a = 0;
+            mj.b = int(sc.bodies.size()) - 1;
+            mj.lax = -10.0f * float(k);
+            mj.lay = 8.0f;
+            mj.referenceAngle = 0.5f * float(k);
+            mj.maxForce = 40.0f * float(k);
+            mj.maxTorque = 3.0f;
+            mj.correctionFactor = 0.2f + 0.1f * float(k);
+            sc.joints.push_back(mj);
+        }
         if (k % 3 == 0) {
             // A rope from the bar, slack at first, taut once the body has
             // fallen 2 to 6 m below the bar.

@@ -1267,7 +1267,7 @@ __device__ __forceinline__ bool joint_solve_position(const float4* u, float4& po
 }
 
 // ---------------------------------------------------------------------------
-// Distance, weld, rope and friction joints.  Their records share the revolute record's slots
+// Distance, weld, rope, friction and motor joints.  Their records share the revolute record's slots
 // u[0] (rA, rB), u[3] (impulses), u[4] (inverse masses), u[5].zw (bodies),
 // u[6] (anchors less local centers); u[2].w holds flags | limit state << 8 |
 // kind << 16 (the revolute record's word, kind 0), limit state 0 but for ropes.
@@ -1443,6 +1443,92 @@ __device__ __forceinline__ void build_jc_friction(float4* u, const float4 cf, fl
     u[7] = float4{0.0f, 0.0f, 0.0f, __uint_as_float(prx::kJointFriction)};
 }
 
+// Motor record: u[1] linear mass m00, m01, m10, m11; u[2].xy angular mass,
+// angular error; u[5].xy dt * maximum force, dt * maximum torque; u[7] linear
+// error x, y, (1 / dt) * correction factor; impulses as friction.
+// InitVelocity(MotorJointConf&) less its warm start (MotorJointConf.cpp:79-145).
+__device__ __forceinline__ void build_jc_motor(float4* u, const float4 cf, float4 im, V2 rA, V2 rB, const float4& pA,
+                                               const float4& pB, float mA, float iA, float mB, float iB,
+                                               std::uint32_t flags, const StepParams& c) {
+    const float exx = mA + mB + iA * (rA.y * rA.y) / 1.0f + iB * (rB.y * rB.y) / 1.0f;
+    const float exy = -iA * rA.x * rA.y / 1.0f + -iB * rB.x * rB.y / 1.0f;
+    const float eyy = mA + mB + iA * (rA.x * rA.x) / 1.0f + iB * (rB.x * rB.x) / 1.0f;
+    float4 lm{0.0f, 0.0f, 0.0f, 0.0f};
+    const float cp = cross(V2{exx, exy}, V2{exy, eyy});
+    if (!almost_zero(cp)) {
+        const float inverse = 1.0f / cp;
+        lm = float4{eyy * inverse, -exy * inverse, -exy * inverse, exx * inverse};
+    }
+    const float invRotI = iA + iB;
+    const float angularMass = (invRotI > 0.0f) ? 1.0f / invRotI : 0.0f;
+    const V2 linearError = (V2{pB.x, pB.y} + rB) - (V2{pA.x, pA.y} + rA);
+    const float angularError = (pB.z - pA.z) - cf.x;
+    im = c.warmStart ? float4{im.x * c.dtRatio, im.y * c.dtRatio, im.z * c.dtRatio, 0.0f}
+                     : float4{0.0f, 0.0f, 0.0f, 0.0f};
+    const float invH = (c.dt != 0.0f) ? 1.0f / c.dt : 0.0f;
+    u[1] = lm;
+    u[2] = float4{angularMass, angularError, 0.0f, __uint_as_float(flags | (prx::kJointMotorKind << 16))};
+    u[3] = im;
+    u[5].x = c.dt * cf.y;
+    u[5].y = c.dt * cf.z;
+    u[7] = float4{linearError.x, linearError.y, invH * cf.w, __uint_as_float(prx::kJointMotorKind)};
+}
+
+// SolveVelocity(MotorJointConf&) (MotorJointConf.cpp:160-229): returns
+// whether neither impulse changed (velocities are then left as they were).
+__device__ __forceinline__ bool motor_solve_velocity(float4* u, float4& velA, float4& velB) {
+    const float4 r = u[0], lm = u[1], n = u[2], ms = u[4], mx = u[5], e = u[7];
+    const V2 rA{r.x, r.y}, rB{r.z, r.w};
+    float4 im = u[3];
+    bool solved = true;
+    {
+        const float Cdot = (velB.z - velA.z) + e.z * n.y;
+        const float angularImpulse = -n.x * Cdot;
+        const float old = im.z;
+        const float now = clamp_std(old + angularImpulse, -mx.y, mx.y);
+        im.z = now;
+        const float inc = now - old;
+        if (inc != 0.0f) {
+            // Velocity{LinearVelocity2{}, ...}: the linear parts take a zero.
+            velA.x = velA.x - 0.0f;
+            velA.y = velA.y - 0.0f;
+            velA.z = velA.z - ms.y * inc;
+            velB.x = velB.x + 0.0f;
+            velB.y = velB.y + 0.0f;
+            velB.z = velB.z + ms.w * inc;
+            solved = false;
+        }
+    }
+    {
+        const V2 vb = V2{velB.x, velB.y} + mul(revperp(rB), velB.z / 1.0f);
+        const V2 va = V2{velA.x, velA.y} + mul(revperp(rA), velA.z / 1.0f);
+        const V2 Cdot = (vb - va) + V2{e.z * e.x, e.z * e.y};
+        float tx = 0.0f, ty = 0.0f;
+        tx += lm.x * Cdot.x;
+        tx += lm.y * Cdot.y;
+        ty += lm.z * Cdot.x;
+        ty += lm.w * Cdot.y;
+        const V2 old{im.x, im.y};
+        V2 li{old.x + -tx, old.y + -ty};
+        const float maxImpulse = mx.x;
+        if (magsq(li) > maxImpulse * maxImpulse) {
+            const V2 dir = unit_vector(li, V2{0.0f, 0.0f});
+            li = V2{dir.x * maxImpulse, dir.y * maxImpulse};
+        }
+        im.x = li.x;
+        im.y = li.y;
+        const V2 inc = li - old;
+        const float angA = cross(rA, inc) / 1.0f;
+        const float angB = cross(rB, inc) / 1.0f;
+        if (inc.x != 0.0f || inc.y != 0.0f || angA != 0.0f || angB != 0.0f) {
+            apply_joint_impulse(ms, inc, angA, angB, velA, velB);
+            solved = false;
+        }
+    }
+    u[3] = im;
+    return solved;
+}
+
 // SolveVelocity(RopeJointConf&) (RopeJointConf.cpp:130-167): returns whether
 // the impulse did not change.
 __device__ __forceinline__ bool rope_solve_velocity(float4* u, float4& velA, float4& velB) {
@@ -1516,7 +1602,7 @@ __device__ __forceinline__ void spring_joint_warm_start(const float4* u, std::ui
     if (kind == prx::kJointDistance || kind == prx::kJointRope) {
         const V2 P = mul(im.x, V2{u[1].x, u[1].y});
         apply_joint_impulse(ms, P, cross(rA, P) / 1.0f, cross(rB, P) / 1.0f, velA, velB);
-    } else if (kind == prx::kJointFriction) {
+    } else if (kind == prx::kJointFriction || kind == prx::kJointMotorKind) {
         const V2 P{im.x, im.y};
         apply_joint_impulse(ms, P, cross(rA, P) / 1.0f + im.z, cross(rB, P) / 1.0f + im.z, velA, velB);
     } else {
@@ -1596,7 +1682,8 @@ __device__ __forceinline__ bool spring_joint_solve_position(const float4* u, std
                                                             float4& posB, const StepParams& c) {
     const float4 ms = u[4], la = u[6], cf = u[7];
     const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
-    if (kind == prx::kJointFriction) return true;  // FrictionJointConf.cpp:233-237
+    if (kind == prx::kJointFriction) return true;    // FrictionJointConf.cpp:233-237
+    if (kind == prx::kJointMotorKind) return true;   // MotorJointConf.cpp:231-234
     if (kind == prx::kJointDistance && cf.y > 0.0f) return true;
     const V2 rA = rotate(V2{la.x, la.y}, unit_from_angle(posA.z));
     const V2 rB = rotate(V2{la.z, la.w}, unit_from_angle(posB.z));
@@ -1690,7 +1777,13 @@ __device__ __forceinline__ void build_jc(const DeviceState& d, const WorldCtx& x
         if (kind == prx::kJointDistance) build_jc_distance(u, cf, im, rA, rB, pA, pB, mA, iA, mB, iB, c);
         else if (kind == prx::kJointWeld) build_jc_weld(u, cf, im, rA, rB, pA, pB, mA, iA, mB, iB, flags & 0xffu, c);
         else if (kind == prx::kJointRope) build_jc_rope(u, cf, im, rA, rB, pA, pB, mA, iA, mB, iB, flags & 0xffu, c);
-        else build_jc_friction(u, cf, im, rA, rB, mA, iA, mB, iB, flags & 0xffu, c);
+        else if (kind == prx::kJointFriction) build_jc_friction(u, cf, im, rA, rB, mA, iA, mB, iB, flags & 0xffu, c);
+        else {
+            // Body B's anchor is its origin: rB = Rotate(-localCenterB, qB).
+            const V2 rBm = rotate(V2{-B.lc.x, -B.lc.y}, qB);
+            u[0] = float4{rA.x, rA.y, rBm.x, rBm.y};
+            build_jc_motor(u, cf, im, rA, rBm, pA, pB, mA, iA, mB, iB, flags & 0xffu, c);
+        }
         return;
     }
     const float totInvI = iA + iB;
@@ -2141,7 +2234,8 @@ __device__ __forceinline__ bool joints_velocity_k(float4* jbase, int j0, int j1,
             else if (kind == prx::kJointDistance) ok &= distance_solve_velocity(u, va, vb);
             else if (kind == prx::kJointWeld) ok &= weld_solve_velocity(u, va, vb);
             else if (kind == prx::kJointRope) ok &= rope_solve_velocity(u, va, vb);
-            else ok &= friction_solve_velocity(u, va, vb);
+            else if (kind == prx::kJointFriction) ok &= friction_solve_velocity(u, va, vb);
+            else ok &= motor_solve_velocity(u, va, vb);
             bs.set_vel(bA, va);
             bs.set_vel(bB, vb);
             continue;

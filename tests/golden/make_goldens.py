@@ -37,8 +37,8 @@ CASES = [
 ] + [("pile%d" % w, "pile", 150, [1, w], [1, 150]) for w in range(8)] + [
     ("chain0", "chain", 240, [1, 0], [1, 120, 240]),
     ("chain1", "chain", 240, [1, 1, 12, 24, 40], [1, 240]),
-    # Distance and weld joints, rigid and soft, friction joints on the ground and
-    # ropes from a bar, beside falling bodies.
+    # Distance and weld joints, rigid and soft, friction joints on the ground,
+    # ropes from a bar and motor joints, beside falling bodies.
     ("joints0", "joints", 240, [1, 0], [1, 120, 240]),
     ("joints1", "joints", 240, [2, 1, 12, 40], [1, 240]),
 ]
