@@ -149,6 +149,21 @@ typedef struct prx_motor_joint_conf {
     uint32_t collide_connected;
 } prx_motor_joint_conf;
 
+/* WheelJointConf (d2/WheelJointConf.hpp:71-170): point-to-line along the local
+ * axis of A (a unit vector; the y axis is its reverse perpendicular), a soft
+ * spring along it when frequency > 0, and an optional motor.  The
+ * reference's defaults are axis (1, 0), frequency 2, damping ratio 0.7, motor
+ * off; a negative or NaN frequency is PRX_E_INVALID_ARGUMENT. */
+typedef struct prx_wheel_joint_conf {
+    uint32_t body_a, body_b;
+    float local_anchor_a[2], local_anchor_b[2];
+    float local_axis_a[2];
+    uint32_t enable_motor;
+    float motor_speed, max_motor_torque;
+    float frequency, damping_ratio;
+    uint32_t collide_connected;
+} prx_wheel_joint_conf;
+
 /* StepConf fields the regular step reads (StepConf.hpp:49-392). */
 typedef struct prx_step_conf {
     float delta_time, dt_ratio, min_still_time_to_sleep, linear_slop, angular_slop;
@@ -233,6 +248,8 @@ int prx_rope_joint_create(prx_world* world, const prx_rope_joint_conf* conf, uin
 int prx_friction_joint_create(prx_world* world, const prx_friction_joint_conf* conf, uint32_t* joint_id);
 /* CreateJoint(World&, const MotorJointConf&) (solver MotorJointConf.cpp:79-234). */
 int prx_motor_joint_create(prx_world* world, const prx_motor_joint_conf* conf, uint32_t* joint_id);
+/* CreateJoint(World&, const WheelJointConf&) (solver WheelJointConf.cpp:105-359). */
+int prx_wheel_joint_create(prx_world* world, const prx_wheel_joint_conf* conf, uint32_t* joint_id);
 uint32_t prx_world_joint_count(const prx_world* world);
 /* Builds a named workload scene through the calls above. */
 int prx_scene_build(prx_world* world, const char* name, const int32_t* params, int nparams);

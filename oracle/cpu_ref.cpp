@@ -106,7 +106,7 @@ void Load(Handle<R>& h, In& in) {
         j.enableLimit = (fl & 1u) != 0;
         j.enableMotor = (fl & 2u) != 0;
         j.collideConnected = (fl & 4u) != 0;
-        R f[9];
+        R f[12];
         for (auto& x : f) x = R(in.f32());
         j.laA = V2<R>{f[0], f[1]};
         j.laB = V2<R>{f[2], f[3]};
@@ -122,10 +122,17 @@ void Load(Handle<R>& h, In& in) {
             // Friction: maximum force, maximum torque.  Motor: linear offset,
             // -, angular offset, maximum force, maximum torque, correction
             // factor.
-            j.enableLimit = j.enableMotor = false;
+            j.enableLimit = false;
+            if (j.kind != 6) j.enableMotor = false;
             if (j.kind == 1 || j.kind == 3) j.length = f[4];
             if (j.kind != 2 && j.kind != 5) j.referenceAngle = 0;
-            if (j.kind <= 2) {
+            if (j.kind == 6) {
+                // Wheel: frequency, damping ratio, -, motor speed, maximum
+                // motor torque, local x axis of A.
+                j.frequency = f[4];
+                j.dampingRatio = f[5];
+                j.localXAxis = V2<R>{f[9], f[10]};
+            } else if (j.kind <= 2) {
                 j.frequency = f[5];
                 j.dampingRatio = f[6];
             } else if (j.kind == 4) {
@@ -138,7 +145,8 @@ void Load(Handle<R>& h, In& in) {
                 j.maxTorque = f[6];
                 j.correctionFactor = f[7];
             }
-            j.lower = j.upper = j.motorSpeed = j.maxMotorTorque = 0;
+            j.lower = j.upper = 0;
+            if (j.kind != 6) j.motorSpeed = j.maxMotorTorque = 0;
         }
         h.world.AddJoint(j);
     }

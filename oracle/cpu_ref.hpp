@@ -795,6 +795,10 @@ struct Joint {
     // impulses as friction.
     R correctionFactor = 0, angularError = 0;
     V2<R> linearError;
+    // kind 6 wheel: impulse.x impulse, impulse.y spring impulse, impulse.z
+    // angular (motor) impulse; dmass the point-to-line mass.
+    V2<R> localXAxis, ax, ay;
+    R sAx = 0, sAy = 0, sBx = 0, sBy = 0, springMass = 0;
 };
 
 template <class R>
@@ -2023,6 +2027,144 @@ public:
         return false;
     }
 
+    // InitVelocity(WheelJointConf&) (WheelJointConf.cpp:105-230).
+    static void WheelInitVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        const R mA = A.invMass, iA = A.invRotI, mB = B.invMass, iB = B.invRotI;
+        Vel<R> velA = A.vel, velB = B.vel;
+        const V2<R> qA = UnitFromAngle(A.pos.ang);
+        const V2<R> rA = Rotate(j.laA - A.lc, qA);
+        const V2<R> rB = Rotate(j.laB - B.lc, UnitFromAngle(B.pos.ang));
+        const V2<R> dd = (B.pos.lin + rB) - (A.pos.lin + rA);
+        {
+            j.ay = Rotate(RevPerp(j.localXAxis), qA);
+            j.sAy = Cross(dd + rA, j.ay);
+            j.sBy = Cross(rB, j.ay);
+            const R invMass = mA + mB + iA * (j.sAy * j.sAy) / R(1) + iB * (j.sBy * j.sBy) / R(1);
+            j.dmass = (invMass > R(0)) ? R(1) / invMass : R(0);
+        }
+        j.springMass = 0;
+        j.bias = 0;
+        j.gamma = 0;
+        if (j.frequency > R(0)) {
+            j.ax = Rotate(j.localXAxis, qA);
+            j.sAx = Cross(dd + rA, j.ax);
+            j.sBx = Cross(rB, j.ax);
+            const R invMass = mA + mB + iA * (j.sAx * j.sAx) / R(1) + iB * (j.sBx * j.sBx) / R(1);
+            if (invMass > R(0)) {
+                j.springMass = R(1) / invMass;
+                const R C = Dot(dd, j.ax);
+                const R omega = R(2) * kPi * j.frequency;
+                const R d = R(2) * j.springMass * j.dampingRatio * omega;
+                const R k = j.springMass * omega * omega;
+                const R h = conf.dt;
+                const R invGamma = h * (d + h * k);
+                j.gamma = (invGamma > R(0)) ? R(1) / invGamma : R(0);
+                j.bias = C * h * k * j.gamma;
+                const R total = invMass + j.gamma;
+                j.springMass = (total > R(0)) ? R(1) / total : R(0);
+            }
+        } else {
+            j.impulse.y = 0;
+            j.ax = V2<R>{R(0), R(0)};
+            j.sAx = 0;
+            j.sBx = 0;
+        }
+        if (j.enableMotor) {
+            const R invRotI = iA + iB;
+            j.angularMass = (invRotI > R(0)) ? R(1) / invRotI : R(0);
+        } else {
+            j.angularMass = 0;
+            j.impulse.z = 0;
+        }
+        if (conf.warmStart) {
+            j.impulse.x *= conf.dtRatio;
+            j.impulse.y *= conf.dtRatio;
+            j.impulse.z *= conf.dtRatio;
+            const V2<R> P = j.impulse.x * j.ay + j.impulse.y * j.ax;
+            const R LA = (j.impulse.x * j.sAy + j.impulse.y * j.sAx) / R(1) + j.impulse.z;
+            const R LB = (j.impulse.x * j.sBy + j.impulse.y * j.sBx) / R(1) + j.impulse.z;
+            ApplyImpulse(A, B, P, LA, LB, velA, velB);
+        } else {
+            j.impulse = V3<R>{};
+        }
+        A.vel = velA;
+        B.vel = velB;
+    }
+
+    // SolveVelocity(WheelJointConf&) (WheelJointConf.cpp:232-307).
+    static bool WheelSolveVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        const Vel<R> oldVelA = A.vel, oldVelB = B.vel;
+        Vel<R> velA = oldVelA, velB = oldVelB;
+        const R iA = A.invRotI, iB = B.invRotI;
+        {
+            const R dot = Dot(j.ax, velB.lin - velA.lin);
+            const R Cdot = dot + j.sBx * velB.ang / R(1) - j.sAx * velA.ang / R(1);
+            const R impulse = -j.springMass * (Cdot + j.bias + j.gamma * j.impulse.y);
+            j.impulse.y += impulse;
+            const V2<R> P = impulse * j.ax;
+            ApplyImpulse(A, B, P, impulse * j.sAx / R(1), impulse * j.sBx / R(1), velA, velB);
+        }
+        {
+            const R Cdot = velB.ang - velA.ang - j.motorSpeed;
+            R impulse = -j.angularMass * Cdot;
+            const R old = j.impulse.z;
+            const R maxImpulse = conf.dt * j.maxMotorTorque;
+            j.impulse.z = Clamp(j.impulse.z + impulse, -maxImpulse, maxImpulse);
+            impulse = j.impulse.z - old;
+            velA.ang = velA.ang - iA * impulse;
+            velB.ang = velB.ang + iB * impulse;
+        }
+        {
+            const R dot = Dot(j.ay, velB.lin - velA.lin);
+            const R Cdot = dot + j.sBy * velB.ang / R(1) - j.sAy * velA.ang / R(1);
+            const R impulse = -j.dmass * Cdot;
+            j.impulse.x += impulse;
+            const V2<R> P = impulse * j.ay;
+            ApplyImpulse(A, B, P, impulse * j.sAy / R(1), impulse * j.sBy / R(1), velA, velB);
+        }
+        const bool changed = !(velA.lin == oldVelA.lin) || velA.ang != oldVelA.ang || !(velB.lin == oldVelB.lin) ||
+                             velB.ang != oldVelB.ang;
+        if (changed) {
+            A.vel = velA;
+            B.vel = velB;
+            return false;
+        }
+        return true;
+    }
+
+    // SolvePosition(const WheelJointConf&) (WheelJointConf.cpp:309-359).  The
+    // effective mass uses the sAy, sBy found by InitVelocity, as there.
+    static bool WheelSolvePosition(const Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        Pos<R> posA = A.pos, posB = B.pos;
+        const R mA = A.invMass, iA = A.invRotI, mB = B.invMass, iB = B.invRotI;
+        const V2<R> qA = UnitFromAngle(posA.ang);
+        const V2<R> rA = Rotate(j.laA - A.lc, qA);
+        const V2<R> rB = Rotate(j.laB - B.lc, UnitFromAngle(posB.ang));
+        const V2<R> d = (posB.lin - posA.lin) + (rB - rA);
+        const V2<R> ay = Rotate(RevPerp(j.localXAxis), qA);
+        const R sAy = Cross(d + rA, ay);
+        const R sBy = Cross(rB, ay);
+        const R C = Dot(d, ay);
+        const R k = mA + mB + iA * (j.sAy * j.sAy) / R(1) + iB * (j.sBy * j.sBy) / R(1);
+        const R impulse = (k != R(0)) ? -(C / k) : R(0);
+        const V2<R> P = impulse * ay;
+        const R LA = impulse * sAy / R(1);
+        const R LB = impulse * sBy / R(1);
+        posA.lin = posA.lin - mA * P;
+        posA.ang = posA.ang - iA * LA;
+        posB.lin = posB.lin + mB * P;
+        posB.ang = posB.ang + iB * LB;
+        A.pos = posA;
+        B.pos = posB;
+        return std::abs(C) <= conf.linearSlop;
+    }
+
     // InitVelocity(RevoluteJointConf&) (RevoluteJointConf.cpp:127-226).
     static void JointInitVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
         if (j.kind == 1) return DistanceInitVelocity(j, bcs, conf);
@@ -2030,6 +2172,7 @@ public:
         if (j.kind == 3) return RopeInitVelocity(j, bcs, conf);
         if (j.kind == 4) return FrictionInitVelocity(j, bcs, conf);
         if (j.kind == 5) return MotorInitVelocity(j, bcs, conf);
+        if (j.kind == 6) return WheelInitVelocity(j, bcs, conf);
         BodyConstraint<R>& A = bcs[j.bodyA];
         BodyConstraint<R>& B = bcs[j.bodyB];
         const R mA = A.invMass, iA = A.invRotI, aA = A.pos.ang;
@@ -2093,6 +2236,7 @@ public:
         if (j.kind == 3) return RopeSolveVelocity(j, bcs, conf);
         if (j.kind == 4) return FrictionSolveVelocity(j, bcs, conf);
         if (j.kind == 5) return MotorSolveVelocity(j, bcs, conf);
+        if (j.kind == 6) return WheelSolveVelocity(j, bcs, conf);
         BodyConstraint<R>& A = bcs[j.bodyA];
         BodyConstraint<R>& B = bcs[j.bodyB];
         const Vel<R> oldVelA = A.vel, oldVelB = B.vel;
@@ -2169,6 +2313,7 @@ public:
         if (j.kind == 3) return RopeSolvePosition(j, bcs, conf);
         if (j.kind == 4) return true;  // FrictionJointConf.cpp:233-237
         if (j.kind == 5) return true;  // MotorJointConf.cpp:231-234
+        if (j.kind == 6) return WheelSolvePosition(j, bcs, conf);
         BodyConstraint<R>& A = bcs[j.bodyA];
         BodyConstraint<R>& B = bcs[j.bodyB];
         Pos<R> posA = A.pos, posB = B.pos;

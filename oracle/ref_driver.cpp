@@ -41,6 +41,7 @@
 #include <playrho/d2/RopeJointConf.hpp>
 #include <playrho/d2/FrictionJointConf.hpp>
 #include <playrho/d2/MotorJointConf.hpp>
+#include <playrho/d2/WheelJointConf.hpp>
 #include <playrho/d2/WorldJoint.hpp>
 #include <playrho/StepConf.hpp>
 
@@ -175,7 +176,7 @@ void WriteDescription(Out& o, const World& world) {
             o.u32(to_underlying(GetBodyA(*j)));
             o.u32(to_underlying(GetBodyB(*j)));
             o.u32((j->enableLimit ? 1u : 0u) | (j->enableMotor ? 2u : 0u) | (j->collideConnected ? 4u : 0u));
-            const float f[9] = {GetX(j->localAnchorA), GetY(j->localAnchorA), GetX(j->localAnchorB),
+            const float f[12] = {GetX(j->localAnchorA), GetY(j->localAnchorA), GetX(j->localAnchorB),
                                 GetY(j->localAnchorB), j->referenceAngle, j->lowerAngle, j->upperAngle,
                                 j->motorSpeed, j->maxMotorTorque};
             o.put(f, sizeof(f));
@@ -183,14 +184,14 @@ void WriteDescription(Out& o, const World& world) {
             o.u32(to_underlying(GetBodyA(*j)));
             o.u32(to_underlying(GetBodyB(*j)));
             o.u32((j->collideConnected ? 4u : 0u) | (1u << 8));
-            const float f[9] = {GetX(j->localAnchorA), GetY(j->localAnchorA), GetX(j->localAnchorB),
+            const float f[12] = {GetX(j->localAnchorA), GetY(j->localAnchorA), GetX(j->localAnchorB),
                                 GetY(j->localAnchorB), j->length, j->frequency, j->dampingRatio, 0.0f, 0.0f};
             o.put(f, sizeof(f));
         } else if (const auto* j = TypeCast<const WeldJointConf>(&joint)) {
             o.u32(to_underlying(GetBodyA(*j)));
             o.u32(to_underlying(GetBodyB(*j)));
             o.u32((j->collideConnected ? 4u : 0u) | (2u << 8));
-            const float f[9] = {GetX(j->localAnchorA), GetY(j->localAnchorA), GetX(j->localAnchorB),
+            const float f[12] = {GetX(j->localAnchorA), GetY(j->localAnchorA), GetX(j->localAnchorB),
                                 GetY(j->localAnchorB), j->referenceAngle, j->frequency, j->dampingRatio,
                                 0.0f, 0.0f};
             o.put(f, sizeof(f));
@@ -198,21 +199,29 @@ void WriteDescription(Out& o, const World& world) {
             o.u32(to_underlying(GetBodyA(*j)));
             o.u32(to_underlying(GetBodyB(*j)));
             o.u32((j->collideConnected ? 4u : 0u) | (3u << 8));
-            const float f[9] = {GetX(j->localAnchorA), GetY(j->localAnchorA), GetX(j->localAnchorB),
+            const float f[12] = {GetX(j->localAnchorA), GetY(j->localAnchorA), GetX(j->localAnchorB),
                                 GetY(j->localAnchorB), j->maxLength, 0.0f, 0.0f, 0.0f, 0.0f};
             o.put(f, sizeof(f));
         } else if (const auto* j = TypeCast<const FrictionJointConf>(&joint)) {
             o.u32(to_underlying(GetBodyA(*j)));
             o.u32(to_underlying(GetBodyB(*j)));
             o.u32((j->collideConnected ? 4u : 0u) | (4u << 8));
-            const float f[9] = {GetX(j->localAnchorA), GetY(j->localAnchorA), GetX(j->localAnchorB),
+            const float f[12] = {GetX(j->localAnchorA), GetY(j->localAnchorA), GetX(j->localAnchorB),
                                 GetY(j->localAnchorB), j->maxForce, j->maxTorque, 0.0f, 0.0f, 0.0f};
+            o.put(f, sizeof(f));
+        } else if (const auto* j = TypeCast<const WheelJointConf>(&joint)) {
+            o.u32(to_underlying(GetBodyA(*j)));
+            o.u32(to_underlying(GetBodyB(*j)));
+            o.u32((j->enableMotor ? 2u : 0u) | (j->collideConnected ? 4u : 0u) | (6u << 8));
+            const float f[12] = {GetX(j->localAnchorA), GetY(j->localAnchorA), GetX(j->localAnchorB),
+                                 GetY(j->localAnchorB), j->frequency, j->dampingRatio, 0.0f, j->motorSpeed,
+                                 j->maxMotorTorque, GetX(j->localXAxisA), GetY(j->localXAxisA), 0.0f};
             o.put(f, sizeof(f));
         } else if (const auto* j = TypeCast<const MotorJointConf>(&joint)) {
             o.u32(to_underlying(GetBodyA(*j)));
             o.u32(to_underlying(GetBodyB(*j)));
             o.u32((j->collideConnected ? 4u : 0u) | (5u << 8));
-            const float f[9] = {GetX(j->linearOffset), GetY(j->linearOffset), 0.0f, 0.0f, j->angularOffset,
+            const float f[12] = {GetX(j->linearOffset), GetY(j->linearOffset), 0.0f, 0.0f, j->angularOffset,
                                 j->maxForce, j->maxTorque, j->correctionFactor, 0.0f};
             o.put(f, sizeof(f));
         }
@@ -416,6 +425,10 @@ void WriteState(Out& o, const World& world, std::uint32_t step, const StepStats&
             o.u32(0u);
             const float f[4] = {GetX(j->linearImpulse), GetY(j->linearImpulse), j->angularImpulse, 0.0f};
             o.put(f, sizeof(f));
+        } else if (const auto* j = TypeCast<const WheelJointConf>(&joint)) {
+            o.u32(0u);
+            const float f[4] = {j->impulse, j->springImpulse, j->angularImpulse, 0.0f};
+            o.put(f, sizeof(f));
         } else if (const auto* j = TypeCast<const MotorJointConf>(&joint)) {
             o.u32(0u);
             const float f[4] = {GetX(j->linearImpulse), GetY(j->linearImpulse), j->angularImpulse, 0.0f};
@@ -471,6 +484,13 @@ World BuildWorld(const prx_scene::SceneSpec& spec) {
         } else if (js.kind == prx_scene::kFriction) {
             auto jc = FrictionJointConf{ba, bb, Length2{js.lax, js.lay}, Length2{js.lbx, js.lby}};
             jc.UseMaxForce(js.maxForce).UseMaxTorque(js.maxTorque);
+            jc.UseCollideConnected(js.collideConnected);
+            CreateJoint(world, jc);
+        } else if (js.kind == prx_scene::kWheel) {
+            auto jc = WheelJointConf{ba, bb, Length2{js.lax, js.lay}, Length2{js.lbx, js.lby},
+                                     UnitVec::Get(js.axisAngle)};
+            jc.UseEnableMotor(js.enableMotor).UseMotorSpeed(js.motorSpeed).UseMaxMotorTorque(js.maxMotorTorque);
+            jc.UseFrequency(js.frequency).UseDampingRatio(js.dampingRatio);
             jc.UseCollideConnected(js.collideConnected);
             CreateJoint(world, jc);
         } else if (js.kind == prx_scene::kMotor) {

@@ -204,6 +204,23 @@ class MotorJointConf(ctypes.Structure):
         super().__init__(**kw)
 
 
+class WheelJointConf(ctypes.Structure):
+    """WheelJointConf (d2/WheelJointConf.hpp:71-170); the reference's defaults:
+    axis (1, 0), frequency 2, damping ratio 0.7, motor off."""
+    _fields_ = [("body_a", ctypes.c_uint32), ("body_b", ctypes.c_uint32),
+                ("local_anchor_a", ctypes.c_float * 2), ("local_anchor_b", ctypes.c_float * 2),
+                ("local_axis_a", ctypes.c_float * 2), ("enable_motor", ctypes.c_uint32),
+                ("motor_speed", ctypes.c_float), ("max_motor_torque", ctypes.c_float),
+                ("frequency", ctypes.c_float), ("damping_ratio", ctypes.c_float),
+                ("collide_connected", ctypes.c_uint32)]
+
+    def __init__(self, **kw):
+        kw.setdefault("local_axis_a", (1.0, 0.0))
+        kw.setdefault("frequency", 2.0)
+        kw.setdefault("damping_ratio", 0.7)
+        super().__init__(**kw)
+
+
 class WeldJointConf(ctypes.Structure):
     """WeldJointConf (d2/WeldJointConf.hpp:62-130); frequency 0 is rigid."""
     _fields_ = [("body_a", ctypes.c_uint32), ("body_b", ctypes.c_uint32),
@@ -260,6 +277,7 @@ _SIGS = {
     "prx_rope_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
     "prx_friction_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
     "prx_motor_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
+    "prx_wheel_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
     "prx_world_joint_count": (ctypes.c_uint32, [ctypes.c_void_p]),
     "prx_world_shape_count": (ctypes.c_uint32, [ctypes.c_void_p]),
     "prx_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
@@ -395,13 +413,14 @@ class World:
 
     def create_joint(self, conf):
         """CreateJoint with a RevoluteJointConf, DistanceJointConf,
-        WeldJointConf, RopeJointConf, FrictionJointConf or MotorJointConf;
-        returns the joint id
+        WeldJointConf, RopeJointConf, FrictionJointConf, MotorJointConf or
+        WheelJointConf; returns the joint id
         (shared by all kinds)."""
         fn = {RevoluteJointConf: lib().prx_joint_create, DistanceJointConf: lib().prx_distance_joint_create,
               WeldJointConf: lib().prx_weld_joint_create, RopeJointConf: lib().prx_rope_joint_create,
               FrictionJointConf: lib().prx_friction_joint_create,
-              MotorJointConf: lib().prx_motor_joint_create}.get(type(conf))
+              MotorJointConf: lib().prx_motor_joint_create,
+              WheelJointConf: lib().prx_wheel_joint_create}.get(type(conf))
         if fn is None:
             raise TypeError("not a joint conf: %r" % type(conf))
         jid = ctypes.c_uint32()
@@ -737,7 +756,7 @@ def desc_joint_offsets(desc):
     else:
         off = shapes[-1] + 4 + 16 + 12 + 16 * u32(shapes[-1]) + 4 if shapes else 8
     nj = u32(off)
-    return [off + 4 + 48 * k for k in range(nj)]
+    return [off + 4 + 60 * k for k in range(nj)]
 
 
 def desc_with(desc, velocities=None, friction=None, restitution=None, motor_torque=None):

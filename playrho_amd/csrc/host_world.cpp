@@ -263,9 +263,12 @@ struct HostJoint {
     // frequency, damping ratio, 0.  Weld: reference angle, frequency, damping
     // ratio, 0.  Rope: maximum length, 0, 0, 0.  Friction: maximum force,
     // maximum torque, 0, 0.  Motor: angular offset, maximum force, maximum
-    // torque, correction factor (linear offset in la[0..1]).
+    // torque, correction factor (linear offset in la[0..1]).  Wheel: frequency,
+    // damping ratio, 0, motor speed (maximum motor torque in maxTorque, axis in
+    // ext).
     float cfg[4] = {0, 0, 0, 0};
     float maxTorque = 0;
+    float ext[3] = {0, 0, 0};               // wheel: local x axis of A (x, y), -
     float imp[4] = {0, 0, 0, 0};            // impulse x, y, z, angular motor impulse
     std::uint32_t limit = 0;
 };
@@ -648,6 +651,28 @@ int AddJoint(prx_world* w, const prx_revolute_joint_conf& c, std::uint32_t* out)
     return AddJointRecord(w, j, out);
 }
 
+// WheelJointConf: anchors, (frequency, damping ratio, -, motor speed), the
+// maximum motor torque, and the local x axis of A.
+int AddWheelJoint(prx_world* w, const prx_wheel_joint_conf& c, std::uint32_t* out) {
+    if (!(c.frequency >= 0.0f)) return Fail(PRX_E_INVALID_ARGUMENT, "frequency must be non-negative");
+    HostJoint j;
+    j.a = c.body_a;
+    j.b = c.body_b;
+    j.flags = (c.enable_motor ? prx::kJointMotor : 0u) | (c.collide_connected ? prx::kJointCollideConnected : 0u) |
+              (prx::kJointWheel << 8);
+    j.la[0] = c.local_anchor_a[0];
+    j.la[1] = c.local_anchor_a[1];
+    j.la[2] = c.local_anchor_b[0];
+    j.la[3] = c.local_anchor_b[1];
+    j.cfg[0] = c.frequency;
+    j.cfg[1] = c.damping_ratio;
+    j.cfg[3] = c.motor_speed;
+    j.maxTorque = c.max_motor_torque;
+    j.ext[0] = c.local_axis_a[0];
+    j.ext[1] = c.local_axis_a[1];
+    return AddJointRecord(w, j, out);
+}
+
 // MotorJointConf: the linear offset takes the anchor A slot, then (angular
 // offset, maximum force, maximum torque, correction factor).
 int AddMotorJoint(prx_world* w, const prx_motor_joint_conf& c, std::uint32_t* out) {
@@ -882,7 +907,7 @@ int UploadBatch(prx_batch* b) {
                 jcfg[g] = float4{j.cfg[0], j.cfg[1], j.cfg[2], j.cfg[3]};
                 float fl;
                 std::memcpy(&fl, &j.flags, 4);
-                jcfg2[g] = float4{j.maxTorque, fl, 0.0f, 0.0f};
+                jcfg2[g] = float4{j.maxTorque, fl, j.ext[0], j.ext[1]};
                 if (j.flags >> 8) b->joint_kinds = true;
                 jimp[g] = float4{j.imp[0], j.imp[1], j.imp[2], j.imp[3]};
                 jlimit[g] = j.limit;
@@ -1395,6 +1420,26 @@ int prx_scene_build(prx_world* w, const char* name, const std::int32_t* params, 
         if (int st = AddBody(w, c, nullptr)) return st;
     }
     for (const auto& js : spec.joints) {
+        if (js.kind == prx_scene::kWheel) {
+            prx_wheel_joint_conf wc{};
+            wc.body_a = std::uint32_t(js.a);
+            wc.body_b = std::uint32_t(js.b);
+            wc.local_anchor_a[0] = js.lax;
+            wc.local_anchor_a[1] = js.lay;
+            wc.local_anchor_b[0] = js.lbx;
+            wc.local_anchor_b[1] = js.lby;
+            const V2 axis = UnitFromAngle(js.axisAngle);
+            wc.local_axis_a[0] = axis.x;
+            wc.local_axis_a[1] = axis.y;
+            wc.enable_motor = js.enableMotor ? 1u : 0u;
+            wc.motor_speed = js.motorSpeed;
+            wc.max_motor_torque = js.maxMotorTorque;
+            wc.frequency = js.frequency;
+            wc.damping_ratio = js.dampingRatio;
+            wc.collide_connected = js.collideConnected ? 1u : 0u;
+            if (int st = AddWheelJoint(w, wc, nullptr)) return st;
+            continue;
+        }
         if (js.kind == prx_scene::kMotor) {
             prx_motor_joint_conf mc{};
             mc.body_a = std::uint32_t(js.a);
@@ -1468,6 +1513,13 @@ int prx_distance_joint_create(prx_world* w, const prx_distance_joint_conf* conf,
     return AddSpringJoint(w, prx::kJointDistance, conf->body_a, conf->body_b, conf->local_anchor_a,
                           conf->local_anchor_b, conf->length, conf->frequency, conf->damping_ratio,
                           conf->collide_connected != 0, id);
+}
+
+int prx_wheel_joint_create(prx_world* w, const prx_wheel_joint_conf* conf, std::uint32_t* id) {
+    if (int st = CheckWorld(w)) return st;
+    if (!conf) return Fail(PRX_E_INVALID_ARGUMENT, "null joint conf");
+    if (int st = Detach(w)) return st;
+    return AddWheelJoint(w, *conf, id);
 }
 
 int prx_motor_joint_create(prx_world* w, const prx_motor_joint_conf* conf, std::uint32_t* id) {
@@ -1627,11 +1679,12 @@ int prx_world_load_desc(prx_world* w, const std::uint8_t* desc, std::size_t len)
         j.b = u32();
         j.flags = u32() & 0xff07u;
         if (j.a >= fresh.bodies.size() || j.b >= fresh.bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
-        if ((j.flags >> 8) > prx::kJointMotorKind) return Fail(PRX_E_INVALID_ARGUMENT, "unknown joint kind");
-        if (j.flags >> 8) j.flags &= ~(prx::kJointLimit | prx::kJointMotor);
+        if ((j.flags >> 8) > prx::kJointWheel) return Fail(PRX_E_INVALID_ARGUMENT, "unknown joint kind");
+        if (j.flags >> 8) j.flags &= ~(prx::kJointLimit | ((j.flags >> 8) == prx::kJointWheel ? 0u : prx::kJointMotor));
         for (float& v : j.la) v = f32();
         for (float& v : j.cfg) v = f32();
         j.maxTorque = f32();
+        for (float& v : j.ext) v = f32();
         fresh.joints.push_back(j);
     }
     if (bad) return Fail(PRX_E_INVALID_ARGUMENT, "truncated description");
@@ -1692,6 +1745,7 @@ int prx_world_save_desc(prx_world* w, std::uint8_t* buf, std::size_t cap, std::s
         put(j.la, sizeof(j.la));
         put(j.cfg, sizeof(j.cfg));
         f32(j.maxTorque);
+        put(j.ext, sizeof(j.ext));
     }
     if (len) *len = o.size();
     if (buf && cap >= o.size()) std::memcpy(buf, o.data(), o.size());

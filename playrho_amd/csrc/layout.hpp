@@ -200,9 +200,10 @@ struct DeviceState {
                          // angle, frequency, damping ratio, -; rope: maximum length, -, -, -;
                          // friction: maximum force, maximum torque, -, -; motor: angular
                          // offset, maximum force, maximum torque, correction factor (the
-                         // linear offset in j_anchor.xy)
+                         // linear offset in j_anchor.xy); wheel: frequency, damping ratio,
+                         // -, motor speed
     float4* j_cfg2;      // max motor torque, flags (limit 1, motor 2, collideConnected 4,
-                         // kind << 8), -, -
+                         // kind << 8), wheel: local x axis of A (x, y)
     float4* j_imp;       // impulse x, y, z, angular motor impulse
     std::uint32_t* j_limit;  // limit state (0 inactive, 1 lower, 2 upper, 3 equal)
     int2* bjoint;        // per body: first entry in bjoint_list, count (world-local)
@@ -284,7 +285,8 @@ enum : std::uint32_t {
     kJointWeld = 2u,
     kJointRope = 3u,
     kJointFriction = 4u,
-    kJointMotorKind = 5u  // MotorJointConf (kJointMotor is the revolute motor flag)
+    kJointMotorKind = 5u,  // MotorJointConf (kJointMotor is the motor-enable flag)
+    kJointWheel = 6u
 };
 // Constraint records staged through LDS per chunk by the wave-wide sweeps;
 // the stage sits at the end of the dynamic LDS.

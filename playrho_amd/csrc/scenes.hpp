@@ -53,13 +53,14 @@ struct BodySpec {
 // (referenceAngle, frequency, dampingRatio), 3 RopeJointConf (length is the
 // maximum length), 4 FrictionJointConf (maxForce, maxTorque), 5 MotorJointConf
 // (linear offset lax, lay; angular offset referenceAngle; maxForce,
-// maxTorque, correctionFactor).
-enum JointKind : int { kRevolute = 0, kDistance = 1, kWeld = 2, kRope = 3, kFriction = 4, kMotor = 5 };
+// maxTorque, correctionFactor), 6 WheelJointConf (anchors, axis at axisAngle,
+// frequency, dampingRatio, enableMotor, motorSpeed, maxMotorTorque).
+enum JointKind : int { kRevolute = 0, kDistance = 1, kWeld = 2, kRope = 3, kFriction = 4, kMotor = 5, kWheel = 6 };
 struct JointSpec {
     int kind = kRevolute;
     int a = 0, b = 0;
     float length = 1.0f, frequency = 0.0f, dampingRatio = 0.0f;
-    float maxForce = 0.0f, maxTorque = 0.0f, correctionFactor = 0.3f;
+    float maxForce = 0.0f, maxTorque = 0.0f, correctionFactor = 0.3f, axisAngle = 0.0f;
     float lax = 0, lay = 0, lbx = 0, lby = 0;
     float referenceAngle = 0;
     bool enableLimit = false;
@@ -420,6 +421,44 @@ inline SceneSpec Joints(std::uint32_t seed, int w, int n = 8, int falling = 24) 
             wj.dampingRatio = 0.5f;
         }
         sc.joints.push_back(wj);
+    }
+    // A cart: a box chassis on two disk wheels (wheel joints along a tilted
+    // axis, soft or rigid, one with a motor).
+    {
+        sc.shapes.push_back(Box(1.2f, 0.25f, 1.0f, 0.6f, 0.0f));
+        const int chassisShape = int(sc.shapes.size()) - 1;
+        sc.shapes.push_back(Disk(0.35f, 1.0f, 0.9f, 0.0f));
+        const int wheelShape = int(sc.shapes.size()) - 1;
+        BodySpec ch;
+        ch.type = kDynamic;
+        ch.x = -30.0f;
+        ch.y = 1.4f;
+        ch.ay = -10.0f;
+        ch.shapes = {chassisShape};
+        sc.bodies.push_back(ch);
+        const int chassis = int(sc.bodies.size()) - 1;
+        for (int k = 0; k < 2; ++k) {
+            BodySpec wb;
+            wb.type = kDynamic;
+            wb.x = -30.0f + (k ? 0.9f : -0.9f);
+            wb.y = 0.5f;
+            wb.ay = -10.0f;
+            wb.shapes = {wheelShape};
+            sc.bodies.push_back(wb);
+            JointSpec wj;
+            wj.kind = kWheel;
+            wj.a = chassis;
+            wj.b = int(sc.bodies.size()) - 1;
+            wj.lax = k ? 0.9f : -0.9f;
+            wj.lay = -0.9f;
+            wj.axisAngle = 1.5707963f + (k ? 0.2f : -0.1f);
+            wj.frequency = k ? 4.0f : 0.0f;
+            wj.dampingRatio = 0.7f;
+            wj.enableMotor = (k == 1);
+            wj.motorSpeed = -6.0f;
+            wj.maxMotorTorque = 8.0f;
+            sc.joints.push_back(wj);
+        }
     }
     for (int k = 0; k < 4; ++k) {
         BodySpec b;

@@ -1267,8 +1267,8 @@ __device__ __forceinline__ bool joint_solve_position(const float4* u, float4& po
 }
 
 // ---------------------------------------------------------------------------
-// Distance, weld, rope, friction and motor joints.  Their records share the revolute record's slots
-// u[0] (rA, rB), u[3] (impulses), u[4] (inverse masses), u[5].zw (bodies),
+// Distance, weld, rope, friction, motor and wheel joints.  Their records share the revolute record's slots
+// u[0] (rA, rB; a wheel holds its axes there), u[3] (impulses), u[4] (inverse masses), u[5].zw (bodies),
 // u[6] (anchors less local centers); u[2].w holds flags | limit state << 8 |
 // kind << 16 (the revolute record's word, kind 0), limit state 0 but for ropes.
 //   distance: u[1] direction x, y, mass, invGamma   u[5].x bias
@@ -1443,6 +1443,102 @@ __device__ __forceinline__ void build_jc_friction(float4* u, const float4 cf, fl
     u[7] = float4{0.0f, 0.0f, 0.0f, __uint_as_float(prx::kJointFriction)};
 }
 
+// Wheel record: u[0] ay x, y, ax x, y; u[1] sAy, sBy, sAx, sBx; u[2].xyz
+// point-to-line mass, spring mass, angular (motor) mass; u[3] impulse, spring
+// impulse, angular impulse; u[5].xy bias, gamma; u[7] motor speed,
+// dt * maximum motor torque, local x axis of A.
+// InitVelocity(WheelJointConf&) less its warm start (WheelJointConf.cpp:105-211).
+__device__ __forceinline__ void build_jc_wheel(float4* u, const float4 cf, const float4 cf2, float4 im, V2 rA, V2 rB,
+                                               V2 qA, const float4& pA, const float4& pB, float mA, float iA,
+                                               float mB, float iB, std::uint32_t flags, const StepParams& c) {
+    const V2 axis{cf2.z, cf2.w};
+    const V2 dd = (V2{pB.x, pB.y} + rB) - (V2{pA.x, pA.y} + rA);
+    const V2 ay = rotate(revperp(axis), qA);
+    const float sAy = cross(dd + rA, ay);
+    const float sBy = cross(rB, ay);
+    const float invMass = mA + mB + iA * (sAy * sAy) / 1.0f + iB * (sBy * sBy) / 1.0f;
+    const float mass = (invMass > 0.0f) ? 1.0f / invMass : 0.0f;
+    float springMass = 0.0f, bias = 0.0f, gamma = 0.0f, sAx = 0.0f, sBx = 0.0f;
+    V2 ax{0.0f, 0.0f};
+    if (cf.x > 0.0f) {
+        ax = rotate(axis, qA);
+        sAx = cross(dd + rA, ax);
+        sBx = cross(rB, ax);
+        const float invMass2 = mA + mB + iA * (sAx * sAx) / 1.0f + iB * (sBx * sBx) / 1.0f;
+        if (invMass2 > 0.0f) {
+            springMass = 1.0f / invMass2;
+            const float C = dot(dd, ax);
+            const float omega = 2.0f * prx::kPiF * cf.x;
+            const float d = 2.0f * springMass * cf.y * omega;
+            const float k = springMass * omega * omega;
+            const float h = c.dt;
+            const float invGamma = h * (d + h * k);
+            gamma = (invGamma > 0.0f) ? 1.0f / invGamma : 0.0f;
+            bias = C * h * k * gamma;
+            const float total = invMass2 + gamma;
+            springMass = (total > 0.0f) ? 1.0f / total : 0.0f;
+        }
+    } else {
+        im.y = 0.0f;
+    }
+    float angularMass = 0.0f;
+    if (flags & prx::kJointMotor) {
+        const float invRotI = iA + iB;
+        angularMass = (invRotI > 0.0f) ? 1.0f / invRotI : 0.0f;
+    } else {
+        im.z = 0.0f;
+    }
+    im = c.warmStart ? float4{im.x * c.dtRatio, im.y * c.dtRatio, im.z * c.dtRatio, 0.0f}
+                     : float4{0.0f, 0.0f, 0.0f, 0.0f};
+    u[0] = float4{ay.x, ay.y, ax.x, ax.y};
+    u[1] = float4{sAy, sBy, sAx, sBx};
+    u[2] = float4{mass, springMass, angularMass, __uint_as_float(flags | (prx::kJointWheel << 16))};
+    u[3] = im;
+    u[5].x = bias;
+    u[5].y = gamma;
+    u[7] = float4{cf.w, c.dt * cf2.x, axis.x, axis.y};
+}
+
+// SolveVelocity(WheelJointConf&) (WheelJointConf.cpp:232-307): returns whether
+// the bodies' velocities were left unchanged.
+__device__ __forceinline__ bool wheel_solve_velocity(float4* u, float4& velA, float4& velB) {
+    const float4 a = u[0], s = u[1], n = u[2], ms = u[4], gb = u[5], mo = u[7];
+    const V2 ay{a.x, a.y}, ax{a.z, a.w};
+    const float4 oldA = velA, oldB = velB;
+    float4 im = u[3];
+    {
+        const float dv = dot(ax, V2{velB.x, velB.y} - V2{velA.x, velA.y});
+        const float Cdot = dv + s.w * velB.z / 1.0f - s.z * velA.z / 1.0f;
+        const float impulse = -n.y * (Cdot + gb.x + gb.y * im.y);
+        im.y = im.y + impulse;
+        apply_joint_impulse(ms, mul(impulse, ax), impulse * s.z / 1.0f, impulse * s.w / 1.0f, velA, velB);
+    }
+    {
+        const float Cdot = velB.z - velA.z - mo.x;
+        float impulse = -n.z * Cdot;
+        const float old = im.z;
+        im.z = clamp_std(im.z + impulse, -mo.y, mo.y);
+        impulse = im.z - old;
+        velA.z = velA.z - ms.y * impulse;
+        velB.z = velB.z + ms.w * impulse;
+    }
+    {
+        const float dv = dot(ay, V2{velB.x, velB.y} - V2{velA.x, velA.y});
+        const float Cdot = dv + s.y * velB.z / 1.0f - s.x * velA.z / 1.0f;
+        const float impulse = -n.x * Cdot;
+        im.x = im.x + impulse;
+        apply_joint_impulse(ms, mul(impulse, ay), impulse * s.x / 1.0f, impulse * s.y / 1.0f, velA, velB);
+    }
+    u[3] = im;
+    const bool same = velA.x == oldA.x && velA.y == oldA.y && velA.z == oldA.z && velB.x == oldB.x &&
+                      velB.y == oldB.y && velB.z == oldB.z;
+    if (same) {
+        velA = oldA;
+        velB = oldB;
+    }
+    return same;
+}
+
 // Motor record: u[1] linear mass m00, m01, m10, m11; u[2].xy angular mass,
 // angular error; u[5].xy dt * maximum force, dt * maximum torque; u[7] linear
 // error x, y, (1 / dt) * correction factor; impulses as friction.
@@ -1602,6 +1698,12 @@ __device__ __forceinline__ void spring_joint_warm_start(const float4* u, std::ui
     if (kind == prx::kJointDistance || kind == prx::kJointRope) {
         const V2 P = mul(im.x, V2{u[1].x, u[1].y});
         apply_joint_impulse(ms, P, cross(rA, P) / 1.0f, cross(rB, P) / 1.0f, velA, velB);
+    } else if (kind == prx::kJointWheel) {
+        // WheelJointConf.cpp:212-223.
+        const float4 a = u[0], s = u[1];
+        const V2 P{im.x * a.x + im.y * a.z, im.x * a.y + im.y * a.w};
+        apply_joint_impulse(ms, P, (im.x * s.x + im.y * s.z) / 1.0f + im.z, (im.x * s.y + im.y * s.w) / 1.0f + im.z,
+                            velA, velB);
     } else if (kind == prx::kJointFriction || kind == prx::kJointMotorKind) {
         const V2 P{im.x, im.y};
         apply_joint_impulse(ms, P, cross(rA, P) / 1.0f + im.z, cross(rB, P) / 1.0f + im.z, velA, velB);
@@ -1685,13 +1787,28 @@ __device__ __forceinline__ bool spring_joint_solve_position(const float4* u, std
     if (kind == prx::kJointFriction) return true;    // FrictionJointConf.cpp:233-237
     if (kind == prx::kJointMotorKind) return true;   // MotorJointConf.cpp:231-234
     if (kind == prx::kJointDistance && cf.y > 0.0f) return true;
-    const V2 rA = rotate(V2{la.x, la.y}, unit_from_angle(posA.z));
+    const V2 qA = unit_from_angle(posA.z);
+    const V2 rA = rotate(V2{la.x, la.y}, qA);
     const V2 rB = rotate(V2{la.z, la.w}, unit_from_angle(posB.z));
     const V2 d = (V2{posB.x, posB.y} + rB) - (V2{posA.x, posA.y} + rA);
     V2 P;
     float LA, LB;
     bool ok;
-    if (kind == prx::kJointRope) {
+    if (kind == prx::kJointWheel) {
+        // WheelJointConf.cpp:309-359; the mass uses InitVelocity's sAy, sBy.
+        const V2 dw = (V2{posB.x, posB.y} - V2{posA.x, posA.y}) + (rB - rA);
+        const V2 ay = rotate(revperp(V2{cf.z, cf.w}), qA);
+        const float sAy = cross(dw + rA, ay);
+        const float sBy = cross(rB, ay);
+        const float C = dot(dw, ay);
+        const float4 s = u[1];
+        const float k = mA + mB + iA * (s.x * s.x) / 1.0f + iB * (s.y * s.y) / 1.0f;
+        const float impulse = (k != 0.0f) ? -(C / k) : 0.0f;
+        P = mul(impulse, ay);
+        LA = impulse * sAy / 1.0f;
+        LB = impulse * sBy / 1.0f;
+        ok = __builtin_fabsf(C) <= c.linearSlop;
+    } else if (kind == prx::kJointRope) {
         // RopeJointConf.cpp:169-212.
         float length;
         const V2 dir = unit_polar(d, length);
@@ -1778,6 +1895,8 @@ __device__ __forceinline__ void build_jc(const DeviceState& d, const WorldCtx& x
         else if (kind == prx::kJointWeld) build_jc_weld(u, cf, im, rA, rB, pA, pB, mA, iA, mB, iB, flags & 0xffu, c);
         else if (kind == prx::kJointRope) build_jc_rope(u, cf, im, rA, rB, pA, pB, mA, iA, mB, iB, flags & 0xffu, c);
         else if (kind == prx::kJointFriction) build_jc_friction(u, cf, im, rA, rB, mA, iA, mB, iB, flags & 0xffu, c);
+        else if (kind == prx::kJointWheel)
+            build_jc_wheel(u, cf, cf2, im, rA, rB, qA, pA, pB, mA, iA, mB, iB, flags & 0xffu, c);
         else {
             // Body B's anchor is its origin: rB = Rotate(-localCenterB, qB).
             const V2 rBm = rotate(V2{-B.lc.x, -B.lc.y}, qB);
@@ -2235,6 +2354,7 @@ __device__ __forceinline__ bool joints_velocity_k(float4* jbase, int j0, int j1,
             else if (kind == prx::kJointWeld) ok &= weld_solve_velocity(u, va, vb);
             else if (kind == prx::kJointRope) ok &= rope_solve_velocity(u, va, vb);
             else if (kind == prx::kJointFriction) ok &= friction_solve_velocity(u, va, vb);
+            else if (kind == prx::kJointWheel) ok &= wheel_solve_velocity(u, va, vb);
             else ok &= motor_solve_velocity(u, va, vb);
             bs.set_vel(bA, va);
             bs.set_vel(bB, vb);

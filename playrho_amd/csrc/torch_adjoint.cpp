@@ -140,7 +140,7 @@ Tensor billiards_loss(const Tensor& v0, const Tensor& friction, const Tensor& re
 
 // Byte offsets of the joint records' maximum motor torque in a world
 // description (layout of prx_world_save_desc: shapes, bodies, then joints of
-// 48 bytes each with the torque last).
+// 60 bytes each: bodies and flags, 12 floats with the torque the ninth).
 std::vector<std::size_t> TorqueOffsets(const std::uint8_t* d, std::size_t len) {
     auto u32 = [&](std::size_t o) {
         TORCH_CHECK(o + 4 <= len, "truncated world description");
@@ -157,7 +157,7 @@ std::vector<std::size_t> TorqueOffsets(const std::uint8_t* d, std::size_t len) {
     const std::uint32_t nj = u32(off);
     off += 4;
     std::vector<std::size_t> out;
-    for (std::uint32_t k = 0; k < nj; ++k) out.push_back(off + 48 * std::size_t(k) + 44);
+    for (std::uint32_t k = 0; k < nj; ++k) out.push_back(off + 60 * std::size_t(k) + 44);
     TORCH_CHECK(out.empty() || out.back() + 4 <= len, "truncated world description");
     return out;
 }

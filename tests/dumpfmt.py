@@ -17,7 +17,7 @@ BODY_F = 10
 CONTACT_U = 4
 CONTACT_T = 3
 CONTACT_F = 14
-JOINT_DESC_F = 9
+JOINT_DESC_F = 12
 JOINT_F = 4
 # Listener kinds: 0 begin, 1 end, 2 pre-solve (old manifold: type | count << 8,
 # two contact features, 12 floats), 3 post-solve (count, solved position

@@ -38,7 +38,8 @@ CASES = [
     ("chain0", "chain", 240, [1, 0], [1, 120, 240]),
     ("chain1", "chain", 240, [1, 1, 12, 24, 40], [1, 240]),
     # Distance and weld joints, rigid and soft, friction joints on the ground,
-    # ropes from a bar and motor joints, beside falling bodies.
+    # ropes from a bar, motor joints and a cart on wheel joints, beside falling
+    # bodies.
     ("joints0", "joints", 240, [1, 0], [1, 120, 240]),
     ("joints1", "joints", 240, [2, 1, 12, 40], [1, 240]),
 ]

@@ -172,18 +172,19 @@ def test_contact_event_recording_switch_host_side():
     w.set_contact_listener()  # clearing with nothing set is allowed
 
 
-def test_distance_weld_rope_friction_motor_joint_creation_matches_scene():
-    """Distance, weld, rope, friction and motor joints created through
+def test_non_revolute_joint_creation_matches_scene():
+    """Distance, weld, rope, friction, motor and wheel joints created through
     prx_distance_joint_create / prx_weld_joint_create / prx_rope_joint_create /
-    prx_friction_joint_create / prx_motor_joint_create give the same world
-    description as the scene the reference built (kind in bits 8-15 of the
-    flags, then length or reference angle, frequency, damping ratio; maximum
-    length; maximum force and torque; offsets and correction factor)."""
+    prx_friction_joint_create / prx_motor_joint_create / prx_wheel_joint_create
+    give the same world description as the scene the reference built (kind in
+    bits 8-15 of the flags, then length or reference angle, frequency, damping
+    ratio; maximum length; maximum force and torque; offsets and correction
+    factor; axis, motor)."""
     g = golden_cases.load("joints0")
     desc = _desc_only(g["desc_conf"])
     recs = dumpfmt.read_description(dumpfmt.Reader(desc))[2]
     kinds = {r["flags"] >> 8 for r in recs}
-    assert kinds == {1, 2, 3, 4, 5}
+    assert kinds == {1, 2, 3, 4, 5, 6}
     offs = pa.desc_joint_offsets(desc)
     bare = desc[:offs[0] - 4] + struct.pack("<I", 0)
     w = pa.World.from_desc(bare)
@@ -199,6 +200,10 @@ def test_distance_weld_rope_friction_motor_joint_creation_matches_scene():
             w.create_joint(pa.WeldJointConf(reference_angle=v[4], **common))
         elif kind == 3:
             w.create_joint(pa.RopeJointConf(max_length=v[4], **base))
+        elif kind == 6:
+            w.create_joint(pa.WheelJointConf(local_axis_a=(v[9], v[10]), enable_motor=(r["flags"] >> 1) & 1,
+                                             motor_speed=v[7], max_motor_torque=v[8], frequency=v[4],
+                                             damping_ratio=v[5], **base))
         elif kind == 5:
             w.create_joint(pa.MotorJointConf(body_a=r["a"], body_b=r["b"], linear_offset=(v[0], v[1]),
                                              angular_offset=v[4], max_force=v[5], max_torque=v[6],
@@ -217,3 +222,5 @@ def test_distance_weld_rope_friction_motor_joint_creation_matches_scene():
     assert tuple(pa.RopeJointConf().local_anchor_a) == (-1.0, 0.0)
     m = pa.MotorJointConf()
     assert (m.max_force, m.max_torque, round(m.correction_factor, 6)) == (1.0, 1.0, 0.3)
+    wh = pa.WheelJointConf()
+    assert (tuple(wh.local_axis_a), wh.frequency, round(wh.damping_ratio, 6)) == ((1.0, 0.0), 2.0, 0.7)
