@@ -164,6 +164,18 @@ typedef struct prx_wheel_joint_conf {
     uint32_t collide_connected;
 } prx_wheel_joint_conf;
 
+/* PulleyJointConf (d2/PulleyJointConf.hpp:65-137): lengthA + ratio * lengthB
+ * held at its initial value.  The reference's defaults are ground anchors
+ * (-1, 1) and (1, 1), local anchors (-1, 0) and (1, 0), lengths 0, ratio 1,
+ * collide_connected 1. */
+typedef struct prx_pulley_joint_conf {
+    uint32_t body_a, body_b;
+    float ground_anchor_a[2], ground_anchor_b[2];
+    float local_anchor_a[2], local_anchor_b[2];
+    float length_a, length_b, ratio;
+    uint32_t collide_connected;
+} prx_pulley_joint_conf;
+
 /* StepConf fields the regular step reads (StepConf.hpp:49-392). */
 typedef struct prx_step_conf {
     float delta_time, dt_ratio, min_still_time_to_sleep, linear_slop, angular_slop;
@@ -250,6 +262,9 @@ int prx_friction_joint_create(prx_world* world, const prx_friction_joint_conf* c
 int prx_motor_joint_create(prx_world* world, const prx_motor_joint_conf* conf, uint32_t* joint_id);
 /* CreateJoint(World&, const WheelJointConf&) (solver WheelJointConf.cpp:105-359). */
 int prx_wheel_joint_create(prx_world* world, const prx_wheel_joint_conf* conf, uint32_t* joint_id);
+/* CreateJoint(World&, const PulleyJointConf&) (solver PulleyJointConf.cpp:92-253;
+ * prx_world_shift_origin moves its ground anchors, PulleyJointConf.cpp:255-260). */
+int prx_pulley_joint_create(prx_world* world, const prx_pulley_joint_conf* conf, uint32_t* joint_id);
 uint32_t prx_world_joint_count(const prx_world* world);
 /* Builds a named workload scene through the calls above. */
 int prx_scene_build(prx_world* world, const char* name, const int32_t* params, int nparams);

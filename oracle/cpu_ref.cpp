@@ -126,7 +126,15 @@ void Load(Handle<R>& h, In& in) {
             if (j.kind != 6) j.enableMotor = false;
             if (j.kind == 1 || j.kind == 3) j.length = f[4];
             if (j.kind != 2 && j.kind != 5) j.referenceAngle = 0;
-            if (j.kind == 6) {
+            if (j.kind == 7) {
+                // Pulley: ground anchors, ratio, lengths.
+                j.groundA = V2<R>{f[4], f[5]};
+                j.groundB = V2<R>{f[6], f[7]};
+                j.ratio = f[8];
+                j.length = f[9];
+                j.lengthB = f[10];
+                j.frequency = j.dampingRatio = 0;
+            } else if (j.kind == 6) {
                 // Wheel: frequency, damping ratio, -, motor speed, maximum
                 // motor torque, local x axis of A.
                 j.frequency = f[4];

@@ -799,6 +799,10 @@ struct Joint {
     // angular (motor) impulse; dmass the point-to-line mass.
     V2<R> localXAxis, ax, ay;
     R sAx = 0, sAy = 0, sBx = 0, sBy = 0, springMass = 0;
+    // kind 7 pulley: ground anchors, lengths (length, lengthB), ratio; uA in
+    // u, uB in ay; impulse in impulse.x, mass in dmass.
+    V2<R> groundA, groundB;
+    R lengthB = 0, ratio = 1;
 };
 
 template <class R>
@@ -2165,6 +2169,91 @@ public:
         return std::abs(C) <= conf.linearSlop;
     }
 
+    // InitVelocity(PulleyJointConf&) (PulleyJointConf.cpp:92-153).
+    static void PulleyInitVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        const R mA = A.invMass, iA = A.invRotI, mB = B.invMass, iB = B.invRotI;
+        Vel<R> velA = A.vel, velB = B.vel;
+        j.rA = Rotate(j.laA - A.lc, UnitFromAngle(A.pos.ang));
+        j.rB = Rotate(j.laB - B.lc, UnitFromAngle(B.pos.ang));
+        j.u = UnitVector((A.pos.lin + j.rA) - j.groundA, V2<R>{R(0), R(0)});
+        j.ay = UnitVector((B.pos.lin + j.rB) - j.groundB, V2<R>{R(0), R(0)});
+        const R ruA = Cross(j.rA, j.u);
+        const R ruB = Cross(j.rB, j.ay);
+        const R tA = mA + (iA * (ruA * ruA)) / R(1);
+        const R tB = mB + (iB * (ruB * ruB)) / R(1);
+        const R total = tA + j.ratio * j.ratio * tB;
+        j.dmass = (total > R(0)) ? R(1) / total : R(0);
+        if (conf.warmStart) {
+            j.impulse.x *= conf.dtRatio;
+            const V2<R> PA = (-j.impulse.x) * j.u;
+            const V2<R> PB = (-j.ratio * j.impulse.x) * j.ay;
+            velA.lin = velA.lin + mA * PA;
+            velA.ang = velA.ang + iA * Cross(j.rA, PA) / R(1);
+            velB.lin = velB.lin + mB * PB;
+            velB.ang = velB.ang + iB * Cross(j.rB, PB) / R(1);
+        } else {
+            j.impulse.x = 0;
+        }
+        A.vel = velA;
+        B.vel = velB;
+    }
+
+    // SolveVelocity(PulleyJointConf&) (PulleyJointConf.cpp:155-190).
+    static bool PulleySolveVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        Vel<R> velA = A.vel, velB = B.vel;
+        const V2<R> vpA = velA.lin + RevPerp(j.rA) * (velA.ang / R(1));
+        const V2<R> vpB = velB.lin + RevPerp(j.rB) * (velB.ang / R(1));
+        const R Cdot = -Dot(j.u, vpA) - j.ratio * Dot(j.ay, vpB);
+        const R impulse = -j.dmass * Cdot;
+        j.impulse.x += impulse;
+        const V2<R> PA = -impulse * j.u;
+        const V2<R> PB = -j.ratio * impulse * j.ay;
+        velA.lin = velA.lin + A.invMass * PA;
+        velA.ang = velA.ang + A.invRotI * Cross(j.rA, PA) / R(1);
+        velB.lin = velB.lin + B.invMass * PB;
+        velB.ang = velB.ang + B.invRotI * Cross(j.rB, PB) / R(1);
+        A.vel = velA;
+        B.vel = velB;
+        return impulse == R(0);
+    }
+
+    // SolvePosition(const PulleyJointConf&) (PulleyJointConf.cpp:192-253).
+    static bool PulleySolvePosition(const Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        const R mA = A.invMass, iA = A.invRotI, mB = B.invMass, iB = B.invRotI;
+        Pos<R> posA = A.pos, posB = B.pos;
+        const V2<R> rA = Rotate(j.laA - A.lc, UnitFromAngle(posA.ang));
+        const V2<R> rB = Rotate(j.laB - B.lc, UnitFromAngle(posB.ang));
+        R lengthA, lengthB;
+        const V2<R> uA = UnitPolar((posA.lin + rA) - j.groundA, lengthA);
+        const V2<R> uB = UnitPolar((posB.lin + rB) - j.groundB, lengthB);
+        const R ruA = Cross(rA, uA);
+        const R ruB = Cross(rB, uB);
+        const R tA = mA + iA * (ruA * ruA) / R(1);
+        const R tB = mB + iB * (ruB * ruB) / R(1);
+        const R total = tA + (j.ratio * j.ratio) * tB;
+        const R mass = (total > R(0)) ? R(1) / total : R(0);
+        const R src = j.length + j.ratio * j.lengthB;
+        const R dst = lengthA + j.ratio * lengthB;
+        const R C = src - dst;
+        const R linearError = std::abs(C);
+        const R impulse = -mass * C;
+        const V2<R> PA = -impulse * uA;
+        const V2<R> PB = -j.ratio * impulse * uB;
+        posA.lin = posA.lin + mA * PA;
+        posA.ang = posA.ang + iA * Cross(rA, PA) / R(1);
+        posB.lin = posB.lin + mB * PB;
+        posB.ang = posB.ang + iB * Cross(rB, PB) / R(1);
+        A.pos = posA;
+        B.pos = posB;
+        return linearError < conf.linearSlop;
+    }
+
     // InitVelocity(RevoluteJointConf&) (RevoluteJointConf.cpp:127-226).
     static void JointInitVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
         if (j.kind == 1) return DistanceInitVelocity(j, bcs, conf);
@@ -2173,6 +2262,7 @@ public:
         if (j.kind == 4) return FrictionInitVelocity(j, bcs, conf);
         if (j.kind == 5) return MotorInitVelocity(j, bcs, conf);
         if (j.kind == 6) return WheelInitVelocity(j, bcs, conf);
+        if (j.kind == 7) return PulleyInitVelocity(j, bcs, conf);
         BodyConstraint<R>& A = bcs[j.bodyA];
         BodyConstraint<R>& B = bcs[j.bodyB];
         const R mA = A.invMass, iA = A.invRotI, aA = A.pos.ang;
@@ -2237,6 +2327,7 @@ public:
         if (j.kind == 4) return FrictionSolveVelocity(j, bcs, conf);
         if (j.kind == 5) return MotorSolveVelocity(j, bcs, conf);
         if (j.kind == 6) return WheelSolveVelocity(j, bcs, conf);
+        if (j.kind == 7) return PulleySolveVelocity(j, bcs);
         BodyConstraint<R>& A = bcs[j.bodyA];
         BodyConstraint<R>& B = bcs[j.bodyB];
         const Vel<R> oldVelA = A.vel, oldVelB = B.vel;
@@ -2314,6 +2405,7 @@ public:
         if (j.kind == 4) return true;  // FrictionJointConf.cpp:233-237
         if (j.kind == 5) return true;  // MotorJointConf.cpp:231-234
         if (j.kind == 6) return WheelSolvePosition(j, bcs, conf);
+        if (j.kind == 7) return PulleySolvePosition(j, bcs, conf);
         BodyConstraint<R>& A = bcs[j.bodyA];
         BodyConstraint<R>& B = bcs[j.bodyB];
         Pos<R> posA = A.pos, posB = B.pos;
@@ -3301,6 +3393,13 @@ public:
     void SetAngle(std::uint32_t bid, R a) { SetSweepPosition(bid, Pos<R>{bodies[bid].pos1.lin, a}); }
     // ShiftOrigin (AabbTreeWorld.cpp:2221-2242, DynamicTree.cpp:650-658).
     void ShiftOrigin(V2<R> o) {
+        // ShiftOrigin(PulleyJointConf&) (PulleyJointConf.cpp:255-260); the
+        // other joint kinds hold nothing in world coordinates.
+        for (Joint<R>& j : joints) {
+            if (j.kind != 7) continue;
+            j.groundA = j.groundA - o;
+            j.groundB = j.groundB - o;
+        }
         for (std::uint32_t bid : bodyOrder) {
             Body<R>& b = bodies[bid];
             b.pos0.lin = b.pos0.lin - o;

@@ -42,6 +42,7 @@
 #include <playrho/d2/FrictionJointConf.hpp>
 #include <playrho/d2/MotorJointConf.hpp>
 #include <playrho/d2/WheelJointConf.hpp>
+#include <playrho/d2/PulleyJointConf.hpp>
 #include <playrho/d2/WorldJoint.hpp>
 #include <playrho/StepConf.hpp>
 
@@ -208,6 +209,15 @@ void WriteDescription(Out& o, const World& world) {
             o.u32((j->collideConnected ? 4u : 0u) | (4u << 8));
             const float f[12] = {GetX(j->localAnchorA), GetY(j->localAnchorA), GetX(j->localAnchorB),
                                 GetY(j->localAnchorB), j->maxForce, j->maxTorque, 0.0f, 0.0f, 0.0f};
+            o.put(f, sizeof(f));
+        } else if (const auto* j = TypeCast<const PulleyJointConf>(&joint)) {
+            o.u32(to_underlying(GetBodyA(*j)));
+            o.u32(to_underlying(GetBodyB(*j)));
+            o.u32((j->collideConnected ? 4u : 0u) | (7u << 8));
+            const float f[12] = {GetX(j->localAnchorA),  GetY(j->localAnchorA),  GetX(j->localAnchorB),
+                                 GetY(j->localAnchorB),  GetX(j->groundAnchorA), GetY(j->groundAnchorA),
+                                 GetX(j->groundAnchorB), GetY(j->groundAnchorB), j->ratio,
+                                 j->lengthA,             j->lengthB,             0.0f};
             o.put(f, sizeof(f));
         } else if (const auto* j = TypeCast<const WheelJointConf>(&joint)) {
             o.u32(to_underlying(GetBodyA(*j)));
@@ -425,6 +435,10 @@ void WriteState(Out& o, const World& world, std::uint32_t step, const StepStats&
             o.u32(0u);
             const float f[4] = {GetX(j->linearImpulse), GetY(j->linearImpulse), j->angularImpulse, 0.0f};
             o.put(f, sizeof(f));
+        } else if (const auto* j = TypeCast<const PulleyJointConf>(&joint)) {
+            o.u32(0u);
+            const float f[4] = {j->impulse, 0.0f, 0.0f, 0.0f};
+            o.put(f, sizeof(f));
         } else if (const auto* j = TypeCast<const WheelJointConf>(&joint)) {
             o.u32(0u);
             const float f[4] = {j->impulse, j->springImpulse, j->angularImpulse, 0.0f};
@@ -484,6 +498,12 @@ World BuildWorld(const prx_scene::SceneSpec& spec) {
         } else if (js.kind == prx_scene::kFriction) {
             auto jc = FrictionJointConf{ba, bb, Length2{js.lax, js.lay}, Length2{js.lbx, js.lby}};
             jc.UseMaxForce(js.maxForce).UseMaxTorque(js.maxTorque);
+            jc.UseCollideConnected(js.collideConnected);
+            CreateJoint(world, jc);
+        } else if (js.kind == prx_scene::kPulley) {
+            auto jc = PulleyJointConf{ba, bb, Length2{js.gax, js.gay}, Length2{js.gbx, js.gby},
+                                      Length2{js.lax, js.lay}, Length2{js.lbx, js.lby}, js.length, js.lengthB};
+            jc.UseRatio(js.ratio);
             jc.UseCollideConnected(js.collideConnected);
             CreateJoint(world, jc);
         } else if (js.kind == prx_scene::kWheel) {

@@ -221,6 +221,26 @@ class WheelJointConf(ctypes.Structure):
         super().__init__(**kw)
 
 
+class PulleyJointConf(ctypes.Structure):
+    """PulleyJointConf (d2/PulleyJointConf.hpp:65-137); the reference's defaults:
+    ground anchors (-1, 1), (1, 1), local anchors (-1, 0), (1, 0), lengths 0,
+    ratio 1, collide_connected 1."""
+    _fields_ = [("body_a", ctypes.c_uint32), ("body_b", ctypes.c_uint32),
+                ("ground_anchor_a", ctypes.c_float * 2), ("ground_anchor_b", ctypes.c_float * 2),
+                ("local_anchor_a", ctypes.c_float * 2), ("local_anchor_b", ctypes.c_float * 2),
+                ("length_a", ctypes.c_float), ("length_b", ctypes.c_float), ("ratio", ctypes.c_float),
+                ("collide_connected", ctypes.c_uint32)]
+
+    def __init__(self, **kw):
+        kw.setdefault("ground_anchor_a", (-1.0, 1.0))
+        kw.setdefault("ground_anchor_b", (1.0, 1.0))
+        kw.setdefault("local_anchor_a", (-1.0, 0.0))
+        kw.setdefault("local_anchor_b", (1.0, 0.0))
+        kw.setdefault("ratio", 1.0)
+        kw.setdefault("collide_connected", 1)
+        super().__init__(**kw)
+
+
 class WeldJointConf(ctypes.Structure):
     """WeldJointConf (d2/WeldJointConf.hpp:62-130); frequency 0 is rigid."""
     _fields_ = [("body_a", ctypes.c_uint32), ("body_b", ctypes.c_uint32),
@@ -278,6 +298,7 @@ _SIGS = {
     "prx_friction_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
     "prx_motor_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
     "prx_wheel_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
+    "prx_pulley_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
     "prx_world_joint_count": (ctypes.c_uint32, [ctypes.c_void_p]),
     "prx_world_shape_count": (ctypes.c_uint32, [ctypes.c_void_p]),
     "prx_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
@@ -413,14 +434,15 @@ class World:
 
     def create_joint(self, conf):
         """CreateJoint with a RevoluteJointConf, DistanceJointConf,
-        WeldJointConf, RopeJointConf, FrictionJointConf, MotorJointConf or
-        WheelJointConf; returns the joint id
+        WeldJointConf, RopeJointConf, FrictionJointConf, MotorJointConf,
+        WheelJointConf or PulleyJointConf; returns the joint id
         (shared by all kinds)."""
         fn = {RevoluteJointConf: lib().prx_joint_create, DistanceJointConf: lib().prx_distance_joint_create,
               WeldJointConf: lib().prx_weld_joint_create, RopeJointConf: lib().prx_rope_joint_create,
               FrictionJointConf: lib().prx_friction_joint_create,
               MotorJointConf: lib().prx_motor_joint_create,
-              WheelJointConf: lib().prx_wheel_joint_create}.get(type(conf))
+              WheelJointConf: lib().prx_wheel_joint_create,
+              PulleyJointConf: lib().prx_pulley_joint_create}.get(type(conf))
         if fn is None:
             raise TypeError("not a joint conf: %r" % type(conf))
         jid = ctypes.c_uint32()

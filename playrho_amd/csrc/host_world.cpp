@@ -265,7 +265,7 @@ struct HostJoint {
     // maximum torque, 0, 0.  Motor: angular offset, maximum force, maximum
     // torque, correction factor (linear offset in la[0..1]).  Wheel: frequency,
     // damping ratio, 0, motor speed (maximum motor torque in maxTorque, axis in
-    // ext).
+    // ext).  Pulley: ground anchors A, B (ratio in maxTorque, lengths in ext).
     float cfg[4] = {0, 0, 0, 0};
     float maxTorque = 0;
     float ext[3] = {0, 0, 0};               // wheel: local x axis of A (x, y), -
@@ -648,6 +648,27 @@ int AddJoint(prx_world* w, const prx_revolute_joint_conf& c, std::uint32_t* out)
     j.cfg[2] = c.upper_angle;
     j.cfg[3] = c.motor_speed;
     j.maxTorque = c.max_motor_torque;
+    return AddJointRecord(w, j, out);
+}
+
+// PulleyJointConf: anchors, ground anchors in cfg, the ratio in the maximum
+// torque slot, the lengths in ext.
+int AddPulleyJoint(prx_world* w, const prx_pulley_joint_conf& c, std::uint32_t* out) {
+    HostJoint j;
+    j.a = c.body_a;
+    j.b = c.body_b;
+    j.flags = (c.collide_connected ? prx::kJointCollideConnected : 0u) | (prx::kJointPulley << 8);
+    j.la[0] = c.local_anchor_a[0];
+    j.la[1] = c.local_anchor_a[1];
+    j.la[2] = c.local_anchor_b[0];
+    j.la[3] = c.local_anchor_b[1];
+    j.cfg[0] = c.ground_anchor_a[0];
+    j.cfg[1] = c.ground_anchor_a[1];
+    j.cfg[2] = c.ground_anchor_b[0];
+    j.cfg[3] = c.ground_anchor_b[1];
+    j.maxTorque = c.ratio;
+    j.ext[0] = c.length_a;
+    j.ext[1] = c.length_b;
     return AddJointRecord(w, j, out);
 }
 
@@ -1420,6 +1441,25 @@ int prx_scene_build(prx_world* w, const char* name, const std::int32_t* params, 
         if (int st = AddBody(w, c, nullptr)) return st;
     }
     for (const auto& js : spec.joints) {
+        if (js.kind == prx_scene::kPulley) {
+            prx_pulley_joint_conf pc{};
+            pc.body_a = std::uint32_t(js.a);
+            pc.body_b = std::uint32_t(js.b);
+            pc.ground_anchor_a[0] = js.gax;
+            pc.ground_anchor_a[1] = js.gay;
+            pc.ground_anchor_b[0] = js.gbx;
+            pc.ground_anchor_b[1] = js.gby;
+            pc.local_anchor_a[0] = js.lax;
+            pc.local_anchor_a[1] = js.lay;
+            pc.local_anchor_b[0] = js.lbx;
+            pc.local_anchor_b[1] = js.lby;
+            pc.length_a = js.length;
+            pc.length_b = js.lengthB;
+            pc.ratio = js.ratio;
+            pc.collide_connected = js.collideConnected ? 1u : 0u;
+            if (int st = AddPulleyJoint(w, pc, nullptr)) return st;
+            continue;
+        }
         if (js.kind == prx_scene::kWheel) {
             prx_wheel_joint_conf wc{};
             wc.body_a = std::uint32_t(js.a);
@@ -1513,6 +1553,13 @@ int prx_distance_joint_create(prx_world* w, const prx_distance_joint_conf* conf,
     return AddSpringJoint(w, prx::kJointDistance, conf->body_a, conf->body_b, conf->local_anchor_a,
                           conf->local_anchor_b, conf->length, conf->frequency, conf->damping_ratio,
                           conf->collide_connected != 0, id);
+}
+
+int prx_pulley_joint_create(prx_world* w, const prx_pulley_joint_conf* conf, std::uint32_t* id) {
+    if (int st = CheckWorld(w)) return st;
+    if (!conf) return Fail(PRX_E_INVALID_ARGUMENT, "null joint conf");
+    if (int st = Detach(w)) return st;
+    return AddPulleyJoint(w, *conf, id);
 }
 
 int prx_wheel_joint_create(prx_world* w, const prx_wheel_joint_conf* conf, std::uint32_t* id) {
@@ -1679,7 +1726,7 @@ int prx_world_load_desc(prx_world* w, const std::uint8_t* desc, std::size_t len)
         j.b = u32();
         j.flags = u32() & 0xff07u;
         if (j.a >= fresh.bodies.size() || j.b >= fresh.bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
-        if ((j.flags >> 8) > prx::kJointWheel) return Fail(PRX_E_INVALID_ARGUMENT, "unknown joint kind");
+        if ((j.flags >> 8) > prx::kJointPulley) return Fail(PRX_E_INVALID_ARGUMENT, "unknown joint kind");
         if (j.flags >> 8) j.flags &= ~(prx::kJointLimit | ((j.flags >> 8) == prx::kJointWheel ? 0u : prx::kJointMotor));
         for (float& v : j.la) v = f32();
         for (float& v : j.cfg) v = f32();
@@ -1853,7 +1900,12 @@ int prx_world_shift_origin(prx_world* w, const float origin[2]) {
     if (!origin) return Fail(PRX_E_INVALID_ARGUMENT, "null origin");
     if (int st = Detach(w)) return st;
     // ShiftOrigin (AabbTreeWorld.cpp:2221-2242): sweeps, every contact flagged
-    // for updating, and the tree's AABBs (DynamicTree.cpp:650-658).
+    // for updating, and the tree's AABBs (DynamicTree.cpp:650-658).  Pulleys
+    // move their ground anchors (PulleyJointConf.cpp:255-260).
+    for (HostJoint& j : w->joints) {
+        if ((j.flags >> 8) != prx::kJointPulley) continue;
+        for (int k = 0; k < 4; ++k) j.cfg[k] = j.cfg[k] - origin[k & 1];
+    }
     for (HostBody& b : w->bodies) {
         for (int k = 0; k < 2; ++k) {
             b.pos0[k] = b.pos0[k] - origin[k];

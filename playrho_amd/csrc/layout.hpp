@@ -201,7 +201,8 @@ struct DeviceState {
                          // friction: maximum force, maximum torque, -, -; motor: angular
                          // offset, maximum force, maximum torque, correction factor (the
                          // linear offset in j_anchor.xy); wheel: frequency, damping ratio,
-                         // -, motor speed
+                         // -, motor speed; pulley: ground anchors A, B (j_cfg2: ratio, flags,
+                         // lengths A, B)
     float4* j_cfg2;      // max motor torque, flags (limit 1, motor 2, collideConnected 4,
                          // kind << 8), wheel: local x axis of A (x, y)
     float4* j_imp;       // impulse x, y, z, angular motor impulse
@@ -286,7 +287,8 @@ enum : std::uint32_t {
     kJointRope = 3u,
     kJointFriction = 4u,
     kJointMotorKind = 5u,  // MotorJointConf (kJointMotor is the motor-enable flag)
-    kJointWheel = 6u
+    kJointWheel = 6u,
+    kJointPulley = 7u
 };
 // Constraint records staged through LDS per chunk by the wave-wide sweeps;
 // the stage sits at the end of the dynamic LDS.

@@ -54,13 +54,18 @@ struct BodySpec {
 // maximum length), 4 FrictionJointConf (maxForce, maxTorque), 5 MotorJointConf
 // (linear offset lax, lay; angular offset referenceAngle; maxForce,
 // maxTorque, correctionFactor), 6 WheelJointConf (anchors, axis at axisAngle,
-// frequency, dampingRatio, enableMotor, motorSpeed, maxMotorTorque).
-enum JointKind : int { kRevolute = 0, kDistance = 1, kWeld = 2, kRope = 3, kFriction = 4, kMotor = 5, kWheel = 6 };
+// frequency, dampingRatio, enableMotor, motorSpeed, maxMotorTorque), 7
+// PulleyJointConf (anchors, ground anchors gax, gay, gbx, gby, lengths
+// length, lengthB, ratio).
+enum JointKind : int {
+    kRevolute = 0, kDistance = 1, kWeld = 2, kRope = 3, kFriction = 4, kMotor = 5, kWheel = 6, kPulley = 7
+};
 struct JointSpec {
     int kind = kRevolute;
     int a = 0, b = 0;
     float length = 1.0f, frequency = 0.0f, dampingRatio = 0.0f;
     float maxForce = 0.0f, maxTorque = 0.0f, correctionFactor = 0.3f, axisAngle = 0.0f;
+    float gax = 0, gay = 0, gbx = 0, gby = 0, lengthB = 0, ratio = 1;
     float lax = 0, lay = 0, lbx = 0, lby = 0;
     float referenceAngle = 0;
     bool enableLimit = false;
@@ -459,6 +464,34 @@ inline SceneSpec Joints(std::uint32_t seed, int w, int n = 8, int falling = 24) 
             wj.maxMotorTorque = 8.0f;
             sc.joints.push_back(wj);
         }
+    }
+    // Two boxes on a pulley (ratio 1.5) from ground anchors at y = 28.
+    {
+        sc.shapes.push_back(Box(0.5f, 0.5f, 1.0f, 0.5f, 0.0f));
+        const int boxShape = int(sc.shapes.size()) - 1;
+        const float xs[2] = {35.0f, 40.0f}, ys[2] = {20.0f, 22.0f};
+        for (int k = 0; k < 2; ++k) {
+            BodySpec pb;
+            pb.type = kDynamic;
+            pb.x = xs[k];
+            pb.y = ys[k];
+            pb.ay = -10.0f;
+            pb.shapes = {boxShape};
+            sc.bodies.push_back(pb);
+        }
+        JointSpec pj;
+        pj.kind = kPulley;
+        pj.a = int(sc.bodies.size()) - 2;
+        pj.b = int(sc.bodies.size()) - 1;
+        pj.gax = xs[0];
+        pj.gay = 28.0f;
+        pj.gbx = xs[1];
+        pj.gby = 28.0f;
+        pj.length = 28.0f - ys[0];
+        pj.lengthB = 28.0f - ys[1];
+        pj.ratio = 1.5f;
+        pj.collideConnected = true;
+        sc.joints.push_back(pj);
     }
     for (int k = 0; k < 4; ++k) {
         BodySpec b;

@@ -1267,7 +1267,7 @@ __device__ __forceinline__ bool joint_solve_position(const float4* u, float4& po
 }
 
 // ---------------------------------------------------------------------------
-// Distance, weld, rope, friction, motor and wheel joints.  Their records share the revolute record's slots
+// Distance, weld, rope, friction, motor, wheel and pulley joints.  Their records share the revolute record's slots
 // u[0] (rA, rB; a wheel holds its axes there), u[3] (impulses), u[4] (inverse masses), u[5].zw (bodies),
 // u[6] (anchors less local centers); u[2].w holds flags | limit state << 8 |
 // kind << 16 (the revolute record's word, kind 0), limit state 0 but for ropes.
@@ -1539,6 +1539,58 @@ __device__ __forceinline__ bool wheel_solve_velocity(float4* u, float4& velA, fl
     return same;
 }
 
+// Pulley record: u[1] uA x, y, uB x, y; u[2].xy mass, ratio; u[5].xy lengths
+// A, B; u[7] ground anchors A, B; impulse in u[3].x.
+// InitVelocity(PulleyJointConf&) less its warm start (PulleyJointConf.cpp:92-141).
+__device__ __forceinline__ void build_jc_pulley(float4* u, const float4 cf, const float4 cf2, float4 im, V2 rA, V2 rB,
+                                                const float4& pA, const float4& pB, float mA, float iA, float mB,
+                                                float iB, std::uint32_t flags, const StepParams& c) {
+    const float ratio = cf2.x;
+    const V2 uA = unit_vector((V2{pA.x, pA.y} + rA) - V2{cf.x, cf.y}, V2{0.0f, 0.0f});
+    const V2 uB = unit_vector((V2{pB.x, pB.y} + rB) - V2{cf.z, cf.w}, V2{0.0f, 0.0f});
+    const float ruA = cross(rA, uA);
+    const float ruB = cross(rB, uB);
+    const float tA = mA + (iA * (ruA * ruA)) / 1.0f;
+    const float tB = mB + (iB * (ruB * ruB)) / 1.0f;
+    const float total = tA + ratio * ratio * tB;
+    const float mass = (total > 0.0f) ? 1.0f / total : 0.0f;
+    im = c.warmStart ? float4{im.x * c.dtRatio, 0.0f, 0.0f, 0.0f} : float4{0.0f, 0.0f, 0.0f, 0.0f};
+    u[1] = float4{uA.x, uA.y, uB.x, uB.y};
+    u[2] = float4{mass, ratio, 0.0f, __uint_as_float(flags | (prx::kJointPulley << 16))};
+    u[3] = im;
+    u[5].x = cf2.z;
+    u[5].y = cf2.w;
+    u[7] = cf;
+}
+
+// Applies a pulley impulse: both bodies gain (PulleyJointConf.cpp:135-140,
+// 178-186).
+__device__ __forceinline__ void pulley_apply(const float4* u, float impulse, float4& velA, float4& velB) {
+    const float4 r = u[0], uv = u[1], ms = u[4];
+    const float ratio = u[2].y;
+    const V2 PA = mul(-impulse, V2{uv.x, uv.y});
+    const V2 PB = mul(-ratio * impulse, V2{uv.z, uv.w});
+    velA.x = velA.x + ms.x * PA.x;
+    velA.y = velA.y + ms.x * PA.y;
+    velA.z = velA.z + ms.y * cross(V2{r.x, r.y}, PA) / 1.0f;
+    velB.x = velB.x + ms.z * PB.x;
+    velB.y = velB.y + ms.z * PB.y;
+    velB.z = velB.z + ms.w * cross(V2{r.z, r.w}, PB) / 1.0f;
+}
+
+// SolveVelocity(PulleyJointConf&) (PulleyJointConf.cpp:155-190): returns
+// whether the impulse was zero.
+__device__ __forceinline__ bool pulley_solve_velocity(float4* u, float4& velA, float4& velB) {
+    const float4 r = u[0], uv = u[1], n = u[2];
+    const V2 vpA = V2{velA.x, velA.y} + mul(revperp(V2{r.x, r.y}), velA.z / 1.0f);
+    const V2 vpB = V2{velB.x, velB.y} + mul(revperp(V2{r.z, r.w}), velB.z / 1.0f);
+    const float Cdot = -dot(V2{uv.x, uv.y}, vpA) - n.y * dot(V2{uv.z, uv.w}, vpB);
+    const float impulse = -n.x * Cdot;
+    u[3].x = u[3].x + impulse;
+    pulley_apply(u, impulse, velA, velB);
+    return impulse == 0.0f;
+}
+
 // Motor record: u[1] linear mass m00, m01, m10, m11; u[2].xy angular mass,
 // angular error; u[5].xy dt * maximum force, dt * maximum torque; u[7] linear
 // error x, y, (1 / dt) * correction factor; impulses as friction.
@@ -1698,6 +1750,8 @@ __device__ __forceinline__ void spring_joint_warm_start(const float4* u, std::ui
     if (kind == prx::kJointDistance || kind == prx::kJointRope) {
         const V2 P = mul(im.x, V2{u[1].x, u[1].y});
         apply_joint_impulse(ms, P, cross(rA, P) / 1.0f, cross(rB, P) / 1.0f, velA, velB);
+    } else if (kind == prx::kJointPulley) {
+        pulley_apply(u, im.x, velA, velB);
     } else if (kind == prx::kJointWheel) {
         // WheelJointConf.cpp:212-223.
         const float4 a = u[0], s = u[1];
@@ -1785,6 +1839,32 @@ __device__ __forceinline__ bool spring_joint_solve_position(const float4* u, std
     const float4 ms = u[4], la = u[6], cf = u[7];
     const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
     if (kind == prx::kJointFriction) return true;    // FrictionJointConf.cpp:233-237
+    if (kind == prx::kJointPulley) {
+        // PulleyJointConf.cpp:192-253.
+        const float ratio = u[2].y;
+        const V2 rA = rotate(V2{la.x, la.y}, unit_from_angle(posA.z));
+        const V2 rB = rotate(V2{la.z, la.w}, unit_from_angle(posB.z));
+        float lengthA, lengthB;
+        const V2 uA = unit_polar((V2{posA.x, posA.y} + rA) - V2{cf.x, cf.y}, lengthA);
+        const V2 uB = unit_polar((V2{posB.x, posB.y} + rB) - V2{cf.z, cf.w}, lengthB);
+        const float ruA = cross(rA, uA);
+        const float ruB = cross(rB, uB);
+        const float tA = mA + iA * (ruA * ruA) / 1.0f;
+        const float tB = mB + iB * (ruB * ruB) / 1.0f;
+        const float total = tA + (ratio * ratio) * tB;
+        const float mass = (total > 0.0f) ? 1.0f / total : 0.0f;
+        const float C = (u[5].x + ratio * u[5].y) - (lengthA + ratio * lengthB);
+        const float impulse = -mass * C;
+        const V2 PA = mul(-impulse, uA);
+        const V2 PB = mul(-ratio * impulse, uB);
+        posA.x = posA.x + mA * PA.x;
+        posA.y = posA.y + mA * PA.y;
+        posA.z = posA.z + iA * cross(rA, PA) / 1.0f;
+        posB.x = posB.x + mB * PB.x;
+        posB.y = posB.y + mB * PB.y;
+        posB.z = posB.z + iB * cross(rB, PB) / 1.0f;
+        return __builtin_fabsf(C) < c.linearSlop;
+    }
     if (kind == prx::kJointMotorKind) return true;   // MotorJointConf.cpp:231-234
     if (kind == prx::kJointDistance && cf.y > 0.0f) return true;
     const V2 qA = unit_from_angle(posA.z);
@@ -1897,6 +1977,8 @@ __device__ __forceinline__ void build_jc(const DeviceState& d, const WorldCtx& x
         else if (kind == prx::kJointFriction) build_jc_friction(u, cf, im, rA, rB, mA, iA, mB, iB, flags & 0xffu, c);
         else if (kind == prx::kJointWheel)
             build_jc_wheel(u, cf, cf2, im, rA, rB, qA, pA, pB, mA, iA, mB, iB, flags & 0xffu, c);
+        else if (kind == prx::kJointPulley)
+            build_jc_pulley(u, cf, cf2, im, rA, rB, pA, pB, mA, iA, mB, iB, flags & 0xffu, c);
         else {
             // Body B's anchor is its origin: rB = Rotate(-localCenterB, qB).
             const V2 rBm = rotate(V2{-B.lc.x, -B.lc.y}, qB);
@@ -2355,6 +2437,7 @@ __device__ __forceinline__ bool joints_velocity_k(float4* jbase, int j0, int j1,
             else if (kind == prx::kJointRope) ok &= rope_solve_velocity(u, va, vb);
             else if (kind == prx::kJointFriction) ok &= friction_solve_velocity(u, va, vb);
             else if (kind == prx::kJointWheel) ok &= wheel_solve_velocity(u, va, vb);
+            else if (kind == prx::kJointPulley) ok &= pulley_solve_velocity(u, va, vb);
             else ok &= motor_solve_velocity(u, va, vb);
             bs.set_vel(bA, va);
             bs.set_vel(bB, vb);

@@ -38,8 +38,8 @@ CASES = [
     ("chain0", "chain", 240, [1, 0], [1, 120, 240]),
     ("chain1", "chain", 240, [1, 1, 12, 24, 40], [1, 240]),
     # Distance and weld joints, rigid and soft, friction joints on the ground,
-    # ropes from a bar, motor joints and a cart on wheel joints, beside falling
-    # bodies.
+    # ropes from a bar, motor joints, a cart on wheel joints and a pulley, beside
+    # falling bodies.
     ("joints0", "joints", 240, [1, 0], [1, 120, 240]),
     ("joints1", "joints", 240, [2, 1, 12, 40], [1, 240]),
 ]
@@ -51,6 +51,9 @@ EDIT_CASES = [
       (90, "L", 0, 0.25, 0.0, 0)]),
     ("drop300_edit_toi", "drop", 120, [300, 20, 40, 7], [1, 120], True,
      [(15, "T", 40, 0.0, 12.0, -1.0), (30, "O", 0, -2.0, 1.5, 0), (45, "A", 77, 3.0, 0, 0), (60, "L", 120, 5.0, 9.0, 0)]),
+    # ShiftOrigin with every joint kind present (the pulley's ground anchors move).
+    ("joints0_edit", "joints", 120, [1, 0], [1, 120], False,
+     [(40, "O", 0, 3.0, -2.0, 0), (80, "L", 30, -5.0, 20.0, 0)]),
 ]
 # The same scenes with the TOI phase (the reference's default doToi = true).
 TOI_CASES = [

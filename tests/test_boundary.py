@@ -173,18 +173,17 @@ def test_contact_event_recording_switch_host_side():
 
 
 def test_non_revolute_joint_creation_matches_scene():
-    """Distance, weld, rope, friction, motor and wheel joints created through
-    prx_distance_joint_create / prx_weld_joint_create / prx_rope_joint_create /
-    prx_friction_joint_create / prx_motor_joint_create / prx_wheel_joint_create
-    give the same world description as the scene the reference built (kind in
-    bits 8-15 of the flags, then length or reference angle, frequency, damping
-    ratio; maximum length; maximum force and torque; offsets and correction
-    factor; axis, motor)."""
+    """Distance, weld, rope, friction, motor, wheel and pulley joints created
+    through their prx_*_joint_create entry points give the same world
+    description as the scene the reference built (kind in bits 8-15 of the
+    flags, then length or reference angle, frequency, damping ratio; maximum
+    length; maximum force and torque; offsets and correction factor; axis,
+    motor; ground anchors, ratio, lengths)."""
     g = golden_cases.load("joints0")
     desc = _desc_only(g["desc_conf"])
     recs = dumpfmt.read_description(dumpfmt.Reader(desc))[2]
     kinds = {r["flags"] >> 8 for r in recs}
-    assert kinds == {1, 2, 3, 4, 5, 6}
+    assert kinds == {1, 2, 3, 4, 5, 6, 7}
     offs = pa.desc_joint_offsets(desc)
     bare = desc[:offs[0] - 4] + struct.pack("<I", 0)
     w = pa.World.from_desc(bare)
@@ -200,6 +199,11 @@ def test_non_revolute_joint_creation_matches_scene():
             w.create_joint(pa.WeldJointConf(reference_angle=v[4], **common))
         elif kind == 3:
             w.create_joint(pa.RopeJointConf(max_length=v[4], **base))
+        elif kind == 7:
+            w.create_joint(pa.PulleyJointConf(body_a=r["a"], body_b=r["b"], local_anchor_a=(v[0], v[1]),
+                                              local_anchor_b=(v[2], v[3]), ground_anchor_a=(v[4], v[5]),
+                                              ground_anchor_b=(v[6], v[7]), ratio=v[8], length_a=v[9],
+                                              length_b=v[10], collide_connected=(r["flags"] >> 2) & 1))
         elif kind == 6:
             w.create_joint(pa.WheelJointConf(local_axis_a=(v[9], v[10]), enable_motor=(r["flags"] >> 1) & 1,
                                              motor_speed=v[7], max_motor_torque=v[8], frequency=v[4],
