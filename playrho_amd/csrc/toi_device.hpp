@@ -228,9 +228,13 @@ __device__ __forceinline__ SimplexCache empty_cache() {
 __device__ __forceinline__ SimplexCache cache_of(const Simplex& s) {
     SimplexCache c = empty_cache();
     c.metric = simplex_metric(s.e, s.n);
-    for (std::uint32_t i = 0; i < s.n; ++i) {
-        c.ia[i] = s.e[i].iA;
-        c.ib[i] = s.e[i].iB;
+    // Constant indices after unrolling keep the simplex in registers.
+#pragma unroll
+    for (std::uint32_t i = 0; i < 3; ++i) {
+        if (i < s.n) {
+            c.ia[i] = s.e[i].iA;
+            c.ib[i] = s.e[i].iB;
+        }
     }
     return c;
 }
@@ -255,7 +259,9 @@ __device__ __forceinline__ DistOut gjk_distance(const Proxy& pA, const Xf& xfA, 
     std::uint32_t sb[3] = {cache.ib[0], cache.ib[1], cache.ib[2]};
     SEdge e[3];
     std::uint32_t n = cache.count();
-    for (std::uint32_t i = 0; i < n; ++i) e[i] = edge_of(pA, xfA, sa[i], pB, xfB, sb[i]);
+#pragma unroll
+    for (std::uint32_t i = 0; i < 3; ++i)
+        if (i < n) e[i] = edge_of(pA, xfA, sa[i], pB, xfB, sb[i]);
     if (n > 1) {
         const float metric1 = cache.metric;
         const float metric2 = simplex_metric(e, n);
@@ -275,7 +281,9 @@ __device__ __forceinline__ DistOut gjk_distance(const Proxy& pA, const Xf& xfA, 
         ++iter;
         simplex = solve_simplex(e, n);
         n = simplex.n;
-        for (std::uint32_t i = 0; i < n; ++i) e[i] = simplex.e[i];
+#pragma unroll
+        for (std::uint32_t i = 0; i < 3; ++i)
+            if (i < n) e[i] = simplex.e[i];
         if (n == 3) {
             state = kDistMaxPoints;
             break;
@@ -302,8 +310,12 @@ __device__ __forceinline__ DistOut gjk_distance(const Proxy& pA, const Xf& xfA, 
             state = kDistDuplicateIndexPair;
             break;
         }
-        e[n] = edge_of(pA, xfA, iA, pB, xfB, iB);
+        const SEdge added = edge_of(pA, xfA, iA, pB, xfB, iB);
+#pragma unroll
+        for (std::uint32_t k = 0; k < 3; ++k)
+            if (k == n) e[k] = added;
         ++n;
+#pragma unroll
         for (std::uint32_t k = 0; k < 3; ++k) {
             sa[k] = k < n ? e[k].iA : 255u;
             sb[k] = k < n ? e[k].iB : 255u;
@@ -315,9 +327,12 @@ __device__ __forceinline__ DistOut gjk_distance(const Proxy& pA, const Xf& xfA, 
 // GetDelta(GetWitnessPoints(simplex)) (Distance.cpp:75-93, Distance.hpp:55-58).
 __device__ __forceinline__ V2 witness_delta(const Simplex& s) {
     V2 a{0.0f, 0.0f}, b{0.0f, 0.0f};
-    for (std::uint32_t i = 0; i < s.n; ++i) {
-        a = a + mul(s.e[i].wA, s.c[i]);
-        b = b + mul(s.e[i].wB, s.c[i]);
+#pragma unroll
+    for (std::uint32_t i = 0; i < 3; ++i) {
+        if (i < s.n) {
+            a = a + mul(s.e[i].wA, s.c[i]);
+            b = b + mul(s.e[i].wB, s.c[i]);
+        }
     }
     return b - a;
 }
