@@ -8,8 +8,14 @@ Workload: 4096 independent "pile" worlds, world w with a seeded body count
 drawn log-uniform in [8, 512] (boxes, disks and 3..8-gons in stacks or piles
 on a ground edge), PlayRho's regular step (dt = 1/60, 8 velocity and 3
 position iterations, doToi = false), all worlds advanced by one hipGraph
-launch per step.  W untimed warmup steps are followed by K timed steps; inputs
-are resident in device memory when the timed region starts.
+launch per step, for a run of 500 steps.
+
+Timed window: the K timed steps are the last K steps of the 500-step run.
+The run's first 500 - W - K steps (the settle phase: bodies fall and piles
+form) and the W warmup steps are stepped untimed, then the launch schedule is
+autotuned on that settled state, then K steps are timed.  `full_run` times
+all 500 steps of a fresh batch under the same schedule.  Inputs are resident
+in device memory when a timed region starts.
 
 Prints one JSON line (rank 0, one GPU).
 """
@@ -25,6 +31,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "body-steps/s at 4096 batched worlds vs plain GPU and CPU; adjoint steps/s"
 HBM_PEAK_GBS = 8000.0
+RUN_STEPS = 500  # steps of the batched pile run
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_step_worlds_pile4096_traffic.json")
 
 
 def parse():
@@ -34,6 +42,13 @@ def parse():
     p.add_argument("--warmup", type=int, default=30)
     p.add_argument("--worlds", type=int, default=4096)
     p.add_argument("--seed", type=int, default=1)
+    p.add_argument("--settle", type=int, default=-1,
+                   help="untimed steps before the warmup (-1: 500 - warmup - steps, so that the timed steps "
+                        "end at step 500 of the run)")
+    p.add_argument("--full-run", type=int, default=1, help="also time all 500 steps of a fresh batch (0: skip)")
+    p.add_argument("--phase-table", action="store_true",
+                   help="only print the per-phase cycle table of the 16 longest worlds over the timed window "
+                        "(phase timers on; a separate run, since the timers cost time)")
     p.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cores)")
     p.add_argument("--cpu-worlds", type=int, default=1024, help="worlds in the CPU sample")
     p.add_argument("--no-cpu", action="store_true")
@@ -75,15 +90,27 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes": n * 4, "copies": 5, "ok": bool(torch.equal(a, b))}))
 
 
-def measured_traffic():
+def settle_steps(args):
+    return args.settle if args.settle >= 0 else max(0, RUN_STEPS - args.warmup - args.steps)
+
+
+def timed_window(args):
+    first = settle_steps(args) + args.warmup + 1
+    return [first, first + args.steps - 1]
+
+
+def measured_traffic(args):
     """HBM bytes per step launch from the committed counter summary (two
     rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE, scaled by a plain copy's
-    counters), or None."""
-    path = os.path.join(ROOT, "profiles", "k_step_worlds_pile4096_traffic.json")
-    if not os.path.exists(path):
+    counters), when that summary was taken over this run's timed window of
+    the same workload; otherwise None."""
+    if not os.path.exists(TRAFFIC_PROFILE):
         return None
-    with open(path) as f:
-        return json.load(f)
+    with open(TRAFFIC_PROFILE) as f:
+        tr = json.load(f)
+    if tr.get("window") != timed_window(args) or tr.get("worlds") != args.worlds or tr.get("seed") != args.seed:
+        return None
+    return tr
 
 
 def cpu_threads(args):
@@ -98,16 +125,18 @@ def algorithmic_bytes(awake, proxies, shape_bytes, contacts, joints=0):
 def cpu_baseline(args, threads, do_toi=False):
     ref = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
     n = min(args.cpu_worlds, args.worlds)
+    untimed = settle_steps(args) + args.warmup
+    first, last = timed_window(args)
     if os.path.exists(ref):
-        out = subprocess.run([ref, "--time", "pile", str(args.warmup), str(args.steps), "1" if do_toi else "0",
+        out = subprocess.run([ref, "--time", "pile", str(untimed), str(args.steps), "1" if do_toi else "0",
                               str(threads), str(n), str(args.seed), "0"],
                              check=True, capture_output=True, text=True).stdout
         r = json.loads(out.strip().splitlines()[-1])
         return dict(value=r["body_steps"] / r["seconds"], unit="body-steps/s", cores=threads, kind="reference",
                     sample="reference PlayRho float build (oracle/_ref, g++ -O3), worlds 0..%d of the same "
-                           "workload, steps %d..%d timed after %d warmup, %d std::threads, one world per "
-                           "thread at a time, doToi=%s" % (n - 1, args.warmup + 1, args.warmup + args.steps,
-                                                           args.warmup, threads, "true" if do_toi else "false"),
+                           "workload, steps %d..%d timed after %d untimed steps, %d std::threads, one world per "
+                           "thread at a time, doToi=%s" % (n - 1, first, last, untimed, threads,
+                                                           "true" if do_toi else "false"),
                     seconds=r["seconds"])
     from tests import oracle_lib
     import playrho_amd as pa
@@ -130,17 +159,19 @@ def toi_bench(args, pa, threads, regular_value):
     worlds = [pa.World.scene("pile", [args.seed, w]) for w in range(n)]
     batch = pa.Batch(worlds, use_graph=True, coop_min=args.coop_min or None, lds_bytes=args.lds_bytes or None)
     dyn = batch.dynamic_bodies()
-    batch.step(conf, args.warmup)
+    batch.step(conf, settle_steps(args) + args.warmup)
     batch.sync()
     t1 = time.perf_counter()
     batch.step(conf, args.steps)
     batch.sync()
     wall = time.perf_counter() - t1
     kernel_ms, launches = batch.last_timing()
+    first, last = timed_window(args)
     out = {"value": dyn * args.steps / wall, "unit": "body-steps/s", "ms_per_step": 1e3 * wall / args.steps,
            "workload": "%d batched uneven pile worlds (seed %d), dt=1/60 8/3 iterations, doToi=true "
-                       "(TOI defaults of StepConf), steps %d..%d" % (n, args.seed, args.warmup + 1,
-                                                                     args.warmup + args.steps),
+                       "(TOI defaults of StepConf), steps %d..%d of the 500-step run timed" % (n, args.seed, first,
+                                                                                             last),
+           "kernel_ms_per_step": kernel_ms / max(1, launches),
            "ratio_vs_doToi_false": (dyn * args.steps / wall) / regular_value}
     if not args.no_cpu:
         cb = cpu_baseline(args, threads, do_toi=True)
@@ -320,6 +351,47 @@ def conf_record(pa, do_toi=False):
             struct.pack("<I", 5) + struct.pack("<5f", *toi_f) + struct.pack("<6I", *toi_i))
 
 
+def full_run(args, pa, conf, schedule, dyn):
+    """All 500 steps of a fresh batch of the same worlds, timed from step 1,
+    under the schedule the autotuner chose."""
+    worlds = [pa.World.scene("pile", [args.seed, w]) for w in range(args.worlds)]
+    kw = {}
+    if schedule:
+        kw = dict(order=schedule["order"], coop_min=schedule["coop_min_contacts"],
+                  lds_bytes=schedule["lds_bytes"] or None)
+    batch = pa.Batch(worlds, use_graph=True, **kw)
+    batch.sync()
+    t0 = time.perf_counter()
+    batch.step(conf, RUN_STEPS)
+    batch.sync()
+    wall = time.perf_counter() - t0
+    kernel_ms, launches = batch.last_timing()
+    return {"steps": [1, RUN_STEPS], "value": dyn * RUN_STEPS / wall, "unit": "body-steps/s",
+            "ms_per_step": 1e3 * wall / RUN_STEPS, "kernel_ms_per_launch": kernel_ms / max(1, launches),
+            "note": "the whole 500-step run (free fall, piling, settling) of a fresh batch, timed from step 1"}
+
+
+def phase_table(args, pa, batch, conf, first, last):
+    """Per-phase shader-clock cycles of the 16 longest worlds over the timed
+    window (phase timers on), as one JSON line."""
+    import numpy as np
+    batch.phase_cycles(True)
+    batch.sync()
+    t0 = time.perf_counter()
+    batch.step(conf, args.steps)
+    batch.sync()
+    wall = time.perf_counter() - t0
+    pc = batch.phase_cycles(False).astype(np.float64) / args.steps
+    nb = [w.body_count for w in batch.worlds]
+    tot = pc[:, 13]
+    order = np.argsort(-tot)[:16]
+    names = pa.Batch.PHASES
+    rows = [dict(world=int(w), bodies=int(nb[w]), **{names[i]: int(pc[w, i]) for i in range(14)}) for w in order]
+    print(json.dumps({"window": [first, last], "ms_per_step_with_timers": 1e3 * wall / args.steps,
+                      "cycles_per_step": rows,
+                      "sum_over_worlds": {names[i]: int(pc[:, i].sum()) for i in range(14)}}))
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -344,13 +416,18 @@ def main():
     upload_s = time.perf_counter() - t_up
     dyn = batch.dynamic_bodies()
 
-    batch.step(conf, args.warmup)
+    settle = settle_steps(args)
+    first, last = timed_window(args)
+    batch.step(conf, settle + args.warmup)
     batch.sync()
     schedule = None
     if args.autotune > 0:
         t_at = time.time()
         schedule = batch.autotune(conf, args.autotune)
         schedule["autotune_s"] = round(time.time() - t_at, 3)
+    if args.phase_table:
+        phase_table(args, pa, batch, conf, first, last)
+        return
     batch.download()
     c0 = batch.counts()
 
@@ -386,11 +463,15 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "%d batched uneven pile worlds (bodies log-uniform in [8,512], seed %d), regular "
-                        "step dt=1/60 8/3 iterations doToi=false, one hipGraph launch per step"
-                        % (args.worlds, args.seed),
+                        "step dt=1/60 8/3 iterations doToi=false, one hipGraph launch per step; a 500-step "
+                        "run whose steps %d..%d are timed (steps 1..%d untimed: %d settle + %d warmup)"
+                        % (args.worlds, args.seed, first, last, first - 1, settle, args.warmup),
             "worlds": args.worlds,
             "bodies": total_bodies,
             "dynamic_bodies": int(dyn),
+            "run_steps": RUN_STEPS,
+            "settle_steps": settle,
+            "timed_steps": [first, last],
             "schedule": schedule,
         },
         "contacts_per_s": contacts_avg * args.steps / wall,
@@ -409,15 +490,26 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
             "kernel": "k_step_worlds",
+            "window": [first, last],
             "algorithmic_bytes_per_launch": bytes_step,
             "kernel_ms_per_launch": ms_launch,
+            "note": "achieved = algorithmic bytes per launch (mean of the formula at the window's first and "
+                    "last state) / mean k_step_worlds duration over the timed window (HIP events on the "
+                    "launch stream)",
         },
     }
 
-    tr = measured_traffic()
+    tr = measured_traffic(args)
     if tr is not None:
         result["roofline"]["traffic"] = tr["bytes_per_launch"]
         result["roofline"]["traffic_note"] = tr["note"]
+    else:
+        result["roofline"]["traffic_note"] = ("no HBM counter profile of steps %d..%d of this workload is "
+                                              "committed (profiles/k_step_worlds_pile4096_traffic.json holds "
+                                              "one window)" % (first, last))
+
+    if args.full_run:
+        result["full_run"] = full_run(args, pa, conf, schedule, dyn)
 
     if args.toi_worlds > 0:
         result["toi"] = toi_bench(args, pa, cpu_threads(args), value)
@@ -440,9 +532,12 @@ def main():
         return
     step_idx = max(1, args.worlds // max(1, args.plain_worlds))
     sample = [pa.World.scene("pile", [args.seed, w]) for w in range(0, args.worlds, step_idx)][:args.plain_worlds]
-    for w in sample:
-        for _ in range(args.warmup):
-            w.step(conf)
+    # The sample is brought to the timed window's first step by a batch of
+    # its own (released afterwards), then stepped world by world.
+    sb = pa.Batch(sample, use_graph=True)
+    sb.step(conf, first - 1)
+    sb.sync()
+    del sb
     nb = sum(sum(1 for f in w.bodies()[1] if f & 2) for w in sample)
     t2 = time.perf_counter()
     for w in sample:
@@ -452,8 +547,7 @@ def main():
     plain = nb * args.plain_steps / plain_s
     result["plain_gpu"] = {"value": plain, "unit": "body-steps/s",
                            "sample": "%d worlds (every %dth), one synchronised launch per world per step, "
-                                     "%d steps after %d warmup" % (len(sample), step_idx, args.plain_steps,
-                                                                  args.warmup)}
+                                     "steps %d..%d" % (len(sample), step_idx, first, first + args.plain_steps - 1)}
     result["ratio_vs_plain_gpu"] = value / plain
 
     if not args.no_cpu:
