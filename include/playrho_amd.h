@@ -436,6 +436,19 @@ int prx_big_metrics(prx_big* big, float out[4]);
  * *count receives the number found. */
 int prx_lbvh_pairs(const float* aabbs, const int32_t* bodies, const uint8_t* moved, int n, int32_t* pairs,
                    int cap, int* count);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): prx_lbvh_pairs
+ * on DEVICE buffers, with no hipMalloc, hipFree or other allocation.  The
+ * LBVH's working arrays live in a fixed per-device array of about 6.6 MB
+ * (held until the process ends) sized for at most 65536 proxies; more is
+ * PRX_E_LENGTH.  aabbs (4 floats per proxy), bodies, moved, pairs (8-byte
+ * aligned) and count are device pointers; the work is queued on `stream` (a
+ * hipStream_t, null for the null stream) and not waited for.  *count receives
+ * the number of pairs found (above cap, only the first cap were written), or
+ * -1 if the tree exceeded its depth bound.  The pairs and their order equal
+ * prx_lbvh_pairs's.  Calls that overlap on one device race on that array. */
+int prx_lbvh_pairs_noalloc(const float* aabbs, const int32_t* bodies, const uint8_t* moved, int n, int32_t* pairs,
+                           int cap, int* count, void* stream);
 
 /* Adjoint of K regular steps of a batch.  No reference counterpart: it
  * differentiates Step (AabbTreeWorld.cpp:1563-1682; contact solves

@@ -23,6 +23,9 @@
 // Order-sensitive bookkeeping (stable contact compaction, scans, the sort and
 // the colouring) runs in single-workgroup kernels so every run is
 // deterministic.
+#include <mutex>
+
+#include "../../include/playrho_amd.h"
 #include "big_world.hpp"
 #include "step_device.hpp"
 
@@ -1123,4 +1126,162 @@ extern "C" hipError_t prx_big_launch_metrics(const prx::DeviceState* dp, const p
     using namespace prxd;
     hipLaunchKernelGGL(k_big_metrics, dim3(Blocks(gp->cap, kGrid)), dim3(kGrid), 0, s, *dp, *gp);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Pair finding without allocation.  The LBVH's working arrays for up to
+// kNoAllocMaxProxies proxies live in one file-scope device array; the caller
+// passes device buffers for the inputs and outputs and a stream.  Each device
+// holds its own copy of the array, whose address is looked up once per
+// device (hipGetSymbolAddress) and used for every kernel argument.
+
+namespace {
+
+constexpr int kNoAllocMaxProxies = 1 << 16;
+
+constexpr std::size_t Align256(std::size_t n) { return (n + 255) & ~std::size_t(255); }
+
+// Byte offsets of the working arrays inside the scratch array (sized for
+// kNoAllocMaxProxies; a call with fewer proxies uses the front of each).
+struct NoAllocLayout {
+    static constexpr std::size_t N = std::size_t(kNoAllocMaxProxies);
+    static constexpr std::size_t hdr = 0;
+    static constexpr std::size_t paabb = Align256(hdr + sizeof(prx::WorldHeader));
+    static constexpr std::size_t pflags = Align256(paabb + 16 * N);
+    static constexpr std::size_t morton = Align256(pflags + 4 * N);
+    static constexpr std::size_t morton2 = Align256(morton + 4 * N);
+    static constexpr std::size_t order = Align256(morton2 + 4 * N);
+    static constexpr std::size_t order2 = Align256(order + 4 * N);
+    static constexpr std::size_t left = Align256(order2 + 4 * N);
+    static constexpr std::size_t right = Align256(left + 4 * N);
+    static constexpr std::size_t parent = Align256(right + 4 * N);
+    static constexpr std::size_t escape = Align256(parent + 8 * N);
+    static constexpr std::size_t visit = Align256(escape + 8 * N);
+    static constexpr std::size_t naabb = Align256(visit + 4 * N);
+    static constexpr std::size_t bounds = Align256(naabb + 32 * N);
+    static constexpr std::size_t pcount = Align256(bounds + 16);
+    static constexpr std::size_t counters = Align256(pcount + 4 * (N + 1));
+    static constexpr std::size_t bytes = Align256(counters + 4 * prx::kBigCounters);
+};
+
+}  // namespace
+
+// About 6.6 MB per device, held until the process ends.
+static __device__ __align__(256) unsigned char scratch[NoAllocLayout::bytes];
+
+namespace prxd {
+
+// Inputs into the LBVH's proxy arrays: fat AABBs (4 floats each, any 4-byte
+// alignment), moved flags; the header carries the proxy count and a cleared
+// status (a fill by kernel, so the call can be captured in a graph).
+__global__ void k_noalloc_prep(DeviceState d, const float* aabbs, const std::uint8_t* moved, int n) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i == 0) {
+        d.hdr[0].nproxies = n;
+        d.hdr[0].status = 0;
+    }
+    if (i >= n) return;
+    d.paabb[i] = float4{aabbs[4 * i], aabbs[4 * i + 1], aabbs[4 * i + 2], aabbs[4 * i + 3]};
+    d.pflags[i] = moved[i] ? prx::kProxyMoved : 0u;
+}
+
+// The pair count for the caller: the number found (above cap: only cap were
+// written), or -1 when the tree walk exceeded its depth bound.
+__global__ void k_noalloc_count(DeviceState d, BigState g, int* count) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int total = g.pcount[g.P];
+    const bool depth = (d.hdr[0].status & prx::kStatusScratchOverflow) != 0 && total <= g.pairCap;
+    *count = depth ? -1 : total;
+}
+
+}  // namespace prxd
+
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mutex;
+unsigned char* g_noalloc_ptr[kNoAllocMaxDevices] = {};
+
+// The current device's copy of the scratch array, looked up on the first
+// call on that device.
+hipError_t NoAllocScratch(unsigned char** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_noalloc_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_noalloc_ptr[dev];
+    return hipSuccess;
+}
+
+}  // namespace
+
+// Returns a PRX_* status; *err names the failure.
+extern "C" int prx_lbvh_pairs_noalloc_impl(const float* aabbs, const std::int32_t* bodies, const std::uint8_t* moved,
+                                           int n, std::int32_t* pairs, int cap, int* count, void* stream,
+                                           const char** err) {
+    using namespace prxd;
+    if (!aabbs || !bodies || !moved || n <= 0 || !count || cap < 0 || (cap > 0 && !pairs)) {
+        *err = "bad argument";
+        return PRX_E_INVALID_ARGUMENT;
+    }
+    if (n > kNoAllocMaxProxies) {
+        *err = "more proxies than the allocation-free pair finder holds (65536)";
+        return PRX_E_LENGTH;
+    }
+    if ((reinterpret_cast<std::uintptr_t>(pairs) & 7u) || (reinterpret_cast<std::uintptr_t>(bodies) & 3u) ||
+        (reinterpret_cast<std::uintptr_t>(aabbs) & 3u) || (reinterpret_cast<std::uintptr_t>(count) & 3u)) {
+        *err = "misaligned buffer (pairs need 8-byte, the others 4-byte alignment)";
+        return PRX_E_INVALID_ARGUMENT;
+    }
+    unsigned char* base = nullptr;
+    if (NoAllocScratch(&base) != hipSuccess) {
+        *err = "device scratch lookup failed";
+        return PRX_E_DEVICE;
+    }
+    using L = NoAllocLayout;
+    prx::DeviceState d{};
+    d.nworlds = 1;
+    d.hdr = reinterpret_cast<prx::WorldHeader*>(base + L::hdr);
+    d.paabb = reinterpret_cast<float4*>(base + L::paabb);
+    d.pflags = reinterpret_cast<std::uint32_t*>(base + L::pflags);
+    d.pbody = const_cast<int*>(reinterpret_cast<const int*>(bodies));
+    prx::BigState g{};
+    g.P = n;
+    g.B = 1;
+    g.cap = 1;
+    g.pairCap = cap;
+    g.morton = reinterpret_cast<std::uint32_t*>(base + L::morton);
+    g.morton2 = reinterpret_cast<std::uint32_t*>(base + L::morton2);
+    g.order = reinterpret_cast<int*>(base + L::order);
+    g.order2 = reinterpret_cast<int*>(base + L::order2);
+    g.left = reinterpret_cast<int*>(base + L::left);
+    g.right = reinterpret_cast<int*>(base + L::right);
+    g.parent = reinterpret_cast<int*>(base + L::parent);
+    g.escape = reinterpret_cast<int*>(base + L::escape);
+    g.visit = reinterpret_cast<int*>(base + L::visit);
+    g.naabb = reinterpret_cast<float4*>(base + L::naabb);
+    g.bounds = reinterpret_cast<float4*>(base + L::bounds);
+    g.pcount = reinterpret_cast<int*>(base + L::pcount);
+    g.counters = reinterpret_cast<int*>(base + L::counters);
+    g.pairs = reinterpret_cast<int2*>(pairs);
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_noalloc_prep, dim3(Blocks(n, kGrid)), dim3(kGrid), 0, s, d, aabbs, moved, n);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = prx_big_launch_pairs(&d, &g, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_noalloc_count, dim3(1), dim3(64), 0, s, d, g, count);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) {
+        *err = hipGetErrorString(e);
+        return PRX_E_DEVICE;
+    }
+    return PRX_OK;
 }

@@ -34,6 +34,9 @@ extern "C" hipError_t prx_launch_step(const prx::DeviceState* d, const prx::Step
 extern "C" hipError_t prx_big_launch_step(const prx::DeviceState* d, const prx::BigState* g,
                                           const prx::StepParams* c, hipStream_t s);
 extern "C" hipError_t prx_big_launch_pairs(const prx::DeviceState* d, const prx::BigState* g, hipStream_t s);
+extern "C" int prx_lbvh_pairs_noalloc_impl(const float* aabbs, const std::int32_t* bodies, const std::uint8_t* moved,
+                                           int n, std::int32_t* pairs, int cap, int* count, void* stream,
+                                           const char** err);
 extern "C" hipError_t prx_big_launch_metrics(const prx::DeviceState* d, const prx::BigState* g, hipStream_t s);
 extern "C" hipError_t prx_launch_step_tape(const prx::DeviceState* d, const prx::StepParams* c, int count,
                                            hipStream_t stream);
@@ -2514,6 +2517,13 @@ int prx_lbvh_pairs(const float* aabbs, const std::int32_t* bodies, const std::ui
     if (m > 0) PRX_HIP(hipMemcpy(pairs, g.pairs, std::size_t(m) * sizeof(int2), hipMemcpyDeviceToHost));
     if (total > g.pairCap) return Fail(PRX_E_LENGTH, "more pairs than the pair capacity");
     return PRX_OK;
+}
+
+int prx_lbvh_pairs_noalloc(const float* aabbs, const std::int32_t* bodies, const std::uint8_t* moved, int n,
+                           std::int32_t* pairs, int cap, int* count, void* stream) {
+    const char* err = "";
+    const int st = prx_lbvh_pairs_noalloc_impl(aabbs, bodies, moved, n, pairs, cap, count, stream, &err);
+    return st ? Fail(st, err) : PRX_OK;
 }
 
 
