@@ -150,6 +150,44 @@ def cpu_baseline(args, threads, do_toi=False):
                 seconds=secs)
 
 
+def fp64_report(args, pa, conf, body_counts):
+    """Device single-step error against the FP64 oracle from the same state,
+    at the timed window's first step, on the two largest worlds and a
+    median-sized one: each is stepped on the device to the step before, its
+    state checked bit for bit against the float reference restatement, then
+    one device step is compared with the FP64 oracle's step from that state
+    widened exactly (tests/fp64.py states the metric and its bounds)."""
+    from tests import dumpfmt, fp64, oracle_lib
+    first, _ = timed_window(args)
+    order = sorted(range(len(body_counts)), key=lambda w: -body_counts[w])
+    ids = [order[0], order[1], order[len(order) // 2]]
+    worlds = [pa.World.scene("pile", [args.seed, w]) for w in ids]
+    oracles = [oracle_lib.OracleWorld(w.save_desc() + dumpfmt.conf_record(conf), 32) for w in worlds]
+    sb = pa.Batch(worlds, use_graph=True)
+    sb.step(conf, first - 1)
+    sb.sync()
+    del sb
+    worst, exact = None, True
+    for w, o in zip(worlds, oracles):
+        o.step(first - 1, record=False)
+        got = dumpfmt.read_state(dumpfmt.Reader(w.state_record()))
+        want = o.state()
+        exact = exact and all(dumpfmt.digest(got, (p,)) == dumpfmt.digest(want, (p,))
+                              for p in ("bodies", "contacts", "proxies"))
+        d = o.widen()
+        w.step(conf)
+        d.step(1, record=False)
+        after = dumpfmt.read_state(dumpfmt.Reader(w.state_record()))
+        e = fp64.errors(fp64.state_bodies(after), d.bodies_f64(w.body_count), fp64.dynamic_mask(after))
+        worst = e if worst is None else {k: max(worst[k], e[k]) for k in e}
+    return {"worlds": ids, "bodies": [body_counts[i] for i in ids], "step": first,
+            "device_equals_float_reference_before_the_step": exact, "max_error": worst,
+            "bounds": {"position": fp64.POS_BOUND, "angle": fp64.ANGLE_BOUND, "velocity": fp64.VEL_BOUND},
+            "within_bounds": fp64.within_bounds(worst),
+            "metric": "per body |x - x64| / max(|x64|, 1) for position (m), angle (rad), velocity (m/s) and "
+                      "spin (rad/s), max over dynamic bodies; x64 from the FP64 oracle's step from the same state"}
+
+
 def toi_bench(args, pa, threads, regular_value):
     """The batched pile workload stepped with the TOI phase (doToi = true, the
     reference's default StepConf): the regular step kernel then the TOI kernel
@@ -553,6 +591,7 @@ def main():
     if not args.no_cpu:
         threads = cpu_threads(args)
         cb = cpu_baseline(args, threads)
+        cb["fp64_single_step_error"] = fp64_report(args, pa, conf, [w.body_count for w in worlds])
         result["cpu_baseline"] = cb
         result["ratio_vs_cpu"] = value / cb["value"]
     result["setup_s"] = build_s

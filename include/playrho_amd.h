@@ -367,6 +367,14 @@ int prx_world_set_contact_listener(prx_world* world, const prx_contact_listener*
  * recorded calls also end each state record).  Not for a world inside a
  * multi-world batch (PRX_E_WRONG_STATE): switch it before batching. */
 int prx_world_record_contact_events(prx_world* world, int on);
+/* Initial size in 32-bit words of the world's record of one step's listener
+ * calls (0: the default, 48 words per contact slot, which covers every call
+ * of the regular phase).  No reference counterpart: the reference calls its
+ * listeners directly.  A step whose calls need more words (the TOI phase can
+ * update a contact once per TOI event) is stepped again from the state before
+ * it with the record grown to fit, so no call is lost.  Not for a world
+ * inside a multi-world batch (PRX_E_WRONG_STATE). */
+int prx_world_set_event_record_words(prx_world* world, uint32_t words);
 /* The recorded calls of the world's last step: ncalls calls in nwords 32-bit
  * words, each u32 kind (0 begin, 1 end, 2 pre-solve, 3 post-solve), u32
  * contact id, then for pre-solve u32 type | point count << 8, two contact

@@ -306,6 +306,217 @@ unsigned char* Copy(const std::vector<unsigned char>& v, size_t* n) {
     return p;
 }
 
+// Widening of a float world's whole state to the double instantiation, for
+// single-step comparisons from the same input state: every real converts
+// exactly; each body's cached transformation is recomputed in double from its
+// position (Body::SetTransformation with UnitVec::Get(angle), Body.cpp), as a
+// double build holding these positions would have it.
+template <class D, class S> oracle::V2<D> Wd(const oracle::V2<S>& v) { return {D(v.x), D(v.y)}; }
+template <class D, class S> oracle::V3<D> Wd(const oracle::V3<S>& v) { return {D(v.x), D(v.y), D(v.z)}; }
+template <class D, class S> oracle::Pos<D> Wd(const oracle::Pos<S>& v) { return {Wd<D>(v.lin), D(v.ang)}; }
+template <class D, class S> oracle::Vel<D> Wd(const oracle::Vel<S>& v) { return {Wd<D>(v.lin), D(v.ang)}; }
+
+template <class D, class S>
+void Widen(const Handle<S>& s, Handle<D>& d) {
+    using namespace oracle;
+    const World<S>& a = s.world;
+    World<D>& b = d.world;
+    b.joints.clear();
+    for (const auto& j : a.joints) {
+        Joint<D> k;
+        k.bodyA = j.bodyA;
+        k.bodyB = j.bodyB;
+        k.enableLimit = j.enableLimit;
+        k.enableMotor = j.enableMotor;
+        k.collideConnected = j.collideConnected;
+        k.laA = Wd<D>(j.laA);
+        k.laB = Wd<D>(j.laB);
+        k.referenceAngle = D(j.referenceAngle);
+        k.lower = D(j.lower);
+        k.upper = D(j.upper);
+        k.motorSpeed = D(j.motorSpeed);
+        k.maxMotorTorque = D(j.maxMotorTorque);
+        k.impulse = Wd<D>(j.impulse);
+        k.motorImpulse = D(j.motorImpulse);
+        k.limitState = j.limitState;
+        k.rA = Wd<D>(j.rA);
+        k.rB = Wd<D>(j.rB);
+        k.mass = M33<D>{Wd<D>(j.mass.ex), Wd<D>(j.mass.ey), Wd<D>(j.mass.ez)};
+        k.angularMass = D(j.angularMass);
+        k.kind = j.kind;
+        k.length = D(j.length);
+        k.frequency = D(j.frequency);
+        k.dampingRatio = D(j.dampingRatio);
+        k.u = Wd<D>(j.u);
+        k.dmass = D(j.dmass);
+        k.invGamma = D(j.invGamma);
+        k.bias = D(j.bias);
+        k.gamma = D(j.gamma);
+        k.curLength = D(j.curLength);
+        k.maxForce = D(j.maxForce);
+        k.maxTorque = D(j.maxTorque);
+        k.lm00 = D(j.lm00);
+        k.lm01 = D(j.lm01);
+        k.lm10 = D(j.lm10);
+        k.lm11 = D(j.lm11);
+        k.correctionFactor = D(j.correctionFactor);
+        k.angularError = D(j.angularError);
+        k.linearError = Wd<D>(j.linearError);
+        k.localXAxis = Wd<D>(j.localXAxis);
+        k.ax = Wd<D>(j.ax);
+        k.ay = Wd<D>(j.ay);
+        k.sAx = D(j.sAx);
+        k.sAy = D(j.sAy);
+        k.sBx = D(j.sBx);
+        k.sBy = D(j.sBy);
+        k.springMass = D(j.springMass);
+        k.groundA = Wd<D>(j.groundA);
+        k.groundB = Wd<D>(j.groundB);
+        k.lengthB = D(j.lengthB);
+        k.ratio = D(j.ratio);
+        b.joints.push_back(k);
+    }
+    b.bodyJoints = a.bodyJoints;
+    b.islandedJoints = a.islandedJoints;
+    b.shapes.clear();
+    for (const auto& sh : a.shapes) {
+        Shape<D> t;
+        for (const auto& v : sh.v) t.v.push_back(Wd<D>(v));
+        for (const auto& v : sh.nrm) t.nrm.push_back(Wd<D>(v));
+        t.radius = D(sh.radius);
+        t.density = D(sh.density);
+        t.friction = D(sh.friction);
+        t.restitution = D(sh.restitution);
+        t.sensor = sh.sensor;
+        t.catmask = sh.catmask;
+        t.group = sh.group;
+        b.shapes.push_back(t);
+    }
+    b.bodies.clear();
+    for (const auto& bo : a.bodies) {
+        Body<D> t;
+        t.type = bo.type;
+        t.flags = bo.flags;
+        t.pos0 = Wd<D>(bo.pos0);
+        t.pos1 = Wd<D>(bo.pos1);
+        t.lc = Wd<D>(bo.lc);
+        t.vel = Wd<D>(bo.vel);
+        t.lacc = Wd<D>(bo.lacc);
+        t.aacc = D(bo.aacc);
+        t.ldamp = D(bo.ldamp);
+        t.adamp = D(bo.adamp);
+        t.invMass = D(bo.invMass);
+        t.invRotI = D(bo.invRotI);
+        t.underActive = D(bo.underActive);
+        t.alpha0 = D(bo.alpha0);
+        t.xf = GetXf(t.pos1, t.lc);
+        t.shapes = bo.shapes;
+        t.proxies = bo.proxies;
+        b.bodies.push_back(t);
+    }
+    b.bodyOrder = a.bodyOrder;
+    b.tree.clear();
+    for (const auto& lf : a.tree) {
+        Leaf<D> t;
+        t.used = lf.used;
+        t.body = lf.body;
+        t.shape = lf.shape;
+        t.child = lf.child;
+        for (int i = 0; i < 2; ++i) {
+            t.lo[i] = D(lf.lo[i]);
+            t.hi[i] = D(lf.hi[i]);
+        }
+        b.tree.push_back(t);
+    }
+    b.leafCount = a.leafCount;
+    b.fixturesForProxies = a.fixturesForProxies;
+    b.bodiesForSync = a.bodiesForSync;
+    b.proxiesForContacts = a.proxiesForContacts;
+    b.contactBuf.clear();
+    for (const auto& c : a.contactBuf) {
+        Contact<D> t;
+        t.kmin = c.kmin;
+        t.kmax = c.kmax;
+        t.bodyA = c.bodyA;
+        t.bodyB = c.bodyB;
+        t.shapeA = c.shapeA;
+        t.shapeB = c.shapeB;
+        t.flags = c.flags;
+        t.friction = D(c.friction);
+        t.restitution = D(c.restitution);
+        t.tangentSpeed = D(c.tangentSpeed);
+        t.toi = D(c.toi);
+        t.toiValid = c.toiValid;
+        t.toiCount = c.toiCount;
+        b.contactBuf.push_back(t);
+    }
+    b.manifoldBuf.clear();
+    for (const auto& m : a.manifoldBuf) {
+        Manifold<D> t;
+        t.type = m.type;
+        t.n = m.n;
+        t.ln = Wd<D>(m.ln);
+        t.lpt = Wd<D>(m.lpt);
+        for (int i = 0; i < 2; ++i) {
+            t.pts[i].lp = Wd<D>(m.pts[i].lp);
+            t.pts[i].cf = m.pts[i].cf;
+            t.pts[i].ni = D(m.pts[i].ni);
+            t.pts[i].ti = D(m.pts[i].ti);
+        }
+        b.manifoldBuf.push_back(t);
+    }
+    b.contactFree = a.contactFree;
+    b.contacts.clear();
+    for (const auto& k : a.contacts) b.contacts.push_back({k.kmin, k.kmax, k.id});
+    b.bodyContacts.clear();
+    for (const auto& lst : a.bodyContacts) {
+        std::vector<typename World<D>::KeyedId> t;
+        for (const auto& k : lst) t.push_back({k.kmin, k.kmax, k.id});
+        b.bodyContacts.push_back(t);
+    }
+    b.islandedBodies = a.islandedBodies;
+    b.islandedContacts = a.islandedContacts;
+    b.events.clear();
+    b.nevents = 0;
+    const StepConf<S>& c = s.conf;
+    StepConf<D>& e = d.conf;
+    e.dt = D(c.dt);
+    e.dtRatio = D(c.dtRatio);
+    e.minStillTimeToSleep = D(c.minStillTimeToSleep);
+    e.linearSlop = D(c.linearSlop);
+    e.angularSlop = D(c.angularSlop);
+    e.regResolutionRate = D(c.regResolutionRate);
+    e.regMinSeparation = D(c.regMinSeparation);
+    e.regMinMomentum = D(c.regMinMomentum);
+    e.velocityThreshold = D(c.velocityThreshold);
+    e.maxTranslation = D(c.maxTranslation);
+    e.maxRotation = D(c.maxRotation);
+    e.maxLinearCorrection = D(c.maxLinearCorrection);
+    e.maxAngularCorrection = D(c.maxAngularCorrection);
+    e.linearSleepTolerance = D(c.linearSleepTolerance);
+    e.angularSleepTolerance = D(c.angularSleepTolerance);
+    e.displaceMultiplier = D(c.displaceMultiplier);
+    e.aabbExtension = D(c.aabbExtension);
+    e.maxCirclesRatio = D(c.maxCirclesRatio);
+    e.velIters = c.velIters;
+    e.posIters = c.posIters;
+    e.warmStart = c.warmStart;
+    e.toi = c.toi;
+    e.blockSolve = c.blockSolve;
+    e.toiResolutionRate = D(c.toiResolutionRate);
+    e.toiMinSeparation = D(c.toiMinSeparation);
+    e.toiMinMomentum = D(c.toiMinMomentum);
+    e.targetDepth = D(c.targetDepth);
+    e.tolerance = D(c.tolerance);
+    e.toiVelIters = c.toiVelIters;
+    e.toiPosIters = c.toiPosIters;
+    e.maxToiRootIters = c.maxToiRootIters;
+    e.maxToiIters = c.maxToiIters;
+    e.maxDistanceIters = c.maxDistanceIters;
+    e.maxSubSteps = c.maxSubSteps;
+    d.steps = s.steps;
+}
+
 }  // namespace
 
 // A world edit between steps: kind 'L' SetLocation(body, (a, b)), 'A'
@@ -342,6 +553,19 @@ void* oracle_create(const unsigned char* desc, size_t len, int precision) {
         a->h = h;
     }
     return a;
+}
+
+// A double-precision copy of a float world's whole state (the FP64 oracle
+// stepping from the float world's current state); null for a double world.
+void* oracle_widen(void* p) {
+    auto* a = static_cast<AnyHandle*>(p);
+    if (!a || a->precision == 64) return nullptr;
+    auto* h = new Handle<double>;
+    Widen(*static_cast<Handle<float>*>(a->h), *h);
+    auto* out = new AnyHandle;
+    out->precision = 64;
+    out->h = h;
+    return out;
 }
 
 void oracle_destroy(void* p) {

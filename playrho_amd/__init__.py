@@ -351,6 +351,7 @@ _SIGS = {
     "prx_big_metrics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "prx_lbvh_pairs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                       ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "prx_world_set_event_record_words": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     "prx_lbvh_pairs_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                               ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
 }
@@ -528,6 +529,11 @@ class World:
     def record_contact_events(self, on=True):
         """Records each step's contact listener calls (before batching the world)."""
         _check(lib().prx_world_record_contact_events(self.h, 1 if on else 0))
+
+    def set_event_record_words(self, words):
+        """Initial size of the per-step listener-call record in 32-bit words
+        (0: the default); a step needing more grows it and is stepped again."""
+        _check(lib().prx_world_set_event_record_words(self.h, int(words)))
 
     def contact_event_words(self):
         """Raw recorded listener calls of the last step: (uint32 words, calls)."""

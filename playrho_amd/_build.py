@@ -41,11 +41,18 @@ def build_library(out=LIB):
     objs = []
     tmp = os.path.join(PKG, "_build")
     os.makedirs(tmp, exist_ok=True)
+    # The translation units compile in parallel.
+    procs = []
     for src in ("step_kernel.hip", "big_world.hip", "adjoint.hip", "host_world.cpp"):
         obj = os.path.join(tmp, src + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
-        _run([HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
+        cmd = [HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)))
         objs.append(obj)
+    for cmd, p in procs:
+        so, se = p.communicate()
+        if p.returncode != 0:
+            raise RuntimeError("build failed: %s\n%s\n%s" % (" ".join(cmd), so[-4000:], se[-8000:]))
     tmp_out = out + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp_out] + objs)
     os.replace(tmp_out, out)

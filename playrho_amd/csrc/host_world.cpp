@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -326,6 +327,7 @@ struct prx_world {
     prx_batch* solo = nullptr;   // single-world batch used by prx_step
     prx_step_stats last_stats = DefaultStats();  // StepStats{}
     bool record_events = false;  // the device records the contact listener calls of each step
+    std::uint32_t event_words = 0;  // initial words of the call record (0: kEvWordsPerSlot per contact slot)
     bool has_listener = false;
     prx_contact_listener listener{};
     std::vector<std::uint32_t> events;  // listener calls of the last step (device word format)
@@ -372,6 +374,7 @@ struct prx_batch {
     std::size_t totB = 0, totP = 0, totS = 0, totV = 0, totC = 0, totJ = 0, totE = 0;
     int lds_bytes = 16 * 1024;
     int lds_min = 16 * 1024;
+    std::vector<void*> snapshot;  // state before a recorded step (listener record growth)
     ~prx_batch() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -806,7 +809,10 @@ int BuildBatch(prx_batch* b) {
     for (int i = 0; i < nw; ++i) {
         auto& h = b->hdr[std::size_t(i)];
         h.ev_off = int(E);
-        h.ev_cap = b->worlds[std::size_t(i)]->record_events ? prx::kEvWordsPerSlot * h.contact_cap + 16 : 0;
+        const prx_world* wi = b->worlds[std::size_t(i)];
+        h.ev_cap = !wi->record_events ? 0
+                   : wi->event_words  ? int(wi->event_words)
+                                      : prx::kEvWordsPerSlot * h.contact_cap + 16;
         E += std::size_t(h.ev_cap);
         if (E > std::size_t(0x7fffffff)) return Fail(PRX_E_LENGTH, "contact listener records exceed 2^31 words");
     }
@@ -1166,9 +1172,21 @@ int CreateBatch(prx_world* const* worlds, int n, const prx_batch_conf* conf, prx
 }
 
 int CheckStatus(prx_batch* b) {
-    // Status words are checked after each synchronising call.
+    // Status words are checked after each synchronising call.  A lost
+    // listener call or a TOI phase cut short affects the steps since the last
+    // check only: those bits are reported once, then cleared, so the worlds
+    // step on.  Contact and scratch overflows leave a world's contact set
+    // incomplete and stay set.
     std::vector<prx::WorldHeader> hdr(b->worlds.size());
     PRX_HIP(hipMemcpy(hdr.data(), b->d.hdr, hdr.size() * sizeof(prx::WorldHeader), hipMemcpyDeviceToHost));
+    constexpr std::uint32_t kRecoverable = prx::kStatusEventOverflow | prx::kStatusToiLimit;
+    for (std::size_t i = 0; i < hdr.size(); ++i) {
+        if (std::uint32_t(hdr[i].status) & kRecoverable) {
+            const int cleared = int(std::uint32_t(hdr[i].status) & ~kRecoverable);
+            PRX_HIP(hipMemcpy(reinterpret_cast<char*>(b->d.hdr + i) + offsetof(prx::WorldHeader, status), &cleared,
+                              sizeof(int), hipMemcpyHostToDevice));
+        }
+    }
     for (const auto& h : hdr) {
         if (h.status & prx::kStatusContactOverflow)
             return Fail(PRX_E_LENGTH, "contact capacity of a world exceeded (raise contact_capacity_per_proxy)");
@@ -1295,6 +1313,66 @@ bool HasListener(const prx_batch* b) {
     for (const prx_world* w : b->worlds)
         if (w && w->has_listener) return true;
     return false;
+}
+
+std::vector<std::pair<void*, std::size_t>> DynamicArrays(prx_batch* b);
+
+// One step of a batch whose worlds record listener calls.  A step whose calls
+// exceed a world's call record (the TOI phase's calls have no fixed bound per
+// contact slot) is stepped again from a device copy of the state taken before
+// it, with that world's record grown to what the step needed; the calls
+// delivered are then the complete list.
+int StepRecorded(prx_batch* b, const prx_step_conf* conf) {
+    if (b->totE == 0) {
+        if (int st = StepBatch(b, conf, 1)) return st;
+        PRX_HIP(hipStreamSynchronize(b->stream));
+        return CheckStatus(b);
+    }
+    std::vector<std::pair<void*, std::size_t>> arrays = DynamicArrays(b);
+    arrays.pop_back();  // the previous step's calls (the last entry) need no copy
+    if (b->snapshot.empty()) {
+        for (const auto& a : arrays) {
+            unsigned char* p = nullptr;
+            PRX_HIP(b->arena.alloc(p, a.second));
+            b->snapshot.push_back(p);
+        }
+    }
+    for (int attempt = 0;; ++attempt) {
+        for (std::size_t i = 0; i < arrays.size(); ++i)
+            PRX_HIP(hipMemcpyAsync(b->snapshot[i], arrays[i].first, arrays[i].second, hipMemcpyDeviceToDevice,
+                                   b->stream));
+        if (int st = StepBatch(b, conf, 1)) return st;
+        PRX_HIP(hipStreamSynchronize(b->stream));
+        std::vector<prx::WorldHeader> hdr;
+        if (int st = Download(hdr, b->d.hdr, b->worlds.size())) return st;
+        bool over = false;
+        for (const auto& h : hdr) over = over || (h.status & prx::kStatusEventOverflow) != 0;
+        if (!over || attempt >= 8) return CheckStatus(b);
+        // Back to the state before the step, then a larger record for every
+        // world that ran out (new offsets for all: the segments are contiguous).
+        for (std::size_t i = 0; i < arrays.size(); ++i)
+            PRX_HIP(hipMemcpyAsync(arrays[i].first, b->snapshot[i], arrays[i].second, hipMemcpyDeviceToDevice,
+                                   b->stream));
+        PRX_HIP(hipStreamSynchronize(b->stream));
+        std::vector<prx::WorldHeader> cur;
+        if (int st = Download(cur, b->d.hdr, b->worlds.size())) return st;
+        std::size_t E = 0;
+        for (std::size_t i = 0; i < cur.size(); ++i) {
+            int cap = cur[i].ev_cap;
+            if (cap > 0 && (hdr[i].status & prx::kStatusEventOverflow))
+                cap = std::max(2 * cap, hdr[i].ev_words + 16);
+            cur[i].ev_off = int(E);
+            cur[i].ev_cap = cap;
+            b->hdr[i].ev_off = cur[i].ev_off;
+            b->hdr[i].ev_cap = cur[i].ev_cap;
+            E += std::size_t(cap);
+            if (E > std::size_t(0x7fffffff)) return Fail(PRX_E_LENGTH, "contact listener records exceed 2^31 words");
+        }
+        PRX_HIP(b->arena.alloc(b->d.ev, E));
+        b->totE = E;
+        if (int st = Upload(b->d.hdr, cur)) return st;
+        b->graph_valid = false;  // the graph holds the old record's address
+    }
 }
 
 }  // namespace
@@ -1821,9 +1899,7 @@ int prx_step(prx_world* w, const prx_step_conf* conf, prx_step_stats* stats) {
         if (int st = CreateBatch(one, 1, &bc, &b)) return st;
         w->solo = b;
     }
-    if (int st = StepBatch(w->solo, conf, 1)) return st;
-    PRX_HIP(hipStreamSynchronize(w->solo->stream));
-    if (int st = CheckStatus(w->solo)) return st;
+    if (int st = StepRecorded(w->solo, conf)) return st;
     if (int st = DeliverEvents(w->solo)) return st;
     if (stats) {
         prx::WorldHeader h;
@@ -1929,6 +2005,7 @@ int prx_world_shift_origin(prx_world* w, const float origin[2]) {
 
 int prx_world_set_velocity(prx_world* w, std::uint32_t body, const float v[3]) {
     if (int st = CheckWorld(w)) return st;
+    if (!v) return Fail(PRX_E_INVALID_ARGUMENT, "null velocity");
     if (body >= w->bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
     if (int st = Detach(w)) return st;
     HostBody& b = w->bodies[body];
@@ -2029,6 +2106,16 @@ int prx_world_record_contact_events(prx_world* w, int on) {
         w->events.clear();
         w->nevents = 0;
     }
+    return PRX_OK;
+}
+
+int prx_world_set_event_record_words(prx_world* w, std::uint32_t words) {
+    if (int st = CheckWorld(w)) return st;
+    if (words > 0x40000000u) return Fail(PRX_E_INVALID_ARGUMENT, "record size above 2^30 words");
+    if (w->owner && w->owner != w->solo)
+        return Fail(PRX_E_WRONG_STATE, "the world is in a batch: size its call record before batching it");
+    if (int st = Detach(w)) return st;
+    w->event_words = words;
     return PRX_OK;
 }
 
@@ -2152,9 +2239,7 @@ int prx_batch_step(prx_batch* b, const prx_step_conf* conf, int nsteps) {
     // Worlds with listeners: one step at a time, each step's calls delivered
     // before the next step.
     for (int s = 0; s < nsteps; ++s) {
-        if (int st = StepBatch(b, conf, 1)) return st;
-        PRX_HIP(hipStreamSynchronize(b->stream));
-        if (int st = CheckStatus(b)) return st;
+        if (int st = StepRecorded(b, conf)) return st;
         if (int st = DeliverEvents(b)) return st;
     }
     return PRX_OK;

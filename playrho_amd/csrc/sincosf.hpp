@@ -11,11 +11,36 @@
 //   * |x| >= 120      : Payne-Hanek style reduction with a 4/pi bit table
 //
 // This is the published method of ARM's optimized-routines sincosf (Szabolcs
-// Nagy, 2018; MIT / Apache-2.0 with LLVM exception), which glibc adopted.  The
-// x86-64 glibc selects a build of it compiled with FMA contraction on CPUs that
-// support FMA; PRX_SINCOS_FMA picks that evaluation (fused multiply-adds in
-// the reduction and the polynomials).  Exhaustive CPU comparison against the
-// host library is in tests/test_sincosf.py.
+// Nagy, 2018), which glibc adopted.  The x86-64 glibc selects a build of it
+// compiled with FMA contraction on CPUs that support FMA; PRX_SINCOS_FMA picks
+// that evaluation (fused multiply-adds in the reduction and the polynomials).
+// Checks: tests/test_oracle_golden.py compares this restatement on the host
+// with the host C library over a dense sample of angles; the device build is
+// compared with the host library in tests/test_gpu_parity.py
+// (test_device_sincos_matches_host_library).
+//
+// The polynomial coefficients, the reduction constants and the 4/pi bit table
+// below are those of optimized-routines math/sincosf.h and
+// math/sincosf_data.c, used under its licence:
+//
+//   Copyright (c) 2018, Arm Limited.
+//   SPDX-License-Identifier: MIT OR Apache-2.0 WITH LLVM-exception
+//
+//   MIT License: Permission is hereby granted, free of charge, to any person
+//   obtaining a copy of this software and associated documentation files (the
+//   "Software"), to deal in the Software without restriction, including
+//   without limitation the rights to use, copy, modify, merge, publish,
+//   distribute, sublicense, and/or sell copies of the Software, and to permit
+//   persons to whom the Software is furnished to do so, subject to the
+//   following conditions: The above copyright notice and this permission
+//   notice shall be included in all copies or substantial portions of the
+//   Software.  THE SOFTWARE IS PROVIDED "AS IS", WITHOUT WARRANTY OF ANY KIND,
+//   EXPRESS OR IMPLIED, INCLUDING BUT NOT LIMITED TO THE WARRANTIES OF
+//   MERCHANTABILITY, FITNESS FOR A PARTICULAR PURPOSE AND NONINFRINGEMENT. IN
+//   NO EVENT SHALL THE AUTHORS OR COPYRIGHT HOLDERS BE LIABLE FOR ANY CLAIM,
+//   DAMAGES OR OTHER LIABILITY, WHETHER IN AN ACTION OF CONTRACT, TORT OR
+//   OTHERWISE, ARISING FROM, OUT OF OR IN CONNECTION WITH THE SOFTWARE OR THE
+//   USE OR OTHER DEALINGS IN THE SOFTWARE.
 #pragma once
 
 #include <cstdint>

@@ -25,6 +25,8 @@ def lib():
         L.oracle_create.restype = ctypes.c_void_p
         L.oracle_create.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int]
         L.oracle_destroy.argtypes = [ctypes.c_void_p]
+        L.oracle_widen.restype = ctypes.c_void_p
+        L.oracle_widen.argtypes = [ctypes.c_void_p]
         L.oracle_step.restype = ctypes.c_void_p
         L.oracle_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                   ctypes.POINTER(ctypes.c_size_t)]
@@ -52,9 +54,17 @@ def _take(ptr, n):
 class OracleWorld:
     """One oracle world built from a description+conf blob."""
 
-    def __init__(self, desc_conf: bytes, precision=32):
-        self.h = lib().oracle_create(desc_conf, len(desc_conf), precision)
+    def __init__(self, desc_conf: bytes, precision=32, handle=None):
+        self.h = handle if handle is not None else lib().oracle_create(desc_conf, len(desc_conf), precision)
+        self.precision = precision
         self.nbodies = None
+
+    def widen(self):
+        """The FP64 oracle stepping on from this float world's current state
+        (every real widened exactly; cached transformations recomputed in
+        double)."""
+        assert self.precision == 32
+        return OracleWorld(b"", 64, handle=lib().oracle_widen(self.h))
 
     def __del__(self):
         if getattr(self, "h", None):

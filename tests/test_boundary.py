@@ -86,6 +86,16 @@ def test_error_codes():
     assert e.value.code == 4
     conf = pa.StepConf(do_toi=1)
     assert conf.do_toi == 1
+    # Null value pointers of the setters are InvalidArgument.
+    w2 = pa.World.scene("pyramid", [2])
+    lib = pa.lib()
+    assert lib.prx_world_set_velocity(w2.h, 1, None) == 1
+    assert lib.prx_world_set_transformation(w2.h, 1, None) == 1
+    assert lib.prx_world_set_location(w2.h, 1, None) == 1
+    assert lib.prx_world_shift_origin(w2.h, None) == 1
+    # The listener-call record size is set before batching.
+    assert lib.prx_world_set_event_record_words(w2.h, 64) == 0
+    assert lib.prx_world_set_event_record_words(w2.h, 0x40000001) == 1
 
 
 def test_step_conf_defaults_match_reference():

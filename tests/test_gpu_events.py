@@ -125,3 +125,32 @@ def test_recording_switch_inside_batch_is_wrong_state():
         ws[0].record_contact_events(True)
     assert e.value.code == 3
     del batch
+
+
+@pytest.mark.parametrize("name,steps", [("pile3_toi", 150), ("billiards0_toi", 120)])
+def test_call_record_grows_without_losing_calls(name, steps):
+    """A world whose call record starts at 24 words: every step with more
+    calls (with doToi, TOI events add updates and post-solve reports per
+    event) is stepped again from the state before it with a larger record.
+    The calls delivered and the state after every step still equal the
+    oracle's and the reference's."""
+    g = golden_cases.load(name)
+    conf = pa.StepConf(do_toi=1 if g["do_toi"] else 0)
+    print("case", name, "steps", steps, "record 24 words", flush=True)
+    w = pa.World.from_desc(g["desc_conf"])
+    w.set_event_record_words(24)
+    col = Collector()
+    col.attach(w)
+    ow = oracle_lib.OracleWorld(g["desc_conf"])
+    most = 0
+    for step in range(1, steps + 1):
+        w.step(conf)
+        want = _oracle_calls(ow.step(1)[0])
+        got = col.take()
+        assert got == want, "%s step %d: %d calls, oracle %d" % (name, step, len(got), len(want))
+        most = max(most, w.contact_event_words()[0].size)
+        s = dumpfmt.read_state(dumpfmt.Reader(w.state_record()))
+        for k, part in enumerate(golden_cases.PARTS[:4]):
+            assert dumpfmt.digest(s, (part,)) == g["digests"][step, k].tobytes(), (step, part)
+    print("largest step record: %d words" % most, flush=True)
+    assert most > 24
