@@ -42,6 +42,11 @@ CASES = [
     # falling bodies.
     ("joints0", "joints", 240, [1, 0], [1, 120, 240]),
     ("joints1", "joints", 240, [2, 1, 12, 40], [1, 240]),
+    # The two largest worlds of the 4096-world pile batch (seed 1): 513 and
+    # 511 bodies, through piling and settling (their settled islands take the
+    # level form of the wave-wide sweeps on the device).
+    ("pile3301", "pile", 300, [1, 3301], [1, 300]),
+    ("pile859", "pile", 300, [1, 859], [1, 300]),
 ]
 # World edits between steps (step, kind, body, a, b, c): SetLocation 'L',
 # SetAngle 'A', SetTransformation 'T' (c the rotation's angle), ShiftOrigin 'O'.
@@ -63,6 +68,8 @@ TOI_CASES = [
     ("pile3_toi", "pile", 150, [1, 3], [1, 150]),
     ("chain0_toi", "chain", 240, [1, 0], [1, 240]),
     ("joints0_toi", "joints", 240, [1, 0], [1, 240]),
+    ("pile3301_toi", "pile", 300, [1, 3301], [1, 300]),
+    ("pile859_toi", "pile", 300, [1, 859], [1, 300]),
 ]
 
 

@@ -199,3 +199,42 @@ def test_world_edits_match_reference_every_step(name):
         for k, part in enumerate(golden_cases.PARTS):
             if dumpfmt.digest(s, (part,)) != g["digests"][step, k].tobytes():
                 pytest.fail("%s: step %d differs from the reference in %s" % (name, step, part))
+
+
+@pytest.mark.parametrize("toi", [False, True])
+def test_largest_worlds_inside_autotuned_4096_world_batch(toi):
+    """The worlds that set the batched step's time: the 4096-world pile batch
+    of the bench (seed 1) stepped 500 steps by one hipGraph launch per step,
+    with the schedule autotuned on its state at step 200.  The two largest
+    worlds (513 and 511 bodies) and eight others of all sizes equal the
+    oracle every 50 steps, and the reference goldens of the two largest at
+    every golden step reached (steps 1..300, checked at the 50-step marks)."""
+    conf = pa.StepConf(do_toi=1 if toi else 0)
+    worlds = [pa.World.scene("pile", [1, w]) for w in range(4096)]
+    sizes = [w.body_count for w in worlds]
+    big = sorted(range(4096), key=lambda i: -sizes[i])[:2]
+    assert sorted(sizes[i] for i in big) == [511, 513]
+    picks = big + [7, 100, 777, 1500, 2048, 2999, 3500, 4095]
+    for i in picks:
+        _recording(worlds[i])
+    oracles = {i: oracle_lib.OracleWorld(worlds[i].save_desc() + dumpfmt.conf_record(conf)) for i in picks}
+    golden = {3301: golden_cases.load("pile3301" + ("_toi" if toi else "")),
+              859: golden_cases.load("pile859" + ("_toi" if toi else ""))}
+    batch = pa.Batch(worlds, use_graph=True)
+    done = 0
+    for mark in range(50, 501, 50):
+        batch.step(conf, mark - done)
+        if mark == 200:
+            batch.sync()
+            print("autotune at step 200:", batch.autotune(conf, nsteps=10), flush=True)
+        done = mark
+        batch.download()
+        print("step", mark, "toi", toi, flush=True)
+        for i in picks:
+            got = _state(worlds[i])
+            want = oracles[i].step(50)[-1]
+            for part in golden_cases.PARTS:
+                assert dumpfmt.digest(got, (part,)) == dumpfmt.digest(want, (part,)), (i, mark, part)
+            if i in golden and mark <= 300:
+                for k, part in enumerate(golden_cases.PARTS):
+                    assert dumpfmt.digest(got, (part,)) == golden[i]["digests"][mark, k].tobytes(), (i, mark, part)
