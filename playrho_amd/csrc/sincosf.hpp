@@ -54,44 +54,28 @@
 
 namespace prx_math {
 
+// The published method's constants: 2/pi * 2^24 and pi/2 for the reduction,
+// the cosine (c0..c4) and sine (s1..s3) polynomial coefficients.
 struct SinCosTab {
-    double sign[4];
     double hpi_inv;  // 2/pi * 2^24
     double hpi;      // pi/2
     double c0, c1, c2, c3, c4;
     double s1, s2, s3;
 };
 
-PRX_HD const SinCosTab& Tab(int i) {
-#ifdef __HIP_DEVICE_COMPILE__
-    __constant__ static const SinCosTab t[2] = {
-#else
-    static const SinCosTab t[2] = {
-#endif
-        {{1.0, -1.0, -1.0, 1.0},
-         0x1.45F306DC9C883p+23,
-         0x1.921FB54442D18p0,
-         0x1p0,
-         -0x1.ffffffd0c621cp-2,
-         0x1.55553e1068f19p-5,
-         -0x1.6c087e89a359dp-10,
-         0x1.99343027bf8c3p-16,
-         -0x1.555545995a603p-3,
-         0x1.1107605230bc4p-7,
-         -0x1.994eb3774cf24p-13},
-        {{1.0, -1.0, -1.0, 1.0},
-         0x1.45F306DC9C883p+23,
-         0x1.921FB54442D18p0,
-         -0x1p0,
-         0x1.ffffffd0c621cp-2,
-         -0x1.55553e1068f19p-5,
-         0x1.6c087e89a359dp-10,
-         -0x1.99343027bf8c3p-16,
-         -0x1.555545995a603p-3,
-         0x1.1107605230bc4p-7,
-         -0x1.994eb3774cf24p-13},
-    };
-    return t[i];
+// By value from a constant expression, so the coefficients fold into the
+// code as immediates.
+PRX_HD constexpr SinCosTab Tab() {
+    return SinCosTab{0x1.45F306DC9C883p+23,
+                     0x1.921FB54442D18p0,
+                     0x1p0,
+                     -0x1.ffffffd0c621cp-2,
+                     0x1.55553e1068f19p-5,
+                     -0x1.6c087e89a359dp-10,
+                     0x1.99343027bf8c3p-16,
+                     -0x1.555545995a603p-3,
+                     0x1.1107605230bc4p-7,
+                     -0x1.994eb3774cf24p-13};
 }
 
 PRX_HD std::uint32_t InvPio4(int i) {
@@ -133,9 +117,16 @@ PRX_HD double MulAdd(double a, double b, double c) {
     }
 }
 
-// Polynomial for sin (n even) or cos (n odd) on the reduced argument.
+// Polynomial for sin (n even) or cos (n odd) on the reduced argument.  The
+// second table of the published method holds the cosine coefficients
+// negated (and the same sine coefficients), so evaluating with it equals
+// negating the cosine polynomial of the first: every step is a product or a
+// sum of sign-symmetric roundings, and the cosine polynomial on the reduced
+// range stays near cos(r) >= 0.7, never 0.  `negCos` selects that table.
+// Selecting by a flag instead of a table pointer keeps every coefficient a
+// compile-time constant: no per-lane table load inside the solver chains.
 template <bool Fma>
-PRX_HD float Poly(double x, double x2, const SinCosTab& p, int n) {
+PRX_HD float Poly(double x, double x2, const SinCosTab& p, int n, bool negCos) {
     if ((n & 1) == 0) {
         const double x3 = x * x2;
         const double s1 = MulAdd<Fma>(x2, p.s3, p.s2);
@@ -148,8 +139,12 @@ PRX_HD float Poly(double x, double x2, const SinCosTab& p, int n) {
     const double c1 = MulAdd<Fma>(x2, p.c1, p.c0);
     const double x6 = x4 * x2;
     const double c = MulAdd<Fma>(x4, p.c2, c1);
-    return float(MulAdd<Fma>(x6, c2, c));
+    const float r = float(MulAdd<Fma>(x6, c2, c));
+    return negCos ? -r : r;
 }
+
+// The published sign table {1, -1, -1, 1}[i & 3].
+PRX_HD double QuadrantSign(int i) { return ((i + 1) & 2) ? -1.0 : 1.0; }
 
 template <bool Fma>
 PRX_HD double ReduceFast(double x, const SinCosTab& p, int* np) {
@@ -179,51 +174,47 @@ PRX_HD double ReduceLarge(std::uint32_t xi, int* np) {
 template <bool Fma>
 PRX_HD float SinF(float y) {
     double x = y;
-    const SinCosTab* p = &Tab(0);
+    constexpr SinCosTab p = Tab();
     int n;
     const std::uint32_t t = AbsTop12(y);
     if (t < AbsTop12(0x1.921FB6p-1f)) {
         const double s = x * x;
         if (t < AbsTop12(0x1p-12f)) return y;
-        return Poly<Fma>(x, s, *p, 0);
+        return Poly<Fma>(x, s, p, 0, false);
     }
     if (t < AbsTop12(120.0f)) {
-        x = ReduceFast<Fma>(x, *p, &n);
-        const double s = p->sign[n & 3];
-        if (n & 2) p = &Tab(1);
-        return Poly<Fma>(x * s, x * x, *p, n);
+        x = ReduceFast<Fma>(x, p, &n);
+        const double s = QuadrantSign(n);
+        return Poly<Fma>(x * s, x * x, p, n, (n & 2) != 0);
     }
     const std::uint32_t xi = AsU32(y);
     const int sign = int(xi >> 31);
     x = ReduceLarge(xi, &n);
-    const double s = p->sign[(n + sign) & 3];
-    if ((n + sign) & 2) p = &Tab(1);
-    return Poly<Fma>(x * s, x * x, *p, n);
+    const double s = QuadrantSign(n + sign);
+    return Poly<Fma>(x * s, x * x, p, n, ((n + sign) & 2) != 0);
 }
 
 template <bool Fma>
 PRX_HD float CosF(float y) {
     double x = y;
-    const SinCosTab* p = &Tab(0);
+    constexpr SinCosTab p = Tab();
     int n;
     const std::uint32_t t = AbsTop12(y);
     if (t < AbsTop12(0x1.921FB6p-1f)) {
         const double x2 = x * x;
         if (t < AbsTop12(0x1p-12f)) return 1.0f;
-        return Poly<Fma>(x, x2, *p, 1);
+        return Poly<Fma>(x, x2, p, 1, false);
     }
     if (t < AbsTop12(120.0f)) {
-        x = ReduceFast<Fma>(x, *p, &n);
-        const double s = p->sign[n & 3];
-        if (n & 2) p = &Tab(1);
-        return Poly<Fma>(x * s, x * x, *p, n ^ 1);
+        x = ReduceFast<Fma>(x, p, &n);
+        const double s = QuadrantSign(n);
+        return Poly<Fma>(x * s, x * x, p, n ^ 1, (n & 2) != 0);
     }
     const std::uint32_t xi = AsU32(y);
     const int sign = int(xi >> 31);
     x = ReduceLarge(xi, &n);
-    const double s = p->sign[(n + sign) & 3];
-    if ((n + sign) & 2) p = &Tab(1);
-    return Poly<Fma>(x * s, x * x, *p, n ^ 1);
+    const double s = QuadrantSign(n + sign);
+    return Poly<Fma>(x * s, x * x, p, n ^ 1, ((n + sign) & 2) != 0);
 }
 
 // Both values with one argument reduction; each equals SinF / CosF above
@@ -231,7 +222,7 @@ PRX_HD float CosF(float y) {
 template <bool Fma>
 PRX_HD void SinCosF(float y, float* sinOut, float* cosOut) {
     double x = y;
-    const SinCosTab* p = &Tab(0);
+    constexpr SinCosTab p = Tab();
     int n;
     const std::uint32_t t = AbsTop12(y);
     if (t < AbsTop12(0x1.921FB6p-1f)) {
@@ -241,25 +232,26 @@ PRX_HD void SinCosF(float y, float* sinOut, float* cosOut) {
             *cosOut = 1.0f;
             return;
         }
-        *sinOut = Poly<Fma>(x, x2, *p, 0);
-        *cosOut = Poly<Fma>(x, x2, *p, 1);
+        *sinOut = Poly<Fma>(x, x2, p, 0, false);
+        *cosOut = Poly<Fma>(x, x2, p, 1, false);
         return;
     }
     double s;
+    bool neg;
     if (t < AbsTop12(120.0f)) {
-        x = ReduceFast<Fma>(x, *p, &n);
-        s = p->sign[n & 3];
-        if (n & 2) p = &Tab(1);
+        x = ReduceFast<Fma>(x, p, &n);
+        s = QuadrantSign(n);
+        neg = (n & 2) != 0;
     } else {
         const std::uint32_t xi = AsU32(y);
         const int sign = int(xi >> 31);
         x = ReduceLarge(xi, &n);
-        s = p->sign[(n + sign) & 3];
-        if ((n + sign) & 2) p = &Tab(1);
+        s = QuadrantSign(n + sign);
+        neg = ((n + sign) & 2) != 0;
     }
     const double xs = x * s, x2 = x * x;
-    *sinOut = Poly<Fma>(xs, x2, *p, n);
-    *cosOut = Poly<Fma>(xs, x2, *p, n ^ 1);
+    *sinOut = Poly<Fma>(xs, x2, p, n, neg);
+    *cosOut = Poly<Fma>(xs, x2, p, n ^ 1, neg);
 }
 
 }  // namespace prx_math

@@ -104,6 +104,10 @@ __device__ __forceinline__ float4 proxy_aabb(const Proxy& px, const Xf& xf0, con
     return float4{mnx, mny, mxx, mxy};
 }
 
+// Dynamic LDS of the step kernels (size StepParams::ldsBytes in k_step_worlds,
+// none in k_toi_worlds).
+extern __shared__ float4 g_smem[];
+
 // IsIntersecting per axis (Interval.hpp:292-298).
 __device__ __forceinline__ bool overlap(float4 a, float4 b) {
     const float mx0 = fmax_std(a.x, b.x);
@@ -404,7 +408,15 @@ __device__ __forceinline__ bool joint_allows(const DeviceState& d, const WorldCt
 // FindContacts (AabbTreeWorld.cpp:709-751) + AddContacts (:2424-2547).
 // Candidate keys pack world-local proxy indices (min << 16 | max); proxy
 // indices increase with tree leaf ids, so key order equals ContactKey order.
-__device__ __forceinline__ std::uint32_t phase_find_add(const DeviceState& d, const WorldCtx& x, Counters& k, const StepParams& c) {
+__device__ __forceinline__ int readlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ float readlane_f(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// ldsAvail: bytes of dynamic LDS the caller's kernel has (free during this
+// phase).  When the candidates fit, they are sorted and looked up there.
+__device__ __forceinline__ std::uint32_t phase_find_add(const DeviceState& d, const WorldCtx& x, Counters& k, const StepParams& c,
+                                                      int ldsAvail) {
     // Moved proxies, in index order.
     int nmoved = 0;
     for (int base = 0; base < x.np; base += kWave) {
@@ -422,7 +434,10 @@ __device__ __forceinline__ std::uint32_t phase_find_add(const DeviceState& d, co
     if (nmoved == 0) return 0;
     int* moved = d.s_isl_body + x.bo + x.co;
 
-    // Candidate pairs: every proxy q against every moved proxy m.
+    // Candidate pairs: every proxy q (one per lane) against every moved proxy
+    // m.  The moved proxies come in chunks of one per lane (index, body and
+    // fat AABB in registers) and the wave reads them one at a time from the
+    // holding lane, so the inner loop does no memory access.
     int ncand = 0;
     bool overflow = false;
     for (int base = 0; base < x.np; base += kWave) {
@@ -436,24 +451,31 @@ __device__ __forceinline__ std::uint32_t phase_find_add(const DeviceState& d, co
             qb = d.pbody[x.po + q];
             qmoved = (d.pflags[x.po + q] & prx::kProxyMoved) != 0;
         }
-        for (int j = 0; j < nmoved; ++j) {
-            const int m = moved[j];
-            bool hit = false;
-            if (qv && q != m) {
+        for (int mb = 0; mb < nmoved; mb += kWave) {
+            const int cnt = (nmoved - mb < kWave) ? nmoved - mb : kWave;
+            int mi = 0, mbody = -1;
+            float4 ma = float4{0, 0, 0, 0};
+            if (x.lane < cnt) {
+                mi = moved[mb + x.lane];
+                mbody = d.pbody[x.po + mi];
+                ma = d.paabb[x.po + mi];
+            }
+            for (int j = 0; j < cnt; ++j) {
+                const int m = readlane_i(mi, j);
+                const int mbd = readlane_i(mbody, j);
+                const float4 am{readlane_f(ma.x, j), readlane_f(ma.y, j), readlane_f(ma.z, j), readlane_f(ma.w, j)};
                 // A pair of two moved proxies is produced once, from the smaller.
-                if (!(qmoved && q < m)) {
-                    if (qb != d.pbody[x.po + m] && overlap(d.paabb[x.po + m], qa)) hit = true;
+                const bool hit = qv && q != m && !(qmoved && q < m) && qb != mbd && overlap(am, qa);
+                const unsigned long long hm = __ballot(hit);
+                if (hit) {
+                    const int r = ncand + __popcll(hm & lanes_below(x.lane));
+                    if (r < x.cap) {
+                        const int lo = q < m ? q : m, hi = q < m ? m : q;
+                        d.s_keys[x.co + r] = (std::uint32_t(lo) << 16) | std::uint32_t(hi);
+                    }
                 }
+                ncand += __popcll(hm);
             }
-            const unsigned long long hm = __ballot(hit);
-            if (hit) {
-                const int r = ncand + __popcll(hm & lanes_below(x.lane));
-                if (r < x.cap) {
-                    const int lo = q < m ? q : m, hi = q < m ? m : q;
-                    d.s_keys[x.co + r] = (std::uint32_t(lo) << 16) | std::uint32_t(hi);
-                }
-            }
-            ncand += __popcll(hm);
         }
     }
     if (ncand > x.cap) {
@@ -467,29 +489,98 @@ __device__ __forceinline__ std::uint32_t phase_find_add(const DeviceState& d, co
         if (overflow) k.status |= prx::kStatusScratchOverflow;
         return 0;
     }
-    // Rank sort of the (unique) candidate keys.
-    for (int i = x.lane; i < ncand; i += kWave) {
-        const std::uint32_t key = d.s_keys[x.co + i];
-        int rank = 0;
-        for (int j = 0; j < ncand; ++j) rank += (d.s_keys[x.co + j] < key) ? 1 : 0;
-        d.s_keys2[x.co + rank] = key;
-    }
-    __syncthreads();
-    // Existing contacts: mark candidates already present (binary search).
-    for (int i = x.lane; i < ncand; i += kWave) d.s_isl_c[x.co + i] = 0;
-    __syncthreads();
-    for (int i = x.lane; i < k.nc; i += kWave) {
-        const int2 pr = d.c_prox[x.co + i];
-        const std::uint32_t key = (std::uint32_t(pr.x) << 16) | std::uint32_t(pr.y);
-        int lo = 0, hi = ncand;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (d.s_keys2[x.co + mid] < key) lo = mid + 1;
-            else hi = mid;
+    // The (unique) candidate keys in ascending order, and a mark on those
+    // that are contacts already.
+    const bool ldsSort = x.np <= 0x7fff && 4 * (ncand + 2 * x.np) <= ldsAvail;
+    std::uint32_t* skeys = reinterpret_cast<std::uint32_t*>(g_smem);  // sorted keys, bit 31: exists
+    if (ldsSort) {
+        // Counting sort by the lower proxy index in LDS (histogram, prefix
+        // sum, scatter), then each lower index's bucket (the few partners of
+        // one proxy) sorted on one lane.  Keys have bit 31 clear (indices
+        // below 2^15).
+        int* bcnt = reinterpret_cast<int*>(skeys + ncand);
+        int* bend = bcnt + x.np;
+        for (int i = x.lane; i < x.np; i += kWave) bcnt[i] = 0;
+        __syncthreads();
+        for (int i = x.lane; i < ncand; i += kWave) atomicAdd(&bcnt[d.s_keys[x.co + i] >> 16], 1);
+        __syncthreads();
+        {
+            int carry = 0;
+            for (int b0 = 0; b0 < x.np; b0 += kWave) {
+                const int i = b0 + x.lane;
+                const int v = (i < x.np) ? bcnt[i] : 0;
+                const int incl = wave_incl_scan(v, x.lane);
+                if (i < x.np) bend[i] = carry + incl - v;
+                carry += __shfl(incl, kWave - 1);
+            }
         }
-        if (lo < ncand && d.s_keys2[x.co + lo] == key) d.s_isl_c[x.co + lo] = 1;
+        __syncthreads();
+        for (int i = x.lane; i < ncand; i += kWave) {
+            const std::uint32_t key = d.s_keys[x.co + i];
+            skeys[atomicAdd(&bend[key >> 16], 1)] = key;
+        }
+        __syncthreads();
+        for (int b = x.lane; b < x.np; b += kWave) {
+            const int e = bend[b], s0 = e - bcnt[b];
+            for (int i = s0 + 1; i < e; ++i) {
+                const std::uint32_t key = skeys[i];
+                int j = i - 1;
+                while (j >= s0 && skeys[j] > key) {
+                    skeys[j + 1] = skeys[j];
+                    --j;
+                }
+                skeys[j + 1] = key;
+            }
+        }
+        __syncthreads();
+        // Existing contacts: binary search in LDS, mark with bit 31 (each
+        // contact's key is unique, so no two lanes write one slot).
+        for (int i = x.lane; i < k.nc; i += kWave) {
+            const int2 pr = d.c_prox[x.co + i];
+            const std::uint32_t key = (std::uint32_t(pr.x) << 16) | std::uint32_t(pr.y);
+            int lo = 0, hi = ncand;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if ((skeys[mid] & 0x7fffffffu) < key) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo < ncand && skeys[lo] == key) skeys[lo] = key | 0x80000000u;
+        }
+        __syncthreads();
+    } else {
+        // Rank sort in global scratch.
+        for (int i = x.lane; i < ncand; i += kWave) {
+            const std::uint32_t key = d.s_keys[x.co + i];
+            int rank = 0;
+            for (int j = 0; j < ncand; ++j) rank += (d.s_keys[x.co + j] < key) ? 1 : 0;
+            d.s_keys2[x.co + rank] = key;
+        }
+        __syncthreads();
+        // Existing contacts: mark candidates already present (binary search).
+        for (int i = x.lane; i < ncand; i += kWave) d.s_isl_c[x.co + i] = 0;
+        __syncthreads();
+        for (int i = x.lane; i < k.nc; i += kWave) {
+            const int2 pr = d.c_prox[x.co + i];
+            const std::uint32_t key = (std::uint32_t(pr.x) << 16) | std::uint32_t(pr.y);
+            int lo = 0, hi = ncand;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (d.s_keys2[x.co + mid] < key) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo < ncand && d.s_keys2[x.co + lo] == key) d.s_isl_c[x.co + lo] = 1;
+        }
+        __syncthreads();
     }
-    __syncthreads();
+    auto sorted_key = [&](int i, bool& exists) -> std::uint32_t {
+        if (ldsSort) {
+            const std::uint32_t v = skeys[i];
+            exists = (v >> 31) != 0;
+            return v & 0x7fffffffu;
+        }
+        exists = d.s_isl_c[x.co + i] != 0;
+        return d.s_keys2[x.co + i];
+    };
     // Filter, then append in key order.
     std::uint32_t added = 0;
     const int nc0 = k.nc;
@@ -498,7 +589,8 @@ __device__ __forceinline__ std::uint32_t phase_find_add(const DeviceState& d, co
         bool acc = false;
         int pa = 0, pb = 0, ba = 0, bb = 0, sa = 0, sb = 0;
         if (i < ncand) {
-            const std::uint32_t key = d.s_keys2[x.co + i];
+            bool exists = false;
+            const std::uint32_t key = sorted_key(i, exists);
             pa = int(key >> 16);
             pb = int(key & 0xffffu);
             ba = d.pbody[x.po + pa];
@@ -506,7 +598,7 @@ __device__ __forceinline__ std::uint32_t phase_find_add(const DeviceState& d, co
             sa = d.pshape[x.po + pa];
             sb = d.pshape[x.po + pb];
             const std::uint32_t fa = d.bflags[x.bo + ba], fb = d.bflags[x.bo + bb];
-            acc = ((fa | fb) & prx::kAccelerable) != 0 && should_collide(d, sa, sb) && d.s_isl_c[x.co + i] == 0 &&
+            acc = ((fa | fb) & prx::kAccelerable) != 0 && should_collide(d, sa, sb) && !exists &&
                   joint_allows(d, x, ba, bb);
         }
         const unsigned long long am = __ballot(acc);
@@ -2084,8 +2176,6 @@ __device__ __forceinline__ float4 integrated_velocity(const DeviceState& d, int 
 // Body-constraint positions and velocities live in LDS when the world fits
 // (32 bytes per body), else in global scratch.
 
-extern __shared__ float4 g_smem[];
-
 template <bool kLds>
 struct Bodies {
     float4* gpos;
@@ -2966,17 +3056,42 @@ __device__ __forceinline__ LevelPlan level_plan_of(const DeviceState& d, const W
     return LevelPlan{n, L[0], level_rounds(L, n), L + n};
 }
 
+// Round bounds of a plan during its passes.  Round k covers level positions
+// [end(k - 1), end(k)) (rounds are contiguous).  The ends are staged in LDS
+// (the record stage is free while the passes run) when they fit in 16 bits
+// and in the stage, so that no round waits on a global load for its bounds;
+// otherwise they are read from the plan in global scratch.
+struct RoundEnds {
+    const unsigned short* lds;
+    const int2* rb;
+    int rounds;
+    __device__ __forceinline__ int end(int k) const { return lds ? int(lds[k]) : rb[k].y; }
+    __device__ __forceinline__ int2 round(int k) const {
+        return int2{k > 0 ? end(k - 1) : 0, end(k)};
+    }
+};
+
+__device__ __forceinline__ RoundEnds stage_round_ends(const LevelPlan& P, float4* stage, int lane) {
+    constexpr int kCap = prx::kStageFloat4 * 8;  // 16-bit ends in the stage
+    if (P.rounds > kCap || P.n > 0xffff) return RoundEnds{nullptr, P.rb, P.rounds};
+    unsigned short* e = reinterpret_cast<unsigned short*>(stage);
+    for (int k = lane; k < P.rounds; k += kWave) e[k] = static_cast<unsigned short>(P.rb[k].y);
+    __syncthreads();
+    return RoundEnds{e, P.rb, P.rounds};
+}
+
 // Warm start and velocity sweeps of one island in level form.  Round k+1's
 // records are loaded while round k is solved (a record is read and written
 // by its own update only, once per pass).
 template <bool kLds>
 __device__ __forceinline__ VelResult island_velocity_level(const DeviceState& d, const WorldCtx& x,
                                                            const Bodies<kLds>& bs, const LevelPlan& P, int c0,
-                                                           const StepParams& c) {
+                                                           const StepParams& c, float4* stage) {
     VelResult res{0.0f, int(c.velIters)};
     float4* vbase = d.s_vc + (std::size_t(x.co) + std::size_t(c0)) * prx::kVcStride;
     float4* v2 = d.s_vc2 + (std::size_t(x.co) + std::size_t(c0)) * prx::kVcStride;
     const int npass = int(c.velIters) + (c.warmStart ? 1 : 0);
+    const RoundEnds R = stage_round_ends(P, stage, x.lane);
     auto fetch = [&](int2 r, float4 (&buf)[prx::kVcStride]) {
         const int pos = r.x + x.lane;
         if (pos < r.y) {
@@ -2989,10 +3104,10 @@ __device__ __forceinline__ VelResult island_velocity_level(const DeviceState& d,
         const bool warm = c.warmStart && pass == 0;
         float newInc = 0.0f;
         float4 cur[prx::kVcStride], nxt[prx::kVcStride];
-        int2 r = (P.rounds > 0) ? P.rb[0] : int2{0, 0};
+        int2 r = (P.rounds > 0) ? R.round(0) : int2{0, 0};
         fetch(r, cur);
         for (int k = 0; k < P.rounds; ++k) {
-            const int2 rn = (k + 1 < P.rounds) ? P.rb[k + 1] : int2{0, 0};
+            const int2 rn = (k + 1 < P.rounds) ? int2{r.y, R.end(k + 1)} : int2{0, 0};
             fetch(rn, nxt);
             const int pos = r.x + x.lane;
             if (pos < r.y) {
@@ -3037,8 +3152,10 @@ __device__ __forceinline__ VelResult island_velocity_level(const DeviceState& d,
 // ahead); returns whether solved.
 template <bool kLds>
 __device__ __forceinline__ PosResult island_position_level(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
-                                                      const LevelPlan& P, int c0, const StepParams& c) {
+                                                      const LevelPlan& P, int c0, const StepParams& c,
+                                                      float4* stage) {
     const float4* p2 = d.s_pc2 + (std::size_t(x.co) + std::size_t(c0)) * prx::kPcStride;
+    const RoundEnds R = stage_round_ends(P, stage, x.lane);
     auto fetch = [&](int2 r, float4 (&buf)[prx::kPcStride]) {
         const int pos = r.x + x.lane;
         if (pos < r.y) {
@@ -3051,10 +3168,10 @@ __device__ __forceinline__ PosResult island_position_level(const DeviceState& d,
     for (std::uint32_t it = 0; it < c.posIters; ++it) {
         float minSep = __builtin_inff();
         float4 cur[prx::kPcStride], nxt[prx::kPcStride];
-        int2 r = (P.rounds > 0) ? P.rb[0] : int2{0, 0};
+        int2 r = (P.rounds > 0) ? R.round(0) : int2{0, 0};
         fetch(r, cur);
         for (int k = 0; k < P.rounds; ++k) {
-            const int2 rn = (k + 1 < P.rounds) ? P.rb[k + 1] : int2{0, 0};
+            const int2 rn = (k + 1 < P.rounds) ? int2{r.y, R.end(k + 1)} : int2{0, 0};
             fetch(rn, nxt);
             if (r.x + x.lane < r.y) {
                 const Pc pc = load_pc(cur);
@@ -3227,7 +3344,7 @@ __device__ __forceinline__ SolveSummary solve_islands(const DeviceState& d, cons
             VelResult r;
             if (level_form(x, c0, c1, S.jstart[isl], S.jstart[isl + 1])) {
                 const LevelPlan P = level_plan(d, x, stage, c0, c1);
-                r = island_velocity_level<kLds>(d, x, bs, P, c0, c);
+                r = island_velocity_level<kLds>(d, x, bs, P, c0, c, stage);
             } else {
                 r = island_velocity_wave<kLds>(d, x, bs, stage, c0, c1, S.jstart[isl], S.jstart[isl + 1], c);
             }
@@ -3264,7 +3381,8 @@ __device__ __forceinline__ SolveSummary solve_islands(const DeviceState& d, cons
             const int isl = base + __ffsll(m) - 1;
             const int c0 = S.istart[isl].y, c1 = S.istart[isl + 1].y;
             const PosResult r = level_form(x, c0, c1, S.jstart[isl], S.jstart[isl + 1])
-                                    ? island_position_level<kLds>(d, x, bs, level_plan_of(d, x, c0, c1), c0, c)
+                                    ? island_position_level<kLds>(d, x, bs, level_plan_of(d, x, c0, c1), c0, c,
+                                                                  stage)
                                     : island_position_wave<kLds>(d, x, bs, stage, c0, c1, S.jstart[isl],
                                                                  S.jstart[isl + 1], c);
             if (x.lane == 0) {
@@ -3427,6 +3545,15 @@ __device__ __forceinline__ void post_solve_calls(const DeviceState& d, const Wor
 // The TOI phase of one world after its regular phase (a kernel of its own:
 // the serial event solves would otherwise set the register budget of the
 // regular step).  Counters and listener calls continue from the header.
+// Instruction-issue priority of a world's wavefront on its SIMD: the step
+// time of a batch is the time of its longest worlds, so larger worlds issue
+// ahead of the smaller ones that share their SIMD (these have slack).
+__device__ __forceinline__ void world_priority(int nbodies) {
+    if (nbodies >= 384) __builtin_amdgcn_s_setprio(3);
+    else if (nbodies >= 256) __builtin_amdgcn_s_setprio(2);
+    else if (nbodies >= 128) __builtin_amdgcn_s_setprio(1);
+}
+
 __device__ __forceinline__ void toi_world(const DeviceState& d, int w, const StepParams& c) {
     WorldHeader& H = d.hdr[w];
     WorldCtx x;
@@ -3449,6 +3576,7 @@ __device__ __forceinline__ void toi_world(const DeviceState& d, int w, const Ste
     k.status = std::uint32_t(H.status);
     k.evw = H.ev_words;
     k.evn = H.ev_count;
+    world_priority(x.nb);
     IslandScratch S;
     S.off = d.s_csr_off + x.bo + w;
     S.csr = reinterpret_cast<int2*>(d.s_csr) + 2 * x.co;
@@ -3504,6 +3632,7 @@ __device__ __forceinline__ void step_world(const DeviceState& d, int w, const St
         H.stats[prx::kStRegMinSeparation] = 0x7f800000u;  // +infinity: no position iteration ran
         H.stats[prx::kStToiMinSeparation] = 0x7f800000u;
     }
+    world_priority(x.nb);
     PhaseTimer T;
     T.begin(d.phase_cycles ? d.phase_cycles + std::size_t(w) * prx::kPhases : nullptr);
 
@@ -3546,7 +3675,7 @@ __device__ __forceinline__ void step_world(const DeviceState& d, int w, const St
     }
     __syncthreads();
     T.mark(prx::kPhUpdate);
-    stat(prx::kStPreAdded, phase_find_add(d, x, k, c));
+    stat(prx::kStPreAdded, phase_find_add(d, x, k, c, int(c.ldsBytes)));
     __syncthreads();
     T.mark(prx::kPhFindAdd);
 
@@ -3607,7 +3736,7 @@ __device__ __forceinline__ void step_world(const DeviceState& d, int w, const St
         }
         __syncthreads();
         T.mark(prx::kPhUpdate2);
-        stat(prx::kStRegAdded, phase_find_add(d, x, k, c));
+        stat(prx::kStRegAdded, phase_find_add(d, x, k, c, int(c.ldsBytes)));
         __syncthreads();
         T.mark(prx::kPhFindAdd2);
     }

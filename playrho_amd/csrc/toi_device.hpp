@@ -988,7 +988,7 @@ __device__ __forceinline__ void toi_phase(const DeviceState& d, const WorldCtx& 
         for (int b = x.lane; b < x.nb; b += kWave) d.t_isl_b[x.bo + b] = 0;
         __syncthreads();
         const int nc0 = k.nc;
-        const std::uint32_t added = phase_find_add(d, x, k, c);
+        const std::uint32_t added = phase_find_add(d, x, k, c, 0);
         if (x.lane == 0) st[prx::kStToiAdded] += added;
         __syncthreads();
         if (k.nc > nc0) {

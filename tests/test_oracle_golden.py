@@ -89,8 +89,12 @@ int main() {
       float x; std::memcpy(&x, &u, 4);
       float c = std::cos(x), s = std::sin(x);
       float c1 = prx_math::CosF<true>(x), s1 = prx_math::SinF<true>(x);
+      float s2, c2;
+      prx_math::SinCosF<true>(x, &s2, &c2);
       bad += std::memcmp(&c, &c1, 4) != 0;
       bad += std::memcmp(&s, &s1, 4) != 0;
+      bad += std::memcmp(&c, &c2, 4) != 0;
+      bad += std::memcmp(&s, &s2, 4) != 0;
     }
   }
   std::printf("%ld\n", bad);
