@@ -29,6 +29,12 @@ __device__ __forceinline__ std::uint32_t flip_cf(std::uint32_t c) {
 
 enum : std::uint32_t { kUnset = 0, kCircles = 1, kFaceA = 2, kFaceB = 3 };
 
+// Vertex counts up to which a shape's vertices and normals are loaded into
+// registers in one batch before the separation loops (static indices, so no
+// load waits inside the loops); larger polygons take a loop with a load per
+// iteration.
+constexpr std::uint32_t kStagedVerts = 8;
+
 struct MPoint {
     V2 lp;
     std::uint32_t cf;
@@ -219,12 +225,36 @@ __device__ __forceinline__ Manifold poly_circle_manifold(bool flipped, float tot
     const V2 cLocal = invtransform(transform(point, xf), sxf);
     std::uint32_t iMax = 0;
     float maxSep = -3.40282347e+38f;
-    for (std::uint32_t i = 0; i < n; ++i) {
-        const float sep = dot(s.norm(i), cLocal - s.vert(i));
-        if (sep > totalRadius) return empty_manifold();
-        if (maxSep < sep) {
-            maxSep = sep;
-            iMax = i;
+    if (n <= kStagedVerts) {
+        // Vertices and normals loaded in one batch (static indices).
+        V2 sv[kStagedVerts], sn[kStagedVerts];
+#pragma unroll
+        for (std::uint32_t i = 0; i < kStagedVerts; ++i) {
+            sv[i] = (i < n) ? s.vert(i) : V2{0, 0};
+            sn[i] = (i < n) ? s.norm(i) : V2{0, 0};
+        }
+        bool apart = false;
+#pragma unroll
+        for (std::uint32_t i = 0; i < kStagedVerts; ++i) {
+            if (i < n && !apart) {
+                const float sep = dot(sn[i], cLocal - sv[i]);
+                if (sep > totalRadius) {
+                    apart = true;
+                } else if (maxSep < sep) {
+                    maxSep = sep;
+                    iMax = i;
+                }
+            }
+        }
+        if (apart) return empty_manifold();
+    } else {
+        for (std::uint32_t i = 0; i < n; ++i) {
+            const float sep = dot(s.norm(i), cLocal - s.vert(i));
+            if (sep > totalRadius) return empty_manifold();
+            if (maxSep < sep) {
+                maxSep = sep;
+                iMax = i;
+            }
         }
     }
     const std::uint32_t iMax2 = mod_next(iMax, n);
@@ -297,7 +327,39 @@ __device__ __forceinline__ SepInfo max_sep_4x4(const Proxy& p1, const Xf& xf1, c
     return out;
 }
 
+__device__ __forceinline__ SepInfo max_sep_staged(const Proxy& p1, const Xf& xf1, const Proxy& p2, const Xf& xf2) {
+    SepInfo out{-__builtin_inff(), kInvalidVertex, kInvalidVertex, kInvalidVertex};
+    const Xf xf = mulT(xf2, xf1);
+    V2 v2[kStagedVerts], v1[kStagedVerts], n1[kStagedVerts];
+#pragma unroll
+    for (std::uint32_t k = 0; k < kStagedVerts; ++k) {
+        v2[k] = (k < p2.n) ? p2.vert(k) : V2{0, 0};
+        v1[k] = (k < p1.n) ? p1.vert(k) : V2{0, 0};
+        n1[k] = (k < p1.n) ? p1.norm(k) : V2{0, 0};
+    }
+#pragma unroll
+    for (std::uint32_t i = 0; i < kStagedVerts; ++i) {
+        if (i < p1.n) {
+            const V2 origin = transform(v1[i], xf);
+            const V2 normal = rotate(n1[i], xf.q);
+            float d = __builtin_inff();
+            std::uint32_t f = kInvalidVertex, s = kInvalidVertex;
+#pragma unroll
+            for (std::uint32_t k = 0; k < kStagedVerts; ++k)
+                if (k < p2.n) min_sep_step(dot(normal, v2[k] - origin), k, d, f, s);
+            if (out.d < d) {
+                out.d = d;
+                out.s0 = f;
+                out.s1 = s;
+                out.first = i;
+            }
+        }
+    }
+    return out;
+}
+
 __device__ __forceinline__ SepInfo max_sep(const Proxy& p1, const Xf& xf1, const Proxy& p2, const Xf& xf2) {
+    if (p1.n <= kStagedVerts && p2.n <= kStagedVerts) return max_sep_staged(p1, xf1, p2, xf2);
     SepInfo out{-__builtin_inff(), kInvalidVertex, kInvalidVertex, kInvalidVertex};
     const Xf xf = mulT(xf2, xf1);
     for (std::uint32_t i = 0; i < p1.n; ++i) {

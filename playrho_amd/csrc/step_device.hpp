@@ -2409,6 +2409,171 @@ __device__ __forceinline__ int find_islands(const DeviceState& d, const WorldCtx
 }
 
 
+// Islands with everything in LDS (ldsAvail bytes): the per-body adjacency
+// (entries packed as contact index << 16 | other body, for worlds under 2^16
+// contacts and bodies), the DFS stack and the flags.  Only the lists of
+// speedable bodies are built: static bodies are never expanded (the DFS only
+// marks them).  Each list is sorted by contact index (the body's contact
+// list order).  Same islands, order and counts as find_islands; returns -1,
+// with nothing written outside LDS, when the world does not fit.
+__device__ __forceinline__ int find_islands_lds(const DeviceState& d, const WorldCtx& x, int nc, const IslandScratch& S,
+                                                int& maxBodies, int ldsAvail) {
+    __shared__ int s_cnt[2];
+    if (x.nb >= 0xffff || nc >= 0xffff) return -1;
+    const int nb = x.nb;
+    // off: nb + 1 ints; stack: nb ints (first the scatter cursors); entries.
+    if (4 * (2 * nb + 1) > ldsAvail) return -1;
+    int* off = reinterpret_cast<int*>(g_smem);
+    int* stack = off + nb + 1;
+    std::uint32_t* ent = reinterpret_cast<std::uint32_t*>(stack + nb);
+    auto listed = [&](int b) { return (d.bflags[x.bo + b] & prx::kSpeedable) != 0; };
+    for (int b = x.lane; b <= nb; b += kWave) off[b] = 0;
+    __syncthreads();
+    for (int i = x.lane; i < nc; i += kWave) {
+        const std::uint32_t f = d.c_flags[x.co + i];
+        if ((f & prx::kCEnabled) && (f & prx::kTouching) && !(f & prx::kSensor)) {
+            const int2 bd = d.c_body[x.co + i];
+            if (listed(bd.x)) atomicAdd(&off[bd.x], 1);
+            if (listed(bd.y)) atomicAdd(&off[bd.y], 1);
+        }
+    }
+    __syncthreads();
+    {
+        int carry = 0;
+        for (int b0 = 0; b0 <= nb; b0 += kWave) {
+            const int b = b0 + x.lane;
+            const int v = (b <= nb) ? off[b] : 0;
+            const int incl = wave_incl_scan(v, x.lane);
+            if (b <= nb) off[b] = carry + incl - v;
+            carry += __shfl(incl, kWave - 1);
+        }
+    }
+    __syncthreads();
+    const int nent = off[nb];
+    std::uint8_t* islb = reinterpret_cast<std::uint8_t*>(ent + nent);
+    std::uint8_t* islc = islb + nb;
+    std::uint8_t* islj = islc + nc;
+    if (4 * (2 * nb + 1 + nent) + nb + nc + x.nj > ldsAvail) return -1;
+    for (int b = x.lane; b < nb; b += kWave) stack[b] = off[b];
+    __syncthreads();
+    for (int i = x.lane; i < nc; i += kWave) {
+        const std::uint32_t f = d.c_flags[x.co + i];
+        if ((f & prx::kCEnabled) && (f & prx::kTouching) && !(f & prx::kSensor)) {
+            const int2 bd = d.c_body[x.co + i];
+            if (listed(bd.x)) ent[atomicAdd(&stack[bd.x], 1)] = (std::uint32_t(i) << 16) | std::uint32_t(bd.y);
+            if (listed(bd.y)) ent[atomicAdd(&stack[bd.y], 1)] = (std::uint32_t(i) << 16) | std::uint32_t(bd.x);
+        }
+    }
+    __syncthreads();
+    for (int b = x.lane; b < nb; b += kWave) {
+        const int s0 = off[b], e0 = off[b + 1];
+        for (int i = s0 + 1; i < e0; ++i) {
+            const std::uint32_t v = ent[i];
+            int j = i - 1;
+            while (j >= s0 && ent[j] > v) {
+                ent[j + 1] = ent[j];
+                --j;
+            }
+            ent[j + 1] = v;
+        }
+        islb[b] = listed(b) ? 0 : 2;
+        S.bisland[b] = -1;
+    }
+    for (int i = x.lane; i < nc; i += kWave) islc[i] = 0;
+    for (int i = x.lane; i < x.nj; i += kWave) islj[i] = 0;
+    __syncthreads();
+    // One-body islands.
+    int nsingle = 0;
+    for (int base = 0; base < nb; base += kWave) {
+        const int b = base + x.lane;
+        bool single = false;
+        if (b < nb) {
+            const std::uint32_t f = d.bflags[x.bo + b];
+            single = (f & prx::kAwake) && (f & prx::kEnabled) && (f & prx::kSpeedable) && off[b] == off[b + 1] &&
+                     (x.nj == 0 || d.bjoint[x.bo + b].y == 0);
+        }
+        const unsigned long long m = __ballot(single);
+        if (single) {
+            const int r = nsingle + __popcll(m & lanes_below(x.lane));
+            S.ibody[r] = b;
+            S.istart[r] = int2{r, 0};
+            S.jstart[r] = 0;
+            S.bisland[b] = r;
+            islb[b] = 1;
+        }
+        nsingle += __popcll(m);
+    }
+    __syncthreads();
+    if (x.lane == 0) {
+        int nisl = nsingle, nbe = nsingle, nce = 0, nje = 0;
+        // Reached static bodies are marked 3 while counted and listed from
+        // the top of the stack array (AabbTreeWorld.cpp:1386-1398).
+        int maxB = nsingle > 0 ? 1 : 0;
+        for (int seed = 0; seed < nb; ++seed) {
+            if (islb[seed]) continue;
+            const std::uint32_t sf = d.bflags[x.bo + seed];
+            if (!((sf & prx::kAwake) && (sf & prx::kEnabled))) continue;
+            S.istart[nisl] = int2{nbe, nce};
+            S.jstart[nisl] = nje;
+            const int nbe0 = nbe;
+            int nst = 0;
+            int sp = 0;
+            stack[sp++] = seed;
+            islb[seed] = 1;
+            while (sp > 0) {
+                const int bb = stack[--sp];
+                S.ibody[nbe++] = bb;
+                S.bisland[bb] = nisl;
+                const int e1 = off[bb + 1];
+                for (int e = off[bb]; e < e1; ++e) {
+                    const std::uint32_t en = ent[e];
+                    const int ci = int(en >> 16), ob = int(en & 0xffffu);
+                    if (islc[ci]) continue;
+                    S.icont[nce++] = ci;
+                    islc[ci] = 1;
+                    if (islb[ob] == 0) {
+                        islb[ob] = 1;
+                        stack[sp++] = ob;
+                    } else if (islb[ob] == 2) {
+                        islb[ob] = 3;
+                        stack[nb - 1 - nst++] = ob;
+                    }
+                }
+                // AddJointsToIsland (AabbTreeWorld.cpp:1446-1470).
+                if (x.nj) {
+                    const int2 bj = d.bjoint[x.bo + bb];
+                    for (int k = 0; k < bj.y; ++k) {
+                        const int2 je = d.bjoint_list[2 * x.jo + bj.x + k];
+                        if (islj[je.x]) continue;
+                        if (!(d.bflags[x.bo + je.y] & prx::kEnabled)) continue;
+                        islj[je.x] = 1;
+                        S.ijoint[nje++] = je.x;
+                        if (islb[je.y] == 0) {
+                            islb[je.y] = 1;
+                            stack[sp++] = je.y;
+                        } else if (islb[je.y] == 2) {
+                            islb[je.y] = 3;
+                            stack[nb - 1 - nst++] = je.y;
+                        }
+                    }
+                }
+            }
+            for (int q = 0; q < nst; ++q) islb[stack[nb - 1 - q]] = 2;
+            maxB = (nbe - nbe0 + nst > maxB) ? nbe - nbe0 + nst : maxB;
+            ++nisl;
+        }
+        S.istart[nisl] = int2{nbe, nce};
+        S.jstart[nisl] = nje;
+        s_cnt[0] = nisl;
+        s_cnt[1] = maxB;
+    }
+    __syncthreads();
+    maxBodies = s_cnt[1];
+    const int n = s_cnt[0];
+    __syncthreads();
+    return n;
+}
+
 // Position-constraint record (see the layout above).
 struct Pc {
     Manifold m;
@@ -3691,14 +3856,17 @@ __device__ __forceinline__ void step_world(const DeviceState& d, int w, const St
         S.bisland = d.s_stack + 2 * x.bo + x.nb;
         S.isl_word = d.s_keys + x.co;
         S.isl_under = d.s_keys2 + x.co;
-        build_adjacency(d, x, k.nc, S);
+        int maxBodies = 0;
+        int nisl = find_islands_lds(d, x, k.nc, S, maxBodies, int(c.ldsBytes));
         T.mark(prx::kPhAdjacency);
-        const int nent = S.off[x.nb];
-        const int ldsAdj = 4 * (((x.nb + 2) >> 1) * 2) + 8 * nent + x.nb + k.nc + x.nj;
-        int nisl, maxBodies;
-        if (ldsAdj <= int(c.ldsBytes)) nisl = find_islands<2>(d, x, k.nc, S, maxBodies);
-        else if (x.nb + k.nc + x.nj <= int(c.ldsBytes)) nisl = find_islands<1>(d, x, k.nc, S, maxBodies);
-        else nisl = find_islands<0>(d, x, k.nc, S, maxBodies);
+        if (nisl < 0) {
+            build_adjacency(d, x, k.nc, S);
+            const int nent = S.off[x.nb];
+            const int ldsAdj = 4 * (((x.nb + 2) >> 1) * 2) + 8 * nent + x.nb + k.nc + x.nj;
+            if (ldsAdj <= int(c.ldsBytes)) nisl = find_islands<2>(d, x, k.nc, S, maxBodies);
+            else if (x.nb + k.nc + x.nj <= int(c.ldsBytes)) nisl = find_islands<1>(d, x, k.nc, S, maxBodies);
+            else nisl = find_islands<0>(d, x, k.nc, S, maxBodies);
+        }
         stat(prx::kStRegIslandsFound, std::uint32_t(nisl));
         stat(prx::kStRegMaxIslandBodies, std::uint32_t(maxBodies));
         T.mark(prx::kPhIslands);
