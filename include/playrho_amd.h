@@ -290,6 +290,10 @@ int prx_world_get_bodies(prx_world* world, float* out, uint32_t* flags);
 int prx_world_get_transformation(prx_world* world, uint32_t body, float out[4]);
 /* SetVelocity(World&, BodyID, Velocity) (WorldBody.hpp). */
 int prx_world_set_velocity(prx_world* world, uint32_t body, const float v[3]);
+/* SetAcceleration(World&, BodyID, LinearAcceleration2, AngularAcceleration)
+ * (d2/WorldBody.hpp; Body::SetAcceleration, Body.cpp:203-238): a = linear x,
+ * y, angular. */
+int prx_world_set_acceleration(prx_world* world, uint32_t body, const float a[3]);
 /* SetTransformation(World&, BodyID, Transformation) (WorldBody.hpp;
  * Body.cpp:270-273 through SetBody, AabbTreeWorld.cpp:2723-2795): xf = p.x,
  * p.y, q.x, q.y.  The body's sweep becomes the single position (p, the angle
@@ -318,6 +322,38 @@ int prx_shape_set_restitution(prx_world* world, uint32_t shape, float restitutio
  * the golden vectors. */
 int prx_world_state_record(prx_world* world, uint8_t* buf, size_t cap, size_t* len);
 uint32_t prx_world_contact_count(prx_world* world);
+
+/* Getters for a world model over this engine (GetBody's mass data,
+ * GetContacts / GetContact / GetManifold, GetTree's leaves).
+ *
+ * A body's mass data as the world holds it (ResetMassData, WorldBody.cpp:
+ * 360-399): out = inverse mass, inverse rotational inertia, local centre x, y. */
+int prx_world_get_body_mass(prx_world* world, uint32_t body, float out[4]);
+/* One entry of GetContacts(World&) with its Contact and Manifold values
+ * (d2/WorldContact.hpp; Contact.hpp, Manifold.hpp).  key_min / key_max: the
+ * ContactKey's proxy (tree leaf) ids; flags: bit 0 touching, 1 enabled,
+ * 2 sensor, 3 impenetrable, 4 needs updating. */
+typedef struct prx_contact_info {
+    uint32_t key_min, key_max, id, flags;
+    uint32_t body_a, body_b, shape_a, shape_b;
+    float friction, restitution;
+    uint32_t manifold_type, point_count;
+    float local_normal[2], local_point[2];
+    struct {
+        float local_point[2];
+        uint32_t contact_feature;
+        float normal_impulse, tangent_impulse;
+    } points[2];
+} prx_contact_info;
+/* The world's contacts in its contact-list order: up to cap into out (null
+ * when cap is 0); *count receives the number of contacts. */
+int prx_world_get_contacts(prx_world* world, prx_contact_info* out, size_t cap, size_t* count);
+/* The world's proxies (DynamicTree leaves: a body's shape, AabbTreeWorld.cpp:
+ * 557-579) in creation order: leaf id, body, shape and fat AABB (lo x, lo y,
+ * hi x, hi y); up to cap (arrays may be null when cap is 0), *count receives
+ * the number of proxies. */
+int prx_world_get_proxies(prx_world* world, uint32_t* ids, uint32_t* bodies, uint32_t* shapes, float* aabbs,
+                          size_t cap, size_t* count);
 
 /* Contact listeners.
  * A contact's manifold as a pre-solve listener receives it: the manifold the

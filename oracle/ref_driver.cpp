@@ -676,19 +676,40 @@ int main(int argc, char** argv) {
 
     const auto conf = RegularStepConf(doToi);
 
+    // PRX_DUMP_STEPS="a,b,...": write only the initial state and those steps
+    // (large worlds); otherwise every step.
+    std::vector<int> only;
+    if (const char* env = std::getenv("PRX_DUMP_STEPS")) {
+        std::stringstream in(env);
+        std::string item;
+        while (std::getline(in, item, ',')) only.push_back(std::atoi(item.c_str()));
+    }
+    auto wanted = [&](int s) {
+        if (only.empty()) return true;
+        for (int k : only)
+            if (k == s) return true;
+        return false;
+    };
+    std::uint32_t nstates = 1;
+    for (int s = 1; s <= steps; ++s) nstates += wanted(s) ? 1u : 0u;
     Out o;
     o.put("PRXD", 4);
     o.u32(3);
     WriteDescription(o, world);
     WriteConf(o, conf);
-    o.u32(std::uint32_t(steps + 1));
+    o.u32(nstates);
     WriteState(o, world, 0, StepStats{});
     const auto edits = ReadEdits();
     for (int s = 1; s <= steps; ++s) {
         for (const auto& e : edits)
             if (e.step == s) ApplyEdit(world, e);
         const auto st = Step(world, conf);
-        WriteState(o, world, std::uint32_t(s), st);
+        if (wanted(s)) {
+            WriteState(o, world, std::uint32_t(s), st);
+        } else {
+            g_events.o.buf.clear();  // the listener calls of an unwritten step
+            g_events.n = 0;
+        }
     }
     FILE* f = std::fopen(argv[4], "wb");
     if (!f) {

@@ -2021,6 +2021,34 @@ int prx_world_set_velocity(prx_world* w, std::uint32_t body, const float v[3]) {
     return PRX_OK;
 }
 
+// SetAcceleration(World&, BodyID, ...) through Body::SetAcceleration
+// (Body.cpp:203-238): no change, nothing; a body that is not accelerable
+// takes zero only; a larger linear or angular acceleration, a new direction
+// or a new sign of the angular one wakes the body (under-active time reset).
+int prx_world_set_acceleration(prx_world* w, std::uint32_t body, const float a[3]) {
+    if (int st = CheckWorld(w)) return st;
+    if (!a) return Fail(PRX_E_INVALID_ARGUMENT, "null acceleration");
+    if (body >= w->bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
+    if (int st = Detach(w)) return st;
+    HostBody& b = w->bodies[body];
+    if (b.acc[0] == a[0] && b.acc[1] == a[1] && b.acc[2] == a[2]) return PRX_OK;
+    if (!(b.flags & prx::kAccelerable)) {
+        if (a[0] != 0.0f || a[1] != 0.0f || a[2] != 0.0f) return PRX_OK;
+    } else {
+        const float m0 = b.acc[0] * b.acc[0] + b.acc[1] * b.acc[1];
+        const float m1 = a[0] * a[0] + a[1] * a[1];
+        if (b.acc[2] < a[2] || m0 < m1 || std::atan2(b.acc[1], b.acc[0]) != std::atan2(a[1], a[0]) ||
+            std::signbit(b.acc[2]) != std::signbit(a[2])) {
+            b.flags |= prx::kAwake;
+            b.under = 0;
+        }
+    }
+    b.acc[0] = a[0];
+    b.acc[1] = a[1];
+    b.acc[2] = a[2];
+    return PRX_OK;
+}
+
 namespace {
 
 // SetShape (AabbTreeWorld.cpp:1284-1357) for a change of friction or
@@ -2142,6 +2170,84 @@ int prx_world_contact_events(prx_world* w, std::uint32_t* words, std::size_t cap
         if (cap < w->events.size()) return Fail(PRX_E_LENGTH, "buffer too small");
         if (!w->events.empty()) std::memcpy(words, w->events.data(), w->events.size() * 4);
     }
+    return PRX_OK;
+}
+
+int prx_world_get_body_mass(prx_world* w, std::uint32_t body, float out[4]) {
+    if (int st = CheckWorld(w)) return st;
+    if (!out) return Fail(PRX_E_INVALID_ARGUMENT, "null output");
+    if (body >= w->bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
+    const HostBody& b = w->bodies[body];
+    out[0] = b.invMass;
+    out[1] = b.invRotI;
+    out[2] = b.lc.x;
+    out[3] = b.lc.y;
+    return PRX_OK;
+}
+
+int prx_world_get_contacts(prx_world* w, prx_contact_info* out, std::size_t cap, std::size_t* count) {
+    if (int st = CheckWorld(w)) return st;
+    if (!count || (cap > 0 && !out)) return Fail(PRX_E_INVALID_ARGUMENT, "null output");
+    if (int st = Refresh(w)) return st;
+    const HostContacts& hc = w->contacts;
+    *count = hc.size();
+    for (std::size_t i = 0; i < hc.size() && i < cap; ++i) {
+        prx_contact_info c{};
+        c.key_min = w->proxies[std::size_t(hc.prox[i].x)].id;
+        c.key_max = w->proxies[std::size_t(hc.prox[i].y)].id;
+        c.id = hc.id[i];
+        c.flags = hc.flags[i];
+        c.body_a = std::uint32_t(hc.body[i].x);
+        c.body_b = std::uint32_t(hc.body[i].y);
+        c.shape_a = std::uint32_t(hc.shape[i].x);
+        c.shape_b = std::uint32_t(hc.shape[i].y);
+        c.friction = hc.mat[i].x;
+        c.restitution = hc.mat[i].y;
+        const uint4 h = hc.hdr[i];
+        c.manifold_type = h.x & 0xffu;
+        c.point_count = (h.x >> 8) & 0xffu;
+        if (c.manifold_type == 2 || c.manifold_type == 3) {
+            c.local_normal[0] = hc.geo[i].x;
+            c.local_normal[1] = hc.geo[i].y;
+        }
+        if (c.manifold_type != 0) {
+            c.local_point[0] = hc.geo[i].z;
+            c.local_point[1] = hc.geo[i].w;
+        }
+        const float4 pts[2] = {hc.p0[i], hc.p1[i]};
+        const std::uint32_t cfs[2] = {h.y, h.z};
+        for (std::uint32_t k = 0; k < 2; ++k) {
+            c.points[k].contact_feature = cfs[k];
+            if (k < c.point_count) {
+                c.points[k].local_point[0] = pts[k].x;
+                c.points[k].local_point[1] = pts[k].y;
+                c.points[k].normal_impulse = pts[k].z;
+                c.points[k].tangent_impulse = pts[k].w;
+            }
+        }
+        out[i] = c;
+    }
+    return PRX_OK;
+}
+
+int prx_world_get_proxies(prx_world* w, std::uint32_t* ids, std::uint32_t* bodies, std::uint32_t* shapes,
+                          float* aabbs, std::size_t cap, std::size_t* count) {
+    if (int st = CheckWorld(w)) return st;
+    if (!count || (cap > 0 && (!ids || !bodies || !shapes || !aabbs)))
+        return Fail(PRX_E_INVALID_ARGUMENT, "null output");
+    if (int st = Refresh(w)) return st;
+    std::size_t k = 0;
+    for (const auto& p : w->proxies) {
+        if (p.flags & prx::kProxyPending) continue;
+        if (k < cap) {
+            ids[k] = p.id;
+            bodies[k] = std::uint32_t(p.body);
+            shapes[k] = std::uint32_t(p.shape);
+            for (int j = 0; j < 4; ++j) aabbs[4 * k + j] = p.aabb[j];
+        }
+        ++k;
+    }
+    *count = k;
     return PRX_OK;
 }
 

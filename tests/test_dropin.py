@@ -1,0 +1,54 @@
+"""The literal drop-in: the reference's own API (CreateShape, CreateBody,
+CreateJoint, Step, GetTransformation, GetContacts, GetManifold, SetVelocity,
+SetTransformation, contact listeners) on playrho::d2::World{GpuWorld{}}
+(playrho_amd/dropin/GpuWorld.cpp, a model of detail::WorldConcept over the C
+ABI) beside the reference's own World.  The check program
+(tests/dropin/dropin_check.cpp) is compiled and linked against the reference
+library by oracle/Makefile.ref where the reference checkout is present; the
+tests skip where that binary was not built."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CHECK = os.path.join(ROOT, "oracle", "_ref", "dropin_check")
+EXPECTED = os.path.join(ROOT, "tests", "golden", "hello_expected.txt")
+
+needs_check = pytest.mark.skipif(not os.path.exists(CHECK), reason="oracle/_ref/dropin_check not built here")
+
+
+@needs_check
+def test_dropin_world_construction_equals_reference():
+    """World{GpuWorld{}} built through the reference API: every body's
+    transformation, velocity and mass data, and every shape, equal the
+    reference World's (no device needed)."""
+    r = subprocess.run([CHECK, "cpu"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "cpu: OK" in r.stdout
+
+
+@needs_check
+@pytest.mark.gpu
+def test_dropin_hello_world_output():
+    """HelloWorld's program on World{GpuWorld{}} prints the reference's
+    HelloWorld/ExpectedOutput.txt (kept as tests/golden/hello_expected.txt)."""
+    print("dropin hello", flush=True)
+    r = subprocess.run([CHECK, "hello"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    with open(EXPECTED) as f:
+        want = f.read().splitlines()
+    assert r.stdout.splitlines() == want
+
+
+@needs_check
+@pytest.mark.gpu
+def test_dropin_pyramid_steps_equal_reference():
+    """150 steps of the pyramid through Step(World&, const StepConf&) on both
+    worlds, with SetVelocity and SetTransformation edits: transformations,
+    velocities, contact keys, manifolds and begin/end listener calls equal the
+    reference bit for bit after every step."""
+    print("dropin gpu", flush=True)
+    r = subprocess.run([CHECK, "gpu"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr[-3000:]
+    assert "gpu: OK" in r.stdout
