@@ -465,7 +465,12 @@ int prx_batch_autotune(prx_batch* batch, const prx_step_conf* step, int nsteps,
 typedef struct prx_big prx_big;
 int prx_big_create(prx_world* world, prx_big** out);
 void prx_big_destroy(prx_big* big);
-/* At most 16 velocity and 16 position iterations (PRX_E_UNSUPPORTED otherwise). */
+/* At most 16 velocity and 16 position iterations (PRX_E_UNSUPPORTED otherwise).
+ * With doToi, a reduced TOI phase (AabbTreeWorld.cpp:1947-2063 restricted to
+ * static shapes): up to two time-of-impact events per awake dynamic body and
+ * step against static shapes, each solved as the island of the body and the
+ * static body at toiResolutionRate with toiPositionIters / toiVelocityIters,
+ * then the rest of the step's motion. */
 int prx_big_step(prx_big* big, const prx_step_conf* conf, int nsteps);
 int prx_big_sync(prx_big* big);
 int prx_big_last_timing(prx_big* big, float* ms, int* launches);

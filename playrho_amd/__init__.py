@@ -68,6 +68,18 @@ class Manifold(ctypes.Structure):
                 ("points", ManifoldPoint * 2)]
 
 
+class ContactInfo(ctypes.Structure):
+    """One entry of GetContacts with its Contact and Manifold values
+    (d2/WorldContact.hpp; Contact.hpp, Manifold.hpp).  flags: bit 0 touching,
+    1 enabled, 2 sensor, 3 impenetrable, 4 needs updating."""
+    _fields_ = [("key_min", ctypes.c_uint32), ("key_max", ctypes.c_uint32), ("id", ctypes.c_uint32),
+                ("flags", ctypes.c_uint32), ("body_a", ctypes.c_uint32), ("body_b", ctypes.c_uint32),
+                ("shape_a", ctypes.c_uint32), ("shape_b", ctypes.c_uint32), ("friction", ctypes.c_float),
+                ("restitution", ctypes.c_float), ("manifold_type", ctypes.c_uint32),
+                ("point_count", ctypes.c_uint32), ("local_normal", ctypes.c_float * 2),
+                ("local_point", ctypes.c_float * 2), ("points", ManifoldPoint * 2)]
+
+
 class ContactImpulses(ctypes.Structure):
     """ContactImpulsesList (ContactImpulsesList.hpp:47-100)."""
     _fields_ = [("count", ctypes.c_uint32), ("normal", ctypes.c_float * 2), ("tangent", ctypes.c_float * 2)]
@@ -312,6 +324,13 @@ _SIGS = {
     "prx_world_state_record": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
                                               ctypes.POINTER(ctypes.c_size_t)]),
     "prx_world_contact_count": (ctypes.c_uint32, [ctypes.c_void_p]),
+    "prx_world_get_body_mass": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float)]),
+    "prx_world_get_contacts": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                              ctypes.POINTER(ctypes.c_size_t)]),
+    "prx_world_get_proxies": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "prx_world_set_acceleration": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32,
+                                                  ctypes.POINTER(ctypes.c_float)]),
     "prx_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_void_p,
                                         ctypes.POINTER(ctypes.c_void_p)]),
     "prx_batch_destroy": (None, [ctypes.c_void_p]),
@@ -481,6 +500,39 @@ class World:
     def set_velocity(self, body, v):
         arr = (ctypes.c_float * 3)(*v)
         _check(lib().prx_world_set_velocity(self.h, body, arr))
+
+    def set_acceleration(self, body, a):
+        """SetAcceleration (Body.cpp:203-238): a = (linear x, y, angular)."""
+        arr = (ctypes.c_float * 3)(*a)
+        _check(lib().prx_world_set_acceleration(self.h, body, arr))
+
+    def body_mass(self, body):
+        """(inverse mass, inverse rotational inertia, local centre x, y)
+        (ResetMassData, WorldBody.cpp:360-399)."""
+        out = (ctypes.c_float * 4)()
+        _check(lib().prx_world_get_body_mass(self.h, body, out))
+        return tuple(out)
+
+    def contacts(self):
+        """GetContacts in the world's contact-list order, as ContactInfo."""
+        n = ctypes.c_size_t()
+        _check(lib().prx_world_get_contacts(self.h, None, 0, ctypes.byref(n)))
+        out = (ContactInfo * max(n.value, 1))()
+        _check(lib().prx_world_get_contacts(self.h, out, n.value, ctypes.byref(n)))
+        return list(out[:n.value])
+
+    def proxies(self):
+        """The tree's leaves in creation order: (ids, bodies, shapes, fat AABBs)."""
+        n = ctypes.c_size_t()
+        _check(lib().prx_world_get_proxies(self.h, None, None, None, None, 0, ctypes.byref(n)))
+        k = n.value
+        ids = np.zeros(k, dtype=np.uint32)
+        bodies = np.zeros(k, dtype=np.uint32)
+        shapes = np.zeros(k, dtype=np.uint32)
+        aabbs = np.zeros((k, 4), dtype=np.float32)
+        _check(lib().prx_world_get_proxies(self.h, ids.ctypes.data, bodies.ctypes.data, shapes.ctypes.data,
+                                           aabbs.ctypes.data, k, ctypes.byref(n)))
+        return ids, bodies, shapes, aabbs
 
     def set_transformation(self, body, xf):
         """SetTransformation (Body.cpp:270-273): xf = (p.x, p.y, q.x, q.y)."""
@@ -666,7 +718,7 @@ class BigWorld:
             self.h = None
 
     def step(self, conf=None, nsteps=1):
-        conf = conf or StepConf(do_toi=0)  # no TOI phase in the coloured mode
+        conf = conf or StepConf()
         _check(lib().prx_big_step(self.h, ctypes.byref(conf), int(nsteps)))
 
     def sync(self):

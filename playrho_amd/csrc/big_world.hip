@@ -37,6 +37,7 @@ constexpr int kBlock = prx::kBigBlock;  // threads of the single-workgroup kerne
 constexpr int kGrid = 256;              // threads per workgroup of the grid-wide kernels
 constexpr int kSweepBlocks = 64;        // workgroups of a colour sweep launch (16,384 threads)
 constexpr std::uint64_t kEmptyKey = ~0ull;
+constexpr int kBigToiPasses = 2;  // time-of-impact passes per step against static shapes
 // Bound on tree depth for the parent walks (a Karras tree over 30-bit codes
 // with index tie-breaks is at most 30 + 32 deep).
 constexpr int kMaxDepth = 96;
@@ -1003,6 +1004,128 @@ __global__ void k_big_clear_bodyflag(DeviceState d) {
     if (b < d.hdr[0].nbodies) d.bflags[b] &= ~0x100u;
 }
 
+// Coloured mode with doToi: continuous collision of dynamic bodies against
+// static shapes, after the regular phase.  Every enabled non-sensor contact
+// between an awake dynamic body and a static body gets its time of impact
+// over the step's sweep (GetToiViaSat, TimeOfImpact.cpp:35-251, as in the
+// per-world TOI phase); each body keeps its earliest (ties: the lower contact
+// index).  That body is advanced to its pose at the time of impact (Body::
+// Advance, Body.hpp:1004-1009), the contact's velocity constraint is built
+// and solved there against the static body with the TOI velocity iterations
+// (VelocityConstraint, SolveVelocityConstraint with restitution and friction,
+// as SolveToiViaGS does for the contact, AabbTreeWorld.cpp:1991-2057) after
+// its position iterations at the TOI resolution rate, and the body is
+// integrated over the rest of the step with the new velocity.  Its
+// proxies and contacts are then synchronised and updated.  A second pass
+// checks the sweep from the solved pose to the new end pose.  This is a
+// reduced form of the reference's TOI phase: at most two events per body and
+// step, the island is the body and the static body of its earliest contact,
+// and contacts between dynamic bodies get no time of impact.
+__global__ void k_big_toi_static(DeviceState d, BigState g, StepParams c) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= d.hdr[0].ncontacts) return;
+    const std::uint32_t f = d.c_flags[i];
+    if (!(f & prx::kCEnabled) || (f & prx::kSensor)) return;
+    const int2 bd = d.c_body[i];
+    const std::uint32_t fa = d.bflags[bd.x], fb = d.bflags[bd.y];
+    const bool aDyn = (fa & prx::kAccelerable) && !(fb & prx::kSpeedable);
+    const bool bDyn = (fb & prx::kAccelerable) && !(fa & prx::kSpeedable);
+    if (!aDyn && !bDyn) return;
+    const int dyn = aDyn ? bd.x : bd.y;
+    if (!(d.bflags[dyn] & prx::kAwake)) return;
+    const SweepD sA = normalized(sweep_of(d, bd.x));
+    const SweepD sB = normalized(sweep_of(d, bd.y));
+    const int2 sh = d.c_shape[i];
+    const ToiRes r = toi_via_sat(shape_proxy(d, sh.x), sA, shape_proxy(d, sh.y), sB, c);
+    if (r.time >= 0.0f && r.time < 1.0f)
+        atomicMin(&g.toiKey[dyn], (static_cast<unsigned long long>(__float_as_uint(r.time)) << 32) |
+                                      static_cast<unsigned long long>(std::uint32_t(i)));
+}
+
+__global__ void k_big_toi_reset(DeviceState d, BigState g) {
+    const int b = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (b < d.hdr[0].nbodies) g.toiLeft[b] = 1.0f;
+}
+
+__global__ void k_big_toi_apply(DeviceState d, BigState g, StepParams c) {
+    const int b = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (b >= d.hdr[0].nbodies) return;
+    const unsigned long long key = g.toiKey[b];
+    if (key == ~0ull) return;
+    g.toiKey[b] = ~0ull;
+    const float t = __uint_as_float(std::uint32_t(key >> 32));  // a fraction of the sweep's time
+    const int i = int(std::uint32_t(key & 0xffffffffull));
+    const int2 bd = d.c_body[i];
+    const int2 ss = d.c_shape[i];
+    const int o = (bd.x == b) ? bd.y : bd.x;  // the static body
+    float4 p = lerp_pos(d.pos0[b], d.pos1[b], t);
+    const BodyC Bd = load_bodyc(d, b), Bo = load_bodyc(d, o);
+    const float4 po = d.pos1[o];
+    const Xf xfD = get_xf(p.x, p.y, p.z, Bd.lc);
+    const Xf xfO = get_xf(po.x, po.y, po.z, Bo.lc);
+    const bool dynIsA = (bd.x == b);
+    const Xf& xfA = dynIsA ? xfD : xfO;
+    const Xf& xfB = dynIsA ? xfO : xfD;
+    Manifold m = collide_shapes(shape_proxy(d, ss.x), xfA, shape_proxy(d, ss.y), xfB, c.linearSlop, c.maxCirclesRatio);
+    float4 v = d.vel[b];
+    if (m.n > 0) {
+        const float rA = d.shape_f[ss.x].x, rB = d.shape_f[ss.y].x;
+        const BodyC& A = dynIsA ? Bd : Bo;
+        const BodyC& B = dynIsA ? Bo : Bd;
+        // Position iterations at the TOI resolution rate until the TOI minimum
+        // separation (the TOI island's position solve, SolveToiViaGS).
+        StepParams ps = c;
+        ps.regResolutionRate = c.toiResolutionRate;
+        float4 qo = po;
+        for (std::uint32_t it = 0; it < c.toiPosIters; ++it) {
+            float4& qa = dynIsA ? p : qo;
+            float4& qb = dynIsA ? qo : p;
+            const float sep = solve_pc(m, rA + rB, A, B, qa, qb, ps);
+            if (sep >= c.toiMinSeparation) break;
+        }
+        // Velocity constraint at the solved pose, no warm start, TOI iterations.
+        const float4 pA = dynIsA ? p : po, pB = dynIsA ? po : p;
+        const Xf xA = get_xf(pA.x, pA.y, pA.z, A.lc), xB = get_xf(pB.x, pB.y, pB.z, B.lc);
+        const WM wm = world_manifold(m, xA, rA, xB, rB);
+        const float4 mt = d.c_mat[i];
+        const float4 vo{0.0f, 0.0f, 0.0f, 0.0f};
+        float4 vA = dynIsA ? v : vo, vB = dynIsA ? vo : v;
+        StepParams vconf = c;
+        vconf.warmStart = 0u;
+        Vc vc = make_vc(wm, mt.x, mt.y, mt.z, bd.x, bd.y, A, pA, vA, B, pB, vB, vconf);
+        for (std::uint32_t it = 0; it < c.toiVelIters; ++it) {
+            float inc = solve_dir(vc, true, A, B, vA, vB);
+            if (vc.n == 1 || (vc.K00 == 0.0f && vc.K11 == 0.0f && vc.K01 == 0.0f))
+                inc = fmax_std(inc, solve_dir(vc, false, A, B, vA, vB));
+            else
+                inc = fmax_std(inc, block_solve(vc, A, B, vA, vB));
+            if (inc <= c.toiMinMomentum) break;
+        }
+        const float4 vn = dynIsA ? vA : vB;
+        v.x = vn.x;
+        v.y = vn.y;
+        v.z = vn.z;
+        d.vel[b] = v;
+    }
+    // The rest of the step with the new velocity; the sweep from the solved
+    // pose to the new end pose is what a further pass of this step checks.
+    const float left = g.toiLeft[b] * (1.0f - t);
+    g.toiLeft[b] = left;
+    const float h = left * c.dt;
+    d.pos0[b] = p;
+    p.x = p.x + h * v.x;
+    p.y = p.y + h * v.y;
+    p.z = p.z + h * v.z;
+    d.pos1[b] = p;
+    const Xf xf = get_xf(p.x, p.y, p.z, Bd.lc);
+    d.xf[b] = float4{xf.p.x, xf.p.y, xf.q.x, xf.q.y};
+    d.bflags[b] |= 0x100u;  // contacts to update
+    const WorldCtx x = big_ctx(d);
+    const std::uint32_t moved = synchronize_body(d, x, b, c);
+    atomicAdd(&d.hdr[0].stats[prx::kStToiIslandsSolved], 1u);
+    if (moved) atomicAdd(&d.hdr[0].stats[prx::kStToiProxiesMoved], moved);
+}
+
 // Smallest separation over touching contacts at the current poses (a
 // diagnostic; separation below zero is penetration).
 __global__ void k_big_metrics(DeviceState d, BigState g) {
@@ -1099,6 +1222,20 @@ extern "C" hipError_t prx_big_launch_step(const prx::DeviceState* dp, const prx:
         hipLaunchKernelGGL(k_big_clear_bodyflag, dim3(Blocks(B, kGrid)), dim3(kGrid), 0, s, d);
         hipLaunchKernelGGL(k_big_update, dim3(Blocks(cap, kGrid)), dim3(kGrid), 0, s, d, *c, -1);
         FindAdd(d, g, *c, prx::kStRegAdded, s);
+        if (c->doToi) {
+            // Two passes: a body's second event in a step (another static
+            // shape reached while moving on) is found from the sweep the
+            // first left.
+            hipLaunchKernelGGL(k_big_toi_reset, dim3(Blocks(B, kGrid)), dim3(kGrid), 0, s, d, g);
+            for (int pass = 0; pass < kBigToiPasses; ++pass) {
+                hipLaunchKernelGGL(k_big_toi_static, dim3(Blocks(cap, kGrid)), dim3(kGrid), 0, s, d, g, *c);
+                hipLaunchKernelGGL(k_big_toi_apply, dim3(Blocks(B, kGrid)), dim3(kGrid), 0, s, d, g, *c);
+                hipLaunchKernelGGL(k_big_flag_contacts, dim3(Blocks(cap, kGrid)), dim3(kGrid), 0, s, d);
+                hipLaunchKernelGGL(k_big_clear_bodyflag, dim3(Blocks(B, kGrid)), dim3(kGrid), 0, s, d);
+                hipLaunchKernelGGL(k_big_update, dim3(Blocks(cap, kGrid)), dim3(kGrid), 0, s, d, *c, -1);
+                FindAdd(d, g, *c, prx::kStToiAdded, s);
+            }
+        }
     }
     return hipGetLastError();
 }

@@ -59,6 +59,10 @@ struct BigState {
     int* tent;        // cap: proposed colour
     int* win;         // cap
     int* colorStart;  // kBigMaxColors + 2
+    // B: the earliest time of impact against a static shape, (float bits << 32
+    // | contact index), ~0 for none.
+    unsigned long long* toiKey;
+    float* toiLeft;  // B: the part of the step a body has left to move (1 before its first event)
     int* sweep;       // kBigSweepSlots
     int* counters;    // kBigCounters
     float* metrics;   // kBigMetrics

@@ -2541,6 +2541,9 @@ int AllocBig(DeviceArena& a, prx::BigState& g, int P, int B, int cap) {
     PRX_HIP(a.alloc(g.tent, std::size_t(g.cap)));
     PRX_HIP(a.alloc(g.win, std::size_t(g.cap)));
     PRX_HIP(a.alloc(g.colorStart, std::size_t(prx::kBigMaxColors + 2)));
+    PRX_HIP(a.alloc(g.toiKey, std::size_t(g.B)));
+    PRX_HIP(hipMemset(g.toiKey, 0xff, sizeof(unsigned long long) * std::size_t(g.B)));
+    PRX_HIP(a.alloc(g.toiLeft, std::size_t(g.B)));
     PRX_HIP(a.alloc(g.sweep, std::size_t(prx::kBigSweepSlots)));
     PRX_HIP(a.alloc(g.counters, std::size_t(prx::kBigCounters)));
     PRX_HIP(a.alloc(g.metrics, std::size_t(prx::kBigMetrics)));
@@ -2601,7 +2604,6 @@ void prx_big_destroy(prx_big* big) {
 
 int prx_big_step(prx_big* big, const prx_step_conf* conf, int nsteps) {
     if (!big || !conf) return Fail(PRX_E_INVALID_ARGUMENT, "null argument");
-    if (conf->do_toi) return Fail(PRX_E_UNSUPPORTED, "the coloured large-world mode has no TOI phase (doToi = true)");
     if (nsteps < 0) return Fail(PRX_E_INVALID_ARGUMENT, "negative step count");
     prx_batch* b = big->batch;
     if (b->worlds[0] != big->world || big->world->owner != b)

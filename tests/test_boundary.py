@@ -118,6 +118,32 @@ def test_step_conf_defaults_match_reference():
     assert [getattr(c, f) for f in dumpfmt.TOI_ITERS] == list(ref["toi_iters"])
 
 
+def test_mass_and_acceleration_host_side():
+    """GetMassData of a disk body against its closed form (mass = density
+    pi r^2, I = m r^2 / 2 about the centre, WorldBody.cpp:360-399), and
+    SetAcceleration landing in the body (Body.cpp:203-238); contacts and
+    proxies are empty before the first step creates them."""
+    import math
+    w = pa.World.scene("billiards", [1, 0])
+    shapes, bodies, _ = dumpfmt.read_description(dumpfmt.Reader(w.save_desc()))
+    sh = shapes[int(bodies[5]["shapes"][0])]
+    r, rho = float(sh["radius"]), float(sh["density"])
+    m = rho * math.pi * r * r
+    inv_m, inv_i, cx, cy = w.body_mass(5)
+    assert inv_m == pytest.approx(1.0 / m, rel=1e-6)
+    assert inv_i == pytest.approx(1.0 / (0.5 * m * r * r), rel=1e-6)
+    assert (cx, cy) == (0.0, 0.0)
+    w.set_acceleration(5, (1.5, -2.5, 0.25))
+    _, bodies2, _ = dumpfmt.read_description(dumpfmt.Reader(w.save_desc()))
+    assert list(bodies2[5]["vals"][11:14]) == [1.5, -2.5, 0.25]
+    assert w.contacts() == []
+    assert len(w.proxies()[0]) == 0
+    with pytest.raises(pa.PrxError) as e:
+        w.body_mass(len(bodies) + 2)
+    assert e.value.code == 4
+    assert pa.lib().prx_world_set_acceleration(w.h, 5, None) == 1
+
+
 def test_shape_material_setters():
     """SetFriction / SetRestitution (WorldShape.cpp:79-96): the value lands in
     the shape, a body holding the shape is woken with its under-active time

@@ -2,6 +2,8 @@
 brute force, the pre-phase (contact creation and narrowphase) against the
 reference restatement bit for bit, and the colour-parallel solve judged by
 penetration, containment and energy against the reference's exact solve."""
+import os
+
 import numpy as np
 import pytest
 
@@ -211,3 +213,56 @@ def test_big_deterministic():
     ba.sync()
     bb.sync()
     assert a.state_record() == b.state_record()
+
+
+@pytest.mark.parametrize("toi", [False, True])
+def test_big_world_20k_against_reference_measures(toi):
+    """The one-large-world workload at its real size (20,000 bodies dropped
+    into the 400 x 400 box, seed 1) in the coloured mode, against the
+    reference build's measures of the same world at steps 100, 300 and 600
+    (tests/golden/drop20k.json, written by tests/golden/make_big_fixture.py),
+    with doToi false and true (the reference's default; the coloured mode's
+    TOI pass covers dynamic bodies against static shapes).  Judged:
+    containment, the penetration distribution, resting height, momentum and
+    kinetic energy (tests/bigmetrics.py)."""
+    import json
+    from tests import bigmetrics
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "drop20k.json")))
+    w = pa.World.scene(fx["scene"], fx["params"])
+    info = bigmetrics.WorldInfo(w.save_desc())
+    big = pa.BigWorld(w)
+    conf = pa.StepConf(do_toi=1 if toi else 0)
+    ref = fx["reference"]["doToi_true" if toi else "doToi_false"]
+    done = 0
+    for s in fx["steps"]:
+        big.step(conf, s - done)
+        big.sync()
+        done = s
+        got = bigmetrics.measures(info, dumpfmt.read_state(dumpfmt.Reader(w.state_record())))
+        r = ref[str(s)]
+        print("step", s, "doToi", toi, flush=True)
+        for k in ("escaped", "sep_min", "sep_p01", "sep_p05", "sep_p50", "sep_deepest10", "sep_below_5cm",
+                  "mean_height", "momentum", "angular_momentum", "kinetic_energy", "touching_points"):
+            print("  %-16s coloured %-28s reference %s" % (k, got[k], r[k]), flush=True)
+        # Containment: with doToi the reference loses no body; without it a
+        # few fast bodies tunnel through the floor in both engines.
+        if toi:
+            assert got["escaped"] == r["escaped"] == 0
+        else:
+            assert got["escaped"] <= r["escaped"] + 3
+        # The penetration distribution (metres): its bulk within millimetres,
+        # its deep tail no more than 5 cm deeper than the reference's.
+        assert abs(got["sep_p50"] - r["sep_p50"]) <= 0.002
+        assert abs(got["sep_p05"] - r["sep_p05"]) <= 0.006
+        assert abs(got["sep_p01"] - r["sep_p01"]) <= 0.012
+        assert got["sep_deepest10"] >= r["sep_deepest10"] - 0.05
+        # The pile's resting height, momentum and contact count.
+        assert abs(got["mean_height"] - r["mean_height"]) <= 0.02 * r["mean_height"]
+        assert abs(got["momentum"][1] - r["momentum"][1]) <= 0.03 * abs(r["momentum"][1])
+        assert abs(got["momentum"][0]) <= 0.01 * abs(r["momentum"][1]) + 500.0
+        assert abs(got["touching_points"] - r["touching_points"]) <= 0.06 * r["touching_points"]
+        # Kinetic energy: at step 600 most of it is in the escaped bodies'
+        # fall, so its tolerance follows the escaped count.
+        ke_tol = 0.01 if (toi or s < 600) else 0.15
+        assert abs(got["kinetic_energy"] - r["kinetic_energy"]) <= ke_tol * r["kinetic_energy"]
+        assert abs(got["total_mass"] - r["total_mass"]) <= 1e-6 * r["total_mass"]
