@@ -3245,64 +3245,118 @@ __device__ __forceinline__ RoundEnds stage_round_ends(const LevelPlan& P, float4
     return RoundEnds{e, P.rb, P.rounds};
 }
 
-// Warm start and velocity sweeps of one island in level form.  Round k+1's
-// records are loaded while round k is solved (a record is read and written
-// by its own update only, once per pass).
-template <bool kLds>
-__device__ __forceinline__ VelResult island_velocity_level(const DeviceState& d, const WorldCtx& x,
-                                                           const Bodies<kLds>& bs, const LevelPlan& P, int c0,
-                                                           const StepParams& c, float4* stage) {
+// One pass of an island's level form.  Round k covers level positions
+// [end(k - 1), end(k)); its records are loaded while round k - 1 is solved,
+// and the end of round k + 1 is read while round k - 1 is solved, so that no
+// round waits on memory it did not ask for a round earlier.  The records are
+// read and the impulses written by buffer instructions that every lane
+// issues in every round, lanes without an update at an offset past the
+// buffer's end (no memory is touched for them): with no branch around them,
+// the waits the compiler places count exactly the operations in flight, so
+// a round's impulse stores retire while the next round runs.
+constexpr int kOutOfRange = 0x40000000;  // a buffer offset past every island's records
+
+template <int kStride, bool kLdsEnds>
+struct LevelRounds {
+    __amdgpu_buffer_rsrc_t rec;
+    const unsigned short* e16;
+    const int2* rb;
+    int n, rounds, lane;
+    __device__ __forceinline__ int end(int k) const {
+        if (k >= rounds) return n;
+        if constexpr (kLdsEnds) return int(e16[k]);
+        else return rb[k].y;
+    }
+    __device__ __forceinline__ void fetch(int start, int stop, float4 (&buf)[kStride]) const {
+        const int pos = start + lane;
+        const int off = (pos < stop) ? pos * (kStride * 16) : kOutOfRange;
+#pragma unroll
+        for (int f = 0; f < kStride; ++f)
+            buf[f] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rec, off + 16 * f, 0, 0));
+    }
+    // Runs body(start, end, records) for every round, two rounds per trip
+    // (the two record buffers alternate without copies).
+    template <class F>
+    __device__ __forceinline__ void run(F&& body) const {
+        float4 a[kStride], b[kStride];
+        int s = 0;
+        int e = end(0);
+        fetch(0, e, a);
+        for (int k = 0; k < rounds; k += 2) {
+            const int e1 = end(k + 1);
+            fetch(e, e1, b);
+            body(s, e, a);
+            if (k + 1 >= rounds) break;
+            const int e2 = end(k + 2);
+            fetch(e1, e2, a);
+            body(e, e1, b);
+            s = e1;
+            e = e2;
+        }
+    }
+};
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t record_rsrc(const float4* base, int n, int stride) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(base), 0, n * stride * 16, 0x00020000);
+}
+
+using U4 = __attribute__((ext_vector_type(4))) unsigned;
+
+template <bool kWarm, bool kLdsEnds, bool kLds>
+__device__ __forceinline__ float velocity_level_pass(const LevelRounds<prx::kVcStride, kLdsEnds>& R,
+                                                     const Bodies<kLds>& bs) {
+    float newInc = 0.0f;
+    R.run([&](int s, int e, float4 (&cur)[prx::kVcStride]) {
+        const int pos = s + R.lane;
+        const bool act = pos < e;
+        float4 o4 = cur[4], o7 = cur[7];
+        if (act) {
+            BodyC A, B;
+            Vc vc = load_vc(cur, A, B);
+            if constexpr (kWarm) {
+                warm_start_one(vc, A, B, bs);
+            } else {
+                const float mi = solve_velocity_one(vc, A, B, bs);
+                o4 = float4{vc.p[0].ni, vc.p[0].ti, vc.p[0].nm, vc.p[0].tm};
+                o7 = float4{vc.p[1].ni, vc.p[1].ti, vc.p[1].nm, vc.p[1].tm};
+                newInc = fmax_std(newInc, mi);
+            }
+        }
+        if constexpr (!kWarm) {
+            const int off = act ? pos * (prx::kVcStride * 16) : kOutOfRange;
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U4, o4), R.rec, off + 64, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U4, o7), R.rec, off + 112, 0, 0);
+        }
+        __syncthreads();
+    });
+    return newInc;
+}
+
+// Warm start and velocity sweeps of one island in level form.
+template <bool kLdsEnds, bool kLds>
+__device__ __forceinline__ VelResult island_velocity_level_run(const DeviceState& d, const WorldCtx& x,
+                                                               const Bodies<kLds>& bs, const LevelPlan& P,
+                                                               int rounds, const unsigned short* e16, int c0,
+                                                               const StepParams& c) {
     VelResult res{0.0f, int(c.velIters)};
     float4* vbase = d.s_vc + (std::size_t(x.co) + std::size_t(c0)) * prx::kVcStride;
     float4* v2 = d.s_vc2 + (std::size_t(x.co) + std::size_t(c0)) * prx::kVcStride;
-    const int npass = int(c.velIters) + (c.warmStart ? 1 : 0);
-    const RoundEnds R = stage_round_ends(P, stage, x.lane);
-    auto fetch = [&](int2 r, float4 (&buf)[prx::kVcStride]) {
-        const int pos = r.x + x.lane;
-        if (pos < r.y) {
-            const float4* src = v2 + std::size_t(pos) * prx::kVcStride;
+    const LevelRounds<prx::kVcStride, kLdsEnds> R{record_rsrc(v2, P.n, prx::kVcStride), e16, P.rb, P.n, rounds,
+                                                  x.lane};
+    if (c.warmStart) velocity_level_pass<true, kLdsEnds, kLds>(R, bs);
+    for (int it = 0; it < int(c.velIters); ++it) {
+        float newInc = velocity_level_pass<false, kLdsEnds, kLds>(R, bs);
 #pragma unroll
-            for (int f = 0; f < prx::kVcStride; ++f) buf[f] = src[f];
-        }
-    };
-    for (int pass = 0; pass < npass; ++pass) {
-        const bool warm = c.warmStart && pass == 0;
-        float newInc = 0.0f;
-        float4 cur[prx::kVcStride], nxt[prx::kVcStride];
-        int2 r = (P.rounds > 0) ? R.round(0) : int2{0, 0};
-        fetch(r, cur);
-        for (int k = 0; k < P.rounds; ++k) {
-            const int2 rn = (k + 1 < P.rounds) ? int2{r.y, R.end(k + 1)} : int2{0, 0};
-            fetch(rn, nxt);
-            const int pos = r.x + x.lane;
-            if (pos < r.y) {
-                BodyC A, B;
-                Vc vc = load_vc(cur, A, B);
-                if (warm) {
-                    warm_start_one(vc, A, B, bs);
-                } else {
-                    const float mi = solve_velocity_one(vc, A, B, bs);
-                    float4* rec = v2 + std::size_t(pos) * prx::kVcStride;
-                    rec[4] = float4{vc.p[0].ni, vc.p[0].ti, vc.p[0].nm, vc.p[0].tm};
-                    rec[7] = float4{vc.p[1].ni, vc.p[1].ti, vc.p[1].nm, vc.p[1].tm};
-                    newInc = fmax_std(newInc, mi);
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int f = 0; f < prx::kVcStride; ++f) cur[f] = nxt[f];
-            r = rn;
-        }
-        if (!warm) {
-#pragma unroll
-            for (int off = kWave / 2; off > 0; off >>= 1) newInc = fmax_std(newInc, __shfl_xor(newInc, off));
-            res.maxInc = fmax_std(res.maxInc, newInc);
-            if (newInc <= c.regMinMomentum) {
-                res.iters = pass - (c.warmStart ? 1 : 0) + 1;
-                break;
-            }
+        for (int off = kWave / 2; off > 0; off >>= 1) newInc = fmax_std(newInc, __shfl_xor(newInc, off));
+        res.maxInc = fmax_std(res.maxInc, newInc);
+        if (newInc <= c.regMinMomentum) {
+            res.iters = it + 1;
+            break;
         }
     }
+    // The passes' stores complete before other lanes read the records back:
+    // s_waitcnt vmcnt(0) (expcnt and lgkmcnt at their maxima).
+    __builtin_amdgcn_s_waitcnt(0x0F70);
     for (int pos = x.lane; pos < P.n; pos += kWave) {
         float4* dst = vbase + std::size_t(P.order[pos]) * prx::kVcStride;
         const float4* src = v2 + std::size_t(pos) * prx::kVcStride;
@@ -3313,32 +3367,30 @@ __device__ __forceinline__ VelResult island_velocity_level(const DeviceState& d,
     return res;
 }
 
-// Position sweeps of one island in level form (records prefetched a round
-// ahead); returns whether solved.
 template <bool kLds>
-__device__ __forceinline__ PosResult island_position_level(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
-                                                      const LevelPlan& P, int c0, const StepParams& c,
-                                                      float4* stage) {
-    const float4* p2 = d.s_pc2 + (std::size_t(x.co) + std::size_t(c0)) * prx::kPcStride;
+__device__ __forceinline__ VelResult island_velocity_level(const DeviceState& d, const WorldCtx& x,
+                                                           const Bodies<kLds>& bs, const LevelPlan& P, int c0,
+                                                           const StepParams& c, float4* stage) {
     const RoundEnds R = stage_round_ends(P, stage, x.lane);
-    auto fetch = [&](int2 r, float4 (&buf)[prx::kPcStride]) {
-        const int pos = r.x + x.lane;
-        if (pos < r.y) {
-            const float4* src = p2 + std::size_t(pos) * prx::kPcStride;
-#pragma unroll
-            for (int f = 0; f < prx::kPcStride; ++f) buf[f] = src[f];
-        }
-    };
+    const int rounds = __builtin_amdgcn_readfirstlane(P.rounds);
+    if (R.lds) return island_velocity_level_run<true, kLds>(d, x, bs, P, rounds, R.lds, c0, c);
+    return island_velocity_level_run<false, kLds>(d, x, bs, P, rounds, nullptr, c0, c);
+}
+
+// Position sweeps of one island in level form; returns whether solved.
+template <bool kLdsEnds, bool kLds>
+__device__ __forceinline__ PosResult island_position_level_run(const DeviceState& d, const WorldCtx& x,
+                                                               const Bodies<kLds>& bs, const LevelPlan& P,
+                                                               int rounds, const unsigned short* e16, int c0,
+                                                               const StepParams& c) {
+    const float4* p2 = d.s_pc2 + (std::size_t(x.co) + std::size_t(c0)) * prx::kPcStride;
+    const LevelRounds<prx::kPcStride, kLdsEnds> R{record_rsrc(p2, P.n, prx::kPcStride), e16, P.rb, P.n, rounds,
+                                                  x.lane};
     PosResult res{__builtin_inff(), 0};
     for (std::uint32_t it = 0; it < c.posIters; ++it) {
         float minSep = __builtin_inff();
-        float4 cur[prx::kPcStride], nxt[prx::kPcStride];
-        int2 r = (P.rounds > 0) ? R.round(0) : int2{0, 0};
-        fetch(r, cur);
-        for (int k = 0; k < P.rounds; ++k) {
-            const int2 rn = (k + 1 < P.rounds) ? int2{r.y, R.end(k + 1)} : int2{0, 0};
-            fetch(rn, nxt);
-            if (r.x + x.lane < r.y) {
+        R.run([&](int s, int e, float4 (&cur)[prx::kPcStride]) {
+            if (s + x.lane < e) {
                 const Pc pc = load_pc(cur);
                 float4 pa = bs.pos(pc.bA);
                 float4 pb = bs.pos(pc.bB);
@@ -3348,10 +3400,7 @@ __device__ __forceinline__ PosResult island_position_level(const DeviceState& d,
                 minSep = fmin_std(minSep, sep);
             }
             __syncthreads();
-#pragma unroll
-            for (int f = 0; f < prx::kPcStride; ++f) cur[f] = nxt[f];
-            r = rn;
-        }
+        });
 #pragma unroll
         for (int off = kWave / 2; off > 0; off >>= 1) minSep = fmin_std(minSep, __shfl_xor(minSep, off));
         res.minSep = fmin_std(res.minSep, minSep);
@@ -3361,6 +3410,16 @@ __device__ __forceinline__ PosResult island_position_level(const DeviceState& d,
         }
     }
     return res;
+}
+
+template <bool kLds>
+__device__ __forceinline__ PosResult island_position_level(const DeviceState& d, const WorldCtx& x, const Bodies<kLds>& bs,
+                                                      const LevelPlan& P, int c0, const StepParams& c,
+                                                      float4* stage) {
+    const RoundEnds R = stage_round_ends(P, stage, x.lane);
+    const int rounds = __builtin_amdgcn_readfirstlane(P.rounds);
+    if (R.lds) return island_position_level_run<true, kLds>(d, x, bs, P, rounds, R.lds, c0, c);
+    return island_position_level_run<false, kLds>(d, x, bs, P, rounds, nullptr, c0, c);
 }
 
 __device__ __forceinline__ bool level_form(const WorldCtx& x, int c0, int c1, int j0, int j1) {

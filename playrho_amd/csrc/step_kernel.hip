@@ -24,7 +24,9 @@
 
 namespace prxd {
 
-__global__ __launch_bounds__(64) void k_step_worlds(DeviceState d, StepParams c) {
+// At most 168 VGPRs, so that three waves share a SIMD.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_step_worlds(DeviceState d,
+                                                                                          StepParams c) {
     const int w = d.schedule[blockIdx.x];
     step_world<false>(d, w, c);
 }
