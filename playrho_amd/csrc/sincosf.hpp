@@ -218,24 +218,18 @@ PRX_HD float CosF(float y) {
 }
 
 // Both values with one argument reduction; each equals SinF / CosF above
-// (same reduction, same table, polynomials of opposite parity).
+// (same reduction, same table, polynomials of opposite parity).  Below pi/4
+// the fast reduction gives quadrant 0 and leaves the argument unchanged
+// (fma(-0, pi/2, x) = x), so the small-argument polynomials are the fast
+// path's and one path serves every |y| < 120 (the tiny-argument results are
+// selected at the end): lanes of a wave with angles on either side of pi/4
+// run the same instructions.
 template <bool Fma>
 PRX_HD void SinCosF(float y, float* sinOut, float* cosOut) {
     double x = y;
     constexpr SinCosTab p = Tab();
     int n;
     const std::uint32_t t = AbsTop12(y);
-    if (t < AbsTop12(0x1.921FB6p-1f)) {
-        const double x2 = x * x;
-        if (t < AbsTop12(0x1p-12f)) {
-            *sinOut = y;
-            *cosOut = 1.0f;
-            return;
-        }
-        *sinOut = Poly<Fma>(x, x2, p, 0, false);
-        *cosOut = Poly<Fma>(x, x2, p, 1, false);
-        return;
-    }
     double s;
     bool neg;
     if (t < AbsTop12(120.0f)) {
@@ -250,8 +244,11 @@ PRX_HD void SinCosF(float y, float* sinOut, float* cosOut) {
         neg = ((n + sign) & 2) != 0;
     }
     const double xs = x * s, x2 = x * x;
-    *sinOut = Poly<Fma>(xs, x2, p, n, neg);
-    *cosOut = Poly<Fma>(xs, x2, p, n ^ 1, neg);
+    const bool tiny = t < AbsTop12(0x1p-12f);
+    const float sv = Poly<Fma>(xs, x2, p, n, neg);
+    const float cv = Poly<Fma>(xs, x2, p, n ^ 1, neg);
+    *sinOut = tiny ? y : sv;
+    *cosOut = tiny ? 1.0f : cv;
 }
 
 }  // namespace prx_math

@@ -1027,25 +1027,27 @@ struct PSM {
     V2 normal, point;
     float sep;
 };
+// The three manifold types share one form, so that lanes of different types
+// run the same instructions: with X the transformation of the body holding
+// the manifold's local point (A for circles and face A, B for face B) and Y
+// the other, P1 = X(local point), P2 = Y(the manifold point) and the
+// separation is dot(P2 - P1, normal).  Circles take the normal from
+// P2 - P1 and the midpoint as the point; the face types take the rotated
+// local normal and P2, and face B reports the normal negated.
 __device__ __forceinline__ PSM get_psm(const Manifold& m, std::uint32_t idx, const Xf& xfA, const Xf& xfB) {
     const V2 plp = idx ? m.p[1].lp : m.p[0].lp;
-    if (m.type == kCircles) {
-        const V2 pA = transform(m.lpt, xfA);
-        const V2 pB = transform(plp, xfB);
-        const V2 delta = pB - pA;
-        const V2 normal = unit_vector(delta, V2{0, 0});
-        return PSM{normal, divs(pA + pB, 2.0f), dot(delta, normal)};
-    }
-    if (m.type == kFaceA) {
-        const V2 plane = transform(m.lpt, xfA);
-        const V2 normal = rotate(m.ln, xfA.q);
-        const V2 clip = transform(plp, xfB);
-        return PSM{normal, clip, dot(clip - plane, normal)};
-    }
-    const V2 plane = transform(m.lpt, xfB);
-    const V2 normal = rotate(m.ln, xfB.q);
-    const V2 clip = transform(plp, xfA);
-    return PSM{-normal, clip, dot(clip - plane, normal)};
+    const bool faceB = m.type == kFaceB;
+    const bool circles = m.type == kCircles;
+    const Xf& X = faceB ? xfB : xfA;
+    const Xf& Y = faceB ? xfA : xfB;
+    const V2 p1 = transform(m.lpt, X);
+    const V2 p2 = transform(plp, Y);
+    const V2 delta = p2 - p1;
+    V2 normal = rotate(m.ln, X.q);
+    if (circles) normal = unit_vector(delta, V2{0, 0});
+    const float sep = dot(delta, normal);
+    const V2 point = circles ? divs(p1 + p2, 2.0f) : p2;
+    return PSM{faceB ? -normal : normal, point, sep};
 }
 
 // GaussSeidel::SolvePositionConstraint (ContactSolver.cpp:494-643).
@@ -1072,57 +1074,35 @@ __device__ __forceinline__ float solve_pc(const Manifold& m, float totalRadius, 
         const float LB = cross(rB, P) / 1.0f;
         return Sol{-mul(A.invMass, P), -(A.invRotI * LA), mul(B.invMass, P), B.invRotI * LB, separation};
     };
-    V2 pAl{posA.x, posA.y}, pBl{posB.x, posB.y};
-    float pAa = posA.z, pBa = posB.z;
-    float minSep = __builtin_inff();
-    if (m.n == 1) {
-        const PSM psm0 = get_psm(m, 0, get_xf(pAl.x, pAl.y, pAa, A.lc), get_xf(pBl.x, pBl.y, pBa, B.lc));
-        const Sol s = solve(psm0, pAl, pBl);
-        pAl = pAl + s.la;
-        pAa = pAa + s.aa;
-        pBl = pBl + s.lb;
-        pBa = pBa + s.ab;
-        minSep = 0.0f + s.sep;
-    } else if (m.n == 2) {
-        const Xf xfA = get_xf(pAl.x, pAl.y, pAa, A.lc);
-        const Xf xfB = get_xf(pBl.x, pBl.y, pBa, B.lc);
-        const PSM psm0 = get_psm(m, 0, xfA, xfB);
-        const PSM psm1 = get_psm(m, 1, xfA, xfB);
-        if (almost_equal(psm0.sep, psm1.sep)) {
-            const Sol s0 = solve(psm0, pAl, pBl);
-            const Sol s1 = solve(psm1, pAl, pBl);
-            pAl = pAl + s0.la + s1.la;
-            pAa = pAa + s0.aa + s1.aa;
-            pBl = pBl + s0.lb + s1.lb;
-            pBa = pBa + s0.ab + s1.ab;
-            minSep = s0.sep;
-        } else if (psm0.sep < psm1.sep) {
-            const Sol s0 = solve(psm0, pAl, pBl);
-            pAl = pAl + s0.la;
-            pAa = pAa + s0.aa;
-            pBl = pBl + s0.lb;
-            pBa = pBa + s0.ab;
-            const PSM p1 = get_psm(m, 1, get_xf(pAl.x, pAl.y, pAa, A.lc), get_xf(pBl.x, pBl.y, pBa, B.lc));
-            const Sol s1 = solve(p1, pAl, pBl);
-            pAl = pAl + s1.la;
-            pAa = pAa + s1.aa;
-            pBl = pBl + s1.lb;
-            pBa = pBa + s1.ab;
-            minSep = s0.sep;
-        } else if (psm1.sep < psm0.sep) {
-            const Sol s1 = solve(psm1, pAl, pBl);
-            pAl = pAl + s1.la;
-            pAa = pAa + s1.aa;
-            pBl = pBl + s1.lb;
-            pBa = pBa + s1.ab;
-            const PSM p0 = get_psm(m, 0, get_xf(pAl.x, pAl.y, pAa, A.lc), get_xf(pBl.x, pBl.y, pBa, B.lc));
-            const Sol s0 = solve(p0, pAl, pBl);
-            pAl = pAl + s0.la;
-            pAa = pAa + s0.aa;
-            pBl = pBl + s0.lb;
-            pBa = pBa + s0.ab;
-            minSep = s1.sep;
-        }
+    // One path for every lane: the first point solved is point 0 (one
+    // point, equal separations, or point 0 the deeper) or point 1 (point 1
+    // the deeper); the second point is solved at the positions and
+    // transformations before the first's correction when the separations are
+    // almost equal, else after it.
+    const V2 oAl{posA.x, posA.y}, oBl{posB.x, posB.y};
+    const float oAa = posA.z, oBa = posB.z;
+    const Xf xfA = get_xf(oAl.x, oAl.y, oAa, A.lc);
+    const Xf xfB = get_xf(oBl.x, oBl.y, oBa, B.lc);
+    const PSM psm0 = get_psm(m, 0, xfA, xfB);
+    const PSM psm1 = get_psm(m, 1, xfA, xfB);
+    const bool two = m.n == 2;
+    const bool eq = two && almost_equal(psm0.sep, psm1.sep);
+    const bool lt = two && !eq && psm0.sep < psm1.sep;
+    const bool gt = two && !eq && psm1.sep < psm0.sep;
+    if (!(m.n == 1 || eq || lt || gt)) return __builtin_inff();
+    const Sol s0 = solve(gt ? psm1 : psm0, oAl, oBl);
+    V2 pAl = oAl + s0.la;
+    float pAa = oAa + s0.aa;
+    V2 pBl = oBl + s0.lb;
+    float pBa = oBa + s0.ab;
+    const float minSep = two ? s0.sep : 0.0f + s0.sep;
+    if (two) {
+        const PSM next = get_psm(m, gt ? 0u : 1u, get_xf(pAl.x, pAl.y, pAa, A.lc), get_xf(pBl.x, pBl.y, pBa, B.lc));
+        const Sol s1 = solve(eq ? psm1 : next, eq ? oAl : pAl, eq ? oBl : pBl);
+        pAl = pAl + s1.la;
+        pAa = pAa + s1.aa;
+        pBl = pBl + s1.lb;
+        pBa = pBa + s1.ab;
     }
     posA.x = pAl.x;
     posA.y = pAl.y;
