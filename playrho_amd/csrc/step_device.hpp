@@ -281,9 +281,67 @@ __device__ __forceinline__ void ev_update(const DeviceState& d, const WorldCtx& 
 }
 
 // UpdateContacts (AabbTreeWorld.cpp:2333-2421).  Returns contacts updated.
+// The contacts that need an update, grouped by the pair of their shapes'
+// vertex counts (disk 1, edge 2, polygons 3..16), so that the lanes of a
+// wave take the same narrowphase path: each contact's update depends on that
+// contact alone, so the grouping leaves every result as it is.  A counting
+// sort in LDS (ldsAvail bytes); returns -1 where it does not fit, else the
+// number of listed contacts.
+constexpr int kUpdKeys = 17 * 17;
+
+__device__ __forceinline__ int update_key(const DeviceState& d, int gi) {
+    if (d.c_flags[gi] & prx::kSensor) return 0;
+    const int2 ss = d.c_shape[gi];
+    const int na = d.shape_i[ss.x].x, nb = d.shape_i[ss.y].x;
+    const int lo = (na < nb) ? na : nb, hi = (na < nb) ? nb : na;
+    return lo * 17 + hi;
+}
+
+__device__ __forceinline__ int group_updates(const DeviceState& d, const WorldCtx& x, int nc, int ldsAvail) {
+    if (4 * (kUpdKeys + 1) + 6 * nc > ldsAvail) return -1;
+    int* start = reinterpret_cast<int*>(g_smem);   // kUpdKeys + 1 counters
+    int* list = start + kUpdKeys + 1;               // nc contact indices
+    unsigned short* key = reinterpret_cast<unsigned short*>(list + nc);  // nc keys
+    for (int k = x.lane; k <= kUpdKeys; k += kWave) start[k] = 0;
+    __syncthreads();
+    for (int i = x.lane; i < nc; i += kWave) {
+        int kk = kUpdKeys;  // no update
+        if (d.c_flags[x.co + i] & prx::kNeedsUpdating) {
+            kk = update_key(d, x.co + i);
+            atomicAdd(&start[kk], 1);
+        }
+        key[i] = static_cast<unsigned short>(kk);
+    }
+    __syncthreads();
+    int carry = 0;
+    for (int base = 0; base < kUpdKeys; base += kWave) {
+        const int k = base + x.lane;
+        const int v = (k < kUpdKeys) ? start[k] : 0;
+        const int incl = wave_incl_scan(v, x.lane);
+        if (k < kUpdKeys) start[k] = carry + incl - v;
+        carry += __shfl(incl, kWave - 1);
+    }
+    __syncthreads();
+    for (int i = x.lane; i < nc; i += kWave) {
+        const int kk = key[i];
+        if (kk < kUpdKeys) list[atomicAdd(&start[kk], 1)] = i;
+    }
+    __syncthreads();
+    return carry;
+}
+
 __device__ __forceinline__ std::uint32_t phase_update(const DeviceState& d, const WorldCtx& x, Counters& k,
-                                                      const StepParams& c) {
+                                                      const StepParams& c, int ldsAvail) {
     const int nc = k.nc;
+    if (x.ecap == 0) {
+        const int n = group_updates(d, x, nc, ldsAvail);
+        if (n >= 0) {
+            const int* list = reinterpret_cast<const int*>(g_smem) + kUpdKeys + 1;
+            for (int r = x.lane; r < n; r += kWave) update_contact(d, x.bo, x.co + list[r], c);
+            __syncthreads();
+            return std::uint32_t(n);
+        }
+    }
     std::uint32_t updated = 0;
     for (int base = 0; base < nc; base += kWave) {
         const int i = base + x.lane;
@@ -3873,7 +3931,7 @@ __device__ __forceinline__ void step_world(const DeviceState& d, int w, const St
     __syncthreads();
     T.mark(prx::kPhDestroy);
     {
-        const std::uint32_t updated = phase_update(d, x, k, c);
+        const std::uint32_t updated = phase_update(d, x, k, c, int(c.ldsBytes));
         stat(prx::kStPreUpdated, updated);
         stat(prx::kStPreSkipped, std::uint32_t(k.nc) - updated);
     }
@@ -3937,7 +3995,7 @@ __device__ __forceinline__ void step_world(const DeviceState& d, int w, const St
         for (int b = x.lane; b < x.nb; b += kWave) d.bflags[x.bo + b] &= ~0x100u;
         __syncthreads();
         {
-            const std::uint32_t updated = phase_update(d, x, k, c);
+            const std::uint32_t updated = phase_update(d, x, k, c, int(c.ldsBytes));
             stat(prx::kStRegUpdated, updated);
             stat(prx::kStRegSkipped, std::uint32_t(k.nc) - updated);
         }

@@ -1004,7 +1004,7 @@ __device__ __forceinline__ void toi_phase(const DeviceState& d, const WorldCtx& 
     for (int b = x.lane; b < x.nb; b += kWave) d.alpha0[x.bo + b] = 0.0f;
     for (int i = x.lane; i < k.nc; i += kWave) d.c_toiinfo[x.co + i] = 0u;
     __syncthreads();
-    const std::uint32_t updated = phase_update(d, x, k, c);
+    const std::uint32_t updated = phase_update(d, x, k, c, 0);
     if (x.lane == 0) {
         st[prx::kStToiUpdatedTouching] += updated;
         st[prx::kStToiSkippedTouching] += std::uint32_t(k.nc) - updated;
