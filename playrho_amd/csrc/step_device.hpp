@@ -1008,7 +1008,10 @@ __device__ __forceinline__ float solve_dir(Vc& vc, bool tangent, const BodyC& A,
     return maxInc;
 }
 
-// BlockSolveNormalConstraint (ContactSolver.cpp:65-335).
+// BlockSolveNormalConstraint (ContactSolver.cpp:65-335).  The four cases
+// (both points, point 0 alone, point 1 alone, neither) are all evaluated and
+// the first valid one selected, in the reference's order; with no valid case
+// the velocities and impulses stay as they are.
 __device__ __forceinline__ float block_solve(Vc& vc, const BodyC& A, const BodyC& B, float4& velA, float4& velB) {
     const V2 normal = vc.normal;
     V2 bp;
@@ -1027,37 +1030,28 @@ __device__ __forceinline__ float block_solve(Vc& vc, const BodyC& A, const BodyC
         t1 += vc.K11 * vc.p[1].ni;
         bp = V2{b.x - t0, b.y - t1};
     }
-    float n0, n1;
-    bool ok = false;
-    {
-        float x0 = 0.0f;
-        x0 += vc.M00 * bp.x;
-        x0 += vc.M01 * bp.y;
-        float x1 = 0.0f;
-        x1 += vc.M01 * bp.x;
-        x1 += vc.M11 * bp.y;
-        n0 = -x0;
-        n1 = -x1;
-        ok = (n0 >= 0.0f && n1 >= 0.0f);
-    }
-    if (!ok) {
-        n0 = -vc.p[0].nm * bp.x;
-        n1 = 0.0f;
-        const float vn2 = vc.K01 * n0 + bp.y;
-        ok = (n0 >= 0.0f && vn2 >= 0.0f);
-    }
-    if (!ok) {
-        n0 = 0.0f;
-        n1 = -vc.p[1].nm * bp.y;
-        const float vn1 = vc.K01 * n1 + bp.x;
-        ok = (n1 >= 0.0f && vn1 >= 0.0f);
-    }
-    if (!ok) {
-        n0 = 0.0f;
-        n1 = 0.0f;
-        ok = (bp.x >= 0.0f && bp.y >= 0.0f);
-    }
-    if (!ok) return 0.0f;
+    // Case 1: both points (x = -M b').
+    float x0 = 0.0f;
+    x0 += vc.M00 * bp.x;
+    x0 += vc.M01 * bp.y;
+    float x1 = 0.0f;
+    x1 += vc.M01 * bp.x;
+    x1 += vc.M11 * bp.y;
+    const float c1n0 = -x0, c1n1 = -x1;
+    const bool ok1 = c1n0 >= 0.0f && c1n1 >= 0.0f;
+    // Case 2: point 0 alone.
+    const float c2n0 = -vc.p[0].nm * bp.x;
+    const float vn2 = vc.K01 * c2n0 + bp.y;
+    const bool ok2 = c2n0 >= 0.0f && vn2 >= 0.0f;
+    // Case 3: point 1 alone.
+    const float c3n1 = -vc.p[1].nm * bp.y;
+    const float vn1 = vc.K01 * c3n1 + bp.x;
+    const bool ok3 = c3n1 >= 0.0f && vn1 >= 0.0f;
+    // Case 4: neither.
+    const bool ok4 = bp.x >= 0.0f && bp.y >= 0.0f;
+    const float n0 = ok1 ? c1n0 : (ok2 ? c2n0 : 0.0f);
+    const float n1 = ok1 ? c1n1 : (ok2 ? 0.0f : (ok3 ? c3n1 : 0.0f));
+    const bool ok = ok1 || ok2 || ok3 || ok4;
     const float d0 = n0 - vc.p[0].ni;
     const float d1 = n1 - vc.p[1].ni;
     const V2 P0 = mul(d0, normal);
@@ -1069,15 +1063,15 @@ __device__ __forceinline__ float block_solve(Vc& vc, const BodyC& A, const BodyC
     const float dAa = -(A.invRotI * LA);
     const V2 dBl = mul(B.invMass, P);
     const float dBa = B.invRotI * LB;
-    velA.x = velA.x + dAl.x;
-    velA.y = velA.y + dAl.y;
-    velA.z = velA.z + dAa;
-    velB.x = velB.x + dBl.x;
-    velB.y = velB.y + dBl.y;
-    velB.z = velB.z + dBa;
-    vc.p[0].ni = n0;
-    vc.p[1].ni = n1;
-    return fmax_std(__builtin_fabsf(n0), __builtin_fabsf(n1));
+    velA.x = ok ? velA.x + dAl.x : velA.x;
+    velA.y = ok ? velA.y + dAl.y : velA.y;
+    velA.z = ok ? velA.z + dAa : velA.z;
+    velB.x = ok ? velB.x + dBl.x : velB.x;
+    velB.y = ok ? velB.y + dBl.y : velB.y;
+    velB.z = ok ? velB.z + dBa : velB.z;
+    vc.p[0].ni = ok ? n0 : vc.p[0].ni;
+    vc.p[1].ni = ok ? n1 : vc.p[1].ni;
+    return ok ? fmax_std(__builtin_fabsf(n0), __builtin_fabsf(n1)) : 0.0f;
 }
 
 // Position solver manifold (PositionSolverManifold.cpp:42-121).
@@ -3136,13 +3130,236 @@ __device__ __forceinline__ PosResult island_position_wave(const DeviceState& d, 
 }
 
 // ---------------------------------------------------------------------------
+// Paired-lane forms of the contact updates for the level form: lane 2j
+// (side 0) computes update j's body-A terms and lane 2j + 1 (side 1) its
+// body-B terms, exchanging what the other body contributes through a DPP
+// swap within the pair; terms of both bodies (relative velocity, impulses,
+// block-solver cases, position-solver manifolds) are computed by both lanes
+// from the same values.  Body A's changes are the negated body-B form
+// (a - b*c = a + (-1)*(b*c) exactly), so both lanes run one instruction
+// stream and every result equals the single-lane form bit for bit.
+__device__ __forceinline__ float pair_swap(float v) {
+    // quad_perm [1, 0, 3, 2]: each lane reads its pair partner.
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
+}
+__device__ __forceinline__ V2 pair_swap(V2 v) { return V2{pair_swap(v.x), pair_swap(v.y)}; }
+
+// GetContactRelVelocity (d2/Math.cpp:73-103) from the pair's two halves.
+__device__ __forceinline__ V2 pair_rel_vel(V2 v, float w, V2 rel, int side) {
+    const V2 u = v + mul(revperp(rel), w / 1.0f);
+    const V2 o = pair_swap(u);
+    const V2 uA = side ? o : u;
+    const V2 uB = side ? u : o;
+    return uB - uA;
+}
+
+struct PairBody {
+    float m, i, sgn;  // inverse mass, inverse rotational inertia, -1 for body A
+};
+
+__device__ __forceinline__ void pair_apply(V2& v, float& w, const PairBody& b, V2 P, float L) {
+    const V2 mp = mul(b.m, P);
+    v = V2{v.x + b.sgn * mp.x, v.y + b.sgn * mp.y};
+    w = w + b.sgn * (b.i * L);
+}
+
+__device__ __forceinline__ float solve_dir_pair(Vc& vc, bool tangent, const PairBody& me, int side, V2& v, float& w) {
+    float maxInc = 0.0f;
+    const V2 dir = tangent ? fwdperp(vc.normal) : vc.normal;
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+        const int k = 1 - kk;  // point 1 first, then point 0
+        if (k >= int(vc.n)) continue;
+        VcPoint& p = vc.p[k];
+        const V2 rel = side ? p.relB : p.relA;
+        const V2 closing = pair_rel_vel(v, w, rel, side);
+        float oldI, newI;
+        if (tangent) {
+            const float dirVel = vc.tangentSpeed - dot(closing, dir);
+            const float lambda = p.tm * dirVel;
+            const float maxI = vc.friction * p.ni;
+            oldI = p.ti;
+            newI = clamp_std(oldI + lambda, -maxI, maxI);
+        } else {
+            const float dirVel = dot(closing, dir);
+            const float lambda = p.nm * (p.bias - dirVel);
+            oldI = p.ni;
+            newI = fmax_std(oldI + lambda, 0.0f);
+        }
+        const float inc = newI - oldI;
+        const V2 P = mul(inc, dir);
+        pair_apply(v, w, me, P, cross(rel, P) / 1.0f);
+        maxInc = fmax_std(maxInc, __builtin_fabsf(inc));
+        if (tangent) p.ti = oldI + inc;
+        else p.ni = oldI + inc;
+    }
+    return maxInc;
+}
+
+__device__ __forceinline__ float block_solve_pair(Vc& vc, const PairBody& me, int side, V2& v, float& w) {
+    const V2 normal = vc.normal;
+    const V2 rel0 = side ? vc.p[0].relB : vc.p[0].relA;
+    const V2 rel1 = side ? vc.p[1].relB : vc.p[1].relA;
+    V2 bp;
+    {
+        const V2 dv0 = pair_rel_vel(v, w, rel0, side);
+        const V2 dv1 = pair_rel_vel(v, w, rel1, side);
+        const float vn0 = dot(dv0, normal);
+        const float vn1 = dot(dv1, normal);
+        const V2 b{vn0 - vc.p[0].bias, vn1 - vc.p[1].bias};
+        float t0 = 0.0f;
+        t0 += vc.K00 * vc.p[0].ni;
+        t0 += vc.K01 * vc.p[1].ni;
+        float t1 = 0.0f;
+        t1 += vc.K01 * vc.p[0].ni;
+        t1 += vc.K11 * vc.p[1].ni;
+        bp = V2{b.x - t0, b.y - t1};
+    }
+    float x0 = 0.0f;
+    x0 += vc.M00 * bp.x;
+    x0 += vc.M01 * bp.y;
+    float x1 = 0.0f;
+    x1 += vc.M01 * bp.x;
+    x1 += vc.M11 * bp.y;
+    const float c1n0 = -x0, c1n1 = -x1;
+    const bool ok1 = c1n0 >= 0.0f && c1n1 >= 0.0f;
+    const float c2n0 = -vc.p[0].nm * bp.x;
+    const float vn2 = vc.K01 * c2n0 + bp.y;
+    const bool ok2 = c2n0 >= 0.0f && vn2 >= 0.0f;
+    const float c3n1 = -vc.p[1].nm * bp.y;
+    const float vn1 = vc.K01 * c3n1 + bp.x;
+    const bool ok3 = c3n1 >= 0.0f && vn1 >= 0.0f;
+    const bool ok4 = bp.x >= 0.0f && bp.y >= 0.0f;
+    const float n0 = ok1 ? c1n0 : (ok2 ? c2n0 : 0.0f);
+    const float n1 = ok1 ? c1n1 : (ok2 ? 0.0f : (ok3 ? c3n1 : 0.0f));
+    const bool ok = ok1 || ok2 || ok3 || ok4;
+    const float d0 = n0 - vc.p[0].ni;
+    const float d1 = n1 - vc.p[1].ni;
+    const V2 P0 = mul(d0, normal);
+    const V2 P1 = mul(d1, normal);
+    const V2 P = P0 + P1;
+    const float L = (cross(rel0, P0) + cross(rel1, P1)) / 1.0f;
+    V2 nv = v;
+    float nw = w;
+    pair_apply(nv, nw, me, P, L);
+    v = ok ? nv : v;
+    w = ok ? nw : w;
+    vc.p[0].ni = ok ? n0 : vc.p[0].ni;
+    vc.p[1].ni = ok ? n1 : vc.p[1].ni;
+    return ok ? fmax_std(__builtin_fabsf(n0), __builtin_fabsf(n1)) : 0.0f;
+}
+
+template <bool kLds>
+__device__ __forceinline__ float solve_velocity_pair(Vc& vc, const BodyC& A, const BodyC& B, const Bodies<kLds>& bs,
+                                                     int side) {
+    const int b = side ? vc.bB : vc.bA;
+    const PairBody me{side ? B.invMass : A.invMass, side ? B.invRotI : A.invRotI, side ? 1.0f : -1.0f};
+    const float4 vel = bs.vel(b);
+    V2 v{vel.x, vel.y};
+    float w = vel.z;
+    float mi = 0.0f;
+    mi = fmax_std(mi, solve_dir_pair(vc, true, me, side, v, w));
+    float nrm;
+    if (vc.n == 1 || (vc.K00 == 0.0f && vc.K11 == 0.0f && vc.K01 == 0.0f)) nrm = solve_dir_pair(vc, false, me, side, v, w);
+    else nrm = block_solve_pair(vc, me, side, v, w);
+    mi = fmax_std(mi, nrm);
+    bs.set_vel(b, float4{v.x, v.y, w, vel.w});
+    return mi;
+}
+
+template <bool kLds>
+__device__ __forceinline__ void warm_start_pair(const Vc& vc, const BodyC& A, const BodyC& B, const Bodies<kLds>& bs,
+                                                int side) {
+    const int b = side ? vc.bB : vc.bA;
+    const PairBody me{side ? B.invMass : A.invMass, side ? B.invRotI : A.invRotI, side ? 1.0f : -1.0f};
+    const V2 tangent = fwdperp(vc.normal);
+    V2 dl{0, 0};
+    float da = 0.0f;
+#pragma unroll
+    for (std::uint32_t j = 0; j < 2; ++j) {
+        if (j >= vc.n) continue;
+        const VcPoint& p = vc.p[j];
+        const V2 P = mul(p.ni, vc.normal) + mul(p.ti, tangent);
+        pair_apply(dl, da, me, P, cross(side ? p.relB : p.relA, P) / 1.0f);
+    }
+    float4 vel = bs.vel(b);
+    vel.x = vel.x + dl.x;
+    vel.y = vel.y + dl.y;
+    vel.z = vel.z + da;
+    bs.set_vel(b, vel);
+}
+
+// GaussSeidel::SolvePositionConstraint (ContactSolver.cpp:494-643) in the
+// paired form of solve_pc: each lane computes its own body's transformation
+// (the sine and cosine of its angle) and position change.
+__device__ __forceinline__ float solve_pc_pair(const Manifold& m, float totalRadius, const BodyC& A, const BodyC& B,
+                                               float4& pos, int side, const StepParams& c) {
+    const float invMassTotal = A.invMass + B.invMass;
+    const BodyC& me = side ? B : A;
+    const float sgn = side ? 1.0f : -1.0f;
+    auto xfs = [&](V2 pl, float pa, Xf& xfA, Xf& xfB) {
+        const Xf mine = get_xf(pl.x, pl.y, pa, me.lc);
+        const Xf other{pair_swap(mine.p), pair_swap(mine.q)};
+        xfA = side ? other : mine;
+        xfB = side ? mine : other;
+    };
+    struct Sol {
+        V2 dl;
+        float da, sep;
+    };
+    auto solve = [&](const PSM& psm, V2 p) {
+        const float separation = psm.sep - totalRadius;
+        const V2 r = psm.point - p;
+        const float cr = cross(r, psm.normal) / 1.0f;
+        const float co = pair_swap(cr);
+        const float crA = side ? co : cr;
+        const float crB = side ? cr : co;
+        const float K = invMassTotal + A.invRotI * (crA * crA) + B.invRotI * (crB * crB);
+        const float C = -clamp_std(c.regResolutionRate * (separation + c.linearSlop), -c.maxLinearCorrection, 0.0f);
+        const V2 P = divs(mul(psm.normal, C), K);
+        const float L = cross(r, P) / 1.0f;
+        const V2 mp = mul(me.invMass, P);
+        return Sol{V2{sgn * mp.x, sgn * mp.y}, sgn * (me.invRotI * L), separation};
+    };
+    const V2 op{pos.x, pos.y};
+    const float oa = pos.z;
+    Xf xfA, xfB;
+    xfs(op, oa, xfA, xfB);
+    const PSM psm0 = get_psm(m, 0, xfA, xfB);
+    const PSM psm1 = get_psm(m, 1, xfA, xfB);
+    const bool two = m.n == 2;
+    const bool eq = two && almost_equal(psm0.sep, psm1.sep);
+    const bool lt = two && !eq && psm0.sep < psm1.sep;
+    const bool gt = two && !eq && psm1.sep < psm0.sep;
+    if (!(m.n == 1 || eq || lt || gt)) return __builtin_inff();
+    const Sol s0 = solve(gt ? psm1 : psm0, op);
+    V2 p = op + s0.dl;
+    float a = oa + s0.da;
+    const float minSep = two ? s0.sep : 0.0f + s0.sep;
+    if (two) {
+        Xf nA, nB;
+        xfs(p, a, nA, nB);
+        const PSM next = get_psm(m, gt ? 0u : 1u, nA, nB);
+        const Sol s1 = solve(eq ? psm1 : next, eq ? op : p);
+        p = p + s1.dl;
+        a = a + s1.da;
+    }
+    pos.x = p.x;
+    pos.y = p.y;
+    pos.z = a;
+    return minSep;
+}
+
+// ---------------------------------------------------------------------------
 // Level form of the wave-wide sweeps, for islands without joints in worlds of
 // at most kLevelMaxBodies bodies.  The Gauss-Seidel order is kept exactly: an
 // update's level is one more than the last level that touched either of its
 // bodies (bodies with zero inverse mass and inertia are never changed by an
 // update, so they order nothing), and the updates of one level touch
 // distinct bodies.  The wave runs each level's updates in parallel, level
-// after level, with the inputs the serial sweep gives them; warm-start
+// after level, two lanes to an update (each lane computes its own body's
+// terms, the pair exchanging partial results), with the inputs the serial
+// sweep gives them; warm-start
 // deltas reach each body in the same order, and the early exits reduce the
 // same maxima and minima.  The plan (levels, level order, rounds of at most
 // one wave) is built once per step and serves the warm start, the velocity
@@ -3150,6 +3367,8 @@ __device__ __forceinline__ PosResult island_position_wave(const DeviceState& d, 
 // order so that a round reads consecutive records.
 constexpr int kLevelMaxBodies = prx::kStageFloat4 * 4;
 constexpr int kLevelMinConstraints = 8;
+// Updates per round: two lanes serve one update (one lane per body).
+constexpr int kLevelRound = kWave / 2;
 
 struct LevelPlan {
     int n, rounds;
@@ -3228,11 +3447,11 @@ __device__ __forceinline__ LevelPlan level_plan(const DeviceState& d, const Worl
     for (int base = 0; base < levels; base += kWave) {
         const int k = base + lane;
         const int cnt = (k < levels) ? start[k + 1] - start[k] : 0;
-        const int nr = (cnt + kWave - 1) / kWave;
+        const int nr = (cnt + kLevelRound - 1) / kLevelRound;
         const int incl = wave_incl_scan(nr, lane);
         for (int i = 0; i < nr; ++i) {
-            const int b = start[k] + i * kWave;
-            const int e = (b + kWave < start[k + 1]) ? b + kWave : start[k + 1];
+            const int b = start[k] + i * kLevelRound;
+            const int e = (b + kLevelRound < start[k + 1]) ? b + kLevelRound : start[k + 1];
             rb[rounds + incl - nr + i] = int2{b, e};
         }
         rounds += __shfl(incl, kWave - 1);
@@ -3306,7 +3525,7 @@ struct LevelRounds {
         else return rb[k].y;
     }
     __device__ __forceinline__ void fetch(int start, int stop, float4 (&buf)[kStride]) const {
-        const int pos = start + lane;
+        const int pos = start + (lane >> 1);
         const int off = (pos < stop) ? pos * (kStride * 16) : kOutOfRange;
 #pragma unroll
         for (int f = 0; f < kStride; ++f)
@@ -3344,24 +3563,26 @@ template <bool kWarm, bool kLdsEnds, bool kLds>
 __device__ __forceinline__ float velocity_level_pass(const LevelRounds<prx::kVcStride, kLdsEnds>& R,
                                                      const Bodies<kLds>& bs) {
     float newInc = 0.0f;
+    const int side = R.lane & 1;
     R.run([&](int s, int e, float4 (&cur)[prx::kVcStride]) {
-        const int pos = s + R.lane;
+        const int pos = s + (R.lane >> 1);
         const bool act = pos < e;
         float4 o4 = cur[4], o7 = cur[7];
         if (act) {
             BodyC A, B;
             Vc vc = load_vc(cur, A, B);
             if constexpr (kWarm) {
-                warm_start_one(vc, A, B, bs);
+                warm_start_pair(vc, A, B, bs, side);
             } else {
-                const float mi = solve_velocity_one(vc, A, B, bs);
+                const float mi = solve_velocity_pair(vc, A, B, bs, side);
                 o4 = float4{vc.p[0].ni, vc.p[0].ti, vc.p[0].nm, vc.p[0].tm};
                 o7 = float4{vc.p[1].ni, vc.p[1].ti, vc.p[1].nm, vc.p[1].tm};
                 newInc = fmax_std(newInc, mi);
             }
         }
         if constexpr (!kWarm) {
-            const int off = act ? pos * (prx::kVcStride * 16) : kOutOfRange;
+            // The pair's lanes hold the same impulses: the body-A lane stores.
+            const int off = (act && side == 0) ? pos * (prx::kVcStride * 16) : kOutOfRange;
             __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U4, o4), R.rec, off + 64, 0, 0);
             __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U4, o7), R.rec, off + 112, 0, 0);
         }
@@ -3427,14 +3648,14 @@ __device__ __forceinline__ PosResult island_position_level_run(const DeviceState
     PosResult res{__builtin_inff(), 0};
     for (std::uint32_t it = 0; it < c.posIters; ++it) {
         float minSep = __builtin_inff();
+        const int side = x.lane & 1;
         R.run([&](int s, int e, float4 (&cur)[prx::kPcStride]) {
-            if (s + x.lane < e) {
+            if (s + (x.lane >> 1) < e) {
                 const Pc pc = load_pc(cur);
-                float4 pa = bs.pos(pc.bA);
-                float4 pb = bs.pos(pc.bB);
-                const float sep = solve_pc(pc.m, pc.totalRadius, pc.A, pc.B, pa, pb, c);
-                bs.set_pos(pc.bA, pa);
-                bs.set_pos(pc.bB, pb);
+                const int mine = side ? pc.bB : pc.bA;
+                float4 p = bs.pos(mine);
+                const float sep = solve_pc_pair(pc.m, pc.totalRadius, pc.A, pc.B, p, side, c);
+                bs.set_pos(mine, p);
                 minSep = fmin_std(minSep, sep);
             }
             __syncthreads();
