@@ -176,6 +176,23 @@ typedef struct prx_pulley_joint_conf {
     uint32_t collide_connected;
 } prx_pulley_joint_conf;
 
+/* PrismaticJointConf (d2/PrismaticJointConf.hpp:61-217): body B slides along
+ * the local x axis of A (a unit vector) with the relative rotation held at the
+ * reference angle, with an optional translation limit [lower, upper] and an
+ * optional motor (speed, maximum force).  The reference's defaults are axis
+ * (1, 0), reference angle 0, limit and motor off. */
+typedef struct prx_prismatic_joint_conf {
+    uint32_t body_a, body_b;
+    float local_anchor_a[2], local_anchor_b[2];
+    float local_axis_a[2];
+    float reference_angle;
+    uint32_t enable_limit;
+    float lower_translation, upper_translation;
+    uint32_t enable_motor;
+    float motor_speed, max_motor_force;
+    uint32_t collide_connected;
+} prx_prismatic_joint_conf;
+
 /* StepConf fields the regular step reads (StepConf.hpp:49-392). */
 typedef struct prx_step_conf {
     float delta_time, dt_ratio, min_still_time_to_sleep, linear_slop, angular_slop;
@@ -265,6 +282,8 @@ int prx_wheel_joint_create(prx_world* world, const prx_wheel_joint_conf* conf, u
 /* CreateJoint(World&, const PulleyJointConf&) (solver PulleyJointConf.cpp:92-253;
  * prx_world_shift_origin moves its ground anchors, PulleyJointConf.cpp:255-260). */
 int prx_pulley_joint_create(prx_world* world, const prx_pulley_joint_conf* conf, uint32_t* joint_id);
+/* CreateJoint(World&, const PrismaticJointConf&) (solver PrismaticJointConf.cpp:155-531). */
+int prx_prismatic_joint_create(prx_world* world, const prx_prismatic_joint_conf* conf, uint32_t* joint_id);
 uint32_t prx_world_joint_count(const prx_world* world);
 /* Builds a named workload scene through the calls above. */
 int prx_scene_build(prx_world* world, const char* name, const int32_t* params, int nparams);

@@ -116,7 +116,11 @@ void Load(Handle<R>& h, In& in) {
         j.motorSpeed = f[7];
         j.maxMotorTorque = f[8];
         j.kind = (fl >> 8) & 0xffu;
-        if (j.kind != 0) {
+        if (j.kind == 8) {
+            // Prismatic: reference angle, lower and upper translation, motor
+            // speed, maximum motor force (in maxMotorTorque), local x axis of A.
+            j.localXAxis = V2<R>{f[9], f[10]};
+        } else if (j.kind != 0) {
             // Distance: length, frequency, damping ratio.  Weld: reference
             // angle, frequency, damping ratio.  Rope: maximum length.
             // Friction: maximum force, maximum torque.  Motor: linear offset,
@@ -374,6 +378,11 @@ void Widen(const Handle<S>& s, Handle<D>& d) {
         k.groundB = Wd<D>(j.groundB);
         k.lengthB = D(j.lengthB);
         k.ratio = D(j.ratio);
+        k.s1 = D(j.s1);
+        k.s2 = D(j.s2);
+        k.a1 = D(j.a1);
+        k.a2 = D(j.a2);
+        k.motorMass = D(j.motorMass);
         b.joints.push_back(k);
     }
     b.bodyJoints = a.bodyJoints;

@@ -803,6 +803,10 @@ struct Joint {
     // u, uB in ay; impulse in impulse.x, mass in dmass.
     V2<R> groundA, groundB;
     R lengthB = 0, ratio = 1;
+    // kind 8 prismatic: local x axis of A in localXAxis; translation limits in
+    // lower, upper; the maximum motor force in maxMotorTorque.  Solver data:
+    // axis in ax, perp in ay, s1, s2, a1, a2, K in mass, motorMass.
+    R s1 = 0, s2 = 0, a1 = 0, a2 = 0, motorMass = 0;
 };
 
 template <class R>
@@ -2254,6 +2258,221 @@ public:
         return linearError < conf.linearSlop;
     }
 
+    // InitVelocity(PrismaticJointConf&) (PrismaticJointConf.cpp:155-282).
+    static void PrismaticInitVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        const R mA = A.invMass, iA = A.invRotI, mB = B.invMass, iB = B.invRotI;
+        Vel<R> velA = A.vel, velB = B.vel;
+        const V2<R> qA = UnitFromAngle(A.pos.ang);
+        const V2<R> qB = UnitFromAngle(B.pos.ang);
+        const V2<R> rA = Rotate(j.laA - A.lc, qA);
+        const V2<R> rB = Rotate(j.laB - B.lc, qB);
+        const V2<R> d = ((B.pos.lin - A.pos.lin) + rB) - rA;
+        j.ax = Rotate(j.localXAxis, qA);
+        j.a1 = Cross(d + rA, j.ax);
+        j.a2 = Cross(rB, j.ax);
+        const R invRotMassA = iA * (j.a1 * j.a1) / R(1);
+        const R invRotMassB = iB * (j.a2 * j.a2) / R(1);
+        const R totalInvMass = mA + mB + invRotMassA + invRotMassB;
+        j.motorMass = (totalInvMass > R(0)) ? R(1) / totalInvMass : R(0);
+        {
+            j.ay = Rotate(RevPerp(j.localXAxis), qA);
+            j.s1 = Cross(d + rA, j.ay);
+            j.s2 = Cross(rB, j.ay);
+            const R invRotMassA2 = iA * (j.s1 * j.s1) / R(1);
+            const R invRotMassB2 = iB * (j.s2 * j.s2) / R(1);
+            const R k11 = mA + mB + invRotMassA2 + invRotMassB2;
+            const R k12 = (iA * j.s1 + iB * j.s2);
+            const R k13 = (iA * j.s1 * j.a1 + iB * j.s2 * j.a2) / R(1);
+            const R totalInvRotI = iA + iB;
+            const R k22 = (totalInvRotI == R(0)) ? R(1) : totalInvRotI;
+            const R k23 = (iA * j.a1 + iB * j.a2);
+            const R k33 = totalInvMass;
+            j.mass = M33<R>{V3<R>{k11, k12, k13}, V3<R>{k12, k22, k23}, V3<R>{k13, k23, k33}};
+        }
+        if (j.enableLimit) {
+            const R translation = Dot(j.ax, d);
+            if (std::abs(j.upper - j.lower) < (conf.linearSlop * R(2))) {
+                j.limitState = kLimitEqual;
+            } else if (translation <= j.lower) {
+                if (j.limitState != kLimitLower) {
+                    j.limitState = kLimitLower;
+                    j.impulse.z = 0;
+                }
+            } else if (translation >= j.upper) {
+                if (j.limitState != kLimitUpper) {
+                    j.limitState = kLimitUpper;
+                    j.impulse.z = 0;
+                }
+            } else {
+                j.limitState = kLimitInactive;
+                j.impulse.z = 0;
+            }
+        } else {
+            j.limitState = kLimitInactive;
+            j.impulse.z = 0;
+        }
+        if (!j.enableMotor) j.motorImpulse = 0;
+        if (conf.warmStart) {
+            j.impulse.x *= conf.dtRatio;
+            j.impulse.y *= conf.dtRatio;
+            j.impulse.z *= conf.dtRatio;
+            j.motorImpulse *= conf.dtRatio;
+            const V2<R> Px = j.impulse.x * j.ay;
+            const R Pxs1 = j.impulse.x * j.s1;
+            const R Pxs2 = j.impulse.x * j.s2;
+            const R PzLength = j.motorImpulse + j.impulse.z;
+            const V2<R> Pz = PzLength * j.ax;
+            const V2<R> P = Px + Pz;
+            const R L = j.impulse.y;
+            const R LA = L + (Pxs1 + PzLength * j.a1);
+            const R LB = L + (Pxs2 + PzLength * j.a2);
+            ApplyImpulse(A, B, P, LA, LB, velA, velB);
+        } else {
+            j.impulse = V3<R>{};
+            j.motorImpulse = 0;
+        }
+        A.vel = velA;
+        B.vel = velB;
+    }
+
+    // SolveVelocity(PrismaticJointConf&) (PrismaticJointConf.cpp:284-398).
+    static bool PrismaticSolveVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        const Vel<R> oldVelA = A.vel, oldVelB = B.vel;
+        Vel<R> velA = oldVelA, velB = oldVelB;
+        if (j.enableMotor && j.limitState != kLimitEqual) {
+            const R vDot = Dot(j.ax, velB.lin - velA.lin);
+            const R Cdot = vDot + (j.a2 * velB.ang - j.a1 * velA.ang);
+            R impulse = j.motorMass * (j.motorSpeed - Cdot);
+            const R oldImpulse = j.motorImpulse;
+            const R maxImpulse = conf.dt * j.maxMotorTorque;
+            j.motorImpulse = Clamp(j.motorImpulse + impulse, -maxImpulse, maxImpulse);
+            impulse = j.motorImpulse - oldImpulse;
+            const V2<R> P = impulse * j.ax;
+            ApplyImpulse(A, B, P, impulse * j.a1, impulse * j.a2, velA, velB);
+        }
+        const V2<R> velDelta = velB.lin - velA.lin;
+        const R sRotSpeed = (j.s2 * velB.ang - j.s1 * velA.ang);
+        const V2<R> Cdot1{Dot(j.ay, velDelta) + sRotSpeed, velB.ang - velA.ang};
+        if (j.enableLimit && j.limitState != kLimitInactive) {
+            const R deltaDot = Dot(j.ax, velDelta);
+            const R aRotSpeed = (j.a2 * velB.ang - j.a1 * velA.ang);
+            const R Cdot2 = deltaDot + aRotSpeed;
+            const V3<R> f1 = j.impulse;
+            const V3<R> s = Solve33(j.mass, V3<R>{-Cdot1.x, -Cdot1.y, -Cdot2});
+            j.impulse = V3<R>{j.impulse.x + s.x, j.impulse.y + s.y, j.impulse.z + s.z};
+            if (j.limitState == kLimitLower) j.impulse.z = Max(j.impulse.z, R(0));
+            else if (j.limitState == kLimitUpper) j.impulse.z = Min(j.impulse.z, R(0));
+            const R dz = j.impulse.z - f1.z;
+            const V2<R> b{-Cdot1.x - dz * j.mass.ez.x, -Cdot1.y - dz * j.mass.ez.y};
+            const V2<R> f2r = Solve22(j.mass, b) + V2<R>{f1.x, f1.y};
+            j.impulse.x = f2r.x;
+            j.impulse.y = f2r.y;
+            const V3<R> df{j.impulse.x - f1.x, j.impulse.y - f1.y, j.impulse.z - f1.z};
+            const V2<R> P = df.x * j.ay + df.z * j.ax;
+            const R LA = (df.x * j.s1 + df.y + df.z * j.a1);
+            const R LB = (df.x * j.s2 + df.y + df.z * j.a2);
+            ApplyImpulse(A, B, P, LA, LB, velA, velB);
+        } else {
+            const V2<R> df = Solve22(j.mass, V2<R>{-Cdot1.x, -Cdot1.y});
+            j.impulse.x += df.x;
+            j.impulse.y += df.y;
+            const V2<R> P = df.x * j.ay;
+            const R LA = (df.x * j.s1 + df.y);
+            const R LB = (df.x * j.s2 + df.y);
+            ApplyImpulse(A, B, P, LA, LB, velA, velB);
+        }
+        const bool changed = !(velA.lin == oldVelA.lin) || velA.ang != oldVelA.ang || !(velB.lin == oldVelB.lin) ||
+                             velB.ang != oldVelB.ang;
+        if (changed) {
+            A.vel = velA;
+            B.vel = velB;
+            return false;
+        }
+        return true;
+    }
+
+    // SolvePosition(const PrismaticJointConf&) (PrismaticJointConf.cpp:409-531).
+    static bool PrismaticSolvePosition(const Joint<R>& j, std::vector<BodyConstraint<R>>& bcs,
+                                       const StepConf<R>& conf) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        Pos<R> posA = A.pos, posB = B.pos;
+        const R mA = A.invMass, iA = A.invRotI, mB = B.invMass, iB = B.invRotI;
+        const V2<R> qA = UnitFromAngle(posA.ang);
+        const V2<R> qB = UnitFromAngle(posB.ang);
+        const V2<R> rA = Rotate(j.laA - A.lc, qA);
+        const V2<R> rB = Rotate(j.laB - B.lc, qB);
+        const V2<R> d = (posB.lin + rB) - (posA.lin + rA);
+        const V2<R> axis = Rotate(j.localXAxis, qA);
+        const R a1 = Cross(d + rA, axis);
+        const R a2 = Cross(rB, axis);
+        const V2<R> perp = Rotate(RevPerp(j.localXAxis), qA);
+        const R s1 = Cross(d + rA, perp);
+        const R s2 = Cross(rB, perp);
+        const V2<R> C1{Dot(perp, d), posB.ang - posA.ang - j.referenceAngle};
+        R linearError = std::abs(C1.x);
+        const R angularError = std::abs(C1.y);
+        bool active = false;
+        R C2 = 0;
+        if (j.enableLimit) {
+            const R translation = Dot(axis, d);
+            if (std::abs(j.upper - j.lower) < (R(2) * conf.linearSlop)) {
+                C2 = Clamp(translation, -conf.maxLinearCorrection, conf.maxLinearCorrection);
+                linearError = Max(linearError, std::abs(translation));
+                active = true;
+            } else if (translation <= j.lower) {
+                C2 = Clamp(translation - j.lower + conf.linearSlop, -conf.maxLinearCorrection, R(0));
+                linearError = Max(linearError, j.lower - translation);
+                active = true;
+            } else if (translation >= j.upper) {
+                C2 = Clamp(translation - j.upper - conf.linearSlop, R(0), conf.maxLinearCorrection);
+                linearError = Max(linearError, translation - j.upper);
+                active = true;
+            }
+        }
+        V3<R> impulse;
+        if (active) {
+            const R k11 = mA + iA * (s1 * s1) / R(1) + mB + iB * (s2 * s2) / R(1);
+            const R k12 = iA * s1 + iB * s2;
+            const R k13 = iA * s1 * a1 / R(1) + iB * s2 * a2 / R(1);
+            R k22 = iA + iB;
+            if (k22 == R(0)) k22 = R(1);
+            const R k23 = iA * a1 + iB * a2;
+            const R k33 = mA + iA * (a1 * a1) / R(1) + mB + iB * (a2 * a2) / R(1);
+            const M33<R> K{V3<R>{k11, k12, k13}, V3<R>{k12, k22, k23}, V3<R>{k13, k23, k33}};
+            impulse = Solve33(K, V3<R>{-C1.x, -C1.y, -C2});
+        } else {
+            const R k11 = mA + iA * s1 * s1 / R(1) + mB + iB * s2 * s2 / R(1);
+            const R k12 = iA * s1 + iB * s2;
+            R k22 = iA + iB;
+            if (k22 == R(0)) k22 = R(1);
+            // Solve(Mat22, Vec2) (Math.hpp:413-426) with the columns (k11,
+            // k12) and (k12, k22).
+            const R cp = k11 * k22 - k12 * k12;
+            const V2<R> b{-C1.x, -C1.y};
+            V2<R> sol{R(0), R(0)};
+            if (!AlmostZero(cp)) {
+                const R inverse = R(1) / cp;
+                sol = V2<R>{(k22 * b.x - k12 * b.y) * inverse, (k11 * b.y - k12 * b.x) * inverse};
+            }
+            impulse = V3<R>{sol.x, sol.y, R(0)};
+        }
+        const V2<R> P = impulse.x * perp + impulse.z * axis;
+        const R LA = (impulse.x * s1 + impulse.y + impulse.z * a1);
+        const R LB = (impulse.x * s2 + impulse.y + impulse.z * a2);
+        posA.lin = posA.lin - mA * P;
+        posA.ang = posA.ang - iA * LA;
+        posB.lin = posB.lin + mB * P;
+        posB.ang = posB.ang + iB * LB;
+        A.pos = posA;
+        B.pos = posB;
+        return (linearError <= conf.linearSlop) && (angularError <= conf.angularSlop);
+    }
+
     // InitVelocity(RevoluteJointConf&) (RevoluteJointConf.cpp:127-226).
     static void JointInitVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
         if (j.kind == 1) return DistanceInitVelocity(j, bcs, conf);
@@ -2263,6 +2482,7 @@ public:
         if (j.kind == 5) return MotorInitVelocity(j, bcs, conf);
         if (j.kind == 6) return WheelInitVelocity(j, bcs, conf);
         if (j.kind == 7) return PulleyInitVelocity(j, bcs, conf);
+        if (j.kind == 8) return PrismaticInitVelocity(j, bcs, conf);
         BodyConstraint<R>& A = bcs[j.bodyA];
         BodyConstraint<R>& B = bcs[j.bodyB];
         const R mA = A.invMass, iA = A.invRotI, aA = A.pos.ang;
@@ -2328,6 +2548,7 @@ public:
         if (j.kind == 5) return MotorSolveVelocity(j, bcs, conf);
         if (j.kind == 6) return WheelSolveVelocity(j, bcs, conf);
         if (j.kind == 7) return PulleySolveVelocity(j, bcs);
+        if (j.kind == 8) return PrismaticSolveVelocity(j, bcs, conf);
         BodyConstraint<R>& A = bcs[j.bodyA];
         BodyConstraint<R>& B = bcs[j.bodyB];
         const Vel<R> oldVelA = A.vel, oldVelB = B.vel;
@@ -2406,6 +2627,7 @@ public:
         if (j.kind == 5) return true;  // MotorJointConf.cpp:231-234
         if (j.kind == 6) return WheelSolvePosition(j, bcs, conf);
         if (j.kind == 7) return PulleySolvePosition(j, bcs, conf);
+        if (j.kind == 8) return PrismaticSolvePosition(j, bcs, conf);
         BodyConstraint<R>& A = bcs[j.bodyA];
         BodyConstraint<R>& B = bcs[j.bodyB];
         Pos<R> posA = A.pos, posB = B.pos;

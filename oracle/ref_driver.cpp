@@ -43,6 +43,7 @@
 #include <playrho/d2/MotorJointConf.hpp>
 #include <playrho/d2/WheelJointConf.hpp>
 #include <playrho/d2/PulleyJointConf.hpp>
+#include <playrho/d2/PrismaticJointConf.hpp>
 #include <playrho/d2/WorldJoint.hpp>
 #include <playrho/StepConf.hpp>
 
@@ -218,6 +219,16 @@ void WriteDescription(Out& o, const World& world) {
                                  GetY(j->localAnchorB),  GetX(j->groundAnchorA), GetY(j->groundAnchorA),
                                  GetX(j->groundAnchorB), GetY(j->groundAnchorB), j->ratio,
                                  j->lengthA,             j->lengthB,             0.0f};
+            o.put(f, sizeof(f));
+        } else if (const auto* j = TypeCast<const PrismaticJointConf>(&joint)) {
+            o.u32(to_underlying(GetBodyA(*j)));
+            o.u32(to_underlying(GetBodyB(*j)));
+            o.u32((j->enableLimit ? 1u : 0u) | (j->enableMotor ? 2u : 0u) | (j->collideConnected ? 4u : 0u) |
+                  (8u << 8));
+            const float f[12] = {GetX(j->localAnchorA), GetY(j->localAnchorA),  GetX(j->localAnchorB),
+                                 GetY(j->localAnchorB), j->referenceAngle,       j->lowerTranslation,
+                                 j->upperTranslation,   j->motorSpeed,           j->maxMotorForce,
+                                 GetX(j->localXAxisA),  GetY(j->localXAxisA),    0.0f};
             o.put(f, sizeof(f));
         } else if (const auto* j = TypeCast<const WheelJointConf>(&joint)) {
             o.u32(to_underlying(GetBodyA(*j)));
@@ -439,6 +450,10 @@ void WriteState(Out& o, const World& world, std::uint32_t step, const StepStats&
             o.u32(0u);
             const float f[4] = {j->impulse, 0.0f, 0.0f, 0.0f};
             o.put(f, sizeof(f));
+        } else if (const auto* j = TypeCast<const PrismaticJointConf>(&joint)) {
+            o.u32(std::uint32_t(j->limitState));
+            const float f[4] = {GetX(j->impulse), GetY(j->impulse), GetZ(j->impulse), j->motorImpulse};
+            o.put(f, sizeof(f));
         } else if (const auto* j = TypeCast<const WheelJointConf>(&joint)) {
             o.u32(0u);
             const float f[4] = {j->impulse, j->springImpulse, j->angularImpulse, 0.0f};
@@ -504,6 +519,13 @@ World BuildWorld(const prx_scene::SceneSpec& spec) {
             auto jc = PulleyJointConf{ba, bb, Length2{js.gax, js.gay}, Length2{js.gbx, js.gby},
                                       Length2{js.lax, js.lay}, Length2{js.lbx, js.lby}, js.length, js.lengthB};
             jc.UseRatio(js.ratio);
+            jc.UseCollideConnected(js.collideConnected);
+            CreateJoint(world, jc);
+        } else if (js.kind == prx_scene::kPrismatic) {
+            auto jc = PrismaticJointConf{ba, bb, Length2{js.lax, js.lay}, Length2{js.lbx, js.lby},
+                                         UnitVec::Get(js.axisAngle), js.referenceAngle};
+            jc.UseEnableLimit(js.enableLimit).UseLowerLength(js.lower).UseUpperLength(js.upper);
+            jc.UseEnableMotor(js.enableMotor).UseMotorSpeed(js.motorSpeed).UseMaxMotorForce(js.maxForce);
             jc.UseCollideConnected(js.collideConnected);
             CreateJoint(world, jc);
         } else if (js.kind == prx_scene::kWheel) {

@@ -233,6 +233,22 @@ class WheelJointConf(ctypes.Structure):
         super().__init__(**kw)
 
 
+class PrismaticJointConf(ctypes.Structure):
+    """PrismaticJointConf (d2/PrismaticJointConf.hpp:61-217); the reference's
+    defaults: axis (1, 0), reference angle 0, limit and motor off."""
+    _fields_ = [("body_a", ctypes.c_uint32), ("body_b", ctypes.c_uint32),
+                ("local_anchor_a", ctypes.c_float * 2), ("local_anchor_b", ctypes.c_float * 2),
+                ("local_axis_a", ctypes.c_float * 2), ("reference_angle", ctypes.c_float),
+                ("enable_limit", ctypes.c_uint32), ("lower_translation", ctypes.c_float),
+                ("upper_translation", ctypes.c_float), ("enable_motor", ctypes.c_uint32),
+                ("motor_speed", ctypes.c_float), ("max_motor_force", ctypes.c_float),
+                ("collide_connected", ctypes.c_uint32)]
+
+    def __init__(self, **kw):
+        kw.setdefault("local_axis_a", (1.0, 0.0))
+        super().__init__(**kw)
+
+
 class PulleyJointConf(ctypes.Structure):
     """PulleyJointConf (d2/PulleyJointConf.hpp:65-137); the reference's defaults:
     ground anchors (-1, 1), (1, 1), local anchors (-1, 0), (1, 0), lengths 0,
@@ -311,6 +327,8 @@ _SIGS = {
     "prx_motor_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
     "prx_wheel_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
     "prx_pulley_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
+    "prx_prismatic_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                                  ctypes.POINTER(ctypes.c_uint32)]),
     "prx_world_joint_count": (ctypes.c_uint32, [ctypes.c_void_p]),
     "prx_world_shape_count": (ctypes.c_uint32, [ctypes.c_void_p]),
     "prx_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
@@ -457,14 +475,15 @@ class World:
     def create_joint(self, conf):
         """CreateJoint with a RevoluteJointConf, DistanceJointConf,
         WeldJointConf, RopeJointConf, FrictionJointConf, MotorJointConf,
-        WheelJointConf or PulleyJointConf; returns the joint id
+        WheelJointConf, PulleyJointConf or PrismaticJointConf; returns the joint id
         (shared by all kinds)."""
         fn = {RevoluteJointConf: lib().prx_joint_create, DistanceJointConf: lib().prx_distance_joint_create,
               WeldJointConf: lib().prx_weld_joint_create, RopeJointConf: lib().prx_rope_joint_create,
               FrictionJointConf: lib().prx_friction_joint_create,
               MotorJointConf: lib().prx_motor_joint_create,
               WheelJointConf: lib().prx_wheel_joint_create,
-              PulleyJointConf: lib().prx_pulley_joint_create}.get(type(conf))
+              PulleyJointConf: lib().prx_pulley_joint_create,
+              PrismaticJointConf: lib().prx_prismatic_joint_create}.get(type(conf))
         if fn is None:
             raise TypeError("not a joint conf: %r" % type(conf))
         jid = ctypes.c_uint32()

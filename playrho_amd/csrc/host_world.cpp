@@ -700,6 +700,29 @@ int AddWheelJoint(prx_world* w, const prx_wheel_joint_conf& c, std::uint32_t* ou
     return AddJointRecord(w, j, out);
 }
 
+// PrismaticJointConf: anchors, (reference angle, lower, upper, motor speed),
+// the maximum motor force, and the local x axis of A (the revolute joint's
+// slots plus the wheel joint's axis).
+int AddPrismaticJoint(prx_world* w, const prx_prismatic_joint_conf& c, std::uint32_t* out) {
+    HostJoint j;
+    j.a = c.body_a;
+    j.b = c.body_b;
+    j.flags = (c.enable_limit ? prx::kJointLimit : 0u) | (c.enable_motor ? prx::kJointMotor : 0u) |
+              (c.collide_connected ? prx::kJointCollideConnected : 0u) | (prx::kJointPrismatic << 8);
+    j.la[0] = c.local_anchor_a[0];
+    j.la[1] = c.local_anchor_a[1];
+    j.la[2] = c.local_anchor_b[0];
+    j.la[3] = c.local_anchor_b[1];
+    j.cfg[0] = c.reference_angle;
+    j.cfg[1] = c.lower_translation;
+    j.cfg[2] = c.upper_translation;
+    j.cfg[3] = c.motor_speed;
+    j.maxTorque = c.max_motor_force;
+    j.ext[0] = c.local_axis_a[0];
+    j.ext[1] = c.local_axis_a[1];
+    return AddJointRecord(w, j, out);
+}
+
 // MotorJointConf: the linear offset takes the anchor A slot, then (angular
 // offset, maximum force, maximum torque, correction factor).
 int AddMotorJoint(prx_world* w, const prx_motor_joint_conf& c, std::uint32_t* out) {
@@ -1541,6 +1564,28 @@ int prx_scene_build(prx_world* w, const char* name, const std::int32_t* params, 
             if (int st = AddPulleyJoint(w, pc, nullptr)) return st;
             continue;
         }
+        if (js.kind == prx_scene::kPrismatic) {
+            prx_prismatic_joint_conf pc{};
+            pc.body_a = std::uint32_t(js.a);
+            pc.body_b = std::uint32_t(js.b);
+            pc.local_anchor_a[0] = js.lax;
+            pc.local_anchor_a[1] = js.lay;
+            pc.local_anchor_b[0] = js.lbx;
+            pc.local_anchor_b[1] = js.lby;
+            const V2 axis = UnitFromAngle(js.axisAngle);
+            pc.local_axis_a[0] = axis.x;
+            pc.local_axis_a[1] = axis.y;
+            pc.reference_angle = js.referenceAngle;
+            pc.enable_limit = js.enableLimit ? 1u : 0u;
+            pc.lower_translation = js.lower;
+            pc.upper_translation = js.upper;
+            pc.enable_motor = js.enableMotor ? 1u : 0u;
+            pc.motor_speed = js.motorSpeed;
+            pc.max_motor_force = js.maxForce;
+            pc.collide_connected = js.collideConnected ? 1u : 0u;
+            if (int st = AddPrismaticJoint(w, pc, nullptr)) return st;
+            continue;
+        }
         if (js.kind == prx_scene::kWheel) {
             prx_wheel_joint_conf wc{};
             wc.body_a = std::uint32_t(js.a);
@@ -1641,6 +1686,13 @@ int prx_pulley_joint_create(prx_world* w, const prx_pulley_joint_conf* conf, std
     if (!conf) return Fail(PRX_E_INVALID_ARGUMENT, "null joint conf");
     if (int st = Detach(w)) return st;
     return AddPulleyJoint(w, *conf, id);
+}
+
+int prx_prismatic_joint_create(prx_world* w, const prx_prismatic_joint_conf* conf, std::uint32_t* id) {
+    if (int st = CheckWorld(w)) return st;
+    if (!conf) return Fail(PRX_E_INVALID_ARGUMENT, "null joint conf");
+    if (int st = Detach(w)) return st;
+    return AddPrismaticJoint(w, *conf, id);
 }
 
 int prx_wheel_joint_create(prx_world* w, const prx_wheel_joint_conf* conf, std::uint32_t* id) {
@@ -1807,8 +1859,11 @@ int prx_world_load_desc(prx_world* w, const std::uint8_t* desc, std::size_t len)
         j.b = u32();
         j.flags = u32() & 0xff07u;
         if (j.a >= fresh.bodies.size() || j.b >= fresh.bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
-        if ((j.flags >> 8) > prx::kJointPulley) return Fail(PRX_E_INVALID_ARGUMENT, "unknown joint kind");
-        if (j.flags >> 8) j.flags &= ~(prx::kJointLimit | ((j.flags >> 8) == prx::kJointWheel ? 0u : prx::kJointMotor));
+        if ((j.flags >> 8) > prx::kJointPrismatic) return Fail(PRX_E_INVALID_ARGUMENT, "unknown joint kind");
+        // Limits belong to revolute and prismatic joints, motors to those and wheels.
+        const std::uint32_t kind = j.flags >> 8;
+        if (kind && kind != prx::kJointPrismatic)
+            j.flags &= ~(prx::kJointLimit | (kind == prx::kJointWheel ? 0u : prx::kJointMotor));
         for (float& v : j.la) v = f32();
         for (float& v : j.cfg) v = f32();
         j.maxTorque = f32();

@@ -288,7 +288,8 @@ enum : std::uint32_t {
     kJointFriction = 4u,
     kJointMotorKind = 5u,  // MotorJointConf (kJointMotor is the motor-enable flag)
     kJointWheel = 6u,
-    kJointPulley = 7u
+    kJointPulley = 7u,
+    kJointPrismatic = 8u
 };
 // Constraint records staged through LDS per chunk by the wave-wide sweeps;
 // the stage sits at the end of the dynamic LDS.

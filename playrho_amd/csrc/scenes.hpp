@@ -56,9 +56,12 @@ struct BodySpec {
 // maxTorque, correctionFactor), 6 WheelJointConf (anchors, axis at axisAngle,
 // frequency, dampingRatio, enableMotor, motorSpeed, maxMotorTorque), 7
 // PulleyJointConf (anchors, ground anchors gax, gay, gbx, gby, lengths
-// length, lengthB, ratio).
+// length, lengthB, ratio), 8 PrismaticJointConf (anchors, axis at axisAngle,
+// referenceAngle, enableLimit, lower, upper, enableMotor, motorSpeed,
+// maxForce as the maximum motor force).
 enum JointKind : int {
-    kRevolute = 0, kDistance = 1, kWeld = 2, kRope = 3, kFriction = 4, kMotor = 5, kWheel = 6, kPulley = 7
+    kRevolute = 0, kDistance = 1, kWeld = 2, kRope = 3, kFriction = 4, kMotor = 5, kWheel = 6, kPulley = 7,
+    kPrismatic = 8
 };
 struct JointSpec {
     int kind = kRevolute;
@@ -568,10 +571,125 @@ inline SceneSpec Joints(std::uint32_t seed, int w, int n = 8, int falling = 24) 
     return sc;
 }
 
+// Prismatic joints: `n` slabs slide on rails from the ground body (axes at
+// seeded angles; cycling through a translation limit, a motor, and both; the
+// last with equal limits), a carriage on a limited horizontal rail carries a
+// motorised vertical slider (a joint between two dynamic bodies), and
+// `falling` seeded boxes and disks drop onto them and onto a ground edge.
+inline SceneSpec Sliders(std::uint32_t seed, int w, int n = 6, int falling = 24) {
+    SceneSpec sc;
+    Rng rng(seed * 2654435761u + std::uint32_t(w) * 40503u + 29u);
+    sc.shapes.push_back(Edge(-50.0f, 0.0f, 50.0f, 0.0f));
+    BodySpec ground;
+    ground.shapes = {0};
+    sc.bodies.push_back(ground);
+    sc.shapes.push_back(Box(0.6f, 0.3f, 1.0f, 0.5f, 0.0f));
+    const int slab = int(sc.shapes.size()) - 1;
+    for (int i = 0; i < n; ++i) {
+        BodySpec b;
+        b.type = kDynamic;
+        b.x = float(6 * i) - 3.0f * float(n);
+        b.y = 4.0f + rng.uniform(0.0, 3.0);
+        b.angle = rng.uniform(-0.2, 0.2);
+        b.ay = -10.0f;
+        b.shapes = {slab};
+        sc.bodies.push_back(b);
+        JointSpec pj;
+        pj.kind = kPrismatic;
+        pj.a = 0;
+        pj.b = int(sc.bodies.size()) - 1;
+        pj.lax = b.x;
+        pj.lay = b.y;
+        pj.axisAngle = rng.uniform(0.0, 3.14159);
+        pj.referenceAngle = b.angle;
+        const int m = i % 3;
+        if (m != 1) {
+            pj.enableLimit = true;
+            pj.lower = -rng.uniform(0.5, 2.0);
+            pj.upper = rng.uniform(0.5, 2.0);
+        }
+        if (m != 0) {
+            pj.enableMotor = true;
+            pj.motorSpeed = rng.uniform(-3.0, 3.0);
+            pj.maxForce = rng.uniform(20.0, 200.0);
+        }
+        if (i == n - 1) {
+            pj.enableLimit = true;
+            pj.lower = 0.25f;
+            pj.upper = 0.25f;
+        }
+        sc.joints.push_back(pj);
+    }
+    {
+        sc.shapes.push_back(Box(1.5f, 0.3f, 1.0f, 0.6f, 0.0f));
+        const int carriageShape = int(sc.shapes.size()) - 1;
+        sc.shapes.push_back(Box(0.4f, 0.4f, 1.0f, 0.6f, 0.0f));
+        const int sliderShape = int(sc.shapes.size()) - 1;
+        BodySpec cb;
+        cb.type = kDynamic;
+        cb.x = 0.0f;
+        cb.y = 12.0f;
+        cb.ay = -10.0f;
+        cb.shapes = {carriageShape};
+        sc.bodies.push_back(cb);
+        const int carriage = int(sc.bodies.size()) - 1;
+        JointSpec rail;
+        rail.kind = kPrismatic;
+        rail.a = 0;
+        rail.b = carriage;
+        rail.lax = 0.0f;
+        rail.lay = 12.0f;
+        rail.enableLimit = true;
+        rail.lower = -8.0f;
+        rail.upper = 8.0f;
+        sc.joints.push_back(rail);
+        BodySpec sb;
+        sb.type = kDynamic;
+        sb.x = 0.5f;
+        sb.y = 13.0f;
+        sb.ay = -10.0f;
+        sb.shapes = {sliderShape};
+        sc.bodies.push_back(sb);
+        JointSpec up;
+        up.kind = kPrismatic;
+        up.a = carriage;
+        up.b = int(sc.bodies.size()) - 1;
+        up.lax = 0.5f;
+        up.lay = 1.0f;
+        up.axisAngle = 1.5707963f;
+        up.enableLimit = true;
+        up.lower = -0.5f;
+        up.upper = 2.5f;
+        up.enableMotor = true;
+        up.motorSpeed = 1.5f;
+        up.maxForce = 30.0f;
+        sc.joints.push_back(up);
+    }
+    for (int k = 0; k < falling; ++k) {
+        BodySpec b;
+        b.type = kDynamic;
+        b.x = rng.uniform(-20.0, 20.0);
+        b.y = rng.uniform(15.0, 40.0);
+        b.ay = -10.0f;
+        const float friction = rng.uniform(0.2, 0.8);
+        if (rng.below(2) == 0) {
+            const float hx = rng.uniform(0.3, 0.6);
+            const float hy = rng.uniform(0.3, 0.6);
+            sc.shapes.push_back(Box(hx, hy, 1.0f, friction, 0.0f));
+        } else {
+            sc.shapes.push_back(Disk(rng.uniform(0.3, 0.6), 1.0f, friction, 0.0f));
+        }
+        b.shapes = {int(sc.shapes.size()) - 1};
+        sc.bodies.push_back(b);
+    }
+    return sc;
+}
+
 // Named scenes with integer parameters, as used by tests and benchmarks.
 //   "hello", "pyramid"(rows), "drop"(n, half, height, seed),
 //   "pile"(seed, w, nmin, nmax), "billiards"(seed, w),
-//   "chain"(seed, w, links, falling, torque), "joints"(seed, w, n, falling)
+//   "chain"(seed, w, links, falling, torque), "joints"(seed, w, n, falling),
+//   "sliders"(seed, w, n, falling)
 inline bool Make(const std::string& name, const int* p, int np, SceneSpec& out) {
     auto arg = [&](int i, int dflt) { return (i < np) ? p[i] : dflt; };
     if (name == "hello") {
@@ -588,6 +706,8 @@ inline bool Make(const std::string& name, const int* p, int np, SceneSpec& out) 
         out = Chain(std::uint32_t(arg(0, 1)), arg(1, 0), arg(2, 32), arg(3, 64), float(arg(4, 2)));
     } else if (name == "joints") {
         out = Joints(std::uint32_t(arg(0, 1)), arg(1, 0), arg(2, 8), arg(3, 24));
+    } else if (name == "sliders") {
+        out = Sliders(std::uint32_t(arg(0, 1)), arg(1, 0), arg(2, 6), arg(3, 24));
     } else {
         return false;
     }

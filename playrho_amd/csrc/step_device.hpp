@@ -1663,6 +1663,169 @@ __device__ __forceinline__ bool wheel_solve_velocity(float4* u, float4& velA, fl
     return same;
 }
 
+// Prismatic record: u[0] axis x, y, perp x, y; u[1] s1, s2, a1, a2; u[2].xy
+// motor speed, dt * maximum motor force, u[2].w flags | limit state << 8 |
+// kind << 16; u[3] impulse x, y, z, motor impulse; u[5].x upper translation;
+// u[7] local x axis of A, reference angle, lower translation.  K and the motor
+// mass are recomputed from u[1] and u[4] where used, by InitVelocity's own
+// operations (PrismaticJointConf.cpp:196-230).
+struct PrisK {
+    float k11, k12, k13, k22, k23, k33;
+};
+__device__ __forceinline__ PrisK prismatic_k(float4 ms, float s1, float s2, float a1, float a2) {
+    const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
+    const float invRotMassA = iA * (a1 * a1) / 1.0f;
+    const float invRotMassB = iB * (a2 * a2) / 1.0f;
+    const float invRotMassA2 = iA * (s1 * s1) / 1.0f;
+    const float invRotMassB2 = iB * (s2 * s2) / 1.0f;
+    PrisK k;
+    k.k11 = mA + mB + invRotMassA2 + invRotMassB2;
+    k.k12 = iA * s1 + iB * s2;
+    k.k13 = (iA * s1 * a1 + iB * s2 * a2) / 1.0f;
+    const float tot = iA + iB;
+    k.k22 = (tot == 0.0f) ? 1.0f : tot;
+    k.k23 = iA * a1 + iB * a2;
+    k.k33 = mA + mB + invRotMassA + invRotMassB;
+    return k;
+}
+__device__ __forceinline__ Jc prismatic_jc(const PrisK& k) {
+    Jc j;
+    j.exx = k.k11;
+    j.eyx = k.k12;
+    j.ezx = k.k13;
+    j.eyy = k.k22;
+    j.ezy = k.k23;
+    j.ezz = k.k33;
+    return j;
+}
+
+// InitVelocity(PrismaticJointConf&) less its warm start (PrismaticJointConf.cpp:155-264).
+__device__ __forceinline__ void build_jc_prismatic(float4* u, const float4 cf, const float4 cf2, float4 im, V2 rA,
+                                                   V2 rB, V2 qA, const float4& pA, const float4& pB,
+                                                   std::uint32_t flags, std::uint32_t limit, const StepParams& c) {
+    const V2 axisA{cf2.z, cf2.w};
+    const V2 d = ((V2{pB.x, pB.y} - V2{pA.x, pA.y}) + rB) - rA;
+    const V2 axis = rotate(axisA, qA);
+    const float a1 = cross(d + rA, axis);
+    const float a2 = cross(rB, axis);
+    const V2 perp = rotate(revperp(axisA), qA);
+    const float s1 = cross(d + rA, perp);
+    const float s2 = cross(rB, perp);
+    if (flags & prx::kJointLimit) {
+        const float translation = dot(axis, d);
+        if (__builtin_fabsf(cf.z - cf.y) < (c.linearSlop * 2.0f)) {
+            limit = 3u;
+        } else if (translation <= cf.y) {
+            if (limit != 1u) {
+                limit = 1u;
+                im.z = 0.0f;
+            }
+        } else if (translation >= cf.z) {
+            if (limit != 2u) {
+                limit = 2u;
+                im.z = 0.0f;
+            }
+        } else {
+            limit = 0u;
+            im.z = 0.0f;
+        }
+    } else {
+        limit = 0u;
+        im.z = 0.0f;
+    }
+    if (!(flags & prx::kJointMotor)) im.w = 0.0f;
+    im = c.warmStart ? float4{im.x * c.dtRatio, im.y * c.dtRatio, im.z * c.dtRatio, im.w * c.dtRatio}
+                     : float4{0.0f, 0.0f, 0.0f, 0.0f};
+    u[0] = float4{axis.x, axis.y, perp.x, perp.y};
+    u[1] = float4{s1, s2, a1, a2};
+    u[2] = float4{cf.w, c.dt * cf2.x, 0.0f, __uint_as_float(flags | (limit << 8) | (prx::kJointPrismatic << 16))};
+    u[3] = im;
+    u[5].x = cf.z;
+    u[7] = float4{axisA.x, axisA.y, cf.x, cf.y};
+}
+
+// Warm start of InitVelocity (PrismaticJointConf.cpp:241-262).
+__device__ __forceinline__ void prismatic_warm_start(const float4* u, float4& velA, float4& velB) {
+    const float4 a = u[0], s = u[1], im = u[3];
+    const V2 axis{a.x, a.y}, perp{a.z, a.w};
+    const V2 Px = mul(im.x, perp);
+    const float Pxs1 = im.x * s.x;
+    const float Pxs2 = im.x * s.y;
+    const float PzLength = im.w + im.z;
+    const V2 Pz = mul(PzLength, axis);
+    const V2 P = Px + Pz;
+    const float L = im.y;
+    const float LA = L + (Pxs1 + PzLength * s.z);
+    const float LB = L + (Pxs2 + PzLength * s.w);
+    apply_joint_impulse(u[4], P, LA, LB, velA, velB);
+}
+
+// SolveVelocity(PrismaticJointConf&) (PrismaticJointConf.cpp:270-392):
+// returns whether the bodies' velocities were left unchanged.
+__device__ __forceinline__ bool prismatic_solve_velocity(float4* u, float4& velA, float4& velB) {
+    const float4 a = u[0], s = u[1], mo = u[2], ms = u[4];
+    const V2 axis{a.x, a.y}, perp{a.z, a.w};
+    const float s1 = s.x, s2 = s.y, a1 = s.z, a2 = s.w;
+    const std::uint32_t word = __float_as_uint(mo.w);
+    const std::uint32_t flags = word & 0xffu, limit = (word >> 8) & 0xffu;
+    const PrisK K = prismatic_k(ms, s1, s2, a1, a2);
+    const Jc kj = prismatic_jc(K);
+    const float4 oldA = velA, oldB = velB;
+    float4 im = u[3];
+    if ((flags & prx::kJointMotor) && limit != 3u) {
+        const float vDot = dot(axis, V2{velB.x, velB.y} - V2{velA.x, velA.y});
+        const float Cdot = vDot + (a2 * velB.z - a1 * velA.z);
+        const float motorMass = (K.k33 > 0.0f) ? 1.0f / K.k33 : 0.0f;
+        float impulse = motorMass * (mo.x - Cdot);
+        const float old = im.w;
+        im.w = clamp_std(im.w + impulse, -mo.y, mo.y);
+        impulse = im.w - old;
+        apply_joint_impulse(ms, mul(impulse, axis), impulse * a1, impulse * a2, velA, velB);
+    }
+    const V2 velDelta = V2{velB.x, velB.y} - V2{velA.x, velA.y};
+    const float sRotSpeed = (s2 * velB.z - s1 * velA.z);
+    const V2 Cdot1{dot(perp, velDelta) + sRotSpeed, velB.z - velA.z};
+    if ((flags & prx::kJointLimit) && limit != 0u) {
+        const float deltaDot = dot(axis, velDelta);
+        const float aRotSpeed = (a2 * velB.z - a1 * velA.z);
+        const float Cdot2 = deltaDot + aRotSpeed;
+        const float4 f1 = im;
+        float sx, sy, sz;
+        solve33(kj, -Cdot1.x, -Cdot1.y, -Cdot2, sx, sy, sz);
+        im.x = im.x + sx;
+        im.y = im.y + sy;
+        im.z = im.z + sz;
+        if (limit == 1u) im.z = fmax_std(im.z, 0.0f);
+        else if (limit == 2u) im.z = fmin_std(im.z, 0.0f);
+        const float dz = im.z - f1.z;
+        const V2 b{-Cdot1.x - dz * K.k13, -Cdot1.y - dz * K.k23};
+        const V2 f2r = solve22(kj, b) + V2{f1.x, f1.y};
+        im.x = f2r.x;
+        im.y = f2r.y;
+        const float dfx = im.x - f1.x, dfy = im.y - f1.y, dfz = im.z - f1.z;
+        const V2 P = mul(dfx, perp) + mul(dfz, axis);
+        const float LA = (dfx * s1 + dfy + dfz * a1);
+        const float LB = (dfx * s2 + dfy + dfz * a2);
+        apply_joint_impulse(ms, P, LA, LB, velA, velB);
+    } else {
+        const V2 df = solve22(kj, V2{-Cdot1.x, -Cdot1.y});
+        im.x = im.x + df.x;
+        im.y = im.y + df.y;
+        const V2 P = mul(df.x, perp);
+        const float LA = (df.x * s1 + df.y);
+        const float LB = (df.x * s2 + df.y);
+        apply_joint_impulse(ms, P, LA, LB, velA, velB);
+    }
+    u[3] = im;
+    const bool same = velA.x == oldA.x && velA.y == oldA.y && velA.z == oldA.z && velB.x == oldB.x &&
+                      velB.y == oldB.y && velB.z == oldB.z;
+    if (same) {
+        velA = oldA;
+        velB = oldB;
+    }
+    return same;
+}
+
 // Pulley record: u[1] uA x, y, uB x, y; u[2].xy mass, ratio; u[5].xy lengths
 // A, B; u[7] ground anchors A, B; impulse in u[3].x.
 // InitVelocity(PulleyJointConf&) less its warm start (PulleyJointConf.cpp:92-141).
@@ -1876,6 +2039,8 @@ __device__ __forceinline__ void spring_joint_warm_start(const float4* u, std::ui
         apply_joint_impulse(ms, P, cross(rA, P) / 1.0f, cross(rB, P) / 1.0f, velA, velB);
     } else if (kind == prx::kJointPulley) {
         pulley_apply(u, im.x, velA, velB);
+    } else if (kind == prx::kJointPrismatic) {
+        prismatic_warm_start(u, velA, velB);
     } else if (kind == prx::kJointWheel) {
         // WheelJointConf.cpp:212-223.
         const float4 a = u[0], s = u[1];
@@ -1998,7 +2163,70 @@ __device__ __forceinline__ bool spring_joint_solve_position(const float4* u, std
     V2 P;
     float LA, LB;
     bool ok;
-    if (kind == prx::kJointWheel) {
+    if (kind == prx::kJointPrismatic) {
+        // PrismaticJointConf.cpp:409-531.
+        const V2 axisA{cf.x, cf.y};
+        const float lower = cf.w, upper = u[5].x;
+        const V2 axis = rotate(axisA, qA);
+        const float a1 = cross(d + rA, axis);
+        const float a2 = cross(rB, axis);
+        const V2 perp = rotate(revperp(axisA), qA);
+        const float s1 = cross(d + rA, perp);
+        const float s2 = cross(rB, perp);
+        const V2 C1{dot(perp, d), posB.z - posA.z - cf.z};
+        float linearError = __builtin_fabsf(C1.x);
+        const float angularError = __builtin_fabsf(C1.y);
+        bool active = false;
+        float C2 = 0.0f;
+        if (__float_as_uint(u[2].w) & prx::kJointLimit) {
+            const float translation = dot(axis, d);
+            if (__builtin_fabsf(upper - lower) < (2.0f * c.linearSlop)) {
+                C2 = clamp_std(translation, -c.maxLinearCorrection, c.maxLinearCorrection);
+                linearError = fmax_std(linearError, __builtin_fabsf(translation));
+                active = true;
+            } else if (translation <= lower) {
+                C2 = clamp_std(translation - lower + c.linearSlop, -c.maxLinearCorrection, 0.0f);
+                linearError = fmax_std(linearError, lower - translation);
+                active = true;
+            } else if (translation >= upper) {
+                C2 = clamp_std(translation - upper - c.linearSlop, 0.0f, c.maxLinearCorrection);
+                linearError = fmax_std(linearError, translation - upper);
+                active = true;
+            }
+        }
+        float ix, iy, iz;
+        if (active) {
+            Jc kj;
+            kj.exx = mA + iA * (s1 * s1) / 1.0f + mB + iB * (s2 * s2) / 1.0f;
+            kj.eyx = iA * s1 + iB * s2;
+            kj.ezx = iA * s1 * a1 / 1.0f + iB * s2 * a2 / 1.0f;
+            const float k22 = iA + iB;
+            kj.eyy = (k22 == 0.0f) ? 1.0f : k22;
+            kj.ezy = iA * a1 + iB * a2;
+            kj.ezz = mA + iA * (a1 * a1) / 1.0f + mB + iB * (a2 * a2) / 1.0f;
+            solve33(kj, -C1.x, -C1.y, -C2, ix, iy, iz);
+        } else {
+            const float k11 = mA + iA * s1 * s1 / 1.0f + mB + iB * s2 * s2 / 1.0f;
+            const float k12 = iA * s1 + iB * s2;
+            float k22 = iA + iB;
+            if (k22 == 0.0f) k22 = 1.0f;
+            // Solve(Mat22, Vec2) (Math.hpp:413-426), columns (k11, k12) and (k12, k22).
+            const float cp = k11 * k22 - k12 * k12;
+            const V2 bb{-C1.x, -C1.y};
+            ix = 0.0f;
+            iy = 0.0f;
+            if (!almost_zero(cp)) {
+                const float inverse = 1.0f / cp;
+                ix = (k22 * bb.x - k12 * bb.y) * inverse;
+                iy = (k11 * bb.y - k12 * bb.x) * inverse;
+            }
+            iz = 0.0f;
+        }
+        P = mul(ix, perp) + mul(iz, axis);
+        LA = (ix * s1 + iy + iz * a1);
+        LB = (ix * s2 + iy + iz * a2);
+        ok = (linearError <= c.linearSlop) && (angularError <= c.angularSlop);
+    } else if (kind == prx::kJointWheel) {
         // WheelJointConf.cpp:309-359; the mass uses InitVelocity's sAy, sBy.
         const V2 dw = (V2{posB.x, posB.y} - V2{posA.x, posA.y}) + (rB - rA);
         const V2 ay = rotate(revperp(V2{cf.z, cf.w}), qA);
@@ -2103,6 +2331,8 @@ __device__ __forceinline__ void build_jc(const DeviceState& d, const WorldCtx& x
             build_jc_wheel(u, cf, cf2, im, rA, rB, qA, pA, pB, mA, iA, mB, iB, flags & 0xffu, c);
         else if (kind == prx::kJointPulley)
             build_jc_pulley(u, cf, cf2, im, rA, rB, pA, pB, mA, iA, mB, iB, flags & 0xffu, c);
+        else if (kind == prx::kJointPrismatic)
+            build_jc_prismatic(u, cf, cf2, im, rA, rB, qA, pA, pB, flags & 0xffu, limit, c);
         else {
             // Body B's anchor is its origin: rB = Rotate(-localCenterB, qB).
             const V2 rBm = rotate(V2{-B.lc.x, -B.lc.y}, qB);
@@ -2725,6 +2955,7 @@ __device__ __forceinline__ bool joints_velocity_k(float4* jbase, int j0, int j1,
             else if (kind == prx::kJointFriction) ok &= friction_solve_velocity(u, va, vb);
             else if (kind == prx::kJointWheel) ok &= wheel_solve_velocity(u, va, vb);
             else if (kind == prx::kJointPulley) ok &= pulley_solve_velocity(u, va, vb);
+            else if (kind == prx::kJointPrismatic) ok &= prismatic_solve_velocity(u, va, vb);
             else ok &= motor_solve_velocity(u, va, vb);
             bs.set_vel(bA, va);
             bs.set_vel(bB, vb);

@@ -16,6 +16,7 @@
 #include <playrho/d2/FrictionJointConf.hpp>
 #include <playrho/d2/MotorJointConf.hpp>
 #include <playrho/d2/PolygonShapeConf.hpp>
+#include <playrho/d2/PrismaticJointConf.hpp>
 #include <playrho/d2/PulleyJointConf.hpp>
 #include <playrho/d2/RevoluteJointConf.hpp>
 #include <playrho/d2/RopeJointConf.hpp>
@@ -653,8 +654,25 @@ JointID CreateJoint(GpuWorld& g, const Joint& def) {
         c.ratio = F(j->ratio);
         c.collide_connected = cc;
         Check(prx_pulley_joint_create(g.m_world, &c, &id));
+    } else if (const auto* j = TypeCast<const PrismaticJointConf>(&def)) {
+        prx_prismatic_joint_conf c{};
+        c.body_a = bA;
+        c.body_b = bB;
+        xy(j->localAnchorA, c.local_anchor_a);
+        xy(j->localAnchorB, c.local_anchor_b);
+        c.local_axis_a[0] = F(GetX(j->localXAxisA));
+        c.local_axis_a[1] = F(GetY(j->localXAxisA));
+        c.reference_angle = F(Real(j->referenceAngle / Radian));
+        c.enable_limit = j->enableLimit ? 1u : 0u;
+        c.lower_translation = F(Real(j->lowerTranslation / Meter));
+        c.upper_translation = F(Real(j->upperTranslation / Meter));
+        c.enable_motor = j->enableMotor ? 1u : 0u;
+        c.motor_speed = F(Real(j->motorSpeed / RadianPerSecond));
+        c.max_motor_force = F(Real(j->maxMotorForce / Newton));
+        c.collide_connected = cc;
+        Check(prx_prismatic_joint_create(g.m_world, &c, &id));
     } else {
-        Unsupported("this joint type (prismatic, gear and target joints)");
+        Unsupported("this joint type (gear and target joints)");
     }
     g.m_joints.push_back(def);
     return JointID(static_cast<JointID::underlying_type>(id));

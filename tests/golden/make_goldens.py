@@ -47,6 +47,10 @@ CASES = [
     # level form of the wave-wide sweeps on the device).
     ("pile3301", "pile", 300, [1, 3301], [1, 300]),
     ("pile859", "pile", 300, [1, 859], [1, 300]),
+    # Prismatic joints: limits, motors, both, equal limits, a carriage with a
+    # motorised slider between two dynamic bodies, beside falling bodies.
+    ("sliders0", "sliders", 240, [1, 0], [1, 120, 240]),
+    ("sliders1", "sliders", 240, [2, 1, 9, 40], [1, 240]),
 ]
 # World edits between steps (step, kind, body, a, b, c): SetLocation 'L',
 # SetAngle 'A', SetTransformation 'T' (c the rotation's angle), ShiftOrigin 'O'.
@@ -70,6 +74,7 @@ TOI_CASES = [
     ("joints0_toi", "joints", 240, [1, 0], [1, 240]),
     ("pile3301_toi", "pile", 300, [1, 3301], [1, 300]),
     ("pile859_toi", "pile", 300, [1, 859], [1, 300]),
+    ("sliders0_toi", "sliders", 240, [1, 0], [1, 240]),
 ]
 
 

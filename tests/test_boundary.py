@@ -264,3 +264,29 @@ def test_non_revolute_joint_creation_matches_scene():
     assert (m.max_force, m.max_torque, round(m.correction_factor, 6)) == (1.0, 1.0, 0.3)
     wh = pa.WheelJointConf()
     assert (tuple(wh.local_axis_a), wh.frequency, round(wh.damping_ratio, 6)) == ((1.0, 0.0), 2.0, 0.7)
+
+
+def test_prismatic_joint_creation_matches_scene():
+    """Prismatic joints created through prx_prismatic_joint_create give the
+    same world description as the scene the reference built (kind 8; limit,
+    motor and collide-connected flags; reference angle, lower and upper
+    translation, motor speed, maximum motor force, local x axis of A)."""
+    g = golden_cases.load("sliders0")
+    desc = _desc_only(g["desc_conf"])
+    recs = dumpfmt.read_description(dumpfmt.Reader(desc))[2]
+    assert {r["flags"] >> 8 for r in recs} == {8}
+    assert any(r["flags"] & 1 for r in recs) and any(r["flags"] & 2 for r in recs)
+    offs = pa.desc_joint_offsets(desc)
+    bare = desc[:offs[0] - 4] + struct.pack("<I", 0)
+    w = pa.World.from_desc(bare)
+    for r in recs:
+        v = [float(x) for x in r["vals"]]
+        w.create_joint(pa.PrismaticJointConf(
+            body_a=r["a"], body_b=r["b"], local_anchor_a=(v[0], v[1]), local_anchor_b=(v[2], v[3]),
+            reference_angle=v[4], enable_limit=r["flags"] & 1, lower_translation=v[5], upper_translation=v[6],
+            enable_motor=(r["flags"] >> 1) & 1, motor_speed=v[7], max_motor_force=v[8], local_axis_a=(v[9], v[10]),
+            collide_connected=(r["flags"] >> 2) & 1))
+    assert w.joint_count == len(recs)
+    assert w.save_desc() == desc
+    p = pa.PrismaticJointConf()
+    assert (tuple(p.local_axis_a), p.enable_limit, p.enable_motor) == ((1.0, 0.0), 0, 0)
