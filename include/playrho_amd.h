@@ -193,6 +193,18 @@ typedef struct prx_prismatic_joint_conf {
     uint32_t collide_connected;
 } prx_prismatic_joint_conf;
 
+/* TargetJointConf (d2/TargetJointConf.hpp:56-161): pulls a point of one body
+ * (the local anchor) towards a world target with a soft spring (frequency,
+ * damping ratio) whose impulse is limited by dt * max_force; the joint has no
+ * body A.  The reference's defaults are frequency 5, damping ratio 0.7,
+ * maximum force 0; negative or NaN values are PRX_E_INVALID_ARGUMENT. */
+typedef struct prx_target_joint_conf {
+    uint32_t body;
+    float target[2], local_anchor[2];
+    float max_force, frequency, damping_ratio;
+    uint32_t collide_connected;
+} prx_target_joint_conf;
+
 /* StepConf fields the regular step reads (StepConf.hpp:49-392). */
 typedef struct prx_step_conf {
     float delta_time, dt_ratio, min_still_time_to_sleep, linear_slop, angular_slop;
@@ -282,6 +294,8 @@ int prx_wheel_joint_create(prx_world* world, const prx_wheel_joint_conf* conf, u
 /* CreateJoint(World&, const PulleyJointConf&) (solver PulleyJointConf.cpp:92-253;
  * prx_world_shift_origin moves its ground anchors, PulleyJointConf.cpp:255-260). */
 int prx_pulley_joint_create(prx_world* world, const prx_pulley_joint_conf* conf, uint32_t* joint_id);
+/* CreateJoint(World&, const TargetJointConf&) (solver TargetJointConf.cpp:60-184). */
+int prx_target_joint_create(prx_world* world, const prx_target_joint_conf* conf, uint32_t* joint_id);
 /* CreateJoint(World&, const PrismaticJointConf&) (solver PrismaticJointConf.cpp:155-531). */
 int prx_prismatic_joint_create(prx_world* world, const prx_prismatic_joint_conf* conf, uint32_t* joint_id);
 uint32_t prx_world_joint_count(const prx_world* world);

@@ -120,6 +120,15 @@ void Load(Handle<R>& h, In& in) {
             // Prismatic: reference angle, lower and upper translation, motor
             // speed, maximum motor force (in maxMotorTorque), local x axis of A.
             j.localXAxis = V2<R>{f[9], f[10]};
+        } else if (j.kind == 9) {
+            // Target: no body A; anchor on B, target, maximum force,
+            // frequency, damping ratio.
+            j.enableLimit = j.enableMotor = false;
+            j.groundA = V2<R>{f[4], f[5]};
+            j.maxForce = f[6];
+            j.frequency = f[7];
+            j.dampingRatio = f[8];
+            j.referenceAngle = j.lower = j.upper = j.motorSpeed = j.maxMotorTorque = 0;
         } else if (j.kind != 0) {
             // Distance: length, frequency, damping ratio.  Weld: reference
             // angle, frequency, damping ratio.  Rope: maximum length.

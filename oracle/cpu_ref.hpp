@@ -807,7 +807,14 @@ struct Joint {
     // lower, upper; the maximum motor force in maxMotorTorque.  Solver data:
     // axis in ax, perp in ay, s1, s2, a1, a2, K in mass, motorMass.
     R s1 = 0, s2 = 0, a1 = 0, a2 = 0, motorMass = 0;
+    // kind 9 target: no body A (bodyA = kNoBody); the target in groundA, the
+    // anchor on B in laB, maxForce, frequency, dampingRatio.  Solver data: rB,
+    // gamma, the effective mass in lm00..lm11, C in linearError; the impulse
+    // in impulse.x, y.
 };
+
+// InvalidBodyID's index (BodyID.hpp:50): a target joint's body A.
+constexpr std::uint32_t kNoBody = 0xffffu;
 
 template <class R>
 class World {
@@ -2473,6 +2480,85 @@ public:
         return (linearError <= conf.linearSlop) && (angularError <= conf.angularSlop);
     }
 
+    // InitVelocity(TargetJointConf&) (TargetJointConf.cpp:85-141), with
+    // GetEffectiveMassMatrix (TargetJointConf.cpp:60-83).
+    static void TargetInitVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        Vel<R> velB = B.vel;
+        const V2<R> qB = UnitFromAngle(B.pos.ang);
+        const R mass = (B.invMass != R(0)) ? (R(1) / B.invMass) : R(0);
+        const R omega = R(2) * kPi * j.frequency;
+        const R d = R(2) * mass * j.dampingRatio * omega;
+        const R k = mass * (omega * omega);
+        const R h = conf.dt;
+        const R tmp = d + h * k;
+        const R invGamma = h * tmp;
+        j.gamma = (invGamma != R(0)) ? R(1) / invGamma : R(0);
+        const R beta = h * k * j.gamma;
+        j.rB = Rotate(j.laB - B.lc, qB);
+        {
+            const R invMass = B.invMass, invRotI = B.invRotI;
+            const R exx = invMass + (invRotI * (j.rB.y * j.rB.y) / R(1)) + j.gamma;
+            const R exy = -invRotI * j.rB.x * j.rB.y / R(1);
+            const R eyy = invMass + (invRotI * (j.rB.x * j.rB.x) / R(1)) + j.gamma;
+            // Invert(Matrix22) (Math.hpp:429-442) of columns (exx, exy), (exy, eyy).
+            const R cp = Cross(V2<R>{exx, exy}, V2<R>{exy, eyy});
+            if (!AlmostZero(cp)) {
+                const R inverse = R(1) / cp;
+                j.lm00 = eyy * inverse;
+                j.lm01 = -exy * inverse;
+                j.lm10 = -exy * inverse;
+                j.lm11 = exx * inverse;
+            } else {
+                j.lm00 = j.lm01 = j.lm10 = j.lm11 = 0;
+            }
+        }
+        j.linearError = ((B.pos.lin + j.rB) - j.groundA) * beta;
+        velB.ang *= R(0.98f);
+        if (conf.warmStart) {
+            j.impulse.x *= conf.dtRatio;
+            j.impulse.y *= conf.dtRatio;
+            const V2<R> P{j.impulse.x, j.impulse.y};
+            const R crossBP = Cross(j.rB, P) / R(1);
+            velB.lin = velB.lin + B.invMass * P;
+            velB.ang = velB.ang + B.invRotI * crossBP;
+        } else {
+            j.impulse = V3<R>{};
+        }
+        B.vel = velB;
+    }
+
+    // SolveVelocity(TargetJointConf&) (TargetJointConf.cpp:143-178).
+    static bool TargetSolveVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        Vel<R> velB = B.vel;
+        const V2<R> Cdot = velB.lin + RevPerp(j.rB) * (velB.ang / R(1));
+        const V2<R> ev = Cdot + (j.linearError + j.gamma * V2<R>{j.impulse.x, j.impulse.y});
+        const V2<R> old{j.impulse.x, j.impulse.y};
+        // Transform(-ev, mass): the matrix's vectors dotted with -ev from zero
+        // (Vector.hpp:455-470).
+        const V2<R> nev = -ev;
+        R tx = 0, ty = 0;
+        tx += j.lm00 * nev.x;
+        tx += j.lm01 * nev.y;
+        ty += j.lm10 * nev.x;
+        ty += j.lm11 * nev.y;
+        V2<R> imp{old.x + tx, old.y + ty};
+        const R maxImpulse = conf.dt * j.maxForce;
+        if (MagSq(imp) > maxImpulse * maxImpulse) {
+            const V2<R> dir = UnitVector(imp, V2<R>{R(0), R(0)});
+            imp = V2<R>{dir.x * maxImpulse, dir.y * maxImpulse};
+        }
+        j.impulse.x = imp.x;
+        j.impulse.y = imp.y;
+        const V2<R> inc = imp - old;
+        const R angImpulseB = Cross(j.rB, inc) / R(1);
+        velB.lin = velB.lin + B.invMass * inc;
+        velB.ang = velB.ang + B.invRotI * angImpulseB;
+        B.vel = velB;
+        return inc.x == R(0) && inc.y == R(0);
+    }
+
     // InitVelocity(RevoluteJointConf&) (RevoluteJointConf.cpp:127-226).
     static void JointInitVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
         if (j.kind == 1) return DistanceInitVelocity(j, bcs, conf);
@@ -2483,6 +2569,7 @@ public:
         if (j.kind == 6) return WheelInitVelocity(j, bcs, conf);
         if (j.kind == 7) return PulleyInitVelocity(j, bcs, conf);
         if (j.kind == 8) return PrismaticInitVelocity(j, bcs, conf);
+        if (j.kind == 9) return TargetInitVelocity(j, bcs, conf);
         BodyConstraint<R>& A = bcs[j.bodyA];
         BodyConstraint<R>& B = bcs[j.bodyB];
         const R mA = A.invMass, iA = A.invRotI, aA = A.pos.ang;
@@ -2549,6 +2636,7 @@ public:
         if (j.kind == 6) return WheelSolveVelocity(j, bcs, conf);
         if (j.kind == 7) return PulleySolveVelocity(j, bcs);
         if (j.kind == 8) return PrismaticSolveVelocity(j, bcs, conf);
+        if (j.kind == 9) return TargetSolveVelocity(j, bcs, conf);
         BodyConstraint<R>& A = bcs[j.bodyA];
         BodyConstraint<R>& B = bcs[j.bodyB];
         const Vel<R> oldVelA = A.vel, oldVelB = B.vel;
@@ -2628,6 +2716,7 @@ public:
         if (j.kind == 6) return WheelSolvePosition(j, bcs, conf);
         if (j.kind == 7) return PulleySolvePosition(j, bcs, conf);
         if (j.kind == 8) return PrismaticSolvePosition(j, bcs, conf);
+        if (j.kind == 9) return true;  // TargetJointConf.cpp:180-184
         BodyConstraint<R>& A = bcs[j.bodyA];
         BodyConstraint<R>& B = bcs[j.bodyB];
         Pos<R> posA = A.pos, posB = B.pos;
@@ -2716,10 +2805,10 @@ public:
                 const std::uint32_t jid = bj.second;
                 if (islandedJoints[jid]) continue;
                 const std::uint32_t other = bj.first;
-                if (bodies[other].flags & kEnabled) {
+                if (other == kNoBody || (bodies[other].flags & kEnabled)) {
                     islandedJoints[jid] = 1;
                     island.joints.push_back(jid);
-                    if (!islandedBodies[other]) {
+                    if (other != kNoBody && !islandedBodies[other]) {
                         islandedBodies[other] = 1;
                         stack.push_back(other);
                     }
@@ -3682,7 +3771,7 @@ public:
     std::uint32_t AddJoint(const Joint<R>& j) {
         const std::uint32_t id = std::uint32_t(joints.size());
         joints.push_back(j);
-        bodyJoints[j.bodyA].emplace_back(j.bodyB, id);
+        if (j.bodyA != kNoBody) bodyJoints[j.bodyA].emplace_back(j.bodyB, id);
         bodyJoints[j.bodyB].emplace_back(j.bodyA, id);
         return id;
     }

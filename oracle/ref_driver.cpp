@@ -44,6 +44,7 @@
 #include <playrho/d2/WheelJointConf.hpp>
 #include <playrho/d2/PulleyJointConf.hpp>
 #include <playrho/d2/PrismaticJointConf.hpp>
+#include <playrho/d2/TargetJointConf.hpp>
 #include <playrho/d2/WorldJoint.hpp>
 #include <playrho/StepConf.hpp>
 
@@ -219,6 +220,14 @@ void WriteDescription(Out& o, const World& world) {
                                  GetY(j->localAnchorB),  GetX(j->groundAnchorA), GetY(j->groundAnchorA),
                                  GetX(j->groundAnchorB), GetY(j->groundAnchorB), j->ratio,
                                  j->lengthA,             j->lengthB,             0.0f};
+            o.put(f, sizeof(f));
+        } else if (const auto* j = TypeCast<const TargetJointConf>(&joint)) {
+            o.u32(to_underlying(GetBodyA(*j)));
+            o.u32(to_underlying(GetBodyB(*j)));
+            o.u32((j->collideConnected ? 4u : 0u) | (9u << 8));
+            const float f[12] = {0.0f,          0.0f,           GetX(j->localAnchorB), GetY(j->localAnchorB),
+                                 GetX(j->target), GetY(j->target), j->maxForce,          j->frequency,
+                                 j->dampingRatio, 0.0f,          0.0f,                  0.0f};
             o.put(f, sizeof(f));
         } else if (const auto* j = TypeCast<const PrismaticJointConf>(&joint)) {
             o.u32(to_underlying(GetBodyA(*j)));
@@ -450,6 +459,10 @@ void WriteState(Out& o, const World& world, std::uint32_t step, const StepStats&
             o.u32(0u);
             const float f[4] = {j->impulse, 0.0f, 0.0f, 0.0f};
             o.put(f, sizeof(f));
+        } else if (const auto* j = TypeCast<const TargetJointConf>(&joint)) {
+            o.u32(0u);
+            const float f[4] = {GetX(j->impulse), GetY(j->impulse), 0.0f, 0.0f};
+            o.put(f, sizeof(f));
         } else if (const auto* j = TypeCast<const PrismaticJointConf>(&joint)) {
             o.u32(std::uint32_t(j->limitState));
             const float f[4] = {GetX(j->impulse), GetY(j->impulse), GetZ(j->impulse), j->motorImpulse};
@@ -519,6 +532,12 @@ World BuildWorld(const prx_scene::SceneSpec& spec) {
             auto jc = PulleyJointConf{ba, bb, Length2{js.gax, js.gay}, Length2{js.gbx, js.gby},
                                       Length2{js.lax, js.lay}, Length2{js.lbx, js.lby}, js.length, js.lengthB};
             jc.UseRatio(js.ratio);
+            jc.UseCollideConnected(js.collideConnected);
+            CreateJoint(world, jc);
+        } else if (js.kind == prx_scene::kTarget) {
+            auto jc = TargetJointConf{bb};
+            jc.UseTarget(Length2{js.gax, js.gay}).UseAnchor(Length2{js.lbx, js.lby});
+            jc.UseMaxForce(js.maxForce).UseFrequency(js.frequency).UseDampingRatio(js.dampingRatio);
             jc.UseCollideConnected(js.collideConnected);
             CreateJoint(world, jc);
         } else if (js.kind == prx_scene::kPrismatic) {

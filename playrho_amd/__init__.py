@@ -249,6 +249,20 @@ class PrismaticJointConf(ctypes.Structure):
         super().__init__(**kw)
 
 
+class TargetJointConf(ctypes.Structure):
+    """TargetJointConf (d2/TargetJointConf.hpp:56-161), a joint with body B
+    only; the reference's defaults: frequency 5, damping ratio 0.7, maximum
+    force 0."""
+    _fields_ = [("body", ctypes.c_uint32), ("target", ctypes.c_float * 2), ("local_anchor", ctypes.c_float * 2),
+                ("max_force", ctypes.c_float), ("frequency", ctypes.c_float), ("damping_ratio", ctypes.c_float),
+                ("collide_connected", ctypes.c_uint32)]
+
+    def __init__(self, **kw):
+        kw.setdefault("frequency", 5.0)
+        kw.setdefault("damping_ratio", 0.7)
+        super().__init__(**kw)
+
+
 class PulleyJointConf(ctypes.Structure):
     """PulleyJointConf (d2/PulleyJointConf.hpp:65-137); the reference's defaults:
     ground anchors (-1, 1), (1, 1), local anchors (-1, 0), (1, 0), lengths 0,
@@ -329,6 +343,7 @@ _SIGS = {
     "prx_pulley_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
     "prx_prismatic_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
                                                   ctypes.POINTER(ctypes.c_uint32)]),
+    "prx_target_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
     "prx_world_joint_count": (ctypes.c_uint32, [ctypes.c_void_p]),
     "prx_world_shape_count": (ctypes.c_uint32, [ctypes.c_void_p]),
     "prx_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
@@ -475,7 +490,7 @@ class World:
     def create_joint(self, conf):
         """CreateJoint with a RevoluteJointConf, DistanceJointConf,
         WeldJointConf, RopeJointConf, FrictionJointConf, MotorJointConf,
-        WheelJointConf, PulleyJointConf or PrismaticJointConf; returns the joint id
+        WheelJointConf, PulleyJointConf, PrismaticJointConf or TargetJointConf; returns the joint id
         (shared by all kinds)."""
         fn = {RevoluteJointConf: lib().prx_joint_create, DistanceJointConf: lib().prx_distance_joint_create,
               WeldJointConf: lib().prx_weld_joint_create, RopeJointConf: lib().prx_rope_joint_create,
@@ -483,7 +498,8 @@ class World:
               MotorJointConf: lib().prx_motor_joint_create,
               WheelJointConf: lib().prx_wheel_joint_create,
               PulleyJointConf: lib().prx_pulley_joint_create,
-              PrismaticJointConf: lib().prx_prismatic_joint_create}.get(type(conf))
+              PrismaticJointConf: lib().prx_prismatic_joint_create,
+              TargetJointConf: lib().prx_target_joint_create}.get(type(conf))
         if fn is None:
             raise TypeError("not a joint conf: %r" % type(conf))
         jid = ctypes.c_uint32()

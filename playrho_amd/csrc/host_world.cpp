@@ -625,9 +625,18 @@ prx::StepParams ToParams(const prx_step_conf& c) {
 }
 
 // CreateJoint (AabbTreeWorld.cpp:1132-1175) for any joint kind.
+// A target joint has no body A: its a holds InvalidBodyID's index (BodyID.hpp:50).
+constexpr std::uint32_t kNoBody = 0xffffu;
+
+bool JointBodyValid(const prx_world* w, const HostJoint& j, bool bodyA) {
+    const std::uint32_t id = bodyA ? j.a : j.b;
+    if (bodyA && id == kNoBody && (j.flags >> 8) == prx::kJointTarget) return true;
+    return id < w->bodies.size();
+}
+
 int AddJointRecord(prx_world* w, const HostJoint& j, std::uint32_t* out) {
-    if (j.a >= w->bodies.size() || j.b >= w->bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
-    if (!(j.flags & prx::kJointCollideConnected)) {
+    if (!JointBodyValid(w, j, true) || !JointBodyValid(w, j, false)) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
+    if (!(j.flags & prx::kJointCollideConnected) && j.a != kNoBody) {
         for (std::size_t i = 0; i < w->contacts.size(); ++i) {
             const int2 bd = w->contacts.body[i];
             if ((bd.x == int(j.a) && bd.y == int(j.b)) || (bd.x == int(j.b) && bd.y == int(j.a)))
@@ -720,6 +729,25 @@ int AddPrismaticJoint(prx_world* w, const prx_prismatic_joint_conf& c, std::uint
     j.maxTorque = c.max_motor_force;
     j.ext[0] = c.local_axis_a[0];
     j.ext[1] = c.local_axis_a[1];
+    return AddJointRecord(w, j, out);
+}
+
+// TargetJointConf: no body A; the anchor on B in the anchor B slot, (target x,
+// y, maximum force, frequency), the damping ratio in the maximum torque slot.
+int AddTargetJoint(prx_world* w, const prx_target_joint_conf& c, std::uint32_t* out) {
+    if (!(c.max_force >= 0.0f) || !(c.frequency >= 0.0f) || !(c.damping_ratio >= 0.0f))
+        return Fail(PRX_E_INVALID_ARGUMENT, "maximum force, frequency and damping ratio must be non-negative");
+    HostJoint j;
+    j.a = kNoBody;
+    j.b = c.body;
+    j.flags = (c.collide_connected ? prx::kJointCollideConnected : 0u) | (prx::kJointTarget << 8);
+    j.la[2] = c.local_anchor[0];
+    j.la[3] = c.local_anchor[1];
+    j.cfg[0] = c.target[0];
+    j.cfg[1] = c.target[1];
+    j.cfg[2] = c.max_force;
+    j.cfg[3] = c.frequency;
+    j.maxTorque = c.damping_ratio;
     return AddJointRecord(w, j, out);
 }
 
@@ -955,7 +983,10 @@ int UploadBatch(prx_batch* b) {
             for (std::size_t ji = 0; ji < w->joints.size(); ++ji) {
                 const HostJoint& j = w->joints[ji];
                 const std::size_t g = std::size_t(h.joint_off) + ji;
-                jbody[g] = int2{int(j.a), int(j.b)};
+                // A target joint's record names body B twice (its solver
+                // leaves the body-A copy unchanged, written back first).
+                const bool noA = j.a == kNoBody;
+                jbody[g] = int2{noA ? int(j.b) : int(j.a), int(j.b)};
                 janchor[g] = float4{j.la[0], j.la[1], j.la[2], j.la[3]};
                 jcfg[g] = float4{j.cfg[0], j.cfg[1], j.cfg[2], j.cfg[3]};
                 float fl;
@@ -964,8 +995,8 @@ int UploadBatch(prx_batch* b) {
                 if (j.flags >> 8) b->joint_kinds = true;
                 jimp[g] = float4{j.imp[0], j.imp[1], j.imp[2], j.imp[3]};
                 jlimit[g] = j.limit;
-                lists[j.a].push_back(int2{int(ji), int(j.b)});
-                lists[j.b].push_back(int2{int(ji), int(j.a)});
+                if (!noA) lists[j.a].push_back(int2{int(ji), int(j.b)});
+                lists[j.b].push_back(int2{int(ji), noA ? -1 : int(j.a)});
             }
             int at = 0;
             for (std::size_t bi = 0; bi < w->bodies.size(); ++bi) {
@@ -1564,6 +1595,20 @@ int prx_scene_build(prx_world* w, const char* name, const std::int32_t* params, 
             if (int st = AddPulleyJoint(w, pc, nullptr)) return st;
             continue;
         }
+        if (js.kind == prx_scene::kTarget) {
+            prx_target_joint_conf tc{};
+            tc.body = std::uint32_t(js.b);
+            tc.target[0] = js.gax;
+            tc.target[1] = js.gay;
+            tc.local_anchor[0] = js.lbx;
+            tc.local_anchor[1] = js.lby;
+            tc.max_force = js.maxForce;
+            tc.frequency = js.frequency;
+            tc.damping_ratio = js.dampingRatio;
+            tc.collide_connected = js.collideConnected ? 1u : 0u;
+            if (int st = AddTargetJoint(w, tc, nullptr)) return st;
+            continue;
+        }
         if (js.kind == prx_scene::kPrismatic) {
             prx_prismatic_joint_conf pc{};
             pc.body_a = std::uint32_t(js.a);
@@ -1686,6 +1731,13 @@ int prx_pulley_joint_create(prx_world* w, const prx_pulley_joint_conf* conf, std
     if (!conf) return Fail(PRX_E_INVALID_ARGUMENT, "null joint conf");
     if (int st = Detach(w)) return st;
     return AddPulleyJoint(w, *conf, id);
+}
+
+int prx_target_joint_create(prx_world* w, const prx_target_joint_conf* conf, std::uint32_t* id) {
+    if (int st = CheckWorld(w)) return st;
+    if (!conf) return Fail(PRX_E_INVALID_ARGUMENT, "null joint conf");
+    if (int st = Detach(w)) return st;
+    return AddTargetJoint(w, *conf, id);
 }
 
 int prx_prismatic_joint_create(prx_world* w, const prx_prismatic_joint_conf* conf, std::uint32_t* id) {
@@ -1858,8 +1910,10 @@ int prx_world_load_desc(prx_world* w, const std::uint8_t* desc, std::size_t len)
         j.a = u32();
         j.b = u32();
         j.flags = u32() & 0xff07u;
-        if (j.a >= fresh.bodies.size() || j.b >= fresh.bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
-        if ((j.flags >> 8) > prx::kJointPrismatic) return Fail(PRX_E_INVALID_ARGUMENT, "unknown joint kind");
+        if ((j.flags >> 8) > prx::kJointTarget) return Fail(PRX_E_INVALID_ARGUMENT, "unknown joint kind");
+        if ((j.a >= fresh.bodies.size() && !(j.a == kNoBody && (j.flags >> 8) == prx::kJointTarget)) ||
+            j.b >= fresh.bodies.size())
+            return Fail(PRX_E_OUT_OF_RANGE, "no such body");
         // Limits belong to revolute and prismatic joints, motors to those and wheels.
         const std::uint32_t kind = j.flags >> 8;
         if (kind && kind != prx::kJointPrismatic)

@@ -289,7 +289,8 @@ enum : std::uint32_t {
     kJointMotorKind = 5u,  // MotorJointConf (kJointMotor is the motor-enable flag)
     kJointWheel = 6u,
     kJointPulley = 7u,
-    kJointPrismatic = 8u
+    kJointPrismatic = 8u,
+    kJointTarget = 9u
 };
 // Constraint records staged through LDS per chunk by the wave-wide sweeps;
 // the stage sits at the end of the dynamic LDS.

@@ -58,10 +58,12 @@ struct BodySpec {
 // PulleyJointConf (anchors, ground anchors gax, gay, gbx, gby, lengths
 // length, lengthB, ratio), 8 PrismaticJointConf (anchors, axis at axisAngle,
 // referenceAngle, enableLimit, lower, upper, enableMotor, motorSpeed,
-// maxForce as the maximum motor force).
+// maxForce as the maximum motor force), 9 TargetJointConf (body b only; the
+// target gax, gay, the anchor on B lbx, lby, maxForce, frequency,
+// dampingRatio).
 enum JointKind : int {
     kRevolute = 0, kDistance = 1, kWeld = 2, kRope = 3, kFriction = 4, kMotor = 5, kWheel = 6, kPulley = 7,
-    kPrismatic = 8
+    kPrismatic = 8, kTarget = 9
 };
 struct JointSpec {
     int kind = kRevolute;
@@ -685,11 +687,69 @@ inline SceneSpec Sliders(std::uint32_t seed, int w, int n = 6, int falling = 24)
     return sc;
 }
 
+// Target joints: `n` bodies (boxes and disks) are pulled towards seeded
+// targets by target joints with seeded anchors, maximum forces (some small
+// enough to clamp), frequencies and damping ratios (one with frequency 0, one
+// with maximum force 0), beside `falling` seeded bodies dropped onto them and
+// onto a ground edge.
+inline SceneSpec Targets(std::uint32_t seed, int w, int n = 8, int falling = 24) {
+    SceneSpec sc;
+    Rng rng(seed * 2654435761u + std::uint32_t(w) * 40503u + 31u);
+    sc.shapes.push_back(Edge(-50.0f, 0.0f, 50.0f, 0.0f));
+    BodySpec ground;
+    ground.shapes = {0};
+    sc.bodies.push_back(ground);
+    auto addBody = [&](float x, float y) {
+        BodySpec b;
+        b.type = kDynamic;
+        b.x = x;
+        b.y = y;
+        b.angle = rng.uniform(-0.5, 0.5);
+        b.ay = -10.0f;
+        const float friction = rng.uniform(0.2, 0.8);
+        if (rng.below(2) == 0) {
+            const float hx = rng.uniform(0.3, 0.6);
+            const float hy = rng.uniform(0.3, 0.6);
+            sc.shapes.push_back(Box(hx, hy, 1.0f, friction, 0.0f));
+        } else {
+            sc.shapes.push_back(Disk(rng.uniform(0.3, 0.6), 1.0f, friction, 0.0f));
+        }
+        b.shapes = {int(sc.shapes.size()) - 1};
+        sc.bodies.push_back(b);
+        return int(sc.bodies.size()) - 1;
+    };
+    for (int i = 0; i < n; ++i) {
+        const float x = float(5 * i) - 2.5f * float(n);
+        const float y = rng.uniform(1.0, 6.0);
+        JointSpec tj;
+        tj.kind = kTarget;
+        tj.b = addBody(x, y);
+        tj.a = -1;
+        tj.gax = x + rng.uniform(-3.0, 3.0);
+        tj.gay = rng.uniform(2.0, 12.0);
+        tj.lbx = rng.uniform(-0.3, 0.3);
+        tj.lby = rng.uniform(-0.3, 0.3);
+        tj.maxForce = rng.uniform(5.0, 400.0);
+        tj.frequency = rng.uniform(1.0, 6.0);
+        tj.dampingRatio = rng.uniform(0.2, 1.0);
+        if (i == 1) tj.frequency = 0.0f;
+        if (i == 2) tj.maxForce = 0.0f;
+        sc.joints.push_back(tj);
+    }
+    for (int k = 0; k < falling; ++k) {
+        // Drawn in sequence: argument evaluation order differs between compilers.
+        const float fx = rng.uniform(-20.0, 20.0);
+        const float fy = rng.uniform(15.0, 40.0);
+        addBody(fx, fy);
+    }
+    return sc;
+}
+
 // Named scenes with integer parameters, as used by tests and benchmarks.
 //   "hello", "pyramid"(rows), "drop"(n, half, height, seed),
 //   "pile"(seed, w, nmin, nmax), "billiards"(seed, w),
 //   "chain"(seed, w, links, falling, torque), "joints"(seed, w, n, falling),
-//   "sliders"(seed, w, n, falling)
+//   "sliders"(seed, w, n, falling), "targets"(seed, w, n, falling)
 inline bool Make(const std::string& name, const int* p, int np, SceneSpec& out) {
     auto arg = [&](int i, int dflt) { return (i < np) ? p[i] : dflt; };
     if (name == "hello") {
@@ -708,6 +768,8 @@ inline bool Make(const std::string& name, const int* p, int np, SceneSpec& out) 
         out = Joints(std::uint32_t(arg(0, 1)), arg(1, 0), arg(2, 8), arg(3, 24));
     } else if (name == "sliders") {
         out = Sliders(std::uint32_t(arg(0, 1)), arg(1, 0), arg(2, 6), arg(3, 24));
+    } else if (name == "targets") {
+        out = Targets(std::uint32_t(arg(0, 1)), arg(1, 0), arg(2, 8), arg(3, 24));
     } else {
         return false;
     }

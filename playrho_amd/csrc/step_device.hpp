@@ -1663,6 +1663,88 @@ __device__ __forceinline__ bool wheel_solve_velocity(float4* u, float4& velA, fl
     return same;
 }
 
+// Target record (body A is body B: its copy is left unchanged and written back
+// first): u[0] rB x, y, C x, y; u[1] effective mass m00, m01, m10, m11;
+// u[2].xyz gamma, dt * maximum force, 1 when warm starting; impulse in
+// u[3].xy.  InitVelocity(TargetJointConf&) less its velocity changes
+// (TargetJointConf.cpp:85-132, GetEffectiveMassMatrix :60-83).
+__device__ __forceinline__ void build_jc_target(float4* u, const float4 cf, const float4 cf2, float4 im, V2 rB,
+                                                const float4& pB, float mB, float iB, std::uint32_t flags,
+                                                const StepParams& c) {
+    const float mass = (mB != 0.0f) ? (1.0f / mB) : 0.0f;
+    const float omega = 2.0f * prx::kPiF * cf.w;
+    const float dd = 2.0f * mass * cf2.x * omega;
+    const float k = mass * (omega * omega);
+    const float h = c.dt;
+    const float tmp = dd + h * k;
+    const float invGamma = h * tmp;
+    const float gamma = (invGamma != 0.0f) ? 1.0f / invGamma : 0.0f;
+    const float beta = h * k * gamma;
+    const float exx = mB + (iB * (rB.y * rB.y) / 1.0f) + gamma;
+    const float exy = -iB * rB.x * rB.y / 1.0f;
+    const float eyy = mB + (iB * (rB.x * rB.x) / 1.0f) + gamma;
+    // Invert(Matrix22) (Math.hpp:429-442) of columns (exx, exy), (exy, eyy).
+    const float cp = cross(V2{exx, exy}, V2{exy, eyy});
+    float4 m{0.0f, 0.0f, 0.0f, 0.0f};
+    if (!almost_zero(cp)) {
+        const float inverse = 1.0f / cp;
+        m = float4{eyy * inverse, -exy * inverse, -exy * inverse, exx * inverse};
+    }
+    const V2 C = mul((V2{pB.x, pB.y} + rB) - V2{cf.x, cf.y}, beta);
+    im = c.warmStart ? float4{im.x * c.dtRatio, im.y * c.dtRatio, 0.0f, 0.0f} : float4{0.0f, 0.0f, 0.0f, 0.0f};
+    u[0] = float4{rB.x, rB.y, C.x, C.y};
+    u[1] = m;
+    u[2] = float4{gamma, c.dt * cf.z, c.warmStart ? 1.0f : 0.0f,
+                  __uint_as_float(flags | (prx::kJointTarget << 16))};
+    u[3] = im;
+}
+
+// The velocity half of InitVelocity(TargetJointConf&) (TargetJointConf.cpp:
+// 125-140): the spin damping, then the warm start when warm starting.
+__device__ __forceinline__ void target_warm_start(const float4* u, float4& velB) {
+    const float4 r = u[0], im = u[3], ms = u[4];
+    velB.z = velB.z * 0.98f;
+    if (u[2].z == 0.0f) return;
+    const V2 P{im.x, im.y};
+    const float crossBP = cross(V2{r.x, r.y}, P) / 1.0f;
+    velB.x = velB.x + ms.z * P.x;
+    velB.y = velB.y + ms.z * P.y;
+    velB.z = velB.z + ms.w * crossBP;
+}
+
+// SolveVelocity(TargetJointConf&) (TargetJointConf.cpp:143-178): returns
+// whether the impulse increment was zero.
+__device__ __forceinline__ bool target_solve_velocity(float4* u, float4& velB) {
+    const float4 r = u[0], m = u[1], g = u[2], ms = u[4];
+    const V2 rB{r.x, r.y}, C{r.z, r.w};
+    float4 im = u[3];
+    const V2 Cdot = V2{velB.x, velB.y} + mul(revperp(rB), velB.z / 1.0f);
+    const V2 ev = Cdot + (C + mul(g.x, V2{im.x, im.y}));
+    const V2 old{im.x, im.y};
+    // Transform(-ev, mass): the matrix's vectors dotted with -ev from zero.
+    const V2 nev = -ev;
+    float tx = 0.0f, ty = 0.0f;
+    tx += m.x * nev.x;
+    tx += m.y * nev.y;
+    ty += m.z * nev.x;
+    ty += m.w * nev.y;
+    V2 imp{old.x + tx, old.y + ty};
+    const float maxImpulse = g.y;
+    if (magsq(imp) > maxImpulse * maxImpulse) {
+        const V2 dir = unit_vector(imp, V2{0.0f, 0.0f});
+        imp = V2{dir.x * maxImpulse, dir.y * maxImpulse};
+    }
+    im.x = imp.x;
+    im.y = imp.y;
+    u[3] = im;
+    const V2 inc = imp - old;
+    const float angImpulseB = cross(rB, inc) / 1.0f;
+    velB.x = velB.x + ms.z * inc.x;
+    velB.y = velB.y + ms.z * inc.y;
+    velB.z = velB.z + ms.w * angImpulseB;
+    return inc.x == 0.0f && inc.y == 0.0f;
+}
+
 // Prismatic record: u[0] axis x, y, perp x, y; u[1] s1, s2, a1, a2; u[2].xy
 // motor speed, dt * maximum motor force, u[2].w flags | limit state << 8 |
 // kind << 16; u[3] impulse x, y, z, motor impulse; u[5].x upper translation;
@@ -2041,6 +2123,8 @@ __device__ __forceinline__ void spring_joint_warm_start(const float4* u, std::ui
         pulley_apply(u, im.x, velA, velB);
     } else if (kind == prx::kJointPrismatic) {
         prismatic_warm_start(u, velA, velB);
+    } else if (kind == prx::kJointTarget) {
+        target_warm_start(u, velB);
     } else if (kind == prx::kJointWheel) {
         // WheelJointConf.cpp:212-223.
         const float4 a = u[0], s = u[1];
@@ -2128,6 +2212,7 @@ __device__ __forceinline__ bool spring_joint_solve_position(const float4* u, std
     const float4 ms = u[4], la = u[6], cf = u[7];
     const float mA = ms.x, iA = ms.y, mB = ms.z, iB = ms.w;
     if (kind == prx::kJointFriction) return true;    // FrictionJointConf.cpp:233-237
+    if (kind == prx::kJointTarget) return true;      // TargetJointConf.cpp:180-184
     if (kind == prx::kJointPulley) {
         // PulleyJointConf.cpp:192-253.
         const float ratio = u[2].y;
@@ -2333,6 +2418,8 @@ __device__ __forceinline__ void build_jc(const DeviceState& d, const WorldCtx& x
             build_jc_pulley(u, cf, cf2, im, rA, rB, pA, pB, mA, iA, mB, iB, flags & 0xffu, c);
         else if (kind == prx::kJointPrismatic)
             build_jc_prismatic(u, cf, cf2, im, rA, rB, qA, pA, pB, flags & 0xffu, limit, c);
+        else if (kind == prx::kJointTarget)
+            build_jc_target(u, cf, cf2, im, rB, pB, mB, iB, flags & 0xffu, c);
         else {
             // Body B's anchor is its origin: rB = Rotate(-localCenterB, qB).
             const V2 rBm = rotate(V2{-B.lc.x, -B.lc.y}, qB);
@@ -2641,9 +2728,11 @@ __device__ __forceinline__ int find_islands(const DeviceState& d, const WorldCtx
                     for (int k = 0; k < bj.y; ++k) {
                         const int2 je = d.bjoint_list[2 * x.jo + bj.x + k];
                         if (islj[je.x]) continue;
-                        if (!(d.bflags[x.bo + je.y] & prx::kEnabled)) continue;
+                        // A target joint has no other body (-1).
+                        if (je.y >= 0 && !(d.bflags[x.bo + je.y] & prx::kEnabled)) continue;
                         islj[je.x] = 1;
                         S.ijoint[nje++] = je.x;
+                        if (je.y < 0) continue;
                         if (islb[je.y] == 0) {
                             islb[je.y] = 1;
                             stack[sp++] = je.y;
@@ -2807,9 +2896,11 @@ __device__ __forceinline__ int find_islands_lds(const DeviceState& d, const Worl
                     for (int k = 0; k < bj.y; ++k) {
                         const int2 je = d.bjoint_list[2 * x.jo + bj.x + k];
                         if (islj[je.x]) continue;
-                        if (!(d.bflags[x.bo + je.y] & prx::kEnabled)) continue;
+                        // A target joint has no other body (-1).
+                        if (je.y >= 0 && !(d.bflags[x.bo + je.y] & prx::kEnabled)) continue;
                         islj[je.x] = 1;
                         S.ijoint[nje++] = je.x;
+                        if (je.y < 0) continue;
                         if (islb[je.y] == 0) {
                             islb[je.y] = 1;
                             stack[sp++] = je.y;
@@ -2956,6 +3047,7 @@ __device__ __forceinline__ bool joints_velocity_k(float4* jbase, int j0, int j1,
             else if (kind == prx::kJointWheel) ok &= wheel_solve_velocity(u, va, vb);
             else if (kind == prx::kJointPulley) ok &= pulley_solve_velocity(u, va, vb);
             else if (kind == prx::kJointPrismatic) ok &= prismatic_solve_velocity(u, va, vb);
+            else if (kind == prx::kJointTarget) ok &= target_solve_velocity(u, vb);
             else ok &= motor_solve_velocity(u, va, vb);
             bs.set_vel(bA, va);
             bs.set_vel(bB, vb);
@@ -2974,6 +3066,21 @@ __device__ __forceinline__ bool joints_velocity_k(float4* jbase, int j0, int j1,
         bs.set_vel(j.bB, vb);
     }
     return ok;
+}
+
+// Without warm starting, InitVelocity still damps a target joint's body's spin
+// (TargetJointConf.cpp:125-126): the joints' part of the warm-start pass that
+// remains then.
+template <bool kLds>
+__device__ __forceinline__ void joints_target_damping(const float4* jbase, int j0, int j1, const Bodies<kLds>& bs) {
+    for (int e = j0; e < j1; ++e) {
+        const float4* u = jbase + std::size_t(e) * prx::kJcStride;
+        if (jc_kind(u) != prx::kJointTarget) continue;
+        const int b = __float_as_int(u[5].w);
+        float4 v = bs.vel(b);
+        v.z = v.z * 0.98f;
+        bs.set_vel(b, v);
+    }
 }
 
 // Batches whose joints are all revolute take the loop without the kind
@@ -3099,6 +3206,7 @@ __device__ __forceinline__ VelResult island_velocity(const DeviceState& d, const
         }
     }
     if (c.warmStart) joints_velocity(jbase, j0, j1, bs, true, c);
+    else if (c.jointKinds) joints_target_damping(jbase, j0, j1, bs);
     for (std::uint32_t it = 0; it < c.velIters; ++it) {
         if constexpr (kTape) d.tape.isl[islandSlot].z = int(it) + 1;
         bool jointsOkay;
@@ -3192,6 +3300,7 @@ __device__ __forceinline__ VelResult island_velocity_wave(const DeviceState& d, 
     sg.commit(0);
     __syncthreads();
     const int npass = int(c.velIters) + (c.warmStart ? 1 : 0);
+    if (!c.warmStart && c.jointKinds && x.lane == 0) joints_target_damping(jbase, j0, j1, bs);
     for (int pass = 0; pass < npass; ++pass) {
         const bool warm = c.warmStart && pass == 0;
         float newInc = 0.0f;

@@ -17,6 +17,7 @@
 #include <playrho/d2/MotorJointConf.hpp>
 #include <playrho/d2/PolygonShapeConf.hpp>
 #include <playrho/d2/PrismaticJointConf.hpp>
+#include <playrho/d2/TargetJointConf.hpp>
 #include <playrho/d2/PulleyJointConf.hpp>
 #include <playrho/d2/RevoluteJointConf.hpp>
 #include <playrho/d2/RopeJointConf.hpp>
@@ -671,8 +672,18 @@ JointID CreateJoint(GpuWorld& g, const Joint& def) {
         c.max_motor_force = F(Real(j->maxMotorForce / Newton));
         c.collide_connected = cc;
         Check(prx_prismatic_joint_create(g.m_world, &c, &id));
+    } else if (const auto* j = TypeCast<const TargetJointConf>(&def)) {
+        prx_target_joint_conf c{};
+        c.body = bB;
+        xy(j->target, c.target);
+        xy(j->localAnchorB, c.local_anchor);
+        c.max_force = F(Real(j->maxForce / Newton));
+        c.frequency = F(Real(j->frequency / Hertz));
+        c.damping_ratio = F(j->dampingRatio);
+        c.collide_connected = cc;
+        Check(prx_target_joint_create(g.m_world, &c, &id));
     } else {
-        Unsupported("this joint type (gear and target joints)");
+        Unsupported("this joint type (gear joints)");
     }
     g.m_joints.push_back(def);
     return JointID(static_cast<JointID::underlying_type>(id));

@@ -51,6 +51,9 @@ CASES = [
     # motorised slider between two dynamic bodies, beside falling bodies.
     ("sliders0", "sliders", 240, [1, 0], [1, 120, 240]),
     ("sliders1", "sliders", 240, [2, 1, 9, 40], [1, 240]),
+    # Target joints (no body A): clamped and unclamped forces, frequency 0,
+    # maximum force 0, beside falling bodies.
+    ("targets0", "targets", 240, [1, 0], [1, 120, 240]),
 ]
 # World edits between steps (step, kind, body, a, b, c): SetLocation 'L',
 # SetAngle 'A', SetTransformation 'T' (c the rotation's angle), ShiftOrigin 'O'.
@@ -75,6 +78,7 @@ TOI_CASES = [
     ("pile3301_toi", "pile", 300, [1, 3301], [1, 300]),
     ("pile859_toi", "pile", 300, [1, 859], [1, 300]),
     ("sliders0_toi", "sliders", 240, [1, 0], [1, 240]),
+    ("targets0_toi", "targets", 240, [1, 0], [1, 240]),
 ]
 
 

@@ -290,3 +290,33 @@ def test_prismatic_joint_creation_matches_scene():
     assert w.save_desc() == desc
     p = pa.PrismaticJointConf()
     assert (tuple(p.local_axis_a), p.enable_limit, p.enable_motor) == ((1.0, 0.0), 0, 0)
+
+
+def test_target_joint_creation_matches_scene():
+    """Target joints created through prx_target_joint_create give the same
+    world description as the scene the reference built: kind 9, body A
+    InvalidBodyID's index 65535 (BodyID.hpp:50), the anchor on B, the target,
+    maximum force, frequency and damping ratio; negative values are
+    InvalidArgument."""
+    g = golden_cases.load("targets0")
+    desc = _desc_only(g["desc_conf"])
+    recs = dumpfmt.read_description(dumpfmt.Reader(desc))[2]
+    assert {r["flags"] >> 8 for r in recs} == {9} and {r["a"] for r in recs} == {65535}
+    offs = pa.desc_joint_offsets(desc)
+    bare = desc[:offs[0] - 4] + struct.pack("<I", 0)
+    w = pa.World.from_desc(bare)
+    for r in recs:
+        v = [float(x) for x in r["vals"]]
+        w.create_joint(pa.TargetJointConf(body=r["b"], local_anchor=(v[2], v[3]), target=(v[4], v[5]),
+                                          max_force=v[6], frequency=v[7], damping_ratio=v[8],
+                                          collide_connected=(r["flags"] >> 2) & 1))
+    assert w.joint_count == len(recs)
+    assert w.save_desc() == desc
+    with pytest.raises(pa.PrxError) as e:
+        w.create_joint(pa.TargetJointConf(body=1, max_force=-1.0))
+    assert e.value.code == 1
+    with pytest.raises(pa.PrxError) as e:
+        w.create_joint(pa.TargetJointConf(body=100000))
+    assert e.value.code == 4
+    t = pa.TargetJointConf()
+    assert (t.frequency, round(t.damping_ratio, 6), t.max_force) == (5.0, 0.7, 0.0)

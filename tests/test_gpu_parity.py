@@ -166,6 +166,27 @@ def test_device_matches_oracle(scene, params, steps, toi):
             assert dumpfmt.digest(got, (part,)) == dumpfmt.digest(want, (part,)), (scene, params, step, part)
 
 
+@pytest.mark.parametrize("scene,params,coop_min", [
+    ("targets", [1, 0], None), ("targets", [2, 3, 10, 30], 1), ("sliders", [1, 0], None), ("joints", [1, 0], 1)])
+def test_joints_without_warm_start_match_oracle(scene, params, coop_min):
+    """doWarmStart = false: every joint's impulses restart from zero and a
+    target joint's InitVelocity still damps its body's spin
+    (TargetJointConf.cpp:125-126); the lane form (coop_min None) and the
+    wave form (coop_min 1) against the oracle, every step."""
+    w = _recording(pa.World.scene(scene, params))
+    conf = pa.StepConf(do_toi=0, do_warm_start=0)
+    ow = oracle_lib.OracleWorld(w.save_desc() + dumpfmt.conf_record(conf))
+    batch = pa.Batch([w], use_graph=False, coop_min=coop_min)
+    print("no warm start", scene, params, coop_min, flush=True)
+    for step in range(1, 121):
+        batch.step(conf, 1)
+        batch.download()
+        got = _state(w)
+        want = ow.step(1)[0]
+        for part in golden_cases.PARTS:
+            assert dumpfmt.digest(got, (part,)) == dumpfmt.digest(want, (part,)), (scene, params, step, part)
+
+
 def _conf_bytes():
     """The StepConf record (dump layout) of the default conf."""
     return dumpfmt.conf_record(pa.StepConf(do_toi=0))
