@@ -554,6 +554,20 @@ void prx_adjoint_destroy(prx_adjoint* adj);
  * restitution per batch shape.  The batch is left at its starting state. */
 int prx_adjoint_run(prx_adjoint* adj, const prx_step_conf* conf, const float* targets, float* loss,
                     float* grad_velocity, float* grad_friction, float* grad_restitution);
+/* The two halves of prx_adjoint_run for a loss of the caller's choosing.
+ * prx_adjoint_forward steps the batch nsteps steps with checkpoints and leaves
+ * it at the final state; final_state (may be null) receives 6 floats per batch
+ * body: x, y, angle (the sweep's position 1), vx, vy, w.
+ * prx_adjoint_backward takes the gradient of the loss with respect to that
+ * final state (6 floats per batch body, in the same order; entries of bodies
+ * that do not move are ignored) and returns the gradients with respect to
+ * initial velocities, friction and restitution as prx_adjoint_run does
+ * (joint torques by prx_adjoint_joint_gradients); it reverses the last
+ * forward pass (PRX_E_WRONG_STATE without one) and leaves the batch at its
+ * starting state.  No reference counterpart: the reference has no adjoint. */
+int prx_adjoint_forward(prx_adjoint* adj, const prx_step_conf* conf, float* final_state);
+int prx_adjoint_backward(prx_adjoint* adj, const prx_step_conf* conf, const float* state_bar, float* grad_velocity,
+                         float* grad_friction, float* grad_restitution);
 /* Diagnostic: on != 0 runs a disk-only batch through the general reverse
  * kernel (to check the two against each other); on = 0 returns to the
  * hand-derived one where the batch allows it (else PRX_E_UNSUPPORTED). */

@@ -383,6 +383,9 @@ _SIGS = {
     "prx_adjoint_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "prx_adjoint_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    "prx_adjoint_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "prx_adjoint_backward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_void_p]),
     "prx_adjoint_use_general": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "prx_adjoint_joint_gradients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "prx_adjoint_diagnostics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
@@ -819,6 +822,29 @@ class Adjoint:
         _check(lib().prx_adjoint_run(self.h, ctypes.byref(self.conf), t.ctypes.data, loss.ctypes.data,
                                      gv.ctypes.data, gf.ctypes.data, ge.ctypes.data))
         out = dict(loss=loss, grad_velocity=gv, grad_friction=gf, grad_restitution=ge)
+        if self.njoints:
+            gt = np.zeros(self.njoints, dtype=np.float32)
+            _check(lib().prx_adjoint_joint_gradients(self.h, gt.ctypes.data))
+            out["grad_motor_torque"] = gt
+        return out
+
+    def forward(self):
+        """Steps the batch K steps with checkpoints; returns the final state,
+        (bodies, 6): x, y, angle, vx, vy, w (prx_adjoint_forward)."""
+        out = np.zeros((self.nbodies, 6), dtype=np.float32)
+        _check(lib().prx_adjoint_forward(self.h, ctypes.byref(self.conf), out.ctypes.data))
+        return out
+
+    def backward(self, state_bar):
+        """Gradients of a loss whose gradient with respect to the final state
+        of the last forward() is state_bar (bodies, 6) (prx_adjoint_backward)."""
+        sb = np.ascontiguousarray(state_bar, dtype=np.float32).reshape(self.nbodies, 6)
+        gv = np.zeros((self.nbodies, 3), dtype=np.float32)
+        gf = np.zeros(self.nshapes, dtype=np.float32)
+        ge = np.zeros(self.nshapes, dtype=np.float32)
+        _check(lib().prx_adjoint_backward(self.h, ctypes.byref(self.conf), sb.ctypes.data, gv.ctypes.data,
+                                          gf.ctypes.data, ge.ctypes.data))
+        out = dict(grad_velocity=gv, grad_friction=gf, grad_restitution=ge)
         if self.njoints:
             gt = np.zeros(self.njoints, dtype=np.float32)
             _check(lib().prx_adjoint_joint_gradients(self.h, gt.ctypes.data))

@@ -1,5 +1,5 @@
 """torch binding of the adjoint: the libtorch operators
-torch.ops.playrho_amd.billiards_loss and articulated_loss, C++
+torch.ops.playrho_amd.simulate, billiards_loss and articulated_loss, C++
 torch::autograd::Functions over the library's adjoint
 (csrc/torch_adjoint.cpp, built into libplayrho_amd_torch.so next to this
 file).
@@ -43,6 +43,23 @@ def _pack(descs, v0, targets):
     offsets = np.cumsum([0] + [len(d) for d in descs]).astype(np.int64)
     t = torch.as_tensor(np.asarray(targets, dtype=np.float32)).reshape(v0.shape[0], v0.shape[1], 2)
     return torch.frombuffer(bytearray(blob), dtype=torch.uint8), torch.from_numpy(offsets), t
+
+
+def simulate(v0, friction, restitution, descs, nsteps, every, torque=None):
+    """The final state (worlds, bodies, 6): x, y, angle, vx, vy, w of every
+    body after nsteps steps from initial velocities v0 (worlds, bodies, 3),
+    per-shape friction and restitution (worlds, shapes) and, for worlds with
+    revolute joints, per-joint maximum motor torques (worlds, joints).  The
+    operator torch.ops.playrho_amd.simulate is a torch::autograd::Function:
+    any loss of the state back-propagates to v0, friction, restitution and
+    torque through the adjoint (its backward steps the worlds again from
+    checkpoints every `every` steps and reverses the steps)."""
+    blob = b"".join(descs)
+    offsets = torch.from_numpy(np.cumsum([0] + [len(d) for d in descs]).astype(np.int64))
+    if torque is None:
+        torque = torch.zeros((v0.shape[0], 0), dtype=v0.dtype)
+    return _ops().simulate(v0, friction, restitution, torque, torch.frombuffer(bytearray(blob), dtype=torch.uint8),
+                           offsets, int(nsteps), int(every))
 
 
 def billiards_loss(v0, friction, restitution, descs, targets, nsteps, every):

@@ -2861,7 +2861,119 @@ struct prx_adjoint {
     std::vector<prx::Tape> tapes;
     prx::AdjState a{};
     std::vector<std::vector<void*>> ckpt;
+    bool forwarded = false;  // checkpoints of a forward pass not yet reversed
 };
+
+namespace {
+
+int AdjointParams(prx_adjoint* adj, const prx_step_conf* conf, prx::StepParams& p) {
+    if (conf->do_toi) return Fail(PRX_E_UNSUPPORTED, "the adjoint does not differentiate the TOI phase (doToi = true)");
+    if (int(conf->reg_velocity_iters) > adj->itV || int(conf->reg_position_iters) > adj->itP)
+        return Fail(PRX_E_INVALID_ARGUMENT, "iteration counts above those the adjoint was created for");
+    for (prx_world* w : adj->b->worlds)
+        if (!w) return Fail(PRX_E_WRONG_STATE, "a member world left the batch");
+    p = ToParams(*conf);
+    p.ldsBytes = std::uint32_t(adj->b->lds_bytes);
+    p.coopMin = 0xffffffffu;  // every island on one lane (the recorded form)
+    return PRX_OK;
+}
+
+int AdjointCopy(prx_adjoint* adj, int k, bool save) {
+    prx_batch* b = adj->b;
+    const auto arrays = DynamicArrays(b);
+    for (std::size_t i = 0; i < arrays.size(); ++i) {
+        if (!arrays[i].second) continue;
+        void* ck = adj->ckpt[std::size_t(k)][i];
+        PRX_HIP(hipMemcpyAsync(save ? ck : arrays[i].first, save ? arrays[i].first : ck, arrays[i].second,
+                               hipMemcpyDeviceToDevice, b->stream));
+    }
+    return PRX_OK;
+}
+
+// K steps from the batch's state with a checkpoint every S steps; the batch
+// is left at the final state.
+int AdjointForward(prx_adjoint* adj, const prx::StepParams& p) {
+    prx_batch* b = adj->b;
+    const int nw = int(b->worlds.size());
+    hipStream_t st = b->stream;
+    PRX_HIP(hipEventRecord(b->ev0, st));
+    for (int k = 0; k < adj->K; ++k) {
+        if (k % adj->S == 0)
+            if (int e = AdjointCopy(adj, k / adj->S, true)) return e;
+        const hipError_t le = prx_launch_step(&b->d, &p, 0, nw, st);
+        if (le != hipSuccess) return Fail(PRX_E_DEVICE, std::string("launch: ") + hipGetErrorString(le));
+    }
+    PRX_HIP(hipStreamSynchronize(st));
+    if (int e = CheckStatus(b)) return e;
+    adj->forwarded = true;
+    return PRX_OK;
+}
+
+// The reversed steps from the adjoint of the final positions (x, y, angle)
+// and velocities (vx, vy, w), segment by segment from the checkpoints; the
+// batch is left at its starting state.
+int AdjointBackward(prx_adjoint* adj, const prx::StepParams& p, const std::vector<float4>& xbar,
+                    const std::vector<float4>& vbar, float* grad_velocity, float* grad_friction,
+                    float* grad_restitution) {
+    prx_batch* b = adj->b;
+    const int nw = int(b->worlds.size());
+    const std::size_t B = b->totB, S = b->totS;
+    hipStream_t st = b->stream;
+    PRX_HIP(hipMemcpy(adj->a.xbar, xbar.data(), B * sizeof(float4), hipMemcpyHostToDevice));
+    PRX_HIP(hipMemcpy(adj->a.vbar, vbar.data(), B * sizeof(float4), hipMemcpyHostToDevice));
+    PRX_HIP(hipMemset(adj->a.lambar, 0, b->totC * sizeof(float4)));
+    PRX_HIP(hipMemset(adj->a.fbar, 0, S * sizeof(float)));
+    PRX_HIP(hipMemset(adj->a.ebar, 0, S * sizeof(float)));
+    PRX_HIP(hipMemset(adj->a.status, 0, sizeof(int)));
+    PRX_HIP(hipMemset(adj->a.diag, 0, 4 * sizeof(float)));
+    {
+        const std::size_t J = std::max<std::size_t>(b->totJ, 1);
+        PRX_HIP(hipMemset(adj->a.lamc, 0, 2 * b->totC * sizeof(float4)));
+        PRX_HIP(hipMemset(adj->a.jcarry, 0, J * sizeof(float4)));
+        PRX_HIP(hipMemset(adj->a.tbar, 0, J * sizeof(float)));
+    }
+    for (int seg = adj->nseg - 1; seg >= 0; --seg) {
+        if (int e = AdjointCopy(adj, seg, false)) return e;
+        const int n = std::min(adj->S, adj->K - seg * adj->S);
+        for (int s = 0; s < n; ++s) {
+            prx::DeviceState dd = b->d;
+            dd.tape = adj->tapes[std::size_t(s)];
+            const hipError_t le = prx_launch_step_tape(&dd, &p, nw, st);
+            if (le != hipSuccess) return Fail(PRX_E_DEVICE, std::string("launch: ") + hipGetErrorString(le));
+        }
+        for (int s = n - 1; s >= 0; --s) {
+            const hipError_t le =
+                adj->general ? prx_launch_adjoint_general(&b->d, &adj->tapes[std::size_t(s)], &adj->a, &p, nw, st)
+                             : prx_launch_adjoint_step(&b->d, &adj->tapes[std::size_t(s)], &adj->a, &p, nw, st);
+            if (le != hipSuccess) return Fail(PRX_E_DEVICE, std::string("launch: ") + hipGetErrorString(le));
+        }
+    }
+    PRX_HIP(hipEventRecord(b->ev1, st));
+    b->last_launches = 2 * adj->K;
+    PRX_HIP(hipStreamSynchronize(st));
+    adj->forwarded = false;
+    int status = 0;
+    PRX_HIP(hipMemcpy(&status, adj->a.status, sizeof(int), hipMemcpyDeviceToHost));
+    if (grad_velocity) {
+        std::vector<float4> vb(B);
+        PRX_HIP(hipMemcpy(vb.data(), adj->a.vbar, B * sizeof(float4), hipMemcpyDeviceToHost));
+        for (std::size_t i = 0; i < B; ++i) {
+            grad_velocity[3 * i] = vb[i].x;
+            grad_velocity[3 * i + 1] = vb[i].y;
+            grad_velocity[3 * i + 2] = vb[i].z;
+        }
+    }
+    if (grad_friction) PRX_HIP(hipMemcpy(grad_friction, adj->a.fbar, S * sizeof(float), hipMemcpyDeviceToHost));
+    if (grad_restitution)
+        PRX_HIP(hipMemcpy(grad_restitution, adj->a.ebar, S * sizeof(float), hipMemcpyDeviceToHost));
+    if (int e = AdjointCopy(adj, 0, false)) return e;
+    PRX_HIP(hipStreamSynchronize(st));
+    for (prx_world* w : b->worlds) w->host_stale = true;
+    if (status) return Fail(PRX_E_UNSUPPORTED, "a contact the adjoint does not model (two-point manifold or face B)");
+    return PRX_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -2942,49 +3054,18 @@ void prx_adjoint_destroy(prx_adjoint* adj) { delete adj; }
 int prx_adjoint_run(prx_adjoint* adj, const prx_step_conf* conf, const float* targets, float* loss,
                     float* grad_velocity, float* grad_friction, float* grad_restitution) {
     if (!adj || !conf || !targets) return Fail(PRX_E_INVALID_ARGUMENT, "null argument");
-    if (conf->do_toi) return Fail(PRX_E_UNSUPPORTED, "the adjoint does not differentiate the TOI phase (doToi = true)");
-    if (int(conf->reg_velocity_iters) > adj->itV || int(conf->reg_position_iters) > adj->itP)
-        return Fail(PRX_E_INVALID_ARGUMENT, "iteration counts above those the adjoint was created for");
+    prx::StepParams p;
+    if (int e = AdjointParams(adj, conf, p)) return e;
     prx_batch* b = adj->b;
-    for (prx_world* w : b->worlds)
-        if (!w) return Fail(PRX_E_WRONG_STATE, "a member world left the batch");
     const int nw = int(b->worlds.size());
-    const std::size_t B = b->totB, S = b->totS;
-    hipStream_t st = b->stream;
-    prx::StepParams p = ToParams(*conf);
-    p.ldsBytes = std::uint32_t(b->lds_bytes);
-    p.coopMin = 0xffffffffu;  // every island on one lane (the recorded form)
-    const auto arrays = DynamicArrays(b);
-    auto save = [&](int k) -> int {
-        for (std::size_t i = 0; i < arrays.size(); ++i)
-            if (arrays[i].second)
-                PRX_HIP(hipMemcpyAsync(adj->ckpt[std::size_t(k)][i], arrays[i].first, arrays[i].second,
-                                       hipMemcpyDeviceToDevice, st));
-        return PRX_OK;
-    };
-    auto restore = [&](int k) -> int {
-        for (std::size_t i = 0; i < arrays.size(); ++i)
-            if (arrays[i].second)
-                PRX_HIP(hipMemcpyAsync(arrays[i].first, adj->ckpt[std::size_t(k)][i], arrays[i].second,
-                                       hipMemcpyDeviceToDevice, st));
-        return PRX_OK;
-    };
-    // Forward with checkpoints.
-    PRX_HIP(hipEventRecord(b->ev0, st));
-    for (int k = 0; k < adj->K; ++k) {
-        if (k % adj->S == 0)
-            if (int e = save(k / adj->S)) return e;
-        const hipError_t le = prx_launch_step(&b->d, &p, 0, nw, st);
-        if (le != hipSuccess) return Fail(PRX_E_DEVICE, std::string("launch: ") + hipGetErrorString(le));
-    }
+    const std::size_t B = b->totB;
+    if (int e = AdjointForward(adj, p)) return e;
     // Loss and its gradient with respect to the final positions.
     std::vector<float4> pos(B);
     std::vector<std::uint32_t> flags(B);
-    PRX_HIP(hipStreamSynchronize(st));
-    if (int e = CheckStatus(b)) return e;
     PRX_HIP(hipMemcpy(pos.data(), b->d.pos1, B * sizeof(float4), hipMemcpyDeviceToHost));
     PRX_HIP(hipMemcpy(flags.data(), b->d.bflags, B * sizeof(std::uint32_t), hipMemcpyDeviceToHost));
-    std::vector<float4> xbar(B, float4{0, 0, 0, 0});
+    std::vector<float4> xbar(B, float4{0, 0, 0, 0}), vbar(B, float4{0, 0, 0, 0});
     for (int w = 0; w < nw; ++w) {
         const auto& h = b->hdr[std::size_t(w)];
         double l = 0.0;
@@ -2997,60 +3078,51 @@ int prx_adjoint_run(prx_adjoint* adj, const prx_step_conf* conf, const float* ta
         }
         if (loss) loss[w] = float(l);
     }
-    PRX_HIP(hipMemcpy(adj->a.xbar, xbar.data(), B * sizeof(float4), hipMemcpyHostToDevice));
-    PRX_HIP(hipMemset(adj->a.vbar, 0, B * sizeof(float4)));
-    PRX_HIP(hipMemset(adj->a.lambar, 0, b->totC * sizeof(float4)));
-    PRX_HIP(hipMemset(adj->a.fbar, 0, S * sizeof(float)));
-    PRX_HIP(hipMemset(adj->a.ebar, 0, S * sizeof(float)));
-    PRX_HIP(hipMemset(adj->a.status, 0, sizeof(int)));
-    PRX_HIP(hipMemset(adj->a.diag, 0, 4 * sizeof(float)));
-    {
-        const std::size_t J = std::max<std::size_t>(b->totJ, 1);
-        PRX_HIP(hipMemset(adj->a.lamc, 0, 2 * b->totC * sizeof(float4)));
-        PRX_HIP(hipMemset(adj->a.jcarry, 0, J * sizeof(float4)));
-        PRX_HIP(hipMemset(adj->a.tbar, 0, J * sizeof(float)));
-    }
-    // Backward, segment by segment: recompute the segment from its
-    // checkpoint recording every step, then reverse its steps.
-    for (int seg = adj->nseg - 1; seg >= 0; --seg) {
-        if (int e = restore(seg)) return e;
-        const int n = std::min(adj->S, adj->K - seg * adj->S);
-        for (int s = 0; s < n; ++s) {
-            prx::DeviceState dd = b->d;
-            dd.tape = adj->tapes[std::size_t(s)];
-            const hipError_t le = prx_launch_step_tape(&dd, &p, nw, st);
-            if (le != hipSuccess) return Fail(PRX_E_DEVICE, std::string("launch: ") + hipGetErrorString(le));
-        }
-        for (int s = n - 1; s >= 0; --s) {
-            const hipError_t le =
-                adj->general ? prx_launch_adjoint_general(&b->d, &adj->tapes[std::size_t(s)], &adj->a, &p, nw, st)
-                             : prx_launch_adjoint_step(&b->d, &adj->tapes[std::size_t(s)], &adj->a, &p, nw, st);
-            if (le != hipSuccess) return Fail(PRX_E_DEVICE, std::string("launch: ") + hipGetErrorString(le));
-        }
-    }
-    PRX_HIP(hipEventRecord(b->ev1, st));
-    b->last_launches = 2 * adj->K;
-    PRX_HIP(hipStreamSynchronize(st));
-    int status = 0;
-    PRX_HIP(hipMemcpy(&status, adj->a.status, sizeof(int), hipMemcpyDeviceToHost));
-    if (grad_velocity) {
-        std::vector<float4> vb(B);
-        PRX_HIP(hipMemcpy(vb.data(), adj->a.vbar, B * sizeof(float4), hipMemcpyDeviceToHost));
+    return AdjointBackward(adj, p, xbar, vbar, grad_velocity, grad_friction, grad_restitution);
+}
+
+int prx_adjoint_forward(prx_adjoint* adj, const prx_step_conf* conf, float* final_state) {
+    if (!adj || !conf) return Fail(PRX_E_INVALID_ARGUMENT, "null argument");
+    prx::StepParams p;
+    if (int e = AdjointParams(adj, conf, p)) return e;
+    if (int e = AdjointForward(adj, p)) return e;
+    if (final_state) {
+        prx_batch* b = adj->b;
+        const std::size_t B = b->totB;
+        std::vector<float4> pos(B), vel(B);
+        PRX_HIP(hipMemcpy(pos.data(), b->d.pos1, B * sizeof(float4), hipMemcpyDeviceToHost));
+        PRX_HIP(hipMemcpy(vel.data(), b->d.vel, B * sizeof(float4), hipMemcpyDeviceToHost));
         for (std::size_t i = 0; i < B; ++i) {
-            grad_velocity[3 * i] = vb[i].x;
-            grad_velocity[3 * i + 1] = vb[i].y;
-            grad_velocity[3 * i + 2] = vb[i].z;
+            float* o = final_state + 6 * i;
+            o[0] = pos[i].x;
+            o[1] = pos[i].y;
+            o[2] = pos[i].z;
+            o[3] = vel[i].x;
+            o[4] = vel[i].y;
+            o[5] = vel[i].z;
         }
     }
-    if (grad_friction) PRX_HIP(hipMemcpy(grad_friction, adj->a.fbar, S * sizeof(float), hipMemcpyDeviceToHost));
-    if (grad_restitution)
-        PRX_HIP(hipMemcpy(grad_restitution, adj->a.ebar, S * sizeof(float), hipMemcpyDeviceToHost));
-    // The batch is left at its starting state.
-    if (int e = restore(0)) return e;
-    PRX_HIP(hipStreamSynchronize(st));
-    for (prx_world* w : b->worlds) w->host_stale = true;
-    if (status) return Fail(PRX_E_UNSUPPORTED, "a contact the adjoint does not model (two-point manifold or face B)");
     return PRX_OK;
+}
+
+int prx_adjoint_backward(prx_adjoint* adj, const prx_step_conf* conf, const float* state_bar, float* grad_velocity,
+                         float* grad_friction, float* grad_restitution) {
+    if (!adj || !conf || !state_bar) return Fail(PRX_E_INVALID_ARGUMENT, "null argument");
+    if (!adj->forwarded) return Fail(PRX_E_WRONG_STATE, "no forward pass to reverse");
+    prx::StepParams p;
+    if (int e = AdjointParams(adj, conf, p)) return e;
+    const std::size_t B = adj->b->totB;
+    std::vector<float4> xbar(B), vbar(B);
+    std::vector<std::uint32_t> flags(B);
+    PRX_HIP(hipMemcpy(flags.data(), adj->b->d.bflags, B * sizeof(std::uint32_t), hipMemcpyDeviceToHost));
+    for (std::size_t i = 0; i < B; ++i) {
+        const float* s = state_bar + 6 * i;
+        // Bodies that do not move carry no adjoint (their state is fixed).
+        const bool dyn = (flags[i] & prx::kAccelerable) != 0;
+        xbar[i] = dyn ? float4{s[0], s[1], s[2], 0.0f} : float4{0, 0, 0, 0};
+        vbar[i] = dyn ? float4{s[3], s[4], s[5], 0.0f} : float4{0, 0, 0, 0};
+    }
+    return AdjointBackward(adj, p, xbar, vbar, grad_velocity, grad_friction, grad_restitution);
 }
 
 int prx_adjoint_use_general(prx_adjoint* adj, int on) {
