@@ -3,6 +3,9 @@
 // it through ctypes.  The world description and the state records use the same
 // little-endian layout as oracle/ref_driver.cpp writes, so the reference's own
 // description can be fed in and the outputs digested identically.
+// tangent.hpp first: its std:: overloads must be visible where the
+// restatement's templates name std::sqrt and the like.
+#include "tangent.hpp"
 #include "cpu_ref.hpp"
 
 #include <chrono>
@@ -301,10 +304,28 @@ void WriteState(const Handle<R>& h, Out& o) {
     if (!w.events.empty()) o.put(w.events.data(), w.events.size() * 4);
 }
 
+// precision 32: float, 64: double; 33 and 65: the same with forward-mode
+// derivatives in double (oracle::Tf, oracle::Td); 34: float with derivatives
+// in float (oracle::Tff).
+constexpr int kTangent = 33, kTangent64 = 65, kTangentF = 34;
+
+bool IsTangent(int precision) {
+    return precision == kTangent || precision == kTangent64 || precision == kTangentF;
+}
+
 struct AnyHandle {
     int precision = 32;
     void* h = nullptr;
 };
+
+template <class F>
+void Visit(AnyHandle* a, F&& f) {
+    if (a->precision == 64) f(*static_cast<Handle<double>*>(a->h));
+    else if (a->precision == kTangent) f(*static_cast<Handle<oracle::Tf>*>(a->h));
+    else if (a->precision == kTangent64) f(*static_cast<Handle<oracle::Td>*>(a->h));
+    else if (a->precision == kTangentF) f(*static_cast<Handle<oracle::Tff>*>(a->h));
+    else f(*static_cast<Handle<float>*>(a->h));
+}
 
 template <class R>
 void DoStep(Handle<R>& h) {
@@ -535,6 +556,33 @@ void Widen(const Handle<S>& s, Handle<D>& d) {
     d.steps = s.steps;
 }
 
+// Sets the derivative of one parameter of a dual-number world (the kinds of
+// oracle_tangent_seed).
+template <class W>
+int SeedTangent(W& w, int kind, std::uint32_t index, int comp, double value) {
+    switch (kind) {
+        case 0:
+            if (index >= w.bodies.size() || comp < 0 || comp > 2) return -1;
+            (comp == 0 ? w.bodies[index].vel.lin.x : comp == 1 ? w.bodies[index].vel.lin.y : w.bodies[index].vel.ang)
+                .d = decltype(w.bodies[index].vel.ang.d)(value);
+            return 0;
+        case 1:
+            if (index >= w.shapes.size()) return -1;
+            w.shapes[index].friction.d = decltype(w.shapes[index].friction.d)(value);
+            return 0;
+        case 2:
+            if (index >= w.shapes.size()) return -1;
+            w.shapes[index].restitution.d = decltype(w.shapes[index].restitution.d)(value);
+            return 0;
+        case 3:
+            if (index >= w.joints.size()) return -1;
+            w.joints[index].maxMotorTorque.d = decltype(w.joints[index].maxMotorTorque.d)(value);
+            return 0;
+        default:
+            return -1;
+    }
+}
+
 }  // namespace
 
 // A world edit between steps: kind 'L' SetLocation(body, (a, b)), 'A'
@@ -556,7 +604,8 @@ void Edit(Handle<R>& h, int kind, std::uint32_t body, float a, float b, float c)
 extern "C" {
 
 // Creates an oracle world from a description + conf record (layout of
-// ref_driver.cpp's WriteDescription/WriteConf).  precision: 32 or 64.
+// ref_driver.cpp's WriteDescription/WriteConf).  precision: 32, 64 or 33
+// (float with forward-mode derivatives, seeded by oracle_tangent_seed).
 void* oracle_create(const unsigned char* desc, size_t len, int precision) {
     In in{desc, len};
     auto* a = new AnyHandle;
@@ -565,7 +614,20 @@ void* oracle_create(const unsigned char* desc, size_t len, int precision) {
         auto* h = new Handle<double>;
         Load(*h, in);
         a->h = h;
+    } else if (precision == kTangent) {
+        auto* h = new Handle<oracle::Tf>;
+        Load(*h, in);
+        a->h = h;
+    } else if (precision == kTangent64) {
+        auto* h = new Handle<oracle::Td>;
+        Load(*h, in);
+        a->h = h;
+    } else if (precision == kTangentF) {
+        auto* h = new Handle<oracle::Tff>;
+        Load(*h, in);
+        a->h = h;
     } else {
+        a->precision = 32;
         auto* h = new Handle<float>;
         Load(*h, in);
         a->h = h;
@@ -577,7 +639,7 @@ void* oracle_create(const unsigned char* desc, size_t len, int precision) {
 // stepping from the float world's current state); null for a double world.
 void* oracle_widen(void* p) {
     auto* a = static_cast<AnyHandle*>(p);
-    if (!a || a->precision == 64) return nullptr;
+    if (!a || a->precision != 32) return nullptr;
     auto* h = new Handle<double>;
     Widen(*static_cast<Handle<float>*>(a->h), *h);
     auto* out = new AnyHandle;
@@ -589,15 +651,13 @@ void* oracle_widen(void* p) {
 void oracle_destroy(void* p) {
     auto* a = static_cast<AnyHandle*>(p);
     if (!a) return;
-    if (a->precision == 64) delete static_cast<Handle<double>*>(a->h);
-    else delete static_cast<Handle<float>*>(a->h);
+    Visit(a, [](auto& h) { delete &h; });
     delete a;
 }
 
 int oracle_edit(void* p, int kind, std::uint32_t body, float a, float b, float c) {
     auto* an = static_cast<AnyHandle*>(p);
-    if (an->precision == 64) Edit(*static_cast<Handle<double>*>(an->h), kind, body, a, b, c);
-    else Edit(*static_cast<Handle<float>*>(an->h), kind, body, a, b, c);
+    Visit(an, [&](auto& h) { Edit(h, kind, body, a, b, c); });
     return 0;
 }
 
@@ -607,15 +667,10 @@ unsigned char* oracle_step(void* p, int n, int record, size_t* outlen) {
     auto* a = static_cast<AnyHandle*>(p);
     Out o;
     for (int i = 0; i < n; ++i) {
-        if (a->precision == 64) {
-            auto& h = *static_cast<Handle<double>*>(a->h);
+        Visit(a, [&](auto& h) {
             DoStep(h);
             if (record) WriteState(h, o);
-        } else {
-            auto& h = *static_cast<Handle<float>*>(a->h);
-            DoStep(h);
-            if (record) WriteState(h, o);
-        }
+        });
     }
     return Copy(o.buf, outlen);
 }
@@ -624,8 +679,7 @@ unsigned char* oracle_step(void* p, int n, int record, size_t* outlen) {
 unsigned char* oracle_state(void* p, size_t* outlen) {
     auto* a = static_cast<AnyHandle*>(p);
     Out o;
-    if (a->precision == 64) WriteState(*static_cast<Handle<double>*>(a->h), o);
-    else WriteState(*static_cast<Handle<float>*>(a->h), o);
+    Visit(a, [&](auto& h) { WriteState(h, o); });
     return Copy(o.buf, outlen);
 }
 
@@ -644,8 +698,43 @@ void oracle_bodies_f64(void* p, double* out) {
             out[k++] = double(b.vel.ang);
         }
     };
-    if (a->precision == 64) fill(static_cast<Handle<double>*>(a->h)->world);
-    else fill(static_cast<Handle<float>*>(a->h)->world);
+    Visit(a, [&](auto& h) { fill(h.world); });
+}
+
+// Forward-mode seed of a world created with precision 33, 34 or 65: sets the derivative
+// of one parameter (kind 0: body `index`'s velocity component `comp` (0 vx,
+// 1 vy, 2 w); 1: shape `index`'s friction; 2: its restitution; 3: joint
+// `index`'s maximum motor torque) to `value`.  Returns 0, or -1 for a bad
+// handle or index.
+int oracle_tangent_seed(void* p, int kind, std::uint32_t index, int comp, double value) {
+    auto* a = static_cast<AnyHandle*>(p);
+    if (!a || !IsTangent(a->precision)) return -1;
+    int rc = -1;
+    Visit(a, [&](auto& h) {
+        if constexpr (!std::is_arithmetic_v<std::decay_t<decltype(h.world.bodies[0].vel.ang)>>)
+            rc = SeedTangent(h.world, kind, index, comp, value);
+    });
+    return rc;
+}
+
+// Derivatives of the body state of a world created with precision 33, 34 or 65:
+// out[b*6 + {x, y, angle, vx, vy, w}] (widened exactly).
+void oracle_tangent_bodies(void* p, double* out) {
+    auto* a = static_cast<AnyHandle*>(p);
+    if (!a || !IsTangent(a->precision)) return;
+    Visit(a, [&](auto& h) {
+        if constexpr (!std::is_arithmetic_v<std::decay_t<decltype(h.world.bodies[0].vel.ang)>>) {
+            size_t k = 0;
+            for (const auto& b : h.world.bodies) {
+                out[k++] = double(b.pos1.lin.x.d);
+                out[k++] = double(b.pos1.lin.y.d);
+                out[k++] = double(b.pos1.ang.d);
+                out[k++] = double(b.vel.lin.x.d);
+                out[k++] = double(b.vel.lin.y.d);
+                out[k++] = double(b.vel.ang.d);
+            }
+        }
+    });
 }
 
 void oracle_free(void* p) { std::free(p); }

@@ -50,15 +50,16 @@ __global__ __launch_bounds__(64) void k_adjoint_step(DeviceState d, Tape t, AdjS
     const int4* isl = t.isl + bo + w;
     const int nslots = nisl > 0 ? isl[nisl - 1].y : 0;
     // Impulse adjoints carried in from the next step, by contact id: the
-    // step's final impulses warm-started the next step only if they reached
-    // it unchanged (the contact kept touching; the manifold update carried
-    // them by ContactFeature).
+    // step's final impulses warm-started the next step where the manifold
+    // update after the solve carried them (by ContactFeature, else from the
+    // nearest point; the tape's carry map says which).  The next step wrote
+    // nothing for contacts it did not solve or that started afresh.
     for (int e = lane; e < nslots; e += kWave) {
         const float4* vc = t.vc + (std::size_t(co) + e) * prx::kTapeVc;
         const std::uint32_t id = __float_as_uint(vc[10].y);
         const float4 lb = a.lambar[co + int(id)];
-        const float4 fin = vc[14];
-        const bool carried = lb.z == fin.x && lb.w == fin.y;
+        const std::uint32_t cm = t.cmap[co + int(id)];
+        const bool carried = ((cm >> 8) & 0xfu) >= 1u && (cm & 0xfu) == 0u;
         a.lamtmp[co + e] = carried ? float2{lb.x, lb.y} : float2{0.0f, 0.0f};
     }
     __syncthreads();
@@ -273,9 +274,11 @@ __global__ __launch_bounds__(64) void k_adjoint_step(DeviceState d, Tape t, AdjS
                 a.cbar[slot * 3 + 1] = float4{rBbar.x, rBbar.y, tbar.x, tbar.y};
             }
             // Impulses as carried from the previous step (dtRatio scaling is
-            // part of the record: warm start uses them as stored).
+            // part of the record: warm start uses them as stored); nothing
+            // for a contact created before this step's solve.
             const std::uint32_t id = __float_as_uint(vc[10].y);
-            a.lambar[co + int(id)] = float4{lb.x, lb.y, lnInit, ltInit};
+            const bool fresh = (t.cmap[co + int(id)] & prx::kCarryFresh) != 0u;
+            a.lambar[co + int(id)] = fresh ? float4{0.0f, 0.0f, 0.0f, 0.0f} : float4{lb.x, lb.y, lnInit, ltInit};
         }
         // Constraint creation, per slot.
         for (int e = rng.x; e < rng.y; ++e) {
@@ -493,26 +496,23 @@ __global__ __launch_bounds__(64) void k_adjoint_general(DeviceState d, Tape t, A
     auto jcrec = [&](int e) { return t.jc + (std::size_t(jo) + std::size_t(e)) * prx::kTapeJc; };
 
     // Impulse adjoints carried in from the next step.  Contacts: by pool id,
-    // point by point where the next step started from this step's final
-    // impulses (the manifold update carried them).  Joints: by joint.
+    // each of the next step's points to the point of this step its impulses
+    // came from in the manifold update after the solve (the tape's carry
+    // map: by ContactFeature, else the nearest point; two points may share
+    // one source).  Joints: by joint.
     for (int e = lane; e < nslots; e += kWave) {
         const float4* vc = vcrec(e);
         const std::uint32_t id = __float_as_uint(vc[10].y);
         const float4 lb = a.lamc[(std::size_t(co) + id) * 2];
-        const float4 nv = a.lamc[(std::size_t(co) + id) * 2 + 1];
-        const float4 fin = vc[14];
-        const float fn[2] = {fin.x, fin.z}, ft[2] = {fin.y, fin.w};
-        const float nn[2] = {nv.x, nv.z}, nt[2] = {nv.y, nv.w};
+        const std::uint32_t cm = t.cmap[std::size_t(co) + id];
+        const std::uint32_t nnext = (cm >> 8) & 0xfu;
         const float bn[2] = {lb.x, lb.z}, bt[2] = {lb.y, lb.w};
         float r[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        for (int k = 0; k < 2; ++k) {
-            for (int q = 0; q < 2; ++q) {
-                const int j = (q == 0) ? k : 1 - k;
-                if (nn[j] == fn[k] && nt[j] == ft[k]) {
-                    r[2 * k] = bn[j];
-                    r[2 * k + 1] = bt[j];
-                    break;
-                }
+        for (std::uint32_t j = 0; j < 2; ++j) {
+            const std::uint32_t k = (cm >> (4 * j)) & 0xfu;
+            if (j < nnext && k < 2u) {
+                r[2 * k] += bn[j];
+                r[2 * k + 1] += bt[j];
             }
         }
         a.lam4[std::size_t(co) + e] = float4{r[0], r[1], r[2], r[3]};
@@ -558,21 +558,30 @@ __global__ __launch_bounds__(64) void k_adjoint_general(DeviceState d, Tape t, A
                 __syncthreads();
             }
             for (int e = rng.y - 1; e >= rng.x; --e) {
+                const std::size_t slot = std::size_t(co) + e;
                 const float4* vc = vcrec(e);
                 const Pc pc = load_pc(vc + 17);
-                const float4* g = t.ngs + ((std::size_t(co) + e) * std::size_t(t.itP) + it) * 2;
-                float x[6], yb[6];
+                const float4* g = t.ngs + (slot * std::size_t(t.itP) + it) * 2;
+                // Positions before the update, then the manifold's local
+                // points (their adjoints collect for the constraint creation).
+                float x[10], yb[6];
                 load3(g[0], x);
                 load3(g[1], x + 3);
+                x[6] = pc.m.p[0].lp.x;
+                x[7] = pc.m.p[0].lp.y;
+                x[8] = pc.m.p[1].lp.x;
+                x[9] = pc.m.p[1].lp.y;
                 load3(xbar[pc.bA], yb);
                 load3(xbar[pc.bB], yb + 3);
-                const float r = lane_vjp<6, 6>(x, yb, [&](dl::D* in, dl::D* out) {
-                    dl::solve_pc(pc.m, pc.totalRadius, pc.A, pc.B, in, c);
+                const float r = lane_vjp<10, 6>(x, yb, [&](dl::D* in, dl::D* out) {
+                    const dl::V lpd[2] = {dl::V{in[6], in[7]}, dl::V{in[8], in[9]}};
+                    dl::solve_pc(pc.m, pc.totalRadius, pc.A, pc.B, in, c, lpd);
                     for (int k = 0; k < 6; ++k) out[k] = in[k];
                 });
                 __syncthreads();
                 if (lane < 3) comp(xbar[pc.bA], lane) = r;
                 else if (lane < 6) comp(xbar[pc.bB], lane - 3) = r;
+                else if (lane < 10) a.fld[slot * prx::kAdjFields + 28 + (lane - 6)] += r;
                 __syncthreads();
                 if (lane == 0) atomicAdd(a.diag + 2, 1.0f);
             }
@@ -797,7 +806,7 @@ __global__ __launch_bounds__(64) void k_adjoint_general(DeviceState d, Tape t, A
             const Pc pc = load_pc(vc + 17);
             const std::uint32_t id = __float_as_uint(vc[10].y);
             const float rA = vc[10].w, rB = vc[11].x;
-            float x[18];
+            float x[22];
             load3(vc[15], x);
             load3(vc[16], x + 3);
             load3(vc[12], x + 6);
@@ -805,12 +814,17 @@ __global__ __launch_bounds__(64) void k_adjoint_general(DeviceState d, Tape t, A
             x[12] = vc[0].z;
             x[13] = vc[5].w;
             load4(vc[22], x + 14);
+            x[18] = pc.m.p[0].lp.x;
+            x[19] = pc.m.p[0].lp.y;
+            x[20] = pc.m.p[1].lp.x;
+            x[21] = pc.m.p[1].lp.y;
             float yb[27];
             for (int k = 0; k < 27; ++k) yb[k] = a.fld[slot * prx::kAdjFields + k];
-            const float r = lane_vjp<18, 27>(x, yb, [&](dl::D* in, dl::D* out) {
+            const float r = lane_vjp<22, 27>(x, yb, [&](dl::D* in, dl::D* out) {
                 const dl::X xfA = dl::get_xf(in[0], in[1], in[2], pc.A.lc);
                 const dl::X xfB = dl::get_xf(in[3], in[4], in[5], pc.B.lc);
-                const dl::WMd wm = dl::world_manifold(pc.m, xfA, rA, xfB, rB);
+                const dl::V lpd[2] = {dl::V{in[18], in[19]}, dl::V{in[20], in[21]}};
+                const dl::WMd wm = dl::world_manifold(pc.m, xfA, rA, xfB, rB, lpd);
                 const dl::Vcd v = dl::make_vc(wm, in[12], in[13], vc[0].w, pc.A, in, in + 6, pc.B, in + 3, in + 9,
                                               in + 14, c);
                 out[0] = v.normal.x; out[1] = v.normal.y; out[2] = v.friction;
@@ -842,10 +856,45 @@ __global__ __launch_bounds__(64) void k_adjoint_general(DeviceState d, Tape t, A
                     atomicAdd(a.ebar + ((eA > eB) ? sA : sB), r);
                 }
             } else if (lane < 18) {
-                (&a.lamc[(std::size_t(co) + id) * 2].x)[lane - 14] = r;
+                // Nothing for a contact created before this step's solve.
+                const bool fresh = (t.cmap[std::size_t(co) + id] & prx::kCarryFresh) != 0u;
+                (&a.lamc[(std::size_t(co) + id) * 2].x)[lane - 14] = fresh ? 0.0f : r;
                 (&a.lamc[(std::size_t(co) + id) * 2 + 1].x)[lane - 14] = (&vc[22].x)[lane - 14];
+            } else if (lane < 22) {
+                a.fld[slot * prx::kAdjFields + 28 + (lane - 18)] += r;
             }
             __syncthreads();
+            // The manifold's local points as made by the update before this
+            // step, at the same positions: points the clipping made move
+            // with the bodies.
+            const std::uint32_t mtype = pc.m.type;
+            const float4 cfs = vc[23];
+            if (mtype == kFaceA || mtype == kFaceB) {
+                const bool faceB = mtype == kFaceB;
+                const int sA = __float_as_int(vc[11].y), sB = __float_as_int(vc[11].z);
+                const Proxy s0 = shape_proxy(d, faceB ? sB : sA), s1 = shape_proxy(d, faceB ? sA : sB);
+                float xm[6], ybm[4];
+                load3(vc[15], xm);
+                load3(vc[16], xm + 3);
+                for (int k = 0; k < 4; ++k) ybm[k] = a.fld[slot * prx::kAdjFields + 28 + k];
+                const float rm = lane_vjp<6, 4>(xm, ybm, [&](dl::D* in, dl::D* out) {
+                    const dl::X xfA = dl::get_xf(in[0], in[1], in[2], pc.A.lc);
+                    const dl::X xfB = dl::get_xf(in[3], in[4], in[5], pc.B.lc);
+                    for (std::uint32_t k = 0; k < 2; ++k) {
+                        dl::V lp = dl::vc2(pc.m.p[k].lp);
+                        if (k < pc.m.n)
+                            lp = dl::manifold_local_point(s0, faceB ? xfB : xfA, s1, faceB ? xfA : xfB,
+                                                          __float_as_uint(k ? cfs.y : cfs.x), faceB, pc.m.ln,
+                                                          pc.m.p[k].lp);
+                        out[2 * k] = lp.x;
+                        out[2 * k + 1] = lp.y;
+                    }
+                });
+                __syncthreads();
+                if (lane < 3) comp(xbar[pc.bA], lane) += rm;
+                else if (lane < 6) comp(xbar[pc.bB], lane - 3) += rm;
+                __syncthreads();
+            }
             if (lane == 0) atomicAdd(a.diag + 0, 1.0f);
         }
     }

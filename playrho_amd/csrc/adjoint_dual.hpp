@@ -125,8 +125,10 @@ struct WMd {
     V p[2];
 };
 
-// world_manifold (WorldManifold.cpp:37-130).
-__device__ __forceinline__ WMd world_manifold(const Manifold& m, const X& xfA, float rA, const X& xfB, float rB) {
+// world_manifold (WorldManifold.cpp:37-130); lpd (when given): the
+// manifold's local points as dual values (else its constants).
+__device__ __forceinline__ WMd world_manifold(const Manifold& m, const X& xfA, float rA, const X& xfB, float rB,
+                                              const V* lpd = nullptr) {
     WMd w;
     w.n = 0;
     w.normal = V{cst(0), cst(0)};
@@ -134,7 +136,7 @@ __device__ __forceinline__ WMd world_manifold(const Manifold& m, const X& xfA, f
     w.p[1] = w.normal;
     if (m.type == kCircles) {
         const V pA = transform(vc2(m.lpt), xfA);
-        const V pB = transform(vc2(m.p[0].lp), xfB);
+        const V pB = transform(lpd ? lpd[0] : vc2(m.p[0].lp), xfB);
         const V normal = unit_vector(pB - pA, V2{1.0f, 0.0f});
         const V cA = pA + mul(cst(rA), normal);
         const V cB = pB - mul(cst(rB), normal);
@@ -149,7 +151,7 @@ __device__ __forceinline__ WMd world_manifold(const Manifold& m, const X& xfA, f
 #pragma unroll
         for (std::uint32_t i = 0; i < 2; ++i) {
             if (i >= m.n) continue;
-            const V clip = transform(vc2(m.p[i].lp), xfB);
+            const V clip = transform(lpd ? lpd[i] : vc2(m.p[i].lp), xfB);
             const V cA = clip + mul(cst(rA) - dot(clip - plane, normal), normal);
             const V cB = clip - mul(cst(rB), normal);
             w.p[i] = divs(cA + cB, 2.0f);
@@ -162,7 +164,7 @@ __device__ __forceinline__ WMd world_manifold(const Manifold& m, const X& xfA, f
 #pragma unroll
         for (std::uint32_t i = 0; i < 2; ++i) {
             if (i >= m.n) continue;
-            const V clip = transform(vc2(m.p[i].lp), xfA);
+            const V clip = transform(lpd ? lpd[i] : vc2(m.p[i].lp), xfA);
             const V cB = clip + mul(cst(rB) - dot(clip - plane, normal), normal);
             const V cA = clip - mul(cst(rA), normal);
             w.p[i] = divs(cA + cB, 2.0f);
@@ -424,9 +426,10 @@ struct PSMd {
     V normal, point;
     D sep;
 };
-// get_psm (PositionSolverManifold.cpp:42-121).
-__device__ __forceinline__ PSMd get_psm(const Manifold& m, std::uint32_t idx, const X& xfA, const X& xfB) {
-    const V plp = vc2(idx ? m.p[1].lp : m.p[0].lp);
+// get_psm (PositionSolverManifold.cpp:42-121); lpd as for world_manifold.
+__device__ __forceinline__ PSMd get_psm(const Manifold& m, std::uint32_t idx, const X& xfA, const X& xfB,
+                                        const V* lpd = nullptr) {
+    const V plp = lpd ? lpd[idx ? 1 : 0] : vc2(idx ? m.p[1].lp : m.p[0].lp);
     if (m.type == kCircles) {
         const V pA = transform(vc2(m.lpt), xfA);
         const V pB = transform(plp, xfB);
@@ -448,7 +451,7 @@ __device__ __forceinline__ PSMd get_psm(const Manifold& m, std::uint32_t idx, co
 
 // solve_pc (ContactSolver.cpp:494-643); pos: x, y, angle of A then B, updated.
 __device__ __forceinline__ void solve_pc(const Manifold& m, float totalRadius, const BodyC& A, const BodyC& B, D* pos,
-                                         const StepParams& c) {
+                                         const StepParams& c, const V* lpd = nullptr) {
     const float invMassTotal = A.invMass + B.invMass;
     struct Sol {
         V la;
@@ -479,13 +482,13 @@ __device__ __forceinline__ void solve_pc(const Manifold& m, float totalRadius, c
         pBa = pBa + s.ab;
     };
     if (m.n == 1) {
-        const PSMd psm0 = get_psm(m, 0, get_xf(pAl.x, pAl.y, pAa, A.lc), get_xf(pBl.x, pBl.y, pBa, B.lc));
+        const PSMd psm0 = get_psm(m, 0, get_xf(pAl.x, pAl.y, pAa, A.lc), get_xf(pBl.x, pBl.y, pBa, B.lc), lpd);
         apply(solve(psm0, pAl, pBl));
     } else if (m.n == 2) {
         const X xfA = get_xf(pAl.x, pAl.y, pAa, A.lc);
         const X xfB = get_xf(pBl.x, pBl.y, pBa, B.lc);
-        const PSMd psm0 = get_psm(m, 0, xfA, xfB);
-        const PSMd psm1 = get_psm(m, 1, xfA, xfB);
+        const PSMd psm0 = get_psm(m, 0, xfA, xfB, lpd);
+        const PSMd psm1 = get_psm(m, 1, xfA, xfB, lpd);
         if (almost_equal(psm0.sep.v, psm1.sep.v)) {
             const Sol s0 = solve(psm0, pAl, pBl);
             const Sol s1 = solve(psm1, pAl, pBl);
@@ -495,11 +498,11 @@ __device__ __forceinline__ void solve_pc(const Manifold& m, float totalRadius, c
             pBa = pBa + s0.ab + s1.ab;
         } else if (psm0.sep.v < psm1.sep.v) {
             apply(solve(psm0, pAl, pBl));
-            const PSMd p1 = get_psm(m, 1, get_xf(pAl.x, pAl.y, pAa, A.lc), get_xf(pBl.x, pBl.y, pBa, B.lc));
+            const PSMd p1 = get_psm(m, 1, get_xf(pAl.x, pAl.y, pAa, A.lc), get_xf(pBl.x, pBl.y, pBa, B.lc), lpd);
             apply(solve(p1, pAl, pBl));
         } else if (psm1.sep.v < psm0.sep.v) {
             apply(solve(psm1, pAl, pBl));
-            const PSMd p0 = get_psm(m, 0, get_xf(pAl.x, pAl.y, pAa, A.lc), get_xf(pBl.x, pBl.y, pBa, B.lc));
+            const PSMd p0 = get_psm(m, 0, get_xf(pAl.x, pAl.y, pAa, A.lc), get_xf(pBl.x, pBl.y, pBa, B.lc), lpd);
             apply(solve(p0, pAl, pBl));
         }
     }
@@ -509,6 +512,40 @@ __device__ __forceinline__ void solve_pc(const Manifold& m, float totalRadius, c
     pos[3] = pBl.x;
     pos[4] = pBl.y;
     pos[5] = pBa;
+}
+
+// A manifold point's local coordinates as a function of the two bodies'
+// transformations at the update that made the manifold (poly_manifold,
+// CollidePolygons in CollideShapes.cpp).  A point kept from the incident
+// edge is a vertex of the incident shape: constant.  A point the clipping
+// made (ContactFeature: vertex of the reference shape, face of the incident
+// one) lies where the incident edge crosses the side plane through the
+// reference face's end vertex; with d_i the signed distances of the incident
+// vertices from that plane, lp = v_a + (v_b - v_a) d_a / (d_a - d_b).
+// cf: the point's ContactFeature as stored; faceB: the manifold's reference
+// face is on B (the feature is stored flipped); ln: the reference face's
+// normal in its shape's frame; X0 (X1): the reference (incident) body's
+// transformation.
+__device__ __forceinline__ V manifold_local_point(const Proxy& s0, const X& X0, const Proxy& s1, const X& X1,
+                                                  std::uint32_t cf, bool faceB, V2 ln, V2 lp) {
+    const std::uint32_t u = faceB ? flip_cf(cf) : cf;
+    const std::uint32_t tA = u & 1u, tB = (u >> 1) & 1u, iA = (u >> 8) & 0xffu, iB = (u >> 16) & 0xffu;
+    if (!(tA == kVertex && tB == kFace) || iB >= s1.n) return vc2(lp);
+    std::uint32_t idx0 = s0.n;
+    for (std::uint32_t i = 0; i < s0.n; ++i) {
+        const V2 nr = s0.norm(i);
+        if (idx0 == s0.n && nr.x == ln.x && nr.y == ln.y) idx0 = i;
+    }
+    if (idx0 == s0.n) return vc2(lp);
+    const std::uint32_t idx0Next = mod_next(idx0, s0.n);
+    if (iA != idx0 && iA != idx0Next) return vc2(lp);
+    const V2 va = s1.vert(iB), vb = s1.vert(mod_next(iB, s1.n));
+    const V dir = rotate(vc2(revperp(s0.norm(idx0))), X0.q);
+    const V av = transform(vc2(s0.vert(iA)), X0);
+    const D da = dot(dir, transform(vc2(va), X1) - av);
+    const D db = dot(dir, transform(vc2(vb), X1) - av);
+    const D t = da / (da - db);
+    return V{cst(va.x) + (vb.x - va.x) * t, cst(va.y) + (vb.y - va.y) * t};
 }
 
 // ---------------------------------------------------------------------------

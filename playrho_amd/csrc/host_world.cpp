@@ -3017,6 +3017,7 @@ int prx_adjoint_create(prx_batch* b, int nsteps, int every, const prx_step_conf*
         PRX_HIP(A.alloc(t.jc, J * prx::kTapeJc));
         PRX_HIP(A.alloc(t.jgs, J * std::size_t(adj->itV) * 3));
         PRX_HIP(A.alloc(t.jngs, J * std::size_t(adj->itP) * 2));
+        PRX_HIP(A.alloc(t.cmap, std::max<std::size_t>(C, 1)));
         t.itV = adj->itV;
         t.itP = adj->itP;
     }

@@ -100,14 +100,15 @@ struct StepParams {
 // island-order constraint slot the velocity-constraint record as built plus
 // its geometry inputs, the inputs of every Gauss-Seidel update and of every
 // position-solver update, and the island table.
-constexpr int kTapeVc = 23;
+constexpr int kTapeVc = 24;
 constexpr int kTapeJc = 10;
 struct Tape {
     float4* vc;           // slot * kTapeVc: VelocityConstraint record [0..8], body positions [9],
                           // (vn0, contact id, manifold type, radius A) [10], (radius B, shape A, shape B, -) [11],
                           // velocities of A [12] and B [13] at creation, final impulses (n0, t0, n1, t1) [14],
                           // positions of A [15] and B [16] at creation, position-constraint record [17..21],
-                          // manifold impulses before the dtRatio scaling (n0, t0, n1, t1) [22]
+                          // manifold impulses before the dtRatio scaling (n0, t0, n1, t1) [22],
+                          // ContactFeatures of the manifold's points (bits) [23]
     float4* gs;           // (slot * itV + it) * 3: (vA, wA, point-0 tangent impulse), (vB, wB, point-0 normal
                           // impulse), impulses (n0, t0, n1, t1) before the update
     float4* ngs;          // (slot * itP + it) * 2: position A (w: separation), position B before the update
@@ -119,8 +120,14 @@ struct Tape {
                           // torque, previous limit state) [8], impulses before the set-up [9]
     float4* jgs;          // (joint slot * itV + it) * 3: velocities of A, B and the impulses before the update
     float4* jngs;         // (joint slot * itP + it) * 2: positions of A and B before the update
+    std::uint32_t* cmap;  // per world contact pool id: how the next step's manifold took this step's final
+                          // impulses: bits 0-3 (4-7) the point point 0 (1) took them from (kNoSource: none),
+                          // bits 8-11 the next step's point count; bit 16 (kCarryFresh): the contact was
+                          // created before this step's solve (its impulses owe nothing to the step before)
     int itV, itP;
 };
+constexpr std::uint32_t kNoSource = 0xfu;
+constexpr std::uint32_t kCarryFresh = 0x10000u;
 
 // Adjoint state of a batch (one backward step per launch).
 struct AdjState {
@@ -136,7 +143,8 @@ struct AdjState {
     float* diag;    // counters of the last run: slots reversed, of them with a restitution bias,
                     // position-solver updates reversed, velocity updates reversed
     // General form (polygons, two-point manifolds, joints):
-    float* fld;     // per slot kAdjFields: adjoints of the velocity-constraint values
+    float* fld;     // per slot kAdjFields: adjoints of the velocity-constraint values [0..26] and of
+                    // the manifold's local points (point 0 x, y, point 1 x, y) [28..31]
     float4* lam4;   // per slot: adjoint of the running impulses (n0, t0, n1, t1)
     float4* lamc;   // per contact pool id, 2 float4: adjoint of the raw impulses at the later step's start
                     // and those impulses' values
@@ -145,7 +153,7 @@ struct AdjState {
     float4* jcarry; // per joint: adjoint of the impulses the later step started from
     float* tbar;    // per joint: d loss / d maximum motor torque
 };
-constexpr int kAdjFields = 28;
+constexpr int kAdjFields = 32;
 constexpr int kAdjJointFields = 16;
 
 struct DeviceState {

@@ -153,14 +153,16 @@ __device__ __forceinline__ void store_manifold(const DeviceState& d, int gi, con
 // carried over by ContactFeature, else from the nearest old point.  Returns
 // the listener calls the update makes (:2703-2719): bit 0 begin-contact, bit 1
 // end-contact, bit 2 pre-solve with the manifold before the update (stored
-// to oldOut when given).
+// to oldOut when given).  mapOut (when given) receives which old point each
+// new point took its impulses from (prx::Tape::cmap's low bits).
 __device__ __forceinline__ std::uint32_t update_contact(const DeviceState& d, int bo, int gi, const StepParams& c,
-                                                        Manifold* oldOut = nullptr) {
+                                                        Manifold* oldOut = nullptr, std::uint32_t* mapOut = nullptr) {
     std::uint32_t flags = d.c_flags[gi];
     const Manifold old = load_manifold(d, gi);
     const bool oldTouching = (flags & prx::kTouching) != 0;
     bool newTouching = false;
     Manifold nm = empty_manifold();
+    std::uint32_t src = prx::kNoSource | (prx::kNoSource << 4);
     if (!(flags & prx::kSensor)) {
         const int2 bb = d.c_body[gi];
         const int2 ss = d.c_shape[gi];
@@ -181,6 +183,7 @@ __device__ __forceinline__ std::uint32_t update_contact(const DeviceState& d, in
                     found[i] = true;
                     nm.p[i].ni = old.p[j].ni;
                     nm.p[i].ti = old.p[j].ti;
+                    src = (src & ~(0xfu << (4 * i))) | (j << (4 * i));
                     done = true;
                 }
             }
@@ -197,11 +200,13 @@ __device__ __forceinline__ std::uint32_t update_contact(const DeviceState& d, in
                     least = dd;
                     nm.p[i].ni = old.p[j].ni;
                     nm.p[i].ti = old.p[j].ti;
+                    src = (src & ~(0xfu << (4 * i))) | (j << (4 * i));
                 }
             }
         }
     }
     store_manifold(d, gi, nm);
+    if (mapOut) *mapOut = src | (nm.n << 8);
     flags &= ~prx::kNeedsUpdating;
     std::uint32_t calls = 0;
     if (!oldTouching && newTouching) {
@@ -330,14 +335,28 @@ __device__ __forceinline__ int group_updates(const DeviceState& d, const WorldCt
     return carry;
 }
 
+// cmap (when given: a tape's carry map of the world) receives each updated
+// contact's carry map, by pool id, keeping its kCarryFresh bit.
+__device__ __forceinline__ void update_mapped(const DeviceState& d, const WorldCtx& x, int gi, const StepParams& c,
+                                              std::uint32_t* cmap) {
+    std::uint32_t m = 0;
+    update_contact(d, x.bo, gi, c, nullptr, &m);
+    std::uint32_t& e = cmap[d.c_id[gi]];
+    e = (e & prx::kCarryFresh) | m;
+}
+
 __device__ __forceinline__ std::uint32_t phase_update(const DeviceState& d, const WorldCtx& x, Counters& k,
-                                                      const StepParams& c, int ldsAvail) {
+                                                      const StepParams& c, int ldsAvail,
+                                                      std::uint32_t* cmap = nullptr) {
     const int nc = k.nc;
     if (x.ecap == 0) {
         const int n = group_updates(d, x, nc, ldsAvail);
         if (n >= 0) {
             const int* list = reinterpret_cast<const int*>(g_smem) + kUpdKeys + 1;
-            for (int r = x.lane; r < n; r += kWave) update_contact(d, x.bo, x.co + list[r], c);
+            for (int r = x.lane; r < n; r += kWave) {
+                if (cmap) update_mapped(d, x, x.co + list[r], c, cmap);
+                else update_contact(d, x.bo, x.co + list[r], c);
+            }
             __syncthreads();
             return std::uint32_t(n);
         }
@@ -351,10 +370,16 @@ __device__ __forceinline__ std::uint32_t phase_update(const DeviceState& d, cons
         if (x.ecap > 0) {
             Manifold old = empty_manifold();
             std::uint32_t calls = 0;
-            if (need) calls = update_contact(d, x.bo, x.co + i, c, &old);
+            std::uint32_t m = 0;
+            if (need) calls = update_contact(d, x.bo, x.co + i, c, &old, cmap ? &m : nullptr);
+            if (need && cmap) {
+                std::uint32_t& e = cmap[d.c_id[x.co + i]];
+                e = (e & prx::kCarryFresh) | m;
+            }
             ev_update(d, x, k, calls, need ? d.c_id[x.co + i] : 0u, old);
         } else if (need) {
-            update_contact(d, x.bo, x.co + i, c);
+            if (cmap) update_mapped(d, x, x.co + i, c, cmap);
+            else update_contact(d, x.bo, x.co + i, c);
         }
     }
     return updated;
@@ -4113,6 +4138,7 @@ __device__ __forceinline__ SolveSummary solve_islands(const DeviceState& d, cons
             t[16] = pB;
             store_pc(t + 17, m, bd.x, bd.y, rA + rB, A, B);
             t[22] = float4{m.p[0].ni, m.p[0].ti, m.p[1].ni, m.p[1].ti};
+            t[23] = float4{__uint_as_float(m.p[0].cf), __uint_as_float(m.p[1].cf), 0.0f, 0.0f};
         }
     }
     if constexpr (kTape) {
@@ -4458,6 +4484,12 @@ __device__ __forceinline__ void step_world(const DeviceState& d, int w, const St
     world_priority(x.nb);
     PhaseTimer T;
     T.begin(d.phase_cycles ? d.phase_cycles + std::size_t(w) * prx::kPhases : nullptr);
+    // Carry map of the recorded step: contacts existing before the step owe
+    // their impulses to the step before (kCarryFresh clear).
+    std::uint32_t* const cmap = kTape ? d.tape.cmap + x.co : nullptr;
+    if constexpr (kTape) {
+        for (int i = x.lane; i < k.nc; i += kWave) cmap[d.c_id[x.co + i]] = 0u;
+    }
 
     // A: proxies of newly attached shapes (CreateProxies, AabbTreeWorld.cpp:557-579).
     std::uint32_t created = 0;
@@ -4498,8 +4530,14 @@ __device__ __forceinline__ void step_world(const DeviceState& d, int w, const St
     }
     __syncthreads();
     T.mark(prx::kPhUpdate);
+    const int ncBeforeAdd = k.nc;
     stat(prx::kStPreAdded, phase_find_add(d, x, k, c, int(c.ldsBytes)));
     __syncthreads();
+    if constexpr (kTape) {
+        // Contacts created before the solve start from impulses of their own.
+        for (int i = ncBeforeAdd + x.lane; i < k.nc; i += kWave) cmap[d.c_id[x.co + i]] = prx::kCarryFresh;
+        __syncthreads();
+    }
     T.mark(prx::kPhFindAdd);
 
     if (c.dt != 0.0f) {
@@ -4554,9 +4592,16 @@ __device__ __forceinline__ void step_world(const DeviceState& d, int w, const St
         }
         __syncthreads();
         for (int b = x.lane; b < x.nb; b += kWave) d.bflags[x.bo + b] &= ~0x100u;
+        if constexpr (kTape) {
+            // Contacts the update leaves alone keep their points and impulses.
+            for (int i = x.lane; i < k.nc; i += kWave) {
+                std::uint32_t& e = cmap[d.c_id[x.co + i]];
+                e = (e & prx::kCarryFresh) | 0x10u | (((d.m_hdr[x.co + i].x >> 8) & 0xffu) << 8);
+            }
+        }
         __syncthreads();
         {
-            const std::uint32_t updated = phase_update(d, x, k, c, int(c.ldsBytes));
+            const std::uint32_t updated = phase_update(d, x, k, c, int(c.ldsBytes), cmap);
             stat(prx::kStRegUpdated, updated);
             stat(prx::kStRegSkipped, std::uint32_t(k.nc) - updated);
         }

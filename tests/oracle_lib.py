@@ -37,6 +37,10 @@ def lib():
         L.oracle_sincosf.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
         L.oracle_edit.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_float,
                                   ctypes.c_float, ctypes.c_float]
+        L.oracle_tangent_seed.restype = ctypes.c_int
+        L.oracle_tangent_seed.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_int,
+                                          ctypes.c_double]
+        L.oracle_tangent_bodies.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.oracle_time_batch.restype = ctypes.c_double
         L.oracle_time_batch.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int,
@@ -109,6 +113,31 @@ class OracleWorld:
         out = np.zeros(nbodies * 6, dtype=np.float64)
         lib().oracle_bodies_f64(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         return out.reshape(nbodies, 6)
+
+    # Forward mode (precision TANGENT: the float step as executed, TANGENT64:
+    # the double step), with the derivative of every value along its path in
+    # double.
+    SEED_KINDS = {"velocity": 0, "friction": 1, "restitution": 2, "motor_torque": 3}
+
+    def seed(self, kind, index, comp=0, value=1.0):
+        """Sets d(parameter) for one parameter: kind 'velocity' (comp 0 vx,
+        1 vy, 2 w), 'friction', 'restitution' or 'motor_torque'."""
+        assert self.precision in (TANGENT, TANGENT64, TANGENT_F)
+        rc = lib().oracle_tangent_seed(self.h, self.SEED_KINDS[kind], int(index), int(comp), float(value))
+        if rc != 0:
+            raise IndexError("bad tangent seed %s %d %d" % (kind, index, comp))
+
+    def tangent_f64(self, nbodies):
+        """d(x, y, angle, vx, vy, w) of every body, (nbodies, 6)."""
+        assert self.precision in (TANGENT, TANGENT64, TANGENT_F)
+        out = np.zeros(nbodies * 6, dtype=np.float64)
+        lib().oracle_tangent_bodies(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return out.reshape(nbodies, 6)
+
+
+TANGENT = 33  # oracle_create precision codes of the forward-mode worlds
+TANGENT64 = 65
+TANGENT_F = 34  # float values and float derivatives
 
 
 def split_dump(buf):
