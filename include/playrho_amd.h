@@ -321,8 +321,23 @@ int prx_step(prx_world* world, const prx_step_conf* conf, prx_step_stats* stats)
 int prx_world_get_bodies(prx_world* world, float* out, uint32_t* flags);
 /* GetTransformation(World&, BodyID) (WorldBody.hpp): {px, py, cos, sin}. */
 int prx_world_get_transformation(prx_world* world, uint32_t body, float out[4]);
-/* SetVelocity(World&, BodyID, Velocity) (WorldBody.hpp). */
+/* SetVelocity(World&, BodyID, Velocity) (WorldBody.cpp:263-268; Body::SetVelocity,
+ * Body.cpp:184-194): v = linear x, y, angular; a velocity other than zero is
+ * ignored by a body that is not speedable and wakes the others. */
 int prx_world_set_velocity(prx_world* world, uint32_t body, const float v[3]);
+/* Destroy(World&, JointID) (World.hpp:469; AabbTreeWorld.cpp:1215-1227): both
+ * bodies wake; the id is reused by the next joint created.  An id already
+ * destroyed is left alone. */
+int prx_world_destroy_joint(prx_world* world, uint32_t joint);
+/* Destroy(World&, BodyID) (World.hpp:373; AabbTreeWorld.cpp:1062-1104): the
+ * body's joints, then its contacts (touching ones end: with contact events on,
+ * their end-contact calls come first in the next step's record; bodies they
+ * held wake), then its proxies.  The body keeps its id and is no longer
+ * stepped; edits and getters on it return PRX_E_WRONG_STATE, and the state
+ * record writes it as zeros with flags 0x80.  Creating a body afterwards and
+ * saving the world's description are PRX_E_UNSUPPORTED (the reference would
+ * reuse the id at the end of its body list). */
+int prx_world_destroy_body(prx_world* world, uint32_t body);
 /* SetAcceleration(World&, BodyID, LinearAcceleration2, AngularAcceleration)
  * (d2/WorldBody.hpp; Body::SetAcceleration, Body.cpp:203-238): a = linear x,
  * y, angular. */

@@ -251,6 +251,13 @@ void WriteState(const Handle<R>& h, Out& o) {
     o.put(ts, sizeof(ts));
     o.u32(std::uint32_t(w.bodies.size()));
     for (const auto& b : w.bodies) {
+        if (b.destroyed) {
+            // A destroyed body: zeros and flags 0x80.
+            const float z[10] = {};
+            o.put(z, sizeof(z));
+            o.u32(0x80u);
+            continue;
+        }
         const float v[10] = {float(b.pos0.lin.x), float(b.pos0.lin.y), float(b.pos0.ang),
                              float(b.pos1.lin.x), float(b.pos1.lin.y), float(b.pos1.ang),
                              float(b.vel.lin.x),  float(b.vel.lin.y),  float(b.vel.ang),
@@ -296,6 +303,13 @@ void WriteState(const Handle<R>& h, Out& o) {
     }
     o.u32(std::uint32_t(w.joints.size()));
     for (const auto& j : w.joints) {
+        if (j.destroyed) {
+            // A destroyed joint: limit state 0xffffffff and zeros.
+            o.u32(0xffffffffu);
+            const float z[4] = {};
+            o.put(z, sizeof(z));
+            continue;
+        }
         o.u32(j.limitState);
         const float f[4] = {float(j.impulse.x), float(j.impulse.y), float(j.impulse.z), float(j.motorImpulse)};
         o.put(f, sizeof(f));
@@ -413,6 +427,7 @@ void Widen(const Handle<S>& s, Handle<D>& d) {
         k.a1 = D(j.a1);
         k.a2 = D(j.a2);
         k.motorMass = D(j.motorMass);
+        k.destroyed = j.destroyed;
         b.joints.push_back(k);
     }
     b.bodyJoints = a.bodyJoints;
@@ -451,6 +466,7 @@ void Widen(const Handle<S>& s, Handle<D>& d) {
         t.xf = GetXf(t.pos1, t.lc);
         t.shapes = bo.shapes;
         t.proxies = bo.proxies;
+        t.destroyed = bo.destroyed;
         b.bodies.push_back(t);
     }
     b.bodyOrder = a.bodyOrder;
@@ -468,6 +484,7 @@ void Widen(const Handle<S>& s, Handle<D>& d) {
         b.tree.push_back(t);
     }
     b.leafCount = a.leafCount;
+    b.leafFree = a.leafFree;
     b.fixturesForProxies = a.fixturesForProxies;
     b.bodiesForSync = a.bodiesForSync;
     b.proxiesForContacts = a.proxiesForContacts;
@@ -597,6 +614,9 @@ void Edit(Handle<R>& h, int kind, std::uint32_t body, float a, float b, float c)
     case 'A': w.SetAngle(body, R(a)); break;
     case 'T': w.SetTransformation(body, V2<R>{R(a), R(b)}, UnitFromAngle(R(c))); break;
     case 'O': w.ShiftOrigin(V2<R>{R(a), R(b)}); break;
+    case 'V': w.SetVelocity(body, V2<R>{R(a), R(b)}, R(c)); break;
+    case 'J': w.DestroyJoint(body); break;
+    case 'D': w.DestroyBody(body); break;
     default: break;
     }
 }

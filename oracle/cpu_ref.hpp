@@ -627,6 +627,7 @@ struct Body {
     R aacc = 0, ldamp = 0, adamp = 0, invMass = 0, invRotI = 0, underActive = 0;
     R alpha0 = 0;  // Sweep::alpha0 (Sweep.hpp:67)
     Xf<R> xf;  // cached transform of pos1 (Body.hpp:697-715)
+    bool destroyed = false;  // Destroy(World&, BodyID) took it out of the world
     std::vector<std::uint32_t> shapes;
     std::vector<std::uint32_t> proxies;
     bool awake() const { return (flags & kAwake) != 0; }
@@ -771,6 +772,7 @@ template <class R> inline V2<R> Solve22(const M33<R>& m, V2<R> b) {  // Math.hpp
 template <class R>
 struct Joint {
     std::uint32_t bodyA = 0, bodyB = 0;
+    bool destroyed = false;  // Destroy(World&, JointID) took it out of the world
     bool enableLimit = false, enableMotor = false, collideConnected = false;
     V2<R> laA, laB;
     R referenceAngle = 0, lower = 0, upper = 0, motorSpeed = 0, maxMotorTorque = 0;
@@ -843,14 +845,27 @@ public:
     std::vector<std::vector<KeyedId>> bodyContacts;
     std::vector<char> islandedBodies, islandedContacts;
 
-    // Leaf id allocation of DynamicTree (DynamicTree.cpp:456-537) on a tree
-    // that only ever grows: the first leaf takes node 0, every later leaf the
-    // next free node, followed by its new parent node.
+    // Leaf id allocation of DynamicTree (DynamicTree.cpp:456-537): the node
+    // free list is a stack.  A leaf and its new parent take two nodes
+    // (CreateLeaf :517-537) and DestroyLeaf (:539-569) frees the parent and
+    // then the leaf, so a new leaf takes the node of the leaf destroyed last;
+    // with no freed node, the first leaf takes node 0 and every later leaf the
+    // next unused node, followed by its parent.
+    std::vector<std::uint32_t> leafFree;
     std::uint32_t AllocLeafId() {
+        if (!leafFree.empty()) {
+            const std::uint32_t id = leafFree.back();
+            leafFree.pop_back();
+            return id;
+        }
         const std::uint32_t id = (leafCount == 0) ? 0u : 2u * leafCount - 1u;
         ++leafCount;
         if (tree.size() <= id) tree.resize(id + 1);
         return id;
+    }
+    void FreeLeaf(std::uint32_t id) {
+        tree[id] = Leaf<R>{};
+        leafFree.push_back(id);
     }
 
     // ComputeAABB(proxy, xf0, xf1) (AABB.cpp:51-62) + GetFattenedAABB +
@@ -980,8 +995,9 @@ public:
         for (R v : f) events.push_back(FloatBits(v));
     }
 
-    // InternalDestroy (AabbTreeWorld.cpp:2263-2288), called by DestroyContacts.
-    void InternalDestroy(std::uint32_t cid) {
+    // InternalDestroy (AabbTreeWorld.cpp:2245-2279), called by DestroyContacts
+    // and, with `from` the body being destroyed, by Destroy(World&, BodyID).
+    void InternalDestroy(std::uint32_t cid, std::uint32_t from = 0xffffffffu) {
         Contact<R>& c = contactBuf[cid];
         if (c.flags & kTouching) EventHead(1, cid);  // :2249-2252
         auto eraseFrom = [&](std::uint32_t body) {
@@ -993,8 +1009,8 @@ public:
                 }
             }
         };
-        eraseFrom(c.bodyA);
-        eraseFrom(c.bodyB);
+        if (c.bodyA != from) eraseFrom(c.bodyA);
+        if (c.bodyB != from) eraseFrom(c.bodyB);
         if (manifoldBuf[cid].n > 0 && !(c.flags & kSensor)) {
             bodies[c.bodyA].SetAwake();
             bodies[c.bodyB].SetAwake();
@@ -3707,7 +3723,7 @@ public:
         // ShiftOrigin(PulleyJointConf&) (PulleyJointConf.cpp:255-260); the
         // other joint kinds hold nothing in world coordinates.
         for (Joint<R>& j : joints) {
-            if (j.kind != 7) continue;
+            if (j.kind != 7 || j.destroyed) continue;
             j.groundA = j.groundA - o;
             j.groundB = j.groundB - o;
         }
@@ -3728,11 +3744,95 @@ public:
         }
     }
 
+    // SetVelocity(World&, BodyID, Velocity) (WorldBody.cpp:263-268) with
+    // Body::SetVelocity (Body.cpp:184-194): a velocity other than zero is
+    // ignored by a body that is not speedable and wakes the others (their
+    // under-active time restarts).
+    void SetVelocity(std::uint32_t bid, V2<R> v, R w) {
+        Body<R>& b = bodies[bid];
+        if (!(v.x == R(0) && v.y == R(0) && w == R(0))) {
+            if (!(b.flags & kSpeedable)) return;
+            b.flags |= kAwake;
+            b.underActive = R(0);
+        }
+        b.vel = Vel<R>{v, w};
+    }
+
+    // Destroy(World&, JointID) (AabbTreeWorld.cpp:1215-1227) with
+    // AabbTreeWorld::Remove(JointID) (:1177-1213): both bodies wake and the
+    // joint leaves their joint lists.  Contacts between its bodies would need
+    // filtering, but none exists while a joint keeps its bodies from
+    // colliding (joints here are created before any contact).
+    void DestroyJoint(std::uint32_t jid) {
+        Joint<R>& j = joints[jid];
+        if (j.destroyed) return;
+        auto drop = [&](std::uint32_t body) {
+            bodies[body].SetAwake();
+            auto& lst = bodyJoints[body];
+            for (auto it = lst.begin(); it != lst.end(); ++it) {
+                if (it->second == jid) {
+                    lst.erase(it);
+                    break;
+                }
+            }
+        };
+        if (j.bodyA != kNoBody) drop(j.bodyA);
+        if (j.bodyB != kNoBody) drop(j.bodyB);
+        j.destroyed = true;
+    }
+
+    // Destroy(World&, BodyID) (AabbTreeWorld.cpp:1062-1104): the body's joints
+    // (the first of its list each time), its contacts in its list's order
+    // (AabbTreeWorld::Destroy(ContactID, from) :2281-2288: out of m_contacts,
+    // InternalDestroy leaves the destroyed body's own list alone), its queued
+    // proxy creations, its proxies in leaf order (FindProxies :617-630), then
+    // AabbTreeWorld::Remove(BodyID) (:1046-1060): out of the sync queue and
+    // the body list.
+    void DestroyBody(std::uint32_t bid) {
+        Body<R>& b = bodies[bid];
+        if (b.destroyed) return;
+        FreshEvents();
+        while (!bodyJoints[bid].empty()) DestroyJoint(bodyJoints[bid].front().second);
+        const std::vector<KeyedId> own = bodyContacts[bid];
+        for (const KeyedId& kc : own) {
+            for (auto it = contacts.begin(); it != contacts.end(); ++it) {
+                if (it->id == kc.id) {
+                    contacts.erase(it);
+                    break;
+                }
+            }
+            InternalDestroy(kc.id, bid);
+        }
+        bodyContacts[bid].clear();
+        fixturesForProxies.erase(std::remove_if(fixturesForProxies.begin(), fixturesForProxies.end(),
+                                                [bid](const std::pair<std::uint32_t, std::uint32_t>& f) {
+                                                    return f.first == bid;
+                                                }),
+                                 fixturesForProxies.end());
+        std::vector<std::uint32_t> leaves = b.proxies;
+        std::sort(leaves.begin(), leaves.end());
+        for (std::uint32_t id : leaves) FreeLeaf(id);
+        b.proxies.clear();
+        bodiesForSync.erase(std::remove(bodiesForSync.begin(), bodiesForSync.end(), bid), bodiesForSync.end());
+        bodyOrder.erase(std::remove(bodyOrder.begin(), bodyOrder.end(), bid), bodyOrder.end());
+        b.destroyed = true;
+    }
+
+    // Listener calls made by edits between steps (a destroyed contact's
+    // end-contact call) come before the next step's calls.
+    bool eventsStale = false;
+    void FreshEvents() {
+        if (eventsStale) {
+            events.clear();
+            nevents = 0;
+            eventsStale = false;
+        }
+    }
+
     // Step (AabbTreeWorld.cpp:2144-2219); the TOI phase when conf.toi.
     StepStats Step(const StepConf<R>& conf) {
         StepStats st;
-        events.clear();
-        nevents = 0;
+        FreshEvents();
         for (const auto& fs : fixturesForProxies) {
             CreateProxies(fs.first, fs.second, conf);
             ++st.preProxiesCreated;
@@ -3747,6 +3847,7 @@ public:
             SolveReg(conf, st);
             if (conf.toi) SolveToiPhase(conf, st);
         }
+        eventsStale = true;
         return st;
     }
 

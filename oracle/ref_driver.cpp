@@ -25,6 +25,7 @@
 //                         P, per proxy u32 id, f32[4]
 //                         J, per joint u32 limitState, f32[4] (impulse 3,
 //                           angular motor impulse)
+#include <algorithm>
 #include <playrho/d2/BasicAPI.hpp>
 #include <playrho/d2/World.hpp>
 #include <playrho/d2/WorldBody.hpp>
@@ -367,7 +368,15 @@ void WriteState(Out& o, const World& world, std::uint32_t step, const StepStats&
     o.put(toi, sizeof(toi));
     const auto nbodies = GetBodyRange(world);
     o.u32(nbodies);
+    const auto alive = GetBodies(world);
     for (auto i = 0u; i < nbodies; ++i) {
+        if (std::find(alive.begin(), alive.end(), BodyID(i)) == alive.end()) {
+            // A destroyed body: zeros and flags 0x80.
+            const float v[10] = {};
+            o.put(v, sizeof(v));
+            o.u32(0x80u);
+            continue;
+        }
         const auto b = GetBody(world, BodyID(i));
         const auto sw = GetSweep(b);
         const float v[10] = {GetX(sw.pos0.linear), GetY(sw.pos0.linear), sw.pos0.angular,
@@ -433,7 +442,15 @@ void WriteState(Out& o, const World& world, std::uint32_t step, const StepStats&
     }
     const auto njoints = GetJointRange(world);
     o.u32(njoints);
+    const auto aliveJoints = GetJoints(world);
     for (auto i = 0u; i < njoints; ++i) {
+        if (std::find(aliveJoints.begin(), aliveJoints.end(), JointID(i)) == aliveJoints.end()) {
+            // A destroyed joint: limit state 0xffffffff and zeros.
+            o.u32(0xffffffffu);
+            const float f[4] = {};
+            o.put(f, sizeof(f));
+            continue;
+        }
         const auto joint = GetJoint(world, JointID(i));
         if (const auto* j = TypeCast<const RevoluteJointConf>(&joint)) {
             o.u32(std::uint32_t(j->limitState));
@@ -654,6 +671,9 @@ int TimeWorlds(int argc, char** argv) {
 //   A  SetAngle(world, body, a)
 //   T  SetTransformation(world, body, {(a, b), UnitVec::Get(c)})
 //   O  ShiftOrigin(world, (a, b))
+//   V  SetVelocity(world, body, {(a, b), c})
+//   J  Destroy(world, JointID(body))
+//   D  Destroy(world, BodyID(body))
 struct Edit {
     int step = 0;
     char kind = 0;
@@ -689,6 +709,16 @@ void ApplyEdit(World& world, const Edit& e) {
         break;
     case 'O':
         ShiftOrigin(world, Length2{e.a * Meter, e.b * Meter});
+        break;
+    case 'V':
+        SetVelocity(world, id, Velocity{LinearVelocity2{e.a * MeterPerSecond, e.b * MeterPerSecond},
+                                        e.c * RadianPerSecond});
+        break;
+    case 'J':
+        Destroy(world, JointID(static_cast<JointID::underlying_type>(e.body)));
+        break;
+    case 'D':
+        Destroy(world, id);
         break;
     default:
         break;

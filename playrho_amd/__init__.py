@@ -354,6 +354,8 @@ _SIGS = {
     "prx_shape_set_restitution": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float]),
     "prx_world_set_velocity": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32,
                                               ctypes.POINTER(ctypes.c_float)]),
+    "prx_world_destroy_body": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "prx_world_destroy_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     "prx_world_state_record": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
                                               ctypes.POINTER(ctypes.c_size_t)]),
     "prx_world_contact_count": (ctypes.c_uint32, [ctypes.c_void_p]),
@@ -538,6 +540,14 @@ class World:
     def set_velocity(self, body, v):
         arr = (ctypes.c_float * 3)(*v)
         _check(lib().prx_world_set_velocity(self.h, body, arr))
+
+    def destroy_body(self, body):
+        """Destroy(World&, BodyID) (AabbTreeWorld.cpp:1062-1104)."""
+        _check(lib().prx_world_destroy_body(self.h, int(body)))
+
+    def destroy_joint(self, joint):
+        """Destroy(World&, JointID) (AabbTreeWorld.cpp:1215-1227)."""
+        _check(lib().prx_world_destroy_joint(self.h, int(joint)))
 
     def set_acceleration(self, body, a):
         """SetAcceleration (Body.cpp:203-238): a = (linear x, y, angular)."""

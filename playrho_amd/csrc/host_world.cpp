@@ -250,12 +250,14 @@ struct HostBody {
     float invMass = 0, invRotI = 0;
     std::vector<std::uint32_t> shapes;
     std::vector<int> proxies;  // world-local proxy indices
+    bool destroyed = false;    // Destroy(World&, BodyID): state frozen, out of every list
 };
 
 struct HostProxy {
     int body = 0, shape = 0;
     std::uint32_t id = 0, flags = prx::kProxyPending;
     float aabb[4] = {0, 0, 0, 0};
+    bool dead = false;  // its body was destroyed (uploaded with an empty box, never moved)
 };
 
 struct HostJoint {
@@ -275,6 +277,8 @@ struct HostJoint {
     float ext[3] = {0, 0, 0};               // wheel: local x axis of A (x, y), -
     float imp[4] = {0, 0, 0, 0};            // impulse x, y, z, angular motor impulse
     std::uint32_t limit = 0;
+    std::uint64_t seq = 0;   // creation order (the bodies' joint lists follow it)
+    bool destroyed = false;  // Destroy(World&, JointID)
 };
 
 // Contact records as held on the host between device residencies.
@@ -332,6 +336,13 @@ struct prx_world {
     prx_contact_listener listener{};
     std::vector<std::uint32_t> events;  // listener calls of the last step (device word format)
     std::uint32_t nevents = 0;
+    // Listener calls of edits since the last step (a destroyed body's touching
+    // contacts end): they come before the next step's calls.
+    std::vector<std::uint32_t> pre_events;
+    std::uint32_t pre_nevents = 0;
+    std::vector<std::uint32_t> joint_free;  // destroyed joint ids, reused last first (ObjectPool)
+    std::uint64_t joint_seq = 0;
+    std::uint32_t destroyed_bodies = 0;
 };
 
 namespace {
@@ -487,6 +498,11 @@ int AddShape(prx_world* w, const prx_shape_conf& c, std::uint32_t* out) {
 }
 
 int AddBody(prx_world* w, const prx_body_conf& c, std::uint32_t* out) {
+    // The reference would give the new body the id freed last while putting
+    // it at the end of the body list, and its proxies the tree nodes freed
+    // last; the device keeps bodies in id order.
+    if (w->destroyed_bodies)
+        return Fail(PRX_E_UNSUPPORTED, "creating a body in a world where a body was destroyed");
     if (w->bodies.size() >= 65534) return Fail(PRX_E_LENGTH, "CreateBody: operation would exceed MaxBodies");
     for (std::uint32_t i = 0; i < c.nshapes; ++i)
         if (c.shapes[i] >= w->shapes.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such shape");
@@ -631,7 +647,38 @@ constexpr std::uint32_t kNoBody = 0xffffu;
 bool JointBodyValid(const prx_world* w, const HostJoint& j, bool bodyA) {
     const std::uint32_t id = bodyA ? j.a : j.b;
     if (bodyA && id == kNoBody && (j.flags >> 8) == prx::kJointTarget) return true;
-    return id < w->bodies.size();
+    return id < w->bodies.size() && !w->bodies[id].destroyed;
+}
+
+// A body id the world holds and has not destroyed (the reference throws
+// WasDestroyed for a destroyed one).
+int CheckBody(const prx_world* w, std::uint32_t body) {
+    if (body >= w->bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
+    if (w->bodies[body].destroyed) return Fail(PRX_E_WRONG_STATE, "the body was destroyed");
+    return PRX_OK;
+}
+
+// Body::SetAwake (Body.cpp:163-171).
+void WakeBody(HostBody& b) {
+    if (b.flags & prx::kSpeedable) {
+        b.flags |= prx::kAwake;
+        b.under = 0.0f;
+    }
+}
+
+// AabbTreeWorld::Remove(JointID) (AabbTreeWorld.cpp:1177-1213) and the rest
+// of Destroy(World&, JointID) (:1215-1227): both bodies wake, the joint
+// leaves their joint lists (rebuilt at upload from the live joints in
+// creation order) and its id goes to the pool's free list.  Contacts between
+// its bodies would need filtering, but none exists while a joint keeps its
+// bodies from colliding (AddJointRecord refuses such a joint between bodies
+// that have a contact).
+void DestroyJointRecord(prx_world* w, std::uint32_t joint) {
+    HostJoint& j = w->joints[joint];
+    if (j.a != kNoBody) WakeBody(w->bodies[j.a]);
+    WakeBody(w->bodies[j.b]);
+    j.destroyed = true;
+    w->joint_free.push_back(joint);
 }
 
 int AddJointRecord(prx_world* w, const HostJoint& j, std::uint32_t* out) {
@@ -643,8 +690,19 @@ int AddJointRecord(prx_world* w, const HostJoint& j, std::uint32_t* out) {
                 return Fail(PRX_E_UNSUPPORTED, "joint between bodies that already have a contact");
         }
     }
+    HostJoint k = j;
+    k.seq = w->joint_seq++;
+    k.destroyed = false;
+    // The joint pool reuses the id freed last (ObjectPool::Allocate).
+    if (!w->joint_free.empty()) {
+        const std::uint32_t id = w->joint_free.back();
+        w->joint_free.pop_back();
+        w->joints[id] = k;
+        if (out) *out = id;
+        return PRX_OK;
+    }
     if (out) *out = std::uint32_t(w->joints.size());
-    w->joints.push_back(j);
+    w->joints.push_back(k);
     return PRX_OK;
 }
 
@@ -980,7 +1038,11 @@ int UploadBatch(prx_batch* b) {
         // Joints and per-body joint lists in creation order (m_bodyJoints).
         {
             std::vector<std::vector<int2>> lists(w->bodies.size());
-            for (std::size_t ji = 0; ji < w->joints.size(); ++ji) {
+            std::vector<std::size_t> order(w->joints.size());
+            for (std::size_t ji = 0; ji < order.size(); ++ji) order[ji] = ji;
+            std::stable_sort(order.begin(), order.end(),
+                             [w](std::size_t a, std::size_t c) { return w->joints[a].seq < w->joints[c].seq; });
+            for (std::size_t ji : order) {
                 const HostJoint& j = w->joints[ji];
                 const std::size_t g = std::size_t(h.joint_off) + ji;
                 // A target joint's record names body B twice (its solver
@@ -995,6 +1057,7 @@ int UploadBatch(prx_batch* b) {
                 if (j.flags >> 8) b->joint_kinds = true;
                 jimp[g] = float4{j.imp[0], j.imp[1], j.imp[2], j.imp[3]};
                 jlimit[g] = j.limit;
+                if (j.destroyed) continue;
                 if (!noA) lists[j.a].push_back(int2{int(ji), int(j.b)});
                 lists[j.b].push_back(int2{int(ji), noA ? -1 : int(j.a)});
             }
@@ -1041,8 +1104,10 @@ int UploadBatch(prx_batch* b) {
             pbody[g] = hp.body;
             pshape[g] = sb + hp.shape;
             pid[g] = hp.id;
-            pflags[g] = hp.flags;
-            paabb[g] = float4{hp.aabb[0], hp.aabb[1], hp.aabb[2], hp.aabb[3]};
+            pflags[g] = hp.dead ? 0u : hp.flags;
+            // A dead proxy's box is empty: it overlaps nothing.
+            const float inf = std::numeric_limits<float>::infinity();
+            paabb[g] = hp.dead ? float4{inf, inf, -inf, -inf} : float4{hp.aabb[0], hp.aabb[1], hp.aabb[2], hp.aabb[3]};
         }
         const HostContacts& hc = w->contacts;
         for (std::size_t ci = 0; ci < hc.size(); ++ci) {
@@ -1105,11 +1170,19 @@ int FetchEvents(prx_batch* b, int wi, const prx::WorldHeader& h) {
     if (h.ev_cap == 0) {
         w->events.clear();
         w->nevents = 0;
+        w->pre_events.clear();
+        w->pre_nevents = 0;
         return PRX_OK;
     }
     const std::size_t words = std::size_t(std::min(h.ev_words, h.ev_cap));
     if (int st = Download(w->events, b->d.ev + h.ev_off, words)) return st;
     w->nevents = h.ev_words <= h.ev_cap ? std::uint32_t(h.ev_count) : 0u;
+    if (!w->pre_events.empty()) {
+        w->events.insert(w->events.begin(), w->pre_events.begin(), w->pre_events.end());
+        w->nevents += w->pre_nevents;
+        w->pre_events.clear();
+        w->pre_nevents = 0;
+    }
     return PRX_OK;
 }
 
@@ -1922,6 +1995,7 @@ int prx_world_load_desc(prx_world* w, const std::uint8_t* desc, std::size_t len)
         for (float& v : j.cfg) v = f32();
         j.maxTorque = f32();
         for (float& v : j.ext) v = f32();
+        j.seq = fresh.joints.size();
         fresh.joints.push_back(j);
     }
     if (bad) return Fail(PRX_E_INVALID_ARGUMENT, "truncated description");
@@ -1931,12 +2005,20 @@ int prx_world_load_desc(prx_world* w, const std::uint8_t* desc, std::size_t len)
     w->proxies = std::move(fresh.proxies);
     w->contacts = HostContacts{};
     w->leafCount = fresh.leafCount;
+    w->joint_free.clear();
+    w->joint_seq = w->joints.size();
+    w->destroyed_bodies = 0;
+    w->pre_events.clear();
+    w->pre_nevents = 0;
     return PRX_OK;
 }
 
 int prx_world_save_desc(prx_world* w, std::uint8_t* buf, std::size_t cap, std::size_t* len) {
     if (int st = CheckWorld(w)) return st;
     if (int st = Refresh(w)) return st;
+    // The description has no place for destroyed bodies and joints.
+    if (w->destroyed_bodies || !w->joint_free.empty())
+        return Fail(PRX_E_UNSUPPORTED, "saving the description of a world with destroyed bodies or joints");
     std::vector<std::uint8_t> o;
     auto put = [&](const void* p, std::size_t n) {
         const auto* c = static_cast<const std::uint8_t*>(p);
@@ -2043,7 +2125,7 @@ int prx_world_get_bodies(prx_world* w, float* out, std::uint32_t* flags) {
 
 int prx_world_get_transformation(prx_world* w, std::uint32_t body, float out[4]) {
     if (int st = CheckWorld(w)) return st;
-    if (body >= w->bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
+    if (int st = CheckBody(w, body)) return st;
     if (int st = Refresh(w)) return st;
     const HostBody& b = w->bodies[body];
     const V2 q = UnitFromAngle(b.pos1[2]);
@@ -2058,7 +2140,7 @@ int prx_world_get_transformation(prx_world* w, std::uint32_t body, float out[4])
 int prx_world_set_transformation(prx_world* w, std::uint32_t body, const float xf[4]) {
     if (int st = CheckWorld(w)) return st;
     if (!xf) return Fail(PRX_E_INVALID_ARGUMENT, "null transformation");
-    if (body >= w->bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
+    if (int st = CheckBody(w, body)) return st;
     if (int st = Detach(w)) return st;
     // SetTransformation(Body&, xfm) (Body.cpp:270-273): the angle of q.
     return SetSweepPosition(w, body, xf[0], xf[1], std::atan2(xf[3], xf[2]));
@@ -2067,7 +2149,7 @@ int prx_world_set_transformation(prx_world* w, std::uint32_t body, const float x
 int prx_world_set_location(prx_world* w, std::uint32_t body, const float v[2]) {
     if (int st = CheckWorld(w)) return st;
     if (!v) return Fail(PRX_E_INVALID_ARGUMENT, "null location");
-    if (body >= w->bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
+    if (int st = CheckBody(w, body)) return st;
     if (int st = Detach(w)) return st;
     // SetLocation(Body&, v) (Body.cpp:275-278): the current rotation, re-read as an angle.
     const V2 q = UnitFromAngle(w->bodies[body].pos1[2]);
@@ -2076,7 +2158,7 @@ int prx_world_set_location(prx_world* w, std::uint32_t body, const float v[2]) {
 
 int prx_world_set_angle(prx_world* w, std::uint32_t body, float angle) {
     if (int st = CheckWorld(w)) return st;
-    if (body >= w->bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
+    if (int st = CheckBody(w, body)) return st;
     if (int st = Detach(w)) return st;
     // SetAngle(Body&, a) (Body.cpp:285-288): the sweep's linear position kept.
     const HostBody& b = w->bodies[body];
@@ -2115,7 +2197,7 @@ int prx_world_shift_origin(prx_world* w, const float origin[2]) {
 int prx_world_set_velocity(prx_world* w, std::uint32_t body, const float v[3]) {
     if (int st = CheckWorld(w)) return st;
     if (!v) return Fail(PRX_E_INVALID_ARGUMENT, "null velocity");
-    if (body >= w->bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
+    if (int st = CheckBody(w, body)) return st;
     if (int st = Detach(w)) return st;
     HostBody& b = w->bodies[body];
     const bool nonzero = v[0] != 0 || v[1] != 0 || v[2] != 0;
@@ -2130,6 +2212,99 @@ int prx_world_set_velocity(prx_world* w, std::uint32_t body, const float v[3]) {
     return PRX_OK;
 }
 
+// Destroy(World&, JointID) (AabbTreeWorld.cpp:1215-1227); an id already
+// destroyed is left alone, as the reference leaves an id not in its joint list.
+int prx_world_destroy_joint(prx_world* w, std::uint32_t joint) {
+    if (int st = CheckWorld(w)) return st;
+    if (joint >= w->joints.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such joint");
+    if (w->joints[joint].destroyed) return PRX_OK;
+    if (int st = Detach(w)) return st;
+    DestroyJointRecord(w, joint);
+    return PRX_OK;
+}
+
+// Destroy(World&, BodyID) (AabbTreeWorld.cpp:1062-1104): the body's joints
+// (the first of its list each time), then its contacts in creation order
+// (AabbTreeWorld::Destroy(ContactID, from) :2281-2288 and InternalDestroy
+// :2245-2279: an end-contact call for a touching one, both bodies awake if
+// the manifold has points, the id to the contact pool's free list), then its
+// proxies (their boxes emptied), then AabbTreeWorld::Remove (:1046-1060).
+// The body keeps its id; its state is no longer stepped.  Proxies queued
+// since the last step were never in the reference's tree: they give their
+// leaf ids back when they were the last ones handed out.
+int prx_world_destroy_body(prx_world* w, std::uint32_t body) {
+    if (int st = CheckWorld(w)) return st;
+    if (int st = CheckBody(w, body)) return st;
+    if (int st = Detach(w)) return st;
+    HostBody& hb = w->bodies[body];
+    std::vector<std::uint32_t> pending;
+    for (int pi : hb.proxies)
+        if (w->proxies[std::size_t(pi)].flags & prx::kProxyPending) pending.push_back(w->proxies[std::size_t(pi)].id);
+    if (!pending.empty()) {
+        std::sort(pending.begin(), pending.end());
+        const std::uint32_t k = std::uint32_t(pending.size());
+        for (std::uint32_t i = 0; i < k; ++i) {
+            const std::uint32_t n = w->leafCount - k + i;
+            const std::uint32_t want = (n == 0) ? 0u : 2u * n - 1u;
+            if (pending[i] != want)
+                return Fail(PRX_E_UNSUPPORTED, "destroying a body whose queued proxies precede other queued proxies");
+        }
+        w->leafCount -= k;
+    }
+    for (;;) {
+        std::size_t first = w->joints.size();
+        for (std::size_t ji = 0; ji < w->joints.size(); ++ji) {
+            const HostJoint& j = w->joints[ji];
+            if (j.destroyed || (j.a != body && j.b != body)) continue;
+            if (first == w->joints.size() || j.seq < w->joints[first].seq) first = ji;
+        }
+        if (first == w->joints.size()) break;
+        DestroyJointRecord(w, std::uint32_t(first));
+    }
+    HostContacts& hc = w->contacts;
+    std::size_t out = 0;
+    for (std::size_t i = 0; i < hc.size(); ++i) {
+        const int2 bd = hc.body[i];
+        if (bd.x != int(body) && bd.y != int(body)) {
+            if (out != i) {
+                hc.prox[out] = hc.prox[i];
+                hc.body[out] = hc.body[i];
+                hc.shape[out] = hc.shape[i];
+                hc.id[out] = hc.id[i];
+                hc.flags[out] = hc.flags[i];
+                hc.mat[out] = hc.mat[i];
+                hc.geo[out] = hc.geo[i];
+                hc.p0[out] = hc.p0[i];
+                hc.p1[out] = hc.p1[i];
+                hc.hdr[out] = hc.hdr[i];
+            }
+            ++out;
+            continue;
+        }
+        if (w->record_events && (hc.flags[i] & prx::kTouching)) {
+            w->pre_events.push_back(prx::kEvEnd);
+            w->pre_events.push_back(hc.id[i]);
+            ++w->pre_nevents;
+        }
+        if (((hc.hdr[i].x >> 8) & 0xffu) > 0 && !(hc.flags[i] & prx::kSensor)) {
+            WakeBody(w->bodies[std::size_t(bd.x)]);
+            WakeBody(w->bodies[std::size_t(bd.y)]);
+        }
+        hc.freelist.push_back(hc.id[i]);
+    }
+    hc.resize(out);
+    for (int pi : hb.proxies) {
+        HostProxy& p = w->proxies[std::size_t(pi)];
+        p.dead = true;
+        p.flags = 0;
+    }
+    hb.proxies.clear();
+    hb.flags = 0;
+    hb.destroyed = true;
+    ++w->destroyed_bodies;
+    return PRX_OK;
+}
+
 // SetAcceleration(World&, BodyID, ...) through Body::SetAcceleration
 // (Body.cpp:203-238): no change, nothing; a body that is not accelerable
 // takes zero only; a larger linear or angular acceleration, a new direction
@@ -2137,7 +2312,7 @@ int prx_world_set_velocity(prx_world* w, std::uint32_t body, const float v[3]) {
 int prx_world_set_acceleration(prx_world* w, std::uint32_t body, const float a[3]) {
     if (int st = CheckWorld(w)) return st;
     if (!a) return Fail(PRX_E_INVALID_ARGUMENT, "null acceleration");
-    if (body >= w->bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
+    if (int st = CheckBody(w, body)) return st;
     if (int st = Detach(w)) return st;
     HostBody& b = w->bodies[body];
     if (b.acc[0] == a[0] && b.acc[1] == a[1] && b.acc[2] == a[2]) return PRX_OK;
@@ -2285,7 +2460,7 @@ int prx_world_contact_events(prx_world* w, std::uint32_t* words, std::size_t cap
 int prx_world_get_body_mass(prx_world* w, std::uint32_t body, float out[4]) {
     if (int st = CheckWorld(w)) return st;
     if (!out) return Fail(PRX_E_INVALID_ARGUMENT, "null output");
-    if (body >= w->bodies.size()) return Fail(PRX_E_OUT_OF_RANGE, "no such body");
+    if (int st = CheckBody(w, body)) return st;
     const HostBody& b = w->bodies[body];
     out[0] = b.invMass;
     out[1] = b.invRotI;
@@ -2373,6 +2548,13 @@ int prx_world_state_record(prx_world* w, std::uint8_t* buf, std::size_t cap, std
     put(&w->last_stats, sizeof(prx_step_stats));  // the 34 StepStats words, in order
     u32(std::uint32_t(w->bodies.size()));
     for (const auto& b : w->bodies) {
+        if (b.destroyed) {
+            // A destroyed body: zeros and flags 0x80.
+            const float z[10] = {};
+            put(z, sizeof(z));
+            u32(0x80u);
+            continue;
+        }
         const float v[10] = {b.pos0[0], b.pos0[1], b.pos0[2], b.pos1[0], b.pos1[1],
                              b.pos1[2], b.vel[0],  b.vel[1],  b.vel[2],  b.under};
         put(v, sizeof(v));
@@ -2412,15 +2594,22 @@ int prx_world_state_record(prx_world* w, std::uint8_t* buf, std::size_t cap, std
         put(f, sizeof(f));
     }
     std::uint32_t np = 0;
-    for (const auto& p : w->proxies) np += (p.flags & prx::kProxyPending) ? 0u : 1u;
+    for (const auto& p : w->proxies) np += ((p.flags & prx::kProxyPending) || p.dead) ? 0u : 1u;
     u32(np);
     for (const auto& p : w->proxies) {
-        if (p.flags & prx::kProxyPending) continue;
+        if ((p.flags & prx::kProxyPending) || p.dead) continue;
         u32(p.id);
         put(p.aabb, sizeof(p.aabb));
     }
     u32(std::uint32_t(w->joints.size()));
     for (const auto& j : w->joints) {
+        if (j.destroyed) {
+            // A destroyed joint: limit state 0xffffffff and zeros.
+            const float z[4] = {};
+            u32(0xffffffffu);
+            put(z, sizeof(z));
+            continue;
+        }
         u32(j.limit);
         put(j.imp, sizeof(j.imp));
     }

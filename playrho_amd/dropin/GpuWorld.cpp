@@ -432,7 +432,8 @@ BodyCounter GetBodyRange(const GpuWorld& g) noexcept { return BodyCounter(g.m_bo
 
 std::vector<BodyID> GetBodies(const GpuWorld& g) {
     std::vector<BodyID> out;
-    for (std::size_t i = 0; i < g.m_bodies.size(); ++i) out.push_back(BodyID(static_cast<BodyID::underlying_type>(i)));
+    for (std::size_t i = 0; i < g.m_bodies.size(); ++i)
+        if (!g.m_bodies[i].IsDestroyed()) out.push_back(BodyID(static_cast<BodyID::underlying_type>(i)));
     return out;
 }
 
@@ -478,6 +479,7 @@ BodyID CreateBody(GpuWorld& g, const Body& body) {
 
 Body GetBody(const GpuWorld& g, BodyID id) {
     if (to_underlying(id) >= g.m_bodies.size()) throw OutOfRange<BodyID>(id, "playrho_amd: no such body");
+    if (g.m_bodies[to_underlying(id)].IsDestroyed()) throw WasDestroyed<BodyID>(id, "playrho_amd: the body was destroyed");
     const auto st = ReadBodies(g);
     return BodyFromState(g, id, st.at(to_underlying(id)));
 }
@@ -517,13 +519,24 @@ void SetBody(GpuWorld& g, BodyID id, const Body& value) {
     }
 }
 
-void Destroy(GpuWorld&, BodyID) { throw WrongState("playrho_amd: destroying bodies is not supported by the device engine"); }
+// Destroy (AabbTreeWorld.cpp:1062-1104): the engine destroys the body's
+// joints, contacts and proxies; the body keeps its id.
+void Destroy(GpuWorld& g, BodyID id) {
+    const auto i = to_underlying(id);
+    if (i >= g.m_bodies.size()) throw OutOfRange<BodyID>(id, "playrho_amd: no such body");
+    if (g.m_bodies[i].IsDestroyed()) throw WasDestroyed<BodyID>(id, "playrho_amd: the body was destroyed");
+    Check(prx_world_destroy_body(g.m_world, i));
+    for (auto& j : g.m_joints)
+        if (j.has_value() && (GetBodyA(j) == id || GetBodyB(j) == id)) j = Joint{};
+    SetDestroyed(g.m_bodies[i]);
+}
 
 std::vector<std::pair<BodyID, JointID>> GetJoints(const GpuWorld& g, BodyID id) {
     std::vector<std::pair<BodyID, JointID>> out;
     for (std::size_t j = 0; j < g.m_joints.size(); ++j) {
         const Joint& jt = g.m_joints[j];
         const JointID jid(static_cast<JointID::underlying_type>(j));
+        if (!jt.has_value()) continue;
         if (GetBodyA(jt) == id) out.emplace_back(GetBodyB(jt), jid);
         else if (GetBodyB(jt) == id) out.emplace_back(GetBodyA(jt), jid);
     }
@@ -548,7 +561,8 @@ JointCounter GetJointRange(const GpuWorld& g) noexcept { return JointCounter(g.m
 
 std::vector<JointID> GetJoints(const GpuWorld& g) {
     std::vector<JointID> out;
-    for (std::size_t j = 0; j < g.m_joints.size(); ++j) out.push_back(JointID(static_cast<JointID::underlying_type>(j)));
+    for (std::size_t j = 0; j < g.m_joints.size(); ++j)
+        if (g.m_joints[j].has_value()) out.push_back(JointID(static_cast<JointID::underlying_type>(j)));
     return out;
 }
 
@@ -685,12 +699,16 @@ JointID CreateJoint(GpuWorld& g, const Joint& def) {
     } else {
         Unsupported("this joint type (gear joints)");
     }
-    g.m_joints.push_back(def);
+    // The engine reuses the id destroyed last (ObjectPool).
+    if (id < g.m_joints.size()) g.m_joints[id] = def;
+    else g.m_joints.push_back(def);
     return JointID(static_cast<JointID::underlying_type>(id));
 }
 
 Joint GetJoint(const GpuWorld& g, JointID id) {
     if (to_underlying(id) >= g.m_joints.size()) throw OutOfRange<JointID>(id, "playrho_amd: no such joint");
+    if (!g.m_joints[to_underlying(id)].has_value())
+        throw WasDestroyed<JointID>(id, "playrho_amd: the joint was destroyed");
     return g.m_joints[to_underlying(id)];
 }
 
@@ -698,7 +716,14 @@ void SetJoint(GpuWorld& g, JointID id, const Joint& def) {
     if (!(GetJoint(g, id) == def)) Unsupported("changing a joint after creation");
 }
 
-void Destroy(GpuWorld&, JointID) { throw WrongState("playrho_amd: destroying joints is not supported by the device engine"); }
+// Destroy (AabbTreeWorld.cpp:1215-1227).
+void Destroy(GpuWorld& g, JointID id) {
+    const auto i = to_underlying(id);
+    if (i >= g.m_joints.size()) throw OutOfRange<JointID>(id, "playrho_amd: no such joint");
+    if (!g.m_joints[i].has_value()) return;
+    Check(prx_world_destroy_joint(g.m_world, i));
+    g.m_joints[i] = Joint{};
+}
 
 // ---------------------------------------------------------------------------
 // Shapes (AabbTreeWorld.cpp:1236-1300; d2/WorldShape.hpp).

@@ -148,14 +148,18 @@ int Cpu() {
     // The type-erased copy (Clone_) of a device world and its equality.
     const World copy = gpu;
     Expect(copy == gpu, "copy equals");
-    // Unsupported operations raise the reference's exception types.
-    bool threw = false;
-    try {
-        Destroy(gpu, BodyID(3));
-    } catch (const WrongState&) {
-        threw = true;
-    }
-    Expect(threw, "Destroy(body) throws WrongState");
+    // Destroy(JointID) and Destroy(BodyID) (the last body: its proxies are
+    // still queued), then a new joint, through both worlds.
+    Destroy(ref2, JointID(3));
+    Destroy(gpu2, JointID(3));
+    Destroy(ref2, BodyID(8));
+    Destroy(gpu2, BodyID(8));
+    Expect(GetBodies(ref2) == GetBodies(gpu2), "bodies after destroying");
+    Expect(GetJoints(ref2) == GetJoints(gpu2), "joints after destroying");
+    CompareBodies(ref2, gpu2, "chain after destroying");
+    const auto jc = Joint{RevoluteJointConf{}.UseBodyA(BodyID(2)).UseBodyB(BodyID(5))};
+    Expect(CreateJoint(ref2, jc) == CreateJoint(gpu2, jc), "joint id after destroying");
+    Expect(GetJoints(ref2) == GetJoints(gpu2), "joints after creating");
     std::printf("cpu: %s\n", g_fail ? "FAIL" : "OK");
     return g_fail ? 1 : 0;
 }

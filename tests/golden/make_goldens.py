@@ -56,7 +56,8 @@ CASES = [
     ("targets0", "targets", 240, [1, 0], [1, 120, 240]),
 ]
 # World edits between steps (step, kind, body, a, b, c): SetLocation 'L',
-# SetAngle 'A', SetTransformation 'T' (c the rotation's angle), ShiftOrigin 'O'.
+# SetAngle 'A', SetTransformation 'T' (c the rotation's angle), ShiftOrigin 'O',
+# SetVelocity 'V', Destroy(JointID) 'J', Destroy(BodyID) 'D'.
 EDIT_CASES = [
     ("pile3_edit", "pile", 120, [1, 3], [1, 120], False,
      [(20, "L", 5, 1.25, 6.5, 0), (35, "A", 9, 0.7, 0, 0), (50, "T", 12, -1.5, 8.0, 2.5), (70, "O", 0, 3.0, -0.5, 0),
@@ -66,6 +67,22 @@ EDIT_CASES = [
     # ShiftOrigin with every joint kind present (the pulley's ground anchors move).
     ("joints0_edit", "joints", 120, [1, 0], [1, 120], False,
      [(40, "O", 0, 3.0, -2.0, 0), (80, "L", 30, -5.0, 20.0, 0)]),
+    # SetVelocity 'V' (c the angular velocity): resting and falling bodies,
+    # the static ground (ignored), a zero velocity (no wake-up).
+    ("pile3_vel_edit", "pile", 120, [1, 3], [1, 120], False,
+     [(30, "V", 7, 2.0, 3.0, 1.5), (55, "V", 0, 1.0, 1.0, 0), (80, "V", 12, 0, 0, 0), (100, "V", 20, -4.0, 6.0, -3.0),
+      (100, "V", 21, 0.5, 0, 0)]),
+    # Destroy(JointID) 'J' and Destroy(BodyID) 'D' (body: the id): a weld, the
+    # pulley, a body held by friction and weld joints and resting on the
+    # ground, a wheel of the cart, a falling body among others.
+    ("joints0_destroy_edit", "joints", 160, [1, 0], [1, 160], False,
+     [(40, "J", 3, 0, 0, 0), (60, "J", 18, 0, 0, 0), (90, "D", 24, 0, 0, 0), (110, "D", 19, 0, 0, 0),
+      (130, "D", 50, 0, 0, 0), (130, "D", 45, 0, 0, 0)]),
+    # Bodies destroyed out of a resting pile (their touching contacts end,
+    # their neighbours wake), with the TOI phase.
+    ("pile3_destroy_edit_toi", "pile", 120, [1, 3], [1, 120], True,
+     [(60, "D", 40, 0, 0, 0), (61, "D", 41, 0, 0, 0), (61, "D", 90, 0, 0, 0), (90, "V", 60, 0, 8.0, 0),
+      (90, "D", 100, 0, 0, 0)]),
 ]
 # The same scenes with the TOI phase (the reference's default doToi = true).
 TOI_CASES = [

@@ -320,3 +320,38 @@ def test_target_joint_creation_matches_scene():
     assert e.value.code == 4
     t = pa.TargetJointConf()
     assert (t.frequency, round(t.damping_ratio, 6), t.max_force) == (5.0, 0.7, 0.0)
+
+
+def test_destroy_host_side():
+    """Destroy(BodyID) / Destroy(JointID) on a host-resident world (its
+    proxies still queued): the joints of a destroyed body go with it, a
+    destroyed joint's id is reused by the next joint, edits on a destroyed
+    body and creating a body afterwards are refused, destroying a body whose
+    queued proxies precede others' is refused, and the state record writes
+    the destroyed entries as markers."""
+    from tests import dumpfmt
+    w = pa.World.scene("joints", [1, 0])
+    nj = w.joint_count
+    nb = w.body_count
+    w.destroy_joint(3)
+    w.destroy_joint(3)  # already destroyed: left alone
+    with pytest.raises(pa.PrxError):
+        w.destroy_body(48)  # queued proxies of later bodies follow its own
+    for b in (50, 49, 48):  # the last bodies, last first; rope joint 35 goes with 48
+        w.destroy_body(b)
+    with pytest.raises(pa.PrxError):
+        w.set_velocity(48, (1.0, 0.0, 0.0))
+    with pytest.raises(pa.PrxError):
+        w.destroy_body(48)
+    with pytest.raises(pa.PrxError):
+        w.save_desc()
+    st = dumpfmt.read_state(dumpfmt.Reader(w.state_record()))
+    assert all(st["bflags"][b] == 0x80 and not st["bodies"][b].any() for b in (48, 49, 50))
+    assert [int(j) for j in np.flatnonzero(st["jlimit"] == 0xffffffff)] == [3, 35]
+    assert w.body_count == nb
+    # The next joint takes the id destroyed last (35), then 3.
+    assert w.create_joint(pa.RevoluteJointConf(body_a=2, body_b=30)) == 35
+    assert w.create_joint(pa.RevoluteJointConf(body_a=2, body_b=33)) == 3
+    assert w.joint_count == nj
+    with pytest.raises(pa.PrxError):
+        w.create_body(pa.BodyConf())

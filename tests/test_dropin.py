@@ -22,7 +22,9 @@ needs_check = pytest.mark.skipif(not os.path.exists(CHECK), reason="oracle/_ref/
 def test_dropin_world_construction_equals_reference():
     """World{GpuWorld{}} built through the reference API: every body's
     transformation, velocity and mass data, and every shape, equal the
-    reference World's (no device needed)."""
+    reference World's; after Destroy(JointID) and Destroy(BodyID) the body
+    and joint lists, the bodies and the next joint's id equal too (no device
+    needed)."""
     r = subprocess.run([CHECK, "cpu"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "cpu: OK" in r.stdout

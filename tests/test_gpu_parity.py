@@ -194,9 +194,11 @@ def _conf_bytes():
 
 @pytest.mark.parametrize("name", golden_cases.edit_names())
 def test_world_edits_match_reference_every_step(name):
-    """SetLocation / SetAngle / SetTransformation / ShiftOrigin between steps
-    on a device-resident world (the reference driver applied the same edits
-    before the same steps)."""
+    """SetLocation / SetAngle / SetTransformation / ShiftOrigin /
+    SetVelocity / Destroy(JointID) / Destroy(BodyID) between steps on a
+    device-resident world (the reference driver applied the same edits before
+    the same steps; a destroyed body's touching contacts end before the next
+    step's listener calls)."""
     g = golden_cases.load(name)
     print("case", name, "edits", len(g["edits"]), flush=True)
     w = _recording(pa.World.from_desc(g["desc_conf"]))
@@ -215,6 +217,12 @@ def test_world_edits_match_reference_every_step(name):
                 w.set_transformation(body, (a, b, float(q[0]), float(q[1])))
             elif kind == "O":
                 w.shift_origin((a, b))
+            elif kind == "V":
+                w.set_velocity(body, (a, b, c))
+            elif kind == "J":
+                w.destroy_joint(body)
+            elif kind == "D":
+                w.destroy_body(body)
         w.step(conf)
         s = _state(w)
         for k, part in enumerate(golden_cases.PARTS):
