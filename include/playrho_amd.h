@@ -338,6 +338,12 @@ int prx_world_destroy_joint(prx_world* world, uint32_t joint);
  * saving the world's description are PRX_E_UNSUPPORTED (the reference would
  * reuse the id at the end of its body list). */
 int prx_world_destroy_body(prx_world* world, uint32_t body);
+/* SetType(World&, BodyID, BodyType, resetMassData = true) (WorldBody.hpp:289;
+ * WorldBody.cpp:236-246; Body::SetType, Body.cpp:124-149; SetBody,
+ * AabbTreeWorld.cpp:2739-2759; ResetMassData, WorldBody.cpp:360-400): type
+ * PRX_STATIC, PRX_KINEMATIC or PRX_DYNAMIC.  The body's contacts are destroyed
+ * (as by prx_world_destroy_body) and its mass data recomputed from its shapes. */
+int prx_world_set_type(prx_world* world, uint32_t body, uint32_t type);
 /* SetAcceleration(World&, BodyID, LinearAcceleration2, AngularAcceleration)
  * (d2/WorldBody.hpp; Body::SetAcceleration, Body.cpp:203-238): a = linear x,
  * y, angular. */

@@ -617,6 +617,7 @@ void Edit(Handle<R>& h, int kind, std::uint32_t body, float a, float b, float c)
     case 'V': w.SetVelocity(body, V2<R>{R(a), R(b)}, R(c)); break;
     case 'J': w.DestroyJoint(body); break;
     case 'D': w.DestroyBody(body); break;
+    case 'S': w.SetType(body, std::uint32_t(a)); break;
     default: break;
     }
 }

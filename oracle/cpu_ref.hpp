@@ -3818,6 +3818,186 @@ public:
         b.destroyed = true;
     }
 
+    // Mass data of one shape (MassData.cpp:40-181): a disk (one vertex), an
+    // edge with rounded ends (two), a polygon (more; the vertex radius does
+    // not enter).
+    struct MassDataR {
+        V2<R> center;
+        R mass = 0, I = 0;
+    };
+    static R PolarMoment(const V2<R>* v, std::uint32_t count) {  // GetPolarMoment (Math.cpp:243-274)
+        R sx = 0, sy = 0;
+        for (std::uint32_t i = 0; i < count; ++i) {
+            const V2<R> a = v[i];
+            const V2<R> b = v[(i + 1 < count) ? i + 1 : 0];
+            const R fb = Cross(a, b);
+            sx += ((a.y * a.y) + a.y * b.y + (b.y * b.y)) * fb;
+            sy += ((a.x * a.x) + a.x * b.x + (b.x * b.x)) * fb;
+        }
+        return (sx + sy) * (R(1) / R(12));
+    }
+    static MassDataR ShapeMassData(const Shape<R>& sh) {
+        const std::uint32_t count = std::uint32_t(sh.v.size());
+        const R r = sh.radius, density = sh.density;
+        if (count == 0) return MassDataR{};
+        if (count == 1) {
+            const R r2 = r * r;
+            const R area = r2 * kPiR;
+            const R mass = density * area;
+            const R Iz = area * ((r2 / R(2)) + MagSq(sh.v[0]));
+            return MassDataR{sh.v[0], mass, Iz * density / R(1)};
+        }
+        if (count == 2) {
+            const V2<R> v0 = sh.v[0], v1 = sh.v[1];
+            const R r2 = r * r;
+            const R circleArea = r2 * kPiR;
+            const R circleMass = density * circleArea;
+            const V2<R> d = v1 - v0;
+            const V2<R> offset = RevPerp(UnitVector(d, V2<R>{R(0), R(0)})) * r;
+            const R b = std::sqrt(MagSq(d));
+            const R h = r * R(2);
+            const R rectMass = density * b * h;
+            const R totalMass = circleMass + rectMass;
+            const V2<R> center = (v0 + v1) / R(2);
+            const R halfCircleArea = circleArea / R(2);
+            const R halfR2 = r2 / R(2);
+            const V2<R> verts[4] = {v0 + offset, v0 - offset, v1 - offset, v1 + offset};
+            const R Iz = PolarMoment(verts, 4);
+            const R I0 = halfCircleArea * (halfR2 + MagSq(v0));
+            const R I1 = halfCircleArea * (halfR2 + MagSq(v1));
+            return MassDataR{center, totalMass, (I0 + I1 + Iz) * density / R(1)};
+        }
+        V2<R> center{}, sum{};
+        R area = 0, I = 0;
+        for (const V2<R>& p : sh.v) sum = sum + p;
+        const V2<R> sc = sum / R(count);
+        for (std::uint32_t i = 0; i < count; ++i) {
+            const V2<R> e1 = sh.v[i] - sc;
+            const V2<R> e2 = sh.v[(i + 1 < count) ? i + 1 : 0] - sc;
+            const R D = Cross(e1, e2);
+            const R triangleArea = D * (R(1) / R(2));
+            area += triangleArea;
+            center = center + (triangleArea * (e1 + e2)) * (R(1) / R(3));
+            const R intx2 = (e1.x * e1.x) + e2.x * e1.x + (e2.x * e2.x);
+            const R inty2 = (e1.y * e1.y) + e2.y * e1.y + (e2.y * e2.y);
+            I += D * (intx2 + inty2) * (R(1) / R(12));
+        }
+        const R mass = density * area;
+        center = ((area > R(0)) && !AlmostZero(area)) ? center / area : V2<R>{};
+        const V2<R> mc = center + sc;
+        const R lever = MagSq(mc) - MagSq(center);
+        return MassDataR{mc, mass, ((density * I) + (mass * lever)) / R(1)};
+    }
+    static constexpr R kPiR = R(3.14159265358979323846264338327950288L);  // RealConstants.hpp:57
+
+    // SetBody's handling of a changed transformation (AabbTreeWorld.cpp:2787-2793).
+    void TransformationChanged(std::uint32_t bid, const Xf<R>& old) {
+        const Xf<R>& xf = bodies[bid].xf;
+        if (!(old.p == xf.p) || !(old.q == xf.q)) {
+            FlagForUpdating(bid);
+            bodiesForSync.push_back(bid);
+        }
+    }
+
+    // ResetMassData (WorldBody.cpp:360-363): ComputeMassData over the body's
+    // shapes of positive density (WorldShape.cpp:160-176), then SetMassData
+    // (WorldBody.cpp:365-400).
+    void ResetMassData(std::uint32_t bid) {
+        Body<R>& b = bodies[bid];
+        R mass = 0, I = 0;
+        V2<R> wc{};
+        for (std::uint32_t sid : b.shapes) {
+            const Shape<R>& sh = shapes[sid];
+            if (sh.density > R(0)) {
+                const MassDataR md = ShapeMassData(sh);
+                mass += md.mass;
+                wc = wc + (md.mass / R(1)) * md.center;
+                I += md.I;
+            }
+        }
+        const V2<R> center = (mass > R(0)) ? wc / (mass / R(1)) : V2<R>{};
+        const Xf<R> old = b.xf;
+        if (!(b.flags & kAccelerable)) {
+            b.invMass = R(0);
+            b.invRotI = R(0);
+            if (!(b.flags & kSpeedable)) {
+                // Sweep{Position{GetLocation(body), GetAngle(body)}}: local centre zero.
+                const Pos<R> p{b.xf.p, b.pos1.ang};
+                b.pos0 = p;
+                b.pos1 = p;
+                b.lc = V2<R>{};
+                b.alpha0 = R(0);
+                b.xf = GetXf(b.pos1, b.lc);
+            }
+            TransformationChanged(bid, old);
+            return;
+        }
+        const R m = (mass > R(0)) ? mass : R(1);
+        const R invMass = R(1) / m;
+        R invRotI = R(0) / R(1);
+        if ((I > R(0)) && !(b.flags & kFixedRotation)) {
+            const R lsq = MagSq(center);
+            invRotI = R(1) / (I - (m * lsq) / R(1));
+        }
+        b.invMass = invMass;
+        b.invRotI = invRotI;
+        const V2<R> oldCenter = b.pos1.lin;
+        const Pos<R> p{Transform(center, b.xf), b.pos1.ang};
+        b.pos0 = p;
+        b.pos1 = p;
+        b.lc = center;
+        b.alpha0 = R(0);
+        b.xf = GetXf(b.pos1, b.lc);
+        const V2<R> dc = b.pos1.lin - oldCenter;
+        b.vel.lin = b.vel.lin + RevPerp(dc) * (b.vel.ang / R(1));
+        TransformationChanged(bid, old);
+    }
+
+    // SetType(World&, BodyID, BodyType, resetMassData = true) (WorldBody.cpp:236-246):
+    // Body::SetType (Body.cpp:124-149), SetBody (AabbTreeWorld.cpp:2723-2795)
+    // for the new type (the body's contacts destroyed in its list's order; a
+    // static body queued for synchronisation), then ResetMassData.
+    // type: 0 static, 1 kinematic, 2 dynamic.
+    void SetType(std::uint32_t bid, std::uint32_t type) {
+        Body<R>& b = bodies[bid];
+        if (b.type == type) return;
+        b.type = type;
+        b.flags &= ~(kImpenetrable | kSpeedable | kAccelerable);
+        b.flags |= (type == 2) ? (kSpeedable | kAccelerable) : (type == 1) ? (kImpenetrable | kSpeedable) : kImpenetrable;
+        if (type == 2) {
+            b.flags |= kAwake;
+        } else if (type == 1) {
+            b.flags |= kAwake;
+            b.lacc = V2<R>{};
+            b.aacc = R(0);
+            b.invMass = R(0);
+            b.invRotI = R(0);
+        } else {
+            b.flags &= ~kAwake;
+            b.vel = Vel<R>{};
+            b.pos0 = b.pos1;
+            b.lacc = V2<R>{};
+            b.aacc = R(0);
+            b.invMass = R(0);
+            b.invRotI = R(0);
+        }
+        b.underActive = R(0);
+        FreshEvents();
+        const std::vector<KeyedId> own = bodyContacts[bid];
+        for (const KeyedId& kc : own) {
+            for (auto it = contacts.begin(); it != contacts.end(); ++it) {
+                if (it->id == kc.id) {
+                    contacts.erase(it);
+                    break;
+                }
+            }
+            InternalDestroy(kc.id, bid);
+        }
+        bodyContacts[bid].clear();
+        if (type == 0) bodiesForSync.push_back(bid);
+        ResetMassData(bid);
+    }
+
     // Listener calls made by edits between steps (a destroyed contact's
     // end-contact call) come before the next step's calls.
     bool eventsStale = false;

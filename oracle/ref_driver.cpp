@@ -674,6 +674,7 @@ int TimeWorlds(int argc, char** argv) {
 //   V  SetVelocity(world, body, {(a, b), c})
 //   J  Destroy(world, JointID(body))
 //   D  Destroy(world, BodyID(body))
+//   S  SetType(world, body, BodyType(a)) (0 static, 1 kinematic, 2 dynamic)
 struct Edit {
     int step = 0;
     char kind = 0;
@@ -719,6 +720,9 @@ void ApplyEdit(World& world, const Edit& e) {
         break;
     case 'D':
         Destroy(world, id);
+        break;
+    case 'S':
+        SetType(world, id, static_cast<BodyType>(int(e.a)));
         break;
     default:
         break;

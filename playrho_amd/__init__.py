@@ -356,6 +356,7 @@ _SIGS = {
                                               ctypes.POINTER(ctypes.c_float)]),
     "prx_world_destroy_body": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     "prx_world_destroy_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "prx_world_set_type": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
     "prx_world_state_record": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
                                               ctypes.POINTER(ctypes.c_size_t)]),
     "prx_world_contact_count": (ctypes.c_uint32, [ctypes.c_void_p]),
@@ -544,6 +545,11 @@ class World:
     def destroy_body(self, body):
         """Destroy(World&, BodyID) (AabbTreeWorld.cpp:1062-1104)."""
         _check(lib().prx_world_destroy_body(self.h, int(body)))
+
+    def set_type(self, body, body_type):
+        """SetType(World&, BodyID, BodyType) (WorldBody.cpp:236-246): 0
+        static, 1 kinematic, 2 dynamic; the mass data is reset."""
+        _check(lib().prx_world_set_type(self.h, int(body), int(body_type)))
 
     def destroy_joint(self, joint):
         """Destroy(World&, JointID) (AabbTreeWorld.cpp:1215-1227)."""

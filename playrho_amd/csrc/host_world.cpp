@@ -497,6 +497,63 @@ int AddShape(prx_world* w, const prx_shape_conf& c, std::uint32_t* out) {
     return PRX_OK;
 }
 
+// ResetMassData (WorldBody.cpp:360-363): ComputeMassData over the body's
+// shapes of positive density (WorldShape.cpp:160-176), then SetMassData
+// (WorldBody.cpp:365-400): a body that is not accelerable gets no mass (a
+// static one a sweep at its origin with local centre zero); an accelerable
+// one its mass (1 kg if none), rotational inertia about the centre of mass
+// (none with fixed rotation), the centre of mass moved there and the linear
+// velocity changed by the angular one about the move.  Returns whether the
+// transformation changed (SetBody then flags the contacts and syncs).
+bool ResetMass(const prx_world* w, HostBody& b) {
+    const V2 q = UnitFromAngle(b.pos1[2]);
+    const V2 xfp = V2{b.pos1[0], b.pos1[1]} - Rotate(b.lc, q);
+    if (!(b.flags & prx::kAccelerable)) {
+        b.invMass = 0;
+        b.invRotI = 0;
+        if (b.flags & prx::kSpeedable) return false;
+        b.pos0[0] = b.pos1[0] = xfp.x;
+        b.pos0[1] = b.pos1[1] = xfp.y;
+        b.pos0[2] = b.pos1[2];
+        b.lc = V2{0, 0};
+        const V2 p1 = V2{b.pos1[0], b.pos1[1]} - Rotate(b.lc, q);
+        return !(p1.x == xfp.x && p1.y == xfp.y);
+    }
+    float mass = 0, I = 0;
+    V2 wc{0, 0};
+    for (std::uint32_t sid : b.shapes) {
+        const HostShape& s = w->shapes[sid];
+        if (s.density > 0) {
+            const MassData md = ShapeMass(s);
+            mass += md.mass;
+            wc = wc + Mul(md.mass / 1.0f, md.center);
+            I += md.I;
+        }
+    }
+    const V2 center = (mass > 0) ? DivS(wc, mass / 1.0f) : V2{0, 0};
+    const float m = (mass > 0) ? mass : 1.0f;
+    b.invMass = 1.0f / m;
+    b.invRotI = 0.0f / 1.0f;
+    if ((I > 0) && !(b.flags & prx::kFixedRotation)) {
+        const float lsq = MagSq(center);
+        const float Ic = I - (m * lsq) / 1.0f;
+        b.invRotI = 1.0f / Ic;
+    }
+    // Move the center of mass (sweep) and update the velocity.
+    const V2 oldCenter{b.pos1[0], b.pos1[1]};
+    const V2 newc = Rotate(center, q) + xfp;
+    b.pos0[0] = b.pos1[0] = newc.x;
+    b.pos0[1] = b.pos1[1] = newc.y;
+    b.pos0[2] = b.pos1[2];
+    b.lc = center;
+    const V2 dc = newc - oldCenter;
+    const float wv = b.vel[2] / 1.0f;
+    b.vel[0] += -dc.y * wv;
+    b.vel[1] += dc.x * wv;
+    const V2 p1 = newc - Rotate(b.lc, q);
+    return !(p1.x == xfp.x && p1.y == xfp.y);
+}
+
 int AddBody(prx_world* w, const prx_body_conf& c, std::uint32_t* out) {
     // The reference would give the new body the id freed last while putting
     // it at the end of the body list, and its proxies the tree nodes freed
@@ -545,44 +602,7 @@ int AddBody(prx_world* w, const prx_body_conf& c, std::uint32_t* out) {
         b.acc[2] = c.angular_acceleration;
     }
     b.shapes.assign(c.shapes, c.shapes + c.nshapes);
-    // ResetMassData -> SetMassData (WorldBody.cpp:360-399).
-    if (!(f & prx::kAccelerable)) {
-        b.invMass = 0;
-        b.invRotI = 0;
-    } else {
-        float mass = 0, I = 0;
-        V2 wc{0, 0};
-        for (std::uint32_t sid : b.shapes) {
-            const HostShape& s = w->shapes[sid];
-            if (s.density > 0) {
-                const MassData md = ShapeMass(s);
-                mass += md.mass;
-                wc = wc + Mul(md.mass / 1.0f, md.center);
-                I += md.I;
-            }
-        }
-        const V2 center = (mass > 0) ? DivS(wc, mass / 1.0f) : V2{0, 0};
-        const float m = (mass > 0) ? mass : 1.0f;
-        b.invMass = 1.0f / m;
-        b.invRotI = 0.0f / 1.0f;
-        if ((I > 0) && !(f & prx::kFixedRotation)) {
-            const float lsq = MagSq(center);
-            const float Ic = I - (m * lsq) / 1.0f;
-            b.invRotI = 1.0f / Ic;
-        }
-        // Move the center of mass (sweep) and update the velocity.
-        const V2 q = UnitFromAngle(b.pos1[2]);
-        const V2 xfp = V2{b.pos1[0], b.pos1[1]} - Rotate(V2{0, 0}, q);
-        const V2 oldCenter{b.pos1[0], b.pos1[1]};
-        const V2 newc = Rotate(center, q) + xfp;
-        b.pos0[0] = b.pos1[0] = newc.x;
-        b.pos0[1] = b.pos1[1] = newc.y;
-        b.lc = center;
-        const V2 dc = newc - oldCenter;
-        const float wv = b.vel[2] / 1.0f;
-        b.vel[0] += -dc.y * wv;
-        b.vel[1] += dc.x * wv;
-    }
+    ResetMass(w, b);
     const int bid = int(w->bodies.size());
     if (f & prx::kEnabled) {
         for (std::uint32_t sid : b.shapes) {
@@ -664,6 +684,46 @@ void WakeBody(HostBody& b) {
         b.flags |= prx::kAwake;
         b.under = 0.0f;
     }
+}
+
+// The contacts of a body being destroyed or changing type, in creation
+// order (AabbTreeWorld::Destroy(ContactID, from) :2281-2288 and
+// InternalDestroy :2245-2279): an end-contact call for a touching one (with
+// contact events on; they come before the next step's calls), both bodies
+// awake if the manifold has points, the id to the contact pool's free list.
+void DestroyBodyContacts(prx_world* w, std::uint32_t body) {
+    HostContacts& hc = w->contacts;
+    std::size_t out = 0;
+    for (std::size_t i = 0; i < hc.size(); ++i) {
+        const int2 bd = hc.body[i];
+        if (bd.x != int(body) && bd.y != int(body)) {
+            if (out != i) {
+                hc.prox[out] = hc.prox[i];
+                hc.body[out] = hc.body[i];
+                hc.shape[out] = hc.shape[i];
+                hc.id[out] = hc.id[i];
+                hc.flags[out] = hc.flags[i];
+                hc.mat[out] = hc.mat[i];
+                hc.geo[out] = hc.geo[i];
+                hc.p0[out] = hc.p0[i];
+                hc.p1[out] = hc.p1[i];
+                hc.hdr[out] = hc.hdr[i];
+            }
+            ++out;
+            continue;
+        }
+        if (w->record_events && (hc.flags[i] & prx::kTouching)) {
+            w->pre_events.push_back(prx::kEvEnd);
+            w->pre_events.push_back(hc.id[i]);
+            ++w->pre_nevents;
+        }
+        if (((hc.hdr[i].x >> 8) & 0xffu) > 0 && !(hc.flags[i] & prx::kSensor)) {
+            WakeBody(w->bodies[std::size_t(bd.x)]);
+            WakeBody(w->bodies[std::size_t(bd.y)]);
+        }
+        hc.freelist.push_back(hc.id[i]);
+    }
+    hc.resize(out);
 }
 
 // AabbTreeWorld::Remove(JointID) (AabbTreeWorld.cpp:1177-1213) and the rest
@@ -2261,38 +2321,7 @@ int prx_world_destroy_body(prx_world* w, std::uint32_t body) {
         if (first == w->joints.size()) break;
         DestroyJointRecord(w, std::uint32_t(first));
     }
-    HostContacts& hc = w->contacts;
-    std::size_t out = 0;
-    for (std::size_t i = 0; i < hc.size(); ++i) {
-        const int2 bd = hc.body[i];
-        if (bd.x != int(body) && bd.y != int(body)) {
-            if (out != i) {
-                hc.prox[out] = hc.prox[i];
-                hc.body[out] = hc.body[i];
-                hc.shape[out] = hc.shape[i];
-                hc.id[out] = hc.id[i];
-                hc.flags[out] = hc.flags[i];
-                hc.mat[out] = hc.mat[i];
-                hc.geo[out] = hc.geo[i];
-                hc.p0[out] = hc.p0[i];
-                hc.p1[out] = hc.p1[i];
-                hc.hdr[out] = hc.hdr[i];
-            }
-            ++out;
-            continue;
-        }
-        if (w->record_events && (hc.flags[i] & prx::kTouching)) {
-            w->pre_events.push_back(prx::kEvEnd);
-            w->pre_events.push_back(hc.id[i]);
-            ++w->pre_nevents;
-        }
-        if (((hc.hdr[i].x >> 8) & 0xffu) > 0 && !(hc.flags[i] & prx::kSensor)) {
-            WakeBody(w->bodies[std::size_t(bd.x)]);
-            WakeBody(w->bodies[std::size_t(bd.y)]);
-        }
-        hc.freelist.push_back(hc.id[i]);
-    }
-    hc.resize(out);
+    DestroyBodyContacts(w, body);
     for (int pi : hb.proxies) {
         HostProxy& p = w->proxies[std::size_t(pi)];
         p.dead = true;
@@ -2302,6 +2331,46 @@ int prx_world_destroy_body(prx_world* w, std::uint32_t body) {
     hb.flags = 0;
     hb.destroyed = true;
     ++w->destroyed_bodies;
+    return PRX_OK;
+}
+
+// SetType(World&, BodyID, BodyType, resetMassData = true) (WorldBody.cpp:236-246):
+// Body::SetType (Body.cpp:124-149): the type's flags, awake unless static;
+// kinematic and static bodies lose their accelerations and mass, a static
+// one its velocity (position 0 becomes position 1).  SetBody
+// (AabbTreeWorld.cpp:2739-2759) destroys the body's contacts and queues a
+// static body for synchronisation; ResetMassData follows.
+int prx_world_set_type(prx_world* w, std::uint32_t body, std::uint32_t type) {
+    if (int st = CheckWorld(w)) return st;
+    if (int st = CheckBody(w, body)) return st;
+    if (type > PRX_DYNAMIC) return Fail(PRX_E_INVALID_ARGUMENT, "no such body type");
+    if (int st = Detach(w)) return st;
+    HostBody& b = w->bodies[body];
+    if (b.type == type) return PRX_OK;
+    b.type = type;
+    b.flags &= ~(prx::kImpenetrable | prx::kSpeedable | prx::kAccelerable);
+    if (type == PRX_DYNAMIC) {
+        b.flags |= prx::kSpeedable | prx::kAccelerable | prx::kAwake;
+    } else if (type == PRX_KINEMATIC) {
+        b.flags |= prx::kImpenetrable | prx::kSpeedable | prx::kAwake;
+        b.acc[0] = b.acc[1] = b.acc[2] = 0.0f;
+        b.invMass = b.invRotI = 0.0f;
+    } else {
+        b.flags = (b.flags | prx::kImpenetrable) & ~prx::kAwake;
+        b.vel[0] = b.vel[1] = b.vel[2] = 0.0f;
+        for (int k = 0; k < 3; ++k) b.pos0[k] = b.pos1[k];
+        b.acc[0] = b.acc[1] = b.acc[2] = 0.0f;
+        b.invMass = b.invRotI = 0.0f;
+    }
+    b.under = 0.0f;
+    DestroyBodyContacts(w, body);
+    if (type == PRX_STATIC) b.flags |= prx::kNeedsSync;
+    if (ResetMass(w, b)) {
+        HostContacts& hc = w->contacts;
+        for (std::size_t i = 0; i < hc.size(); ++i)
+            if (hc.body[i].x == int(body) || hc.body[i].y == int(body)) hc.flags[i] |= prx::kNeedsUpdating;
+        b.flags |= prx::kNeedsSync;
+    }
     return PRX_OK;
 }
 

@@ -488,7 +488,15 @@ Body GetBody(const GpuWorld& g, BodyID id) {
 void SetBody(GpuWorld& g, BodyID id, const Body& value) {
     const Body cur = GetBody(g, id);
     const auto i = to_underlying(id);
-    if (GetType(value) != GetType(cur)) Unsupported("changing a body's type");
+    if (GetType(value) != GetType(cur)) {
+        // SetType(World&, BodyID, BodyType) arrives here with Body::SetType
+        // applied; the engine makes the change and resets the mass data, as
+        // SetType's default resetMassData does (its ResetMassData then finds
+        // nothing left to change).
+        Check(prx_world_set_type(g.m_world, i, std::uint32_t(GetType(value))));
+        g.m_bodies[i] = value;  // type, flags and accelerations; the state comes from the engine
+        return;
+    }
     if (value.GetShapes() != cur.GetShapes()) Unsupported("attaching or detaching shapes after creation");
     if (value.GetInvMass() != cur.GetInvMass() || value.GetInvRotInertia() != cur.GetInvRotInertia() ||
         value.GetSweep().localCenter != cur.GetSweep().localCenter)

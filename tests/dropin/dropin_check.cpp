@@ -157,6 +157,13 @@ int Cpu() {
     Expect(GetBodies(ref2) == GetBodies(gpu2), "bodies after destroying");
     Expect(GetJoints(ref2) == GetJoints(gpu2), "joints after destroying");
     CompareBodies(ref2, gpu2, "chain after destroying");
+    // SetType: a link made static and then dynamic again (mass data reset).
+    SetType(ref2, BodyID(4), BodyType::Static);
+    SetType(gpu2, BodyID(4), BodyType::Static);
+    CompareBodies(ref2, gpu2, "chain link made static");
+    SetType(ref2, BodyID(4), BodyType::Dynamic);
+    SetType(gpu2, BodyID(4), BodyType::Dynamic);
+    CompareBodies(ref2, gpu2, "chain link made dynamic");
     const auto jc = Joint{RevoluteJointConf{}.UseBodyA(BodyID(2)).UseBodyB(BodyID(5))};
     Expect(CreateJoint(ref2, jc) == CreateJoint(gpu2, jc), "joint id after destroying");
     Expect(GetJoints(ref2) == GetJoints(gpu2), "joints after creating");

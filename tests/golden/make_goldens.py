@@ -57,7 +57,7 @@ CASES = [
 ]
 # World edits between steps (step, kind, body, a, b, c): SetLocation 'L',
 # SetAngle 'A', SetTransformation 'T' (c the rotation's angle), ShiftOrigin 'O',
-# SetVelocity 'V', Destroy(JointID) 'J', Destroy(BodyID) 'D'.
+# SetVelocity 'V', Destroy(JointID) 'J', Destroy(BodyID) 'D', SetType 'S'.
 EDIT_CASES = [
     ("pile3_edit", "pile", 120, [1, 3], [1, 120], False,
      [(20, "L", 5, 1.25, 6.5, 0), (35, "A", 9, 0.7, 0, 0), (50, "T", 12, -1.5, 8.0, 2.5), (70, "O", 0, 3.0, -0.5, 0),
@@ -78,6 +78,15 @@ EDIT_CASES = [
     ("joints0_destroy_edit", "joints", 160, [1, 0], [1, 160], False,
      [(40, "J", 3, 0, 0, 0), (60, "J", 18, 0, 0, 0), (90, "D", 24, 0, 0, 0), (110, "D", 19, 0, 0, 0),
       (130, "D", 50, 0, 0, 0), (130, "D", 45, 0, 0, 0)]),
+    # SetType 'S' (a: 0 static, 1 kinematic, 2 dynamic): a resting body made
+    # static and dynamic again, one made kinematic (then given a velocity)
+    # and dynamic again; the mass data reset each time.
+    ("pile3_type_edit", "pile", 120, [1, 3], [1, 120], False,
+     [(40, "S", 30, 0, 0, 0), (60, "S", 50, 1, 0, 0), (80, "S", 30, 2, 0, 0), (90, "V", 50, 2.0, 0, 1.0),
+      (100, "S", 50, 2, 0, 0)]),
+    ("pile3_type_edit_toi", "pile", 120, [1, 3], [1, 120], True,
+     [(40, "S", 30, 0, 0, 0), (60, "S", 50, 1, 0, 0), (80, "S", 30, 2, 0, 0), (90, "V", 50, 2.0, 0, 1.0),
+      (100, "S", 50, 2, 0, 0)]),
     # Bodies destroyed out of a resting pile (their touching contacts end,
     # their neighbours wake), with the TOI phase.
     ("pile3_destroy_edit_toi", "pile", 120, [1, 3], [1, 120], True,

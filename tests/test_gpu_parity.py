@@ -195,7 +195,7 @@ def _conf_bytes():
 @pytest.mark.parametrize("name", golden_cases.edit_names())
 def test_world_edits_match_reference_every_step(name):
     """SetLocation / SetAngle / SetTransformation / ShiftOrigin /
-    SetVelocity / Destroy(JointID) / Destroy(BodyID) between steps on a
+    SetVelocity / Destroy(JointID) / Destroy(BodyID) / SetType between steps on a
     device-resident world (the reference driver applied the same edits before
     the same steps; a destroyed body's touching contacts end before the next
     step's listener calls)."""
@@ -223,6 +223,8 @@ def test_world_edits_match_reference_every_step(name):
                 w.destroy_joint(body)
             elif kind == "D":
                 w.destroy_body(body)
+            elif kind == "S":
+                w.set_type(body, int(a))
         w.step(conf)
         s = _state(w)
         for k, part in enumerate(golden_cases.PARTS):
