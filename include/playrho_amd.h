@@ -43,7 +43,7 @@ typedef struct prx_batch prx_batch;
 enum { PRX_STATIC = 0, PRX_KINEMATIC = 1, PRX_DYNAMIC = 2 };
 /* Shape kinds: DiskShapeConf, PolygonShapeConf::Set (hull of points),
  * PolygonShapeConf::SetAsBox, EdgeShapeConf. */
-enum { PRX_SHAPE_DISK = 0, PRX_SHAPE_POLYGON = 1, PRX_SHAPE_BOX = 2, PRX_SHAPE_EDGE = 3 };
+enum { PRX_SHAPE_DISK = 0, PRX_SHAPE_POLYGON = 1, PRX_SHAPE_BOX = 2, PRX_SHAPE_EDGE = 3, PRX_SHAPE_CHAIN = 4, PRX_SHAPE_MULTI = 5 };
 
 /* WorldConf (d2/WorldConf.hpp:40-90): vertex-radius bounds; capacity hint. */
 typedef struct prx_world_conf {
@@ -268,6 +268,17 @@ void prx_world_destroy(prx_world* world);
 
 /* CreateShape(World&, conf) (World.hpp:512,522-526; AabbTreeWorld.cpp:1234-1257). */
 int prx_shape_create(prx_world* world, const prx_shape_conf* conf, uint32_t* shape_id);
+/* CreateShape(World&, ChainShapeConf / MultiShapeConf) (WorldShape.hpp;
+ * ChainShapeConf.cpp:38-160, MultiShapeConf.cpp:35-62, ConvexHull.cpp:30-36):
+ * conf->kind PRX_SHAPE_CHAIN (points: the chain's vertices; one child per
+ * segment) or PRX_SHAPE_MULTI (points: the hulls' points one hull after
+ * another, hull_sizes[nhulls] points each; one child per convex hull).  The
+ * children become consecutive shape ids from *first_shape on (*child_count of
+ * them), one proxy each, with the conf's material and filter; a body takes
+ * the shape by listing all of them, and its mass data counts the shape once
+ * (the chain's or multi shape's own mass formula). */
+int prx_compound_shape_create(prx_world* world, const prx_shape_conf* conf, const uint32_t* hull_sizes,
+                              uint32_t nhulls, uint32_t* first_shape, uint32_t* child_count);
 /* CreateBody(World&, BodyConf, resetMassData = true) (WorldBody.cpp:55;
  * AabbTreeWorld.cpp:1019-1044; mass from WorldBody.cpp:360-392). */
 int prx_body_create(prx_world* world, const prx_body_conf* conf, uint32_t* body_id);

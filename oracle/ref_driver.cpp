@@ -36,6 +36,9 @@
 #include <playrho/d2/PolygonShapeConf.hpp>
 #include <playrho/d2/DiskShapeConf.hpp>
 #include <playrho/d2/EdgeShapeConf.hpp>
+#include <playrho/d2/ChainShapeConf.hpp>
+#include <playrho/d2/MultiShapeConf.hpp>
+#include <playrho/d2/VertexSet.hpp>
 #include <playrho/d2/RevoluteJointConf.hpp>
 #include <playrho/d2/DistanceJointConf.hpp>
 #include <playrho/d2/WeldJointConf.hpp>
@@ -115,6 +118,27 @@ ShapeID MakeShape(World& world, const prx_scene::ShapeSpec& s) {
         conf.UseDensity(s.density).UseFriction(s.friction).UseRestitution(s.restitution);
         return CreateShape(world, conf);
     }
+    case prx_scene::kChain: {
+        auto conf = ChainShapeConf{};
+        if (s.vertexRadius >= 0) conf.UseVertexRadius(s.vertexRadius);
+        std::vector<Length2> pts;
+        for (const auto& p : s.pts) pts.push_back(Length2{p.x, p.y});
+        conf.Set(pts);
+        conf.UseDensity(s.density).UseFriction(s.friction).UseRestitution(s.restitution);
+        return CreateShape(world, conf);
+    }
+    case prx_scene::kMulti: {
+        auto conf = MultiShapeConf{};
+        std::size_t at = 0;
+        for (int n : s.hullSizes) {
+            VertexSet set;
+            for (int k = 0; k < n; ++k, ++at) set.add(Length2{s.pts[at].x, s.pts[at].y});
+            if (s.vertexRadius >= 0) conf.AddConvexHull(set, s.vertexRadius);
+            else conf.AddConvexHull(set);
+        }
+        conf.UseDensity(s.density).UseFriction(s.friction).UseRestitution(s.restitution);
+        return CreateShape(world, conf);
+    }
     default: {
         auto conf = EdgeShapeConf{Length2{s.pts[0].x, s.pts[0].y}, Length2{s.pts[1].x, s.pts[1].y}};
         if (s.vertexRadius >= 0) conf.UseVertexRadius(s.vertexRadius);
@@ -124,12 +148,29 @@ ShapeID MakeShape(World& world, const prx_scene::ShapeSpec& s) {
     }
 }
 
+// A shape with several children (a chain's segments, a multi shape's hulls)
+// is written as one description shape per child, in child order, with the
+// shape's material; bodies list the children.  The shape index of child 0 of
+// each shape is first[shape].
+std::vector<std::uint32_t> ChildBase(const World& world) {
+    std::vector<std::uint32_t> first;
+    std::uint32_t at = 0;
+    for (auto i = 0u; i < GetShapeRange(world); ++i) {
+        first.push_back(at);
+        at += std::uint32_t(GetChildCount(GetShape(world, ShapeID(i))));
+    }
+    first.push_back(at);
+    return first;
+}
+
 void WriteDescription(Out& o, const World& world) {
+    const auto first = ChildBase(world);
     const auto nshapes = GetShapeRange(world);
-    o.u32(nshapes);
-    for (auto i = 0u; i < nshapes; ++i) {
+    o.u32(first.back());
+    for (auto i = 0u; i < nshapes; ++i)
+    for (auto c = 0u; c < first[i + 1] - first[i]; ++c) {
         const auto shape = GetShape(world, ShapeID(i));
-        const auto child = GetChild(shape, 0);
+        const auto child = GetChild(shape, ChildCounter(c));
         const auto n = child.GetVertexCount();
         o.u32(n);
         o.f32(child.GetVertexRadius());
@@ -167,8 +208,11 @@ void WriteDescription(Out& o, const World& world) {
             GetInvMass(b), GetInvRotInertia(b), GetUnderActiveTime(b)};
         o.put(vals, sizeof(vals));
         const auto shapes = GetShapes(world, BodyID(i));
-        o.u32(std::uint32_t(shapes.size()));
-        for (auto s : shapes) o.u32(to_underlying(s));
+        std::uint32_t nchildren = 0;
+        for (auto s : shapes) nchildren += first[to_underlying(s) + 1] - first[to_underlying(s)];
+        o.u32(nchildren);
+        for (auto s : shapes)
+            for (auto c = first[to_underlying(s)]; c < first[to_underlying(s) + 1]; ++c) o.u32(c);
     }
     const auto njoints = GetJointRange(world);
     o.u32(njoints);

@@ -326,6 +326,9 @@ _SIGS = {
     "prx_world_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     "prx_world_destroy": (None, [ctypes.c_void_p]),
     "prx_shape_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
+    "prx_compound_shape_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32),
+                                                 ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32),
+                                                 ctypes.POINTER(ctypes.c_uint32)]),
     "prx_body_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
     "prx_scene_build": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32),
                                        ctypes.c_int]),
@@ -487,6 +490,16 @@ class World:
         sid = ctypes.c_uint32()
         _check(lib().prx_shape_create(self.h, ctypes.byref(conf), ctypes.byref(sid)))
         return sid.value
+
+    def create_compound_shape(self, conf: ShapeConf, hull_sizes=None):
+        """A chain (conf.kind 4) or multi shape (conf.kind 5, hull_sizes: points
+        per hull) as its children's shape ids (prx_compound_shape_create); a
+        body takes the shape by listing all of them."""
+        hs = (ctypes.c_uint32 * max(1, len(hull_sizes or [])))(*(hull_sizes or []))
+        first, count = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib().prx_compound_shape_create(self.h, ctypes.byref(conf), hs, len(hull_sizes or []),
+                                               ctypes.byref(first), ctypes.byref(count)))
+        return list(range(first.value, first.value + count.value))
 
     def create_body(self, conf: BodyConf):
         bid = ctypes.c_uint32()

@@ -107,6 +107,9 @@ struct HostShape {
     float radius = 0.01f, density = 0, friction = 0.2f, restitution = 0;
     std::uint32_t sensor = 0, catmask = 0xffff0001u;
     std::int32_t group = 0;
+    // A child of a chain (1) or multi (2) shape: the children are consecutive
+    // shapes from `first` on, `count` of them, one proxy each.
+    std::uint32_t compound = 0, first = 0, count = 1;
 };
 
 struct MassData {
@@ -497,6 +500,119 @@ int AddShape(prx_world* w, const prx_shape_conf& c, std::uint32_t* out) {
     return PRX_OK;
 }
 
+// A chain or multi shape (ChainShapeConf, MultiShapeConf) as its children,
+// consecutive shapes sharing the conf's material and filter: a chain of n > 1
+// vertices has n - 1 segments (ChainShapeConf::GetChild, ChainShapeConf.cpp:
+// 149-160; each a two-vertex proxy with normals n, -n where
+// n = unit(fwdperp(v[i+1] - v[i])), ChainShapeConf.cpp:38-59), of one vertex a
+// single point; a multi shape has one convex hull per child
+// (MultiShapeConf::AddConvexHull, MultiShapeConf.cpp:57-62; ConvexHull::Get,
+// ConvexHull.cpp:30-36: points welded at the VertexSet default, then the hull
+// and its forward normals).  hull_sizes: points per hull (multi shapes).
+int AddCompound(prx_world* w, const prx_shape_conf& c, const std::uint32_t* hull_sizes, std::uint32_t nhulls,
+                std::uint32_t* first_out, std::uint32_t* count_out) {
+    if (c.kind != PRX_SHAPE_CHAIN && c.kind != PRX_SHAPE_MULTI) return Fail(PRX_E_INVALID_ARGUMENT, "not a chain or multi shape");
+    if (!c.points && c.npoints) return Fail(PRX_E_INVALID_ARGUMENT, "points missing");
+    const float radius = (c.vertex_radius >= 0) ? c.vertex_radius : kDefaultLinearSlop * 2.0f;
+    if (!(radius >= w->conf.min_vertex_radius) || !(radius <= w->conf.max_vertex_radius))
+        return Fail(PRX_E_INVALID_ARGUMENT, "CreateShape: vertex radius out of range");
+    std::vector<std::vector<V2>> children;
+    if (c.kind == PRX_SHAPE_CHAIN) {
+        if (c.npoints == 0) return Fail(PRX_E_INVALID_ARGUMENT, "a chain needs a vertex");
+        if (c.npoints == 1) children.push_back({V2{c.points[0], c.points[1]}});
+        for (std::uint32_t i = 0; i + 1 < c.npoints; ++i)
+            children.push_back({V2{c.points[2 * i], c.points[2 * i + 1]}, V2{c.points[2 * i + 2], c.points[2 * i + 3]}});
+    } else {
+        if (!hull_sizes || nhulls == 0) return Fail(PRX_E_INVALID_ARGUMENT, "a multi shape needs hulls");
+        const float minSepSq = std::sqrt(std::numeric_limits<float>::min());
+        std::uint32_t at = 0;
+        for (std::uint32_t h = 0; h < nhulls; ++h) {
+            if (at + hull_sizes[h] > c.npoints) return Fail(PRX_E_INVALID_ARGUMENT, "hull sizes exceed the points");
+            std::vector<V2> set;
+            for (std::uint32_t k = 0; k < hull_sizes[h]; ++k, ++at) {
+                const V2 p{c.points[2 * at], c.points[2 * at + 1]};
+                bool dup = false;
+                for (const V2& e : set) {
+                    if (MagSq(p - e) <= minSepSq) {
+                        dup = true;
+                        break;
+                    }
+                }
+                if (!dup) set.push_back(p);
+            }
+            std::vector<V2> hull = ConvexHull(set);
+            if (hull.empty()) return Fail(PRX_E_INVALID_ARGUMENT, "empty hull");
+            if (int(hull.size()) > kMaxShapeVertices) return Fail(PRX_E_UNSUPPORTED, "more than 16 vertices");
+            children.push_back(std::move(hull));
+        }
+    }
+    const std::uint32_t first = std::uint32_t(w->shapes.size());
+    for (auto& v : children) {
+        HostShape s;
+        s.radius = radius;
+        s.n = (c.kind == PRX_SHAPE_CHAIN && v.size() == 2) ? std::vector<V2>{} : FwdNormals(v);
+        if (c.kind == PRX_SHAPE_CHAIN && v.size() == 2) {
+            const V2 e = v[1] - v[0];
+            const V2 n = UnitVector(V2{e.y, -e.x}, V2{0, 0});
+            s.n = {n, V2{-n.x, -n.y}};
+        }
+        s.v = std::move(v);
+        s.density = c.density;
+        s.friction = c.friction;
+        s.restitution = c.restitution;
+        s.sensor = c.sensor ? 1u : 0u;
+        s.catmask = std::uint32_t(c.category_bits) | (std::uint32_t(c.mask_bits) << 16);
+        s.group = c.group_index;
+        s.compound = (c.kind == PRX_SHAPE_CHAIN) ? 1u : 2u;
+        s.first = first;
+        s.count = std::uint32_t(children.size());
+        w->shapes.push_back(std::move(s));
+    }
+    if (first_out) *first_out = first;
+    if (count_out) *count_out = std::uint32_t(children.size());
+    return PRX_OK;
+}
+
+// Mass data of a whole chain or multi shape from its children: a chain
+// (ChainShapeConf::GetMassData, ChainShapeConf.cpp:113-147) sums its
+// segments' capsule masses and divides their weighted centres by the summed
+// capsule areas; a multi shape (GetMassData(MultiShapeConf),
+// MultiShapeConf.cpp:35-55) sums its hulls' masses, centres weighted by mass
+// and inertias.
+MassData CompoundMass(const prx_world* w, const HostShape& first) {
+    const HostShape& c0 = w->shapes[first.first];
+    if (c0.compound == 1) {
+        if (c0.count == 1 && c0.v.size() == 1) return DiskMass(c0.radius, c0.density, c0.v[0]);
+        float mass = 0, I = 0, area = 0;
+        V2 center{0, 0};
+        const float circleArea = (c0.radius * c0.radius) * kPi;
+        for (std::uint32_t k = 0; k < c0.count; ++k) {
+            const HostShape& e = w->shapes[first.first + k];
+            const MassData md = EdgeMass(e.radius, e.density, e.v[0], e.v[1]);
+            mass += md.mass;
+            center = center + Mul(md.mass / 1.0f, md.center);
+            I += md.I;
+            const V2 d = e.v[1] - e.v[0];
+            const float b = std::sqrt(MagSq(d));
+            const float h = e.radius * 2.0f;
+            area += b * h + circleArea;
+        }
+        center = V2{center.x / area, center.y / area};
+        return MassData{center, mass, I};
+    }
+    float mass = 0, I = 0;
+    V2 wc{0, 0};
+    for (std::uint32_t k = 0; k < c0.count; ++k) {
+        const HostShape& h = w->shapes[first.first + k];
+        const MassData md = PolygonMass(h.radius, h.density, h.v);
+        mass += md.mass;
+        wc = wc + Mul(md.mass, md.center);
+        I += md.I;
+    }
+    const V2 center = (mass > 0) ? V2{wc.x / mass, wc.y / mass} : V2{0, 0};
+    return MassData{center, mass, I};
+}
+
 // ResetMassData (WorldBody.cpp:360-363): ComputeMassData over the body's
 // shapes of positive density (WorldShape.cpp:160-176), then SetMassData
 // (WorldBody.cpp:365-400): a body that is not accelerable gets no mass (a
@@ -523,8 +639,10 @@ bool ResetMass(const prx_world* w, HostBody& b) {
     V2 wc{0, 0};
     for (std::uint32_t sid : b.shapes) {
         const HostShape& s = w->shapes[sid];
+        // A chain or multi shape counts once, at its first child.
+        if (s.compound && sid != s.first) continue;
         if (s.density > 0) {
-            const MassData md = ShapeMass(s);
+            const MassData md = s.compound ? CompoundMass(w, s) : ShapeMass(s);
             mass += md.mass;
             wc = wc + Mul(md.mass / 1.0f, md.center);
             I += md.I;
@@ -1649,6 +1767,14 @@ int prx_shape_create(prx_world* w, const prx_shape_conf* conf, std::uint32_t* id
     return AddShape(w, *conf, id);
 }
 
+int prx_compound_shape_create(prx_world* w, const prx_shape_conf* conf, const std::uint32_t* hull_sizes,
+                              std::uint32_t nhulls, std::uint32_t* first_shape, std::uint32_t* child_count) {
+    if (int st = CheckWorld(w)) return st;
+    if (!conf) return Fail(PRX_E_INVALID_ARGUMENT, "null shape conf");
+    if (int st = Detach(w)) return st;
+    return AddCompound(w, *conf, hull_sizes, nhulls, first_shape, child_count);
+}
+
 int prx_body_create(prx_world* w, const prx_body_conf* conf, std::uint32_t* id) {
     if (int st = CheckWorld(w)) return st;
     if (!conf) return Fail(PRX_E_INVALID_ARGUMENT, "null body conf");
@@ -1663,7 +1789,7 @@ int prx_scene_build(prx_world* w, const char* name, const std::int32_t* params, 
     if (!name || !prx_scene::Make(name, params, nparams, spec)) return Fail(PRX_E_INVALID_ARGUMENT, "unknown scene");
     // Shapes are created right before the first body using them, matching the
     // order of the golden-vector driver.
-    std::vector<std::uint32_t> ids(spec.shapes.size(), 0xffffffffu);
+    std::vector<std::uint32_t> ids(spec.shapes.size(), 0xffffffffu), counts(spec.shapes.size(), 1u);
     for (const auto& bs : spec.bodies) {
         std::vector<std::uint32_t> bsh;
         for (int si : bs.shapes) {
@@ -1687,9 +1813,16 @@ int prx_scene_build(prx_world* w, const char* name, const std::int32_t* params, 
                 c.restitution = s.restitution;
                 c.category_bits = 1;
                 c.mask_bits = 0xffff;
-                if (int st = AddShape(w, c, &ids[std::size_t(si)])) return st;
+                if (s.kind == prx_scene::kChain || s.kind == prx_scene::kMulti) {
+                    std::vector<std::uint32_t> hs(s.hullSizes.begin(), s.hullSizes.end());
+                    if (int st = AddCompound(w, c, hs.data(), std::uint32_t(hs.size()), &ids[std::size_t(si)],
+                                             &counts[std::size_t(si)]))
+                        return st;
+                } else if (int st = AddShape(w, c, &ids[std::size_t(si)])) {
+                    return st;
+                }
             }
-            bsh.push_back(ids[std::size_t(si)]);
+            for (std::uint32_t k = 0; k < counts[std::size_t(si)]; ++k) bsh.push_back(ids[std::size_t(si)] + k);
         }
         prx_body_conf c{};
         c.type = std::uint32_t(bs.type);

@@ -24,7 +24,7 @@
 
 namespace prx_scene {
 
-enum ShapeKind : int { kDisk = 0, kPolygon = 1, kBox = 2, kEdge = 3 };
+enum ShapeKind : int { kDisk = 0, kPolygon = 1, kBox = 2, kEdge = 3, kChain = 4, kMulti = 5 };
 enum BodyKind : int { kStatic = 0, kKinematic = 1, kDynamic = 2 };
 
 struct Pt {
@@ -35,7 +35,9 @@ struct ShapeSpec {
     int kind = kDisk;
     float radius = 0.01f;        // disk radius (vertex radius of a disk)
     float hx = 0, hy = 0;        // box half extents
-    std::vector<Pt> pts;         // polygon points (hull taken) or the 2 edge ends
+    std::vector<Pt> pts;         // polygon points (hull taken), the 2 edge ends, a chain's vertices,
+                                 // or a multi shape's hull points one hull after another
+    std::vector<int> hullSizes;  // multi shape: points per hull
     float vertexRadius = -1.0f;  // <0: the conf's default (0.01 m)
     float density = 0, friction = 0.2f, restitution = 0;
 };
@@ -745,11 +747,89 @@ inline SceneSpec Targets(std::uint32_t seed, int w, int n = 8, int falling = 24)
     return sc;
 }
 
+// Terrain: a static ground holding a chain shape (a seeded zig-zag line of
+// 16 segments) and a closed chain (a square outline), dynamic bodies made of
+// multi shapes (L, T and cross outlines of two or three box hulls), and
+// `falling` seeded boxes and disks.
+inline SceneSpec Terrain(std::uint32_t seed, int w, int falling = 40) {
+    SceneSpec sc;
+    Rng rng(seed * 2654435761u + std::uint32_t(w) * 40503u + 37u);
+    ShapeSpec line;
+    line.kind = kChain;
+    for (int i = 0; i <= 16; ++i) {
+        const float y = (i == 0 || i == 16) ? 12.0f : rng.uniform(0.0, 3.0);
+        line.pts.push_back(Pt{-32.0f + 4.0f * float(i), y});
+    }
+    sc.shapes.push_back(line);
+    ShapeSpec loop;
+    loop.kind = kChain;
+    loop.pts = {Pt{8.0f, 6.0f}, Pt{11.0f, 6.0f}, Pt{11.0f, 9.0f}, Pt{8.0f, 9.0f}, Pt{8.0f, 6.0f}};
+    sc.shapes.push_back(loop);
+    BodySpec ground;
+    ground.shapes = {0, 1};
+    sc.bodies.push_back(ground);
+    auto hullBox = [](ShapeSpec& m, float cx, float cy, float hx, float hy) {
+        m.pts.push_back(Pt{cx - hx, cy - hy});
+        m.pts.push_back(Pt{cx + hx, cy - hy});
+        m.pts.push_back(Pt{cx + hx, cy + hy});
+        m.pts.push_back(Pt{cx - hx, cy + hy});
+        m.hullSizes.push_back(4);
+    };
+    for (int k = 0; k < 6; ++k) {
+        ShapeSpec m;
+        m.kind = kMulti;
+        m.density = 1.0f;
+        m.friction = rng.uniform(0.2, 0.8);
+        const int form = k % 3;
+        if (form == 0) {  // L
+            hullBox(m, 0.0f, 0.0f, 1.0f, 0.2f);
+            hullBox(m, -0.8f, 0.6f, 0.2f, 0.4f);
+        } else if (form == 1) {  // T
+            hullBox(m, 0.0f, 0.5f, 0.9f, 0.2f);
+            hullBox(m, 0.0f, -0.1f, 0.2f, 0.4f);
+        } else {  // cross
+            hullBox(m, 0.0f, 0.0f, 0.8f, 0.15f);
+            hullBox(m, 0.0f, 0.0f, 0.15f, 0.8f);
+            hullBox(m, 0.5f, 0.5f, 0.2f, 0.2f);
+        }
+        sc.shapes.push_back(m);
+        BodySpec b;
+        b.type = kDynamic;
+        b.x = -24.0f + 8.0f * float(k);
+        b.y = rng.uniform(8.0, 14.0);
+        b.angle = rng.uniform(-1.0, 1.0);
+        b.ay = -10.0f;
+        b.shapes = {int(sc.shapes.size()) - 1};
+        sc.bodies.push_back(b);
+    }
+    for (int k = 0; k < falling; ++k) {
+        BodySpec b;
+        b.type = kDynamic;
+        // Drawn in sequence: argument evaluation order differs between compilers.
+        b.x = rng.uniform(-28.0, 28.0);
+        b.y = rng.uniform(14.0, 40.0);
+        b.angle = rng.uniform(-0.5, 0.5);
+        b.ay = -10.0f;
+        const float friction = rng.uniform(0.2, 0.8);
+        if (rng.below(2) == 0) {
+            const float hx = rng.uniform(0.2, 0.6);
+            const float hy = rng.uniform(0.2, 0.6);
+            sc.shapes.push_back(Box(hx, hy, 1.0f, friction, 0.0f));
+        } else {
+            sc.shapes.push_back(Disk(rng.uniform(0.2, 0.6), 1.0f, friction, 0.0f));
+        }
+        b.shapes = {int(sc.shapes.size()) - 1};
+        sc.bodies.push_back(b);
+    }
+    return sc;
+}
+
 // Named scenes with integer parameters, as used by tests and benchmarks.
 //   "hello", "pyramid"(rows), "drop"(n, half, height, seed),
 //   "pile"(seed, w, nmin, nmax), "billiards"(seed, w),
 //   "chain"(seed, w, links, falling, torque), "joints"(seed, w, n, falling),
-//   "sliders"(seed, w, n, falling), "targets"(seed, w, n, falling)
+//   "sliders"(seed, w, n, falling), "targets"(seed, w, n, falling),
+//   "terrain"(seed, w, falling)
 inline bool Make(const std::string& name, const int* p, int np, SceneSpec& out) {
     auto arg = [&](int i, int dflt) { return (i < np) ? p[i] : dflt; };
     if (name == "hello") {
@@ -770,6 +850,8 @@ inline bool Make(const std::string& name, const int* p, int np, SceneSpec& out) 
         out = Sliders(std::uint32_t(arg(0, 1)), arg(1, 0), arg(2, 6), arg(3, 24));
     } else if (name == "targets") {
         out = Targets(std::uint32_t(arg(0, 1)), arg(1, 0), arg(2, 8), arg(3, 24));
+    } else if (name == "terrain") {
+        out = Terrain(std::uint32_t(arg(0, 1)), arg(1, 0), arg(2, 40));
     } else {
         return false;
     }

@@ -10,9 +10,11 @@
 #include <playrho/LengthError.hpp>
 #include <playrho/OutOfRange.hpp>
 #include <playrho/WrongState.hpp>
+#include <playrho/d2/ChainShapeConf.hpp>
 #include <playrho/d2/DiskShapeConf.hpp>
 #include <playrho/d2/DistanceJointConf.hpp>
 #include <playrho/d2/EdgeShapeConf.hpp>
+#include <playrho/d2/MultiShapeConf.hpp>
 #include <playrho/d2/FrictionJointConf.hpp>
 #include <playrho/d2/MotorJointConf.hpp>
 #include <playrho/d2/PolygonShapeConf.hpp>
@@ -184,7 +186,8 @@ GpuWorld::GpuWorld() : GpuWorld(WorldConf{}) {}
 GpuWorld::GpuWorld(const WorldConf& conf) : m_vertexRadius(conf.vertexRadius) { m_world = NewWorld(m_vertexRadius); }
 
 GpuWorld::GpuWorld(const GpuWorld& o)
-    : m_vertexRadius(o.m_vertexRadius), m_shapes(o.m_shapes), m_bodies(o.m_bodies), m_joints(o.m_joints),
+    : m_vertexRadius(o.m_vertexRadius), m_shapes(o.m_shapes), m_shapeFirst(o.m_shapeFirst),
+      m_shapeCount(o.m_shapeCount), m_engineShape(o.m_engineShape), m_bodies(o.m_bodies), m_joints(o.m_joints),
       m_listeners(o.m_listeners), m_invDeltaTime(o.m_invDeltaTime), m_subStepping(o.m_subStepping) {
     m_world = NewWorld(m_vertexRadius);
     std::size_t len = 0;
@@ -197,7 +200,8 @@ GpuWorld::GpuWorld(const GpuWorld& o)
 
 GpuWorld::GpuWorld(GpuWorld&& o) noexcept
     : m_world(o.m_world), m_vertexRadius(o.m_vertexRadius), m_shapes(std::move(o.m_shapes)),
-      m_bodies(std::move(o.m_bodies)), m_joints(std::move(o.m_joints)), m_listeners(std::move(o.m_listeners)),
+      m_shapeFirst(std::move(o.m_shapeFirst)), m_shapeCount(std::move(o.m_shapeCount)),
+      m_engineShape(std::move(o.m_engineShape)), m_bodies(std::move(o.m_bodies)), m_joints(std::move(o.m_joints)), m_listeners(std::move(o.m_listeners)),
       m_invDeltaTime(o.m_invDeltaTime), m_subStepping(o.m_subStepping) {
     o.m_world = nullptr;
     InstallListeners();
@@ -218,6 +222,9 @@ GpuWorld& GpuWorld::operator=(GpuWorld&& o) noexcept {
         o.m_world = nullptr;
         m_vertexRadius = o.m_vertexRadius;
         m_shapes = std::move(o.m_shapes);
+        m_shapeFirst = std::move(o.m_shapeFirst);
+        m_shapeCount = std::move(o.m_shapeCount);
+        m_engineShape = std::move(o.m_engineShape);
         m_bodies = std::move(o.m_bodies);
         m_joints = std::move(o.m_joints);
         m_listeners = std::move(o.m_listeners);
@@ -302,6 +309,9 @@ void Clear(GpuWorld& g) noexcept {
         if (g.m_world) prx_world_destroy(g.m_world);
         g.m_world = w;
         g.m_shapes.clear();
+        g.m_shapeFirst.clear();
+        g.m_shapeCount.clear();
+        g.m_engineShape.clear();
         g.m_bodies.clear();
         g.m_joints.clear();
         g.InstallListeners();
@@ -408,7 +418,8 @@ const DynamicTree& GetTree(const GpuWorld& g) noexcept {
         for (std::size_t i = 0; i < n; ++i) {
             const AABB aabb{LengthInterval{Real(ab[4 * i]) * Meter, Real(ab[4 * i + 2]) * Meter},
                             LengthInterval{Real(ab[4 * i + 1]) * Meter, Real(ab[4 * i + 3]) * Meter}};
-            g.m_tree.CreateLeaf(aabb, Contactable{BodyID(bodies[i]), ShapeID(shapes[i]), 0u});
+            const auto es = g.m_engineShape.at(shapes[i]);
+            g.m_tree.CreateLeaf(aabb, Contactable{BodyID(bodies[i]), ShapeID(es.first), ChildCounter(es.second)});
         }
     } catch (...) {
         g.m_tree = DynamicTree{};
@@ -468,7 +479,11 @@ BodyID CreateBody(GpuWorld& g, const Body& body) {
     c.bullet = body.IsImpenetrable() ? 1u : 0u;
     c.enabled = body.IsEnabled() ? 1u : 0u;
     std::vector<std::uint32_t> shapes;
-    for (const ShapeID s : body.GetShapes()) shapes.push_back(to_underlying(s));
+    for (const ShapeID s : body.GetShapes()) {
+        const auto i = to_underlying(s);
+        if (i >= g.m_shapes.size()) throw OutOfRange<ShapeID>(s, "playrho_amd: no such shape");
+        for (std::uint32_t k = 0; k < g.m_shapeCount[i]; ++k) shapes.push_back(g.m_shapeFirst[i] + k);
+    }
     c.shapes = shapes.data();
     c.nshapes = std::uint32_t(shapes.size());
     std::uint32_t id = 0;
@@ -738,6 +753,17 @@ void Destroy(GpuWorld& g, JointID id) {
 
 ShapeCounter GetShapeRange(const GpuWorld& g) noexcept { return ShapeCounter(g.m_shapes.size()); }
 
+// The drop-in's record of a shape made of `count` engine shapes from `first` on.
+ShapeID AddShapeRecord(GpuWorld& g, const Shape& def, std::uint32_t first, std::uint32_t count) {
+    const auto id = std::uint32_t(g.m_shapes.size());
+    g.m_shapes.push_back(def);
+    g.m_shapeFirst.push_back(first);
+    g.m_shapeCount.push_back(count);
+    if (g.m_engineShape.size() < first + count) g.m_engineShape.resize(first + count);
+    for (std::uint32_t k = 0; k < count; ++k) g.m_engineShape[first + k] = {id, k};
+    return ShapeID(static_cast<ShapeID::underlying_type>(id));
+}
+
 ShapeID CreateShape(GpuWorld& g, const Shape& def) {
     prx_shape_conf c{};
     c.friction = F(Real(GetFriction(def)));
@@ -769,15 +795,40 @@ ShapeID CreateShape(GpuWorld& g, const Shape& def) {
             pts.push_back(F(Real(GetX(v) / Meter)));
             pts.push_back(F(Real(GetY(v) / Meter)));
         }
+    } else if (const auto* s = TypeCast<const ChainShapeConf>(&def)) {
+        c.kind = PRX_SHAPE_CHAIN;
+        c.vertex_radius = F(Real(Length{s->vertexRadius} / Meter));
+        for (std::size_t i = 0; i < s->GetVertexCount(); ++i) {
+            pts.push_back(F(Real(GetX(s->GetVertex(ChildCounter(i))) / Meter)));
+            pts.push_back(F(Real(GetY(s->GetVertex(ChildCounter(i))) / Meter)));
+        }
+    } else if (const auto* s = TypeCast<const MultiShapeConf>(&def)) {
+        // Each hull's vertices (its convex hull again is itself).
+        c.kind = PRX_SHAPE_MULTI;
+        std::vector<std::uint32_t> sizes;
+        for (const ConvexHull& ch : s->children) {
+            const auto dp = ch.GetDistanceProxy();
+            c.vertex_radius = F(Real(Length{ch.GetVertexRadius()} / Meter));
+            sizes.push_back(std::uint32_t(dp.GetVertexCount()));
+            for (VertexCounter i = 0; i < dp.GetVertexCount(); ++i) {
+                pts.push_back(F(Real(GetX(dp.GetVertex(i)) / Meter)));
+                pts.push_back(F(Real(GetY(dp.GetVertex(i)) / Meter)));
+            }
+        }
+        c.points = pts.empty() ? nullptr : pts.data();
+        c.npoints = std::uint32_t(pts.size() / 2);
+        std::uint32_t first = 0, count = 0;
+        Check(prx_compound_shape_create(g.m_world, &c, sizes.data(), std::uint32_t(sizes.size()), &first, &count));
+        return AddShapeRecord(g, def, first, count);
     } else {
-        Unsupported("this shape type (chain and multi shapes)");
+        Unsupported("this shape type");
     }
     c.points = pts.empty() ? nullptr : pts.data();
     c.npoints = std::uint32_t(pts.size() / 2);
-    std::uint32_t id = 0;
-    Check(prx_shape_create(g.m_world, &c, &id));
-    g.m_shapes.push_back(def);
-    return ShapeID(static_cast<ShapeID::underlying_type>(id));
+    std::uint32_t first = 0, count = 1;
+    if (c.kind == PRX_SHAPE_CHAIN) Check(prx_compound_shape_create(g.m_world, &c, nullptr, 0, &first, &count));
+    else Check(prx_shape_create(g.m_world, &c, &first));
+    return AddShapeRecord(g, def, first, count);
 }
 
 Shape GetShape(const GpuWorld& g, ShapeID id) {
@@ -790,11 +841,13 @@ void SetShape(GpuWorld& g, ShapeID id, const Shape& def) {
     Shape cur = GetShape(g, id);
     const auto i = to_underlying(id);
     if (GetFriction(def) != GetFriction(cur)) {
-        Check(prx_shape_set_friction(g.m_world, i, F(Real(GetFriction(def)))));
+        for (std::uint32_t k = 0; k < g.m_shapeCount[i]; ++k)
+            Check(prx_shape_set_friction(g.m_world, g.m_shapeFirst[i] + k, F(Real(GetFriction(def)))));
         SetFriction(cur, GetFriction(def));
     }
     if (GetRestitution(def) != GetRestitution(cur)) {
-        Check(prx_shape_set_restitution(g.m_world, i, F(Real(GetRestitution(def)))));
+        for (std::uint32_t k = 0; k < g.m_shapeCount[i]; ++k)
+            Check(prx_shape_set_restitution(g.m_world, g.m_shapeFirst[i] + k, F(Real(GetRestitution(def)))));
         SetRestitution(cur, GetRestitution(def));
     }
     if (!(cur == def)) Unsupported("changing a shape other than its friction and restitution");
@@ -826,10 +879,11 @@ std::vector<KeyedContactID> GetContacts(const GpuWorld& g) {
 Contact GetContact(const GpuWorld& g, ContactID id) {
     const auto cs = ReadContacts(g);
     const prx_contact_info& c = FindContact(cs, id);
+    const auto ea = g.m_engineShape.at(c.shape_a), eb = g.m_engineShape.at(c.shape_b);
     Contact out{Contactable{BodyID(static_cast<BodyID::underlying_type>(c.body_a)),
-                            ShapeID(static_cast<ShapeID::underlying_type>(c.shape_a)), 0u},
+                            ShapeID(static_cast<ShapeID::underlying_type>(ea.first)), ChildCounter(ea.second)},
                 Contactable{BodyID(static_cast<BodyID::underlying_type>(c.body_b)),
-                            ShapeID(static_cast<ShapeID::underlying_type>(c.shape_b)), 0u}};
+                            ShapeID(static_cast<ShapeID::underlying_type>(eb.first)), ChildCounter(eb.second)}};
     out.SetFriction(NonNegative<Real>{Real(c.friction)});
     out.SetRestitution(Real(c.restitution));
     if (c.flags & 1u) out.SetTouching();

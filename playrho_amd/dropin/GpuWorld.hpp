@@ -72,6 +72,9 @@ public:
     prx_world* m_world = nullptr;
     playrho::Interval<playrho::Positive<playrho::Length>> m_vertexRadius;
     std::vector<playrho::d2::Shape> m_shapes;
+    // Engine shape ids: a chain or multi shape is one engine shape per child.
+    std::vector<std::uint32_t> m_shapeFirst, m_shapeCount;
+    std::vector<std::pair<std::uint32_t, std::uint32_t>> m_engineShape;  // engine id -> (ShapeID, child)
     std::vector<playrho::d2::Body> m_bodies;  // as created (type, flags, shapes, damping)
     std::vector<playrho::d2::Joint> m_joints;
     Listeners m_listeners;

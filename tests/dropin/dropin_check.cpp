@@ -23,6 +23,9 @@
 #include <string>
 #include <vector>
 
+#include <playrho/d2/VertexSet.hpp>
+#include <playrho/d2/MultiShapeConf.hpp>
+#include <playrho/d2/ChainShapeConf.hpp>
 #include <playrho/WrongState.hpp>
 #include <playrho/d2/BasicAPI.hpp>
 #include <playrho/d2/RevoluteJointConf.hpp>
@@ -167,6 +170,27 @@ int Cpu() {
     const auto jc = Joint{RevoluteJointConf{}.UseBodyA(BodyID(2)).UseBodyB(BodyID(5))};
     Expect(CreateJoint(ref2, jc) == CreateJoint(gpu2, jc), "joint id after destroying");
     Expect(GetJoints(ref2) == GetJoints(gpu2), "joints after creating");
+    // Chain and multi shapes: a ground line and an L-shaped dynamic body.
+    auto ref3 = World{};
+    auto gpu3 = World{playrho_amd::GpuWorld{}};
+    for (World* wp : {&ref3, &gpu3}) {
+        auto line = ChainShapeConf{};
+        line.Set({Length2{-8_m, 2_m}, Length2{-4_m, 0_m}, Length2{0_m, 1_m}, Length2{4_m, 0.5_m}, Length2{8_m, 3_m}});
+        const auto ground = CreateShape(*wp, line);
+        CreateBody(*wp, BodyConf{}.Use(ground));
+        auto ell = MultiShapeConf{};
+        VertexSet a, b;
+        for (const auto& p : {Length2{-1_m, -0.2_m}, Length2{1_m, -0.2_m}, Length2{1_m, 0.2_m}, Length2{-1_m, 0.2_m}}) a.add(p);
+        for (const auto& p : {Length2{-1_m, 0.2_m}, Length2{-0.6_m, 0.2_m}, Length2{-0.6_m, 1_m}, Length2{-1_m, 1_m}}) b.add(p);
+        ell.AddConvexHull(a).AddConvexHull(b);
+        ell.UseDensity(1_kgpm2);
+        const auto body = CreateShape(*wp, ell);
+        CreateBody(*wp, BodyConf{}.Use(BodyType::Dynamic).Use(body).UseLocation(Length2{0_m, 6_m}));
+    }
+    CompareBodies(ref3, gpu3, "chain and multi shapes");
+    Expect(GetShapeRange(ref3) == GetShapeRange(gpu3), "compound shape count");
+    for (auto s = 0u; s < GetShapeRange(ref3); ++s)
+        Expect(GetShape(ref3, ShapeID(s)) == GetShape(gpu3, ShapeID(s)), "compound shape " + std::to_string(s));
     std::printf("cpu: %s\n", g_fail ? "FAIL" : "OK");
     return g_fail ? 1 : 0;
 }

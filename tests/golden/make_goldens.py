@@ -54,6 +54,10 @@ CASES = [
     # Target joints (no body A): clamped and unclamped forces, frequency 0,
     # maximum force 0, beside falling bodies.
     ("targets0", "targets", 240, [1, 0], [1, 120, 240]),
+    # Chain shapes (a zig-zag ground line, a closed square outline) and bodies
+    # of multi shapes (two or three box hulls), beside falling bodies.
+    ("terrain0", "terrain", 200, [1, 0], [1, 100, 200]),
+    ("terrain1", "terrain", 200, [2, 1, 60], [1, 200]),
 ]
 # World edits between steps (step, kind, body, a, b, c): SetLocation 'L',
 # SetAngle 'A', SetTransformation 'T' (c the rotation's angle), ShiftOrigin 'O',
@@ -105,6 +109,7 @@ TOI_CASES = [
     ("pile859_toi", "pile", 300, [1, 859], [1, 300]),
     ("sliders0_toi", "sliders", 240, [1, 0], [1, 240]),
     ("targets0_toi", "targets", 240, [1, 0], [1, 240]),
+    ("terrain0_toi", "terrain", 200, [1, 0], [1, 200]),
 ]
 
 

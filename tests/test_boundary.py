@@ -1,6 +1,7 @@
 """The C-ABI library: it loads, exports every function include/playrho_amd.h
 declares, and builds worlds exactly as the reference does (host side only:
 no device calls)."""
+import ctypes
 import os
 import re
 import struct
@@ -355,3 +356,34 @@ def test_destroy_host_side():
     assert w.joint_count == nj
     with pytest.raises(pa.PrxError):
         w.create_body(pa.BodyConf())
+
+
+def test_compound_shape_creation_matches_scene():
+    """A chain and a multi shape made through prx_compound_shape_create give
+    the children (vertices, normals, material) the terrain scene's reference
+    description holds for its first chain and first multi shape."""
+    from tests import dumpfmt
+    g = golden_cases.load("terrain0")
+    ref_shapes, _, _ = dumpfmt.read_description(dumpfmt.Reader(g["desc_conf"]))
+    w = pa.World()
+    ref_w = pa.World.scene("terrain", [1, 0])
+    scene_shapes, _, _ = dumpfmt.read_description(dumpfmt.Reader(ref_w.save_desc()))
+    chain_pts = np.array([s["verts"][0] for s in scene_shapes[:16]] + [scene_shapes[15]["verts"][1]], np.float32)
+    flat = chain_pts.reshape(-1).astype(np.float32)
+    c = pa.ShapeConf(kind=4, vertex_radius=-1.0, friction=float(scene_shapes[0]["friction"]),
+                     points=flat.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), npoints=len(chain_pts),
+                     category_bits=1, mask_bits=0xffff)
+    ids = w.create_compound_shape(c)
+    assert ids == list(range(16))
+    mine, _, _ = dumpfmt.read_description(dumpfmt.Reader(_with_one_body(w, ids)))
+    for k in range(16):
+        for key in ("verts", "normals", "radius", "friction"):
+            assert np.array_equal(np.asarray(mine[k][key]), np.asarray(ref_shapes[k][key])), (k, key)
+    with pytest.raises(pa.PrxError):
+        w.create_compound_shape(pa.ShapeConf(kind=5, npoints=0))  # no hulls
+
+
+def _with_one_body(w, shape_ids):
+    arr = (ctypes.c_uint32 * len(shape_ids))(*shape_ids)
+    w.create_body(pa.BodyConf(type=0, enabled=1, shapes=arr, nshapes=len(shape_ids)))
+    return w.save_desc()
