@@ -119,7 +119,28 @@ void Load(Handle<R>& h, In& in) {
         j.motorSpeed = f[7];
         j.maxMotorTorque = f[8];
         j.kind = (fl >> 8) & 0xffu;
-        if (j.kind == 8) {
+        if (j.kind == 10) {
+            // Gear: per side (the first joint's A-C in f[0..5], the second's
+            // B-D in f[6..11]) the prismatic data (local anchors A, B, local
+            // x axis; bit 16 + side of the flags) or the revolute reference
+            // angle (first float), then bodies C, D, the ratio and the
+            // constant.
+            for (int k = 0; k < 2; ++k) {
+                const R* g = f + 6 * k;
+                j.gearPris[k] = (fl >> (16 + k)) & 1u;
+                j.gearLaA[k] = V2<R>{g[0], g[1]};
+                j.gearLaB[k] = V2<R>{g[2], g[3]};
+                j.gearAxis[k] = V2<R>{g[4], g[5]};
+                j.gearRef[k] = g[0];
+            }
+            j.bodyC = in.u32();
+            j.bodyD = in.u32();
+            j.ratio = R(in.f32());
+            j.constant = R(in.f32());
+            j.enableLimit = j.enableMotor = false;
+            j.laA = j.laB = V2<R>{R(0), R(0)};
+            j.referenceAngle = j.lower = j.upper = j.motorSpeed = j.maxMotorTorque = 0;
+        } else if (j.kind == 8) {
             // Prismatic: reference angle, lower and upper translation, motor
             // speed, maximum motor force (in maxMotorTorque), local x axis of A.
             j.localXAxis = V2<R>{f[9], f[10]};
@@ -427,6 +448,22 @@ void Widen(const Handle<S>& s, Handle<D>& d) {
         k.a1 = D(j.a1);
         k.a2 = D(j.a2);
         k.motorMass = D(j.motorMass);
+        k.bodyC = j.bodyC;
+        k.bodyD = j.bodyD;
+        for (int e = 0; e < 2; ++e) {
+            k.gearPris[e] = j.gearPris[e];
+            k.gearLaA[e] = Wd<D>(j.gearLaA[e]);
+            k.gearLaB[e] = Wd<D>(j.gearLaB[e]);
+            k.gearAxis[e] = Wd<D>(j.gearAxis[e]);
+            k.gearRef[e] = D(j.gearRef[e]);
+        }
+        k.constant = D(j.constant);
+        k.JvAC = Wd<D>(j.JvAC);
+        k.JvBD = Wd<D>(j.JvBD);
+        k.JwA = D(j.JwA);
+        k.JwB = D(j.JwB);
+        k.JwC = D(j.JwC);
+        k.JwD = D(j.JwD);
         k.destroyed = j.destroyed;
         b.joints.push_back(k);
     }

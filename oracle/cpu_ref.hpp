@@ -813,6 +813,19 @@ struct Joint {
     // anchor on B in laB, maxForce, frequency, dampingRatio.  Solver data: rB,
     // gamma, the effective mass in lm00..lm11, C in linearError; the impulse
     // in impulse.x, y.
+    // kind 10 gear: bodies A, B (in bodyA, bodyB) and C, D; per side (0: the
+    // first joint's A-C, 1: the second's B-D) whether it is prismatic, its
+    // local anchors A, B and local x axis (prismatic) or reference angle
+    // (revolute); the ratio in ratio and the constant.  Solver data: the
+    // Jacobians JvAC, JvBD, JwA..JwD, the mass in dmass, the impulse in
+    // impulse.x.
+    std::uint32_t bodyC = 0, bodyD = 0;
+    bool gearPris[2] = {false, false};
+    V2<R> gearLaA[2], gearLaB[2], gearAxis[2];
+    R gearRef[2] = {0, 0};
+    R constant = 0;
+    V2<R> JvAC, JvBD;
+    R JwA = 0, JwB = 0, JwC = 0, JwD = 0;
 };
 
 // InvalidBodyID's index (BodyID.hpp:50): a target joint's body A.
@@ -2575,6 +2588,196 @@ public:
         return inc.x == R(0) && inc.y == R(0);
     }
 
+    // InitVelocity(GearJointConf&) (GearJointConf.cpp:171-255): per side the
+    // Jacobian of its coordinate (a prismatic side the axis and the arms'
+    // crosses, a revolute one the unit angular terms, side B-D scaled by the
+    // ratio) and the summed inverse mass, then the warm start on all four.
+    static void GearInitVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        BodyConstraint<R>& C = bcs[j.bodyC];
+        BodyConstraint<R>& D = bcs[j.bodyD];
+        Vel<R> velA = A.vel, velB = B.vel, velC = C.vel, velD = D.vel;
+        R invMass = 0;
+        if (j.gearPris[0]) {
+            const V2<R> qA = UnitFromAngle(A.pos.ang);
+            const V2<R> qC = UnitFromAngle(C.pos.ang);
+            const V2<R> u = Rotate(j.gearAxis[0], qC);
+            const V2<R> rC = Rotate(j.gearLaA[0] - C.lc, qC);
+            const V2<R> rA = Rotate(j.gearLaB[0] - A.lc, qA);
+            j.JvAC = V2<R>{R(1) * u.x, R(1) * u.y};
+            j.JwC = Cross(rC, u);
+            j.JwA = Cross(rA, u);
+            const R invRotMassC = C.invRotI * (j.JwC * j.JwC) / R(1);
+            const R invRotMassA = A.invRotI * (j.JwA * j.JwA) / R(1);
+            invMass += C.invMass + A.invMass + invRotMassC + invRotMassA;
+        } else {
+            j.JvAC = V2<R>{R(0), R(0)};
+            j.JwA = R(1);
+            j.JwC = R(1);
+            invMass += A.invRotI + C.invRotI;
+        }
+        if (j.gearPris[1]) {
+            const V2<R> qB = UnitFromAngle(B.pos.ang);
+            const V2<R> qD = UnitFromAngle(D.pos.ang);
+            const V2<R> u = Rotate(j.gearAxis[1], qD);
+            const V2<R> rD = Rotate(j.gearLaA[1] - D.lc, qD);
+            const V2<R> rB = Rotate(j.gearLaB[1] - B.lc, qB);
+            j.JvBD = V2<R>{j.ratio * u.x, j.ratio * u.y};
+            j.JwD = j.ratio * Cross(rD, u);
+            j.JwB = j.ratio * Cross(rB, u);
+            const R invRotMassD = D.invRotI * (j.JwD * j.JwD) / R(1);
+            const R invRotMassB = B.invRotI * (j.JwB * j.JwB) / R(1);
+            invMass += (j.ratio * j.ratio) * (D.invMass + B.invMass) + invRotMassD + invRotMassB;
+        } else {
+            j.JvBD = V2<R>{R(0), R(0)};
+            j.JwB = j.ratio * R(1);
+            j.JwD = j.ratio * R(1);
+            invMass += (j.ratio * j.ratio) * (B.invRotI + D.invRotI);
+        }
+        j.dmass = (invMass > R(0)) ? R(1) / invMass : R(0);
+        if (conf.warmStart) {
+            const R imp = j.impulse.x;
+            velA.lin = velA.lin + (A.invMass * imp) * j.JvAC;
+            velA.ang = velA.ang + A.invRotI * imp * j.JwA / R(1);
+            velB.lin = velB.lin + (B.invMass * imp) * j.JvBD;
+            velB.ang = velB.ang + B.invRotI * imp * j.JwB / R(1);
+            velC.lin = velC.lin - (C.invMass * imp) * j.JvAC;
+            velC.ang = velC.ang - C.invRotI * imp * j.JwC / R(1);
+            velD.lin = velD.lin - (D.invMass * imp) * j.JvBD;
+            velD.ang = velD.ang - D.invRotI * imp * j.JwD / R(1);
+        } else {
+            j.impulse.x = R(0);
+        }
+        A.vel = velA;
+        B.vel = velB;
+        C.vel = velC;
+        D.vel = velD;
+    }
+
+    // SolveVelocity(GearJointConf&) (GearJointConf.cpp:257-296).
+    static bool GearSolveVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        BodyConstraint<R>& C = bcs[j.bodyC];
+        BodyConstraint<R>& D = bcs[j.bodyD];
+        Vel<R> velA = A.vel, velB = B.vel, velC = C.vel, velD = D.vel;
+        const R acDot = Dot(j.JvAC, velA.lin - velC.lin);
+        const R bdDot = Dot(j.JvBD, velB.lin - velD.lin);
+        const R Cdot = acDot + bdDot + (j.JwA * velA.ang - j.JwC * velC.ang) / R(1) +
+                       (j.JwB * velB.ang - j.JwD * velD.ang) / R(1);
+        const R impulse = -j.dmass * R(1) * Cdot;
+        j.impulse.x += impulse;
+        velA.lin = velA.lin + (A.invMass * impulse) * j.JvAC;
+        velA.ang = velA.ang + A.invRotI * impulse * j.JwA / R(1);
+        velB.lin = velB.lin + (B.invMass * impulse) * j.JvBD;
+        velB.ang = velB.ang + B.invRotI * impulse * j.JwB / R(1);
+        velC.lin = velC.lin - (C.invMass * impulse) * j.JvAC;
+        velC.ang = velC.ang - C.invRotI * impulse * j.JwC / R(1);
+        velD.lin = velD.lin - (D.invMass * impulse) * j.JvBD;
+        velD.ang = velD.ang - D.invRotI * impulse * j.JwD / R(1);
+        A.vel = velA;
+        B.vel = velB;
+        C.vel = velC;
+        D.vel = velD;
+        return impulse == R(0);
+    }
+
+    // Cap(Position, ConstraintSolverConf) (d2/Math.cpp:63-71).
+    static Pos<R> CapPosition(Pos<R> p, const StepConf<R>& conf) {
+        const R lsquared = MagSq(p.lin);
+        if (lsquared > conf.maxLinearCorrection * conf.maxLinearCorrection) {
+            const R s = conf.maxLinearCorrection / std::sqrt(lsquared);
+            p.lin = V2<R>{p.lin.x * s, p.lin.y * s};
+        }
+        p.ang = Clamp(p.ang, -conf.maxAngularCorrection, +conf.maxAngularCorrection);
+        return p;
+    }
+
+    // SolvePosition(const GearJointConf&) (GearJointConf.cpp:298-471): the
+    // coordinates from the positions now (a revolute side the angle less its
+    // reference angle, a prismatic one the anchor offset along the axis), one
+    // capped correction of the position error on all four bodies.
+    static bool GearSolvePosition(const Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
+        BodyConstraint<R>& A = bcs[j.bodyA];
+        BodyConstraint<R>& B = bcs[j.bodyB];
+        BodyConstraint<R>& C = bcs[j.bodyC];
+        BodyConstraint<R>& D = bcs[j.bodyD];
+        const Pos<R> posA = A.pos, posB = B.pos, posC = C.pos, posD = D.pos;
+        if (!std::isfinite(j.constant)) return true;
+        V2<R> JvAC{R(0), R(0)}, JvBD{R(0), R(0)};
+        R JwA = 0, JwB = 0, JwC = 0, JwD = 0;
+        R coordinateA = 0, coordinateB = 0, invMass = 0;
+        if (j.gearPris[0]) {
+            const V2<R> qA = UnitFromAngle(posA.ang);
+            const V2<R> qC = UnitFromAngle(posC.ang);
+            const V2<R> u = Rotate(j.gearAxis[0], qC);
+            const V2<R> rC = Rotate(j.gearLaA[0] - C.lc, qC);
+            const V2<R> rA = Rotate(j.gearLaB[0] - A.lc, qA);
+            JvAC = V2<R>{u.x * R(1), u.y * R(1)};
+            JwC = Cross(rC, u);
+            JwA = Cross(rA, u);
+            const R invLinMass = C.invMass + A.invMass;
+            const R invRotMassC = C.invRotI * ((JwC * R(1) / R(1)) * (JwC * R(1) / R(1)));
+            const R invRotMassA = A.invRotI * ((JwA * R(1) / R(1)) * (JwA * R(1) / R(1)));
+            invMass += invLinMass + invRotMassC + invRotMassA;
+            const V2<R> pC = j.gearLaA[0] - C.lc;
+            const V2<R> pA = InvRotate(rA + (posA.lin - posC.lin), qC);
+            coordinateA = Dot(pA - pC, j.gearAxis[0]) / R(1);
+        } else {
+            JwA = R(1);
+            JwC = R(1);
+            invMass += A.invRotI + C.invRotI;
+            coordinateA = ((posA.ang - posC.ang) - j.gearRef[0]) / R(1);
+        }
+        if (j.gearPris[1]) {
+            const V2<R> qB = UnitFromAngle(posB.ang);
+            const V2<R> qD = UnitFromAngle(posD.ang);
+            const V2<R> u = Rotate(j.gearAxis[1], qD);
+            const V2<R> rD = Rotate(j.gearLaA[1] - D.lc, qD);
+            const V2<R> rB = Rotate(j.gearLaB[1] - B.lc, qB);
+            JvBD = V2<R>{j.ratio * u.x, j.ratio * u.y};
+            JwD = j.ratio * Cross(rD, u);
+            JwB = j.ratio * Cross(rB, u);
+            const R invLinMass = (j.ratio * j.ratio) * (D.invMass + B.invMass);
+            const R invRotMassD = D.invRotI * ((JwD * R(1) / R(1)) * (JwD * R(1) / R(1)));
+            const R invRotMassB = B.invRotI * ((JwB * R(1) / R(1)) * (JwB * R(1) / R(1)));
+            invMass += invLinMass + invRotMassD + invRotMassB;
+            const V2<R> pD = j.gearLaA[1] - D.lc;
+            const V2<R> pB = InvRotate(rB + (posB.lin - posD.lin), qD);
+            coordinateB = Dot(pB - pD, j.gearAxis[1]) / R(1);
+        } else {
+            JwB = j.ratio;
+            JwD = j.ratio;
+            invMass += (j.ratio * j.ratio) * (B.invRotI + D.invRotI);
+            coordinateB = ((posB.ang - posD.ang) - j.gearRef[1]) / R(1);
+        }
+        const R Cv = ((coordinateA + j.ratio * coordinateB) - j.constant) * conf.regResolutionRate;
+        const R impulse = ((invMass > R(0)) ? -Cv / invMass : R(0)) * R(1) * R(1);
+        auto delta = [&](const BodyConstraint<R>& b, V2<R> Jv, R Jw) {
+            const R lm = b.invMass * impulse;
+            return CapPosition(Pos<R>{V2<R>{lm * Jv.x, lm * Jv.y}, b.invRotI * impulse * Jw * R(1) / R(1)}, conf);
+        };
+        const Pos<R> dA = delta(A, JvAC, JwA);
+        const Pos<R> dC = delta(C, JvAC, JwC);
+        const Pos<R> dB = delta(B, JvBD, JwB);
+        const Pos<R> dD = delta(D, JvBD, JwD);
+        A.pos = Pos<R>{posA.lin + dA.lin, posA.ang + dA.ang};
+        B.pos = Pos<R>{posB.lin + dB.lin, posB.ang + dB.ang};
+        C.pos = Pos<R>{posC.lin - dC.lin, posC.ang - dC.ang};
+        D.pos = Pos<R>{posD.lin - dD.lin, posD.ang - dD.ang};
+        R linearError = 0, angularError = 0;
+        linearError = std::max(linearError, MagSq(dA.lin));
+        linearError = std::max(linearError, MagSq(dB.lin));
+        linearError = std::max(linearError, MagSq(dC.lin));
+        linearError = std::max(linearError, MagSq(dD.lin));
+        angularError = std::max(angularError, std::abs(dA.ang));
+        angularError = std::max(angularError, std::abs(dB.ang));
+        angularError = std::max(angularError, std::abs(dC.ang));
+        angularError = std::max(angularError, std::abs(dD.ang));
+        return (linearError <= conf.linearSlop * conf.linearSlop) && (angularError <= conf.angularSlop);
+    }
+
     // InitVelocity(RevoluteJointConf&) (RevoluteJointConf.cpp:127-226).
     static void JointInitVelocity(Joint<R>& j, std::vector<BodyConstraint<R>>& bcs, const StepConf<R>& conf) {
         if (j.kind == 1) return DistanceInitVelocity(j, bcs, conf);
@@ -2586,6 +2789,7 @@ public:
         if (j.kind == 7) return PulleyInitVelocity(j, bcs, conf);
         if (j.kind == 8) return PrismaticInitVelocity(j, bcs, conf);
         if (j.kind == 9) return TargetInitVelocity(j, bcs, conf);
+        if (j.kind == 10) return GearInitVelocity(j, bcs, conf);
         BodyConstraint<R>& A = bcs[j.bodyA];
         BodyConstraint<R>& B = bcs[j.bodyB];
         const R mA = A.invMass, iA = A.invRotI, aA = A.pos.ang;
@@ -2653,6 +2857,7 @@ public:
         if (j.kind == 7) return PulleySolveVelocity(j, bcs);
         if (j.kind == 8) return PrismaticSolveVelocity(j, bcs, conf);
         if (j.kind == 9) return TargetSolveVelocity(j, bcs, conf);
+        if (j.kind == 10) return GearSolveVelocity(j, bcs);
         BodyConstraint<R>& A = bcs[j.bodyA];
         BodyConstraint<R>& B = bcs[j.bodyB];
         const Vel<R> oldVelA = A.vel, oldVelB = B.vel;
@@ -2733,6 +2938,7 @@ public:
         if (j.kind == 7) return PulleySolvePosition(j, bcs, conf);
         if (j.kind == 8) return PrismaticSolvePosition(j, bcs, conf);
         if (j.kind == 9) return true;  // TargetJointConf.cpp:180-184
+        if (j.kind == 10) return GearSolvePosition(j, bcs, conf);
         BodyConstraint<R>& A = bcs[j.bodyA];
         BodyConstraint<R>& B = bcs[j.bodyB];
         Pos<R> posA = A.pos, posB = B.pos;

@@ -14,9 +14,12 @@
 //   nbodies, per body:  type, flags, f32[19] (pos0 3, pos1 3, lc 2, vel 3,
 //                       linAcc 2, angAcc, linDamp, angDamp, invMass, invRotI,
 //                       underActive), nshape, shapeIds[]
-//   njoints, per joint: bodyA, bodyB, flags (limit 1, motor 2, collideConnected 4),
-//                       f32[9] (laA 2, laB 2, referenceAngle, lower, upper,
-//                       motorSpeed, maxMotorTorque)
+//   njoints, per joint: bodyA, bodyB, flags (limit 1, motor 2, collideConnected 4,
+//                       kind << 8), f32[12] (revolute: laA 2, laB 2,
+//                       referenceAngle, lower, upper, motorSpeed,
+//                       maxMotorTorque; other kinds see WriteDescription);
+//                       gear joints (kind 10) then u32 bodyC, bodyD, f32
+//                       ratio, constant
 //   conf: f32/u32 StepConf fields used by the step (see WriteConf)
 //   nrecords, per record: step, stats u32[8],
 //                         B, per body f32[10] (pos0, pos1, vel, underActive), flags
@@ -49,6 +52,7 @@
 #include <playrho/d2/PulleyJointConf.hpp>
 #include <playrho/d2/PrismaticJointConf.hpp>
 #include <playrho/d2/TargetJointConf.hpp>
+#include <playrho/d2/GearJointConf.hpp>
 #include <playrho/d2/WorldJoint.hpp>
 #include <playrho/StepConf.hpp>
 
@@ -292,6 +296,34 @@ void WriteDescription(Out& o, const World& world) {
                                  GetY(j->localAnchorB), j->frequency, j->dampingRatio, 0.0f, j->motorSpeed,
                                  j->maxMotorTorque, GetX(j->localXAxisA), GetY(j->localXAxisA), 0.0f};
             o.put(f, sizeof(f));
+        } else if (const auto* j = TypeCast<const GearJointConf>(&joint)) {
+            // Gear: per side (A-C, then B-D) the prismatic data or the
+            // revolute reference angle, bit 16 + side set for prismatic;
+            // then bodies C, D, ratio, constant.
+            float f[12] = {};
+            std::uint32_t pris = 0;
+            const GearJointConf::TypeData* sides[2] = {&j->typeDataAC, &j->typeDataBD};
+            for (int k = 0; k < 2; ++k) {
+                if (const auto* pd = std::get_if<GearJointConf::PrismaticData>(sides[k])) {
+                    pris |= 1u << (16 + k);
+                    f[6 * k + 0] = GetX(pd->localAnchorA);
+                    f[6 * k + 1] = GetY(pd->localAnchorA);
+                    f[6 * k + 2] = GetX(pd->localAnchorB);
+                    f[6 * k + 3] = GetY(pd->localAnchorB);
+                    f[6 * k + 4] = GetX(pd->localAxis);
+                    f[6 * k + 5] = GetY(pd->localAxis);
+                } else if (const auto* rd = std::get_if<GearJointConf::RevoluteData>(sides[k])) {
+                    f[6 * k] = rd->referenceAngle;
+                }
+            }
+            o.u32(to_underlying(GetBodyA(*j)));
+            o.u32(to_underlying(GetBodyB(*j)));
+            o.u32((j->collideConnected ? 4u : 0u) | (10u << 8) | pris);
+            o.put(f, sizeof(f));
+            o.u32(to_underlying(j->bodyC));
+            o.u32(to_underlying(j->bodyD));
+            const float g[2] = {j->ratio, j->constant};
+            o.put(g, sizeof(g));
         } else if (const auto* j = TypeCast<const MotorJointConf>(&joint)) {
             o.u32(to_underlying(GetBodyA(*j)));
             o.u32(to_underlying(GetBodyB(*j)));
@@ -532,6 +564,10 @@ void WriteState(Out& o, const World& world, std::uint32_t step, const StepStats&
             o.u32(0u);
             const float f[4] = {j->impulse, j->springImpulse, j->angularImpulse, 0.0f};
             o.put(f, sizeof(f));
+        } else if (const auto* j = TypeCast<const GearJointConf>(&joint)) {
+            o.u32(0u);
+            const float f[4] = {j->impulse, 0.0f, 0.0f, 0.0f};
+            o.put(f, sizeof(f));
         } else if (const auto* j = TypeCast<const MotorJointConf>(&joint)) {
             o.u32(0u);
             const float f[4] = {GetX(j->linearImpulse), GetY(j->linearImpulse), j->angularImpulse, 0.0f};
@@ -572,7 +608,13 @@ World BuildWorld(const prx_scene::SceneSpec& spec) {
     for (const auto& js : spec.joints) {
         const auto ba = BodyID(static_cast<BodyID::underlying_type>(js.a));
         const auto bb = BodyID(static_cast<BodyID::underlying_type>(js.b));
-        if (js.kind == prx_scene::kDistance) {
+        if (js.kind == prx_scene::kGear) {
+            // a, b: the revolute or prismatic joints the gear couples.
+            auto jc = GetGearJointConf(world, JointID(static_cast<JointID::underlying_type>(js.a)),
+                                       JointID(static_cast<JointID::underlying_type>(js.b)), js.ratio);
+            jc.UseCollideConnected(js.collideConnected);
+            CreateJoint(world, jc);
+        } else if (js.kind == prx_scene::kDistance) {
             auto jc = DistanceJointConf{ba, bb, Length2{js.lax, js.lay}, Length2{js.lbx, js.lby}, js.length};
             jc.UseFrequency(js.frequency).UseDampingRatio(js.dampingRatio);
             jc.UseCollideConnected(js.collideConnected);

@@ -62,10 +62,11 @@ struct BodySpec {
 // referenceAngle, enableLimit, lower, upper, enableMotor, motorSpeed,
 // maxForce as the maximum motor force), 9 TargetJointConf (body b only; the
 // target gax, gay, the anchor on B lbx, lby, maxForce, frequency,
-// dampingRatio).
+// dampingRatio), 10 GearJointConf (a, b: indices of two earlier revolute or
+// prismatic joints, made by GetGearJointConf with `ratio`).
 enum JointKind : int {
     kRevolute = 0, kDistance = 1, kWeld = 2, kRope = 3, kFriction = 4, kMotor = 5, kWheel = 6, kPulley = 7,
-    kPrismatic = 8, kTarget = 9
+    kPrismatic = 8, kTarget = 9, kGear = 10
 };
 struct JointSpec {
     int kind = kRevolute;
@@ -824,12 +825,136 @@ inline SceneSpec Terrain(std::uint32_t seed, int w, int falling = 40) {
     return sc;
 }
 
+// Gear joints: a train of three seeded wheels on revolute joints to the
+// ground (seeded angles and reference angles, the first motorised) meshed by
+// gears of seeded ratios, the last wheel driving a rack on a horizontal
+// prismatic joint (revolute-prismatic gear), two slabs on prismatic rails
+// coupled to each other (prismatic-prismatic gear, one rail limited), a
+// pinion on a revolute joint to a carriage that slides on a limited rail
+// geared to the pinion (the gear's bodies B and C are the same carriage), and
+// `falling` seeded boxes and disks dropped onto them.
+inline SceneSpec Gears(std::uint32_t seed, int w, int falling = 24) {
+    SceneSpec sc;
+    Rng rng(seed * 2654435761u + std::uint32_t(w) * 40503u + 41u);
+    sc.shapes.push_back(Edge(-50.0f, 0.0f, 50.0f, 0.0f));
+    BodySpec ground;
+    ground.shapes = {0};
+    sc.bodies.push_back(ground);
+    auto dynamic = [&](float x, float y, float angle, int shape) {
+        BodySpec b;
+        b.type = kDynamic;
+        b.x = x;
+        b.y = y;
+        b.angle = angle;
+        b.ay = -10.0f;
+        b.shapes = {shape};
+        sc.bodies.push_back(b);
+        return int(sc.bodies.size()) - 1;
+    };
+    auto revolute = [&](int a, int b, float ax, float ay, float ref) {
+        JointSpec j;
+        j.kind = kRevolute;
+        j.a = a;
+        j.b = b;
+        j.lax = ax;
+        j.lay = ay;
+        j.referenceAngle = ref;
+        sc.joints.push_back(j);
+        return int(sc.joints.size()) - 1;
+    };
+    auto prismatic = [&](int a, int b, float ax, float ay, float axisAngle) {
+        JointSpec j;
+        j.kind = kPrismatic;
+        j.a = a;
+        j.b = b;
+        j.lax = ax;
+        j.lay = ay;
+        j.axisAngle = axisAngle;
+        sc.joints.push_back(j);
+        return int(sc.joints.size()) - 1;
+    };
+    auto gear = [&](int j1, int j2, float ratio) {
+        JointSpec j;
+        j.kind = kGear;
+        j.a = j1;
+        j.b = j2;
+        j.ratio = ratio;
+        sc.joints.push_back(j);
+    };
+    // Wheel train.
+    float radius[3];
+    int wheelJoint[3];
+    for (int i = 0; i < 3; ++i) {
+        radius[i] = rng.uniform(0.8, 2.0);
+        sc.shapes.push_back(Disk(radius[i], 1.0f, 0.4f, 0.0f));
+        const float x = -8.0f + 5.0f * float(i);
+        const float angle = rng.uniform(-1.0, 1.0);
+        const int b = dynamic(x, 8.0f, angle, int(sc.shapes.size()) - 1);
+        wheelJoint[i] = revolute(0, b, x, 8.0f, angle + rng.uniform(-0.5, 0.5));
+        if (i == 0) {
+            sc.joints.back().enableMotor = true;
+            sc.joints.back().motorSpeed = rng.uniform(-2.0, 2.0);
+            sc.joints.back().maxMotorTorque = rng.uniform(50.0, 400.0);
+        }
+        if (i > 0) gear(wheelJoint[i - 1], wheelJoint[i], radius[i - 1] / radius[i]);
+    }
+    // Rack under the last wheel.
+    sc.shapes.push_back(Box(3.0f, 0.25f, 1.0f, 0.5f, 0.0f));
+    const int rack = dynamic(2.0f, 8.0f - radius[2] - 0.3f, 0.0f, int(sc.shapes.size()) - 1);
+    const int rackJoint = prismatic(0, rack, 2.0f, 8.0f - radius[2] - 0.3f, 0.0f);
+    gear(wheelJoint[2], rackJoint, -1.0f / radius[2]);
+    // Two slabs on coupled rails.
+    sc.shapes.push_back(Box(0.7f, 0.3f, 1.0f, 0.5f, 0.0f));
+    const int slabShape = int(sc.shapes.size()) - 1;
+    const int s1 = dynamic(14.0f, 5.0f, 0.0f, slabShape);
+    const int p1 = prismatic(0, s1, 14.0f, 5.0f, 1.5707963f);
+    sc.joints.back().enableLimit = true;
+    sc.joints.back().lower = -2.0f;
+    sc.joints.back().upper = 2.0f;
+    const float a2 = rng.uniform(0.3, 2.8);
+    const int s2 = dynamic(20.0f, 6.0f, rng.uniform(-0.3, 0.3), slabShape);
+    const int p2 = prismatic(0, s2, 20.0f, 6.0f, a2);
+    gear(p1, p2, rng.uniform(0.5, 2.0));
+    // Pinion on a carriage.
+    sc.shapes.push_back(Box(1.5f, 0.3f, 1.0f, 0.6f, 0.0f));
+    const int carriage = dynamic(-20.0f, 3.0f, 0.0f, int(sc.shapes.size()) - 1);
+    const int railJoint = prismatic(0, carriage, -20.0f, 3.0f, 0.0f);
+    sc.joints.back().enableLimit = true;
+    sc.joints.back().lower = -5.0f;
+    sc.joints.back().upper = 5.0f;
+    sc.shapes.push_back(Disk(0.6f, 1.0f, 0.5f, 0.0f));
+    const int pinion = dynamic(-20.0f, 4.2f, 0.0f, int(sc.shapes.size()) - 1);
+    JointSpec pj;
+    pj.kind = kRevolute;
+    pj.a = carriage;
+    pj.b = pinion;
+    pj.lax = 0.0f;
+    pj.lay = 1.2f;
+    sc.joints.push_back(pj);
+    const int pinionJoint = int(sc.joints.size()) - 1;
+    gear(pinionJoint, railJoint, rng.uniform(0.5, 1.5));
+    for (int k = 0; k < falling; ++k) {
+        const float x = rng.uniform(-25.0, 25.0);
+        const float y = rng.uniform(12.0, 40.0);
+        const float friction = rng.uniform(0.2, 0.8);
+        if (rng.below(2) == 0) {
+            const float hx = rng.uniform(0.3, 0.6);
+            const float hy = rng.uniform(0.3, 0.6);
+            sc.shapes.push_back(Box(hx, hy, 1.0f, friction, 0.0f));
+        } else {
+            sc.shapes.push_back(Disk(rng.uniform(0.3, 0.6), 1.0f, friction, 0.0f));
+        }
+        dynamic(x, y, 0.0f, int(sc.shapes.size()) - 1);
+    }
+    return sc;
+}
+
 // Named scenes with integer parameters, as used by tests and benchmarks.
 //   "hello", "pyramid"(rows), "drop"(n, half, height, seed),
 //   "pile"(seed, w, nmin, nmax), "billiards"(seed, w),
 //   "chain"(seed, w, links, falling, torque), "joints"(seed, w, n, falling),
 //   "sliders"(seed, w, n, falling), "targets"(seed, w, n, falling),
-//   "terrain"(seed, w, falling)
+//   "terrain"(seed, w, falling), "gears"(seed, w, falling)
 inline bool Make(const std::string& name, const int* p, int np, SceneSpec& out) {
     auto arg = [&](int i, int dflt) { return (i < np) ? p[i] : dflt; };
     if (name == "hello") {
@@ -852,6 +977,8 @@ inline bool Make(const std::string& name, const int* p, int np, SceneSpec& out) 
         out = Targets(std::uint32_t(arg(0, 1)), arg(1, 0), arg(2, 8), arg(3, 24));
     } else if (name == "terrain") {
         out = Terrain(std::uint32_t(arg(0, 1)), arg(1, 0), arg(2, 40));
+    } else if (name == "gears") {
+        out = Gears(std::uint32_t(arg(0, 1)), arg(1, 0), arg(2, 24));
     } else {
         return false;
     }

@@ -18,6 +18,7 @@ CONTACT_U = 4
 CONTACT_T = 3
 CONTACT_F = 14
 JOINT_DESC_F = 12
+GEAR_KIND = 10
 JOINT_F = 4
 # Listener kinds: 0 begin, 1 end, 2 pre-solve (old manifold: type | count << 8,
 # two contact features, 12 floats), 3 post-solve (count, solved position
@@ -76,7 +77,12 @@ def read_description(r):
     for _ in range(r.u32()):
         ab = r.arr("<u4", 3)
         vals = r.arr("<f4", JOINT_DESC_F)
-        joints.append(dict(a=int(ab[0]), b=int(ab[1]), flags=int(ab[2]), vals=vals))
+        j = dict(a=int(ab[0]), b=int(ab[1]), flags=int(ab[2]), vals=vals)
+        if (int(ab[2]) >> 8) & 0xff == GEAR_KIND:
+            # Gear joints: bodies C, D, then ratio and constant.
+            cd = r.arr("<u4", 2)
+            j.update(c=int(cd[0]), d=int(cd[1]), extra=r.arr("<f4", 2))
+        joints.append(j)
     return shapes, bodies, joints
 
 

@@ -58,6 +58,12 @@ CASES = [
     # of multi shapes (two or three box hulls), beside falling bodies.
     ("terrain0", "terrain", 200, [1, 0], [1, 100, 200]),
     ("terrain1", "terrain", 200, [2, 1, 60], [1, 200]),
+    # Gear joints: revolute-revolute (a motorised wheel train), revolute-
+    # prismatic (a rack), prismatic-prismatic (one rail limited) and a pinion
+    # geared to its own carriage's rail (bodies B and C the same), beside
+    # falling bodies.
+    ("gears0", "gears", 240, [1, 0], [1, 120, 240]),
+    ("gears1", "gears", 240, [2, 1, 40], [1, 240]),
 ]
 # World edits between steps (step, kind, body, a, b, c): SetLocation 'L',
 # SetAngle 'A', SetTransformation 'T' (c the rotation's angle), ShiftOrigin 'O',
@@ -110,6 +116,7 @@ TOI_CASES = [
     ("sliders0_toi", "sliders", 240, [1, 0], [1, 240]),
     ("targets0_toi", "targets", 240, [1, 0], [1, 240]),
     ("terrain0_toi", "terrain", 200, [1, 0], [1, 200]),
+    ("gears0_toi", "gears", 240, [1, 0], [1, 240]),
 ]
 
 
