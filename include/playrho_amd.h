@@ -205,6 +205,22 @@ typedef struct prx_target_joint_conf {
     uint32_t collide_connected;
 } prx_target_joint_conf;
 
+/* GearJointConf (d2/GearJointConf.hpp:60-126): keeps coordinate1 + ratio *
+ * coordinate2 equal to `constant`, the coordinates being those of two
+ * revolute (angle) or prismatic (translation) joints.  Bodies A and B are the
+ * two joints' second bodies, C and D their first bodies.  Per side (A-C from
+ * the first joint in ac[], B-D from the second in bd[]): a prismatic side
+ * (prismatic_ac / prismatic_bd set) holds the joint's local anchors A, B and
+ * local x axis in [0..5]; a revolute one its reference angle in [0].  The
+ * reference's defaults are ratio 1, constant 0. */
+typedef struct prx_gear_joint_conf {
+    uint32_t body_a, body_b, body_c, body_d;
+    uint32_t prismatic_ac, prismatic_bd;
+    float ac[6], bd[6];
+    float ratio, constant;
+    uint32_t collide_connected;
+} prx_gear_joint_conf;
+
 /* StepConf fields the regular step reads (StepConf.hpp:49-392). */
 typedef struct prx_step_conf {
     float delta_time, dt_ratio, min_still_time_to_sleep, linear_slop, angular_slop;
@@ -309,6 +325,14 @@ int prx_pulley_joint_create(prx_world* world, const prx_pulley_joint_conf* conf,
 int prx_target_joint_create(prx_world* world, const prx_target_joint_conf* conf, uint32_t* joint_id);
 /* CreateJoint(World&, const PrismaticJointConf&) (solver PrismaticJointConf.cpp:155-531). */
 int prx_prismatic_joint_create(prx_world* world, const prx_prismatic_joint_conf* conf, uint32_t* joint_id);
+/* GetGearJointConf(const World&, JointID, JointID, Real) (GearJointConf.cpp:89-169):
+ * the gear conf for two existing joints at the bodies' present positions (the
+ * world's state is brought back from the device first when it is stale);
+ * PRX_E_INVALID_ARGUMENT unless both are revolute or prismatic. */
+int prx_gear_joint_conf_get(prx_world* world, uint32_t joint1, uint32_t joint2, float ratio,
+                            prx_gear_joint_conf* conf);
+/* CreateJoint(World&, const GearJointConf&) (solver GearJointConf.cpp:171-471). */
+int prx_gear_joint_create(prx_world* world, const prx_gear_joint_conf* conf, uint32_t* joint_id);
 uint32_t prx_world_joint_count(const prx_world* world);
 /* Builds a named workload scene through the calls above. */
 int prx_scene_build(prx_world* world, const char* name, const int32_t* params, int nparams);

@@ -249,6 +249,23 @@ class PrismaticJointConf(ctypes.Structure):
         super().__init__(**kw)
 
 
+class GearJointConf(ctypes.Structure):
+    """GearJointConf (d2/GearJointConf.hpp:60-126): coordinate1 + ratio *
+    coordinate2 = constant over two revolute or prismatic joints; bodies A, B
+    the joints' second bodies, C, D their first; per side (ac, bd) a
+    prismatic joint's local anchors A, B and local x axis, or a revolute
+    joint's reference angle in [0].  World.gear_joint_conf makes one from two
+    joints (GetGearJointConf)."""
+    _fields_ = [("body_a", ctypes.c_uint32), ("body_b", ctypes.c_uint32), ("body_c", ctypes.c_uint32),
+                ("body_d", ctypes.c_uint32), ("prismatic_ac", ctypes.c_uint32), ("prismatic_bd", ctypes.c_uint32),
+                ("ac", ctypes.c_float * 6), ("bd", ctypes.c_float * 6), ("ratio", ctypes.c_float),
+                ("constant", ctypes.c_float), ("collide_connected", ctypes.c_uint32)]
+
+    def __init__(self, **kw):
+        kw.setdefault("ratio", 1.0)
+        super().__init__(**kw)
+
+
 class TargetJointConf(ctypes.Structure):
     """TargetJointConf (d2/TargetJointConf.hpp:56-161), a joint with body B
     only; the reference's defaults: frequency 5, damping ratio 0.7, maximum
@@ -347,6 +364,9 @@ _SIGS = {
     "prx_prismatic_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
                                                   ctypes.POINTER(ctypes.c_uint32)]),
     "prx_target_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
+    "prx_gear_joint_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
+    "prx_gear_joint_conf_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
+                                               ctypes.c_void_p]),
     "prx_world_joint_count": (ctypes.c_uint32, [ctypes.c_void_p]),
     "prx_world_shape_count": (ctypes.c_uint32, [ctypes.c_void_p]),
     "prx_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
@@ -509,8 +529,8 @@ class World:
     def create_joint(self, conf):
         """CreateJoint with a RevoluteJointConf, DistanceJointConf,
         WeldJointConf, RopeJointConf, FrictionJointConf, MotorJointConf,
-        WheelJointConf, PulleyJointConf, PrismaticJointConf or TargetJointConf; returns the joint id
-        (shared by all kinds)."""
+        WheelJointConf, PulleyJointConf, PrismaticJointConf, TargetJointConf or GearJointConf; returns
+        the joint id (shared by all kinds)."""
         fn = {RevoluteJointConf: lib().prx_joint_create, DistanceJointConf: lib().prx_distance_joint_create,
               WeldJointConf: lib().prx_weld_joint_create, RopeJointConf: lib().prx_rope_joint_create,
               FrictionJointConf: lib().prx_friction_joint_create,
@@ -518,12 +538,20 @@ class World:
               WheelJointConf: lib().prx_wheel_joint_create,
               PulleyJointConf: lib().prx_pulley_joint_create,
               PrismaticJointConf: lib().prx_prismatic_joint_create,
-              TargetJointConf: lib().prx_target_joint_create}.get(type(conf))
+              TargetJointConf: lib().prx_target_joint_create,
+              GearJointConf: lib().prx_gear_joint_create}.get(type(conf))
         if fn is None:
             raise TypeError("not a joint conf: %r" % type(conf))
         jid = ctypes.c_uint32()
         _check(fn(self.h, ctypes.byref(conf), ctypes.byref(jid)))
         return jid.value
+
+    def gear_joint_conf(self, joint1, joint2, ratio=1.0):
+        """GetGearJointConf(world, joint1, joint2, ratio) at the bodies'
+        present positions."""
+        conf = GearJointConf()
+        _check(lib().prx_gear_joint_conf_get(self.h, joint1, joint2, ratio, ctypes.byref(conf)))
+        return conf
 
     @property
     def joint_count(self):

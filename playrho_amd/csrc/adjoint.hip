@@ -778,7 +778,7 @@ __global__ __launch_bounds__(64) void k_adjoint_general(DeviceState d, Tape t, A
             const float4* u = jcrec(e);
             const int bA = __float_as_int(u[5].z), bB = __float_as_int(u[5].w);
             const int jg = __float_as_int(u[7].w);
-            const float4 in8 = u[8], raw = u[9];
+            const float4 in8 = u[prx::kJcStride], raw = u[prx::kJcStride + 1];
             float x[7] = {in8.x, in8.y, raw.x, raw.y, raw.z, raw.w, in8.z};
             float yb[16];
             for (int k = 0; k < 16; ++k) yb[k] = a.jfld[js * prx::kAdjJointFields + k];

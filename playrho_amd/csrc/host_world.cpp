@@ -85,6 +85,13 @@ inline float MagSq(V2 v) {
     return r;
 }
 inline V2 Rotate(V2 v, V2 q) { return {(q.x * v.x) - (q.y * v.y), (q.y * v.x) + (q.x * v.y)}; }
+inline V2 InvRotate(V2 v, V2 q) { return {(q.x * v.x) + (q.y * v.y), (q.x * v.y) - (q.y * v.x)}; }
+inline float Dot(V2 a, V2 b) {
+    float r = 0.0f;
+    r += a.x * b.x;
+    r += a.y * b.y;
+    return r;
+}
 inline V2 UnitVector(V2 v, V2 fallback) {
     const float m2 = v.x * v.x + v.y * v.y;
     if (std::isnormal(m2)) {
@@ -279,6 +286,13 @@ struct HostJoint {
     float maxTorque = 0;
     float ext[3] = {0, 0, 0};               // wheel: local x axis of A (x, y), -
     float imp[4] = {0, 0, 0, 0};            // impulse x, y, z, angular motor impulse
+    // Gear (kind 10, prismatic sides in flag bits kGearPrisAC / kGearPrisBD):
+    // bodies C, D, the ratio, the constant, and per side (A-C in gear[0..5],
+    // B-D in gear[6..11]) the prismatic joint's local anchors A, B and local
+    // x axis or the revolute joint's reference angle (first float).
+    std::uint32_t c = 0, d = 0;
+    float ratio = 1, constant = 0;
+    float gear[12] = {};
     std::uint32_t limit = 0;
     std::uint64_t seq = 0;   // creation order (the bodies' joint lists follow it)
     bool destroyed = false;  // Destroy(World&, JointID)
@@ -968,6 +982,87 @@ int AddPrismaticJoint(prx_world* w, const prx_prismatic_joint_conf& c, std::uint
     return AddJointRecord(w, j, out);
 }
 
+// GearJointConf: bodies A, B of the record plus C, D; the sides' data, the
+// ratio and the constant as given (CreateJoint validates bodies A and B only,
+// AabbTreeWorld.cpp:1143-1149; C and D must be bodies of the world too, the
+// solver indexes them).
+int AddGearJoint(prx_world* w, const prx_gear_joint_conf& c, std::uint32_t* out) {
+    if (c.body_c >= w->bodies.size() || c.body_d >= w->bodies.size() || w->bodies[c.body_c].destroyed ||
+        w->bodies[c.body_d].destroyed)
+        return Fail(PRX_E_OUT_OF_RANGE, "no such body");
+    HostJoint j;
+    j.a = c.body_a;
+    j.b = c.body_b;
+    j.c = c.body_c;
+    j.d = c.body_d;
+    j.flags = (c.collide_connected ? prx::kJointCollideConnected : 0u) | (c.prismatic_ac ? prx::kGearPrisAC : 0u) |
+              (c.prismatic_bd ? prx::kGearPrisBD : 0u) | (prx::kJointGear << 8);
+    for (int k = 0; k < 6; ++k) {
+        j.gear[k] = c.ac[k];
+        j.gear[6 + k] = c.bd[k];
+    }
+    j.ratio = c.ratio;
+    j.constant = c.constant;
+    return AddJointRecord(w, j, out);
+}
+
+// GetGearJointConf(const World&, JointID, JointID, Real) (GearJointConf.cpp:
+// 89-169): bodies A, B the joints' second bodies, C, D their first; a revolute
+// side's scalar the angle difference less the reference angle, a prismatic
+// side's the offset of the second body's anchor from the first's along the
+// axis, in the first body's frame; constant = scalar1 + ratio * scalar2.  A
+// body's transformation is derived from its sweep (position less the rotated
+// local centre), as after any step.
+int GearConf(const prx_world* w, std::uint32_t j1, std::uint32_t j2, float ratio, prx_gear_joint_conf& out) {
+    const std::uint32_t ids[2] = {j1, j2};
+    for (std::uint32_t id : ids) {
+        if (id >= w->joints.size() || w->joints[id].destroyed) return Fail(PRX_E_OUT_OF_RANGE, "no such joint");
+        const std::uint32_t kind = w->joints[id].flags >> 8;
+        if (kind != prx::kJointRevolute && kind != prx::kJointPrismatic)
+            return Fail(PRX_E_INVALID_ARGUMENT, "gear joints couple revolute or prismatic joints only");
+    }
+    out = prx_gear_joint_conf{};
+    out.body_a = w->joints[j1].b;
+    out.body_b = w->joints[j2].b;
+    out.body_c = w->joints[j1].a;
+    out.body_d = w->joints[j2].a;
+    auto xf = [&](std::uint32_t body, V2& p, V2& q) {
+        const HostBody& hb = w->bodies[body];
+        q = UnitFromAngle(hb.pos1[2]);
+        p = V2{hb.pos1[0], hb.pos1[1]} - Rotate(hb.lc, q);
+    };
+    float scalar[2];
+    for (int k = 0; k < 2; ++k) {
+        const HostJoint& jt = w->joints[ids[k]];
+        float* side = k ? out.bd : out.ac;
+        const std::uint32_t self = k ? out.body_b : out.body_a;
+        const std::uint32_t other = k ? out.body_d : out.body_c;
+        if ((jt.flags >> 8) == prx::kJointRevolute) {
+            const float referenceAngle = jt.cfg[0];
+            side[0] = referenceAngle;
+            scalar[k] = ((w->bodies[self].pos1[2] - w->bodies[other].pos1[2]) - referenceAngle) / 1.0f;
+        } else {
+            V2 ps, qs, po, qo;
+            xf(self, ps, qs);
+            xf(other, po, qo);
+            const V2 laA{jt.la[0], jt.la[1]}, laB{jt.la[2], jt.la[3]}, lxA{jt.ext[0], jt.ext[1]};
+            side[0] = laA.x;
+            side[1] = laA.y;
+            side[2] = laB.x;
+            side[3] = laB.y;
+            side[4] = lxA.x;
+            side[5] = lxA.y;
+            const V2 pOther = laA;
+            const V2 pSelf = InvRotate(Rotate(laB, qs) + (ps - po), qo);
+            scalar[k] = Dot(pSelf - pOther, lxA) / 1.0f;
+            (k ? out.prismatic_bd : out.prismatic_ac) = 1u;
+        }
+    }
+    out.ratio = ratio;
+    out.constant = scalar[0] + out.ratio * scalar[1];
+    return PRX_OK;
+}
+
 // TargetJointConf: no body A; the anchor on B in the anchor B slot, (target x,
 // y, maximum force, frequency), the damping ratio in the maximum torque slot.
 int AddTargetJoint(prx_world* w, const prx_target_joint_conf& c, std::uint32_t* out) {
@@ -1173,6 +1268,7 @@ int BuildBatch(prx_batch* b) {
     PRX_HIP(a.alloc(d.j_anchor, J));
     PRX_HIP(a.alloc(d.j_cfg, J));
     PRX_HIP(a.alloc(d.j_cfg2, J));
+    PRX_HIP(a.alloc(d.j_gear, 2 * J));
     PRX_HIP(a.alloc(d.j_imp, J));
     PRX_HIP(a.alloc(d.j_limit, J));
     PRX_HIP(a.alloc(d.bjoint, B));
@@ -1207,7 +1303,7 @@ int UploadBatch(prx_batch* b) {
     std::vector<uint4> mhdr(C);
     const std::size_t J = b->totJ;
     std::vector<int2> jbody(J), bjoint(B), bjlist(2 * J);
-    std::vector<float4> janchor(J), jcfg(J), jcfg2(J), jimp(J);
+    std::vector<float4> janchor(J), jcfg(J), jcfg2(J), jimp(J), jgear(2 * J);
     std::vector<std::uint32_t> jlimit(J);
     b->joint_kinds = false;
     for (std::size_t wi = 0; wi < b->worlds.size(); ++wi) {
@@ -1232,6 +1328,17 @@ int UploadBatch(prx_batch* b) {
                 float fl;
                 std::memcpy(&fl, &j.flags, 4);
                 jcfg2[g] = float4{j.maxTorque, fl, j.ext[0], j.ext[1]};
+                if ((j.flags >> 8) == prx::kJointGear) {
+                    janchor[g] = float4{j.gear[0], j.gear[1], j.gear[2], j.gear[3]};
+                    jcfg[g] = float4{j.gear[6], j.gear[7], j.gear[8], j.gear[9]};
+                    jcfg2[g] = float4{0.0f, fl, 0.0f, 0.0f};
+                    float cbits, dbits;
+                    const int ci = int(j.c), di = int(j.d);
+                    std::memcpy(&cbits, &ci, 4);
+                    std::memcpy(&dbits, &di, 4);
+                    jgear[2 * g] = float4{j.ratio, j.constant, cbits, dbits};
+                    jgear[2 * g + 1] = float4{j.gear[4], j.gear[5], j.gear[10], j.gear[11]};
+                }
                 if (j.flags >> 8) b->joint_kinds = true;
                 jimp[g] = float4{j.imp[0], j.imp[1], j.imp[2], j.imp[3]};
                 jlimit[g] = j.limit;
@@ -1328,7 +1435,7 @@ int UploadBatch(prx_batch* b) {
         (st = Upload(d.c_id, cid)) || (st = Upload(d.c_flags, cflags)) || (st = Upload(d.c_mat, cmat)) ||
         (st = Upload(d.m_hdr, mhdr)) || (st = Upload(d.m_geo, mgeo)) || (st = Upload(d.m_p0, mp0)) ||
         (st = Upload(d.m_p1, mp1)) || (st = Upload(d.c_free, cfree)) || (st = Upload(d.j_body, jbody)) ||
-        (st = Upload(d.j_anchor, janchor)) || (st = Upload(d.j_cfg, jcfg)) || (st = Upload(d.j_cfg2, jcfg2)) ||
+        (st = Upload(d.j_anchor, janchor)) || (st = Upload(d.j_cfg, jcfg)) || (st = Upload(d.j_cfg2, jcfg2)) || (st = Upload(d.j_gear, jgear)) ||
         (st = Upload(d.j_imp, jimp)) || (st = Upload(d.j_limit, jlimit)) || (st = Upload(d.bjoint, bjoint)) ||
         (st = Upload(d.bjoint_list, bjlist)))
         return st;
@@ -1861,6 +1968,13 @@ int prx_scene_build(prx_world* w, const char* name, const std::int32_t* params, 
             if (int st = AddPulleyJoint(w, pc, nullptr)) return st;
             continue;
         }
+        if (js.kind == prx_scene::kGear) {
+            prx_gear_joint_conf gc{};
+            if (int st = GearConf(w, std::uint32_t(js.a), std::uint32_t(js.b), js.ratio, gc)) return st;
+            gc.collide_connected = js.collideConnected ? 1u : 0u;
+            if (int st = AddGearJoint(w, gc, nullptr)) return st;
+            continue;
+        }
         if (js.kind == prx_scene::kTarget) {
             prx_target_joint_conf tc{};
             tc.body = std::uint32_t(js.b);
@@ -2004,6 +2118,21 @@ int prx_target_joint_create(prx_world* w, const prx_target_joint_conf* conf, std
     if (!conf) return Fail(PRX_E_INVALID_ARGUMENT, "null joint conf");
     if (int st = Detach(w)) return st;
     return AddTargetJoint(w, *conf, id);
+}
+
+int prx_gear_joint_conf_get(prx_world* w, std::uint32_t joint1, std::uint32_t joint2, float ratio,
+                            prx_gear_joint_conf* out) {
+    if (int st = CheckWorld(w)) return st;
+    if (!out) return Fail(PRX_E_INVALID_ARGUMENT, "null conf");
+    if (int st = Refresh(w)) return st;
+    return GearConf(w, joint1, joint2, ratio, *out);
+}
+
+int prx_gear_joint_create(prx_world* w, const prx_gear_joint_conf* conf, std::uint32_t* id) {
+    if (int st = CheckWorld(w)) return st;
+    if (!conf) return Fail(PRX_E_INVALID_ARGUMENT, "null conf");
+    if (int st = Detach(w)) return st;
+    return AddGearJoint(w, *conf, id);
 }
 
 int prx_prismatic_joint_create(prx_world* w, const prx_prismatic_joint_conf* conf, std::uint32_t* id) {
@@ -2175,8 +2304,9 @@ int prx_world_load_desc(prx_world* w, const std::uint8_t* desc, std::size_t len)
         HostJoint j;
         j.a = u32();
         j.b = u32();
-        j.flags = u32() & 0xff07u;
-        if ((j.flags >> 8) > prx::kJointTarget) return Fail(PRX_E_INVALID_ARGUMENT, "unknown joint kind");
+        const std::uint32_t rawFlags = u32();
+        j.flags = rawFlags & 0xff07u;
+        if ((j.flags >> 8) > prx::kJointGear) return Fail(PRX_E_INVALID_ARGUMENT, "unknown joint kind");
         if ((j.a >= fresh.bodies.size() && !(j.a == kNoBody && (j.flags >> 8) == prx::kJointTarget)) ||
             j.b >= fresh.bodies.size())
             return Fail(PRX_E_OUT_OF_RANGE, "no such body");
@@ -2184,10 +2314,23 @@ int prx_world_load_desc(prx_world* w, const std::uint8_t* desc, std::size_t len)
         const std::uint32_t kind = j.flags >> 8;
         if (kind && kind != prx::kJointPrismatic)
             j.flags &= ~(prx::kJointLimit | (kind == prx::kJointWheel ? 0u : prx::kJointMotor));
-        for (float& v : j.la) v = f32();
-        for (float& v : j.cfg) v = f32();
-        j.maxTorque = f32();
-        for (float& v : j.ext) v = f32();
+        if (kind == prx::kJointGear) {
+            // Gear: the sides' 12 floats, then bodies C, D, ratio, constant;
+            // prismatic sides in bits 16 (A-C) and 17 (B-D) of the flags.
+            for (float& v : j.gear) v = f32();
+            j.c = u32();
+            j.d = u32();
+            j.ratio = f32();
+            j.constant = f32();
+            if (j.c >= fresh.bodies.size() || j.d >= fresh.bodies.size())
+                return Fail(PRX_E_OUT_OF_RANGE, "no such body");
+            j.flags |= ((rawFlags >> 16) & 1u ? prx::kGearPrisAC : 0u) | ((rawFlags >> 17) & 1u ? prx::kGearPrisBD : 0u);
+        } else {
+            for (float& v : j.la) v = f32();
+            for (float& v : j.cfg) v = f32();
+            j.maxTorque = f32();
+            for (float& v : j.ext) v = f32();
+        }
         j.seq = fresh.joints.size();
         fresh.joints.push_back(j);
     }
@@ -2253,6 +2396,16 @@ int prx_world_save_desc(prx_world* w, std::uint8_t* buf, std::size_t cap, std::s
     for (const auto& j : w->joints) {
         u32(j.a);
         u32(j.b);
+        if ((j.flags >> 8) == prx::kJointGear) {
+            u32((j.flags & 0xff07u) | ((j.flags & prx::kGearPrisAC) ? 1u << 16 : 0u) |
+                ((j.flags & prx::kGearPrisBD) ? 1u << 17 : 0u));
+            put(j.gear, sizeof(j.gear));
+            u32(j.c);
+            u32(j.d);
+            f32(j.ratio);
+            f32(j.constant);
+            continue;
+        }
         u32(j.flags);
         put(j.la, sizeof(j.la));
         put(j.cfg, sizeof(j.cfg));
@@ -2429,6 +2582,10 @@ int prx_world_destroy_body(prx_world* w, std::uint32_t body) {
     if (int st = CheckWorld(w)) return st;
     if (int st = CheckBody(w, body)) return st;
     if (int st = Detach(w)) return st;
+    for (const HostJoint& j : w->joints)
+        if (!j.destroyed && (j.flags >> 8) == prx::kJointGear && (j.c == body || j.d == body) && j.a != body &&
+            j.b != body)
+            return Fail(PRX_E_UNSUPPORTED, "destroying body C or D of a gear joint (destroy the gear joint first)");
     HostBody& hb = w->bodies[body];
     std::vector<std::uint32_t> pending;
     for (int pi : hb.proxies)

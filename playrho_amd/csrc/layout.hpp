@@ -101,7 +101,8 @@ struct StepParams {
 // its geometry inputs, the inputs of every Gauss-Seidel update and of every
 // position-solver update, and the island table.
 constexpr int kTapeVc = 24;
-constexpr int kTapeJc = 10;
+constexpr int kJcStride = 10;
+constexpr int kTapeJc = kJcStride + 2;
 struct Tape {
     float4* vc;           // slot * kTapeVc: VelocityConstraint record [0..8], body positions [9],
                           // (vn0, contact id, manifold type, radius A) [10], (radius B, shape A, shape B, -) [11],
@@ -116,8 +117,9 @@ struct Tape {
     int2* islj;           // per island: first and end joint slot
     int* nisl;            // per world
     std::uint8_t* inisl;  // per body: part of an island
-    float4* jc;           // joint slot * kTapeJc: joint record at set-up [0..7], (angle A, angle B, maximum motor
-                          // torque, previous limit state) [8], impulses before the set-up [9]
+    float4* jc;           // joint slot * kTapeJc: joint record at set-up [0..kJcStride-1], (angle A, angle B,
+                          // maximum motor torque, previous limit state) [kJcStride], impulses before the
+                          // set-up [kJcStride + 1]
     float4* jgs;          // (joint slot * itV + it) * 3: velocities of A, B and the impulses before the update
     float4* jngs;         // (joint slot * itP + it) * 2: positions of A and B before the update
     std::uint32_t* cmap;  // per world contact pool id: how the next step's manifold took this step's final
@@ -213,6 +215,9 @@ struct DeviceState {
                          // lengths A, B)
     float4* j_cfg2;      // max motor torque, flags (limit 1, motor 2, collideConnected 4,
                          // kind << 8), wheel: local x axis of A (x, y)
+    float4* j_gear;      // gear joints, 2 per joint: (ratio, constant, body C, body D (world-local int
+                         // bits)), (local x axis of the A-C side, of the B-D side); the sides' anchors in
+                         // j_anchor (A-C) and j_cfg (B-D), a revolute side's reference angle in .x
     float4* j_imp;       // impulse x, y, z, angular motor impulse
     std::uint32_t* j_limit;  // limit state (0 inactive, 1 lower, 2 upper, 3 equal)
     int2* bjoint;        // per body: first entry in bjoint_list, count (world-local)
@@ -283,7 +288,6 @@ constexpr int kEvWordsPerSlot = 48;
 
 constexpr int kVcStride = 9;
 constexpr int kPcStride = 5;
-constexpr int kJcStride = 8;
 enum : std::uint32_t { kJointLimit = 1u, kJointMotor = 2u, kJointCollideConnected = 4u };
 // Pi as the float build's Real (RealConstants.hpp:57).
 constexpr float kPiF = 3.14159265358979323846f;
@@ -298,8 +302,11 @@ enum : std::uint32_t {
     kJointWheel = 6u,
     kJointPulley = 7u,
     kJointPrismatic = 8u,
-    kJointTarget = 9u
+    kJointTarget = 9u,
+    kJointGear = 10u
 };
+// Gear joints: flag bits set when the A-C (B-D) side is a prismatic joint.
+enum : std::uint32_t { kGearPrisAC = 8u, kGearPrisBD = 16u };
 // Constraint records staged through LDS per chunk by the wave-wide sweeps;
 // the stage sits at the end of the dynamic LDS.
 constexpr int kStageRecords = 16;

@@ -2410,6 +2410,226 @@ __device__ __forceinline__ bool spring_joint_solve_position(const float4* u, std
     return ok;
 }
 
+// Gear record (kJcStride float4): u[0] (JvAC, JvBD); u[1] (JwA, JwB, JwC,
+// JwD); u[2] (mass, ratio, constant, flags | kind << 16); u[3] (impulse, 0,
+// 0, 0); u[4] (mA, iA, mB, iB); u[5] (mC, iC, body A, body B); u[6] (mD, iD,
+// body C, body D); u[7] the A-C side's anchors less the local centres (local
+// anchor A - centre of C, local anchor B - centre of A; a revolute side's
+// reference angle in .x); u[8] the B-D side's (anchor A - centre of D, anchor
+// B - centre of B); u[9] the sides' local x axes.  Revolute sides have no
+// linear terms and angular terms 1 (A-C) or the ratio (B-D).
+
+// A prismatic side's Jacobian at the given angles (GearJointConf.cpp:188-204,
+// 213-229): the axis rotated by the other body (C or D) scaled by s, and the
+// crosses of both arms with it (scaled by s on the B-D side only).
+struct GearSide {
+    V2 Jv;
+    float JwSelf, JwOther;
+};
+__device__ __forceinline__ GearSide gear_side_prismatic(V2 axis, V2 lOther, V2 lSelf, float angSelf, float angOther,
+                                                       float s, bool scaled) {
+    const V2 qSelf = unit_from_angle(angSelf);
+    const V2 qOther = unit_from_angle(angOther);
+    const V2 u = rotate(axis, qOther);
+    const V2 rOther = rotate(lOther, qOther);
+    const V2 rSelf = rotate(lSelf, qSelf);
+    GearSide g;
+    if (scaled) {
+        g.Jv = V2{s * u.x, s * u.y};
+        g.JwOther = s * cross(rOther, u);
+        g.JwSelf = s * cross(rSelf, u);
+    } else {
+        g.Jv = V2{1.0f * u.x, 1.0f * u.y};
+        g.JwOther = cross(rOther, u);
+        g.JwSelf = cross(rSelf, u);
+    }
+    return g;
+}
+
+// InitVelocity(GearJointConf&) less its warm start (GearJointConf.cpp:171-237).
+__device__ __forceinline__ void build_jc_gear(const DeviceState& d, int jg, float4* u, const BodyC& A,
+                                              const float4& pA, const BodyC& B, const float4& pB, const BodyC& C,
+                                              const float4& pC, const BodyC& D, const float4& pD, int bA, int bB,
+                                              int bC, int bD, std::uint32_t flags, const StepParams& c) {
+    const float4 g0 = d.j_gear[2 * std::size_t(jg)], ax = d.j_gear[2 * std::size_t(jg) + 1];
+    const float4 ac = d.j_anchor[jg], bd = d.j_cfg[jg];
+    const float ratio = g0.x;
+    float invMass = 0.0f;
+    float4 sideAC = ac, sideBD = bd;
+    V2 JvAC{0.0f, 0.0f}, JvBD{0.0f, 0.0f};
+    float JwA, JwB, JwC, JwD;
+    if (flags & prx::kGearPrisAC) {
+        const V2 lC = V2{ac.x, ac.y} - C.lc, lA = V2{ac.z, ac.w} - A.lc;
+        sideAC = float4{lC.x, lC.y, lA.x, lA.y};
+        const GearSide g = gear_side_prismatic(V2{ax.x, ax.y}, lC, lA, pA.z, pC.z, 1.0f, false);
+        JvAC = g.Jv;
+        JwC = g.JwOther;
+        JwA = g.JwSelf;
+        const float invRotMassC = C.invRotI * (JwC * JwC) / 1.0f;
+        const float invRotMassA = A.invRotI * (JwA * JwA) / 1.0f;
+        invMass += C.invMass + A.invMass + invRotMassC + invRotMassA;
+    } else {
+        JwA = 1.0f;
+        JwC = 1.0f;
+        invMass += A.invRotI + C.invRotI;
+    }
+    if (flags & prx::kGearPrisBD) {
+        const V2 lD = V2{bd.x, bd.y} - D.lc, lB = V2{bd.z, bd.w} - B.lc;
+        sideBD = float4{lD.x, lD.y, lB.x, lB.y};
+        const GearSide g = gear_side_prismatic(V2{ax.z, ax.w}, lD, lB, pB.z, pD.z, ratio, true);
+        JvBD = g.Jv;
+        JwD = g.JwOther;
+        JwB = g.JwSelf;
+        const float invRotMassD = D.invRotI * (JwD * JwD) / 1.0f;
+        const float invRotMassB = B.invRotI * (JwB * JwB) / 1.0f;
+        invMass += (ratio * ratio) * (D.invMass + B.invMass) + invRotMassD + invRotMassB;
+    } else {
+        JwB = ratio * 1.0f;
+        JwD = ratio * 1.0f;
+        invMass += (ratio * ratio) * (B.invRotI + D.invRotI);
+    }
+    const float mass = (invMass > 0.0f) ? 1.0f / invMass : 0.0f;
+    u[0] = float4{JvAC.x, JvAC.y, JvBD.x, JvBD.y};
+    u[1] = float4{JwA, JwB, JwC, JwD};
+    u[2] = float4{mass, ratio, g0.y, __uint_as_float((flags & 0xffu) | (prx::kJointGear << 16))};
+    u[3] = float4{c.warmStart ? d.j_imp[jg].x : 0.0f, 0.0f, 0.0f, 0.0f};
+    u[4] = float4{A.invMass, A.invRotI, B.invMass, B.invRotI};
+    u[5] = float4{C.invMass, C.invRotI, __int_as_float(bA), __int_as_float(bB)};
+    u[6] = float4{D.invMass, D.invRotI, __int_as_float(bC), __int_as_float(bD)};
+    u[7] = sideAC;
+    u[8] = sideBD;
+    u[9] = ax;
+}
+
+// The velocity change of an impulse on the four bodies (GearJointConf.cpp:
+// 226-235 and 281-288): A and B gain, C and D lose.
+__device__ __forceinline__ void gear_apply(const float4* u, float imp, float4& vA, float4& vB, float4& vC,
+                                           float4& vD) {
+    const float4 J = u[0], W = u[1], mab = u[4], mc = u[5], md = u[6];
+    const float lA = mab.x * imp, lB = mab.z * imp, lC = mc.x * imp, lD = md.x * imp;
+    vA.x = vA.x + lA * J.x;
+    vA.y = vA.y + lA * J.y;
+    vA.z = vA.z + mab.y * imp * W.x / 1.0f;
+    vB.x = vB.x + lB * J.z;
+    vB.y = vB.y + lB * J.w;
+    vB.z = vB.z + mab.w * imp * W.y / 1.0f;
+    vC.x = vC.x - lC * J.x;
+    vC.y = vC.y - lC * J.y;
+    vC.z = vC.z - mc.y * imp * W.z / 1.0f;
+    vD.x = vD.x - lD * J.z;
+    vD.y = vD.y - lD * J.w;
+    vD.z = vD.z - md.y * imp * W.w / 1.0f;
+}
+
+// SolveVelocity(GearJointConf&) (GearJointConf.cpp:257-296): returns whether
+// the impulse was zero.
+__device__ __forceinline__ bool gear_solve_velocity(float4* u, float4& vA, float4& vB, float4& vC, float4& vD) {
+    const float4 J = u[0], W = u[1];
+    const float acDot = dot(V2{J.x, J.y}, V2{vA.x - vC.x, vA.y - vC.y});
+    const float bdDot = dot(V2{J.z, J.w}, V2{vB.x - vD.x, vB.y - vD.y});
+    const float Cdot = acDot + bdDot + (W.x * vA.z - W.z * vC.z) / 1.0f + (W.y * vB.z - W.w * vD.z) / 1.0f;
+    const float impulse = -u[2].x * 1.0f * Cdot;
+    u[3].x = u[3].x + impulse;
+    gear_apply(u, impulse, vA, vB, vC, vD);
+    return impulse == 0.0f;
+}
+
+// Cap(Position, ConstraintSolverConf) (d2/Math.cpp:63-71).
+__device__ __forceinline__ float4 cap_position(float4 p, const StepParams& c) {
+    const float lsquared = magsq(V2{p.x, p.y});
+    if (lsquared > c.maxLinearCorrection * c.maxLinearCorrection) {
+        const float s = c.maxLinearCorrection / __builtin_sqrtf(lsquared);
+        p.x = p.x * s;
+        p.y = p.y * s;
+    }
+    p.z = clamp_std(p.z, -c.maxAngularCorrection, +c.maxAngularCorrection);
+    return p;
+}
+
+// SolvePosition(const GearJointConf&) (GearJointConf.cpp:298-471): the
+// coordinates at the positions now, one capped correction on all four
+// bodies; returns whether it is within tolerance.
+__device__ __forceinline__ bool gear_solve_position(const float4* u, float4& pA, float4& pB, float4& pC, float4& pD,
+                                                    const StepParams& c) {
+    const float4 w2 = u[2], mab = u[4], mc = u[5], md = u[6], sAC = u[7], sBD = u[8], ax = u[9];
+    const float ratio = w2.y, constant = w2.z;
+    const std::uint32_t flags = __float_as_uint(w2.w);
+    if (!__builtin_isfinite(constant)) return true;
+    V2 JvAC{0.0f, 0.0f}, JvBD{0.0f, 0.0f};
+    float JwA, JwB, JwC, JwD, coordinateA, coordinateB;
+    float invMass = 0.0f;
+    if (flags & prx::kGearPrisAC) {
+        const V2 lC{sAC.x, sAC.y}, lA{sAC.z, sAC.w}, axis{ax.x, ax.y};
+        const V2 qA = unit_from_angle(pA.z);
+        const V2 qC = unit_from_angle(pC.z);
+        const V2 uu = rotate(axis, qC);
+        const V2 rC = rotate(lC, qC);
+        const V2 rA = rotate(lA, qA);
+        JvAC = V2{uu.x * 1.0f, uu.y * 1.0f};
+        JwC = cross(rC, uu);
+        JwA = cross(rA, uu);
+        const float invLinMass = mc.x + mab.x;
+        const float sc = JwC * 1.0f / 1.0f, sa = JwA * 1.0f / 1.0f;
+        const float invRotMassC = mc.y * (sc * sc);
+        const float invRotMassA = mab.y * (sa * sa);
+        invMass += invLinMass + invRotMassC + invRotMassA;
+        const V2 p = invrotate(rA + (V2{pA.x, pA.y} - V2{pC.x, pC.y}), qC);
+        coordinateA = dot(p - lC, axis) / 1.0f;
+    } else {
+        JwA = 1.0f;
+        JwC = 1.0f;
+        invMass += mab.y + mc.y;
+        coordinateA = ((pA.z - pC.z) - sAC.x) / 1.0f;
+    }
+    if (flags & prx::kGearPrisBD) {
+        const V2 lD{sBD.x, sBD.y}, lB{sBD.z, sBD.w}, axis{ax.z, ax.w};
+        const V2 qB = unit_from_angle(pB.z);
+        const V2 qD = unit_from_angle(pD.z);
+        const V2 uu = rotate(axis, qD);
+        const V2 rD = rotate(lD, qD);
+        const V2 rB = rotate(lB, qB);
+        JvBD = V2{ratio * uu.x, ratio * uu.y};
+        JwD = ratio * cross(rD, uu);
+        JwB = ratio * cross(rB, uu);
+        const float invLinMass = (ratio * ratio) * (md.x + mab.z);
+        const float sd = JwD * 1.0f / 1.0f, sb = JwB * 1.0f / 1.0f;
+        const float invRotMassD = md.y * (sd * sd);
+        const float invRotMassB = mab.w * (sb * sb);
+        invMass += invLinMass + invRotMassD + invRotMassB;
+        const V2 p = invrotate(rB + (V2{pB.x, pB.y} - V2{pD.x, pD.y}), qD);
+        coordinateB = dot(p - lD, axis) / 1.0f;
+    } else {
+        JwB = ratio;
+        JwD = ratio;
+        invMass += (ratio * ratio) * (mab.w + md.y);
+        coordinateB = ((pB.z - pD.z) - sBD.x) / 1.0f;
+    }
+    const float C = ((coordinateA + ratio * coordinateB) - constant) * c.regResolutionRate;
+    const float impulse = ((invMass > 0.0f) ? -C / invMass : 0.0f) * 1.0f * 1.0f;
+    auto delta = [&](float m, float i, V2 Jv, float Jw) {
+        const float l = m * impulse;
+        return cap_position(float4{l * Jv.x, l * Jv.y, i * impulse * Jw * 1.0f / 1.0f, 0.0f}, c);
+    };
+    const float4 dA = delta(mab.x, mab.y, JvAC, JwA);
+    const float4 dC = delta(mc.x, mc.y, JvAC, JwC);
+    const float4 dB = delta(mab.z, mab.w, JvBD, JwB);
+    const float4 dD = delta(md.x, md.y, JvBD, JwD);
+    pA = float4{pA.x + dA.x, pA.y + dA.y, pA.z + dA.z, pA.w};
+    pB = float4{pB.x + dB.x, pB.y + dB.y, pB.z + dB.z, pB.w};
+    pC = float4{pC.x - dC.x, pC.y - dC.y, pC.z - dC.z, pC.w};
+    pD = float4{pD.x - dD.x, pD.y - dD.y, pD.z - dD.z, pD.w};
+    float linearError = 0.0f, angularError = 0.0f;
+    linearError = fmax_std(linearError, magsq(V2{dA.x, dA.y}));
+    linearError = fmax_std(linearError, magsq(V2{dB.x, dB.y}));
+    linearError = fmax_std(linearError, magsq(V2{dC.x, dC.y}));
+    linearError = fmax_std(linearError, magsq(V2{dD.x, dD.y}));
+    angularError = fmax_std(angularError, __builtin_fabsf(dA.z));
+    angularError = fmax_std(angularError, __builtin_fabsf(dB.z));
+    angularError = fmax_std(angularError, __builtin_fabsf(dC.z));
+    angularError = fmax_std(angularError, __builtin_fabsf(dD.z));
+    return (linearError <= c.linearSlop * c.linearSlop) && (angularError <= c.angularSlop);
+}
+
 // Per-step joint set-up half of InitVelocity (RevoluteJointConf.cpp:127-197),
 // writes the joint's solver record.
 __device__ __forceinline__ void build_jc(const DeviceState& d, const WorldCtx& x, int jg, float4* u, const BodyC& A,
@@ -3059,6 +3279,20 @@ __device__ __forceinline__ bool joints_velocity_k(float4* jbase, int j0, int j1,
     for (int e = j0; e < j1; ++e) {
         float4* u = jbase + std::size_t(e) * prx::kJcStride;
         const std::uint32_t kind = kKinds ? jc_kind(u) : prx::kJointRevolute;
+        if (kind == prx::kJointGear) {
+            // Four bodies, stored back in the order A, B, C, D (a body that is
+            // two of them keeps the last store, as the reference's does).
+            const int bA = __float_as_int(u[5].z), bB = __float_as_int(u[5].w);
+            const int bC = __float_as_int(u[6].z), bD = __float_as_int(u[6].w);
+            float4 va = bs.vel(bA), vb = bs.vel(bB), vc = bs.vel(bC), vd = bs.vel(bD);
+            if (warm) gear_apply(u, u[3].x, va, vb, vc, vd);
+            else ok &= gear_solve_velocity(u, va, vb, vc, vd);
+            bs.set_vel(bA, va);
+            bs.set_vel(bB, vb);
+            bs.set_vel(bC, vc);
+            bs.set_vel(bD, vd);
+            continue;
+        }
         if (kind != prx::kJointRevolute) {
             const float4 mo = u[5];
             const int bA = __float_as_int(mo.z), bB = __float_as_int(mo.w);
@@ -3150,9 +3384,19 @@ __device__ __forceinline__ bool joints_position_k(const float4* jbase, int j0, i
         const float4* u = jbase + std::size_t(e) * prx::kJcStride;
         const float4 mo = u[5];
         const int bA = __float_as_int(mo.z), bB = __float_as_int(mo.w);
+        const std::uint32_t kind = kKinds ? jc_kind(u) : prx::kJointRevolute;
+        if (kind == prx::kJointGear) {
+            const int bC = __float_as_int(u[6].z), bD = __float_as_int(u[6].w);
+            float4 pa = bs.pos(bA), pb = bs.pos(bB), pc = bs.pos(bC), pd = bs.pos(bD);
+            ok &= gear_solve_position(u, pa, pb, pc, pd, c);
+            bs.set_pos(bA, pa);
+            bs.set_pos(bB, pb);
+            bs.set_pos(bC, pc);
+            bs.set_pos(bD, pd);
+            continue;
+        }
         float4 pa = bs.pos(bA);
         float4 pb = bs.pos(bB);
-        const std::uint32_t kind = kKinds ? jc_kind(u) : prx::kJointRevolute;
         if (kind == prx::kJointRevolute) ok &= joint_solve_position(u, pa, pb, c);
         else ok &= spring_joint_solve_position(u, kind, pa, pb, c);
         bs.set_pos(bA, pa);
@@ -4157,11 +4401,20 @@ __device__ __forceinline__ SolveSummary solve_islands(const DeviceState& d, cons
         const BodyC A = load_bodyc(d, bo + bd.x), B = load_bodyc(d, bo + bd.y);
         if constexpr (kTape) {
             float4* t = d.tape.jc + (std::size_t(x.jo) + std::size_t(e)) * prx::kTapeJc;
-            t[8] = float4{bs.pos(bd.x).z, bs.pos(bd.y).z, d.j_cfg2[jg].x, __uint_as_float(d.j_limit[jg])};
-            t[9] = d.j_imp[jg];
+            t[prx::kJcStride] = float4{bs.pos(bd.x).z, bs.pos(bd.y).z, d.j_cfg2[jg].x, __uint_as_float(d.j_limit[jg])};
+            t[prx::kJcStride + 1] = d.j_imp[jg];
         }
-        build_jc(d, x, jg, d.s_jc + (std::size_t(x.jo) + std::size_t(e)) * prx::kJcStride, A, bs.pos(bd.x), B,
-                 bs.pos(bd.y), bd.x, bd.y, c);
+        const std::uint32_t jflags = __float_as_uint(d.j_cfg2[jg].y);
+        if ((jflags >> 8) == prx::kJointGear) {
+            const float4 g0 = d.j_gear[2 * std::size_t(jg)];
+            const int bC = __float_as_int(g0.z), bD = __float_as_int(g0.w);
+            const BodyC C = load_bodyc(d, bo + bC), D = load_bodyc(d, bo + bD);
+            build_jc_gear(d, jg, d.s_jc + (std::size_t(x.jo) + std::size_t(e)) * prx::kJcStride, A, bs.pos(bd.x), B,
+                          bs.pos(bd.y), C, bs.pos(bC), D, bs.pos(bD), bd.x, bd.y, bC, bD, jflags, c);
+        } else {
+            build_jc(d, x, jg, d.s_jc + (std::size_t(x.jo) + std::size_t(e)) * prx::kJcStride, A, bs.pos(bd.x), B,
+                     bs.pos(bd.y), bd.x, bd.y, c);
+        }
         if constexpr (kTape) {
             const float4* u = d.s_jc + (std::size_t(x.jo) + std::size_t(e)) * prx::kJcStride;
             float4* t = d.tape.jc + (std::size_t(x.jo) + std::size_t(e)) * prx::kTapeJc;

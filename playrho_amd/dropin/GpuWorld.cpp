@@ -20,6 +20,7 @@
 #include <playrho/d2/PolygonShapeConf.hpp>
 #include <playrho/d2/PrismaticJointConf.hpp>
 #include <playrho/d2/TargetJointConf.hpp>
+#include <playrho/d2/GearJointConf.hpp>
 #include <playrho/d2/PulleyJointConf.hpp>
 #include <playrho/d2/RevoluteJointConf.hpp>
 #include <playrho/d2/RopeJointConf.hpp>
@@ -719,8 +720,34 @@ JointID CreateJoint(GpuWorld& g, const Joint& def) {
         c.damping_ratio = F(j->dampingRatio);
         c.collide_connected = cc;
         Check(prx_target_joint_create(g.m_world, &c, &id));
+    } else if (const auto* j = TypeCast<const GearJointConf>(&def)) {
+        // The conf as GetGearJointConf made it (sides, ratio, constant).
+        prx_gear_joint_conf c{};
+        c.body_a = bA;
+        c.body_b = bB;
+        c.body_c = std::uint32_t(to_underlying(j->bodyC));
+        c.body_d = std::uint32_t(to_underlying(j->bodyD));
+        const GearJointConf::TypeData* sides[2] = {&j->typeDataAC, &j->typeDataBD};
+        for (int k = 0; k < 2; ++k) {
+            float* out = k ? c.bd : c.ac;
+            if (const auto* pd = std::get_if<GearJointConf::PrismaticData>(sides[k])) {
+                (k ? c.prismatic_bd : c.prismatic_ac) = 1u;
+                xy(pd->localAnchorA, out);
+                xy(pd->localAnchorB, out + 2);
+                out[4] = F(GetX(pd->localAxis));
+                out[5] = F(GetY(pd->localAxis));
+            } else if (const auto* rd = std::get_if<GearJointConf::RevoluteData>(sides[k])) {
+                out[0] = F(Real(rd->referenceAngle / Radian));
+            } else {
+                throw InvalidArgument("playrho_amd: a gear joint side that is neither revolute nor prismatic");
+            }
+        }
+        c.ratio = F(j->ratio);
+        c.constant = F(j->constant);
+        c.collide_connected = cc;
+        Check(prx_gear_joint_create(g.m_world, &c, &id));
     } else {
-        Unsupported("this joint type (gear joints)");
+        Unsupported("this joint type");
     }
     // The engine reuses the id destroyed last (ObjectPool).
     if (id < g.m_joints.size()) g.m_joints[id] = def;

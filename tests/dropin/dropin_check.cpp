@@ -29,6 +29,8 @@
 #include <playrho/WrongState.hpp>
 #include <playrho/d2/BasicAPI.hpp>
 #include <playrho/d2/RevoluteJointConf.hpp>
+#include <playrho/d2/PrismaticJointConf.hpp>
+#include <playrho/d2/GearJointConf.hpp>
 #include <playrho/d2/WorldContact.hpp>
 
 #include "../../playrho_amd/dropin/GpuWorld.hpp"
@@ -90,6 +92,39 @@ void BuildChain(World& world) {
         CreateJoint(world, Joint{jc});
         prev = b;
     }
+}
+
+// Two wheels on revolute joints to the ground, the first motorised, meshed
+// by a gear, and a rack on a horizontal prismatic joint geared to the second
+// wheel (GetGearJointConf on the world being built), beside two falling boxes.
+// Returns the gear joints.
+std::vector<JointID> BuildGears(World& world) {
+    const auto ground = CreateShape(world, EdgeShapeConf{Length2{-40_m, 0_m}, Length2{40_m, 0_m}});
+    const auto g = CreateBody(world, BodyConf{}.Use(ground));
+    const auto gravity = LinearAcceleration2{0, -10 * MeterPerSquareSecond};
+    std::vector<JointID> revs, gears;
+    for (int i = 0; i < 2; ++i) {
+        const auto disk = CreateShape(world, DiskShapeConf{}.UseRadius(Real(1.0f + 0.5f * i) * Meter).UseDensity(1_kgpm2));
+        const auto at = Length2{Real(-4.0f + 5.0f * i) * Meter, 6_m};
+        const auto b = CreateBody(world, BodyConf{}.Use(BodyType::Dynamic).Use(disk).UseLocation(at)
+                                             .UseAngle(Real(0.3f * i) * Radian).UseLinearAcceleration(gravity));
+        auto jc = RevoluteJointConf{}.UseBodyA(g).UseBodyB(b);
+        jc.localAnchorA = at;
+        jc.referenceAngle = Real(0.3f * i) * Radian;
+        if (i == 0) jc.UseEnableMotor(true).UseMotorSpeed(1.5_rad / 1_s).UseMaxMotorTorque(200_Nm);
+        revs.push_back(CreateJoint(world, Joint{jc}));
+    }
+    gears.push_back(CreateJoint(world, Joint{GetGearJointConf(world, revs[0], revs[1], 1.0f / 1.5f)}));
+    const auto bar = CreateShape(world, PolygonShapeConf{}.UseDensity(1_kgpm2).SetAsBox(3_m, 0.25_m));
+    const auto rack = CreateBody(world, BodyConf{}.Use(BodyType::Dynamic).Use(bar).UseLocation(Length2{1_m, 4.2_m})
+                                            .UseLinearAcceleration(gravity));
+    const auto pj = CreateJoint(world, Joint{PrismaticJointConf{g, rack, Length2{1_m, 4.2_m}, Length2{}, UnitVec::GetRight()}});
+    gears.push_back(CreateJoint(world, Joint{GetGearJointConf(world, revs[1], pj, -1.0f / 1.5f)}));
+    const auto box = CreateShape(world, PolygonShapeConf{}.UseDensity(1_kgpm2).SetAsBox(0.4_m, 0.4_m));
+    for (int i = 0; i < 2; ++i)
+        CreateBody(world, BodyConf{}.Use(BodyType::Dynamic).Use(box).UseLocation(Length2{Real(-4.5f + 6.0f * i) * Meter, 12_m})
+                              .UseLinearAcceleration(gravity));
+    return gears;
 }
 
 void CompareBodies(const World& a, const World& b, const std::string& when) {
@@ -191,6 +226,17 @@ int Cpu() {
     Expect(GetShapeRange(ref3) == GetShapeRange(gpu3), "compound shape count");
     for (auto s = 0u; s < GetShapeRange(ref3); ++s)
         Expect(GetShape(ref3, ShapeID(s)) == GetShape(gpu3, ShapeID(s)), "compound shape " + std::to_string(s));
+    // Gear joints made by GetGearJointConf through each world's queries.
+    auto ref4 = World{};
+    auto gpu4 = World{playrho_amd::GpuWorld{}};
+    const auto refGears = BuildGears(ref4);
+    const auto gpuGears = BuildGears(gpu4);
+    Expect(refGears == gpuGears, "gear joint ids");
+    for (const auto id : refGears) {
+        Expect(GetJoint(ref4, id) == GetJoint(gpu4, id), "gear joint " + std::to_string(to_underlying(id)));
+        Expect(GetType(gpu4, id) == GetTypeID<GearJointConf>(), "gear joint type");
+    }
+    CompareBodies(ref4, gpu4, "gear world");
     std::printf("cpu: %s\n", g_fail ? "FAIL" : "OK");
     return g_fail ? 1 : 0;
 }
@@ -256,6 +302,16 @@ int Gpu() {
         CompareBodies(ref, gpu, when);
         CompareContacts(ref, gpu, when);
         Expect(refCalls == gpuCalls, when + " listener calls");
+    }
+    // Geared bodies stepped on the device against the reference.
+    auto ref4 = World{};
+    auto gpu4 = World{playrho_amd::GpuWorld{}};
+    BuildGears(ref4);
+    BuildGears(gpu4);
+    for (int s = 1; s <= 120; ++s) {
+        Step(ref4, conf);
+        Step(gpu4, conf);
+        CompareBodies(ref4, gpu4, "gear world step " + std::to_string(s));
     }
     std::printf("gpu: %s (%zu listener calls)\n", g_fail ? "FAIL" : "OK", refCalls.size());
     return g_fail ? 1 : 0;

@@ -387,3 +387,40 @@ def _with_one_body(w, shape_ids):
     arr = (ctypes.c_uint32 * len(shape_ids))(*shape_ids)
     w.create_body(pa.BodyConf(type=0, enabled=1, shapes=arr, nshapes=len(shape_ids)))
     return w.save_desc()
+
+
+def test_gear_joint_conf_matches_reference():
+    """GetGearJointConf restated (prx_gear_joint_conf_get) on the gears
+    scene's reference description gives the reference's gear records bit for
+    bit: revolute-revolute, revolute-prismatic, prismatic-prismatic and a
+    pinion geared to its own carriage's rail; a gear created from the conf is
+    saved as the same record.  Gears of other gears, and destroying a gear's
+    body C or D while the gear lives, are refused."""
+    from tests import dumpfmt
+    g = golden_cases.load("gears0")
+    desc = _desc_only(g["desc_conf"])
+    _, _, joints = dumpfmt.read_description(dumpfmt.Reader(desc))
+    # The scene's gears and the joints each couples (scenes.hpp Gears).
+    pairs = {2: (0, 1), 4: (1, 3), 6: (3, 5), 9: (7, 8), 12: (11, 10)}
+    assert sorted(i for i, j in enumerate(joints) if (j["flags"] >> 8) & 0xff == 10) == sorted(pairs)
+    w = pa.World.from_desc(g["desc_conf"])
+    for gi, (j1, j2) in pairs.items():
+        ref = joints[gi]
+        conf = w.gear_joint_conf(j1, j2, float(ref["extra"][0]))
+        assert (conf.body_a, conf.body_b, conf.body_c, conf.body_d) == (ref["a"], ref["b"], ref["c"], ref["d"]), gi
+        assert conf.prismatic_ac == (ref["flags"] >> 16) & 1 and conf.prismatic_bd == (ref["flags"] >> 17) & 1, gi
+        sides = np.array(list(conf.ac) + list(conf.bd), np.float32)
+        assert np.array_equal(sides.view("<u4"), ref["vals"].view("<u4")), (gi, sides, ref["vals"])
+        got = np.array([conf.ratio, conf.constant], np.float32)
+        assert np.array_equal(got.view("<u4"), ref["extra"].view("<u4")), (gi, got, ref["extra"])
+    conf = w.gear_joint_conf(7, 8, 1.5)
+    jid = w.create_joint(conf)
+    assert jid == len(joints)
+    _, _, again = dumpfmt.read_description(dumpfmt.Reader(w.save_desc()))
+    new = again[jid]
+    assert (new["a"], new["b"], new["c"], new["d"]) == (conf.body_a, conf.body_b, conf.body_c, conf.body_d)
+    assert np.array_equal(new["extra"], np.array([1.5, conf.constant], np.float32))
+    with pytest.raises(pa.PrxError):
+        w.gear_joint_conf(2, 0)  # joint 2 is a gear joint
+    with pytest.raises(pa.PrxError):
+        w.destroy_body(0)  # the ground is body C or D of every gear

@@ -167,7 +167,8 @@ def test_device_matches_oracle(scene, params, steps, toi):
 
 
 @pytest.mark.parametrize("scene,params,coop_min", [
-    ("targets", [1, 0], None), ("targets", [2, 3, 10, 30], 1), ("sliders", [1, 0], None), ("joints", [1, 0], 1)])
+    ("targets", [1, 0], None), ("targets", [2, 3, 10, 30], 1), ("sliders", [1, 0], None), ("joints", [1, 0], 1),
+    ("gears", [1, 0], None), ("gears", [2, 1, 40], 1)])
 def test_joints_without_warm_start_match_oracle(scene, params, coop_min):
     """doWarmStart = false: every joint's impulses restart from zero and a
     target joint's InitVelocity still damps its body's spin
