@@ -85,6 +85,11 @@ EDIT_CASES = [
     # Destroy(JointID) 'J' and Destroy(BodyID) 'D' (body: the id): a weld, the
     # pulley, a body held by friction and weld joints and resting on the
     # ground, a wheel of the cart, a falling body among others.
+    # Gear joints under edits: a wheel's spin set (the gear train carries it),
+    # a wheel-wheel gear and the pinion's gear destroyed, then a slab that is
+    # body B of a prismatic-prismatic gear destroyed with its joints.
+    ("gears0_edit", "gears", 200, [1, 0], [1, 200], False,
+     [(50, "V", 2, 0.0, 0.0, 4.0), (80, "J", 4, 0, 0, 0), (110, "J", 12, 0, 0, 0), (140, "D", 6, 0, 0, 0)]),
     ("joints0_destroy_edit", "joints", 160, [1, 0], [1, 160], False,
      [(40, "J", 3, 0, 0, 0), (60, "J", 18, 0, 0, 0), (90, "D", 24, 0, 0, 0), (110, "D", 19, 0, 0, 0),
       (130, "D", 50, 0, 0, 0), (130, "D", 45, 0, 0, 0)]),
